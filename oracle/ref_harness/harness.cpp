@@ -1,0 +1,895 @@
+// Oracle harness (test infrastructure only).
+//
+// Drives the reference's own header-only templates (pederson/fdtd, fdtd.h and
+// include/*.hpp, taken with -I<reference root>) on small seeded scenarios and
+// writes golden fixtures: one JSON file (parameters + array index) and one raw
+// little-endian float64 .bin per scenario.  It also has a timing mode used as
+// the measured CPU baseline of the reference's own update functors.
+//
+// Nothing here is shipped: the binary lands in oracle/_ref/ and is built only
+// where the reference tree exists.  The step composition follows the
+// reference's own idiom (tests/testTM.cpp:102-113, tests/test3D.cpp:146-171,
+// include/Source.hpp:413-435): one sweep per functor over the interior cells
+// (indices 1..N-2), outermost layer never updated.
+#include <fdtd.h>
+
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace fdtd {
+// The primary DispersiveAtomicUpdate (include/DispersiveMaterials.hpp:259-282)
+// puts std::enable_if_t<implicit> on a class template parameter, which is a
+// hard error as soon as the class is instantiated.  This partial
+// specialisation supplies the implicit branch with exactly the argument list
+// of the primary (flux, field, polarization, current, then the storage
+// arguments), so the reference's own *::implicit_update arithmetic runs
+// unchanged.
+template <typename U, typename F>
+struct DispersiveAtomicUpdate<U, true, F> {
+  static constexpr Dir d = FieldDir<F>::value;
+  static constexpr FieldType ft = F::field_type;
+  template <typename T, typename... A>
+  static void get(T&& f, A&&... a) {
+    U::implicit_update(GetField<typename FluxComponent<ft, d>::type>::get(f),
+                       GetField<typename FieldComponent<ft, d>::type>::get(f),
+                       GetField<typename PolarizationComponent<ft, d>::type>::get(f),
+                       GetField<typename CurrentComponent<ft, d>::type>::get(f),
+                       std::forward<A>(a)...);
+  }
+};
+}  // namespace fdtd
+
+using namespace fdtd;
+
+// ---------------------------------------------------------------- utilities
+struct SplitMix64 {
+  uint64_t s;
+  explicit SplitMix64(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+  int randint(int lo, int hi) { return lo + int(next() % uint64_t(hi - lo + 1)); }
+};
+
+struct Fixture {
+  std::string name;
+  std::string params;  // JSON object body (without braces)
+  std::vector<std::pair<std::string, std::vector<size_t>>> index;
+  std::vector<double> data;
+  void p(const std::string& k, double v) {
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.17g", v);
+    add(k, buf);
+  }
+  void pi(const std::string& k, long v) { add(k, std::to_string(v)); }
+  void ps(const std::string& k, const std::string& v) { add(k, "\"" + v + "\""); }
+  void add(const std::string& k, const std::string& raw) {
+    if (!params.empty()) params += ",\n    ";
+    params += "\"" + k + "\": " + raw;
+  }
+  void arr(const std::string& k, const std::vector<double>& v, std::vector<size_t> shape) {
+    size_t n = 1;
+    for (auto s : shape) n *= s;
+    if (n != v.size()) {
+      fprintf(stderr, "shape mismatch for %s\n", k.c_str());
+      exit(3);
+    }
+    index.push_back({k, shape});
+    data.insert(data.end(), v.begin(), v.end());
+  }
+  void write(const std::string& dir) const {
+    std::string js = "{\n  \"name\": \"" + name + "\",\n  \"params\": {\n    " + params + "\n  },\n  \"arrays\": [";
+    size_t off = 0;
+    for (size_t a = 0; a < index.size(); a++) {
+      size_t n = 1;
+      std::string sh;
+      for (size_t s = 0; s < index[a].second.size(); s++) {
+        n *= index[a].second[s];
+        sh += (s ? ", " : "") + std::to_string(index[a].second[s]);
+      }
+      js += std::string(a ? "," : "") + "\n    {\"name\": \"" + index[a].first + "\", \"shape\": [" + sh +
+            "], \"offset\": " + std::to_string(off) + "}";
+      off += n;
+    }
+    js += "\n  ]\n}\n";
+    FILE* f = fopen((dir + "/" + name + ".json").c_str(), "w");
+    fputs(js.c_str(), f);
+    fclose(f);
+    f = fopen((dir + "/" + name + ".bin").c_str(), "wb");
+    fwrite(data.data(), sizeof(double), data.size(), f);
+    fclose(f);
+  }
+};
+
+template <class Base>
+struct Nb : public Base {
+  Nb* mn[3] = {nullptr, nullptr, nullptr};
+  Nb* mx[3] = {nullptr, nullptr, nullptr};
+  Nb& getNeighborMin(int d) const { return *mn[d]; }
+  Nb& getNeighborMax(int d) const { return *mx[d]; }
+};
+
+typedef Nb<YeeCell<YeeFields<TM, double, std::array>, DoublePolarization<double>, VacuumMagnetization<double>,
+                   NewPML<TM, double>>>
+    CellTM;
+typedef Nb<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<double>, VacuumMagnetization<double>,
+                   NewPML<ThreeD, double>>>
+    Cell3;
+
+// material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz
+struct Mat {
+  int kind;
+  double eps, freq, gamma, delta;
+};
+
+template <class Mode, class C>
+void material_E(C& c, const Mat& m, double dt) {
+  switch (m.kind) {
+    case 0: VacuumUpdate<Mode, FieldType::Electric>()(c); break;
+    case 1: ConstantUpdate<Mode, FieldType::Electric>(m.eps)(c); break;
+    case 2: DrudeUpdate<Mode, FieldType::Electric>(m.eps, m.freq, m.gamma, dt)(c); break;
+    case 3: LorentzUpdate<Mode, FieldType::Electric>(m.eps, m.delta, m.freq, m.gamma, dt)(c); break;
+  }
+}
+
+struct PmlSpec {
+  int n;
+  double m, ma, smax, kmax, amax;
+};
+
+// PML placement idiom of tests/testTM.cpp:46-71 and include/Source.hpp:380-389:
+// for i = 1..n+1, x = (n-i+1)/n, xm = x - 0.5/n; E parameters at i and L-i,
+// H parameters at i and L-1-i.  `set(idx, electric, K, S, A)` applies to every
+// cell whose coordinate along the axis equals idx.
+template <class Setter>
+void place_pml(int L, const PmlSpec& s, Setter set) {
+  PMLParameterModel p(s.m, s.ma, s.smax, s.kmax, s.amax);
+  for (int i = 1; i < s.n + 2; i++) {
+    double x = static_cast<double>(s.n - i + 1) / static_cast<double>(s.n);
+    double xm = x - 0.5 / static_cast<double>(s.n);
+    set(i, true, p.K(x), p.S(x), p.A(x));
+    set(i, false, p.K(xm), p.S(xm), p.A(xm));
+    set(L - i, true, p.K(x), p.S(x), p.A(x));
+    set(L - 1 - i, false, p.K(xm), p.S(xm), p.A(xm));
+  }
+}
+
+static void put_pml(Fixture& fx, const PmlSpec& s) {
+  fx.pi("pml_n", s.n);
+  fx.p("pml_m", s.m);
+  fx.p("pml_ma", s.ma);
+  fx.p("pml_smax", s.smax);
+  fx.p("pml_kmax", s.kmax);
+  fx.p("pml_amax", s.amax);
+}
+
+// ------------------------------------------------------- TFSF plane iterator
+template <class C>
+struct PlaneEntry {
+  C* cell;
+  double x, y, z;
+};
+template <class C>
+struct PlaneIt {
+  PlaneEntry<C>* p;
+  C& operator*() const { return *p->cell; }
+  PlaneIt& operator++() {
+    ++p;
+    return *this;
+  }
+  PlaneIt operator++(int) {
+    PlaneIt t = *this;
+    ++p;
+    return t;
+  }
+  bool operator!=(const PlaneIt& o) const { return p != o.p; }
+  bool operator==(const PlaneIt& o) const { return p == o.p; }
+};
+struct LocTM {
+  std::array<double, 2> operator()(const PlaneIt<CellTM>& it) const { return {it.p->x, it.p->y}; }
+};
+
+// ------------------------------------------------------------ scenario: PML
+static void scen_pml_coeffs(const std::string& dir) {
+  Fixture fx;
+  fx.name = "pml_coeffs";
+  // placement idiom on a 1-D line of TM cells for three (dx, dt, n) sets
+  const double dxs[3] = {1e-6, 1.0, 1e-8};
+  const int ns[3] = {10, 12, 8};
+  const double cfls[3] = {0.5 / std::sqrt(3.0), 0.5, 0.9 / std::sqrt(2.0)};
+  const int L = 40;
+  for (int t = 0; t < 3; t++) {
+    double dx = dxs[t], dt = cfls[t] * dx / c0;
+    PMLParameterModel dflt(dx);
+    PmlSpec s{ns[t], dflt.mM, dflt.mMa, dflt.msMax, dflt.mkMax, dflt.maMax};
+    std::vector<NewPML<TM, double>> line(L);
+    place_pml(L, s, [&](int idx, bool e, double K, double S, double A) {
+      if (e)
+        line[idx].setPMLParameters<FieldType::Electric, Dir::X>(K, S, A, dt);
+      else
+        line[idx].setPMLParameters<FieldType::Magnetic, Dir::X>(K, S, A, dt);
+    });
+    std::vector<double> out;
+    for (int i = 0; i < L; i++) {
+      auto& c = line[i];
+      double v[14] = {c.pmlK<FieldType::Electric, Dir::X>(), c.pmlS<FieldType::Electric, Dir::X>(),
+                      c.pmlA<FieldType::Electric, Dir::X>(), c.pmlB<FieldType::Electric, Dir::X>(),
+                      c.pmlC<FieldType::Electric, Dir::X>(), c.pmlF<FieldType::Electric, Dir::X>(),
+                      c.pmlG<FieldType::Electric, Dir::X>(), c.pmlK<FieldType::Magnetic, Dir::X>(),
+                      c.pmlS<FieldType::Magnetic, Dir::X>(), c.pmlA<FieldType::Magnetic, Dir::X>(),
+                      c.pmlB<FieldType::Magnetic, Dir::X>(), c.pmlC<FieldType::Magnetic, Dir::X>(),
+                      c.pmlF<FieldType::Magnetic, Dir::X>(), c.pmlG<FieldType::Magnetic, Dir::X>()};
+      out.insert(out.end(), v, v + 14);
+    }
+    std::string k = "set" + std::to_string(t);
+    fx.p(k + "_dx", dx);
+    fx.p(k + "_dt", dt);
+    put_pml(fx, s);  // last one wins for the scalar keys; per-set keys below
+    fx.pi(k + "_n", s.n);
+    fx.p(k + "_smax", s.smax);
+    fx.arr(k + "_KSABCFG_EH", out, {size_t(L), 14});
+  }
+  fx.pi("L", L);
+  // model evaluations at sample x (incl. x<0 and x>1 branches)
+  PMLParameterModel pm(3.0, 1.0, 123.0, 5.0, 0.05);
+  std::vector<double> xs = {-0.25, -0.05, 0.0, 0.05, 0.1, 0.333, 0.5, 0.95, 1.0, 1.05, 2.0};
+  std::vector<double> ev;
+  for (double x : xs) {
+    ev.push_back(x);
+    ev.push_back(pm.K(x));
+    ev.push_back(pm.S(x));
+    ev.push_back(pm.A(x));
+  }
+  fx.arr("model_x_K_S_A", ev, {xs.size(), 4});
+  fx.p("imp0", imp0);
+  fx.p("eps0", eps0);
+  fx.p("mu0", mu0);
+  fx.p("c0", c0);
+  fx.write(dir);
+}
+
+// ----------------------------------------------------------- scenario: Aux1D
+static void scen_aux1d(const std::string& dir) {
+  Fixture fx;
+  fx.name = "aux1d";
+  const int ncells = 40, nsteps = 220;
+  const double dx = 1e-6, dt = 0.5 * dx / c0;
+  const double f0 = c0 / (20 * dx), tw = 2.0 / f0, t0 = 3.0 * tw;
+  std::vector<double> st, sv;
+  auto sig = [&](double t) {
+    double a = (t - t0) / tw;
+    double v = std::exp(-a * a) * std::sin(2 * pi * f0 * (t - t0));
+    st.push_back(t);
+    sv.push_back(v);
+    return v;
+  };
+  Aux1D<double> aux(ncells, dx, dt, sig);
+  const int L = ncells + 2 * 15 + 20;
+  std::vector<double> eall, hall, locs = {-20.0, -7.25, 0.0, 3.5, 11.0, 19.75};
+  std::vector<double> trace;
+  for (int n = 0; n < nsteps; n++) {
+    aux.time_step(dt);
+    for (double l : locs) {
+      trace.push_back(aux.interpolateE(l * dx));
+      trace.push_back(aux.interpolateH(l * dx));
+    }
+  }
+  // whole state read back through the interpolators at node positions
+  for (int c = 0; c + 1 < L; c++) {
+    double loc = (static_cast<double>(c) - L / 2.0) * dx;
+    eall.push_back(aux.interpolateE(loc));
+    hall.push_back(aux.interpolateH(loc + 0.5 * dx));
+  }
+  fx.pi("ncells", ncells);
+  fx.pi("L", L);
+  fx.pi("nsteps", nsteps);
+  fx.p("dx", dx);
+  fx.p("dt", dt);
+  fx.p("f0", f0);
+  fx.p("tw", tw);
+  fx.p("t0", t0);
+  fx.arr("signal_t", st, {st.size()});
+  fx.arr("signal_v", sv, {sv.size()});
+  fx.arr("locs", locs, {locs.size()});
+  fx.arr("trace_EH", trace, {size_t(nsteps), locs.size(), 2});
+  fx.arr("final_E_nodes", eall, {eall.size()});
+  fx.arr("final_H_nodes", hall, {hall.size()});
+  fx.write(dir);
+}
+
+// ------------------------------------------------------------ scenario: 2D TM
+struct TMScenario {
+  std::string name;
+  int nx, ny, nsteps;
+  double dx, dt;
+  PmlSpec pml;
+  bool pmlx, pmly;
+  std::vector<Mat> mats;
+  std::vector<int> matid;  // nx*ny
+  int src_i, src_j;
+  std::vector<double> src;  // per step, added to Dz before the PML/material sweep
+  std::vector<std::pair<int, int>> probes;
+  bool tfsf = false;
+  int bx0, bx1, by0, by1;
+  double kx, ky;
+  int aux_ncells;
+  double f0, tw, t0;
+  bool save_full = true;
+};
+
+static void run_tm(const TMScenario& S, const std::string& dir) {
+  const int nx = S.nx, ny = S.ny;
+  std::vector<CellTM> cells(size_t(nx) * ny);
+  auto at = [&](int i, int j) -> CellTM& { return cells[size_t(i) + size_t(nx) * j]; };
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      CellTM& c = at(i, j);
+      c.mn[0] = &at(i - 1, j);
+      c.mx[0] = &at(i + 1, j);
+      c.mn[1] = &at(i, j - 1);
+      c.mx[1] = &at(i, j + 1);
+    }
+  const double dt = S.dt, dx = S.dx;
+  if (S.pmlx)
+    place_pml(nx, S.pml, [&](int idx, bool e, double K, double Sg, double A) {
+      for (int j = 0; j < ny; j++) {
+        if (e)
+          at(idx, j).setPMLParameters<FieldType::Electric, Dir::X>(K, Sg, A, dt);
+        else
+          at(idx, j).setPMLParameters<FieldType::Magnetic, Dir::X>(K, Sg, A, dt);
+      }
+    });
+  if (S.pmly)
+    place_pml(ny, S.pml, [&](int idx, bool e, double K, double Sg, double A) {
+      for (int i = 0; i < nx; i++) {
+        if (e)
+          at(i, idx).setPMLParameters<FieldType::Electric, Dir::Y>(K, Sg, A, dt);
+        else
+          at(i, idx).setPMLParameters<FieldType::Magnetic, Dir::Y>(K, Sg, A, dt);
+      }
+    });
+
+  // TFSF: one plane per face of the box, each with its own Aux1D
+  // (include/Source.hpp:595-609 steps the aux inside updateD).
+  typedef TFSFPlane<TM, LocTM, PlaneIt<CellTM>, Aux1D<double>> PlaneT;
+  std::vector<std::vector<PlaneEntry<CellTM>>> faces;
+  std::vector<PlaneT> planes;
+  std::vector<double> aux_sig;
+  const double cx = std::floor(0.5 * (S.bx0 + S.bx1)) + 0.5, cy = std::floor(0.5 * (S.by0 + S.by1)) + 0.5;
+  if (S.tfsf) {
+    faces.resize(4);
+    for (int j = S.by0 + 1; j <= S.by1; j++) {
+      faces[0].push_back({&at(S.bx0, j), (S.bx0 - cx) * dx, (j - cy) * dx, 0});
+      faces[1].push_back({&at(S.bx1, j), (S.bx1 - cx) * dx, (j - cy) * dx, 0});
+    }
+    for (int i = S.bx0 + 1; i <= S.bx1; i++) {
+      faces[2].push_back({&at(i, S.by0), (i - cx) * dx, (S.by0 - cy) * dx, 0});
+      faces[3].push_back({&at(i, S.by1), (i - cx) * dx, (S.by1 - cy) * dx, 0});
+    }
+    const double f0 = S.f0, tw = S.tw, t0 = S.t0;
+    for (int fidx = 0; fidx < 4; fidx++) {
+      auto sig = [f0, tw, t0, fidx, &aux_sig](double t) {
+        double a = (t - t0) / tw;
+        double v = std::exp(-a * a) * std::sin(2 * pi * f0 * (t - t0));
+        if (fidx == 0) aux_sig.push_back(v);
+        return v;
+      };
+      auto aux = std::make_shared<Aux1D<double>>(S.aux_ncells, dx, dt, sig);
+      std::array<double, 2> K = {S.kx, S.ky};
+      PlaneIt<CellTM> b{faces[fidx].data()}, e{faces[fidx].data() + faces[fidx].size()};
+      if (fidx == 0) planes.push_back(make_tfsf_plane<TM, Dir::X, Orientation::MIN>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+      if (fidx == 1) planes.push_back(make_tfsf_plane<TM, Dir::X, Orientation::MAX>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+      if (fidx == 2) planes.push_back(make_tfsf_plane<TM, Dir::Y, Orientation::MIN>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+      if (fidx == 3) planes.push_back(make_tfsf_plane<TM, Dir::Y, Orientation::MAX>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+    }
+  }
+
+  std::vector<double> ptrace;
+  for (int n = 0; n < S.nsteps; n++) {
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TM, Dz>(dt, dx)(at(i, j));
+    if (S.src_i >= 0) at(S.src_i, S.src_j).Dz() += S.src[n];
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TM, FieldType::Electric, Dir::X>(dt, dx)(at(i, j));
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TM, FieldType::Electric, Dir::Y>(dt, dx)(at(i, j));
+    for (auto& pl : planes) pl.updateD(dt, dx);
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) material_E<TM>(at(i, j), S.mats[S.matid[size_t(i) + size_t(nx) * j]], dt);
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TM, Bx>(dt, dx)(at(i, j));
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TM, By>(dt, dx)(at(i, j));
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TM, FieldType::Magnetic, Dir::X>(dt, dx)(at(i, j));
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TM, FieldType::Magnetic, Dir::Y>(dt, dx)(at(i, j));
+    for (auto& pl : planes) pl.updateB(dt, dx);
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) VacuumUpdate<TM, FieldType::Magnetic>()(at(i, j));
+    for (auto& pr : S.probes) {
+      ptrace.push_back(at(pr.first, pr.second).Ez());
+      ptrace.push_back(at(pr.first, pr.second).Hx());
+      ptrace.push_back(at(pr.first, pr.second).Hy());
+    }
+  }
+
+  Fixture fx;
+  fx.name = S.name;
+  fx.ps("mode", "TM");
+  fx.pi("nx", nx);
+  fx.pi("ny", ny);
+  fx.pi("nsteps", S.nsteps);
+  fx.p("dx", dx);
+  fx.p("dt", dt);
+  put_pml(fx, S.pml);
+  fx.pi("pmlx", S.pmlx);
+  fx.pi("pmly", S.pmly);
+  fx.pi("src_i", S.src_i);
+  fx.pi("src_j", S.src_j);
+  fx.pi("tfsf", S.tfsf);
+  if (S.tfsf) {
+    fx.pi("bx0", S.bx0);
+    fx.pi("bx1", S.bx1);
+    fx.pi("by0", S.by0);
+    fx.pi("by1", S.by1);
+    fx.p("kx", S.kx);
+    fx.p("ky", S.ky);
+    fx.pi("aux_ncells", S.aux_ncells);
+    fx.p("cx", cx);
+    fx.p("cy", cy);
+    fx.p("f0", S.f0);
+    fx.p("tw", S.tw);
+    fx.p("t0", S.t0);
+    fx.arr("aux_signal", aux_sig, {aux_sig.size()});
+  }
+  std::vector<double> mt;
+  for (auto& m : S.mats) {
+    mt.push_back(m.kind);
+    mt.push_back(m.eps);
+    mt.push_back(m.freq);
+    mt.push_back(m.gamma);
+    mt.push_back(m.delta);
+  }
+  fx.arr("mats_kind_eps_freq_gamma_delta", mt, {S.mats.size(), 5});
+  std::vector<double> mid(S.matid.begin(), S.matid.end());
+  if (S.save_full) fx.arr("matid", mid, {size_t(ny), size_t(nx)});
+  if (S.src_i >= 0) fx.arr("src", S.src, {S.src.size()});
+  std::vector<double> pr;
+  for (auto& p : S.probes) {
+    pr.push_back(p.first);
+    pr.push_back(p.second);
+  }
+  if (!S.probes.empty()) {
+    fx.arr("probes", pr, {S.probes.size(), 2});
+    fx.arr("probe_trace_Ez_Hx_Hy", ptrace, {size_t(S.nsteps), S.probes.size(), 3});
+  }
+  std::vector<double> Ez, Dz, Hx, Hy, Bx, By;
+  for (auto& c : cells) {
+    Ez.push_back(c.Ez());
+    Dz.push_back(c.Dz());
+    Hx.push_back(c.Hx());
+    Hy.push_back(c.Hy());
+    Bx.push_back(c.Bx());
+    By.push_back(c.By());
+  }
+  std::vector<size_t> sh = {size_t(ny), size_t(nx)};
+  if (S.save_full) {
+    fx.arr("Ez", Ez, sh);
+    fx.arr("Dz", Dz, sh);
+    fx.arr("Hx", Hx, sh);
+    fx.arr("Hy", Hy, sh);
+    fx.arr("Bx", Bx, sh);
+    fx.arr("By", By, sh);
+  } else {
+    // row and column through the centre plus per-field sums of squares
+    std::vector<double> row, col, ss(6, 0.0);
+    for (int i = 0; i < nx; i++) row.push_back(at(i, ny / 2).Ez());
+    for (int j = 0; j < ny; j++) col.push_back(at(nx / 2, j).Ez());
+    for (size_t q = 0; q < cells.size(); q++) {
+      ss[0] += Ez[q] * Ez[q];
+      ss[1] += Dz[q] * Dz[q];
+      ss[2] += Hx[q] * Hx[q];
+      ss[3] += Hy[q] * Hy[q];
+      ss[4] += Bx[q] * Bx[q];
+      ss[5] += By[q] * By[q];
+    }
+    fx.arr("Ez_row_mid", row, {row.size()});
+    fx.arr("Ez_col_mid", col, {col.size()});
+    fx.arr("sumsq_Ez_Dz_Hx_Hy_Bx_By", ss, {6});
+  }
+  fx.write(dir);
+}
+
+// ------------------------------------------------------------ scenario: 3-D
+struct Scenario3 {
+  std::string name;
+  int nx, ny, nz, nsteps;
+  double dx, dt;
+  PmlSpec pml;
+  std::vector<Mat> mats;
+  std::vector<int> matid;  // nx*ny*nz
+  int si, sj, sk;
+  std::vector<double> src;
+  std::vector<std::array<int, 3>> probes;
+};
+
+static void run_3d(const Scenario3& S, const std::string& dir, bool timing_only = false, int warm = 0,
+                   double* seconds = nullptr) {
+  const int nx = S.nx, ny = S.ny, nz = S.nz;
+  std::vector<Cell3> cells(size_t(nx) * ny * nz);
+  auto id = [&](int i, int j, int k) { return size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k); };
+  auto at = [&](int i, int j, int k) -> Cell3& { return cells[id(i, j, k)]; };
+  for (int k = 1; k < nz - 1; k++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) {
+        Cell3& c = at(i, j, k);
+        c.mn[0] = &at(i - 1, j, k);
+        c.mx[0] = &at(i + 1, j, k);
+        c.mn[1] = &at(i, j - 1, k);
+        c.mx[1] = &at(i, j + 1, k);
+        c.mn[2] = &at(i, j, k - 1);
+        c.mx[2] = &at(i, j, k + 1);
+      }
+  const double dt = S.dt, dx = S.dx;
+  place_pml(nx, S.pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++) {
+        if (e)
+          at(idx, j, k).setPMLParameters<FieldType::Electric, Dir::X>(K, Sg, A, dt);
+        else
+          at(idx, j, k).setPMLParameters<FieldType::Magnetic, Dir::X>(K, Sg, A, dt);
+      }
+  });
+  place_pml(ny, S.pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int k = 0; k < nz; k++)
+      for (int i = 0; i < nx; i++) {
+        if (e)
+          at(i, idx, k).setPMLParameters<FieldType::Electric, Dir::Y>(K, Sg, A, dt);
+        else
+          at(i, idx, k).setPMLParameters<FieldType::Magnetic, Dir::Y>(K, Sg, A, dt);
+      }
+  });
+  place_pml(nz, S.pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        if (e)
+          at(i, j, idx).setPMLParameters<FieldType::Electric, Dir::Z>(K, Sg, A, dt);
+        else
+          at(i, j, idx).setPMLParameters<FieldType::Magnetic, Dir::Z>(K, Sg, A, dt);
+      }
+  });
+  std::vector<double> ptrace;
+  auto interior = [&](auto&& fn) {
+    for (int k = 1; k < nz - 1; k++)
+      for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) fn(i, j, k);
+  };
+  auto t_start = std::chrono::steady_clock::now();
+  for (int n = 0; n < S.nsteps; n++) {
+    if (timing_only && n == warm) t_start = std::chrono::steady_clock::now();
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dx>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dy>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dz>(dt, dx)(at(i, j, k)); });
+    if (S.si >= 0) at(S.si, S.sj, S.sk).Dz() += S.src[n];
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Electric, Dir::X>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Electric, Dir::Y>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Electric, Dir::Z>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { material_E<ThreeD>(at(i, j, k), S.mats[S.matid[id(i, j, k)]], dt); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Bx>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, By>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Bz>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Magnetic, Dir::X>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Magnetic, Dir::Y>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Magnetic, Dir::Z>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { VacuumUpdate<ThreeD, FieldType::Magnetic>()(at(i, j, k)); });
+    for (auto& p : S.probes) {
+      Cell3& c = at(p[0], p[1], p[2]);
+      ptrace.push_back(c.Ex());
+      ptrace.push_back(c.Ey());
+      ptrace.push_back(c.Ez());
+      ptrace.push_back(c.Hx());
+      ptrace.push_back(c.Hy());
+      ptrace.push_back(c.Hz());
+    }
+  }
+  if (timing_only) {
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return;
+  }
+  Fixture fx;
+  fx.name = S.name;
+  fx.ps("mode", "ThreeD");
+  fx.pi("nx", nx);
+  fx.pi("ny", ny);
+  fx.pi("nz", nz);
+  fx.pi("nsteps", S.nsteps);
+  fx.p("dx", dx);
+  fx.p("dt", dt);
+  put_pml(fx, S.pml);
+  fx.pi("src_i", S.si);
+  fx.pi("src_j", S.sj);
+  fx.pi("src_k", S.sk);
+  std::vector<double> mt;
+  for (auto& m : S.mats) {
+    mt.push_back(m.kind);
+    mt.push_back(m.eps);
+    mt.push_back(m.freq);
+    mt.push_back(m.gamma);
+    mt.push_back(m.delta);
+  }
+  fx.arr("mats_kind_eps_freq_gamma_delta", mt, {S.mats.size(), 5});
+  std::vector<double> mid(S.matid.begin(), S.matid.end());
+  std::vector<size_t> sh = {size_t(nz), size_t(ny), size_t(nx)};
+  fx.arr("matid", mid, sh);
+  fx.arr("src", S.src, {S.src.size()});
+  std::vector<double> pr;
+  for (auto& p : S.probes) pr.insert(pr.end(), {double(p[0]), double(p[1]), double(p[2])});
+  fx.arr("probes", pr, {S.probes.size(), 3});
+  fx.arr("probe_trace_E_H", ptrace, {size_t(S.nsteps), S.probes.size(), 6});
+  const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+  for (int f = 0; f < 6; f++) {
+    std::vector<double> v;
+    v.reserve(cells.size());
+    for (auto& c : cells) {
+      Cell3& cc = const_cast<Cell3&>(c);
+      double x = f == 0 ? cc.Ex() : f == 1 ? cc.Ey() : f == 2 ? cc.Ez() : f == 3 ? cc.Hx() : f == 4 ? cc.Hy() : cc.Hz();
+      v.push_back(x);
+    }
+    fx.arr(names[f], v, sh);
+  }
+  // dispersive state where the material is dispersive: D, P, J per cell, flattened in cell order
+  std::vector<double> disp;
+  for (size_t q = 0; q < cells.size(); q++) {
+    if (S.mats[S.matid[q]].kind >= 2) {
+      Cell3& c = cells[q];
+      disp.insert(disp.end(), {double(q), c.Dx(), c.Dy(), c.Dz(), c.Px(), c.Py(), c.Pz(), c.Jx(), c.Jy(), c.Jz()});
+    }
+  }
+  if (!disp.empty()) fx.arr("dispersive_q_D_P_J", disp, {disp.size() / 10, 10});
+  fx.write(dir);
+}
+
+// seeded axis-aligned blocks (later blocks override earlier ones)
+static void seed_blocks(std::vector<int>& matid, int nx, int ny, int nz, int nblocks, int lo, int hi, int side_lo,
+                        int side_hi, SplitMix64& rng, std::vector<Mat>& mats, double eps_lo, double eps_hi,
+                        std::vector<double>* boxes_out) {
+  for (int b = 0; b < nblocks; b++) {
+    int sx = rng.randint(side_lo, side_hi), sy = rng.randint(side_lo, side_hi), sz = rng.randint(side_lo, side_hi);
+    int x0 = rng.randint(lo, std::max(lo, nx - hi - sx)), y0 = rng.randint(lo, std::max(lo, ny - hi - sy));
+    int z0 = nz > 1 ? rng.randint(lo, std::max(lo, nz - hi - sz)) : 0;
+    double eps = eps_lo + (eps_hi - eps_lo) * rng.uniform();
+    mats.push_back({1, eps, 0, 0, 0});
+    int m = int(mats.size()) - 1;
+    int x1 = std::min(nx - hi, x0 + sx), y1 = std::min(ny - hi, y0 + sy), z1 = nz > 1 ? std::min(nz - hi, z0 + sz) : 1;
+    for (int k = z0; k < z1; k++)
+      for (int j = y0; j < y1; j++)
+        for (int i = x0; i < x1; i++) matid[size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k)] = m;
+    if (boxes_out) boxes_out->insert(boxes_out->end(), {double(x0), double(x1), double(y0), double(y1), double(z0), double(z1), eps});
+  }
+}
+
+static double gauss_pulse(int n, double t0, double spread) {
+  double a = (n - t0) / spread;
+  return std::exp(-a * a);
+}
+
+int main(int argc, char** argv) {
+  if (argc >= 2 && !strcmp(argv[1], "bench3d")) {
+    // timing of the reference's own functors on a bounded sample of the 3-D
+    // CPML + dielectric-block workload: N^3 cells, `steps` timed steps after
+    // one untimed step.
+    int N = argc > 2 ? atoi(argv[2]) : 256;
+    int steps = argc > 3 ? atoi(argv[3]) : 2;
+    int npml = argc > 4 ? atoi(argv[4]) : 12;
+    Scenario3 S;
+    S.name = "bench";
+    S.nx = S.ny = S.nz = N;
+    S.nsteps = steps + 1;
+    S.dx = 1e-6;
+    S.dt = 0.5 * S.dx / (c0 * std::sqrt(3.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {npml, 3, 1, dm.msMax, 5, 0.05};
+    S.mats = {{0, 1, 0, 0, 0}};
+    S.matid.assign(size_t(N) * N * N, 0);
+    SplitMix64 rng(2);
+    seed_blocks(S.matid, N, N, N, 48, npml + 2, npml + 2, std::max(2, N / 32), std::max(3, N / 4), rng, S.mats, 1, 12,
+                nullptr);
+    S.si = S.sj = S.sk = N / 2;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 40, 12));
+    double secs = 0;
+    run_3d(S, "", true, 1, &secs);
+    double cells = double(N - 2) * (N - 2) * (N - 2) * steps;
+    printf("{\"N\": %d, \"steps\": %d, \"seconds\": %.6f, \"cell_updates_per_s\": %.6e}\n", N, steps, secs, cells / secs);
+    return 0;
+  }
+  if (argc < 2) {
+    fprintf(stderr, "usage: harness <outdir> | harness bench3d N steps [npml]\n");
+    return 2;
+  }
+  std::string dir = argv[1];
+  scen_pml_coeffs(dir);
+  scen_aux1d(dir);
+
+  // 2D TM, small, non-square, CPML x+y, two dielectric blocks, one Drude and
+  // one Lorentz block, Gaussian soft source on Dz.
+  {
+    TMScenario S;
+    S.name = "tm2d_small";
+    S.nx = 48;
+    S.ny = 40;
+    S.nsteps = 150;
+    S.dx = 1e-6;
+    S.dt = 0.9 * S.dx / (c0 * std::sqrt(2.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {10, 3, 1, dm.msMax, 5, 0.05};
+    S.pmlx = S.pmly = true;
+    S.mats = {{0, 1, 0, 0, 0}, {1, 4.0, 0, 0, 0}, {1, 2.25, 0, 0, 0}};
+    // Drude and Lorentz with parameters resolved at this dt
+    S.mats.push_back({2, 1.5, 0.3 / S.dt, 0.02 / S.dt, 0});
+    S.mats.push_back({3, 2.0, 0.2 / S.dt, 0.01 / S.dt, 0.7});
+    S.matid.assign(size_t(S.nx) * S.ny, 0);
+    auto fill = [&](int x0, int x1, int y0, int y1, int m) {
+      for (int j = y0; j < y1; j++)
+        for (int i = x0; i < x1; i++) S.matid[size_t(i) + size_t(S.nx) * j] = m;
+    };
+    fill(14, 22, 13, 19, 1);
+    fill(27, 33, 22, 29, 2);
+    fill(15, 20, 24, 30, 3);
+    fill(28, 34, 12, 17, 4);
+    S.src_i = 24;
+    S.src_j = 20;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 25, 8));
+    S.probes = {{24, 20}, {17, 15}, {30, 25}, {12, 30}, {3, 3}};
+    run_tm(S, dir);
+  }
+  // 2D TM at the 256^2 / 1000-step shape: dx = 1 m, c0 dt/dx = 0.5, 10-cell
+  // CPML on all edges, vacuum, Dz[128,128] += eps0 exp(-((n-40)/12)^2),
+  // 4 seeded probes in the non-PML interior.
+  {
+    TMScenario S;
+    S.name = "tm2d_c1";
+    S.nx = S.ny = 256;
+    S.nsteps = 1000;
+    S.dx = 1.0;
+    S.dt = 0.5 * S.dx / c0;
+    PMLParameterModel dm(S.dx);
+    S.pml = {10, 3, 1, dm.msMax, 5, 0.05};
+    S.pmlx = S.pmly = true;
+    S.mats = {{0, 1, 0, 0, 0}};
+    S.matid.assign(size_t(S.nx) * S.ny, 0);
+    S.src_i = S.src_j = 128;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 40, 12));
+    SplitMix64 rng(1);
+    for (int p = 0; p < 4; p++) S.probes.push_back({rng.randint(12, 243), rng.randint(12, 243)});
+    S.save_full = false;
+    run_tm(S, dir);
+  }
+  // 2D TM TFSF box (inset 14) around a dielectric block, x-propagating and
+  // oblique plane waves.
+  for (int variant = 0; variant < 2; variant++) {
+    TMScenario S;
+    S.name = variant == 0 ? "tm2d_tfsf_x" : "tm2d_tfsf_oblique";
+    S.nx = 64;
+    S.ny = 60;
+    S.nsteps = 160;
+    S.dx = 1e-6;
+    S.dt = 0.5 * S.dx / (c0 * std::sqrt(2.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {10, 3, 1, dm.msMax, 5, 0.05};
+    S.pmlx = S.pmly = true;
+    S.mats = {{0, 1, 0, 0, 0}, {1, 3.0, 0, 0, 0}};
+    S.matid.assign(size_t(S.nx) * S.ny, 0);
+    for (int j = 26; j < 33; j++)
+      for (int i = 28; i < 36; i++) S.matid[size_t(i) + size_t(S.nx) * j] = variant;  // block only in variant 1
+    S.src_i = -1;
+    S.src_j = -1;
+    S.tfsf = true;
+    S.bx0 = 14;
+    S.bx1 = 49;
+    S.by0 = 14;
+    S.by1 = 45;
+    S.kx = variant == 0 ? 1.0 : std::cos(pi / 6);
+    S.ky = variant == 0 ? 0.0 : std::sin(pi / 6);
+    S.aux_ncells = 100;
+    S.f0 = c0 / (20 * S.dx);
+    S.tw = 2.0 / S.f0;
+    S.t0 = 3.0 * S.tw;
+    S.probes = {{30, 30}, {5 + 10, 30}, {14, 30}, {13, 30}, {49, 20}, {50, 20}, {31, 14}, {31, 13}};
+    run_tm(S, dir);
+  }
+  // 3-D, small and anisotropic, CPML on all faces, two dielectric blocks,
+  // Gaussian soft source on Dz.
+  {
+    Scenario3 S;
+    S.name = "grid3d_small";
+    S.nx = 22;
+    S.ny = 19;
+    S.nz = 17;
+    S.nsteps = 50;
+    S.dx = 1e-6;
+    S.dt = 0.5 * S.dx / (c0 * std::sqrt(3.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {4, 3, 1, dm.msMax, 5, 0.05};
+    S.mats = {{0, 1, 0, 0, 0}};
+    S.matid.assign(size_t(S.nx) * S.ny * S.nz, 0);
+    SplitMix64 rng(7);
+    std::vector<double> boxes;
+    seed_blocks(S.matid, S.nx, S.ny, S.nz, 3, 6, 6, 2, 5, rng, S.mats, 1, 12, &boxes);
+    S.si = 11;
+    S.sj = 9;
+    S.sk = 8;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 15, 5));
+    S.probes = {{11, 9, 8}, {7, 7, 7}, {15, 12, 10}, {2, 3, 4}, {19, 16, 14}};
+    run_3d(S, dir);
+  }
+  // 3-D with single-pole Drude and Lorentz blocks (the reference holds one
+  // P/J pair per cell, include/DispersiveMaterials.hpp:72-83).
+  {
+    Scenario3 S;
+    S.name = "grid3d_disp";
+    S.nx = 18;
+    S.ny = 17;
+    S.nz = 16;
+    S.nsteps = 60;
+    S.dx = 1e-8;
+    S.dt = 0.5 * S.dx / (c0 * std::sqrt(3.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {4, 3, 1, dm.msMax, 5, 0.05};
+    S.mats = {{0, 1, 0, 0, 0}, {2, 1.0, 1.37e16, 1e14, 0}, {3, 1.5, 4e15, 1e14, 1.5}, {1, 2.5, 0, 0, 0}};
+    S.matid.assign(size_t(S.nx) * S.ny * S.nz, 0);
+    auto fill = [&](int x0, int x1, int y0, int y1, int z0, int z1, int m) {
+      for (int k = z0; k < z1; k++)
+        for (int j = y0; j < y1; j++)
+          for (int i = x0; i < x1; i++) S.matid[size_t(i) + size_t(S.nx) * (size_t(j) + size_t(S.ny) * k)] = m;
+    };
+    fill(6, 10, 6, 11, 6, 9, 1);
+    fill(10, 13, 5, 9, 8, 11, 2);
+    fill(6, 9, 9, 12, 9, 11, 3);
+    S.si = 9;
+    S.sj = 8;
+    S.sk = 8;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 12, 4));
+    S.probes = {{9, 8, 8}, {7, 8, 7}, {11, 7, 9}, {7, 10, 10}};
+    run_3d(S, dir);
+  }
+  // 0-D material recursions on one TM cell: prescribed D_n, reference functors.
+  {
+    Fixture fx;
+    fx.name = "cell_ade";
+    const double dt = 1e-17;
+    std::vector<Mat> ms = {{2, 1.0, 1.37e16, 1e14, 0}, {3, 2.0, 3e15, 1e14, 0.8}, {2, 3.0, 5e15, 2e15, 0},
+                           {3, 1.0, 6e15, 0, 1.5}, {1, 7.5, 0, 0, 0}};
+    std::vector<double> out, mt;
+    const int nsteps = 300;
+    for (auto& m : ms) {
+      CellTM c;
+      for (int n = 0; n < nsteps; n++) {
+        c.Dz() = eps0 * (std::sin(0.07 * n) + 0.3 * std::cos(0.011 * n * n));
+        material_E<TM>(c, m, dt);
+        out.insert(out.end(), {c.Dz(), c.Ez(), c.Pz(), c.Jz()});
+      }
+      mt.insert(mt.end(), {double(m.kind), m.eps, m.freq, m.gamma, m.delta});
+    }
+    fx.p("dt", dt);
+    fx.pi("nsteps", nsteps);
+    fx.arr("mats_kind_eps_freq_gamma_delta", mt, {ms.size(), 5});
+    fx.arr("trace_D_E_P_J", out, {ms.size(), size_t(nsteps), 4});
+    fx.write(dir);
+  }
+  printf("fixtures written to %s\n", dir.c_str());
+  return 0;
+}

@@ -1,0 +1,797 @@
+/* refcpu.c — FP64 CPU restatement of the reference hot path (THE ORACLE).
+ *
+ * Test infrastructure only: loaded by tests/, the smoke check and bench.py's
+ * cpu_baseline leg as the checker.  The product (fdtd_amd/) never links it.
+ *
+ * Every update keeps the reference's state (D/B fluxes, E/H derived by the
+ * material step), its sweep order and its floating-point operation order, so
+ * that results agree with the reference's own templates to the last bits
+ * (compiled with -ffp-contract=off).  File:line citations point into
+ * pederson/fdtd.
+ */
+#include "refcpu.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* include/FDTDConstants.hpp:394-398 */
+static const double RC_PI = 3.14159265358979323846264338327950288;
+static const double RC_EPS0 = 8.854e-12;
+static const double RC_C0 = 2.99792458e+8;
+static double rc_mu0_v(void) { return RC_PI * 4.0e-7; }
+/* constexpr Newton-Raphson square root, FDTDConstants.hpp:136-156 */
+static double rc_newton_sqrt(double x) {
+  double curr = x, prev = 0;
+  while (curr != prev) {
+    double nxt = 0.5 * (curr + x / curr);
+    prev = curr;
+    curr = nxt;
+  }
+  return curr;
+}
+double rc_eps0(void) { return RC_EPS0; }
+double rc_mu0(void) { return rc_mu0_v(); }
+double rc_c0(void) { return RC_C0; }
+double rc_imp0(void) { return rc_newton_sqrt(rc_mu0_v() / RC_EPS0); }
+
+/* Levi-Civita for distinct axes (FDTDConstants.hpp:602-637) */
+static int lc(int I, int J, int K) {
+  if ((I == 0 && J == 1 && K == 2) || (I == 1 && J == 2 && K == 0) || (I == 2 && J == 0 && K == 1)) return 1;
+  return -1;
+}
+/* d1 = next(I), d2 = the remaining axis (YeeUpdates.hpp:397-398) */
+static const int D1[3] = {1, 2, 0};
+static const int D2[3] = {2, 0, 1};
+static int third(int I, int J) { return 3 - I - J; }
+
+/* ---------------------------------------------------------------- PML */
+/* NewPML::setPMLParameters, include/PML.hpp:85-102 */
+void rc_pml_coeffs(double K, double S, double A, double dt, double out[4]) {
+  double nu = 1.0 / RC_EPS0 * (S / K + A);
+  double B = exp(-dt * nu);
+  double C = (nu == 0) ? 0.0 : S / K * 1.0 / (S + K * A) * (B - 1.0);
+  double u = -S / (RC_EPS0 * K * K);
+  double F = (nu == 0) ? 1.0 / K : 1.0 / K - 1.0 / dt * (1.0 - B - nu * dt) * u / (nu * nu);
+  double G = (nu == 0) ? 0.0 : 1.0 / dt * (1.0 - B) / nu;
+  out[0] = B;
+  out[1] = C;
+  out[2] = F;
+  out[3] = G;
+}
+
+/* PMLParameterModel::K/S/A, include/PML.hpp:378-391 */
+void rc_pml_model(double x, double m, double ma, double smax, double kmax, double amax, double out[3]) {
+  if (x < 0.0) {
+    out[0] = 1.0;
+    out[1] = 0.0;
+    out[2] = 0.0;
+    return;
+  }
+  out[0] = (x > 1.0 ? 1.0 : 1.0 + pow(x, m) * (kmax - 1.0));
+  out[1] = (x > 1.0 ? 0.0 : pow(x, m) * smax);
+  out[2] = (x > 1.0 ? 0.0 : pow(1.0 - x, ma) * amax);
+}
+
+static void set_row(double* row, double K, double S, double A, double dt) {
+  double bcfg[4];
+  rc_pml_coeffs(K, S, A, dt, bcfg);
+  row[0] = K;
+  row[1] = S;
+  row[2] = A;
+  row[3] = bcfg[0];
+  row[4] = bcfg[1];
+  row[5] = bcfg[2];
+  row[6] = bcfg[3];
+}
+
+/* placement idiom: tests/testTM.cpp:46-71, include/Source.hpp:380-389 */
+void rc_pml_profile(int L, int n, double m, double ma, double smax, double kmax, double amax, double dt,
+                    double* outE, double* outH) {
+  for (int i = 0; i < L; i++) {
+    double d[7] = {1, 0, 0, 0, 0, 1, 0}; /* NewPML() defaults, PML.hpp:40-55 */
+    memcpy(outE + 7 * i, d, sizeof d);
+    memcpy(outH + 7 * i, d, sizeof d);
+  }
+  for (int i = 1; i < n + 2; i++) {
+    double x = (double)(n - i + 1) / (double)n;
+    double xm = x - 0.5 / (double)n;
+    double e[3], h[3];
+    rc_pml_model(x, m, ma, smax, kmax, amax, e);
+    rc_pml_model(xm, m, ma, smax, kmax, amax, h);
+    set_row(outE + 7 * i, e[0], e[1], e[2], dt);
+    set_row(outH + 7 * i, h[0], h[1], h[2], dt);
+    set_row(outE + 7 * (L - i), e[0], e[1], e[2], dt);
+    set_row(outH + 7 * (L - 1 - i), h[0], h[1], h[2], dt);
+  }
+}
+
+/* ---------------------------------------------------------------- Aux1D */
+struct rc_aux {
+  int L;
+  double dx, dt, t;
+  double *Dz, *Ez, *By, *Hy, *psiE, *psiH;
+  double *pE, *pH; /* [L][7] */
+};
+
+rc_aux* rc_aux_create(int ncells, double dx, double dt) {
+  rc_aux* a = (rc_aux*)calloc(1, sizeof(rc_aux));
+  const int npml = 15; /* Source.hpp:362 */
+  a->L = ncells + 2 * npml + 20; /* Source.hpp:404 */
+  a->dx = dx;
+  a->dt = dt;
+  int L = a->L;
+  a->Dz = calloc(L, sizeof(double));
+  a->Ez = calloc(L, sizeof(double));
+  a->By = calloc(L, sizeof(double));
+  a->Hy = calloc(L, sizeof(double));
+  a->psiE = calloc(L, sizeof(double));
+  a->psiH = calloc(L, sizeof(double));
+  a->pE = malloc(sizeof(double) * 7 * L);
+  a->pH = malloc(sizeof(double) * 7 * L);
+  /* PMLParameterModel(dx) defaults, PML.hpp:372-373 */
+  double m = 3, smax = 0.8 * (m + 1) / (rc_imp0() * dx);
+  rc_pml_profile(L, npml, m, 1, smax, 5, 0.05, dt, a->pE, a->pH);
+  return a;
+}
+
+void rc_aux_destroy(rc_aux* a) {
+  if (!a) return;
+  free(a->Dz);
+  free(a->Ez);
+  free(a->By);
+  free(a->Hy);
+  free(a->psiE);
+  free(a->psiH);
+  free(a->pE);
+  free(a->pH);
+  free(a);
+}
+
+int rc_aux_length(const rc_aux* a) { return a->L; }
+
+/* Aux1D::time_step, Source.hpp:413-435.  Ghost neighbours outside [0,L) hold zero. */
+void rc_aux_step(rc_aux* a, double source_value) {
+  const int L = a->L;
+  const double dt = a->dt, dx = a->dx;
+  const int npml = 15;
+  a->t += dt;
+  /* YeeUpdate<TEM,Dz>: Dz += dt*(F^E_x/dx*(Hy - Hy[-1]) + 0) */
+  for (int n = 0; n < L; n++) {
+    double hm = n > 0 ? a->Hy[n - 1] : 0.0;
+    double ax = 1 * a->pE[7 * n + 5] / (dx) * (a->Hy[n] - hm);
+    a->Dz[n] = a->Dz[n] + 1.0 * 1.0 * dt * (ax + 0.0);
+  }
+  /* UpdatePML<TEM,Electric,X> on [0,npml+2) and [L-(npml+3),L) */
+  for (int pass = 0; pass < 2; pass++) {
+    int b = pass == 0 ? 0 : L - (npml + 3), e = pass == 0 ? npml + 2 : L;
+    for (int n = b; n < e; n++) {
+      double hm = n > 0 ? a->Hy[n - 1] : 0.0;
+      a->Dz[n] += 1.0 * dt * a->psiE[n] * a->pE[7 * n + 6];
+      a->psiE[n] = a->pE[7 * n + 3] * a->psiE[n] + a->pE[7 * n + 4] * 1 / dx * (a->Hy[n] - hm);
+    }
+  }
+  for (int n = 0; n < L; n++) a->Ez[n] = a->Dz[n] / (1.0 * RC_EPS0);
+  a->Ez[npml + 5] += source_value; /* Source.hpp:425 */
+  /* YeeUpdate<TEM,By>: By += -dt*(0 + (-F^H_x/dx)*(Ez[+1]-Ez)) */
+  for (int n = 0; n < L; n++) {
+    double ep = n + 1 < L ? a->Ez[n + 1] : 0.0;
+    double ax = -1 * a->pH[7 * n + 5] / (dx) * (ep - a->Ez[n]);
+    a->By[n] = a->By[n] + -1.0 * 1.0 * dt * (0.0 + ax);
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    int b = pass == 0 ? 0 : L - (npml + 3), e = pass == 0 ? npml + 2 : L;
+    for (int n = b; n < e; n++) {
+      double ep = n + 1 < L ? a->Ez[n + 1] : 0.0;
+      a->By[n] += -1.0 * dt * a->psiH[n] * a->pH[7 * n + 6];
+      a->psiH[n] = a->pH[7 * n + 3] * a->psiH[n] + a->pH[7 * n + 4] * -1 / dx * (ep - a->Ez[n]);
+    }
+  }
+  const double mu0 = rc_mu0_v();
+  for (int n = 0; n < L; n++) a->Hy[n] = a->By[n] / (1.0 * mu0);
+}
+
+/* Aux1D::interpolateE / interpolateH, Source.hpp:442-458 */
+double rc_aux_interp_E(const rc_aux* a, double loc) {
+  double nd = (double)a->L / 2.0 + loc / a->dx;
+  size_t n = (size_t)nd;
+  double frac = (nd - n);
+  return (1.0 - frac) * a->Ez[n] + frac * a->Ez[n + 1];
+}
+double rc_aux_interp_H(const rc_aux* a, double loc) {
+  double nd = (double)a->L / 2.0 + loc / a->dx - 0.5;
+  size_t n = (size_t)nd;
+  double frac = (nd - n);
+  return (1.0 - frac) * a->Hy[n] + frac * a->Hy[n + 1];
+}
+void rc_aux_state(const rc_aux* a, double* Ez, double* Hy) {
+  memcpy(Ez, a->Ez, sizeof(double) * a->L);
+  memcpy(Hy, a->Hy, sizeof(double) * a->L);
+}
+
+/* ------------------------------------------------------------ grid sim */
+typedef struct {
+  double eps, mu;
+  int np;
+  int kind[RC_MAXPOLES];
+  double freq[RC_MAXPOLES], gamma[RC_MAXPOLES], weight[RC_MAXPOLES];
+} rc_mat;
+
+typedef struct {
+  int field, comp;
+  size_t cell;
+  double* table;
+  int n;
+} rc_src;
+
+typedef struct {
+  int region[6];
+  int mask, nfreq, ncomp, nphase;
+  size_t ncells;
+  double *phE, *phH, *acc;
+} rc_dft;
+
+typedef struct {
+  int field, comp;
+  size_t cell;
+} rc_probe;
+
+struct rc_sim {
+  int mode, nx, ny, nz;
+  size_t ncell;
+  double dx, dt;
+  int actE[3], actH[3], axes[3];
+  double *D[3], *E[3], *B[3], *H[3];
+  double* psi[2][3][3];
+  double* pml[2][3]; /* [ft][axis] -> L*4 = B,C,F,G */
+  uint8_t* matid;
+  rc_mat mat[RC_MAXMAT];
+  double *P[RC_MAXPOLES][3], *J[RC_MAXPOLES][3];
+  int npoles_alloc;
+  rc_src* src;
+  int nsrc;
+  int tfsf;
+  int box[6];
+  double kvec[3], pol[3], polH[3], origin[3];
+  rc_aux* aux;
+  double* signal;
+  int nsignal;
+  rc_dft* dft;
+  int ndft;
+  rc_probe* probes;
+  int nprobes;
+  double* ptrace;
+  int ptrace_cap;
+  long step;
+};
+
+static size_t cidx(const rc_sim* s, int i, int j, int k) {
+  return (size_t)i + (size_t)s->nx * ((size_t)j + (size_t)s->ny * (size_t)k);
+}
+
+rc_sim* rc_create(int mode, int nx, int ny, int nz, double dx, double dt) {
+  rc_sim* s = (rc_sim*)calloc(1, sizeof(rc_sim));
+  s->mode = mode;
+  s->nx = nx;
+  s->ny = mode == RC_TEM ? 1 : ny;
+  s->nz = mode == RC_3D ? nz : 1;
+  s->ncell = (size_t)s->nx * s->ny * s->nz;
+  s->dx = dx;
+  s->dt = dt;
+  int aE[3] = {0, 0, 0}, aH[3] = {0, 0, 0}, ax[3] = {1, 0, 0};
+  if (mode == RC_3D) {
+    aE[0] = aE[1] = aE[2] = aH[0] = aH[1] = aH[2] = 1;
+    ax[1] = ax[2] = 1;
+  } else if (mode == RC_TM) {
+    aE[2] = aH[0] = aH[1] = 1;
+    ax[1] = 1;
+  } else if (mode == RC_TE) {
+    aE[0] = aE[1] = aH[2] = 1;
+    ax[1] = 1;
+  } else {
+    aE[2] = aH[1] = 1;
+  }
+  for (int c = 0; c < 3; c++) {
+    s->actE[c] = aE[c];
+    s->actH[c] = aH[c];
+    s->axes[c] = ax[c];
+    if (aE[c]) {
+      s->D[c] = calloc(s->ncell, sizeof(double));
+      s->E[c] = calloc(s->ncell, sizeof(double));
+    }
+    if (aH[c]) {
+      s->B[c] = calloc(s->ncell, sizeof(double));
+      s->H[c] = calloc(s->ncell, sizeof(double));
+    }
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < 3; J++)
+        if ((ft == 0 ? aE[I] : aH[I]) && I != J && ax[J]) s->psi[ft][I][J] = calloc(s->ncell, sizeof(double));
+  int len[3] = {s->nx, s->ny, s->nz};
+  for (int ft = 0; ft < 2; ft++)
+    for (int a = 0; a < 3; a++) {
+      s->pml[ft][a] = malloc(sizeof(double) * 4 * len[a]);
+      for (int i = 0; i < len[a]; i++) {
+        s->pml[ft][a][4 * i + 0] = 0.0; /* NewPML() defaults B=0,C=0,F=1,G=0 */
+        s->pml[ft][a][4 * i + 1] = 0.0;
+        s->pml[ft][a][4 * i + 2] = 1.0;
+        s->pml[ft][a][4 * i + 3] = 0.0;
+      }
+    }
+  s->matid = calloc(s->ncell, 1);
+  for (int m = 0; m < RC_MAXMAT; m++) {
+    s->mat[m].eps = 1.0;
+    s->mat[m].mu = 1.0;
+  }
+  return s;
+}
+
+void rc_destroy(rc_sim* s) {
+  if (!s) return;
+  for (int c = 0; c < 3; c++) {
+    free(s->D[c]);
+    free(s->E[c]);
+    free(s->B[c]);
+    free(s->H[c]);
+    for (int p = 0; p < RC_MAXPOLES; p++) {
+      free(s->P[p][c]);
+      free(s->J[p][c]);
+    }
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++) {
+      s->pml[ft][I] ? free(s->pml[ft][I]) : (void)0;
+      for (int J = 0; J < 3; J++) free(s->psi[ft][I][J]);
+    }
+  free(s->matid);
+  for (int q = 0; q < s->nsrc; q++) free(s->src[q].table);
+  free(s->src);
+  rc_aux_destroy(s->aux);
+  free(s->signal);
+  for (int q = 0; q < s->ndft; q++) {
+    free(s->dft[q].phE);
+    free(s->dft[q].phH);
+    free(s->dft[q].acc);
+  }
+  free(s->dft);
+  free(s->probes);
+  free(s->ptrace);
+  free(s);
+}
+
+int rc_set_pml_axis(rc_sim* s, int axis, int ft, const double* coeffs) {
+  int len[3] = {s->nx, s->ny, s->nz};
+  if (axis < 0 || axis > 2 || ft < 0 || ft > 1) return -1;
+  memcpy(s->pml[ft][axis], coeffs, sizeof(double) * 4 * len[axis]);
+  return 0;
+}
+
+int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, const double* poles) {
+  if (id < 0 || id >= RC_MAXMAT || npoles < 0 || npoles > RC_MAXPOLES) return -1;
+  rc_mat* m = &s->mat[id];
+  m->eps = eps_r;
+  m->mu = mu_r;
+  m->np = npoles;
+  for (int p = 0; p < npoles; p++) {
+    m->kind[p] = (int)poles[4 * p];
+    m->freq[p] = poles[4 * p + 1];
+    m->gamma[p] = poles[4 * p + 2];
+    m->weight[p] = poles[4 * p + 3];
+  }
+  if (npoles > s->npoles_alloc) {
+    for (int p = s->npoles_alloc; p < npoles; p++)
+      for (int c = 0; c < 3; c++)
+        if (s->actE[c]) {
+          s->P[p][c] = calloc(s->ncell, sizeof(double));
+          s->J[p][c] = calloc(s->ncell, sizeof(double));
+        }
+    s->npoles_alloc = npoles;
+  }
+  return 0;
+}
+
+int rc_set_matid(rc_sim* s, const uint8_t* ids) {
+  memcpy(s->matid, ids, s->ncell);
+  return 0;
+}
+
+int rc_add_source(rc_sim* s, int field, int comp, int i, int j, int k, const double* table, int ntable) {
+  if (field < 0 || field > 1 || comp < 0 || comp > 2) return -1;
+  if (!(field == 0 ? s->actE[comp] : s->actH[comp])) return -1;
+  s->src = realloc(s->src, sizeof(rc_src) * (s->nsrc + 1));
+  rc_src* q = &s->src[s->nsrc++];
+  q->field = field;
+  q->comp = comp;
+  q->cell = cidx(s, i, j, k);
+  q->n = ntable;
+  q->table = malloc(sizeof(double) * ntable);
+  memcpy(q->table, table, sizeof(double) * ntable);
+  return 0;
+}
+
+int rc_set_tfsf(rc_sim* s, const int box[6], const double kvec[3], const double pol[3], const double polH[3],
+                const double origin[3], int aux_ncells, const double* signal, int nsignal) {
+  s->tfsf = 1;
+  memcpy(s->box, box, sizeof(int) * 6);
+  /* TFSFPlane normalises the wave vector, Source.hpp:566-570 */
+  int nd = s->mode == RC_3D ? 3 : (s->mode == RC_TEM ? 1 : 2);
+  double K = 0;
+  for (int d = 0; d < nd; d++) K += kvec[d] * kvec[d];
+  K = sqrt(K);
+  for (int d = 0; d < 3; d++) s->kvec[d] = d < nd ? kvec[d] / K : 0.0;
+  memcpy(s->pol, pol, sizeof(double) * 3);
+  memcpy(s->polH, polH, sizeof(double) * 3);
+  memcpy(s->origin, origin, sizeof(double) * 3);
+  rc_aux_destroy(s->aux);
+  s->aux = rc_aux_create(aux_ncells, s->dx, s->dt);
+  free(s->signal);
+  s->signal = malloc(sizeof(double) * nsignal);
+  memcpy(s->signal, signal, sizeof(double) * nsignal);
+  s->nsignal = nsignal;
+  return 0;
+}
+
+int rc_add_dft(rc_sim* s, const int region[6], int comp_mask, int nfreq, const double* phaseE, const double* phaseH,
+               int nphase) {
+  s->dft = realloc(s->dft, sizeof(rc_dft) * (s->ndft + 1));
+  rc_dft* d = &s->dft[s->ndft++];
+  memcpy(d->region, region, sizeof(int) * 6);
+  d->mask = comp_mask;
+  d->nfreq = nfreq;
+  d->nphase = nphase;
+  d->ncomp = 0;
+  for (int c = 0; c < 6; c++) d->ncomp += (comp_mask >> c) & 1;
+  d->ncells = (size_t)(region[1] - region[0]) * (region[3] - region[2]) * (region[5] - region[4]);
+  d->phE = malloc(sizeof(double) * 2 * nfreq * nphase);
+  d->phH = malloc(sizeof(double) * 2 * nfreq * nphase);
+  memcpy(d->phE, phaseE, sizeof(double) * 2 * nfreq * nphase);
+  memcpy(d->phH, phaseH, sizeof(double) * 2 * nfreq * nphase);
+  d->acc = calloc((size_t)d->ncomp * nfreq * 2 * d->ncells, sizeof(double));
+  return s->ndft - 1;
+}
+
+int rc_get_dft(const rc_sim* s, int idx, double* out) {
+  if (idx < 0 || idx >= s->ndft) return -1;
+  const rc_dft* d = &s->dft[idx];
+  memcpy(out, d->acc, sizeof(double) * (size_t)d->ncomp * d->nfreq * 2 * d->ncells);
+  return 0;
+}
+
+int rc_add_probe(rc_sim* s, int field, int comp, int i, int j, int k) {
+  s->probes = realloc(s->probes, sizeof(rc_probe) * (s->nprobes + 1));
+  rc_probe* p = &s->probes[s->nprobes++];
+  p->field = field;
+  p->comp = comp;
+  p->cell = cidx(s, i, j, k);
+  return 0;
+}
+
+int rc_get_probes(const rc_sim* s, double* out, int max_steps) {
+  long n = s->step < max_steps ? s->step : max_steps;
+  if (s->ptrace) memcpy(out, s->ptrace, sizeof(double) * n * s->nprobes);
+  return (int)n;
+}
+
+static double* field_ptr(const rc_sim* s, int field, int comp, int pole) {
+  switch (field) {
+    case 0: return s->D[comp];
+    case 1: return s->E[comp];
+    case 2: return s->B[comp];
+    case 3: return s->H[comp];
+    case 4: return pole < s->npoles_alloc ? s->P[pole][comp] : NULL;
+    case 5: return pole < s->npoles_alloc ? s->J[pole][comp] : NULL;
+  }
+  return NULL;
+}
+
+int rc_get_field(const rc_sim* s, int field, int comp, int pole, double* out) {
+  double* p = (comp >= 0 && comp < 3) ? field_ptr(s, field, comp, pole) : NULL;
+  if (!p) return -1;
+  memcpy(out, p, sizeof(double) * s->ncell);
+  return 0;
+}
+
+int rc_set_field(rc_sim* s, int field, int comp, int pole, const double* in) {
+  double* p = (comp >= 0 && comp < 3) ? field_ptr(s, field, comp, pole) : NULL;
+  if (!p) return -1;
+  memcpy(p, in, sizeof(double) * s->ncell);
+  return 0;
+}
+
+int rc_nthreads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+/* interior index ranges: 1..N-2 on every axis of the mode (testTM.cpp:103) */
+static void interior(const rc_sim* s, int lo[3], int hi[3]) {
+  int len[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; a++) {
+    lo[a] = s->axes[a] ? 1 : 0;
+    hi[a] = s->axes[a] ? len[a] - 1 : 1;
+  }
+}
+
+static size_t stride(const rc_sim* s, int a) { return a == 0 ? 1 : a == 1 ? (size_t)s->nx : (size_t)s->nx * s->ny; }
+
+/* Tangential TFSF range of a component (offset 0 or -1/2 along axis T):
+ * the total-field region along T is the open interval (lo-1/2, hi). */
+static void tf_range(int lo, int hi, int half, int* a, int* b) {
+  if (half) {
+    *a = lo + 1;
+    *b = hi;
+  } else {
+    *a = lo;
+    *b = hi - 1;
+  }
+}
+
+/* one TFSF face correction pass; ft 0 = updateD (uses aux H, polH), 1 = updateB (aux E, pol).
+ * The per-cell arithmetic is TFSFPlane::field_update, Source.hpp:506-553. */
+static void tfsf_faces(rc_sim* s, int ft) {
+  const double dt = s->dt, dx = s->dx;
+  const int* box = s->box;
+  for (int J = 0; J < 3; J++) {
+    if (!s->axes[J]) continue;
+    for (int side = 0; side < 2; side++) {
+      const int layer = box[2 * J + side];
+      const double nrm = side == 0 ? -1.0 : 1.0;
+      for (int I = 0; I < 3; I++) {
+        if (I == J) continue;
+        if (!(ft == 0 ? s->actE[I] : s->actH[I])) continue;
+        const int K = third(I, J);
+        if (!(ft == 0 ? s->actH[K] : s->actE[K])) continue;
+        /* ranges along the two tangential axes */
+        int lo[3], hi[3];
+        for (int T = 0; T < 3; T++) {
+          if (T == J) {
+            lo[T] = layer;
+            hi[T] = layer;
+            continue;
+          }
+          if (!s->axes[T]) {
+            lo[T] = 0;
+            hi[T] = 0;
+            continue;
+          }
+          /* flux offsets: D_I is -1/2 on axes != I, B_I is -1/2 on axis I */
+          int half = ft == 0 ? (T != I) : (T == I);
+          tf_range(box[2 * T], box[2 * T + 1], half, &lo[T], &hi[T]);
+        }
+        double* F = ft == 0 ? s->D[I] : s->B[I];
+        const double* pml = s->pml[ft][J];
+        const double* polv = ft == 0 ? s->polH : s->pol;
+        const int e = lc(I, J, K);
+        for (int k = lo[2]; k <= hi[2]; k++)
+          for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++) {
+              int ci[3] = {i, j, k};
+              /* curl-field location: cell centre + offset of the K component */
+              double src_loc[3];
+              double proj = 0;
+              for (int dd = 0; dd < 3; dd++) {
+                if (!s->axes[dd]) continue;
+                double off;
+                if (ft == 0)
+                  off = (dd == K) ? -0.5 : 0.0; /* H_K offsets */
+                else
+                  off = (dd != K) ? -0.5 : 0.0; /* E_K offsets */
+                src_loc[dd] = (ci[dd] - s->origin[dd]) * dx;
+                src_loc[dd] += off * dx;
+                proj += s->kvec[dd] * src_loc[dd];
+              }
+              double srcval = ft == 0 ? rc_aux_interp_H(s->aux, proj) : rc_aux_interp_E(s->aux, proj);
+              double term = e * pml[4 * ci[J] + 2] / (dx)*nrm * srcval * polv[K];
+              size_t c = cidx(s, i, j, k);
+              double coeff = ft == 0 ? 1.0 : -1.0;
+              F[c] += coeff * 1.0 * dt * (term + 0.0);
+            }
+      }
+    }
+  }
+}
+
+int rc_step(rc_sim* s, int nsteps) {
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const double dt = s->dt, dx = s->dx;
+  const double mu0 = rc_mu0_v();
+  for (int it = 0; it < nsteps; it++) {
+    const long n = s->step;
+    /* ---- D curl, YeeUpdate<Mode,D_I> (YeeUpdates.hpp:395-405) */
+    for (int I = 0; I < 3; I++) {
+      if (!s->actE[I]) continue;
+      const int j1 = D1[I], j2 = D2[I];
+      const int k1 = third(I, j1), k2 = third(I, j2);
+      const int h1 = s->actH[k1] && s->axes[j1], h2 = s->actH[k2] && s->axes[j2];
+      const size_t s1 = stride(s, j1), s2 = stride(s, j2);
+      double* Df = s->D[I];
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            size_t c = cidx(s, i, j, k);
+            int ci[3] = {i, j, k};
+            double a1 = 0.0, a2 = 0.0;
+            if (h1) a1 = lc(I, j1, k1) * s->pml[0][j1][4 * ci[j1] + 2] / (dx) * (s->H[k1][c] - s->H[k1][c - s1]);
+            if (h2) a2 = lc(I, j2, k2) * s->pml[0][j2][4 * ci[j2] + 2] / (dx) * (s->H[k2][c] - s->H[k2][c - s2]);
+            Df[c] = Df[c] + 1.0 * 1.0 * dt * (a1 + a2);
+          }
+    }
+    for (int q = 0; q < s->nsrc; q++)
+      if (s->src[q].field == 0 && n < s->src[q].n) s->D[s->src[q].comp][s->src[q].cell] += s->src[q].table[n];
+    /* ---- UpdatePML<Mode,Electric,J> (PML.hpp:315-323), J = x, y, z */
+    for (int J = 0; J < 3; J++) {
+      if (!s->axes[J]) continue;
+      const size_t sJ = stride(s, J);
+      const double* pm = s->pml[0][J];
+      for (int I = 2; I >= 0; I--) {
+        if (I == J || !s->actE[I]) continue;
+        const int K = third(I, J);
+        double* ps = s->psi[0][I][J];
+        double* Df = s->D[I];
+        const double* Hk = s->H[K];
+        const int e = lc(I, J, K);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = lo[2]; k < hi[2]; k++)
+          for (int j = lo[1]; j < hi[1]; j++)
+            for (int i = lo[0]; i < hi[0]; i++) {
+              size_t c = cidx(s, i, j, k);
+              int ci[3] = {i, j, k};
+              const double* p = pm + 4 * ci[J];
+              Df[c] += 1.0 * dt * ps[c] * p[3];
+              ps[c] = p[0] * ps[c] + p[1] * e / dx * (Hk[c] - Hk[c - sJ]);
+            }
+      }
+    }
+    /* ---- TFSF updateD, then the aux line advances (Source.hpp:595-609) */
+    if (s->tfsf) {
+      tfsf_faces(s, 0);
+      rc_aux_step(s->aux, n < s->nsignal ? s->signal[n] : 0.0);
+    }
+    /* ---- E material (DispersiveMaterials.hpp:354-359, 710-735, 979-998) */
+    {
+      const int np = s->npoles_alloc;
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            size_t c = cidx(s, i, j, k);
+            const rc_mat* m = &s->mat[s->matid[c]];
+            for (int I = 2; I >= 0; I--) {
+              if (!s->actE[I]) continue;
+              if (m->np == 0 || np == 0) {
+                s->E[I][c] = s->D[I][c] / (m->eps * RC_EPS0);
+                continue;
+              }
+              for (int p = 0; p < m->np; p++) s->P[p][I][c] += dt * s->J[p][I][c];
+              double num = s->D[I][c];
+              for (int p = 0; p < m->np; p++) num = num - s->P[p][I][c];
+              double Ev = num / (m->eps * RC_EPS0);
+              s->E[I][c] = Ev;
+              for (int p = 0; p < m->np; p++) {
+                double* Jp = &s->J[p][I][c];
+                double* Pp = &s->P[p][I][c];
+                const double wgt_eps0 = m->weight[p] * RC_EPS0;
+                if (m->kind[p] == 1) {
+                  double b = m->gamma[p] * dt * 0.5;
+                  double w = dt * m->freq[p];
+                  *Jp = *Jp * (1.0 - b) / (1.0 + b) + w * w / dt * wgt_eps0 * Ev;
+                } else {
+                  double b = 1.0 + m->gamma[p] * dt;
+                  double bm = 1.0 - m->gamma[p] * dt;
+                  double w = m->freq[p] * dt;
+                  *Jp = bm / b * *Jp + m->freq[p] * w / b * wgt_eps0 * Ev - m->freq[p] * w / b * *Pp;
+                }
+              }
+            }
+          }
+    }
+    /* ---- B curl */
+    for (int I = 0; I < 3; I++) {
+      if (!s->actH[I]) continue;
+      const int j1 = D1[I], j2 = D2[I];
+      const int k1 = third(I, j1), k2 = third(I, j2);
+      const int e1 = s->actE[k1] && s->axes[j1], e2 = s->actE[k2] && s->axes[j2];
+      const size_t s1 = stride(s, j1), s2 = stride(s, j2);
+      double* Bf = s->B[I];
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            size_t c = cidx(s, i, j, k);
+            int ci[3] = {i, j, k};
+            double a1 = 0.0, a2 = 0.0;
+            if (e1) a1 = lc(I, j1, k1) * s->pml[1][j1][4 * ci[j1] + 2] / (dx) * (s->E[k1][c + s1] - s->E[k1][c]);
+            if (e2) a2 = lc(I, j2, k2) * s->pml[1][j2][4 * ci[j2] + 2] / (dx) * (s->E[k2][c + s2] - s->E[k2][c]);
+            Bf[c] = Bf[c] + -1.0 * 1.0 * dt * (a1 + a2);
+          }
+    }
+    for (int q = 0; q < s->nsrc; q++)
+      if (s->src[q].field == 1 && n < s->src[q].n) s->B[s->src[q].comp][s->src[q].cell] += s->src[q].table[n];
+    /* ---- UpdatePML<Mode,Magnetic,J> */
+    for (int J = 0; J < 3; J++) {
+      if (!s->axes[J]) continue;
+      const size_t sJ = stride(s, J);
+      const double* pm = s->pml[1][J];
+      for (int I = 2; I >= 0; I--) {
+        if (I == J || !s->actH[I]) continue;
+        const int K = third(I, J);
+        double* ps = s->psi[1][I][J];
+        double* Bf = s->B[I];
+        const double* Ek = s->E[K];
+        const int e = lc(I, J, K);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = lo[2]; k < hi[2]; k++)
+          for (int j = lo[1]; j < hi[1]; j++)
+            for (int i = lo[0]; i < hi[0]; i++) {
+              size_t c = cidx(s, i, j, k);
+              int ci[3] = {i, j, k};
+              const double* p = pm + 4 * ci[J];
+              Bf[c] += -1.0 * dt * ps[c] * p[3];
+              ps[c] = p[0] * ps[c] + p[1] * e / dx * (Ek[c + sJ] - Ek[c]);
+            }
+      }
+    }
+    if (s->tfsf) tfsf_faces(s, 1);
+    /* ---- H material: H = B/(mu_r mu0) */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = lo[2]; k < hi[2]; k++)
+      for (int j = lo[1]; j < hi[1]; j++)
+        for (int i = lo[0]; i < hi[0]; i++) {
+          size_t c = cidx(s, i, j, k);
+          const rc_mat* m = &s->mat[s->matid[c]];
+          for (int I = 2; I >= 0; I--)
+            if (s->actH[I]) s->H[I][c] = s->B[I][c] / (m->mu * mu0);
+        }
+    /* ---- running DFT (build-defined): F += f(t) e^{-i w t} dt, E at (n+1)dt, H at (n+1.5)dt */
+    for (int q = 0; q < s->ndft; q++) {
+      rc_dft* d = &s->dft[q];
+      if (n >= d->nphase) continue;
+      const int* r = d->region;
+      int slot = 0;
+      for (int comp = 0; comp < 6; comp++) {
+        if (!((d->mask >> comp) & 1)) continue;
+        const double* fld = comp < 3 ? s->E[comp] : s->H[comp - 3];
+        const double* ph = (comp < 3 ? d->phE : d->phH) + 2 * (size_t)d->nfreq * n;
+        for (int f = 0; f < d->nfreq; f++) {
+          double* re = d->acc + ((size_t)slot * d->nfreq + f) * 2 * d->ncells;
+          double* im = re + d->ncells;
+          size_t q2 = 0;
+          for (int k = r[4]; k < r[5]; k++)
+            for (int j = r[2]; j < r[3]; j++)
+              for (int i = r[0]; i < r[1]; i++, q2++) {
+                double v = fld ? fld[cidx(s, i, j, k)] : 0.0;
+                re[q2] += v * ph[2 * f];
+                im[q2] += v * ph[2 * f + 1];
+              }
+        }
+        slot++;
+      }
+    }
+    /* ---- probes */
+    if (s->nprobes) {
+      if (s->step + 1 > s->ptrace_cap) {
+        int cap = s->ptrace_cap ? 2 * s->ptrace_cap : 1024;
+        while (cap < s->step + 1) cap *= 2;
+        s->ptrace = realloc(s->ptrace, sizeof(double) * cap * s->nprobes);
+        s->ptrace_cap = cap;
+      }
+      for (int p = 0; p < s->nprobes; p++) {
+        const double* f = field_ptr(s, s->probes[p].field, s->probes[p].comp, 0);
+        s->ptrace[(size_t)s->step * s->nprobes + p] = f ? f[s->probes[p].cell] : 0.0;
+      }
+    }
+    s->step++;
+  }
+  return 0;
+}

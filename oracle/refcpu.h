@@ -1,0 +1,92 @@
+/* refcpu — FP64 CPU restatement of the reference's per-time-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Only tests/, the smoke check and the
+ * CPU-baseline leg of bench.py load this library; the product path never does.
+ *
+ * It restates, cell by cell and in the reference's own operation order:
+ *   YeeUpdate curl           include/YeeUpdates.hpp:343-405
+ *   UpdatePML (psi + flux)   include/PML.hpp:296-355
+ *   NewPML coefficients      include/PML.hpp:84-102
+ *   PMLParameterModel        include/PML.hpp:368-393
+ *   Constant/Vacuum update   include/DispersiveMaterials.hpp:354-359
+ *   Drude implicit update    include/DispersiveMaterials.hpp:979-998
+ *   Lorentz implicit update  include/DispersiveMaterials.hpp:710-735
+ *   Aux1D                    include/Source.hpp:280-458
+ *   TFSFPlane updateD/B      include/Source.hpp:489-619
+ *   step composition         tests/testTM.cpp:102-113, tests/test3D.cpp:146-171
+ * plus the build-defined pieces the reference lacks (running DFT, multi-pole
+ * dispersion with a Lorentz weight, per-component TFSF face ranges).
+ */
+#ifndef FDTD_REFCPU_H
+#define FDTD_REFCPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RC_MAXPOLES 3
+#define RC_MAXMAT 256
+
+/* modes: which components exist (FDTDConstants.hpp:424-445, 844-874) */
+enum { RC_TEM = 1, RC_TM = 2, RC_3D = 3, RC_TE = 4 };
+
+typedef struct rc_sim rc_sim;
+typedef struct rc_aux rc_aux;
+
+/* constants of include/FDTDConstants.hpp:394-398 */
+double rc_eps0(void);
+double rc_mu0(void);
+double rc_c0(void);
+double rc_imp0(void);
+
+/* NewPML::setPMLParameters closed form: out = {B, C, F, G} */
+void rc_pml_coeffs(double K, double S, double A, double dt, double out[4]);
+/* PMLParameterModel::K/S/A at x: out = {K, S, A} */
+void rc_pml_model(double x, double m, double ma, double smax, double kmax, double amax, double out[3]);
+/* placement idiom along one axis of length L: outE/outH are [L][7] = K,S,A,B,C,F,G
+ * (defaults K=1,S=A=B=C=G=0,F=1 where not placed) */
+void rc_pml_profile(int L, int n, double m, double ma, double smax, double kmax, double amax, double dt,
+                    double* outE, double* outH);
+
+/* Aux1D<double> (TEM line, 15-cell CPML each end, source cell 20) */
+rc_aux* rc_aux_create(int ncells, double dx, double dt);
+void rc_aux_destroy(rc_aux* a);
+int rc_aux_length(const rc_aux* a);
+void rc_aux_step(rc_aux* a, double source_value); /* one Aux1D::time_step */
+double rc_aux_interp_E(const rc_aux* a, double loc);
+double rc_aux_interp_H(const rc_aux* a, double loc);
+void rc_aux_state(const rc_aux* a, double* Ez, double* Hy); /* L values each */
+
+/* grid simulation */
+rc_sim* rc_create(int mode, int nx, int ny, int nz, double dx, double dt);
+void rc_destroy(rc_sim* s);
+/* per-axis, per-index PML coefficients; coeffs is [L][4] = B,C,F,G; ft 0=E 1=H */
+int rc_set_pml_axis(rc_sim* s, int axis, int ft, const double* coeffs);
+/* material table: eps_r, mu_r, npoles, then per pole {kind(1 Drude,2 Lorentz), freq, gamma, weight} */
+int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, const double* poles);
+int rc_set_matid(rc_sim* s, const uint8_t* ids); /* nx*ny*nz, x fastest */
+/* soft source: adds table[n] to D_comp (field 0) or B_comp (field 1) at cell after the curl of step n */
+int rc_add_source(rc_sim* s, int field, int comp, int i, int j, int k, const double* table, int ntable);
+/* TFSF box; faces at x0,x1 (etc); kvec/pol/polH 3-vectors; origin (cell units); aux line of ncells;
+ * signal[n] is the aux source value at its n-th step */
+int rc_set_tfsf(rc_sim* s, const int box[6], const double kvec[3], const double pol[3], const double polH[3],
+                const double origin[3], int aux_ncells, const double* signal, int nsignal);
+/* running DFT over a box region; comps bitmask (bit c: E_c for c<3, H_{c-3}); phases [nsteps][nfreq][2]
+ * for E and for H (already multiplied by dt) */
+int rc_add_dft(rc_sim* s, const int region[6], int comp_mask, int nfreq, const double* phaseE,
+               const double* phaseH, int nphase);
+int rc_get_dft(const rc_sim* s, int idx, double* out); /* [ncomp][nfreq][2][ncells] */
+int rc_add_probe(rc_sim* s, int field, int comp, int i, int j, int k);
+int rc_get_probes(const rc_sim* s, double* out, int max_steps); /* [steps][nprobes] */
+
+/* field: 0 D, 1 E, 2 B, 3 H, 4 P(pole p), 5 J(pole p) ; comp 0..2 */
+int rc_get_field(const rc_sim* s, int field, int comp, int pole, double* out);
+int rc_set_field(rc_sim* s, int field, int comp, int pole, const double* in);
+int rc_step(rc_sim* s, int nsteps);
+int rc_nthreads(int n); /* set OpenMP threads (0 = leave); returns the count in use */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,215 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/refcpu.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` as the checker.  The product package
+(``fdtd_amd``) never imports this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+MODES = {"TEM": 1, "TM": 2, "ThreeD": 3, "TE": 4}
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "librefcpu.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library missing: run `make -C oracle refcpu`")
+        L = ctypes.CDLL(path)
+        for name, res, args in [
+            ("rc_eps0", _d, []), ("rc_mu0", _d, []), ("rc_c0", _d, []), ("rc_imp0", _d, []),
+            ("rc_pml_coeffs", None, [_d, _d, _d, _d, _dp]),
+            ("rc_pml_model", None, [_d, _d, _d, _d, _d, _d, _dp]),
+            ("rc_pml_profile", None, [_i, _i, _d, _d, _d, _d, _d, _d, _dp, _dp]),
+            ("rc_aux_create", _p, [_i, _d, _d]), ("rc_aux_destroy", None, [_p]),
+            ("rc_aux_length", _i, [_p]), ("rc_aux_step", None, [_p, _d]),
+            ("rc_aux_interp_E", _d, [_p, _d]), ("rc_aux_interp_H", _d, [_p, _d]),
+            ("rc_aux_state", None, [_p, _dp, _dp]),
+            ("rc_create", _p, [_i, _i, _i, _i, _d, _d]), ("rc_destroy", None, [_p]),
+            ("rc_set_pml_axis", _i, [_p, _i, _i, _dp]),
+            ("rc_set_material", _i, [_p, _i, _d, _d, _i, _dp]),
+            ("rc_set_matid", _i, [_p, ctypes.c_char_p]),
+            ("rc_add_source", _i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
+            ("rc_set_tfsf", _i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
+            ("rc_add_dft", _i, [_p, _ip, _i, _i, _dp, _dp, _i]),
+            ("rc_get_dft", _i, [_p, _i, _dp]),
+            ("rc_add_probe", _i, [_p, _i, _i, _i, _i, _i]),
+            ("rc_get_probes", _i, [_p, _dp, _i]),
+            ("rc_get_field", _i, [_p, _i, _i, _i, _dp]),
+            ("rc_set_field", _i, [_p, _i, _i, _i, _dp]),
+            ("rc_step", _i, [_p, _i]), ("rc_nthreads", _i, [_i]),
+        ]:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def constants():
+    L = lib()
+    return {"eps0": L.rc_eps0(), "mu0": L.rc_mu0(), "c0": L.rc_c0(), "imp0": L.rc_imp0()}
+
+
+def pml_coeffs(K, S, A, dt):
+    out = np.zeros(4)
+    lib().rc_pml_coeffs(K, S, A, dt, _dptr(out))
+    return out
+
+
+def pml_model(x, m, ma, smax, kmax, amax):
+    out = np.zeros(3)
+    lib().rc_pml_model(x, m, ma, smax, kmax, amax, _dptr(out))
+    return out
+
+
+def pml_profile(L, n, m, ma, smax, kmax, amax, dt):
+    """[L,7] arrays (K,S,A,B,C,F,G) for E and H of the placement idiom."""
+    e = np.zeros((L, 7))
+    h = np.zeros((L, 7))
+    lib().rc_pml_profile(L, n, m, ma, smax, kmax, amax, dt, _dptr(e), _dptr(h))
+    return e, h
+
+
+class Aux:
+    def __init__(self, ncells, dx, dt):
+        self.h = lib().rc_aux_create(ncells, dx, dt)
+        self.L = lib().rc_aux_length(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rc_aux_destroy(self.h)
+            self.h = None
+
+    def step(self, s):
+        lib().rc_aux_step(self.h, s)
+
+    def interp_E(self, loc):
+        return lib().rc_aux_interp_E(self.h, loc)
+
+    def interp_H(self, loc):
+        return lib().rc_aux_interp_H(self.h, loc)
+
+    def state(self):
+        e = np.zeros(self.L)
+        h = np.zeros(self.L)
+        lib().rc_aux_state(self.h, _dptr(e), _dptr(h))
+        return e, h
+
+
+class Sim:
+    """FP64 oracle grid.  Field codes: 0 D, 1 E, 2 B, 3 H, 4 P, 5 J."""
+
+    def __init__(self, mode, nx, ny=1, nz=1, dx=1.0, dt=1.0):
+        self.mode = mode
+        self.shape = (nz if mode == "ThreeD" else 1, ny if mode != "TEM" else 1, nx)
+        self.h = lib().rc_create(MODES[mode], nx, ny, nz, dx, dt)
+        self.dft_meta = []
+        self._dt = dt
+        self._dx = dx
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rc_destroy(self.h)
+            self.h = None
+
+    def set_pml_axis(self, axis, ft, coeffs):
+        c = _f64(coeffs)
+        assert lib().rc_set_pml_axis(self.h, axis, ft, _dptr(c)) == 0
+
+    def set_pml_idiom(self, axis, n, m, ma, smax, kmax, amax):
+        """PML placement idiom (tests/testTM.cpp:46-71) along one axis."""
+        L = (self.shape[2], self.shape[1], self.shape[0])[axis]
+        e, h = pml_profile(L, n, m, ma, smax, kmax, amax, self._dt)
+        self.set_pml_axis(axis, 0, e[:, 3:7])
+        self.set_pml_axis(axis, 1, h[:, 3:7])
+
+    def set_material(self, mid, eps_r, mu_r=1.0, poles=()):
+        arr = _f64(np.array(poles, dtype=np.float64).reshape(-1)) if len(poles) else np.zeros(1)
+        assert lib().rc_set_material(self.h, mid, eps_r, mu_r, len(poles), _dptr(arr)) == 0
+
+    def set_matid(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.uint8).reshape(-1)
+        assert ids.size == int(np.prod(self.shape))
+        assert lib().rc_set_matid(self.h, ids.tobytes()) == 0
+
+    def add_source(self, field, comp, i, j, k, table):
+        t = _f64(table)
+        assert lib().rc_add_source(self.h, field, comp, i, j, k, _dptr(t), t.size) == 0
+
+    def set_tfsf(self, box, kvec, pol, polH, origin, aux_ncells, signal):
+        b = (ctypes.c_int * 6)(*box)
+        k, p, ph, o, s = (_f64(x) for x in (kvec, pol, polH, origin, signal))
+        assert lib().rc_set_tfsf(self.h, b, _dptr(k), _dptr(p), _dptr(ph), _dptr(o), aux_ncells,
+                                 _dptr(s), s.size) == 0
+
+    def add_dft(self, region, comp_mask, phaseE, phaseH):
+        pe, ph = _f64(phaseE), _f64(phaseH)
+        nphase, nfreq = pe.shape[0], pe.shape[1]
+        r = (ctypes.c_int * 6)(*region)
+        idx = lib().rc_add_dft(self.h, r, comp_mask, nfreq, _dptr(pe), _dptr(ph), nphase)
+        ncomp = bin(comp_mask).count("1")
+        ncell = (region[1] - region[0]) * (region[3] - region[2]) * (region[5] - region[4])
+        self.dft_meta.append((ncomp, nfreq, ncell, region))
+        return idx
+
+    def get_dft(self, idx):
+        ncomp, nfreq, ncell, region = self.dft_meta[idx]
+        out = np.zeros(ncomp * nfreq * 2 * ncell)
+        assert lib().rc_get_dft(self.h, idx, _dptr(out)) == 0
+        shp = (region[5] - region[4], region[3] - region[2], region[1] - region[0])
+        return out.reshape(ncomp, nfreq, 2, *shp)
+
+    def add_probe(self, field, comp, i, j, k=0):
+        lib().rc_add_probe(self.h, field, comp, i, j, k)
+        self._nprobes = getattr(self, "_nprobes", 0) + 1
+
+    def probes(self, max_steps=1 << 20):
+        out = np.zeros(max_steps * self._nprobes) if max_steps < (1 << 20) else None
+        if out is None:
+            out = np.zeros(self._steps_done() * self._nprobes)
+            max_steps = self._steps_done()
+        n = lib().rc_get_probes(self.h, _dptr(out), max_steps)
+        return out[: n * self._nprobes].reshape(n, self._nprobes)
+
+    def _steps_done(self):
+        return getattr(self, "_nstep", 0)
+
+    def get(self, field, comp, pole=0):
+        out = np.zeros(int(np.prod(self.shape)))
+        if lib().rc_get_field(self.h, field, comp, pole, _dptr(out)) != 0:
+            return None
+        return out.reshape(self.shape)
+
+    def set(self, field, comp, arr, pole=0):
+        a = _f64(arr).reshape(-1)
+        assert lib().rc_set_field(self.h, field, comp, pole, _dptr(a)) == 0
+
+    def step(self, n=1):
+        assert lib().rc_step(self.h, n) == 0
+        self._nstep = self._steps_done() + n
+
+
+def nthreads(n=0):
+    return lib().rc_nthreads(n)
+

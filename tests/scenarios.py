@@ -1,0 +1,156 @@
+"""Scenario descriptions shared by the oracle tests and the GPU parity tests.
+
+A scenario is a plain dict.  ``apply(sim, sc)`` drives any object exposing the
+common set-up methods (the oracle's ``Sim`` and the product's ``Grid`` both
+do): set_pml_idiom, set_material, set_matid, add_source, set_tfsf, add_probe,
+add_dft.
+"""
+import math
+
+import numpy as np
+
+from golden_io import load
+
+EPS0 = 8.854e-12
+C0 = 2.99792458e8
+MU0 = math.pi * 4.0e-7
+IMP0 = math.sqrt(MU0 / EPS0)
+
+# field codes used by both sims
+D, E, B, H, P, J = 0, 1, 2, 3, 4, 5
+
+
+class SplitMix64:
+    """Same generator as the fixture harness (oracle/ref_harness/harness.cpp)."""
+
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+    def randint(self, lo, hi):
+        return lo + int(self.next() % (hi - lo + 1))
+
+
+def ref_material(kind, eps, freq, gamma, delta):
+    """Map a reference material (kind 0 vacuum, 1 constant, 2 Drude, 3 Lorentz) to
+    (eps_r, mu_r, poles).  The reference's Lorentz update never reads its delta
+    (include/DispersiveMaterials.hpp:711-735), so its pole weight is 1."""
+    kind = int(kind)
+    if kind == 0:
+        return (1.0, 1.0, [])
+    if kind == 1:
+        return (eps, 1.0, [])
+    if kind == 2:
+        return (eps, 1.0, [(1, freq, gamma, 1.0)])
+    return (eps, 1.0, [(2, freq, gamma, 1.0)])
+
+
+def from_golden(name):
+    p, a = load(name)
+    mode = p["mode"]
+    nx, ny = int(p["nx"]), int(p["ny"])
+    nz = int(p.get("nz", 1))
+    sc = {
+        "name": name, "mode": mode, "nx": nx, "ny": ny, "nz": nz,
+        "dx": p["dx"], "dt": p["dt"], "nsteps": int(p["nsteps"]),
+        "pml": dict(n=int(p["pml_n"]), m=p["pml_m"], ma=p["pml_ma"], smax=p["pml_smax"],
+                    kmax=p["pml_kmax"], amax=p["pml_amax"]),
+        "pml_axes": [bool(p.get("pmlx", 1)), bool(p.get("pmly", 1)), mode == "ThreeD"],
+        "materials": [ref_material(*row) for row in a["mats_kind_eps_freq_gamma_delta"]],
+        "sources": [], "probes": [], "dfts": [], "tfsf": None,
+    }
+    shape = (nz if mode == "ThreeD" else 1, ny, nx)
+    if "matid" in a:
+        sc["matid"] = a["matid"].reshape(shape).astype(np.uint8)
+    else:
+        sc["matid"] = np.zeros(shape, np.uint8)
+    if int(p.get("src_i", -1)) >= 0:
+        sc["sources"].append((D, 2, int(p["src_i"]), int(p["src_j"]), int(p.get("src_k", 0)), a["src"]))
+    if "probes" in a:
+        for pr in a["probes"]:
+            ijk = [int(v) for v in pr] + [0] * (3 - len(pr))
+            if mode == "ThreeD":
+                for f, c in [(E, 0), (E, 1), (E, 2), (H, 0), (H, 1), (H, 2)]:
+                    sc["probes"].append((f, c, *ijk))
+            else:
+                for f, c in [(E, 2), (H, 0), (H, 1)]:
+                    sc["probes"].append((f, c, *ijk))
+    if int(p.get("tfsf", 0)):
+        kx, ky = p["kx"], p["ky"]
+        sc["tfsf"] = dict(
+            box=[int(p["bx0"]), int(p["bx1"]), int(p["by0"]), int(p["by1"]), 0, 0],
+            kvec=[kx, ky, 0.0],
+            # the reference's TM defaults, include/Source.hpp:577-580
+            pol=[0.0, 0.0, 1.0], polH=None,
+            origin=[p["cx"], p["cy"], 0.0], aux_ncells=int(p["aux_ncells"]), signal=a["aux_signal"])
+        k = np.array([kx, ky])
+        k = k / math.sqrt(k[0] * k[0] + k[1] * k[1])
+        sc["tfsf"]["polH"] = [k[1], -k[0], 0.0]
+    return sc, p, a
+
+
+def apply(sim, sc):
+    pm = sc["pml"]
+    for ax, on in enumerate(sc["pml_axes"]):
+        if on:
+            sim.set_pml_idiom(ax, pm["n"], pm["m"], pm["ma"], pm["smax"], pm["kmax"], pm["amax"])
+    for mid, (eps, mu, poles) in enumerate(sc["materials"]):
+        sim.set_material(mid, eps, mu, poles)
+    sim.set_matid(sc["matid"])
+    for s in sc["sources"]:
+        sim.add_source(*s)
+    if sc.get("tfsf"):
+        t = sc["tfsf"]
+        sim.set_tfsf(t["box"], t["kvec"], t["pol"], t["polH"], t["origin"], t["aux_ncells"], t["signal"])
+    for pr in sc["probes"]:
+        sim.add_probe(*pr)
+    for d in sc.get("dfts", []):
+        sim.add_dft(*d)
+    return sim
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.linalg.norm(b.reshape(-1))
+    num = np.linalg.norm((a - b).reshape(-1))
+    if den == 0:
+        return num
+    return num / den
+
+
+def gauss_table(nsteps, t0, spread, scale=EPS0):
+    n = np.arange(nsteps, dtype=np.float64)
+    return scale * np.exp(-((n - t0) / spread) ** 2)
+
+
+def gmsin_signal(nsteps, dt, f0, tw, t0):
+    """Gaussian-modulated sinusoid sampled at the aux line's accumulated time
+    (t += dt before each use, include/Source.hpp:414)."""
+    out = np.zeros(nsteps)
+    t = 0.0
+    for n in range(nsteps):
+        t += dt
+        a = (t - t0) / tw
+        out[n] = math.exp(-a * a) * math.sin(2 * math.pi * f0 * (t - t0))
+    return out
+
+
+def dft_phases(nsteps, dt, freqs):
+    """[nsteps][nfreq][2] phase tables (cos, -sin)(w t) * dt for E (t=(n+1)dt) and H (t=(n+1.5)dt)."""
+    n = np.arange(nsteps, dtype=np.float64)[:, None]
+    w = 2 * np.pi * np.asarray(freqs, dtype=np.float64)[None, :]
+    tE = (n + 1.0) * dt
+    tH = (n + 1.5) * dt
+    pe = np.stack([np.cos(w * tE) * dt, -np.sin(w * tE) * dt], axis=-1)
+    ph = np.stack([np.cos(w * tH) * dt, -np.sin(w * tH) * dt], axis=-1)
+    return pe, ph

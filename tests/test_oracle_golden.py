@@ -1,0 +1,101 @@
+"""The FP64 oracle (oracle/refcpu.c) against golden vectors written by the
+reference's own templates (tests/golden/, made by oracle/ref_harness)."""
+import numpy as np
+import pytest
+
+from golden_io import load
+from oracle import refcpu
+from scenarios import D, E, B, H, apply, from_golden, rel_l2
+
+
+def test_constants():
+    p, _ = load("pml_coeffs")
+    c = refcpu.constants()
+    for k in ("eps0", "mu0", "c0", "imp0"):
+        assert c[k] == p[k], k
+
+
+def test_pml_coefficients_and_model():
+    p, a = load("pml_coeffs")
+    L = int(p["L"])
+    for t in range(3):
+        dx, dt, n, smax = p[f"set{t}_dx"], p[f"set{t}_dt"], int(p[f"set{t}_n"]), p[f"set{t}_smax"]
+        e, h = refcpu.pml_profile(L, n, 3.0, 1.0, smax, 5.0, 0.05, dt)
+        ref = a[f"set{t}_KSABCFG_EH"]
+        np.testing.assert_array_equal(e, ref[:, :7])
+        np.testing.assert_array_equal(h, ref[:, 7:])
+    for x, K, S, A in a["model_x_K_S_A"]:
+        np.testing.assert_array_equal(refcpu.pml_model(x, 3.0, 1.0, 123.0, 5.0, 0.05), [K, S, A])
+
+
+def test_aux1d_matches_reference():
+    p, a = load("aux1d")
+    aux = refcpu.Aux(int(p["ncells"]), p["dx"], p["dt"])
+    assert aux.L == int(p["L"])
+    locs = a["locs"]
+    tr = np.zeros((int(p["nsteps"]), len(locs), 2))
+    for n in range(int(p["nsteps"])):
+        aux.step(a["signal_v"][n])
+        for q, l in enumerate(locs):
+            tr[n, q] = aux.interp_E(l * p["dx"]), aux.interp_H(l * p["dx"])
+    np.testing.assert_array_equal(tr, a["trace_EH"])
+
+
+def test_cell_ade_recursions():
+    p, a = load("cell_ade")
+    dt = p["dt"]
+    mats = a["mats_kind_eps_freq_gamma_delta"]
+    tr = a["trace_D_E_P_J"]
+    from scenarios import ref_material
+    for m, row in enumerate(mats):
+        eps, mu, poles = ref_material(*row)
+        sim = refcpu.Sim("TM", 3, 3, 1, 1.0, dt)
+        sim.set_material(0, eps, mu, poles)
+        out = []
+        for n in range(tr.shape[1]):
+            d = np.zeros((1, 3, 3))
+            d[0, 1, 1] = tr[m, n, 0]
+            sim.set(D, 2, d)
+            for c in (0, 1):  # zero magnetic state: the D curl adds exactly 0, only the material acts
+                sim.set(B, c, np.zeros((1, 3, 3)))
+                sim.set(H, c, np.zeros((1, 3, 3)))
+            sim.step(1)
+            e = sim.get(E, 2)[0, 1, 1]
+            pz = sim.get(4, 2)[0, 1, 1] if poles else 0.0
+            jz = sim.get(5, 2)[0, 1, 1] if poles else 0.0
+            out.append((e, pz, jz))
+        np.testing.assert_array_equal(np.array(out), tr[m, :, 1:])
+
+
+@pytest.mark.parametrize("name", ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small",
+                                  "grid3d_disp", "tm2d_c1"])
+def test_grid_scenarios_bit_exact(name):
+    sc, p, a = from_golden(name)
+    sim = refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])
+    apply(sim, sc)
+    sim.step(sc["nsteps"])
+    if sc["mode"] == "TM":
+        comps = [("Ez", E, 2), ("Dz", D, 2), ("Hx", H, 0), ("Hy", H, 1), ("Bx", B, 0), ("By", B, 1)]
+    else:
+        comps = [("Ex", E, 0), ("Ey", E, 1), ("Ez", E, 2), ("Hx", H, 0), ("Hy", H, 1), ("Hz", H, 2)]
+    checked = 0
+    for nm, f, c in comps:
+        if nm in a:
+            np.testing.assert_array_equal(sim.get(f, c).reshape(a[nm].shape), a[nm], err_msg=nm)
+            checked += 1
+    if "Ez_row_mid" in a:
+        ez = sim.get(E, 2)[0]
+        np.testing.assert_array_equal(ez[sc["ny"] // 2, :], a["Ez_row_mid"])
+        np.testing.assert_array_equal(ez[:, sc["nx"] // 2], a["Ez_col_mid"])
+        checked += 1
+    key = [k for k in a if k.startswith("probe_trace")][0]
+    pr = sim.probes()
+    np.testing.assert_array_equal(pr, a[key].reshape(pr.shape))
+    assert checked > 0
+    if "dispersive_q_D_P_J" in a:
+        dq = a["dispersive_q_D_P_J"]
+        q = dq[:, 0].astype(int)
+        for c in range(3):
+            np.testing.assert_array_equal(sim.get(D, c).reshape(-1)[q], dq[:, 1 + c])
+            np.testing.assert_array_equal(sim.get(4, c).reshape(-1)[q], dq[:, 4 + c])
+            np.testing.assert_array_equal(sim.get(5, c).reshape(-1)[q], dq[:, 7 + c])
