@@ -1,0 +1,942 @@
+// Host side of the C ABI declared in include/fdtd_hip.h.
+//
+// A grid handle owns its device memory; set-up calls stage small tables on the
+// host and upload them, fdtd_step() launches the per-step passes on the
+// grid's stream.  The reference has no device layer at all: its "grid" is a
+// user container of YeeCell objects linked by neighbour pointers
+// (tests/test3D.cpp:33-49), swept once per functor on one CPU thread.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fdtd_hip.h"
+#include "fdtd_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) return fail(FDTD_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// include/FDTDConstants.hpp:394-398 (imp0 via the same Newton iteration)
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kEps0 = 8.854e-12;
+const double kMu0 = kPi * 4.0e-7;
+double newton_sqrt(double x) {
+  double curr = x, prev = 0;
+  while (curr != prev) {
+    double nxt = 0.5 * (curr + x / curr);
+    prev = curr;
+    curr = nxt;
+  }
+  return curr;
+}
+const double kImp0 = newton_sqrt(kMu0 / kEps0);
+
+// NewPML::setPMLParameters, include/PML.hpp:85-102: {B, C, F, G}
+void pml_bcfg(double K, double S, double A, double dt, double out[4]) {
+  double nu = 1.0 / kEps0 * (S / K + A);
+  double B = std::exp(-dt * nu);
+  out[0] = B;
+  out[1] = (nu == 0) ? 0.0 : S / K * 1.0 / (S + K * A) * (B - 1.0);
+  double u = -S / (kEps0 * K * K);
+  out[2] = (nu == 0) ? 1.0 / K : 1.0 / K - 1.0 / dt * (1.0 - B - nu * dt) * u / (nu * nu);
+  out[3] = (nu == 0) ? 0.0 : 1.0 / dt * (1.0 - B) / nu;
+}
+
+// PMLParameterModel K/S/A, include/PML.hpp:378-391
+void pml_ksa(double x, double m, double ma, double smax, double kmax, double amax, double out[3]) {
+  if (x < 0.0) {
+    out[0] = 1.0;
+    out[1] = 0.0;
+    out[2] = 0.0;
+    return;
+  }
+  out[0] = x > 1.0 ? 1.0 : 1.0 + std::pow(x, m) * (kmax - 1.0);
+  out[1] = x > 1.0 ? 0.0 : std::pow(x, m) * smax;
+  out[2] = x > 1.0 ? 0.0 : std::pow(1.0 - x, ma) * amax;
+}
+
+// placement idiom (tests/testTM.cpp:46-71, include/Source.hpp:380-389): per-index
+// {B,C,F,G} rows for E and H along an axis of length L; unplaced rows keep the
+// NewPML() defaults B=C=G=0, F=1.
+void pml_idiom_rows(int L, int n, double m, double ma, double smax, double kmax, double amax, double dt,
+                    std::vector<double>& e, std::vector<double>& h) {
+  e.assign(4 * L, 0.0);
+  h.assign(4 * L, 0.0);
+  for (int i = 0; i < L; i++) e[4 * i + 2] = h[4 * i + 2] = 1.0;
+  for (int i = 1; i < n + 2; i++) {
+    double x = double(n - i + 1) / double(n);
+    double xm = x - 0.5 / double(n);
+    double ke[3], kh[3], r[4];
+    pml_ksa(x, m, ma, smax, kmax, amax, ke);
+    pml_ksa(xm, m, ma, smax, kmax, amax, kh);
+    auto put = [&](std::vector<double>& v, int idx, const double* ksa) {
+      if (idx < 0 || idx >= L) return;
+      pml_bcfg(ksa[0], ksa[1], ksa[2], dt, r);
+      std::copy(r, r + 4, v.begin() + 4 * idx);
+    };
+    put(e, i, ke);
+    put(h, i, kh);
+    put(e, L - i, ke);
+    put(h, L - 1 - i, kh);
+  }
+}
+
+constexpr int kChunk = 256;  // steps per incident-table chunk
+
+// tile of the per-step passes: threads per block in x and y, z planes per thread
+struct Tile {
+  int bx, by, kc;
+};
+constexpr Tile kDefaultTile3D = {64, 4, 16};
+constexpr Tile kDefaultTileTM = {64, 4, 1};
+
+}  // namespace
+
+struct MatHost {
+  double eps = 1.0, mu = 1.0;
+  int np = 0;
+  int kind[FDTD_MAX_POLES] = {0, 0, 0};
+  double freq[FDTD_MAX_POLES] = {0, 0, 0}, gamma[FDTD_MAX_POLES] = {0, 0, 0}, weight[FDTD_MAX_POLES] = {0, 0, 0};
+};
+
+struct fdtd_grid {
+  fdtd_grid_desc d{};
+  int ndim = 3;
+  bool f64 = false;
+  size_t esz = 4;
+  long ncell = 0;
+  int len[3] = {1, 1, 1};
+  hipStream_t stream = nullptr;
+  bool actE[3] = {false, false, false}, actH[3] = {false, false, false};
+  std::vector<void*> allocs;
+
+  void* E[3] = {nullptr, nullptr, nullptr};
+  void* H[3] = {nullptr, nullptr, nullptr};
+  uint8_t* mat = nullptr;
+  std::vector<uint8_t> mat_host;
+  void* mats_dev = nullptr;
+  MatHost mats[FDTD_MAX_MATERIALS];
+
+  // CPML rows {B,C,F,G} per [ft][axis]
+  std::vector<double> pml[2][3];
+  void* coef[2][3][4] = {};  // f, g, b, c device arrays of T
+  void* psi[2][3][3] = {};
+  int slab_a[3] = {0, 0, 0}, slab_b[3] = {1, 1, 1}, slab_S[3] = {0, 0, 0};
+
+  // dispersive box
+  int dbox[6] = {0, 0, 0, 0, 0, 0};
+  int max_np = 0;
+  void* Dd[3] = {};
+  void* Pd[FDTD_MAX_POLES][3] = {};
+  void* Jd[FDTD_MAX_POLES][3] = {};
+
+  // sources
+  int nsrc = 0;
+  fdtdk::SrcDev src[FDTD_MAX_SOURCES];
+
+  // TFSF
+  bool tfsf = false;
+  int box[6] = {};
+  double kvec[3] = {}, pol[3] = {}, polH[3] = {}, origin[3] = {};
+  int auxL = 0;
+  double *aux_state = nullptr, *aux_pE = nullptr, *aux_pH = nullptr, *aux_sig = nullptr;
+  int nsig = 0;
+  double *incE = nullptr, *incH = nullptr;
+
+  // DFT
+  int ndft = 0;
+  struct Dft {
+    int r[6];
+    int mask, nfreq, ncomp, nphase;
+    long ncells;
+    void* acc;
+    double *phE, *phH;
+  } dft[FDTD_MAX_DFT];
+
+  // probes
+  int nprobe = 0;
+  fdtdk::ProbeDev probe[FDTD_MAX_PROBES];
+  double* trace = nullptr;
+  long trace_cap = 0;
+
+  Tile tile = {0, 0, 0};
+  bool finalized = false;
+  bool coef_dirty = true;
+  bool mats_dirty = true;
+  long step = 0;
+
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, int>> ev_pairs;  // (kernel id, index of start event)
+  size_t ev_used = 0;
+  double kms[3] = {0, 0, 0};
+  long kcount[3] = {0, 0, 0};
+
+  ~fdtd_grid() {
+    for (void* p : allocs) (void)hipFree(p);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+  }
+
+  int alloc(void** p, size_t bytes, bool zero = true) {
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      *p = nullptr;
+      return fail(FDTD_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    }
+    allocs.push_back(*p);
+    if (zero && hipMemset(*p, 0, bytes) != hipSuccess) return fail(FDTD_EDEVICE, "hipMemset failed");
+    return 0;
+  }
+  void release(void* p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+  }
+  long q(int i, int j, int k) const { return long(i) + long(d.nx) * (long(j) + long(d.ny) * k); }
+};
+
+namespace {
+
+// host -> device array of T from doubles
+int upload_T(fdtd_grid* g, void* dst, const double* src, size_t n) {
+  if (g->f64) {
+    HIP_TRY(hipMemcpy(dst, src, n * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(src, src + n);
+    HIP_TRY(hipMemcpy(dst, tmp.data(), n * 4, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+int download_T(fdtd_grid* g, double* dst, const void* src, size_t n) {
+  if (g->f64) {
+    HIP_TRY(hipMemcpy(dst, src, n * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(n);
+    HIP_TRY(hipMemcpy(tmp.data(), src, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) dst[i] = tmp[i];
+  }
+  return 0;
+}
+
+template <typename T>
+int upload_materials(fdtd_grid* g) {
+  std::vector<fdtdk::MatDev<T>> m(FDTD_MAX_MATERIALS);
+  const double dt = g->d.dt;
+  for (int id = 0; id < FDTD_MAX_MATERIALS; id++) {
+    const MatHost& h = g->mats[id];
+    m[id].inv_eps = T(1.0 / (h.eps * kEps0));
+    m[id].inv_mu = T(1.0 / (h.mu * kMu0));
+    m[id].np = h.np;
+    for (int p = 0; p < fdtdk::kMaxPoles; p++) {
+      double cJ = 0, cE = 0, cP = 0;
+      if (p < h.np && h.kind[p] == FDTD_DRUDE) {
+        // DrudeUpdate::implicit_update, DispersiveMaterials.hpp:986-996
+        double b = h.gamma[p] * dt * 0.5, w = dt * h.freq[p];
+        cJ = (1.0 - b) / (1.0 + b);
+        cE = w * w / dt * (h.weight[p] * kEps0);
+      } else if (p < h.np) {
+        // LorentzUpdate::implicit_update, DispersiveMaterials.hpp:719-732
+        double b = 1.0 + h.gamma[p] * dt, bm = 1.0 - h.gamma[p] * dt, w = h.freq[p] * dt;
+        cJ = bm / b;
+        cE = h.freq[p] * w / b * (h.weight[p] * kEps0);
+        cP = -(h.freq[p] * w / b);
+      }
+      m[id].cJ[p] = T(cJ);
+      m[id].cE[p] = T(cE);
+      m[id].cP[p] = T(cP);
+    }
+  }
+  HIP_TRY(hipMemcpy(g->mats_dev, m.data(), sizeof(fdtdk::MatDev<T>) * m.size(), hipMemcpyHostToDevice));
+  g->mats_dirty = false;
+  return 0;
+}
+
+// per-axis device coefficients: f = dt F/dx, g = dt G, b = B, c = C/dx
+int upload_coefs(fdtd_grid* g) {
+  const double dt = g->d.dt, dx = g->d.dx;
+  for (int ft = 0; ft < 2; ft++)
+    for (int a = 0; a < 3; a++) {
+      const int L = g->len[a];
+      std::vector<double> f(L), gg(L), b(L), c(L);
+      for (int i = 0; i < L; i++) {
+        const double* r = &g->pml[ft][a][4 * i];
+        f[i] = dt * r[2] / dx;
+        gg[i] = dt * r[3];
+        b[i] = r[0];
+        c[i] = r[1] / dx;
+      }
+      const std::vector<double>* src[4] = {&f, &gg, &b, &c};
+      for (int w = 0; w < 4; w++) {
+        int rc = upload_T(g, g->coef[ft][a][w], src[w]->data(), L);
+        if (rc) return rc;
+      }
+    }
+  g->coef_dirty = false;
+  return 0;
+}
+
+// CPML slabs: psi is kept only where some C coefficient of the axis is non-zero
+// (elsewhere psi stays exactly zero in the reference as well).
+int finalize(fdtd_grid* g) {
+  for (int a = 0; a < g->ndim; a++) {
+    const int L = g->len[a];
+    int lo = 0, hi = L;
+    for (int i = 0; i < L; i++) {
+      bool nz = g->pml[0][a][4 * i + 1] != 0.0 || g->pml[1][a][4 * i + 1] != 0.0;
+      if (!nz) continue;
+      if (i < L / 2)
+        lo = std::max(lo, i + 1);
+      else
+        hi = std::min(hi, i);
+    }
+    if (lo > hi) lo = hi;
+    g->slab_a[a] = lo;
+    g->slab_b[a] = hi;
+    g->slab_S[a] = lo + (L - hi);
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < g->ndim; J++) {
+        bool act = ft == 0 ? g->actE[I] : g->actH[I];
+        if (!act || I == J || g->slab_S[J] == 0) continue;
+        long n = long(g->slab_S[J]);
+        for (int a = 0; a < 3; a++)
+          if (a != J) n *= g->len[a];
+        int rc = g->alloc(&g->psi[ft][I][J], size_t(n) * g->esz);
+        if (rc) return rc;
+      }
+  // dispersive box: bounding box of interior cells whose material has poles
+  int mx = 0;
+  for (int id = 0; id < FDTD_MAX_MATERIALS; id++) mx = std::max(mx, g->mats[id].np);
+  g->max_np = mx;
+  if (mx > 0) {
+    int b[6] = {g->len[0], -1, g->len[1], -1, g->len[2], -1};
+    const int nz = g->ndim == 3 ? g->d.nz : 1;
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < g->d.ny; j++)
+        for (int i = 0; i < g->d.nx; i++)
+          if (g->mats[g->mat_host[g->q(i, j, k)]].np > 0) {
+            b[0] = std::min(b[0], i);
+            b[1] = std::max(b[1], i);
+            b[2] = std::min(b[2], j);
+            b[3] = std::max(b[3], j);
+            b[4] = std::min(b[4], k);
+            b[5] = std::max(b[5], k);
+          }
+    if (b[1] < 0) {
+      g->max_np = 0;
+    } else {
+      g->dbox[0] = b[0];
+      g->dbox[1] = b[1] + 1;
+      g->dbox[2] = b[2];
+      g->dbox[3] = b[3] + 1;
+      g->dbox[4] = b[4];
+      g->dbox[5] = b[5] + 1;
+      long nb = long(g->dbox[1] - g->dbox[0]) * (g->dbox[3] - g->dbox[2]) * (g->dbox[5] - g->dbox[4]);
+      // D = eps_r eps0 E from the current E (zero unless uploaded)
+      for (int c = 0; c < 3; c++) {
+        if (!g->actE[c]) continue;
+        int rc = g->alloc(&g->Dd[c], size_t(nb) * g->esz);
+        if (rc) return rc;
+        std::vector<double> e(g->ncell), dbx(nb);
+        rc = download_T(g, e.data(), g->E[c], g->ncell);
+        if (rc) return rc;
+        long t = 0;
+        for (int k = g->dbox[4]; k < g->dbox[5]; k++)
+          for (int j = g->dbox[2]; j < g->dbox[3]; j++)
+            for (int i = g->dbox[0]; i < g->dbox[1]; i++, t++) {
+              long qq = g->q(i, j, k);
+              dbx[t] = g->mats[g->mat_host[qq]].eps * kEps0 * e[qq];
+            }
+        rc = upload_T(g, g->Dd[c], dbx.data(), nb);
+        if (rc) return rc;
+        for (int p = 0; p < mx; p++) {
+          if ((rc = g->alloc(&g->Pd[p][c], size_t(nb) * g->esz))) return rc;
+          if ((rc = g->alloc(&g->Jd[p][c], size_t(nb) * g->esz))) return rc;
+        }
+      }
+    }
+  }
+  g->finalized = true;
+  return 0;
+}
+
+template <typename T>
+void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
+  memset(&a, 0, sizeof a);
+  for (int c = 0; c < 3; c++) {
+    a.E[c] = (T*)g->E[c];
+    a.H[c] = (T*)g->H[c];
+  }
+  a.mat = g->mat;
+  a.mats = (const fdtdk::MatDev<T>*)g->mats_dev;
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++) {
+      a.cf[ft][I] = (const T*)g->coef[ft][I][0];
+      a.cg[ft][I] = (const T*)g->coef[ft][I][1];
+      a.cb[ft][I] = (const T*)g->coef[ft][I][2];
+      a.cc[ft][I] = (const T*)g->coef[ft][I][3];
+      for (int J = 0; J < 3; J++) a.psi[ft][I][J] = (T*)g->psi[ft][I][J];
+    }
+  for (int x = 0; x < 3; x++) {
+    a.slab_a[x] = g->ndim > x ? g->slab_a[x] : 0;
+    a.slab_b[x] = g->ndim > x ? g->slab_b[x] : 1 << 30;
+    a.slab_S[x] = g->slab_S[x];
+  }
+  a.nx = g->d.nx;
+  a.ny = g->d.ny;
+  a.nz = g->ndim == 3 ? g->d.nz : 1;
+  a.sy = g->d.nx;
+  a.sz = long(g->d.nx) * g->d.ny;
+  for (int x = 0; x < 6; x++) a.dbox[x] = g->dbox[x];
+  for (int c = 0; c < 3; c++) {
+    a.Dd[c] = (T*)g->Dd[c];
+    for (int p = 0; p < fdtdk::kMaxPoles; p++) {
+      a.Pd[p][c] = (T*)g->Pd[p][c];
+      a.Jd[p][c] = (T*)g->Jd[p][c];
+    }
+  }
+  a.dt = T(g->d.dt);
+  a.nsrc = g->nsrc;
+  for (int s = 0; s < g->nsrc; s++) a.src[s] = g->src[s];
+  a.tfsf = g->tfsf;
+  for (int x = 0; x < 6; x++) a.box[x] = g->box[x];
+  for (int x = 0; x < 3; x++) {
+    a.kvec[x] = g->kvec[x];
+    a.pol[x] = g->pol[x];
+    a.polH[x] = g->polH[x];
+    a.origin[x] = g->origin[x];
+  }
+  a.dx = g->d.dx;
+  a.auxL = g->auxL;
+  a.ndft = g->ndft;
+  for (int q = 0; q < g->ndft; q++) {
+    const auto& h = g->dft[q];
+    auto& dd = a.dft[q];
+    for (int x = 0; x < 6; x++) dd.r[x] = h.r[x];
+    dd.mask = h.mask;
+    dd.nfreq = h.nfreq;
+    dd.ncomp = h.ncomp;
+    dd.ncells = h.ncells;
+    dd.acc = (T*)h.acc;
+    dd.phE = h.phE;
+    dd.phH = h.phH;
+    dd.nphase = h.nphase;
+  }
+  a.nprobe = g->nprobe;
+  for (int p = 0; p < g->nprobe; p++) a.probe[p] = g->probe[p];
+  a.trace = g->trace;
+}
+
+hipEvent_t next_event(fdtd_grid* g) {
+  if (g->ev_used == g->ev_pool.size()) {
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    g->ev_pool.push_back(e);
+  }
+  return g->ev_pool[g->ev_used++];
+}
+
+int collect_events(fdtd_grid* g) {
+  if (!g->prof || g->ev_pairs.empty()) return 0;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  for (auto& pr : g->ev_pairs) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, g->ev_pool[pr.second], g->ev_pool[pr.second + 1]));
+    g->kms[pr.first] += ms;
+    g->kcount[pr.first] += 1;
+  }
+  g->ev_pairs.clear();
+  g->ev_used = 0;
+  return 0;
+}
+
+template <typename K, typename... A>
+int launch(fdtd_grid* g, int kid, K kern, dim3 grid, dim3 block, const A&... args) {
+  if (g->prof) {
+    if (g->ev_used + 2 > 4096) {
+      int rc = collect_events(g);
+      if (rc) return rc;
+    }
+    size_t s = g->ev_used;
+    hipEvent_t e0 = next_event(g), e1 = next_event(g);
+    HIP_TRY(hipEventRecord(e0, g->stream));
+    hipLaunchKernelGGL(kern, grid, block, 0, g->stream, args...);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, g->stream));
+    g->ev_pairs.push_back({kid, int(s)});
+    return 0;
+  }
+  hipLaunchKernelGGL(kern, grid, block, 0, g->stream, args...);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int run_aux(fdtd_grid* g, long step0, int nsteps) {
+  fdtdk::AuxArgs a;
+  a.L = g->auxL;
+  a.npml = 15;
+  a.dt = g->d.dt;
+  a.dx = g->d.dx;
+  a.eps0 = kEps0;
+  a.mu0 = kMu0;
+  double* s = g->aux_state;
+  a.Dz = s;
+  a.Ez = s + g->auxL;
+  a.By = s + 2 * g->auxL;
+  a.Hy = s + 3 * g->auxL;
+  a.psiE = s + 4 * g->auxL;
+  a.psiH = s + 5 * g->auxL;
+  a.pE = g->aux_pE;
+  a.pH = g->aux_pH;
+  a.signal = g->aux_sig;
+  a.nsignal = g->nsig;
+  a.step0 = step0;
+  a.nsteps = nsteps;
+  a.outE = g->incE;
+  a.outH = g->incH;
+  int threads = std::min(1024, ((g->auxL + 63) / 64) * 64);
+  return launch(g, 2, fdtdk::aux_line, dim3(1), dim3(threads), a);
+}
+
+template <typename T>
+int step_impl(fdtd_grid* g, int nsteps) {
+  if (g->mats_dirty) {
+    int rc = upload_materials<T>(g);
+    if (rc) return rc;
+  }
+  if (g->coef_dirty) {
+    int rc = upload_coefs(g);
+    if (rc) return rc;
+  }
+  if (g->nprobe && g->trace_cap < g->step + nsteps) {
+    long cap = std::max<long>(g->step + nsteps, 2 * g->trace_cap);
+    double* nt = nullptr;
+    int rc = g->alloc((void**)&nt, size_t(cap) * g->nprobe * 8);
+    if (rc) return rc;
+    if (g->trace) {
+      HIP_TRY(hipMemcpyAsync(nt, g->trace, size_t(g->step) * g->nprobe * 8, hipMemcpyDeviceToDevice, g->stream));
+      HIP_TRY(hipStreamSynchronize(g->stream));
+      g->release(g->trace);
+    }
+    g->trace = nt;
+    g->trace_cap = cap;
+  }
+  fdtdk::StepArgs<T> a;
+  fill_args<T>(g, a);
+  Tile t = g->tile.bx ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
+  if (g->ndim == 2) t.kc = 1;
+  const int nzi = g->ndim == 3 ? g->d.nz - 2 : 1;
+  dim3 block(t.bx, t.by, 1);
+  dim3 grid((g->d.nx - 2 + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, (nzi + t.kc - 1) / t.kc);
+  int done = 0;
+  while (done < nsteps) {
+    const int chunk = std::min(kChunk, nsteps - done);
+    if (g->tfsf) {
+      int rc = run_aux(g, g->step, chunk);
+      if (rc) return rc;
+    }
+    for (int s = 0; s < chunk; s++) {
+      a.step = g->step;
+      if (g->tfsf) {
+        a.incE = g->incE + long(s) * g->auxL;
+        a.incH = g->incH + long(s) * g->auxL;
+      }
+      int rc;
+      if (g->ndim == 3) {
+        rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, a, t.kc);
+        if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, a, t.kc);
+      } else {
+        rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, a, 1);
+        if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, a, 1);
+      }
+      if (rc) return rc;
+      g->step++;
+    }
+    done += chunk;
+  }
+  return 0;
+}
+
+bool valid_comp(const fdtd_grid* g, int field, int comp) {
+  if (comp < 0 || comp > 2) return false;
+  if (field == FDTD_E || field == FDTD_D || field == FDTD_P || field == FDTD_J) return g->actE[comp];
+  if (field == FDTD_H || field == FDTD_B) return g->actH[comp];
+  return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fdtd_last_error(void) { return g_err.c_str(); }
+
+int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
+  if (!desc || !out) return fail(FDTD_EINVAL, "null argument");
+  *out = nullptr;
+  if (desc->mode != FDTD_TM && desc->mode != FDTD_3D) return fail(FDTD_EINVAL, "mode must be FDTD_TM or FDTD_3D");
+  if (desc->precision != FDTD_F32 && desc->precision != FDTD_F64) return fail(FDTD_EINVAL, "bad precision");
+  const int nz = desc->mode == FDTD_3D ? desc->nz : 1;
+  if (desc->nx < 3 || desc->ny < 3 || (desc->mode == FDTD_3D && nz < 3))
+    return fail(FDTD_EINVAL, "grid must have at least 3 cells per axis");
+  if (!(desc->dx > 0) || !(desc->dt > 0)) return fail(FDTD_EINVAL, "dx and dt must be positive");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(FDTD_EDEVICE, "no HIP device");
+  fdtd_grid* g = new fdtd_grid();
+  g->d = *desc;
+  g->d.nz = nz;
+  g->ndim = desc->mode == FDTD_3D ? 3 : 2;
+  g->f64 = desc->precision == FDTD_F64;
+  g->esz = g->f64 ? 8 : 4;
+  g->len[0] = desc->nx;
+  g->len[1] = desc->ny;
+  g->len[2] = nz;
+  g->ncell = long(desc->nx) * desc->ny * nz;
+  if (g->ndim == 3) {
+    for (int c = 0; c < 3; c++) g->actE[c] = g->actH[c] = true;
+  } else {
+    g->actE[2] = true;
+    g->actH[0] = g->actH[1] = true;
+  }
+  int rc = 0;
+  for (int c = 0; c < 3 && !rc; c++) {
+    if (g->actE[c]) rc = g->alloc(&g->E[c], g->ncell * g->esz);
+    if (!rc && g->actH[c]) rc = g->alloc(&g->H[c], g->ncell * g->esz);
+  }
+  if (!rc) rc = g->alloc((void**)&g->mat, g->ncell);
+  g->mat_host.assign(g->ncell, 0);
+  if (!rc) rc = g->alloc(&g->mats_dev, sizeof(fdtdk::MatDev<double>) * FDTD_MAX_MATERIALS);
+  for (int ft = 0; ft < 2 && !rc; ft++)
+    for (int a = 0; a < 3 && !rc; a++) {
+      g->pml[ft][a].assign(4 * g->len[a], 0.0);
+      for (int i = 0; i < g->len[a]; i++) g->pml[ft][a][4 * i + 2] = 1.0;
+      for (int w = 0; w < 4 && !rc; w++) rc = g->alloc(&g->coef[ft][a][w], size_t(g->len[a]) * g->esz);
+    }
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return 0;
+}
+
+int fdtd_grid_destroy(fdtd_grid* g) {
+  if (!g) return 0;
+  (void)hipStreamSynchronize(g->stream);
+  delete g;
+  return 0;
+}
+
+int fdtd_set_stream(fdtd_grid* g, void* s) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  g->stream = (hipStream_t)s;
+  return 0;
+}
+
+int fdtd_set_pml_axis(fdtd_grid* g, int axis, int ft, const double* K, const double* S, const double* A) {
+  if (!g || !K || !S || !A) return fail(FDTD_EINVAL, "null argument");
+  if (axis < 0 || axis >= g->ndim || ft < 0 || ft > 1) return fail(FDTD_EINVAL, "bad axis/ft");
+  if (g->finalized) return fail(FDTD_ESTATE, "CPML profiles are fixed once stepping has started");
+  for (int i = 0; i < g->len[axis]; i++) pml_bcfg(K[i], S[i], A[i], g->d.dt, &g->pml[ft][axis][4 * i]);
+  g->coef_dirty = true;
+  return 0;
+}
+
+int fdtd_set_pml_idiom(fdtd_grid* g, int axis, int n, double m, double ma, double smax, double kmax, double amax) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (axis < 0 || axis >= g->ndim || n < 1 || 2 * n + 4 > g->len[axis]) return fail(FDTD_EINVAL, "bad axis or layer count");
+  if (g->finalized) return fail(FDTD_ESTATE, "CPML profiles are fixed once stepping has started");
+  pml_idiom_rows(g->len[axis], n, m, ma, smax, kmax, amax, g->d.dt, g->pml[0][axis], g->pml[1][axis]);
+  g->coef_dirty = true;
+  return 0;
+}
+
+int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npoles, const double* poles) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (id < 0 || id >= FDTD_MAX_MATERIALS || npoles < 0 || npoles > FDTD_MAX_POLES || (npoles && !poles))
+    return fail(FDTD_EINVAL, "bad material id or pole count");
+  if (!(eps_r > 0) || !(mu_r > 0)) return fail(FDTD_EINVAL, "eps_r and mu_r must be positive");
+  MatHost& m = g->mats[id];
+  if (g->finalized && (npoles > 0) != (m.np > 0)) return fail(FDTD_ESTATE, "dispersive layout is fixed once stepping has started");
+  if (g->finalized && npoles > g->max_np) return fail(FDTD_ESTATE, "pole count exceeds the allocated dispersive state");
+  for (int p = 0; p < npoles; p++)
+    if (int(poles[4 * p]) != FDTD_DRUDE && int(poles[4 * p]) != FDTD_LORENTZ) return fail(FDTD_EINVAL, "bad pole kind");
+  m.eps = eps_r;
+  m.mu = mu_r;
+  m.np = npoles;
+  for (int p = 0; p < npoles; p++) {
+    m.kind[p] = int(poles[4 * p]);
+    m.freq[p] = poles[4 * p + 1];
+    m.gamma[p] = poles[4 * p + 2];
+    m.weight[p] = poles[4 * p + 3];
+  }
+  g->mats_dirty = true;
+  return 0;
+}
+
+int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids) {
+  if (!g || !ids) return fail(FDTD_EINVAL, "null argument");
+  if (g->finalized && g->max_np > 0) {
+    for (long i = 0; i < g->ncell; i++)
+      if ((g->mats[ids[i]].np > 0) != (g->mats[g->mat_host[i]].np > 0))
+        return fail(FDTD_ESTATE, "dispersive layout is fixed once stepping has started");
+  }
+  std::copy(ids, ids + g->ncell, g->mat_host.begin());
+  HIP_TRY(hipMemcpy(g->mat, ids, g->ncell, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, const double* table, int ntable) {
+  if (!g || !table || ntable < 0) return fail(FDTD_EINVAL, "null argument");
+  if (g->nsrc >= FDTD_MAX_SOURCES) return fail(FDTD_EINVAL, "too many sources (max %d)", FDTD_MAX_SOURCES);
+  if ((field != FDTD_D && field != FDTD_B) || !valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "bad source field/component");
+  if (i < 1 || j < 1 || i >= g->d.nx - 1 || j >= g->d.ny - 1 || (g->ndim == 3 && (k < 1 || k >= g->d.nz - 1)) ||
+      (g->ndim == 2 && k != 0))
+    return fail(FDTD_EINVAL, "source cell must be an interior cell");
+  double* t = nullptr;
+  int rc = g->alloc((void**)&t, size_t(std::max(ntable, 1)) * 8);
+  if (rc) return rc;
+  if (ntable) HIP_TRY(hipMemcpy(t, table, size_t(ntable) * 8, hipMemcpyHostToDevice));
+  fdtdk::SrcDev& s = g->src[g->nsrc++];
+  s.cell = g->q(i, j, k);
+  s.field = field;
+  s.comp = comp;
+  s.table = t;
+  s.n = ntable;
+  return 0;
+}
+
+int fdtd_set_tfsf(fdtd_grid* g, const int box[6], const double kvec[3], const double pol[3], const double polH[3],
+                  const double origin[3], int aux_ncells, const double* signal, int nsignal) {
+  if (!g || !box || !kvec || !pol || !polH || !origin || (!signal && nsignal)) return fail(FDTD_EINVAL, "null argument");
+  if (g->tfsf) return fail(FDTD_ESTATE, "TFSF already set");
+  for (int a = 0; a < g->ndim; a++)
+    if (box[2 * a] < 1 || box[2 * a + 1] > g->len[a] - 2 || box[2 * a] >= box[2 * a + 1])
+      return fail(FDTD_EINVAL, "TFSF box must lie on interior cells with lo < hi");
+  if (aux_ncells < 2 || (aux_ncells + 50) > 8192) return fail(FDTD_EINVAL, "bad aux line length");
+  g->tfsf = true;
+  for (int x = 0; x < 6; x++) g->box[x] = g->ndim == 3 || x < 4 ? box[x] : 0;
+  // TFSFPlane normalises the wave vector (include/Source.hpp:566-570)
+  double K = 0;
+  for (int d = 0; d < g->ndim; d++) K += kvec[d] * kvec[d];
+  K = std::sqrt(K);
+  if (!(K > 0)) return fail(FDTD_EINVAL, "zero wave vector");
+  for (int d = 0; d < 3; d++) {
+    g->kvec[d] = d < g->ndim ? kvec[d] / K : 0.0;
+    g->pol[d] = pol[d];
+    g->polH[d] = polH[d];
+    g->origin[d] = origin[d];
+  }
+  g->auxL = aux_ncells + 2 * 15 + 20;
+  const int L = g->auxL;
+  int rc = g->alloc((void**)&g->aux_state, size_t(6) * L * 8);
+  if (!rc) rc = g->alloc((void**)&g->aux_pE, size_t(4) * L * 8);
+  if (!rc) rc = g->alloc((void**)&g->aux_pH, size_t(4) * L * 8);
+  if (!rc) rc = g->alloc((void**)&g->aux_sig, size_t(std::max(nsignal, 1)) * 8);
+  if (!rc) rc = g->alloc((void**)&g->incE, size_t(kChunk) * L * 8);
+  if (!rc) rc = g->alloc((void**)&g->incH, size_t(kChunk) * L * 8);
+  if (rc) return rc;
+  // Aux1D's own CPML: PMLParameterModel(dx) defaults, 15 layers (Source.hpp:362, 380-389)
+  std::vector<double> e, h;
+  const double m = 3;
+  pml_idiom_rows(L, 15, m, 1, 0.8 * (m + 1) / (kImp0 * g->d.dx), 5, 0.05, g->d.dt, e, h);
+  HIP_TRY(hipMemcpy(g->aux_pE, e.data(), e.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g->aux_pH, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  if (nsignal) HIP_TRY(hipMemcpy(g->aux_sig, signal, size_t(nsignal) * 8, hipMemcpyHostToDevice));
+  g->nsig = nsignal;
+  return 0;
+}
+
+int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, const double* phaseE,
+                 const double* phaseH, int nphase, int* idx) {
+  if (!g || !region || !phaseE || !phaseH || nfreq < 1 || nphase < 0) return fail(FDTD_EINVAL, "bad argument");
+  if (g->ndft >= FDTD_MAX_DFT) return fail(FDTD_EINVAL, "too many DFT monitors (max %d)", FDTD_MAX_DFT);
+  int r[6];
+  for (int x = 0; x < 6; x++) r[x] = region[x];
+  if (g->ndim == 2) {
+    r[4] = 0;
+    r[5] = 1;
+  }
+  for (int a = 0; a < 3; a++)
+    if (r[2 * a] < 0 || r[2 * a + 1] > g->len[a] || r[2 * a] >= r[2 * a + 1]) return fail(FDTD_EINVAL, "bad DFT region");
+  for (int c = 0; c < 6; c++)
+    if (((comp_mask >> c) & 1) && !(c < 3 ? g->actE[c] : g->actH[c - 3]))
+      return fail(FDTD_EINVAL, "DFT component %d does not exist in this mode", c);
+  if (comp_mask <= 0 || comp_mask >= 64) return fail(FDTD_EINVAL, "bad component mask");
+  auto& d = g->dft[g->ndft];
+  for (int x = 0; x < 6; x++) d.r[x] = r[x];
+  d.mask = comp_mask;
+  d.nfreq = nfreq;
+  d.nphase = nphase;
+  d.ncomp = __builtin_popcount(comp_mask);
+  d.ncells = long(r[1] - r[0]) * (r[3] - r[2]) * (r[5] - r[4]);
+  int rc = g->alloc(&d.acc, size_t(d.ncomp) * nfreq * 2 * d.ncells * g->esz);
+  if (!rc) rc = g->alloc((void**)&d.phE, size_t(std::max(nphase, 1)) * nfreq * 2 * 8);
+  if (!rc) rc = g->alloc((void**)&d.phH, size_t(std::max(nphase, 1)) * nfreq * 2 * 8);
+  if (rc) return rc;
+  if (nphase) {
+    HIP_TRY(hipMemcpy(d.phE, phaseE, size_t(nphase) * nfreq * 2 * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d.phH, phaseH, size_t(nphase) * nfreq * 2 * 8, hipMemcpyHostToDevice));
+  }
+  if (idx) *idx = g->ndft;
+  g->ndft++;
+  return 0;
+}
+
+int fdtd_get_dft(fdtd_grid* g, int idx, double* out) {
+  if (!g || !out || idx < 0 || idx >= g->ndft) return fail(FDTD_EINVAL, "bad DFT index");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  const auto& d = g->dft[idx];
+  return download_T(g, out, d.acc, size_t(d.ncomp) * d.nfreq * 2 * d.ncells);
+}
+
+int fdtd_add_probe(fdtd_grid* g, int field, int comp, int i, int j, int k) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (g->nprobe >= FDTD_MAX_PROBES) return fail(FDTD_EINVAL, "too many probes (max %d)", FDTD_MAX_PROBES);
+  if ((field != FDTD_E && field != FDTD_H) || !valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "probes record E or H components");
+  if (i < 1 || j < 1 || i >= g->d.nx - 1 || j >= g->d.ny - 1 || (g->ndim == 3 && (k < 1 || k >= g->d.nz - 1)) ||
+      (g->ndim == 2 && k != 0))
+    return fail(FDTD_EINVAL, "probe cell must be an interior cell");
+  if (g->step > 0) return fail(FDTD_ESTATE, "probes must be added before stepping");
+  g->probe[g->nprobe].cell = g->q(i, j, k);
+  g->probe[g->nprobe].field = field;
+  g->probe[g->nprobe].comp = comp;
+  g->nprobe++;
+  return 0;
+}
+
+int fdtd_get_probes(fdtd_grid* g, double* out, int max_steps, int* nsteps_out) {
+  if (!g || !out) return fail(FDTD_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  long n = std::min<long>(g->step, max_steps);
+  if (n > 0 && g->nprobe) HIP_TRY(hipMemcpy(out, g->trace, size_t(n) * g->nprobe * 8, hipMemcpyDeviceToHost));
+  if (nsteps_out) *nsteps_out = int(n);
+  return 0;
+}
+
+int fdtd_upload_field(fdtd_grid* g, int field, int comp, int pole, const double* host) {
+  if (!g || !host) return fail(FDTD_EINVAL, "null argument");
+  if (!valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "component does not exist in this mode");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (field == FDTD_E) return upload_T(g, g->E[comp], host, g->ncell);
+  if (field == FDTD_H) return upload_T(g, g->H[comp], host, g->ncell);
+  return fail(FDTD_EINVAL, "only E and H can be uploaded");
+}
+
+int fdtd_download_field(fdtd_grid* g, int field, int comp, int pole, double* host) {
+  if (!g || !host) return fail(FDTD_EINVAL, "null argument");
+  if (!valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "component does not exist in this mode");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (field == FDTD_E) return download_T(g, host, g->E[comp], g->ncell);
+  if (field == FDTD_H) return download_T(g, host, g->H[comp], g->ncell);
+  if (field == FDTD_B) {
+    int rc = download_T(g, host, g->H[comp], g->ncell);
+    if (rc) return rc;
+    for (long i = 0; i < g->ncell; i++) host[i] *= g->mats[g->mat_host[i]].mu * kMu0;
+    return 0;
+  }
+  // D, P, J: dispersive box values where stored, eps_r eps0 E (D) or 0 elsewhere
+  if (field == FDTD_D) {
+    int rc = download_T(g, host, g->E[comp], g->ncell);
+    if (rc) return rc;
+    for (long i = 0; i < g->ncell; i++) host[i] *= g->mats[g->mat_host[i]].eps * kEps0;
+  } else {
+    std::fill(host, host + g->ncell, 0.0);
+    if (pole < 0 || pole >= FDTD_MAX_POLES) return fail(FDTD_EINVAL, "bad pole index");
+  }
+  void* src = field == FDTD_D ? g->Dd[comp] : (field == FDTD_P ? g->Pd[pole][comp] : g->Jd[pole][comp]);
+  if (!src || g->max_np == 0) return 0;
+  long nb = long(g->dbox[1] - g->dbox[0]) * (g->dbox[3] - g->dbox[2]) * (g->dbox[5] - g->dbox[4]);
+  std::vector<double> bx(nb);
+  int rc = download_T(g, bx.data(), src, nb);
+  if (rc) return rc;
+  long t = 0;
+  for (int k = g->dbox[4]; k < g->dbox[5]; k++)
+    for (int j = g->dbox[2]; j < g->dbox[3]; j++)
+      for (int i = g->dbox[0]; i < g->dbox[1]; i++, t++) {
+        long qq = g->q(i, j, k);
+        if (g->mats[g->mat_host[qq]].np > 0) host[qq] = bx[t];
+      }
+  return 0;
+}
+
+int fdtd_step(fdtd_grid* g, int nsteps) {
+  if (!g || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
+  if (nsteps == 0) return 0;
+  if (!g->finalized) {
+    int rc = finalize(g);
+    if (rc) return rc;
+  }
+  return g->f64 ? step_impl<double>(g, nsteps) : step_impl<float>(g, nsteps);
+}
+
+int fdtd_synchronize(fdtd_grid* g) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return collect_events(g);
+}
+
+long fdtd_step_count(const fdtd_grid* g) { return g ? g->step : -1; }
+
+int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (bx <= 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
+    return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
+  g->tile = Tile{bx, by, kc};
+  return 0;
+}
+
+int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  Tile t = g->tile.bx ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
+  if (bx) *bx = t.bx;
+  if (by) *by = t.by;
+  if (kc) *kc = t.kc;
+  return 0;
+}
+
+int fdtd_set_profiling(fdtd_grid* g, int on) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  int rc = collect_events(g);
+  g->prof = on != 0;
+  for (int k = 0; k < 3; k++) {
+    g->kms[k] = 0;
+    g->kcount[k] = 0;
+  }
+  return rc;
+}
+
+int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches) {
+  if (!g || kernel < 0 || kernel > 2) return fail(FDTD_EINVAL, "bad argument");
+  int rc = collect_events(g);
+  if (rc) return rc;
+  if (ms) *ms = g->kms[kernel];
+  if (launches) *launches = g->kcount[kernel];
+  return 0;
+}
+
+}  // extern "C"

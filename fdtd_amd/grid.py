@@ -1,0 +1,229 @@
+"""Python binding of the C ABI (include/fdtd_hip.h) over ctypes.
+
+``Grid`` owns one ``fdtd_grid`` handle.  Method names follow the set-up steps
+of the reference's user time loop (PML placement, materials, soft sources,
+TFSF planes, probes) so the same scenario description drives this class and
+the CPU oracle.  Loading fails loudly when the HIP library is missing: there
+is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfdtd_hip.so")
+
+TM, THREED = 2, 3
+F32, F64 = 0, 1
+D, E, B, H, P, J = 0, 1, 2, 3, 4, 5
+DRUDE, LORENTZ = 1, 2
+
+# every symbol declared in include/fdtd_hip.h
+ABI_SYMBOLS = [
+    "fdtd_grid_create", "fdtd_grid_destroy", "fdtd_set_stream", "fdtd_set_pml_axis", "fdtd_set_pml_idiom",
+    "fdtd_set_material", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_dft", "fdtd_get_dft",
+    "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
+    "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
+    "fdtd_set_tile", "fdtd_get_tile",
+]
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("precision", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("nz", ctypes.c_int), ("dx", ctypes.c_double), ("dt", ctypes.c_double)]
+
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_p = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_LIB = None
+
+
+def load_library():
+    """Load libfdtd_hip.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m fdtd_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "fdtd_grid_create": (_i, [ctypes.POINTER(GridDesc), ctypes.POINTER(_p)]),
+        "fdtd_grid_destroy": (_i, [_p]),
+        "fdtd_set_stream": (_i, [_p, _p]),
+        "fdtd_set_pml_axis": (_i, [_p, _i, _i, _dp, _dp, _dp]),
+        "fdtd_set_pml_idiom": (_i, [_p, _i, _i, _d, _d, _d, _d, _d]),
+        "fdtd_set_material": (_i, [_p, _i, _d, _d, _i, _dp]),
+        "fdtd_set_matid": (_i, [_p, ctypes.c_char_p]),
+        "fdtd_add_source": (_i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
+        "fdtd_set_tfsf": (_i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
+        "fdtd_add_dft": (_i, [_p, _ip, _i, _i, _dp, _dp, _i, _ip]),
+        "fdtd_get_dft": (_i, [_p, _i, _dp]),
+        "fdtd_add_probe": (_i, [_p, _i, _i, _i, _i, _i]),
+        "fdtd_get_probes": (_i, [_p, _dp, _i, _ip]),
+        "fdtd_upload_field": (_i, [_p, _i, _i, _i, _dp]),
+        "fdtd_download_field": (_i, [_p, _i, _i, _i, _dp]),
+        "fdtd_step": (_i, [_p, _i]),
+        "fdtd_synchronize": (_i, [_p]),
+        "fdtd_step_count": (ctypes.c_long, [_p]),
+        "fdtd_set_profiling": (_i, [_p, _i]),
+        "fdtd_kernel_time": (_i, [_p, _i, _dp, ctypes.POINTER(ctypes.c_long)]),
+        "fdtd_last_error": (ctypes.c_char_p, []),
+        "fdtd_set_tile": (_i, [_p, _i, _i, _i]),
+        "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+class FdtdError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load_library().fdtd_last_error().decode()
+        raise FdtdError(f"fdtd error {rc}: {msg}")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Grid:
+    """One simulation on the GPU.  ``mode`` is "TM" or "ThreeD";
+    ``precision`` "f32" or "f64"."""
+
+    def __init__(self, mode, nx, ny, nz=1, dx=1.0, dt=1.0, precision="f64", stream=None):
+        L = load_library()
+        self.mode = mode
+        self.shape = (nz if mode == "ThreeD" else 1, ny, nx)
+        desc = GridDesc(THREED if mode == "ThreeD" else TM, F64 if precision == "f64" else F32, nx, ny,
+                        nz if mode == "ThreeD" else 1, dx, dt)
+        h = _p()
+        _check(L.fdtd_grid_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+        self.dft_meta = []
+        self._nprobes = 0
+        if stream is not None:
+            _check(L.fdtd_set_stream(self.h, stream))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().fdtd_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up
+    def set_pml_idiom(self, axis, n, m, ma, smax, kmax, amax):
+        _check(load_library().fdtd_set_pml_idiom(self.h, axis, n, m, ma, smax, kmax, amax))
+
+    def set_pml_axis(self, axis, ft, K, S, A):
+        K, S, A = _f64(K), _f64(S), _f64(A)
+        _check(load_library().fdtd_set_pml_axis(self.h, axis, ft, _dptr(K), _dptr(S), _dptr(A)))
+
+    def set_material(self, mid, eps_r, mu_r=1.0, poles=()):
+        arr = _f64(np.array(poles, dtype=np.float64).reshape(-1)) if len(poles) else np.zeros(1)
+        _check(load_library().fdtd_set_material(self.h, mid, eps_r, mu_r, len(poles), _dptr(arr)))
+
+    def set_matid(self, ids):
+        ids = np.ascontiguousarray(ids, dtype=np.uint8).reshape(-1)
+        if ids.size != int(np.prod(self.shape)):
+            raise ValueError("matid size mismatch")
+        _check(load_library().fdtd_set_matid(self.h, ids.tobytes()))
+
+    def add_source(self, field, comp, i, j, k, table):
+        t = _f64(table)
+        _check(load_library().fdtd_add_source(self.h, field, comp, i, j, k, _dptr(t), t.size))
+
+    def set_tfsf(self, box, kvec, pol, polH, origin, aux_ncells, signal):
+        b = (ctypes.c_int * 6)(*box)
+        k, p, ph, o, s = (_f64(x) for x in (kvec, pol, polH, origin, signal))
+        _check(load_library().fdtd_set_tfsf(self.h, b, _dptr(k), _dptr(p), _dptr(ph), _dptr(o), aux_ncells,
+                                            _dptr(s), s.size))
+
+    def add_dft(self, region, comp_mask, phaseE, phaseH):
+        pe, ph = _f64(phaseE), _f64(phaseH)
+        nphase, nfreq = pe.shape[0], pe.shape[1]
+        r = (ctypes.c_int * 6)(*region)
+        idx = ctypes.c_int()
+        _check(load_library().fdtd_add_dft(self.h, r, comp_mask, nfreq, _dptr(pe), _dptr(ph), nphase,
+                                           ctypes.byref(idx)))
+        region = list(region)
+        if self.mode != "ThreeD":
+            region[4], region[5] = 0, 1
+        ncomp = bin(comp_mask).count("1")
+        ncell = (region[1] - region[0]) * (region[3] - region[2]) * (region[5] - region[4])
+        self.dft_meta.append((ncomp, nfreq, ncell, region))
+        return idx.value
+
+    def get_dft(self, idx):
+        ncomp, nfreq, ncell, region = self.dft_meta[idx]
+        out = np.zeros(ncomp * nfreq * 2 * ncell)
+        _check(load_library().fdtd_get_dft(self.h, idx, _dptr(out)))
+        shp = (region[5] - region[4], region[3] - region[2], region[1] - region[0])
+        return out.reshape(ncomp, nfreq, 2, *shp)
+
+    def add_probe(self, field, comp, i, j, k=0):
+        _check(load_library().fdtd_add_probe(self.h, field, comp, i, j, k))
+        self._nprobes += 1
+
+    def probes(self):
+        n = self.step_count()
+        out = np.zeros(max(1, n * self._nprobes))
+        got = ctypes.c_int()
+        _check(load_library().fdtd_get_probes(self.h, _dptr(out), n, ctypes.byref(got)))
+        return out[: got.value * self._nprobes].reshape(got.value, self._nprobes)
+
+    # ---- fields
+    def get(self, field, comp, pole=0):
+        out = np.zeros(int(np.prod(self.shape)))
+        _check(load_library().fdtd_download_field(self.h, field, comp, pole, _dptr(out)))
+        return out.reshape(self.shape)
+
+    def set(self, field, comp, arr, pole=0):
+        a = _f64(arr).reshape(-1)
+        _check(load_library().fdtd_upload_field(self.h, field, comp, pole, _dptr(a)))
+
+    # ---- stepping
+    def step(self, n=1):
+        _check(load_library().fdtd_step(self.h, n))
+
+    def synchronize(self):
+        _check(load_library().fdtd_synchronize(self.h))
+
+    def step_count(self):
+        return load_library().fdtd_step_count(self.h)
+
+    def set_tile(self, bx, by, kc):
+        _check(load_library().fdtd_set_tile(self.h, bx, by, kc))
+
+    def get_tile(self):
+        v = [ctypes.c_int() for _ in range(3)]
+        _check(load_library().fdtd_get_tile(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_profiling(self, on=True):
+        _check(load_library().fdtd_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_time(self, kernel):
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        _check(load_library().fdtd_kernel_time(self.h, kernel, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,146 @@
+/* fdtd_hip.h — C ABI of the MI355X FDTD hot path.
+ *
+ * One opaque grid handle owns all device state of one simulation: SoA field
+ * arrays, CPML auxiliary (psi) slabs, per-axis CPML profiles, a uint8
+ * material-id volume with a material table, a dense dispersive box (D, P, J),
+ * the TFSF incident line, running-DFT accumulators and probe traces.
+ * fdtd_step() runs the reference's whole step composition, which the
+ * reference spells as one functor sweep per stage (tests/testTM.cpp:102-113,
+ * tests/test3D.cpp:146-171):
+ *   D curl -> soft sources -> CPML(E) -> TFSF updateD + aux step -> E material
+ *   -> B curl -> sources -> CPML(H) -> TFSF updateB -> H material
+ * plus the running DFT and probe capture.  All pointers here are host
+ * pointers; the library copies to/from device memory.  Every function returns
+ * 0 on success or a negative FDTD_E* code; fdtd_last_error() then holds a
+ * thread-local message.  No exceptions cross this boundary.
+ *
+ * Index convention: cell (i,j,k), x fastest: q = i + nx*(j + ny*k).  The
+ * outermost layer of cells (index 0 and N-1 on every axis of the mode) is
+ * never updated, as in the reference (tests/testTM.cpp:103).
+ */
+#ifndef FDTD_HIP_H
+#define FDTD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fdtd_status {
+  FDTD_OK = 0,
+  FDTD_EINVAL = -1,   /* bad argument */
+  FDTD_ENOMEM = -2,   /* device allocation failed */
+  FDTD_EDEVICE = -3,  /* HIP runtime error */
+  FDTD_ESTATE = -4    /* call not valid in the grid's current state */
+};
+
+/* Modes, as in include/FDTDConstants.hpp:424-445 (TEM exists only as the
+ * TFSF incident line inside the library). */
+enum fdtd_mode { FDTD_TM = 2, FDTD_3D = 3 };
+enum fdtd_precision { FDTD_F32 = 0, FDTD_F64 = 1 };
+/* field codes */
+enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5 };
+/* material pole kinds */
+enum fdtd_pole { FDTD_DRUDE = 1, FDTD_LORENTZ = 2 };
+
+#define FDTD_MAX_MATERIALS 256
+#define FDTD_MAX_POLES 3
+#define FDTD_MAX_SOURCES 16
+#define FDTD_MAX_DFT 8
+#define FDTD_MAX_PROBES 32
+
+typedef struct fdtd_grid fdtd_grid;
+
+typedef struct {
+  int mode;      /* FDTD_TM or FDTD_3D */
+  int precision; /* FDTD_F32 or FDTD_F64: storage and arithmetic type */
+  int nx, ny, nz; /* cells per axis including the outer layer; nz = 1 for TM */
+  double dx;     /* cell size [m] */
+  double dt;     /* time step [s] */
+} fdtd_grid_desc;
+
+/* Replaces the user-built std::vector<YeeCell> grid + neighbour pointers of
+ * tests/test3D.cpp:33-49 (YeeCell.hpp:74-97, YeeFields.hpp:11-174). */
+int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out);
+int fdtd_grid_destroy(fdtd_grid* g);
+/* hipStream_t to launch on (NULL = the default stream). */
+int fdtd_set_stream(fdtd_grid* g, void* hip_stream);
+
+/* NewPML::setPMLParameters (include/PML.hpp:84-102) for every index of one
+ * axis: K, S, A are arrays of the axis length; ft 0 = electric, 1 = magnetic.
+ * The reference stores these per cell; here they are per-axis profiles. */
+int fdtd_set_pml_axis(fdtd_grid* g, int axis, int ft, const double* K, const double* S, const double* A);
+/* PMLParameterModel (include/PML.hpp:368-393) with the reference's placement
+ * idiom (tests/testTM.cpp:46-71): n graded cells at both ends of `axis`. */
+int fdtd_set_pml_idiom(fdtd_grid* g, int axis, int n, double m, double ma, double smax, double kmax, double amax);
+
+/* Material table entry `id`: replaces MaterialPair/MaterialMap
+ * (include/MaterialMap.hpp:45-284) and the Vacuum/Constant/Drude/Lorentz
+ * update functors (include/DispersiveMaterials.hpp:342-497, 692-884,
+ * 898-1217).  poles: npoles x {kind, freq [rad/s], gamma [1/s], weight}.
+ * One Drude pole of weight 1 is the reference's DrudeUpdate; one Lorentz pole
+ * of weight 1 is its LorentzUpdate (which never reads its delta). */
+int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npoles, const double* poles);
+/* per-cell material id, nx*ny*nz bytes */
+int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids);
+
+/* make_soft_source + SoftSource::apply (include/Source.hpp:183-261): at step
+ * n, table[n] is added to D_comp (field FDTD_D) or B_comp (FDTD_B) of cell
+ * (i,j,k) right after the curl, before CPML and the material step. */
+int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, const double* table, int ntable);
+
+/* TFSF box = make_tfsf_plane x 2*dim faces sharing one Aux1D history
+ * (include/Source.hpp:280-634).  Faces lie on cell layers box[0],box[1]
+ * (x), box[2],box[3] (y), box[4],box[5] (z).  kvec is normalised as in
+ * TFSFPlane's constructor.  pol multiplies the aux E (updateB), polH the aux
+ * H (updateD).  origin: cell-unit coordinates of the location-functor origin
+ * (cell centre r = (i - origin_x) dx, ...).  aux_ncells sizes the Aux1D line;
+ * signal[n] is its soft-source value at its n-th step. */
+int fdtd_set_tfsf(fdtd_grid* g, const int box[6], const double kvec[3], const double pol[3], const double polH[3],
+                  const double origin[3], int aux_ncells, const double* signal, int nsignal);
+
+/* Running DFT over the half-open region [i0,i1)x[j0,j1)x[k0,k1) (no
+ * counterpart in the reference: include/DataCapture.hpp:61-191 is empty).
+ * comp_mask bit c selects E_c (c<3) or H_{c-3}.  phaseE/phaseH:
+ * [nphase][nfreq][2] = (cos, -sin)(w t) dt with E sampled after step n at
+ * t=(n+1)dt and H at t=(n+1.5)dt.  *idx receives the monitor index. */
+int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, const double* phaseE,
+                 const double* phaseH, int nphase, int* idx);
+/* [ncomp][nfreq][2][cells of region] as float64 */
+int fdtd_get_dft(fdtd_grid* g, int idx, double* out);
+
+/* Probe: field value at one cell recorded after every step. */
+int fdtd_add_probe(fdtd_grid* g, int field, int comp, int i, int j, int k);
+/* out: [nsteps][nprobes] float64, nsteps <= max_steps; *nsteps_out set */
+int fdtd_get_probes(fdtd_grid* g, double* out, int max_steps, int* nsteps_out);
+
+/* Host<->device copies of one component (nx*ny*nz float64 values).  D and B
+ * read back as eps_r eps0 E and mu_r mu0 H outside the dispersive box. */
+int fdtd_upload_field(fdtd_grid* g, int field, int comp, int pole, const double* host);
+int fdtd_download_field(fdtd_grid* g, int field, int comp, int pole, double* host);
+
+/* Advance nsteps whole time steps (asynchronous on the grid's stream). */
+int fdtd_step(fdtd_grid* g, int nsteps);
+int fdtd_synchronize(fdtd_grid* g);
+/* steps taken so far */
+long fdtd_step_count(const fdtd_grid* g);
+
+/* Tile of the per-step passes: bx*by threads per block (a multiple of 64,
+ * at most 256) covering bx x by cells of an xy plane, each thread marching kc
+ * z planes.  Results do not depend on the tile; only speed does. */
+int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc);
+int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc);
+
+/* Per-kernel timing with hipEvents on the grid's stream (off by default).
+ * fdtd_kernel_time returns accumulated milliseconds and launches for
+ * kernel id 0 = E pass, 1 = H pass, 2 = aux line. */
+int fdtd_set_profiling(fdtd_grid* g, int on);
+int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches);
+
+const char* fdtd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

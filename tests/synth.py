@@ -1,0 +1,139 @@
+"""Seeded synthetic scenarios of the benchmark shapes, scaled by size.
+
+grid3d_cpml_tfsf_dft     3-D CPML + dielectric blocks + TFSF plane wave + DFT planes
+                         (256^3 x 2000 steps at full size)
+grid3d_dispersive_sphere 3-D Drude + 2-Lorentz sphere, dipole source, DFT box surface
+                         (192^3 x 3000 steps at full size)
+tm2d_c1                  2-D TM 256^2, 10-cell CPML, centre Gaussian pulse, 4 probes
+"""
+import math
+
+import numpy as np
+
+from scenarios import C0, D, E, EPS0, IMP0, H, SplitMix64, dft_phases, gauss_table, gmsin_signal
+
+DRUDE, LORENTZ = 1, 2
+
+
+def _pml(n, dx):
+    return dict(n=n, m=3.0, ma=1.0, smax=0.8 * (3.0 + 1) / (IMP0 * dx), kmax=5.0, amax=0.05)
+
+
+def _dims(n):
+    return (n, n, n) if isinstance(n, int) else tuple(n)
+
+
+def seed_blocks(matid, nblocks, lo_margin, rng, side_lo, side_hi, eps_lo=1.0, eps_hi=12.0, first_id=1):
+    """Axis-aligned blocks inside the non-PML interior; later blocks override earlier."""
+    nz, ny, nx = matid.shape
+    mats = []
+    for b in range(nblocks):
+        s = [rng.randint(side_lo, side_hi) for _ in range(3)]
+        lo = []
+        for ax, L in enumerate((nx, ny, nz)):
+            hi = max(lo_margin, L - lo_margin - s[ax])
+            lo.append(rng.randint(lo_margin, hi))
+        eps = eps_lo + (eps_hi - eps_lo) * rng.uniform()
+        mid = first_id + b
+        mats.append((eps, 1.0, []))
+        x1 = min(nx - lo_margin, lo[0] + s[0])
+        y1 = min(ny - lo_margin, lo[1] + s[1])
+        z1 = min(nz - lo_margin, lo[2] + s[2])
+        matid[lo[2]:z1, lo[1]:y1, lo[0]:x1] = mid
+    return mats
+
+
+def grid3d_cpml_tfsf_dft(n=256, nsteps=2000, npml=None, nblocks=None, nfreq=None, inset=None, seed=2,
+                         tfsf=True, dft=True):
+    nx, ny, nz = _dims(n)
+    small = min(nx, ny, nz) < 128
+    npml = npml if npml is not None else (5 if small else 12)
+    nblocks = nblocks if nblocks is not None else (6 if small else 48)
+    nfreq = nfreq if nfreq is not None else (4 if small else 8)
+    inset = inset if inset is not None else (max(npml + 3, min(nx, ny, nz) // 6) if small else 20)
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(3.0))
+    rng = SplitMix64(seed)
+    matid = np.zeros((nz, ny, nx), np.uint8)
+    side_hi = max(3, min(nx, ny, nz) // 4)
+    blocks = seed_blocks(matid, nblocks, npml + 2, rng, max(2, side_hi // 8), side_hi)
+    sc = {
+        "name": f"grid3d_cpml_tfsf_dft_{nx}x{ny}x{nz}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz,
+        "dx": dx, "dt": dt, "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, True, True],
+        "materials": [(1.0, 1.0, [])] + blocks, "matid": matid, "sources": [], "probes": [], "dfts": [],
+        "tfsf": None,
+    }
+    if tfsf:
+        box = [inset, nx - 1 - inset, inset, ny - 1 - inset, inset, nz - 1 - inset]
+        f0 = C0 / (20 * dx)
+        tw = 2.0 / f0
+        t0 = 3.0 * tw
+        origin = [math.floor(0.5 * (box[0] + box[1])) + 0.5, math.floor(0.5 * (box[2] + box[3])) + 0.5,
+                  math.floor(0.5 * (box[4] + box[5])) + 0.5]
+        ext = max(box[1] - box[0], box[3] - box[2], box[5] - box[4]) + 12
+        ext += ext % 2
+        sc["tfsf"] = dict(box=box, kvec=[1.0, 0.0, 0.0], pol=[0.0, 0.0, 1.0],
+                          polH=[0.0, 1.0, 0.0],  # pol x k: consistent with the aux line's (Ez, Hy)
+                          origin=origin, aux_ncells=ext, signal=gmsin_signal(nsteps, dt, f0, tw, t0))
+    else:
+        sc["sources"].append((D, 2, nx // 2, ny // 2, nz // 2, gauss_table(nsteps, 40, 12)))
+    if dft:
+        freqs = np.linspace(0.6, 1.4, nfreq) * C0 / (20 * dx)
+        pe, ph = dft_phases(nsteps, dt, freqs)
+        for kz in (nz // 3, (2 * nz) // 3):
+            sc["dfts"].append(([1, nx - 1, 1, ny - 1, kz, kz + 1], 63, pe, ph))
+    sc["probes"] = [(E, 2, nx // 2, ny // 2, nz // 2), (H, 1, nx // 2 + 1, ny // 2, nz // 2)]
+    return sc
+
+
+def grid3d_dispersive_sphere(n=192, radius=40, nsteps=3000, npml=None, nfreq=None, seed=3):
+    nx, ny, nz = _dims(n)
+    small = min(nx, ny, nz) < 96
+    npml = npml if npml is not None else (5 if small else 12)
+    nfreq = nfreq if nfreq is not None else (4 if small else 16)
+    dx = 1e-8  # 10 nm cells: omega_p dt ~ 0.13
+    dt = 0.5 * dx / (C0 * math.sqrt(3.0))
+    rng = SplitMix64(seed)
+    w0 = [2e15 + 4e15 * rng.uniform() for _ in range(2)]
+    poles = [(DRUDE, 1.37e16, 1e14, 1.0), (LORENTZ, w0[0], 1e14, 1.5), (LORENTZ, w0[1], 1e14, 0.8)]
+    cx, cy, cz = nx // 2, ny // 2, nz // 2
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    inside = (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 <= radius * radius
+    matid = np.where(inside, 1, 0).astype(np.uint8)
+    src_x = min(nx - npml - 3, cx + radius + 3)
+    sc = {
+        "name": f"grid3d_dispersive_sphere_{nx}x{ny}x{nz}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz,
+        "dx": dx, "dt": dt, "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, True, True],
+        "materials": [(1.0, 1.0, []), (1.0, 1.0, poles)], "matid": matid,
+        "sources": [(D, 2, src_x, cy, cz, gauss_table(nsteps, 30, 10))], "probes": [], "dfts": [], "tfsf": None,
+    }
+    freqs = np.linspace(1e14, 1.2e15, nfreq)
+    pe, ph = dft_phases(nsteps, dt, freqs)
+    r = radius + 2
+    b = [max(1, cx - r), min(nx - 1, cx + r + 1), max(1, cy - r), min(ny - 1, cy + r + 1), max(1, cz - r),
+         min(nz - 1, cz + r + 1)]
+    faces = [
+        [b[0], b[0] + 1, b[2], b[3], b[4], b[5]], [b[1] - 1, b[1], b[2], b[3], b[4], b[5]],
+        [b[0], b[1], b[2], b[2] + 1, b[4], b[5]], [b[0], b[1], b[3] - 1, b[3], b[4], b[5]],
+        [b[0], b[1], b[2], b[3], b[4], b[4] + 1], [b[0], b[1], b[2], b[3], b[5] - 1, b[5]],
+    ]
+    for f in faces:
+        sc["dfts"].append((f, 63, pe, ph))
+    sc["probes"] = [(E, 2, cx, cy, cz), (E, 2, src_x, cy, cz)]
+    return sc
+
+
+def tm2d_c1(nsteps=1000):
+    """2-D TM 256^2, FP64 in the reference, dx = 1 m, c0 dt/dx = 0.5, 10-cell CPML,
+    Dz[128,128] += eps0 exp(-((n-40)/12)^2), 4 seeded probes in the non-PML interior."""
+    nx = ny = 256
+    dx = 1.0
+    dt = 0.5 * dx / C0
+    rng = SplitMix64(1)
+    probes = [(rng.randint(12, 243), rng.randint(12, 243)) for _ in range(4)]
+    return {
+        "name": "tm2d_c1", "mode": "TM", "nx": nx, "ny": ny, "nz": 1, "dx": dx, "dt": dt, "nsteps": nsteps,
+        "pml": _pml(10, dx), "pml_axes": [True, True, False], "materials": [(1.0, 1.0, [])],
+        "matid": np.zeros((1, ny, nx), np.uint8), "sources": [(D, 2, 128, 128, 0, gauss_table(nsteps, 40, 12))],
+        "probes": [(E, 2, i, j, 0) for i, j in probes], "dfts": [], "tfsf": None,
+    }

@@ -1,0 +1,120 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle and the
+reference's golden vectors, on the same seeded inputs.
+
+Tolerances (relative L2 per field, per DFT bin block):
+  FP64 GPU vs FP64 oracle: 1e-12
+  FP32 GPU vs FP64 oracle: 1e-5
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import refcpu
+from scenarios import D, E, B, H, P, J, apply, from_golden, rel_l2
+import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-12, "f32": 1e-5}
+
+
+def _grid(sc, precision):
+    from fdtd_amd import Grid
+    return Grid(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"], precision=precision)
+
+
+def _oracle(sc):
+    return refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])
+
+
+def _comps(sc):
+    if sc["mode"] == "TM":
+        return [("Ez", E, 2), ("Hx", H, 0), ("Hy", H, 1)]
+    return [("Ex", E, 0), ("Ey", E, 1), ("Ez", E, 2), ("Hx", H, 0), ("Hy", H, 1), ("Hz", H, 2)]
+
+
+def run_pair(sc, precision, nsteps=None):
+    print(f"[gpu-case] {sc['name']} {precision}", flush=True)
+    n = sc["nsteps"] if nsteps is None else nsteps
+    g = apply(_grid(sc, precision), sc)
+    o = apply(_oracle(sc), sc)
+    g.step(n)
+    o.step(n)
+    g.synchronize()
+    return g, o
+
+
+def field_errors(g, o, sc):
+    return {nm: rel_l2(g.get(f, c), o.get(f, c)) for nm, f, c in _comps(sc)}
+
+
+GOLDEN = ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small", "grid3d_disp", "tm2d_c1"]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("name", GOLDEN)
+def test_golden_scenarios(name, precision):
+    sc, p, a = from_golden(name)
+    g, o = run_pair(sc, precision)
+    errs = field_errors(g, o, sc)
+    print(name, precision, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= TOL[precision], (k, v)
+    # probes against the oracle's and, for FP64, the reference's own traces
+    pr_g, pr_o = g.probes(), o.probes()
+    assert pr_g.shape == pr_o.shape
+    assert rel_l2(pr_g, pr_o) <= TOL[precision] * 10
+    # fields against the reference's golden arrays directly
+    for nm, f, c in _comps(sc):
+        if nm in a:
+            assert rel_l2(g.get(f, c).reshape(a[nm].shape), a[nm]) <= TOL[precision], nm
+    if "dispersive_q_D_P_J" in a:
+        dq = a["dispersive_q_D_P_J"]
+        q = dq[:, 0].astype(int)
+        for c in range(3):
+            assert rel_l2(g.get(D, c).reshape(-1)[q], dq[:, 1 + c]) <= TOL[precision]
+            assert rel_l2(g.get(P, c).reshape(-1)[q], dq[:, 4 + c]) <= TOL[precision] * 10
+            assert rel_l2(g.get(J, c).reshape(-1)[q], dq[:, 7 + c]) <= TOL[precision] * 10
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_3d_cpml_tfsf_dft(precision):
+    sc = synth.grid3d_cpml_tfsf_dft(n=40, nsteps=120)
+    g, o = run_pair(sc, precision)
+    errs = field_errors(g, o, sc)
+    print(precision, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= TOL[precision], (k, v)
+    for idx in range(len(sc["dfts"])):
+        dg, do = g.get_dft(idx), o.get_dft(idx)
+        for comp in range(dg.shape[0]):
+            for f in range(dg.shape[1]):
+                e = rel_l2(dg[comp, f], do[comp, f])
+                assert e <= TOL[precision] * 10, (idx, comp, f, e)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_3d_multipole_sphere(precision):
+    sc = synth.grid3d_dispersive_sphere(n=32, radius=8, nsteps=150)
+    g, o = run_pair(sc, precision)
+    errs = field_errors(g, o, sc)
+    print(precision, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= TOL[precision], (k, v)
+    for pole in range(3):
+        for c in range(3):
+            assert rel_l2(g.get(P, c, pole), o.get(P, c, pole)) <= TOL[precision] * 10
+
+
+def test_anisotropic_3d_f64_chunked_steps():
+    """Stepping in uneven chunks equals one long call (incident table chunking)."""
+    sc = synth.grid3d_cpml_tfsf_dft(n=(30, 26, 22), nsteps=300)
+    from fdtd_amd import Grid
+    g1 = apply(_grid(sc, "f64"), sc)
+    g2 = apply(_grid(sc, "f64"), sc)
+    g1.step(300)
+    for n in (1, 255, 44):
+        g2.step(n)
+    for nm, f, c in _comps(sc):
+        np.testing.assert_array_equal(g1.get(f, c), g2.get(f, c))
