@@ -109,7 +109,7 @@ constexpr int kChunk = 256;  // steps per incident-table chunk
 struct Tile {
   int bx, by, kc;
 };
-constexpr Tile kDefaultTile3D = {64, 4, 16};
+constexpr Tile kDefaultTile3D = {256, 1, 1};
 constexpr Tile kDefaultTileTM = {64, 4, 1};
 
 }  // namespace
@@ -137,6 +137,8 @@ struct fdtd_grid {
   uint8_t* mat = nullptr;
   std::vector<uint8_t> mat_host;
   void* mats_dev = nullptr;
+  void* mie_dev = nullptr;  // T[256]: 1/(eps_r eps0), negated for dispersive ids
+  void* mim_dev = nullptr;  // T[256]: 1/(mu_r mu0)
   MatHost mats[FDTD_MAX_MATERIALS];
 
   // CPML rows {B,C,F,G} per [ft][axis]
@@ -271,6 +273,13 @@ int upload_materials(fdtd_grid* g) {
     }
   }
   HIP_TRY(hipMemcpy(g->mats_dev, m.data(), sizeof(fdtdk::MatDev<T>) * m.size(), hipMemcpyHostToDevice));
+  std::vector<T> ie(FDTD_MAX_MATERIALS), im(FDTD_MAX_MATERIALS);
+  for (int id = 0; id < FDTD_MAX_MATERIALS; id++) {
+    ie[id] = g->mats[id].np > 0 ? -m[id].inv_eps : m[id].inv_eps;
+    im[id] = m[id].inv_mu;
+  }
+  HIP_TRY(hipMemcpy(g->mie_dev, ie.data(), sizeof(T) * ie.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g->mim_dev, im.data(), sizeof(T) * im.size(), hipMemcpyHostToDevice));
   g->mats_dirty = false;
   return 0;
 }
@@ -394,6 +403,8 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   }
   a.mat = g->mat;
   a.mats = (const fdtdk::MatDev<T>*)g->mats_dev;
+  a.mie = (const T*)g->mie_dev;
+  a.mim = (const T*)g->mim_dev;
   for (int ft = 0; ft < 2; ft++)
     for (int I = 0; I < 3; I++) {
       a.cf[ft][I] = (const T*)g->coef[ft][I][0];
@@ -630,6 +641,8 @@ int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   if (!rc) rc = g->alloc((void**)&g->mat, g->ncell);
   g->mat_host.assign(g->ncell, 0);
   if (!rc) rc = g->alloc(&g->mats_dev, sizeof(fdtdk::MatDev<double>) * FDTD_MAX_MATERIALS);
+  if (!rc) rc = g->alloc(&g->mie_dev, 8 * FDTD_MAX_MATERIALS);
+  if (!rc) rc = g->alloc(&g->mim_dev, 8 * FDTD_MAX_MATERIALS);
   for (int ft = 0; ft < 2 && !rc; ft++)
     for (int a = 0; a < 3 && !rc; a++) {
       g->pml[ft][a].assign(4 * g->len[a], 0.0);

@@ -66,6 +66,8 @@ struct StepArgs {
   T* H[3];
   const uint8_t* mat;
   const MatDev<T>* mats;
+  const T* mie;  // 1/(eps_r eps0) per material id, negated for dispersive materials
+  const T* mim;  // 1/(mu_r mu0) per material id
   // CPML: psi[ft][I][J] slab arrays and per-axis coefficient profiles
   T* psi[2][3][3];
   const T* cf[2][3];  // dt F / dx
@@ -170,126 +172,173 @@ __device__ void tfsf_cell(const StepArgs<T>& a, int ft, int i, int j, int k, con
 }
 
 // ---------------------------------------------------------------- E pass
-template <typename T, int MODE>
-__device__ __forceinline__ void e_material(const StepArgs<T>& a, long c, int i, int j, int k, T d[3]) {
-  const MatDev<T>& m = a.mats[a.mat[c]];
-  if (m.np == 0) {
-    if (MODE == 3) {
-      a.E[0][c] += d[0] * m.inv_eps;
-      a.E[1][c] += d[1] * m.inv_eps;
-    }
-    a.E[2][c] += d[2] * m.inv_eps;
-    return;
-  }
-  // dispersive: D += delta, P += dt J, E = (D - sum P)/(eps_r eps0), J update
-  long bq = (long)(i - a.dbox[0]) + (long)(a.dbox[1] - a.dbox[0]) *
-                                        ((long)(j - a.dbox[2]) + (long)(a.dbox[3] - a.dbox[2]) * (long)(k - a.dbox[4]));
-  for (int I = (MODE == 3 ? 0 : 2); I < 3; I++) {
-    T D = a.Dd[I][bq] + d[I];
-    a.Dd[I][bq] = D;
-    T num = D;
-    T Pv[kMaxPoles], Jv[kMaxPoles];
-    for (int p = 0; p < m.np; p++) {
-      Jv[p] = a.Jd[p][I][bq];
-      Pv[p] = a.Pd[p][I][bq] + a.dt * Jv[p];
-      num -= Pv[p];
-    }
-    T Ev = num * m.inv_eps;
-    a.E[I][c] = Ev;
-    for (int p = 0; p < m.np; p++) {
-      a.Pd[p][I][bq] = Pv[p];
-      a.Jd[p][I][bq] = m.cJ[p] * Jv[p] + m.cE[p] * Ev + m.cP[p] * Pv[p];
-    }
-  }
+template <typename T>
+__device__ __forceinline__ T pick3(T v0, T v1, T v2, int c) {
+  return c == 0 ? v0 : (c == 1 ? v1 : v2);
 }
 
 template <typename T>
 __device__ __forceinline__ void add_sources(const StepArgs<T>& a, int field, long c, T d[3]) {
-  for (int s = 0; s < a.nsrc; s++)
-    if (a.src[s].cell == c && a.src[s].field == field && a.step < a.src[s].n) d[a.src[s].comp] += (T)a.src[s].table[a.step];
+  for (int s = 0; s < a.nsrc; s++) {
+    const SrcDev& q = a.src[s];
+    if (q.cell == c && q.field == field && a.step < q.n) {
+      const T v = (T)q.table[a.step];
+      if (q.comp == 0)
+        d[0] += v;
+      else if (q.comp == 1)
+        d[1] += v;
+      else
+        d[2] += v;
+    }
+  }
 }
 
+// Dispersive cell, one component: D += delta; P_p += dt J_p;
+// E = (D - sum_p P_p)/(eps_r eps0); J_p = cJ J_p + cE E + cP P_p
+// (DrudeUpdate / LorentzUpdate implicit_update, DispersiveMaterials.hpp:710-735, 979-998).
+template <typename T, int I>
+__device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>& m, long bq, T d) {
+  const T D = a.Dd[I][bq] + d;
+  a.Dd[I][bq] = D;
+  T num = D;
+  T Pv[kMaxPoles], Jv[kMaxPoles];
+#pragma unroll
+  for (int p = 0; p < kMaxPoles; p++) {
+    if (p < m.np) {
+      Jv[p] = a.Jd[p][I][bq];
+      Pv[p] = a.Pd[p][I][bq] + a.dt * Jv[p];
+      num -= Pv[p];
+    }
+  }
+  const T Ev = num * m.inv_eps;
+#pragma unroll
+  for (int p = 0; p < kMaxPoles; p++) {
+    if (p < m.np) {
+      a.Pd[p][I][bq] = Pv[p];
+      a.Jd[p][I][bq] = m.cJ[p] * Jv[p] + m.cE[p] * Ev + m.cP[p] * Pv[p];
+    }
+  }
+  return Ev;
+}
+
+template <typename T>
+__device__ __forceinline__ long dbox_index(const StepArgs<T>& a, int i, int j, int k) {
+  return (long)(i - a.dbox[0]) +
+         (long)(a.dbox[1] - a.dbox[0]) * ((long)(j - a.dbox[2]) + (long)(a.dbox[3] - a.dbox[2]) * (long)(k - a.dbox[4]));
+}
+
+// One thread per cell and z plane: every load of the cell is issued before
+// any arithmetic, so a wave carries ~12 independent loads in flight and the
+// stores go out at the end.  mie[m] < 0 marks a dispersive material.
 template <typename T>
 __global__ __launch_bounds__(256) void e_pass_3d(StepArgs<T> a, int kc) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k0 = 1 + blockIdx.z * kc;
   const int k1 = min(k0 + kc, a.nz - 1);
-  if (i >= a.nx - 1 || j >= a.ny - 1 || k0 >= k1) return;
+  if (i >= a.nx - 1 || j >= a.ny - 1) return;
+  const T* __restrict__ Hx = a.H[0];
+  const T* __restrict__ Hy = a.H[1];
+  const T* __restrict__ Hz = a.H[2];
+  T* __restrict__ Ex = a.E[0];
+  T* __restrict__ Ey = a.E[1];
+  T* __restrict__ Ez = a.E[2];
+  const uint8_t* __restrict__ mat = a.mat;
+  const T* __restrict__ mie = a.mie;
   const long sy = a.sy, sz = a.sz;
-  long c = (long)i + (long)a.nx * ((long)j + (long)a.ny * (k0 - 1));
-  T hx_p = a.H[0][c], hy_p = a.H[1][c];
   const T fx = a.cf[0][0][i], fy = a.cf[0][1][j];
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   for (int k = k0; k < k1; k++) {
-    c += sz;
-    const T hx = a.H[0][c], hy = a.H[1][c], hz = a.H[2][c];
-    const T dxy = hz - a.H[2][c - sy];  // Ex, J=y, +1
-    const T dxz = hy - hy_p;            // Ex, J=z, -1
-    const T dyz = hx - hx_p;            // Ey, J=z, +1
-    const T dyx = hz - a.H[2][c - 1];   // Ey, J=x, -1
-    const T dzx = hy - a.H[1][c - 1];   // Ez, J=x, +1
-    const T dzy = hx - a.H[0][c - sy];  // Ez, J=y, -1
-    hx_p = hx;
-    hy_p = hy;
+    const long c = (long)i + sy * (long)j + sz * (long)k;
+    const T hx = Hx[c], hy = Hy[c], hz = Hz[c];
+    const T hz_ym = Hz[c - sy], hz_xm = Hz[c - 1];
+    const T hy_xm = Hy[c - 1], hy_zm = Hy[c - sz];
+    const T hx_ym = Hx[c - sy], hx_zm = Hx[c - sz];
+    const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    const T ie = mie[mat[c]];
     const T fz = a.cf[0][2][k];
+    const T dxy = hz - hz_ym;  // Ex, J=y, +1
+    const T dxz = hy - hy_zm;  // Ex, J=z, -1
+    const T dyz = hx - hx_zm;  // Ey, J=z, +1
+    const T dyx = hz - hz_xm;  // Ey, J=x, -1
+    const T dzx = hy - hy_xm;  // Ez, J=x, +1
+    const T dzy = hx - hx_ym;  // Ez, J=y, -1
     T d[3];
     d[0] = fy * dxy - fz * dxz;
     d[1] = fz * dyz - fx * dyx;
     d[2] = fx * dzx - fy * dzy;
     if (a.nsrc) add_sources(a, 0, c, d);
     if (sxi >= 0) {
-      long o = slab_off(a, 0, sxi, i, j, k);
+      const long o = slab_off(a, 0, sxi, i, j, k);
+      T* __restrict__ q1 = a.psi[0][1][0];
+      T* __restrict__ q2 = a.psi[0][2][0];
       const T g = a.cg[0][0][i], b = a.cb[0][0][i], cc = a.cc[0][0][i];
-      T p1 = a.psi[0][1][0][o], p2 = a.psi[0][2][0][o];
+      const T p1 = q1[o], p2 = q2[o];
       d[1] += g * p1;
       d[2] += g * p2;
-      a.psi[0][1][0][o] = b * p1 - cc * dyx;
-      a.psi[0][2][0][o] = b * p2 + cc * dzx;
+      q1[o] = b * p1 - cc * dyx;
+      q2[o] = b * p2 + cc * dzx;
     }
     if (syi >= 0) {
-      long o = slab_off(a, 1, syi, i, j, k);
+      const long o = slab_off(a, 1, syi, i, j, k);
+      T* __restrict__ q0 = a.psi[0][0][1];
+      T* __restrict__ q2 = a.psi[0][2][1];
       const T g = a.cg[0][1][j], b = a.cb[0][1][j], cc = a.cc[0][1][j];
-      T p0 = a.psi[0][0][1][o], p2 = a.psi[0][2][1][o];
+      const T p0 = q0[o], p2 = q2[o];
       d[0] += g * p0;
       d[2] += g * p2;
-      a.psi[0][0][1][o] = b * p0 + cc * dxy;
-      a.psi[0][2][1][o] = b * p2 - cc * dzy;
+      q0[o] = b * p0 + cc * dxy;
+      q2[o] = b * p2 - cc * dzy;
     }
     const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
     if (szi >= 0) {
-      long o = slab_off(a, 2, szi, i, j, k);
+      const long o = slab_off(a, 2, szi, i, j, k);
+      T* __restrict__ q0 = a.psi[0][0][2];
+      T* __restrict__ q1 = a.psi[0][1][2];
       const T g = a.cg[0][2][k], b = a.cb[0][2][k], cc = a.cc[0][2][k];
-      T p0 = a.psi[0][0][2][o], p1 = a.psi[0][1][2][o];
+      const T p0 = q0[o], p1 = q1[o];
       d[0] += g * p0;
       d[1] += g * p1;
-      a.psi[0][0][2][o] = b * p0 - cc * dxz;
-      a.psi[0][1][2][o] = b * p1 + cc * dyz;
+      q0[o] = b * p0 - cc * dxz;
+      q1[o] = b * p1 + cc * dyz;
     }
     if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
       const bool act[3] = {true, true, true};
       tfsf_cell<T, 3>(a, 0, i, j, k, act, act, d);
     }
-    e_material<T, 3>(a, c, i, j, k, d);
+    if (ie > T(0)) {
+      Ex[c] = ex + d[0] * ie;
+      Ey[c] = ey + d[1] * ie;
+      Ez[c] = ez + d[2] * ie;
+    } else {
+      const MatDev<T>& m = a.mats[mat[c]];
+      const long bq = dbox_index(a, i, j, k);
+      Ex[c] = ade_component<T, 0>(a, m, bq, d[0]);
+      Ey[c] = ade_component<T, 1>(a, m, bq, d[1]);
+      Ez[c] = ade_component<T, 2>(a, m, bq, d[2]);
+    }
   }
 }
 
 // ---------------------------------------------------------------- H pass
-template <typename T, int MODE>
+template <typename T>
 __device__ __forceinline__ void dft_probe(const StepArgs<T>& a, long c, int i, int j, int k, const T e[3], const T h[3]) {
   for (int q = 0; q < a.ndft; q++) {
     const DftDev<T>& d = a.dft[q];
     if (i < d.r[0] || i >= d.r[1] || j < d.r[2] || j >= d.r[3] || k < d.r[4] || k >= d.r[5]) continue;
     if (a.step >= d.nphase) continue;
-    long lq = (long)(i - d.r[0]) + (long)(d.r[1] - d.r[0]) * ((long)(j - d.r[2]) + (long)(d.r[3] - d.r[2]) * (k - d.r[4]));
+    const long lq =
+        (long)(i - d.r[0]) + (long)(d.r[1] - d.r[0]) * ((long)(j - d.r[2]) + (long)(d.r[3] - d.r[2]) * (k - d.r[4]));
     int slot = 0;
+#pragma unroll
     for (int comp = 0; comp < 6; comp++) {
       if (!((d.mask >> comp) & 1)) continue;
       const T v = comp < 3 ? e[comp] : h[comp - 3];
       const double* ph = (comp < 3 ? d.phE : d.phH) + 2L * d.nfreq * a.step;
+      T* acc = d.acc + (long)slot * d.nfreq * 2 * d.ncells + lq;
       for (int f = 0; f < d.nfreq; f++) {
-        T* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
+        T* re = acc + (long)f * 2 * d.ncells;
         re[0] += v * (T)ph[2 * f];
         re[d.ncells] += v * (T)ph[2 * f + 1];
       }
@@ -299,8 +348,8 @@ __device__ __forceinline__ void dft_probe(const StepArgs<T>& a, long c, int i, i
   for (int p = 0; p < a.nprobe; p++)
     if (a.probe[p].cell == c) {
       const int comp = a.probe[p].comp;
-      double v = a.probe[p].field == 1 ? (double)e[comp] : (double)h[comp];
-      a.trace[a.step * a.nprobe + p] = v;
+      const T v = a.probe[p].field == 1 ? pick3(e[0], e[1], e[2], comp) : pick3(h[0], h[1], h[2], comp);
+      a.trace[a.step * a.nprobe + p] = (double)v;
     }
 }
 
@@ -310,76 +359,89 @@ __global__ __launch_bounds__(256) void h_pass_3d(StepArgs<T> a, int kc) {
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k0 = 1 + blockIdx.z * kc;
   const int k1 = min(k0 + kc, a.nz - 1);
-  if (i >= a.nx - 1 || j >= a.ny - 1 || k0 >= k1) return;
+  if (i >= a.nx - 1 || j >= a.ny - 1) return;
+  const T* __restrict__ Ex = a.E[0];
+  const T* __restrict__ Ey = a.E[1];
+  const T* __restrict__ Ez = a.E[2];
+  T* __restrict__ Hx = a.H[0];
+  T* __restrict__ Hy = a.H[1];
+  T* __restrict__ Hz = a.H[2];
+  const uint8_t* __restrict__ mat = a.mat;
+  const T* __restrict__ mim = a.mim;
   const long sy = a.sy, sz = a.sz;
-  long c = (long)i + (long)a.nx * ((long)j + (long)a.ny * k0);
-  T ex = a.E[0][c], ey = a.E[1][c], ez = a.E[2][c];
   const T fx = a.cf[1][0][i], fy = a.cf[1][1][j];
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   const bool extra = a.ndft || a.nprobe;
   for (int k = k0; k < k1; k++) {
-    const T ex_n = a.E[0][c + sz], ey_n = a.E[1][c + sz];
-    const T dxy = a.E[2][c + sy] - ez;  // Hx, J=y, +1
-    const T dxz = ey_n - ey;            // Hx, J=z, -1
-    const T dyz = ex_n - ex;            // Hy, J=z, +1
-    const T dyx = a.E[2][c + 1] - ez;   // Hy, J=x, -1
-    const T dzx = a.E[1][c + 1] - ey;   // Hz, J=x, +1
-    const T dzy = a.E[0][c + sy] - ex;  // Hz, J=y, -1
+    const long c = (long)i + sy * (long)j + sz * (long)k;
+    const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    const T ez_yp = Ez[c + sy], ez_xp = Ez[c + 1];
+    const T ey_xp = Ey[c + 1], ey_zp = Ey[c + sz];
+    const T ex_yp = Ex[c + sy], ex_zp = Ex[c + sz];
+    const T hx = Hx[c], hy = Hy[c], hz = Hz[c];
+    const T im = mim[mat[c]];
     const T fz = a.cf[1][2][k];
+    const T dxy = ez_yp - ez;  // Hx, J=y, +1
+    const T dxz = ey_zp - ey;  // Hx, J=z, -1
+    const T dyz = ex_zp - ex;  // Hy, J=z, +1
+    const T dyx = ez_xp - ez;  // Hy, J=x, -1
+    const T dzx = ey_xp - ey;  // Hz, J=x, +1
+    const T dzy = ex_yp - ex;  // Hz, J=y, -1
     T d[3];
     d[0] = fz * dxz - fy * dxy;
     d[1] = fx * dyx - fz * dyz;
     d[2] = fy * dzy - fx * dzx;
     if (a.nsrc) add_sources(a, 2, c, d);
     if (sxi >= 0) {
-      long o = slab_off(a, 0, sxi, i, j, k);
+      const long o = slab_off(a, 0, sxi, i, j, k);
+      T* __restrict__ q1 = a.psi[1][1][0];
+      T* __restrict__ q2 = a.psi[1][2][0];
       const T g = a.cg[1][0][i], b = a.cb[1][0][i], cc = a.cc[1][0][i];
-      T p1 = a.psi[1][1][0][o], p2 = a.psi[1][2][0][o];
+      const T p1 = q1[o], p2 = q2[o];
       d[1] -= g * p1;
       d[2] -= g * p2;
-      a.psi[1][1][0][o] = b * p1 - cc * dyx;
-      a.psi[1][2][0][o] = b * p2 + cc * dzx;
+      q1[o] = b * p1 - cc * dyx;
+      q2[o] = b * p2 + cc * dzx;
     }
     if (syi >= 0) {
-      long o = slab_off(a, 1, syi, i, j, k);
+      const long o = slab_off(a, 1, syi, i, j, k);
+      T* __restrict__ q0 = a.psi[1][0][1];
+      T* __restrict__ q2 = a.psi[1][2][1];
       const T g = a.cg[1][1][j], b = a.cb[1][1][j], cc = a.cc[1][1][j];
-      T p0 = a.psi[1][0][1][o], p2 = a.psi[1][2][1][o];
+      const T p0 = q0[o], p2 = q2[o];
       d[0] -= g * p0;
       d[2] -= g * p2;
-      a.psi[1][0][1][o] = b * p0 + cc * dxy;
-      a.psi[1][2][1][o] = b * p2 - cc * dzy;
+      q0[o] = b * p0 + cc * dxy;
+      q2[o] = b * p2 - cc * dzy;
     }
     const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
     if (szi >= 0) {
-      long o = slab_off(a, 2, szi, i, j, k);
+      const long o = slab_off(a, 2, szi, i, j, k);
+      T* __restrict__ q0 = a.psi[1][0][2];
+      T* __restrict__ q1 = a.psi[1][1][2];
       const T g = a.cg[1][2][k], b = a.cb[1][2][k], cc = a.cc[1][2][k];
-      T p0 = a.psi[1][0][2][o], p1 = a.psi[1][1][2][o];
+      const T p0 = q0[o], p1 = q1[o];
       d[0] -= g * p0;
       d[1] -= g * p1;
-      a.psi[1][0][2][o] = b * p0 - cc * dxz;
-      a.psi[1][1][2][o] = b * p1 + cc * dyz;
+      q0[o] = b * p0 - cc * dxz;
+      q1[o] = b * p1 + cc * dyz;
     }
     if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
       const bool act[3] = {true, true, true};
       tfsf_cell<T, 3>(a, 1, i, j, k, act, act, d);
     }
-    const T im = a.mats[a.mat[c]].inv_mu;
     T h[3];
-    h[0] = a.H[0][c] + d[0] * im;
-    h[1] = a.H[1][c] + d[1] * im;
-    h[2] = a.H[2][c] + d[2] * im;
-    a.H[0][c] = h[0];
-    a.H[1][c] = h[1];
-    a.H[2][c] = h[2];
+    h[0] = hx + d[0] * im;
+    h[1] = hy + d[1] * im;
+    h[2] = hz + d[2] * im;
+    Hx[c] = h[0];
+    Hy[c] = h[1];
+    Hz[c] = h[2];
     if (extra) {
       const T e[3] = {ex, ey, ez};
-      dft_probe<T, 3>(a, c, i, j, k, e, h);
+      dft_probe<T>(a, c, i, j, k, e, h);
     }
-    c += sz;
-    ex = ex_n;
-    ey = ey_n;
-    if (k + 1 < k1) ez = a.E[2][c];
   }
 }
 
@@ -390,32 +452,44 @@ __global__ __launch_bounds__(256) void e_pass_tm(StepArgs<T> a, int) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const long c = (long)i + (long)a.nx * j;
-  const T hx = a.H[0][c], hy = a.H[1][c];
-  const T dzx = hy - a.H[1][c - 1];     // J=x, +1
-  const T dzy = hx - a.H[0][c - a.sy];  // J=y, -1
-  T d[3] = {0, 0, 0};
+  const T* __restrict__ Hx = a.H[0];
+  const T* __restrict__ Hy = a.H[1];
+  T* __restrict__ Ez = a.E[2];
+  const long c = (long)i + a.sy * j;
+  const T hx = Hx[c], hy = Hy[c], hy_xm = Hy[c - 1], hx_ym = Hx[c - a.sy];
+  const T ez = Ez[c];
+  const T ie = a.mie[a.mat[c]];
+  const T dzx = hy - hy_xm;  // J=x, +1
+  const T dzy = hx - hx_ym;  // J=y, -1
+  T d[3] = {T(0), T(0), T(0)};
   d[2] = a.cf[0][0][i] * dzx - a.cf[0][1][j] * dzy;
   if (a.nsrc) add_sources(a, 0, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
-    long o = slab_off(a, 0, sxi, i, j, 0);
-    T p = a.psi[0][2][0][o];
+    const long o = slab_off(a, 0, sxi, i, j, 0);
+    T* __restrict__ q = a.psi[0][2][0];
+    const T p = q[o];
     d[2] += a.cg[0][0][i] * p;
-    a.psi[0][2][0][o] = a.cb[0][0][i] * p + a.cc[0][0][i] * dzx;
+    q[o] = a.cb[0][0][i] * p + a.cc[0][0][i] * dzx;
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
-    long o = slab_off(a, 1, syi, i, j, 0);
-    T p = a.psi[0][2][1][o];
+    const long o = slab_off(a, 1, syi, i, j, 0);
+    T* __restrict__ q = a.psi[0][2][1];
+    const T p = q[o];
     d[2] += a.cg[0][1][j] * p;
-    a.psi[0][2][1][o] = a.cb[0][1][j] * p - a.cc[0][1][j] * dzy;
+    q[o] = a.cb[0][1][j] * p - a.cc[0][1][j] * dzy;
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
     tfsf_cell<T, 2>(a, 0, i, j, 0, aF, aC, d);
   }
-  e_material<T, 2>(a, c, i, j, 0, d);
+  if (ie > T(0)) {
+    Ez[c] = ez + d[2] * ie;
+  } else {
+    const MatDev<T>& m = a.mats[a.mat[c]];
+    Ez[c] = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2]);
+  }
 }
 
 template <typename T>
@@ -423,39 +497,45 @@ __global__ __launch_bounds__(256) void h_pass_tm(StepArgs<T> a, int) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const long c = (long)i + (long)a.nx * j;
-  const T ez = a.E[2][c];
-  const T dxy = a.E[2][c + a.sy] - ez;  // Hx, J=y, +1
-  const T dyx = a.E[2][c + 1] - ez;     // Hy, J=x, -1
-  T d[3] = {0, 0, 0};
+  const T* __restrict__ Ez = a.E[2];
+  T* __restrict__ Hx = a.H[0];
+  T* __restrict__ Hy = a.H[1];
+  const long c = (long)i + a.sy * j;
+  const T ez = Ez[c], ez_yp = Ez[c + a.sy], ez_xp = Ez[c + 1];
+  const T hx = Hx[c], hy = Hy[c];
+  const T im = a.mim[a.mat[c]];
+  const T dxy = ez_yp - ez;  // Hx, J=y, +1
+  const T dyx = ez_xp - ez;  // Hy, J=x, -1
+  T d[3] = {T(0), T(0), T(0)};
   d[0] = -a.cf[1][1][j] * dxy;
   d[1] = a.cf[1][0][i] * dyx;
   if (a.nsrc) add_sources(a, 2, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
-    long o = slab_off(a, 0, sxi, i, j, 0);
-    T p = a.psi[1][1][0][o];
+    const long o = slab_off(a, 0, sxi, i, j, 0);
+    T* __restrict__ q = a.psi[1][1][0];
+    const T p = q[o];
     d[1] -= a.cg[1][0][i] * p;
-    a.psi[1][1][0][o] = a.cb[1][0][i] * p - a.cc[1][0][i] * dyx;
+    q[o] = a.cb[1][0][i] * p - a.cc[1][0][i] * dyx;
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
-    long o = slab_off(a, 1, syi, i, j, 0);
-    T p = a.psi[1][0][1][o];
+    const long o = slab_off(a, 1, syi, i, j, 0);
+    T* __restrict__ q = a.psi[1][0][1];
+    const T p = q[o];
     d[0] -= a.cg[1][1][j] * p;
-    a.psi[1][0][1][o] = a.cb[1][1][j] * p + a.cc[1][1][j] * dxy;
+    q[o] = a.cb[1][1][j] * p + a.cc[1][1][j] * dxy;
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {true, true, false}, aC[3] = {false, false, true};
     tfsf_cell<T, 2>(a, 1, i, j, 0, aF, aC, d);
   }
-  const T im = a.mats[a.mat[c]].inv_mu;
-  T h[3] = {a.H[0][c] + d[0] * im, a.H[1][c] + d[1] * im, 0};
-  a.H[0][c] = h[0];
-  a.H[1][c] = h[1];
+  T h[3] = {hx + d[0] * im, hy + d[1] * im, T(0)};
+  Hx[c] = h[0];
+  Hy[c] = h[1];
   if (a.ndft || a.nprobe) {
-    const T e[3] = {0, 0, ez};
-    dft_probe<T, 2>(a, c, i, j, 0, e, h);
+    const T e[3] = {T(0), T(0), ez};
+    dft_probe<T>(a, c, i, j, 0, e, h);
   }
 }
 
