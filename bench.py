@@ -99,6 +99,7 @@ def main():
     sc = synth.grid3d_cpml_tfsf_dft(n=args.size, nsteps=total)
     esz = 4 if args.precision == "f32" else 8
     g = apply(Grid("ThreeD", args.size, args.size, args.size, sc["dx"], sc["dt"], precision=args.precision), sc)
+    tile = g.autotune(trial_steps=10)
     g.step(args.warmup)
     g.synchronize()
     torch.cuda.synchronize()
@@ -127,7 +128,7 @@ def main():
         "config": {"workload": f"3-D {args.size}^3 {args.precision.upper()} forward: 12-cell CPML + 48 eps blocks "
                                f"+ TFSF box inset 20 (x-propagating Ez plane wave) + running DFT of 6 comps on "
                                f"2 z-planes x 8 freqs",
-                   "cells": args.size ** 3, "interior_cells": interior},
+                   "cells": args.size ** 3, "interior_cells": interior, "tile": list(tile)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "e_pass_3d + h_pass_3d (one step)",

@@ -131,6 +131,7 @@ struct fdtd_grid {
   hipStream_t stream = nullptr;
   bool actE[3] = {false, false, false}, actH[3] = {false, false, false};
   std::vector<void*> allocs;
+  std::vector<size_t> alloc_bytes;
 
   void* E[3] = {nullptr, nullptr, nullptr};
   void* H[3] = {nullptr, nullptr, nullptr};
@@ -209,13 +210,17 @@ struct fdtd_grid {
       return fail(FDTD_ENOMEM, "hipMalloc(%zu) failed", bytes);
     }
     allocs.push_back(*p);
+    alloc_bytes.push_back(bytes);
     if (zero && hipMemset(*p, 0, bytes) != hipSuccess) return fail(FDTD_EDEVICE, "hipMemset failed");
     return 0;
   }
   void release(void* p) {
     if (!p) return;
     auto it = std::find(allocs.begin(), allocs.end(), p);
-    if (it != allocs.end()) allocs.erase(it);
+    if (it != allocs.end()) {
+      alloc_bytes.erase(alloc_bytes.begin() + (it - allocs.begin()));
+      allocs.erase(it);
+    }
     (void)hipFree(p);
   }
   long q(int i, int j, int k) const { return long(i) + long(d.nx) * (long(j) + long(d.ny) * k); }
@@ -507,6 +512,23 @@ int launch(fdtd_grid* g, int kid, K kern, dim3 grid, dim3 block, const A&... arg
   return 0;
 }
 
+// grow the probe trace so that it holds `steps` rows
+int ensure_trace(fdtd_grid* g, long steps) {
+  if (!g->nprobe || g->trace_cap >= steps) return 0;
+  long cap = std::max<long>(steps, 2 * g->trace_cap);
+  double* nt = nullptr;
+  int rc = g->alloc((void**)&nt, size_t(cap) * g->nprobe * 8);
+  if (rc) return rc;
+  if (g->trace) {
+    HIP_TRY(hipMemcpyAsync(nt, g->trace, size_t(g->step) * g->nprobe * 8, hipMemcpyDeviceToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->release(g->trace);
+  }
+  g->trace = nt;
+  g->trace_cap = cap;
+  return 0;
+}
+
 int run_aux(fdtd_grid* g, long step0, int nsteps) {
   fdtdk::AuxArgs a;
   a.L = g->auxL;
@@ -544,19 +566,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
     int rc = upload_coefs(g);
     if (rc) return rc;
   }
-  if (g->nprobe && g->trace_cap < g->step + nsteps) {
-    long cap = std::max<long>(g->step + nsteps, 2 * g->trace_cap);
-    double* nt = nullptr;
-    int rc = g->alloc((void**)&nt, size_t(cap) * g->nprobe * 8);
-    if (rc) return rc;
-    if (g->trace) {
-      HIP_TRY(hipMemcpyAsync(nt, g->trace, size_t(g->step) * g->nprobe * 8, hipMemcpyDeviceToDevice, g->stream));
-      HIP_TRY(hipStreamSynchronize(g->stream));
-      g->release(g->trace);
-    }
-    g->trace = nt;
-    g->trace_cap = cap;
-  }
+  int rc0 = ensure_trace(g, g->step + nsteps);
+  if (rc0) return rc0;
   fdtdk::StepArgs<T> a;
   fill_args<T>(g, a);
   Tile t = g->tile.bx ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
@@ -920,6 +931,80 @@ int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
   if (bx <= 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
     return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
   g->tile = Tile{bx, by, kc};
+  return 0;
+}
+
+int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
+  if (!g || trial_steps < 1) return fail(FDTD_EINVAL, "bad argument");
+  int rc = 0;
+  if (!g->finalized && (rc = finalize(g))) return rc;
+  static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {128, 2, 1}, {128, 2, 2},
+                               {128, 1, 1}, {64, 4, 1},  {64, 4, 4},  {64, 1, 1}};
+  static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
+  const Tile* cand = g->ndim == 3 ? cand3 : cand2;
+  const int ncand = g->ndim == 3 ? int(sizeof cand3 / sizeof cand3[0]) : int(sizeof cand2 / sizeof cand2[0]);
+  const long step0 = g->step;
+  if ((rc = ensure_trace(g, step0 + long(ncand) * (trial_steps + 1)))) return rc;
+  // pending table uploads go first: the snapshot must hold the tables the
+  // trials will run with, or the restore would roll them back to stale contents
+  if (g->mats_dirty && (rc = g->f64 ? upload_materials<double>(g) : upload_materials<float>(g))) return rc;
+  if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
+  // snapshot every device allocation (fields, psi, dispersive state, DFT
+  // accumulators, aux line, probe trace) so the trials leave no trace
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  std::vector<void*> ptrs = g->allocs;
+  std::vector<size_t> sizes = g->alloc_bytes;
+  size_t total = 0;
+  for (size_t s : sizes) total += (s + 255) & ~size_t(255);
+  char* snap = nullptr;
+  if (hipMalloc((void**)&snap, total) != hipSuccess) return fail(FDTD_ENOMEM, "autotune snapshot (%zu bytes)", total);
+  size_t off = 0;
+  for (size_t q = 0; q < ptrs.size(); q++) {
+    (void)hipMemcpyAsync(snap + off, ptrs[q], sizes[q], hipMemcpyDeviceToDevice, g->stream);
+    off += (sizes[q] + 255) & ~size_t(255);
+  }
+  const bool prof = g->prof;
+  double kms[3];
+  long kcount[3];
+  std::copy(g->kms, g->kms + 3, kms);
+  std::copy(g->kcount, g->kcount + 3, kcount);
+  const Tile saved = g->tile;
+  double best = 1e300;
+  Tile bt = cand[0];
+  for (int c = 0; c < ncand && !rc; c++) {
+    g->tile = cand[c];
+    if ((rc = fdtd_step(g, 1))) break;
+    if ((rc = fdtd_set_profiling(g, 1))) break;
+    if ((rc = fdtd_step(g, trial_steps))) break;
+    double e = 0, h = 0;
+    long ne = 0, nh = 0;
+    if ((rc = fdtd_kernel_time(g, 0, &e, &ne)) || (rc = fdtd_kernel_time(g, 1, &h, &nh))) break;
+    double t = e / std::max(ne, 1L) + h / std::max(nh, 1L);
+    if (t < best) {
+      best = t;
+      bt = cand[c];
+    }
+  }
+  (void)fdtd_set_profiling(g, 0);
+  off = 0;
+  for (size_t q = 0; q < ptrs.size(); q++) {
+    (void)hipMemcpyAsync(ptrs[q], snap + off, sizes[q], hipMemcpyDeviceToDevice, g->stream);
+    off += (sizes[q] + 255) & ~size_t(255);
+  }
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(snap);
+  g->step = step0;
+  g->prof = prof;
+  std::copy(kms, kms + 3, g->kms);
+  std::copy(kcount, kcount + 3, g->kcount);
+  if (rc) {
+    g->tile = saved;
+    return rc;
+  }
+  g->tile = bt;
+  if (bx) *bx = bt.bx;
+  if (by) *by = bt.by;
+  if (kc) *kc = bt.kc;
   return 0;
 }
 

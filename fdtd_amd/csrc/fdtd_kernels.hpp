@@ -311,7 +311,7 @@ __global__ __launch_bounds__(256) void e_pass_3d(StepArgs<T> a, int kc) {
       Ex[c] = ex + d[0] * ie;
       Ey[c] = ey + d[1] * ie;
       Ez[c] = ez + d[2] * ie;
-    } else {
+    } else if (ie < T(0) && a.Dd[0]) {
       const MatDev<T>& m = a.mats[mat[c]];
       const long bq = dbox_index(a, i, j, k);
       Ex[c] = ade_component<T, 0>(a, m, bq, d[0]);
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void e_pass_tm(StepArgs<T> a, int) {
   }
   if (ie > T(0)) {
     Ez[c] = ez + d[2] * ie;
-  } else {
+  } else if (ie < T(0) && a.Dd[2]) {
     const MatDev<T>& m = a.mats[a.mat[c]];
     Ez[c] = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2]);
   }

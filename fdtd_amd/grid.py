@@ -25,7 +25,7 @@ ABI_SYMBOLS = [
     "fdtd_set_material", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_dft", "fdtd_get_dft",
     "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
-    "fdtd_set_tile", "fdtd_get_tile",
+    "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
 ]
 
 
@@ -74,6 +74,7 @@ def load_library():
         "fdtd_last_error": (ctypes.c_char_p, []),
         "fdtd_set_tile": (_i, [_p, _i, _i, _i]),
         "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
+        "fdtd_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -217,6 +218,12 @@ class Grid:
     def get_tile(self):
         v = [ctypes.c_int() for _ in range(3)]
         _check(load_library().fdtd_get_tile(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune(self, trial_steps=20):
+        """Pick the fastest tile by timing; the simulation state is left unchanged."""
+        v = [ctypes.c_int() for _ in range(3)]
+        _check(load_library().fdtd_autotune(self.h, trial_steps, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
     def set_profiling(self, on=True):

@@ -131,6 +131,11 @@ long fdtd_step_count(const fdtd_grid* g);
  * z planes.  Results do not depend on the tile; only speed does. */
 int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc);
 int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc);
+/* Times each candidate tile over trial_steps steps, keeps the fastest and
+ * restores the complete device state and step count afterwards (it
+ * snapshots every allocation of the grid, so it needs as much free device
+ * memory again).  The chosen tile is returned through bx, by, kc. */
+int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
 
 /* Per-kernel timing with hipEvents on the grid's stream (off by default).
  * fdtd_kernel_time returns accumulated milliseconds and launches for

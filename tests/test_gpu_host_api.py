@@ -1,0 +1,28 @@
+"""The C++ host surface (fdtd_amd/include/fdtd_hip.hpp) driven like the
+reference's own user loop (tests/testTM.cpp), checked against the reference's
+golden probe trace of the 2-D TM 256^2 x 1000-step case."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from golden_io import load
+from scenarios import rel_l2
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "bin", "tm_c1_host_api")
+
+
+def test_tm_c1_through_cpp_host_surface():
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/bin/tm_c1_host_api not built (run __graft_entry__.build())")
+    p, a = load("tm2d_c1")
+    coords = [str(int(v)) for pr in a["probes"] for v in pr]
+    print("[gpu-case] tm_c1_host_api", flush=True)
+    r = subprocess.run([BIN, *coords], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
+    ref = a["probe_trace_Ez_Hx_Hy"][:, :, 0]
+    assert got.shape == ref.shape
+    assert rel_l2(got, ref) <= 1e-12

@@ -118,3 +118,56 @@ def test_anisotropic_3d_f64_chunked_steps():
         g2.step(n)
     for nm, f, c in _comps(sc):
         np.testing.assert_array_equal(g1.get(f, c), g2.get(f, c))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_tile_independence_and_autotune_restores_state(precision):
+    """Every tile gives bitwise-identical fields (no halo/ordering races), and
+    the autotuner's trial steps leave the simulation state untouched."""
+    sc = synth.grid3d_cpml_tfsf_dft(n=(40, 34, 30), nsteps=80)
+    ref = apply(_grid(sc, precision), sc)
+    ref.step(80)
+    outs = {nm: ref.get(f, c) for nm, f, c in _comps(sc)}
+    for tile in [(64, 4, 1), (128, 2, 4), (64, 1, 8)]:
+        g = apply(_grid(sc, precision), sc)
+        g.set_tile(*tile)
+        g.step(30)
+        g.set_tile(256, 1, 2)
+        g.step(50)
+        for nm, f, c in _comps(sc):
+            np.testing.assert_array_equal(g.get(f, c), outs[nm], err_msg=f"{tile} {nm}")
+        g.close()
+    g = apply(_grid(sc, precision), sc)
+    g.step(10)
+    tile = g.autotune(trial_steps=3)
+    assert g.step_count() == 10
+    g.step(70)
+    for nm, f, c in _comps(sc):
+        np.testing.assert_array_equal(g.get(f, c), outs[nm], err_msg=f"autotune {tile} {nm}")
+    np.testing.assert_array_equal(g.probes(), ref.probes())
+    for idx in range(len(sc["dfts"])):
+        np.testing.assert_array_equal(g.get_dft(idx), ref.get_dft(idx))
+
+
+def test_tm2d_c1_probes_match_reference_trace():
+    """2-D TM 256^2 x 1000 steps (the reference's CPU-runnable case): the four
+    probe traces on the GPU in FP64 against the reference's own trace."""
+    sc, p, a = from_golden("tm2d_c1")
+    g = apply(_grid(sc, "f64"), sc)
+    g.step(sc["nsteps"])
+    ref = a["probe_trace_Ez_Hx_Hy"].reshape(sc["nsteps"], -1)
+    assert rel_l2(g.probes(), ref) <= 1e-12
+
+
+def test_autotune_as_first_call():
+    """Regression: autotune before any step must leave the material and CPML
+    tables uploaded (the snapshot is taken after pending uploads)."""
+    sc = synth.grid3d_cpml_tfsf_dft(n=(36, 30, 28), nsteps=60)
+    ref = apply(_grid(sc, "f32"), sc)
+    ref.step(60)
+    g = apply(_grid(sc, "f32"), sc)
+    g.autotune(trial_steps=2)
+    assert g.step_count() == 0
+    g.step(60)
+    for nm, f, c in _comps(sc):
+        np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=nm)
