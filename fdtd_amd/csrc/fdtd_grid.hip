@@ -106,6 +106,7 @@ void pml_idiom_rows(int L, int n, double m, double ma, double smax, double kmax,
 constexpr int kChunk = 256;  // steps per incident-table chunk
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
+// bx == 0 selects the fused single-pass schedule with `by` rows per workgroup
 struct Tile {
   int bx, by, kc;
 };
@@ -154,6 +155,14 @@ struct fdtd_grid {
   void* Dd[3] = {};
   void* Pd[FDTD_MAX_POLES][3] = {};
   void* Jd[FDTD_MAX_POLES][3] = {};
+  // second copies of the stepped state for the fused (ping-pong) schedule
+  bool pingpong = false;
+  void* E2[3] = {};
+  void* H2[3] = {};
+  void* psi2[2][3][3] = {};
+  void* Dd2[3] = {};
+  void* Pd2[FDTD_MAX_POLES][3] = {};
+  void* Jd2[FDTD_MAX_POLES][3] = {};
 
   // sources
   int nsrc = 0;
@@ -189,18 +198,33 @@ struct fdtd_grid {
   bool coef_dirty = true;
   bool mats_dirty = true;
   long step = 0;
+  long nswap = 0;  // role swaps of the two state copies
+
+  // kernel arguments live in device memory (two variants per chunk: the
+  // fused pass alternates read/write roles); a ring of slots lets the host
+  // fill the next chunk while earlier chunks still run
+  static constexpr int kRing = 8;
+  static constexpr size_t kArgsBytes = (sizeof(fdtdk::StepArgs<double>) + 255) & ~size_t(255);
+  char* args_dev = nullptr;
+  char* args_host = nullptr;
+  hipEvent_t args_ev[kRing] = {};
+  bool args_ev_used[kRing] = {};
+  int args_slot = 0;
 
   // profiling
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_pairs;  // (kernel id, index of start event)
   size_t ev_used = 0;
-  double kms[3] = {0, 0, 0};
-  long kcount[3] = {0, 0, 0};
+  double kms[4] = {0, 0, 0, 0};
+  long kcount[4] = {0, 0, 0, 0};
 
   ~fdtd_grid() {
     for (void* p : allocs) (void)hipFree(p);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    for (auto e : args_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (args_host) (void)hipHostFree(args_host);
   }
 
   int alloc(void** p, size_t bytes, bool zero = true) {
@@ -399,13 +423,80 @@ int finalize(fdtd_grid* g) {
   return 0;
 }
 
+// the state pointers: read from the current buffers, write to the same ones
+// (in-place passes) or to the second copies (fused pass)
 template <typename T>
-void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
-  memset(&a, 0, sizeof a);
+void set_state(fdtd_grid* g, fdtdk::StepArgs<T>& a, bool pingpong) {
   for (int c = 0; c < 3; c++) {
     a.E[c] = (T*)g->E[c];
     a.H[c] = (T*)g->H[c];
+    a.Eo[c] = (T*)(pingpong ? g->E2[c] : g->E[c]);
+    a.Ho[c] = (T*)(pingpong ? g->H2[c] : g->H[c]);
+    a.Dd[c] = (T*)g->Dd[c];
+    a.Ddo[c] = (T*)(pingpong ? g->Dd2[c] : g->Dd[c]);
+    for (int p = 0; p < fdtdk::kMaxPoles; p++) {
+      a.Pd[p][c] = (T*)g->Pd[p][c];
+      a.Jd[p][c] = (T*)g->Jd[p][c];
+      a.Pdo[p][c] = (T*)(pingpong ? g->Pd2[p][c] : g->Pd[p][c]);
+      a.Jdo[p][c] = (T*)(pingpong ? g->Jd2[p][c] : g->Jd[p][c]);
+    }
   }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < 3; J++) {
+        a.psi[ft][I][J] = (T*)g->psi[ft][I][J];
+        a.psio[ft][I][J] = (T*)(pingpong ? g->psi2[ft][I][J] : g->psi[ft][I][J]);
+      }
+}
+
+void swap_state(fdtd_grid* g) {
+  g->nswap++;
+  for (int c = 0; c < 3; c++) {
+    std::swap(g->E[c], g->E2[c]);
+    std::swap(g->H[c], g->H2[c]);
+    std::swap(g->Dd[c], g->Dd2[c]);
+    for (int p = 0; p < FDTD_MAX_POLES; p++) {
+      std::swap(g->Pd[p][c], g->Pd2[p][c]);
+      std::swap(g->Jd[p][c], g->Jd2[p][c]);
+    }
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < 3; J++) std::swap(g->psi[ft][I][J], g->psi2[ft][I][J]);
+}
+
+// allocate the second copies (zero: the fused pass writes every interior cell,
+// the outer layer stays zero in both copies)
+int ensure_pingpong(fdtd_grid* g) {
+  if (g->pingpong) return 0;
+  auto twin = [&](void* src, void** dst) -> int {
+    if (!src) return 0;
+    auto it = std::find(g->allocs.begin(), g->allocs.end(), src);
+    if (it == g->allocs.end()) return fail(FDTD_ESTATE, "untracked allocation");
+    return g->alloc(dst, g->alloc_bytes[it - g->allocs.begin()]);
+  };
+  int rc = 0;
+  for (int c = 0; c < 3 && !rc; c++) {
+    if (!rc) rc = twin(g->E[c], &g->E2[c]);
+    if (!rc) rc = twin(g->H[c], &g->H2[c]);
+    if (!rc) rc = twin(g->Dd[c], &g->Dd2[c]);
+    for (int p = 0; p < FDTD_MAX_POLES && !rc; p++) {
+      rc = twin(g->Pd[p][c], &g->Pd2[p][c]);
+      if (!rc) rc = twin(g->Jd[p][c], &g->Jd2[p][c]);
+    }
+  }
+  for (int ft = 0; ft < 2 && !rc; ft++)
+    for (int I = 0; I < 3 && !rc; I++)
+      for (int J = 0; J < 3 && !rc; J++) rc = twin(g->psi[ft][I][J], &g->psi2[ft][I][J]);
+  if (rc) return rc;
+  g->pingpong = true;
+  return 0;
+}
+
+template <typename T>
+void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
+  memset(&a, 0, sizeof a);
+  set_state(g, a, false);
   a.mat = g->mat;
   a.mats = (const fdtdk::MatDev<T>*)g->mats_dev;
   a.mie = (const T*)g->mie_dev;
@@ -416,7 +507,6 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
       a.cg[ft][I] = (const T*)g->coef[ft][I][1];
       a.cb[ft][I] = (const T*)g->coef[ft][I][2];
       a.cc[ft][I] = (const T*)g->coef[ft][I][3];
-      for (int J = 0; J < 3; J++) a.psi[ft][I][J] = (T*)g->psi[ft][I][J];
     }
   for (int x = 0; x < 3; x++) {
     a.slab_a[x] = g->ndim > x ? g->slab_a[x] : 0;
@@ -429,13 +519,6 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   a.sy = g->d.nx;
   a.sz = long(g->d.nx) * g->d.ny;
   for (int x = 0; x < 6; x++) a.dbox[x] = g->dbox[x];
-  for (int c = 0; c < 3; c++) {
-    a.Dd[c] = (T*)g->Dd[c];
-    for (int p = 0; p < fdtdk::kMaxPoles; p++) {
-      a.Pd[p][c] = (T*)g->Pd[p][c];
-      a.Jd[p][c] = (T*)g->Jd[p][c];
-    }
-  }
   a.dt = T(g->d.dt);
   a.nsrc = g->nsrc;
   for (int s = 0; s < g->nsrc; s++) a.src[s] = g->src[s];
@@ -570,11 +653,26 @@ int step_impl(fdtd_grid* g, int nsteps) {
   if (rc0) return rc0;
   fdtdk::StepArgs<T> a;
   fill_args<T>(g, a);
-  Tile t = g->tile.bx ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
+  Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (g->ndim == 2) t.kc = 1;
+  const bool fused = g->ndim == 3 && t.bx == 0;
+  if (fused) {
+    int rc = ensure_pingpong(g);
+    if (rc) return rc;
+  }
   const int nzi = g->ndim == 3 ? g->d.nz - 2 : 1;
-  dim3 block(t.bx, t.by, 1);
-  dim3 grid((g->d.nx - 2 + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, (nzi + t.kc - 1) / t.kc);
+  dim3 block, grid;
+  int ntx = 1, nty = 1, nkc = 1;
+  if (fused) {
+    ntx = (g->d.nx - 2 + fdtdk::kFusedCols - 1) / fdtdk::kFusedCols;
+    nty = (g->d.ny - 2 + t.by - 1) / t.by;
+    nkc = (nzi + t.kc - 1) / t.kc;
+    block = dim3(256, 1, 1);
+    grid = dim3(ntx * nty * nkc, 1, 1);
+  } else {
+    block = dim3(t.bx, t.by, 1);
+    grid = dim3((g->d.nx - 2 + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, (nzi + t.kc - 1) / t.kc);
+  }
   int done = 0;
   while (done < nsteps) {
     const int chunk = std::min(kChunk, nsteps - done);
@@ -582,23 +680,51 @@ int step_impl(fdtd_grid* g, int nsteps) {
       int rc = run_aux(g, g->step, chunk);
       if (rc) return rc;
     }
+    a.step0 = g->step;
+    a.incE = g->incE;
+    a.incH = g->incH;
+    // stage this chunk's arguments: variant 0 reads the current state,
+    // variant 1 (fused pass only) the second copy
+    const int slot = g->args_slot;
+    g->args_slot = (slot + 1) % fdtd_grid::kRing;
+    if (g->args_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->args_ev[slot]));
+    auto* host = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
+    auto* host1 = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+    set_state<T>(g, a, fused);
+    *host = a;
+    if (fused) {
+      swap_state(g);
+      set_state<T>(g, a, true);
+      *host1 = a;
+      swap_state(g);
+    }
+    const auto* dev0 = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
+    const auto* dev1 =
+        reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+    HIP_TRY(hipMemcpyAsync((void*)dev0, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
     for (int s = 0; s < chunk; s++) {
-      a.step = g->step;
-      if (g->tfsf) {
-        a.incE = g->incE + long(s) * g->auxL;
-        a.incH = g->incH + long(s) * g->auxL;
-      }
       int rc;
-      if (g->ndim == 3) {
-        rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, a, t.kc);
-        if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, a, t.kc);
+      if (fused) {
+        const auto* ap = (s % 2) ? dev1 : dev0;
+        if (t.by == 2)
+          rc = launch(g, 3, fdtdk::fused_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        else if (t.by == 4)
+          rc = launch(g, 3, fdtdk::fused_3d<T, 4>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        else
+          rc = launch(g, 3, fdtdk::fused_3d<T, 8>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        swap_state(g);
+      } else if (g->ndim == 3) {
+        rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s);
+        if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s);
       } else {
-        rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, a, 1);
-        if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, a, 1);
+        rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
+        if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
       }
       if (rc) return rc;
       g->step++;
     }
+    HIP_TRY(hipEventRecord(g->args_ev[slot], g->stream));
+    g->args_ev_used[slot] = true;
     done += chunk;
   }
   return 0;
@@ -654,6 +780,12 @@ int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   if (!rc) rc = g->alloc(&g->mats_dev, sizeof(fdtdk::MatDev<double>) * FDTD_MAX_MATERIALS);
   if (!rc) rc = g->alloc(&g->mie_dev, 8 * FDTD_MAX_MATERIALS);
   if (!rc) rc = g->alloc(&g->mim_dev, 8 * FDTD_MAX_MATERIALS);
+  if (!rc) rc = g->alloc((void**)&g->args_dev, fdtd_grid::kRing * 2 * fdtd_grid::kArgsBytes);
+  if (!rc && hipHostMalloc((void**)&g->args_host, fdtd_grid::kRing * 2 * fdtd_grid::kArgsBytes) != hipSuccess)
+    rc = fail(FDTD_ENOMEM, "pinned staging for kernel arguments");
+  for (int r = 0; r < fdtd_grid::kRing && !rc; r++)
+    if (hipEventCreateWithFlags(&g->args_ev[r], hipEventDisableTiming) != hipSuccess)
+      rc = fail(FDTD_EDEVICE, "hipEventCreateWithFlags failed");
   for (int ft = 0; ft < 2 && !rc; ft++)
     for (int a = 0; a < 3 && !rc; a++) {
       g->pml[ft][a].assign(4 * g->len[a], 0.0);
@@ -928,8 +1060,12 @@ long fdtd_step_count(const fdtd_grid* g) { return g ? g->step : -1; }
 
 int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
-  if (bx <= 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
+  if (bx == 0) {
+    if (g->ndim != 3 || (by != 2 && by != 4 && by != 8) || kc <= 0)
+      return fail(FDTD_EINVAL, "the fused schedule (bx = 0) is 3-D only, with by in {2, 4, 8} rows and kc >= 1");
+  } else if (bx < 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0) {
     return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
+  }
   g->tile = Tile{bx, by, kc};
   return 0;
 }
@@ -938,8 +1074,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   if (!g || trial_steps < 1) return fail(FDTD_EINVAL, "bad argument");
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
-  static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {128, 2, 1}, {128, 2, 2},
-                               {128, 1, 1}, {64, 4, 1},  {64, 4, 4},  {64, 1, 1}};
+  static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {128, 2, 1}, {128, 2, 2}, {64, 4, 1},
+                               {0, 2, 8},   {0, 2, 16},  {0, 4, 8},   {0, 4, 16},  {0, 4, 32},  {0, 8, 16}};
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;
   const int ncand = g->ndim == 3 ? int(sizeof cand3 / sizeof cand3[0]) : int(sizeof cand2 / sizeof cand2[0]);
@@ -949,6 +1085,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   // trials will run with, or the restore would roll them back to stale contents
   if (g->mats_dirty && (rc = g->f64 ? upload_materials<double>(g) : upload_materials<float>(g))) return rc;
   if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
+  if (g->ndim == 3 && (rc = ensure_pingpong(g))) return rc;
+  const long swaps0 = g->nswap;
   // snapshot every device allocation (fields, psi, dispersive state, DFT
   // accumulators, aux line, probe trace) so the trials leave no trace
   HIP_TRY(hipStreamSynchronize(g->stream));
@@ -964,10 +1102,10 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     off += (sizes[q] + 255) & ~size_t(255);
   }
   const bool prof = g->prof;
-  double kms[3];
-  long kcount[3];
-  std::copy(g->kms, g->kms + 3, kms);
-  std::copy(g->kcount, g->kcount + 3, kcount);
+  double kms[4];
+  long kcount[4];
+  std::copy(g->kms, g->kms + 4, kms);
+  std::copy(g->kcount, g->kcount + 4, kcount);
   const Tile saved = g->tile;
   double best = 1e300;
   Tile bt = cand[0];
@@ -976,10 +1114,12 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     if ((rc = fdtd_step(g, 1))) break;
     if ((rc = fdtd_set_profiling(g, 1))) break;
     if ((rc = fdtd_step(g, trial_steps))) break;
-    double e = 0, h = 0;
-    long ne = 0, nh = 0;
-    if ((rc = fdtd_kernel_time(g, 0, &e, &ne)) || (rc = fdtd_kernel_time(g, 1, &h, &nh))) break;
-    double t = e / std::max(ne, 1L) + h / std::max(nh, 1L);
+    double e = 0, h = 0, f = 0;
+    long ne = 0, nh = 0, nf = 0;
+    if ((rc = fdtd_kernel_time(g, 0, &e, &ne)) || (rc = fdtd_kernel_time(g, 1, &h, &nh)) ||
+        (rc = fdtd_kernel_time(g, 3, &f, &nf)))
+      break;
+    double t = e / std::max(ne, 1L) + h / std::max(nh, 1L) + f / std::max(nf, 1L);
     if (t < best) {
       best = t;
       bt = cand[c];
@@ -993,10 +1133,12 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   }
   (void)hipStreamSynchronize(g->stream);
   (void)hipFree(snap);
+  // fused trials swap the roles of the two state copies; put them back
+  if ((g->nswap - swaps0) % 2) swap_state(g);
   g->step = step0;
   g->prof = prof;
-  std::copy(kms, kms + 3, g->kms);
-  std::copy(kcount, kcount + 3, g->kcount);
+  std::copy(kms, kms + 4, g->kms);
+  std::copy(kcount, kcount + 4, g->kcount);
   if (rc) {
     g->tile = saved;
     return rc;
@@ -1010,7 +1152,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
 
 int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
-  Tile t = g->tile.bx ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
+  Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (bx) *bx = t.bx;
   if (by) *by = t.by;
   if (kc) *kc = t.kc;
@@ -1021,7 +1163,7 @@ int fdtd_set_profiling(fdtd_grid* g, int on) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
   int rc = collect_events(g);
   g->prof = on != 0;
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < 4; k++) {
     g->kms[k] = 0;
     g->kcount[k] = 0;
   }
@@ -1029,7 +1171,7 @@ int fdtd_set_profiling(fdtd_grid* g, int on) {
 }
 
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches) {
-  if (!g || kernel < 0 || kernel > 2) return fail(FDTD_EINVAL, "bad argument");
+  if (!g || kernel < 0 || kernel > 3) return fail(FDTD_EINVAL, "bad argument");
   int rc = collect_events(g);
   if (rc) return rc;
   if (ms) *ms = g->kms[kernel];

@@ -1,15 +1,18 @@
 // Device side of the FDTD hot path for gfx950.
 //
-// Two passes per time step, each one HBM stream over the grid:
-//   e_pass: D curl + soft sources + CPML(E) + TFSF updateD + E material/ADE
-//   h_pass: B curl + sources + CPML(H) + TFSF updateB + H material, then the
-//           running DFT and the probes of this step
-// Fields are structure-of-arrays (one array per component, x fastest), so a
-// 64-lane wavefront reads 64 consecutive cells of one component.  Each thread
-// owns one (i,j) column and marches a chunk of z planes, keeping the plane it
-// just read in registers, so the z-neighbour of the curl costs no reload.
-// E and H are stored directly (E = D/(eps_r eps0) folded into the update);
-// D, P and J exist only inside the dispersive box.
+// The arithmetic of one cell lives in e_cell_3d / h_cell_3d (and the 2-D TM
+// pair); two schedules call them:
+//   * two passes per step (e_pass_3d, then h_pass_3d), in place;
+//   * one fused pass per step (fused_3d): each workgroup owns a 254-column x
+//     TY-row x kc-plane block, marches z, recomputes the new E of one halo
+//     column, one halo row and one halo plane from the old state, and writes
+//     E and H of its own cells to the "next" buffers (ping-pong), so every
+//     cell's E and H are read once and written once per step.
+// Both schedules feed the same inputs to the same functions, so they give
+// bitwise-identical fields.  Fields are structure-of-arrays, x fastest, so a
+// 64-lane wavefront touches 64 consecutive cells of one component.  E and H
+// are stored directly (E = D/(eps_r eps0) folded into the update); D, P and J
+// exist only inside the dispersive box.
 //
 // Reference arithmetic restated here (pederson/fdtd):
 //   curl      include/YeeUpdates.hpp:363-405   (E-type fluxes take backward
@@ -28,6 +31,7 @@ constexpr int kMaxPoles = 3;
 constexpr int kMaxSrc = 16;
 constexpr int kMaxDft = 8;
 constexpr int kMaxProbe = 32;
+constexpr int kFusedCols = 254;  // owned columns of a fused-pass workgroup (256 threads)
 
 template <typename T>
 struct MatDev {
@@ -49,8 +53,8 @@ struct DftDev {
   int r[6];
   int mask, nfreq, ncomp;
   long ncells;
-  T* acc;               // [ncomp][nfreq][2][ncells]
-  const double* phE;    // [nphase][nfreq][2]
+  T* acc;             // [ncomp][nfreq][2][ncells]
+  const double* phE;  // [nphase][nfreq][2]
   const double* phH;
   int nphase;
 };
@@ -62,14 +66,17 @@ struct ProbeDev {
 
 template <typename T>
 struct StepArgs {
-  T* E[3];
+  T* E[3];   // state read by the step
   T* H[3];
+  T* Eo[3];  // state written by the step (the same arrays for in-place passes)
+  T* Ho[3];
   const uint8_t* mat;
   const MatDev<T>* mats;
   const T* mie;  // 1/(eps_r eps0) per material id, negated for dispersive materials
   const T* mim;  // 1/(mu_r mu0) per material id
-  // CPML: psi[ft][I][J] slab arrays and per-axis coefficient profiles
+  // CPML: psi[ft][I][J] slab arrays (read / written) and per-axis profiles
   T* psi[2][3][3];
+  T* psio[2][3][3];
   const T* cf[2][3];  // dt F / dx
   const T* cg[2][3];  // dt G
   const T* cb[2][3];  // B
@@ -77,21 +84,24 @@ struct StepArgs {
   int slab_a[3], slab_b[3], slab_S[3];
   int nx, ny, nz;
   long sy, sz;
-  // dispersive box (dense): [x0,x1)x[y0,y1)x[z0,z1)
+  // dispersive box (dense): [x0,x1)x[y0,y1)x[z0,z1), read / written
   int dbox[6];
   T* Dd[3];
   T* Pd[kMaxPoles][3];
   T* Jd[kMaxPoles][3];
+  T* Ddo[3];
+  T* Pdo[kMaxPoles][3];
+  T* Jdo[kMaxPoles][3];
   T dt;
   // soft sources
   int nsrc;
   SrcDev src[kMaxSrc];
-  // TFSF
+  // TFSF: incident rows of the current chunk, row s = step - step0
   int tfsf;
   int box[6];
   double kvec[3], pol[3], polH[3], origin[3], dx;
   int auxL;
-  const double* incE;  // aux Ez after this step's aux advance
+  const double* incE;  // aux Ez after each step's aux advance
   const double* incH;  // aux Hy before it
   // running DFT
   int ndft;
@@ -100,7 +110,7 @@ struct StepArgs {
   int nprobe;
   ProbeDev probe[kMaxProbe];
   double* trace;  // [step][nprobe]
-  long step;
+  long step0;     // global step of row 0 of the incident rows
 };
 
 __device__ __forceinline__ int slab_index(int idx, int a, int b) { return idx < a ? idx : (idx >= b ? a + idx - b : -1); }
@@ -135,9 +145,17 @@ __device__ __forceinline__ double lc3(int I, int J) {
 // ft 0: D fluxes from aux H with polH; ft 1: B fluxes from aux E with pol.
 // The total-field region along an axis is the open interval (lo-1/2, hi);
 // a component sits at offset 0 or -1/2 on each axis.
+// Cold paths (TFSF faces, sources, dispersive cells, DFT monitors).  Inlined:
+// out-of-line calls measured more registers (call clobbers) plus scratch for
+// the argument arrays.  Kernel arguments are read through a pointer to device
+// memory, so inlined code never copies the large argument struct.
+#ifndef FDTD_COLD
+#define FDTD_COLD __forceinline__
+#endif
+
 template <typename T, int NDIM>
-__device__ void tfsf_cell(const StepArgs<T>& a, int ft, int i, int j, int k, const bool actF[3], const bool actC[3],
-                          T delta[3]) {
+__device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a, const double* row, int ft, int i, int j, int k, const bool actF[3],
+                          const bool actC[3], T delta[3]) {
   const int ci[3] = {i, j, k};
   for (int J = 0; J < NDIM; J++) {
     int side = (ci[J] == a.box[2 * J]) ? 0 : ((ci[J] == a.box[2 * J + 1]) ? 1 : -1);
@@ -162,7 +180,7 @@ __device__ void tfsf_cell(const StepArgs<T>& a, int ft, int i, int j, int k, con
         double loc = ((double)ci[dd] - a.origin[dd]) * a.dx + off * a.dx;
         proj += a.kvec[dd] * loc;
       }
-      double inc = ft == 0 ? interp_row(a.incH, a.auxL, proj, a.dx, true) : interp_row(a.incE, a.auxL, proj, a.dx, false);
+      double inc = interp_row(row, a.auxL, proj, a.dx, ft == 0);
       double pv = ft == 0 ? a.polH[K] : a.pol[K];
       double f = (double)a.cf[ft][J][ci[J]];
       double term = lc3(I, J) * f * nrm * inc * pv;
@@ -171,18 +189,17 @@ __device__ void tfsf_cell(const StepArgs<T>& a, int ft, int i, int j, int k, con
   }
 }
 
-// ---------------------------------------------------------------- E pass
 template <typename T>
 __device__ __forceinline__ T pick3(T v0, T v1, T v2, int c) {
   return c == 0 ? v0 : (c == 1 ? v1 : v2);
 }
 
 template <typename T>
-__device__ __forceinline__ void add_sources(const StepArgs<T>& a, int field, long c, T d[3]) {
+__device__ FDTD_COLD void add_sources(const StepArgs<T>& a, long step, int field, long c, T d[3]) {
   for (int s = 0; s < a.nsrc; s++) {
     const SrcDev& q = a.src[s];
-    if (q.cell == c && q.field == field && a.step < q.n) {
-      const T v = (T)q.table[a.step];
+    if (q.cell == c && q.field == field && step < q.n) {
+      const T v = (T)q.table[step];
       if (q.comp == 0)
         d[0] += v;
       else if (q.comp == 1)
@@ -197,9 +214,8 @@ __device__ __forceinline__ void add_sources(const StepArgs<T>& a, int field, lon
 // E = (D - sum_p P_p)/(eps_r eps0); J_p = cJ J_p + cE E + cP P_p
 // (DrudeUpdate / LorentzUpdate implicit_update, DispersiveMaterials.hpp:710-735, 979-998).
 template <typename T, int I>
-__device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>& m, long bq, T d) {
+__device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>& m, long bq, T d, bool own) {
   const T D = a.Dd[I][bq] + d;
-  a.Dd[I][bq] = D;
   T num = D;
   T Pv[kMaxPoles], Jv[kMaxPoles];
 #pragma unroll
@@ -211,11 +227,14 @@ __device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>
     }
   }
   const T Ev = num * m.inv_eps;
+  if (own) {
+    a.Ddo[I][bq] = D;
 #pragma unroll
-  for (int p = 0; p < kMaxPoles; p++) {
-    if (p < m.np) {
-      a.Pd[p][I][bq] = Pv[p];
-      a.Jd[p][I][bq] = m.cJ[p] * Jv[p] + m.cE[p] * Ev + m.cP[p] * Pv[p];
+    for (int p = 0; p < kMaxPoles; p++) {
+      if (p < m.np) {
+        a.Pdo[p][I][bq] = Pv[p];
+        a.Jdo[p][I][bq] = m.cJ[p] * Jv[p] + m.cE[p] * Ev + m.cP[p] * Pv[p];
+      }
     }
   }
   return Ev;
@@ -227,107 +246,164 @@ __device__ __forceinline__ long dbox_index(const StepArgs<T>& a, int i, int j, i
          (long)(a.dbox[1] - a.dbox[0]) * ((long)(j - a.dbox[2]) + (long)(a.dbox[3] - a.dbox[2]) * (long)(k - a.dbox[4]));
 }
 
-// One thread per cell and z plane: every load of the cell is issued before
-// any arithmetic, so a wave carries ~12 independent loads in flight and the
-// stores go out at the end.  mie[m] < 0 marks a dispersive material.
+// dispersive cell: the three components
 template <typename T>
-__global__ __launch_bounds__(256) void e_pass_3d(StepArgs<T> a, int kc) {
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k0 = 1 + blockIdx.z * kc;
-  const int k1 = min(k0 + kc, a.nz - 1);
-  if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const T* __restrict__ Hx = a.H[0];
-  const T* __restrict__ Hy = a.H[1];
-  const T* __restrict__ Hz = a.H[2];
-  T* __restrict__ Ex = a.E[0];
-  T* __restrict__ Ey = a.E[1];
-  T* __restrict__ Ez = a.E[2];
-  const uint8_t* __restrict__ mat = a.mat;
-  const T* __restrict__ mie = a.mie;
-  const long sy = a.sy, sz = a.sz;
-  const T fx = a.cf[0][0][i], fy = a.cf[0][1][j];
+__device__ FDTD_COLD void ade_cell(const StepArgs<T>& a, int mid, int i, int j, int k, const T d[3], bool own, T out[3]) {
+  const MatDev<T>& m = a.mats[mid];
+  const long bq = dbox_index(a, i, j, k);
+  out[0] = ade_component<T, 0>(a, m, bq, d[0], own);
+  out[1] = ade_component<T, 1>(a, m, bq, d[1], own);
+  out[2] = ade_component<T, 2>(a, m, bq, d[2], own);
+}
+
+// H values around a cell that the E update reads (backward neighbours)
+template <typename T>
+struct HStencil {
+  T hx, hy, hz, hz_ym, hz_xm, hy_xm, hy_zm, hx_ym, hx_zm;
+};
+// E values around a cell that the H update reads (forward neighbours)
+template <typename T>
+struct EStencil {
+  T ex, ey, ez, ez_yp, ez_xp, ey_xp, ey_zp, ex_yp, ex_zp;
+};
+
+// ---------------------------------------------------------------- E of one cell
+// New E of interior cell (i,j,k).  psi and the dispersive state are read from
+// the current buffers and, when `own`, written to the next buffers.
+template <typename T>
+__device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const double* incH, int i, int j, int k,
+                                          long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, bool own,
+                                          T out[3]) {
+  const T ie = a.mie[mid];
+  const T fx = a.cf[0][0][i], fy = a.cf[0][1][j], fz = a.cf[0][2][k];
+  const T dxy = h.hz - h.hz_ym;  // Ex, J=y, +1
+  const T dxz = h.hy - h.hy_zm;  // Ex, J=z, -1
+  const T dyz = h.hx - h.hx_zm;  // Ey, J=z, +1
+  const T dyx = h.hz - h.hz_xm;  // Ey, J=x, -1
+  const T dzx = h.hy - h.hy_xm;  // Ez, J=x, +1
+  const T dzy = h.hx - h.hx_ym;  // Ez, J=y, -1
+  T d[3];
+  d[0] = fy * dxy - fz * dxz;
+  d[1] = fz * dyz - fx * dyx;
+  d[2] = fx * dzx - fy * dzy;
+  if (a.nsrc) add_sources(a, step, 0, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
+  if (sxi >= 0) {
+    const long o = slab_off(a, 0, sxi, i, j, k);
+    const T g = a.cg[0][0][i], b = a.cb[0][0][i], cc = a.cc[0][0][i];
+    const T p1 = a.psi[0][1][0][o], p2 = a.psi[0][2][0][o];
+    d[1] += g * p1;
+    d[2] += g * p2;
+    if (own) {
+      a.psio[0][1][0][o] = b * p1 - cc * dyx;
+      a.psio[0][2][0][o] = b * p2 + cc * dzx;
+    }
+  }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
-  for (int k = k0; k < k1; k++) {
-    const long c = (long)i + sy * (long)j + sz * (long)k;
-    const T hx = Hx[c], hy = Hy[c], hz = Hz[c];
-    const T hz_ym = Hz[c - sy], hz_xm = Hz[c - 1];
-    const T hy_xm = Hy[c - 1], hy_zm = Hy[c - sz];
-    const T hx_ym = Hx[c - sy], hx_zm = Hx[c - sz];
-    const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
-    const T ie = mie[mat[c]];
-    const T fz = a.cf[0][2][k];
-    const T dxy = hz - hz_ym;  // Ex, J=y, +1
-    const T dxz = hy - hy_zm;  // Ex, J=z, -1
-    const T dyz = hx - hx_zm;  // Ey, J=z, +1
-    const T dyx = hz - hz_xm;  // Ey, J=x, -1
-    const T dzx = hy - hy_xm;  // Ez, J=x, +1
-    const T dzy = hx - hx_ym;  // Ez, J=y, -1
-    T d[3];
-    d[0] = fy * dxy - fz * dxz;
-    d[1] = fz * dyz - fx * dyx;
-    d[2] = fx * dzx - fy * dzy;
-    if (a.nsrc) add_sources(a, 0, c, d);
-    if (sxi >= 0) {
-      const long o = slab_off(a, 0, sxi, i, j, k);
-      T* __restrict__ q1 = a.psi[0][1][0];
-      T* __restrict__ q2 = a.psi[0][2][0];
-      const T g = a.cg[0][0][i], b = a.cb[0][0][i], cc = a.cc[0][0][i];
-      const T p1 = q1[o], p2 = q2[o];
-      d[1] += g * p1;
-      d[2] += g * p2;
-      q1[o] = b * p1 - cc * dyx;
-      q2[o] = b * p2 + cc * dzx;
+  if (syi >= 0) {
+    const long o = slab_off(a, 1, syi, i, j, k);
+    const T g = a.cg[0][1][j], b = a.cb[0][1][j], cc = a.cc[0][1][j];
+    const T p0 = a.psi[0][0][1][o], p2 = a.psi[0][2][1][o];
+    d[0] += g * p0;
+    d[2] += g * p2;
+    if (own) {
+      a.psio[0][0][1][o] = b * p0 + cc * dxy;
+      a.psio[0][2][1][o] = b * p2 - cc * dzy;
     }
-    if (syi >= 0) {
-      const long o = slab_off(a, 1, syi, i, j, k);
-      T* __restrict__ q0 = a.psi[0][0][1];
-      T* __restrict__ q2 = a.psi[0][2][1];
-      const T g = a.cg[0][1][j], b = a.cb[0][1][j], cc = a.cc[0][1][j];
-      const T p0 = q0[o], p2 = q2[o];
-      d[0] += g * p0;
-      d[2] += g * p2;
-      q0[o] = b * p0 + cc * dxy;
-      q2[o] = b * p2 - cc * dzy;
+  }
+  const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
+  if (szi >= 0) {
+    const long o = slab_off(a, 2, szi, i, j, k);
+    const T g = a.cg[0][2][k], b = a.cb[0][2][k], cc = a.cc[0][2][k];
+    const T p0 = a.psi[0][0][2][o], p1 = a.psi[0][1][2][o];
+    d[0] += g * p0;
+    d[1] += g * p1;
+    if (own) {
+      a.psio[0][0][2][o] = b * p0 - cc * dxz;
+      a.psio[0][1][2][o] = b * p1 + cc * dyz;
     }
-    const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
-    if (szi >= 0) {
-      const long o = slab_off(a, 2, szi, i, j, k);
-      T* __restrict__ q0 = a.psi[0][0][2];
-      T* __restrict__ q1 = a.psi[0][1][2];
-      const T g = a.cg[0][2][k], b = a.cb[0][2][k], cc = a.cc[0][2][k];
-      const T p0 = q0[o], p1 = q1[o];
-      d[0] += g * p0;
-      d[1] += g * p1;
-      q0[o] = b * p0 - cc * dxz;
-      q1[o] = b * p1 + cc * dyz;
-    }
-    if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
-      const bool act[3] = {true, true, true};
-      tfsf_cell<T, 3>(a, 0, i, j, k, act, act, d);
-    }
-    if (ie > T(0)) {
-      Ex[c] = ex + d[0] * ie;
-      Ey[c] = ey + d[1] * ie;
-      Ez[c] = ez + d[2] * ie;
-    } else if (ie < T(0) && a.Dd[0]) {
-      const MatDev<T>& m = a.mats[mat[c]];
-      const long bq = dbox_index(a, i, j, k);
-      Ex[c] = ade_component<T, 0>(a, m, bq, d[0]);
-      Ey[c] = ade_component<T, 1>(a, m, bq, d[1]);
-      Ez[c] = ade_component<T, 2>(a, m, bq, d[2]);
-    }
+  }
+  if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
+    const bool act[3] = {true, true, true};
+    tfsf_cell<T, 3>(a, incH, 0, i, j, k, act, act, d);
+  }
+  if (ie > T(0)) {
+    out[0] = ex + d[0] * ie;
+    out[1] = ey + d[1] * ie;
+    out[2] = ez + d[2] * ie;
+  } else if (ie < T(0) && a.Dd[0]) {
+    ade_cell(a, mid, i, j, k, d, own, out);
+  } else {
+    out[0] = ex;
+    out[1] = ey;
+    out[2] = ez;
   }
 }
 
-// ---------------------------------------------------------------- H pass
+// ---------------------------------------------------------------- H of one cell
 template <typename T>
-__device__ __forceinline__ void dft_probe(const StepArgs<T>& a, long c, int i, int j, int k, const T e[3], const T h[3]) {
+__device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const double* incE, int i, int j, int k,
+                                          long c, const EStencil<T>& e, T hx, T hy, T hz, int mid, T out[3]) {
+  const T im = a.mim[mid];
+  const T fx = a.cf[1][0][i], fy = a.cf[1][1][j], fz = a.cf[1][2][k];
+  const T dxy = e.ez_yp - e.ez;  // Hx, J=y, +1
+  const T dxz = e.ey_zp - e.ey;  // Hx, J=z, -1
+  const T dyz = e.ex_zp - e.ex;  // Hy, J=z, +1
+  const T dyx = e.ez_xp - e.ez;  // Hy, J=x, -1
+  const T dzx = e.ey_xp - e.ey;  // Hz, J=x, +1
+  const T dzy = e.ex_yp - e.ex;  // Hz, J=y, -1
+  T d[3];
+  d[0] = fz * dxz - fy * dxy;
+  d[1] = fx * dyx - fz * dyz;
+  d[2] = fy * dzy - fx * dzx;
+  if (a.nsrc) add_sources(a, step, 2, c, d);
+  const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
+  if (sxi >= 0) {
+    const long o = slab_off(a, 0, sxi, i, j, k);
+    const T g = a.cg[1][0][i], b = a.cb[1][0][i], cc = a.cc[1][0][i];
+    const T p1 = a.psi[1][1][0][o], p2 = a.psi[1][2][0][o];
+    d[1] -= g * p1;
+    d[2] -= g * p2;
+    a.psio[1][1][0][o] = b * p1 - cc * dyx;
+    a.psio[1][2][0][o] = b * p2 + cc * dzx;
+  }
+  const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
+  if (syi >= 0) {
+    const long o = slab_off(a, 1, syi, i, j, k);
+    const T g = a.cg[1][1][j], b = a.cb[1][1][j], cc = a.cc[1][1][j];
+    const T p0 = a.psi[1][0][1][o], p2 = a.psi[1][2][1][o];
+    d[0] -= g * p0;
+    d[2] -= g * p2;
+    a.psio[1][0][1][o] = b * p0 + cc * dxy;
+    a.psio[1][2][1][o] = b * p2 - cc * dzy;
+  }
+  const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
+  if (szi >= 0) {
+    const long o = slab_off(a, 2, szi, i, j, k);
+    const T g = a.cg[1][2][k], b = a.cb[1][2][k], cc = a.cc[1][2][k];
+    const T p0 = a.psi[1][0][2][o], p1 = a.psi[1][1][2][o];
+    d[0] -= g * p0;
+    d[1] -= g * p1;
+    a.psio[1][0][2][o] = b * p0 - cc * dxz;
+    a.psio[1][1][2][o] = b * p1 + cc * dyz;
+  }
+  if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
+    const bool act[3] = {true, true, true};
+    tfsf_cell<T, 3>(a, incE, 1, i, j, k, act, act, d);
+  }
+  out[0] = hx + d[0] * im;
+  out[1] = hy + d[1] * im;
+  out[2] = hz + d[2] * im;
+}
+
+// running DFT and probes of one cell after its step
+template <typename T>
+__device__ FDTD_COLD void dft_probe(const StepArgs<T>& a, long step, long c, int i, int j, int k, const T e[3],
+                                          const T h[3]) {
   for (int q = 0; q < a.ndft; q++) {
     const DftDev<T>& d = a.dft[q];
     if (i < d.r[0] || i >= d.r[1] || j < d.r[2] || j >= d.r[3] || k < d.r[4] || k >= d.r[5]) continue;
-    if (a.step >= d.nphase) continue;
+    if (step >= d.nphase) continue;
     const long lq =
         (long)(i - d.r[0]) + (long)(d.r[1] - d.r[0]) * ((long)(j - d.r[2]) + (long)(d.r[3] - d.r[2]) * (k - d.r[4]));
     int slot = 0;
@@ -335,7 +411,7 @@ __device__ __forceinline__ void dft_probe(const StepArgs<T>& a, long c, int i, i
     for (int comp = 0; comp < 6; comp++) {
       if (!((d.mask >> comp) & 1)) continue;
       const T v = comp < 3 ? e[comp] : h[comp - 3];
-      const double* ph = (comp < 3 ? d.phE : d.phH) + 2L * d.nfreq * a.step;
+      const double* ph = (comp < 3 ? d.phE : d.phH) + 2L * d.nfreq * step;
       T* acc = d.acc + (long)slot * d.nfreq * 2 * d.ncells + lq;
       for (int f = 0; f < d.nfreq; f++) {
         T* re = acc + (long)f * 2 * d.ncells;
@@ -349,193 +425,383 @@ __device__ __forceinline__ void dft_probe(const StepArgs<T>& a, long c, int i, i
     if (a.probe[p].cell == c) {
       const int comp = a.probe[p].comp;
       const T v = a.probe[p].field == 1 ? pick3(e[0], e[1], e[2], comp) : pick3(h[0], h[1], h[2], comp);
-      a.trace[a.step * a.nprobe + p] = (double)v;
+      a.trace[step * a.nprobe + p] = (double)v;
     }
 }
 
+// ---------------------------------------------------------------- two passes
+// One thread per cell and z plane: every load of the cell is issued before
+// any arithmetic, so a wave carries ~12 independent loads in flight.
 template <typename T>
-__global__ __launch_bounds__(256) void h_pass_3d(StepArgs<T> a, int kc) {
+__global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
+  const StepArgs<T>& a = *ap;
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k0 = 1 + blockIdx.z * kc;
   const int k1 = min(k0 + kc, a.nz - 1);
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
+  const T* __restrict__ Hx = a.H[0];
+  const T* __restrict__ Hy = a.H[1];
+  const T* __restrict__ Hz = a.H[2];
   const T* __restrict__ Ex = a.E[0];
   const T* __restrict__ Ey = a.E[1];
   const T* __restrict__ Ez = a.E[2];
-  T* __restrict__ Hx = a.H[0];
-  T* __restrict__ Hy = a.H[1];
-  T* __restrict__ Hz = a.H[2];
-  const uint8_t* __restrict__ mat = a.mat;
-  const T* __restrict__ mim = a.mim;
   const long sy = a.sy, sz = a.sz;
-  const T fx = a.cf[1][0][i], fy = a.cf[1][1][j];
-  const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
-  const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
+  const long step = a.step0 + s;
+  const double* incH = a.incH + (long)s * a.auxL;
+  for (int k = k0; k < k1; k++) {
+    const long c = (long)i + sy * (long)j + sz * (long)k;
+    HStencil<T> h;
+    h.hx = Hx[c];
+    h.hy = Hy[c];
+    h.hz = Hz[c];
+    h.hz_ym = Hz[c - sy];
+    h.hz_xm = Hz[c - 1];
+    h.hy_xm = Hy[c - 1];
+    h.hy_zm = Hy[c - sz];
+    h.hx_ym = Hx[c - sy];
+    h.hx_zm = Hx[c - sz];
+    const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
+    const int mid = a.mat[c];
+    T o[3];
+    e_cell_3d(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, true, o);
+    a.Eo[0][c] = o[0];
+    a.Eo[1][c] = o[1];
+    a.Eo[2][c] = o[2];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
+  const StepArgs<T>& a = *ap;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k0 = 1 + blockIdx.z * kc;
+  const int k1 = min(k0 + kc, a.nz - 1);
+  if (i >= a.nx - 1 || j >= a.ny - 1) return;
+  // the E pass wrote Eo; the H pass reads it
+  const T* __restrict__ Ex = a.Eo[0];
+  const T* __restrict__ Ey = a.Eo[1];
+  const T* __restrict__ Ez = a.Eo[2];
+  const T* __restrict__ Hx = a.H[0];
+  const T* __restrict__ Hy = a.H[1];
+  const T* __restrict__ Hz = a.H[2];
+  const long sy = a.sy, sz = a.sz;
+  const long step = a.step0 + s;
+  const double* incE = a.incE + (long)s * a.auxL;
   const bool extra = a.ndft || a.nprobe;
   for (int k = k0; k < k1; k++) {
     const long c = (long)i + sy * (long)j + sz * (long)k;
-    const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
-    const T ez_yp = Ez[c + sy], ez_xp = Ez[c + 1];
-    const T ey_xp = Ey[c + 1], ey_zp = Ey[c + sz];
-    const T ex_yp = Ex[c + sy], ex_zp = Ex[c + sz];
+    EStencil<T> e;
+    e.ex = Ex[c];
+    e.ey = Ey[c];
+    e.ez = Ez[c];
+    e.ez_yp = Ez[c + sy];
+    e.ez_xp = Ez[c + 1];
+    e.ey_xp = Ey[c + 1];
+    e.ey_zp = Ey[c + sz];
+    e.ex_yp = Ex[c + sy];
+    e.ex_zp = Ex[c + sz];
     const T hx = Hx[c], hy = Hy[c], hz = Hz[c];
-    const T im = mim[mat[c]];
-    const T fz = a.cf[1][2][k];
-    const T dxy = ez_yp - ez;  // Hx, J=y, +1
-    const T dxz = ey_zp - ey;  // Hx, J=z, -1
-    const T dyz = ex_zp - ex;  // Hy, J=z, +1
-    const T dyx = ez_xp - ez;  // Hy, J=x, -1
-    const T dzx = ey_xp - ey;  // Hz, J=x, +1
-    const T dzy = ex_yp - ex;  // Hz, J=y, -1
-    T d[3];
-    d[0] = fz * dxz - fy * dxy;
-    d[1] = fx * dyx - fz * dyz;
-    d[2] = fy * dzy - fx * dzx;
-    if (a.nsrc) add_sources(a, 2, c, d);
-    if (sxi >= 0) {
-      const long o = slab_off(a, 0, sxi, i, j, k);
-      T* __restrict__ q1 = a.psi[1][1][0];
-      T* __restrict__ q2 = a.psi[1][2][0];
-      const T g = a.cg[1][0][i], b = a.cb[1][0][i], cc = a.cc[1][0][i];
-      const T p1 = q1[o], p2 = q2[o];
-      d[1] -= g * p1;
-      d[2] -= g * p2;
-      q1[o] = b * p1 - cc * dyx;
-      q2[o] = b * p2 + cc * dzx;
-    }
-    if (syi >= 0) {
-      const long o = slab_off(a, 1, syi, i, j, k);
-      T* __restrict__ q0 = a.psi[1][0][1];
-      T* __restrict__ q2 = a.psi[1][2][1];
-      const T g = a.cg[1][1][j], b = a.cb[1][1][j], cc = a.cc[1][1][j];
-      const T p0 = q0[o], p2 = q2[o];
-      d[0] -= g * p0;
-      d[2] -= g * p2;
-      q0[o] = b * p0 + cc * dxy;
-      q2[o] = b * p2 - cc * dzy;
-    }
-    const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
-    if (szi >= 0) {
-      const long o = slab_off(a, 2, szi, i, j, k);
-      T* __restrict__ q0 = a.psi[1][0][2];
-      T* __restrict__ q1 = a.psi[1][1][2];
-      const T g = a.cg[1][2][k], b = a.cb[1][2][k], cc = a.cc[1][2][k];
-      const T p0 = q0[o], p1 = q1[o];
-      d[0] -= g * p0;
-      d[1] -= g * p1;
-      q0[o] = b * p0 - cc * dxz;
-      q1[o] = b * p1 + cc * dyz;
-    }
-    if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
-      const bool act[3] = {true, true, true};
-      tfsf_cell<T, 3>(a, 1, i, j, k, act, act, d);
-    }
-    T h[3];
-    h[0] = hx + d[0] * im;
-    h[1] = hy + d[1] * im;
-    h[2] = hz + d[2] * im;
-    Hx[c] = h[0];
-    Hy[c] = h[1];
-    Hz[c] = h[2];
+    const int mid = a.mat[c];
+    T o[3];
+    h_cell_3d(a, step, incE, i, j, k, c, e, hx, hy, hz, mid, o);
+    a.Ho[0][c] = o[0];
+    a.Ho[1][c] = o[1];
+    a.Ho[2][c] = o[2];
     if (extra) {
-      const T e[3] = {ex, ey, ez};
-      dft_probe<T>(a, c, i, j, k, e, h);
+      const T ev[3] = {e.ex, e.ey, e.ez};
+      dft_probe(a, step, c, i, j, k, ev, o);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- fused pass
+// Workgroup = 256 threads along x; thread tx holds column i = i0 - 1 + tx.
+// Columns i0..i0+253 are owned; column i0-1 only supplies H(i-1) and column
+// i0+254 is the halo whose new E the owned column i0+253 needs.  Rows
+// j0..j0+TY-1 are owned and row j0+TY is the halo row; planes k0..k1-1 are
+// owned and plane k1 is the halo plane.  Per z plane: load H(k+1) of the
+// block's rows, publish Hz/Hy through LDS for the x-neighbour, compute the new
+// E of plane k+1 (halo included), then the new H of plane k from the new E of
+// planes k and k+1 (x-neighbour E from LDS).  The old state is only read and
+// the new state only written, so halo recomputation never races.
+template <typename T, int TY>
+__device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, long step, const double* incH, int i, int tx, int j0,
+                                              int kk, bool own_plane, bool own_col, T (&lh)[2][TY + 1][256],
+                                              const T (&hzx)[TY + 1], const T (&hzy)[TY + 1], T (&hn)[TY + 2][3],
+                                              T (&en)[TY + 1][3]) {
+  const int nx = a.nx, ny = a.ny;
+  const long sy = a.sy, sz = a.sz;
+  const bool col_mem = i < nx;
+  // loads: H(kk) rows j0-1..j0+TY, E(kk) rows j0..j0+TY, material ids
+  T eo[TY + 1][3];
+  int mid[TY + 1];
+#pragma unroll
+  for (int r = 0; r < TY + 2; r++) {
+    const int j = j0 - 1 + r;
+    if (col_mem && j < ny) {
+      const long c = (long)i + sy * (long)j + sz * (long)kk;
+      hn[r][0] = a.H[0][c];
+      hn[r][1] = a.H[1][c];
+      hn[r][2] = a.H[2][c];
+    } else {
+      hn[r][0] = hn[r][1] = hn[r][2] = T(0);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < TY + 1; r++) {
+    const int j = j0 + r;
+    if (col_mem && j < ny) {
+      const long c = (long)i + sy * (long)j + sz * (long)kk;
+      eo[r][0] = a.E[0][c];
+      eo[r][1] = a.E[1][c];
+      eo[r][2] = a.E[2][c];
+      mid[r] = a.mat[c];
+    } else {
+      eo[r][0] = eo[r][1] = eo[r][2] = T(0);
+      mid[r] = 0;
+    }
+    lh[0][r][tx] = hn[r + 1][2];  // Hz
+    lh[1][r][tx] = hn[r + 1][1];  // Hy
+  }
+  __syncthreads();
+  const bool kin = kk >= 1 && kk <= a.nz - 2;
+#pragma unroll
+  for (int r = 0; r < TY + 1; r++) {
+    const int j = j0 + r;
+    en[r][0] = eo[r][0];
+    en[r][1] = eo[r][1];
+    en[r][2] = eo[r][2];
+    if (tx >= 1 && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && kin) {
+      const long c = (long)i + sy * (long)j + sz * (long)kk;
+      HStencil<T> h;
+      h.hx = hn[r + 1][0];
+      h.hy = hn[r + 1][1];
+      h.hz = hn[r + 1][2];
+      h.hz_ym = hn[r][2];
+      h.hx_ym = hn[r][0];
+      h.hz_xm = lh[0][r][tx - 1];
+      h.hy_xm = lh[1][r][tx - 1];
+      h.hy_zm = hzy[r];
+      h.hx_zm = hzx[r];
+      const bool own = own_plane && own_col && r < TY;
+      T o[3];
+      e_cell_3d(a, step, incH, i, j, kk, c, h, eo[r][0], eo[r][1], eo[r][2], mid[r], own, o);
+      en[r][0] = o[0];
+      en[r][1] = o[1];
+      en[r][2] = o[2];
+      if (own) {
+        a.Eo[0][c] = o[0];
+        a.Eo[1][c] = o[1];
+        a.Eo[2][c] = o[2];
+      }
+    }
+  }
+}
+
+// XCD-aware tile order: the hardware hands consecutive workgroup ids to the 8
+// XCDs in turn; this bijection gives each XCD a contiguous run of tiles, so
+// y-neighbouring tiles (which share halo rows) meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb / 8, r = nb % 8;
+  const int x = bid % 8, loc = bid / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(256) void fused_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
+                                                  int nkc) {
+  const StepArgs<T>& a = *ap;
+  __shared__ T lh[2][TY + 1][256];
+  __shared__ T le[2][TY][256];
+  const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+  const int tx = threadIdx.x;
+  const int i0 = 1 + bx * kFusedCols;
+  const int i = i0 - 1 + tx;
+  const int j0 = 1 + by * TY;
+  const int k0 = 1 + bz * kc;
+  const int k1 = min(k0 + kc, a.nz - 1);
+  const int nx = a.nx, ny = a.ny;
+  const long sy = a.sy, sz = a.sz;
+  const long step = a.step0 + s;
+  const double* incH = a.incH + (long)s * a.auxL;
+  const double* incE = a.incE + (long)s * a.auxL;
+  const bool own_col = tx >= 1 && tx <= kFusedCols && i <= nx - 2;
+  const bool extra = a.ndft || a.nprobe;
+
+  T hk[TY + 1][3];   // old H of plane k, rows j0..j0+TY
+  T ek[TY + 1][3];   // new E of plane k
+  T hn[TY + 2][3];   // old H of plane k+1, rows j0-1..j0+TY
+  T en[TY + 1][3];   // new E of plane k+1
+  T hzx[TY + 1], hzy[TY + 1];
+  // prologue: old Hx, Hy of plane k0-1, then the new E of plane k0
+#pragma unroll
+  for (int r = 0; r < TY + 1; r++) {
+    const int j = j0 + r;
+    if (i < nx && j < ny) {
+      const long c = (long)i + sy * (long)j + sz * (long)(k0 - 1);
+      hzx[r] = a.H[0][c];
+      hzy[r] = a.H[1][c];
+    } else {
+      hzx[r] = hzy[r] = T(0);
+    }
+  }
+  fused_e_plane<T, TY>(a, step, incH, i, tx, j0, k0, true, own_col, lh, hzx, hzy, hn, ek);
+#pragma unroll
+  for (int r = 0; r < TY + 1; r++) {
+    hk[r][0] = hn[r + 1][0];
+    hk[r][1] = hn[r + 1][1];
+    hk[r][2] = hn[r + 1][2];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < TY; r++) {
+    le[0][r][tx] = ek[r][1];  // Ey
+    le[1][r][tx] = ek[r][2];  // Ez
+  }
+  for (int k = k0; k < k1; k++) {
+#pragma unroll
+    for (int r = 0; r < TY + 1; r++) {
+      hzx[r] = hk[r][0];
+      hzy[r] = hk[r][1];
+    }
+    fused_e_plane<T, TY>(a, step, incH, i, tx, j0, k + 1, k + 1 < k1, own_col, lh, hzx, hzy, hn, en);
+    // new H of plane k on the owned cells
+#pragma unroll
+    for (int r = 0; r < TY; r++) {
+      const int j = j0 + r;
+      if (own_col && j <= ny - 2) {
+        const long c = (long)i + sy * (long)j + sz * (long)k;
+        EStencil<T> e;
+        e.ex = ek[r][0];
+        e.ey = ek[r][1];
+        e.ez = ek[r][2];
+        e.ez_yp = ek[r + 1][2];
+        e.ex_yp = ek[r + 1][0];
+        e.ey_xp = le[0][r][tx + 1];
+        e.ez_xp = le[1][r][tx + 1];
+        e.ey_zp = en[r][1];
+        e.ex_zp = en[r][0];
+        T o[3];
+        h_cell_3d(a, step, incE, i, j, k, c, e, hk[r][0], hk[r][1], hk[r][2], (int)a.mat[c], o);
+        a.Ho[0][c] = o[0];
+        a.Ho[1][c] = o[1];
+        a.Ho[2][c] = o[2];
+        if (extra) {
+          const T ev[3] = {e.ex, e.ey, e.ez};
+          dft_probe(a, step, c, i, j, k, ev, o);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < TY + 1; r++) {
+      ek[r][0] = en[r][0];
+      ek[r][1] = en[r][1];
+      ek[r][2] = en[r][2];
+      hk[r][0] = hn[r + 1][0];
+      hk[r][1] = hn[r + 1][1];
+      hk[r][2] = hn[r + 1][2];
+    }
+#pragma unroll
+    for (int r = 0; r < TY; r++) {
+      le[0][r][tx] = ek[r][1];
+      le[1][r][tx] = ek[r][2];
     }
   }
 }
 
 // ---------------------------------------------------------------- 2-D TM
-// Ez, Hx, Hy (include/YeeFields.hpp:123-148); CPML along x and y.
+// Ez, Hx, Hy (include/YeeFields.hpp:123-148); CPML along x and y; in place.
 template <typename T>
-__global__ __launch_bounds__(256) void e_pass_tm(StepArgs<T> a, int) {
+__global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
+  const StepArgs<T>& a = *ap;
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
   const T* __restrict__ Hx = a.H[0];
   const T* __restrict__ Hy = a.H[1];
-  T* __restrict__ Ez = a.E[2];
+  const long step = a.step0 + s;
   const long c = (long)i + a.sy * j;
   const T hx = Hx[c], hy = Hy[c], hy_xm = Hy[c - 1], hx_ym = Hx[c - a.sy];
-  const T ez = Ez[c];
-  const T ie = a.mie[a.mat[c]];
+  const T ez = a.E[2][c];
+  const int mid = a.mat[c];
+  const T ie = a.mie[mid];
   const T dzx = hy - hy_xm;  // J=x, +1
   const T dzy = hx - hx_ym;  // J=y, -1
   T d[3] = {T(0), T(0), T(0)};
   d[2] = a.cf[0][0][i] * dzx - a.cf[0][1][j] * dzy;
-  if (a.nsrc) add_sources(a, 0, c, d);
+  if (a.nsrc) add_sources(a, step, 0, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, 0);
-    T* __restrict__ q = a.psi[0][2][0];
-    const T p = q[o];
+    const T p = a.psi[0][2][0][o];
     d[2] += a.cg[0][0][i] * p;
-    q[o] = a.cb[0][0][i] * p + a.cc[0][0][i] * dzx;
+    a.psio[0][2][0][o] = a.cb[0][0][i] * p + a.cc[0][0][i] * dzx;
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
     const long o = slab_off(a, 1, syi, i, j, 0);
-    T* __restrict__ q = a.psi[0][2][1];
-    const T p = q[o];
+    const T p = a.psi[0][2][1][o];
     d[2] += a.cg[0][1][j] * p;
-    q[o] = a.cb[0][1][j] * p - a.cc[0][1][j] * dzy;
+    a.psio[0][2][1][o] = a.cb[0][1][j] * p - a.cc[0][1][j] * dzy;
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
-    tfsf_cell<T, 2>(a, 0, i, j, 0, aF, aC, d);
+    tfsf_cell<T, 2>(a, a.incH + (long)s * a.auxL, 0, i, j, 0, aF, aC, d);
   }
   if (ie > T(0)) {
-    Ez[c] = ez + d[2] * ie;
+    a.Eo[2][c] = ez + d[2] * ie;
   } else if (ie < T(0) && a.Dd[2]) {
-    const MatDev<T>& m = a.mats[a.mat[c]];
-    Ez[c] = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2]);
+    const MatDev<T>& m = a.mats[mid];
+    a.Eo[2][c] = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], true);
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void h_pass_tm(StepArgs<T> a, int) {
+__global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
+  const StepArgs<T>& a = *ap;
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const T* __restrict__ Ez = a.E[2];
-  T* __restrict__ Hx = a.H[0];
-  T* __restrict__ Hy = a.H[1];
+  const T* __restrict__ Ez = a.Eo[2];
+  const long step = a.step0 + s;
   const long c = (long)i + a.sy * j;
   const T ez = Ez[c], ez_yp = Ez[c + a.sy], ez_xp = Ez[c + 1];
-  const T hx = Hx[c], hy = Hy[c];
+  const T hx = a.H[0][c], hy = a.H[1][c];
   const T im = a.mim[a.mat[c]];
   const T dxy = ez_yp - ez;  // Hx, J=y, +1
   const T dyx = ez_xp - ez;  // Hy, J=x, -1
   T d[3] = {T(0), T(0), T(0)};
   d[0] = -a.cf[1][1][j] * dxy;
   d[1] = a.cf[1][0][i] * dyx;
-  if (a.nsrc) add_sources(a, 2, c, d);
+  if (a.nsrc) add_sources(a, step, 2, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, 0);
-    T* __restrict__ q = a.psi[1][1][0];
-    const T p = q[o];
+    const T p = a.psi[1][1][0][o];
     d[1] -= a.cg[1][0][i] * p;
-    q[o] = a.cb[1][0][i] * p - a.cc[1][0][i] * dyx;
+    a.psio[1][1][0][o] = a.cb[1][0][i] * p - a.cc[1][0][i] * dyx;
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
     const long o = slab_off(a, 1, syi, i, j, 0);
-    T* __restrict__ q = a.psi[1][0][1];
-    const T p = q[o];
+    const T p = a.psi[1][0][1][o];
     d[0] -= a.cg[1][1][j] * p;
-    q[o] = a.cb[1][1][j] * p + a.cc[1][1][j] * dxy;
+    a.psio[1][0][1][o] = a.cb[1][1][j] * p + a.cc[1][1][j] * dxy;
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {true, true, false}, aC[3] = {false, false, true};
-    tfsf_cell<T, 2>(a, 1, i, j, 0, aF, aC, d);
+    tfsf_cell<T, 2>(a, a.incE + (long)s * a.auxL, 1, i, j, 0, aF, aC, d);
   }
   T h[3] = {hx + d[0] * im, hy + d[1] * im, T(0)};
-  Hx[c] = h[0];
-  Hy[c] = h[1];
+  a.Ho[0][c] = h[0];
+  a.Ho[1][c] = h[1];
   if (a.ndft || a.nprobe) {
     const T e[3] = {T(0), T(0), ez};
-    dft_probe<T>(a, c, i, j, 0, e, h);
+    dft_probe(a, step, c, i, j, 0, e, h);
   }
 }
 

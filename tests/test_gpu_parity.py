@@ -171,3 +171,53 @@ def test_autotune_as_first_call():
     g.step(60)
     for nm, f, c in _comps(sc):
         np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=nm)
+
+
+FUSED = [(0, 2, 8), (0, 4, 16), (0, 8, 5)]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_fused_pass_equals_two_pass(precision):
+    """The fused single pass (ping-pong state, halo recomputation) gives the
+    same fields as the two in-place passes on an anisotropic grid whose x
+    extent spans two fused tiles, with CPML, blocks, TFSF, DFT and probes;
+    schedules may be switched between steps."""
+    sc = synth.grid3d_cpml_tfsf_dft(n=(300, 23, 21), nsteps=40, npml=4, inset=6)
+    ref = apply(_grid(sc, precision), sc)
+    ref.set_tile(256, 1, 1)
+    ref.step(40)
+    for tile in FUSED:
+        print(f"[gpu-case] fused {tile} {precision}", flush=True)
+        g = apply(_grid(sc, precision), sc)
+        g.set_tile(*tile)
+        g.step(17)
+        g.set_tile(128, 2, 2)
+        g.step(3)
+        g.set_tile(*tile)
+        g.step(20)
+        for nm, f, c in _comps(sc):
+            np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=f"{tile} {nm}")
+        np.testing.assert_array_equal(g.probes(), ref.probes())
+        for idx in range(len(sc["dfts"])):
+            np.testing.assert_array_equal(g.get_dft(idx), ref.get_dft(idx))
+        g.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_fused_pass_against_oracle(precision):
+    """Fused schedule on the dispersive sphere and on a CPML+TFSF+DFT grid,
+    against the FP64 oracle."""
+    for sc in (synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90),
+               synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90)):
+        print(f"[gpu-case] fused-vs-oracle {sc['name']} {precision}", flush=True)
+        g = apply(_grid(sc, precision), sc)
+        g.set_tile(0, 4, 8)
+        o = apply(_oracle(sc), sc)
+        g.step(sc["nsteps"])
+        o.step(sc["nsteps"])
+        errs = field_errors(g, o, sc)
+        for k, v in errs.items():
+            assert v <= TOL[precision], (sc["name"], k, v)
+        if sc["name"].startswith("grid3d_dispersive"):
+            for c in range(3):
+                assert rel_l2(g.get(P, c, 1), o.get(P, c, 1)) <= TOL[precision] * 10
