@@ -667,7 +667,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
     ntx = (g->d.nx - 2 + fdtdk::kFusedCols - 1) / fdtdk::kFusedCols;
     nty = (g->d.ny - 2 + t.by - 1) / t.by;
     nkc = (nzi + t.kc - 1) / t.kc;
-    block = dim3(256, 1, 1);
+    block = dim3(256, t.by + 1, 1);
     grid = dim3(ntx * nty * nkc, 1, 1);
   } else {
     block = dim3(t.bx, t.by, 1);
@@ -706,12 +706,12 @@ int step_impl(fdtd_grid* g, int nsteps) {
       int rc;
       if (fused) {
         const auto* ap = (s % 2) ? dev1 : dev0;
-        if (t.by == 2)
+        if (t.by == 1)
+          rc = launch(g, 3, fdtdk::fused_3d<T, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        else if (t.by == 2)
           rc = launch(g, 3, fdtdk::fused_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-        else if (t.by == 4)
-          rc = launch(g, 3, fdtdk::fused_3d<T, 4>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         else
-          rc = launch(g, 3, fdtdk::fused_3d<T, 8>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         swap_state(g);
       } else if (g->ndim == 3) {
         rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s);
@@ -1061,8 +1061,8 @@ long fdtd_step_count(const fdtd_grid* g) { return g ? g->step : -1; }
 int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
   if (bx == 0) {
-    if (g->ndim != 3 || (by != 2 && by != 4 && by != 8) || kc <= 0)
-      return fail(FDTD_EINVAL, "the fused schedule (bx = 0) is 3-D only, with by in {2, 4, 8} rows and kc >= 1");
+    if (g->ndim != 3 || by < 1 || by > 3 || kc <= 0)
+      return fail(FDTD_EINVAL, "the fused schedule (bx = 0) is 3-D only, with by in {1, 2, 3} rows and kc >= 1");
   } else if (bx < 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0) {
     return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
   }
@@ -1075,7 +1075,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
   static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {128, 2, 1}, {128, 2, 2}, {64, 4, 1},
-                               {0, 2, 8},   {0, 2, 16},  {0, 4, 8},   {0, 4, 16},  {0, 4, 32},  {0, 8, 16}};
+                               {0, 1, 16},  {0, 2, 8},   {0, 2, 16},  {0, 3, 8},   {0, 3, 16},  {0, 3, 32}};
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;
   const int ncand = g->ndim == 3 ? int(sizeof cand3 / sizeof cand3[0]) : int(sizeof cand2 / sizeof cand2[0]);

@@ -474,9 +474,13 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
 template <typename T>
 __global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
   const StepArgs<T>& a = *ap;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k0 = 1 + blockIdx.z * kc;
+  // Workgroups run in reverse order of the E pass: the first tiles of this
+  // pass read the E and H the E pass touched last, which are still in the
+  // Infinity Cache; the next E pass then starts where this pass ended.
+  const int bxr = gridDim.x - 1 - blockIdx.x, byr = gridDim.y - 1 - blockIdx.y, bzr = gridDim.z - 1 - blockIdx.z;
+  const int i = 1 + bxr * blockDim.x + threadIdx.x;
+  const int j = 1 + byr * blockDim.y + threadIdx.y;
+  const int k0 = 1 + bzr * kc;
   const int k1 = min(k0 + kc, a.nz - 1);
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
   // the E pass wrote Eo; the H pass reads it
@@ -517,85 +521,84 @@ __global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__
 }
 
 // ---------------------------------------------------------------- fused pass
-// Workgroup = 256 threads along x; thread tx holds column i = i0 - 1 + tx.
-// Columns i0..i0+253 are owned; column i0-1 only supplies H(i-1) and column
-// i0+254 is the halo whose new E the owned column i0+253 needs.  Rows
-// j0..j0+TY-1 are owned and row j0+TY is the halo row; planes k0..k1-1 are
-// owned and plane k1 is the halo plane.  Per z plane: load H(k+1) of the
-// block's rows, publish Hz/Hy through LDS for the x-neighbour, compute the new
-// E of plane k+1 (halo included), then the new H of plane k from the new E of
-// planes k and k+1 (x-neighbour E from LDS).  The old state is only read and
-// the new state only written, so halo recomputation never races.
-template <typename T, int TY>
-__device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, long step, const double* incH, int i, int tx, int j0,
-                                              int kk, bool own_plane, bool own_col, T (&lh)[2][TY + 1][256],
-                                              const T (&hzx)[TY + 1], const T (&hzy)[TY + 1], T (&hn)[TY + 2][3],
-                                              T (&en)[TY + 1][3]) {
-  const int nx = a.nx, ny = a.ny;
-  const long sy = a.sy, sz = a.sz;
-  const bool col_mem = i < nx;
-  // loads: H(kk) rows j0-1..j0+TY, E(kk) rows j0..j0+TY, material ids
-  T eo[TY + 1][3];
-  int mid[TY + 1];
-#pragma unroll
-  for (int r = 0; r < TY + 2; r++) {
-    const int j = j0 - 1 + r;
-    if (col_mem && j < ny) {
-      const long c = (long)i + sy * (long)j + sz * (long)kk;
-      hn[r][0] = a.H[0][c];
-      hn[r][1] = a.H[1][c];
-      hn[r][2] = a.H[2][c];
-    } else {
-      hn[r][0] = hn[r][1] = hn[r][2] = T(0);
-    }
+// Workgroup = 256 x (TY+1) threads.  Thread (tx, ty) holds column
+// i = i0 - 1 + tx and row j = j0 + ty.  Columns i0..i0+253 and rows
+// j0..j0+TY-1 are owned; column i0-1 only supplies H(i-1), column i0+254 and
+// row j0+TY are the halo whose new E the owned cells need, and plane k1 is
+// the halo plane.  Per z plane: load the old H of plane k+1 and publish it in
+// LDS, compute the new E of plane k+1 (halo included), then the new H of
+// plane k from the new E of planes k and k+1 (x and y neighbours from LDS),
+// then publish the new E of plane k+1.  The old state is only read and the
+// new state only written, so halo recomputation never races.
+// old state of one plane at this thread's (i,j), loaded one plane ahead
+template <typename T>
+struct PlaneLoad {
+  T h[3], e[3];
+  T lx, lz;  // Hx, Hz of row j0-1 (first row of threads only)
+  int mid;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_plane(const StepArgs<T>& a, int i, int j, int kk, bool mem, bool lead,
+                                           PlaneLoad<T>& p) {
+  const long c = (long)i + a.sy * (long)j + a.sz * (long)kk;
+  p.h[0] = p.h[1] = p.h[2] = T(0);
+  p.e[0] = p.e[1] = p.e[2] = T(0);
+  p.lx = p.lz = T(0);
+  p.mid = 0;
+  if (mem) {
+    p.h[0] = a.H[0][c];
+    p.h[1] = a.H[1][c];
+    p.h[2] = a.H[2][c];
+    p.e[0] = a.E[0][c];
+    p.e[1] = a.E[1][c];
+    p.e[2] = a.E[2][c];
+    p.mid = a.mat[c];
   }
-#pragma unroll
-  for (int r = 0; r < TY + 1; r++) {
-    const int j = j0 + r;
-    if (col_mem && j < ny) {
-      const long c = (long)i + sy * (long)j + sz * (long)kk;
-      eo[r][0] = a.E[0][c];
-      eo[r][1] = a.E[1][c];
-      eo[r][2] = a.E[2][c];
-      mid[r] = a.mat[c];
-    } else {
-      eo[r][0] = eo[r][1] = eo[r][2] = T(0);
-      mid[r] = 0;
-    }
-    lh[0][r][tx] = hn[r + 1][2];  // Hz
-    lh[1][r][tx] = hn[r + 1][1];  // Hy
+  if (lead) {  // row j0-1 of this column: the y-neighbour of row j0
+    p.lx = a.H[0][c - a.sy];
+    p.lz = a.H[2][c - a.sy];
+  }
+}
+
+// publish the old H of plane kk in LDS, then the new E of (i,j,kk)
+template <typename T, int TY>
+__device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, long step, const double* incH, int i, int j, int tx,
+                                              int ty, int kk, bool own_e, bool ecell, const PlaneLoad<T>& p, T hzx,
+                                              T hzy, T (&sH)[3][TY + 2][256], T en[3]) {
+  sH[0][ty + 1][tx] = p.h[0];
+  sH[1][ty + 1][tx] = p.h[1];
+  sH[2][ty + 1][tx] = p.h[2];
+  if (ty == 0) {
+    sH[0][0][tx] = p.lx;
+    sH[1][0][tx] = T(0);
+    sH[2][0][tx] = p.lz;
   }
   __syncthreads();
-  const bool kin = kk >= 1 && kk <= a.nz - 2;
-#pragma unroll
-  for (int r = 0; r < TY + 1; r++) {
-    const int j = j0 + r;
-    en[r][0] = eo[r][0];
-    en[r][1] = eo[r][1];
-    en[r][2] = eo[r][2];
-    if (tx >= 1 && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && kin) {
-      const long c = (long)i + sy * (long)j + sz * (long)kk;
-      HStencil<T> h;
-      h.hx = hn[r + 1][0];
-      h.hy = hn[r + 1][1];
-      h.hz = hn[r + 1][2];
-      h.hz_ym = hn[r][2];
-      h.hx_ym = hn[r][0];
-      h.hz_xm = lh[0][r][tx - 1];
-      h.hy_xm = lh[1][r][tx - 1];
-      h.hy_zm = hzy[r];
-      h.hx_zm = hzx[r];
-      const bool own = own_plane && own_col && r < TY;
-      T o[3];
-      e_cell_3d(a, step, incH, i, j, kk, c, h, eo[r][0], eo[r][1], eo[r][2], mid[r], own, o);
-      en[r][0] = o[0];
-      en[r][1] = o[1];
-      en[r][2] = o[2];
-      if (own) {
-        a.Eo[0][c] = o[0];
-        a.Eo[1][c] = o[1];
-        a.Eo[2][c] = o[2];
-      }
+  en[0] = p.e[0];
+  en[1] = p.e[1];
+  en[2] = p.e[2];
+  if (ecell && kk >= 1 && kk <= a.nz - 2) {
+    const long c = (long)i + a.sy * (long)j + a.sz * (long)kk;
+    HStencil<T> h;
+    h.hx = p.h[0];
+    h.hy = p.h[1];
+    h.hz = p.h[2];
+    h.hz_ym = sH[2][ty][tx];
+    h.hx_ym = sH[0][ty][tx];
+    h.hz_xm = sH[2][ty + 1][tx - 1];
+    h.hy_xm = sH[1][ty + 1][tx - 1];
+    h.hy_zm = hzy;
+    h.hx_zm = hzx;
+    T o[3];
+    e_cell_3d(a, step, incH, i, j, kk, c, h, p.e[0], p.e[1], p.e[2], p.mid, own_e, o);
+    en[0] = o[0];
+    en[1] = o[1];
+    en[2] = o[2];
+    if (own_e) {
+      a.Eo[0][c] = o[0];
+      a.Eo[1][c] = o[1];
+      a.Eo[2][c] = o[2];
     }
   }
 }
@@ -610,17 +613,18 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 }
 
 template <typename T, int TY>
-__global__ __launch_bounds__(256) void fused_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
-                                                  int nkc) {
+__global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx,
+                                                            int nty, int nkc) {
   const StepArgs<T>& a = *ap;
-  __shared__ T lh[2][TY + 1][256];
-  __shared__ T le[2][TY][256];
+  __shared__ T sH[3][TY + 2][256];
+  __shared__ T sE[3][TY + 1][256];
   const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-  const int tx = threadIdx.x;
+  const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = 1 + bx * kFusedCols;
   const int i = i0 - 1 + tx;
   const int j0 = 1 + by * TY;
+  const int j = j0 + ty;
   const int k0 = 1 + bz * kc;
   const int k1 = min(k0 + kc, a.nz - 1);
   const int nx = a.nx, ny = a.ny;
@@ -628,88 +632,71 @@ __global__ __launch_bounds__(256) void fused_3d(const StepArgs<T>* __restrict__ 
   const long step = a.step0 + s;
   const double* incH = a.incH + (long)s * a.auxL;
   const double* incE = a.incE + (long)s * a.auxL;
-  const bool own_col = tx >= 1 && tx <= kFusedCols && i <= nx - 2;
+  const bool mem = i < nx && j < ny;
+  const bool ecell = tx >= 1 && i <= nx - 2 && j <= ny - 2;
+  const bool own = ecell && tx <= kFusedCols && ty < TY;
+  const bool lead = ty == 0 && i < nx;
   const bool extra = a.ndft || a.nprobe;
 
-  T hk[TY + 1][3];   // old H of plane k, rows j0..j0+TY
-  T ek[TY + 1][3];   // new E of plane k
-  T hn[TY + 2][3];   // old H of plane k+1, rows j0-1..j0+TY
-  T en[TY + 1][3];   // new E of plane k+1
-  T hzx[TY + 1], hzy[TY + 1];
-  // prologue: old Hx, Hy of plane k0-1, then the new E of plane k0
-#pragma unroll
-  for (int r = 0; r < TY + 1; r++) {
-    const int j = j0 + r;
-    if (i < nx && j < ny) {
-      const long c = (long)i + sy * (long)j + sz * (long)(k0 - 1);
-      hzx[r] = a.H[0][c];
-      hzy[r] = a.H[1][c];
-    } else {
-      hzx[r] = hzy[r] = T(0);
-    }
+  T hk[3], ek[3], en[3];
+  int midk = 0;
+  T hzx = T(0), hzy = T(0);
+  if (mem) {  // old Hx, Hy of plane k0-1 (z-neighbour of plane k0)
+    const long c = (long)i + sy * (long)j + sz * (long)(k0 - 1);
+    hzx = a.H[0][c];
+    hzy = a.H[1][c];
   }
-  fused_e_plane<T, TY>(a, step, incH, i, tx, j0, k0, true, own_col, lh, hzx, hzy, hn, ek);
-#pragma unroll
-  for (int r = 0; r < TY + 1; r++) {
-    hk[r][0] = hn[r + 1][0];
-    hk[r][1] = hn[r + 1][1];
-    hk[r][2] = hn[r + 1][2];
-  }
+  PlaneLoad<T> cur, nxt;
+  load_plane(a, i, j, k0, mem, lead, cur);
+  load_plane(a, i, j, k0 + 1, mem, lead, nxt);  // plane k1 <= nz-1 always exists
+  fused_e_plane<T, TY>(a, step, incH, i, j, tx, ty, k0, own, ecell, cur, hzx, hzy, sH, ek);
+  hk[0] = cur.h[0];
+  hk[1] = cur.h[1];
+  hk[2] = cur.h[2];
+  midk = cur.mid;
+  cur = nxt;
   __syncthreads();
-#pragma unroll
-  for (int r = 0; r < TY; r++) {
-    le[0][r][tx] = ek[r][1];  // Ey
-    le[1][r][tx] = ek[r][2];  // Ez
-  }
+  sE[0][ty][tx] = ek[0];
+  sE[1][ty][tx] = ek[1];
+  sE[2][ty][tx] = ek[2];
   for (int k = k0; k < k1; k++) {
-#pragma unroll
-    for (int r = 0; r < TY + 1; r++) {
-      hzx[r] = hk[r][0];
-      hzy[r] = hk[r][1];
-    }
-    fused_e_plane<T, TY>(a, step, incH, i, tx, j0, k + 1, k + 1 < k1, own_col, lh, hzx, hzy, hn, en);
-    // new H of plane k on the owned cells
-#pragma unroll
-    for (int r = 0; r < TY; r++) {
-      const int j = j0 + r;
-      if (own_col && j <= ny - 2) {
-        const long c = (long)i + sy * (long)j + sz * (long)k;
-        EStencil<T> e;
-        e.ex = ek[r][0];
-        e.ey = ek[r][1];
-        e.ez = ek[r][2];
-        e.ez_yp = ek[r + 1][2];
-        e.ex_yp = ek[r + 1][0];
-        e.ey_xp = le[0][r][tx + 1];
-        e.ez_xp = le[1][r][tx + 1];
-        e.ey_zp = en[r][1];
-        e.ex_zp = en[r][0];
-        T o[3];
-        h_cell_3d(a, step, incE, i, j, k, c, e, hk[r][0], hk[r][1], hk[r][2], (int)a.mat[c], o);
-        a.Ho[0][c] = o[0];
-        a.Ho[1][c] = o[1];
-        a.Ho[2][c] = o[2];
-        if (extra) {
-          const T ev[3] = {e.ex, e.ey, e.ez};
-          dft_probe(a, step, c, i, j, k, ev, o);
-        }
+    // issue the loads of plane k+2 before working on planes k+1 and k
+    if (k + 2 <= k1) load_plane(a, i, j, k + 2, mem, lead, nxt);
+    fused_e_plane<T, TY>(a, step, incH, i, j, tx, ty, k + 1, own && k + 1 < k1, ecell, cur, hk[0], hk[1], sH, en);
+    if (own) {  // new H of plane k
+      const long c = (long)i + sy * (long)j + sz * (long)k;
+      EStencil<T> e;
+      e.ex = ek[0];
+      e.ey = ek[1];
+      e.ez = ek[2];
+      e.ez_yp = sE[2][ty + 1][tx];
+      e.ex_yp = sE[0][ty + 1][tx];
+      e.ey_xp = sE[1][ty][tx + 1];
+      e.ez_xp = sE[2][ty][tx + 1];
+      e.ey_zp = en[1];
+      e.ex_zp = en[0];
+      T o[3];
+      h_cell_3d(a, step, incE, i, j, k, c, e, hk[0], hk[1], hk[2], midk, o);
+      a.Ho[0][c] = o[0];
+      a.Ho[1][c] = o[1];
+      a.Ho[2][c] = o[2];
+      if (extra) {
+        const T ev[3] = {e.ex, e.ey, e.ez};
+        dft_probe(a, step, c, i, j, k, ev, o);
       }
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < TY + 1; r++) {
-      ek[r][0] = en[r][0];
-      ek[r][1] = en[r][1];
-      ek[r][2] = en[r][2];
-      hk[r][0] = hn[r + 1][0];
-      hk[r][1] = hn[r + 1][1];
-      hk[r][2] = hn[r + 1][2];
-    }
-#pragma unroll
-    for (int r = 0; r < TY; r++) {
-      le[0][r][tx] = ek[r][1];
-      le[1][r][tx] = ek[r][2];
-    }
+    sE[0][ty][tx] = en[0];
+    sE[1][ty][tx] = en[1];
+    sE[2][ty][tx] = en[2];
+    ek[0] = en[0];
+    ek[1] = en[1];
+    ek[2] = en[2];
+    hk[0] = cur.h[0];
+    hk[1] = cur.h[1];
+    hk[2] = cur.h[2];
+    midk = cur.mid;
+    cur = nxt;
   }
 }
 

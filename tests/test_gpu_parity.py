@@ -173,7 +173,7 @@ def test_autotune_as_first_call():
         np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=nm)
 
 
-FUSED = [(0, 2, 8), (0, 4, 16), (0, 8, 5)]
+FUSED = [(0, 1, 8), (0, 2, 16), (0, 3, 5)]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -211,7 +211,7 @@ def test_fused_pass_against_oracle(precision):
                synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90)):
         print(f"[gpu-case] fused-vs-oracle {sc['name']} {precision}", flush=True)
         g = apply(_grid(sc, precision), sc)
-        g.set_tile(0, 4, 8)
+        g.set_tile(0, 3, 8)
         o = apply(_oracle(sc), sc)
         g.step(sc["nsteps"])
         o.step(sc["nsteps"])
