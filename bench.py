@@ -74,6 +74,121 @@ def cpu_baseline(size, sample_steps):
             "kind": "port", "sample": f"{size}^3 full workload, {sample_steps} timed steps, FP64 oracle"}
 
 
+def timed_steps(g, steps):
+    """hipEvent-timed steps on the grid's stream; returns (ms_total, per-kernel avg ms)."""
+    import torch
+    g.synchronize()
+    torch.cuda.synchronize()
+    g.set_profiling(True)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    g.step(steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    g.synchronize()
+    avg = {}
+    for kid, name in ((0, "e_pass"), (1, "h_pass"), (2, "aux_line"), (3, "fused_pass")):
+        ms, n = g.kernel_time(kid)
+        if n:
+            avg[name] = round(ms / n, 5)
+    g.set_profiling(False)
+    return ev0.elapsed_time(ev1), avg
+
+
+def base_line(args, metric_value, unit, ms_step, dtype, config, higher=True):
+    return {"metric": METRIC, "value": metric_value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def run_c1(args):
+    """2-D TM 256^2 FP64, 10-cell CPML, centre Gaussian pulse, 4 probes (cache-resident: Gcell/s only)."""
+    from fdtd_amd import Grid
+    import synth
+    from scenarios import apply
+    sc = synth.tm2d_c1(nsteps=args.warmup + args.steps)
+    g = apply(Grid("TM", 256, 256, 1, sc["dx"], sc["dt"], precision="f64"), sc)
+    tile = g.autotune(trial_steps=20)
+    g.step(args.warmup)
+    ms, avg = timed_steps(g, args.steps)
+    interior = 254 * 254
+    out = base_line(args, round(interior * args.steps / (ms / 1e3) / 1e9, 4), "Gcell-updates/s",
+                    round(ms / args.steps, 5), "f64",
+                    {"workload": "2-D TM 256^2 FP64, 10-cell CPML, Dz Gaussian soft source, 4 Ez probes",
+                     "tile": list(tile), "kernel_avg_ms": avg,
+                     "note": "state (~3.5 MB) is cache-resident: no HBM roofline"})
+    return out
+
+
+def run_c3(args):
+    """3-D 192^3 FP32 Drude + 2-Lorentz sphere (r=40), 12-cell CPML, dipole, DFT box surface x 16 freqs."""
+    from fdtd_amd import Grid
+    import synth
+    from scenarios import apply
+    n = args.size if args.size != 256 else 192
+    sc = synth.grid3d_dispersive_sphere(n=n, radius=40 if n >= 160 else n // 5, nsteps=args.warmup + args.steps)
+    g = apply(Grid("ThreeD", n, n, n, sc["dx"], sc["dt"], precision=args.precision), sc)
+    tile = g.autotune(trial_steps=10)
+    g.step(args.warmup)
+    ms, avg = timed_steps(g, args.steps)
+    esz = 4 if args.precision == "f32" else 8
+    abytes, interior = algorithmic_bytes(sc, esz)
+    disp = int((sc["matid"] == 1).sum())
+    abytes += disp * 3 * (1 + 2 * 3) * 2 * esz  # D and 3 poles x (P, J), read + written
+    ms_step = ms / args.steps
+    achieved = abytes / (ms_step / 1e3) / 1e9
+    return base_line(args, round(interior * args.steps / (ms / 1e3) / 1e9, 4), "Gcell-updates/s", round(ms_step, 5),
+                     args.precision,
+                     {"workload": f"3-D {n}^3 {args.precision.upper()}: Drude + 2 Lorentz sphere r=40, 12-cell CPML, "
+                                  f"z-dipole soft source, DFT of 6 comps on the sphere's box surface x 16 freqs",
+                      "tile": list(tile), "dispersive_cells": disp, "kernel_avg_ms": avg,
+                      "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}})
+
+
+def run_c4(args):
+    """64 uneven 3-D FP32 grids: one batch (tile table, hipGraph chunks) vs the same grids stepped one by one."""
+    import torch
+    from fdtd_amd import Batch, Grid
+    import synth
+    from scenarios import apply
+    nsteps = args.steps
+    scs = synth.batch_c4(nsteps=nsteps + args.warmup)
+    mk = lambda: [apply(Grid("ThreeD", sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"], precision="f32"), sc)
+                  for sc in scs]
+    seq = mk()
+    for g in seq:
+        g.step(args.warmup)
+        g.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for g in seq:
+        g.step(nsteps)
+    for g in seq:
+        g.synchronize()
+    t_seq = time.perf_counter() - t0
+    for g in seq:
+        g.close()
+    bat = mk()
+    b = Batch(bat)
+    b.step(args.warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(nsteps)
+    b.synchronize()
+    t_bat = time.perf_counter() - t0
+    cells = sum((sc["nx"] - 2) * (sc["ny"] - 2) * (sc["nz"] - 2) for sc in scs)
+    sims_bat = len(scs) / t_bat
+    return base_line(args, round(sims_bat, 4), "sims/s", round(t_bat / nsteps * 1e3, 5), "f32",
+                     {"workload": f"{len(scs)} independent 3-D FP32 grids, edges U[64,160], 8-cell CPML, 10 eps "
+                                  f"blocks, Dz point source, 4-freq DFT plane each, {nsteps} steps",
+                      "total_interior_cells": cells, "batched_sims_per_s": round(sims_bat, 4),
+                      "sequential_sims_per_s": round(len(scs) / t_seq, 4), "speedup": round(t_seq / t_bat, 3),
+                      "batched_gcell_per_s": round(cells * nsteps / t_bat / 1e9, 3),
+                      "sequential_gcell_per_s": round(cells * nsteps / t_seq / 1e9, 3)})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +198,9 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+                    help="c2 (default): the headline 3-D 256^3 forward; c1: 2-D TM 256^2; c3: dispersive sphere; "
+                         "c4: 64 batched grids vs sequential")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (z-slab sharding is a design "
@@ -95,6 +213,10 @@ def main():
     from scenarios import apply
 
     torch.cuda.set_device(0)
+    if args.workload != "c2":
+        fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
     total = args.warmup + args.steps
     sc = synth.grid3d_cpml_tfsf_dft(n=args.size, nsteps=total)
     esz = 4 if args.precision == "f32" else 8

@@ -3,6 +3,6 @@
 The compute path is HIP (fdtd_amd/csrc, built into libfdtd_hip.so); this
 package is its Python binding.
 """
-from .grid import Grid, FdtdError, load_library  # noqa: F401
+from .grid import Batch, Grid, FdtdError, load_library  # noqa: F401
 
-__all__ = ["Grid", "FdtdError", "load_library"]
+__all__ = ["Batch", "Grid", "FdtdError", "load_library"]

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/fdtd_hip.h"
@@ -1176,6 +1177,183 @@ int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches) {
   if (rc) return rc;
   if (ms) *ms = g->kms[kernel];
   if (launches) *launches = g->kcount[kernel];
+  return 0;
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Independent grids of one precision stepped together: one launch per pass
+// covers the tiles of every grid (a tile table maps workgroups to grids), and
+// each chunk of steps is captured once in a hipGraph and replayed.
+struct fdtd_batch {
+  std::vector<fdtd_grid*> grids;
+  std::vector<hipStream_t> saved_streams;
+  bool f64 = false;
+  Tile tile = {64, 4, 2};
+  hipStream_t stream = nullptr;
+  int4* tiles = nullptr;
+  int ntiles = 0;
+  static constexpr int kRing = 2;
+  char* args_dev = nullptr;
+  char* args_host = nullptr;
+  hipEvent_t ev[kRing] = {};
+  bool ev_used[kRing] = {};
+  int slot = 0;
+  bool use_graph = true;
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (slot, steps) -> executable graph
+
+  ~fdtd_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (tiles) (void)hipFree(tiles);
+    if (args_dev) (void)hipFree(args_dev);
+    if (args_host) (void)hipHostFree(args_host);
+    for (size_t q = 0; q < grids.size(); q++) grids[q]->stream = saved_streams[q];
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+template <typename T>
+int batch_step_impl(fdtd_batch* b, int nsteps) {
+  const size_t n = b->grids.size();
+  for (fdtd_grid* g : b->grids) {
+    int rc = 0;
+    if (g->mats_dirty && (rc = upload_materials<T>(g))) return rc;
+    if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
+    if ((rc = ensure_trace(g, g->step + nsteps))) return rc;
+  }
+  int done = 0;
+  while (done < nsteps) {
+    const int chunk = std::min(kChunk, nsteps - done);
+    for (fdtd_grid* g : b->grids)
+      if (g->tfsf) {
+        int rc = run_aux(g, g->step, chunk);
+        if (rc) return rc;
+      }
+    const int slot = b->slot;
+    b->slot = (slot + 1) % fdtd_batch::kRing;
+    if (b->ev_used[slot]) HIP_TRY(hipEventSynchronize(b->ev[slot]));
+    auto* host = reinterpret_cast<fdtdk::StepArgs<T>*>(b->args_host + slot * n * sizeof(fdtdk::StepArgs<T>));
+    for (size_t q = 0; q < n; q++) {
+      fdtd_grid* g = b->grids[q];
+      fdtdk::StepArgs<T> a;
+      fill_args<T>(g, a);
+      a.step0 = g->step;
+      a.incE = g->incE;
+      a.incH = g->incH;
+      host[q] = a;
+    }
+    auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + slot * n * sizeof(fdtdk::StepArgs<T>));
+    HIP_TRY(hipMemcpyAsync((void*)dev, host, n * sizeof(fdtdk::StepArgs<T>), hipMemcpyHostToDevice, b->stream));
+    const dim3 block(b->tile.bx, b->tile.by, 1), grid(b->ntiles, 1, 1);
+    if (b->use_graph) {
+      auto key = std::make_pair(slot, chunk);
+      auto it = b->graphs.find(key);
+      if (it == b->graphs.end()) {
+        hipGraph_t graph;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int s = 0; s < chunk; s++) {
+          hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+          hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+        }
+        HIP_TRY(hipStreamEndCapture(b->stream, &graph));
+        hipGraphExec_t exec;
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(FDTD_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        it = b->graphs.emplace(key, exec).first;
+      }
+      HIP_TRY(hipGraphLaunch(it->second, b->stream));
+    } else {
+      for (int s = 0; s < chunk; s++) {
+        hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+        hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+      }
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(b->ev[slot], b->stream));
+    b->ev_used[slot] = true;
+    for (fdtd_grid* g : b->grids) g->step += chunk;
+    done += chunk;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
+  if (!grids || n < 1 || !out) return fail(FDTD_EINVAL, "bad argument");
+  *out = nullptr;
+  for (int q = 0; q < n; q++) {
+    if (!grids[q]) return fail(FDTD_EINVAL, "null grid in batch");
+    if (grids[q]->ndim != 3) return fail(FDTD_EINVAL, "batches hold 3-D grids");
+    if (grids[q]->f64 != grids[0]->f64) return fail(FDTD_EINVAL, "batched grids must share one precision");
+    for (int r = 0; r < q; r++)
+      if (grids[r] == grids[q]) return fail(FDTD_EINVAL, "a grid appears twice in the batch");
+  }
+  fdtd_batch* b = new fdtd_batch();
+  b->f64 = grids[0]->f64;
+  int rc = 0;
+  if (hipStreamCreate(&b->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipStreamCreate failed");
+  std::vector<int4> tab;
+  for (int q = 0; q < n && !rc; q++) {
+    fdtd_grid* g = grids[q];
+    (void)hipStreamSynchronize(g->stream);
+    b->grids.push_back(g);
+    b->saved_streams.push_back(g->stream);
+    g->stream = b->stream;
+    if (!g->finalized) rc = finalize(g);
+    const int tx = (g->d.nx - 2 + b->tile.bx - 1) / b->tile.bx, ty = (g->d.ny - 2 + b->tile.by - 1) / b->tile.by;
+    const int tz = (g->d.nz - 2 + b->tile.kc - 1) / b->tile.kc;
+    for (int z = 0; z < tz; z++)
+      for (int y = 0; y < ty; y++)
+        for (int x = 0; x < tx; x++) tab.push_back(make_int4(q, x, y, z));
+  }
+  b->ntiles = int(tab.size());
+  if (!rc && hipMalloc((void**)&b->tiles, tab.size() * sizeof(int4)) != hipSuccess) rc = fail(FDTD_ENOMEM, "tile table");
+  if (!rc && hipMemcpy(b->tiles, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(FDTD_EDEVICE, "tile table upload");
+  const size_t abytes = size_t(fdtd_batch::kRing) * n * sizeof(fdtdk::StepArgs<double>);
+  if (!rc && hipMalloc((void**)&b->args_dev, abytes) != hipSuccess) rc = fail(FDTD_ENOMEM, "batch arguments");
+  if (!rc && hipHostMalloc((void**)&b->args_host, abytes) != hipSuccess) rc = fail(FDTD_ENOMEM, "batch staging");
+  for (int r = 0; r < fdtd_batch::kRing && !rc; r++)
+    if (hipEventCreateWithFlags(&b->ev[r], hipEventDisableTiming) != hipSuccess) rc = fail(FDTD_EDEVICE, "event");
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int fdtd_batch_destroy(fdtd_batch* b) {
+  delete b;
+  return 0;
+}
+
+int fdtd_batch_set_graph(fdtd_batch* b, int on) {
+  if (!b) return fail(FDTD_EINVAL, "null batch");
+  b->use_graph = on != 0;
+  return 0;
+}
+
+int fdtd_batch_step(fdtd_batch* b, int nsteps) {
+  if (!b || nsteps < 0) return fail(FDTD_EINVAL, "bad argument");
+  if (nsteps == 0) return 0;
+  return b->f64 ? batch_step_impl<double>(b, nsteps) : batch_step_impl<float>(b, nsteps);
+}
+
+int fdtd_batch_synchronize(fdtd_batch* b) {
+  if (!b) return fail(FDTD_EINVAL, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return 0;
 }
 

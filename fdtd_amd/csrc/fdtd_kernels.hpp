@@ -433,12 +433,7 @@ __device__ FDTD_COLD void dft_probe(const StepArgs<T>& a, long step, long c, int
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
 template <typename T>
-__global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
-  const StepArgs<T>& a = *ap;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k0 = 1 + blockIdx.z * kc;
-  const int k1 = min(k0 + kc, a.nz - 1);
+__device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, int i, int j, int k0, int k1, int s) {
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
   const T* __restrict__ Hx = a.H[0];
   const T* __restrict__ Hy = a.H[1];
@@ -472,16 +467,7 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
-  const StepArgs<T>& a = *ap;
-  // Workgroups run in reverse order of the E pass: the first tiles of this
-  // pass read the E and H the E pass touched last, which are still in the
-  // Infinity Cache; the next E pass then starts where this pass ended.
-  const int bxr = gridDim.x - 1 - blockIdx.x, byr = gridDim.y - 1 - blockIdx.y, bzr = gridDim.z - 1 - blockIdx.z;
-  const int i = 1 + bxr * blockDim.x + threadIdx.x;
-  const int j = 1 + byr * blockDim.y + threadIdx.y;
-  const int k0 = 1 + bzr * kc;
-  const int k1 = min(k0 + kc, a.nz - 1);
+__device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, int i, int j, int k0, int k1, int s) {
   if (i >= a.nx - 1 || j >= a.ny - 1) return;
   // the E pass wrote Eo; the H pass reads it
   const T* __restrict__ Ex = a.Eo[0];
@@ -518,6 +504,50 @@ __global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__
       dft_probe(a, step, c, i, j, k, ev, o);
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
+  const StepArgs<T>& a = *ap;
+  const int k0 = 1 + blockIdx.z * kc;
+  e_pass_cells(a, 1 + blockIdx.x * blockDim.x + threadIdx.x, 1 + blockIdx.y * blockDim.y + threadIdx.y, k0,
+               min(k0 + kc, a.nz - 1), s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
+  const StepArgs<T>& a = *ap;
+  // Workgroups run in reverse order of the E pass: the first tiles of this
+  // pass read the E and H the E pass touched last, which are still in the
+  // Infinity Cache; the next E pass then starts where this pass ended.
+  const int bxr = gridDim.x - 1 - blockIdx.x, byr = gridDim.y - 1 - blockIdx.y, bzr = gridDim.z - 1 - blockIdx.z;
+  const int k0 = 1 + bzr * kc;
+  h_pass_cells(a, 1 + bxr * blockDim.x + threadIdx.x, 1 + byr * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
+               s);
+}
+
+// ---------------------------------------------------------------- batched passes
+// Many independent grids in one launch per pass: workgroup b takes tile
+// tiles[b] = (grid index, tile x, tile y, tile z) of a table built on the host,
+// each grid reading its own arguments.  Same cell arithmetic as above.
+template <typename T>
+__global__ __launch_bounds__(256) void e_pass_batch(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
+                                                    int kc, int s) {
+  const int4 t = tiles[blockIdx.x];
+  const StepArgs<T>& a = args[t.x];
+  const int k0 = 1 + t.w * kc;
+  e_pass_cells(a, 1 + t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
+               s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void h_pass_batch(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
+                                                    int kc, int s) {
+  const int4 t = tiles[gridDim.x - 1 - blockIdx.x];
+  const StepArgs<T>& a = args[t.x];
+  const int k0 = 1 + t.w * kc;
+  h_pass_cells(a, 1 + t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
+               s);
 }
 
 // ---------------------------------------------------------------- fused pass

@@ -26,6 +26,7 @@ ABI_SYMBOLS = [
     "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
+    "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
 ]
 
 
@@ -75,6 +76,11 @@ def load_library():
         "fdtd_set_tile": (_i, [_p, _i, _i, _i]),
         "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
         "fdtd_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
+        "fdtd_batch_create": (_i, [ctypes.POINTER(_p), _i, ctypes.POINTER(_p)]),
+        "fdtd_batch_destroy": (_i, [_p]),
+        "fdtd_batch_set_graph": (_i, [_p, _i]),
+        "fdtd_batch_step": (_i, [_p, _i]),
+        "fdtd_batch_synchronize": (_i, [_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -234,3 +240,35 @@ class Grid:
         n = ctypes.c_long()
         _check(load_library().fdtd_kernel_time(self.h, kernel, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+class Batch:
+    """Independent 3-D grids stepped together (one launch per pass for all,
+    hipGraph-captured chunks).  The grids must outlive the batch."""
+
+    def __init__(self, grids, graph=True):
+        L = load_library()
+        self.grids = list(grids)
+        arr = (_p * len(self.grids))(*[g.h for g in self.grids])
+        h = _p()
+        _check(L.fdtd_batch_create(arr, len(self.grids), ctypes.byref(h)))
+        self.h = h
+        if not graph:
+            _check(L.fdtd_batch_set_graph(self.h, 0))
+
+    def step(self, n=1):
+        _check(load_library().fdtd_batch_step(self.h, n))
+
+    def synchronize(self):
+        _check(load_library().fdtd_batch_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().fdtd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -143,6 +143,19 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
 int fdtd_set_profiling(fdtd_grid* g, int on);
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches);
 
+/* Batches: independent 3-D grids of one precision stepped together, one
+ * launch per pass covering the tiles of every grid, each chunk of steps
+ * captured once in a hipGraph and replayed.  The grids stay owned by the
+ * caller and must outlive the batch; while batched they run on the batch's
+ * stream and must not be stepped on their own.  No counterpart in the
+ * reference (its user loop steps one grid, tests/test3D.cpp:146-171). */
+typedef struct fdtd_batch fdtd_batch;
+int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out);
+int fdtd_batch_destroy(fdtd_batch* b);
+int fdtd_batch_set_graph(fdtd_batch* b, int on);
+int fdtd_batch_step(fdtd_batch* b, int nsteps);
+int fdtd_batch_synchronize(fdtd_batch* b);
+
 const char* fdtd_last_error(void);
 
 #ifdef __cplusplus

@@ -137,3 +137,29 @@ def tm2d_c1(nsteps=1000):
         "matid": np.zeros((1, ny, nx), np.uint8), "sources": [(D, 2, 128, 128, 0, gauss_table(nsteps, 40, 12))],
         "probes": [(E, 2, i, j, 0) for i, j in probes], "dfts": [], "tfsf": None,
     }
+
+
+def batch_c4(nsims=64, nsteps=1500, lo=64, hi=160, seed=4):
+    """64 independent 3-D FP32 grids, edges seeded uniform in [64,160] per axis,
+    8-cell CPML, 10 eps blocks U[1,12], own Dz point source, 4-frequency DFT
+    of all components on the mid z-plane."""
+    rng = SplitMix64(seed)
+    out = []
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(3.0))
+    freqs = np.linspace(0.7, 1.3, 4) * C0 / (20 * dx)
+    pe, ph = dft_phases(nsteps, dt, freqs)
+    for q in range(nsims):
+        nx, ny, nz = (rng.randint(lo, hi) for _ in range(3))
+        matid = np.zeros((nz, ny, nx), np.uint8)
+        side_hi = max(3, min(nx, ny, nz) // 4)
+        blocks = seed_blocks(matid, 10, 8 + 2, rng, 2, side_hi)
+        sx, sy, sz = rng.randint(10, nx - 11), rng.randint(10, ny - 11), rng.randint(10, nz - 11)
+        out.append({
+            "name": f"batch{q}_{nx}x{ny}x{nz}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
+            "nsteps": nsteps, "pml": _pml(8, dx), "pml_axes": [True, True, True],
+            "materials": [(1.0, 1.0, [])] + blocks, "matid": matid,
+            "sources": [(D, 2, sx, sy, sz, gauss_table(nsteps, 30 + q % 7, 10))], "probes": [],
+            "dfts": [([1, nx - 1, 1, ny - 1, nz // 2, nz // 2 + 1], 63, pe, ph)], "tfsf": None,
+        })
+    return out

@@ -221,3 +221,33 @@ def test_fused_pass_against_oracle(precision):
         if sc["name"].startswith("grid3d_dispersive"):
             for c in range(3):
                 assert rel_l2(g.get(P, c, 1), o.get(P, c, 1)) <= TOL[precision] * 10
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_standalone(graph):
+    """Grids of uneven sizes stepped as one batch (tile table, hipGraph chunks)
+    give the same fields, DFTs and probes as each grid stepped on its own."""
+    from fdtd_amd import Batch
+    scs = synth.batch_c4(nsims=5, nsteps=300, lo=28, hi=44, seed=11)
+    tf = synth.grid3d_cpml_tfsf_dft(n=(34, 30, 26), nsteps=300)  # one member with TFSF + probes
+    scs.append(tf)
+    alone = []
+    for sc in scs:
+        g = apply(_grid(sc, "f32"), sc)
+        g.step(300)
+        alone.append(g)
+    batched = [apply(_grid(sc, "f32"), sc) for sc in scs]
+    print(f"[gpu-case] batch graph={graph}", flush=True)
+    b = Batch(batched, graph=graph)
+    b.step(7)
+    b.step(256)
+    b.step(37)
+    b.synchronize()
+    b.close()
+    for sc, g, r in zip(scs, batched, alone):
+        assert g.step_count() == 300
+        for nm, f, c in _comps(sc):
+            np.testing.assert_array_equal(g.get(f, c), r.get(f, c), err_msg=f"{sc['name']} {nm}")
+        for idx in range(len(sc["dfts"])):
+            np.testing.assert_array_equal(g.get_dft(idx), r.get_dft(idx))
+    np.testing.assert_array_equal(batched[-1].probes(), alone[-1].probes())
