@@ -217,8 +217,8 @@ struct fdtd_grid {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_pairs;  // (kernel id, index of start event)
   size_t ev_used = 0;
-  double kms[4] = {0, 0, 0, 0};
-  long kcount[4] = {0, 0, 0, 0};
+  double kms[5] = {0, 0, 0, 0, 0};
+  long kcount[5] = {0, 0, 0, 0, 0};
 
   ~fdtd_grid() {
     for (void* p : allocs) (void)hipFree(p);
@@ -546,6 +546,8 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     dd.phE = h.phE;
     dd.phH = h.phH;
     dd.nphase = h.nphase;
+    dd.base = a.dft_work;
+    a.dft_work += long(h.nfreq) * h.ncells;
   }
   a.nprobe = g->nprobe;
   for (int p = 0; p < g->nprobe; p++) a.probe[p] = g->probe[p];
@@ -703,10 +705,11 @@ int step_impl(fdtd_grid* g, int nsteps) {
     const auto* dev1 =
         reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
     HIP_TRY(hipMemcpyAsync((void*)dev0, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
+    const long dft_work = host->dft_work;
     for (int s = 0; s < chunk; s++) {
       int rc;
+      const auto* ap = (fused && (s % 2)) ? dev1 : dev0;
       if (fused) {
-        const auto* ap = (s % 2) ? dev1 : dev0;
         if (t.by == 1)
           rc = launch(g, 3, fdtdk::fused_3d<T, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         else if (t.by == 2)
@@ -721,6 +724,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
         rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
         if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
       }
+      if (!rc && dft_work > 0)
+        rc = launch(g, 4, fdtdk::dft_accumulate<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), ap, s);
       if (rc) return rc;
       g->step++;
     }
@@ -1103,10 +1108,10 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     off += (sizes[q] + 255) & ~size_t(255);
   }
   const bool prof = g->prof;
-  double kms[4];
-  long kcount[4];
-  std::copy(g->kms, g->kms + 4, kms);
-  std::copy(g->kcount, g->kcount + 4, kcount);
+  double kms[5];
+  long kcount[5];
+  std::copy(g->kms, g->kms + 5, kms);
+  std::copy(g->kcount, g->kcount + 5, kcount);
   const Tile saved = g->tile;
   double best = 1e300;
   Tile bt = cand[0];
@@ -1115,12 +1120,14 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     if ((rc = fdtd_step(g, 1))) break;
     if ((rc = fdtd_set_profiling(g, 1))) break;
     if ((rc = fdtd_step(g, trial_steps))) break;
-    double e = 0, h = 0, f = 0;
-    long ne = 0, nh = 0, nf = 0;
-    if ((rc = fdtd_kernel_time(g, 0, &e, &ne)) || (rc = fdtd_kernel_time(g, 1, &h, &nh)) ||
-        (rc = fdtd_kernel_time(g, 3, &f, &nf)))
-      break;
-    double t = e / std::max(ne, 1L) + h / std::max(nh, 1L) + f / std::max(nf, 1L);
+    double t = 0;
+    for (int kid : {0, 1, 3, 4}) {
+      double ms = 0;
+      long nl = 0;
+      if ((rc = fdtd_kernel_time(g, kid, &ms, &nl))) break;
+      t += ms / std::max(nl, 1L);
+    }
+    if (rc) break;
     if (t < best) {
       best = t;
       bt = cand[c];
@@ -1138,8 +1145,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   if ((g->nswap - swaps0) % 2) swap_state(g);
   g->step = step0;
   g->prof = prof;
-  std::copy(kms, kms + 4, g->kms);
-  std::copy(kcount, kcount + 4, g->kcount);
+  std::copy(kms, kms + 5, g->kms);
+  std::copy(kcount, kcount + 5, g->kcount);
   if (rc) {
     g->tile = saved;
     return rc;
@@ -1164,7 +1171,7 @@ int fdtd_set_profiling(fdtd_grid* g, int on) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
   int rc = collect_events(g);
   g->prof = on != 0;
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < 5; k++) {
     g->kms[k] = 0;
     g->kcount[k] = 0;
   }
@@ -1172,7 +1179,7 @@ int fdtd_set_profiling(fdtd_grid* g, int on) {
 }
 
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches) {
-  if (!g || kernel < 0 || kernel > 3) return fail(FDTD_EINVAL, "bad argument");
+  if (!g || kernel < 0 || kernel > 4) return fail(FDTD_EINVAL, "bad argument");
   int rc = collect_events(g);
   if (rc) return rc;
   if (ms) *ms = g->kms[kernel];
@@ -1195,6 +1202,8 @@ struct fdtd_batch {
   hipStream_t stream = nullptr;
   int4* tiles = nullptr;
   int ntiles = 0;
+  long* dft_offs = nullptr;  // n+1 prefix sums of each grid's DFT items
+  long dft_total = 0;
   static constexpr int kRing = 2;
   char* args_dev = nullptr;
   char* args_host = nullptr;
@@ -1210,6 +1219,7 @@ struct fdtd_batch {
     for (auto e : ev)
       if (e) (void)hipEventDestroy(e);
     if (tiles) (void)hipFree(tiles);
+    if (dft_offs) (void)hipFree(dft_offs);
     if (args_dev) (void)hipFree(args_dev);
     if (args_host) (void)hipHostFree(args_host);
     for (size_t q = 0; q < grids.size(); q++) grids[q]->stream = saved_streams[q];
@@ -1261,6 +1271,9 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
         for (int s = 0; s < chunk; s++) {
           hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
           hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+          if (b->dft_total)
+            hipLaunchKernelGGL(fdtdk::dft_accumulate_batch<T>, dim3(unsigned((b->dft_total + 255) / 256)), dim3(256), 0,
+                               b->stream, dev, b->dft_offs, int(n), s);
         }
         HIP_TRY(hipStreamEndCapture(b->stream, &graph));
         hipGraphExec_t exec;
@@ -1274,6 +1287,9 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
       for (int s = 0; s < chunk; s++) {
         hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
         hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+        if (b->dft_total)
+          hipLaunchKernelGGL(fdtdk::dft_accumulate_batch<T>, dim3(unsigned((b->dft_total + 255) / 256)), dim3(256), 0,
+                             b->stream, dev, b->dft_offs, int(n), s);
       }
       HIP_TRY(hipGetLastError());
     }
@@ -1318,6 +1334,16 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
         for (int x = 0; x < tx; x++) tab.push_back(make_int4(q, x, y, z));
   }
   b->ntiles = int(tab.size());
+  std::vector<long> offs(1, 0);
+  for (int q = 0; q < n; q++) {
+    long w = 0;
+    for (int m = 0; m < grids[q]->ndft; m++) w += long(grids[q]->dft[m].nfreq) * grids[q]->dft[m].ncells;
+    offs.push_back(offs.back() + w);
+  }
+  b->dft_total = offs.back();
+  if (!rc && hipMalloc((void**)&b->dft_offs, offs.size() * sizeof(long)) != hipSuccess) rc = fail(FDTD_ENOMEM, "offsets");
+  if (!rc && hipMemcpy(b->dft_offs, offs.data(), offs.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(FDTD_EDEVICE, "offset upload");
   if (!rc && hipMalloc((void**)&b->tiles, tab.size() * sizeof(int4)) != hipSuccess) rc = fail(FDTD_ENOMEM, "tile table");
   if (!rc && hipMemcpy(b->tiles, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(FDTD_EDEVICE, "tile table upload");

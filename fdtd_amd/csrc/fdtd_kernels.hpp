@@ -57,6 +57,7 @@ struct DftDev {
   const double* phE;  // [nphase][nfreq][2]
   const double* phH;
   int nphase;
+  long base;  // offset of this monitor's (frequency, cell) items in the DFT launch
 };
 
 struct ProbeDev {
@@ -105,6 +106,7 @@ struct StepArgs {
   const double* incH;  // aux Hy before it
   // running DFT
   int ndft;
+  long dft_work;  // sum over monitors of nfreq x cells
   DftDev<T> dft[kMaxDft];
   // probes
   int nprobe;
@@ -396,37 +398,78 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const
   out[2] = hz + d[2] * im;
 }
 
-// running DFT and probes of one cell after its step
+// probes of one cell after its step
 template <typename T>
 __device__ FDTD_COLD void dft_probe(const StepArgs<T>& a, long step, long c, int i, int j, int k, const T e[3],
-                                          const T h[3]) {
-  for (int q = 0; q < a.ndft; q++) {
-    const DftDev<T>& d = a.dft[q];
-    if (i < d.r[0] || i >= d.r[1] || j < d.r[2] || j >= d.r[3] || k < d.r[4] || k >= d.r[5]) continue;
-    if (step >= d.nphase) continue;
-    const long lq =
-        (long)(i - d.r[0]) + (long)(d.r[1] - d.r[0]) * ((long)(j - d.r[2]) + (long)(d.r[3] - d.r[2]) * (k - d.r[4]));
-    int slot = 0;
-#pragma unroll
-    for (int comp = 0; comp < 6; comp++) {
-      if (!((d.mask >> comp) & 1)) continue;
-      const T v = comp < 3 ? e[comp] : h[comp - 3];
-      const double* ph = (comp < 3 ? d.phE : d.phH) + 2L * d.nfreq * step;
-      T* acc = d.acc + (long)slot * d.nfreq * 2 * d.ncells + lq;
-      for (int f = 0; f < d.nfreq; f++) {
-        T* re = acc + (long)f * 2 * d.ncells;
-        re[0] += v * (T)ph[2 * f];
-        re[d.ncells] += v * (T)ph[2 * f + 1];
-      }
-      slot++;
-    }
-  }
+                                    const T h[3]) {
   for (int p = 0; p < a.nprobe; p++)
     if (a.probe[p].cell == c) {
       const int comp = a.probe[p].comp;
       const T v = a.probe[p].field == 1 ? pick3(e[0], e[1], e[2], comp) : pick3(h[0], h[1], h[2], comp);
       a.trace[step * a.nprobe + p] = (double)v;
     }
+}
+
+// ---------------------------------------------------------------- running DFT
+// F_c(w) += f_c(t_n) (cos, -sin)(w t_n) dt, E at t = (n+1) dt and H at
+// (n+1.5) dt (the reference's DataCapture, include/DataCapture.hpp:61-191, is
+// empty; this is the build's definition).  One thread per (monitor,
+// frequency, cell) after the step, so surface work spreads over the whole GPU
+// instead of serialising in the few threads that own monitor cells; it reads
+// the new E and H of the monitor cells only.
+template <typename T>
+__device__ __forceinline__ void dft_item(const StepArgs<T>& a, long g, long step) {
+  int q = 0;
+  while (q + 1 < a.ndft && g >= a.dft[q + 1].base) q++;
+  const DftDev<T>& d = a.dft[q];
+  if (step >= d.nphase) return;
+  const long r = g - d.base;
+  const int f = (int)(r / d.ncells);
+  const long lq = r - (long)f * d.ncells;
+  const int wx = d.r[1] - d.r[0], wy = d.r[3] - d.r[2];
+  const int i = d.r[0] + (int)(lq % wx);
+  const int j = d.r[2] + (int)((lq / wx) % wy);
+  const int k = d.r[4] + (int)(lq / ((long)wx * wy));
+  const long c = (long)i + a.sy * (long)j + a.sz * (long)k;
+  const double* pe = d.phE + 2L * (d.nfreq * step + f);
+  const double* ph = d.phH + 2L * (d.nfreq * step + f);
+  const T ce = (T)pe[0], se = (T)pe[1], chh = (T)ph[0], sh = (T)ph[1];
+  int slot = 0;
+#pragma unroll
+  for (int comp = 0; comp < 6; comp++) {
+    if (!((d.mask >> comp) & 1)) continue;
+    const T* src = comp < 3 ? a.Eo[comp] : a.Ho[comp - 3];
+    const T v = src ? src[c] : T(0);
+    T* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
+    re[0] += v * (comp < 3 ? ce : chh);
+    re[d.ncells] += v * (comp < 3 ? se : sh);
+    slot++;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void dft_accumulate(const StepArgs<T>* __restrict__ ap, int s) {
+  const StepArgs<T>& a = *ap;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < a.dft_work) dft_item(a, g, a.step0 + s);
+}
+
+// batched grids: `offs` holds each grid's first item (prefix sums, n+1 entries)
+template <typename T>
+__global__ __launch_bounds__(256) void dft_accumulate_batch(const StepArgs<T>* __restrict__ args,
+                                                            const long* __restrict__ offs, int n, int s) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= offs[n]) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (offs[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const StepArgs<T>& a = args[lo];
+  dft_item(a, g - offs[lo], a.step0 + s);
 }
 
 // ---------------------------------------------------------------- two passes
@@ -479,7 +522,7 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, int i, int j,
   const long sy = a.sy, sz = a.sz;
   const long step = a.step0 + s;
   const double* incE = a.incE + (long)s * a.auxL;
-  const bool extra = a.ndft || a.nprobe;
+  const bool extra = a.nprobe > 0;
   for (int k = k0; k < k1; k++) {
     const long c = (long)i + sy * (long)j + sz * (long)k;
     EStencil<T> e;
@@ -666,7 +709,7 @@ __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __
   const bool ecell = tx >= 1 && i <= nx - 2 && j <= ny - 2;
   const bool own = ecell && tx <= kFusedCols && ty < TY;
   const bool lead = ty == 0 && i < nx;
-  const bool extra = a.ndft || a.nprobe;
+  const bool extra = a.nprobe > 0;
 
   T hk[3], ek[3], en[3];
   int midk = 0;
@@ -816,7 +859,7 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   T h[3] = {hx + d[0] * im, hy + d[1] * im, T(0)};
   a.Ho[0][c] = h[0];
   a.Ho[1][c] = h[1];
-  if (a.ndft || a.nprobe) {
+  if (a.nprobe) {
     const T e[3] = {T(0), T(0), ez};
     dft_probe(a, step, c, i, j, 0, e, h);
   }
