@@ -189,19 +189,124 @@ def run_c4(args):
                       "sequential_gcell_per_s": round(cells * nsteps / t_seq / 1e9, 3)})
 
 
+def run_c5(args):
+    """Adjoint eps gradient, 3-D 128^3 FP32: 10-cell CPML, TFSF continuous wave at one frequency, 64x64x16
+    design box (eps_r U[1,12]), objective sum |F[Ez]|^2 on a 64x64 plane, checkpoint every 100 steps.
+    value = gradient time / plain forward time (both from step 0 over the same steps)."""
+    import numpy as np
+    import torch
+    from fdtd_amd import Grid
+    import synth
+    from scenarios import apply
+    n = args.size if args.size != 256 else 128
+    nsteps = args.steps
+    sc = synth.grid3d_adjoint(n=n, nsteps=nsteps)
+    box, eps = sc["design"]
+    mk = lambda prec="f32": apply(Grid("ThreeD", n, n, n, sc["dx"], sc["dt"], precision=prec), sc)
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    # plain forward at its autotuned tile
+    g = mk()
+    tile = g.autotune(trial_steps=10)
+    g.step(args.warmup)
+    g.synchronize()
+    g.close()
+    g = mk()
+    g.set_tile(*tile)
+    t_fwd, _ = wall(lambda: (g.step(nsteps), g.synchronize()))
+    g.close()
+    two_pass = tile if tile[0] > 0 else (256, 1, 1)
+    res = {}
+    for name, K, mb in (("checkpointed", 100, 0.0), ("history_resident", 100, 1e5)):
+        # timed run (no per-kernel events), then a profiled run for the kernel averages
+        g = mk()
+        g.set_tile(*two_pass)
+        t, (J, grad) = wall(lambda: g.adjoint_eps_gradient(nsteps, 0, 4, checkpoint_every=K, history_mb=mb))
+        g.close()
+        g = mk()
+        g.set_tile(*two_pass)
+        g.set_profiling(True)
+        g.adjoint_eps_gradient(nsteps, 0, 4, checkpoint_every=K, history_mb=mb)
+        avg = {}
+        for kid, kn in ((0, "e_pass"), (1, "h_pass"), (2, "aux_line"), (4, "dft"), (5, "adj_e_pass"),
+                        (6, "adj_h_pass")):
+            ms, cnt = g.kernel_time(kid)
+            if cnt:
+                avg[kn] = round(ms / cnt, 5)
+        g.close()
+        res[name] = {"seconds": round(t, 4), "ratio_to_forward": round(t / t_fwd, 3), "checkpoint_every": K,
+                     "history_mb": mb, "kernel_avg_ms": avg, "objective": J}
+    # central finite differences (h = 1e-3) of FP64 forward runs on 16 random design cells
+    rng = np.random.default_rng(16)
+    cells = rng.choice(eps.size, 16, replace=False)
+
+    def objective(e):
+        q = apply(Grid("ThreeD", n, n, n, sc["dx"], sc["dt"], precision="f64"), dict(sc, design=(box, e)))
+        q.set_tile(*two_pass)
+        q.step(nsteps)
+        F = q.get_dft(0)
+        q.close()
+        return float(np.sum(F * F))
+
+    fd = []
+    for c in cells:
+        ep = eps.reshape(-1).copy()
+        em = eps.reshape(-1).copy()
+        ep[c] += 1e-3
+        em[c] -= 1e-3
+        fd.append((objective(ep.reshape(eps.shape)) - objective(em.reshape(eps.shape))) / 2e-3)
+    fd = np.array(fd)
+    got = grad.reshape(-1)[cells]
+    fd_err = float(np.max(np.abs(got - fd)) / np.max(np.abs(fd)))
+    # roofline of the adjoint pair: the forward's per-step bytes (fields, material, psi) plus, per design
+    # cell, the recorded E increment (3 reads) and the gradient read-modify-write (FP64)
+    esz = 4
+    abytes, interior = algorithmic_bytes(dict(sc, dfts=[]), esz)
+    ndes = int(eps.size)
+    abytes += ndes * (3 * esz + 16)
+    adj = res["checkpointed"]["kernel_avg_ms"]
+    adj_ms = adj.get("adj_e_pass", 0) + adj.get("adj_h_pass", 0)
+    achieved = abytes / (adj_ms / 1e3) / 1e9 if adj_ms else None
+    out = base_line(args, res["checkpointed"]["ratio_to_forward"], "x forward time", round(t_fwd / nsteps * 1e3, 5),
+                    "f32",
+                    {"workload": f"adjoint eps gradient, 3-D {n}^3 FP32, 10-cell CPML, TFSF continuous wave at one "
+                                 f"frequency, design box {list(eps.shape[::-1])} eps_r U[1,12], objective sum |F[Ez]|^2 "
+                                 f"on a {sc['dfts'][0][0][3] - sc['dfts'][0][0][2]}^2 plane, {nsteps} steps",
+                     "forward_seconds": round(t_fwd, 4), "forward_tile": list(tile),
+                     "forward_gcell_per_s": round(interior * nsteps / t_fwd / 1e9, 3),
+                     "gradient": res, "fd_check": {"cells": 16, "h": 1e-3, "reference": "FP64 GPU forward runs",
+                                                   "max_abs_err_over_max_abs_fd": fd_err},
+                     "batch_of_designs": "not implemented: designs run one grid per call"},
+                    higher=False)
+    if achieved:
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": "adjoint step: adj_e_pass + adj_h_pass", "algorithmic_bytes_per_step": abytes}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 2000; c4 1500; c5 3000, the workload's own step count)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c2 (default): the headline 3-D 256^3 forward; c1: 2-D TM 256^2; c3: dispersive sphere; "
-                         "c4: 64 batched grids vs sequential")
+                         "c4: 64 batched grids vs sequential; c5: adjoint eps gradient 128^3")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"c4": 1500, "c5": 3000}.get(args.workload, 2000)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (z-slab sharding is a design "
               "note, not code); rerun with --gpus 1", file=sys.stderr)
@@ -214,7 +319,7 @@ def main():
 
     torch.cuda.set_device(0)
     if args.workload != "c2":
-        fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4}[args.workload]
+        fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     total = args.warmup + args.steps

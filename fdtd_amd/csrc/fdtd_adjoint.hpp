@@ -1,0 +1,273 @@
+// Discrete adjoint (reverse-time) passes of the 3-D two-pass step, for the
+// gradient of a running-DFT objective with respect to per-cell eps_r.
+//
+// The forward step n maps (E, H, psiE, psiH) to the next state through
+//   psiE' = B psiE + C/dx sigma diff^-(H)         E' = E + ie (dt F/dx curl^-(H) + dt G psiE + s_n)
+//   psiH' = B psiH + C/dx sigma diff^+(E')        H' = H - im (dt F/dx curl^+(E') + dt G psiH) + ...
+// (e_cell_3d / h_cell_3d); sources and TFSF terms do not depend on the state.
+// Its transpose, applied from the last step down to the first, is again a
+// Yee-like pair of gather passes with the difference directions swapped:
+//   adjoint E pass: nu += ie (objective source + curl^-T of w), w = dt F/dx mu - C/dx psiH-adjoint
+//   adjoint H pass: mu += im curl^+T of v,                      v = dt F/dx nu + C/dx psiE-adjoint
+// with the stored adjoint variables scaled as nu = ie * adj(E), mu = im * adj(H)
+// so that no pass needs a neighbour's material.  The psiH adjoint is updated
+// (own cell) in the adjoint H pass, the psiE adjoint (own cell) in the next
+// adjoint E pass, so that no pass writes what another thread of it reads.
+// d J / d eps_r = -(1/eps_r) sum_n adj(E^{n+1}) . (E^{n+1} - E^n) = -eps0 sum_n nu . dE.
+// The reference has no adjoint; this is the build's own.
+#pragma once
+#include "fdtd_kernels.hpp"
+
+namespace fdtdk {
+
+template <typename T>
+struct AdjArgs {
+  T* nu[3];          // ie * adjoint of E
+  T* mu[3];          // im * adjoint of H
+  T* lpsi[2][3][3];  // adjoint of psi[ft][I][J] (the forward slab layout)
+  double* grad;      // [ndes]: d J / d eps_r of the design cells
+  double eps0;
+  // objective J = sum over the monitor's cells, slots and frequencies of |F|^2
+  int obj_r[6];
+  int obj_comp[3];  // E component of objective slot s (-1: unused)
+  int obj_nfreq;
+  long obj_ncells;
+  const T* obj_w;        // [slot][nfreq][2][ncells]: 2 Re F, 2 Im F
+  const double* obj_ph;  // E phases of the monitor, [nphase][nfreq][2]
+  int obj_nphase;
+};
+
+// objective source of E at (i,j,k) for the step's E sample
+template <typename T>
+__device__ FDTD_COLD void adj_objective(const AdjArgs<T>& q, long step, int i, int j, int k, T s[3]) {
+  const int di = i - q.obj_r[0], dj = j - q.obj_r[2], dk = k - q.obj_r[4];
+  const int wx = q.obj_r[1] - q.obj_r[0], wy = q.obj_r[3] - q.obj_r[2], wz = q.obj_r[5] - q.obj_r[4];
+  if ((unsigned)di >= (unsigned)wx || (unsigned)dj >= (unsigned)wy || (unsigned)dk >= (unsigned)wz) return;
+  if (step >= q.obj_nphase) return;
+  const long lq = (long)di + (long)wx * ((long)dj + (long)wy * dk);
+  for (int slot = 0; slot < 3; slot++) {
+    const int comp = q.obj_comp[slot];
+    if (comp < 0) continue;
+    T acc = T(0);
+    for (int f = 0; f < q.obj_nfreq; f++) {
+      const double* ph = q.obj_ph + 2L * (q.obj_nfreq * step + f);
+      const T* w = q.obj_w + ((long)slot * q.obj_nfreq + f) * 2 * q.obj_ncells + lq;
+      acc += w[0] * (T)ph[0] + w[q.obj_ncells] * (T)ph[1];
+    }
+    if (comp == 0)
+      s[0] += acc;
+    else if (comp == 1)
+      s[1] += acc;
+    else
+      s[2] += acc;
+  }
+}
+
+// Adjoint E pass of step `step` over one column of cells.  hist: the step's
+// row of design-cell E increments (null: no design box).
+template <typename T>
+__device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<T>& q, int i, int j, int k0, int k1,
+                                            long step, const T* __restrict__ hist) {
+  if (i >= a.nx - 1 || j >= a.ny - 1) return;
+  const T* __restrict__ M0 = q.mu[0];
+  const T* __restrict__ M1 = q.mu[1];
+  const T* __restrict__ M2 = q.mu[2];
+  T* __restrict__ N0 = q.nu[0];
+  T* __restrict__ N1 = q.nu[1];
+  T* __restrict__ N2 = q.nu[2];
+  const long sy = a.sy, sz = a.sz;
+  const T fx = a.cf[1][0][i], fy = a.cf[1][1][j];
+  const T fxm = a.cf[1][0][i - 1], fym = a.cf[1][1][j - 1];
+  for (int k = k0; k < k1; k++) {
+    const long c = (long)i + sy * (long)j + sz * (long)k;
+    const T m0 = M0[c], m1 = M1[c], m2 = M2[c];
+    const T m1_zm = M1[c - sz], m2_ym = M2[c - sy], m0_zm = M0[c - sz];
+    const T m2_xm = M2[c - 1], m0_ym = M0[c - sy], m1_xm = M1[c - 1];
+    T n0 = N0[c], n1 = N1[c], n2 = N2[c];
+    const int mid = a.mat[c];
+    const T ie = cell_inv_eps(a, mid, i, j, k);
+    const T fz = a.cf[1][2][k], fzm = a.cf[1][2][k - 1];
+    // w_KJ at the cell and at its lower J neighbour
+    T w12 = fz * m1, w12m = fzm * m1_zm;
+    T w21 = fy * m2, w21m = fym * m2_ym;
+    T w02 = fz * m0, w02m = fzm * m0_zm;
+    T w20 = fx * m2, w20m = fxm * m2_xm;
+    T w01 = fy * m0, w01m = fym * m0_ym;
+    T w10 = fx * m1, w10m = fxm * m1_xm;
+    // x slabs: H psi (K=1,2; J=0) and the E psi adjoint (I=1,2; J=0) of the later step
+    int s = slab_index(i, a.slab_a[0], a.slab_b[0]);
+    if (s >= 0) {
+      const long o = slab_off(a, 0, s, i, j, k);
+      const T cc = a.cc[1][0][i];
+      w10 -= cc * q.lpsi[1][1][0][o];
+      w20 -= cc * q.lpsi[1][2][0][o];
+      const T b = a.cb[0][0][i], g = a.cg[0][0][i];
+      q.lpsi[0][1][0][o] = b * q.lpsi[0][1][0][o] + g * n1;
+      q.lpsi[0][2][0][o] = b * q.lpsi[0][2][0][o] + g * n2;
+    }
+    s = slab_index(i - 1, a.slab_a[0], a.slab_b[0]);
+    if (s >= 0) {
+      const long o = slab_off(a, 0, s, i - 1, j, k);
+      const T cc = a.cc[1][0][i - 1];
+      w10m -= cc * q.lpsi[1][1][0][o];
+      w20m -= cc * q.lpsi[1][2][0][o];
+    }
+    s = slab_index(j, a.slab_a[1], a.slab_b[1]);
+    if (s >= 0) {
+      const long o = slab_off(a, 1, s, i, j, k);
+      const T cc = a.cc[1][1][j];
+      w01 -= cc * q.lpsi[1][0][1][o];
+      w21 -= cc * q.lpsi[1][2][1][o];
+      const T b = a.cb[0][1][j], g = a.cg[0][1][j];
+      q.lpsi[0][0][1][o] = b * q.lpsi[0][0][1][o] + g * n0;
+      q.lpsi[0][2][1][o] = b * q.lpsi[0][2][1][o] + g * n2;
+    }
+    s = slab_index(j - 1, a.slab_a[1], a.slab_b[1]);
+    if (s >= 0) {
+      const long o = slab_off(a, 1, s, i, j - 1, k);
+      const T cc = a.cc[1][1][j - 1];
+      w01m -= cc * q.lpsi[1][0][1][o];
+      w21m -= cc * q.lpsi[1][2][1][o];
+    }
+    s = slab_index(k, a.slab_a[2], a.slab_b[2]);
+    if (s >= 0) {
+      const long o = slab_off(a, 2, s, i, j, k);
+      const T cc = a.cc[1][2][k];
+      w02 -= cc * q.lpsi[1][0][2][o];
+      w12 -= cc * q.lpsi[1][1][2][o];
+      const T b = a.cb[0][2][k], g = a.cg[0][2][k];
+      q.lpsi[0][0][2][o] = b * q.lpsi[0][0][2][o] + g * n0;
+      q.lpsi[0][1][2][o] = b * q.lpsi[0][1][2][o] + g * n1;
+    }
+    s = slab_index(k - 1, a.slab_a[2], a.slab_b[2]);
+    if (s >= 0) {
+      const long o = slab_off(a, 2, s, i, j, k - 1);
+      const T cc = a.cc[1][2][k - 1];
+      w02m -= cc * q.lpsi[1][0][2][o];
+      w12m -= cc * q.lpsi[1][1][2][o];
+    }
+    T src[3] = {T(0), T(0), T(0)};
+    if (q.obj_w) adj_objective(q, step, i, j, k, src);
+    n0 += ie * (src[0] + (w12 - w12m) - (w21 - w21m));
+    n1 += ie * (src[1] + (w20 - w20m) - (w02 - w02m));
+    n2 += ie * (src[2] + (w01 - w01m) - (w10 - w10m));
+    N0[c] = n0;
+    N1[c] = n1;
+    N2[c] = n2;
+    if (hist) {
+      const long l = design_index(a, i, j, k);
+      if (l >= 0) {
+        const long nd = (long)(a.dsg[1] - a.dsg[0]) * (a.dsg[3] - a.dsg[2]) * (a.dsg[5] - a.dsg[4]);
+        const T dot = n0 * hist[l] + n1 * hist[nd + l] + n2 * hist[2 * nd + l];
+        q.grad[l] -= q.eps0 * (double)dot;
+      }
+    }
+  }
+}
+
+// Adjoint H pass of one step over one column of cells.
+template <typename T>
+__device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<T>& q, int i, int j, int k0, int k1) {
+  if (i >= a.nx - 1 || j >= a.ny - 1) return;
+  const T* __restrict__ N0 = q.nu[0];
+  const T* __restrict__ N1 = q.nu[1];
+  const T* __restrict__ N2 = q.nu[2];
+  T* __restrict__ M0 = q.mu[0];
+  T* __restrict__ M1 = q.mu[1];
+  T* __restrict__ M2 = q.mu[2];
+  const long sy = a.sy, sz = a.sz;
+  const T fx = a.cf[0][0][i], fy = a.cf[0][1][j];
+  const T fxp = a.cf[0][0][i + 1], fyp = a.cf[0][1][j + 1];
+  for (int k = k0; k < k1; k++) {
+    const long c = (long)i + sy * (long)j + sz * (long)k;
+    const T n0 = N0[c], n1 = N1[c], n2 = N2[c];
+    const T n1_zp = N1[c + sz], n2_yp = N2[c + sy], n0_zp = N0[c + sz];
+    const T n2_xp = N2[c + 1], n0_yp = N0[c + sy], n1_xp = N1[c + 1];
+    const T h0 = M0[c], h1 = M1[c], h2 = M2[c];
+    const T im = a.mim[a.mat[c]];
+    const T fz = a.cf[0][2][k], fzp = a.cf[0][2][k + 1];
+    // v_IJ at the cell and at its upper J neighbour
+    T v12 = fz * n1, v12p = fzp * n1_zp;
+    T v21 = fy * n2, v21p = fyp * n2_yp;
+    T v02 = fz * n0, v02p = fzp * n0_zp;
+    T v20 = fx * n2, v20p = fxp * n2_xp;
+    T v01 = fy * n0, v01p = fyp * n0_yp;
+    T v10 = fx * n1, v10p = fxp * n1_xp;
+    // x slabs: E psi adjoint (I=1,2; J=0) and the H psi adjoint update (K=1,2; J=0)
+    int s = slab_index(i, a.slab_a[0], a.slab_b[0]);
+    if (s >= 0) {
+      const long o = slab_off(a, 0, s, i, j, k);
+      const T cc = a.cc[0][0][i];
+      v10 += cc * q.lpsi[0][1][0][o];
+      v20 += cc * q.lpsi[0][2][0][o];
+      const T b = a.cb[1][0][i], g = a.cg[1][0][i];
+      q.lpsi[1][1][0][o] = b * q.lpsi[1][1][0][o] - g * h1;
+      q.lpsi[1][2][0][o] = b * q.lpsi[1][2][0][o] - g * h2;
+    }
+    s = slab_index(i + 1, a.slab_a[0], a.slab_b[0]);
+    if (s >= 0) {
+      const long o = slab_off(a, 0, s, i + 1, j, k);
+      const T cc = a.cc[0][0][i + 1];
+      v10p += cc * q.lpsi[0][1][0][o];
+      v20p += cc * q.lpsi[0][2][0][o];
+    }
+    s = slab_index(j, a.slab_a[1], a.slab_b[1]);
+    if (s >= 0) {
+      const long o = slab_off(a, 1, s, i, j, k);
+      const T cc = a.cc[0][1][j];
+      v01 += cc * q.lpsi[0][0][1][o];
+      v21 += cc * q.lpsi[0][2][1][o];
+      const T b = a.cb[1][1][j], g = a.cg[1][1][j];
+      q.lpsi[1][0][1][o] = b * q.lpsi[1][0][1][o] - g * h0;
+      q.lpsi[1][2][1][o] = b * q.lpsi[1][2][1][o] - g * h2;
+    }
+    s = slab_index(j + 1, a.slab_a[1], a.slab_b[1]);
+    if (s >= 0) {
+      const long o = slab_off(a, 1, s, i, j + 1, k);
+      const T cc = a.cc[0][1][j + 1];
+      v01p += cc * q.lpsi[0][0][1][o];
+      v21p += cc * q.lpsi[0][2][1][o];
+    }
+    s = slab_index(k, a.slab_a[2], a.slab_b[2]);
+    if (s >= 0) {
+      const long o = slab_off(a, 2, s, i, j, k);
+      const T cc = a.cc[0][2][k];
+      v02 += cc * q.lpsi[0][0][2][o];
+      v12 += cc * q.lpsi[0][1][2][o];
+      const T b = a.cb[1][2][k], g = a.cg[1][2][k];
+      q.lpsi[1][0][2][o] = b * q.lpsi[1][0][2][o] - g * h0;
+      q.lpsi[1][1][2][o] = b * q.lpsi[1][1][2][o] - g * h1;
+    }
+    s = slab_index(k + 1, a.slab_a[2], a.slab_b[2]);
+    if (s >= 0) {
+      const long o = slab_off(a, 2, s, i, j, k + 1);
+      const T cc = a.cc[0][2][k + 1];
+      v02p += cc * q.lpsi[0][0][2][o];
+      v12p += cc * q.lpsi[0][1][2][o];
+    }
+    M0[c] = h0 + im * ((v12 - v12p) - (v21 - v21p));
+    M1[c] = h1 + im * ((v20 - v20p) - (v02 - v02p));
+    M2[c] = h2 + im * ((v01 - v01p) - (v10 - v10p));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void adj_e_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
+                                                     int kc, long step, const T* __restrict__ hist) {
+  const int k0 = 1 + blockIdx.z * kc;
+  adj_e_cells(*ap, *qp, 1 + blockIdx.x * blockDim.x + threadIdx.x, 1 + blockIdx.y * blockDim.y + threadIdx.y, k0,
+              min(k0 + kc, ap->nz - 1), step, hist);
+}
+
+// reverse workgroup order, as the forward H pass (the first tiles read what
+// the adjoint E pass touched last)
+template <typename T>
+__global__ __launch_bounds__(256) void adj_h_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
+                                                     int kc) {
+  const int bxr = gridDim.x - 1 - blockIdx.x, byr = gridDim.y - 1 - blockIdx.y, bzr = gridDim.z - 1 - blockIdx.z;
+  const int k0 = 1 + bzr * kc;
+  adj_h_cells(*ap, *qp, 1 + bxr * blockDim.x + threadIdx.x, 1 + byr * blockDim.y + threadIdx.y, k0,
+              min(k0 + kc, ap->nz - 1));
+}
+
+}  // namespace fdtdk

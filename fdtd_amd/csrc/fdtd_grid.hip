@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/fdtd_hip.h"
+#include "fdtd_adjoint.hpp"
 #include "fdtd_kernels.hpp"
 
 namespace {
@@ -105,6 +106,9 @@ void pml_idiom_rows(int L, int n, double m, double ma, double smax, double kmax,
 }
 
 constexpr int kChunk = 256;  // steps per incident-table chunk
+// timed kernels: 0 E pass, 1 H pass, 2 aux line, 3 fused pass, 4 DFT,
+// 5 adjoint E pass, 6 adjoint H pass
+constexpr int kNumKernels = 7;
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
 // bx == 0 selects the fused single-pass schedule with `by` rows per workgroup
@@ -194,6 +198,21 @@ struct fdtd_grid {
   double* trace = nullptr;
   long trace_cap = 0;
 
+  // design box: per-cell eps_r replacing the material table's value there
+  bool design = false;
+  int dsg[6] = {0, 0, 0, 0, 0, 0};
+  long ndes = 0;
+  std::vector<double> dsg_eps;
+  void* dsg_ie_dev = nullptr;  // T[ndes] = 1/(eps_r eps0)
+
+  // forward-step controls used by the adjoint
+  bool force_two_pass = false;  // the design increments are recorded by the two-pass E pass
+  bool recompute = false;       // segment recompute: no DFT accumulation, no probes
+  void* hist = nullptr;         // T[hist_rows][3][ndes]
+  long hist_step0 = 0, hist_rows = 0;
+  double *inc_allE = nullptr, *inc_allH = nullptr;  // incident rows of steps [0, inc_rows)
+  long inc_rows = 0;
+
   Tile tile = {0, 0, 0};
   bool finalized = false;
   bool coef_dirty = true;
@@ -217,8 +236,8 @@ struct fdtd_grid {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, int>> ev_pairs;  // (kernel id, index of start event)
   size_t ev_used = 0;
-  double kms[5] = {0, 0, 0, 0, 0};
-  long kcount[5] = {0, 0, 0, 0, 0};
+  double kms[kNumKernels] = {};
+  long kcount[kNumKernels] = {};
 
   ~fdtd_grid() {
     for (void* p : allocs) (void)hipFree(p);
@@ -552,6 +571,16 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   a.nprobe = g->nprobe;
   for (int p = 0; p < g->nprobe; p++) a.probe[p] = g->probe[p];
   a.trace = g->trace;
+  a.dsg_ie = (const T*)g->dsg_ie_dev;
+  for (int x = 0; x < 6; x++) a.dsg[x] = g->dsg[x];
+  a.hist = (T*)g->hist;
+  a.hist_step0 = g->hist_step0;
+  a.hist_rows = g->hist_rows;
+  if (g->recompute) {
+    a.ndft = 0;
+    a.dft_work = 0;
+    a.nprobe = 0;
+  }
 }
 
 hipEvent_t next_event(fdtd_grid* g) {
@@ -658,6 +687,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
   fill_args<T>(g, a);
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (g->ndim == 2) t.kc = 1;
+  if (g->force_two_pass && t.bx == 0) t = kDefaultTile3D;
   const bool fused = g->ndim == 3 && t.bx == 0;
   if (fused) {
     int rc = ensure_pingpong(g);
@@ -679,13 +709,18 @@ int step_impl(fdtd_grid* g, int nsteps) {
   int done = 0;
   while (done < nsteps) {
     const int chunk = std::min(kChunk, nsteps - done);
-    if (g->tfsf) {
-      int rc = run_aux(g, g->step, chunk);
-      if (rc) return rc;
-    }
     a.step0 = g->step;
     a.incE = g->incE;
     a.incH = g->incH;
+    if (g->tfsf && g->inc_rows) {
+      // incident rows computed ahead for every step (adjoint forward and recompute)
+      if (g->step + chunk > g->inc_rows) return fail(FDTD_ESTATE, "step beyond the precomputed incident rows");
+      a.incE = g->inc_allE + g->step * g->auxL;
+      a.incH = g->inc_allH + g->step * g->auxL;
+    } else if (g->tfsf) {
+      int rc = run_aux(g, g->step, chunk);
+      if (rc) return rc;
+    }
     // stage this chunk's arguments: variant 0 reads the current state,
     // variant 1 (fused pass only) the second copy
     const int slot = g->args_slot;
@@ -741,6 +776,223 @@ bool valid_comp(const fdtd_grid* g, int field, int comp) {
   if (field == FDTD_E || field == FDTD_D || field == FDTD_P || field == FDTD_J) return g->actE[comp];
   if (field == FDTD_H || field == FDTD_B) return g->actH[comp];
   return false;
+}
+
+// eps_r of a cell: the design value inside the design box, else the material's
+double cell_eps(const fdtd_grid* g, int i, int j, int k) {
+  if (g->design && i >= g->dsg[0] && i < g->dsg[1] && j >= g->dsg[2] && j < g->dsg[3] && k >= g->dsg[4] &&
+      k < g->dsg[5]) {
+    const long wx = g->dsg[1] - g->dsg[0], wy = g->dsg[3] - g->dsg[2];
+    return g->dsg_eps[(i - g->dsg[0]) + wx * ((j - g->dsg[2]) + wy * (k - g->dsg[4]))];
+  }
+  return g->mats[g->mat_host[g->q(i, j, k)]].eps;
+}
+
+// ---------------------------------------------------------------- adjoint
+// The stepped state of a non-dispersive grid (E, H, CPML psi), in a fixed order.
+void state_arrays(fdtd_grid* g, std::vector<std::pair<void*, size_t>>& out) {
+  out.clear();
+  auto add = [&](void* p) {
+    if (!p) return;
+    auto it = std::find(g->allocs.begin(), g->allocs.end(), p);
+    if (it != g->allocs.end()) out.push_back({p, g->alloc_bytes[it - g->allocs.begin()]});
+  };
+  for (int c = 0; c < 3; c++) {
+    add(g->E[c]);
+    add(g->H[c]);
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < 3; J++) add(g->psi[ft][I][J]);
+}
+
+// Temporaries and forward-step controls of one adjoint call, undone on every exit.
+struct AdjointScope {
+  fdtd_grid* g;
+  std::vector<void*> ptrs;
+  explicit AdjointScope(fdtd_grid* gg) : g(gg) {}
+  ~AdjointScope() {
+    (void)hipStreamSynchronize(g->stream);
+    for (void* p : ptrs) g->release(p);
+    g->force_two_pass = false;
+    g->recompute = false;
+    g->hist = nullptr;
+    g->hist_rows = 0;
+    g->inc_allE = g->inc_allH = nullptr;
+    g->inc_rows = 0;
+  }
+  int get(void** p, size_t bytes) {
+    int rc = g->alloc(p, bytes);
+    if (!rc) ptrs.push_back(*p);
+    return rc;
+  }
+};
+
+// Gradient of J = sum |F|^2 (monitor `didx`, E components in `mask`, all its
+// frequencies) over steps [0, nsteps) with respect to the design cells' eps_r.
+// Segments of K steps: the design increments of the last R segments (as many
+// as `history_mb` holds, at least one) are recorded during the forward pass;
+// every earlier segment is checkpointed (E, H, psi) and recomputed from its
+// checkpoint, into the second state copy, just before its adjoint steps.
+template <typename T>
+int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, int mask, double* objective,
+                 double* grad) {
+  AdjointScope sc(g);
+  int rc = 0;
+  g->force_two_pass = true;
+  if (g->tfsf) {
+    if ((rc = sc.get((void**)&g->inc_allE, size_t(nsteps) * g->auxL * 8))) return rc;
+    if ((rc = sc.get((void**)&g->inc_allH, size_t(nsteps) * g->auxL * 8))) return rc;
+    double *e0 = g->incE, *h0 = g->incH;
+    g->incE = g->inc_allE;
+    g->incH = g->inc_allH;
+    rc = run_aux(g, 0, nsteps);
+    g->incE = e0;
+    g->incH = h0;
+    if (rc) return rc;
+    g->inc_rows = nsteps;
+  }
+  const int S = (nsteps + K - 1) / K;
+  const long nd = g->ndes;
+  const size_t row_bytes = size_t(3) * nd * g->esz;
+  const size_t seg_bytes = size_t(K) * row_bytes;
+  long R = long(history_mb * 1e6 / double(seg_bytes));
+  R = std::max(1L, std::min<long>(R, S));
+  const int nck = S - int(R);
+  char* hist = nullptr;
+  if ((rc = sc.get((void**)&hist, size_t(R) * seg_bytes))) return rc;
+  std::vector<std::pair<void*, size_t>> st;
+  state_arrays(g, st);
+  size_t st_bytes = 0;
+  for (auto& p : st) st_bytes += (p.second + 255) & ~size_t(255);
+  char* ck = nullptr;
+  if (nck && (rc = sc.get((void**)&ck, size_t(nck) * st_bytes))) return rc;
+  if (nck && (rc = ensure_pingpong(g))) return rc;
+  auto copy_state = [&](char* base, bool save) -> int {
+    state_arrays(g, st);
+    size_t off = 0;
+    for (auto& p : st) {
+      void* dst = save ? (void*)(base + off) : p.first;
+      const void* src = save ? (const void*)p.first : (const void*)(base + off);
+      HIP_TRY(hipMemcpyAsync(dst, src, p.second, hipMemcpyDeviceToDevice, g->stream));
+      off += (p.second + 255) & ~size_t(255);
+    }
+    return 0;
+  };
+
+  // forward pass with checkpoints / recorded increments
+  for (int s = 0; s < S && !rc; s++) {
+    const long n0 = long(s) * K;
+    const int len = int(std::min<long>(K, nsteps - n0));
+    if (s < nck) {
+      rc = copy_state(ck + size_t(s) * st_bytes, true);
+    } else {
+      g->hist = hist + size_t(s - nck) * seg_bytes;
+      g->hist_step0 = n0;
+      g->hist_rows = len;
+    }
+    if (!rc) rc = step_impl<T>(g, len);
+    g->hist = nullptr;
+    g->hist_rows = 0;
+  }
+  if (rc) return rc;
+
+  // objective and its adjoint source weights 2 F
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  const auto& d = g->dft[didx];
+  const size_t per_slot = size_t(d.nfreq) * 2 * d.ncells;
+  std::vector<double> acc(size_t(d.ncomp) * per_slot);
+  if ((rc = download_T(g, acc.data(), d.acc, acc.size()))) return rc;
+  fdtdk::AdjArgs<T> q;
+  memset(&q, 0, sizeof q);
+  q.eps0 = kEps0;
+  for (int x = 0; x < 6; x++) q.obj_r[x] = d.r[x];
+  q.obj_nfreq = d.nfreq;
+  q.obj_ncells = d.ncells;
+  q.obj_ph = d.phE;
+  q.obj_nphase = d.nphase;
+  double J = 0;
+  std::vector<double> w;
+  int ns = 0;
+  for (int c = 0; c < 3; c++) q.obj_comp[c] = -1;
+  for (int c = 0; c < 3; c++) {
+    if (!((mask >> c) & 1)) continue;
+    const int sm = __builtin_popcount(unsigned(d.mask) & ((1u << c) - 1));
+    const double* src = acc.data() + size_t(sm) * per_slot;
+    for (size_t x = 0; x < per_slot; x++) {
+      J += src[x] * src[x];
+      w.push_back(2.0 * src[x]);
+    }
+    q.obj_comp[ns++] = c;
+  }
+  void* wdev = nullptr;
+  if ((rc = sc.get(&wdev, w.size() * g->esz))) return rc;
+  if ((rc = upload_T(g, wdev, w.data(), w.size()))) return rc;
+  q.obj_w = (const T*)wdev;
+
+  // adjoint state (zero: the adjoint of the state after the last step)
+  for (int c = 0; c < 3; c++) {
+    if ((rc = sc.get((void**)&q.nu[c], size_t(g->ncell) * g->esz))) return rc;
+    if ((rc = sc.get((void**)&q.mu[c], size_t(g->ncell) * g->esz))) return rc;
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int Jx = 0; Jx < 3; Jx++) {
+        void* p = g->psi[ft][I][Jx];
+        if (!p) continue;
+        auto it = std::find(g->allocs.begin(), g->allocs.end(), p);
+        if ((rc = sc.get((void**)&q.lpsi[ft][I][Jx], g->alloc_bytes[it - g->allocs.begin()]))) return rc;
+      }
+  if ((rc = sc.get((void**)&q.grad, size_t(nd) * 8))) return rc;
+  fdtdk::StepArgs<T> a;
+  fill_args<T>(g, a);
+  void *adev = nullptr, *qdev = nullptr;
+  if ((rc = sc.get(&adev, sizeof a))) return rc;
+  if ((rc = sc.get(&qdev, sizeof q))) return rc;
+  HIP_TRY(hipMemcpy(adev, &a, sizeof a, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(qdev, &q, sizeof q, hipMemcpyHostToDevice));
+  const auto* ap = (const fdtdk::StepArgs<T>*)adev;
+  const auto* qp = (const fdtdk::AdjArgs<T>*)qdev;
+
+  const Tile t = (g->tile.by && g->tile.bx > 0) ? g->tile : kDefaultTile3D;
+  const int nzi = g->d.nz - 2;
+  const dim3 block(t.bx, t.by, 1);
+  const dim3 grd((g->d.nx - 2 + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, (nzi + t.kc - 1) / t.kc);
+  for (int s = S - 1; s >= 0; s--) {
+    const long n0 = long(s) * K;
+    const int len = int(std::min<long>(K, nsteps - n0));
+    char* hb = hist;
+    if (s >= nck) {
+      hb = hist + size_t(s - nck) * seg_bytes;
+    } else {
+      // recompute this segment's increments from its checkpoint in the second copy
+      swap_state(g);
+      rc = copy_state(ck + size_t(s) * st_bytes, false);
+      const long step_saved = g->step;
+      g->step = n0;
+      g->recompute = true;
+      g->hist = hist;
+      g->hist_step0 = n0;
+      g->hist_rows = len;
+      if (!rc) rc = step_impl<T>(g, len);
+      g->step = step_saved;
+      g->recompute = false;
+      g->hist = nullptr;
+      g->hist_rows = 0;
+      swap_state(g);
+      if (rc) return rc;
+    }
+    for (long n = n0 + len - 1; n >= n0 && !rc; n--) {
+      const T* hrow = (const T*)(hb + size_t(n - n0) * row_bytes);
+      rc = launch(g, 5, fdtdk::adj_e_pass_3d<T>, grd, block, ap, qp, t.kc, n, hrow);
+      if (!rc) rc = launch(g, 6, fdtdk::adj_h_pass_3d<T>, grd, block, ap, qp, t.kc);
+    }
+    if (rc) return rc;
+  }
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  HIP_TRY(hipMemcpy(grad, q.grad, size_t(nd) * 8, hipMemcpyDeviceToHost));
+  *objective = J;
+  return collect_events(g);
 }
 
 }  // namespace
@@ -1025,7 +1277,9 @@ int fdtd_download_field(fdtd_grid* g, int field, int comp, int pole, double* hos
   if (field == FDTD_D) {
     int rc = download_T(g, host, g->E[comp], g->ncell);
     if (rc) return rc;
-    for (long i = 0; i < g->ncell; i++) host[i] *= g->mats[g->mat_host[i]].eps * kEps0;
+    for (int k = 0; k < g->d.nz; k++)
+      for (int j = 0; j < g->d.ny; j++)
+        for (int i = 0; i < g->d.nx; i++) host[g->q(i, j, k)] *= cell_eps(g, i, j, k) * kEps0;
   } else {
     std::fill(host, host + g->ncell, 0.0);
     if (pole < 0 || pole >= FDTD_MAX_POLES) return fail(FDTD_EINVAL, "bad pole index");
@@ -1108,10 +1362,10 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     off += (sizes[q] + 255) & ~size_t(255);
   }
   const bool prof = g->prof;
-  double kms[5];
-  long kcount[5];
-  std::copy(g->kms, g->kms + 5, kms);
-  std::copy(g->kcount, g->kcount + 5, kcount);
+  double kms[kNumKernels];
+  long kcount[kNumKernels];
+  std::copy(g->kms, g->kms + kNumKernels, kms);
+  std::copy(g->kcount, g->kcount + kNumKernels, kcount);
   const Tile saved = g->tile;
   double best = 1e300;
   Tile bt = cand[0];
@@ -1145,8 +1399,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   if ((g->nswap - swaps0) % 2) swap_state(g);
   g->step = step0;
   g->prof = prof;
-  std::copy(kms, kms + 5, g->kms);
-  std::copy(kcount, kcount + 5, g->kcount);
+  std::copy(kms, kms + kNumKernels, g->kms);
+  std::copy(kcount, kcount + kNumKernels, g->kcount);
   if (rc) {
     g->tile = saved;
     return rc;
@@ -1171,7 +1425,7 @@ int fdtd_set_profiling(fdtd_grid* g, int on) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
   int rc = collect_events(g);
   g->prof = on != 0;
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < kNumKernels; k++) {
     g->kms[k] = 0;
     g->kcount[k] = 0;
   }
@@ -1179,12 +1433,63 @@ int fdtd_set_profiling(fdtd_grid* g, int on) {
 }
 
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches) {
-  if (!g || kernel < 0 || kernel > 4) return fail(FDTD_EINVAL, "bad argument");
+  if (!g || kernel < 0 || kernel >= kNumKernels) return fail(FDTD_EINVAL, "bad argument");
   int rc = collect_events(g);
   if (rc) return rc;
   if (ms) *ms = g->kms[kernel];
   if (launches) *launches = g->kcount[kernel];
   return 0;
+}
+
+int fdtd_set_design(fdtd_grid* g, const int box[6], const double* eps_r) {
+  if (!g || !box || !eps_r) return fail(FDTD_EINVAL, "null argument");
+  int b[6];
+  for (int x = 0; x < 6; x++) b[x] = box[x];
+  if (g->ndim == 2) {
+    b[4] = 0;
+    b[5] = 1;
+  }
+  for (int a = 0; a < g->ndim; a++)
+    if (b[2 * a] < 1 || b[2 * a + 1] > g->len[a] - 1 || b[2 * a] >= b[2 * a + 1])
+      return fail(FDTD_EINVAL, "design box must cover interior cells only, with lo < hi");
+  const long n = long(b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]);
+  for (long i = 0; i < n; i++)
+    if (!(eps_r[i] > 0)) return fail(FDTD_EINVAL, "design eps_r must be positive");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (g->design && n != g->ndes) {
+    g->release(g->dsg_ie_dev);
+    g->dsg_ie_dev = nullptr;
+  }
+  if (!g->dsg_ie_dev) {
+    int rc = g->alloc(&g->dsg_ie_dev, size_t(n) * g->esz);
+    if (rc) return rc;
+  }
+  std::vector<double> ie(n);
+  for (long i = 0; i < n; i++) ie[i] = 1.0 / (eps_r[i] * kEps0);
+  int rc = upload_T(g, g->dsg_ie_dev, ie.data(), n);
+  if (rc) return rc;
+  g->dsg_eps.assign(eps_r, eps_r + n);
+  for (int x = 0; x < 6; x++) g->dsg[x] = b[x];
+  g->ndes = n;
+  g->design = true;
+  return 0;
+}
+
+int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, double history_mb, int dft,
+                              int comp_mask, double* objective, double* grad) {
+  if (!g || !objective || !grad) return fail(FDTD_EINVAL, "null argument");
+  if (nsteps < 1 || checkpoint_every < 1 || !(history_mb >= 0)) return fail(FDTD_EINVAL, "bad step counts or budget");
+  if (g->ndim != 3) return fail(FDTD_EINVAL, "the adjoint gradient is implemented for 3-D grids");
+  if (!g->design) return fail(FDTD_ESTATE, "no design box (fdtd_set_design)");
+  if (g->step != 0) return fail(FDTD_ESTATE, "the adjoint gradient runs from the initial state (step 0)");
+  if (dft < 0 || dft >= g->ndft) return fail(FDTD_EINVAL, "bad DFT monitor index");
+  if (comp_mask < 1 || comp_mask > 7 || (comp_mask & ~g->dft[dft].mask))
+    return fail(FDTD_EINVAL, "objective components must be E components recorded by the monitor");
+  int rc = 0;
+  if (!g->finalized && (rc = finalize(g))) return rc;
+  if (g->max_np > 0) return fail(FDTD_EINVAL, "the adjoint gradient does not cover dispersive materials");
+  return g->f64 ? adjoint_impl<double>(g, nsteps, checkpoint_every, history_mb, dft, comp_mask, objective, grad)
+                : adjoint_impl<float>(g, nsteps, checkpoint_every, history_mb, dft, comp_mask, objective, grad);
 }
 
 

@@ -113,7 +113,35 @@ struct StepArgs {
   ProbeDev probe[kMaxProbe];
   double* trace;  // [step][nprobe]
   long step0;     // global step of row 0 of the incident rows
+  // design box [x0,x1)x[y0,y1)x[z0,z1): per-cell 1/(eps_r eps0) replaces the
+  // material table's value there (null when the grid has no design box)
+  const T* dsg_ie;
+  int dsg[6];
+  // E increment of the design cells, one row [3][ndes] per step of
+  // [hist_step0, hist_step0 + hist_rows) (null = not recorded)
+  T* hist;
+  long hist_step0, hist_rows;
 };
+
+// local index of (i,j,k) in the design box, -1 outside
+template <typename T>
+__device__ __forceinline__ long design_index(const StepArgs<T>& a, int i, int j, int k) {
+  const int di = i - a.dsg[0], dj = j - a.dsg[2], dk = k - a.dsg[4];
+  const int wx = a.dsg[1] - a.dsg[0], wy = a.dsg[3] - a.dsg[2], wz = a.dsg[5] - a.dsg[4];
+  if ((unsigned)di >= (unsigned)wx || (unsigned)dj >= (unsigned)wy || (unsigned)dk >= (unsigned)wz) return -1;
+  return (long)di + (long)wx * ((long)dj + (long)wy * dk);
+}
+
+// 1/(eps_r eps0) of a cell: the design value inside the design box, else the material's
+template <typename T>
+__device__ __forceinline__ T cell_inv_eps(const StepArgs<T>& a, int mid, int i, int j, int k) {
+  T ie = a.mie[mid];
+  if (a.dsg_ie) {
+    const long l = design_index(a, i, j, k);
+    if (l >= 0) ie = a.dsg_ie[l];
+  }
+  return ie;
+}
 
 __device__ __forceinline__ int slab_index(int idx, int a, int b) { return idx < a ? idx : (idx >= b ? a + idx - b : -1); }
 
@@ -276,7 +304,7 @@ template <typename T>
 __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const double* incH, int i, int j, int k,
                                           long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, bool own,
                                           T out[3]) {
-  const T ie = a.mie[mid];
+  const T ie = cell_inv_eps(a, mid, i, j, k);
   const T fx = a.cf[0][0][i], fy = a.cf[0][1][j], fz = a.cf[0][2][k];
   const T dxy = h.hz - h.hz_ym;  // Ex, J=y, +1
   const T dxz = h.hy - h.hy_zm;  // Ex, J=z, -1
@@ -472,6 +500,20 @@ __global__ __launch_bounds__(256) void dft_accumulate_batch(const StepArgs<T>* _
   dft_item(a, g - offs[lo], a.step0 + s);
 }
 
+// E increment of a design cell at `step`, kept for the adjoint's eps gradient
+template <typename T>
+__device__ FDTD_COLD void record_increment(const StepArgs<T>& a, long step, int i, int j, int k, T d0, T d1, T d2) {
+  const long row = step - a.hist_step0;
+  if (row < 0 || row >= a.hist_rows) return;
+  const long l = design_index(a, i, j, k);
+  if (l < 0) return;
+  const long nd = (long)(a.dsg[1] - a.dsg[0]) * (a.dsg[3] - a.dsg[2]) * (a.dsg[5] - a.dsg[4]);
+  T* r = a.hist + row * 3 * nd + l;
+  r[0] = d0;
+  r[nd] = d1;
+  r[2 * nd] = d2;
+}
+
 // ---------------------------------------------------------------- two passes
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
@@ -506,6 +548,7 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, int i, int j,
     a.Eo[0][c] = o[0];
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
+    if (a.hist) record_increment(a, step, i, j, k, o[0] - ex, o[1] - ey, o[2] - ez);
   }
 }
 
@@ -788,7 +831,7 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
   const T hx = Hx[c], hy = Hy[c], hy_xm = Hy[c - 1], hx_ym = Hx[c - a.sy];
   const T ez = a.E[2][c];
   const int mid = a.mat[c];
-  const T ie = a.mie[mid];
+  const T ie = cell_inv_eps(a, mid, i, j, 0);
   const T dzx = hy - hy_xm;  // J=x, +1
   const T dzy = hx - hx_ym;  // J=y, -1
   T d[3] = {T(0), T(0), T(0)};

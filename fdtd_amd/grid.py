@@ -27,6 +27,7 @@ ABI_SYMBOLS = [
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
+    "fdtd_set_design", "fdtd_adjoint_eps_gradient",
 ]
 
 
@@ -81,6 +82,8 @@ def load_library():
         "fdtd_batch_set_graph": (_i, [_p, _i]),
         "fdtd_batch_step": (_i, [_p, _i]),
         "fdtd_batch_synchronize": (_i, [_p]),
+        "fdtd_set_design": (_i, [_p, _ip, _dp]),
+        "fdtd_adjoint_eps_gradient": (_i, [_p, _i, _i, _d, _i, _i, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -240,6 +243,23 @@ class Grid:
         n = ctypes.c_long()
         _check(load_library().fdtd_kernel_time(self.h, kernel, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    # ---- design and adjoint
+    def set_design(self, box, eps_r):
+        """Per-cell eps_r over box [i0,i1,j0,j1,k0,k1); eps_r shaped (k, j, i) or flat, x fastest."""
+        b = (ctypes.c_int * 6)(*box)
+        e = _f64(eps_r).reshape(-1)
+        _check(load_library().fdtd_set_design(self.h, b, _dptr(e)))
+        self.design_box = list(box)
+        self.design_shape = (box[5] - box[4], box[3] - box[2], box[1] - box[0])
+
+    def adjoint_eps_gradient(self, nsteps, dft, comp_mask=4, checkpoint_every=100, history_mb=4096.0):
+        """Step nsteps from step 0 and return (J, dJ/d eps_r) for J = sum |F|^2 of monitor `dft`."""
+        grad = np.zeros(int(np.prod(self.design_shape)))
+        obj = ctypes.c_double()
+        _check(load_library().fdtd_adjoint_eps_gradient(self.h, nsteps, checkpoint_every, float(history_mb), dft,
+                                                        comp_mask, ctypes.byref(obj), _dptr(grad)))
+        return obj.value, grad.reshape(self.design_shape)
 
 
 class Batch:

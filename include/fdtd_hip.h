@@ -139,9 +139,35 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
 
 /* Per-kernel timing with hipEvents on the grid's stream (off by default).
  * fdtd_kernel_time returns accumulated milliseconds and launches for
- * kernel id 0 = E pass, 1 = H pass, 2 = aux line, 3 = fused pass, 4 = DFT. */
+ * kernel id 0 = E pass, 1 = H pass, 2 = aux line, 3 = fused pass, 4 = DFT,
+ * 5 = adjoint E pass, 6 = adjoint H pass. */
 int fdtd_set_profiling(fdtd_grid* g, int on);
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches);
+
+/* Design box [i0,i1)x[j0,j1)x[k0,k1) of interior cells whose relative
+ * permittivity is given per cell (eps_r, x fastest), replacing the material
+ * table's eps_r there; such cells are non-dispersive, mu_r stays the
+ * material's.  The reference sets eps_r per material only
+ * (include/DispersiveMaterials.hpp:342-359); this per-cell form exists for
+ * the adjoint gradient below.  Calling it again replaces the values. */
+int fdtd_set_design(fdtd_grid* g, const int box[6], const double* eps_r);
+
+/* Discrete adjoint gradient (no counterpart in the reference).  From the
+ * initial state (step 0) steps the grid nsteps forward, then returns the
+ * objective J = sum over the cells of DFT monitor `dft`, the E components of
+ * comp_mask (bit c = E_c, recorded by that monitor) and its frequencies of
+ * |F|^2, and grad[cell] = dJ/d eps_r for every design cell (same order as
+ * fdtd_set_design).  Reverse steps are the transpose of the forward step
+ * (curl, CPML psi, per-cell material; sources and TFSF terms are
+ * state-independent).  The design cells' E increments are kept for segments
+ * of checkpoint_every steps: the last segments that fit history_mb megabytes
+ * (at least one) are recorded during the forward pass, every earlier segment
+ * restarts from a checkpoint of E, H and psi taken at its first step and is
+ * recomputed just before its reverse steps.  On return the grid holds the
+ * forward state after nsteps steps and the monitor holds F.  3-D,
+ * non-dispersive grids only. */
+int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, double history_mb, int dft,
+                              int comp_mask, double* objective, double* grad);
 
 /* Batches: independent 3-D grids of one precision stepped together, one
  * launch per pass covering the tiles of every grid, each chunk of steps

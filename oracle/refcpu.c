@@ -266,10 +266,27 @@ struct rc_sim {
   double* ptrace;
   int ptrace_cap;
   long step;
+  /* design box: per-cell eps_r (build-defined) */
+  int has_design;
+  int dsg[6];
+  double* dsg_eps;
+  /* adjoint state: D, B and psi adjoints carried across steps; E and H
+   * adjoints are per-step accumulators of the derived fields */
+  double *Db[3], *Bb[3], *Eb[3], *Hb[3];
+  double* Pb[2][3][3];
+  int obj_dft, obj_mask;
 };
 
 static size_t cidx(const rc_sim* s, int i, int j, int k) {
   return (size_t)i + (size_t)s->nx * ((size_t)j + (size_t)s->ny * (size_t)k);
+}
+
+/* index of (i,j,k) in the design box (x fastest), -1 outside */
+static long design_local(const rc_sim* s, int i, int j, int k) {
+  if (!s->has_design) return -1;
+  const int* b = s->dsg;
+  if (i < b[0] || i >= b[1] || j < b[2] || j >= b[3] || k < b[4] || k >= b[5]) return -1;
+  return (long)(i - b[0]) + (long)(b[1] - b[0]) * ((long)(j - b[2]) + (long)(b[3] - b[2]) * (long)(k - b[4]));
 }
 
 rc_sim* rc_create(int mode, int nx, int ny, int nz, double dx, double dt) {
@@ -360,6 +377,16 @@ void rc_destroy(rc_sim* s) {
   free(s->dft);
   free(s->probes);
   free(s->ptrace);
+  free(s->dsg_eps);
+  for (int c = 0; c < 3; c++) {
+    free(s->Db[c]);
+    free(s->Bb[c]);
+    free(s->Eb[c]);
+    free(s->Hb[c]);
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < 3; J++) free(s->Pb[ft][I][J]);
   free(s);
 }
 
@@ -477,6 +504,8 @@ int rc_get_probes(const rc_sim* s, double* out, int max_steps) {
 }
 
 static double* field_ptr(const rc_sim* s, int field, int comp, int pole) {
+  if (field == 6 || field == 7) return (comp >= 0 && comp < 9) ? s->psi[field - 6][comp / 3][comp % 3] : NULL;
+  if (comp < 0 || comp > 2) return NULL;
   switch (field) {
     case 0: return s->D[comp];
     case 1: return s->E[comp];
@@ -489,14 +518,14 @@ static double* field_ptr(const rc_sim* s, int field, int comp, int pole) {
 }
 
 int rc_get_field(const rc_sim* s, int field, int comp, int pole, double* out) {
-  double* p = (comp >= 0 && comp < 3) ? field_ptr(s, field, comp, pole) : NULL;
+  double* p = field_ptr(s, field, comp, pole);
   if (!p) return -1;
   memcpy(out, p, sizeof(double) * s->ncell);
   return 0;
 }
 
 int rc_set_field(rc_sim* s, int field, int comp, int pole, const double* in) {
-  double* p = (comp >= 0 && comp < 3) ? field_ptr(s, field, comp, pole) : NULL;
+  double* p = field_ptr(s, field, comp, pole);
   if (!p) return -1;
   memcpy(p, in, sizeof(double) * s->ncell);
   return 0;
@@ -667,8 +696,13 @@ int rc_step(rc_sim* s, int nsteps) {
           for (int i = lo[0]; i < hi[0]; i++) {
             size_t c = cidx(s, i, j, k);
             const rc_mat* m = &s->mat[s->matid[c]];
+            const long dl = design_local(s, i, j, k);
             for (int I = 2; I >= 0; I--) {
               if (!s->actE[I]) continue;
+              if (dl >= 0) { /* design cell: non-dispersive, its own eps_r */
+                s->E[I][c] = s->D[I][c] / (s->dsg_eps[dl] * RC_EPS0);
+                continue;
+              }
               if (m->np == 0 || np == 0) {
                 s->E[I][c] = s->D[I][c] / (m->eps * RC_EPS0);
                 continue;
@@ -793,5 +827,307 @@ int rc_step(rc_sim* s, int nsteps) {
     }
     s->step++;
   }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- adjoint
+ * Build-defined (the reference has no adjoint).  Reverse mode of rc_step for
+ * non-dispersive cells, op by op in the D/B form above: every forward update
+ * y += a x becomes x_bar += a y_bar, run in the reverse order of the forward
+ * ops; sources and TFSF terms do not depend on the state and drop out.  Each
+ * material op E = D/(eps_r eps0) contributes dJ/d eps_r += E_bar (-E/eps_r).
+ * Written independently of the device adjoint, which works on E and H
+ * increments with gathered (not scattered) transposes. */
+
+int rc_set_design(rc_sim* s, const int box[6], const double* eps_r) {
+  const size_t n = (size_t)(box[1] - box[0]) * (box[3] - box[2]) * (box[5] - box[4]);
+  free(s->dsg_eps);
+  s->dsg_eps = malloc(sizeof(double) * n);
+  memcpy(s->dsg_eps, eps_r, sizeof(double) * n);
+  memcpy(s->dsg, box, sizeof(int) * 6);
+  s->has_design = 1;
+  return 0;
+}
+
+static double cell_eps(const rc_sim* s, int i, int j, int k) {
+  const long dl = design_local(s, i, j, k);
+  return dl >= 0 ? s->dsg_eps[dl] : s->mat[s->matid[cidx(s, i, j, k)]].eps;
+}
+
+void rc_adjoint_reset(rc_sim* s) {
+  for (int c = 0; c < 3; c++) {
+    if (s->actE[c]) {
+      if (!s->Db[c]) s->Db[c] = malloc(sizeof(double) * s->ncell);
+      if (!s->Eb[c]) s->Eb[c] = malloc(sizeof(double) * s->ncell);
+      memset(s->Db[c], 0, sizeof(double) * s->ncell);
+      memset(s->Eb[c], 0, sizeof(double) * s->ncell);
+    }
+    if (s->actH[c]) {
+      if (!s->Bb[c]) s->Bb[c] = malloc(sizeof(double) * s->ncell);
+      if (!s->Hb[c]) s->Hb[c] = malloc(sizeof(double) * s->ncell);
+      memset(s->Bb[c], 0, sizeof(double) * s->ncell);
+      memset(s->Hb[c], 0, sizeof(double) * s->ncell);
+    }
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < 3; J++)
+        if (s->psi[ft][I][J]) {
+          if (!s->Pb[ft][I][J]) s->Pb[ft][I][J] = malloc(sizeof(double) * s->ncell);
+          memset(s->Pb[ft][I][J], 0, sizeof(double) * s->ncell);
+        }
+}
+
+/* Reverse of forward step n.  w: objective weights 2F of monitor obj_dft
+ * ([slot][nfreq][2][ncells]) or NULL; Edes: E after step n at the design
+ * cells ([3][ndes]) or NULL; grad accumulates dJ/d eps_r. */
+static void adj_step(rc_sim* s, long n, const double* w, const double* Edes, double* grad) {
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const double dt = s->dt, dx = s->dx, mu0 = rc_mu0_v();
+  const long nd = s->has_design ? (long)(s->dsg[1] - s->dsg[0]) * (s->dsg[3] - s->dsg[2]) * (s->dsg[5] - s->dsg[4]) : 0;
+  /* DFT of E after step n: J = sum |F|^2, dJ/dE_n = 2 Re(conj(F) phase_n) */
+  if (w) {
+    const rc_dft* d = &s->dft[s->obj_dft];
+    if (n < d->nphase) {
+      const double* ph = d->phE + 2 * (size_t)d->nfreq * n;
+      int slot = 0;
+      for (int comp = 0; comp < 3; comp++) {
+        if (!((s->obj_mask >> comp) & 1)) continue;
+        for (int f = 0; f < d->nfreq; f++) {
+          const double* wr = w + ((size_t)slot * d->nfreq + f) * 2 * d->ncells;
+          const double* wi = wr + d->ncells;
+          size_t q2 = 0;
+          const int* r = d->region;
+          for (int k = r[4]; k < r[5]; k++)
+            for (int j = r[2]; j < r[3]; j++)
+              for (int i = r[0]; i < r[1]; i++, q2++) s->Eb[comp][cidx(s, i, j, k)] += wr[q2] * ph[2 * f] + wi[q2] * ph[2 * f + 1];
+        }
+        slot++;
+      }
+    }
+  }
+  /* H = B/(mu_r mu0) */
+  for (int k = lo[2]; k < hi[2]; k++)
+    for (int j = lo[1]; j < hi[1]; j++)
+      for (int i = lo[0]; i < hi[0]; i++) {
+        const size_t c = cidx(s, i, j, k);
+        const double mu = s->mat[s->matid[c]].mu;
+        for (int I = 0; I < 3; I++)
+          if (s->actH[I]) s->Bb[I][c] += s->Hb[I][c] / (mu * mu0);
+      }
+  for (int I = 0; I < 3; I++)
+    if (s->actH[I]) memset(s->Hb[I], 0, sizeof(double) * s->ncell);
+  /* UpdatePML magnetic: B_I += -dt G psi; psi = B psi + C e/dx (E_K[c+sJ] - E_K[c]) */
+  for (int J = 2; J >= 0; J--) {
+    if (!s->axes[J]) continue;
+    const size_t sJ = stride(s, J);
+    const double* pm = s->pml[1][J];
+    for (int I = 0; I < 3; I++) {
+      if (I == J || !s->actH[I]) continue;
+      const int K = third(I, J);
+      const int e = lc(I, J, K);
+      double* pb = s->Pb[1][I][J];
+      const double* bb = s->Bb[I];
+      double* ekb = s->Eb[K];
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            const size_t c = cidx(s, i, j, k);
+            const int ci[3] = {i, j, k};
+            const double* p = pm + 4 * ci[J];
+            const double t = pb[c];
+            if (ekb) {
+              const double a = p[1] * e / dx * t;
+              ekb[c + sJ] += a;
+              ekb[c] -= a;
+            }
+            pb[c] = p[0] * t - dt * p[3] * bb[c];
+          }
+    }
+  }
+  /* B curl: B_I += -dt e F/dx (E_K[c+sJ] - E_K[c]) */
+  for (int I = 2; I >= 0; I--) {
+    if (!s->actH[I]) continue;
+    const int js[2] = {D1[I], D2[I]};
+    for (int t = 0; t < 2; t++) {
+      const int Jx = js[t], K = third(I, Jx);
+      if (!(s->actE[K] && s->axes[Jx])) continue;
+      const size_t sJ = stride(s, Jx);
+      const int e = lc(I, Jx, K);
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            const size_t c = cidx(s, i, j, k);
+            const int ci[3] = {i, j, k};
+            const double a = -dt * e * s->pml[1][Jx][4 * ci[Jx] + 2] / dx * s->Bb[I][c];
+            s->Eb[K][c + sJ] += a;
+            s->Eb[K][c] -= a;
+          }
+    }
+  }
+  /* E = D/(eps_r eps0): D_bar += E_bar/(eps_r eps0), dJ/d eps_r += E_bar (-E/eps_r) */
+  for (int k = lo[2]; k < hi[2]; k++)
+    for (int j = lo[1]; j < hi[1]; j++)
+      for (int i = lo[0]; i < hi[0]; i++) {
+        const size_t c = cidx(s, i, j, k);
+        const double eps = cell_eps(s, i, j, k);
+        const long dl = design_local(s, i, j, k);
+        for (int I = 0; I < 3; I++) {
+          if (!s->actE[I]) continue;
+          if (dl >= 0 && grad && Edes) grad[dl] += s->Eb[I][c] * (-Edes[(size_t)I * nd + dl] / eps);
+          s->Db[I][c] += s->Eb[I][c] / (eps * RC_EPS0);
+        }
+      }
+  for (int I = 0; I < 3; I++)
+    if (s->actE[I]) memset(s->Eb[I], 0, sizeof(double) * s->ncell);
+  /* UpdatePML electric: D_I += dt G psi; psi = B psi + C e/dx (H_K[c] - H_K[c-sJ]) */
+  for (int J = 2; J >= 0; J--) {
+    if (!s->axes[J]) continue;
+    const size_t sJ = stride(s, J);
+    const double* pm = s->pml[0][J];
+    for (int I = 0; I < 3; I++) {
+      if (I == J || !s->actE[I]) continue;
+      const int K = third(I, J);
+      const int e = lc(I, J, K);
+      double* pb = s->Pb[0][I][J];
+      const double* db = s->Db[I];
+      double* hkb = s->Hb[K];
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            const size_t c = cidx(s, i, j, k);
+            const int ci[3] = {i, j, k};
+            const double* p = pm + 4 * ci[J];
+            const double t = pb[c];
+            if (hkb) {
+              const double a = p[1] * e / dx * t;
+              hkb[c] += a;
+              hkb[c - sJ] -= a;
+            }
+            pb[c] = p[0] * t + dt * p[3] * db[c];
+          }
+    }
+  }
+  /* D curl: D_I += dt e F/dx (H_K[c] - H_K[c-sJ]) */
+  for (int I = 2; I >= 0; I--) {
+    if (!s->actE[I]) continue;
+    const int js[2] = {D1[I], D2[I]};
+    for (int t = 0; t < 2; t++) {
+      const int Jx = js[t], K = third(I, Jx);
+      if (!(s->actH[K] && s->axes[Jx])) continue;
+      const size_t sJ = stride(s, Jx);
+      const int e = lc(I, Jx, K);
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            const size_t c = cidx(s, i, j, k);
+            const int ci[3] = {i, j, k};
+            const double a = dt * e * s->pml[0][Jx][4 * ci[Jx] + 2] / dx * s->Db[I][c];
+            s->Hb[K][c] += a;
+            s->Hb[K][c - sJ] -= a;
+          }
+    }
+  }
+}
+
+void rc_adjoint_step(rc_sim* s, long n) { adj_step(s, n, NULL, NULL, NULL); }
+
+void rc_adjoint_finish(rc_sim* s) {
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const double mu0 = rc_mu0_v();
+  for (int k = lo[2]; k < hi[2]; k++)
+    for (int j = lo[1]; j < hi[1]; j++)
+      for (int i = lo[0]; i < hi[0]; i++) {
+        const size_t c = cidx(s, i, j, k);
+        for (int I = 0; I < 3; I++)
+          if (s->actH[I]) s->Bb[I][c] += s->Hb[I][c] / (s->mat[s->matid[c]].mu * mu0);
+      }
+  for (int I = 0; I < 3; I++)
+    if (s->actH[I]) memset(s->Hb[I], 0, sizeof(double) * s->ncell);
+}
+
+void rc_sync_derived(rc_sim* s) {
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const double mu0 = rc_mu0_v();
+  for (int k = lo[2]; k < hi[2]; k++)
+    for (int j = lo[1]; j < hi[1]; j++)
+      for (int i = lo[0]; i < hi[0]; i++) {
+        const size_t c = cidx(s, i, j, k);
+        for (int I = 0; I < 3; I++) {
+          if (s->actE[I]) s->E[I][c] = s->D[I][c] / (cell_eps(s, i, j, k) * RC_EPS0);
+          if (s->actH[I]) s->H[I][c] = s->B[I][c] / (s->mat[s->matid[c]].mu * mu0);
+        }
+      }
+}
+
+static double* adj_ptr(const rc_sim* s, int field, int comp) {
+  if (field == 0 && comp >= 0 && comp < 3) return s->Db[comp];
+  if (field == 2 && comp >= 0 && comp < 3) return s->Bb[comp];
+  if ((field == 6 || field == 7) && comp >= 0 && comp < 9) return s->Pb[field - 6][comp / 3][comp % 3];
+  return NULL;
+}
+
+int rc_get_adjoint(const rc_sim* s, int field, int comp, double* out) {
+  const double* p = adj_ptr(s, field, comp);
+  if (!p) return -1;
+  memcpy(out, p, sizeof(double) * s->ncell);
+  return 0;
+}
+
+int rc_set_adjoint(rc_sim* s, int field, int comp, const double* in) {
+  double* p = adj_ptr(s, field, comp);
+  if (!p) return -1;
+  memcpy(p, in, sizeof(double) * s->ncell);
+  return 0;
+}
+
+int rc_adjoint_eps_gradient(rc_sim* s, int nsteps, int dft, int comp_mask, double* objective, double* grad) {
+  if (!s->has_design || s->step != 0 || nsteps < 1 || dft < 0 || dft >= s->ndft) return -1;
+  if (comp_mask < 1 || comp_mask > 7 || (comp_mask & ~s->dft[dft].mask)) return -1;
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  for (int k = lo[2]; k < hi[2]; k++)
+    for (int j = lo[1]; j < hi[1]; j++)
+      for (int i = lo[0]; i < hi[0]; i++)
+        if (s->npoles_alloc && s->mat[s->matid[cidx(s, i, j, k)]].np > 0 && design_local(s, i, j, k) < 0) return -1;
+  const long nd = (long)(s->dsg[1] - s->dsg[0]) * (s->dsg[3] - s->dsg[2]) * (s->dsg[5] - s->dsg[4]);
+  double* hist = calloc((size_t)nsteps * 3 * nd, sizeof(double));
+  for (int n = 0; n < nsteps; n++) {
+    rc_step(s, 1);
+    for (int k = s->dsg[4]; k < s->dsg[5]; k++)
+      for (int j = s->dsg[2]; j < s->dsg[3]; j++)
+        for (int i = s->dsg[0]; i < s->dsg[1]; i++) {
+          const long dl = design_local(s, i, j, k);
+          for (int I = 0; I < 3; I++)
+            if (s->actE[I]) hist[((size_t)n * 3 + I) * nd + dl] = s->E[I][cidx(s, i, j, k)];
+        }
+  }
+  const rc_dft* d = &s->dft[dft];
+  const size_t per = (size_t)d->nfreq * 2 * d->ncells;
+  double* w = calloc(3 * per, sizeof(double));
+  double J = 0;
+  int slot = 0;
+  for (int c = 0; c < 3; c++) {
+    if (!((comp_mask >> c) & 1)) continue;
+    int sm = 0;
+    for (int b = 0; b < c; b++) sm += (d->mask >> b) & 1;
+    const double* a = d->acc + (size_t)sm * per;
+    for (size_t x = 0; x < per; x++) {
+      J += a[x] * a[x];
+      w[(size_t)slot * per + x] = 2.0 * a[x];
+    }
+    slot++;
+  }
+  s->obj_dft = dft;
+  s->obj_mask = comp_mask;
+  rc_adjoint_reset(s);
+  memset(grad, 0, sizeof(double) * nd);
+  for (long n = nsteps - 1; n >= 0; n--) adj_step(s, n, w, hist + (size_t)n * 3 * nd, grad);
+  free(hist);
+  free(w);
+  *objective = J;
   return 0;
 }

@@ -80,10 +80,27 @@ int rc_get_dft(const rc_sim* s, int idx, double* out); /* [ncomp][nfreq][2][ncel
 int rc_add_probe(rc_sim* s, int field, int comp, int i, int j, int k);
 int rc_get_probes(const rc_sim* s, double* out, int max_steps); /* [steps][nprobes] */
 
-/* field: 0 D, 1 E, 2 B, 3 H, 4 P(pole p), 5 J(pole p) ; comp 0..2 */
+/* field: 0 D, 1 E, 2 B, 3 H, 4 P(pole p), 5 J(pole p) ; comp 0..2; 6 psi E, 7 psi H with comp = 3 I + J */
 int rc_get_field(const rc_sim* s, int field, int comp, int pole, double* out);
 int rc_set_field(rc_sim* s, int field, int comp, int pole, const double* in);
 int rc_step(rc_sim* s, int nsteps);
+
+/* ---- build-defined adjoint (the reference has none) */
+/* per-cell eps_r over box [i0,i1)x[j0,j1)x[k0,k1) (x fastest); design cells are non-dispersive */
+int rc_set_design(rc_sim* s, const int box[6], const double* eps_r);
+/* E = D/(eps_r eps0), H = B/(mu_r mu0) on interior cells (non-dispersive) */
+void rc_sync_derived(rc_sim* s);
+/* Gradient from step 0 over nsteps of J = sum over the cells of DFT monitor `dft`, its E components in
+ * comp_mask and its frequencies of |F|^2, with respect to every design cell's eps_r. */
+int rc_adjoint_eps_gradient(rc_sim* s, int nsteps, int dft, int comp_mask, double* objective, double* grad);
+/* Transposed step, for the dot-product test: adjoint state D_bar (field 0), B_bar (2), psi_bar (6 E, 7 H;
+ * comp = 3 I + J).  reset zeroes it; step applies the transpose of the homogeneous forward step n;
+ * finish folds the adjoint of the step's input H = B/(mu_r mu0) into B_bar. */
+void rc_adjoint_reset(rc_sim* s);
+void rc_adjoint_step(rc_sim* s, long n);
+void rc_adjoint_finish(rc_sim* s);
+int rc_get_adjoint(const rc_sim* s, int field, int comp, double* out);
+int rc_set_adjoint(rc_sim* s, int field, int comp, const double* in);
 int rc_nthreads(int n); /* set OpenMP threads (0 = leave); returns the count in use */
 
 #ifdef __cplusplus

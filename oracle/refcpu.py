@@ -50,6 +50,11 @@ def lib():
             ("rc_get_field", _i, [_p, _i, _i, _i, _dp]),
             ("rc_set_field", _i, [_p, _i, _i, _i, _dp]),
             ("rc_step", _i, [_p, _i]), ("rc_nthreads", _i, [_i]),
+            ("rc_set_design", _i, [_p, _ip, _dp]), ("rc_sync_derived", None, [_p]),
+            ("rc_adjoint_eps_gradient", _i, [_p, _i, _i, _i, _dp, _dp]),
+            ("rc_adjoint_reset", None, [_p]), ("rc_adjoint_step", None, [_p, ctypes.c_long]),
+            ("rc_adjoint_finish", None, [_p]),
+            ("rc_get_adjoint", _i, [_p, _i, _i, _dp]), ("rc_set_adjoint", _i, [_p, _i, _i, _dp]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -208,6 +213,42 @@ class Sim:
     def step(self, n=1):
         assert lib().rc_step(self.h, n) == 0
         self._nstep = self._steps_done() + n
+
+    # ---- adjoint (build-defined)
+    def set_design(self, box, eps_r):
+        b = (ctypes.c_int * 6)(*box)
+        e = _f64(eps_r).reshape(-1)
+        assert lib().rc_set_design(self.h, b, _dptr(e)) == 0
+        self.design_shape = (box[5] - box[4], box[3] - box[2], box[1] - box[0])
+
+    def sync_derived(self):
+        lib().rc_sync_derived(self.h)
+
+    def adjoint_eps_gradient(self, nsteps, dft, comp_mask=4):
+        grad = np.zeros(int(np.prod(self.design_shape)))
+        obj = ctypes.c_double()
+        assert lib().rc_adjoint_eps_gradient(self.h, nsteps, dft, comp_mask, ctypes.byref(obj), _dptr(grad)) == 0
+        self._nstep = self._steps_done() + nsteps
+        return obj.value, grad.reshape(self.design_shape)
+
+    def adjoint_reset(self):
+        lib().rc_adjoint_reset(self.h)
+
+    def adjoint_step(self, n):
+        lib().rc_adjoint_step(self.h, n)
+
+    def adjoint_finish(self):
+        lib().rc_adjoint_finish(self.h)
+
+    def get_adjoint(self, field, comp):
+        out = np.zeros(int(np.prod(self.shape)))
+        if lib().rc_get_adjoint(self.h, field, comp, _dptr(out)) != 0:
+            return None
+        return out.reshape(self.shape)
+
+    def set_adjoint(self, field, comp, arr):
+        a = _f64(arr).reshape(-1)
+        assert lib().rc_set_adjoint(self.h, field, comp, _dptr(a)) == 0
 
 
 def nthreads(n=0):

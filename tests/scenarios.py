@@ -115,6 +115,8 @@ def apply(sim, sc):
         sim.add_probe(*pr)
     for d in sc.get("dfts", []):
         sim.add_dft(*d)
+    if sc.get("design"):
+        sim.set_design(*sc["design"])
     return sim
 
 
@@ -142,6 +144,18 @@ def gmsin_signal(nsteps, dt, f0, tw, t0):
         t += dt
         a = (t - t0) / tw
         out[n] = math.exp(-a * a) * math.sin(2 * math.pi * f0 * (t - t0))
+    return out
+
+
+def cw_signal(nsteps, dt, f0, ramp_periods=3.0):
+    """sin(2 pi f0 t) switched on by 1 - exp(-(t/tau)^2), tau = ramp_periods / f0,
+    at the aux line's accumulated time (t += dt before each use)."""
+    out = np.zeros(nsteps)
+    tau = ramp_periods / f0
+    t = 0.0
+    for n in range(nsteps):
+        t += dt
+        out[n] = (1.0 - math.exp(-(t / tau) ** 2)) * math.sin(2 * math.pi * f0 * t)
     return out
 
 

@@ -5,12 +5,15 @@ grid3d_cpml_tfsf_dft     3-D CPML + dielectric blocks + TFSF plane wave + DFT pl
 grid3d_dispersive_sphere 3-D Drude + 2-Lorentz sphere, dipole source, DFT box surface
                          (192^3 x 3000 steps at full size)
 tm2d_c1                  2-D TM 256^2, 10-cell CPML, centre Gaussian pulse, 4 probes
+batch_c4                 64 uneven 3-D grids (edges 64..160), point source, 4-frequency plane each
+grid3d_adjoint           3-D CPML + TFSF continuous wave, per-cell eps design box, Ez DFT plane
+                         at one frequency (128^3, design 64x64x16, 3000 steps at full size)
 """
 import math
 
 import numpy as np
 
-from scenarios import C0, D, E, EPS0, IMP0, H, SplitMix64, dft_phases, gauss_table, gmsin_signal
+from scenarios import C0, D, E, EPS0, IMP0, H, SplitMix64, cw_signal, dft_phases, gauss_table, gmsin_signal
 
 DRUDE, LORENTZ = 1, 2
 
@@ -163,3 +166,38 @@ def batch_c4(nsims=64, nsteps=1500, lo=64, hi=160, seed=4):
             "dfts": [([1, nx - 1, 1, ny - 1, nz // 2, nz // 2 + 1], 63, pe, ph)], "tfsf": None,
         })
     return out
+
+
+def grid3d_adjoint(n=128, nsteps=3000, design=(64, 64, 16), npml=10, monitor=64, seed=5):
+    """Inverse-design gradient workload: x-propagating, Ez-polarised continuous
+    wave (TFSF, one frequency f0 = c0/(20 dx)) through a design box of per-cell
+    eps_r seeded U[1,12] centred in the grid; objective sum |F[Ez](f0)|^2 over a
+    monitor x-plane of monitor x monitor cells behind the design box."""
+    nx = ny = nz = n
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(3.0))
+    rng = SplitMix64(seed)
+    wx, wy, wz = design
+    x0, y0, z0 = (nx - wx) // 2, (ny - wy) // 2, (nz - wz) // 2
+    dbox = [x0, x0 + wx, y0, y0 + wy, z0, z0 + wz]
+    eps = np.array([1.0 + 11.0 * rng.uniform() for _ in range(wx * wy * wz)]).reshape(wz, wy, wx)
+    inset = npml + 2
+    box = [inset, nx - 1 - inset, inset, ny - 1 - inset, inset, nz - 1 - inset]
+    f0 = C0 / (20 * dx)
+    origin = [math.floor(0.5 * (box[0] + box[1])) + 0.5, math.floor(0.5 * (box[2] + box[3])) + 0.5,
+              math.floor(0.5 * (box[4] + box[5])) + 0.5]
+    ext = max(box[1] - box[0], box[3] - box[2], box[5] - box[4]) + 12
+    ext += ext % 2
+    mx = min(box[1] - 1, x0 + wx + max(1, (box[1] - x0 - wx) // 2))
+    m = min(monitor, ny - 2 * inset - 2, nz - 2 * inset - 2)
+    my0, mz0 = (ny - m) // 2, (nz - m) // 2
+    pe, ph = dft_phases(nsteps, dt, [f0])
+    return {
+        "name": f"grid3d_adjoint_{n}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
+        "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, True, True], "materials": [(1.0, 1.0, [])],
+        "matid": np.zeros((nz, ny, nx), np.uint8), "sources": [], "probes": [],
+        "tfsf": dict(box=box, kvec=[1.0, 0.0, 0.0], pol=[0.0, 0.0, 1.0], polH=[0.0, 1.0, 0.0], origin=origin,
+                     aux_ncells=ext, signal=cw_signal(nsteps, dt, f0)),
+        "dfts": [([mx, mx + 1, my0, my0 + m, mz0, mz0 + m], 4, pe, ph)],
+        "design": (dbox, eps), "f0": f0,
+    }

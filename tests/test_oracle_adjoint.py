@@ -1,0 +1,132 @@
+"""Pins of the oracle's adjoint (oracle/refcpu.c, build-defined: the reference
+has no adjoint) without any GPU:
+
+* dot-product test <A v, w> = <v, A^T w> of one homogeneous step, block by
+  block (D, B, psiE, psiH), so that no block's scale hides another's error;
+* central finite differences (FP64, h = 1e-3) of the eps gradient of a
+  CPML + TFSF continuous-wave scenario with a design box;
+* a design box holding the material table's values steps bitwise like the table.
+"""
+import numpy as np
+import pytest
+
+import synth
+from oracle import refcpu
+from scenarios import apply
+
+PSI_E, PSI_H = 6, 7
+
+
+def _dot_sim():
+    sc = synth.grid3d_adjoint(n=14, nsteps=4, design=(4, 4, 3), npml=3, monitor=6)
+    sc["tfsf"] = None
+    sc["dfts"] = []
+    rng = np.random.default_rng(7)
+    # non-uniform materials (eps and mu) so both material transposes matter
+    sc["materials"] = [(1.0, 1.0, [])] + [(1.0 + 11.0 * rng.random(), 0.5 + rng.random(), []) for _ in range(3)]
+    matid = np.zeros((14, 14, 14), np.uint8)
+    matid[2:9, 3:10, 4:12] = 1
+    matid[6:12, 1:7, 2:8] = 2
+    matid[1:5, 8:13, 7:13] = 3
+    sc["matid"] = matid
+    return sc
+
+
+def _blocks(sim):
+    """name -> (getter, setter, list of component codes) of the stepped state."""
+    out = {"D": [(0, c) for c in range(3)], "B": [(2, c) for c in range(3)]}
+    for f, name in ((PSI_E, "psiE"), (PSI_H, "psiH")):
+        out[name] = [(f, 3 * I + J) for I in range(3) for J in range(3) if I != J and sim.get(f, 3 * I + J) is not None]
+    return out
+
+
+def _interior(shape):
+    m = np.zeros(shape, bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m
+
+
+@pytest.mark.parametrize("src", ["D", "B", "psiE", "psiH"])
+@pytest.mark.parametrize("dst", ["D", "B", "psiE", "psiH"])
+def test_adjoint_step_dot_product(src, dst):
+    sc = _dot_sim()
+    sim = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    blocks = _blocks(sim)
+    inner = _interior(sim.shape)
+    rng = np.random.default_rng(hash((src, dst)) % (1 << 32))
+    # v: random in block `src` (interior cells), zero elsewhere; H, E derived
+    v = {}
+    for f, c in blocks[src]:
+        a = np.where(inner, rng.standard_normal(sim.shape), 0.0)
+        v[(f, c)] = a
+        sim.set(f, c, a)
+    sim.sync_derived()
+    sim.step(1)
+    w = {}
+    lhs = 0.0
+    for f, c in blocks[dst]:
+        a = np.where(inner, rng.standard_normal(sim.shape), 0.0)
+        w[(f, c)] = a
+        lhs += float(np.sum(sim.get(f, c) * a))
+    sim.adjoint_reset()
+    for (f, c), a in w.items():
+        sim.set_adjoint(f, c, a)
+    sim.adjoint_step(0)
+    sim.adjoint_finish()
+    rhs = sum(float(np.sum(a * sim.get_adjoint(f, c))) for (f, c), a in v.items())
+    # scale of the terms summed: a mistake in a transposed coefficient moves rhs by one of them
+    terms = sum(float(np.sum(np.abs(a * sim.get_adjoint(f, c)))) for (f, c), a in v.items())
+    assert terms > 0 or lhs == 0
+    assert abs(lhs - rhs) <= 1e-12 * max(terms, 1e-300), (src, dst, lhs, rhs, terms)
+
+
+def _grad_scenario():
+    return synth.grid3d_adjoint(n=20, nsteps=140, design=(4, 4, 3), npml=4, monitor=8)
+
+
+def _objective(sc, eps):
+    sim = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    sim.set_design(sc["design"][0], eps)
+    sim.step(sc["nsteps"])
+    F = sim.get_dft(0)
+    return float(np.sum(F * F))
+
+
+def test_oracle_gradient_matches_central_differences():
+    sc = _grad_scenario()
+    box, eps = sc["design"]
+    sim = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    J, g = sim.adjoint_eps_gradient(sc["nsteps"], 0, 4)
+    assert J > 0
+    assert J == pytest.approx(_objective(sc, eps), rel=1e-13)
+    rng = np.random.default_rng(11)
+    flat = g.reshape(-1)
+    cells = rng.choice(flat.size, 8, replace=False)
+    h = 1e-3
+    fd = []
+    for q in cells:
+        ep = eps.copy().reshape(-1)
+        em = eps.copy().reshape(-1)
+        ep[q] += h
+        em[q] -= h
+        fd.append((_objective(sc, ep.reshape(eps.shape)) - _objective(sc, em.reshape(eps.shape))) / (2 * h))
+    fd = np.array(fd)
+    err = np.abs(flat[cells] - fd)
+    # central differences carry an O(h^2) error; the adjoint is exact to roundoff
+    assert np.max(err) <= 1e-5 * np.max(np.abs(fd)), (flat[cells], fd)
+
+
+def test_design_box_equals_material_table():
+    sc = synth.grid3d_cpml_tfsf_dft(n=18, nsteps=30)
+    box = [6, 12, 5, 11, 6, 10]
+    m = sc["matid"]
+    eps_tab = np.array([e for e, _, _ in sc["materials"]])
+    eps_box = eps_tab[m[box[4]:box[5], box[2]:box[3], box[0]:box[1]]]
+    a = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    b = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    b.set_design(box, eps_box)
+    a.step(30)
+    b.step(30)
+    for c in range(3):
+        assert np.array_equal(a.get(1, c), b.get(1, c))
+        assert np.array_equal(a.get(3, c), b.get(3, c))
