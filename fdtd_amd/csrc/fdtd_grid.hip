@@ -594,9 +594,10 @@ hipEvent_t next_event(fdtd_grid* g) {
 
 int collect_events(fdtd_grid* g) {
   if (!g->prof || g->ev_pairs.empty()) return 0;
-  HIP_TRY(hipStreamSynchronize(g->stream));
   for (auto& pr : g->ev_pairs) {
     float ms = 0;
+    // the pair may sit on another stream than the grid's (the adjoint's recompute stream)
+    HIP_TRY(hipEventSynchronize(g->ev_pool[pr.second + 1]));
     HIP_TRY(hipEventElapsedTime(&ms, g->ev_pool[pr.second], g->ev_pool[pr.second + 1]));
     g->kms[pr.first] += ms;
     g->kcount[pr.first] += 1;
@@ -809,10 +810,19 @@ void state_arrays(fdtd_grid* g, std::vector<std::pair<void*, size_t>>& out) {
 // Temporaries and forward-step controls of one adjoint call, undone on every exit.
 struct AdjointScope {
   fdtd_grid* g;
+  hipStream_t main;
   std::vector<void*> ptrs;
-  explicit AdjointScope(fdtd_grid* gg) : g(gg) {}
+  hipStream_t rs = nullptr;  // recompute stream
+  std::vector<hipEvent_t> evs;
+  explicit AdjointScope(fdtd_grid* gg) : g(gg), main(gg->stream) {}
   ~AdjointScope() {
+    g->stream = main;
     (void)hipStreamSynchronize(g->stream);
+    if (rs) {
+      (void)hipStreamSynchronize(rs);
+      (void)hipStreamDestroy(rs);
+    }
+    for (auto e : evs) (void)hipEventDestroy(e);
     for (void* p : ptrs) g->release(p);
     g->force_two_pass = false;
     g->recompute = false;
@@ -826,6 +836,11 @@ struct AdjointScope {
     if (!rc) ptrs.push_back(*p);
     return rc;
   }
+  int event(hipEvent_t* e) {
+    if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return fail(FDTD_EDEVICE, "hipEventCreate");
+    evs.push_back(*e);
+    return 0;
+  }
 };
 
 // Gradient of J = sum |F|^2 (monitor `didx`, E components in `mask`, all its
@@ -833,7 +848,8 @@ struct AdjointScope {
 // Segments of K steps: the design increments of the last R segments (as many
 // as `history_mb` holds, at least one) are recorded during the forward pass;
 // every earlier segment is checkpointed (E, H, psi) and recomputed from its
-// checkpoint, into the second state copy, just before its adjoint steps.
+// checkpoint, into the second state copy, while the next later segment's
+// adjoint steps run.
 template <typename T>
 int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, int mask, double* objective,
                  double* grad) {
@@ -958,37 +974,72 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   const int nzi = g->d.nz - 2;
   const dim3 block(t.bx, t.by, 1);
   const dim3 grd((g->d.nx - 2 + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, (nzi + t.kc - 1) / t.kc);
+
+  // Checkpointed segments are recomputed on a second stream, one segment
+  // ahead of the reverse steps on the grid's stream, into two alternating
+  // increment buffers: segment s-1 is recomputed while segment s is reversed.
+  // ev_r[b]: buffer b holds its segment; ev_a[b]: the reverse steps that read
+  // buffer b are done.
+  hipStream_t M = g->stream;
+  char* X[2] = {nullptr, nullptr};
+  hipEvent_t ev_fwd = nullptr, ev_r[2] = {nullptr, nullptr}, ev_a[2] = {nullptr, nullptr};
+  if (nck) {
+    if (hipStreamCreateWithFlags(&sc.rs, hipStreamNonBlocking) != hipSuccess)
+      return fail(FDTD_EDEVICE, "hipStreamCreateWithFlags failed");
+    for (int b = 0; b < 2; b++) {
+      if ((rc = sc.get((void**)&X[b], seg_bytes))) return rc;
+      if ((rc = sc.event(&ev_r[b])) || (rc = sc.event(&ev_a[b]))) return rc;
+    }
+    if ((rc = sc.event(&ev_fwd))) return rc;
+    HIP_TRY(hipEventRecord(ev_fwd, M));
+    HIP_TRY(hipStreamWaitEvent(sc.rs, ev_fwd, 0));
+  }
+  auto recompute = [&](int s) -> int {
+    const long n0 = long(s) * K;
+    const int len = int(std::min<long>(K, nsteps - n0));
+    const int b = s % 2;
+    if (s + 2 < nck) HIP_TRY(hipStreamWaitEvent(sc.rs, ev_a[b], 0));
+    g->stream = sc.rs;
+    swap_state(g);
+    int r = copy_state(ck + size_t(s) * st_bytes, false);
+    const long step_saved = g->step;
+    g->step = n0;
+    g->recompute = true;
+    g->hist = X[b];
+    g->hist_step0 = n0;
+    g->hist_rows = len;
+    if (!r) r = step_impl<T>(g, len);
+    g->step = step_saved;
+    g->recompute = false;
+    g->hist = nullptr;
+    g->hist_rows = 0;
+    swap_state(g);
+    g->stream = M;
+    if (r) return r;
+    HIP_TRY(hipEventRecord(ev_r[b], sc.rs));
+    return 0;
+  };
+  if (nck && (rc = recompute(nck - 1))) return rc;
   for (int s = S - 1; s >= 0; s--) {
     const long n0 = long(s) * K;
     const int len = int(std::min<long>(K, nsteps - n0));
-    char* hb = hist;
+    char* hb;
     if (s >= nck) {
       hb = hist + size_t(s - nck) * seg_bytes;
     } else {
-      // recompute this segment's increments from its checkpoint in the second copy
-      swap_state(g);
-      rc = copy_state(ck + size_t(s) * st_bytes, false);
-      const long step_saved = g->step;
-      g->step = n0;
-      g->recompute = true;
-      g->hist = hist;
-      g->hist_step0 = n0;
-      g->hist_rows = len;
-      if (!rc) rc = step_impl<T>(g, len);
-      g->step = step_saved;
-      g->recompute = false;
-      g->hist = nullptr;
-      g->hist_rows = 0;
-      swap_state(g);
-      if (rc) return rc;
+      hb = X[s % 2];
+      HIP_TRY(hipStreamWaitEvent(M, ev_r[s % 2], 0));
     }
+    if (s >= 1 && s < nck && (rc = recompute(s - 1))) return rc;
     for (long n = n0 + len - 1; n >= n0 && !rc; n--) {
       const T* hrow = (const T*)(hb + size_t(n - n0) * row_bytes);
       rc = launch(g, 5, fdtdk::adj_e_pass_3d<T>, grd, block, ap, qp, t.kc, n, hrow);
       if (!rc) rc = launch(g, 6, fdtdk::adj_h_pass_3d<T>, grd, block, ap, qp, t.kc);
     }
     if (rc) return rc;
+    if (s < nck) HIP_TRY(hipEventRecord(ev_a[s % 2], M));
   }
+  if (sc.rs) HIP_TRY(hipStreamSynchronize(sc.rs));
   HIP_TRY(hipStreamSynchronize(g->stream));
   HIP_TRY(hipMemcpy(grad, q.grad, size_t(nd) * 8, hipMemcpyDeviceToHost));
   *objective = J;

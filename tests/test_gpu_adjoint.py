@@ -52,7 +52,8 @@ def test_checkpoint_recompute_is_bitwise(precision):
     sc = synth.grid3d_adjoint(n=24, nsteps=150, design=(6, 5, 3), npml=4, monitor=8)
     print(f"[gpu-case] adjoint recompute {sc['name']} {precision}", flush=True)
     runs = []
-    for K, mb in [(150, 1e3), (13, 0.0), (13, 1e3), (40, 0.0)]:
+    # (K, history MB): all recorded, 11 of 12 segments recomputed, none, 2 of 4, and a few resident
+    for K, mb in [(150, 1e3), (13, 0.0), (13, 1e3), (40, 0.0), (13, 0.08)]:
         g = apply(_grid(sc, precision), sc)
         J, grad = g.adjoint_eps_gradient(sc["nsteps"], 0, 4, checkpoint_every=K, history_mb=mb)
         fields = [g.get(E, c) for c in range(3)] + [g.get(H, c) for c in range(3)]
