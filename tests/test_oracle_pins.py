@@ -1,0 +1,88 @@
+"""CPU pins of the oracle's build-defined pieces (the reference has no 3-D TFSF
+polarisation, no running DFT and no multi-pole weights), each against what the
+mathematics fixes rather than a retyped formula:
+
+* 3-D TFSF: with polH = pol x k (the consistent Yee plane wave of the aux
+  line's (Ez, Hy) pair) and an axis-aligned k, the scattered-field region of
+  an empty grid stays at round-off (leakage invariant), for +-x, y and z;
+* running DFT: every accumulator equals the brute-force sum of the stored
+  probe trace of that cell times the phase table;
+* multi-pole ADE: two identical poles of half weight step like one pole of
+  full weight (the polarisation equations are linear in the weight).
+"""
+import numpy as np
+import pytest
+
+import synth
+from oracle import refcpu
+from scenarios import E, H, apply, dft_phases, rel_l2
+
+
+def _sim(sc):
+    return apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+
+
+@pytest.mark.parametrize("kvec,pol", [
+    ((1, 0, 0), (0, 0, 1)), ((-1, 0, 0), (0, 0, 1)), ((0, 1, 0), (0, 0, 1)), ((0, 0, 1), (1, 0, 0)),
+])
+def test_tfsf_3d_leakage_invariant(kvec, pol):
+    sc = synth.grid3d_cpml_tfsf_dft(n=32, nsteps=90, nblocks=0, dft=False, inset=10)
+    sc["materials"] = [(1.0, 1.0, [])]
+    sc["matid"][:] = 0
+    sc["probes"] = []
+    k = np.array(kvec, float)
+    p = np.array(pol, float)
+    sc["tfsf"] = dict(sc["tfsf"], kvec=list(k), pol=list(p), polH=list(np.cross(p, k)))
+    sim = _sim(sc)
+    sim.step(sc["nsteps"])
+    box = sc["tfsf"]["box"]
+    n = sc["nx"]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    inside = ((x >= box[0]) & (x <= box[1]) & (y >= box[2]) & (y <= box[3]) & (z >= box[4]) & (z <= box[5]))
+    # scattered-field cells two or more cells outside the box and outside the CPML
+    lo, hi = sc["pml"]["n"] + 2, n - sc["pml"]["n"] - 3
+    far = ((x < box[0] - 1) | (x > box[1] + 1) | (y < box[2] - 1) | (y > box[3] + 1) | (z < box[4] - 1) |
+           (z > box[5] + 1))
+    body = (x >= lo) & (x <= hi) & (y >= lo) & (y <= hi) & (z >= lo) & (z <= hi)
+    for f in (E, H):
+        tot = max(np.max(np.abs(sim.get(f, c)[inside])) for c in range(3))
+        leak = max(np.max(np.abs(sim.get(f, c)[far & body])) for c in range(3))
+        assert tot > 0
+        assert leak <= 1e-12 * tot, (f, leak / tot)
+
+
+def test_running_dft_equals_bruteforce_of_probe_traces():
+    sc = synth.grid3d_cpml_tfsf_dft(n=20, nsteps=70, dft=False)
+    nsteps = sc["nsteps"]
+    freqs = np.array([0.7, 1.0, 1.3]) * 2.99792458e8 / (20 * sc["dx"])
+    pe, ph = dft_phases(nsteps, sc["dt"], freqs)
+    region = [7, 10, 8, 10, 9, 10]
+    sc["dfts"] = [(region, 0b100100, pe, ph)]  # Ez and Hz
+    cells = [(i, j, k) for k in range(region[4], region[5]) for j in range(region[2], region[3])
+             for i in range(region[0], region[1])]
+    sc["probes"] = [(E, 2, i, j, k) for i, j, k in cells] + [(H, 2, i, j, k) for i, j, k in cells]
+    sim = _sim(sc)
+    sim.step(nsteps)
+    tr = sim.probes()
+    assert tr.shape == (nsteps, 2 * len(cells))
+    F = sim.get_dft(0).reshape(2, len(freqs), 2, len(cells))
+    for slot, table in ((0, pe), (1, ph)):
+        trace = tr[:, slot * len(cells):(slot + 1) * len(cells)]
+        brute = np.einsum("nc,nfr->frc", trace, table)
+        assert rel_l2(F[slot], brute) <= 1e-13
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_split_pole_equals_single_pole(kind):
+    sc = synth.grid3d_dispersive_sphere(n=22, radius=5, nsteps=60, npml=4, nfreq=2)
+    w0 = 4e15 if kind == 2 else 1.37e16
+    one = dict(sc, materials=[(1.0, 1.0, []), (2.0, 1.0, [(kind, w0, 1e14, 1.4)])])
+    two = dict(sc, materials=[(1.0, 1.0, []), (2.0, 1.0, [(kind, w0, 1e14, 0.7), (kind, w0, 1e14, 0.7)])])
+    a, b = _sim(one), _sim(two)
+    a.step(60)
+    b.step(60)
+    for f in (E, H):
+        for c in range(3):
+            ref = a.get(f, c)
+            assert np.max(np.abs(ref)) > 0
+            assert rel_l2(b.get(f, c), ref) <= 1e-13
