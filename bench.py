@@ -291,6 +291,29 @@ def run_c5(args):
     return out
 
 
+def run_copy(args):
+    """Plain device copy of a 1 GiB FP32 buffer into another (torch): the calibration kernel for the HBM
+    counters (FETCH_SIZE / WRITE_SIZE of a known 1 GiB read + 1 GiB write) and the measured copy bandwidth."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    gbs = 2 * n * 4 * args.steps / (ms / 1e3) / 1e9
+    return base_line(args, round(gbs, 1), "GB/s", round(ms / args.steps, 5), "f32",
+                     {"workload": "device copy, 1 GiB FP32 -> 1 GiB FP32", "bytes_per_copy": 2 * n * 4})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,12 +324,14 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "copy"],
                     help="c2 (default): the headline 3-D 256^3 forward; c1: 2-D TM 256^2; c3: dispersive sphere; "
-                         "c4: 64 batched grids vs sequential; c5: adjoint eps gradient 128^3")
+                         "c4: 64 batched grids vs sequential; c5: adjoint eps gradient 128^3; copy: 1 GiB device copy")
+    ap.add_argument("--tile", default=None,
+                    help="bx,by,kc: fixed tile instead of the autotuner (c2; bx = 0 selects the fused pass)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"c4": 1500, "c5": 3000}.get(args.workload, 2000)
+        args.steps = {"c4": 1500, "c5": 3000, "copy": 20}.get(args.workload, 2000)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (z-slab sharding is a design "
               "note, not code); rerun with --gpus 1", file=sys.stderr)
@@ -319,14 +344,18 @@ def main():
 
     torch.cuda.set_device(0)
     if args.workload != "c2":
-        fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload]
+        fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "copy": run_copy}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     total = args.warmup + args.steps
     sc = synth.grid3d_cpml_tfsf_dft(n=args.size, nsteps=total)
     esz = 4 if args.precision == "f32" else 8
     g = apply(Grid("ThreeD", args.size, args.size, args.size, sc["dx"], sc["dt"], precision=args.precision), sc)
-    tile = g.autotune(trial_steps=10)
+    if args.tile:
+        tile = tuple(int(v) for v in args.tile.split(","))
+        g.set_tile(*tile)
+    else:
+        tile = g.autotune(trial_steps=10)
     g.step(args.warmup)
     g.synchronize()
     torch.cuda.synchronize()

@@ -22,18 +22,18 @@ namespace fdtdk {
 
 template <typename T>
 struct AdjArgs {
-  T* nu[3];          // ie * adjoint of E
-  T* mu[3];          // im * adjoint of H
-  T* lpsi[2][3][3];  // adjoint of psi[ft][I][J] (the forward slab layout)
-  double* grad;      // [ndes]: d J / d eps_r of the design cells
+  FDTD_G T* nu[3];          // ie * adjoint of E
+  FDTD_G T* mu[3];          // im * adjoint of H
+  FDTD_G T* lpsi[2][3][3];  // adjoint of psi[ft][I][J] (the forward slab layout)
+  FDTD_G double* grad;      // [ndes]: d J / d eps_r of the design cells
   double eps0;
   // objective J = sum over the monitor's cells, slots and frequencies of |F|^2
   int obj_r[6];
   int obj_comp[3];  // E component of objective slot s (-1: unused)
   int obj_nfreq;
   long obj_ncells;
-  const T* obj_w;        // [slot][nfreq][2][ncells]: 2 Re F, 2 Im F
-  const double* obj_ph;  // E phases of the monitor, [nphase][nfreq][2]
+  const FDTD_G T* obj_w;        // [slot][nfreq][2][ncells]: 2 Re F, 2 Im F
+  const FDTD_G double* obj_ph;  // E phases of the monitor, [nphase][nfreq][2]
   int obj_nphase;
 };
 
@@ -50,8 +50,8 @@ __device__ FDTD_COLD void adj_objective(const AdjArgs<T>& q, long step, int i, i
     if (comp < 0) continue;
     T acc = T(0);
     for (int f = 0; f < q.obj_nfreq; f++) {
-      const double* ph = q.obj_ph + 2L * (q.obj_nfreq * step + f);
-      const T* w = q.obj_w + ((long)slot * q.obj_nfreq + f) * 2 * q.obj_ncells + lq;
+      const FDTD_G double* ph = q.obj_ph + 2L * (q.obj_nfreq * step + f);
+      const FDTD_G T* w = q.obj_w + ((long)slot * q.obj_nfreq + f) * 2 * q.obj_ncells + lq;
       acc += w[0] * (T)ph[0] + w[q.obj_ncells] * (T)ph[1];
     }
     if (comp == 0)
@@ -66,15 +66,15 @@ __device__ FDTD_COLD void adj_objective(const AdjArgs<T>& q, long step, int i, i
 // Adjoint E pass of step `step` over one column of cells.  hist: the step's
 // row of design-cell E increments (null: no design box).
 template <typename T>
-__device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<T>& q, int i, int j, int k0, int k1,
-                                            long step, const T* __restrict__ hist) {
-  if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const T* __restrict__ M0 = q.mu[0];
-  const T* __restrict__ M1 = q.mu[1];
-  const T* __restrict__ M2 = q.mu[2];
-  T* __restrict__ N0 = q.nu[0];
-  T* __restrict__ N1 = q.nu[1];
-  T* __restrict__ N2 = q.nu[2];
+__device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<T>& q, const T* tie, int i, int j, int k0,
+                                            int k1, long step, const T* __restrict__ hist) {
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
+  const FDTD_G T* __restrict__ M0 = q.mu[0];
+  const FDTD_G T* __restrict__ M1 = q.mu[1];
+  const FDTD_G T* __restrict__ M2 = q.mu[2];
+  FDTD_G T* __restrict__ N0 = q.nu[0];
+  FDTD_G T* __restrict__ N1 = q.nu[1];
+  FDTD_G T* __restrict__ N2 = q.nu[2];
   const long sy = a.sy, sz = a.sz;
   const T fx = a.cf[1][0][i], fy = a.cf[1][1][j];
   const T fxm = a.cf[1][0][i - 1], fym = a.cf[1][1][j - 1];
@@ -85,7 +85,7 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
     const T m2_xm = M2[c - 1], m0_ym = M0[c - sy], m1_xm = M1[c - 1];
     T n0 = N0[c], n1 = N1[c], n2 = N2[c];
     const int mid = a.mat[c];
-    const T ie = cell_inv_eps(a, mid, i, j, k);
+    const T ie = cell_inv_eps(a, tie, mid, i, j, k);
     const T fz = a.cf[1][2][k], fzm = a.cf[1][2][k - 1];
     // w_KJ at the cell and at its lower J neighbour
     T w12 = fz * m1, w12m = fzm * m1_zm;
@@ -167,14 +167,15 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
 
 // Adjoint H pass of one step over one column of cells.
 template <typename T>
-__device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<T>& q, int i, int j, int k0, int k1) {
-  if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const T* __restrict__ N0 = q.nu[0];
-  const T* __restrict__ N1 = q.nu[1];
-  const T* __restrict__ N2 = q.nu[2];
-  T* __restrict__ M0 = q.mu[0];
-  T* __restrict__ M1 = q.mu[1];
-  T* __restrict__ M2 = q.mu[2];
+__device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<T>& q, const T* tim, int i, int j, int k0,
+                                            int k1) {
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
+  const FDTD_G T* __restrict__ N0 = q.nu[0];
+  const FDTD_G T* __restrict__ N1 = q.nu[1];
+  const FDTD_G T* __restrict__ N2 = q.nu[2];
+  FDTD_G T* __restrict__ M0 = q.mu[0];
+  FDTD_G T* __restrict__ M1 = q.mu[1];
+  FDTD_G T* __restrict__ M2 = q.mu[2];
   const long sy = a.sy, sz = a.sz;
   const T fx = a.cf[0][0][i], fy = a.cf[0][1][j];
   const T fxp = a.cf[0][0][i + 1], fyp = a.cf[0][1][j + 1];
@@ -184,7 +185,7 @@ __device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<
     const T n1_zp = N1[c + sz], n2_yp = N2[c + sy], n0_zp = N0[c + sz];
     const T n2_xp = N2[c + 1], n0_yp = N0[c + sy], n1_xp = N1[c + 1];
     const T h0 = M0[c], h1 = M1[c], h2 = M2[c];
-    const T im = a.mim[a.mat[c]];
+    const T im = tim[a.mat[c]];
     const T fz = a.cf[0][2][k], fzp = a.cf[0][2][k + 1];
     // v_IJ at the cell and at its upper J neighbour
     T v12 = fz * n1, v12p = fzp * n1_zp;
@@ -251,22 +252,30 @@ __device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<
   }
 }
 
+// launched like the forward passes: ntx x nty x nkc tiles, XCD-aware order,
+// the H pass walking each XCD's range in reverse
 template <typename T>
 __global__ __launch_bounds__(256) void adj_e_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
-                                                     int kc, long step, const T* __restrict__ hist) {
-  const int k0 = 1 + blockIdx.z * kc;
-  adj_e_cells(*ap, *qp, 1 + blockIdx.x * blockDim.x + threadIdx.x, 1 + blockIdx.y * blockDim.y + threadIdx.y, k0,
+                                                     int kc, int ntx, int nty, int nkc, long step,
+                                                     const T* __restrict__ hist) {
+  __shared__ MatLds<T> sm;
+  stage_materials(*ap, sm);
+  const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+  const int k0 = 1 + bz * kc;
+  adj_e_cells(*ap, *qp, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
               min(k0 + kc, ap->nz - 1), step, hist);
 }
 
-// reverse workgroup order, as the forward H pass (the first tiles read what
-// the adjoint E pass touched last)
 template <typename T>
 __global__ __launch_bounds__(256) void adj_h_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
-                                                     int kc) {
-  const int bxr = gridDim.x - 1 - blockIdx.x, byr = gridDim.y - 1 - blockIdx.y, bzr = gridDim.z - 1 - blockIdx.z;
-  const int k0 = 1 + bzr * kc;
-  adj_h_cells(*ap, *qp, 1 + bxr * blockDim.x + threadIdx.x, 1 + byr * blockDim.y + threadIdx.y, k0,
+                                                     int kc, int ntx, int nty, int nkc) {
+  __shared__ MatLds<T> sm;
+  stage_materials(*ap, sm);
+  const int t = xcd_tile_rev(blockIdx.x, ntx * nty * nkc);
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+  const int k0 = 1 + bz * kc;
+  adj_h_cells(*ap, *qp, sm.im, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
               min(k0 + kc, ap->nz - 1));
 }
 

@@ -111,7 +111,9 @@ constexpr int kChunk = 256;  // steps per incident-table chunk
 constexpr int kNumKernels = 7;
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
-// bx == 0 selects the fused single-pass schedule with `by` rows per workgroup
+// bx == 0 selects the LDS fused single pass with `by` rows per workgroup,
+// bx == 1 the wave-fused single pass with `by` rows per wave; bx >= 2 the
+// two passes with bx x by threads per workgroup
 struct Tile {
   int bx, by, kc;
 };
@@ -271,6 +273,13 @@ struct fdtd_grid {
 };
 
 namespace {
+
+// Store a host-side device pointer into a kernel-argument field (those fields
+// are global-address-space pointers in the device compile).
+template <typename D, typename S>
+inline void gput(D& dst, S src) {
+  dst = (D)src;
+}
 
 // host -> device array of T from doubles
 int upload_T(fdtd_grid* g, void* dst, const double* src, size_t n) {
@@ -448,24 +457,24 @@ int finalize(fdtd_grid* g) {
 template <typename T>
 void set_state(fdtd_grid* g, fdtdk::StepArgs<T>& a, bool pingpong) {
   for (int c = 0; c < 3; c++) {
-    a.E[c] = (T*)g->E[c];
-    a.H[c] = (T*)g->H[c];
-    a.Eo[c] = (T*)(pingpong ? g->E2[c] : g->E[c]);
-    a.Ho[c] = (T*)(pingpong ? g->H2[c] : g->H[c]);
-    a.Dd[c] = (T*)g->Dd[c];
-    a.Ddo[c] = (T*)(pingpong ? g->Dd2[c] : g->Dd[c]);
+    gput(a.E[c], (T*)g->E[c]);
+    gput(a.H[c], (T*)g->H[c]);
+    gput(a.Eo[c], (T*)(pingpong ? g->E2[c] : g->E[c]));
+    gput(a.Ho[c], (T*)(pingpong ? g->H2[c] : g->H[c]));
+    gput(a.Dd[c], (T*)g->Dd[c]);
+    gput(a.Ddo[c], (T*)(pingpong ? g->Dd2[c] : g->Dd[c]));
     for (int p = 0; p < fdtdk::kMaxPoles; p++) {
-      a.Pd[p][c] = (T*)g->Pd[p][c];
-      a.Jd[p][c] = (T*)g->Jd[p][c];
-      a.Pdo[p][c] = (T*)(pingpong ? g->Pd2[p][c] : g->Pd[p][c]);
-      a.Jdo[p][c] = (T*)(pingpong ? g->Jd2[p][c] : g->Jd[p][c]);
+      gput(a.Pd[p][c], (T*)g->Pd[p][c]);
+      gput(a.Jd[p][c], (T*)g->Jd[p][c]);
+      gput(a.Pdo[p][c], (T*)(pingpong ? g->Pd2[p][c] : g->Pd[p][c]));
+      gput(a.Jdo[p][c], (T*)(pingpong ? g->Jd2[p][c] : g->Jd[p][c]));
     }
   }
   for (int ft = 0; ft < 2; ft++)
     for (int I = 0; I < 3; I++)
       for (int J = 0; J < 3; J++) {
-        a.psi[ft][I][J] = (T*)g->psi[ft][I][J];
-        a.psio[ft][I][J] = (T*)(pingpong ? g->psi2[ft][I][J] : g->psi[ft][I][J]);
+        gput(a.psi[ft][I][J], (T*)g->psi[ft][I][J]);
+        gput(a.psio[ft][I][J], (T*)(pingpong ? g->psi2[ft][I][J] : g->psi[ft][I][J]));
       }
 }
 
@@ -517,16 +526,16 @@ template <typename T>
 void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   memset(&a, 0, sizeof a);
   set_state(g, a, false);
-  a.mat = g->mat;
-  a.mats = (const fdtdk::MatDev<T>*)g->mats_dev;
-  a.mie = (const T*)g->mie_dev;
-  a.mim = (const T*)g->mim_dev;
+  gput(a.mat, g->mat);
+  gput(a.mats, (const fdtdk::MatDev<T>*)g->mats_dev);
+  gput(a.mie, (const T*)g->mie_dev);
+  gput(a.mim, (const T*)g->mim_dev);
   for (int ft = 0; ft < 2; ft++)
     for (int I = 0; I < 3; I++) {
-      a.cf[ft][I] = (const T*)g->coef[ft][I][0];
-      a.cg[ft][I] = (const T*)g->coef[ft][I][1];
-      a.cb[ft][I] = (const T*)g->coef[ft][I][2];
-      a.cc[ft][I] = (const T*)g->coef[ft][I][3];
+      gput(a.cf[ft][I], (const T*)g->coef[ft][I][0]);
+      gput(a.cg[ft][I], (const T*)g->coef[ft][I][1]);
+      gput(a.cb[ft][I], (const T*)g->coef[ft][I][2]);
+      gput(a.cc[ft][I], (const T*)g->coef[ft][I][3]);
     }
   for (int x = 0; x < 3; x++) {
     a.slab_a[x] = g->ndim > x ? g->slab_a[x] : 0;
@@ -561,19 +570,24 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     dd.nfreq = h.nfreq;
     dd.ncomp = h.ncomp;
     dd.ncells = h.ncells;
-    dd.acc = (T*)h.acc;
-    dd.phE = h.phE;
-    dd.phH = h.phH;
+    gput(dd.acc, (T*)h.acc);
+    gput(dd.phE, h.phE);
+    gput(dd.phH, h.phH);
     dd.nphase = h.nphase;
     dd.base = a.dft_work;
     a.dft_work += long(h.nfreq) * h.ncells;
   }
   a.nprobe = g->nprobe;
   for (int p = 0; p < g->nprobe; p++) a.probe[p] = g->probe[p];
-  a.trace = g->trace;
-  a.dsg_ie = (const T*)g->dsg_ie_dev;
+  gput(a.trace, g->trace);
+  // byte sizes of one (padded) field array and of the id volume, for the
+  // buffer-addressed wave-fused pass (0 when they do not fit 31 bits)
+  const size_t fb = size_t(g->ncell + fdtdk::field_pad(g->d.nx)) * sizeof(T);
+  a.field_bytes = fb < (size_t(1) << 31) ? unsigned(fb) : 0u;
+  a.mat_bytes = a.field_bytes ? unsigned(g->ncell + fdtdk::field_pad(g->d.nx)) : 0u;
+  gput(a.dsg_ie, (const T*)g->dsg_ie_dev);
   for (int x = 0; x < 6; x++) a.dsg[x] = g->dsg[x];
-  a.hist = (T*)g->hist;
+  gput(a.hist, (T*)g->hist);
   a.hist_step0 = g->hist_step0;
   a.hist_rows = g->hist_rows;
   if (g->recompute) {
@@ -688,8 +702,9 @@ int step_impl(fdtd_grid* g, int nsteps) {
   fill_args<T>(g, a);
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (g->ndim == 2) t.kc = 1;
-  if (g->force_two_pass && t.bx == 0) t = kDefaultTile3D;
-  const bool fused = g->ndim == 3 && t.bx == 0;
+  if (g->force_two_pass && t.bx <= 1) t = kDefaultTile3D;
+  const bool wavef = g->ndim == 3 && t.bx == 1;
+  const bool fused = g->ndim == 3 && t.bx <= 1;
   if (fused) {
     int rc = ensure_pingpong(g);
     if (rc) return rc;
@@ -697,27 +712,39 @@ int step_impl(fdtd_grid* g, int nsteps) {
   const int nzi = g->ndim == 3 ? g->d.nz - 2 : 1;
   dim3 block, grid;
   int ntx = 1, nty = 1, nkc = 1;
-  if (fused) {
+  if (wavef) {
+    ntx = (g->d.nx - 2 + fdtdk::kWaveCols - 1) / fdtdk::kWaveCols;
+    nty = (g->d.ny - 2 + fdtdk::kWavesY * t.by - 1) / (fdtdk::kWavesY * t.by);
+    nkc = (nzi + t.kc - 1) / t.kc;
+    block = dim3(64, fdtdk::kWavesY, 1);
+    grid = dim3(ntx * nty * nkc, 1, 1);
+  } else if (fused) {
     ntx = (g->d.nx - 2 + fdtdk::kFusedCols - 1) / fdtdk::kFusedCols;
     nty = (g->d.ny - 2 + t.by - 1) / t.by;
     nkc = (nzi + t.kc - 1) / t.kc;
     block = dim3(256, t.by + 1, 1);
     grid = dim3(ntx * nty * nkc, 1, 1);
+  } else if (g->ndim == 3) {
+    ntx = (g->d.nx + t.bx - 1) / t.bx;
+    nty = (g->d.ny - 2 + t.by - 1) / t.by;
+    nkc = (nzi + t.kc - 1) / t.kc;
+    block = dim3(t.bx, t.by, 1);
+    grid = dim3(ntx * nty * nkc, 1, 1);
   } else {
     block = dim3(t.bx, t.by, 1);
-    grid = dim3((g->d.nx - 2 + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, (nzi + t.kc - 1) / t.kc);
+    grid = dim3((g->d.nx + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, 1);
   }
   int done = 0;
   while (done < nsteps) {
     const int chunk = std::min(kChunk, nsteps - done);
     a.step0 = g->step;
-    a.incE = g->incE;
-    a.incH = g->incH;
+    gput(a.incE, g->incE);
+    gput(a.incH, g->incH);
     if (g->tfsf && g->inc_rows) {
       // incident rows computed ahead for every step (adjoint forward and recompute)
       if (g->step + chunk > g->inc_rows) return fail(FDTD_ESTATE, "step beyond the precomputed incident rows");
-      a.incE = g->inc_allE + g->step * g->auxL;
-      a.incH = g->inc_allH + g->step * g->auxL;
+      gput(a.incE, g->inc_allE + g->step * g->auxL);
+      gput(a.incH, g->inc_allH + g->step * g->auxL);
     } else if (g->tfsf) {
       int rc = run_aux(g, g->step, chunk);
       if (rc) return rc;
@@ -745,7 +772,15 @@ int step_impl(fdtd_grid* g, int nsteps) {
     for (int s = 0; s < chunk; s++) {
       int rc;
       const auto* ap = (fused && (s % 2)) ? dev1 : dev0;
-      if (fused) {
+      if (wavef) {
+        if (t.by == 2)
+          rc = launch(g, 3, fdtdk::fusedw_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        else if (t.by == 4)
+          rc = launch(g, 3, fdtdk::fusedw_3d<T, 4>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        else
+          rc = launch(g, 3, fdtdk::fusedw_3d<T, 6>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        swap_state(g);
+      } else if (fused) {
         if (t.by == 1)
           rc = launch(g, 3, fdtdk::fused_3d<T, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         else if (t.by == 2)
@@ -754,8 +789,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
           rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         swap_state(g);
       } else if (g->ndim == 3) {
-        rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s);
-        if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s);
+        rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+        if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
       } else {
         rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
         if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
@@ -925,7 +960,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   for (int x = 0; x < 6; x++) q.obj_r[x] = d.r[x];
   q.obj_nfreq = d.nfreq;
   q.obj_ncells = d.ncells;
-  q.obj_ph = d.phE;
+  gput(q.obj_ph, d.phE);
   q.obj_nphase = d.nphase;
   double J = 0;
   std::vector<double> w;
@@ -944,7 +979,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   void* wdev = nullptr;
   if ((rc = sc.get(&wdev, w.size() * g->esz))) return rc;
   if ((rc = upload_T(g, wdev, w.data(), w.size()))) return rc;
-  q.obj_w = (const T*)wdev;
+  gput(q.obj_w, (const T*)wdev);
 
   // adjoint state (zero: the adjoint of the state after the last step)
   for (int c = 0; c < 3; c++) {
@@ -970,10 +1005,11 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   const auto* ap = (const fdtdk::StepArgs<T>*)adev;
   const auto* qp = (const fdtdk::AdjArgs<T>*)qdev;
 
-  const Tile t = (g->tile.by && g->tile.bx > 0) ? g->tile : kDefaultTile3D;
+  const Tile t = (g->tile.by && g->tile.bx >= 2) ? g->tile : kDefaultTile3D;
   const int nzi = g->d.nz - 2;
   const dim3 block(t.bx, t.by, 1);
-  const dim3 grd((g->d.nx - 2 + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, (nzi + t.kc - 1) / t.kc);
+  const int ntx = (g->d.nx + t.bx - 1) / t.bx, nty = (g->d.ny - 2 + t.by - 1) / t.by, nkc = (nzi + t.kc - 1) / t.kc;
+  const dim3 grd(ntx * nty * nkc, 1, 1);
 
   // Checkpointed segments are recomputed on a second stream, one segment
   // ahead of the reverse steps on the grid's stream, into two alternating
@@ -1033,8 +1069,8 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
     if (s >= 1 && s < nck && (rc = recompute(s - 1))) return rc;
     for (long n = n0 + len - 1; n >= n0 && !rc; n--) {
       const T* hrow = (const T*)(hb + size_t(n - n0) * row_bytes);
-      rc = launch(g, 5, fdtdk::adj_e_pass_3d<T>, grd, block, ap, qp, t.kc, n, hrow);
-      if (!rc) rc = launch(g, 6, fdtdk::adj_h_pass_3d<T>, grd, block, ap, qp, t.kc);
+      rc = launch(g, 5, fdtdk::adj_e_pass_3d<T>, grd, block, ap, qp, t.kc, ntx, nty, nkc, n, hrow);
+      if (!rc) rc = launch(g, 6, fdtdk::adj_h_pass_3d<T>, grd, block, ap, qp, t.kc, ntx, nty, nkc);
     }
     if (rc) return rc;
     if (s < nck) HIP_TRY(hipEventRecord(ev_a[s % 2], M));
@@ -1081,10 +1117,10 @@ int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   }
   int rc = 0;
   for (int c = 0; c < 3 && !rc; c++) {
-    if (g->actE[c]) rc = g->alloc(&g->E[c], g->ncell * g->esz);
-    if (!rc && g->actH[c]) rc = g->alloc(&g->H[c], g->ncell * g->esz);
+    if (g->actE[c]) rc = g->alloc(&g->E[c], (g->ncell + fdtdk::field_pad(desc->nx)) * g->esz);
+    if (!rc && g->actH[c]) rc = g->alloc(&g->H[c], (g->ncell + fdtdk::field_pad(desc->nx)) * g->esz);
   }
-  if (!rc) rc = g->alloc((void**)&g->mat, g->ncell);
+  if (!rc) rc = g->alloc((void**)&g->mat, g->ncell + fdtdk::field_pad(desc->nx));
   g->mat_host.assign(g->ncell, 0);
   if (!rc) rc = g->alloc(&g->mats_dev, sizeof(fdtdk::MatDev<double>) * FDTD_MAX_MATERIALS);
   if (!rc) rc = g->alloc(&g->mie_dev, 8 * FDTD_MAX_MATERIALS);
@@ -1190,7 +1226,7 @@ int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, cons
   s.cell = g->q(i, j, k);
   s.field = field;
   s.comp = comp;
-  s.table = t;
+  gput(s.table, t);
   s.n = ntable;
   return 0;
 }
@@ -1374,6 +1410,9 @@ int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
   if (bx == 0) {
     if (g->ndim != 3 || by < 1 || by > 3 || kc <= 0)
       return fail(FDTD_EINVAL, "the fused schedule (bx = 0) is 3-D only, with by in {1, 2, 3} rows and kc >= 1");
+  } else if (bx == 1) {
+    if (g->ndim != 3 || (by != 2 && by != 4 && by != 6) || kc <= 0)
+      return fail(FDTD_EINVAL, "the wave-fused schedule (bx = 1) is 3-D only, with by in {2, 4, 6} rows and kc >= 1");
   } else if (bx < 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0) {
     return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
   }
@@ -1386,7 +1425,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
   static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {128, 2, 1}, {128, 2, 2}, {64, 4, 1},
-                               {0, 1, 16},  {0, 2, 8},   {0, 2, 16},  {0, 3, 8},   {0, 3, 16},  {0, 3, 32}};
+                               {0, 2, 16},  {0, 3, 16},  {1, 2, 16},  {1, 2, 32},  {1, 4, 8},   {1, 4, 16},
+                               {1, 4, 32},  {1, 6, 16}};
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;
   const int ncand = g->ndim == 3 ? int(sizeof cand3 / sizeof cand3[0]) : int(sizeof cand2 / sizeof cand2[0]);
@@ -1611,8 +1651,8 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
       fdtdk::StepArgs<T> a;
       fill_args<T>(g, a);
       a.step0 = g->step;
-      a.incE = g->incE;
-      a.incH = g->incH;
+      gput(a.incE, g->incE);
+      gput(a.incH, g->incH);
       host[q] = a;
     }
     auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + slot * n * sizeof(fdtdk::StepArgs<T>));
@@ -1683,7 +1723,7 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
     b->saved_streams.push_back(g->stream);
     g->stream = b->stream;
     if (!g->finalized) rc = finalize(g);
-    const int tx = (g->d.nx - 2 + b->tile.bx - 1) / b->tile.bx, ty = (g->d.ny - 2 + b->tile.by - 1) / b->tile.by;
+    const int tx = (g->d.nx + b->tile.bx - 1) / b->tile.bx, ty = (g->d.ny - 2 + b->tile.by - 1) / b->tile.by;
     const int tz = (g->d.nz - 2 + b->tile.kc - 1) / b->tile.kc;
     for (int z = 0; z < tz; z++)
       for (int y = 0; y < ty; y++)

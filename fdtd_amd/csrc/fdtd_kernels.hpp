@@ -25,6 +25,17 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Pointers that kernels read out of argument blocks in device memory are
+// declared in the global address space for the device compile, so every
+// access through them is a global load/store (not a flat one, which would
+// also tie up the LDS counter and force full waits); the host compile sees
+// plain pointers of the same size and layout.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FDTD_G __attribute__((address_space(1)))
+#else
+#define FDTD_G
+#endif
+
 namespace fdtdk {
 
 constexpr int kMaxPoles = 3;
@@ -44,7 +55,7 @@ struct MatDev {
 struct SrcDev {
   long cell;
   int field, comp;
-  const double* table;
+  const FDTD_G double* table;
   int n;
 };
 
@@ -53,9 +64,9 @@ struct DftDev {
   int r[6];
   int mask, nfreq, ncomp;
   long ncells;
-  T* acc;             // [ncomp][nfreq][2][ncells]
-  const double* phE;  // [nphase][nfreq][2]
-  const double* phH;
+  FDTD_G T* acc;             // [ncomp][nfreq][2][ncells]
+  const FDTD_G double* phE;  // [nphase][nfreq][2]
+  const FDTD_G double* phH;
   int nphase;
   long base;  // offset of this monitor's (frequency, cell) items in the DFT launch
 };
@@ -67,32 +78,32 @@ struct ProbeDev {
 
 template <typename T>
 struct StepArgs {
-  T* E[3];   // state read by the step
-  T* H[3];
-  T* Eo[3];  // state written by the step (the same arrays for in-place passes)
-  T* Ho[3];
-  const uint8_t* mat;
-  const MatDev<T>* mats;
-  const T* mie;  // 1/(eps_r eps0) per material id, negated for dispersive materials
-  const T* mim;  // 1/(mu_r mu0) per material id
+  FDTD_G T* E[3];   // state read by the step
+  FDTD_G T* H[3];
+  FDTD_G T* Eo[3];  // state written by the step (the same arrays for in-place passes)
+  FDTD_G T* Ho[3];
+  const FDTD_G uint8_t* mat;
+  const FDTD_G MatDev<T>* mats;
+  const FDTD_G T* mie;  // 1/(eps_r eps0) per material id, negated for dispersive materials
+  const FDTD_G T* mim;  // 1/(mu_r mu0) per material id
   // CPML: psi[ft][I][J] slab arrays (read / written) and per-axis profiles
-  T* psi[2][3][3];
-  T* psio[2][3][3];
-  const T* cf[2][3];  // dt F / dx
-  const T* cg[2][3];  // dt G
-  const T* cb[2][3];  // B
-  const T* cc[2][3];  // C / dx
+  FDTD_G T* psi[2][3][3];
+  FDTD_G T* psio[2][3][3];
+  const FDTD_G T* cf[2][3];  // dt F / dx
+  const FDTD_G T* cg[2][3];  // dt G
+  const FDTD_G T* cb[2][3];  // B
+  const FDTD_G T* cc[2][3];  // C / dx
   int slab_a[3], slab_b[3], slab_S[3];
   int nx, ny, nz;
   long sy, sz;
   // dispersive box (dense): [x0,x1)x[y0,y1)x[z0,z1), read / written
   int dbox[6];
-  T* Dd[3];
-  T* Pd[kMaxPoles][3];
-  T* Jd[kMaxPoles][3];
-  T* Ddo[3];
-  T* Pdo[kMaxPoles][3];
-  T* Jdo[kMaxPoles][3];
+  FDTD_G T* Dd[3];
+  FDTD_G T* Pd[kMaxPoles][3];
+  FDTD_G T* Jd[kMaxPoles][3];
+  FDTD_G T* Ddo[3];
+  FDTD_G T* Pdo[kMaxPoles][3];
+  FDTD_G T* Jdo[kMaxPoles][3];
   T dt;
   // soft sources
   int nsrc;
@@ -102,8 +113,8 @@ struct StepArgs {
   int box[6];
   double kvec[3], pol[3], polH[3], origin[3], dx;
   int auxL;
-  const double* incE;  // aux Ez after each step's aux advance
-  const double* incH;  // aux Hy before it
+  const FDTD_G double* incE;  // aux Ez after each step's aux advance
+  const FDTD_G double* incH;  // aux Hy before it
   // running DFT
   int ndft;
   long dft_work;  // sum over monitors of nfreq x cells
@@ -111,17 +122,23 @@ struct StepArgs {
   // probes
   int nprobe;
   ProbeDev probe[kMaxProbe];
-  double* trace;  // [step][nprobe]
+  FDTD_G double* trace;  // [step][nprobe]
   long step0;     // global step of row 0 of the incident rows
   // design box [x0,x1)x[y0,y1)x[z0,z1): per-cell 1/(eps_r eps0) replaces the
   // material table's value there (null when the grid has no design box)
-  const T* dsg_ie;
+  const FDTD_G T* dsg_ie;
   int dsg[6];
   // E increment of the design cells, one row [3][ndes] per step of
   // [hist_step0, hist_step0 + hist_rows) (null = not recorded)
-  T* hist;
+  FDTD_G T* hist;
   long hist_step0, hist_rows;
+  unsigned field_bytes, mat_bytes;  // padded array sizes for buffer addressing (0: too large)
 };
+
+// Extra elements allocated after every field array and the id volume: the
+// wave-fused pass reads whole halo rows and columns of its last tiles
+// (at most 62 columns past nx and 7 rows past ny in the last plane).
+__host__ __device__ constexpr long field_pad(int nx) { return 8L * nx + 128; }
 
 // local index of (i,j,k) in the design box, -1 outside
 template <typename T>
@@ -133,9 +150,9 @@ __device__ __forceinline__ long design_index(const StepArgs<T>& a, int i, int j,
 }
 
 // 1/(eps_r eps0) of a cell: the design value inside the design box, else the material's
-template <typename T>
-__device__ __forceinline__ T cell_inv_eps(const StepArgs<T>& a, int mid, int i, int j, int k) {
-  T ie = a.mie[mid];
+template <typename T, typename Tab>
+__device__ __forceinline__ T cell_inv_eps(const StepArgs<T>& a, const Tab& tab, int mid, int i, int j, int k) {
+  T ie = tab[mid];
   if (a.dsg_ie) {
     const long l = design_index(a, i, j, k);
     if (l >= 0) ie = a.dsg_ie[l];
@@ -155,7 +172,7 @@ __device__ __forceinline__ long slab_off(const StepArgs<T>& a, int J, int s, int
 
 // Aux1D::interpolateE / interpolateH (include/Source.hpp:442-458) on a row of
 // the incident table.
-__device__ __forceinline__ double interp_row(const double* row, int L, double loc, double dx, bool h) {
+__device__ __forceinline__ double interp_row(const FDTD_G double* row, int L, double loc, double dx, bool h) {
   double nd = (double)L / 2.0 + loc / dx;
   if (h) nd -= 0.5;
   long n = (long)nd;
@@ -184,7 +201,7 @@ __device__ __forceinline__ double lc3(int I, int J) {
 #endif
 
 template <typename T, int NDIM>
-__device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a, const double* row, int ft, int i, int j, int k, const bool actF[3],
+__device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a, const FDTD_G double* row, int ft, int i, int j, int k, const bool actF[3],
                           const bool actC[3], T delta[3]) {
   const int ci[3] = {i, j, k};
   for (int J = 0; J < NDIM; J++) {
@@ -301,11 +318,9 @@ struct EStencil {
 // New E of interior cell (i,j,k).  psi and the dispersive state are read from
 // the current buffers and, when `own`, written to the next buffers.
 template <typename T>
-__device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const double* incH, int i, int j, int k,
-                                          long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, bool own,
-                                          T out[3]) {
-  const T ie = cell_inv_eps(a, mid, i, j, k);
-  const T fx = a.cf[0][0][i], fy = a.cf[0][1][j], fz = a.cf[0][2][k];
+__device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incH, int i, int j, int k,
+                                          long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy,
+                                          T fz, bool own, T out[3]) {
   const T dxy = h.hz - h.hz_ym;  // Ex, J=y, +1
   const T dxz = h.hy - h.hy_zm;  // Ex, J=z, -1
   const T dyz = h.hx - h.hx_zm;  // Ey, J=z, +1
@@ -372,10 +387,9 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
 
 // ---------------------------------------------------------------- H of one cell
 template <typename T>
-__device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const double* incE, int i, int j, int k,
-                                          long c, const EStencil<T>& e, T hx, T hy, T hz, int mid, T out[3]) {
-  const T im = a.mim[mid];
-  const T fx = a.cf[1][0][i], fy = a.cf[1][1][j], fz = a.cf[1][2][k];
+__device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incE, int i, int j, int k,
+                                          long c, const EStencil<T>& e, T hx, T hy, T hz, T im, T fx, T fy, T fz,
+                                          T out[3]) {
   const T dxy = e.ez_yp - e.ez;  // Hx, J=y, +1
   const T dxz = e.ey_zp - e.ey;  // Hx, J=z, -1
   const T dyz = e.ex_zp - e.ex;  // Hy, J=z, +1
@@ -459,16 +473,16 @@ __device__ __forceinline__ void dft_item(const StepArgs<T>& a, long g, long step
   const int j = d.r[2] + (int)((lq / wx) % wy);
   const int k = d.r[4] + (int)(lq / ((long)wx * wy));
   const long c = (long)i + a.sy * (long)j + a.sz * (long)k;
-  const double* pe = d.phE + 2L * (d.nfreq * step + f);
-  const double* ph = d.phH + 2L * (d.nfreq * step + f);
+  const FDTD_G double* pe = d.phE + 2L * (d.nfreq * step + f);
+  const FDTD_G double* ph = d.phH + 2L * (d.nfreq * step + f);
   const T ce = (T)pe[0], se = (T)pe[1], chh = (T)ph[0], sh = (T)ph[1];
   int slot = 0;
 #pragma unroll
   for (int comp = 0; comp < 6; comp++) {
     if (!((d.mask >> comp) & 1)) continue;
-    const T* src = comp < 3 ? a.Eo[comp] : a.Ho[comp - 3];
+    const FDTD_G T* src = comp < 3 ? a.Eo[comp] : a.Ho[comp - 3];
     const T v = src ? src[c] : T(0);
-    T* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
+    FDTD_G T* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
     re[0] += v * (comp < 3 ? ce : chh);
     re[d.ncells] += v * (comp < 3 ? se : sh);
     slot++;
@@ -508,27 +522,64 @@ __device__ FDTD_COLD void record_increment(const StepArgs<T>& a, long step, int 
   const long l = design_index(a, i, j, k);
   if (l < 0) return;
   const long nd = (long)(a.dsg[1] - a.dsg[0]) * (a.dsg[3] - a.dsg[2]) * (a.dsg[5] - a.dsg[4]);
-  T* r = a.hist + row * 3 * nd + l;
+  FDTD_G T* r = a.hist + row * 3 * nd + l;
   r[0] = d0;
   r[nd] = d1;
   r[2 * nd] = d2;
+}
+
+// Workgroup -> tile, XCD-aware.  The dispatcher deals consecutive workgroups
+// round-robin to the 8 XCDs, each with its own L2; this bijection on [0, nb)
+// gives XCD x one contiguous range of tiles instead, so the neighbour rows
+// and planes a tile reads were mostly loaded by the same XCD.  The reverse
+// form walks each XCD's range from its end (the H pass starts where the E
+// pass ended, on the same XCD).
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb / 8, r = nb % 8;
+  const int x = bid % 8, loc = bid / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+}
+__device__ __forceinline__ int xcd_tile_rev(int bid, int nb) {
+  const int q = nb / 8, r = nb % 8;
+  const int x = bid % 8, loc = bid / 8;
+  const int cnt = q + (x < r ? 1 : 0);
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (cnt - 1 - loc);
+}
+
+// The 256-entry material inverses, staged in LDS once per workgroup: the
+// per-cell lookup by material id then costs an LDS read instead of a global
+// load that, issued behind a batch of field loads, would wait for all of them.
+template <typename T>
+struct MatLds {
+  T ie[256], im[256];
+};
+
+template <typename T>
+__device__ __forceinline__ void stage_materials(const StepArgs<T>& a, MatLds<T>& m) {
+  const int n = blockDim.x * blockDim.y, t = threadIdx.x + blockDim.x * threadIdx.y;
+  for (int q = t; q < 256; q += n) {
+    m.ie[q] = a.mie[q];
+    m.im[q] = a.mim[q];
+  }
+  __syncthreads();
 }
 
 // ---------------------------------------------------------------- two passes
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
 template <typename T>
-__device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, int i, int j, int k0, int k1, int s) {
-  if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const T* __restrict__ Hx = a.H[0];
-  const T* __restrict__ Hy = a.H[1];
-  const T* __restrict__ Hz = a.H[2];
-  const T* __restrict__ Ex = a.E[0];
-  const T* __restrict__ Ey = a.E[1];
-  const T* __restrict__ Ez = a.E[2];
+__device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie, int i, int j, int k0, int k1, int s) {
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
+  const FDTD_G T* __restrict__ Hx = a.H[0];
+  const FDTD_G T* __restrict__ Hy = a.H[1];
+  const FDTD_G T* __restrict__ Hz = a.H[2];
+  const FDTD_G T* __restrict__ Ex = a.E[0];
+  const FDTD_G T* __restrict__ Ey = a.E[1];
+  const FDTD_G T* __restrict__ Ez = a.E[2];
   const long sy = a.sy, sz = a.sz;
   const long step = a.step0 + s;
-  const double* incH = a.incH + (long)s * a.auxL;
+  const FDTD_G double* incH = a.incH + (long)s * a.auxL;
+  const T fx = a.cf[0][0][i], fy = a.cf[0][1][j];
   for (int k = k0; k < k1; k++) {
     const long c = (long)i + sy * (long)j + sz * (long)k;
     HStencil<T> h;
@@ -543,8 +594,9 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, int i, int j,
     h.hx_zm = Hx[c - sz];
     const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
     const int mid = a.mat[c];
+    const T fz = a.cf[0][2][k];
     T o[3];
-    e_cell_3d(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, true, o);
+    e_cell_3d(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, cell_inv_eps(a, tie, mid, i, j, k), fx, fy, fz, true, o);
     a.Eo[0][c] = o[0];
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
@@ -552,20 +604,21 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, int i, int j,
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, int i, int j, int k0, int k1, int s) {
-  if (i >= a.nx - 1 || j >= a.ny - 1) return;
+template <typename T, typename Tab>
+__device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int i, int j, int k0, int k1, int s) {
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   // the E pass wrote Eo; the H pass reads it
-  const T* __restrict__ Ex = a.Eo[0];
-  const T* __restrict__ Ey = a.Eo[1];
-  const T* __restrict__ Ez = a.Eo[2];
-  const T* __restrict__ Hx = a.H[0];
-  const T* __restrict__ Hy = a.H[1];
-  const T* __restrict__ Hz = a.H[2];
+  const FDTD_G T* __restrict__ Ex = a.Eo[0];
+  const FDTD_G T* __restrict__ Ey = a.Eo[1];
+  const FDTD_G T* __restrict__ Ez = a.Eo[2];
+  const FDTD_G T* __restrict__ Hx = a.H[0];
+  const FDTD_G T* __restrict__ Hy = a.H[1];
+  const FDTD_G T* __restrict__ Hz = a.H[2];
   const long sy = a.sy, sz = a.sz;
   const long step = a.step0 + s;
-  const double* incE = a.incE + (long)s * a.auxL;
+  const FDTD_G double* incE = a.incE + (long)s * a.auxL;
   const bool extra = a.nprobe > 0;
+  const T fx = a.cf[1][0][i], fy = a.cf[1][1][j];
   for (int k = k0; k < k1; k++) {
     const long c = (long)i + sy * (long)j + sz * (long)k;
     EStencil<T> e;
@@ -580,8 +633,9 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, int i, int j,
     e.ex_zp = Ex[c + sz];
     const T hx = Hx[c], hy = Hy[c], hz = Hz[c];
     const int mid = a.mat[c];
+    const T fz = a.cf[1][2][k];
     T o[3];
-    h_cell_3d(a, step, incE, i, j, k, c, e, hx, hy, hz, mid, o);
+    h_cell_3d(a, step, incE, i, j, k, c, e, hx, hy, hz, tim[mid], fx, fy, fz, o);
     a.Ho[0][c] = o[0];
     a.Ho[1][c] = o[1];
     a.Ho[2][c] = o[2];
@@ -592,47 +646,61 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, int i, int j,
   }
 }
 
+// One-dimensional launch of ntx x nty x nkc tiles of blockDim.x x blockDim.y
+// columns and kc planes, XCD-aware (tiles x fastest, then y, then z chunks).
 template <typename T>
-__global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
+__global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
+                                                 int nkc) {
   const StepArgs<T>& a = *ap;
-  const int k0 = 1 + blockIdx.z * kc;
-  e_pass_cells(a, 1 + blockIdx.x * blockDim.x + threadIdx.x, 1 + blockIdx.y * blockDim.y + threadIdx.y, k0,
-               min(k0 + kc, a.nz - 1), s);
+  __shared__ MatLds<T> sm;
+  stage_materials(a, sm);
+  const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+  const int k0 = 1 + bz * kc;
+  e_pass_cells(a, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
+               s);
 }
 
+// Each XCD walks its tile range in reverse: its first tiles read the E and H
+// that its E pass touched last (still in its L2 / the Infinity Cache); the
+// next E pass then starts where this pass ended.
 template <typename T>
-__global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s) {
+__global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
+                                                 int nkc) {
   const StepArgs<T>& a = *ap;
-  // Workgroups run in reverse order of the E pass: the first tiles of this
-  // pass read the E and H the E pass touched last, which are still in the
-  // Infinity Cache; the next E pass then starts where this pass ended.
-  const int bxr = gridDim.x - 1 - blockIdx.x, byr = gridDim.y - 1 - blockIdx.y, bzr = gridDim.z - 1 - blockIdx.z;
-  const int k0 = 1 + bzr * kc;
-  h_pass_cells(a, 1 + bxr * blockDim.x + threadIdx.x, 1 + byr * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
+  const int t = xcd_tile_rev(blockIdx.x, ntx * nty * nkc);
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+  const int k0 = 1 + bz * kc;
+  // the H pass reads 1/(mu_r mu0) from global memory: staging it in LDS
+  // measured slower here (the E pass, with its larger stencil, gains)
+  h_pass_cells(a, a.mim, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
                s);
 }
 
 // ---------------------------------------------------------------- batched passes
-// Many independent grids in one launch per pass: workgroup b takes tile
-// tiles[b] = (grid index, tile x, tile y, tile z) of a table built on the host,
-// each grid reading its own arguments.  Same cell arithmetic as above.
+// Many independent grids in one launch per pass: workgroup b takes an entry
+// (grid index, tile x, tile y, tile z) of a table built on the host, in the
+// XCD-aware order of the single-grid passes, each grid reading its own
+// arguments.  Same cell arithmetic as above.
 template <typename T>
 __global__ __launch_bounds__(256) void e_pass_batch(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
                                                     int kc, int s) {
-  const int4 t = tiles[blockIdx.x];
+  const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const StepArgs<T>& a = args[t.x];
+  __shared__ MatLds<T> sm;
+  stage_materials(a, sm);
   const int k0 = 1 + t.w * kc;
-  e_pass_cells(a, 1 + t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
+  e_pass_cells(a, sm.ie, t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
                s);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void h_pass_batch(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
                                                     int kc, int s) {
-  const int4 t = tiles[gridDim.x - 1 - blockIdx.x];
+  const int4 t = tiles[xcd_tile_rev(blockIdx.x, gridDim.x)];
   const StepArgs<T>& a = args[t.x];
   const int k0 = 1 + t.w * kc;
-  h_pass_cells(a, 1 + t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
+  h_pass_cells(a, a.mim, t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
                s);
 }
 
@@ -679,9 +747,9 @@ __device__ __forceinline__ void load_plane(const StepArgs<T>& a, int i, int j, i
 
 // publish the old H of plane kk in LDS, then the new E of (i,j,kk)
 template <typename T, int TY>
-__device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, long step, const double* incH, int i, int j, int tx,
-                                              int ty, int kk, bool own_e, bool ecell, const PlaneLoad<T>& p, T hzx,
-                                              T hzy, T (&sH)[3][TY + 2][256], T en[3]) {
+__device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, const T* tie, long step, const FDTD_G double* incH,
+                                              int i, int j, int tx, int ty, int kk, bool own_e, bool ecell,
+                                              const PlaneLoad<T>& p, T hzx, T hzy, T (&sH)[3][TY + 2][256], T en[3]) {
   sH[0][ty + 1][tx] = p.h[0];
   sH[1][ty + 1][tx] = p.h[1];
   sH[2][ty + 1][tx] = p.h[2];
@@ -707,7 +775,8 @@ __device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, long step, c
     h.hy_zm = hzy;
     h.hx_zm = hzx;
     T o[3];
-    e_cell_3d(a, step, incH, i, j, kk, c, h, p.e[0], p.e[1], p.e[2], p.mid, own_e, o);
+    e_cell_3d(a, step, incH, i, j, kk, c, h, p.e[0], p.e[1], p.e[2], p.mid, cell_inv_eps(a, tie, p.mid, i, j, kk),
+              a.cf[0][0][i], a.cf[0][1][j], a.cf[0][2][kk], own_e, o);
     en[0] = o[0];
     en[1] = o[1];
     en[2] = o[2];
@@ -719,14 +788,6 @@ __device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, long step, c
   }
 }
 
-// XCD-aware tile order: the hardware hands consecutive workgroup ids to the 8
-// XCDs in turn; this bijection gives each XCD a contiguous run of tiles, so
-// y-neighbouring tiles (which share halo rows) meet in the same L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nb) {
-  const int q = nb / 8, r = nb % 8;
-  const int x = bid % 8, loc = bid / 8;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
-}
 
 template <typename T, int TY>
 __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx,
@@ -734,6 +795,8 @@ __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __
   const StepArgs<T>& a = *ap;
   __shared__ T sH[3][TY + 2][256];
   __shared__ T sE[3][TY + 1][256];
+  __shared__ MatLds<T> sm;
+  stage_materials(a, sm);
   const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -746,8 +809,8 @@ __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __
   const int nx = a.nx, ny = a.ny;
   const long sy = a.sy, sz = a.sz;
   const long step = a.step0 + s;
-  const double* incH = a.incH + (long)s * a.auxL;
-  const double* incE = a.incE + (long)s * a.auxL;
+  const FDTD_G double* incH = a.incH + (long)s * a.auxL;
+  const FDTD_G double* incE = a.incE + (long)s * a.auxL;
   const bool mem = i < nx && j < ny;
   const bool ecell = tx >= 1 && i <= nx - 2 && j <= ny - 2;
   const bool own = ecell && tx <= kFusedCols && ty < TY;
@@ -765,7 +828,7 @@ __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __
   PlaneLoad<T> cur, nxt;
   load_plane(a, i, j, k0, mem, lead, cur);
   load_plane(a, i, j, k0 + 1, mem, lead, nxt);  // plane k1 <= nz-1 always exists
-  fused_e_plane<T, TY>(a, step, incH, i, j, tx, ty, k0, own, ecell, cur, hzx, hzy, sH, ek);
+  fused_e_plane<T, TY>(a, sm.ie, step, incH, i, j, tx, ty, k0, own, ecell, cur, hzx, hzy, sH, ek);
   hk[0] = cur.h[0];
   hk[1] = cur.h[1];
   hk[2] = cur.h[2];
@@ -778,7 +841,8 @@ __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __
   for (int k = k0; k < k1; k++) {
     // issue the loads of plane k+2 before working on planes k+1 and k
     if (k + 2 <= k1) load_plane(a, i, j, k + 2, mem, lead, nxt);
-    fused_e_plane<T, TY>(a, step, incH, i, j, tx, ty, k + 1, own && k + 1 < k1, ecell, cur, hk[0], hk[1], sH, en);
+    fused_e_plane<T, TY>(a, sm.ie, step, incH, i, j, tx, ty, k + 1, own && k + 1 < k1, ecell, cur, hk[0], hk[1], sH,
+                         en);
     if (own) {  // new H of plane k
       const long c = (long)i + sy * (long)j + sz * (long)k;
       EStencil<T> e;
@@ -792,7 +856,8 @@ __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __
       e.ey_zp = en[1];
       e.ex_zp = en[0];
       T o[3];
-      h_cell_3d(a, step, incE, i, j, k, c, e, hk[0], hk[1], hk[2], midk, o);
+      h_cell_3d(a, step, incE, i, j, k, c, e, hk[0], hk[1], hk[2], sm.im[midk], a.cf[1][0][i], a.cf[1][1][j],
+                a.cf[1][2][k], o);
       a.Ho[0][c] = o[0];
       a.Ho[1][c] = o[1];
       a.Ho[2][c] = o[2];
@@ -816,22 +881,244 @@ __global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __
   }
 }
 
+// ---------------------------------------------------------------- wave-fused pass
+// One pass per step with no LDS and no barrier.  A wavefront owns a
+// 62-column x TY-row x kc-plane brick: lane l holds column i = i0 - 1 + l
+// (lane 0 only supplies H(i-1); lane 63 is the +x halo whose new E the owned
+// cells need) and rows j0 .. j0+TY in registers (row j0+TY is the +y halo,
+// row j0-1 supplies H(j-1)); it marches z keeping the previous plane's old H
+// and the current plane's new E in registers.  x neighbours move between
+// lanes by shuffles, y and z neighbours are registers of the same lane, so
+// every wave streams its loads independently.  Old state is only read and new
+// state only written (ping-pong buffers), so recomputing halo cells never
+// races; the cell arithmetic is e_cell_3d / h_cell_3d, so results are
+// bitwise those of the two passes.  A workgroup stacks kWavesY waves in y.
+constexpr int kWaveCols = 62;
+constexpr int kWavesY = 4;
+
+// Raw buffer access (32-bit byte offsets: one lane offset shared by every
+// array, the row/plane part in a scalar register), which keeps the many rows
+// of a wave's brick from each holding 64-bit addresses per array.  Every
+// access lands inside the (padded) allocation.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const FDTD_G void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so);
+template <>
+__device__ __forceinline__ float buf_ld<float>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+}
+template <>
+__device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, 0));
+}
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                        r, (int)vo, (int)so, 0);
+}
+
+struct WaveBufs {
+  __amdgpu_buffer_rsrc_t h[3], e[3], ho[3], eo[3], m;
+};
+
+template <typename T, int TY>
+struct WavePlane {
+  T hx[TY + 2], hz[TY + 2];  // rows -1 .. TY
+  T hy[TY + 1];              // rows 0 .. TY
+  T e[3][TY + 1];
+  int mid[TY + 1];
+};
+
+// old state of plane kk, rows jw-1 .. jw+TY of this lane's column; vo / vm:
+// byte offsets of (i, jw-1, 0) in a field array / the id volume
+template <typename T, int TY>
+__device__ __forceinline__ void wave_load(const WaveBufs& B, unsigned vo, unsigned vm, long sy, long sz, int kk,
+                                          WavePlane<T, TY>& p) {
+#pragma unroll
+  for (int r = -1; r <= TY; r++) {
+    const unsigned so = (unsigned)((long)(r + 1) * sy + (long)kk * sz);
+    p.hx[r + 1] = buf_ld<T>(B.h[0], vo, so * (unsigned)sizeof(T));
+    p.hz[r + 1] = buf_ld<T>(B.h[2], vo, so * (unsigned)sizeof(T));
+    if (r >= 0) {
+      p.hy[r] = buf_ld<T>(B.h[1], vo, so * (unsigned)sizeof(T));
+      p.e[0][r] = buf_ld<T>(B.e[0], vo, so * (unsigned)sizeof(T));
+      p.e[1][r] = buf_ld<T>(B.e[1], vo, so * (unsigned)sizeof(T));
+      p.e[2][r] = buf_ld<T>(B.e[2], vo, so * (unsigned)sizeof(T));
+      p.mid[r] = (int)__builtin_amdgcn_raw_buffer_load_b8(B.m, (int)vm, (int)so, 0);
+    }
+  }
+}
+
+// new E of plane kk, rows 0..TY (old E where the cell is not updated); the
+// owned cells' E (and psi / dispersive state) are written to the next buffers
+template <typename T, int TY>
+__device__ __forceinline__ void wave_e(const StepArgs<T>& a, const T* tie, T fx, const WaveBufs& B, unsigned vo,
+                                       long step, const FDTD_G double* incH, int i, int jw, int kk, bool ecol,
+                                       bool owncol, bool own_plane, const WavePlane<T, TY>& p, const T (&zx)[TY + 1],
+                                       const T (&zy)[TY + 1], T (&en)[3][TY + 1]) {
+#pragma unroll
+  for (int r = 0; r <= TY; r++) {
+    const T hz_xm = __shfl_up(p.hz[r + 1], 1);
+    const T hy_xm = __shfl_up(p.hy[r], 1);
+    en[0][r] = p.e[0][r];
+    en[1][r] = p.e[1][r];
+    en[2][r] = p.e[2][r];
+    const int j = jw + r;
+    if (ecol && j <= a.ny - 2 && kk >= 1 && kk <= a.nz - 2) {
+      const long c = (long)i + a.sy * (long)j + a.sz * (long)kk;
+      HStencil<T> h;
+      h.hx = p.hx[r + 1];
+      h.hy = p.hy[r];
+      h.hz = p.hz[r + 1];
+      h.hz_ym = p.hz[r];
+      h.hx_ym = p.hx[r];
+      h.hz_xm = hz_xm;
+      h.hy_xm = hy_xm;
+      h.hy_zm = zy[r];
+      h.hx_zm = zx[r];
+      const bool own = owncol && r < TY && own_plane;
+      T o[3];
+      e_cell_3d(a, step, incH, i, j, kk, c, h, p.e[0][r], p.e[1][r], p.e[2][r], p.mid[r],
+                cell_inv_eps(a, tie, p.mid[r], i, j, kk), fx, a.cf[0][1][j], a.cf[0][2][kk], own, o);
+      en[0][r] = o[0];
+      en[1][r] = o[1];
+      en[2][r] = o[2];
+      if (own) {
+        const unsigned so = (unsigned)((long)(r + 1) * a.sy + (long)kk * a.sz) * (unsigned)sizeof(T);
+        buf_st(o[0], B.eo[0], vo, so);
+        buf_st(o[1], B.eo[1], vo, so);
+        buf_st(o[2], B.eo[2], vo, so);
+      }
+    }
+  }
+}
+
+template <typename T, int TY>
+__global__ __launch_bounds__(64 * kWavesY) void fusedw_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx,
+                                                          int nty, int nkc) {
+  const StepArgs<T>& a = *ap;
+  const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+  const int lane = threadIdx.x;
+  const int i = bx * kWaveCols + lane;
+  const int jw = 1 + (by * kWavesY + (int)threadIdx.y) * TY;
+  __shared__ MatLds<T> sm;
+  stage_materials(a, sm);
+  if (jw > a.ny - 2) return;  // the whole wave lies beyond the interior rows
+  const int k0 = 1 + bz * kc;
+  const int k1 = min(k0 + kc, a.nz - 1);
+  const long step = a.step0 + s;
+  const FDTD_G double* incH = a.incH + (long)s * a.auxL;
+  const FDTD_G double* incE = a.incE + (long)s * a.auxL;
+  const bool ecol = lane >= 1 && i <= a.nx - 2;
+  const bool owncol = ecol && lane <= kWaveCols;
+  const bool extra = a.nprobe > 0;
+  const long sy = a.sy, sz = a.sz;
+  const T fxE = ecol ? a.cf[0][0][i] : T(0), fxH = ecol ? a.cf[1][0][i] : T(0);
+  WaveBufs B;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    B.h[q] = buf_rsrc(a.H[q], a.field_bytes);
+    B.e[q] = buf_rsrc(a.E[q], a.field_bytes);
+    B.ho[q] = buf_rsrc(a.Ho[q], a.field_bytes);
+    B.eo[q] = buf_rsrc(a.Eo[q], a.field_bytes);
+  }
+  B.m = buf_rsrc(a.mat, a.mat_bytes);
+  const unsigned vm = (unsigned)((long)i + sy * (long)(jw - 1));
+  const unsigned vo = vm * (unsigned)sizeof(T);
+
+  // old Hx, Hy of plane k0-1: the z-1 neighbours of plane k0
+  T chx[TY + 1], chy[TY + 1], chz[TY];
+  int cmid[TY];
+#pragma unroll
+  for (int r = 0; r <= TY; r++) {
+    const unsigned so = (unsigned)((long)(r + 1) * sy + (long)(k0 - 1) * sz) * (unsigned)sizeof(T);
+    chx[r] = buf_ld<T>(B.h[0], vo, so);
+    chy[r] = buf_ld<T>(B.h[1], vo, so);
+  }
+  WavePlane<T, TY> p;
+  wave_load<T, TY>(B, vo, vm, sy, sz, k0, p);
+  T en[3][TY + 1];
+  wave_e<T, TY>(a, sm.ie, fxE, B, vo, step, incH, i, jw, k0, ecol, owncol, true, p, chx, chy, en);
+#pragma unroll
+  for (int r = 0; r <= TY; r++) {
+    chx[r] = p.hx[r + 1];
+    chy[r] = p.hy[r];
+    if (r < TY) {
+      chz[r] = p.hz[r + 1];
+      cmid[r] = p.mid[r];
+    }
+  }
+  for (int k = k0; k < k1; k++) {
+    wave_load<T, TY>(B, vo, vm, sy, sz, k + 1, p);  // plane k+1 <= k1 <= nz-1 exists
+    T en1[3][TY + 1];
+    wave_e<T, TY>(a, sm.ie, fxE, B, vo, step, incH, i, jw, k + 1, ecol, owncol, k + 1 < k1, p, chx, chy, en1);
+    // new H of plane k from the new E of planes k and k+1
+#pragma unroll
+    for (int r = 0; r < TY; r++) {
+      const T ez_xp = __shfl_down(en[2][r], 1);
+      const T ey_xp = __shfl_down(en[1][r], 1);
+      const int j = jw + r;
+      if (owncol && j <= a.ny - 2) {
+        const long c = (long)i + sy * (long)j + sz * (long)k;
+        EStencil<T> e;
+        e.ex = en[0][r];
+        e.ey = en[1][r];
+        e.ez = en[2][r];
+        e.ez_yp = en[2][r + 1];
+        e.ex_yp = en[0][r + 1];
+        e.ez_xp = ez_xp;
+        e.ey_xp = ey_xp;
+        e.ey_zp = en1[1][r];
+        e.ex_zp = en1[0][r];
+        T o[3];
+        h_cell_3d(a, step, incE, i, j, k, c, e, chx[r], chy[r], chz[r], sm.im[cmid[r]], fxH, a.cf[1][1][j],
+                  a.cf[1][2][k], o);
+        const unsigned so = (unsigned)((long)(r + 1) * sy + (long)k * sz) * (unsigned)sizeof(T);
+        buf_st(o[0], B.ho[0], vo, so);
+        buf_st(o[1], B.ho[1], vo, so);
+        buf_st(o[2], B.ho[2], vo, so);
+        if (extra) {
+          const T ev[3] = {e.ex, e.ey, e.ez};
+          dft_probe(a, step, c, i, j, k, ev, o);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r <= TY; r++) {
+      en[0][r] = en1[0][r];
+      en[1][r] = en1[1][r];
+      en[2][r] = en1[2][r];
+      chx[r] = p.hx[r + 1];
+      chy[r] = p.hy[r];
+      if (r < TY) {
+        chz[r] = p.hz[r + 1];
+        cmid[r] = p.mid[r];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- 2-D TM
 // Ez, Hx, Hy (include/YeeFields.hpp:123-148); CPML along x and y; in place.
 template <typename T>
 __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
   const StepArgs<T>& a = *ap;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const T* __restrict__ Hx = a.H[0];
-  const T* __restrict__ Hy = a.H[1];
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
+  const FDTD_G T* __restrict__ Hx = a.H[0];
+  const FDTD_G T* __restrict__ Hy = a.H[1];
   const long step = a.step0 + s;
   const long c = (long)i + a.sy * j;
   const T hx = Hx[c], hy = Hy[c], hy_xm = Hy[c - 1], hx_ym = Hx[c - a.sy];
   const T ez = a.E[2][c];
   const int mid = a.mat[c];
-  const T ie = cell_inv_eps(a, mid, i, j, 0);
+  const T ie = cell_inv_eps(a, a.mie, mid, i, j, 0);
   const T dzx = hy - hy_xm;  // J=x, +1
   const T dzy = hx - hx_ym;  // J=y, -1
   T d[3] = {T(0), T(0), T(0)};
@@ -866,10 +1153,10 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
 template <typename T>
 __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
   const StepArgs<T>& a = *ap;
-  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.nx - 1 || j >= a.ny - 1) return;
-  const T* __restrict__ Ez = a.Eo[2];
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
+  const FDTD_G T* __restrict__ Ez = a.Eo[2];
   const long step = a.step0 + s;
   const long c = (long)i + a.sy * j;
   const T ez = Ez[c], ez_yp = Ez[c + a.sy], ez_xp = Ez[c + 1];

@@ -173,7 +173,9 @@ def test_autotune_as_first_call():
         np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=nm)
 
 
-FUSED = [(0, 1, 8), (0, 2, 16), (0, 3, 5)]
+# (0, TY, kc): LDS fused pass; (1, TY, kc): wave-fused pass (ny=23 gives a partly
+# empty last wave row block, nx=300 five 62-column wave tiles)
+FUSED = [(0, 1, 8), (0, 2, 16), (0, 3, 5), (1, 2, 7), (1, 4, 16), (1, 6, 5)]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -207,11 +209,13 @@ def test_fused_pass_equals_two_pass(precision):
 def test_fused_pass_against_oracle(precision):
     """Fused schedule on the dispersive sphere and on a CPML+TFSF+DFT grid,
     against the FP64 oracle."""
-    for sc in (synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90),
-               synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90)):
-        print(f"[gpu-case] fused-vs-oracle {sc['name']} {precision}", flush=True)
+    for sc, tile in ((synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (0, 3, 8)),
+                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (0, 3, 8)),
+                     (synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (1, 4, 8)),
+                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (1, 2, 16))):
+        print(f"[gpu-case] fused-vs-oracle {sc['name']} {tile} {precision}", flush=True)
         g = apply(_grid(sc, precision), sc)
-        g.set_tile(0, 3, 8)
+        g.set_tile(*tile)
         o = apply(_oracle(sc), sc)
         g.step(sc["nsteps"])
         o.step(sc["nsteps"])
