@@ -191,8 +191,10 @@ struct fdtd_grid {
     int mask, nfreq, ncomp, nphase;
     long ncells;
     void* acc;
+    void* smp;  // samples of the steps not yet folded into acc
     double *phE, *phH;
   } dft[FDTD_MAX_DFT];
+  long dft_done = 0;  // steps folded into the DFT accumulators
 
   // probes
   int nprobe = 0;
@@ -576,6 +578,9 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     dd.nphase = h.nphase;
     dd.base = a.dft_work;
     a.dft_work += long(h.nfreq) * h.ncells;
+    dd.cbase = a.dft_cells;
+    a.dft_cells += h.ncells;
+    gput(dd.smp, (T*)h.smp);
   }
   a.nprobe = g->nprobe;
   for (int p = 0; p < g->nprobe; p++) a.probe[p] = g->probe[p];
@@ -593,6 +598,7 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   if (g->recompute) {
     a.ndft = 0;
     a.dft_work = 0;
+    a.dft_cells = 0;
     a.nprobe = 0;
   }
 }
@@ -768,7 +774,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
     const auto* dev1 =
         reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
     HIP_TRY(hipMemcpyAsync((void*)dev0, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
-    const long dft_work = host->dft_work;
+    const long dft_work = host->dft_work, dft_cells = host->dft_cells;
     for (int s = 0; s < chunk; s++) {
       int rc;
       const auto* ap = (fused && (s % 2)) ? dev1 : dev0;
@@ -795,8 +801,14 @@ int step_impl(fdtd_grid* g, int nsteps) {
         rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
         if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
       }
-      if (!rc && dft_work > 0)
-        rc = launch(g, 4, fdtdk::dft_accumulate<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), ap, s);
+      if (!rc && dft_work > 0) {
+        rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), ap, s);
+        if (!rc && (g->step + 1) % fdtdk::kDftBlock == 0) {
+          rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), ap, g->dft_done,
+                      g->step + 1);
+          g->dft_done = g->step + 1;
+        }
+      }
       if (rc) return rc;
       g->step++;
     }
@@ -806,6 +818,35 @@ int step_impl(fdtd_grid* g, int nsteps) {
   }
   return 0;
 }
+
+// Fold the samples of steps [dft_done, step) into the DFT accumulators: done
+// before the accumulators are read and before the monitors or the way the
+// grid is stepped change.
+template <typename T>
+int dft_flush_impl(fdtd_grid* g) {
+  if (g->ndft == 0 || g->dft_done >= g->step) {
+    g->dft_done = g->step;
+    return 0;
+  }
+  fdtdk::StepArgs<T> a;
+  fill_args<T>(g, a);
+  const int slot = g->args_slot;
+  g->args_slot = (slot + 1) % fdtd_grid::kRing;
+  if (g->args_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->args_ev[slot]));
+  auto* host = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
+  *host = a;
+  const auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
+  HIP_TRY(hipMemcpyAsync((void*)dev, host, sizeof(a), hipMemcpyHostToDevice, g->stream));
+  int rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((a.dft_work + 255) / 256)), dim3(256), dev, g->dft_done,
+                  g->step);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(g->args_ev[slot], g->stream));
+  g->args_ev_used[slot] = true;
+  g->dft_done = g->step;
+  return 0;
+}
+
+int dft_flush(fdtd_grid* g) { return g->f64 ? dft_flush_impl<double>(g) : dft_flush_impl<float>(g); }
 
 bool valid_comp(const fdtd_grid* g, int field, int comp) {
   if (comp < 0 || comp > 2) return false;
@@ -949,6 +990,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   if (rc) return rc;
 
   // objective and its adjoint source weights 2 F
+  if ((rc = dft_flush_impl<T>(g))) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   const auto& d = g->dft[didx];
   const size_t per_slot = size_t(d.nfreq) * 2 * d.ncells;
@@ -1288,6 +1330,8 @@ int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, co
     if (((comp_mask >> c) & 1) && !(c < 3 ? g->actE[c] : g->actH[c - 3]))
       return fail(FDTD_EINVAL, "DFT component %d does not exist in this mode", c);
   if (comp_mask <= 0 || comp_mask >= 64) return fail(FDTD_EINVAL, "bad component mask");
+  int rc0 = dft_flush(g);
+  if (rc0) return rc0;
   auto& d = g->dft[g->ndft];
   for (int x = 0; x < 6; x++) d.r[x] = r[x];
   d.mask = comp_mask;
@@ -1296,6 +1340,7 @@ int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, co
   d.ncomp = __builtin_popcount(comp_mask);
   d.ncells = long(r[1] - r[0]) * (r[3] - r[2]) * (r[5] - r[4]);
   int rc = g->alloc(&d.acc, size_t(d.ncomp) * nfreq * 2 * d.ncells * g->esz);
+  if (!rc) rc = g->alloc(&d.smp, size_t(fdtdk::kDftBlock) * d.ncomp * d.ncells * g->esz);
   if (!rc) rc = g->alloc((void**)&d.phE, size_t(std::max(nphase, 1)) * nfreq * 2 * 8);
   if (!rc) rc = g->alloc((void**)&d.phH, size_t(std::max(nphase, 1)) * nfreq * 2 * 8);
   if (rc) return rc;
@@ -1310,6 +1355,8 @@ int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, co
 
 int fdtd_get_dft(fdtd_grid* g, int idx, double* out) {
   if (!g || !out || idx < 0 || idx >= g->ndft) return fail(FDTD_EINVAL, "bad DFT index");
+  int rc = dft_flush(g);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   const auto& d = g->dft[idx];
   return download_T(g, out, d.acc, size_t(d.ncomp) * d.nfreq * 2 * d.ncells);
@@ -1430,7 +1477,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;
   const int ncand = g->ndim == 3 ? int(sizeof cand3 / sizeof cand3[0]) : int(sizeof cand2 / sizeof cand2[0]);
-  const long step0 = g->step;
+  const long step0 = g->step, dft_done0 = g->dft_done;
   if ((rc = ensure_trace(g, step0 + long(ncand) * (trial_steps + 1)))) return rc;
   // pending table uploads go first: the snapshot must hold the tables the
   // trials will run with, or the restore would roll them back to stale contents
@@ -1489,6 +1536,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   // fused trials swap the roles of the two state copies; put them back
   if ((g->nswap - swaps0) % 2) swap_state(g);
   g->step = step0;
+  g->dft_done = dft_done0;
   g->prof = prof;
   std::copy(kms, kms + kNumKernels, g->kms);
   std::copy(kcount, kcount + kNumKernels, g->kcount);
@@ -1691,7 +1739,10 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
     }
     HIP_TRY(hipEventRecord(b->ev[slot], b->stream));
     b->ev_used[slot] = true;
-    for (fdtd_grid* g : b->grids) g->step += chunk;
+    for (fdtd_grid* g : b->grids) {
+      g->step += chunk;
+      g->dft_done = g->step;  // batches accumulate the DFT step by step
+    }
     done += chunk;
   }
   return 0;
@@ -1718,6 +1769,7 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
   std::vector<int4> tab;
   for (int q = 0; q < n && !rc; q++) {
     fdtd_grid* g = grids[q];
+    if (!rc) rc = dft_flush(g);  // on the grid's own stream, finished before the batch's stream takes over
     (void)hipStreamSynchronize(g->stream);
     b->grids.push_back(g);
     b->saved_streams.push_back(g->stream);

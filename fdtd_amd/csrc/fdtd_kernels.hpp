@@ -68,8 +68,14 @@ struct DftDev {
   const FDTD_G double* phE;  // [nphase][nfreq][2]
   const FDTD_G double* phH;
   int nphase;
-  long base;  // offset of this monitor's (frequency, cell) items in the DFT launch
+  long base;      // offset of this monitor's (frequency, cell) items in the DFT launch
+  long cbase;     // offset of this monitor's cells in the sampling launch
+  FDTD_G T* smp;  // [kDftBlock][ncomp][ncells]: samples of steps not yet folded in
 };
+
+// steps of field samples gathered before they are folded into the DFT
+// accumulators (which are then read and written once per block)
+constexpr int kDftBlock = 16;
 
 struct ProbeDev {
   long cell;
@@ -117,7 +123,8 @@ struct StepArgs {
   const FDTD_G double* incH;  // aux Hy before it
   // running DFT
   int ndft;
-  long dft_work;  // sum over monitors of nfreq x cells
+  long dft_work;   // sum over monitors of nfreq x cells
+  long dft_cells;  // sum over monitors of cells
   DftDev<T> dft[kMaxDft];
   // probes
   int nprobe;
@@ -459,6 +466,16 @@ __device__ FDTD_COLD void dft_probe(const StepArgs<T>& a, long step, long c, int
 // frequency, cell) after the step, so surface work spreads over the whole GPU
 // instead of serialising in the few threads that own monitor cells; it reads
 // the new E and H of the monitor cells only.
+// field component (0..5) recorded in slot `slot` of a monitor with component mask `mask`
+__device__ __forceinline__ int comp_of_slot(int mask, int slot) {
+  for (int comp = 0; comp < 6; comp++)
+    if ((mask >> comp) & 1) {
+      if (slot == 0) return comp;
+      slot--;
+    }
+  return 5;
+}
+
 template <typename T>
 __device__ __forceinline__ void dft_item(const StepArgs<T>& a, long g, long step) {
   int q = 0;
@@ -496,6 +513,65 @@ __global__ __launch_bounds__(256) void dft_accumulate(const StepArgs<T>* __restr
   if (g < a.dft_work) dft_item(a, g, a.step0 + s);
 }
 
+// Time-blocked form of the same sums.  dft_sample stores, after step n, the
+// monitored components of every monitor cell in slot n % kDftBlock;
+// dft_fold adds the samples of steps [m0, m1) (at most kDftBlock of them) to
+// the accumulators, one thread per (monitor, frequency, cell) keeping its
+// accumulators in registers across the block.  The additions happen in the
+// same order as step-by-step accumulation, so the sums are bitwise equal.
+template <typename T>
+__global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict__ ap, int s) {
+  const StepArgs<T>& a = *ap;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.dft_cells) return;
+  int q = 0;
+  while (q + 1 < a.ndft && g >= a.dft[q + 1].cbase) q++;
+  const DftDev<T>& d = a.dft[q];
+  const long lq = g - d.cbase;
+  const int wx = d.r[1] - d.r[0], wy = d.r[3] - d.r[2];
+  const int i = d.r[0] + (int)(lq % wx);
+  const int j = d.r[2] + (int)((lq / wx) % wy);
+  const int k = d.r[4] + (int)(lq / ((long)wx * wy));
+  const long c = (long)i + a.sy * (long)j + a.sz * (long)k;
+  const long slot0 = ((a.step0 + s) % kDftBlock) * d.ncomp;
+  int slot = 0;
+#pragma unroll
+  for (int comp = 0; comp < 6; comp++) {
+    if (!((d.mask >> comp) & 1)) continue;
+    const FDTD_G T* src = comp < 3 ? a.Eo[comp] : a.Ho[comp - 3];
+    d.smp[(slot0 + slot) * d.ncells + lq] = src ? src[c] : T(0);
+    slot++;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void dft_fold(const StepArgs<T>* __restrict__ ap, long m0, long m1) {
+  const StepArgs<T>& a = *ap;
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.dft_work) return;
+  int q = 0;
+  while (q + 1 < a.ndft && g >= a.dft[q + 1].base) q++;
+  const DftDev<T>& d = a.dft[q];
+  const long r = g - d.base;
+  const int f = (int)(r / d.ncells);
+  const long lq = r - (long)f * d.ncells;
+  const long mend = m1 < (long)d.nphase ? m1 : (long)d.nphase;
+  for (int slot = 0; slot < d.ncomp; slot++) {
+    const bool efield = comp_of_slot(d.mask, slot) < 3;
+    const FDTD_G double* ph = efield ? d.phE : d.phH;
+    FDTD_G T* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
+    T ar = re[0], ai = re[d.ncells];
+    for (long m = m0; m < mend; m++) {
+      const T v = d.smp[((m % kDftBlock) * d.ncomp + slot) * d.ncells + lq];
+      const FDTD_G double* pm = ph + 2L * (d.nfreq * m + f);
+      ar += v * (T)pm[0];
+      ai += v * (T)pm[1];
+    }
+    re[0] = ar;
+    re[d.ncells] = ai;
+  }
+}
+
 // batched grids: `offs` holds each grid's first item (prefix sums, n+1 entries)
 template <typename T>
 __global__ __launch_bounds__(256) void dft_accumulate_batch(const StepArgs<T>* __restrict__ args,
@@ -529,22 +605,23 @@ __device__ FDTD_COLD void record_increment(const StepArgs<T>& a, long step, int 
 }
 
 // Workgroup -> tile, XCD-aware.  The dispatcher deals consecutive workgroups
-// round-robin to the 8 XCDs, each with its own L2; this bijection on [0, nb)
-// gives XCD x one contiguous range of tiles instead, so the neighbour rows
-// and planes a tile reads were mostly loaded by the same XCD.  The reverse
-// form walks each XCD's range from its end (the H pass starts where the E
-// pass ended, on the same XCD).
+// round-robin to the 8 XCDs, each with its own L2.  This bijection on
+// [0, nb) gives each XCD runs of kXcdRun consecutive tiles (neighbouring rows
+// of one z chunk, whose shared halo rows then meet in one L2), the runs dealt
+// round-robin so that unevenly loaded regions (dispersive cells, TFSF faces)
+// still spread over all XCDs.  Tiles past the last whole group of 8 runs map
+// to themselves.  The reverse form runs the same map from the last workgroup
+// down, so a pass launched after another starts on the tiles the other ended
+// with (still in the Infinity Cache).
+constexpr int kXcdRun = 16;
+
 __device__ __forceinline__ int xcd_tile(int bid, int nb) {
-  const int q = nb / 8, r = nb % 8;
+  const int group = 8 * kXcdRun;
+  if (bid >= (nb / group) * group) return bid;
   const int x = bid % 8, loc = bid / 8;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + loc;
+  return ((loc / kXcdRun) * 8 + x) * kXcdRun + loc % kXcdRun;
 }
-__device__ __forceinline__ int xcd_tile_rev(int bid, int nb) {
-  const int q = nb / 8, r = nb % 8;
-  const int x = bid % 8, loc = bid / 8;
-  const int cnt = q + (x < r ? 1 : 0);
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (cnt - 1 - loc);
-}
+__device__ __forceinline__ int xcd_tile_rev(int bid, int nb) { return xcd_tile(nb - 1 - bid, nb); }
 
 // The 256-entry material inverses, staged in LDS once per workgroup: the
 // per-cell lookup by material id then costs an LDS read instead of a global
