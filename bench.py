@@ -278,7 +278,8 @@ def run_c5(args):
     abytes, interior = algorithmic_bytes(dict(sc, dfts=[]), esz)
     ndes = int(eps.size)
     abytes += ndes * (3 * esz + 16)
-    adj = res["checkpointed"]["kernel_avg_ms"]
+    # adjoint kernels timed where nothing runs beside them (with history resident there is no recompute)
+    adj = res["history_resident"]["kernel_avg_ms"]
     adj_ms = adj.get("adj_e_pass", 0) + adj.get("adj_h_pass", 0)
     achieved = abytes / (adj_ms / 1e3) / 1e9 if adj_ms else None
     out = base_line(args, res["checkpointed"]["ratio_to_forward"], "x forward time", round(t_fwd / nsteps * 1e3, 5),
@@ -322,6 +323,66 @@ def run_copy(args):
                      {"workload": "device copy, 1 GiB FP32 -> 1 GiB FP32", "bytes_per_copy": 2 * n * 4})
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3d256_f32.json")
+
+
+def _short(name):
+    for k in ("e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "dft_accumulate", "fusedw_3d", "fused_3d",
+              "aux_line", "copyBuffer"):
+        if k in name:
+            return k
+    return name.split("(")[0][:60]
+
+
+def _counter_means(d, counter):
+    import csv
+    agg = {}
+    with open(os.path.join(d, "run_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                agg.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in agg.items()}
+
+
+def traffic_summary(args):
+    """HBM bytes per launch of each kernel from rocprofv3 --pmc runs of this bench: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch) of the workload, each scaled by known bytes / counter for the 1 GiB device copy
+    (`--workload copy`, the runtime's copyBuffer kernel: 1 GiB read + 1 GiB written)."""
+    fetch_dir, write_dir, cfetch_dir, cwrite_dir = args.traffic_summary
+    known = float(1 << 30)
+    cf = _counter_means(cfetch_dir, "FETCH_SIZE")["copyBuffer"] * 1024.0
+    cw = _counter_means(cwrite_dir, "WRITE_SIZE")["copyBuffer"] * 1024.0
+    ff, wf = known / cf, known / cw
+    fetch = _counter_means(fetch_dir, "FETCH_SIZE")
+    write = _counter_means(write_dir, "WRITE_SIZE")
+    kern = {k: {"fetch_bytes": round(fetch[k] * 1024.0 * ff), "write_bytes": round(write.get(k, 0.0) * 1024.0 * wf)}
+            for k in fetch if k in ("e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedw_3d", "fused_3d")}
+    out = {"workload": "3-D 256^3 FP32 forward (bench.py default workload)", "tile": args.tile,
+           "fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
+           "copy_counters_kib": {"FETCH_SIZE": cf / 1024.0, "WRITE_SIZE": cw / 1024.0}, "kernels": kern}
+    with open(args.traffic_out, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out), flush=True)
+
+
+def step_traffic(tile, kernels_per_step):
+    """HBM bytes of one step from the committed counter summary (taken with one two-pass tile; the bytes a
+    two-pass step moves do not depend on the tile to first order), for a two-pass tile."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if tile[0] < 2:
+        return None
+    total = 0.0
+    for k, per_step in kernels_per_step.items():
+        if k not in t["kernels"]:
+            return None
+        total += per_step * (t["kernels"][k]["fetch_bytes"] + t["kernels"][k]["write_bytes"])
+    return round(total)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,7 +398,14 @@ def main():
                          "c4: 64 batched grids vs sequential; c5: adjoint eps gradient 128^3; copy: 1 GiB device copy")
     ap.add_argument("--tile", default=None,
                     help="bx,by,kc: fixed tile instead of the autotuner (c2; bx = 0 selects the fused pass)")
+    ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench and of --workload copy into "
+                         "HBM bytes per launch (no GPU work)")
+    ap.add_argument("--traffic-out", default="hbm_traffic.json")
     args = ap.parse_args()
+    if args.traffic_summary:
+        traffic_summary(args)
+        return
     if args.steps is None:
         args.steps = {"c4": 1500, "c5": 3000, "copy": 20}.get(args.workload, 2000)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -383,7 +451,10 @@ def main():
                                f"2 z-planes x 8 freqs",
                    "cells": args.size ** 3, "interior_cells": interior, "tile": list(tile)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": step_traffic(tile, {"e_pass_3d": 1.0, "h_pass_3d": 1.0, "dft_sample": 1.0,
+                                                    "dft_fold": 1.0 / 16}) if args.size == 256 and
+                     args.precision == "f32" and tile[0] >= 2 else None,
                      "kernel": "one step: " + " + ".join(k for k in avg if k != "aux_line"),
                      "algorithmic_bytes_per_step": abytes, "avg_ms": avg, "ms_per_step_by_kernel": per_step},
     }
