@@ -14,6 +14,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "../../include/fdtd_hip.h"
@@ -848,6 +849,55 @@ int dft_flush_impl(fdtd_grid* g) {
 
 int dft_flush(fdtd_grid* g) { return g->f64 ? dft_flush_impl<double>(g) : dft_flush_impl<float>(g); }
 
+// Copy of every device allocation of a grid (fields, psi, dispersive state,
+// DFT accumulators and samples, aux line, probe trace) and of the host-side
+// step state, so that trial steps can be undone.
+struct GridSnapshot {
+  char* buf = nullptr;
+  std::vector<void*> ptrs;
+  std::vector<size_t> sizes;
+  long step = 0, dft_done = 0, nswap = 0;
+};
+
+int snapshot_take(fdtd_grid* g, GridSnapshot& s) {
+  int rc = 0;
+  // pending table uploads go first: the snapshot must hold the tables the
+  // trials will run with, or the restore would roll them back to stale contents
+  if (g->mats_dirty && (rc = g->f64 ? upload_materials<double>(g) : upload_materials<float>(g))) return rc;
+  if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  s.ptrs = g->allocs;
+  s.sizes = g->alloc_bytes;
+  size_t total = 0;
+  for (size_t b : s.sizes) total += (b + 255) & ~size_t(255);
+  if (hipMalloc((void**)&s.buf, total) != hipSuccess) return fail(FDTD_ENOMEM, "snapshot (%zu bytes)", total);
+  size_t off = 0;
+  for (size_t q = 0; q < s.ptrs.size(); q++) {
+    HIP_TRY(hipMemcpyAsync(s.buf + off, s.ptrs[q], s.sizes[q], hipMemcpyDeviceToDevice, g->stream));
+    off += (s.sizes[q] + 255) & ~size_t(255);
+  }
+  s.step = g->step;
+  s.dft_done = g->dft_done;
+  s.nswap = g->nswap;
+  return 0;
+}
+
+void snapshot_restore(fdtd_grid* g, GridSnapshot& s) {
+  if (!s.buf) return;
+  size_t off = 0;
+  for (size_t q = 0; q < s.ptrs.size(); q++) {
+    (void)hipMemcpyAsync(s.ptrs[q], s.buf + off, s.sizes[q], hipMemcpyDeviceToDevice, g->stream);
+    off += (s.sizes[q] + 255) & ~size_t(255);
+  }
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(s.buf);
+  s.buf = nullptr;
+  // fused trials swap the roles of the two state copies; put them back
+  if ((g->nswap - s.nswap) % 2) swap_state(g);
+  g->step = s.step;
+  g->dft_done = s.dft_done;
+}
+
 bool valid_comp(const fdtd_grid* g, int field, int comp) {
   if (comp < 0 || comp > 2) return false;
   if (field == FDTD_E || field == FDTD_D || field == FDTD_P || field == FDTD_J) return g->actE[comp];
@@ -893,13 +943,13 @@ struct AdjointScope {
   explicit AdjointScope(fdtd_grid* gg) : g(gg), main(gg->stream) {}
   ~AdjointScope() {
     g->stream = main;
-    (void)hipStreamSynchronize(g->stream);
     if (rs) {
       (void)hipStreamSynchronize(rs);
       (void)hipStreamDestroy(rs);
     }
+    for (void* p : ptrs) (void)hipFreeAsync(p, g->stream);
+    (void)hipStreamSynchronize(g->stream);
     for (auto e : evs) (void)hipEventDestroy(e);
-    for (void* p : ptrs) g->release(p);
     g->force_two_pass = false;
     g->recompute = false;
     g->hist = nullptr;
@@ -907,10 +957,18 @@ struct AdjointScope {
     g->inc_allE = g->inc_allH = nullptr;
     g->inc_rows = 0;
   }
+  // Temporaries come from the stream-ordered allocator: hipMalloc / hipFree
+  // synchronise the whole device and would serialise adjoints of several
+  // designs running side by side on their own streams.
   int get(void** p, size_t bytes) {
-    int rc = g->alloc(p, bytes);
-    if (!rc) ptrs.push_back(*p);
-    return rc;
+    if (bytes == 0) bytes = 16;
+    if (hipMallocAsync(p, bytes, g->stream) != hipSuccess) {
+      *p = nullptr;
+      return fail(FDTD_ENOMEM, "hipMallocAsync(%zu) failed", bytes);
+    }
+    ptrs.push_back(*p);
+    HIP_TRY(hipMemsetAsync(*p, 0, bytes, g->stream));
+    return 0;
   }
   int event(hipEvent_t* e) {
     if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) return fail(FDTD_EDEVICE, "hipEventCreate");
@@ -1020,6 +1078,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   }
   void* wdev = nullptr;
   if ((rc = sc.get(&wdev, w.size() * g->esz))) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));  // the stream-ordered allocation exists before the copy
   if ((rc = upload_T(g, wdev, w.data(), w.size()))) return rc;
   gput(q.obj_w, (const T*)wdev);
 
@@ -1042,6 +1101,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   void *adev = nullptr, *qdev = nullptr;
   if ((rc = sc.get(&adev, sizeof a))) return rc;
   if ((rc = sc.get(&qdev, sizeof q))) return rc;
+  HIP_TRY(hipStreamSynchronize(g->stream));  // the stream-ordered allocations exist before the copies
   HIP_TRY(hipMemcpy(adev, &a, sizeof a, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(qdev, &q, sizeof q, hipMemcpyHostToDevice));
   const auto* ap = (const fdtdk::StepArgs<T>*)adev;
@@ -1471,34 +1531,16 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   if (!g || trial_steps < 1) return fail(FDTD_EINVAL, "bad argument");
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
-  static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {128, 2, 1}, {128, 2, 2}, {64, 4, 1},
-                               {0, 2, 16},  {0, 3, 16},  {1, 2, 16},  {1, 2, 32},  {1, 4, 8},   {1, 4, 16},
-                               {1, 4, 32},  {1, 6, 16}};
+  static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {256, 1, 8}, {128, 2, 2}, {128, 2, 4},
+                               {64, 4, 1},  {64, 4, 2},  {0, 2, 16},  {0, 3, 16},  {1, 2, 8},   {1, 2, 16},
+                               {1, 4, 8},   {1, 4, 16}};
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;
   const int ncand = g->ndim == 3 ? int(sizeof cand3 / sizeof cand3[0]) : int(sizeof cand2 / sizeof cand2[0]);
-  const long step0 = g->step, dft_done0 = g->dft_done;
-  if ((rc = ensure_trace(g, step0 + long(ncand) * (trial_steps + 1)))) return rc;
-  // pending table uploads go first: the snapshot must hold the tables the
-  // trials will run with, or the restore would roll them back to stale contents
-  if (g->mats_dirty && (rc = g->f64 ? upload_materials<double>(g) : upload_materials<float>(g))) return rc;
-  if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
+  if ((rc = ensure_trace(g, g->step + long(ncand) * (trial_steps + 1)))) return rc;
   if (g->ndim == 3 && (rc = ensure_pingpong(g))) return rc;
-  const long swaps0 = g->nswap;
-  // snapshot every device allocation (fields, psi, dispersive state, DFT
-  // accumulators, aux line, probe trace) so the trials leave no trace
-  HIP_TRY(hipStreamSynchronize(g->stream));
-  std::vector<void*> ptrs = g->allocs;
-  std::vector<size_t> sizes = g->alloc_bytes;
-  size_t total = 0;
-  for (size_t s : sizes) total += (s + 255) & ~size_t(255);
-  char* snap = nullptr;
-  if (hipMalloc((void**)&snap, total) != hipSuccess) return fail(FDTD_ENOMEM, "autotune snapshot (%zu bytes)", total);
-  size_t off = 0;
-  for (size_t q = 0; q < ptrs.size(); q++) {
-    (void)hipMemcpyAsync(snap + off, ptrs[q], sizes[q], hipMemcpyDeviceToDevice, g->stream);
-    off += (sizes[q] + 255) & ~size_t(255);
-  }
+  GridSnapshot snap;
+  if ((rc = snapshot_take(g, snap))) return rc;
   const bool prof = g->prof;
   double kms[kNumKernels];
   long kcount[kNumKernels];
@@ -1512,12 +1554,12 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     if ((rc = fdtd_step(g, 1))) break;
     if ((rc = fdtd_set_profiling(g, 1))) break;
     if ((rc = fdtd_step(g, trial_steps))) break;
-    double t = 0;
+    double t = 0;  // device time of the step's kernels per step
     for (int kid : {0, 1, 3, 4}) {
       double ms = 0;
       long nl = 0;
       if ((rc = fdtd_kernel_time(g, kid, &ms, &nl))) break;
-      t += ms / std::max(nl, 1L);
+      t += ms / trial_steps;
     }
     if (rc) break;
     if (t < best) {
@@ -1526,17 +1568,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     }
   }
   (void)fdtd_set_profiling(g, 0);
-  off = 0;
-  for (size_t q = 0; q < ptrs.size(); q++) {
-    (void)hipMemcpyAsync(ptrs[q], snap + off, sizes[q], hipMemcpyDeviceToDevice, g->stream);
-    off += (sizes[q] + 255) & ~size_t(255);
-  }
-  (void)hipStreamSynchronize(g->stream);
-  (void)hipFree(snap);
-  // fused trials swap the roles of the two state copies; put them back
-  if ((g->nswap - swaps0) % 2) swap_state(g);
-  g->step = step0;
-  g->dft_done = dft_done0;
+  snapshot_restore(g, snap);
   g->prof = prof;
   std::copy(kms, kms + kNumKernels, g->kms);
   std::copy(kcount, kcount + kNumKernels, g->kcount);
@@ -1632,6 +1664,66 @@ int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, do
 }
 
 
+int fdtd_adjoint_eps_gradient_batch(fdtd_grid* const* grids, int n, int nsteps, int checkpoint_every,
+                                    double history_mb, int dft, int comp_mask, double* objectives,
+                                    double* const* grads) {
+  if (!grids || n < 1 || !objectives || !grads) return fail(FDTD_EINVAL, "null argument");
+  for (int q = 0; q < n; q++) {
+    if (!grids[q] || !grads[q]) return fail(FDTD_EINVAL, "null grid or gradient buffer in batch");
+    for (int r = 0; r < q; r++)
+      if (grids[r] == grids[q]) return fail(FDTD_EINVAL, "a grid appears twice in the batch");
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  // one host thread and one non-blocking stream per design, so the designs'
+  // kernels share the GPU (a 128^3 design leaves most of it idle)
+  std::vector<hipStream_t> own(n, nullptr), saved(n, nullptr);
+  int rc = 0;
+  for (int q = 0; q < n && !rc; q++) {
+    saved[q] = grids[q]->stream;
+    if (!saved[q]) {
+      if (hipStreamCreateWithFlags(&own[q], hipStreamNonBlocking) != hipSuccess)
+        rc = fail(FDTD_EDEVICE, "hipStreamCreateWithFlags failed");
+      else
+        grids[q]->stream = own[q];
+    }
+  }
+  // device allocations that synchronise the device happen here, before the
+  // designs run side by side
+  for (int q = 0; q < n && !rc; q++) {
+    if (!grids[q]->finalized) rc = finalize(grids[q]);
+    if (!rc && grids[q]->ndim == 3) rc = ensure_pingpong(grids[q]);
+  }
+  std::vector<int> rcs(n, 0);
+  std::vector<std::string> errs(n);
+  if (!rc) {
+    std::vector<std::thread> th;
+    for (int q = 0; q < n; q++)
+      th.emplace_back([&, q]() {
+        if (hipSetDevice(dev) != hipSuccess) {
+          rcs[q] = FDTD_EDEVICE;
+          errs[q] = "hipSetDevice failed";
+          return;
+        }
+        rcs[q] = fdtd_adjoint_eps_gradient(grids[q], nsteps, checkpoint_every, history_mb, dft, comp_mask,
+                                           &objectives[q], grads[q]);
+        if (rcs[q]) errs[q] = g_err;
+      });
+    for (auto& t : th) t.join();
+  }
+  for (int q = 0; q < n; q++) {
+    if (own[q]) {
+      (void)hipStreamSynchronize(own[q]);
+      (void)hipStreamDestroy(own[q]);
+    }
+    grids[q]->stream = saved[q];
+  }
+  if (rc) return rc;
+  for (int q = 0; q < n; q++)
+    if (rcs[q]) return fail(rcs[q], "design %d: %s", q, errs[q].c_str());
+  return 0;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------- batches
@@ -1672,6 +1764,30 @@ struct fdtd_batch {
 };
 
 namespace {
+
+// (Re)build the tile table of a batch for tile t: every grid's two-pass
+// tiles, grid by grid; captured graphs refer to the old table and are dropped.
+int batch_tiles(fdtd_batch* b, Tile t) {
+  std::vector<int4> tab;
+  for (size_t q = 0; q < b->grids.size(); q++) {
+    const fdtd_grid* g = b->grids[q];
+    const int tx = (g->d.nx + t.bx - 1) / t.bx, ty = (g->d.ny - 2 + t.by - 1) / t.by;
+    const int tz = (g->d.nz - 2 + t.kc - 1) / t.kc;
+    for (int z = 0; z < tz; z++)
+      for (int y = 0; y < ty; y++)
+        for (int x = 0; x < tx; x++) tab.push_back(make_int4(int(q), x, y, z));
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+  if (b->tiles) (void)hipFree(b->tiles);
+  b->tiles = nullptr;
+  if (hipMalloc((void**)&b->tiles, tab.size() * sizeof(int4)) != hipSuccess) return fail(FDTD_ENOMEM, "tile table");
+  HIP_TRY(hipMemcpy(b->tiles, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
+  b->ntiles = int(tab.size());
+  b->tile = t;
+  return 0;
+}
 
 template <typename T>
 int batch_step_impl(fdtd_batch* b, int nsteps) {
@@ -1766,7 +1882,6 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
   b->f64 = grids[0]->f64;
   int rc = 0;
   if (hipStreamCreate(&b->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipStreamCreate failed");
-  std::vector<int4> tab;
   for (int q = 0; q < n && !rc; q++) {
     fdtd_grid* g = grids[q];
     if (!rc) rc = dft_flush(g);  // on the grid's own stream, finished before the batch's stream takes over
@@ -1775,13 +1890,8 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
     b->saved_streams.push_back(g->stream);
     g->stream = b->stream;
     if (!g->finalized) rc = finalize(g);
-    const int tx = (g->d.nx + b->tile.bx - 1) / b->tile.bx, ty = (g->d.ny - 2 + b->tile.by - 1) / b->tile.by;
-    const int tz = (g->d.nz - 2 + b->tile.kc - 1) / b->tile.kc;
-    for (int z = 0; z < tz; z++)
-      for (int y = 0; y < ty; y++)
-        for (int x = 0; x < tx; x++) tab.push_back(make_int4(q, x, y, z));
   }
-  b->ntiles = int(tab.size());
+  if (!rc) rc = batch_tiles(b, b->tile);
   std::vector<long> offs(1, 0);
   for (int q = 0; q < n; q++) {
     long w = 0;
@@ -1792,9 +1902,6 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
   if (!rc && hipMalloc((void**)&b->dft_offs, offs.size() * sizeof(long)) != hipSuccess) rc = fail(FDTD_ENOMEM, "offsets");
   if (!rc && hipMemcpy(b->dft_offs, offs.data(), offs.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(FDTD_EDEVICE, "offset upload");
-  if (!rc && hipMalloc((void**)&b->tiles, tab.size() * sizeof(int4)) != hipSuccess) rc = fail(FDTD_ENOMEM, "tile table");
-  if (!rc && hipMemcpy(b->tiles, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
-    rc = fail(FDTD_EDEVICE, "tile table upload");
   const size_t abytes = size_t(fdtd_batch::kRing) * n * sizeof(fdtdk::StepArgs<double>);
   if (!rc && hipMalloc((void**)&b->args_dev, abytes) != hipSuccess) rc = fail(FDTD_ENOMEM, "batch arguments");
   if (!rc && hipHostMalloc((void**)&b->args_host, abytes) != hipSuccess) rc = fail(FDTD_ENOMEM, "batch staging");
@@ -1810,6 +1917,60 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
 
 int fdtd_batch_destroy(fdtd_batch* b) {
   delete b;
+  return 0;
+}
+
+int fdtd_batch_set_tile(fdtd_batch* b, int bx, int by, int kc) {
+  if (!b) return fail(FDTD_EINVAL, "null batch");
+  if (bx < 2 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
+    return fail(FDTD_EINVAL, "batch tiles are two-pass tiles: bx >= 2, bx*by a multiple of 64 up to 256, kc >= 1");
+  return batch_tiles(b, Tile{bx, by, kc});
+}
+
+int fdtd_batch_autotune(fdtd_batch* b, int trial_steps, int* bx, int* by, int* kc) {
+  if (!b || trial_steps < 1) return fail(FDTD_EINVAL, "bad argument");
+  static const Tile cand[] = {{64, 4, 1}, {64, 4, 2}, {64, 2, 4}, {128, 2, 2}, {128, 2, 4}, {128, 1, 4},
+                              {256, 1, 2}, {256, 1, 4}};
+  int rc = 0;
+  std::vector<GridSnapshot> snaps(b->grids.size());
+  for (size_t q = 0; q < b->grids.size() && !rc; q++) {
+    fdtd_grid* g = b->grids[q];
+    if (!rc) rc = ensure_trace(g, g->step + long(sizeof cand / sizeof cand[0]) * (trial_steps + 1));
+    if (!rc) rc = snapshot_take(g, snaps[q]);
+  }
+  const Tile saved = b->tile;
+  const bool graph = b->use_graph;
+  b->use_graph = false;  // no graphs for one-off trial tiles
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(FDTD_EDEVICE, "hipEventCreate failed");
+  double best = 1e300;
+  Tile bt = saved;
+  for (const Tile& t : cand) {
+    if (rc) break;
+    if ((rc = batch_tiles(b, t))) break;
+    if ((rc = fdtd_batch_step(b, 1))) break;
+    if (hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventRecord");
+    if (!rc) rc = fdtd_batch_step(b, trial_steps);
+    if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventRecord");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventSynchronize");
+    float ms = 0;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventElapsedTime");
+    if (!rc && ms < best) {
+      best = ms;
+      bt = t;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (size_t q = 0; q < b->grids.size(); q++) snapshot_restore(b->grids[q], snaps[q]);
+  b->use_graph = graph;
+  int rc2 = batch_tiles(b, rc ? saved : bt);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (bx) *bx = bt.bx;
+  if (by) *by = bt.by;
+  if (kc) *kc = bt.kc;
   return 0;
 }
 

@@ -27,7 +27,8 @@ ABI_SYMBOLS = [
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
-    "fdtd_set_design", "fdtd_adjoint_eps_gradient",
+    "fdtd_set_design", "fdtd_adjoint_eps_gradient", "fdtd_adjoint_eps_gradient_batch",
+    "fdtd_batch_set_tile", "fdtd_batch_autotune",
 ]
 
 
@@ -82,8 +83,12 @@ def load_library():
         "fdtd_batch_set_graph": (_i, [_p, _i]),
         "fdtd_batch_step": (_i, [_p, _i]),
         "fdtd_batch_synchronize": (_i, [_p]),
+        "fdtd_batch_set_tile": (_i, [_p, _i, _i, _i]),
+        "fdtd_batch_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
         "fdtd_set_design": (_i, [_p, _ip, _dp]),
         "fdtd_adjoint_eps_gradient": (_i, [_p, _i, _i, _d, _i, _i, _dp, _dp]),
+        "fdtd_adjoint_eps_gradient_batch": (_i, [ctypes.POINTER(_p), _i, _i, _i, _d, _i, _i, _dp,
+                                                 ctypes.POINTER(_dp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -262,6 +267,19 @@ class Grid:
         return obj.value, grad.reshape(self.design_shape)
 
 
+def adjoint_eps_gradient_batch(grids, nsteps, dft, comp_mask=4, checkpoint_every=100, history_mb=4096.0):
+    """Gradients of several designs (grids at step 0 with a design box each) computed concurrently;
+    returns a list of (J, dJ/d eps_r) in grid order."""
+    n = len(grids)
+    hs = (_p * n)(*[g.h for g in grids])
+    grads = [np.zeros(int(np.prod(g.design_shape))) for g in grids]
+    gp = (_dp * n)(*[_dptr(x) for x in grads])
+    obj = np.zeros(n)
+    _check(load_library().fdtd_adjoint_eps_gradient_batch(hs, n, nsteps, checkpoint_every, float(history_mb), dft,
+                                                          comp_mask, _dptr(obj), gp))
+    return [(float(obj[q]), grads[q].reshape(grids[q].design_shape)) for q in range(n)]
+
+
 class Batch:
     """Independent 3-D grids stepped together (one launch per pass for all,
     hipGraph-captured chunks).  The grids must outlive the batch."""
@@ -281,6 +299,15 @@ class Batch:
 
     def synchronize(self):
         _check(load_library().fdtd_batch_synchronize(self.h))
+
+    def set_tile(self, bx, by, kc):
+        _check(load_library().fdtd_batch_set_tile(self.h, bx, by, kc))
+
+    def autotune(self, trial_steps=10):
+        """Pick the fastest tile by timing whole batch steps; the grids' states are left unchanged."""
+        v = [ctypes.c_int() for _ in range(3)]
+        _check(load_library().fdtd_batch_autotune(self.h, trial_steps, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def close(self):
         if getattr(self, "h", None):

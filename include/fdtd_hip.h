@@ -126,9 +126,14 @@ int fdtd_synchronize(fdtd_grid* g);
 /* steps taken so far */
 long fdtd_step_count(const fdtd_grid* g);
 
-/* Tile of the per-step passes: bx*by threads per block (a multiple of 64,
- * at most 256) covering bx x by cells of an xy plane, each thread marching kc
- * z planes.  Results do not depend on the tile; only speed does. */
+/* Schedule and tile of the per-step kernels.  bx >= 2: two passes (E, then
+ * H) with bx*by threads per block (a multiple of 64, at most 256) covering
+ * bx x by cells of an xy plane, each thread marching kc z planes.  bx = 0: one
+ * fused pass per step, workgroups of 254 columns x `by` rows (1..3) sharing
+ * neighbours in LDS.  bx = 1: one fused pass per step, each wavefront owning
+ * 62 columns x `by` rows (2, 4 or 6) and exchanging neighbours by lane
+ * shuffles.  Fused passes keep two copies of the stepped state.  Results do
+ * not depend on the schedule or tile; only speed does. */
 int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc);
 int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc);
 /* Times each candidate tile over trial_steps steps, keeps the fastest and
@@ -168,6 +173,14 @@ int fdtd_set_design(fdtd_grid* g, const int box[6], const double* eps_r);
  * non-dispersive grids only. */
 int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, double history_mb, int dft,
                               int comp_mask, double* objective, double* grad);
+/* The same gradient for n independent grids (designs) at once: each runs on
+ * its own host thread and stream (a non-blocking one is made for a grid on
+ * the default stream), so the designs share the GPU.  objectives[q] and
+ * grads[q] receive grid q's results; per design they equal single calls.
+ * No counterpart in the reference. */
+int fdtd_adjoint_eps_gradient_batch(fdtd_grid* const* grids, int n, int nsteps, int checkpoint_every,
+                                    double history_mb, int dft, int comp_mask, double* objectives,
+                                    double* const* grads);
 
 /* Batches: independent 3-D grids of one precision stepped together, one
  * launch per pass covering the tiles of every grid, each chunk of steps
@@ -179,6 +192,12 @@ typedef struct fdtd_batch fdtd_batch;
 int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out);
 int fdtd_batch_destroy(fdtd_batch* b);
 int fdtd_batch_set_graph(fdtd_batch* b, int on);
+/* Two-pass tile of the batch's passes (as fdtd_set_tile with bx >= 2; the
+ * default is 64 x 4 x 2). */
+int fdtd_batch_set_tile(fdtd_batch* b, int bx, int by, int kc);
+/* Times candidate tiles over trial_steps steps of the whole batch, keeps the
+ * fastest and restores every grid's device state and step count. */
+int fdtd_batch_autotune(fdtd_batch* b, int trial_steps, int* bx, int* by, int* kc);
 int fdtd_batch_step(fdtd_batch* b, int nsteps);
 int fdtd_batch_synchronize(fdtd_batch* b);
 

@@ -230,7 +230,8 @@ def test_fused_pass_against_oracle(precision):
 @pytest.mark.parametrize("graph", [True, False])
 def test_batch_equals_standalone(graph):
     """Grids of uneven sizes stepped as one batch (tile table, hipGraph chunks)
-    give the same fields, DFTs and probes as each grid stepped on its own."""
+    give the same fields, DFTs and probes as each grid stepped on its own,
+    whatever tiles the batch uses and after its autotuner ran."""
     from fdtd_amd import Batch
     scs = synth.batch_c4(nsims=5, nsteps=300, lo=28, hi=44, seed=11)
     tf = synth.grid3d_cpml_tfsf_dft(n=(34, 30, 26), nsteps=300)  # one member with TFSF + probes
@@ -243,8 +244,11 @@ def test_batch_equals_standalone(graph):
     batched = [apply(_grid(sc, "f32"), sc) for sc in scs]
     print(f"[gpu-case] batch graph={graph}", flush=True)
     b = Batch(batched, graph=graph)
+    b.autotune(trial_steps=3)  # trial steps must leave every grid as it was
     b.step(7)
+    b.set_tile(128, 2, 4)  # the tile changes speed only
     b.step(256)
+    b.set_tile(64, 4, 2)
     b.step(37)
     b.synchronize()
     b.close()
