@@ -123,6 +123,24 @@ constexpr Tile kDefaultTileTM = {64, 4, 1};
 
 }  // namespace
 
+// what a captured chunk depends on beyond the staged arguments: the argument
+// slot, the chunk length, the step phase within a DFT block, the schedule and
+// the DFT launch sizes
+struct GraphKey {
+  int slot, chunk, phase, bx, by, kc;
+  long dft_work, dft_cells;
+  bool operator<(const GraphKey& o) const {
+    if (slot != o.slot) return slot < o.slot;
+    if (chunk != o.chunk) return chunk < o.chunk;
+    if (phase != o.phase) return phase < o.phase;
+    if (bx != o.bx) return bx < o.bx;
+    if (by != o.by) return by < o.by;
+    if (kc != o.kc) return kc < o.kc;
+    if (dft_work != o.dft_work) return dft_work < o.dft_work;
+    return dft_cells < o.dft_cells;
+  }
+};
+
 struct MatHost {
   double eps = 1.0, mu = 1.0;
   int np = 0;
@@ -236,6 +254,11 @@ struct fdtd_grid {
   bool args_ev_used[kRing] = {};
   int args_slot = 0;
 
+  // captured chunks (fdtd_set_graph)
+  bool use_graph = false;
+  hipStream_t own_stream = nullptr;  // made for graph capture when the grid was on the default stream
+  std::map<GraphKey, hipGraphExec_t> graphs;
+
   // profiling
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -245,6 +268,8 @@ struct fdtd_grid {
   long kcount[kNumKernels] = {};
 
   ~fdtd_grid() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
     for (void* p : allocs) (void)hipFree(p);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     for (auto e : args_ev)
@@ -717,6 +742,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
     if (rc) return rc;
   }
   const int nzi = g->ndim == 3 ? g->d.nz - 2 : 1;
+  const bool graphs = g->use_graph && !g->prof;
   dim3 block, grid;
   int ntx = 1, nty = 1, nkc = 1;
   if (wavef) {
@@ -757,9 +783,11 @@ int step_impl(fdtd_grid* g, int nsteps) {
       if (rc) return rc;
     }
     // stage this chunk's arguments: variant 0 reads the current state,
-    // variant 1 (fused pass only) the second copy
-    const int slot = g->args_slot;
-    g->args_slot = (slot + 1) % fdtd_grid::kRing;
+    // variant 1 (fused pass only) the second copy; captured chunks alternate
+    // between two slots so that their graphs are replayed
+    const int slot = graphs ? g->args_slot % 2 : g->args_slot;
+    g->args_slot = graphs ? (slot + 1) % 2 : (slot + 1) % fdtd_grid::kRing;
+    a.dft_done0 = g->dft_done;
     if (g->args_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->args_ev[slot]));
     auto* host = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
     auto* host1 = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
@@ -776,43 +804,73 @@ int step_impl(fdtd_grid* g, int nsteps) {
         reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
     HIP_TRY(hipMemcpyAsync((void*)dev0, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
     const long dft_work = host->dft_work, dft_cells = host->dft_cells;
-    for (int s = 0; s < chunk; s++) {
-      int rc;
-      const auto* ap = (fused && (s % 2)) ? dev1 : dev0;
-      if (wavef) {
-        if (t.by == 2)
-          rc = launch(g, 3, fdtdk::fusedw_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-        else if (t.by == 4)
-          rc = launch(g, 3, fdtdk::fusedw_3d<T, 4>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-        else
-          rc = launch(g, 3, fdtdk::fusedw_3d<T, 6>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-        swap_state(g);
-      } else if (fused) {
-        if (t.by == 1)
-          rc = launch(g, 3, fdtdk::fused_3d<T, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-        else if (t.by == 2)
-          rc = launch(g, 3, fdtdk::fused_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-        else
-          rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-        swap_state(g);
-      } else if (g->ndim == 3) {
-        rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-        if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-      } else {
-        rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
-        if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
-      }
-      if (!rc && dft_work > 0) {
-        rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), ap, s);
-        if (!rc && (g->step + 1) % fdtdk::kDftBlock == 0) {
-          rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), ap, g->dft_done,
-                      g->step + 1);
-          g->dft_done = g->step + 1;
+    const long step0 = g->step;
+    // the chunk's launches; every step-dependent value comes from the staged
+    // arguments and the step index s, so the sequence can be captured once
+    // and replayed for later chunks of the same shape
+    auto enqueue = [&]() -> int {
+      for (int s = 0; s < chunk; s++) {
+        int rc;
+        const auto* ap = (fused && (s % 2)) ? dev1 : dev0;
+        if (wavef) {
+          if (t.by == 2)
+            rc = launch(g, 3, fdtdk::fusedw_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          else if (t.by == 4)
+            rc = launch(g, 3, fdtdk::fusedw_3d<T, 4>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          else
+            rc = launch(g, 3, fdtdk::fusedw_3d<T, 6>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        } else if (fused) {
+          if (t.by == 1)
+            rc = launch(g, 3, fdtdk::fused_3d<T, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          else if (t.by == 2)
+            rc = launch(g, 3, fdtdk::fused_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          else
+            rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        } else if (g->ndim == 3) {
+          rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+        } else {
+          rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
+          if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
         }
+        if (!rc && dft_work > 0) {
+          rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), ap, s);
+          if (!rc && (step0 + s + 1) % fdtdk::kDftBlock == 0)
+            rc = launch(g, 4, fdtdk::dft_fold_step<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), ap, s);
+        }
+        if (rc) return rc;
       }
+      return 0;
+    };
+    if (graphs) {
+      const GraphKey key{slot, chunk, int(step0 % fdtdk::kDftBlock), t.bx, t.by, t.kc, dft_work, dft_cells};
+      auto it = g->graphs.find(key);
+      if (it == g->graphs.end()) {
+        hipGraph_t graph;
+        HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue();
+        hipError_t e = hipStreamEndCapture(g->stream, &graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(FDTD_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        hipGraphExec_t exec;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(FDTD_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        it = g->graphs.emplace(key, exec).first;
+      }
+      HIP_TRY(hipGraphLaunch(it->second, g->stream));
+    } else {
+      int rc = enqueue();
       if (rc) return rc;
-      g->step++;
     }
+    // host-side effects of the chunk
+    if (fused && (chunk % 2)) swap_state(g);
+    if (fused) g->nswap += chunk - (chunk % 2);
+    if (dft_work > 0) {
+      const long last = ((step0 + chunk) / fdtdk::kDftBlock) * fdtdk::kDftBlock;  // last block end in the chunk
+      if (last > step0) g->dft_done = last;
+    }
+    g->step += chunk;
     HIP_TRY(hipEventRecord(g->args_ev[slot], g->stream));
     g->args_ev_used[slot] = true;
     done += chunk;
@@ -1609,6 +1667,19 @@ int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches) {
   if (rc) return rc;
   if (ms) *ms = g->kms[kernel];
   if (launches) *launches = g->kcount[kernel];
+  return 0;
+}
+
+int fdtd_set_graph(fdtd_grid* g, int on) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (on && !g->stream) {
+    // capture needs a stream other than the legacy default one; a blocking
+    // stream still orders itself with work on the default stream
+    if (!g->own_stream && hipStreamCreate(&g->own_stream) != hipSuccess)
+      return fail(FDTD_EDEVICE, "hipStreamCreate failed");
+    g->stream = g->own_stream;
+  }
+  g->use_graph = on != 0;
   return 0;
 }
 

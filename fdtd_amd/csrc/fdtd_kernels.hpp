@@ -125,6 +125,7 @@ struct StepArgs {
   int ndft;
   long dft_work;   // sum over monitors of nfreq x cells
   long dft_cells;  // sum over monitors of cells
+  long dft_done0;  // steps folded into the accumulators when this chunk started
   DftDev<T> dft[kMaxDft];
   // probes
   int nprobe;
@@ -545,8 +546,7 @@ __global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict_
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void dft_fold(const StepArgs<T>* __restrict__ ap, long m0, long m1) {
-  const StepArgs<T>& a = *ap;
+__device__ __forceinline__ void dft_fold_items(const StepArgs<T>& a, long m0, long m1) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.dft_work) return;
   int q = 0;
@@ -570,6 +570,21 @@ __global__ __launch_bounds__(256) void dft_fold(const StepArgs<T>* __restrict__ 
     re[0] = ar;
     re[d.ncells] = ai;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void dft_fold(const StepArgs<T>* __restrict__ ap, long m0, long m1) {
+  dft_fold_items(*ap, m0, m1);
+}
+
+// the fold after step s of a chunk whose step is the last of a block: the
+// range comes from the staged arguments, so a captured chunk can be replayed
+template <typename T>
+__global__ __launch_bounds__(256) void dft_fold_step(const StepArgs<T>* __restrict__ ap, int s) {
+  const StepArgs<T>& a = *ap;
+  const long m1 = a.step0 + s + 1;
+  const long m0 = m1 - kDftBlock > a.dft_done0 ? m1 - kDftBlock : a.dft_done0;
+  dft_fold_items(a, m0, m1);
 }
 
 // batched grids: `offs` holds each grid's first item (prefix sums, n+1 entries)

@@ -28,7 +28,7 @@ ABI_SYMBOLS = [
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
     "fdtd_set_design", "fdtd_adjoint_eps_gradient", "fdtd_adjoint_eps_gradient_batch",
-    "fdtd_batch_set_tile", "fdtd_batch_autotune",
+    "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph",
 ]
 
 
@@ -76,6 +76,7 @@ def load_library():
         "fdtd_kernel_time": (_i, [_p, _i, _dp, ctypes.POINTER(ctypes.c_long)]),
         "fdtd_last_error": (ctypes.c_char_p, []),
         "fdtd_set_tile": (_i, [_p, _i, _i, _i]),
+        "fdtd_set_graph": (_i, [_p, _i]),
         "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
         "fdtd_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
         "fdtd_batch_create": (_i, [ctypes.POINTER(_p), _i, ctypes.POINTER(_p)]),
@@ -228,6 +229,10 @@ class Grid:
 
     def set_tile(self, bx, by, kc):
         _check(load_library().fdtd_set_tile(self.h, bx, by, kc))
+
+    def set_graph(self, on=True):
+        """Capture chunks of steps in hipGraphs and replay them (fewer launches for small grids)."""
+        _check(load_library().fdtd_set_graph(self.h, 1 if on else 0))
 
     def get_tile(self):
         v = [ctypes.c_int() for _ in range(3)]

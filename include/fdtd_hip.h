@@ -142,6 +142,12 @@ int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc);
  * memory again).  The chosen tile is returned through bx, by, kc. */
 int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
 
+/* Capture each chunk of steps (up to 256) in a hipGraph once and replay it
+ * for later chunks of the same shape (off by default: plain launches).  Pays
+ * off where launches dominate (small grids).  A grid on the default stream
+ * gets a stream of its own for capture.  Results are unchanged. */
+int fdtd_set_graph(fdtd_grid* g, int on);
+
 /* Per-kernel timing with hipEvents on the grid's stream (off by default).
  * fdtd_kernel_time returns accumulated milliseconds and launches for
  * kernel id 0 = E pass, 1 = H pass, 2 = aux line, 3 = fused pass, 4 = DFT,

@@ -259,3 +259,39 @@ def test_batch_equals_standalone(graph):
         for idx in range(len(sc["dfts"])):
             np.testing.assert_array_equal(g.get_dft(idx), r.get_dft(idx))
     np.testing.assert_array_equal(batched[-1].probes(), alone[-1].probes())
+
+
+@pytest.mark.parametrize("tile", [(256, 1, 2), (1, 2, 8)])
+def test_graph_chunks_equal_plain_launches(tile):
+    """Chunks captured in hipGraphs and replayed give the fields, DFTs and
+    probes of plain launches, across uneven step calls and a DFT read-back
+    in the middle of a DFT block."""
+    sc = synth.grid3d_cpml_tfsf_dft(n=(40, 33, 29), nsteps=700, npml=4, inset=8)
+    print(f"[gpu-case] graph chunks {tile}", flush=True)
+    ref = apply(_grid(sc, "f32"), sc)
+    ref.set_tile(*tile)
+    ref.step(700)
+    g = apply(_grid(sc, "f32"), sc)
+    g.set_tile(*tile)
+    g.set_graph(True)
+    g.step(300)
+    g.step(13)
+    g.get_dft(0)
+    g.step(387)
+    for nm, f, c in _comps(sc):
+        np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=nm)
+    for idx in range(len(sc["dfts"])):
+        np.testing.assert_array_equal(g.get_dft(idx), ref.get_dft(idx))
+    np.testing.assert_array_equal(g.probes(), ref.probes())
+
+
+def test_graph_chunks_tm():
+    sc = synth.tm2d_c1(nsteps=600)
+    print("[gpu-case] graph chunks TM", flush=True)
+    ref = apply(_grid(sc, "f64"), sc)
+    ref.step(600)
+    g = apply(_grid(sc, "f64"), sc)
+    g.set_graph(True)
+    g.step(600)
+    np.testing.assert_array_equal(g.probes(), ref.probes())
+    np.testing.assert_array_equal(g.get(E, 2), ref.get(E, 2))
