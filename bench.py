@@ -75,18 +75,19 @@ def cpu_baseline(size, sample_steps):
 
 
 def timed_steps(g, steps):
-    """hipEvent-timed steps on the grid's stream; returns (ms_total, per-kernel ms per launch, per-kernel ms per
-    step).  The DFT entry covers the per-step sampling and the fold every 16 steps."""
+    """Times `steps` steps with no per-launch events (the measured value), then the same number of steps again
+    with a hipEvent pair around every kernel launch on the grid's stream; returns (ms_total of the first run,
+    per-kernel ms per launch, per-kernel ms per step).  The DFT entry covers the per-step sampling and the fold
+    every 16 steps."""
     import torch
     g.synchronize()
     torch.cuda.synchronize()
-    g.set_profiling(True)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record()
+    t0 = time.perf_counter()
     g.step(steps)
-    ev1.record()
-    torch.cuda.synchronize()
+    g.synchronize()
+    ms_total = (time.perf_counter() - t0) * 1e3
+    g.set_profiling(True)
+    g.step(steps)
     g.synchronize()
     avg, per_step = {}, {}
     for kid, name in ((0, "e_pass"), (1, "h_pass"), (2, "aux_line"), (3, "fused_pass"), (4, "dft")):
@@ -95,7 +96,7 @@ def timed_steps(g, steps):
             avg[name] = round(ms / n, 5)
             per_step[name] = round(ms / steps, 5)
     g.set_profiling(False)
-    return ev0.elapsed_time(ev1), avg, per_step
+    return ms_total, avg, per_step
 
 
 def base_line(args, metric_value, unit, ms_step, dtype, config, higher=True):
@@ -110,7 +111,7 @@ def run_c1(args):
     import synth
     from scenarios import apply
     import torch
-    sc = synth.tm2d_c1(nsteps=2 * (args.warmup + args.steps))
+    sc = synth.tm2d_c1(nsteps=3 * (args.warmup + args.steps))
     g = apply(Grid("TM", 256, 256, 1, sc["dx"], sc["dt"], precision="f64"), sc)
     tile = g.autotune(trial_steps=20)
     g.step(args.warmup)
@@ -140,7 +141,7 @@ def run_c3(args):
     import synth
     from scenarios import apply
     n = args.size if args.size != 256 else 192
-    sc = synth.grid3d_dispersive_sphere(n=n, radius=40 if n >= 160 else n // 5, nsteps=args.warmup + args.steps)
+    sc = synth.grid3d_dispersive_sphere(n=n, radius=40 if n >= 160 else n // 5, nsteps=args.warmup + 2 * args.steps)
     g = apply(Grid("ThreeD", n, n, n, sc["dx"], sc["dt"], precision=args.precision), sc)
     if args.tile:
         tile = tuple(int(v) for v in args.tile.split(","))
@@ -434,6 +435,7 @@ def main():
                          "c4: 64 batched grids vs sequential; c5: adjoint eps gradient 128^3; copy: 1 GiB device copy")
     ap.add_argument("--tile", default=None,
                     help="bx,by,kc: fixed tile instead of the autotuner (c2; bx = 0 selects the fused pass)")
+    ap.add_argument("--graph", action="store_true", help="capture chunks of steps in hipGraphs (c2)")
     ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench and of --workload copy into "
                          "HBM bytes per launch (no GPU work)")
@@ -459,7 +461,7 @@ def main():
         fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "copy": run_copy}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
-    total = args.warmup + args.steps
+    total = args.warmup + 2 * args.steps  # timed run + profiled rerun
     sc = synth.grid3d_cpml_tfsf_dft(n=args.size, nsteps=total)
     esz = 4 if args.precision == "f32" else 8
     g = apply(Grid("ThreeD", args.size, args.size, args.size, sc["dx"], sc["dt"], precision=args.precision), sc)
@@ -468,6 +470,8 @@ def main():
         g.set_tile(*tile)
     else:
         tile = g.autotune(trial_steps=30)
+    if args.graph:
+        g.set_graph(True)
     g.step(args.warmup)
     g.synchronize()
     torch.cuda.synchronize()
@@ -485,7 +489,8 @@ def main():
         "config": {"workload": f"3-D {args.size}^3 {args.precision.upper()} forward: 12-cell CPML + 48 eps blocks "
                                f"+ TFSF box inset 20 (x-propagating Ez plane wave) + running DFT of 6 comps on "
                                f"2 z-planes x 8 freqs",
-                   "cells": args.size ** 3, "interior_cells": interior, "tile": list(tile)},
+                   "cells": args.size ** 3, "interior_cells": interior, "tile": list(tile),
+                   "graph_chunks": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": step_traffic(tile, {"e_pass_3d": 1.0, "h_pass_3d": 1.0, "dft_sample": 1.0,

@@ -756,9 +756,13 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
 // Each XCD walks its tile range in reverse: its first tiles read the E and H
 // that its E pass touched last (still in its L2 / the Infinity Cache); the
 // next E pass then starts where this pass ended.
+//
+// FP32 instances are compiled for 7 waves per SIMD (the register count they
+// need anyway): at 256^3 on MI355X that schedule runs the H pass ~10 % faster
+// (0.151 vs 0.168 ms per step) than the compiler's default.
 template <typename T>
-__global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
-                                                 int nkc) {
+__device__ __forceinline__ void h_pass_3d_body(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
+                                               int nkc) {
   const StepArgs<T>& a = *ap;
   const int t = xcd_tile_rev(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
@@ -767,6 +771,17 @@ __global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__
   // measured slower here (the E pass, with its larger stencil, gains)
   h_pass_cells(a, a.mim, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
                s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
+                                                 int nkc) {
+  h_pass_3d_body(ap, kc, s, ntx, nty, nkc);
+}
+template <>
+__global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_3d<float>(
+    const StepArgs<float>* __restrict__ ap, int kc, int s, int ntx, int nty, int nkc) {
+  h_pass_3d_body(ap, kc, s, ntx, nty, nkc);
 }
 
 // ---------------------------------------------------------------- batched passes
@@ -787,13 +802,24 @@ __global__ __launch_bounds__(256) void e_pass_batch(const StepArgs<T>* __restric
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void h_pass_batch(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
-                                                    int kc, int s) {
+__device__ __forceinline__ void h_pass_batch_body(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
+                                                  int kc, int s) {
   const int4 t = tiles[xcd_tile_rev(blockIdx.x, gridDim.x)];
   const StepArgs<T>& a = args[t.x];
   const int k0 = 1 + t.w * kc;
   h_pass_cells(a, a.mim, t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
                s);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void h_pass_batch(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
+                                                    int kc, int s) {
+  h_pass_batch_body(args, tiles, kc, s);
+}
+// FP32: the single-grid H pass's 7-waves-per-SIMD schedule
+template <>
+__global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_batch<float>(
+    const StepArgs<float>* __restrict__ args, const int4* __restrict__ tiles, int kc, int s) {
+  h_pass_batch_body(args, tiles, kc, s);
 }
 
 // ---------------------------------------------------------------- fused pass
