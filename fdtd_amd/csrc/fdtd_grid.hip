@@ -109,7 +109,7 @@ void pml_idiom_rows(int L, int n, double m, double ma, double smax, double kmax,
 constexpr int kChunk = 256;  // steps per incident-table chunk
 // timed kernels: 0 E pass, 1 H pass, 2 aux line, 3 fused pass, 4 DFT,
 // 5 adjoint E pass, 6 adjoint H pass
-constexpr int kNumKernels = 7;
+constexpr int kNumKernels = 8;
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
 // bx == 0 selects the LDS fused single pass with `by` rows per workgroup,
@@ -230,6 +230,8 @@ struct fdtd_grid {
 
   // forward-step controls used by the adjoint
   bool force_two_pass = false;  // the design increments are recorded by the two-pass E pass
+  int bc[3][2] = {};            // boundary condition per [axis][side] (FDTD_BC_*)
+  int nbc = 0;                  // faces with a condition (they need the two-pass schedule)
   bool recompute = false;       // segment recompute: no DFT accumulation, no probes
   void* hist = nullptr;         // T[hist_rows][3][ndes]
   long hist_step0 = 0, hist_rows = 0;
@@ -581,6 +583,8 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   for (int s = 0; s < g->nsrc; s++) a.src[s] = g->src[s];
   a.tfsf = g->tfsf;
   for (int x = 0; x < 6; x++) a.box[x] = g->box[x];
+  for (int x = 0; x < 3; x++)
+    for (int sd = 0; sd < 2; sd++) a.bc[x][sd] = g->bc[x][sd];
   for (int x = 0; x < 3; x++) {
     a.kvec[x] = g->kvec[x];
     a.pol[x] = g->pol[x];
@@ -734,7 +738,19 @@ int step_impl(fdtd_grid* g, int nsteps) {
   fill_args<T>(g, a);
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (g->ndim == 2) t.kc = 1;
-  if (g->force_two_pass && t.bx <= 1) t = kDefaultTile3D;
+  // design recording and ghost-layer boundary writes run between the passes
+  if ((g->force_two_pass || g->nbc) && t.bx <= 1) t = kDefaultTile3D;
+  unsigned bc_blocks = 0;
+  if (g->nbc) {
+    long cells = 0;
+    for (int ax = 0; ax < g->ndim; ax++) {
+      long f = 1;
+      for (int t2 = 0; t2 < g->ndim; t2++)
+        if (t2 != ax) f *= g->len[t2] - 2;
+      cells += 2 * f;
+    }
+    bc_blocks = unsigned((cells + 255) / 256);
+  }
   const bool wavef = g->ndim == 3 && t.bx == 1;
   const bool fused = g->ndim == 3 && t.bx <= 1;
   if (fused) {
@@ -828,10 +844,14 @@ int step_impl(fdtd_grid* g, int nsteps) {
             rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         } else if (g->ndim == 3) {
           rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 0, 3);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 1, 3);
         } else {
           rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
+          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 0, 2);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
+          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 1, 2);
         }
         if (!rc && dft_work > 0) {
           rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), ap, s);
@@ -1613,7 +1633,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     if ((rc = fdtd_set_profiling(g, 1))) break;
     if ((rc = fdtd_step(g, trial_steps))) break;
     double t = 0;  // device time of the step's kernels per step
-    for (int kid : {0, 1, 3, 4}) {
+    for (int kid : {0, 1, 3, 4, 7}) {
       double ms = 0;
       long nl = 0;
       if ((rc = fdtd_kernel_time(g, kid, &ms, &nl))) break;
@@ -1667,6 +1687,18 @@ int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches) {
   if (rc) return rc;
   if (ms) *ms = g->kms[kernel];
   if (launches) *launches = g->kcount[kernel];
+  return 0;
+}
+
+int fdtd_set_boundary(fdtd_grid* g, int axis, int side, int kind) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (axis < 0 || axis >= g->ndim || side < 0 || side > 1) return fail(FDTD_EINVAL, "bad axis or side");
+  if (kind == FDTD_BC_BLOCH)
+    return fail(FDTD_EINVAL, "Bloch-periodic boundaries multiply by a complex phase: they need complex fields");
+  if (kind < FDTD_BC_NONE || kind > FDTD_BC_PERIODIC) return fail(FDTD_EINVAL, "bad boundary kind");
+  if (g->finalized) return fail(FDTD_ESTATE, "boundary conditions are fixed once stepping has started");
+  g->nbc += (kind != FDTD_BC_NONE) - (g->bc[axis][side] != FDTD_BC_NONE);
+  g->bc[axis][side] = kind;
   return 0;
 }
 
@@ -1730,6 +1762,7 @@ int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, do
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
   if (g->max_np > 0) return fail(FDTD_EINVAL, "the adjoint gradient does not cover dispersive materials");
+  if (g->nbc) return fail(FDTD_EINVAL, "the adjoint gradient does not cover boundary conditions");
   return g->f64 ? adjoint_impl<double>(g, nsteps, checkpoint_every, history_mb, dft, comp_mask, objective, grad)
                 : adjoint_impl<float>(g, nsteps, checkpoint_every, history_mb, dft, comp_mask, objective, grad);
 }
@@ -1946,6 +1979,7 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
     if (!grids[q]) return fail(FDTD_EINVAL, "null grid in batch");
     if (grids[q]->ndim != 3) return fail(FDTD_EINVAL, "batches hold 3-D grids");
     if (grids[q]->f64 != grids[0]->f64) return fail(FDTD_EINVAL, "batched grids must share one precision");
+    if (grids[q]->nbc) return fail(FDTD_EINVAL, "batched grids cannot have boundary conditions (step them alone)");
     for (int r = 0; r < q; r++)
       if (grids[r] == grids[q]) return fail(FDTD_EINVAL, "a grid appears twice in the batch");
   }

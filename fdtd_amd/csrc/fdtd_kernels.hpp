@@ -117,6 +117,8 @@ struct StepArgs {
   // TFSF: incident rows of the current chunk, row s = step - step0
   int tfsf;
   int box[6];
+  // boundary condition per [axis][side] (FDTD_BC_*, 0: ghost layer untouched)
+  int bc[3][2];
   double kvec[3], pol[3], polH[3], origin[3], dx;
   int auxL;
   const FDTD_G double* incE;  // aux Ez after each step's aux advance
@@ -782,6 +784,62 @@ template <>
 __global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_3d<float>(
     const StepArgs<float>* __restrict__ ap, int kc, int s, int ntx, int nty, int nkc) {
   h_pass_3d_body(ap, kc, s, ntx, nty, nkc);
+}
+
+// ---------------------------------------------------------------- boundary conditions
+// BoundaryUpdates.hpp: SingleUpdate::atomic_update (258-300) sets, for every
+// component of one field type, ghost = Mask * source, the ghost being the
+// boundary cell's outward neighbour; the source is the boundary cell itself
+// for PEC / PMC / Symmetric / Antisymmetric (masks 527-722) or its periodic
+// image (800-832, mask 1).  Placement (which the reference leaves to the
+// caller): ghost layer 0 / L-1, boundary layer 1 / L-2, periodic image of
+// ghost 0 layer L-2 and of ghost L-1 layer 1; tangential range the interior
+// cells.  One thread per ghost cell of the six faces (faces without a
+// condition return at once); the masks are +-1, so the copies are exact.
+// Launched after the E pass (ft 0) and after the H pass (ft 1).
+__device__ __forceinline__ int bc_sign(int kind, int ft, int comp, int axis) {
+  const bool along = comp == axis;
+  switch (kind) {
+    case 1: return ft == 0 ? (along ? 1 : -1) : (along ? -1 : 1);  // PEC
+    case 2: return ft == 0 ? (along ? -1 : 1) : (along ? 1 : -1);  // PMC
+    case 3: return along ? -1 : 1;                                 // Symmetric
+    case 4: return along ? 1 : -1;                                 // Antisymmetric
+    default: return 1;                                             // Periodic
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void bc_fill(const StepArgs<T>* __restrict__ ap, int ft, int ndim) {
+  const StepArgs<T>& a = *ap;
+  const int n[3] = {a.nx, a.ny, a.nz};
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (int ax = 0; ax < ndim; ax++) {
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    const long e1 = n[t1] - 2, e2 = t2 < ndim ? n[t2] - 2 : 1;
+    const long face = e1 * e2;
+    if (q >= 2 * face) {
+      q -= 2 * face;
+      continue;
+    }
+    const int side = int(q / face);
+    const long r = q % face;
+    const int kind = a.bc[ax][side];
+    if (!kind) return;
+    const int L = n[ax];
+    int cd[3], cs[3];
+    cd[ax] = side ? L - 1 : 0;
+    cs[ax] = kind == 5 ? (side ? 1 : L - 2) : (side ? L - 2 : 1);
+    cd[t1] = cs[t1] = 1 + int(r % e1);
+    cd[t2] = cs[t2] = t2 < ndim ? 1 + int(r / e1) : 0;
+    const long d = (long)cd[0] + a.sy * (long)cd[1] + a.sz * (long)cd[2];
+    const long s = (long)cs[0] + a.sy * (long)cs[1] + a.sz * (long)cs[2];
+    FDTD_G T* const* F = ft == 0 ? a.Eo : a.Ho;
+    for (int c = 0; c < 3; c++) {
+      if (ndim == 2 && (ft == 0) != (c == 2)) continue;  // TM: Ez, Hx, Hy
+      F[c][d] = bc_sign(kind, ft, c, ax) < 0 ? -F[c][s] : F[c][s];
+    }
+    return;
+  }
 }
 
 // ---------------------------------------------------------------- batched passes

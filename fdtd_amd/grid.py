@@ -18,6 +18,8 @@ TM, THREED = 2, 3
 F32, F64 = 0, 1
 D, E, B, H, P, J = 0, 1, 2, 3, 4, 5
 DRUDE, LORENTZ = 1, 2
+# boundary kinds (fdtd_boundary in include/fdtd_hip.h; names of the reference's fdtd::Boundary)
+BOUNDARY = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "Periodic": 5, "BlochPeriodic": 6}
 
 # every symbol declared in include/fdtd_hip.h
 ABI_SYMBOLS = [
@@ -28,7 +30,7 @@ ABI_SYMBOLS = [
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
     "fdtd_set_design", "fdtd_adjoint_eps_gradient", "fdtd_adjoint_eps_gradient_batch",
-    "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph",
+    "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph", "fdtd_set_boundary",
 ]
 
 
@@ -77,6 +79,7 @@ def load_library():
         "fdtd_last_error": (ctypes.c_char_p, []),
         "fdtd_set_tile": (_i, [_p, _i, _i, _i]),
         "fdtd_set_graph": (_i, [_p, _i]),
+        "fdtd_set_boundary": (_i, [_p, _i, _i, _i]),
         "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
         "fdtd_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
         "fdtd_batch_create": (_i, [ctypes.POINTER(_p), _i, ctypes.POINTER(_p)]),
@@ -229,6 +232,12 @@ class Grid:
 
     def set_tile(self, bx, by, kc):
         _check(load_library().fdtd_set_tile(self.h, bx, by, kc))
+
+    def set_boundary(self, axis, side, kind):
+        """Boundary condition of one face: kind is a name of BOUNDARY (the reference's
+        fdtd::Boundary kinds) or its number; axis 0..2, side 0 (min) / 1 (max)."""
+        k = BOUNDARY[kind] if isinstance(kind, str) else int(kind)
+        _check(load_library().fdtd_set_boundary(self.h, int(axis), int(side), k))
 
     def set_graph(self, on=True):
         """Capture chunks of steps in hipGraphs and replay them (fewer launches for small grids)."""

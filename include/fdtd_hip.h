@@ -43,6 +43,16 @@ enum fdtd_precision { FDTD_F32 = 0, FDTD_F64 = 1 };
 enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5 };
 /* material pole kinds */
 enum fdtd_pole { FDTD_DRUDE = 1, FDTD_LORENTZ = 2 };
+/* boundary conditions: the kinds of fdtd::Boundary (include/BoundaryUpdates.hpp:17-26) */
+enum fdtd_boundary {
+  FDTD_BC_NONE = 0,          /* ghost layer left alone (stays as uploaded; zero by default) */
+  FDTD_BC_PEC = 1,
+  FDTD_BC_PMC = 2,
+  FDTD_BC_SYMMETRIC = 3,
+  FDTD_BC_ANTISYMMETRIC = 4,
+  FDTD_BC_PERIODIC = 5,
+  FDTD_BC_BLOCH = 6          /* rejected: needs complex fields */
+};
 
 #define FDTD_MAX_MATERIALS 256
 #define FDTD_MAX_POLES 3
@@ -142,6 +152,18 @@ int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc);
  * memory again).  The chosen tile is returned through bx, by, kc. */
 int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
 
+/* Boundary condition on one face (axis 0..ndim-1, side 0 = min, 1 = max).
+ * Replaces make_pec_boundary / make_pmc_boundary / make_symmetric_boundary /
+ * make_antisymmetric_boundary / make_periodic_boundary + BoundaryData::
+ * updateE / updateH (include/BoundaryUpdates.hpp:527-832): after each E (H)
+ * update, every E (H) component of the face's ghost layer (index 0 or L-1)
+ * becomes the reference's mask (+-1 per component) times the same component
+ * of the adjacent interior layer (1 or L-2), or, periodic, of the layer one
+ * period away (L-2 for ghost 0, 1 for ghost L-1).  Tangential range: the
+ * interior cells.  Set before the first step; not with batches or the
+ * adjoint gradient (FDTD_EINVAL there). */
+int fdtd_set_boundary(fdtd_grid* g, int axis, int side, int kind);
+
 /* Capture each chunk of steps (up to 256) in a hipGraph once and replay it
  * for later chunks of the same shape (off by default: plain launches).  Pays
  * off where launches dominate (small grids).  A grid on the default stream
@@ -151,7 +173,7 @@ int fdtd_set_graph(fdtd_grid* g, int on);
 /* Per-kernel timing with hipEvents on the grid's stream (off by default).
  * fdtd_kernel_time returns accumulated milliseconds and launches for
  * kernel id 0 = E pass, 1 = H pass, 2 = aux line, 3 = fused pass, 4 = DFT,
- * 5 = adjoint E pass, 6 = adjoint H pass. */
+ * 5 = adjoint E pass, 6 = adjoint H pass, 7 = boundary-condition ghost writes. */
 int fdtd_set_profiling(fdtd_grid* g, int on);
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches);
 

@@ -687,6 +687,156 @@ static double gauss_pulse(int n, double t0, double spread) {
   return std::exp(-a * a);
 }
 
+// ------------------------------------------------------------ scenario: boundary updaters
+// The reference's own make_{pec,pmc,symmetric,antisymmetric,periodic}_boundary
+// (include/BoundaryUpdates.hpp:527-832) applied to the planes next to each face
+// of a small neighbour-wired 3-D grid with seeded E and H.  Boundary cells are
+// the interior cells of layer 1 (min) / L-2 (max); their outward neighbours
+// (layer 0 / L-1) receive the update; the periodic image of layer 1 is layer
+// L-2 and vice versa.  Bloch-periodic needs complex fields and is left out.
+template <class C>
+struct CellRefIt {
+  typedef std::ptrdiff_t difference_type;
+  typedef C value_type;
+  typedef C& reference;
+  typedef C* pointer;
+  typedef std::forward_iterator_tag iterator_category;
+  C** p;
+  C& operator*() const { return **p; }
+  C* operator->() const { return *p; }
+  CellRefIt& operator++() {
+    ++p;
+    return *this;
+  }
+  CellRefIt operator++(int) {
+    CellRefIt t = *this;
+    ++p;
+    return t;
+  }
+  bool operator!=(const CellRefIt& o) const { return p != o.p; }
+  bool operator==(const CellRefIt& o) const { return p == o.p; }
+};
+struct PeriodicImage {
+  Cell3** base;
+  Cell3** img;
+  Cell3& operator()(const CellRefIt<Cell3>& it) const { return *img[it.p - base]; }
+};
+
+template <Dir d, Orientation o>
+static BoundaryData make_bc(int type, CellRefIt<Cell3> b, CellRefIt<Cell3> e, const PeriodicImage& pf) {
+  switch (type) {
+    case 1: return make_pec_boundary<ThreeD, d, o>(b, e);
+    case 2: return make_pmc_boundary<ThreeD, d, o>(b, e);
+    case 3: return make_symmetric_boundary<ThreeD, d, o>(b, e);
+    case 4: return make_antisymmetric_boundary<ThreeD, d, o>(b, e);
+    default: return make_periodic_boundary<ThreeD, d, o>(b, e, pf);
+  }
+}
+
+static void scen_boundaries(const std::string& dir) {
+  const int n[3] = {7, 6, 5};
+  const int nx = n[0], ny = n[1], nz = n[2];
+  std::vector<Cell3> cells(size_t(nx) * ny * nz);
+  auto at = [&](int i, int j, int k) -> Cell3& { return cells[size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k)]; };
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        Cell3& c = at(i, j, k);
+        const int ci[3] = {i, j, k};
+        for (int a = 0; a < 3; a++) {
+          int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+          lo[a]--;
+          hi[a]++;
+          c.mn[a] = ci[a] > 0 ? &at(lo[0], lo[1], lo[2]) : nullptr;
+          c.mx[a] = ci[a] < n[a] - 1 ? &at(hi[0], hi[1], hi[2]) : nullptr;
+        }
+      }
+  std::vector<double> init;
+  auto seed = [&]() {
+    SplitMix64 rng(11);
+    init.clear();
+    for (auto& c : cells) {
+      c.Ex() = rng.uniform() - 0.5;
+      c.Ey() = rng.uniform() - 0.5;
+      c.Ez() = rng.uniform() - 0.5;
+      c.Hx() = rng.uniform() - 0.5;
+      c.Hy() = rng.uniform() - 0.5;
+      c.Hz() = rng.uniform() - 0.5;
+    }
+    // initial fields, component-major, x fastest
+    for (int f = 0; f < 6; f++)
+      for (auto& c : cells) init.push_back(f == 0 ? c.Ex() : f == 1 ? c.Ey() : f == 2 ? c.Ez() : f == 3 ? c.Hx() : f == 4 ? c.Hy() : c.Hz());
+  };
+  Fixture fx;
+  fx.name = "boundaries";
+  std::vector<double> cases, ghosts;
+  size_t per_case = 0;
+  for (int type = 1; type <= 5; type++)
+    for (int a = 0; a < 3; a++)
+      for (int side = 0; side < 2; side++)
+        for (int ft = 0; ft < 2; ft++) {
+          seed();
+          const int L = n[a];
+          const int src = side ? L - 2 : 1, dst = side ? L - 1 : 0, img = side ? 1 : L - 2;
+          const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+          std::vector<Cell3*> bnd, per;
+          for (int v = 1; v < n[t2] - 1; v++)
+            for (int u = 1; u < n[t1] - 1; u++) {
+              int c1[3], c2[3];
+              c1[a] = src;
+              c1[t1] = u;
+              c1[t2] = v;
+              c2[a] = img;
+              c2[t1] = u;
+              c2[t2] = v;
+              bnd.push_back(&at(c1[0], c1[1], c1[2]));
+              per.push_back(&at(c2[0], c2[1], c2[2]));
+            }
+          CellRefIt<Cell3> b{bnd.data()}, e{bnd.data() + bnd.size()};
+          PeriodicImage pf{bnd.data(), per.data()};
+          BoundaryData bd = a == 0 ? (side ? make_bc<Dir::X, Orientation::MAX>(type, b, e, pf)
+                                           : make_bc<Dir::X, Orientation::MIN>(type, b, e, pf))
+                            : a == 1 ? (side ? make_bc<Dir::Y, Orientation::MAX>(type, b, e, pf)
+                                             : make_bc<Dir::Y, Orientation::MIN>(type, b, e, pf))
+                                     : (side ? make_bc<Dir::Z, Orientation::MAX>(type, b, e, pf)
+                                             : make_bc<Dir::Z, Orientation::MIN>(type, b, e, pf));
+          if (ft == 0)
+            bd.updateE();
+          else
+            bd.updateH();
+          cases.insert(cases.end(), {double(type), double(a), double(side), double(ft)});
+          size_t cnt = 0;
+          for (int v = 1; v < n[t2] - 1; v++)
+            for (int u = 1; u < n[t1] - 1; u++) {
+              int c1[3];
+              c1[a] = dst;
+              c1[t1] = u;
+              c1[t2] = v;
+              const Cell3& g = at(c1[0], c1[1], c1[2]);
+              if (ft == 0)
+                ghosts.insert(ghosts.end(), {g.Ex(), g.Ey(), g.Ez()});
+              else
+                ghosts.insert(ghosts.end(), {g.Hx(), g.Hy(), g.Hz()});
+              cnt += 3;
+            }
+          per_case = std::max(per_case, cnt);
+          // pad to a common length per case (planes differ in size per axis)
+          while (cnt < size_t(3 * (std::max({nx, ny, nz}) - 2) * (std::max({nx, ny, nz}) - 2))) {
+            ghosts.push_back(0.0);
+            cnt++;
+          }
+        }
+  const size_t stride = size_t(3 * (std::max({nx, ny, nz}) - 2) * (std::max({nx, ny, nz}) - 2));
+  fx.pi("nx", nx);
+  fx.pi("ny", ny);
+  fx.pi("nz", nz);
+  fx.ps("ghost_layout", "per case: the ghost plane's tangential interior cells, lower tangential axis fastest, 3 comps each, zero padded");
+  fx.arr("init_E_H", init, {6, size_t(nz), size_t(ny), size_t(nx)});
+  fx.arr("cases_type_axis_side_ft", cases, {cases.size() / 4, 4});
+  fx.arr("ghosts", ghosts, {cases.size() / 4, stride});
+  fx.write(dir);
+}
+
 int main(int argc, char** argv) {
   if (argc >= 2 && !strcmp(argv[1], "bench3d")) {
     // timing of the reference's own functors on a bounded sample of the 3-D
@@ -722,6 +872,7 @@ int main(int argc, char** argv) {
   }
   std::string dir = argv[1];
   scen_pml_coeffs(dir);
+  scen_boundaries(dir);
   scen_aux1d(dir);
 
   // 2D TM, small, non-square, CPML x+y, two dielectric blocks, one Drude and

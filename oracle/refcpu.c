@@ -275,6 +275,8 @@ struct rc_sim {
   double *Db[3], *Bb[3], *Eb[3], *Hb[3];
   double* Pb[2][3][3];
   int obj_dft, obj_mask;
+  /* boundary condition per [axis][side] (RC_BC_*; 0: ghost layer left alone) */
+  int bc[3][2];
 };
 
 static size_t cidx(const rc_sim* s, int i, int j, int k) {
@@ -552,6 +554,70 @@ static void interior(const rc_sim* s, int lo[3], int hi[3]) {
 
 static size_t stride(const rc_sim* s, int a) { return a == 0 ? 1 : a == 1 ? (size_t)s->nx : (size_t)s->nx * s->ny; }
 
+/* ---------------------------------------------------------------- boundaries
+ * BoundaryUpdates.hpp: SingleUpdate::atomic_update (258-300) writes, for every
+ * component of one field type, dest = decorate() * Mask * source, where dest
+ * is the boundary cell's outward neighbour and source the boundary cell itself
+ * (PEC 527-565, PMC 578-616, Symmetric 629-668, Antisymmetric 681-722) or its
+ * periodic image (make_periodic_boundary, 800-832; mask and decorator 1).
+ * The masks (component along the boundary axis / across it):
+ *   PEC: E +1/-1, H -1/+1;  PMC: E -1/+1, H +1/-1;
+ *   Symmetric: -1/+1 for E and H;  Antisymmetric: +1/-1 for E and H.
+ * Placement, which the reference leaves to the caller, is the build's: the
+ * ghost layer is index 0 (min) / L-1 (max) of the axis, the boundary cell the
+ * adjacent interior layer 1 / L-2, the periodic image of ghost 0 is layer
+ * L-2 and of ghost L-1 layer 1; the tangential range is the interior cells
+ * (the ghost edges are never read by the stencils).  E ghosts are written
+ * after the E update, H ghosts after the H update. */
+double rc_bc_mask(int type, int ft, int comp, int axis) {
+  const int along = comp == axis;
+  switch (type) {
+    case RC_BC_PEC: return ft == 0 ? (along ? 1.0 : -1.0) : (along ? -1.0 : 1.0);
+    case RC_BC_PMC: return ft == 0 ? (along ? -1.0 : 1.0) : (along ? 1.0 : -1.0);
+    case RC_BC_SYMMETRIC: return along ? -1.0 : 1.0;
+    case RC_BC_ANTISYMMETRIC: return along ? 1.0 : -1.0;
+    default: return 1.0;
+  }
+}
+
+int rc_set_boundary(rc_sim* s, int axis, int side, int type) {
+  if (axis < 0 || axis > 2 || side < 0 || side > 1 || type < RC_BC_NONE || type > RC_BC_PERIODIC) return -1;
+  if (!s->axes[axis]) return -1;
+  s->bc[axis][side] = type;
+  return 0;
+}
+
+static void apply_boundaries(rc_sim* s, int ft) {
+  double** F = ft == 0 ? s->E : s->H;
+  const int* act = ft == 0 ? s->actE : s->actH;
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const int len[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; a++)
+    for (int side = 0; side < 2; side++) {
+      const int type = s->bc[a][side];
+      if (!type) continue;
+      const int L = len[a];
+      const int dst = side ? L - 1 : 0;
+      const int src = type == RC_BC_PERIODIC ? (side ? 1 : L - 2) : (side ? L - 2 : 1);
+      int rlo[3] = {lo[0], lo[1], lo[2]}, rhi[3] = {hi[0], hi[1], hi[2]};
+      rlo[a] = 0;
+      rhi[a] = 1;
+      for (int k = rlo[2]; k < rhi[2]; k++)
+        for (int j = rlo[1]; j < rhi[1]; j++)
+          for (int i = rlo[0]; i < rhi[0]; i++) {
+            int cd[3] = {i, j, k}, cs[3] = {i, j, k};
+            cd[a] = dst;
+            cs[a] = src;
+            const size_t d = cidx(s, cd[0], cd[1], cd[2]), q = cidx(s, cs[0], cs[1], cs[2]);
+            for (int c = 0; c < 3; c++)
+              if (act[c]) F[c][d] = 1.0 * rc_bc_mask(type, ft, c, a) * F[c][q];
+          }
+    }
+}
+
+void rc_apply_boundaries(rc_sim* s, int ft) { apply_boundaries(s, ft); }
+
 /* Tangential TFSF range of a component (offset 0 or -1/2 along axis T):
  * the total-field region along T is the open interval (lo-1/2, hi). */
 static void tf_range(int lo, int hi, int half, int* a, int* b) {
@@ -730,6 +796,7 @@ int rc_step(rc_sim* s, int nsteps) {
             }
           }
     }
+    apply_boundaries(s, 0);
     /* ---- B curl */
     for (int I = 0; I < 3; I++) {
       if (!s->actH[I]) continue;
@@ -787,6 +854,7 @@ int rc_step(rc_sim* s, int nsteps) {
           for (int I = 2; I >= 0; I--)
             if (s->actH[I]) s->H[I][c] = s->B[I][c] / (m->mu * mu0);
         }
+    apply_boundaries(s, 1);
     /* ---- running DFT (build-defined): F += f(t) e^{-i w t} dt, E at (n+1)dt, H at (n+1.5)dt */
     for (int q = 0; q < s->ndft; q++) {
       rc_dft* d = &s->dft[q];

@@ -88,6 +88,13 @@ int rc_step(rc_sim* s, int nsteps);
 /* ---- build-defined adjoint (the reference has none) */
 /* per-cell eps_r over box [i0,i1)x[j0,j1)x[k0,k1) (x fastest); design cells are non-dispersive */
 int rc_set_design(rc_sim* s, const int box[6], const double* eps_r);
+
+/* boundary conditions (BoundaryUpdates.hpp:17-26 names; numbering the build's) */
+enum { RC_BC_NONE = 0, RC_BC_PEC = 1, RC_BC_PMC = 2, RC_BC_SYMMETRIC = 3, RC_BC_ANTISYMMETRIC = 4, RC_BC_PERIODIC = 5 };
+int rc_set_boundary(rc_sim* s, int axis, int side, int type);
+double rc_bc_mask(int type, int ft, int comp, int axis);
+/* the ghost-layer writes of one field type (0: E, 1: H), as done inside rc_step */
+void rc_apply_boundaries(rc_sim* s, int ft);
 /* E = D/(eps_r eps0), H = B/(mu_r mu0) on interior cells (non-dispersive) */
 void rc_sync_derived(rc_sim* s);
 /* Gradient from step 0 over nsteps of J = sum over the cells of DFT monitor `dft`, its E components in

@@ -13,6 +13,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 MODES = {"TEM": 1, "TM": 2, "ThreeD": 3, "TE": 4}
+# boundary kinds (names of BoundaryUpdates.hpp:17-26; numbering the build's, same as include/fdtd_hip.h)
+BC = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "Periodic": 5}
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -51,6 +53,9 @@ def lib():
             ("rc_set_field", _i, [_p, _i, _i, _i, _dp]),
             ("rc_step", _i, [_p, _i]), ("rc_nthreads", _i, [_i]),
             ("rc_set_design", _i, [_p, _ip, _dp]), ("rc_sync_derived", None, [_p]),
+            ("rc_set_boundary", _i, [_p, _i, _i, _i]),
+            ("rc_bc_mask", ctypes.c_double, [_i, _i, _i, _i]),
+            ("rc_apply_boundaries", None, [_p, _i]),
             ("rc_adjoint_eps_gradient", _i, [_p, _i, _i, _i, _dp, _dp]),
             ("rc_adjoint_reset", None, [_p]), ("rc_adjoint_step", None, [_p, ctypes.c_long]),
             ("rc_adjoint_finish", None, [_p]),
@@ -159,8 +164,10 @@ class Sim:
         assert lib().rc_set_matid(self.h, ids.tobytes()) == 0
 
     def add_source(self, field, comp, i, j, k, table):
+        """field: 0 (D) or 2 (B), the field codes of this class; rc_add_source numbers them 0 / 1."""
+        assert field in (0, 2), field
         t = _f64(table)
-        assert lib().rc_add_source(self.h, field, comp, i, j, k, _dptr(t), t.size) == 0
+        assert lib().rc_add_source(self.h, 0 if field == 0 else 1, comp, i, j, k, _dptr(t), t.size) == 0
 
     def set_tfsf(self, box, kvec, pol, polH, origin, aux_ncells, signal):
         b = (ctypes.c_int * 6)(*box)
@@ -213,6 +220,15 @@ class Sim:
     def step(self, n=1):
         assert lib().rc_step(self.h, n) == 0
         self._nstep = self._steps_done() + n
+
+    # ---- boundary conditions (BoundaryUpdates.hpp; placement build-defined)
+    def set_boundary(self, axis, side, kind):
+        """kind: one of BC (name or number); axis 0..2, side 0 (min) / 1 (max)."""
+        t = BC[kind] if isinstance(kind, str) else int(kind)
+        assert lib().rc_set_boundary(self.h, axis, side, t) == 0, (axis, side, kind)
+
+    def apply_boundaries(self, ft):
+        lib().rc_apply_boundaries(self.h, ft)
 
     # ---- adjoint (build-defined)
     def set_design(self, box, eps_r):

@@ -117,6 +117,8 @@ def apply(sim, sc):
         sim.add_dft(*d)
     if sc.get("design"):
         sim.set_design(*sc["design"])
+    for axis, side, kind in sc.get("boundaries", ()):
+        sim.set_boundary(axis, side, kind)
     return sim
 
 

@@ -13,7 +13,7 @@ import math
 
 import numpy as np
 
-from scenarios import C0, D, E, EPS0, IMP0, H, SplitMix64, cw_signal, dft_phases, gauss_table, gmsin_signal
+from scenarios import B, C0, D, E, EPS0, IMP0, H, SplitMix64, cw_signal, dft_phases, gauss_table, gmsin_signal
 
 DRUDE, LORENTZ = 1, 2
 
@@ -200,4 +200,62 @@ def grid3d_adjoint(n=128, nsteps=3000, design=(64, 64, 16), npml=10, monitor=64,
                      aux_ncells=ext, signal=cw_signal(nsteps, dt, f0)),
         "dfts": [([mx, mx + 1, my0, my0 + m, mz0, mz0 + m], 4, pe, ph)],
         "design": (dbox, eps), "f0": f0,
+    }
+
+
+def grid3d_boundaries(n=(30, 26, 22), nsteps=120, variant="closed", nfreq=4, seed=6):
+    """3-D grids with boundary conditions (fdtd_set_boundary).
+
+    variant "closed": x PEC (min) / PMC (max), y Symmetric / Antisymmetric, z
+    periodic; no CPML.  variant "unit_cell": x and y periodic (a metasurface
+    unit cell), 5-cell CPML along z only, a DFT plane above the blocks.  Both:
+    seeded dielectric blocks, a Dz and a Bx point source off the symmetry
+    planes, probes next to every face."""
+    nx, ny, nz = _dims(n)
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(3.0))
+    rng = SplitMix64(seed)
+    matid = np.zeros((nz, ny, nx), np.uint8)
+    blocks = seed_blocks(matid, 4, 3, rng, 2, max(3, min(nx, ny, nz) // 4))
+    if variant == "closed":
+        bcs = [(0, 0, "PEC"), (0, 1, "PMC"), (1, 0, "Symmetric"), (1, 1, "Antisymmetric"), (2, 0, "Periodic"),
+               (2, 1, "Periodic")]
+        pml_axes = [False, False, False]
+        npml = 0
+    else:
+        bcs = [(0, 0, "Periodic"), (0, 1, "Periodic"), (1, 0, "Periodic"), (1, 1, "Periodic")]
+        pml_axes = [False, False, True]
+        npml = 5
+    sc = {
+        "name": f"grid3d_boundaries_{variant}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
+        "nsteps": nsteps, "pml": _pml(max(npml, 1), dx), "pml_axes": pml_axes,
+        "materials": [(1.0, 1.0, [])] + blocks, "matid": matid,
+        "sources": [(D, 2, nx // 3, ny // 2 + 1, nz // 2, gauss_table(nsteps, 20, 6)),
+                    (B, 0, 2 * nx // 3, ny // 3, nz // 3, gauss_table(nsteps, 30, 8, scale=4e-7 * math.pi))],
+        "probes": [(E, 2, 2, ny // 2, nz // 2), (E, 1, nx - 2, ny // 2, nz // 2), (H, 0, nx // 2, 1, nz // 2),
+                   (H, 2, nx // 2, ny - 2, nz // 2), (E, 0, nx // 2, ny // 2, 1), (H, 1, nx // 2, ny // 2, nz - 2)],
+        "dfts": [], "tfsf": None, "boundaries": bcs,
+    }
+    if variant != "closed":
+        freqs = np.linspace(0.7, 1.3, nfreq) * C0 / (20 * dx)
+        pe, ph = dft_phases(nsteps, dt, freqs)
+        sc["dfts"].append(([1, nx - 1, 1, ny - 1, nz - npml - 3, nz - npml - 2], 63, pe, ph))
+    return sc
+
+
+def tm2d_boundaries(n=(48, 40), nsteps=200):
+    """2-D TM with x periodic and y PEC (min) / Symmetric (max), 8-cell CPML
+    nowhere (closed in y by the conditions), a dielectric block and a Dz source."""
+    nx, ny = n
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(2.0))
+    matid = np.zeros((1, ny, nx), np.uint8)
+    matid[0, 12:20, 10:18] = 1
+    return {
+        "name": "tm2d_boundaries", "mode": "TM", "nx": nx, "ny": ny, "nz": 1, "dx": dx, "dt": dt, "nsteps": nsteps,
+        "pml": _pml(1, dx), "pml_axes": [False, False, False], "materials": [(1.0, 1.0, []), (4.0, 1.0, [])],
+        "matid": matid, "sources": [(D, 2, nx // 3, ny // 2, 0, gauss_table(nsteps, 25, 8))],
+        "probes": [(E, 2, 1, ny // 2, 0), (E, 2, nx - 2, ny // 3, 0), (H, 0, nx // 2, 1, 0), (H, 1, nx // 2, ny - 2, 0)],
+        "dfts": [], "tfsf": None,
+        "boundaries": [(0, 0, "Periodic"), (0, 1, "Periodic"), (1, 0, "PEC"), (1, 1, "Symmetric")],
     }
