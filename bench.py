@@ -167,6 +167,40 @@ def run_c3(args):
                                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}})
 
 
+def run_unit_cell(args):
+    """3-D 256^3 FP32 periodic unit cell: x and y periodic (boundary ghost writes after each pass), 12-cell CPML
+    along z, 4 dielectric blocks, two point sources, DFT of 6 comps on one z-plane x 4 freqs."""
+    from fdtd_amd import Grid
+    import synth
+    from scenarios import apply
+    n = args.size
+    sc = synth.grid3d_boundaries(n=n, nsteps=args.warmup + 2 * args.steps, variant="unit_cell", npml=12)
+    esz = 4 if args.precision == "f32" else 8
+    g = apply(Grid("ThreeD", n, n, n, sc["dx"], sc["dt"], precision=args.precision), sc)
+    if args.tile:
+        tile = tuple(int(v) for v in args.tile.split(","))
+        g.set_tile(*tile)
+    else:
+        tile = g.autotune(trial_steps=30)
+    g.step(args.warmup)
+    ms, avg, per_step = timed_steps(g, args.steps)
+    abytes, interior = algorithmic_bytes(dict(sc, pml=dict(sc["pml"], n=0)), esz)
+    # CPML along z only, and the ghost writes: 2 periodic axes x 2 faces x (E and H) x 3 comps, read + written
+    abytes += 2 * 12 * (n - 2) * (n - 2) * 4 * 2 * esz
+    abytes += 2 * 2 * 2 * 3 * (n - 2) * (n - 2) * 2 * esz
+    kern_ms = sum(v for k, v in per_step.items() if k != "aux_line")
+    achieved = abytes / (kern_ms / 1e3) / 1e9
+    return base_line(args, round(interior * args.steps / (ms / 1e3) / 1e9, 4), "Gcell-updates/s",
+                     round(ms / args.steps, 5), args.precision,
+                     {"workload": f"3-D {n}^3 {args.precision.upper()} periodic unit cell: x and y periodic, 12-cell "
+                                  f"CPML along z, 4 eps blocks, Dz + Bx point sources, DFT of 6 comps on one z-plane "
+                                  f"x 4 freqs", "tile": list(tile), "kernel_avg_ms": avg,
+                      "kernel_ms_per_step": per_step,
+                      "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                   "algorithmic_bytes_per_step": abytes}})
+
+
 def run_c4(args):
     """64 uneven 3-D FP32 grids: one batch (tile table, hipGraph chunks) vs the same grids stepped one by one."""
     import torch
@@ -443,9 +477,10 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "copy"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "unit_cell", "copy"],
                     help="c2 (default): the headline 3-D 256^3 forward; c1: 2-D TM 256^2; c3: dispersive sphere; "
-                         "c4: 64 batched grids vs sequential; c5: adjoint eps gradient 128^3; copy: 1 GiB device copy")
+                         "c4: 64 batched grids vs sequential; c5: adjoint eps gradient 128^3; unit_cell: 256^3 with periodic "
+                         "x/y boundaries; copy: 1 GiB device copy")
     ap.add_argument("--tile", default=None,
                     help="bx,by,kc: fixed tile instead of the autotuner (c2; bx = 0 selects the fused pass)")
     ap.add_argument("--graph", action="store_true", help="capture chunks of steps in hipGraphs (c2)")
@@ -474,7 +509,8 @@ def main():
 
     torch.cuda.set_device(0)
     if args.workload != "c2":
-        fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "copy": run_copy}[args.workload]
+        fn = {"c1": run_c1, "c3": run_c3, "c4": run_c4, "c5": run_c5, "unit_cell": run_unit_cell,
+              "copy": run_copy}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     total = args.warmup + 2 * args.steps  # timed run + profiled rerun

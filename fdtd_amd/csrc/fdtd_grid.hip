@@ -109,7 +109,7 @@ void pml_idiom_rows(int L, int n, double m, double ma, double smax, double kmax,
 constexpr int kChunk = 256;  // steps per incident-table chunk
 // timed kernels: 0 E pass, 1 H pass, 2 aux line, 3 fused pass, 4 DFT,
 // 5 adjoint E pass, 6 adjoint H pass
-constexpr int kNumKernels = 8;
+constexpr int kNumKernels = 7;
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
 // bx == 0 selects the LDS fused single pass with `by` rows per workgroup,
@@ -552,6 +552,40 @@ int ensure_pingpong(fdtd_grid* g) {
   return 0;
 }
 
+// Boundary faces for the passes (BoundaryUpdates.hpp:258-300 and the masks of
+// 527-722): source layer 1 / L-2 (periodic: L-2 for the min ghost, 1 for the
+// max ghost), ghost layer 0 / L-1, and the reference's per-component signs
+// (+-1 for the component along the face normal / across it): PEC E +-, H -+;
+// PMC E -+, H +-; Symmetric -+ for both; Antisymmetric +- for both; Periodic ++.
+template <typename T>
+void bc_args(const fdtd_grid* g, fdtdk::StepArgs<T>& a) {
+  const long stride[3] = {1, long(g->d.nx), long(g->d.nx) * g->d.ny};
+  a.bc_any = g->nbc > 0;
+  for (int ax = 0; ax < 3; ax++)
+    for (int sd = 0; sd < 2; sd++) {
+      const int kind = ax < g->ndim ? g->bc[ax][sd] : 0;
+      a.bc_src[ax][sd] = -1;
+      a.bc_off[ax][sd] = 0;
+      a.bc_neg[0][ax][sd] = a.bc_neg[1][ax][sd] = 0;
+      if (!kind) continue;
+      const int L = g->len[ax];
+      const int src = kind == FDTD_BC_PERIODIC ? (sd ? 1 : L - 2) : (sd ? L - 2 : 1);
+      const int dst = sd ? L - 1 : 0;
+      a.bc_src[ax][sd] = src;
+      a.bc_off[ax][sd] = long(dst - src) * stride[ax];
+      for (int ft = 0; ft < 2; ft++)
+        for (int q = 0; q < 3; q++) {
+          const bool along = q == ax;
+          bool neg = false;
+          if (kind == FDTD_BC_PEC) neg = ft == 0 ? !along : along;
+          if (kind == FDTD_BC_PMC) neg = ft == 0 ? along : !along;
+          if (kind == FDTD_BC_SYMMETRIC) neg = along;
+          if (kind == FDTD_BC_ANTISYMMETRIC) neg = !along;
+          if (neg) a.bc_neg[ft][ax][sd] |= 1 << q;
+        }
+    }
+}
+
 template <typename T>
 void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   memset(&a, 0, sizeof a);
@@ -583,8 +617,7 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   for (int s = 0; s < g->nsrc; s++) a.src[s] = g->src[s];
   a.tfsf = g->tfsf;
   for (int x = 0; x < 6; x++) a.box[x] = g->box[x];
-  for (int x = 0; x < 3; x++)
-    for (int sd = 0; sd < 2; sd++) a.bc[x][sd] = g->bc[x][sd];
+  bc_args(g, a);
   for (int x = 0; x < 3; x++) {
     a.kvec[x] = g->kvec[x];
     a.pol[x] = g->pol[x];
@@ -738,19 +771,9 @@ int step_impl(fdtd_grid* g, int nsteps) {
   fill_args<T>(g, a);
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (g->ndim == 2) t.kc = 1;
-  // design recording and ghost-layer boundary writes run between the passes
+  // design increments and boundary ghost writes are made by the two-pass kernels
   if ((g->force_two_pass || g->nbc) && t.bx <= 1) t = kDefaultTile3D;
-  unsigned bc_blocks = 0;
-  if (g->nbc) {
-    long cells = 0;
-    for (int ax = 0; ax < g->ndim; ax++) {
-      long f = 1;
-      for (int t2 = 0; t2 < g->ndim; t2++)
-        if (t2 != ax) f *= g->len[t2] - 2;
-      cells += 2 * f;
-    }
-    bc_blocks = unsigned((cells + 255) / 256);
-  }
+
   const bool wavef = g->ndim == 3 && t.bx == 1;
   const bool fused = g->ndim == 3 && t.bx <= 1;
   if (fused) {
@@ -843,15 +866,19 @@ int step_impl(fdtd_grid* g, int nsteps) {
           else
             rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         } else if (g->ndim == 3) {
-          rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 0, 3);
-          if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 1, 3);
+          if (g->nbc) {  // the instances that also write the boundary ghost layers
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+            if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          } else {
+            rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+            if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          }
+        } else if (g->nbc) {
+          rc = launch(g, 0, fdtdk::e_pass_tm<T, true>, grid, block, dev0, 1, s);
+          if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T, true>, grid, block, dev0, 1, s);
         } else {
           rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
-          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 0, 2);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
-          if (!rc && bc_blocks) rc = launch(g, 7, fdtdk::bc_fill<T>, dim3(bc_blocks), dim3(256), dev0, 1, 2);
         }
         if (!rc && dft_work > 0) {
           rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), ap, s);
@@ -1633,7 +1660,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
     if ((rc = fdtd_set_profiling(g, 1))) break;
     if ((rc = fdtd_step(g, trial_steps))) break;
     double t = 0;  // device time of the step's kernels per step
-    for (int kid : {0, 1, 3, 4, 7}) {
+    for (int kid : {0, 1, 3, 4}) {
       double ms = 0;
       long nl = 0;
       if ((rc = fdtd_kernel_time(g, kid, &ms, &nl))) break;

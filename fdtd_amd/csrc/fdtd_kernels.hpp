@@ -117,8 +117,14 @@ struct StepArgs {
   // TFSF: incident rows of the current chunk, row s = step - step0
   int tfsf;
   int box[6];
-  // boundary condition per [axis][side] (FDTD_BC_*, 0: ghost layer untouched)
-  int bc[3][2];
+  // boundary conditions, per [axis][side]: the layer whose new values are
+  // copied into the ghost layer (-1: no condition), the address offset from
+  // a source cell to its ghost, and per field type (E, H) the components
+  // that are negated (bit c); bc_any: some face has a condition
+  int bc_src[3][2];
+  long bc_off[3][2];
+  int bc_neg[2][3][2];
+  int bc_any;
   double kvec[3], pol[3], polH[3], origin[3], dx;
   int auxL;
   const FDTD_G double* incE;  // aux Ez after each step's aux advance
@@ -658,10 +664,42 @@ __device__ __forceinline__ void stage_materials(const StepArgs<T>& a, MatLds<T>&
   __syncthreads();
 }
 
+// ---------------------------------------------------------------- boundary conditions
+// BoundaryUpdates.hpp: SingleUpdate::atomic_update (258-300) sets, for every
+// component of one field type, ghost = Mask * source, the ghost being the
+// boundary cell's outward neighbour; the source is the boundary cell itself
+// for PEC / PMC / Symmetric / Antisymmetric (masks 527-722) or its periodic
+// image (800-832, mask 1).  Placement (which the reference leaves to the
+// caller): ghost layer 0 / L-1, boundary layer 1 / L-2, periodic image of
+// ghost 0 layer L-2 and of ghost L-1 layer 1; tangential range the interior
+// cells.  The pass that computes a source cell's new value also writes the
+// ghost copies (masks are +-1, so they are exact): nothing in the same pass
+// reads a ghost of the field it writes, and the next pass sees them.
+// ghost copies of interior cell (i,j,k) (address c) whose new values are v
+// (ft 0: E, 1: H); ndim 2: TM (Ez, or Hx and Hy)
+template <typename T>
+__device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a, int ft, int ndim, int i, int j, int k, long c,
+                                         const T v[3]) {
+  const int ci[3] = {i, j, k};
+  FDTD_G T* const* F = ft == 0 ? a.Eo : a.Ho;
+  for (int ax = 0; ax < ndim; ax++)
+    for (int side = 0; side < 2; side++) {
+      if (ci[ax] != a.bc_src[ax][side]) continue;
+      const long d = c + a.bc_off[ax][side];
+      const int neg = a.bc_neg[ft][ax][side];
+      for (int q = 0; q < 3; q++) {
+        if (ndim == 2 && (ft == 0) != (q == 2)) continue;  // TM: Ez, Hx, Hy
+        F[q][d] = ((neg >> q) & 1) ? -v[q] : v[q];
+      }
+    }
+}
+
 // ---------------------------------------------------------------- two passes
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
-template <typename T>
+// BC: the instance for grids with boundary conditions (ghost writes); the
+// plain instance carries none of that code, so its registers stay as they were.
+template <typename T, bool BC = false>
 __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie, int i, int j, int k0, int k1, int s) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   const FDTD_G T* __restrict__ Hx = a.H[0];
@@ -695,10 +733,11 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
     if (a.hist) record_increment(a, step, i, j, k, o[0] - ex, o[1] - ey, o[2] - ez);
+    if (BC && a.bc_any) bc_ghost_write(a, 0, 3, i, j, k, c, o);
   }
 }
 
-template <typename T, typename Tab>
+template <typename T, typename Tab, bool BC = false>
 __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int i, int j, int k0, int k1, int s) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   // the E pass wrote Eo; the H pass reads it
@@ -733,6 +772,7 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int 
     a.Ho[0][c] = o[0];
     a.Ho[1][c] = o[1];
     a.Ho[2][c] = o[2];
+    if (BC && a.bc_any) bc_ghost_write(a, 1, 3, i, j, k, c, o);
     if (extra) {
       const T ev[3] = {e.ex, e.ey, e.ez};
       dft_probe(a, step, c, i, j, k, ev, o);
@@ -742,7 +782,7 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int 
 
 // One-dimensional launch of ntx x nty x nkc tiles of blockDim.x x blockDim.y
 // columns and kc planes, XCD-aware (tiles x fastest, then y, then z chunks).
-template <typename T>
+template <typename T, bool BC = false>
 __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                  int nkc) {
   const StepArgs<T>& a = *ap;
@@ -751,8 +791,8 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
   const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
   const int k0 = 1 + bz * kc;
-  e_pass_cells(a, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
-               s);
+  e_pass_cells<T, BC>(a, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
+                      min(k0 + kc, a.nz - 1), s);
 }
 
 // Each XCD walks its tile range in reverse: its first tiles read the E and H
@@ -762,7 +802,7 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
 // FP32 instances are compiled for 7 waves per SIMD (the register count they
 // need anyway): at 256^3 on MI355X that schedule runs the H pass ~10 % faster
 // (0.151 vs 0.168 ms per step) than the compiler's default.
-template <typename T>
+template <typename T, bool BC>
 __device__ __forceinline__ void h_pass_3d_body(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                int nkc) {
   const StepArgs<T>& a = *ap;
@@ -771,75 +811,24 @@ __device__ __forceinline__ void h_pass_3d_body(const StepArgs<T>* __restrict__ a
   const int k0 = 1 + bz * kc;
   // the H pass reads 1/(mu_r mu0) from global memory: staging it in LDS
   // measured slower here (the E pass, with its larger stencil, gains)
-  h_pass_cells(a, a.mim, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
-               s);
+  h_pass_cells<T, decltype(a.mim), BC>(a, a.mim, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y,
+                                       k0, min(k0 + kc, a.nz - 1), s);
 }
 
-template <typename T>
+template <typename T, bool BC = false>
 __global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                  int nkc) {
-  h_pass_3d_body(ap, kc, s, ntx, nty, nkc);
+  h_pass_3d_body<T, BC>(ap, kc, s, ntx, nty, nkc);
 }
 template <>
-__global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_3d<float>(
+__global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_3d<float, false>(
     const StepArgs<float>* __restrict__ ap, int kc, int s, int ntx, int nty, int nkc) {
-  h_pass_3d_body(ap, kc, s, ntx, nty, nkc);
+  h_pass_3d_body<float, false>(ap, kc, s, ntx, nty, nkc);
 }
-
-// ---------------------------------------------------------------- boundary conditions
-// BoundaryUpdates.hpp: SingleUpdate::atomic_update (258-300) sets, for every
-// component of one field type, ghost = Mask * source, the ghost being the
-// boundary cell's outward neighbour; the source is the boundary cell itself
-// for PEC / PMC / Symmetric / Antisymmetric (masks 527-722) or its periodic
-// image (800-832, mask 1).  Placement (which the reference leaves to the
-// caller): ghost layer 0 / L-1, boundary layer 1 / L-2, periodic image of
-// ghost 0 layer L-2 and of ghost L-1 layer 1; tangential range the interior
-// cells.  One thread per ghost cell of the six faces (faces without a
-// condition return at once); the masks are +-1, so the copies are exact.
-// Launched after the E pass (ft 0) and after the H pass (ft 1).
-__device__ __forceinline__ int bc_sign(int kind, int ft, int comp, int axis) {
-  const bool along = comp == axis;
-  switch (kind) {
-    case 1: return ft == 0 ? (along ? 1 : -1) : (along ? -1 : 1);  // PEC
-    case 2: return ft == 0 ? (along ? -1 : 1) : (along ? 1 : -1);  // PMC
-    case 3: return along ? -1 : 1;                                 // Symmetric
-    case 4: return along ? 1 : -1;                                 // Antisymmetric
-    default: return 1;                                             // Periodic
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void bc_fill(const StepArgs<T>* __restrict__ ap, int ft, int ndim) {
-  const StepArgs<T>& a = *ap;
-  const int n[3] = {a.nx, a.ny, a.nz};
-  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  for (int ax = 0; ax < ndim; ax++) {
-    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
-    const long e1 = n[t1] - 2, e2 = t2 < ndim ? n[t2] - 2 : 1;
-    const long face = e1 * e2;
-    if (q >= 2 * face) {
-      q -= 2 * face;
-      continue;
-    }
-    const int side = int(q / face);
-    const long r = q % face;
-    const int kind = a.bc[ax][side];
-    if (!kind) return;
-    const int L = n[ax];
-    int cd[3], cs[3];
-    cd[ax] = side ? L - 1 : 0;
-    cs[ax] = kind == 5 ? (side ? 1 : L - 2) : (side ? L - 2 : 1);
-    cd[t1] = cs[t1] = 1 + int(r % e1);
-    cd[t2] = cs[t2] = t2 < ndim ? 1 + int(r / e1) : 0;
-    const long d = (long)cd[0] + a.sy * (long)cd[1] + a.sz * (long)cd[2];
-    const long s = (long)cs[0] + a.sy * (long)cs[1] + a.sz * (long)cs[2];
-    FDTD_G T* const* F = ft == 0 ? a.Eo : a.Ho;
-    for (int c = 0; c < 3; c++) {
-      if (ndim == 2 && (ft == 0) != (c == 2)) continue;  // TM: Ez, Hx, Hy
-      F[c][d] = bc_sign(kind, ft, c, ax) < 0 ? -F[c][s] : F[c][s];
-    }
-    return;
-  }
+template <>
+__global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_3d<float, true>(
+    const StepArgs<float>* __restrict__ ap, int kc, int s, int ntx, int nty, int nkc) {
+  h_pass_3d_body<float, true>(ap, kc, s, ntx, nty, nkc);
 }
 
 // ---------------------------------------------------------------- batched passes
@@ -1281,7 +1270,7 @@ __global__ __launch_bounds__(64 * kWavesY) void fusedw_3d(const StepArgs<T>* __r
 
 // ---------------------------------------------------------------- 2-D TM
 // Ez, Hx, Hy (include/YeeFields.hpp:123-148); CPML along x and y; in place.
-template <typename T>
+template <typename T, bool BC = false>
 __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
   const StepArgs<T>& a = *ap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1318,15 +1307,22 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
     const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
     tfsf_cell<T, 2>(a, a.incH + (long)s * a.auxL, 0, i, j, 0, aF, aC, d);
   }
+  T ezn = ez;
   if (ie > T(0)) {
-    a.Eo[2][c] = ez + d[2] * ie;
+    ezn = ez + d[2] * ie;
+    a.Eo[2][c] = ezn;
   } else if (ie < T(0) && a.Dd[2]) {
     const MatDev<T>& m = a.mats[mid];
-    a.Eo[2][c] = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], true);
+    ezn = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], true);
+    a.Eo[2][c] = ezn;
+  }
+  if (BC && a.bc_any) {
+    const T v[3] = {T(0), T(0), ezn};
+    bc_ghost_write(a, 0, 2, i, j, 0, c, v);
   }
 }
 
-template <typename T>
+template <typename T, bool BC = false>
 __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
   const StepArgs<T>& a = *ap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1365,6 +1361,7 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   T h[3] = {hx + d[0] * im, hy + d[1] * im, T(0)};
   a.Ho[0][c] = h[0];
   a.Ho[1][c] = h[1];
+  if (BC && a.bc_any) bc_ghost_write(a, 1, 2, i, j, 0, c, h);
   if (a.nprobe) {
     const T e[3] = {T(0), T(0), ez};
     dft_probe(a, step, c, i, j, 0, e, h);

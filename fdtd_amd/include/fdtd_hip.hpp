@@ -15,6 +15,9 @@
 //   make_soft_source<Dz>(signal, cell)              make_soft_source<Dz>(signal, grid.cell(i,j,k))  Source.hpp:258-261
 //   Aux1D<double>(ncells,dx,dt,signal)              Aux1D<double>(ncells,dx,dt,signal)               Source.hpp:280-408
 //   make_tfsf_plane<Mode,Dir,Orient>(b,e,loc,k,aux) make_tfsf_box<Mode>(grid,box,k,aux)              Source.hpp:624-634
+//   make_pec_boundary<Mode,Dir,Orient>(b,e)         make_pec_boundary<Mode,Dir,Orient>(grid)         BoundaryUpdates.hpp:527-565
+//     (pmc, symmetric, antisymmetric, periodic)     same names; periodic image fixed (layer L-2 / 1)  BoundaryUpdates.hpp:578-832
+//   BoundaryData::updateE() / updateH()             Stepper::boundary(bd): ghost writes after E / H   BoundaryUpdates.hpp:340-362
 //
 // Instead of sweeping each functor over a user container, the functors are
 // handed once to a Stepper, which checks that they form the reference's step
@@ -341,6 +344,58 @@ TFSFBox<Mode> make_tfsf_box(YeeGrid<Mode, scalar>& g, std::array<int, 6> box, st
   return t;
 }
 
+// ---------------------------------------------------------------- boundaries
+// fdtd::Boundary (include/BoundaryUpdates.hpp:17-26).  make_*_boundary returns
+// the face description the Stepper hands to fdtd_set_boundary; the reference's
+// iterator pair over the boundary cells becomes the grid itself because the
+// placement is fixed: ghost layer 0 / L-1, boundary layer 1 / L-2, periodic
+// image of ghost 0 layer L-2 and of ghost L-1 layer 1.
+enum class Boundary : char { Periodic = 0, BlochPeriodic, PEC, PMC, Symmetric, Antisymmetric, Parallel, NONE };
+
+struct BoundaryData {
+  Boundary type;
+  Dir dir;
+  Orientation orient;
+};
+
+namespace detail {
+inline int boundary_kind(Boundary b) {
+  switch (b) {
+    case Boundary::PEC: return FDTD_BC_PEC;
+    case Boundary::PMC: return FDTD_BC_PMC;
+    case Boundary::Symmetric: return FDTD_BC_SYMMETRIC;
+    case Boundary::Antisymmetric: return FDTD_BC_ANTISYMMETRIC;
+    case Boundary::Periodic: return FDTD_BC_PERIODIC;
+    case Boundary::BlochPeriodic: return FDTD_BC_BLOCH;  // rejected by the library: needs complex fields
+    case Boundary::NONE: return FDTD_BC_NONE;
+    default: throw Error("Parallel boundaries (MPI halo exchange) have no single-GPU counterpart");
+  }
+}
+}  // namespace detail
+
+template <class Mode, Dir d, Orientation o, class scalar>
+BoundaryData make_pec_boundary(YeeGrid<Mode, scalar>&) { return {Boundary::PEC, d, o}; }
+template <class Mode, Dir d, Orientation o, class scalar>
+BoundaryData make_pmc_boundary(YeeGrid<Mode, scalar>&) { return {Boundary::PMC, d, o}; }
+template <class Mode, Dir d, Orientation o, class scalar>
+BoundaryData make_symmetric_boundary(YeeGrid<Mode, scalar>&) { return {Boundary::Symmetric, d, o}; }
+template <class Mode, Dir d, Orientation o, class scalar>
+BoundaryData make_antisymmetric_boundary(YeeGrid<Mode, scalar>&) { return {Boundary::Antisymmetric, d, o}; }
+template <class Mode, Dir d, Orientation o, class scalar>
+BoundaryData make_periodic_boundary(YeeGrid<Mode, scalar>&) { return {Boundary::Periodic, d, o}; }
+
+// Per-face kinds, PEC everywhere by default as in the reference's
+// BoundaryOptions (include/BoundaryUpdates.hpp:44-69).
+struct BoundaryOptions {
+  Boundary conds[3][2];
+  BoundaryOptions() {
+    for (auto& a : conds) a[0] = a[1] = Boundary::PEC;
+  }
+  Boundary& operator()(Dir d, Orientation o) { return conds[int(d)][int(o)]; }
+  template <Dir d, Orientation o>
+  Boundary& get() { return conds[int(d)][int(o)]; }
+};
+
 // ---------------------------------------------------------------- stepper
 // Collects the stages of the reference's step composition and runs whole
 // steps on the device.  Material functors are given per material id.
@@ -369,6 +424,16 @@ class Stepper {
   }
   Stepper& tfsf(const TFSFBox<Mode>& t) {
     tfsf_.reset(new TFSFBox<Mode>(t));
+    return *this;
+  }
+  // ghost-layer writes of one face after every E and H update (BoundaryData::updateE / updateH)
+  Stepper& boundary(const BoundaryData& b) {
+    bcs_.push_back(b);
+    return *this;
+  }
+  Stepper& boundaries(const BoundaryOptions& bo) {
+    for (int d = 0; d < Mode::dim; d++)
+      for (int o = 0; o < 2; o++) bcs_.push_back({bo.conds[d][o], Dir(d), Orientation(o)});
     return *this;
   }
   template <class M>
@@ -401,6 +466,9 @@ class Stepper {
   }
   void commit(int n, double unit) {
     g_.commit_pml();
+    for (auto& b : bcs_)
+      check(fdtd_set_boundary(g_.handle(), int(b.dir), int(b.orient), detail::boundary_kind(b.type)),
+            "fdtd_set_boundary");
     long n0 = fdtd_step_count(g_.handle());
     for (auto& s : sources_) {
       std::vector<double> tab(size_t(n0 + n));
@@ -428,6 +496,7 @@ class Stepper {
   bool committed_ = false;
   std::vector<SoftSource<Mode, scalar>> sources_;
   std::unique_ptr<TFSFBox<Mode>> tfsf_;
+  std::vector<BoundaryData> bcs_;
 };
 
 }  // namespace hip

@@ -173,7 +173,7 @@ int fdtd_set_graph(fdtd_grid* g, int on);
 /* Per-kernel timing with hipEvents on the grid's stream (off by default).
  * fdtd_kernel_time returns accumulated milliseconds and launches for
  * kernel id 0 = E pass, 1 = H pass, 2 = aux line, 3 = fused pass, 4 = DFT,
- * 5 = adjoint E pass, 6 = adjoint H pass, 7 = boundary-condition ghost writes. */
+ * 5 = adjoint E pass, 6 = adjoint H pass. */
 int fdtd_set_profiling(fdtd_grid* g, int on);
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches);
 

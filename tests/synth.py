@@ -203,12 +203,12 @@ def grid3d_adjoint(n=128, nsteps=3000, design=(64, 64, 16), npml=10, monitor=64,
     }
 
 
-def grid3d_boundaries(n=(30, 26, 22), nsteps=120, variant="closed", nfreq=4, seed=6):
+def grid3d_boundaries(n=(30, 26, 22), nsteps=120, variant="closed", nfreq=4, seed=6, npml=5):
     """3-D grids with boundary conditions (fdtd_set_boundary).
 
     variant "closed": x PEC (min) / PMC (max), y Symmetric / Antisymmetric, z
     periodic; no CPML.  variant "unit_cell": x and y periodic (a metasurface
-    unit cell), 5-cell CPML along z only, a DFT plane above the blocks.  Both:
+    unit cell), `npml`-cell CPML along z only, a DFT plane above the blocks.  Both:
     seeded dielectric blocks, a Dz and a Bx point source off the symmetry
     planes, probes next to every face."""
     nx, ny, nz = _dims(n)
@@ -225,7 +225,6 @@ def grid3d_boundaries(n=(30, 26, 22), nsteps=120, variant="closed", nfreq=4, see
     else:
         bcs = [(0, 0, "Periodic"), (0, 1, "Periodic"), (1, 0, "Periodic"), (1, 1, "Periodic")]
         pml_axes = [False, False, True]
-        npml = 5
     sc = {
         "name": f"grid3d_boundaries_{variant}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
         "nsteps": nsteps, "pml": _pml(max(npml, 1), dx), "pml_axes": pml_axes,

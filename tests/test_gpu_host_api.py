@@ -26,3 +26,25 @@ def test_tm_c1_through_cpp_host_surface():
     ref = a["probe_trace_Ez_Hx_Hy"][:, :, 0]
     assert got.shape == ref.shape
     assert rel_l2(got, ref) <= 1e-12
+
+
+def test_boundaries_through_cpp_host_surface():
+    """Boundary conditions set through BoundaryOptions / make_symmetric_boundary
+    and the Stepper: probe traces equal the oracle's run of the same scenario."""
+    import synth
+    from oracle import refcpu
+    from scenarios import apply
+    exe = os.path.join(os.path.dirname(BIN), "tm_boundaries_host_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/bin/tm_boundaries_host_api not built (run __graft_entry__.build())")
+    print("[gpu-case] tm_boundaries_host_api", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
+    sc = synth.tm2d_boundaries()
+    o = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    o.step(sc["nsteps"])
+    ref = o.probes()
+    assert got.shape == ref.shape
+    assert np.abs(ref).max() > 0
+    assert rel_l2(got, ref) <= 1e-12
