@@ -340,10 +340,16 @@ int upload_materials(fdtd_grid* g) {
     const MatHost& h = g->mats[id];
     m[id].inv_eps = T(1.0 / (h.eps * kEps0));
     m[id].inv_mu = T(1.0 / (h.mu * kMu0));
+    m[id].eps_e0 = T(h.eps * kEps0);
     m[id].np = h.np;
     for (int p = 0; p < fdtdk::kMaxPoles; p++) {
-      double cJ = 0, cE = 0, cP = 0;
-      if (p < h.np && h.kind[p] == FDTD_DRUDE) {
+      double cJ = 0, cE = 0, cP = 0, cD = 0;
+      if (p < h.np && h.kind[p] == FDTD_CONDUCTIVE) {
+        // ConductiveUpdate::implicit_update, DispersiveMaterials.hpp:549-560:
+        // factor = dt cond_freq/(eps_rel eps0); J is not touched
+        cD = dt * h.freq[p] / (h.eps * kEps0);
+        cJ = 1.0;
+      } else if (p < h.np && h.kind[p] == FDTD_DRUDE) {
         // DrudeUpdate::implicit_update, DispersiveMaterials.hpp:986-996
         double b = h.gamma[p] * dt * 0.5, w = dt * h.freq[p];
         cJ = (1.0 - b) / (1.0 + b);
@@ -358,6 +364,7 @@ int upload_materials(fdtd_grid* g) {
       m[id].cJ[p] = T(cJ);
       m[id].cE[p] = T(cE);
       m[id].cP[p] = T(cP);
+      m[id].cD[p] = T(cD);
     }
   }
   HIP_TRY(hipMemcpy(g->mats_dev, m.data(), sizeof(fdtdk::MatDev<T>) * m.size(), hipMemcpyHostToDevice));
@@ -1392,7 +1399,8 @@ int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npole
   if (g->finalized && (npoles > 0) != (m.np > 0)) return fail(FDTD_ESTATE, "dispersive layout is fixed once stepping has started");
   if (g->finalized && npoles > g->max_np) return fail(FDTD_ESTATE, "pole count exceeds the allocated dispersive state");
   for (int p = 0; p < npoles; p++)
-    if (int(poles[4 * p]) != FDTD_DRUDE && int(poles[4 * p]) != FDTD_LORENTZ) return fail(FDTD_EINVAL, "bad pole kind");
+    if (int(poles[4 * p]) != FDTD_DRUDE && int(poles[4 * p]) != FDTD_LORENTZ && int(poles[4 * p]) != FDTD_CONDUCTIVE)
+      return fail(FDTD_EINVAL, "bad pole kind");
   m.eps = eps_r;
   m.mu = mu_r;
   m.np = npoles;

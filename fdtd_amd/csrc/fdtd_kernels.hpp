@@ -50,6 +50,11 @@ struct MatDev {
   T inv_mu;   // 1/(mu_r mu0)
   int np;     // poles; 0 = constant material
   T cJ[kMaxPoles], cE[kMaxPoles], cP[kMaxPoles];  // J = cJ J + cE E + cP P
+  // conductive pole: P = (P + cD D)/(1 + cD) with cD = dt sigma/(eps_r eps0)
+  // (ConductiveUpdate::implicit_update, DispersiveMaterials.hpp:549-560); 0 for
+  // Drude / Lorentz poles, whose P advances by dt J
+  T cD[kMaxPoles];
+  T eps_e0;  // eps_r eps0
 };
 
 struct SrcDev {
@@ -277,15 +282,30 @@ __device__ FDTD_COLD void add_sources(const StepArgs<T>& a, long step, int field
 // E = (D - sum_p P_p)/(eps_r eps0); J_p = cJ J_p + cE E + cP P_p
 // (DrudeUpdate / LorentzUpdate implicit_update, DispersiveMaterials.hpp:710-735, 979-998).
 template <typename T, int I>
-__device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>& m, long bq, T d, bool own) {
+__device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>& m, long bq, T d, T Eold, bool own) {
   const T D = a.Dd[I][bq] + d;
+  if (sizeof(T) == 4 && m.np == 1 && m.cD[0] != T(0)) {
+    // FP32, a lone conductive pole: E_{n+1} = (E_n + d/(eps_r eps0))/(1 + f),
+    // the same update without the cancellation of D - P (P -> D in a good
+    // conductor, which in FP32 leaves E at a few digits); P = D - eps_r eps0 E
+    const T Ev = (Eold + d * m.inv_eps) / (T(1) + m.cD[0]);
+    if (own) {
+      a.Ddo[I][bq] = D;
+      a.Pdo[0][I][bq] = D - m.eps_e0 * Ev;
+      a.Jdo[0][I][bq] = a.Jd[0][I][bq];
+    }
+    return Ev;
+  }
   T num = D;
   T Pv[kMaxPoles], Jv[kMaxPoles];
 #pragma unroll
   for (int p = 0; p < kMaxPoles; p++) {
     if (p < m.np) {
       Jv[p] = a.Jd[p][I][bq];
-      Pv[p] = a.Pd[p][I][bq] + a.dt * Jv[p];
+      if (m.cD[p] != T(0))
+        Pv[p] = (a.Pd[p][I][bq] + m.cD[p] * D) / (T(1) + m.cD[p]);
+      else
+        Pv[p] = a.Pd[p][I][bq] + a.dt * Jv[p];
       num -= Pv[p];
     }
   }
@@ -311,12 +331,13 @@ __device__ __forceinline__ long dbox_index(const StepArgs<T>& a, int i, int j, i
 
 // dispersive cell: the three components
 template <typename T>
-__device__ FDTD_COLD void ade_cell(const StepArgs<T>& a, int mid, int i, int j, int k, const T d[3], bool own, T out[3]) {
+__device__ FDTD_COLD void ade_cell(const StepArgs<T>& a, int mid, int i, int j, int k, const T d[3], const T eold[3],
+                                   bool own, T out[3]) {
   const MatDev<T>& m = a.mats[mid];
   const long bq = dbox_index(a, i, j, k);
-  out[0] = ade_component<T, 0>(a, m, bq, d[0], own);
-  out[1] = ade_component<T, 1>(a, m, bq, d[1], own);
-  out[2] = ade_component<T, 2>(a, m, bq, d[2], own);
+  out[0] = ade_component<T, 0>(a, m, bq, d[0], eold[0], own);
+  out[1] = ade_component<T, 1>(a, m, bq, d[1], eold[1], own);
+  out[2] = ade_component<T, 2>(a, m, bq, d[2], eold[2], own);
 }
 
 // H values around a cell that the E update reads (backward neighbours)
@@ -393,7 +414,8 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
     out[1] = ey + d[1] * ie;
     out[2] = ez + d[2] * ie;
   } else if (ie < T(0) && a.Dd[0]) {
-    ade_cell(a, mid, i, j, k, d, own, out);
+    const T eold[3] = {ex, ey, ez};
+    ade_cell(a, mid, i, j, k, d, eold, own, out);
   } else {
     out[0] = ex;
     out[1] = ey;
@@ -1313,7 +1335,7 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
     a.Eo[2][c] = ezn;
   } else if (ie < T(0) && a.Dd[2]) {
     const MatDev<T>& m = a.mats[mid];
-    ezn = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], true);
+    ezn = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], ez, true);
     a.Eo[2][c] = ezn;
   }
   if (BC && a.bc_any) {

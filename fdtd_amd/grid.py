@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libfdtd_hip.so")
 TM, THREED = 2, 3
 F32, F64 = 0, 1
 D, E, B, H, P, J = 0, 1, 2, 3, 4, 5
-DRUDE, LORENTZ = 1, 2
+DRUDE, LORENTZ, CONDUCTIVE = 1, 2, 3
 # boundary kinds (fdtd_boundary in include/fdtd_hip.h; names of the reference's fdtd::Boundary)
 BOUNDARY = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "Periodic": 5, "BlochPeriodic": 6}
 

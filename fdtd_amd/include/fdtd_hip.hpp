@@ -12,6 +12,7 @@
 //   ConstantUpdate<Mode,ft>(K) / VacuumUpdate       same names                       DispersiveMaterials.hpp:489-497
 //   DrudeUpdate<Mode,ft>(K,freq,gamma,dt)           same                             DispersiveMaterials.hpp:1208-1211
 //   LorentzUpdate<Mode,ft>(K,delta,freq,gamma,dt)   same                             DispersiveMaterials.hpp:881-884
+//   ConductiveUpdate<Mode,ft>(K,freq,dt)            same                             DispersiveMaterials.hpp:674-677
 //   make_soft_source<Dz>(signal, cell)              make_soft_source<Dz>(signal, grid.cell(i,j,k))  Source.hpp:258-261
 //   Aux1D<double>(ncells,dx,dt,signal)              Aux1D<double>(ncells,dx,dt,signal)               Source.hpp:280-408
 //   make_tfsf_plane<Mode,Dir,Orient>(b,e,loc,k,aux) make_tfsf_box<Mode>(grid,box,k,aux)              Source.hpp:624-634
@@ -263,6 +264,15 @@ struct LorentzUpdate {
   double K, delta, freq, gamma, dt;
   LorentzUpdate(double k, double de, double f, double g, double deltat) : K(k), delta(de), freq(f), gamma(g), dt(deltat) {}
   MaterialSpec spec() const { return {K, 1.0, {double(FDTD_LORENTZ), freq, gamma, 1.0}}; }
+};
+// Conductive medium with ConductiveStorage(K, Freq, dt), Freq being the
+// conductivity (include/DispersiveMaterials.hpp:508-680): P relaxes
+// implicitly towards D with factor dt Freq/(K eps0)
+template <class Mode, FieldType ft, bool forward = false>
+struct ConductiveUpdate {
+  double K, freq, dt;
+  ConductiveUpdate(double k, double f, double deltat) : K(k), freq(f), dt(deltat) {}
+  MaterialSpec spec() const { return {K, 1.0, {double(FDTD_CONDUCTIVE), freq, 0.0, 1.0}}; }
 };
 
 // MaterialMap equivalent (include/MaterialMap.hpp:204-284): id -> material.

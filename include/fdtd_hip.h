@@ -42,7 +42,7 @@ enum fdtd_precision { FDTD_F32 = 0, FDTD_F64 = 1 };
 /* field codes */
 enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5 };
 /* material pole kinds */
-enum fdtd_pole { FDTD_DRUDE = 1, FDTD_LORENTZ = 2 };
+enum fdtd_pole { FDTD_DRUDE = 1, FDTD_LORENTZ = 2, FDTD_CONDUCTIVE = 3 };
 /* boundary conditions: the kinds of fdtd::Boundary (include/BoundaryUpdates.hpp:17-26) */
 enum fdtd_boundary {
   FDTD_BC_NONE = 0,          /* ghost layer left alone (stays as uploaded; zero by default) */
@@ -90,7 +90,10 @@ int fdtd_set_pml_idiom(fdtd_grid* g, int axis, int n, double m, double ma, doubl
  * update functors (include/DispersiveMaterials.hpp:342-497, 692-884,
  * 898-1217).  poles: npoles x {kind, freq [rad/s], gamma [1/s], weight}.
  * One Drude pole of weight 1 is the reference's DrudeUpdate; one Lorentz pole
- * of weight 1 is its LorentzUpdate (which never reads its delta). */
+ * of weight 1 is its LorentzUpdate (which never reads its delta); one
+ * FDTD_CONDUCTIVE pole {3, sigma [S/m], unused, unused} is its
+ * ConductiveUpdate (508-680): P = (P + f D)/(1 + f), f = dt sigma/(eps_r eps0),
+ * E = (D - P)/(eps_r eps0). */
 int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npoles, const double* poles);
 /* per-cell material id, nx*ny*nz bytes */
 int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids);

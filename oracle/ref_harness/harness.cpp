@@ -129,7 +129,7 @@ typedef Nb<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<dou
                    NewPML<ThreeD, double>>>
     Cell3;
 
-// material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz
+// material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz, 4 conductive (freq = the conductivity)
 struct Mat {
   int kind;
   double eps, freq, gamma, delta;
@@ -142,6 +142,7 @@ void material_E(C& c, const Mat& m, double dt) {
     case 1: ConstantUpdate<Mode, FieldType::Electric>(m.eps)(c); break;
     case 2: DrudeUpdate<Mode, FieldType::Electric>(m.eps, m.freq, m.gamma, dt)(c); break;
     case 3: LorentzUpdate<Mode, FieldType::Electric>(m.eps, m.delta, m.freq, m.gamma, dt)(c); break;
+    case 4: ConductiveUpdate<Mode, FieldType::Electric>(m.eps, m.freq, dt)(c); break;
   }
 }
 
@@ -1018,12 +1019,9 @@ int main(int argc, char** argv) {
     run_3d(S, dir);
   }
   // 0-D material recursions on one TM cell: prescribed D_n, reference functors.
-  {
+  auto cell_traces = [&](const char* name, double dt, const std::vector<Mat>& ms) {
     Fixture fx;
-    fx.name = "cell_ade";
-    const double dt = 1e-17;
-    std::vector<Mat> ms = {{2, 1.0, 1.37e16, 1e14, 0}, {3, 2.0, 3e15, 1e14, 0.8}, {2, 3.0, 5e15, 2e15, 0},
-                           {3, 1.0, 6e15, 0, 1.5}, {1, 7.5, 0, 0, 0}};
+    fx.name = name;
     std::vector<double> out, mt;
     const int nsteps = 300;
     for (auto& m : ms) {
@@ -1040,6 +1038,38 @@ int main(int argc, char** argv) {
     fx.arr("mats_kind_eps_freq_gamma_delta", mt, {ms.size(), 5});
     fx.arr("trace_D_E_P_J", out, {ms.size(), size_t(nsteps), 4});
     fx.write(dir);
+  };
+  cell_traces("cell_ade", 1e-17,
+              {{2, 1.0, 1.37e16, 1e14, 0}, {3, 2.0, 3e15, 1e14, 0.8}, {2, 3.0, 5e15, 2e15, 0}, {3, 1.0, 6e15, 0, 1.5},
+               {1, 7.5, 0, 0, 0}});
+  // conductive (include/DispersiveMaterials.hpp:508-680): factor = dt sigma/(eps_r eps0) from ~0.01 to ~2
+  cell_traces("cell_conductive", 1e-17, {{4, 1.0, 1e4, 0, 0}, {4, 2.5, 2e5, 0, 0}, {4, 7.0, 1.2e7, 0, 0}});
+  // 2D TM with conductive blocks next to a dielectric one, CPML x+y, Dz source
+  {
+    TMScenario S;
+    S.name = "tm2d_conductive";
+    S.nx = 46;
+    S.ny = 42;
+    S.nsteps = 160;
+    S.dx = 1e-6;
+    S.dt = 0.9 * S.dx / (c0 * std::sqrt(2.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {10, 3, 1, dm.msMax, 5, 0.05};
+    S.pmlx = S.pmly = true;
+    S.mats = {{0, 1, 0, 0, 0}, {4, 2.0, 1e6, 0, 0}, {4, 1.0, 5e4, 0, 0}, {1, 3.0, 0, 0, 0}};
+    S.matid.assign(size_t(S.nx) * S.ny, 0);
+    auto fill = [&](int x0, int x1, int y0, int y1, int m) {
+      for (int j = y0; j < y1; j++)
+        for (int i = x0; i < x1; i++) S.matid[size_t(i) + size_t(S.nx) * j] = m;
+    };
+    fill(14, 21, 13, 20, 1);
+    fill(26, 32, 22, 29, 2);
+    fill(14, 19, 24, 30, 3);
+    S.src_i = 23;
+    S.src_j = 21;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 25, 8));
+    S.probes = {{23, 21}, {17, 16}, {29, 25}, {12, 30}};
+    run_tm(S, dir);
   }
   printf("fixtures written to %s\n", dir.c_str());
   return 0;

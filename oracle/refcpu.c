@@ -773,7 +773,14 @@ int rc_step(rc_sim* s, int nsteps) {
                 s->E[I][c] = s->D[I][c] / (m->eps * RC_EPS0);
                 continue;
               }
-              for (int p = 0; p < m->np; p++) s->P[p][I][c] += dt * s->J[p][I][c];
+              for (int p = 0; p < m->np; p++) {
+                if (m->kind[p] == 3) { /* ConductiveUpdate::implicit_update (DispersiveMaterials.hpp:549-560) */
+                  const double factor = dt * m->freq[p] / (m->eps * RC_EPS0);
+                  s->P[p][I][c] = (s->P[p][I][c] + factor * s->D[I][c]) / (1.0 + factor);
+                } else {
+                  s->P[p][I][c] += dt * s->J[p][I][c];
+                }
+              }
               double num = s->D[I][c];
               for (int p = 0; p < m->np; p++) num = num - s->P[p][I][c];
               double Ev = num / (m->eps * RC_EPS0);
@@ -782,6 +789,7 @@ int rc_step(rc_sim* s, int nsteps) {
                 double* Jp = &s->J[p][I][c];
                 double* Pp = &s->P[p][I][c];
                 const double wgt_eps0 = m->weight[p] * RC_EPS0;
+                if (m->kind[p] == 3) continue; /* conductive: J untouched */
                 if (m->kind[p] == 1) {
                   double b = m->gamma[p] * dt * 0.5;
                   double w = dt * m->freq[p];

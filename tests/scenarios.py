@@ -41,10 +41,13 @@ class SplitMix64:
 
 
 def ref_material(kind, eps, freq, gamma, delta):
-    """Map a reference material (kind 0 vacuum, 1 constant, 2 Drude, 3 Lorentz) to
-    (eps_r, mu_r, poles).  The reference's Lorentz update never reads its delta
+    """Map a reference material (kind 0 vacuum, 1 constant, 2 Drude, 3 Lorentz,
+    4 conductive with freq = the conductivity) to (eps_r, mu_r, poles).  The
+    reference's Lorentz update never reads its delta
     (include/DispersiveMaterials.hpp:711-735), so its pole weight is 1."""
     kind = int(kind)
+    if kind == 4:
+        return (eps, 1.0, [(3, freq, 0.0, 1.0)])
     if kind == 0:
         return (1.0, 1.0, [])
     if kind == 1:

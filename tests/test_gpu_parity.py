@@ -49,7 +49,8 @@ def field_errors(g, o, sc):
     return {nm: rel_l2(g.get(f, c), o.get(f, c)) for nm, f, c in _comps(sc)}
 
 
-GOLDEN = ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small", "grid3d_disp", "tm2d_c1"]
+GOLDEN = ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small", "grid3d_disp", "tm2d_c1",
+          "tm2d_conductive"]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -295,3 +296,21 @@ def test_graph_chunks_tm():
     g.step(600)
     np.testing.assert_array_equal(g.probes(), ref.probes())
     np.testing.assert_array_equal(g.get(E, 2), ref.get(E, 2))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_3d_conductive_block(precision):
+    """A conductive block (ConductiveUpdate, one FDTD_CONDUCTIVE pole) next to a
+    Drude block in a 3-D CPML + TFSF + DFT grid, against the oracle."""
+    sc = synth.grid3d_cpml_tfsf_dft(n=(34, 30, 26), nsteps=120, npml=4, inset=7)
+    sc["materials"][1] = (2.0, 1.0, [(3, 1e6, 0.0, 1.0)])
+    sc["materials"][2] = (1.5, 1.0, [(1, 2e15, 1e14, 1.0)])
+    sc["name"] = "grid3d_conductive"
+    assert (sc["matid"] == 1).any() and (sc["matid"] == 2).any()
+    g, o = run_pair(sc, precision)
+    errs = field_errors(g, o, sc)
+    print(precision, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= TOL[precision], (k, v)
+    for c in range(3):
+        assert rel_l2(g.get(P, c), o.get(P, c)) <= TOL[precision] * 10

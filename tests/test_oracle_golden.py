@@ -41,8 +41,9 @@ def test_aux1d_matches_reference():
     np.testing.assert_array_equal(tr, a["trace_EH"])
 
 
-def test_cell_ade_recursions():
-    p, a = load("cell_ade")
+@pytest.mark.parametrize("name", ["cell_ade", "cell_conductive"])
+def test_cell_ade_recursions(name):
+    p, a = load(name)
     dt = p["dt"]
     mats = a["mats_kind_eps_freq_gamma_delta"]
     tr = a["trace_D_E_P_J"]
@@ -68,7 +69,7 @@ def test_cell_ade_recursions():
 
 
 @pytest.mark.parametrize("name", ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small",
-                                  "grid3d_disp", "tm2d_c1"])
+                                  "grid3d_disp", "tm2d_c1", "tm2d_conductive"])
 def test_grid_scenarios_bit_exact(name):
     sc, p, a = from_golden(name)
     sim = refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])
