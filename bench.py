@@ -184,6 +184,13 @@ def run_unit_cell(args):
         tile = g.autotune(trial_steps=30)
     g.step(args.warmup)
     ms, avg, per_step = timed_steps(g, args.steps)
+    # surface charge density and currents of one z-plane of the cell (a synchronous read-back call)
+    plane = [1, n - 1, 1, n - 1, n // 2, n // 2 + 1]
+    g.surface_quantities(plane)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        g.surface_quantities(plane)
+    sq_ms = (time.perf_counter() - t0) / 10 * 1e3
     abytes, interior = algorithmic_bytes(dict(sc, pml=dict(sc["pml"], n=0)), esz)
     # CPML along z only, and the ghost writes: 2 periodic axes x 2 faces x (E and H) x 3 comps, read + written
     abytes += 2 * 12 * (n - 2) * (n - 2) * 4 * 2 * esz
@@ -195,6 +202,7 @@ def run_unit_cell(args):
                      {"workload": f"3-D {n}^3 {args.precision.upper()} periodic unit cell: x and y periodic, 12-cell "
                                   f"CPML along z, 4 eps blocks, Dz + Bx point sources, DFT of 6 comps on one z-plane "
                                   f"x 4 freqs", "tile": list(tile), "kernel_avg_ms": avg,
+                      "surface_quantities_ms_per_plane_call": round(sq_ms, 4),
                       "kernel_ms_per_step": per_step,
                       "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -1298,6 +1298,25 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
 
 }  // namespace
 
+template <typename T>
+int surface_impl(fdtd_grid* g, const fdtdk::SurfRegion& r, long cells, double* out) {
+  const int nq = g->ndim == 3 ? 7 : 3;
+  double* dev = nullptr;
+  HIP_TRY(hipMallocAsync((void**)&dev, size_t(nq) * cells * sizeof(double), g->stream));
+  const long sy = g->d.nx, sz = long(g->d.nx) * g->d.ny;
+  auto F = [](void* p) { return (const T*)p; };
+  const T* z = nullptr;
+  hipLaunchKernelGGL(fdtdk::surface_quantities<T>, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, g->stream,
+                     g->ndim == 3 ? F(g->E[0]) : z, g->ndim == 3 ? F(g->E[1]) : z, F(g->E[2]), F(g->H[0]), F(g->H[1]),
+                     g->ndim == 3 ? F(g->H[2]) : z, sy, sz, r, g->ndim == 3 ? 1 : 0, g->d.dx, kEps0, dev);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dev, size_t(nq) * cells * sizeof(double), hipMemcpyDeviceToHost, g->stream);
+  (void)hipFreeAsync(dev, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) return fail(FDTD_EDEVICE, "surface quantities: %s", hipGetErrorString(e));
+  return 0;
+}
+
 extern "C" {
 
 const char* fdtd_last_error(void) { return g_err.c_str(); }
@@ -1735,6 +1754,20 @@ int fdtd_set_boundary(fdtd_grid* g, int axis, int side, int kind) {
   g->nbc += (kind != FDTD_BC_NONE) - (g->bc[axis][side] != FDTD_BC_NONE);
   g->bc[axis][side] = kind;
   return 0;
+}
+
+int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out) {
+  if (!g || !region || !out) return fail(FDTD_EINVAL, "null argument");
+  for (int a = 0; a < 3; a++) {
+    const int lo = a < g->ndim ? 1 : 0, hi = a < g->ndim ? g->len[a] - 1 : 1;
+    if (region[2 * a] < lo || region[2 * a + 1] > hi || region[2 * a] >= region[2 * a + 1])
+      return fail(FDTD_EINVAL, "region must be a non-empty box of interior cells");
+  }
+  int rc = 0;
+  if (!g->finalized && (rc = finalize(g))) return rc;
+  const fdtdk::SurfRegion r{region[0], region[1], region[2], region[3], region[4], region[5]};
+  const long cells = long(r.x1 - r.x0) * (r.y1 - r.y0) * (r.z1 - r.z0);
+  return g->f64 ? surface_impl<double>(g, r, cells, out) : surface_impl<float>(g, r, cells, out);
 }
 
 int fdtd_set_graph(fdtd_grid* g, int on) {

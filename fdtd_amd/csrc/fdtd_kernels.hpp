@@ -853,6 +853,47 @@ __global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h
   h_pass_3d_body<float, true>(ap, kc, s, ntx, nty, nkc);
 }
 
+// ---------------------------------------------------------------- surface quantities
+// UpdateSurfaceChargeDensity (SurfaceQuantities.hpp:174-183), UpdateSurfaceCurrentE
+// (233-244; TM 258-266) and UpdateSurfaceCurrentH (293-304; TM 319-327) on the
+// current E and H of the cells of region [x0,x1)x[y0,y1)x[z0,z1): one thread
+// per cell, FP64 arithmetic in the reference's operation order with no FMA
+// contraction, so FP64 grids give the reference's bits.  out: quantity-major,
+// x fastest; 3-D: rho, Jex, Jey, Jez, Jhx, Jhy, Jhz; TM: Jez, Jhx, Jhy.
+struct SurfRegion {
+  int x0, x1, y0, y1, z0, z1;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void surface_quantities(const T* __restrict__ Ex, const T* __restrict__ Ey,
+                                                          const T* __restrict__ Ez, const T* __restrict__ Hx,
+                                                          const T* __restrict__ Hy, const T* __restrict__ Hz, long sy,
+                                                          long sz, SurfRegion r, int three_d, double dx, double eps0,
+                                                          double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const long ex = r.x1 - r.x0, ey = r.y1 - r.y0, ez = r.z1 - r.z0;
+  const long cells = ex * ey * ez;
+  const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= cells) return;
+  const int i = r.x0 + int(q % ex), j = r.y0 + int((q / ex) % ey), k = r.z0 + int(q / (ex * ey));
+  const long c = (long)i + sy * (long)j + sz * (long)k;
+  auto v = [](const T* f, long x) { return (double)f[x]; };
+  if (three_d) {
+    out[0 * cells + q] = eps0 / dx *
+                         (v(Ex, c) + v(Ey, c) + v(Ez, c) - v(Ex, c - 1) - v(Ey, c - sy) - v(Ez, c - sz));
+    out[1 * cells + q] = 1.0 / dx * (v(Hz, c) - v(Hz, c - sy) - v(Hy, c) + v(Hy, c - sz));
+    out[2 * cells + q] = 1.0 / dx * (v(Hx, c) - v(Hx, c - sz) - v(Hz, c) + v(Hz, c - 1));
+    out[3 * cells + q] = 1.0 / dx * (v(Hy, c) - v(Hy, c - 1) - v(Hx, c) + v(Hx, c - sy));
+    out[4 * cells + q] = -1.0 / dx * (v(Ez, c + sy) - v(Ez, c) - v(Ey, c + sz) + v(Ey, c));
+    out[5 * cells + q] = -1.0 / dx * (v(Ex, c + sz) - v(Ex, c) - v(Ez, c + 1) + v(Ez, c));
+    out[6 * cells + q] = -1.0 / dx * (v(Ey, c + 1) - v(Ey, c) - v(Ex, c + sy) + v(Ex, c));
+  } else {
+    out[0 * cells + q] = 1.0 / dx * (v(Hy, c) - v(Hy, c - 1) - v(Hx, c) + v(Hx, c - sy));
+    out[1 * cells + q] = -1.0 / dx * (v(Ez, c + sy) - v(Ez, c));
+    out[2 * cells + q] = -1.0 / dx * (-v(Ez, c + 1) + v(Ez, c));
+  }
+}
+
 // ---------------------------------------------------------------- batched passes
 // Many independent grids in one launch per pass: workgroup b takes an entry
 // (grid index, tile x, tile y, tile z) of a table built on the host, in the

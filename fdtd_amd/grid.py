@@ -31,6 +31,7 @@ ABI_SYMBOLS = [
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
     "fdtd_set_design", "fdtd_adjoint_eps_gradient", "fdtd_adjoint_eps_gradient_batch",
     "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph", "fdtd_set_boundary",
+    "fdtd_surface_quantities",
 ]
 
 
@@ -80,6 +81,7 @@ def load_library():
         "fdtd_set_tile": (_i, [_p, _i, _i, _i]),
         "fdtd_set_graph": (_i, [_p, _i]),
         "fdtd_set_boundary": (_i, [_p, _i, _i, _i]),
+        "fdtd_surface_quantities": (_i, [_p, _ip, _dp]),
         "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
         "fdtd_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
         "fdtd_batch_create": (_i, [ctypes.POINTER(_p), _i, ctypes.POINTER(_p)]),
@@ -238,6 +240,17 @@ class Grid:
         fdtd::Boundary kinds) or its number; axis 0..2, side 0 (min) / 1 (max)."""
         k = BOUNDARY[kind] if isinstance(kind, str) else int(kind)
         _check(load_library().fdtd_set_boundary(self.h, int(axis), int(side), k))
+
+    def surface_quantities(self, region):
+        """Surface charge density and currents of the current state on `region`
+        (x0, x1, y0, y1, z0, z1): array (nq, z, y, x); 3-D rho, Jex, Jey, Jez, Jhx,
+        Jhy, Jhz; TM Jez, Jhx, Jhy."""
+        r = [int(v) for v in region]
+        nq = 7 if self.mode == "ThreeD" else 3
+        shape = (r[5] - r[4], r[3] - r[2], r[1] - r[0])
+        out = np.zeros(nq * int(np.prod(shape)))
+        _check(load_library().fdtd_surface_quantities(self.h, (ctypes.c_int * 6)(*r), _dptr(out)))
+        return out.reshape((nq,) + shape)
 
     def set_graph(self, on=True):
         """Capture chunks of steps in hipGraphs and replay them (fewer launches for small grids)."""

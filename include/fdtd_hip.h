@@ -155,6 +155,15 @@ int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc);
  * memory again).  The chosen tile is returned through bx, by, kc. */
 int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
 
+/* Surface charge density and surface currents of the current state on the
+ * cells of region [x0,x1)x[y0,y1)x[z0,z1) (interior cells; TM: z range
+ * [0,1)): replaces UpdateSurfaceChargeDensity / UpdateSurfaceCurrentE /
+ * UpdateSurfaceCurrentH (include/SurfaceQuantities.hpp:169-356) applied to
+ * those cells.  out: quantity-major, x fastest; 3-D 7 quantities rho, Jex,
+ * Jey, Jez, Jhx, Jhy, Jhz; TM 3: Jez, Jhx, Jhy (the reference's TM charge
+ * density functor does nothing).  FP64 arithmetic; synchronises. */
+int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out);
+
 /* Boundary condition on one face (axis 0..ndim-1, side 0 = min, 1 = max).
  * Replaces make_pec_boundary / make_pmc_boundary / make_symmetric_boundary /
  * make_antisymmetric_boundary / make_periodic_boundary + BoundaryData::

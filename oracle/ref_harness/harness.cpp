@@ -688,6 +688,130 @@ static double gauss_pulse(int n, double t0, double spread) {
   return std::exp(-a * a);
 }
 
+// ------------------------------------------------------------ scenario: surface quantities
+// The reference's UpdateSurfaceChargeDensity / UpdateSurfaceCurrentE /
+// UpdateSurfaceCurrentH (include/SurfaceQuantities.hpp:169-356) evaluated on
+// every interior cell of a seeded 3-D field state and of a seeded TM state.
+// The functors take any cell type with the field accessors, neighbour links
+// and SurfaceQuantities storage, so a thin view over the grid cells is used.
+template <class C, class SQ>
+struct SQView : public SQ {
+  C* c;
+  double& Ex() { return c->Ex(); }
+  double& Ey() { return c->Ey(); }
+  double& Ez() { return c->Ez(); }
+  double& Hx() { return c->Hx(); }
+  double& Hy() { return c->Hy(); }
+  double& Hz() { return c->Hz(); }
+  C& getNeighborMin(int d) { return c->getNeighborMin(d); }
+  C& getNeighborMax(int d) { return c->getNeighborMax(d); }
+};
+
+static void scen_surface(const std::string& dir) {
+  Fixture fx;
+  fx.name = "surface_quantities";
+  const double dx = 1e-6;
+  // 3-D
+  {
+    const int n[3] = {9, 8, 7};
+    const int nx = n[0], ny = n[1], nz = n[2];
+    std::vector<Cell3> cells(size_t(nx) * ny * nz);
+    auto at = [&](int i, int j, int k) -> Cell3& { return cells[size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k)]; };
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          Cell3& c = at(i, j, k);
+          const int ci[3] = {i, j, k};
+          for (int a = 0; a < 3; a++) {
+            int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+            lo[a]--;
+            hi[a]++;
+            c.mn[a] = ci[a] > 0 ? &at(lo[0], lo[1], lo[2]) : nullptr;
+            c.mx[a] = ci[a] < n[a] - 1 ? &at(hi[0], hi[1], hi[2]) : nullptr;
+          }
+        }
+    SplitMix64 rng(13);
+    std::vector<double> init;
+    for (auto& c : cells) {
+      c.Ex() = rng.uniform() - 0.5;
+      c.Ey() = rng.uniform() - 0.5;
+      c.Ez() = rng.uniform() - 0.5;
+      c.Hx() = (rng.uniform() - 0.5) * 2e-3;
+      c.Hy() = (rng.uniform() - 0.5) * 2e-3;
+      c.Hz() = (rng.uniform() - 0.5) * 2e-3;
+    }
+    for (int f = 0; f < 6; f++)
+      for (auto& c : cells) init.push_back(f == 0 ? c.Ex() : f == 1 ? c.Ey() : f == 2 ? c.Ez() : f == 3 ? c.Hx() : f == 4 ? c.Hy() : c.Hz());
+    std::vector<double> out;
+    UpdateSurfaceChargeDensity<ThreeD> rho(dx);
+    UpdateSurfaceCurrentE<ThreeD> je(dx);
+    UpdateSurfaceCurrentH<ThreeD> jh(dx);
+    for (int k = 1; k < nz - 1; k++)
+      for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+          SQView<Cell3, SurfaceQuantities<ThreeD>> v;
+          v.c = &at(i, j, k);
+          rho(v);
+          je(v);
+          jh(v);
+          out.insert(out.end(), {v.rho(), v.Jex(), v.Jey(), v.Jez(), v.Jhx(), v.Jhy(), v.Jhz()});
+        }
+    fx.pi("nx", nx);
+    fx.pi("ny", ny);
+    fx.pi("nz", nz);
+    fx.p("dx", dx);
+    fx.arr("init3_E_H", init, {6, size_t(nz), size_t(ny), size_t(nx)});
+    fx.arr("sq3_rho_Je_Jh", out, {size_t(nz - 2), size_t(ny - 2), size_t(nx - 2), 7});
+  }
+  // TM: Jez, Jhx, Jhy (the TM charge density functor is a no-op)
+  {
+    const int nx = 10, ny = 8;
+    std::vector<CellTM> cells(size_t(nx) * ny);
+    auto at = [&](int i, int j) -> CellTM& { return cells[size_t(i) + size_t(nx) * j]; };
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        CellTM& c = at(i, j);
+        c.mn[0] = i > 0 ? &at(i - 1, j) : nullptr;
+        c.mx[0] = i < nx - 1 ? &at(i + 1, j) : nullptr;
+        c.mn[1] = j > 0 ? &at(i, j - 1) : nullptr;
+        c.mx[1] = j < ny - 1 ? &at(i, j + 1) : nullptr;
+      }
+    SplitMix64 rng(17);
+    std::vector<double> init;
+    for (auto& c : cells) {
+      c.Ez() = rng.uniform() - 0.5;
+      c.Hx() = (rng.uniform() - 0.5) * 2e-3;
+      c.Hy() = (rng.uniform() - 0.5) * 2e-3;
+    }
+    for (int f = 0; f < 3; f++)
+      for (auto& c : cells) init.push_back(f == 0 ? c.Ez() : f == 1 ? c.Hx() : c.Hy());
+    std::vector<double> out;
+    UpdateSurfaceCurrentE<TM> je(dx);
+    UpdateSurfaceCurrentH<TM> jh(dx);
+    struct TMView : public SurfaceQuantities<TM> {
+      CellTM* c;
+      double& Ez() { return c->Ez(); }
+      double& Hx() { return c->Hx(); }
+      double& Hy() { return c->Hy(); }
+      CellTM& getNeighborMin(int d) { return c->getNeighborMin(d); }
+      CellTM& getNeighborMax(int d) { return c->getNeighborMax(d); }
+    };
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) {
+        TMView v;
+        v.c = &at(i, j);
+        je(v);
+        jh(v);
+        out.insert(out.end(), {v.Jez(), v.Jhx(), v.Jhy()});
+      }
+    fx.pi("tm_nx", nx);
+    fx.pi("tm_ny", ny);
+    fx.arr("initTM_Ez_Hx_Hy", init, {3, size_t(ny), size_t(nx)});
+    fx.arr("sqTM_Jez_Jhx_Jhy", out, {size_t(ny - 2), size_t(nx - 2), 3});
+  }
+  fx.write(dir);
+}
+
 // ------------------------------------------------------------ scenario: boundary updaters
 // The reference's own make_{pec,pmc,symmetric,antisymmetric,periodic}_boundary
 // (include/BoundaryUpdates.hpp:527-832) applied to the planes next to each face
@@ -874,6 +998,7 @@ int main(int argc, char** argv) {
   std::string dir = argv[1];
   scen_pml_coeffs(dir);
   scen_boundaries(dir);
+  scen_surface(dir);
   scen_aux1d(dir);
 
   // 2D TM, small, non-square, CPML x+y, two dielectric blocks, one Drude and

@@ -906,6 +906,49 @@ int rc_step(rc_sim* s, int nsteps) {
   return 0;
 }
 
+/* ---------------------------------------------------------------- surface quantities
+ * SurfaceQuantities.hpp: UpdateSurfaceChargeDensity (174-183), UpdateSurfaceCurrentE
+ * (233-244, TM 258-266) and UpdateSurfaceCurrentH (293-304, TM 319-327), in the
+ * reference's operation order, on the current E and H of the cells of
+ * `region` ([x0,x1)x[y0,y1)x[z0,z1), neighbours inside the grid).  out:
+ * quantity-major, x fastest; 3-D: rho, Jex, Jey, Jez, Jhx, Jhy, Jhz; TM: Jez,
+ * Jhx, Jhy (the TM charge-density functor does nothing). */
+int rc_surface_quantities(const rc_sim* s, const int r[6], double* out) {
+  const int n[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; a++) {
+    const int lo = s->axes[a] ? 1 : 0, hi = s->axes[a] ? n[a] - 1 : 1;
+    if (r[2 * a] < lo || r[2 * a + 1] > hi || r[2 * a] >= r[2 * a + 1]) return -1;
+  }
+  const size_t cells = (size_t)(r[1] - r[0]) * (r[3] - r[2]) * (r[5] - r[4]);
+  const double dx = s->dx;
+  const size_t sy = (size_t)s->nx, sz = (size_t)s->nx * s->ny;
+  size_t q = 0;
+  for (int k = r[4]; k < r[5]; k++)
+    for (int j = r[2]; j < r[3]; j++)
+      for (int i = r[0]; i < r[1]; i++, q++) {
+        const size_t c = cidx(s, i, j, k);
+        if (s->mode == 3) {
+          double* const* E = s->E;
+          double* const* H = s->H;
+          out[0 * cells + q] = RC_EPS0 / dx * (E[0][c] + E[1][c] + E[2][c] - E[0][c - 1] - E[1][c - sy] - E[2][c - sz]);
+          out[1 * cells + q] = 1.0 / dx * (H[2][c] - H[2][c - sy] - H[1][c] + H[1][c - sz]);
+          out[2 * cells + q] = 1.0 / dx * (H[0][c] - H[0][c - sz] - H[2][c] + H[2][c - 1]);
+          out[3 * cells + q] = 1.0 / dx * (H[1][c] - H[1][c - 1] - H[0][c] + H[0][c - sy]);
+          out[4 * cells + q] = -1.0 / dx * (E[2][c + sy] - E[2][c] - E[1][c + sz] + E[1][c]);
+          out[5 * cells + q] = -1.0 / dx * (E[0][c + sz] - E[0][c] - E[2][c + 1] + E[2][c]);
+          out[6 * cells + q] = -1.0 / dx * (E[1][c + 1] - E[1][c] - E[0][c + sy] + E[0][c]);
+        } else {
+          const double* Ez = s->E[2];
+          const double* Hx = s->H[0];
+          const double* Hy = s->H[1];
+          out[0 * cells + q] = 1.0 / dx * (Hy[c] - Hy[c - 1] - Hx[c] + Hx[c - sy]);
+          out[1 * cells + q] = -1.0 / dx * (Ez[c + sy] - Ez[c]);
+          out[2 * cells + q] = -1.0 / dx * (-Ez[c + 1] + Ez[c]);
+        }
+      }
+  return 0;
+}
+
 /* ---------------------------------------------------------------- adjoint
  * Build-defined (the reference has no adjoint).  Reverse mode of rc_step for
  * non-dispersive cells, op by op in the D/B form above: every forward update

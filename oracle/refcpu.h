@@ -95,6 +95,9 @@ int rc_set_boundary(rc_sim* s, int axis, int side, int type);
 double rc_bc_mask(int type, int ft, int comp, int axis);
 /* the ghost-layer writes of one field type (0: E, 1: H), as done inside rc_step */
 void rc_apply_boundaries(rc_sim* s, int ft);
+
+/* surface charge density and surface currents on a region (SurfaceQuantities.hpp) */
+int rc_surface_quantities(const rc_sim* s, const int region[6], double* out);
 /* E = D/(eps_r eps0), H = B/(mu_r mu0) on interior cells (non-dispersive) */
 void rc_sync_derived(rc_sim* s);
 /* Gradient from step 0 over nsteps of J = sum over the cells of DFT monitor `dft`, its E components in

@@ -56,6 +56,7 @@ def lib():
             ("rc_set_boundary", _i, [_p, _i, _i, _i]),
             ("rc_bc_mask", ctypes.c_double, [_i, _i, _i, _i]),
             ("rc_apply_boundaries", None, [_p, _i]),
+            ("rc_surface_quantities", _i, [_p, _ip, _dp]),
             ("rc_adjoint_eps_gradient", _i, [_p, _i, _i, _i, _dp, _dp]),
             ("rc_adjoint_reset", None, [_p]), ("rc_adjoint_step", None, [_p, ctypes.c_long]),
             ("rc_adjoint_finish", None, [_p]),
@@ -229,6 +230,15 @@ class Sim:
 
     def apply_boundaries(self, ft):
         lib().rc_apply_boundaries(self.h, ft)
+
+    def surface_quantities(self, region):
+        """(nq, z, y, x) array: 3-D rho, Jex, Jey, Jez, Jhx, Jhy, Jhz; TM Jez, Jhx, Jhy."""
+        r = [int(v) for v in region]
+        nq = 7 if self.mode == "ThreeD" else 3
+        shape = (r[5] - r[4], r[3] - r[2], r[1] - r[0])
+        out = np.zeros(nq * int(np.prod(shape)))
+        assert lib().rc_surface_quantities(self.h, (ctypes.c_int * 6)(*r), _dptr(out)) == 0, region
+        return out.reshape((nq,) + shape)
 
     # ---- adjoint (build-defined)
     def set_design(self, box, eps_r):

@@ -100,3 +100,28 @@ def test_grid_scenarios_bit_exact(name):
             np.testing.assert_array_equal(sim.get(D, c).reshape(-1)[q], dq[:, 1 + c])
             np.testing.assert_array_equal(sim.get(4, c).reshape(-1)[q], dq[:, 4 + c])
             np.testing.assert_array_equal(sim.get(5, c).reshape(-1)[q], dq[:, 7 + c])
+
+
+def test_surface_quantities_match_reference_functors():
+    """rc_surface_quantities against the reference's UpdateSurfaceChargeDensity /
+    UpdateSurfaceCurrentE / UpdateSurfaceCurrentH (include/SurfaceQuantities.hpp)
+    on seeded 3-D and TM field states, bit for bit."""
+    p, a = load("surface_quantities")
+    nx, ny, nz, dx = int(p["nx"]), int(p["ny"]), int(p["nz"]), p["dx"]
+    init = a["init3_E_H"].reshape(6, nz, ny, nx)
+    sim = refcpu.Sim("ThreeD", nx, ny, nz, dx, 1e-15)
+    for c in range(3):
+        sim.set(E, c, init[c])
+        sim.set(H, c, init[3 + c])
+    got = sim.surface_quantities([1, nx - 1, 1, ny - 1, 1, nz - 1])
+    want = a["sq3_rho_Je_Jh"].reshape(nz - 2, ny - 2, nx - 2, 7)
+    np.testing.assert_array_equal(np.moveaxis(got, 0, -1), want)
+    tnx, tny = int(p["tm_nx"]), int(p["tm_ny"])
+    init = a["initTM_Ez_Hx_Hy"].reshape(3, 1, tny, tnx)
+    sim = refcpu.Sim("TM", tnx, tny, 1, dx, 1e-15)
+    sim.set(E, 2, init[0])
+    sim.set(H, 0, init[1])
+    sim.set(H, 1, init[2])
+    got = sim.surface_quantities([1, tnx - 1, 1, tny - 1, 0, 1])
+    want = a["sqTM_Jez_Jhx_Jhy"].reshape(1, tny - 2, tnx - 2, 3)
+    np.testing.assert_array_equal(np.moveaxis(got, 0, -1), want)
