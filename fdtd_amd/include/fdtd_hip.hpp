@@ -282,6 +282,26 @@ void set_material(YeeGrid<Mode, scalar>& g, int id, const MaterialSpec& s) {
         "fdtd_set_material");
 }
 
+// MaterialPair / MaterialMap (include/MaterialMap.hpp:45-284): a named pair
+// of electric and magnetic models.  The device takes the electric model's
+// eps_r and poles and the magnetic model's mu_r (magnetic dispersion has no
+// device counterpart).  The reference's XML form is read and written by
+// fdtd_amd/materials.py.
+struct MaterialPair {
+  MaterialSpec electric, magnetic;
+};
+template <class E, class M>
+MaterialPair make_material_pair(const E& e, const M& m) {
+  return {e.spec(), m.spec()};
+}
+using MaterialMap = std::map<std::string, MaterialPair>;
+inline MaterialSpec combine(const std::string& name, const MaterialPair& p) {
+  if (!p.magnetic.poles.empty()) throw Error("material " + name + ": magnetic dispersion has no device counterpart");
+  MaterialSpec s = p.electric;
+  s.mu = p.magnetic.mu;
+  return s;
+}
+
 // ---------------------------------------------------------------- sources
 // SoftSource (include/Source.hpp:183-261): adds signal(t) to the cell's flux.
 template <class Mode, class scalar>
@@ -449,6 +469,16 @@ class Stepper {
   template <class M>
   Stepper& material(int id, const M& m) {
     set_material(g_, id, m.spec());
+    if (stage_ < 3) stage_ = 3;
+    return *this;
+  }
+  // every named pair of `mm` listed in `ids` (name -> material id)
+  Stepper& materials(const MaterialMap& mm, const std::map<std::string, int>& ids) {
+    for (const auto& kv : ids) {
+      auto it = mm.find(kv.first);
+      if (it == mm.end()) throw Error("no material named " + kv.first);
+      set_material(g_, kv.second, combine(kv.first, it->second));
+    }
     if (stage_ < 3) stage_ = 3;
     return *this;
   }

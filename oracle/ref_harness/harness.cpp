@@ -22,6 +22,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -688,6 +689,30 @@ static double gauss_pulse(int n, double t0, double spread) {
   return std::exp(-a * a);
 }
 
+// ------------------------------------------------------------ scenario: material map XML
+// The reference's MaterialMap / MaterialPair (include/MaterialMap.hpp:45-284)
+// holding one material of every kind the device path covers, written by its
+// own print_summary (the XML form its readXML reads back).  Part of that
+// output goes to std::cout rather than the given stream, so both are captured.
+static void scen_material_map(const std::string& dir) {
+  const double dt = 2e-17;
+  typedef MaterialMap<std::string, CellTM&> MapT;
+  MapT mm;
+  mm.insert(std::make_pair(std::string("vacuum"), make_material_pair<CellTM&>(VacuumUpdate<TM, FieldType::Electric>(), VacuumUpdate<TM, FieldType::Magnetic>())));
+  mm.insert(std::make_pair(std::string("glass"), make_material_pair<CellTM&>(ConstantUpdate<TM, FieldType::Electric>(2.25), ConstantUpdate<TM, FieldType::Magnetic>(1.0))));
+  mm.insert(std::make_pair(std::string("ferrite"), make_material_pair<CellTM&>(ConstantUpdate<TM, FieldType::Electric>(4.5), ConstantUpdate<TM, FieldType::Magnetic>(3.2))));
+  mm.insert(std::make_pair(std::string("conductor"), make_material_pair<CellTM&>(ConductiveUpdate<TM, FieldType::Electric>(2.0, 5e5, dt), VacuumUpdate<TM, FieldType::Magnetic>())));
+  mm.insert(std::make_pair(std::string("silver"), make_material_pair<CellTM&>(DrudeUpdate<TM, FieldType::Electric>(3.7, 1.37e16, 2.73e13, dt), VacuumUpdate<TM, FieldType::Magnetic>())));
+  mm.insert(std::make_pair(std::string("resonator"), make_material_pair<CellTM&>(LorentzUpdate<TM, FieldType::Electric>(1.5, 0.8, 4.1e15, 1.2e14, dt), ConstantUpdate<TM, FieldType::Magnetic>(1.0))));
+  std::ostringstream os;
+  std::streambuf* old = std::cout.rdbuf(os.rdbuf());
+  mm.print_summary(os, 0);
+  std::cout.rdbuf(old);
+  FILE* f = fopen((dir + "/material_map.xml").c_str(), "w");
+  fputs(os.str().c_str(), f);
+  fclose(f);
+}
+
 // ------------------------------------------------------------ scenario: surface quantities
 // The reference's UpdateSurfaceChargeDensity / UpdateSurfaceCurrentE /
 // UpdateSurfaceCurrentH (include/SurfaceQuantities.hpp:169-356) evaluated on
@@ -999,6 +1024,7 @@ int main(int argc, char** argv) {
   scen_pml_coeffs(dir);
   scen_boundaries(dir);
   scen_surface(dir);
+  scen_material_map(dir);
   scen_aux1d(dir);
 
   // 2D TM, small, non-square, CPML x+y, two dielectric blocks, one Drude and

@@ -28,11 +28,14 @@ int main() {
   BoundaryOptions bo;  // PEC on every face by default, as in the reference
   bo(Dir::X, Orientation::MIN) = Boundary::Periodic;
   bo.get<Dir::X, Orientation::MAX>() = Boundary::Periodic;
+  MaterialMap mm;  // the reference's MaterialMap idiom (include/MaterialMap.hpp:30-37)
+  mm["vacuum"] = make_material_pair(VacuumUpdate<TM, FieldType::Electric>(), VacuumUpdate<TM, FieldType::Magnetic>());
+  mm["dielectric"] =
+      make_material_pair(ConstantUpdate<TM, FieldType::Electric>(4.0), ConstantUpdate<TM, FieldType::Magnetic>(1.0));
   Stepper<TM, double> step(grid);
   step.then(YeeUpdate<TM, Dz>(dt, dx))
       .source(src)
-      .material(0, VacuumUpdate<TM, FieldType::Electric>())
-      .material(1, ConstantUpdate<TM, FieldType::Electric>(4.0))
+      .materials(mm, {{"vacuum", 0}, {"dielectric", 1}})
       .then(YeeUpdate<TM, Bx>(dt, dx))
       .then(YeeUpdate<TM, By>(dt, dx))
       .boundaries(bo)

@@ -1,0 +1,72 @@
+"""Material registry (fdtd_amd.materials) against the reference's MaterialMap.
+
+tests/golden/material_map.xml is what the reference's MaterialMap::print_summary
+writes for one material of every model the device path covers
+(oracle/ref_harness/harness.cpp, scen_material_map).  Reading it must give
+those materials; writing them back must give the same text byte for byte; the
+resulting material table must step like hand-set materials.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fdtd_amd.materials import Dispersion, MaterialMap, MaterialPair
+from oracle import refcpu
+from scenarios import D, E, gauss_table
+
+XML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "material_map.xml")
+
+
+def test_reads_the_reference_print_summary():
+    mm = MaterialMap.read_xml(XML)
+    assert sorted(mm) == ["conductor", "ferrite", "glass", "resonator", "silver", "vacuum"]
+    assert mm["vacuum"].spec() == (1.0, 1.0, [])
+    assert mm["glass"].spec() == (2.25, 1.0, [])
+    assert mm["ferrite"].spec() == (4.5, 3.2, [])
+    assert mm["conductor"].spec() == (2.0, 1.0, [(3, 5e5, 0.0, 1.0)])
+    assert mm["silver"].spec() == (3.7, 1.0, [(1, 1.37e16, 2.73e13, 1.0)])
+    # Lorentz: Delta is read and kept, but (as in the reference's update) not used
+    assert mm["resonator"].electric.delta == 0.8
+    assert mm["resonator"].spec() == (1.5, 1.0, [(2, 4.1e15, 1.2e14, 1.0)])
+
+
+def test_writes_the_reference_text():
+    text = open(XML).read()
+    assert MaterialMap.read_xml(text).to_xml() == text
+
+
+def test_unsupported_models_raise():
+    mm = MaterialMap(x=MaterialPair(Dispersion("MagnetizedDrude"), Dispersion("Vacuum")),
+                     y=MaterialPair(Dispersion("Constant", K=2.0), Dispersion("Drude", 1.0, 1e15, 1e13)))
+    with pytest.raises(NotImplementedError):
+        mm["x"].spec()
+    with pytest.raises(NotImplementedError, match="magnetic"):
+        mm["y"].spec()
+    with pytest.raises(ValueError):
+        Dispersion("Plasma")
+
+
+def test_applied_map_steps_like_hand_set_materials():
+    mm = MaterialMap.read_xml(XML)
+    ids = {"vacuum": 0, "glass": 1, "conductor": 2, "silver": 3, "resonator": 4, "ferrite": 5}
+    nx = ny = 24
+    dx = 1e-6
+    dt = 0.5 * dx / (refcpu.constants()["c0"] * np.sqrt(2.0))
+    matid = np.zeros((1, ny, nx), np.uint8)
+    for q, (x0, y0) in enumerate([(3, 3), (12, 3), (3, 12), (12, 12), (17, 17)], start=1):
+        matid[0, y0:y0 + 5, x0:x0 + 5] = q
+    runs = []
+    for hand in (False, True):
+        s = refcpu.Sim("TM", nx, ny, 1, dx, dt)
+        if hand:
+            for name, mid in ids.items():
+                s.set_material(mid, *mm[name].spec())
+        else:
+            mm.apply(s, ids)
+        s.set_matid(matid)
+        s.add_source(D, 2, 11, 11, 0, gauss_table(60, 15, 5))
+        s.step(60)
+        runs.append(s.get(E, 2))
+    np.testing.assert_array_equal(runs[0], runs[1])
+    assert np.abs(runs[0]).max() > 0
