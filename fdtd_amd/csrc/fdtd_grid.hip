@@ -341,6 +341,7 @@ int upload_materials(fdtd_grid* g) {
     m[id].inv_eps = T(1.0 / (h.eps * kEps0));
     m[id].inv_mu = T(1.0 / (h.mu * kMu0));
     m[id].eps_e0 = T(h.eps * kEps0);
+    m[id].cG = T(1.0);
     m[id].np = h.np;
     for (int p = 0; p < fdtdk::kMaxPoles; p++) {
       double cJ = 0, cE = 0, cP = 0, cD = 0;
@@ -365,6 +366,7 @@ int upload_materials(fdtd_grid* g) {
       m[id].cE[p] = T(cE);
       m[id].cP[p] = T(cP);
       m[id].cD[p] = T(cD);
+      if (p == 0) m[id].cG = T(1.0 / (1.0 + cD));
     }
   }
   HIP_TRY(hipMemcpy(g->mats_dev, m.data(), sizeof(fdtdk::MatDev<T>) * m.size(), hipMemcpyHostToDevice));
@@ -873,13 +875,22 @@ int step_impl(fdtd_grid* g, int nsteps) {
           else
             rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         } else if (g->ndim == 3) {
-          if (g->nbc) {  // the instances that also write the boundary ghost layers
-            rc = launch(g, 0, fdtdk::e_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-            if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-          } else {
-            rc = launch(g, 0, fdtdk::e_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-            if (!rc) rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-          }
+          // instances that also write the boundary ghost layers (BC)
+          // and the E-pass instance without the ADE code when no material is dispersive
+          const bool disp = g->max_np > 0;
+          if (g->nbc && disp)
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, true, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          else if (g->nbc)
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, true, false>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          else if (disp)
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, false, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          else
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, false, false>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          if (rc) return rc;
+          if (g->nbc)
+            rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          else
+            rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
         } else if (g->nbc) {
           rc = launch(g, 0, fdtdk::e_pass_tm<T, true>, grid, block, dev0, 1, s);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T, true>, grid, block, dev0, 1, s);

@@ -54,6 +54,7 @@ struct MatDev {
   // (ConductiveUpdate::implicit_update, DispersiveMaterials.hpp:549-560); 0 for
   // Drude / Lorentz poles, whose P advances by dt J
   T cD[kMaxPoles];
+  T cG;      // 1/(1 + cD[0]): the FP32 form of a lone conductive pole
   T eps_e0;  // eps_r eps0
 };
 
@@ -288,7 +289,7 @@ __device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>
     // FP32, a lone conductive pole: E_{n+1} = (E_n + d/(eps_r eps0))/(1 + f),
     // the same update without the cancellation of D - P (P -> D in a good
     // conductor, which in FP32 leaves E at a few digits); P = D - eps_r eps0 E
-    const T Ev = (Eold + d * m.inv_eps) / (T(1) + m.cD[0]);
+    const T Ev = (Eold + d * m.inv_eps) * m.cG;
     if (own) {
       a.Ddo[I][bq] = D;
       a.Pdo[0][I][bq] = D - m.eps_e0 * Ev;
@@ -354,7 +355,9 @@ struct EStencil {
 // ---------------------------------------------------------------- E of one cell
 // New E of interior cell (i,j,k).  psi and the dispersive state are read from
 // the current buffers and, when `own`, written to the next buffers.
-template <typename T>
+// DISP = false: the grid has no dispersive material (the ADE code is left
+// out of that instance, which keeps its register count down).
+template <typename T, bool DISP = true>
 __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incH, int i, int j, int k,
                                           long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy,
                                           T fz, bool own, T out[3]) {
@@ -413,7 +416,7 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
     out[0] = ex + d[0] * ie;
     out[1] = ey + d[1] * ie;
     out[2] = ez + d[2] * ie;
-  } else if (ie < T(0) && a.Dd[0]) {
+  } else if (DISP && ie < T(0) && a.Dd[0]) {
     const T eold[3] = {ex, ey, ez};
     ade_cell(a, mid, i, j, k, d, eold, own, out);
   } else {
@@ -719,9 +722,10 @@ __device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a, int ft, int ndim,
 // ---------------------------------------------------------------- two passes
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
-// BC: the instance for grids with boundary conditions (ghost writes); the
-// plain instance carries none of that code, so its registers stay as they were.
-template <typename T, bool BC = false>
+// BC: the instance for grids with boundary conditions (ghost writes); DISP:
+// for grids with dispersive materials (ADE).  An instance without them
+// carries none of that code, so its registers stay low.
+template <typename T, bool BC = false, bool DISP = true>
 __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie, int i, int j, int k0, int k1, int s) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   const FDTD_G T* __restrict__ Hx = a.H[0];
@@ -750,7 +754,8 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     const int mid = a.mat[c];
     const T fz = a.cf[0][2][k];
     T o[3];
-    e_cell_3d(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, cell_inv_eps(a, tie, mid, i, j, k), fx, fy, fz, true, o);
+    e_cell_3d<T, DISP>(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, cell_inv_eps(a, tie, mid, i, j, k), fx, fy, fz,
+                       true, o);
     a.Eo[0][c] = o[0];
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
@@ -804,7 +809,7 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int 
 
 // One-dimensional launch of ntx x nty x nkc tiles of blockDim.x x blockDim.y
 // columns and kc planes, XCD-aware (tiles x fastest, then y, then z chunks).
-template <typename T, bool BC = false>
+template <typename T, bool BC = false, bool DISP = true>
 __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                  int nkc) {
   const StepArgs<T>& a = *ap;
@@ -813,8 +818,8 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
   const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
   const int k0 = 1 + bz * kc;
-  e_pass_cells<T, BC>(a, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
-                      min(k0 + kc, a.nz - 1), s);
+  e_pass_cells<T, BC, DISP>(a, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
+                            min(k0 + kc, a.nz - 1), s);
 }
 
 // Each XCD walks its tile range in reverse: its first tiles read the E and H
