@@ -44,6 +44,9 @@ int fail(int code, const char* fmt, ...) {
 // include/FDTDConstants.hpp:394-398 (imp0 via the same Newton iteration)
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr double kEps0 = 8.854e-12;
+// electron charge and mass of the reference's Drude models (DispersiveMaterials.hpp:899-900)
+constexpr double kQe = 1.6e-19;
+constexpr double kMe = 9.11e-31;
 const double kMu0 = kPi * 4.0e-7;
 double newton_sqrt(double x) {
   double curr = x, prev = 0;
@@ -146,6 +149,9 @@ struct MatHost {
   int np = 0;
   int kind[FDTD_MAX_POLES] = {0, 0, 0};
   double freq[FDTD_MAX_POLES] = {0, 0, 0}, gamma[FDTD_MAX_POLES] = {0, 0, 0}, weight[FDTD_MAX_POLES] = {0, 0, 0};
+  // magnetized Drude: 0 none, 1 fixed bias field `bias` [T], 2 the cell's own B
+  int mag = 0;
+  double bias[3] = {0, 0, 0};
 };
 
 struct fdtd_grid {
@@ -232,6 +238,7 @@ struct fdtd_grid {
   bool force_two_pass = false;  // the design increments are recorded by the two-pass E pass
   int bc[3][2] = {};            // boundary condition per [axis][side] (FDTD_BC_*)
   int nbc = 0;                  // faces with a condition (they need the two-pass schedule)
+  int nmag = 0;                 // magnetized Drude materials in the table (3-D: two-pass schedule, own E instance)
   bool recompute = false;       // segment recompute: no DFT accumulation, no probes
   void* hist = nullptr;         // T[hist_rows][3][ndes]
   long hist_step0 = 0, hist_rows = 0;
@@ -332,6 +339,14 @@ int download_T(fdtd_grid* g, double* dst, const void* src, size_t n) {
   return 0;
 }
 
+// a 3-D grid whose table holds a magnetized Drude material (nmag is counted
+// by the material setters)
+static bool has_magnetized(const fdtd_grid* g) { return g->ndim == 3 && g->nmag > 0; }
+static void count_magnetized(fdtd_grid* g) {
+  g->nmag = 0;
+  for (int id = 0; id < FDTD_MAX_MATERIALS; id++) g->nmag += g->mats[id].np > 0 && g->mats[id].mag != 0;
+}
+
 template <typename T>
 int upload_materials(fdtd_grid* g) {
   std::vector<fdtdk::MatDev<T>> m(FDTD_MAX_MATERIALS);
@@ -343,6 +358,25 @@ int upload_materials(fdtd_grid* g) {
     m[id].eps_e0 = T(h.eps * kEps0);
     m[id].cG = T(1.0);
     m[id].np = h.np;
+    m[id].mag = 0;
+    for (int q = 0; q < 9; q++) m[id].mM[q] = T(0);
+    m[id].w2dt = m[id].op = m[id].vq = T(0);
+    if (h.mag && g->ndim == 3) {
+      // MagnetizedDrudeUpdate3D::implicit_update, DispersiveMaterials.hpp:1297-1318
+      const double wp = dt * h.freq[0], vc = dt * h.gamma[0], op = 1.0 + vc + wp * wp;
+      m[id].mag = h.mag;
+      m[id].w2dt = T(wp * wp / dt);
+      m[id].op = T(op);
+      m[id].vq = T(dt * kQe / kMe);
+      if (h.mag == 1) {
+        const double vx = dt * h.bias[0] * kQe / kMe, vy = dt * h.bias[1] * kQe / kMe, vz = dt * h.bias[2] * kQe / kMe;
+        const double denom = op * (op * op + (vx * vx + vy * vy + vz * vz));
+        const double A[9] = {op * op + vx * vx, -vz * op + vx * vy, vy * op + vx * vz,
+                             vz * op + vx * vy, op * op + vy * vy,  -vx * op + vy * vz,
+                             -vy * op + vx * vz, vx * op + vy * vz, op * op + vz * vz};
+        for (int q = 0; q < 9; q++) m[id].mM[q] = T(A[q] / denom);
+      }
+    }
     for (int p = 0; p < fdtdk::kMaxPoles; p++) {
       double cJ = 0, cE = 0, cP = 0, cD = 0;
       if (p < h.np && h.kind[p] == FDTD_CONDUCTIVE) {
@@ -351,7 +385,9 @@ int upload_materials(fdtd_grid* g) {
         cD = dt * h.freq[p] / (h.eps * kEps0);
         cJ = 1.0;
       } else if (p < h.np && h.kind[p] == FDTD_DRUDE) {
-        // DrudeUpdate::implicit_update, DispersiveMaterials.hpp:986-996
+        // DrudeUpdate::implicit_update, DispersiveMaterials.hpp:986-996 (also a
+        // magnetized Drude material in TM, whose update the reference routes
+        // through DrudeUpdate per component: MagnetizedDrudeCall, 1508-1537)
         double b = h.gamma[p] * dt * 0.5, w = dt * h.freq[p];
         cJ = (1.0 - b) / (1.0 + b);
         cE = w * w / dt * (h.weight[p] * kEps0);
@@ -781,7 +817,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (g->ndim == 2) t.kc = 1;
   // design increments and boundary ghost writes are made by the two-pass kernels
-  if ((g->force_two_pass || g->nbc) && t.bx <= 1) t = kDefaultTile3D;
+  if ((g->force_two_pass || g->nbc || has_magnetized(g)) && t.bx <= 1) t = kDefaultTile3D;
 
   const bool wavef = g->ndim == 3 && t.bx == 1;
   const bool fused = g->ndim == 3 && t.bx <= 1;
@@ -876,16 +912,21 @@ int step_impl(fdtd_grid* g, int nsteps) {
             rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         } else if (g->ndim == 3) {
           // instances that also write the boundary ghost layers (BC)
-          // and the E-pass instance without the ADE code when no material is dispersive
-          const bool disp = g->max_np > 0;
-          if (g->nbc && disp)
-            rc = launch(g, 0, fdtdk::e_pass_3d<T, true, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          // and the E-pass instance without the ADE code when no material is
+          // dispersive (with the magnetized Drude code only when one is magnetized)
+          const int disp = g->max_np > 0 ? (has_magnetized(g) ? 2 : 1) : 0;
+          if (g->nbc && disp == 2)
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 2>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          else if (g->nbc && disp)
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 1>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           else if (g->nbc)
-            rc = launch(g, 0, fdtdk::e_pass_3d<T, true, false>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 0>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+          else if (disp == 2)
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, false, 2>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           else if (disp)
-            rc = launch(g, 0, fdtdk::e_pass_3d<T, false, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, false, 1>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           else
-            rc = launch(g, 0, fdtdk::e_pass_3d<T, false, false>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+            rc = launch(g, 0, fdtdk::e_pass_3d<T, false, 0>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           if (rc) return rc;
           if (g->nbc)
             rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
@@ -1434,13 +1475,28 @@ int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npole
   m.eps = eps_r;
   m.mu = mu_r;
   m.np = npoles;
+  m.mag = 0;
   for (int p = 0; p < npoles; p++) {
     m.kind[p] = int(poles[4 * p]);
     m.freq[p] = poles[4 * p + 1];
     m.gamma[p] = poles[4 * p + 2];
     m.weight[p] = poles[4 * p + 3];
   }
+  count_magnetized(g);
   g->mats_dirty = true;
+  return 0;
+}
+
+int fdtd_set_magnetized_drude(fdtd_grid* g, int id, double eps_r, double mu_r, double freq, double gamma,
+                              const double* bias) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  const double pole[4] = {double(FDTD_DRUDE), freq, gamma, 1.0};
+  int rc = fdtd_set_material(g, id, eps_r, mu_r, 1, pole);
+  if (rc) return rc;
+  MatHost& m = g->mats[id];
+  m.mag = bias ? 1 : 2;
+  for (int c = 0; c < 3; c++) m.bias[c] = bias ? bias[c] : 0.0;
+  count_magnetized(g);
   return 0;
 }
 
@@ -1977,6 +2033,7 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
   const size_t n = b->grids.size();
   for (fdtd_grid* g : b->grids) {
     int rc = 0;
+    if (has_magnetized(g)) return fail(FDTD_EINVAL, "batched grids cannot hold magnetized Drude materials (step them alone)");
     if (g->mats_dirty && (rc = upload_materials<T>(g))) return rc;
     if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
     if ((rc = ensure_trace(g, g->step + nsteps))) return rc;
@@ -2059,6 +2116,8 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
     if (grids[q]->ndim != 3) return fail(FDTD_EINVAL, "batches hold 3-D grids");
     if (grids[q]->f64 != grids[0]->f64) return fail(FDTD_EINVAL, "batched grids must share one precision");
     if (grids[q]->nbc) return fail(FDTD_EINVAL, "batched grids cannot have boundary conditions (step them alone)");
+    if (has_magnetized(grids[q]))
+      return fail(FDTD_EINVAL, "batched grids cannot hold magnetized Drude materials (step them alone)");
     for (int r = 0; r < q; r++)
       if (grids[r] == grids[q]) return fail(FDTD_EINVAL, "a grid appears twice in the batch");
   }

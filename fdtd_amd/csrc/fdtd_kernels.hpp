@@ -56,6 +56,13 @@ struct MatDev {
   T cD[kMaxPoles];
   T cG;      // 1/(1 + cD[0]): the FP32 form of a lone conductive pole
   T eps_e0;  // eps_r eps0
+  // magnetized Drude (3-D): 0 none, 1 a fixed bias field, 2 the cell's own B
+  // (MagnetizedDrudeUpdate3D::implicit_update, DispersiveMaterials.hpp:1283-1340)
+  int mag;
+  T mM[9];  // fixed bias: A/denom, row-major (A of the implicit Euler system)
+  T w2dt;   // (dt omega_p)^2 / dt
+  T op;     // 1 + dt gamma + (dt omega_p)^2
+  T vq;     // dt q_e/m_e: v = vq B
 };
 
 struct SrcDev {
@@ -330,12 +337,72 @@ __device__ __forceinline__ long dbox_index(const StepArgs<T>& a, int i, int j, i
          (long)(a.dbox[1] - a.dbox[0]) * ((long)(j - a.dbox[2]) + (long)(a.dbox[3] - a.dbox[2]) * (long)(k - a.dbox[4]));
 }
 
-// dispersive cell: the three components
+// Magnetized Drude cell (3-D): the implicit Euler system couples the three
+// components through v = dt (q_e/m_e) B.  With r = J + (wp^2/dt)(D - P) and
+// M = A/denom (A = op^2 I + v v^T + op [v]x, denom = op (op^2 + |v|^2)):
+// J' = M r, P' = P + dt J' (the reference's P' = P + M (dt r), the same
+// vector), E = (D - P')/(eps_r eps0)
+// (MagnetizedDrudeUpdate3D::implicit_update, DispersiveMaterials.hpp:1283-1340).
+// mag = 2 takes B from the cell's own magnetic field, B = mu_r mu0 H
+// (SelfMagnetizedDrudeStorage, 1898-1943).
 template <typename T>
+__device__ FDTD_COLD void magnetized_cell(const StepArgs<T>& a, const MatDev<T>& m, long bq, const T d[3],
+                                          const T hown[3], bool own, T out[3]) {
+  T P[3], s[3], r[3];  // s = D - P
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T D = a.Dd[c][bq] + d[c];
+    if (own) a.Ddo[c][bq] = D;
+    P[c] = a.Pd[0][c][bq];
+    s[c] = D - P[c];
+    r[c] = a.Jd[0][c][bq] + m.w2dt * s[c];
+  }
+  T v[3] = {T(0), T(0), T(0)}, inv = T(0);
+  const T op = m.op;
+  if (m.mag == 2) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = m.vq * (hown[c] / m.inv_mu);
+    inv = T(1) / (op * (op * op + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2])));
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T M0, M1, M2;  // row c of A/denom
+    if (m.mag == 1) {
+      M0 = m.mM[3 * c];
+      M1 = m.mM[3 * c + 1];
+      M2 = m.mM[3 * c + 2];
+    } else {
+      // A = op^2 I + v v^T + op [v]x with [v]x = ((0,-vz,vy),(vz,0,-vx),(-vy,vx,0))
+      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+      T row[3];
+      row[c] = (op * op + v[c] * v[c]) * inv;
+      row[c1] = (-v[c2] * op + v[c] * v[c1]) * inv;
+      row[c2] = (v[c1] * op + v[c] * v[c2]) * inv;
+      M0 = row[0];
+      M1 = row[1];
+      M2 = row[2];
+    }
+    const T Jn = M0 * r[0] + M1 * r[1] + M2 * r[2];
+    out[c] = (s[c] - a.dt * Jn) * m.inv_eps;
+    if (own) {
+      a.Pdo[0][c][bq] = P[c] + a.dt * Jn;
+      a.Jdo[0][c][bq] = Jn;
+    }
+  }
+}
+
+// dispersive cell: the three components (hown: the cell's own H, read by a
+// self-magnetized Drude material; MAG: the instance that carries the
+// magnetized Drude code)
+template <typename T, bool MAG>
 __device__ FDTD_COLD void ade_cell(const StepArgs<T>& a, int mid, int i, int j, int k, const T d[3], const T eold[3],
-                                   bool own, T out[3]) {
+                                   const T hown[3], bool own, T out[3]) {
   const MatDev<T>& m = a.mats[mid];
   const long bq = dbox_index(a, i, j, k);
+  if (MAG && m.mag) {
+    magnetized_cell(a, m, bq, d, hown, own, out);
+    return;
+  }
   out[0] = ade_component<T, 0>(a, m, bq, d[0], eold[0], own);
   out[1] = ade_component<T, 1>(a, m, bq, d[1], eold[1], own);
   out[2] = ade_component<T, 2>(a, m, bq, d[2], eold[2], own);
@@ -355,9 +422,10 @@ struct EStencil {
 // ---------------------------------------------------------------- E of one cell
 // New E of interior cell (i,j,k).  psi and the dispersive state are read from
 // the current buffers and, when `own`, written to the next buffers.
-// DISP = false: the grid has no dispersive material (the ADE code is left
-// out of that instance, which keeps its register count down).
-template <typename T, bool DISP = true>
+// DISP = 0: the grid has no dispersive material (the ADE code is left out of
+// that instance, which keeps its register count down); 1: pole materials;
+// 2: also magnetized Drude materials.
+template <typename T, int DISP = 1>
 __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incH, int i, int j, int k,
                                           long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy,
                                           T fz, bool own, T out[3]) {
@@ -418,7 +486,8 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
     out[2] = ez + d[2] * ie;
   } else if (DISP && ie < T(0) && a.Dd[0]) {
     const T eold[3] = {ex, ey, ez};
-    ade_cell(a, mid, i, j, k, d, eold, own, out);
+    const T hown[3] = {h.hx, h.hy, h.hz};
+    ade_cell<T, DISP == 2>(a, mid, i, j, k, d, eold, hown, own, out);
   } else {
     out[0] = ex;
     out[1] = ey;
@@ -723,9 +792,10 @@ __device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a, int ft, int ndim,
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
 // BC: the instance for grids with boundary conditions (ghost writes); DISP:
-// for grids with dispersive materials (ADE).  An instance without them
-// carries none of that code, so its registers stay low.
-template <typename T, bool BC = false, bool DISP = true>
+// for grids with dispersive materials (1: ADE poles, 2: also magnetized
+// Drude).  An instance without them carries none of that code, so its
+// registers stay low.
+template <typename T, bool BC = false, int DISP = 1>
 __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie, int i, int j, int k0, int k1, int s) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   const FDTD_G T* __restrict__ Hx = a.H[0];
@@ -809,7 +879,7 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int 
 
 // One-dimensional launch of ntx x nty x nkc tiles of blockDim.x x blockDim.y
 // columns and kc planes, XCD-aware (tiles x fastest, then y, then z chunks).
-template <typename T, bool BC = false, bool DISP = true>
+template <typename T, bool BC = false, int DISP = 1>
 __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                  int nkc) {
   const StepArgs<T>& a = *ap;

@@ -13,6 +13,8 @@
 //   DrudeUpdate<Mode,ft>(K,freq,gamma,dt)           same                             DispersiveMaterials.hpp:1208-1211
 //   LorentzUpdate<Mode,ft>(K,delta,freq,gamma,dt)   same                             DispersiveMaterials.hpp:881-884
 //   ConductiveUpdate<Mode,ft>(K,freq,dt)            same                             DispersiveMaterials.hpp:674-677
+//   MagnetizedDrudeUpdate<Mode,ft>(Bx,By,Bz,K,Freq,Gamma,dt)  same                   DispersiveMaterials.hpp:2008-2011
+//   SelfMagnetizedDrudeUpdate<Mode,ft>(dt) + K()/Freq()/Gamma()  same                DispersiveMaterials.hpp:2024-2027
 //   make_soft_source<Dz>(signal, cell)              make_soft_source<Dz>(signal, grid.cell(i,j,k))  Source.hpp:258-261
 //   Aux1D<double>(ncells,dx,dt,signal)              Aux1D<double>(ncells,dx,dt,signal)               Source.hpp:280-408
 //   make_tfsf_plane<Mode,Dir,Orient>(b,e,loc,k,aux) make_tfsf_box<Mode>(grid,box,k,aux)              Source.hpp:624-634
@@ -235,6 +237,8 @@ struct UpdatePML {
 struct MaterialSpec {
   double eps = 1.0, mu = 1.0;
   std::vector<double> poles;  // kind, freq, gamma, weight per pole
+  int mag = 0;                // magnetized Drude: 1 fixed bias, 2 the cell's own B
+  double bias[3] = {0, 0, 0};
 };
 template <class Mode, FieldType ft, bool forward = false>
 struct VacuumUpdate {
@@ -274,10 +278,49 @@ struct ConductiveUpdate {
   ConductiveUpdate(double k, double f, double deltat) : K(k), freq(f), dt(deltat) {}
   MaterialSpec spec() const { return {K, 1.0, {double(FDTD_CONDUCTIVE), freq, 0.0, 1.0}}; }
 };
+// Magnetized Drude plasma with MagnetizedDrudeStorage(Bx, By, Bz, K, Freq,
+// Gamma, dt): in 3-D the three components of P and J advance together in the
+// fixed bias field B [T] (include/DispersiveMaterials.hpp:1263-1340); in TM
+// the reference's update is the Drude one per component, and so is this one
+template <class Mode, FieldType ft, bool forward = false>
+struct MagnetizedDrudeUpdate {
+  double Bx, By, Bz, K, freq, gamma, dt;
+  MagnetizedDrudeUpdate(double bx, double by, double bz, double k, double f, double g, double deltat)
+      : Bx(bx), By(by), Bz(bz), K(k), freq(f), gamma(g), dt(deltat) {}
+  MaterialSpec spec() const {
+    MaterialSpec s{K, 1.0, {double(FDTD_DRUDE), freq, gamma, 1.0}};
+    s.mag = 1;
+    s.bias[0] = Bx;
+    s.bias[1] = By;
+    s.bias[2] = Bz;
+    return s;
+  }
+};
+// The same plasma magnetized by the wave's own field B = mu_r mu0 H at the
+// cell; SelfMagnetizedDrudeStorage(dt) with its K, Freq, Gamma set through
+// the accessors, as in the reference (1898-1943)
+template <class Mode, FieldType ft, bool forward = false>
+struct SelfMagnetizedDrudeUpdate {
+  double k_ = 1.0, freq_ = 0.0, gamma_ = 0.0, dt;
+  explicit SelfMagnetizedDrudeUpdate(double deltat) : dt(deltat) {}
+  double& K() { return k_; }
+  double& Freq() { return freq_; }
+  double& Gamma() { return gamma_; }
+  MaterialSpec spec() const {
+    MaterialSpec s{k_, 1.0, {double(FDTD_DRUDE), freq_, gamma_, 1.0}};
+    s.mag = 2;
+    return s;
+  }
+};
 
 // MaterialMap equivalent (include/MaterialMap.hpp:204-284): id -> material.
 template <class Mode, class scalar>
 void set_material(YeeGrid<Mode, scalar>& g, int id, const MaterialSpec& s) {
+  if (s.mag) {
+    check(fdtd_set_magnetized_drude(g.handle(), id, s.eps, s.mu, s.poles[1], s.poles[2], s.mag == 1 ? s.bias : nullptr),
+          "fdtd_set_magnetized_drude");
+    return;
+  }
   check(fdtd_set_material(g.handle(), id, s.eps, s.mu, int(s.poles.size() / 4), s.poles.empty() ? nullptr : s.poles.data()),
         "fdtd_set_material");
 }

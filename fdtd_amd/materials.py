@@ -12,9 +12,12 @@ writes (tabs, order by identifier, C++ default float format).
 
 A map is put on a grid (``fdtd_amd.Grid``) or the oracle by ``apply(sim, ids)``
 with ids = {identifier: material id}: electric Vacuum / Constant / Conductive
-/ Drude / Lorentz become the material's eps_r and pole, magnetic Vacuum /
-Constant its mu_r.  The reference's other models (FluidDrude, the magnetized
-Drude family, magnetic dispersion) have no device counterpart and raise.
+/ Drude / Lorentz become the material's eps_r and pole, MagnetizedDrude
+(MagnetizedDrudeStorage 1619-1726, its <B>(x, y, z)</B> tag included) and
+SelfMagnetizedDrude (1898-2006) a magnetized Drude material, magnetic Vacuum /
+Constant its mu_r.  The reference's other models (the fluid Drude family,
+whose density and collision rate are fields of a user cell type, and magnetic
+dispersion) have no device counterpart and raise.
 """
 import xml.etree.ElementTree as ET
 from dataclasses import dataclass
@@ -27,19 +30,23 @@ DISPERSIONS = ("Vacuum", "Constant", "Conductive", "Drude", "FluidDrude", "Loren
                "Anisotropic")
 # the tags each model writes, in print_summary order
 _FIELDS = {"Vacuum": (), "Constant": ("Constant",), "Conductive": ("Constant", "Freq"),
-           "Drude": ("Constant", "Freq", "Gamma"), "Lorentz": ("Constant", "Delta", "Freq", "Gamma")}
-_ATTR = {"Constant": "K", "Freq": "freq", "Gamma": "gamma", "Delta": "delta"}
+           "Drude": ("Constant", "Freq", "Gamma"), "Lorentz": ("Constant", "Delta", "Freq", "Gamma"),
+           "MagnetizedDrude": ("B", "Constant", "Freq", "Gamma"),
+           "SelfMagnetizedDrude": ("Constant", "Freq", "Gamma")}
+_ATTR = {"Constant": "K", "Freq": "freq", "Gamma": "gamma", "Delta": "delta", "B": "B"}
 
 
 @dataclass
 class Dispersion:
     """One material model: kind (a DISPERSIONS name) and its coefficients (K the
-    relative constant, freq, gamma, delta as the reference names them)."""
+    relative constant, freq, gamma, delta as the reference names them; B the
+    bias field [T] of a MagnetizedDrude model)."""
     kind: str
     K: float = 1.0
     freq: float = 0.0
     gamma: float = 0.0
     delta: float = 0.0
+    B: tuple = (0.0, 0.0, 0.0)
 
     def __post_init__(self):
         if self.kind not in DISPERSIONS:
@@ -52,8 +59,11 @@ class MaterialPair:
     magnetic: Dispersion
 
     def spec(self):
-        """(eps_r, mu_r, poles) for Grid.set_material / the oracle's set_material."""
+        """(eps_r, mu_r, poles) for Grid.set_material / the oracle's set_material;
+        for a magnetized Drude model (eps_r, mu_r, poles, bias) with bias None for
+        the self-magnetized one (Grid.set_magnetized_drude)."""
         e, m = self.electric, self.magnetic
+        bias = ()
         if e.kind == "Vacuum":
             eps, poles = 1.0, []
         elif e.kind == "Constant":
@@ -64,6 +74,9 @@ class MaterialPair:
             eps, poles = e.K, [(DRUDE, e.freq, e.gamma, 1.0)]
         elif e.kind == "Lorentz":  # the reference's update never reads Delta
             eps, poles = e.K, [(LORENTZ, e.freq, e.gamma, 1.0)]
+        elif e.kind in ("MagnetizedDrude", "SelfMagnetizedDrude"):
+            eps, poles = e.K, [(DRUDE, e.freq, e.gamma, 1.0)]
+            bias = (tuple(float(b) for b in e.B) if e.kind == "MagnetizedDrude" else None,)
         else:
             raise NotImplementedError(f"electric model {e.kind} has no device counterpart")
         if m.kind == "Vacuum":
@@ -72,7 +85,7 @@ class MaterialPair:
             mu = m.K
         else:
             raise NotImplementedError(f"magnetic model {m.kind}: magnetic dispersion has no device counterpart")
-        return eps, mu, poles
+        return (eps, mu, poles) + bias
 
 
 def _fmt(v):
@@ -104,7 +117,9 @@ class MaterialMap(dict):
                 model = node[0]
                 d = Dispersion(model.tag)
                 for child in model:
-                    if child.tag in _ATTR:
+                    if child.tag == "B":  # "(bx, by, bz)", MagnetizedDrudeStorage::readXML
+                        d.B = tuple(float(v) for v in child.text.strip().strip("()").split(","))
+                    elif child.tag in _ATTR:
                         setattr(d, _ATTR[child.tag], float(child.text))
                 models[side] = d
             out[ident] = MaterialPair(models["Electric"], models["Magnetic"])
@@ -127,7 +142,11 @@ class MaterialMap(dict):
                         raise NotImplementedError(f"writing model {d.kind}")
                     lines.append(t(ntabs + 4) + f"<{d.kind}>")
                     for tag in _FIELDS[d.kind]:
-                        lines.append(t(ntabs + 5) + f"<{tag}>{_fmt(getattr(d, _ATTR[tag]))}</{tag}>")
+                        if tag == "B":
+                            v = "(" + ", ".join(_fmt(b) for b in d.B) + ")"
+                        else:
+                            v = _fmt(getattr(d, _ATTR[tag]))
+                        lines.append(t(ntabs + 5) + f"<{tag}>{v}</{tag}>")
                     lines.append(t(ntabs + 4) + f"</{d.kind}>")
                 lines.append(t(ntabs + 3) + f"</{side}>")
             lines += [t(ntabs + 2) + "</MaterialPair>", t(ntabs + 1) + "</Item>"]
@@ -137,6 +156,10 @@ class MaterialMap(dict):
     def apply(self, sim, ids):
         """Set material id ids[name] of `sim` (Grid or oracle Sim) from each named pair."""
         for name, mid in ids.items():
-            eps, mu, poles = self[name].spec()
-            sim.set_material(mid, eps, mu, poles)
+            s = self[name].spec()
+            if len(s) == 4:
+                eps, mu, poles, bias = s
+                sim.set_magnetized_drude(mid, eps, mu, poles[0][1], poles[0][2], bias)
+            else:
+                sim.set_material(mid, *s)
         return sim

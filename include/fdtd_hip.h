@@ -95,6 +95,17 @@ int fdtd_set_pml_idiom(fdtd_grid* g, int axis, int n, double m, double ma, doubl
  * ConductiveUpdate (508-680): P = (P + f D)/(1 + f), f = dt sigma/(eps_r eps0),
  * E = (D - P)/(eps_r eps0). */
 int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npoles, const double* poles);
+/* Material `id` as a magnetized Drude plasma: replaces MagnetizedDrudeUpdate
+ * (bias != NULL: the fixed field bias[3] in tesla, MagnetizedDrudeStorage,
+ * include/DispersiveMaterials.hpp:1619-1726) and SelfMagnetizedDrudeUpdate
+ * (bias == NULL: the cell's own magnetic flux density B = mu_r mu0 H,
+ * SelfMagnetizedDrudeStorage, 1898-2006).  In 3-D the three components of
+ * P and J advance together by MagnetizedDrudeUpdate3D::implicit_update
+ * (1283-1340); in TM the reference falls back to DrudeUpdate per component
+ * (MagnetizedDrudeCall, 1508-1537), and so does this material.  K = eps_r,
+ * Freq = omega_p [rad/s], Gamma [1/s]. */
+int fdtd_set_magnetized_drude(fdtd_grid* g, int id, double eps_r, double mu_r, double freq, double gamma,
+                              const double* bias);
 /* per-cell material id, nx*ny*nz bytes */
 int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids);
 

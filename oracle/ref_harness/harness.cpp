@@ -130,10 +130,12 @@ typedef Nb<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<dou
                    NewPML<ThreeD, double>>>
     Cell3;
 
-// material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz, 4 conductive (freq = the conductivity)
+// material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz, 4 conductive (freq = the conductivity),
+// 5 magnetized Drude with the bias (bx, by, bz) [T], 6 self-magnetized Drude (the cell's own B)
 struct Mat {
   int kind;
   double eps, freq, gamma, delta;
+  double bx = 0, by = 0, bz = 0;
 };
 
 template <class Mode, class C>
@@ -144,6 +146,15 @@ void material_E(C& c, const Mat& m, double dt) {
     case 2: DrudeUpdate<Mode, FieldType::Electric>(m.eps, m.freq, m.gamma, dt)(c); break;
     case 3: LorentzUpdate<Mode, FieldType::Electric>(m.eps, m.delta, m.freq, m.gamma, dt)(c); break;
     case 4: ConductiveUpdate<Mode, FieldType::Electric>(m.eps, m.freq, dt)(c); break;
+    case 5: MagnetizedDrudeUpdate<Mode, FieldType::Electric>(m.bx, m.by, m.bz, m.eps, m.freq, m.gamma, dt)(c); break;
+    case 6: {
+      SelfMagnetizedDrudeUpdate<Mode, FieldType::Electric> u(dt);
+      u.K() = m.eps;
+      u.Freq() = m.freq;
+      u.Gamma() = m.gamma;
+      u(c);
+      break;
+    }
   }
 }
 
@@ -634,6 +645,13 @@ static void run_3d(const Scenario3& S, const std::string& dir, bool timing_only 
     mt.push_back(m.delta);
   }
   fx.arr("mats_kind_eps_freq_gamma_delta", mt, {S.mats.size(), 5});
+  bool any_bias = false;
+  std::vector<double> bias;
+  for (auto& m : S.mats) {
+    any_bias = any_bias || m.kind >= 5;
+    bias.insert(bias.end(), {m.bx, m.by, m.bz});
+  }
+  if (any_bias) fx.arr("mats_bias", bias, {S.mats.size(), 3});
   std::vector<double> mid(S.matid.begin(), S.matid.end());
   std::vector<size_t> sh = {size_t(nz), size_t(ny), size_t(nx)};
   fx.arr("matid", mid, sh);
@@ -704,6 +722,12 @@ static void scen_material_map(const std::string& dir) {
   mm.insert(std::make_pair(std::string("conductor"), make_material_pair<CellTM&>(ConductiveUpdate<TM, FieldType::Electric>(2.0, 5e5, dt), VacuumUpdate<TM, FieldType::Magnetic>())));
   mm.insert(std::make_pair(std::string("silver"), make_material_pair<CellTM&>(DrudeUpdate<TM, FieldType::Electric>(3.7, 1.37e16, 2.73e13, dt), VacuumUpdate<TM, FieldType::Magnetic>())));
   mm.insert(std::make_pair(std::string("resonator"), make_material_pair<CellTM&>(LorentzUpdate<TM, FieldType::Electric>(1.5, 0.8, 4.1e15, 1.2e14, dt), ConstantUpdate<TM, FieldType::Magnetic>(1.0))));
+  mm.insert(std::make_pair(std::string("biased plasma"), make_material_pair<CellTM&>(MagnetizedDrudeUpdate<TM, FieldType::Electric>(0.3, -0.4, 1.25, 1.0, 2e11, 1e9, dt), VacuumUpdate<TM, FieldType::Magnetic>())));
+  SelfMagnetizedDrudeUpdate<TM, FieldType::Electric> self(dt);
+  self.K() = 1.5;
+  self.Freq() = 1.5e11;
+  self.Gamma() = 2e9;
+  mm.insert(std::make_pair(std::string("self plasma"), make_material_pair<CellTM&>(self, VacuumUpdate<TM, FieldType::Magnetic>())));
   std::ostringstream os;
   std::streambuf* old = std::cout.rdbuf(os.rdbuf());
   mm.print_summary(os, 0);
@@ -1195,6 +1219,73 @@ int main(int argc, char** argv) {
                {1, 7.5, 0, 0, 0}});
   // conductive (include/DispersiveMaterials.hpp:508-680): factor = dt sigma/(eps_r eps0) from ~0.01 to ~2
   cell_traces("cell_conductive", 1e-17, {{4, 1.0, 1e4, 0, 0}, {4, 2.5, 2e5, 0, 0}, {4, 7.0, 1.2e7, 0, 0}});
+  // magnetized Drude (include/DispersiveMaterials.hpp:1263-2028), microwave
+  // plasma scales (dt 1 ps: wp = dt omega_p ~ 0.2, v = dt q B/m ~ 0.2 per tesla).
+  // 3-D: one cell with prescribed D_n (and, self-magnetized, B_n) per component.
+  {
+    Fixture fx;
+    fx.name = "cell_magnetized";
+    const double dt = 1e-12;
+    const int nsteps = 200;
+    std::vector<Mat> ms = {{5, 1.0, 2e11, 1e9, 0, 0.0, 0.0, 1.0},
+                           {5, 2.0, 1.5e11, 3e10, 0, 0.6, -1.0, 1.6},
+                           {5, 1.0, 3e11, 0, 0, 2.0, 0.5, 0.0},
+                           {6, 1.0, 2e11, 1e9, 0},
+                           {6, 1.5, 1e11, 5e9, 0}};
+    std::vector<double> out, mt;
+    for (auto& m : ms) {
+      Cell3 c;
+      for (int n = 0; n < nsteps; n++) {
+        c.Dx() = eps0 * (std::sin(0.07 * n) + 0.3 * std::cos(0.011 * n * n));
+        c.Dy() = eps0 * (0.5 * std::cos(0.05 * n + 0.3) - 0.2 * std::sin(0.013 * n * n));
+        c.Dz() = eps0 * (0.8 * std::sin(0.031 * n + 1.1));
+        c.Bx() = 0.7 * std::sin(0.02 * n);
+        c.By() = -1.2 + 0.4 * std::cos(0.045 * n);
+        c.Bz() = 1.5 * std::cos(0.017 * n + 0.4);
+        material_E<ThreeD>(c, m, dt);
+        out.insert(out.end(), {c.Dx(), c.Dy(), c.Dz(), c.Ex(), c.Ey(), c.Ez(), c.Px(), c.Py(), c.Pz(), c.Jx(),
+                               c.Jy(), c.Jz(), c.Bx(), c.By(), c.Bz()});
+      }
+      mt.insert(mt.end(), {double(m.kind), m.eps, m.freq, m.gamma, m.bx, m.by, m.bz});
+    }
+    fx.p("dt", dt);
+    fx.pi("nsteps", nsteps);
+    fx.arr("mats_kind_eps_freq_gamma_bx_by_bz", mt, {ms.size(), 7});
+    fx.arr("trace_D3_E3_P3_J3_B3", out, {ms.size(), size_t(nsteps), 15});
+    fx.write(dir);
+  }
+  // TM: the reference runs DrudeUpdate per component for these (MagnetizedDrudeCall, 1508-1537)
+  cell_traces("cell_magnetized_tm", 1e-12, {{5, 1.0, 2e11, 1e9, 0, 0.0, 0.0, 1.0}, {6, 2.0, 1.5e11, 3e10, 0}});
+  // 3-D grid with a biased and a self-magnetized plasma block, CPML, a strong Dz
+  // source (the self-magnetized block sees fields of ~1e8 V/m, so |v| ~ 0.1)
+  {
+    Scenario3 S;
+    S.name = "grid3d_magnetized";
+    S.nx = 20;
+    S.ny = 18;
+    S.nz = 17;
+    S.nsteps = 70;
+    S.dx = 1e-3;
+    S.dt = 0.5 * S.dx / (c0 * std::sqrt(3.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {4, 3, 1, dm.msMax, 5, 0.05};
+    S.mats = {{0, 1, 0, 0, 0}, {5, 1.0, 2e11, 1e9, 0, 0.3, -0.4, 1.2}, {6, 1.5, 1.5e11, 2e9, 0}, {1, 2.5, 0, 0, 0}};
+    S.matid.assign(size_t(S.nx) * S.ny * S.nz, 0);
+    auto fill = [&](int x0, int x1, int y0, int y1, int z0, int z1, int m) {
+      for (int k = z0; k < z1; k++)
+        for (int j = y0; j < y1; j++)
+          for (int i = x0; i < x1; i++) S.matid[size_t(i) + size_t(S.nx) * (size_t(j) + size_t(S.ny) * k)] = m;
+    };
+    fill(6, 10, 6, 12, 6, 10, 1);
+    fill(11, 14, 6, 10, 8, 11, 2);
+    fill(6, 9, 12, 14, 9, 12, 3);
+    S.si = 10;
+    S.sj = 9;
+    S.sk = 8;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * 3e9 * gauss_pulse(n, 14, 5));
+    S.probes = {{10, 9, 8}, {8, 8, 8}, {12, 8, 9}, {7, 13, 10}};
+    run_3d(S, dir);
+  }
   // 2D TM with conductive blocks next to a dielectric one, CPML x+y, Dz source
   {
     TMScenario S;

@@ -218,6 +218,8 @@ typedef struct {
   int np;
   int kind[RC_MAXPOLES];
   double freq[RC_MAXPOLES], gamma[RC_MAXPOLES], weight[RC_MAXPOLES];
+  int mag;        /* magnetized Drude: 0 none, 1 fixed bias, 2 the cell's own B */
+  double bias[3]; /* [T] */
 } rc_mat;
 
 typedef struct {
@@ -405,6 +407,7 @@ int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, co
   m->eps = eps_r;
   m->mu = mu_r;
   m->np = npoles;
+  m->mag = 0;
   for (int p = 0; p < npoles; p++) {
     m->kind[p] = (int)poles[4 * p];
     m->freq[p] = poles[4 * p + 1];
@@ -421,6 +424,60 @@ int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, co
     s->npoles_alloc = npoles;
   }
   return 0;
+}
+
+int rc_set_magnetized_drude(rc_sim* s, int id, double eps_r, double mu_r, double freq, double gamma,
+                            const double* bias) {
+  const double pole[4] = {1.0, freq, gamma, 1.0};
+  if (rc_set_material(s, id, eps_r, mu_r, 1, pole)) return -1;
+  s->mat[id].mag = bias ? 1 : 2;
+  for (int c = 0; c < 3; c++) s->mat[id].bias[c] = bias ? bias[c] : 0.0;
+  return 0;
+}
+
+/* MagnetizedDrudeUpdate3D::implicit_update (DispersiveMaterials.hpp:1283-1340),
+ * in its own operation order; B is the bias (MagnetizedDrudeStorage) or the
+ * cell's own B (SelfMagnetizedDrudeStorage, 1898-1943). */
+static void magnetized_drude_3d(rc_sim* s, const rc_mat* m, size_t c, double dt) {
+  const double qe = 1.6e-19, me = 9.11e-31; /* drude::q_e, drude::m_e (899-900) */
+  const double Bx = m->mag == 1 ? m->bias[0] : s->B[0][c];
+  const double By = m->mag == 1 ? m->bias[1] : s->B[1][c];
+  const double Bz = m->mag == 1 ? m->bias[2] : s->B[2][c];
+  double *Dx = &s->D[0][c], *Dy = &s->D[1][c], *Dz = &s->D[2][c];
+  double *Px = &s->P[0][0][c], *Py = &s->P[0][1][c], *Pz = &s->P[0][2][c];
+  double *Jx = &s->J[0][0][c], *Jy = &s->J[0][1][c], *Jz = &s->J[0][2][c];
+  const double wp = dt * m->freq[0];
+  const double vc = dt * m->gamma[0];
+  const double vx = dt * Bx * qe / me;
+  const double vy = dt * By * qe / me;
+  const double vz = dt * Bz * qe / me;
+  const double oneplus = (1.0 + vc + wp * wp);
+  const double denom = oneplus * (oneplus * oneplus + (vx * vx + vy * vy + vz * vz));
+  const double a11 = oneplus * oneplus + vx * vx, a21 = vz * oneplus + vx * vy, a31 = -vy * oneplus + vx * vz;
+  const double a12 = -vz * oneplus + vx * vy, a22 = oneplus * oneplus + vy * vy, a32 = vx * oneplus + vy * vz;
+  const double a13 = vy * oneplus + vx * vz, a23 = -vx * oneplus + vy * vz, a33 = oneplus * oneplus + vz * vz;
+  /* J + (wp^2/dt)(D - P), written as the reference groups it */
+  const double rx = *Jx - wp * wp / dt * *Px + wp * wp / dt * *Dx;
+  const double ry = *Jy - wp * wp / dt * *Py + wp * wp / dt * *Dy;
+  const double rz = *Jz - wp * wp / dt * *Pz + wp * wp / dt * *Dz;
+  const double sx = dt * *Jx + wp * wp * *Dx - wp * wp * *Px;
+  const double sy = dt * *Jy + wp * wp * *Dy - wp * wp * *Py;
+  const double sz = dt * *Jz + wp * wp * *Dz - wp * wp * *Pz;
+  const double Jxh = 1.0 / denom * (a11 * rx + a12 * ry + a13 * rz);
+  const double Jyh = 1.0 / denom * (a21 * rx + a22 * ry + a23 * rz);
+  const double Jzh = 1.0 / denom * (a31 * rx + a32 * ry + a33 * rz);
+  const double Pxh = *Px + 1.0 / denom * (a11 * sx + a12 * sy + a13 * sz);
+  const double Pyh = *Py + 1.0 / denom * (a21 * sx + a22 * sy + a23 * sz);
+  const double Pzh = *Pz + 1.0 / denom * (a31 * sx + a32 * sy + a33 * sz);
+  *Jx = Jxh;
+  *Jy = Jyh;
+  *Jz = Jzh;
+  *Px = Pxh;
+  *Py = Pyh;
+  *Pz = Pzh;
+  s->E[0][c] = (*Dx - *Px) / (RC_EPS0 * m->eps);
+  s->E[1][c] = (*Dy - *Py) / (RC_EPS0 * m->eps);
+  s->E[2][c] = (*Dz - *Pz) / (RC_EPS0 * m->eps);
 }
 
 int rc_set_matid(rc_sim* s, const uint8_t* ids) {
@@ -763,6 +820,10 @@ int rc_step(rc_sim* s, int nsteps) {
             size_t c = cidx(s, i, j, k);
             const rc_mat* m = &s->mat[s->matid[c]];
             const long dl = design_local(s, i, j, k);
+            if (m->mag && m->np && np && dl < 0 && s->mode == RC_3D) {
+              magnetized_drude_3d(s, m, c, dt);
+              continue;
+            }
             for (int I = 2; I >= 0; I--) {
               if (!s->actE[I]) continue;
               if (dl >= 0) { /* design cell: non-dispersive, its own eps_r */

@@ -65,6 +65,10 @@ void rc_destroy(rc_sim* s);
 int rc_set_pml_axis(rc_sim* s, int axis, int ft, const double* coeffs);
 /* material table: eps_r, mu_r, npoles, then per pole {kind(1 Drude,2 Lorentz), freq, gamma, weight} */
 int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, const double* poles);
+/* magnetized Drude material (bias NULL: self-magnetized by the cell's own B);
+ * in 3-D the coupled update, otherwise the Drude pole {1, freq, gamma, 1} */
+int rc_set_magnetized_drude(rc_sim* s, int id, double eps_r, double mu_r, double freq, double gamma,
+                            const double* bias);
 int rc_set_matid(rc_sim* s, const uint8_t* ids); /* nx*ny*nz, x fastest */
 /* soft source: adds table[n] to D_comp (field 0) or B_comp (field 1) at cell after the curl of step n */
 int rc_add_source(rc_sim* s, int field, int comp, int i, int j, int k, const double* table, int ntable);

@@ -42,6 +42,7 @@ def lib():
             ("rc_create", _p, [_i, _i, _i, _i, _d, _d]), ("rc_destroy", None, [_p]),
             ("rc_set_pml_axis", _i, [_p, _i, _i, _dp]),
             ("rc_set_material", _i, [_p, _i, _d, _d, _i, _dp]),
+            ("rc_set_magnetized_drude", _i, [_p, _i, _d, _d, _d, _d, _dp]),
             ("rc_set_matid", _i, [_p, ctypes.c_char_p]),
             ("rc_add_source", _i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
             ("rc_set_tfsf", _i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
@@ -158,6 +159,11 @@ class Sim:
     def set_material(self, mid, eps_r, mu_r=1.0, poles=()):
         arr = _f64(np.array(poles, dtype=np.float64).reshape(-1)) if len(poles) else np.zeros(1)
         assert lib().rc_set_material(self.h, mid, eps_r, mu_r, len(poles), _dptr(arr)) == 0
+
+    def set_magnetized_drude(self, mid, eps_r, mu_r, freq, gamma, bias=None):
+        """bias: the fixed field [T] (3 values), or None for the cell's own B."""
+        b = None if bias is None else _f64(np.asarray(bias, dtype=np.float64).reshape(3))
+        assert lib().rc_set_magnetized_drude(self.h, mid, eps_r, mu_r, freq, gamma, None if b is None else _dptr(b)) == 0
 
     def set_matid(self, ids):
         ids = np.ascontiguousarray(ids, dtype=np.uint8).reshape(-1)

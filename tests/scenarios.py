@@ -40,12 +40,16 @@ class SplitMix64:
         return lo + int(self.next() % (hi - lo + 1))
 
 
-def ref_material(kind, eps, freq, gamma, delta):
+def ref_material(kind, eps, freq, gamma, delta, bias=None):
     """Map a reference material (kind 0 vacuum, 1 constant, 2 Drude, 3 Lorentz,
-    4 conductive with freq = the conductivity) to (eps_r, mu_r, poles).  The
+    4 conductive with freq = the conductivity, 5 magnetized Drude with `bias`,
+    6 self-magnetized Drude) to (eps_r, mu_r, poles), or for kinds 5 and 6 to
+    (eps_r, mu_r, poles, bias) with bias None for the cell's own B.  The
     reference's Lorentz update never reads its delta
     (include/DispersiveMaterials.hpp:711-735), so its pole weight is 1."""
     kind = int(kind)
+    if kind in (5, 6):
+        return (eps, 1.0, [(1, freq, gamma, 1.0)], list(bias) if kind == 5 else None)
     if kind == 4:
         return (eps, 1.0, [(3, freq, 0.0, 1.0)])
     if kind == 0:
@@ -55,6 +59,15 @@ def ref_material(kind, eps, freq, gamma, delta):
     if kind == 2:
         return (eps, 1.0, [(1, freq, gamma, 1.0)])
     return (eps, 1.0, [(2, freq, gamma, 1.0)])
+
+
+def set_ref_material(sim, mid, spec):
+    """Apply a ref_material spec to a Grid or an oracle Sim."""
+    if len(spec) == 4:
+        eps, mu, poles, bias = spec
+        sim.set_magnetized_drude(mid, eps, mu, poles[0][1], poles[0][2], bias)
+    else:
+        sim.set_material(mid, *spec)
 
 
 def from_golden(name):
@@ -68,7 +81,8 @@ def from_golden(name):
         "pml": dict(n=int(p["pml_n"]), m=p["pml_m"], ma=p["pml_ma"], smax=p["pml_smax"],
                     kmax=p["pml_kmax"], amax=p["pml_amax"]),
         "pml_axes": [bool(p.get("pmlx", 1)), bool(p.get("pmly", 1)), mode == "ThreeD"],
-        "materials": [ref_material(*row) for row in a["mats_kind_eps_freq_gamma_delta"]],
+        "materials": [ref_material(*row, bias=a["mats_bias"][m] if "mats_bias" in a else None)
+                      for m, row in enumerate(a["mats_kind_eps_freq_gamma_delta"])],
         "sources": [], "probes": [], "dfts": [], "tfsf": None,
     }
     shape = (nz if mode == "ThreeD" else 1, ny, nx)
@@ -106,8 +120,8 @@ def apply(sim, sc):
     for ax, on in enumerate(sc["pml_axes"]):
         if on:
             sim.set_pml_idiom(ax, pm["n"], pm["m"], pm["ma"], pm["smax"], pm["kmax"], pm["amax"])
-    for mid, (eps, mu, poles) in enumerate(sc["materials"]):
-        sim.set_material(mid, eps, mu, poles)
+    for mid, spec in enumerate(sc["materials"]):
+        set_ref_material(sim, mid, spec)
     sim.set_matid(sc["matid"])
     for s in sc["sources"]:
         sim.add_source(*s)

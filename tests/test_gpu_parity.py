@@ -50,7 +50,7 @@ def field_errors(g, o, sc):
 
 
 GOLDEN = ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small", "grid3d_disp", "tm2d_c1",
-          "tm2d_conductive"]
+          "tm2d_conductive", "grid3d_magnetized"]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -314,3 +314,32 @@ def test_3d_conductive_block(precision):
         assert v <= TOL[precision], (k, v)
     for c in range(3):
         assert rel_l2(g.get(P, c), o.get(P, c)) <= TOL[precision] * 10
+
+
+def test_3d_magnetized_schedules_and_errors():
+    """Magnetized Drude grids run the two-pass schedule with the E-pass instance
+    that carries the coupled update: a fused tile request and every two-pass
+    tile give the same bits, captured chunks too; batches refuse such grids."""
+    from fdtd_amd import Batch, FdtdError, Grid
+    sc, p, a = from_golden("grid3d_magnetized")
+
+    def run(tile, graph):
+        print(f"[gpu-case] magnetized tile {tile} graph {graph}", flush=True)
+        g = apply(Grid(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"], precision="f32"), sc)
+        if tile:
+            g.set_tile(*tile)
+        g.set_graph(graph)
+        g.step(31)
+        g.step(sc["nsteps"] - 31)
+        return g
+
+    ref = run(None, False)
+    for tile, graph in (((0, 2, 16), False), ((1, 4, 2), False), ((64, 4, 2), False), ((256, 1, 4), True)):
+        g = run(tile, graph)
+        for f in (E, H, P, J):
+            for c in range(3):
+                np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=f"{tile} {graph} {f} {c}")
+        g.close()
+    other = apply(Grid(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"], precision="f32"), sc)
+    with pytest.raises(FdtdError, match="magnetized"):
+        Batch([ref, other])

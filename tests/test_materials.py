@@ -20,7 +20,8 @@ XML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "materi
 
 def test_reads_the_reference_print_summary():
     mm = MaterialMap.read_xml(XML)
-    assert sorted(mm) == ["conductor", "ferrite", "glass", "resonator", "silver", "vacuum"]
+    assert sorted(mm) == ["biased plasma", "conductor", "ferrite", "glass", "resonator", "self plasma", "silver",
+                          "vacuum"]
     assert mm["vacuum"].spec() == (1.0, 1.0, [])
     assert mm["glass"].spec() == (2.25, 1.0, [])
     assert mm["ferrite"].spec() == (4.5, 3.2, [])
@@ -29,6 +30,8 @@ def test_reads_the_reference_print_summary():
     # Lorentz: Delta is read and kept, but (as in the reference's update) not used
     assert mm["resonator"].electric.delta == 0.8
     assert mm["resonator"].spec() == (1.5, 1.0, [(2, 4.1e15, 1.2e14, 1.0)])
+    assert mm["biased plasma"].spec() == (1.0, 1.0, [(1, 2e11, 1e9, 1.0)], (0.3, -0.4, 1.25))
+    assert mm["self plasma"].spec() == (1.5, 1.0, [(1, 1.5e11, 2e9, 1.0)], None)
 
 
 def test_writes_the_reference_text():
@@ -37,7 +40,7 @@ def test_writes_the_reference_text():
 
 
 def test_unsupported_models_raise():
-    mm = MaterialMap(x=MaterialPair(Dispersion("MagnetizedDrude"), Dispersion("Vacuum")),
+    mm = MaterialMap(x=MaterialPair(Dispersion("FluidDrude"), Dispersion("Vacuum")),
                      y=MaterialPair(Dispersion("Constant", K=2.0), Dispersion("Drude", 1.0, 1e15, 1e13)))
     with pytest.raises(NotImplementedError):
         mm["x"].spec()
@@ -70,3 +73,29 @@ def test_applied_map_steps_like_hand_set_materials():
         runs.append(s.get(E, 2))
     np.testing.assert_array_equal(runs[0], runs[1])
     assert np.abs(runs[0]).max() > 0
+
+
+def test_magnetized_models_apply_in_3d():
+    """The map's MagnetizedDrude / SelfMagnetizedDrude entries become magnetized
+    Drude materials: a 3-D run equals the one with the materials set by hand,
+    and differs from the same plasma without the bias."""
+    mm = MaterialMap.read_xml(XML)
+    n, dx = 14, 1e-3
+    dt = 0.5 * dx / (refcpu.constants()["c0"] * np.sqrt(3.0))
+    matid = np.zeros((n, n, n), np.uint8)
+    matid[3:7, 3:11, 3:11] = 1
+    matid[8:11, 3:11, 3:11] = 2
+    runs = []
+    for how in ("map", "hand", "unbiased"):
+        s = refcpu.Sim("ThreeD", n, n, n, dx, dt)
+        if how == "map":
+            mm.apply(s, {"vacuum": 0, "biased plasma": 1, "self plasma": 2})
+        else:
+            s.set_magnetized_drude(1, 1.0, 1.0, 2e11, 1e9, (0.3, -0.4, 1.25) if how == "hand" else (0.0, 0.0, 0.0))
+            s.set_magnetized_drude(2, 1.5, 1.0, 1.5e11, 2e9, None)
+        s.set_matid(matid)
+        s.add_source(D, 2, 7, 7, 7, 3e9 * gauss_table(50, 12, 4))
+        s.step(50)
+        runs.append(s.get(E, 0))
+    np.testing.assert_array_equal(runs[0], runs[1])
+    assert np.abs(runs[0] - runs[2]).max() > 1e-3 * np.abs(runs[0]).max()

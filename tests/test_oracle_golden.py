@@ -41,17 +41,20 @@ def test_aux1d_matches_reference():
     np.testing.assert_array_equal(tr, a["trace_EH"])
 
 
-@pytest.mark.parametrize("name", ["cell_ade", "cell_conductive"])
+@pytest.mark.parametrize("name", ["cell_ade", "cell_conductive", "cell_magnetized_tm"])
 def test_cell_ade_recursions(name):
+    """cell_magnetized_tm: a magnetized Drude material in TM is the Drude update
+    per component (the reference's MagnetizedDrudeCall, DispersiveMaterials.hpp:1508-1537)."""
     p, a = load(name)
     dt = p["dt"]
     mats = a["mats_kind_eps_freq_gamma_delta"]
     tr = a["trace_D_E_P_J"]
-    from scenarios import ref_material
+    from scenarios import ref_material, set_ref_material
     for m, row in enumerate(mats):
-        eps, mu, poles = ref_material(*row)
+        spec = ref_material(*row, bias=(0.0, 0.0, 1.0))
+        poles = spec[2]
         sim = refcpu.Sim("TM", 3, 3, 1, 1.0, dt)
-        sim.set_material(0, eps, mu, poles)
+        set_ref_material(sim, 0, spec)
         out = []
         for n in range(tr.shape[1]):
             d = np.zeros((1, 3, 3))
@@ -68,8 +71,36 @@ def test_cell_ade_recursions(name):
         np.testing.assert_array_equal(np.array(out), tr[m, :, 1:])
 
 
+def test_cell_magnetized_3d():
+    """MagnetizedDrudeUpdate3D (DispersiveMaterials.hpp:1283-1340) on one 3-D
+    cell with prescribed D_n, for fixed biases along z, oblique and along x, and
+    self-magnetized by prescribed B_n: E, P, J bit for bit."""
+    p, a = load("cell_magnetized")
+    dt = p["dt"]
+    mats, tr = a["mats_kind_eps_freq_gamma_bx_by_bz"], a["trace_D3_E3_P3_J3_B3"]
+    for m, (kind, eps, freq, gamma, bx, by, bz) in enumerate(mats):
+        sim = refcpu.Sim("ThreeD", 3, 3, 3, 1.0, dt)
+        sim.set_magnetized_drude(0, eps, 1.0, freq, gamma, (bx, by, bz) if kind == 5 else None)
+        out = []
+        for n in range(tr.shape[1]):
+            row = tr[m, n]
+            for c in range(3):
+                d = np.zeros((3, 3, 3))
+                d[1, 1, 1] = row[c]
+                sim.set(D, c, d)
+                b = np.zeros((3, 3, 3))
+                b[1, 1, 1] = row[12 + c]
+                sim.set(B, c, b)
+                sim.set(H, c, np.zeros((3, 3, 3)))  # the D curl adds exactly 0
+            sim.step(1)
+            out.append([sim.get(f, c)[1, 1, 1] for f in (E, 4, 5) for c in range(3)])
+        np.testing.assert_array_equal(np.array(out), tr[m, :, 3:12], err_msg=f"material {m}")
+    # the bias matters: a zero-bias material differs from the biased trace
+    assert np.abs(tr[0, :, 9] - tr[2, :, 9]).max() > 0
+
+
 @pytest.mark.parametrize("name", ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small",
-                                  "grid3d_disp", "tm2d_c1", "tm2d_conductive"])
+                                  "grid3d_disp", "tm2d_c1", "tm2d_conductive", "grid3d_magnetized"])
 def test_grid_scenarios_bit_exact(name):
     sc, p, a = from_golden(name)
     sim = refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])
