@@ -152,6 +152,8 @@ struct MatHost {
   // magnetized Drude: 0 none, 1 fixed bias field `bias` [T], 2 the cell's own B
   int mag = 0;
   double bias[3] = {0, 0, 0};
+  // fluid Drude: omega_p and gamma of each cell from fdtd_set_fluid_state
+  int fluid = 0;
 };
 
 struct fdtd_grid {
@@ -185,6 +187,11 @@ struct fdtd_grid {
   int dbox[6] = {0, 0, 0, 0, 0, 0};
   int max_np = 0;
   void* Dd[3] = {};
+  // fluid Drude: per-cell electron density and collision rate (host, whole
+  // grid) and the per-cell coefficients over the dispersive box (device)
+  std::vector<double> fluid_ne, fluid_nu;
+  void* fl[2] = {};
+  bool fluid_dirty = true;
   void* Pd[FDTD_MAX_POLES][3] = {};
   void* Jd[FDTD_MAX_POLES][3] = {};
   // second copies of the stepped state for the fused (ping-pong) schedule
@@ -339,12 +346,55 @@ int download_T(fdtd_grid* g, double* dst, const void* src, size_t n) {
   return 0;
 }
 
-// a 3-D grid whose table holds a magnetized Drude material (nmag is counted
-// by the material setters)
+// a 3-D grid whose table holds a magnetized or fluid Drude material (nmag is
+// counted by the material setters)
 static bool has_magnetized(const fdtd_grid* g) { return g->ndim == 3 && g->nmag > 0; }
 static void count_magnetized(fdtd_grid* g) {
   g->nmag = 0;
-  for (int id = 0; id < FDTD_MAX_MATERIALS; id++) g->nmag += g->mats[id].np > 0 && g->mats[id].mag != 0;
+  for (int id = 0; id < FDTD_MAX_MATERIALS; id++)
+    g->nmag += g->mats[id].np > 0 && (g->mats[id].mag != 0 || g->mats[id].fluid != 0);
+}
+
+// Per-cell coefficients of the fluid Drude cells of the dispersive box:
+// omega_p = K sqrt(ne) with K = sqrt(q_e^2/(m_e eps0)), gamma = nu_m
+// (FluidDrudeStorage, DispersiveMaterials.hpp:1160-1181).  Magnetized in 3-D:
+// {wp^2/dt, 1 + dt gamma + wp^2} (1297-1305); otherwise the Drude recursion's
+// {(1 - b)/(1 + b), w^2/dt eps0} with b = gamma dt/2, w = dt omega_p (986-996).
+template <typename T>
+int upload_fluid(fdtd_grid* g) {
+  g->fluid_dirty = false;
+  if (g->max_np == 0 || !g->fl[0]) return 0;
+  const long nb = long(g->dbox[1] - g->dbox[0]) * (g->dbox[3] - g->dbox[2]) * (g->dbox[5] - g->dbox[4]);
+  std::vector<double> c0(nb, 0.0), c1(nb, 0.0);
+  const double dt = g->d.dt, kval = newton_sqrt(kQe * kQe / (kMe * kEps0));
+  bool any = false;
+  long t = 0;
+  for (int k = g->dbox[4]; k < g->dbox[5]; k++)
+    for (int j = g->dbox[2]; j < g->dbox[3]; j++)
+      for (int i = g->dbox[0]; i < g->dbox[1]; i++, t++) {
+        const long qq = g->q(i, j, k);
+        const MatHost& h = g->mats[g->mat_host[qq]];
+        if (!h.np || !h.fluid) continue;
+        if (g->fluid_ne.empty()) {
+          g->fluid_dirty = true;
+          return fail(FDTD_ESTATE, "a fluid Drude material needs fdtd_set_fluid_state");
+        }
+        any = true;
+        const double freq = kval * std::sqrt(g->fluid_ne[qq]), gamma = g->fluid_nu[qq];
+        if (h.mag && g->ndim == 3) {
+          const double wp = dt * freq, vc = dt * gamma;
+          c0[t] = wp * wp / dt;
+          c1[t] = 1.0 + vc + wp * wp;
+        } else {
+          const double b = gamma * dt * 0.5, w = dt * freq;
+          c0[t] = (1.0 - b) / (1.0 + b);
+          c1[t] = w * w / dt * kEps0;
+        }
+      }
+  if (!any) return 0;
+  int rc = upload_T(g, g->fl[0], c0.data(), nb);
+  if (!rc) rc = upload_T(g, g->fl[1], c1.data(), nb);
+  return rc;
 }
 
 template <typename T>
@@ -359,23 +409,18 @@ int upload_materials(fdtd_grid* g) {
     m[id].cG = T(1.0);
     m[id].np = h.np;
     m[id].mag = 0;
-    for (int q = 0; q < 9; q++) m[id].mM[q] = T(0);
+    m[id].fluid = h.np > 0 && h.fluid;
     m[id].w2dt = m[id].op = m[id].vq = T(0);
+    for (int c = 0; c < 3; c++) m[id].v[c] = T(0);
     if (h.mag && g->ndim == 3) {
-      // MagnetizedDrudeUpdate3D::implicit_update, DispersiveMaterials.hpp:1297-1318
+      // MagnetizedDrudeUpdate3D::implicit_update, DispersiveMaterials.hpp:1297-1305
+      // (a fluid material's wp and gamma are per cell: upload_fluid)
       const double wp = dt * h.freq[0], vc = dt * h.gamma[0], op = 1.0 + vc + wp * wp;
       m[id].mag = h.mag;
       m[id].w2dt = T(wp * wp / dt);
       m[id].op = T(op);
       m[id].vq = T(dt * kQe / kMe);
-      if (h.mag == 1) {
-        const double vx = dt * h.bias[0] * kQe / kMe, vy = dt * h.bias[1] * kQe / kMe, vz = dt * h.bias[2] * kQe / kMe;
-        const double denom = op * (op * op + (vx * vx + vy * vy + vz * vz));
-        const double A[9] = {op * op + vx * vx, -vz * op + vx * vy, vy * op + vx * vz,
-                             vz * op + vx * vy, op * op + vy * vy,  -vx * op + vy * vz,
-                             -vy * op + vx * vz, vx * op + vy * vz, op * op + vz * vz};
-        for (int q = 0; q < 9; q++) m[id].mM[q] = T(A[q] / denom);
-      }
+      for (int c = 0; c < 3; c++) m[id].v[c] = T(dt * h.bias[c] * kQe / kMe);
     }
     for (int p = 0; p < fdtdk::kMaxPoles; p++) {
       double cJ = 0, cE = 0, cP = 0, cD = 0;
@@ -521,6 +566,11 @@ int finalize(fdtd_grid* g) {
           if ((rc = g->alloc(&g->Jd[p][c], size_t(nb) * g->esz))) return rc;
         }
       }
+      for (int w = 0; w < 2; w++) {
+        int rc = g->alloc(&g->fl[w], size_t(nb) * g->esz);
+        if (rc) return rc;
+      }
+      g->fluid_dirty = true;
     }
   }
   g->finalized = true;
@@ -657,6 +707,8 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   a.sy = g->d.nx;
   a.sz = long(g->d.nx) * g->d.ny;
   for (int x = 0; x < 6; x++) a.dbox[x] = g->dbox[x];
+  gput(a.fl[0], (const T*)g->fl[0]);
+  gput(a.fl[1], (const T*)g->fl[1]);
   a.dt = T(g->d.dt);
   a.nsrc = g->nsrc;
   for (int s = 0; s < g->nsrc; s++) a.src[s] = g->src[s];
@@ -804,6 +856,10 @@ template <typename T>
 int step_impl(fdtd_grid* g, int nsteps) {
   if (g->mats_dirty) {
     int rc = upload_materials<T>(g);
+    if (rc) return rc;
+  }
+  if (g->fluid_dirty) {
+    int rc = upload_fluid<T>(g);
     if (rc) return rc;
   }
   if (g->coef_dirty) {
@@ -1028,6 +1084,7 @@ int snapshot_take(fdtd_grid* g, GridSnapshot& s) {
   // pending table uploads go first: the snapshot must hold the tables the
   // trials will run with, or the restore would roll them back to stale contents
   if (g->mats_dirty && (rc = g->f64 ? upload_materials<double>(g) : upload_materials<float>(g))) return rc;
+  if (g->fluid_dirty && (rc = g->f64 ? upload_fluid<double>(g) : upload_fluid<float>(g))) return rc;
   if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   s.ptrs = g->allocs;
@@ -1476,6 +1533,8 @@ int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npole
   m.mu = mu_r;
   m.np = npoles;
   m.mag = 0;
+  m.fluid = 0;
+  g->fluid_dirty = true;
   for (int p = 0; p < npoles; p++) {
     m.kind[p] = int(poles[4 * p]);
     m.freq[p] = poles[4 * p + 1];
@@ -1500,6 +1559,31 @@ int fdtd_set_magnetized_drude(fdtd_grid* g, int id, double eps_r, double mu_r, d
   return 0;
 }
 
+int fdtd_set_fluid_drude(fdtd_grid* g, int id, double mu_r, int magnetization, const double* bias) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (magnetization < 0 || magnetization > 2 || (magnetization == 1 && !bias))
+    return fail(FDTD_EINVAL, "magnetization must be 0 (none), 1 (fixed bias, given) or 2 (the cell's own field)");
+  const double pole[4] = {double(FDTD_DRUDE), 0.0, 0.0, 1.0};
+  int rc = fdtd_set_material(g, id, 1.0, mu_r, 1, pole);  // K = 1 (FluidDrudeStorage::getK)
+  if (rc) return rc;
+  MatHost& m = g->mats[id];
+  m.fluid = 1;
+  m.mag = magnetization;
+  for (int c = 0; c < 3; c++) m.bias[c] = magnetization == 1 ? bias[c] : 0.0;
+  count_magnetized(g);
+  return 0;
+}
+
+int fdtd_set_fluid_state(fdtd_grid* g, const double* ne, const double* nu_m) {
+  if (!g || !ne || !nu_m) return fail(FDTD_EINVAL, "null argument");
+  for (long i = 0; i < g->ncell; i++)
+    if (!(ne[i] >= 0) || !(nu_m[i] >= 0)) return fail(FDTD_EINVAL, "densities and collision rates must be >= 0");
+  g->fluid_ne.assign(ne, ne + g->ncell);
+  g->fluid_nu.assign(nu_m, nu_m + g->ncell);
+  g->fluid_dirty = true;
+  return 0;
+}
+
 int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids) {
   if (!g || !ids) return fail(FDTD_EINVAL, "null argument");
   if (g->finalized && g->max_np > 0) {
@@ -1508,6 +1592,7 @@ int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids) {
         return fail(FDTD_ESTATE, "dispersive layout is fixed once stepping has started");
   }
   std::copy(ids, ids + g->ncell, g->mat_host.begin());
+  g->fluid_dirty = true;
   HIP_TRY(hipMemcpy(g->mat, ids, g->ncell, hipMemcpyHostToDevice));
   return 0;
 }

@@ -59,10 +59,13 @@ struct MatDev {
   // magnetized Drude (3-D): 0 none, 1 a fixed bias field, 2 the cell's own B
   // (MagnetizedDrudeUpdate3D::implicit_update, DispersiveMaterials.hpp:1283-1340)
   int mag;
-  T mM[9];  // fixed bias: A/denom, row-major (A of the implicit Euler system)
-  T w2dt;   // (dt omega_p)^2 / dt
-  T op;     // 1 + dt gamma + (dt omega_p)^2
-  T vq;     // dt q_e/m_e: v = vq B
+  // fluid Drude: omega_p and gamma per cell (from the electron density and
+  // collision rate), its coefficients in StepArgs::fl
+  int fluid;
+  T v[3];  // fixed bias: v = dt (q_e/m_e) B
+  T w2dt;  // (dt omega_p)^2 / dt
+  T op;    // 1 + dt gamma + (dt omega_p)^2
+  T vq;    // dt q_e/m_e: v = vq B for the cell's own B
 };
 
 struct SrcDev {
@@ -123,6 +126,9 @@ struct StepArgs {
   FDTD_G T* Ddo[3];
   FDTD_G T* Pdo[kMaxPoles][3];
   FDTD_G T* Jdo[kMaxPoles][3];
+  // fluid Drude coefficients per dispersive-box cell: {cJ, cE} of the Drude
+  // recursion (TM, or no magnetization) or {w2dt, op} of the magnetized one
+  const FDTD_G T* fl[2];
   T dt;
   // soft sources
   int nsrc;
@@ -337,17 +343,45 @@ __device__ __forceinline__ long dbox_index(const StepArgs<T>& a, int i, int j, i
          (long)(a.dbox[1] - a.dbox[0]) * ((long)(j - a.dbox[2]) + (long)(a.dbox[3] - a.dbox[2]) * (long)(k - a.dbox[4]));
 }
 
+// Fluid Drude, one component, with the cell's own coefficients cJ, cE
+// (FluidDrudeStorage: omega_p = K sqrt(ne), gamma = nu_m, through
+// DrudeUpdate::implicit_update, DispersiveMaterials.hpp:979-998, 1160-1181):
+// P += dt J; E = (D - P)/eps0; J = cJ J + cE E.
+template <typename T, int I>
+__device__ __forceinline__ T fluid_component(const StepArgs<T>& a, const MatDev<T>& m, long bq, T d, bool own) {
+  const T D = a.Dd[I][bq] + d;
+  const T J = a.Jd[0][I][bq];
+  const T P = a.Pd[0][I][bq] + a.dt * J;
+  const T Ev = (D - P) * m.inv_eps;
+  if (own) {
+    a.Ddo[I][bq] = D;
+    a.Pdo[0][I][bq] = P;
+    a.Jdo[0][I][bq] = a.fl[0][bq] * J + a.fl[1][bq] * Ev;
+  }
+  return Ev;
+}
+
 // Magnetized Drude cell (3-D): the implicit Euler system couples the three
 // components through v = dt (q_e/m_e) B.  With r = J + (wp^2/dt)(D - P) and
 // M = A/denom (A = op^2 I + v v^T + op [v]x, denom = op (op^2 + |v|^2)):
 // J' = M r, P' = P + dt J' (the reference's P' = P + M (dt r), the same
 // vector), E = (D - P')/(eps_r eps0)
 // (MagnetizedDrudeUpdate3D::implicit_update, DispersiveMaterials.hpp:1283-1340).
-// mag = 2 takes B from the cell's own magnetic field, B = mu_r mu0 H
-// (SelfMagnetizedDrudeStorage, 1898-1943).
+// B is the fixed bias (MagnetizedDrudeStorage) or the cell's own field
+// B = mu_r mu0 H (the self-magnetized storages, 1816-1943); wp and gamma are
+// the material's or, for the fluid forms, the cell's own (1728-1893).
+// A fluid material without magnetization runs fluid_component.
 template <typename T>
-__device__ FDTD_COLD void magnetized_cell(const StepArgs<T>& a, const MatDev<T>& m, long bq, const T d[3],
-                                          const T hown[3], bool own, T out[3]) {
+__device__ FDTD_COLD void extended_cell(const StepArgs<T>& a, const MatDev<T>& m, long bq, const T d[3],
+                                        const T hown[3], bool own, T out[3]) {
+  if (!m.mag) {
+    out[0] = fluid_component<T, 0>(a, m, bq, d[0], own);
+    out[1] = fluid_component<T, 1>(a, m, bq, d[1], own);
+    out[2] = fluid_component<T, 2>(a, m, bq, d[2], own);
+    return;
+  }
+  const T w2dt = m.fluid ? a.fl[0][bq] : m.w2dt;
+  const T op = m.fluid ? a.fl[1][bq] : m.op;
   T P[3], s[3], r[3];  // s = D - P
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -355,34 +389,21 @@ __device__ FDTD_COLD void magnetized_cell(const StepArgs<T>& a, const MatDev<T>&
     if (own) a.Ddo[c][bq] = D;
     P[c] = a.Pd[0][c][bq];
     s[c] = D - P[c];
-    r[c] = a.Jd[0][c][bq] + m.w2dt * s[c];
+    r[c] = a.Jd[0][c][bq] + w2dt * s[c];
   }
-  T v[3] = {T(0), T(0), T(0)}, inv = T(0);
-  const T op = m.op;
-  if (m.mag == 2) {
+  T v[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = m.vq * (hown[c] / m.inv_mu);
-    inv = T(1) / (op * (op * op + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2])));
-  }
+  for (int c = 0; c < 3; c++) v[c] = m.mag == 1 ? m.v[c] : m.vq * (hown[c] / m.inv_mu);
+  const T inv = T(1) / (op * (op * op + (v[0] * v[0] + v[1] * v[1] + v[2] * v[2])));
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    T M0, M1, M2;  // row c of A/denom
-    if (m.mag == 1) {
-      M0 = m.mM[3 * c];
-      M1 = m.mM[3 * c + 1];
-      M2 = m.mM[3 * c + 2];
-    } else {
-      // A = op^2 I + v v^T + op [v]x with [v]x = ((0,-vz,vy),(vz,0,-vx),(-vy,vx,0))
-      const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-      T row[3];
-      row[c] = (op * op + v[c] * v[c]) * inv;
-      row[c1] = (-v[c2] * op + v[c] * v[c1]) * inv;
-      row[c2] = (v[c1] * op + v[c] * v[c2]) * inv;
-      M0 = row[0];
-      M1 = row[1];
-      M2 = row[2];
-    }
-    const T Jn = M0 * r[0] + M1 * r[1] + M2 * r[2];
+    // row c of A = op^2 I + v v^T + op [v]x, [v]x = ((0,-vz,vy),(vz,0,-vx),(-vy,vx,0))
+    const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+    T row[3];
+    row[c] = op * op + v[c] * v[c];
+    row[c1] = -v[c2] * op + v[c] * v[c1];
+    row[c2] = v[c1] * op + v[c] * v[c2];
+    const T Jn = (row[0] * r[0] + row[1] * r[1] + row[2] * r[2]) * inv;
     out[c] = (s[c] - a.dt * Jn) * m.inv_eps;
     if (own) {
       a.Pdo[0][c][bq] = P[c] + a.dt * Jn;
@@ -393,14 +414,14 @@ __device__ FDTD_COLD void magnetized_cell(const StepArgs<T>& a, const MatDev<T>&
 
 // dispersive cell: the three components (hown: the cell's own H, read by a
 // self-magnetized Drude material; MAG: the instance that carries the
-// magnetized Drude code)
+// magnetized and fluid Drude code)
 template <typename T, bool MAG>
 __device__ FDTD_COLD void ade_cell(const StepArgs<T>& a, int mid, int i, int j, int k, const T d[3], const T eold[3],
                                    const T hown[3], bool own, T out[3]) {
   const MatDev<T>& m = a.mats[mid];
   const long bq = dbox_index(a, i, j, k);
-  if (MAG && m.mag) {
-    magnetized_cell(a, m, bq, d, hown, own, out);
+  if (MAG && (m.mag || m.fluid)) {
+    extended_cell(a, m, bq, d, hown, own, out);
     return;
   }
   out[0] = ade_component<T, 0>(a, m, bq, d[0], eold[0], own);
@@ -424,7 +445,7 @@ struct EStencil {
 // the current buffers and, when `own`, written to the next buffers.
 // DISP = 0: the grid has no dispersive material (the ADE code is left out of
 // that instance, which keeps its register count down); 1: pole materials;
-// 2: also magnetized Drude materials.
+// 2: also magnetized and fluid Drude materials.
 template <typename T, int DISP = 1>
 __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incH, int i, int j, int k,
                                           long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy,
@@ -792,8 +813,8 @@ __device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a, int ft, int ndim,
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
 // BC: the instance for grids with boundary conditions (ghost writes); DISP:
-// for grids with dispersive materials (1: ADE poles, 2: also magnetized
-// Drude).  An instance without them carries none of that code, so its
+// for grids with dispersive materials (1: ADE poles, 2: also magnetized and
+// fluid Drude).  An instance without them carries none of that code, so its
 // registers stay low.
 template <typename T, bool BC = false, int DISP = 1>
 __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie, int i, int j, int k0, int k1, int s) {
@@ -1451,7 +1472,10 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
     a.Eo[2][c] = ezn;
   } else if (ie < T(0) && a.Dd[2]) {
     const MatDev<T>& m = a.mats[mid];
-    ezn = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], ez, true);
+    if (m.fluid)
+      ezn = fluid_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], true);
+    else
+      ezn = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], ez, true);
     a.Eo[2][c] = ezn;
   }
   if (BC && a.bc_any) {

@@ -24,7 +24,7 @@ BOUNDARY = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "
 # every symbol declared in include/fdtd_hip.h
 ABI_SYMBOLS = [
     "fdtd_grid_create", "fdtd_grid_destroy", "fdtd_set_stream", "fdtd_set_pml_axis", "fdtd_set_pml_idiom",
-    "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_dft", "fdtd_get_dft",
+    "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_dft", "fdtd_get_dft",
     "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
@@ -64,6 +64,8 @@ def load_library():
         "fdtd_set_pml_idiom": (_i, [_p, _i, _i, _d, _d, _d, _d, _d]),
         "fdtd_set_material": (_i, [_p, _i, _d, _d, _i, _dp]),
         "fdtd_set_magnetized_drude": (_i, [_p, _i, _d, _d, _d, _d, _dp]),
+        "fdtd_set_fluid_drude": (_i, [_p, _i, _d, _i, _dp]),
+        "fdtd_set_fluid_state": (_i, [_p, _dp, _dp]),
         "fdtd_set_matid": (_i, [_p, ctypes.c_char_p]),
         "fdtd_add_source": (_i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
         "fdtd_set_tfsf": (_i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
@@ -170,6 +172,20 @@ class Grid:
         b = None if bias is None else _f64(np.asarray(bias, dtype=np.float64).reshape(3))
         _check(load_library().fdtd_set_magnetized_drude(self.h, mid, eps_r, mu_r, freq, gamma,
                                                         None if b is None else _dptr(b)))
+
+    def set_fluid_drude(self, mid, mu_r=1.0, magnetization=0, bias=None):
+        """Fluid Drude material (omega_p, gamma per cell from set_fluid_state): magnetization
+        0 FluidDrudeUpdate, 1 MagnetizedFluidDrudeUpdate in `bias` [T], 2 SelfMagnetizedFluidDrudeUpdate."""
+        b = None if bias is None else _f64(np.asarray(bias, dtype=np.float64).reshape(3))
+        _check(load_library().fdtd_set_fluid_drude(self.h, mid, mu_r, int(magnetization),
+                                                   None if b is None else _dptr(b)))
+
+    def set_fluid_state(self, ne, nu_m):
+        """Per-cell electron density [1/m^3] and collision rate [1/s], shape of the grid."""
+        ne, nu_m = _f64(np.asarray(ne).reshape(-1)), _f64(np.asarray(nu_m).reshape(-1))
+        if ne.size != int(np.prod(self.shape)) or nu_m.size != ne.size:
+            raise ValueError("fluid state size mismatch")
+        _check(load_library().fdtd_set_fluid_state(self.h, _dptr(ne), _dptr(nu_m)))
 
     def set_matid(self, ids):
         ids = np.ascontiguousarray(ids, dtype=np.uint8).reshape(-1)

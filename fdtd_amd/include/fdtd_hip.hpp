@@ -15,6 +15,8 @@
 //   ConductiveUpdate<Mode,ft>(K,freq,dt)            same                             DispersiveMaterials.hpp:674-677
 //   MagnetizedDrudeUpdate<Mode,ft>(Bx,By,Bz,K,Freq,Gamma,dt)  same                   DispersiveMaterials.hpp:2008-2011
 //   SelfMagnetizedDrudeUpdate<Mode,ft>(dt) + K()/Freq()/Gamma()  same                DispersiveMaterials.hpp:2024-2027
+//   FluidDrudeUpdate<Mode,ft>(dt), cell.ne()/nu_m() same + grid.setFluidState(ne, nu) DispersiveMaterials.hpp:1214-1217
+//   MagnetizedFluidDrudeUpdate<Mode,ft>(Bx,By,Bz,dt), SelfMagnetizedFluidDrudeUpdate<Mode,ft>(dt)  2014-2021
 //   make_soft_source<Dz>(signal, cell)              make_soft_source<Dz>(signal, grid.cell(i,j,k))  Source.hpp:258-261
 //   Aux1D<double>(ncells,dx,dt,signal)              Aux1D<double>(ncells,dx,dt,signal)               Source.hpp:280-408
 //   make_tfsf_plane<Mode,Dir,Orient>(b,e,loc,k,aux) make_tfsf_box<Mode>(grid,box,k,aux)              Source.hpp:624-634
@@ -178,6 +180,11 @@ class YeeGrid {
   }
 
   void setMaterialIds(const std::vector<uint8_t>& ids) { check(fdtd_set_matid(handle(), ids.data()), "fdtd_set_matid"); }
+  // the cells' ne() and nu_m() read by the fluid Drude materials (nx*ny*nz each)
+  void setFluidState(const std::vector<double>& ne, const std::vector<double>& nu_m) {
+    if (ne.size() != size_t(nx_) * ny_ * nz_ || nu_m.size() != ne.size()) throw Error("setFluidState: size mismatch");
+    check(fdtd_set_fluid_state(handle(), ne.data(), nu_m.data()), "fdtd_set_fluid_state");
+  }
 
   std::vector<double> download(int field, int comp, int pole = 0) const {
     std::vector<double> v(size_t(nx_) * ny_ * nz_);
@@ -239,6 +246,7 @@ struct MaterialSpec {
   std::vector<double> poles;  // kind, freq, gamma, weight per pole
   int mag = 0;                // magnetized Drude: 1 fixed bias, 2 the cell's own B
   double bias[3] = {0, 0, 0};
+  bool fluid = false;         // fluid Drude: omega_p, gamma per cell (YeeGrid::setFluidState)
 };
 template <class Mode, FieldType ft, bool forward = false>
 struct VacuumUpdate {
@@ -312,10 +320,53 @@ struct SelfMagnetizedDrudeUpdate {
     return s;
   }
 };
+// The fluid forms: omega_p = sqrt(q_e^2/(m_e eps0) ne) and gamma = nu_m of
+// each cell (FluidDrudeStorage, MagnetizedFluidDrudeStorage,
+// SelfMagnetizedFluidDrudeStorage; include/DispersiveMaterials.hpp:1160-1199,
+// 1728-1893), K = 1; the cells' ne and nu_m are given by YeeGrid::setFluidState
+template <class Mode, FieldType ft, bool forward = false>
+struct FluidDrudeUpdate {
+  double dt;
+  explicit FluidDrudeUpdate(double deltat) : dt(deltat) {}
+  MaterialSpec spec() const {
+    MaterialSpec s{1.0, 1.0, {double(FDTD_DRUDE), 0.0, 0.0, 1.0}};
+    s.fluid = true;
+    return s;
+  }
+};
+template <class Mode, FieldType ft, bool forward = false>
+struct MagnetizedFluidDrudeUpdate {
+  double Bx, By, Bz, dt;
+  MagnetizedFluidDrudeUpdate(double bx, double by, double bz, double deltat) : Bx(bx), By(by), Bz(bz), dt(deltat) {}
+  MaterialSpec spec() const {
+    MaterialSpec s{1.0, 1.0, {double(FDTD_DRUDE), 0.0, 0.0, 1.0}};
+    s.fluid = true;
+    s.mag = 1;
+    s.bias[0] = Bx;
+    s.bias[1] = By;
+    s.bias[2] = Bz;
+    return s;
+  }
+};
+template <class Mode, FieldType ft, bool forward = false>
+struct SelfMagnetizedFluidDrudeUpdate {
+  double dt;
+  explicit SelfMagnetizedFluidDrudeUpdate(double deltat) : dt(deltat) {}
+  MaterialSpec spec() const {
+    MaterialSpec s{1.0, 1.0, {double(FDTD_DRUDE), 0.0, 0.0, 1.0}};
+    s.fluid = true;
+    s.mag = 2;
+    return s;
+  }
+};
 
 // MaterialMap equivalent (include/MaterialMap.hpp:204-284): id -> material.
 template <class Mode, class scalar>
 void set_material(YeeGrid<Mode, scalar>& g, int id, const MaterialSpec& s) {
+  if (s.fluid) {
+    check(fdtd_set_fluid_drude(g.handle(), id, s.mu, s.mag, s.mag == 1 ? s.bias : nullptr), "fdtd_set_fluid_drude");
+    return;
+  }
   if (s.mag) {
     check(fdtd_set_magnetized_drude(g.handle(), id, s.eps, s.mu, s.poles[1], s.poles[2], s.mag == 1 ? s.bias : nullptr),
           "fdtd_set_magnetized_drude");

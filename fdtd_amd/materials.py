@@ -14,10 +14,11 @@ A map is put on a grid (``fdtd_amd.Grid``) or the oracle by ``apply(sim, ids)``
 with ids = {identifier: material id}: electric Vacuum / Constant / Conductive
 / Drude / Lorentz become the material's eps_r and pole, MagnetizedDrude
 (MagnetizedDrudeStorage 1619-1726, its <B>(x, y, z)</B> tag included) and
-SelfMagnetizedDrude (1898-2006) a magnetized Drude material, magnetic Vacuum /
-Constant its mu_r.  The reference's other models (the fluid Drude family,
-whose density and collision rate are fields of a user cell type, and magnetic
-dispersion) have no device counterpart and raise.
+SelfMagnetizedDrude (1898-2006) a magnetized Drude material, FluidDrude
+(1160-1199), MagnetizedFluidDrude (1728-1814) and SelfMagnetizedFluidDrude
+(1816-1893) a fluid Drude material (its density and collision rate per cell
+come from the grid's fluid state), magnetic Vacuum / Constant its mu_r.
+Magnetic dispersion has no device counterpart and raises.
 """
 import xml.etree.ElementTree as ET
 from dataclasses import dataclass
@@ -32,7 +33,11 @@ DISPERSIONS = ("Vacuum", "Constant", "Conductive", "Drude", "FluidDrude", "Loren
 _FIELDS = {"Vacuum": (), "Constant": ("Constant",), "Conductive": ("Constant", "Freq"),
            "Drude": ("Constant", "Freq", "Gamma"), "Lorentz": ("Constant", "Delta", "Freq", "Gamma"),
            "MagnetizedDrude": ("B", "Constant", "Freq", "Gamma"),
-           "SelfMagnetizedDrude": ("Constant", "Freq", "Gamma")}
+           "SelfMagnetizedDrude": ("Constant", "Freq", "Gamma"),
+           "FluidDrude": (), "MagnetizedFluidDrude": ("B",)}
+# models written as one empty element on one line (their print_summary)
+_ONE_LINE = ("Vacuum", "SelfMagnetizedFluidDrude")
+_FLUID = {"FluidDrude": 0, "MagnetizedFluidDrude": 1, "SelfMagnetizedFluidDrude": 2}
 _ATTR = {"Constant": "K", "Freq": "freq", "Gamma": "gamma", "Delta": "delta", "B": "B"}
 
 
@@ -61,9 +66,12 @@ class MaterialPair:
     def spec(self):
         """(eps_r, mu_r, poles) for Grid.set_material / the oracle's set_material;
         for a magnetized Drude model (eps_r, mu_r, poles, bias) with bias None for
-        the self-magnetized one (Grid.set_magnetized_drude)."""
+        the self-magnetized one (Grid.set_magnetized_drude); for a fluid Drude
+        model {"fluid": magnetization, "mu": mu_r, "bias": B or None}
+        (Grid.set_fluid_drude)."""
         e, m = self.electric, self.magnetic
         bias = ()
+        fluid = None
         if e.kind == "Vacuum":
             eps, poles = 1.0, []
         elif e.kind == "Constant":
@@ -77,6 +85,9 @@ class MaterialPair:
         elif e.kind in ("MagnetizedDrude", "SelfMagnetizedDrude"):
             eps, poles = e.K, [(DRUDE, e.freq, e.gamma, 1.0)]
             bias = (tuple(float(b) for b in e.B) if e.kind == "MagnetizedDrude" else None,)
+        elif e.kind in _FLUID:
+            eps, poles = 1.0, [(DRUDE, 0.0, 0.0, 1.0)]
+            fluid = _FLUID[e.kind]
         else:
             raise NotImplementedError(f"electric model {e.kind} has no device counterpart")
         if m.kind == "Vacuum":
@@ -85,6 +96,8 @@ class MaterialPair:
             mu = m.K
         else:
             raise NotImplementedError(f"magnetic model {m.kind}: magnetic dispersion has no device counterpart")
+        if fluid is not None:
+            return {"fluid": fluid, "mu": mu, "bias": tuple(float(b) for b in e.B) if fluid == 1 else None}
         return (eps, mu, poles) + bias
 
 
@@ -135,8 +148,8 @@ class MaterialMap(dict):
                       t(ntabs + 2) + "<MaterialPair>"]
             for side, d in (("Electric", pair.electric), ("Magnetic", pair.magnetic)):
                 lines.append(t(ntabs + 3) + f"<{side}>")
-                if d.kind == "Vacuum":
-                    lines.append(t(ntabs + 4) + "<Vacuum></Vacuum>")
+                if d.kind in _ONE_LINE:
+                    lines.append(t(ntabs + 4) + f"<{d.kind}></{d.kind}>")
                 else:
                     if d.kind not in _FIELDS:
                         raise NotImplementedError(f"writing model {d.kind}")
@@ -157,7 +170,9 @@ class MaterialMap(dict):
         """Set material id ids[name] of `sim` (Grid or oracle Sim) from each named pair."""
         for name, mid in ids.items():
             s = self[name].spec()
-            if len(s) == 4:
+            if isinstance(s, dict):
+                sim.set_fluid_drude(mid, s["mu"], s["fluid"], s["bias"])
+            elif len(s) == 4:
                 eps, mu, poles, bias = s
                 sim.set_magnetized_drude(mid, eps, mu, poles[0][1], poles[0][2], bias)
             else:

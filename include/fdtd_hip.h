@@ -106,6 +106,20 @@ int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npole
  * Freq = omega_p [rad/s], Gamma [1/s]. */
 int fdtd_set_magnetized_drude(fdtd_grid* g, int id, double eps_r, double mu_r, double freq, double gamma,
                               const double* bias);
+/* Material `id` as a fluid Drude plasma whose plasma frequency and collision
+ * rate are those of each cell: omega_p = sqrt(q_e^2/(m_e eps0)) sqrt(ne),
+ * gamma = nu_m, K = 1 (FluidDrudeStorage, include/DispersiveMaterials.hpp:
+ * 1160-1199).  magnetization 0: FluidDrudeUpdate (the Drude update per
+ * component); 1: MagnetizedFluidDrudeUpdate in the fixed field bias[3] [T]
+ * (1728-1814); 2: SelfMagnetizedFluidDrudeUpdate in the cell's own B
+ * (1816-1893).  In TM all three are the Drude update per component, as in the
+ * reference.  The per-cell values come from fdtd_set_fluid_state. */
+int fdtd_set_fluid_drude(fdtd_grid* g, int id, double mu_r, int magnetization, const double* bias);
+/* The cells' electron density ne [1/m^3] and momentum-transfer collision rate
+ * nu_m [1/s] (the ne() / nu_m() accessors the reference's fluid storages read
+ * from a cell), nx*ny*nz each, read where the material is a fluid Drude one.
+ * May be called again between steps (a coupled fluid model). */
+int fdtd_set_fluid_state(fdtd_grid* g, const double* ne, const double* nu_m);
 /* per-cell material id, nx*ny*nz bytes */
 int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids);
 

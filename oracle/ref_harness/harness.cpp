@@ -123,15 +123,26 @@ struct Nb : public Base {
   Nb& getNeighborMax(int d) const { return *mx[d]; }
 };
 
-typedef Nb<YeeCell<YeeFields<TM, double, std::array>, DoublePolarization<double>, VacuumMagnetization<double>,
-                   NewPML<TM, double>>>
+// the electron density and collision rate that the reference's fluid Drude
+// storages read from a cell (f.ne(), f.nu_m(): DispersiveMaterials.hpp:1166-1168)
+template <class Base>
+struct FluidCell : public Base {
+  double mne = 0, mnu = 0;
+  double& ne() { return mne; }
+  double& nu_m() { return mnu; }
+};
+
+typedef Nb<FluidCell<YeeCell<YeeFields<TM, double, std::array>, DoublePolarization<double>,
+                             VacuumMagnetization<double>, NewPML<TM, double>>>>
     CellTM;
-typedef Nb<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<double>, VacuumMagnetization<double>,
-                   NewPML<ThreeD, double>>>
+typedef Nb<FluidCell<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<double>,
+                             VacuumMagnetization<double>, NewPML<ThreeD, double>>>>
     Cell3;
 
 // material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz, 4 conductive (freq = the conductivity),
-// 5 magnetized Drude with the bias (bx, by, bz) [T], 6 self-magnetized Drude (the cell's own B)
+// 5 magnetized Drude with the bias (bx, by, bz) [T], 6 self-magnetized Drude (the cell's own B),
+// 7 fluid Drude, 8 magnetized fluid Drude (bias), 9 self-magnetized fluid Drude (the fluid
+// forms read omega_p and gamma from the cell's ne() and nu_m())
 struct Mat {
   int kind;
   double eps, freq, gamma, delta;
@@ -155,6 +166,9 @@ void material_E(C& c, const Mat& m, double dt) {
       u(c);
       break;
     }
+    case 7: (FluidDrudeUpdate<Mode, FieldType::Electric>(dt))(c); break;
+    case 8: MagnetizedFluidDrudeUpdate<Mode, FieldType::Electric>(m.bx, m.by, m.bz, dt)(c); break;
+    case 9: (SelfMagnetizedFluidDrudeUpdate<Mode, FieldType::Electric>(dt))(c); break;
   }
 }
 
@@ -332,6 +346,7 @@ struct TMScenario {
   bool pmlx, pmly;
   std::vector<Mat> mats;
   std::vector<int> matid;  // nx*ny
+  std::vector<double> ne, nu;  // fluid density / collision rate per cell (empty: none)
   int src_i, src_j;
   std::vector<double> src;  // per step, added to Dz before the PML/material sweep
   std::vector<std::pair<int, int>> probes;
@@ -347,6 +362,10 @@ static void run_tm(const TMScenario& S, const std::string& dir) {
   const int nx = S.nx, ny = S.ny;
   std::vector<CellTM> cells(size_t(nx) * ny);
   auto at = [&](int i, int j) -> CellTM& { return cells[size_t(i) + size_t(nx) * j]; };
+  for (size_t q = 0; q < S.ne.size(); q++) {
+    cells[q].ne() = S.ne[q];
+    cells[q].nu_m() = S.nu[q];
+  }
   for (int j = 1; j < ny - 1; j++)
     for (int i = 1; i < nx - 1; i++) {
       CellTM& c = at(i, j);
@@ -478,6 +497,19 @@ static void run_tm(const TMScenario& S, const std::string& dir) {
     mt.push_back(m.delta);
   }
   fx.arr("mats_kind_eps_freq_gamma_delta", mt, {S.mats.size(), 5});
+  {
+    bool any_bias = false;
+    std::vector<double> bias;
+    for (auto& m : S.mats) {
+      any_bias = any_bias || m.kind >= 5;
+      bias.insert(bias.end(), {m.bx, m.by, m.bz});
+    }
+    if (any_bias) fx.arr("mats_bias", bias, {S.mats.size(), 3});
+  }
+  if (!S.ne.empty()) {
+    fx.arr("fluid_ne", S.ne, {S.ne.size()});
+    fx.arr("fluid_nu", S.nu, {S.nu.size()});
+  }
   std::vector<double> mid(S.matid.begin(), S.matid.end());
   if (S.save_full) fx.arr("matid", mid, {size_t(ny), size_t(nx)});
   if (S.src_i >= 0) fx.arr("src", S.src, {S.src.size()});
@@ -535,6 +567,7 @@ struct Scenario3 {
   PmlSpec pml;
   std::vector<Mat> mats;
   std::vector<int> matid;  // nx*ny*nz
+  std::vector<double> ne, nu;  // fluid density / collision rate per cell (empty: none)
   int si, sj, sk;
   std::vector<double> src;
   std::vector<std::array<int, 3>> probes;
@@ -545,6 +578,10 @@ static void run_3d(const Scenario3& S, const std::string& dir, bool timing_only 
   const int nx = S.nx, ny = S.ny, nz = S.nz;
   std::vector<Cell3> cells(size_t(nx) * ny * nz);
   auto id = [&](int i, int j, int k) { return size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k); };
+  for (size_t q = 0; q < S.ne.size(); q++) {
+    cells[q].ne() = S.ne[q];
+    cells[q].nu_m() = S.nu[q];
+  }
   auto at = [&](int i, int j, int k) -> Cell3& { return cells[id(i, j, k)]; };
   for (int k = 1; k < nz - 1; k++)
     for (int j = 1; j < ny - 1; j++)
@@ -645,6 +682,10 @@ static void run_3d(const Scenario3& S, const std::string& dir, bool timing_only 
     mt.push_back(m.delta);
   }
   fx.arr("mats_kind_eps_freq_gamma_delta", mt, {S.mats.size(), 5});
+  if (!S.ne.empty()) {
+    fx.arr("fluid_ne", S.ne, {S.ne.size()});
+    fx.arr("fluid_nu", S.nu, {S.nu.size()});
+  }
   bool any_bias = false;
   std::vector<double> bias;
   for (auto& m : S.mats) {
@@ -728,6 +769,9 @@ static void scen_material_map(const std::string& dir) {
   self.Freq() = 1.5e11;
   self.Gamma() = 2e9;
   mm.insert(std::make_pair(std::string("self plasma"), make_material_pair<CellTM&>(self, VacuumUpdate<TM, FieldType::Magnetic>())));
+  mm.insert(std::make_pair(std::string("fluid plasma"), make_material_pair<CellTM&>(FluidDrudeUpdate<TM, FieldType::Electric>(dt), VacuumUpdate<TM, FieldType::Magnetic>())));
+  mm.insert(std::make_pair(std::string("biased fluid"), make_material_pair<CellTM&>(MagnetizedFluidDrudeUpdate<TM, FieldType::Electric>(0.1, 0.2, 0.9, dt), ConstantUpdate<TM, FieldType::Magnetic>(1.0))));
+  mm.insert(std::make_pair(std::string("self fluid"), make_material_pair<CellTM&>(SelfMagnetizedFluidDrudeUpdate<TM, FieldType::Electric>(dt), VacuumUpdate<TM, FieldType::Magnetic>())));
   std::ostringstream os;
   std::streambuf* old = std::cout.rdbuf(os.rdbuf());
   mm.print_summary(os, 0);
@@ -1285,6 +1329,135 @@ int main(int argc, char** argv) {
     for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * 3e9 * gauss_pulse(n, 14, 5));
     S.probes = {{10, 9, 8}, {8, 8, 8}, {12, 8, 9}, {7, 13, 10}};
     run_3d(S, dir);
+  }
+  // fluid Drude family (include/DispersiveMaterials.hpp:1160-1214, 1728-1893):
+  // omega_p = sqrt(q^2/(m eps0) ne) ~ 2e11 rad/s at ne ~ 1.26e19 / m^3
+  {
+    Fixture fx;
+    fx.name = "cell_fluid";
+    const double dt = 1e-12;
+    const int nsteps = 200;
+    std::vector<Mat> ms = {{7, 1, 0, 0, 0}, {8, 1, 0, 0, 0, 0.6, -1.0, 1.6}, {9, 1, 0, 0, 0}};
+    std::vector<double> out, mt;
+    for (auto& m : ms) {
+      Cell3 c;
+      for (int n = 0; n < nsteps; n++) {
+        c.Dx() = eps0 * (std::sin(0.07 * n) + 0.3 * std::cos(0.011 * n * n));
+        c.Dy() = eps0 * (0.5 * std::cos(0.05 * n + 0.3) - 0.2 * std::sin(0.013 * n * n));
+        c.Dz() = eps0 * (0.8 * std::sin(0.031 * n + 1.1));
+        c.Bx() = 0.7 * std::sin(0.02 * n);
+        c.By() = -1.2 + 0.4 * std::cos(0.045 * n);
+        c.Bz() = 1.5 * std::cos(0.017 * n + 0.4);
+        c.ne() = 1.26e19 * (1.0 + 0.5 * std::sin(0.03 * n));
+        c.nu_m() = 1e9 * (1.0 + 0.3 * std::cos(0.05 * n));
+        material_E<ThreeD>(c, m, dt);
+        out.insert(out.end(), {c.Dx(), c.Dy(), c.Dz(), c.Ex(), c.Ey(), c.Ez(), c.Px(), c.Py(), c.Pz(), c.Jx(),
+                               c.Jy(), c.Jz(), c.Bx(), c.By(), c.Bz(), c.ne(), c.nu_m()});
+      }
+      mt.insert(mt.end(), {double(m.kind), m.bx, m.by, m.bz});
+    }
+    fx.p("dt", dt);
+    fx.pi("nsteps", nsteps);
+    fx.arr("mats_kind_bx_by_bz", mt, {ms.size(), 4});
+    fx.arr("trace_D3_E3_P3_J3_B3_ne_nu", out, {ms.size(), size_t(nsteps), 17});
+    fx.write(dir);
+  }
+  {
+    Fixture fx;
+    fx.name = "cell_fluid_tm";
+    const double dt = 1e-12;
+    const int nsteps = 200;
+    std::vector<Mat> ms = {{7, 1, 0, 0, 0}, {8, 1, 0, 0, 0, 0.6, -1.0, 1.6}, {9, 1, 0, 0, 0}};
+    std::vector<double> out, mt;
+    for (auto& m : ms) {
+      CellTM c;
+      for (int n = 0; n < nsteps; n++) {
+        c.Dz() = eps0 * (std::sin(0.07 * n) + 0.3 * std::cos(0.011 * n * n));
+        c.ne() = 1.26e19 * (1.0 + 0.5 * std::sin(0.03 * n));
+        c.nu_m() = 1e9 * (1.0 + 0.3 * std::cos(0.05 * n));
+        material_E<TM>(c, m, dt);
+        out.insert(out.end(), {c.Dz(), c.Ez(), c.Pz(), c.Jz(), c.ne(), c.nu_m()});
+      }
+      mt.insert(mt.end(), {double(m.kind), m.bx, m.by, m.bz});
+    }
+    fx.p("dt", dt);
+    fx.pi("nsteps", nsteps);
+    fx.arr("mats_kind_bx_by_bz", mt, {ms.size(), 4});
+    fx.arr("trace_D_E_P_J_ne_nu", out, {ms.size(), size_t(nsteps), 6});
+    fx.write(dir);
+  }
+  // 3-D grid with the three fluid forms on a graded density, CPML, strong Dz source
+  {
+    Scenario3 S;
+    S.name = "grid3d_fluid";
+    S.nx = 20;
+    S.ny = 18;
+    S.nz = 17;
+    S.nsteps = 70;
+    S.dx = 1e-3;
+    S.dt = 0.5 * S.dx / (c0 * std::sqrt(3.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {4, 3, 1, dm.msMax, 5, 0.05};
+    S.mats = {{0, 1, 0, 0, 0}, {7, 1, 0, 0, 0}, {8, 1, 0, 0, 0, 0.3, -0.4, 1.2}, {9, 1, 0, 0, 0}};
+    const size_t ncell = size_t(S.nx) * S.ny * S.nz;
+    S.matid.assign(ncell, 0);
+    S.ne.assign(ncell, 0.0);
+    S.nu.assign(ncell, 0.0);
+    for (int k = 0; k < S.nz; k++)
+      for (int j = 0; j < S.ny; j++)
+        for (int i = 0; i < S.nx; i++) {
+          const size_t q = size_t(i) + size_t(S.nx) * (size_t(j) + size_t(S.ny) * k);
+          S.ne[q] = 1.26e19 * (0.4 + 0.06 * i + 0.01 * j * (k % 4));
+          S.nu[q] = 1e9 * (1.0 + 0.25 * ((i + 2 * j + 3 * k) % 5));
+        }
+    auto fill = [&](int x0, int x1, int y0, int y1, int z0, int z1, int m) {
+      for (int k = z0; k < z1; k++)
+        for (int j = y0; j < y1; j++)
+          for (int i = x0; i < x1; i++) S.matid[size_t(i) + size_t(S.nx) * (size_t(j) + size_t(S.ny) * k)] = m;
+    };
+    fill(6, 10, 6, 12, 6, 10, 1);
+    fill(11, 14, 6, 10, 8, 11, 2);
+    fill(6, 9, 12, 14, 9, 12, 3);
+    S.si = 10;
+    S.sj = 9;
+    S.sk = 8;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * 3e9 * gauss_pulse(n, 14, 5));
+    S.probes = {{10, 9, 8}, {8, 8, 8}, {12, 8, 9}, {7, 13, 10}};
+    run_3d(S, dir);
+  }
+  // 2D TM with the fluid forms (all the Drude update per component in TM)
+  {
+    TMScenario S;
+    S.name = "tm2d_fluid";
+    S.nx = 40;
+    S.ny = 36;
+    S.nsteps = 120;
+    S.dx = 1e-3;
+    S.dt = 0.9 * S.dx / (c0 * std::sqrt(2.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {8, 3, 1, dm.msMax, 5, 0.05};
+    S.pmlx = S.pmly = true;
+    S.mats = {{0, 1, 0, 0, 0}, {7, 1, 0, 0, 0}, {8, 1, 0, 0, 0, 0.3, -0.4, 1.2}, {9, 1, 0, 0, 0}};
+    S.matid.assign(size_t(S.nx) * S.ny, 0);
+    S.ne.assign(size_t(S.nx) * S.ny, 0.0);
+    S.nu.assign(size_t(S.nx) * S.ny, 0.0);
+    for (int j = 0; j < S.ny; j++)
+      for (int i = 0; i < S.nx; i++) {
+        S.ne[size_t(i) + size_t(S.nx) * j] = 1.26e19 * (0.3 + 0.05 * i + 0.02 * (j % 7));
+        S.nu[size_t(i) + size_t(S.nx) * j] = 2e9 * (1.0 + 0.5 * ((i * j) % 3));
+      }
+    auto fill = [&](int x0, int x1, int y0, int y1, int m) {
+      for (int j = y0; j < y1; j++)
+        for (int i = x0; i < x1; i++) S.matid[size_t(i) + size_t(S.nx) * j] = m;
+    };
+    fill(12, 18, 11, 17, 1);
+    fill(22, 28, 19, 25, 2);
+    fill(12, 17, 21, 26, 3);
+    S.src_i = 20;
+    S.src_j = 18;
+    for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 20, 6));
+    S.probes = {{20, 18}, {15, 14}, {25, 22}, {10, 28}};
+    run_tm(S, dir);
   }
   // 2D TM with conductive blocks next to a dielectric one, CPML x+y, Dz source
   {

@@ -220,6 +220,7 @@ typedef struct {
   double freq[RC_MAXPOLES], gamma[RC_MAXPOLES], weight[RC_MAXPOLES];
   int mag;        /* magnetized Drude: 0 none, 1 fixed bias, 2 the cell's own B */
   double bias[3]; /* [T] */
+  int fluid;      /* fluid Drude: omega_p, gamma from the cell's ne, nu_m */
 } rc_mat;
 
 typedef struct {
@@ -279,6 +280,7 @@ struct rc_sim {
   int obj_dft, obj_mask;
   /* boundary condition per [axis][side] (RC_BC_*; 0: ghost layer left alone) */
   int bc[3][2];
+  double *fluid_ne, *fluid_nu; /* per cell, NULL until rc_set_fluid_state */
 };
 
 static size_t cidx(const rc_sim* s, int i, int j, int k) {
@@ -391,6 +393,8 @@ void rc_destroy(rc_sim* s) {
   for (int ft = 0; ft < 2; ft++)
     for (int I = 0; I < 3; I++)
       for (int J = 0; J < 3; J++) free(s->Pb[ft][I][J]);
+  free(s->fluid_ne);
+  free(s->fluid_nu);
   free(s);
 }
 
@@ -408,6 +412,7 @@ int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, co
   m->mu = mu_r;
   m->np = npoles;
   m->mag = 0;
+  m->fluid = 0;
   for (int p = 0; p < npoles; p++) {
     m->kind[p] = (int)poles[4 * p];
     m->freq[p] = poles[4 * p + 1];
@@ -435,19 +440,56 @@ int rc_set_magnetized_drude(rc_sim* s, int id, double eps_r, double mu_r, double
   return 0;
 }
 
+int rc_set_fluid_drude(rc_sim* s, int id, double mu_r, int magnetization, const double* bias) {
+  const double pole[4] = {1.0, 0.0, 0.0, 1.0};
+  if (magnetization < 0 || magnetization > 2 || (magnetization == 1 && !bias)) return -1;
+  if (rc_set_material(s, id, 1.0, mu_r, 1, pole)) return -1;
+  s->mat[id].fluid = 1;
+  s->mat[id].mag = magnetization;
+  for (int c = 0; c < 3; c++) s->mat[id].bias[c] = magnetization == 1 ? bias[c] : 0.0;
+  return 0;
+}
+
+int rc_set_fluid_state(rc_sim* s, const double* ne, const double* nu_m) {
+  if (!s->fluid_ne) {
+    s->fluid_ne = malloc(s->ncell * sizeof(double));
+    s->fluid_nu = malloc(s->ncell * sizeof(double));
+  }
+  memcpy(s->fluid_ne, ne, s->ncell * sizeof(double));
+  memcpy(s->fluid_nu, nu_m, s->ncell * sizeof(double));
+  return 0;
+}
+
+/* omega_p and gamma of material m at cell c: the fluid storages' getFreq =
+ * Kval sqrt(ne), Kval = sqrt(q_e^2/(m_e eps0)) by the reference's constexpr
+ * Newton iteration, and getGamma = nu_m (DispersiveMaterials.hpp:1160-1170) */
+static void plasma_params(const rc_sim* s, const rc_mat* m, size_t c, double* freq, double* gamma) {
+  if (m->fluid) {
+    const double kval = rc_newton_sqrt(1.6e-19 * 1.6e-19 / (9.11e-31 * RC_EPS0));
+    *freq = kval * sqrt(s->fluid_ne[c]);
+    *gamma = s->fluid_nu[c];
+  } else {
+    *freq = m->freq[0];
+    *gamma = m->gamma[0];
+  }
+}
+
 /* MagnetizedDrudeUpdate3D::implicit_update (DispersiveMaterials.hpp:1283-1340),
- * in its own operation order; B is the bias (MagnetizedDrudeStorage) or the
- * cell's own B (SelfMagnetizedDrudeStorage, 1898-1943). */
+ * in its own operation order; B is the bias (MagnetizedDrudeStorage,
+ * MagnetizedFluidDrudeStorage) or the cell's own B (the self-magnetized
+ * storages, 1816-1943). */
 static void magnetized_drude_3d(rc_sim* s, const rc_mat* m, size_t c, double dt) {
   const double qe = 1.6e-19, me = 9.11e-31; /* drude::q_e, drude::m_e (899-900) */
+  double freq, gamma;
+  plasma_params(s, m, c, &freq, &gamma);
   const double Bx = m->mag == 1 ? m->bias[0] : s->B[0][c];
   const double By = m->mag == 1 ? m->bias[1] : s->B[1][c];
   const double Bz = m->mag == 1 ? m->bias[2] : s->B[2][c];
   double *Dx = &s->D[0][c], *Dy = &s->D[1][c], *Dz = &s->D[2][c];
   double *Px = &s->P[0][0][c], *Py = &s->P[0][1][c], *Pz = &s->P[0][2][c];
   double *Jx = &s->J[0][0][c], *Jy = &s->J[0][1][c], *Jz = &s->J[0][2][c];
-  const double wp = dt * m->freq[0];
-  const double vc = dt * m->gamma[0];
+  const double wp = dt * freq;
+  const double vc = dt * gamma;
   const double vx = dt * Bx * qe / me;
   const double vy = dt * By * qe / me;
   const double vz = dt * Bz * qe / me;
@@ -852,8 +894,10 @@ int rc_step(rc_sim* s, int nsteps) {
                 const double wgt_eps0 = m->weight[p] * RC_EPS0;
                 if (m->kind[p] == 3) continue; /* conductive: J untouched */
                 if (m->kind[p] == 1) {
-                  double b = m->gamma[p] * dt * 0.5;
-                  double w = dt * m->freq[p];
+                  double fq = m->freq[p], gm = m->gamma[p];
+                  if (m->fluid) plasma_params(s, m, c, &fq, &gm);
+                  double b = gm * dt * 0.5;
+                  double w = dt * fq;
                   *Jp = *Jp * (1.0 - b) / (1.0 + b) + w * w / dt * wgt_eps0 * Ev;
                 } else {
                   double b = 1.0 + m->gamma[p] * dt;

@@ -69,6 +69,10 @@ int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, co
  * in 3-D the coupled update, otherwise the Drude pole {1, freq, gamma, 1} */
 int rc_set_magnetized_drude(rc_sim* s, int id, double eps_r, double mu_r, double freq, double gamma,
                             const double* bias);
+/* fluid Drude material (K = 1; magnetization 0 none, 1 bias, 2 the cell's own
+ * B) and the per-cell electron density / collision rate it reads */
+int rc_set_fluid_drude(rc_sim* s, int id, double mu_r, int magnetization, const double* bias);
+int rc_set_fluid_state(rc_sim* s, const double* ne, const double* nu_m);
 int rc_set_matid(rc_sim* s, const uint8_t* ids); /* nx*ny*nz, x fastest */
 /* soft source: adds table[n] to D_comp (field 0) or B_comp (field 1) at cell after the curl of step n */
 int rc_add_source(rc_sim* s, int field, int comp, int i, int j, int k, const double* table, int ntable);

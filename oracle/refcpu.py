@@ -43,6 +43,8 @@ def lib():
             ("rc_set_pml_axis", _i, [_p, _i, _i, _dp]),
             ("rc_set_material", _i, [_p, _i, _d, _d, _i, _dp]),
             ("rc_set_magnetized_drude", _i, [_p, _i, _d, _d, _d, _d, _dp]),
+            ("rc_set_fluid_drude", _i, [_p, _i, _d, _i, _dp]),
+            ("rc_set_fluid_state", _i, [_p, _dp, _dp]),
             ("rc_set_matid", _i, [_p, ctypes.c_char_p]),
             ("rc_add_source", _i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
             ("rc_set_tfsf", _i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
@@ -164,6 +166,15 @@ class Sim:
         """bias: the fixed field [T] (3 values), or None for the cell's own B."""
         b = None if bias is None else _f64(np.asarray(bias, dtype=np.float64).reshape(3))
         assert lib().rc_set_magnetized_drude(self.h, mid, eps_r, mu_r, freq, gamma, None if b is None else _dptr(b)) == 0
+
+    def set_fluid_drude(self, mid, mu_r=1.0, magnetization=0, bias=None):
+        b = None if bias is None else _f64(np.asarray(bias, dtype=np.float64).reshape(3))
+        assert lib().rc_set_fluid_drude(self.h, mid, mu_r, magnetization, None if b is None else _dptr(b)) == 0
+
+    def set_fluid_state(self, ne, nu_m):
+        ne, nu_m = _f64(np.asarray(ne).reshape(-1)), _f64(np.asarray(nu_m).reshape(-1))
+        assert ne.size == nu_m.size == int(np.prod(self.shape))
+        assert lib().rc_set_fluid_state(self.h, _dptr(ne), _dptr(nu_m)) == 0
 
     def set_matid(self, ids):
         ids = np.ascontiguousarray(ids, dtype=np.uint8).reshape(-1)

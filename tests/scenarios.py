@@ -43,11 +43,15 @@ class SplitMix64:
 def ref_material(kind, eps, freq, gamma, delta, bias=None):
     """Map a reference material (kind 0 vacuum, 1 constant, 2 Drude, 3 Lorentz,
     4 conductive with freq = the conductivity, 5 magnetized Drude with `bias`,
-    6 self-magnetized Drude) to (eps_r, mu_r, poles), or for kinds 5 and 6 to
-    (eps_r, mu_r, poles, bias) with bias None for the cell's own B.  The
-    reference's Lorentz update never reads its delta
-    (include/DispersiveMaterials.hpp:711-735), so its pole weight is 1."""
+    6 self-magnetized Drude, 7 / 8 / 9 fluid Drude without magnetization / in
+    `bias` / self-magnetized) to (eps_r, mu_r, poles), for kinds 5 and 6 to
+    (eps_r, mu_r, poles, bias) with bias None for the cell's own B, for kinds
+    7-9 to {"fluid": magnetization, "mu": 1, "bias": ...}.  The reference's
+    Lorentz update never reads its delta (include/DispersiveMaterials.hpp:711-735),
+    so its pole weight is 1."""
     kind = int(kind)
+    if kind in (7, 8, 9):
+        return {"fluid": kind - 7, "mu": 1.0, "bias": list(bias) if kind == 8 else None}
     if kind in (5, 6):
         return (eps, 1.0, [(1, freq, gamma, 1.0)], list(bias) if kind == 5 else None)
     if kind == 4:
@@ -63,7 +67,9 @@ def ref_material(kind, eps, freq, gamma, delta, bias=None):
 
 def set_ref_material(sim, mid, spec):
     """Apply a ref_material spec to a Grid or an oracle Sim."""
-    if len(spec) == 4:
+    if isinstance(spec, dict):
+        sim.set_fluid_drude(mid, spec["mu"], spec["fluid"], spec["bias"])
+    elif len(spec) == 4:
         eps, mu, poles, bias = spec
         sim.set_magnetized_drude(mid, eps, mu, poles[0][1], poles[0][2], bias)
     else:
@@ -90,6 +96,8 @@ def from_golden(name):
         sc["matid"] = a["matid"].reshape(shape).astype(np.uint8)
     else:
         sc["matid"] = np.zeros(shape, np.uint8)
+    if "fluid_ne" in a:
+        sc["fluid"] = (a["fluid_ne"].reshape(shape), a["fluid_nu"].reshape(shape))
     if int(p.get("src_i", -1)) >= 0:
         sc["sources"].append((D, 2, int(p["src_i"]), int(p["src_j"]), int(p.get("src_k", 0)), a["src"]))
     if "probes" in a:
@@ -123,6 +131,8 @@ def apply(sim, sc):
     for mid, spec in enumerate(sc["materials"]):
         set_ref_material(sim, mid, spec)
     sim.set_matid(sc["matid"])
+    if sc.get("fluid") is not None:
+        sim.set_fluid_state(*sc["fluid"])
     for s in sc["sources"]:
         sim.add_source(*s)
     if sc.get("tfsf"):

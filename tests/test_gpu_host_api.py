@@ -50,18 +50,20 @@ def test_boundaries_through_cpp_host_surface():
     assert rel_l2(got, ref) <= 1e-12
 
 
-def test_magnetized_plasma_through_cpp_host_surface():
-    """MagnetizedDrudeUpdate / SelfMagnetizedDrudeUpdate in a MaterialMap on a
-    3-D Stepper: probe traces equal the reference's own run of the scenario
-    (golden fixture grid3d_magnetized) to 1e-12."""
-    exe = os.path.join(os.path.dirname(BIN), "grid3d_magnetized_host_api")
+@pytest.mark.parametrize("name", ["grid3d_magnetized", "grid3d_fluid"])
+def test_plasmas_through_cpp_host_surface(name):
+    """MagnetizedDrudeUpdate / SelfMagnetizedDrudeUpdate, and the fluid forms
+    with YeeGrid::setFluidState, in a MaterialMap on a 3-D Stepper: probe traces
+    equal the reference's own run of the scenario (the golden fixture of the
+    same name) to 1e-12."""
+    exe = os.path.join(os.path.dirname(BIN), name + "_host_api")
     if not os.path.exists(exe):
-        pytest.fail("tests/cpp/bin/grid3d_magnetized_host_api not built (run __graft_entry__.build())")
-    print("[gpu-case] grid3d_magnetized_host_api", flush=True)
+        pytest.fail(f"tests/cpp/bin/{name}_host_api not built (run __graft_entry__.build())")
+    print(f"[gpu-case] {name}_host_api", flush=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
-    p, a = load("grid3d_magnetized")
+    p, a = load(name)
     ref = a["probe_trace_E_H"].reshape(got.shape)
     assert np.abs(ref).max() > 0
     assert rel_l2(got, ref) <= 1e-12

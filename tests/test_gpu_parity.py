@@ -50,7 +50,7 @@ def field_errors(g, o, sc):
 
 
 GOLDEN = ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small", "grid3d_disp", "tm2d_c1",
-          "tm2d_conductive", "grid3d_magnetized"]
+          "tm2d_conductive", "grid3d_magnetized", "grid3d_fluid", "tm2d_fluid"]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -343,3 +343,29 @@ def test_3d_magnetized_schedules_and_errors():
     other = apply(Grid(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"], precision="f32"), sc)
     with pytest.raises(FdtdError, match="magnetized"):
         Batch([ref, other])
+
+
+def test_fluid_state_changes_between_steps():
+    """A fluid Drude grid whose density and collision rate change between steps
+    (a coupled fluid model) follows the oracle doing the same (FP64 1e-12,
+    FP32 1e-5); a fluid material without a fluid state is refused."""
+    from fdtd_amd import FdtdError, Grid
+    sc, p, a = from_golden("grid3d_fluid")
+    ne, nu = sc["fluid"]
+    for precision in ("f64", "f32"):
+        print(f"[gpu-case] fluid state update {precision}", flush=True)
+        g = apply(Grid(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"], precision=precision), sc)
+        o = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+        for sim in (g, o):
+            sim.step(30)
+            sim.set_fluid_state(1.5 * ne, 0.5 * nu)
+            sim.step(40)
+        for f in (E, H, P, J):
+            for c in range(3):
+                assert rel_l2(g.get(f, c), o.get(f, c)) <= TOL[precision] * (10 if f in (P, J) else 1), (f, c)
+        g.close()
+    g = Grid(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"], precision="f32")
+    g.set_fluid_drude(1, 1.0, 0, None)
+    g.set_matid(sc["matid"])
+    with pytest.raises(FdtdError, match="fluid"):
+        g.step(1)

@@ -20,8 +20,8 @@ XML = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "materi
 
 def test_reads_the_reference_print_summary():
     mm = MaterialMap.read_xml(XML)
-    assert sorted(mm) == ["biased plasma", "conductor", "ferrite", "glass", "resonator", "self plasma", "silver",
-                          "vacuum"]
+    assert sorted(mm) == ["biased fluid", "biased plasma", "conductor", "ferrite", "fluid plasma", "glass",
+                          "resonator", "self fluid", "self plasma", "silver", "vacuum"]
     assert mm["vacuum"].spec() == (1.0, 1.0, [])
     assert mm["glass"].spec() == (2.25, 1.0, [])
     assert mm["ferrite"].spec() == (4.5, 3.2, [])
@@ -32,6 +32,9 @@ def test_reads_the_reference_print_summary():
     assert mm["resonator"].spec() == (1.5, 1.0, [(2, 4.1e15, 1.2e14, 1.0)])
     assert mm["biased plasma"].spec() == (1.0, 1.0, [(1, 2e11, 1e9, 1.0)], (0.3, -0.4, 1.25))
     assert mm["self plasma"].spec() == (1.5, 1.0, [(1, 1.5e11, 2e9, 1.0)], None)
+    assert mm["fluid plasma"].spec() == {"fluid": 0, "mu": 1.0, "bias": None}
+    assert mm["biased fluid"].spec() == {"fluid": 1, "mu": 1.0, "bias": (0.1, 0.2, 0.9)}
+    assert mm["self fluid"].spec() == {"fluid": 2, "mu": 1.0, "bias": None}
 
 
 def test_writes_the_reference_text():
@@ -40,7 +43,7 @@ def test_writes_the_reference_text():
 
 
 def test_unsupported_models_raise():
-    mm = MaterialMap(x=MaterialPair(Dispersion("FluidDrude"), Dispersion("Vacuum")),
+    mm = MaterialMap(x=MaterialPair(Dispersion("Anisotropic"), Dispersion("Vacuum")),
                      y=MaterialPair(Dispersion("Constant", K=2.0), Dispersion("Drude", 1.0, 1e15, 1e13)))
     with pytest.raises(NotImplementedError):
         mm["x"].spec()

@@ -99,8 +99,56 @@ def test_cell_magnetized_3d():
     assert np.abs(tr[0, :, 9] - tr[2, :, 9]).max() > 0
 
 
+def test_cell_fluid():
+    """The fluid Drude forms (FluidDrudeUpdate, MagnetizedFluidDrudeUpdate,
+    SelfMagnetizedFluidDrudeUpdate; DispersiveMaterials.hpp:1160-1214, 1728-2028)
+    on one 3-D cell and one TM cell with prescribed D_n, B_n and a density and
+    collision rate that change every step: E, P, J bit for bit."""
+    p, a = load("cell_fluid")
+    dt = p["dt"]
+    mats, tr = a["mats_kind_bx_by_bz"], a["trace_D3_E3_P3_J3_B3_ne_nu"]
+    for m, (kind, bx, by, bz) in enumerate(mats):
+        sim = refcpu.Sim("ThreeD", 3, 3, 3, 1.0, dt)
+        sim.set_fluid_drude(0, 1.0, int(kind) - 7, (bx, by, bz) if kind == 8 else None)
+        out = []
+        for n in range(tr.shape[1]):
+            row = tr[m, n]
+            ne, nu = np.zeros((3, 3, 3)), np.zeros((3, 3, 3))
+            ne[1, 1, 1], nu[1, 1, 1] = row[15], row[16]
+            sim.set_fluid_state(ne, nu)
+            for c in range(3):
+                d = np.zeros((3, 3, 3))
+                d[1, 1, 1] = row[c]
+                sim.set(D, c, d)
+                b = np.zeros((3, 3, 3))
+                b[1, 1, 1] = row[12 + c]
+                sim.set(B, c, b)
+                sim.set(H, c, np.zeros((3, 3, 3)))
+            sim.step(1)
+            out.append([sim.get(f, c)[1, 1, 1] for f in (E, 4, 5) for c in range(3)])
+        np.testing.assert_array_equal(np.array(out), tr[m, :, 3:12], err_msg=f"material {m}")
+    p, a = load("cell_fluid_tm")
+    mats, tr = a["mats_kind_bx_by_bz"], a["trace_D_E_P_J_ne_nu"]
+    for m, (kind, bx, by, bz) in enumerate(mats):
+        sim = refcpu.Sim("TM", 3, 3, 1, 1.0, p["dt"])
+        sim.set_fluid_drude(0, 1.0, int(kind) - 7, (bx, by, bz) if kind == 8 else None)
+        out = []
+        for n in range(tr.shape[1]):
+            ne, nu, d = np.zeros((1, 3, 3)), np.zeros((1, 3, 3)), np.zeros((1, 3, 3))
+            ne[0, 1, 1], nu[0, 1, 1], d[0, 1, 1] = tr[m, n, 4], tr[m, n, 5], tr[m, n, 0]
+            sim.set_fluid_state(ne, nu)
+            sim.set(D, 2, d)
+            for c in (0, 1):
+                sim.set(B, c, np.zeros((1, 3, 3)))
+                sim.set(H, c, np.zeros((1, 3, 3)))
+            sim.step(1)
+            out.append((sim.get(E, 2)[0, 1, 1], sim.get(4, 2)[0, 1, 1], sim.get(5, 2)[0, 1, 1]))
+        np.testing.assert_array_equal(np.array(out), tr[m, :, 1:4], err_msg=f"TM material {m}")
+
+
 @pytest.mark.parametrize("name", ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small",
-                                  "grid3d_disp", "tm2d_c1", "tm2d_conductive", "grid3d_magnetized"])
+                                  "grid3d_disp", "tm2d_c1", "tm2d_conductive", "grid3d_magnetized",
+                                  "grid3d_fluid", "tm2d_fluid"])
 def test_grid_scenarios_bit_exact(name):
     sc, p, a = from_golden(name)
     sim = refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])
