@@ -112,7 +112,7 @@ void pml_idiom_rows(int L, int n, double m, double ma, double smax, double kmax,
 constexpr int kChunk = 256;  // steps per incident-table chunk
 // timed kernels: 0 E pass, 1 H pass, 2 aux line, 3 fused pass, 4 DFT,
 // 5 adjoint E pass, 6 adjoint H pass
-constexpr int kNumKernels = 7;
+constexpr int kNumKernels = 8;
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
 // bx == 0 selects the LDS fused single pass with `by` rows per workgroup,
@@ -225,6 +225,7 @@ struct fdtd_grid {
     void* acc;
     void* smp;  // samples of the steps not yet folded into acc
     double *phE, *phH;
+    void *acc_im, *smp_im;  // the imaginary part's (complex grids)
   } dft[FDTD_MAX_DFT];
   long dft_done = 0;  // steps folded into the DFT accumulators
 
@@ -232,7 +233,13 @@ struct fdtd_grid {
   int nprobe = 0;
   fdtdk::ProbeDev probe[FDTD_MAX_PROBES];
   double* trace = nullptr;
+  double* trace_im = nullptr;  // the imaginary part's (complex grids)
   long trace_cap = 0;
+
+  // complex fields (a Bloch-periodic face): the imaginary part lives in the
+  // second copies of the stepped state (E2, H2, psi2, Dd2, Pd2, Jd2)
+  bool cplx = false;
+  double bloch[3][2][2] = {};  // phase factor (re, im) of E per [axis][side]
 
   // design box: per-cell eps_r replacing the material table's value there
   bool design = false;
@@ -486,6 +493,9 @@ int upload_coefs(fdtd_grid* g) {
   return 0;
 }
 
+int ensure_pingpong(fdtd_grid* g);
+int ensure_dft_im(fdtd_grid* g);
+
 // CPML slabs: psi is kept only where some C coefficient of the axis is non-zero
 // (elsewhere psi stays exactly zero in the reference as well).
 int finalize(fdtd_grid* g) {
@@ -572,6 +582,11 @@ int finalize(fdtd_grid* g) {
       }
       g->fluid_dirty = true;
     }
+  }
+  if (g->cplx) {
+    int rc = ensure_pingpong(g);  // the imaginary part's state
+    if (!rc) rc = ensure_dft_im(g);
+    if (rc) return rc;
   }
   g->finalized = true;
   return 0;
@@ -662,7 +677,7 @@ void bc_args(const fdtd_grid* g, fdtdk::StepArgs<T>& a) {
       a.bc_src[ax][sd] = -1;
       a.bc_off[ax][sd] = 0;
       a.bc_neg[0][ax][sd] = a.bc_neg[1][ax][sd] = 0;
-      if (!kind) continue;
+      if (!kind || kind == FDTD_BC_BLOCH) continue;  // Bloch faces: bloch_ghosts
       const int L = g->len[ax];
       const int src = kind == FDTD_BC_PERIODIC ? (sd ? 1 : L - 2) : (sd ? L - 2 : 1);
       const int dst = sd ? L - 1 : 0;
@@ -679,6 +694,65 @@ void bc_args(const fdtd_grid* g, fdtdk::StepArgs<T>& a) {
           if (neg) a.bc_neg[ft][ax][sd] |= 1 << q;
         }
     }
+}
+
+// The imaginary part's arguments (complex grids): the real part's with the
+// state, DFT and probe buffers of the imaginary part, and no sources or TFSF
+// (the excitation is real).
+template <typename T>
+void im_args(const fdtd_grid* g, const fdtdk::StepArgs<T>& re, fdtdk::StepArgs<T>& a) {
+  a = re;
+  for (int c = 0; c < 3; c++) {
+    gput(a.E[c], (T*)g->E2[c]);
+    gput(a.Eo[c], (T*)g->E2[c]);
+    gput(a.H[c], (T*)g->H2[c]);
+    gput(a.Ho[c], (T*)g->H2[c]);
+    gput(a.Dd[c], (T*)g->Dd2[c]);
+    gput(a.Ddo[c], (T*)g->Dd2[c]);
+    for (int p = 0; p < fdtdk::kMaxPoles; p++) {
+      gput(a.Pd[p][c], (T*)g->Pd2[p][c]);
+      gput(a.Pdo[p][c], (T*)g->Pd2[p][c]);
+      gput(a.Jd[p][c], (T*)g->Jd2[p][c]);
+      gput(a.Jdo[p][c], (T*)g->Jd2[p][c]);
+    }
+  }
+  for (int ft = 0; ft < 2; ft++)
+    for (int I = 0; I < 3; I++)
+      for (int J = 0; J < 3; J++) {
+        gput(a.psi[ft][I][J], (T*)g->psi2[ft][I][J]);
+        gput(a.psio[ft][I][J], (T*)g->psi2[ft][I][J]);
+      }
+  a.nsrc = 0;
+  a.tfsf = 0;
+  for (int q = 0; q < g->ndft; q++) {
+    gput(a.dft[q].acc, (T*)g->dft[q].acc_im);
+    gput(a.dft[q].smp, (T*)g->dft[q].smp_im);
+  }
+  gput(a.trace, g->trace_im);
+}
+
+// the Bloch faces of a complex grid
+fdtdk::BlochFaces bloch_faces(const fdtd_grid* g, long* maxcells) {
+  fdtdk::BlochFaces bf{};
+  bf.len[0] = g->d.nx;
+  bf.len[1] = g->d.ny;
+  bf.len[2] = g->ndim == 3 ? g->d.nz : 1;
+  *maxcells = 0;
+  for (int ax = 0; ax < g->ndim; ax++)
+    for (int sd = 0; sd < 2; sd++) {
+      if (g->bc[ax][sd] != FDTD_BC_BLOCH) continue;
+      const int L = g->len[ax];
+      fdtdk::BlochFace& f = bf.f[bf.n++];
+      f.axis = ax;
+      f.src = sd ? 1 : L - 2;  // the periodic image
+      f.dst = sd ? L - 1 : 0;
+      f.fr = g->bloch[ax][sd][0];
+      f.fi = g->bloch[ax][sd][1];
+      const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+      const long n = long(bf.len[t1] - 2) * (bf.len[t2] > 1 ? bf.len[t2] - 2 : 1);
+      *maxcells = std::max(*maxcells, n);
+    }
+  return bf;
 }
 
 template <typename T>
@@ -812,16 +886,32 @@ int launch(fdtd_grid* g, int kid, K kern, dim3 grid, dim3 block, const A&... arg
 int ensure_trace(fdtd_grid* g, long steps) {
   if (!g->nprobe || g->trace_cap >= steps) return 0;
   long cap = std::max<long>(steps, 2 * g->trace_cap);
-  double* nt = nullptr;
-  int rc = g->alloc((void**)&nt, size_t(cap) * g->nprobe * 8);
-  if (rc) return rc;
-  if (g->trace) {
-    HIP_TRY(hipMemcpyAsync(nt, g->trace, size_t(g->step) * g->nprobe * 8, hipMemcpyDeviceToDevice, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    g->release(g->trace);
+  for (int part = 0; part < (g->cplx ? 2 : 1); part++) {
+    double*& tr = part ? g->trace_im : g->trace;
+    double* nt = nullptr;
+    int rc = g->alloc((void**)&nt, size_t(cap) * g->nprobe * 8);
+    if (rc) return rc;
+    if (tr) {
+      HIP_TRY(hipMemcpyAsync(nt, tr, size_t(g->step) * g->nprobe * 8, hipMemcpyDeviceToDevice, g->stream));
+      HIP_TRY(hipStreamSynchronize(g->stream));
+      g->release(tr);
+    }
+    tr = nt;
   }
-  g->trace = nt;
   g->trace_cap = cap;
+  return 0;
+}
+
+// the imaginary part's DFT accumulators and sample rings (complex grids)
+int ensure_dft_im(fdtd_grid* g) {
+  if (!g->cplx) return 0;
+  for (int q = 0; q < g->ndft; q++) {
+    auto& d = g->dft[q];
+    if (d.acc_im) continue;
+    int rc = g->alloc(&d.acc_im, size_t(d.ncomp) * d.nfreq * 2 * d.ncells * g->esz);
+    if (!rc) rc = g->alloc(&d.smp_im, size_t(fdtdk::kDftBlock) * d.ncomp * d.ncells * g->esz);
+    if (rc) return rc;
+  }
   return 0;
 }
 
@@ -938,6 +1028,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
       set_state<T>(g, a, true);
       *host1 = a;
       swap_state(g);
+    } else if (g->cplx) {
+      im_args<T>(g, a, *host1);  // variant 1: the imaginary part
     }
     const auto* dev0 = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
     const auto* dev1 =
@@ -945,6 +1037,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
     HIP_TRY(hipMemcpyAsync((void*)dev0, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
     const long dft_work = host->dft_work, dft_cells = host->dft_cells;
     const long step0 = g->step;
+    long bloch_cells = 0;
+    const fdtdk::BlochFaces bfaces = bloch_faces(g, &bloch_cells);
     // the chunk's launches; every step-dependent value comes from the staged
     // arguments and the step index s, so the sequence can be captured once
     // and replayed for later chunks of the same shape
@@ -966,6 +1060,30 @@ int step_impl(fdtd_grid* g, int nsteps) {
             rc = launch(g, 3, fdtdk::fused_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
           else
             rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+        } else if (g->cplx) {
+          // complex fields: each pass over both parts, then the Bloch ghosts
+          // from both parts' new values
+          const int disp = g->max_np > 0 ? (has_magnetized(g) ? 2 : 1) : 0;
+          for (int ft = 0; ft < 2; ft++) {
+            for (int part = 0; part < 2; part++) {
+              const auto* p = part ? dev1 : dev0;
+              if (g->ndim == 2)
+                rc = ft ? launch(g, 1, fdtdk::h_pass_tm<T, true>, grid, block, p, 1, s)
+                        : launch(g, 0, fdtdk::e_pass_tm<T, true>, grid, block, p, 1, s);
+              else if (ft)
+                rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, p, t.kc, s, ntx, nty, nkc);
+              else if (disp == 2)
+                rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 2>, grid, block, p, t.kc, s, ntx, nty, nkc);
+              else if (disp)
+                rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 1>, grid, block, p, t.kc, s, ntx, nty, nkc);
+              else
+                rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 0>, grid, block, p, t.kc, s, ntx, nty, nkc);
+              if (rc) return rc;
+            }
+            rc = launch(g, 7, fdtdk::bloch_ghosts<T>, dim3(unsigned((bloch_cells + 255) / 256), unsigned(bfaces.n)),
+                        dim3(256), dev0, dev1, bfaces, ft);
+            if (rc) return rc;
+          }
         } else if (g->ndim == 3) {
           // instances that also write the boundary ghost layers (BC)
           // and the E-pass instance without the ADE code when no material is
@@ -995,10 +1113,11 @@ int step_impl(fdtd_grid* g, int nsteps) {
           rc = launch(g, 0, fdtdk::e_pass_tm<T>, grid, block, dev0, 1, s);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T>, grid, block, dev0, 1, s);
         }
-        if (!rc && dft_work > 0) {
-          rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), ap, s);
+        for (int part = 0; part < (g->cplx ? 2 : 1) && !rc && dft_work > 0; part++) {
+          const auto* dp = part ? dev1 : ap;
+          rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), dp, s);
           if (!rc && (step0 + s + 1) % fdtdk::kDftBlock == 0)
-            rc = launch(g, 4, fdtdk::dft_fold_step<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), ap, s);
+            rc = launch(g, 4, fdtdk::dft_fold_step<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), dp, s);
         }
         if (rc) return rc;
       }
@@ -1056,11 +1175,19 @@ int dft_flush_impl(fdtd_grid* g) {
   if (g->args_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->args_ev[slot]));
   auto* host = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
   *host = a;
+  if (g->cplx) {
+    auto* host1 = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+    im_args<T>(g, a, *host1);
+  }
   const auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
-  HIP_TRY(hipMemcpyAsync((void*)dev, host, sizeof(a), hipMemcpyHostToDevice, g->stream));
-  int rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((a.dft_work + 255) / 256)), dim3(256), dev, g->dft_done,
-                  g->step);
-  if (rc) return rc;
+  const auto* dev1 =
+      reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+  HIP_TRY(hipMemcpyAsync((void*)dev, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
+  for (int part = 0; part < (g->cplx ? 2 : 1); part++) {
+    int rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((a.dft_work + 255) / 256)), dim3(256), part ? dev1 : dev,
+                    g->dft_done, g->step);
+    if (rc) return rc;
+  }
   HIP_TRY(hipEventRecord(g->args_ev[slot], g->stream));
   g->args_ev_used[slot] = true;
   g->dft_done = g->step;
@@ -1694,7 +1821,7 @@ int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, co
   }
   if (idx) *idx = g->ndft;
   g->ndft++;
-  return 0;
+  return g->finalized ? ensure_dft_im(g) : 0;
 }
 
 int fdtd_get_dft(fdtd_grid* g, int idx, double* out) {
@@ -1703,7 +1830,21 @@ int fdtd_get_dft(fdtd_grid* g, int idx, double* out) {
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));
   const auto& d = g->dft[idx];
-  return download_T(g, out, d.acc, size_t(d.ncomp) * d.nfreq * 2 * d.ncells);
+  const size_t n = size_t(d.ncomp) * d.nfreq * 2 * d.ncells;
+  rc = download_T(g, out, d.acc, n);
+  if (rc || !g->cplx) return rc;
+  // complex field: F = F_re + i F_im, each [comp][freq][re, im][cell]
+  std::vector<double> fi(n);
+  if ((rc = download_T(g, fi.data(), d.acc_im, n))) return rc;
+  for (size_t b = 0; b < size_t(d.ncomp) * d.nfreq; b++) {
+    double* o = out + b * 2 * d.ncells;
+    const double* q = fi.data() + b * 2 * d.ncells;
+    for (long c = 0; c < d.ncells; c++) {
+      o[c] -= q[d.ncells + c];
+      o[d.ncells + c] += q[c];
+    }
+  }
+  return 0;
 }
 
 int fdtd_add_probe(fdtd_grid* g, int field, int comp, int i, int j, int k) {
@@ -1730,8 +1871,26 @@ int fdtd_get_probes(fdtd_grid* g, double* out, int max_steps, int* nsteps_out) {
   return 0;
 }
 
+int fdtd_get_probes_imag(fdtd_grid* g, double* out, int max_steps, int* nsteps_out) {
+  if (!g || !out) return fail(FDTD_EINVAL, "null argument");
+  if (!g->cplx) return fail(FDTD_EINVAL, "the grid has real fields (no Bloch-periodic face)");
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  long n = std::min<long>(g->step, max_steps);
+  if (n > 0 && g->nprobe) HIP_TRY(hipMemcpy(out, g->trace_im, size_t(n) * g->nprobe * 8, hipMemcpyDeviceToHost));
+  if (nsteps_out) *nsteps_out = int(n);
+  return 0;
+}
+
 int fdtd_upload_field(fdtd_grid* g, int field, int comp, int pole, const double* host) {
   if (!g || !host) return fail(FDTD_EINVAL, "null argument");
+  if (field & FDTD_IMAG) {
+    field &= ~FDTD_IMAG;
+    if (!g->cplx || !g->finalized) return fail(FDTD_ESTATE, "the imaginary part exists once a grid with a Bloch face is stepped");
+    if (!valid_comp(g, field, comp) || (field != FDTD_E && field != FDTD_H))
+      return fail(FDTD_EINVAL, "only E and H can be uploaded");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return upload_T(g, field == FDTD_E ? g->E2[comp] : g->H2[comp], host, g->ncell);
+  }
   if (!valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "component does not exist in this mode");
   HIP_TRY(hipStreamSynchronize(g->stream));
   if (field == FDTD_E) return upload_T(g, g->E[comp], host, g->ncell);
@@ -1741,6 +1900,18 @@ int fdtd_upload_field(fdtd_grid* g, int field, int comp, int pole, const double*
 
 int fdtd_download_field(fdtd_grid* g, int field, int comp, int pole, double* host) {
   if (!g || !host) return fail(FDTD_EINVAL, "null argument");
+  if (field & FDTD_IMAG) {
+    field &= ~FDTD_IMAG;
+    if (!g->cplx) return fail(FDTD_EINVAL, "the grid has real fields (no Bloch-periodic face)");
+    if (!valid_comp(g, field, comp) || (field != FDTD_E && field != FDTD_H))
+      return fail(FDTD_EINVAL, "the imaginary part is read for E and H");
+    if (!g->finalized) {
+      std::fill(host, host + g->ncell, 0.0);
+      return 0;
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return download_T(g, host, field == FDTD_E ? g->E2[comp] : g->H2[comp], g->ncell);
+  }
   if (!valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "component does not exist in this mode");
   HIP_TRY(hipStreamSynchronize(g->stream));
   if (field == FDTD_E) return download_T(g, host, g->E[comp], g->ncell);
@@ -1900,11 +2071,25 @@ int fdtd_set_boundary(fdtd_grid* g, int axis, int side, int kind) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
   if (axis < 0 || axis >= g->ndim || side < 0 || side > 1) return fail(FDTD_EINVAL, "bad axis or side");
   if (kind == FDTD_BC_BLOCH)
-    return fail(FDTD_EINVAL, "Bloch-periodic boundaries multiply by a complex phase: they need complex fields");
+    return fail(FDTD_EINVAL, "Bloch-periodic faces take a complex phase factor: use fdtd_set_bloch_boundary");
   if (kind < FDTD_BC_NONE || kind > FDTD_BC_PERIODIC) return fail(FDTD_EINVAL, "bad boundary kind");
   if (g->finalized) return fail(FDTD_ESTATE, "boundary conditions are fixed once stepping has started");
   g->nbc += (kind != FDTD_BC_NONE) - (g->bc[axis][side] != FDTD_BC_NONE);
   g->bc[axis][side] = kind;
+  g->cplx = false;
+  for (int a = 0; a < g->ndim; a++) g->cplx = g->cplx || g->bc[a][0] == FDTD_BC_BLOCH || g->bc[a][1] == FDTD_BC_BLOCH;
+  return 0;
+}
+
+int fdtd_set_bloch_boundary(fdtd_grid* g, int axis, int side, double factor_re, double factor_im) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (axis < 0 || axis >= g->ndim || side < 0 || side > 1) return fail(FDTD_EINVAL, "bad axis or side");
+  if (g->finalized) return fail(FDTD_ESTATE, "boundary conditions are fixed once stepping has started");
+  g->nbc += (g->bc[axis][side] == FDTD_BC_NONE);
+  g->bc[axis][side] = FDTD_BC_BLOCH;
+  g->bloch[axis][side][0] = factor_re;
+  g->bloch[axis][side][1] = factor_im;
+  g->cplx = true;
   return 0;
 }
 

@@ -949,6 +949,51 @@ __global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h
   h_pass_3d_body<float, true>(ap, kc, s, ntx, nty, nkc);
 }
 
+// ---------------------------------------------------------------- Bloch-periodic faces
+// A complex field is held as two real parts, each stepped by the passes with
+// its own arguments (re, im).  Every Bloch face writes its ghost layer from
+// the periodic image of both parts: ghost = f * image with f = fr + i fi for
+// E and conj(f) for H (BlochDecorator and make_bloch_periodic_boundary,
+// BoundaryUpdates.hpp:845-885; the mask of this kind is 1 for every
+// component).  One thread per tangential interior cell of a face (blockIdx.y
+// = face); run after both parts' E pass (ft 0) and after both H passes (ft 1).
+struct BlochFace {
+  int axis, src, dst;  // normal axis, image layer, ghost layer
+  double fr, fi;       // phase factor of E (H takes its conjugate)
+};
+struct BlochFaces {
+  int n;
+  BlochFace f[6];
+  int len[3];
+};
+template <typename T>
+__global__ __launch_bounds__(256) void bloch_ghosts(const StepArgs<T>* __restrict__ re,
+                                                    const StepArgs<T>* __restrict__ im, BlochFaces bf, int ft) {
+  const BlochFace fc = bf.f[blockIdx.y];
+  const int a = fc.axis, t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+  const int n1 = bf.len[t1] - 2, n2 = bf.len[t2] > 1 ? bf.len[t2] - 2 : 1;  // TM: one plane, k = 0
+  const long u = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (u >= long(n1) * n2) return;
+  int c[3];
+  c[t1] = 1 + int(u % n1);
+  c[t2] = bf.len[t2] > 1 ? 1 + int(u / n1) : 0;
+  c[a] = fc.src;
+  const long sy = bf.len[0], sz = long(bf.len[0]) * bf.len[1];
+  const long q = c[0] + sy * c[1] + sz * c[2];
+  c[a] = fc.dst;
+  const long d = c[0] + sy * c[1] + sz * c[2];
+  const T fr = T(fc.fr), fi = ft == 0 ? T(fc.fi) : T(-fc.fi);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    FDTD_G T* xr = ft == 0 ? re->E[k] : re->H[k];
+    FDTD_G T* xi = ft == 0 ? im->E[k] : im->H[k];
+    if (!xr) continue;
+    const T sr = xr[q], si = xi[q];
+    xr[d] = fr * sr - fi * si;
+    xi[d] = fi * sr + fr * si;
+  }
+}
+
 // ---------------------------------------------------------------- surface quantities
 // UpdateSurfaceChargeDensity (SurfaceQuantities.hpp:174-183), UpdateSurfaceCurrentE
 // (233-244; TM 258-266) and UpdateSurfaceCurrentH (293-304; TM 319-327) on the

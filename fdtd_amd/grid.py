@@ -18,6 +18,7 @@ TM, THREED = 2, 3
 F32, F64 = 0, 1
 D, E, B, H, P, J = 0, 1, 2, 3, 4, 5
 DRUDE, LORENTZ, CONDUCTIVE = 1, 2, 3
+IMAG = 16  # field-code flag: the imaginary part of a complex grid (FDTD_IMAG)
 # boundary kinds (fdtd_boundary in include/fdtd_hip.h; names of the reference's fdtd::Boundary)
 BOUNDARY = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "Periodic": 5, "BlochPeriodic": 6}
 
@@ -31,6 +32,7 @@ ABI_SYMBOLS = [
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
     "fdtd_set_design", "fdtd_adjoint_eps_gradient", "fdtd_adjoint_eps_gradient_batch",
     "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph", "fdtd_set_boundary",
+    "fdtd_set_bloch_boundary", "fdtd_get_probes_imag",
     "fdtd_surface_quantities",
 ]
 
@@ -73,6 +75,8 @@ def load_library():
         "fdtd_get_dft": (_i, [_p, _i, _dp]),
         "fdtd_add_probe": (_i, [_p, _i, _i, _i, _i, _i]),
         "fdtd_get_probes": (_i, [_p, _dp, _i, _ip]),
+        "fdtd_get_probes_imag": (_i, [_p, _dp, _i, _ip]),
+        "fdtd_set_bloch_boundary": (_i, [_p, _i, _i, _d, _d]),
         "fdtd_upload_field": (_i, [_p, _i, _i, _i, _dp]),
         "fdtd_download_field": (_i, [_p, _i, _i, _i, _dp]),
         "fdtd_step": (_i, [_p, _i]),
@@ -229,22 +233,25 @@ class Grid:
         _check(load_library().fdtd_add_probe(self.h, field, comp, i, j, k))
         self._nprobes += 1
 
-    def probes(self):
+    def probes(self, imag=False):
+        """[steps][probes]; imag: the imaginary part of a complex grid's probes."""
         n = self.step_count()
         out = np.zeros(max(1, n * self._nprobes))
         got = ctypes.c_int()
-        _check(load_library().fdtd_get_probes(self.h, _dptr(out), n, ctypes.byref(got)))
+        fn = load_library().fdtd_get_probes_imag if imag else load_library().fdtd_get_probes
+        _check(fn(self.h, _dptr(out), n, ctypes.byref(got)))
         return out[: got.value * self._nprobes].reshape(got.value, self._nprobes)
 
     # ---- fields
-    def get(self, field, comp, pole=0):
+    def get(self, field, comp, pole=0, imag=False):
+        """Field array (z, y, x); imag: the imaginary part of E or H of a complex grid."""
         out = np.zeros(int(np.prod(self.shape)))
-        _check(load_library().fdtd_download_field(self.h, field, comp, pole, _dptr(out)))
+        _check(load_library().fdtd_download_field(self.h, field | (IMAG if imag else 0), comp, pole, _dptr(out)))
         return out.reshape(self.shape)
 
-    def set(self, field, comp, arr, pole=0):
+    def set(self, field, comp, arr, pole=0, imag=False):
         a = _f64(arr).reshape(-1)
-        _check(load_library().fdtd_upload_field(self.h, field, comp, pole, _dptr(a)))
+        _check(load_library().fdtd_upload_field(self.h, field | (IMAG if imag else 0), comp, pole, _dptr(a)))
 
     # ---- stepping
     def step(self, n=1):
@@ -264,6 +271,12 @@ class Grid:
         fdtd::Boundary kinds) or its number; axis 0..2, side 0 (min) / 1 (max)."""
         k = BOUNDARY[kind] if isinstance(kind, str) else int(kind)
         _check(load_library().fdtd_set_boundary(self.h, int(axis), int(side), k))
+
+    def set_bloch_boundary(self, axis, side, factor):
+        """Bloch-periodic face with the complex phase factor `factor`: the grid's
+        fields become complex (make_bloch_periodic_boundary)."""
+        f = complex(factor)
+        _check(load_library().fdtd_set_bloch_boundary(self.h, int(axis), int(side), f.real, f.imag))
 
     def surface_quantities(self, region):
         """Surface charge density and currents of the current state on `region`

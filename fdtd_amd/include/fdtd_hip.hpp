@@ -23,6 +23,8 @@
 //   make_pec_boundary<Mode,Dir,Orient>(b,e)         make_pec_boundary<Mode,Dir,Orient>(grid)         BoundaryUpdates.hpp:527-565
 //     (pmc, symmetric, antisymmetric, periodic)     same names; periodic image fixed (layer L-2 / 1)  BoundaryUpdates.hpp:578-832
 //   BoundaryData::updateE() / updateH()             Stepper::boundary(bd): ghost writes after E / H   BoundaryUpdates.hpp:340-362
+//   make_bloch_periodic_boundary<Mode,Dir,Orient>(b,e,f,factor)  make_bloch_periodic_boundary<Mode,Dir,Orient>(grid, factor)
+//     (complex cells)                               complex fields; grid.download(field | FDTD_IMAG, ...)  BoundaryUpdates.hpp:845-885
 //
 // Instead of sweeping each functor over a user container, the functors are
 // handed once to a Stepper, which checks that they form the reference's step
@@ -33,6 +35,7 @@
 
 #include <array>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -480,6 +483,7 @@ struct BoundaryData {
   Boundary type;
   Dir dir;
   Orientation orient;
+  std::complex<double> factor = 1.0;  // BlochPeriodic: the phase factor of E (H takes its conjugate)
 };
 
 namespace detail {
@@ -490,7 +494,7 @@ inline int boundary_kind(Boundary b) {
     case Boundary::Symmetric: return FDTD_BC_SYMMETRIC;
     case Boundary::Antisymmetric: return FDTD_BC_ANTISYMMETRIC;
     case Boundary::Periodic: return FDTD_BC_PERIODIC;
-    case Boundary::BlochPeriodic: return FDTD_BC_BLOCH;  // rejected by the library: needs complex fields
+    case Boundary::BlochPeriodic: return FDTD_BC_BLOCH;  // set through fdtd_set_bloch_boundary
     case Boundary::NONE: return FDTD_BC_NONE;
     default: throw Error("Parallel boundaries (MPI halo exchange) have no single-GPU counterpart");
   }
@@ -507,6 +511,12 @@ template <class Mode, Dir d, Orientation o, class scalar>
 BoundaryData make_antisymmetric_boundary(YeeGrid<Mode, scalar>&) { return {Boundary::Antisymmetric, d, o}; }
 template <class Mode, Dir d, Orientation o, class scalar>
 BoundaryData make_periodic_boundary(YeeGrid<Mode, scalar>&) { return {Boundary::Periodic, d, o}; }
+// the grid's fields become complex (the reference needs cells with a complex
+// scalar for this face); the imaginary part is read with FDTD_IMAG
+template <class Mode, Dir d, Orientation o, class scalar>
+BoundaryData make_bloch_periodic_boundary(YeeGrid<Mode, scalar>&, std::complex<double> factor) {
+  return {Boundary::BlochPeriodic, d, o, factor};
+}
 
 // Per-face kinds, PEC everywhere by default as in the reference's
 // BoundaryOptions (include/BoundaryUpdates.hpp:44-69).
@@ -600,9 +610,14 @@ class Stepper {
   }
   void commit(int n, double unit) {
     g_.commit_pml();
-    for (auto& b : bcs_)
-      check(fdtd_set_boundary(g_.handle(), int(b.dir), int(b.orient), detail::boundary_kind(b.type)),
-            "fdtd_set_boundary");
+    for (auto& b : bcs_) {
+      if (b.type == Boundary::BlochPeriodic)
+        check(fdtd_set_bloch_boundary(g_.handle(), int(b.dir), int(b.orient), b.factor.real(), b.factor.imag()),
+              "fdtd_set_bloch_boundary");
+      else
+        check(fdtd_set_boundary(g_.handle(), int(b.dir), int(b.orient), detail::boundary_kind(b.type)),
+              "fdtd_set_boundary");
+    }
     long n0 = fdtd_step_count(g_.handle());
     for (auto& s : sources_) {
       std::vector<double> tab(size_t(n0 + n));

@@ -41,6 +41,9 @@ enum fdtd_mode { FDTD_TM = 2, FDTD_3D = 3 };
 enum fdtd_precision { FDTD_F32 = 0, FDTD_F64 = 1 };
 /* field codes */
 enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5 };
+/* OR'ed into the field of fdtd_upload_field / fdtd_download_field: the
+ * imaginary part of E or H of a complex grid (one with a Bloch-periodic face) */
+#define FDTD_IMAG 16
 /* material pole kinds */
 enum fdtd_pole { FDTD_DRUDE = 1, FDTD_LORENTZ = 2, FDTD_CONDUCTIVE = 3 };
 /* boundary conditions: the kinds of fdtd::Boundary (include/BoundaryUpdates.hpp:17-26) */
@@ -51,7 +54,7 @@ enum fdtd_boundary {
   FDTD_BC_SYMMETRIC = 3,
   FDTD_BC_ANTISYMMETRIC = 4,
   FDTD_BC_PERIODIC = 5,
-  FDTD_BC_BLOCH = 6          /* rejected: needs complex fields */
+  FDTD_BC_BLOCH = 6          /* set by fdtd_set_bloch_boundary (a complex phase factor) */
 };
 
 #define FDTD_MAX_MATERIALS 256
@@ -152,6 +155,8 @@ int fdtd_get_dft(fdtd_grid* g, int idx, double* out);
 int fdtd_add_probe(fdtd_grid* g, int field, int comp, int i, int j, int k);
 /* out: [nsteps][nprobes] float64, nsteps <= max_steps; *nsteps_out set */
 int fdtd_get_probes(fdtd_grid* g, double* out, int max_steps, int* nsteps_out);
+/* the imaginary part of the probes of a complex grid (same layout) */
+int fdtd_get_probes_imag(fdtd_grid* g, double* out, int max_steps, int* nsteps_out);
 
 /* Host<->device copies of one component (nx*ny*nz float64 values).  D and B
  * read back as eps_r eps0 E and mu_r mu0 H outside the dispersive box. */
@@ -200,6 +205,15 @@ int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out);
  * interior cells.  Set before the first step; not with batches or the
  * adjoint gradient (FDTD_EINVAL there). */
 int fdtd_set_boundary(fdtd_grid* g, int axis, int side, int kind);
+/* A Bloch-periodic face: make_bloch_periodic_boundary (include/BoundaryUpdates.hpp:
+ * 845-885), which needs complex fields.  The grid's fields become complex
+ * (the excitation, sources and TFSF, stays real): the ghost layer of the face
+ * takes factor * (the periodic image) for E and conj(factor) * (image) for H,
+ * the reference's decorator (mask 1 for every component).  The imaginary
+ * part is read with FDTD_IMAG and fdtd_get_probes_imag; fdtd_get_dft returns
+ * the spectrum of the complex field.  Before stepping; complex grids run the
+ * two-pass schedule and cannot be batched or differentiated. */
+int fdtd_set_bloch_boundary(fdtd_grid* g, int axis, int side, double factor_re, double factor_im);
 
 /* Capture each chunk of steps (up to 256) in a hipGraph once and replay it
  * for later chunks of the same shape (off by default: plain launches).  Pays
@@ -210,7 +224,7 @@ int fdtd_set_graph(fdtd_grid* g, int on);
 /* Per-kernel timing with hipEvents on the grid's stream (off by default).
  * fdtd_kernel_time returns accumulated milliseconds and launches for
  * kernel id 0 = E pass, 1 = H pass, 2 = aux line, 3 = fused pass, 4 = DFT,
- * 5 = adjoint E pass, 6 = adjoint H pass. */
+ * 5 = adjoint E pass, 6 = adjoint H pass, 7 = Bloch ghost layers. */
 int fdtd_set_profiling(fdtd_grid* g, int on);
 int fdtd_kernel_time(fdtd_grid* g, int kernel, double* ms, long* launches);
 

@@ -15,6 +15,7 @@
 
 #include <array>
 #include <chrono>
+#include <complex>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1055,6 +1056,169 @@ static void scen_boundaries(const std::string& dir) {
   fx.write(dir);
 }
 
+// ------------------------------------------------------------ scenario: Bloch-periodic faces
+// make_bloch_periodic_boundary (include/BoundaryUpdates.hpp:845-885) needs
+// complex fields: the reference's own functors on cells whose scalar is
+// std::complex<double> (curl, CPML, materials and the Bloch ghost writes all
+// take the complex scalar; sources stay real).  3-D grid: x and y
+// Bloch-periodic (phase e^{-/+ i theta} at the min / max face), CPML along z
+// (ghost layers there stay zero), a dielectric and a Drude block, a soft
+// source on Dz.  Probes record the real and imaginary parts every step.
+typedef std::complex<double> cplx;
+typedef Nb<FluidCell<YeeCell<YeeFields<ThreeD, cplx, std::array>, DoublePolarization<cplx>,
+                             VacuumMagnetization<cplx>, NewPML<ThreeD, cplx>>>>
+    Cell3c;
+struct PeriodicImageC {
+  Cell3c** base;
+  Cell3c** img;
+  Cell3c& operator()(const CellRefIt<Cell3c>& it) const { return *img[it.p - base]; }
+};
+
+static void scen_bloch(const std::string& dir) {
+  const int nx = 14, ny = 12, nz = 16, nsteps = 80;
+  const double dx = 1e-8, dt = 0.5 * dx / (c0 * std::sqrt(3.0));
+  const double th[2] = {0.9, -0.4};  // x, y phase across one period
+  std::vector<Cell3c> cells(size_t(nx) * ny * nz);
+  auto id = [&](int i, int j, int k) { return size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k); };
+  auto at = [&](int i, int j, int k) -> Cell3c& { return cells[id(i, j, k)]; };
+  const int n[3] = {nx, ny, nz};
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        Cell3c& c = at(i, j, k);
+        const int ci[3] = {i, j, k};
+        for (int a = 0; a < 3; a++) {
+          int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+          lo[a]--;
+          hi[a]++;
+          c.mn[a] = ci[a] > 0 ? &at(lo[0], lo[1], lo[2]) : nullptr;
+          c.mx[a] = ci[a] < n[a] - 1 ? &at(hi[0], hi[1], hi[2]) : nullptr;
+        }
+      }
+  PMLParameterModel dm(dx);
+  const PmlSpec pml = {4, 3, 1, dm.msMax, 5, 0.05};
+  place_pml(nz, pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        if (e)
+          at(i, j, idx).setPMLParameters<FieldType::Electric, Dir::Z>(K, Sg, A, dt);
+        else
+          at(i, j, idx).setPMLParameters<FieldType::Magnetic, Dir::Z>(K, Sg, A, dt);
+      }
+  });
+  std::vector<Mat> mats = {{0, 1, 0, 0, 0}, {1, 3.0, 0, 0, 0}, {2, 1.0, 1.37e16, 1e14, 0}};
+  std::vector<int> matid(cells.size(), 0);
+  for (int k = 6; k < 10; k++)
+    for (int j = 3; j < 8; j++)
+      for (int i = 2; i < 7; i++) matid[id(i, j, k)] = 1;
+  for (int k = 9; k < 12; k++)
+    for (int j = 5; j < 10; j++)
+      for (int i = 8; i < 12; i++) matid[id(i, j, k)] = 2;
+  // Bloch faces: the boundary layer (1 / L-2) and its periodic image (L-2 / 1)
+  std::vector<std::vector<Cell3c*>> bnd(4), img(4);
+  std::vector<BoundaryData> faces;
+  for (int a = 0; a < 2; a++)
+    for (int side = 0; side < 2; side++) {
+      const int L = n[a], src = side ? L - 2 : 1, im = side ? 1 : L - 2;
+      const int t = a == 0 ? 1 : 0;
+      auto& B = bnd[2 * a + side];
+      auto& I = img[2 * a + side];
+      for (int k = 1; k < nz - 1; k++)
+        for (int u = 1; u < n[t] - 1; u++) {
+          int c1[3], c2[3];
+          c1[a] = src;
+          c1[t] = u;
+          c1[2] = k;
+          c2[a] = im;
+          c2[t] = u;
+          c2[2] = k;
+          B.push_back(&at(c1[0], c1[1], c1[2]));
+          I.push_back(&at(c2[0], c2[1], c2[2]));
+        }
+    }
+  for (int a = 0; a < 2; a++)
+    for (int side = 0; side < 2; side++) {
+      auto& B = bnd[2 * a + side];
+      auto& I = img[2 * a + side];
+      CellRefIt<Cell3c> b{B.data()}, e{B.data() + B.size()};
+      PeriodicImageC pf{B.data(), I.data()};
+      const cplx f = std::polar(1.0, side ? th[a] : -th[a]);
+      if (a == 0)
+        faces.push_back(side ? make_bloch_periodic_boundary<ThreeD, Dir::X, Orientation::MAX>(b, e, pf, f)
+                             : make_bloch_periodic_boundary<ThreeD, Dir::X, Orientation::MIN>(b, e, pf, f));
+      else
+        faces.push_back(side ? make_bloch_periodic_boundary<ThreeD, Dir::Y, Orientation::MAX>(b, e, pf, f)
+                             : make_bloch_periodic_boundary<ThreeD, Dir::Y, Orientation::MIN>(b, e, pf, f));
+    }
+  auto interior = [&](auto&& fn) {
+    for (int k = 1; k < nz - 1; k++)
+      for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) fn(i, j, k);
+  };
+  const int probes[4][3] = {{7, 6, 8}, {1, 3, 5}, {12, 10, 9}, {4, 1, 11}};
+  std::vector<double> src, ptrace;
+  for (int step = 0; step < nsteps; step++) src.push_back(eps0 * gauss_pulse(step, 16, 5));
+  for (int step = 0; step < nsteps; step++) {
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dx>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dy>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dz>(dt, dx)(at(i, j, k)); });
+    at(5, 4, 7).Dz() += src[step];
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Electric, Dir::Z>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { material_E<ThreeD>(at(i, j, k), mats[matid[id(i, j, k)]], dt); });
+    for (auto& f : faces) f.updateE();
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Bx>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, By>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Bz>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Magnetic, Dir::Z>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { VacuumUpdate<ThreeD, FieldType::Magnetic>()(at(i, j, k)); });
+    for (auto& f : faces) f.updateH();
+    for (auto& p : probes) {
+      Cell3c& c = at(p[0], p[1], p[2]);
+      for (cplx v : {c.Ex(), c.Ey(), c.Ez(), c.Hx(), c.Hy(), c.Hz()}) ptrace.insert(ptrace.end(), {v.real(), v.imag()});
+    }
+  }
+  Fixture fx;
+  fx.name = "grid3d_bloch";
+  fx.ps("mode", "ThreeD");
+  fx.pi("nx", nx);
+  fx.pi("ny", ny);
+  fx.pi("nz", nz);
+  fx.pi("nsteps", nsteps);
+  fx.p("dx", dx);
+  fx.p("dt", dt);
+  put_pml(fx, pml);
+  fx.pi("pmlx", 0);
+  fx.pi("pmly", 0);
+  fx.p("theta_x", th[0]);
+  fx.p("theta_y", th[1]);
+  fx.pi("src_i", 5);
+  fx.pi("src_j", 4);
+  fx.pi("src_k", 7);
+  std::vector<double> mt;
+  for (auto& m : mats) mt.insert(mt.end(), {double(m.kind), m.eps, m.freq, m.gamma, m.delta});
+  fx.arr("mats_kind_eps_freq_gamma_delta", mt, {mats.size(), 5});
+  std::vector<double> mid(matid.begin(), matid.end());
+  std::vector<size_t> sh = {size_t(nz), size_t(ny), size_t(nx)};
+  fx.arr("matid", mid, sh);
+  fx.arr("src", src, {src.size()});
+  std::vector<double> pr;
+  for (auto& p : probes) pr.insert(pr.end(), {double(p[0]), double(p[1]), double(p[2])});
+  fx.arr("probes", pr, {4, 3});
+  fx.arr("probe_trace_E_H_reim", ptrace, {size_t(nsteps), 4, 6, 2});
+  const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+  for (int f = 0; f < 6; f++)
+    for (int part = 0; part < 2; part++) {
+      std::vector<double> v;
+      for (auto& c : cells) {
+        Cell3c& cc = const_cast<Cell3c&>(c);
+        cplx x = f == 0 ? cc.Ex() : f == 1 ? cc.Ey() : f == 2 ? cc.Ez() : f == 3 ? cc.Hx() : f == 4 ? cc.Hy() : cc.Hz();
+        v.push_back(part ? x.imag() : x.real());
+      }
+      fx.arr(std::string(names[f]) + (part ? "_im" : "_re"), v, sh);
+    }
+  fx.write(dir);
+}
+
 int main(int argc, char** argv) {
   if (argc >= 2 && !strcmp(argv[1], "bench3d")) {
     // timing of the reference's own functors on a bounded sample of the 3-D
@@ -1091,6 +1255,7 @@ int main(int argc, char** argv) {
   std::string dir = argv[1];
   scen_pml_coeffs(dir);
   scen_boundaries(dir);
+  scen_bloch(dir);
   scen_surface(dir);
   scen_material_map(dir);
   scen_aux1d(dir);

@@ -281,6 +281,10 @@ struct rc_sim {
   /* boundary condition per [axis][side] (RC_BC_*; 0: ghost layer left alone) */
   int bc[3][2];
   double *fluid_ne, *fluid_nu; /* per cell, NULL until rc_set_fluid_state */
+  /* complex fields (a Bloch-periodic face): the imaginary part, a copy of the
+   * configuration without sources or TFSF, made at the first step */
+  double bloch[3][2][2];
+  struct rc_sim* im;
 };
 
 static size_t cidx(const rc_sim* s, int i, int j, int k) {
@@ -395,6 +399,7 @@ void rc_destroy(rc_sim* s) {
       for (int J = 0; J < 3; J++) free(s->Pb[ft][I][J]);
   free(s->fluid_ne);
   free(s->fluid_nu);
+  rc_destroy(s->im);
   free(s);
 }
 
@@ -428,6 +433,7 @@ int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, co
         }
     s->npoles_alloc = npoles;
   }
+  if (s->im) rc_set_material(s->im, id, eps_r, mu_r, npoles, poles);
   return 0;
 }
 
@@ -437,6 +443,7 @@ int rc_set_magnetized_drude(rc_sim* s, int id, double eps_r, double mu_r, double
   if (rc_set_material(s, id, eps_r, mu_r, 1, pole)) return -1;
   s->mat[id].mag = bias ? 1 : 2;
   for (int c = 0; c < 3; c++) s->mat[id].bias[c] = bias ? bias[c] : 0.0;
+  if (s->im) s->im->mat[id] = s->mat[id];
   return 0;
 }
 
@@ -447,6 +454,7 @@ int rc_set_fluid_drude(rc_sim* s, int id, double mu_r, int magnetization, const 
   s->mat[id].fluid = 1;
   s->mat[id].mag = magnetization;
   for (int c = 0; c < 3; c++) s->mat[id].bias[c] = magnetization == 1 ? bias[c] : 0.0;
+  if (s->im) s->im->mat[id] = s->mat[id];
   return 0;
 }
 
@@ -457,6 +465,7 @@ int rc_set_fluid_state(rc_sim* s, const double* ne, const double* nu_m) {
   }
   memcpy(s->fluid_ne, ne, s->ncell * sizeof(double));
   memcpy(s->fluid_nu, nu_m, s->ncell * sizeof(double));
+  if (s->im) rc_set_fluid_state(s->im, ne, nu_m);
   return 0;
 }
 
@@ -524,6 +533,7 @@ static void magnetized_drude_3d(rc_sim* s, const rc_mat* m, size_t c, double dt)
 
 int rc_set_matid(rc_sim* s, const uint8_t* ids) {
   memcpy(s->matid, ids, s->ncell);
+  if (s->im) rc_set_matid(s->im, ids);
   return 0;
 }
 
@@ -579,13 +589,23 @@ int rc_add_dft(rc_sim* s, const int region[6], int comp_mask, int nfreq, const d
   memcpy(d->phE, phaseE, sizeof(double) * 2 * nfreq * nphase);
   memcpy(d->phH, phaseH, sizeof(double) * 2 * nfreq * nphase);
   d->acc = calloc((size_t)d->ncomp * nfreq * 2 * d->ncells, sizeof(double));
+  if (s->im) rc_add_dft(s->im, region, comp_mask, nfreq, phaseE, phaseH, nphase);
   return s->ndft - 1;
 }
 
+/* the spectrum of the (complex) field: F = F_re + i F_im for a complex grid */
 int rc_get_dft(const rc_sim* s, int idx, double* out) {
   if (idx < 0 || idx >= s->ndft) return -1;
   const rc_dft* d = &s->dft[idx];
   memcpy(out, d->acc, sizeof(double) * (size_t)d->ncomp * d->nfreq * 2 * d->ncells);
+  if (s->im) {
+    const double* q = s->im->dft[idx].acc;
+    for (size_t b = 0; b < (size_t)d->ncomp * d->nfreq; b++)
+      for (size_t c = 0; c < d->ncells; c++) {
+        out[b * 2 * d->ncells + c] -= q[b * 2 * d->ncells + d->ncells + c];
+        out[b * 2 * d->ncells + d->ncells + c] += q[b * 2 * d->ncells + c];
+      }
+  }
   return 0;
 }
 
@@ -596,6 +616,10 @@ int rc_add_probe(rc_sim* s, int field, int comp, int i, int j, int k) {
   p->comp = comp;
   p->cell = cidx(s, i, j, k);
   return 0;
+}
+
+int rc_get_probes_imag(const rc_sim* s, double* out, int max_steps) {
+  return s->im ? rc_get_probes(s->im, out, max_steps) : -1;
 }
 
 int rc_get_probes(const rc_sim* s, double* out, int max_steps) {
@@ -619,6 +643,14 @@ static double* field_ptr(const rc_sim* s, int field, int comp, int pole) {
 }
 
 int rc_get_field(const rc_sim* s, int field, int comp, int pole, double* out) {
+  if (field & 16) { /* the imaginary part of a complex grid (zero before its first step) */
+    if (!s->im) {
+      if (!field_ptr(s, field & ~16, comp, pole)) return -1;
+      memset(out, 0, sizeof(double) * s->ncell);
+      return 0;
+    }
+    return rc_get_field(s->im, field & ~16, comp, pole, out);
+  }
   double* p = field_ptr(s, field, comp, pole);
   if (!p) return -1;
   memcpy(out, p, sizeof(double) * s->ncell);
@@ -686,6 +718,89 @@ int rc_set_boundary(rc_sim* s, int axis, int side, int type) {
   return 0;
 }
 
+/* Bloch-periodic face (make_bloch_periodic_boundary, BoundaryUpdates.hpp:845-885):
+ * ghost = factor * image for E, conj(factor) * image for H (mask 1), on
+ * complex fields held as a real and an imaginary part */
+int rc_set_bloch_boundary(rc_sim* s, int axis, int side, double fr, double fi) {
+  if (axis < 0 || axis > 2 || side < 0 || side > 1 || !s->axes[axis] || s->step > 0) return -1;
+  s->bc[axis][side] = RC_BC_BLOCH;
+  s->bloch[axis][side][0] = fr;
+  s->bloch[axis][side][1] = fi;
+  return 0;
+}
+
+static int has_bloch(const rc_sim* s) {
+  for (int a = 0; a < 3; a++)
+    if (s->bc[a][0] == RC_BC_BLOCH || s->bc[a][1] == RC_BC_BLOCH) return 1;
+  return 0;
+}
+
+/* the imaginary part: same grid, PML, materials, fluid state, boundaries,
+ * DFT monitors and probes; no sources and no TFSF (the excitation is real) */
+static rc_sim* clone_part(const rc_sim* s) {
+  rc_sim* t = rc_create(s->mode, s->nx, s->ny, s->nz, s->dx, s->dt);
+  const int len[3] = {s->nx, s->ny, s->nz};
+  for (int ft = 0; ft < 2; ft++)
+    for (int a = 0; a < 3; a++) memcpy(t->pml[ft][a], s->pml[ft][a], sizeof(double) * 4 * len[a]);
+  memcpy(t->matid, s->matid, s->ncell);
+  memcpy(t->mat, s->mat, sizeof s->mat);
+  for (int p = 0; p < s->npoles_alloc; p++)
+    for (int c = 0; c < 3; c++)
+      if (t->actE[c]) {
+        t->P[p][c] = calloc(t->ncell, sizeof(double));
+        t->J[p][c] = calloc(t->ncell, sizeof(double));
+      }
+  t->npoles_alloc = s->npoles_alloc;
+  memcpy(t->bc, s->bc, sizeof s->bc);
+  memcpy(t->bloch, s->bloch, sizeof s->bloch);
+  if (s->fluid_ne) rc_set_fluid_state(t, s->fluid_ne, s->fluid_nu);
+  for (int q = 0; q < s->ndft; q++) {
+    const rc_dft* d = &s->dft[q];
+    rc_add_dft(t, d->region, d->mask, d->nfreq, d->phE, d->phH, d->nphase);
+  }
+  for (int p = 0; p < s->nprobes; p++) {
+    const rc_probe* q = &s->probes[p];
+    t->probes = realloc(t->probes, sizeof(rc_probe) * (t->nprobes + 1));
+    t->probes[t->nprobes++] = *q;
+  }
+  return t;
+}
+
+/* the Bloch ghost layers of both parts from both parts' periodic images */
+static void bloch_ghosts(rc_sim* s, int ft) {
+  rc_sim* m = s->im;
+  double** Fr = ft == 0 ? s->E : s->H;
+  double** Fi = ft == 0 ? m->E : m->H;
+  const int* act = ft == 0 ? s->actE : s->actH;
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const int len[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; a++)
+    for (int side = 0; side < 2; side++) {
+      if (s->bc[a][side] != RC_BC_BLOCH) continue;
+      const double fr = s->bloch[a][side][0], fi = ft == 0 ? s->bloch[a][side][1] : -s->bloch[a][side][1];
+      const int L = len[a], dst = side ? L - 1 : 0, src = side ? 1 : L - 2;
+      int rlo[3] = {lo[0], lo[1], lo[2]}, rhi[3] = {hi[0], hi[1], hi[2]};
+      rlo[a] = 0;
+      rhi[a] = 1;
+      for (int k = rlo[2]; k < rhi[2]; k++)
+        for (int j = rlo[1]; j < rhi[1]; j++)
+          for (int i = rlo[0]; i < rhi[0]; i++) {
+            int cd[3] = {i, j, k}, cs[3] = {i, j, k};
+            cd[a] = dst;
+            cs[a] = src;
+            const size_t d = cidx(s, cd[0], cd[1], cd[2]), q = cidx(s, cs[0], cs[1], cs[2]);
+            for (int c = 0; c < 3; c++)
+              if (act[c]) {
+                /* (fr + i fi)(sr + i si), std::complex's product */
+                const double sr = Fr[c][q], si = Fi[c][q];
+                Fr[c][d] = fr * sr - fi * si;
+                Fi[c][d] = fr * si + fi * sr;
+              }
+          }
+    }
+}
+
 static void apply_boundaries(rc_sim* s, int ft) {
   double** F = ft == 0 ? s->E : s->H;
   const int* act = ft == 0 ? s->actE : s->actH;
@@ -695,7 +810,7 @@ static void apply_boundaries(rc_sim* s, int ft) {
   for (int a = 0; a < 3; a++)
     for (int side = 0; side < 2; side++) {
       const int type = s->bc[a][side];
-      if (!type) continue;
+      if (!type || type == RC_BC_BLOCH) continue; /* Bloch faces: bloch_ghosts */
       const int L = len[a];
       const int dst = side ? L - 1 : 0;
       const int src = type == RC_BC_PERIODIC ? (side ? 1 : L - 2) : (side ? L - 2 : 1);
@@ -794,219 +909,257 @@ static void tfsf_faces(rc_sim* s, int ft) {
   }
 }
 
-int rc_step(rc_sim* s, int nsteps) {
+/* One step in three phases (the E half step with its boundaries, the H half
+ * step with its boundaries, then DFT and probes), so that the two parts of a
+ * complex grid advance in lockstep with the Bloch coupling between them. */
+static void step_e(rc_sim* s) {
   int lo[3], hi[3];
   interior(s, lo, hi);
   const double dt = s->dt, dx = s->dx;
   const double mu0 = rc_mu0_v();
-  for (int it = 0; it < nsteps; it++) {
-    const long n = s->step;
-    /* ---- D curl, YeeUpdate<Mode,D_I> (YeeUpdates.hpp:395-405) */
-    for (int I = 0; I < 3; I++) {
-      if (!s->actE[I]) continue;
-      const int j1 = D1[I], j2 = D2[I];
-      const int k1 = third(I, j1), k2 = third(I, j2);
-      const int h1 = s->actH[k1] && s->axes[j1], h2 = s->actH[k2] && s->axes[j2];
-      const size_t s1 = stride(s, j1), s2 = stride(s, j2);
+  const long n = s->step;
+  (void)lo; (void)hi; (void)dt; (void)dx; (void)mu0; (void)n;
+  /* ---- D curl, YeeUpdate<Mode,D_I> (YeeUpdates.hpp:395-405) */
+  for (int I = 0; I < 3; I++) {
+    if (!s->actE[I]) continue;
+    const int j1 = D1[I], j2 = D2[I];
+    const int k1 = third(I, j1), k2 = third(I, j2);
+    const int h1 = s->actH[k1] && s->axes[j1], h2 = s->actH[k2] && s->axes[j2];
+    const size_t s1 = stride(s, j1), s2 = stride(s, j2);
+    double* Df = s->D[I];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = lo[2]; k < hi[2]; k++)
+      for (int j = lo[1]; j < hi[1]; j++)
+        for (int i = lo[0]; i < hi[0]; i++) {
+          size_t c = cidx(s, i, j, k);
+          int ci[3] = {i, j, k};
+          double a1 = 0.0, a2 = 0.0;
+          if (h1) a1 = lc(I, j1, k1) * s->pml[0][j1][4 * ci[j1] + 2] / (dx) * (s->H[k1][c] - s->H[k1][c - s1]);
+          if (h2) a2 = lc(I, j2, k2) * s->pml[0][j2][4 * ci[j2] + 2] / (dx) * (s->H[k2][c] - s->H[k2][c - s2]);
+          Df[c] = Df[c] + 1.0 * 1.0 * dt * (a1 + a2);
+        }
+  }
+  for (int q = 0; q < s->nsrc; q++)
+    if (s->src[q].field == 0 && n < s->src[q].n) s->D[s->src[q].comp][s->src[q].cell] += s->src[q].table[n];
+  /* ---- UpdatePML<Mode,Electric,J> (PML.hpp:315-323), J = x, y, z */
+  for (int J = 0; J < 3; J++) {
+    if (!s->axes[J]) continue;
+    const size_t sJ = stride(s, J);
+    const double* pm = s->pml[0][J];
+    for (int I = 2; I >= 0; I--) {
+      if (I == J || !s->actE[I]) continue;
+      const int K = third(I, J);
+      double* ps = s->psi[0][I][J];
       double* Df = s->D[I];
+      const double* Hk = s->H[K];
+      const int e = lc(I, J, K);
 #pragma omp parallel for collapse(2) schedule(static)
       for (int k = lo[2]; k < hi[2]; k++)
         for (int j = lo[1]; j < hi[1]; j++)
           for (int i = lo[0]; i < hi[0]; i++) {
             size_t c = cidx(s, i, j, k);
             int ci[3] = {i, j, k};
-            double a1 = 0.0, a2 = 0.0;
-            if (h1) a1 = lc(I, j1, k1) * s->pml[0][j1][4 * ci[j1] + 2] / (dx) * (s->H[k1][c] - s->H[k1][c - s1]);
-            if (h2) a2 = lc(I, j2, k2) * s->pml[0][j2][4 * ci[j2] + 2] / (dx) * (s->H[k2][c] - s->H[k2][c - s2]);
-            Df[c] = Df[c] + 1.0 * 1.0 * dt * (a1 + a2);
+            const double* p = pm + 4 * ci[J];
+            Df[c] += 1.0 * dt * ps[c] * p[3];
+            ps[c] = p[0] * ps[c] + p[1] * e / dx * (Hk[c] - Hk[c - sJ]);
           }
     }
-    for (int q = 0; q < s->nsrc; q++)
-      if (s->src[q].field == 0 && n < s->src[q].n) s->D[s->src[q].comp][s->src[q].cell] += s->src[q].table[n];
-    /* ---- UpdatePML<Mode,Electric,J> (PML.hpp:315-323), J = x, y, z */
-    for (int J = 0; J < 3; J++) {
-      if (!s->axes[J]) continue;
-      const size_t sJ = stride(s, J);
-      const double* pm = s->pml[0][J];
-      for (int I = 2; I >= 0; I--) {
-        if (I == J || !s->actE[I]) continue;
-        const int K = third(I, J);
-        double* ps = s->psi[0][I][J];
-        double* Df = s->D[I];
-        const double* Hk = s->H[K];
-        const int e = lc(I, J, K);
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int k = lo[2]; k < hi[2]; k++)
-          for (int j = lo[1]; j < hi[1]; j++)
-            for (int i = lo[0]; i < hi[0]; i++) {
-              size_t c = cidx(s, i, j, k);
-              int ci[3] = {i, j, k};
-              const double* p = pm + 4 * ci[J];
-              Df[c] += 1.0 * dt * ps[c] * p[3];
-              ps[c] = p[0] * ps[c] + p[1] * e / dx * (Hk[c] - Hk[c - sJ]);
-            }
-      }
-    }
-    /* ---- TFSF updateD, then the aux line advances (Source.hpp:595-609) */
-    if (s->tfsf) {
-      tfsf_faces(s, 0);
-      rc_aux_step(s->aux, n < s->nsignal ? s->signal[n] : 0.0);
-    }
-    /* ---- E material (DispersiveMaterials.hpp:354-359, 710-735, 979-998) */
-    {
-      const int np = s->npoles_alloc;
-#pragma omp parallel for collapse(2) schedule(static)
-      for (int k = lo[2]; k < hi[2]; k++)
-        for (int j = lo[1]; j < hi[1]; j++)
-          for (int i = lo[0]; i < hi[0]; i++) {
-            size_t c = cidx(s, i, j, k);
-            const rc_mat* m = &s->mat[s->matid[c]];
-            const long dl = design_local(s, i, j, k);
-            if (m->mag && m->np && np && dl < 0 && s->mode == RC_3D) {
-              magnetized_drude_3d(s, m, c, dt);
-              continue;
-            }
-            for (int I = 2; I >= 0; I--) {
-              if (!s->actE[I]) continue;
-              if (dl >= 0) { /* design cell: non-dispersive, its own eps_r */
-                s->E[I][c] = s->D[I][c] / (s->dsg_eps[dl] * RC_EPS0);
-                continue;
-              }
-              if (m->np == 0 || np == 0) {
-                s->E[I][c] = s->D[I][c] / (m->eps * RC_EPS0);
-                continue;
-              }
-              for (int p = 0; p < m->np; p++) {
-                if (m->kind[p] == 3) { /* ConductiveUpdate::implicit_update (DispersiveMaterials.hpp:549-560) */
-                  const double factor = dt * m->freq[p] / (m->eps * RC_EPS0);
-                  s->P[p][I][c] = (s->P[p][I][c] + factor * s->D[I][c]) / (1.0 + factor);
-                } else {
-                  s->P[p][I][c] += dt * s->J[p][I][c];
-                }
-              }
-              double num = s->D[I][c];
-              for (int p = 0; p < m->np; p++) num = num - s->P[p][I][c];
-              double Ev = num / (m->eps * RC_EPS0);
-              s->E[I][c] = Ev;
-              for (int p = 0; p < m->np; p++) {
-                double* Jp = &s->J[p][I][c];
-                double* Pp = &s->P[p][I][c];
-                const double wgt_eps0 = m->weight[p] * RC_EPS0;
-                if (m->kind[p] == 3) continue; /* conductive: J untouched */
-                if (m->kind[p] == 1) {
-                  double fq = m->freq[p], gm = m->gamma[p];
-                  if (m->fluid) plasma_params(s, m, c, &fq, &gm);
-                  double b = gm * dt * 0.5;
-                  double w = dt * fq;
-                  *Jp = *Jp * (1.0 - b) / (1.0 + b) + w * w / dt * wgt_eps0 * Ev;
-                } else {
-                  double b = 1.0 + m->gamma[p] * dt;
-                  double bm = 1.0 - m->gamma[p] * dt;
-                  double w = m->freq[p] * dt;
-                  *Jp = bm / b * *Jp + m->freq[p] * w / b * wgt_eps0 * Ev - m->freq[p] * w / b * *Pp;
-                }
-              }
-            }
-          }
-    }
-    apply_boundaries(s, 0);
-    /* ---- B curl */
-    for (int I = 0; I < 3; I++) {
-      if (!s->actH[I]) continue;
-      const int j1 = D1[I], j2 = D2[I];
-      const int k1 = third(I, j1), k2 = third(I, j2);
-      const int e1 = s->actE[k1] && s->axes[j1], e2 = s->actE[k2] && s->axes[j2];
-      const size_t s1 = stride(s, j1), s2 = stride(s, j2);
-      double* Bf = s->B[I];
-#pragma omp parallel for collapse(2) schedule(static)
-      for (int k = lo[2]; k < hi[2]; k++)
-        for (int j = lo[1]; j < hi[1]; j++)
-          for (int i = lo[0]; i < hi[0]; i++) {
-            size_t c = cidx(s, i, j, k);
-            int ci[3] = {i, j, k};
-            double a1 = 0.0, a2 = 0.0;
-            if (e1) a1 = lc(I, j1, k1) * s->pml[1][j1][4 * ci[j1] + 2] / (dx) * (s->E[k1][c + s1] - s->E[k1][c]);
-            if (e2) a2 = lc(I, j2, k2) * s->pml[1][j2][4 * ci[j2] + 2] / (dx) * (s->E[k2][c + s2] - s->E[k2][c]);
-            Bf[c] = Bf[c] + -1.0 * 1.0 * dt * (a1 + a2);
-          }
-    }
-    for (int q = 0; q < s->nsrc; q++)
-      if (s->src[q].field == 1 && n < s->src[q].n) s->B[s->src[q].comp][s->src[q].cell] += s->src[q].table[n];
-    /* ---- UpdatePML<Mode,Magnetic,J> */
-    for (int J = 0; J < 3; J++) {
-      if (!s->axes[J]) continue;
-      const size_t sJ = stride(s, J);
-      const double* pm = s->pml[1][J];
-      for (int I = 2; I >= 0; I--) {
-        if (I == J || !s->actH[I]) continue;
-        const int K = third(I, J);
-        double* ps = s->psi[1][I][J];
-        double* Bf = s->B[I];
-        const double* Ek = s->E[K];
-        const int e = lc(I, J, K);
-#pragma omp parallel for collapse(2) schedule(static)
-        for (int k = lo[2]; k < hi[2]; k++)
-          for (int j = lo[1]; j < hi[1]; j++)
-            for (int i = lo[0]; i < hi[0]; i++) {
-              size_t c = cidx(s, i, j, k);
-              int ci[3] = {i, j, k};
-              const double* p = pm + 4 * ci[J];
-              Bf[c] += -1.0 * dt * ps[c] * p[3];
-              ps[c] = p[0] * ps[c] + p[1] * e / dx * (Ek[c + sJ] - Ek[c]);
-            }
-      }
-    }
-    if (s->tfsf) tfsf_faces(s, 1);
-    /* ---- H material: H = B/(mu_r mu0) */
+  }
+  /* ---- TFSF updateD, then the aux line advances (Source.hpp:595-609) */
+  if (s->tfsf) {
+    tfsf_faces(s, 0);
+    rc_aux_step(s->aux, n < s->nsignal ? s->signal[n] : 0.0);
+  }
+  /* ---- E material (DispersiveMaterials.hpp:354-359, 710-735, 979-998) */
+  {
+    const int np = s->npoles_alloc;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int k = lo[2]; k < hi[2]; k++)
       for (int j = lo[1]; j < hi[1]; j++)
         for (int i = lo[0]; i < hi[0]; i++) {
           size_t c = cidx(s, i, j, k);
           const rc_mat* m = &s->mat[s->matid[c]];
-          for (int I = 2; I >= 0; I--)
-            if (s->actH[I]) s->H[I][c] = s->B[I][c] / (m->mu * mu0);
-        }
-    apply_boundaries(s, 1);
-    /* ---- running DFT (build-defined): F += f(t) e^{-i w t} dt, E at (n+1)dt, H at (n+1.5)dt */
-    for (int q = 0; q < s->ndft; q++) {
-      rc_dft* d = &s->dft[q];
-      if (n >= d->nphase) continue;
-      const int* r = d->region;
-      int slot = 0;
-      for (int comp = 0; comp < 6; comp++) {
-        if (!((d->mask >> comp) & 1)) continue;
-        const double* fld = comp < 3 ? s->E[comp] : s->H[comp - 3];
-        const double* ph = (comp < 3 ? d->phE : d->phH) + 2 * (size_t)d->nfreq * n;
-        for (int f = 0; f < d->nfreq; f++) {
-          double* re = d->acc + ((size_t)slot * d->nfreq + f) * 2 * d->ncells;
-          double* im = re + d->ncells;
-          size_t q2 = 0;
-          for (int k = r[4]; k < r[5]; k++)
-            for (int j = r[2]; j < r[3]; j++)
-              for (int i = r[0]; i < r[1]; i++, q2++) {
-                double v = fld ? fld[cidx(s, i, j, k)] : 0.0;
-                re[q2] += v * ph[2 * f];
-                im[q2] += v * ph[2 * f + 1];
+          const long dl = design_local(s, i, j, k);
+          if (m->mag && m->np && np && dl < 0 && s->mode == RC_3D) {
+            magnetized_drude_3d(s, m, c, dt);
+            continue;
+          }
+          for (int I = 2; I >= 0; I--) {
+            if (!s->actE[I]) continue;
+            if (dl >= 0) { /* design cell: non-dispersive, its own eps_r */
+              s->E[I][c] = s->D[I][c] / (s->dsg_eps[dl] * RC_EPS0);
+              continue;
+            }
+            if (m->np == 0 || np == 0) {
+              s->E[I][c] = s->D[I][c] / (m->eps * RC_EPS0);
+              continue;
+            }
+            for (int p = 0; p < m->np; p++) {
+              if (m->kind[p] == 3) { /* ConductiveUpdate::implicit_update (DispersiveMaterials.hpp:549-560) */
+                const double factor = dt * m->freq[p] / (m->eps * RC_EPS0);
+                s->P[p][I][c] = (s->P[p][I][c] + factor * s->D[I][c]) / (1.0 + factor);
+              } else {
+                s->P[p][I][c] += dt * s->J[p][I][c];
               }
+            }
+            double num = s->D[I][c];
+            for (int p = 0; p < m->np; p++) num = num - s->P[p][I][c];
+            double Ev = num / (m->eps * RC_EPS0);
+            s->E[I][c] = Ev;
+            for (int p = 0; p < m->np; p++) {
+              double* Jp = &s->J[p][I][c];
+              double* Pp = &s->P[p][I][c];
+              const double wgt_eps0 = m->weight[p] * RC_EPS0;
+              if (m->kind[p] == 3) continue; /* conductive: J untouched */
+              if (m->kind[p] == 1) {
+                double fq = m->freq[p], gm = m->gamma[p];
+                if (m->fluid) plasma_params(s, m, c, &fq, &gm);
+                double b = gm * dt * 0.5;
+                double w = dt * fq;
+                *Jp = *Jp * (1.0 - b) / (1.0 + b) + w * w / dt * wgt_eps0 * Ev;
+              } else {
+                double b = 1.0 + m->gamma[p] * dt;
+                double bm = 1.0 - m->gamma[p] * dt;
+                double w = m->freq[p] * dt;
+                *Jp = bm / b * *Jp + m->freq[p] * w / b * wgt_eps0 * Ev - m->freq[p] * w / b * *Pp;
+              }
+            }
+          }
         }
-        slot++;
-      }
+  }
+  apply_boundaries(s, 0);
+}
+
+static void step_h(rc_sim* s) {
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const double dt = s->dt, dx = s->dx;
+  const double mu0 = rc_mu0_v();
+  const long n = s->step;
+  (void)lo; (void)hi; (void)dt; (void)dx; (void)mu0; (void)n;
+  /* ---- B curl */
+  for (int I = 0; I < 3; I++) {
+    if (!s->actH[I]) continue;
+    const int j1 = D1[I], j2 = D2[I];
+    const int k1 = third(I, j1), k2 = third(I, j2);
+    const int e1 = s->actE[k1] && s->axes[j1], e2 = s->actE[k2] && s->axes[j2];
+    const size_t s1 = stride(s, j1), s2 = stride(s, j2);
+    double* Bf = s->B[I];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = lo[2]; k < hi[2]; k++)
+      for (int j = lo[1]; j < hi[1]; j++)
+        for (int i = lo[0]; i < hi[0]; i++) {
+          size_t c = cidx(s, i, j, k);
+          int ci[3] = {i, j, k};
+          double a1 = 0.0, a2 = 0.0;
+          if (e1) a1 = lc(I, j1, k1) * s->pml[1][j1][4 * ci[j1] + 2] / (dx) * (s->E[k1][c + s1] - s->E[k1][c]);
+          if (e2) a2 = lc(I, j2, k2) * s->pml[1][j2][4 * ci[j2] + 2] / (dx) * (s->E[k2][c + s2] - s->E[k2][c]);
+          Bf[c] = Bf[c] + -1.0 * 1.0 * dt * (a1 + a2);
+        }
+  }
+  for (int q = 0; q < s->nsrc; q++)
+    if (s->src[q].field == 1 && n < s->src[q].n) s->B[s->src[q].comp][s->src[q].cell] += s->src[q].table[n];
+  /* ---- UpdatePML<Mode,Magnetic,J> */
+  for (int J = 0; J < 3; J++) {
+    if (!s->axes[J]) continue;
+    const size_t sJ = stride(s, J);
+    const double* pm = s->pml[1][J];
+    for (int I = 2; I >= 0; I--) {
+      if (I == J || !s->actH[I]) continue;
+      const int K = third(I, J);
+      double* ps = s->psi[1][I][J];
+      double* Bf = s->B[I];
+      const double* Ek = s->E[K];
+      const int e = lc(I, J, K);
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = lo[2]; k < hi[2]; k++)
+        for (int j = lo[1]; j < hi[1]; j++)
+          for (int i = lo[0]; i < hi[0]; i++) {
+            size_t c = cidx(s, i, j, k);
+            int ci[3] = {i, j, k};
+            const double* p = pm + 4 * ci[J];
+            Bf[c] += -1.0 * dt * ps[c] * p[3];
+            ps[c] = p[0] * ps[c] + p[1] * e / dx * (Ek[c + sJ] - Ek[c]);
+          }
     }
-    /* ---- probes */
-    if (s->nprobes) {
-      if (s->step + 1 > s->ptrace_cap) {
-        int cap = s->ptrace_cap ? 2 * s->ptrace_cap : 1024;
-        while (cap < s->step + 1) cap *= 2;
-        s->ptrace = realloc(s->ptrace, sizeof(double) * cap * s->nprobes);
-        s->ptrace_cap = cap;
+  }
+  if (s->tfsf) tfsf_faces(s, 1);
+  /* ---- H material: H = B/(mu_r mu0) */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = lo[2]; k < hi[2]; k++)
+    for (int j = lo[1]; j < hi[1]; j++)
+      for (int i = lo[0]; i < hi[0]; i++) {
+        size_t c = cidx(s, i, j, k);
+        const rc_mat* m = &s->mat[s->matid[c]];
+        for (int I = 2; I >= 0; I--)
+          if (s->actH[I]) s->H[I][c] = s->B[I][c] / (m->mu * mu0);
       }
-      for (int p = 0; p < s->nprobes; p++) {
-        const double* f = field_ptr(s, s->probes[p].field, s->probes[p].comp, 0);
-        s->ptrace[(size_t)s->step * s->nprobes + p] = f ? f[s->probes[p].cell] : 0.0;
+  apply_boundaries(s, 1);
+}
+
+static void step_out(rc_sim* s) {
+  int lo[3], hi[3];
+  interior(s, lo, hi);
+  const double dt = s->dt, dx = s->dx;
+  const double mu0 = rc_mu0_v();
+  const long n = s->step;
+  (void)lo; (void)hi; (void)dt; (void)dx; (void)mu0; (void)n;
+  /* ---- running DFT (build-defined): F += f(t) e^{-i w t} dt, E at (n+1)dt, H at (n+1.5)dt */
+  for (int q = 0; q < s->ndft; q++) {
+    rc_dft* d = &s->dft[q];
+    if (n >= d->nphase) continue;
+    const int* r = d->region;
+    int slot = 0;
+    for (int comp = 0; comp < 6; comp++) {
+      if (!((d->mask >> comp) & 1)) continue;
+      const double* fld = comp < 3 ? s->E[comp] : s->H[comp - 3];
+      const double* ph = (comp < 3 ? d->phE : d->phH) + 2 * (size_t)d->nfreq * n;
+      for (int f = 0; f < d->nfreq; f++) {
+        double* re = d->acc + ((size_t)slot * d->nfreq + f) * 2 * d->ncells;
+        double* im = re + d->ncells;
+        size_t q2 = 0;
+        for (int k = r[4]; k < r[5]; k++)
+          for (int j = r[2]; j < r[3]; j++)
+            for (int i = r[0]; i < r[1]; i++, q2++) {
+              double v = fld ? fld[cidx(s, i, j, k)] : 0.0;
+              re[q2] += v * ph[2 * f];
+              im[q2] += v * ph[2 * f + 1];
+            }
       }
+      slot++;
     }
-    s->step++;
+  }
+  /* ---- probes */
+  if (s->nprobes) {
+    if (s->step + 1 > s->ptrace_cap) {
+      int cap = s->ptrace_cap ? 2 * s->ptrace_cap : 1024;
+      while (cap < s->step + 1) cap *= 2;
+      s->ptrace = realloc(s->ptrace, sizeof(double) * cap * s->nprobes);
+      s->ptrace_cap = cap;
+    }
+    for (int p = 0; p < s->nprobes; p++) {
+      const double* f = field_ptr(s, s->probes[p].field, s->probes[p].comp, 0);
+      s->ptrace[(size_t)s->step * s->nprobes + p] = f ? f[s->probes[p].cell] : 0.0;
+    }
+  }
+  s->step++;
+}
+
+int rc_step(rc_sim* s, int nsteps) {
+  if (has_bloch(s) && !s->im) s->im = clone_part(s);
+  for (int it = 0; it < nsteps; it++) {
+    step_e(s);
+    if (s->im) {
+      step_e(s->im);
+      bloch_ghosts(s, 0);
+    }
+    step_h(s);
+    if (s->im) {
+      step_h(s->im);
+      bloch_ghosts(s, 1);
+    }
+    step_out(s);
+    if (s->im) step_out(s->im);
   }
   return 0;
 }

@@ -98,8 +98,13 @@ int rc_step(rc_sim* s, int nsteps);
 int rc_set_design(rc_sim* s, const int box[6], const double* eps_r);
 
 /* boundary conditions (BoundaryUpdates.hpp:17-26 names; numbering the build's) */
-enum { RC_BC_NONE = 0, RC_BC_PEC = 1, RC_BC_PMC = 2, RC_BC_SYMMETRIC = 3, RC_BC_ANTISYMMETRIC = 4, RC_BC_PERIODIC = 5 };
+enum { RC_BC_NONE = 0, RC_BC_PEC = 1, RC_BC_PMC = 2, RC_BC_SYMMETRIC = 3, RC_BC_ANTISYMMETRIC = 4, RC_BC_PERIODIC = 5,
+       RC_BC_BLOCH = 6 };
 int rc_set_boundary(rc_sim* s, int axis, int side, int type);
+/* Bloch-periodic face with phase factor fr + i fi: the fields become complex
+ * (field code | 16 reads the imaginary part; rc_get_probes_imag) */
+int rc_set_bloch_boundary(rc_sim* s, int axis, int side, double fr, double fi);
+int rc_get_probes_imag(const rc_sim* s, double* out, int max_steps);
 double rc_bc_mask(int type, int ft, int comp, int axis);
 /* the ghost-layer writes of one field type (0: E, 1: H), as done inside rc_step */
 void rc_apply_boundaries(rc_sim* s, int ft);

@@ -44,6 +44,8 @@ def lib():
             ("rc_set_material", _i, [_p, _i, _d, _d, _i, _dp]),
             ("rc_set_magnetized_drude", _i, [_p, _i, _d, _d, _d, _d, _dp]),
             ("rc_set_fluid_drude", _i, [_p, _i, _d, _i, _dp]),
+            ("rc_set_bloch_boundary", _i, [_p, _i, _i, _d, _d]),
+            ("rc_get_probes_imag", _i, [_p, _dp, _i]),
             ("rc_set_fluid_state", _i, [_p, _dp, _dp]),
             ("rc_set_matid", _i, [_p, ctypes.c_char_p]),
             ("rc_add_source", _i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
@@ -214,20 +216,25 @@ class Sim:
         lib().rc_add_probe(self.h, field, comp, i, j, k)
         self._nprobes = getattr(self, "_nprobes", 0) + 1
 
-    def probes(self, max_steps=1 << 20):
+    def probes(self, max_steps=1 << 20, imag=False):
         out = np.zeros(max_steps * self._nprobes) if max_steps < (1 << 20) else None
         if out is None:
             out = np.zeros(self._steps_done() * self._nprobes)
             max_steps = self._steps_done()
-        n = lib().rc_get_probes(self.h, _dptr(out), max_steps)
+        n = (lib().rc_get_probes_imag if imag else lib().rc_get_probes)(self.h, _dptr(out), max_steps)
+        assert n >= 0
         return out[: n * self._nprobes].reshape(n, self._nprobes)
+
+    def set_bloch_boundary(self, axis, side, factor):
+        f = complex(factor)
+        assert lib().rc_set_bloch_boundary(self.h, axis, side, f.real, f.imag) == 0
 
     def _steps_done(self):
         return getattr(self, "_nstep", 0)
 
-    def get(self, field, comp, pole=0):
+    def get(self, field, comp, pole=0, imag=False):
         out = np.zeros(int(np.prod(self.shape)))
-        if lib().rc_get_field(self.h, field, comp, pole, _dptr(out)) != 0:
+        if lib().rc_get_field(self.h, field | (16 if imag else 0), comp, pole, _dptr(out)) != 0:
             return None
         return out.reshape(self.shape)
 

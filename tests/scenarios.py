@@ -96,6 +96,9 @@ def from_golden(name):
         sc["matid"] = a["matid"].reshape(shape).astype(np.uint8)
     else:
         sc["matid"] = np.zeros(shape, np.uint8)
+    if "theta_x" in p:  # Bloch-periodic x and y: e^{-/+ i theta} at the min / max face
+        sc["bloch"] = [(ax, side, complex(math.cos(t), math.sin(t)) if side else complex(math.cos(t), -math.sin(t)))
+                       for ax, t in ((0, p["theta_x"]), (1, p["theta_y"])) for side in (0, 1)]
     if "fluid_ne" in a:
         sc["fluid"] = (a["fluid_ne"].reshape(shape), a["fluid_nu"].reshape(shape))
     if int(p.get("src_i", -1)) >= 0:
@@ -146,6 +149,8 @@ def apply(sim, sc):
         sim.set_design(*sc["design"])
     for axis, side, kind in sc.get("boundaries", ()):
         sim.set_boundary(axis, side, kind)
+    for axis, side, factor in sc.get("bloch", ()):
+        sim.set_bloch_boundary(axis, side, factor)
     return sim
 
 

@@ -258,3 +258,40 @@ def tm2d_boundaries(n=(48, 40), nsteps=200):
         "dfts": [], "tfsf": None,
         "boundaries": [(0, 0, "Periodic"), (0, 1, "Periodic"), (1, 0, "PEC"), (1, 1, "Symmetric")],
     }
+
+
+def tm2d_bloch(n=(40, 44), nsteps=150, theta=1.1):
+    """2-D TM unit cell: x Bloch-periodic (e^{-/+ i theta} at the min / max face),
+    10-cell CPML along y, a dielectric block, a Dz source and a 2-frequency DFT
+    line across the cell."""
+    nx, ny = n
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(2.0))
+    matid = np.zeros((1, ny, nx), np.uint8)
+    matid[0, 14:22, 6:19] = 1
+    f = complex(math.cos(theta), math.sin(theta))
+    return {
+        "name": "tm2d_bloch", "mode": "TM", "nx": nx, "ny": ny, "nz": 1, "dx": dx, "dt": dt, "nsteps": nsteps,
+        "pml": _pml(10, dx), "pml_axes": [False, True, False], "materials": [(1.0, 1.0, []), (2.25, 1.0, [])],
+        "matid": matid, "sources": [(D, 2, nx // 3, ny // 2, 0, gauss_table(nsteps, 25, 8))],
+        "probes": [(E, 2, 1, ny // 2, 0), (E, 2, nx - 2, ny // 3, 0), (H, 0, nx // 2, 12, 0), (H, 1, 1, ny - 13, 0)],
+        "dfts": [_dft_line(nx, ny // 2 + 5, dt, nsteps)], "tfsf": None,
+        "bloch": [(0, 0, f.conjugate()), (0, 1, f)],
+    }
+
+
+def _dft_line(nx, j, dt, nsteps):
+    """DFT monitor of Ez, Hx, Hy on row j at two frequencies (region, mask, phE, phH)."""
+    return dft_monitor([1, nx - 1, j, j + 1, 0, 1], (1 << 2) | (1 << 3) | (1 << 4), dt, nsteps)
+
+
+def dft_monitor(region, mask, dt, nsteps, freqs=(0.02, 0.035)):
+    """(region, mask, phE, phH) of a running DFT at the given frequencies (units of
+    1/dt): E sampled at (n+1) dt, H at (n+1.5) dt, weights e^{-i w t} dt."""
+    f = np.asarray(freqs) / dt
+    n = np.arange(nsteps)
+    phE = np.stack([np.cos(2 * np.pi * f[None, :] * (n[:, None] + 1) * dt) * dt,
+                    -np.sin(2 * np.pi * f[None, :] * (n[:, None] + 1) * dt) * dt], -1)
+    phH = np.stack([np.cos(2 * np.pi * f[None, :] * (n[:, None] + 1.5) * dt) * dt,
+                    -np.sin(2 * np.pi * f[None, :] * (n[:, None] + 1.5) * dt) * dt], -1)
+    return (list(region), mask, phE, phH)

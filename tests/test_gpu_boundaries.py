@@ -96,3 +96,69 @@ def test_boundary_errors():
     g.step(2)
     with pytest.raises(FdtdError, match="fixed"):
         g.set_boundary(0, 0, "PEC")
+
+
+def _bloch_scenario(name):
+    from scenarios import from_golden
+    if name == "tm2d_bloch":
+        return synth.tm2d_bloch(), None
+    sc, p, a = from_golden("grid3d_bloch")
+    sc["dfts"] = [synth.dft_monitor([1, sc["nx"] - 1, 1, sc["ny"] - 1, 10, 11], 63, sc["dt"], sc["nsteps"])]
+    return sc, a
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("name", ["grid3d_bloch", "tm2d_bloch"])
+def test_bloch_complex_fields(name, precision):
+    """Bloch-periodic faces on complex fields: real and imaginary parts of every
+    field, the probes and the complex DFT against the oracle (1e-12 / 1e-5), and
+    for the 3-D case against the reference's own complex run."""
+    sc, a = _bloch_scenario(name)
+    print(f"[gpu-case] bloch {name} {precision}", flush=True)
+    g = apply(_grid(sc, precision), sc)
+    o = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    g.step(sc["nsteps"])
+    o.step(sc["nsteps"])
+    for nm, f, c in _comps(sc):
+        for imag in (False, True):
+            e = rel_l2(g.get(f, c, imag=imag), o.get(f, c, imag=imag))
+            assert e <= TOL[precision], (nm, imag, e)
+            if a is not None:
+                ref = a[nm + ("_im" if imag else "_re")]
+                assert rel_l2(g.get(f, c, imag=imag).reshape(ref.shape), ref) <= TOL[precision], (nm, imag)
+    assert np.abs(o.get(*_comps(sc)[0][1:], imag=True)).max() > 0
+    for imag in (False, True):
+        assert rel_l2(g.probes(imag=imag), o.probes(imag=imag)) <= 10 * TOL[precision]
+    for idx in range(len(sc["dfts"])):
+        assert rel_l2(g.get_dft(idx), o.get_dft(idx)) <= 10 * TOL[precision]
+
+
+def test_bloch_schedule_invariance_and_errors():
+    """Every tile (fused requests fall back to two passes) and captured chunks
+    give the same bits for both parts; batches and the adjoint refuse complex
+    grids; the phase is fixed once stepping has started."""
+    from fdtd_amd import Batch, FdtdError
+    sc, _ = _bloch_scenario("grid3d_bloch")
+    ref = apply(_grid(sc, "f32"), sc)
+    ref.step(sc["nsteps"])
+    for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 2, 16), False), ((128, 2, 1), True)):
+        print(f"[gpu-case] bloch tile {tile} graph {graph}", flush=True)
+        g = apply(_grid(sc, "f32"), sc)
+        g.set_tile(*tile)
+        g.set_graph(graph)
+        g.step(37)
+        g.step(sc["nsteps"] - 37)
+        for nm, f, c in _comps(sc):
+            for imag in (False, True):
+                np.testing.assert_array_equal(g.get(f, c, imag=imag), ref.get(f, c, imag=imag), err_msg=f"{tile} {nm}")
+        np.testing.assert_array_equal(g.probes(imag=True), ref.probes(imag=True))
+        np.testing.assert_array_equal(g.get_dft(0), ref.get_dft(0))
+        g.close()
+    with pytest.raises(FdtdError, match="fixed"):
+        ref.set_bloch_boundary(0, 0, 1j)
+    other = apply(_grid(sc, "f32"), sc)
+    with pytest.raises(FdtdError, match="boundary"):
+        Batch([ref, other])
+    real = apply(_grid(synth.tm2d_boundaries(), "f32"), synth.tm2d_boundaries())
+    with pytest.raises(FdtdError, match="real fields"):
+        real.probes(imag=True)

@@ -106,3 +106,46 @@ def test_pec_and_pmc_walls_are_exact():
     for f in (Ex, Hy, Hz):
         assert np.all(f[:, :, -2] == 0.0)
         assert np.abs(f[:, :, -3]).max() > 0
+
+
+def test_bloch_complex_grid_matches_reference():
+    """Bloch-periodic x and y faces on complex fields (the reference's own
+    functors on std::complex<double> cells: make_bloch_periodic_boundary,
+    BoundaryUpdates.hpp:845-885, with curl, CPML, a dielectric and a Drude
+    block): the oracle's real and imaginary parts equal the reference's bit for
+    bit, fields and probe traces."""
+    from scenarios import from_golden, apply, E, H
+    sc, p, a = from_golden("grid3d_bloch")
+    o = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    o.step(sc["nsteps"])
+    for nm, f, c in (("Ex", E, 0), ("Ey", E, 1), ("Ez", E, 2), ("Hx", H, 0), ("Hy", H, 1), ("Hz", H, 2)):
+        np.testing.assert_array_equal(o.get(f, c), a[nm + "_re"], err_msg=nm)
+        np.testing.assert_array_equal(o.get(f, c, imag=True), a[nm + "_im"], err_msg=nm)
+    tr = a["probe_trace_E_H_reim"].reshape(sc["nsteps"], -1, 2)
+    np.testing.assert_array_equal(o.probes(), tr[:, :, 0])
+    np.testing.assert_array_equal(o.probes(imag=True), tr[:, :, 1])
+    assert np.abs(tr[:, :, 1]).max() > 1e-3 * np.abs(tr[:, :, 0]).max()  # the phase does reach the fields
+
+
+def test_bloch_tm_invariants():
+    """TM Bloch cell (synth.tm2d_bloch): with theta = 0 the complex grid is the
+    periodic real grid (imaginary part exactly zero); with theta = pi the
+    imaginary part stays zero too (the factor is -1); otherwise the phase
+    couples the parts, and the complex DFT is F_re + i F_im of the parts."""
+    import synth
+    from scenarios import apply, E
+    base = synth.tm2d_bloch(nsteps=80)
+    per = dict(base, bloch=[], boundaries=[(0, 0, "Periodic"), (0, 1, "Periodic")])
+    o0 = apply(refcpu.Sim("TM", base["nx"], base["ny"], 1, base["dx"], base["dt"]), dict(base, bloch=[(0, 0, 1.0), (0, 1, 1.0)]))
+    op = apply(refcpu.Sim("TM", base["nx"], base["ny"], 1, base["dx"], base["dt"]), per)
+    o0.step(80)
+    op.step(80)
+    np.testing.assert_array_equal(o0.get(E, 2), op.get(E, 2))
+    assert np.all(o0.get(E, 2, imag=True) == 0)
+    np.testing.assert_array_equal(o0.get_dft(0), op.get_dft(0))
+    om = apply(refcpu.Sim("TM", base["nx"], base["ny"], 1, base["dx"], base["dt"]), dict(base, bloch=[(0, 0, -1.0), (0, 1, -1.0)]))
+    om.step(80)
+    assert np.all(om.get(E, 2, imag=True) == 0) and np.abs(om.get(E, 2) - op.get(E, 2)).max() > 0
+    o = apply(refcpu.Sim("TM", base["nx"], base["ny"], 1, base["dx"], base["dt"]), base)
+    o.step(80)
+    assert np.abs(o.get(E, 2, imag=True)).max() > 1e-3 * np.abs(o.get(E, 2)).max()
