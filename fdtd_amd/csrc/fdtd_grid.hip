@@ -814,6 +814,8 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     a.dft_work += long(h.nfreq) * h.ncells;
     dd.cbase = a.dft_cells;
     a.dft_cells += h.ncells;
+    dd.sbase = a.dft_slots;
+    a.dft_slots += long(h.ncomp) * fdtdk::fold_segment(h.ncells);
     gput(dd.smp, (T*)h.smp);
   }
   a.nprobe = g->nprobe;
@@ -833,6 +835,7 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     a.ndft = 0;
     a.dft_work = 0;
     a.dft_cells = 0;
+    a.dft_slots = 0;
     a.nprobe = 0;
   }
 }
@@ -1035,7 +1038,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
     const auto* dev1 =
         reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
     HIP_TRY(hipMemcpyAsync((void*)dev0, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
-    const long dft_work = host->dft_work, dft_cells = host->dft_cells;
+    const long dft_work = host->dft_work, dft_cells = host->dft_cells, dft_slots = host->dft_slots;
     const long step0 = g->step;
     long bloch_cells = 0;
     const fdtdk::BlochFaces bfaces = bloch_faces(g, &bloch_cells);
@@ -1117,7 +1120,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
           const auto* dp = part ? dev1 : ap;
           rc = launch(g, 4, fdtdk::dft_sample<T>, dim3(unsigned((dft_cells + 255) / 256)), dim3(256), dp, s);
           if (!rc && (step0 + s + 1) % fdtdk::kDftBlock == 0)
-            rc = launch(g, 4, fdtdk::dft_fold_step<T>, dim3(unsigned((dft_work + 255) / 256)), dim3(256), dp, s);
+            rc = launch(g, 4, fdtdk::dft_fold_step<T>, dim3(unsigned((dft_slots + 255) / 256)), dim3(256), dp, s);
         }
         if (rc) return rc;
       }
@@ -1184,7 +1187,7 @@ int dft_flush_impl(fdtd_grid* g) {
       reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
   HIP_TRY(hipMemcpyAsync((void*)dev, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
   for (int part = 0; part < (g->cplx ? 2 : 1); part++) {
-    int rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((a.dft_work + 255) / 256)), dim3(256), part ? dev1 : dev,
+    int rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((a.dft_slots + 255) / 256)), dim3(256), part ? dev1 : dev,
                     g->dft_done, g->step);
     if (rc) return rc;
   }
@@ -1810,6 +1813,7 @@ int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, co
   d.nphase = nphase;
   d.ncomp = __builtin_popcount(comp_mask);
   d.ncells = long(r[1] - r[0]) * (r[3] - r[2]) * (r[5] - r[4]);
+  if (d.ncells * d.ncomp * fdtdk::kDftBlock >= (1L << 31)) return fail(FDTD_EINVAL, "DFT monitor too large");
   int rc = g->alloc(&d.acc, size_t(d.ncomp) * nfreq * 2 * d.ncells * g->esz);
   if (!rc) rc = g->alloc(&d.smp, size_t(fdtdk::kDftBlock) * d.ncomp * d.ncells * g->esz);
   if (!rc) rc = g->alloc((void**)&d.phE, size_t(std::max(nphase, 1)) * nfreq * 2 * 8);

@@ -86,6 +86,7 @@ struct DftDev {
   int nphase;
   long base;      // offset of this monitor's (frequency, cell) items in the DFT launch
   long cbase;     // offset of this monitor's cells in the sampling launch
+  long sbase;     // offset of this monitor's (component, padded cell) items in the fold launch
   FDTD_G T* smp;  // [kDftBlock][ncomp][ncells]: samples of steps not yet folded in
 };
 
@@ -152,6 +153,7 @@ struct StepArgs {
   int ndft;
   long dft_work;   // sum over monitors of nfreq x cells
   long dft_cells;  // sum over monitors of cells
+  long dft_slots;  // the fold's items: per monitor, components x its cells padded to whole blocks
   long dft_done0;  // steps folded into the accumulators when this chunk started
   DftDev<T> dft[kMaxDft];
   // probes
@@ -651,13 +653,13 @@ __global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict_
   int q = 0;
   while (q + 1 < a.ndft && g >= a.dft[q + 1].cbase) q++;
   const DftDev<T>& d = a.dft[q];
-  const long lq = g - d.cbase;
+  const int lq = (int)(g - d.cbase);  // a monitor holds fewer than 2^31 cells (host-checked): 32-bit index math
   const int wx = d.r[1] - d.r[0], wy = d.r[3] - d.r[2];
-  const int i = d.r[0] + (int)(lq % wx);
-  const int j = d.r[2] + (int)((lq / wx) % wy);
-  const int k = d.r[4] + (int)(lq / ((long)wx * wy));
+  const int lj = lq / wx, i = d.r[0] + (lq - lj * wx);
+  const int lk = lj / wy, j = d.r[2] + (lj - lk * wy);
+  const int k = d.r[4] + lk;
   const long c = (long)i + a.sy * (long)j + a.sz * (long)k;
-  const long slot0 = ((a.step0 + s) % kDftBlock) * d.ncomp;
+  const long slot0 = (long)((int)((a.step0 + s) % kDftBlock) * d.ncomp);
   int slot = 0;
 #pragma unroll
   for (int comp = 0; comp < 6; comp++) {
@@ -668,30 +670,74 @@ __global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict_
   }
 }
 
+// The fold: one thread per (component, monitor cell), cells fastest; each
+// (monitor, component) segment is padded to whole 256-thread blocks, so a
+// block stages the block's phase rows of its component's table in LDS once
+// and every lane reads them from there.  Each sample is read once and folded
+// into every frequency of its accumulators, four frequencies at a time
+// (independent chains; each accumulator takes the samples in step order).
+constexpr int kFoldBlock = 256;
+constexpr int kFoldLdsFreq = 64;  // phase rows staged in LDS up to this many frequencies
+__host__ __device__ inline long fold_segment(long ncells) { return (ncells + kFoldBlock - 1) / kFoldBlock * kFoldBlock; }
+
 template <typename T>
 __device__ __forceinline__ void dft_fold_items(const StepArgs<T>& a, long m0, long m1) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.dft_work) return;
+  __shared__ double sph[kDftBlock * kFoldLdsFreq * 2];
+  const long b0 = (long)blockIdx.x * kFoldBlock;  // block-uniform: one (monitor, component) per block
+  if (b0 >= a.dft_slots) return;
   int q = 0;
-  while (q + 1 < a.ndft && g >= a.dft[q + 1].base) q++;
+  while (q + 1 < a.ndft && b0 >= a.dft[q + 1].sbase) q++;
   const DftDev<T>& d = a.dft[q];
-  const long r = g - d.base;
-  const int f = (int)(r / d.ncells);
-  const long lq = r - (long)f * d.ncells;
+  const long seg = fold_segment(d.ncells);
+  const int slot = (int)((b0 - d.sbase) / seg);
+  const int lq = (int)(b0 - d.sbase - (long)slot * seg) + (int)threadIdx.x;
+  const int nc = (int)d.ncells;
   const long mend = m1 < (long)d.nphase ? m1 : (long)d.nphase;
-  for (int slot = 0; slot < d.ncomp; slot++) {
-    const bool efield = comp_of_slot(d.mask, slot) < 3;
-    const FDTD_G double* ph = efield ? d.phE : d.phH;
-    FDTD_G T* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
-    T ar = re[0], ai = re[d.ncells];
-    for (long m = m0; m < mend; m++) {
-      const T v = d.smp[((m % kDftBlock) * d.ncomp + slot) * d.ncells + lq];
-      const FDTD_G double* pm = ph + 2L * (d.nfreq * m + f);
-      ar += v * (T)pm[0];
-      ai += v * (T)pm[1];
+  int nm = mend > m0 ? (int)(mend - m0) : 0;
+  if (nm > kDftBlock) nm = kDftBlock;
+  const int r0 = (int)(m0 % kDftBlock);
+  const int nf = d.nfreq;
+  const bool efield = comp_of_slot(d.mask, slot) < 3;
+  const FDTD_G double* ph = (efield ? d.phE : d.phH) + 2L * nf * m0;
+  const bool lds = nf <= kFoldLdsFreq;
+  if (lds) {
+    for (int x = threadIdx.x; x < nm * nf * 2; x += kFoldBlock) sph[x] = ph[x];
+    __syncthreads();
+  }
+  if (lq >= nc) return;
+  T v[kDftBlock];
+#pragma unroll
+  for (int t = 0; t < kDftBlock; t++)
+    v[t] = t < nm ? d.smp[(long)(((r0 + t) % kDftBlock) * d.ncomp + slot) * nc + lq] : T(0);
+  FDTD_G T* acc = d.acc + (long)slot * nf * 2 * nc + lq;
+  for (int f0 = 0; f0 < nf; f0 += 4) {
+    const int nu = nf - f0 < 4 ? nf - f0 : 4;
+    T ar[4], ai[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (u < nu) {
+        ar[u] = acc[(long)(f0 + u) * 2 * nc];
+        ai[u] = acc[(long)(f0 + u) * 2 * nc + nc];
+      }
+#pragma unroll
+    for (int t = 0; t < kDftBlock; t++) {
+      if (t < nm) {
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (u < nu) {
+            const int x = (t * nf + f0 + u) * 2;
+            const double pr = lds ? sph[x] : ph[x], pi = lds ? sph[x + 1] : ph[x + 1];
+            ar[u] += v[t] * (T)pr;
+            ai[u] += v[t] * (T)pi;
+          }
+      }
     }
-    re[0] = ar;
-    re[d.ncells] = ai;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (u < nu) {
+        acc[(long)(f0 + u) * 2 * nc] = ar[u];
+        acc[(long)(f0 + u) * 2 * nc + nc] = ai[u];
+      }
   }
 }
 

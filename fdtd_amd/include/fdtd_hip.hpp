@@ -182,6 +182,16 @@ class YeeGrid {
     return v;
   }
 
+  // the imaginary part of the probes of a complex grid (a Bloch-periodic face)
+  std::vector<double> probes_imag() const {
+    long n = fdtd_step_count(handle());
+    std::vector<double> v(size_t(std::max(1L, n)) * std::max(1, nprobes_));
+    int got = 0;
+    check(fdtd_get_probes_imag(handle(), v.data(), int(n), &got), "fdtd_get_probes_imag");
+    v.resize(size_t(got) * nprobes_);
+    return v;
+  }
+
   void setMaterialIds(const std::vector<uint8_t>& ids) { check(fdtd_set_matid(handle(), ids.data()), "fdtd_set_matid"); }
   // the cells' ne() and nu_m() read by the fluid Drude materials (nx*ny*nz each)
   void setFluidState(const std::vector<double>& ne, const std::vector<double>& nu_m) {

@@ -67,3 +67,20 @@ def test_plasmas_through_cpp_host_surface(name):
     ref = a["probe_trace_E_H"].reshape(got.shape)
     assert np.abs(ref).max() > 0
     assert rel_l2(got, ref) <= 1e-12
+
+
+def test_bloch_through_cpp_host_surface():
+    """make_bloch_periodic_boundary on the C++ Stepper (complex fields): real and
+    imaginary probe traces equal the reference's own complex run (golden fixture
+    grid3d_bloch) to 1e-12."""
+    exe = os.path.join(os.path.dirname(BIN), "grid3d_bloch_host_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/bin/grid3d_bloch_host_api not built (run __graft_entry__.build())")
+    print("[gpu-case] grid3d_bloch_host_api", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
+    p, a = load("grid3d_bloch")
+    tr = a["probe_trace_E_H_reim"].reshape(got.shape[0], -1, 2)
+    assert rel_l2(got[:, :24], tr[:, :, 0]) <= 1e-12
+    assert rel_l2(got[:, 24:], tr[:, :, 1]) <= 1e-12
