@@ -2336,6 +2336,9 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
     auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + slot * n * sizeof(fdtdk::StepArgs<T>));
     HIP_TRY(hipMemcpyAsync((void*)dev, host, n * sizeof(fdtdk::StepArgs<T>), hipMemcpyHostToDevice, b->stream));
     const dim3 block(b->tile.bx, b->tile.by, 1), grid(b->ntiles, 1, 1);
+    // the E-pass instance without the ADE code when no grid is dispersive
+    bool disp = false;
+    for (fdtd_grid* g : b->grids) disp = disp || g->max_np > 0;
     if (b->use_graph) {
       auto key = std::make_pair(slot, chunk);
       auto it = b->graphs.find(key);
@@ -2343,7 +2346,10 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
         hipGraph_t graph;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         for (int s = 0; s < chunk; s++) {
-          hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+          if (disp)
+            hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+          else
+            hipLaunchKernelGGL((fdtdk::e_pass_batch<T, 0>), grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
           hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
           if (b->dft_total)
             hipLaunchKernelGGL(fdtdk::dft_accumulate_batch<T>, dim3(unsigned((b->dft_total + 255) / 256)), dim3(256), 0,
@@ -2359,7 +2365,10 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
       HIP_TRY(hipGraphLaunch(it->second, b->stream));
     } else {
       for (int s = 0; s < chunk; s++) {
-        hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+        if (disp)
+          hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+        else
+          hipLaunchKernelGGL((fdtdk::e_pass_batch<T, 0>), grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
         hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
         if (b->dft_total)
           hipLaunchKernelGGL(fdtdk::dft_accumulate_batch<T>, dim3(unsigned((b->dft_total + 255) / 256)), dim3(256), 0,

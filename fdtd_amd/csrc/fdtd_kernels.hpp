@@ -1086,7 +1086,8 @@ __global__ __launch_bounds__(256) void surface_quantities(const T* __restrict__ 
 // (grid index, tile x, tile y, tile z) of a table built on the host, in the
 // XCD-aware order of the single-grid passes, each grid reading its own
 // arguments.  Same cell arithmetic as above.
-template <typename T>
+// DISP: 0 when no grid of the batch holds a dispersive material (no ADE code)
+template <typename T, int DISP = 1>
 __global__ __launch_bounds__(256) void e_pass_batch(const StepArgs<T>* __restrict__ args, const int4* __restrict__ tiles,
                                                     int kc, int s) {
   const int4 t = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -1094,8 +1095,8 @@ __global__ __launch_bounds__(256) void e_pass_batch(const StepArgs<T>* __restric
   __shared__ MatLds<T> sm;
   stage_materials(a, sm);
   const int k0 = 1 + t.w * kc;
-  e_pass_cells(a, sm.ie, t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0, min(k0 + kc, a.nz - 1),
-               s);
+  e_pass_cells<T, false, DISP>(a, sm.ie, t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0,
+                               min(k0 + kc, a.nz - 1), s);
 }
 
 template <typename T>
