@@ -255,9 +255,9 @@ __device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<
 // launched like the forward passes: ntx x nty x nkc tiles, XCD-aware order,
 // the H pass walking each XCD's range in reverse
 template <typename T>
-__global__ __launch_bounds__(256) void adj_e_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
-                                                     int kc, int ntx, int nty, int nkc, long step,
-                                                     const T* __restrict__ hist) {
+__device__ __forceinline__ void adj_e_pass_body(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
+                                                int kc, int ntx, int nty, int nkc, long step,
+                                                const T* __restrict__ hist) {
   __shared__ MatLds<T> sm;
   stage_materials(*ap, sm);
   const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
@@ -266,10 +266,9 @@ __global__ __launch_bounds__(256) void adj_e_pass_3d(const StepArgs<T>* __restri
   adj_e_cells(*ap, *qp, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
               min(k0 + kc, ap->nz - 1), step, hist);
 }
-
 template <typename T>
-__global__ __launch_bounds__(256) void adj_h_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
-                                                     int kc, int ntx, int nty, int nkc) {
+__device__ __forceinline__ void adj_h_pass_body(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
+                                                int kc, int ntx, int nty, int nkc) {
   __shared__ MatLds<T> sm;
   stage_materials(*ap, sm);
   const int t = xcd_tile_rev(blockIdx.x, ntx * nty * nkc);
@@ -277,6 +276,18 @@ __global__ __launch_bounds__(256) void adj_h_pass_3d(const StepArgs<T>* __restri
   const int k0 = 1 + bz * kc;
   adj_h_cells(*ap, *qp, sm.im, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
               min(k0 + kc, ap->nz - 1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void adj_e_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
+                                                     int kc, int ntx, int nty, int nkc, long step,
+                                                     const T* __restrict__ hist) {
+  adj_e_pass_body(ap, qp, kc, ntx, nty, nkc, step, hist);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void adj_h_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
+                                                     int kc, int ntx, int nty, int nkc) {
+  adj_h_pass_body(ap, qp, kc, ntx, nty, nkc);
 }
 
 }  // namespace fdtdk
