@@ -947,6 +947,11 @@ int run_aux(fdtd_grid* g, long step0, int nsteps) {
 
 template <typename T>
 int step_impl(fdtd_grid* g, int nsteps) {
+  if (g->cplx)
+    for (int id = 0; id < FDTD_MAX_MATERIALS; id++)
+      if (g->mats[id].np > 0 && g->mats[id].mag == 2 && g->ndim == 3)
+        return fail(FDTD_EINVAL,
+                    "a self-magnetized material is nonlinear in the field: complex (Bloch) grids cannot hold one");
   if (g->mats_dirty) {
     int rc = upload_materials<T>(g);
     if (rc) return rc;

@@ -212,7 +212,10 @@ int fdtd_set_boundary(fdtd_grid* g, int axis, int side, int kind);
  * the reference's decorator (mask 1 for every component).  The imaginary
  * part is read with FDTD_IMAG and fdtd_get_probes_imag; fdtd_get_dft returns
  * the spectrum of the complex field.  Before stepping; complex grids run the
- * two-pass schedule and cannot be batched or differentiated. */
+ * two-pass schedule and cannot be batched or differentiated.  The two parts
+ * are stepped separately, which needs every update linear with real
+ * coefficients: a 3-D grid with a self-magnetized (Drude or fluid) material
+ * is refused. */
 int fdtd_set_bloch_boundary(fdtd_grid* g, int axis, int side, double factor_re, double factor_im);
 
 /* Capture each chunk of steps (up to 256) in a hipGraph once and replay it

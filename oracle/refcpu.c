@@ -1146,7 +1146,12 @@ static void step_out(rc_sim* s) {
 }
 
 int rc_step(rc_sim* s, int nsteps) {
-  if (has_bloch(s) && !s->im) s->im = clone_part(s);
+  if (has_bloch(s)) {
+    /* the parts are stepped apart: only updates linear with real coefficients */
+    for (int m = 0; m < RC_MAXMAT; m++)
+      if (s->mat[m].np && s->mat[m].mag == 2 && s->mode == RC_3D) return -1;
+    if (!s->im) s->im = clone_part(s);
+  }
   for (int it = 0; it < nsteps; it++) {
     step_e(s);
     if (s->im) {

@@ -162,3 +162,16 @@ def test_bloch_schedule_invariance_and_errors():
     real = apply(_grid(synth.tm2d_boundaries(), "f32"), synth.tm2d_boundaries())
     with pytest.raises(FdtdError, match="real fields"):
         real.probes(imag=True)
+
+
+def test_bloch_refuses_self_magnetized_material():
+    """The real and imaginary parts are stepped apart, which holds only for
+    updates linear with real coefficients: a self-magnetized plasma (its bias is
+    the complex field itself) on a complex grid is refused."""
+    from fdtd_amd import FdtdError
+    sc, _ = _bloch_scenario("grid3d_bloch")
+    print("[gpu-case] bloch self-magnetized refusal", flush=True)
+    g = apply(_grid(sc, "f32"), sc)
+    g.set_magnetized_drude(2, 1.0, 1.0, 1.37e16, 1e14, None)
+    with pytest.raises(FdtdError, match="self-magnetized"):
+        g.step(1)
