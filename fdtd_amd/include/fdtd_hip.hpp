@@ -117,10 +117,39 @@ struct PMLParameterModel {
 template <class Mode, class scalar = double>
 class YeeGrid;
 
+// One cell of a grid, with the reference's cell accessors (YeeFields.hpp:46-78,
+// DispersiveMaterials.hpp:62-77, DefaultInterfaces.hpp:65-79, PML.hpp:84-102)
+// over the grid's host mirror: E and H are writable (copied to the device by
+// YeeGrid::sync_device), D, B, P, J (pole 0) are read-only views of the state
+// YeeGrid::sync_host copied last.  Neighbours are index arithmetic, not links.
 template <class Mode, class scalar>
 struct CellRef {
   YeeGrid<Mode, scalar>* grid;
   int i, j, k;
+  double& Ex() const;
+  double& Ey() const;
+  double& Ez() const;
+  double& Hx() const;
+  double& Hy() const;
+  double& Hz() const;
+  double Dx() const;
+  double Dy() const;
+  double Dz() const;
+  double Bx() const;
+  double By() const;
+  double Bz() const;
+  double Px() const;
+  double Py() const;
+  double Pz() const;
+  double Jx() const;
+  double Jy() const;
+  double Jz() const;
+  CellRef getNeighborMin(int d) const;
+  CellRef getNeighborMax(int d) const;
+  // NewPML::setPMLParameters of this cell: the grid keeps per-axis profiles,
+  // so this sets them for every cell sharing the cell's coordinate along d
+  template <FieldType ft, Dir d>
+  void setPMLParameters(double K, double S, double A, double dt) const;
 };
 
 template <class Mode, class scalar>
@@ -193,6 +222,29 @@ class YeeGrid {
   }
 
   void setMaterialIds(const std::vector<uint8_t>& ids) { check(fdtd_set_matid(handle(), ids.data()), "fdtd_set_matid"); }
+
+  // Host mirror (explicit sync): sync_host copies E, H, D, B, P, J (pole 0)
+  // of every existing component to the host; sync_device copies the mirror's
+  // E and H back.  Cell accessors read and write the mirror only.
+  void sync_host() {
+    const int fields[6] = {FDTD_E, FDTD_H, FDTD_D, FDTD_B, FDTD_P, FDTD_J};
+    for (int f : fields)
+      for (int c = 0; c < 3; c++) {
+        auto& m = mirror_[f][c];
+        m.assign(size_t(nx_) * ny_ * nz_, 0.0);
+        if (fdtd_download_field(handle(), f, c, 0, m.data()) != FDTD_OK) m.clear();  // component absent in this mode
+      }
+  }
+  void sync_device() {
+    for (int f : {FDTD_E, FDTD_H})
+      for (int c = 0; c < 3; c++)
+        if (!mirror_[f][c].empty()) upload(f, c, mirror_[f][c]);
+  }
+  double& mirror(int field, int comp, int i, int j, int k) {
+    auto& m = mirror_[field][comp];
+    if (m.empty()) throw Error("cell access: component absent in this mode, or sync_host() not called");
+    return m[size_t(i) + size_t(nx_) * (size_t(j) + size_t(ny_) * k)];
+  }
   // the cells' ne() and nu_m() read by the fluid Drude materials (nx*ny*nz each)
   void setFluidState(const std::vector<double>& ne, const std::vector<double>& nu_m) {
     if (ne.size() != size_t(nx_) * ny_ * nz_ || nu_m.size() != ne.size()) throw Error("setFluidState: size mismatch");
@@ -236,7 +288,58 @@ class YeeGrid {
   std::vector<double> ksa_[2][3];
   bool pml_dirty_[3] = {false, false, false};
   int nprobes_ = 0;
+  std::vector<double> mirror_[6][3];  // [field code][component]
 };
+
+template <class Mode, class scalar>
+double& CellRef<Mode, scalar>::Ex() const { return grid->mirror(FDTD_E, 0, i, j, k); }
+template <class Mode, class scalar>
+double& CellRef<Mode, scalar>::Ey() const { return grid->mirror(FDTD_E, 1, i, j, k); }
+template <class Mode, class scalar>
+double& CellRef<Mode, scalar>::Ez() const { return grid->mirror(FDTD_E, 2, i, j, k); }
+template <class Mode, class scalar>
+double& CellRef<Mode, scalar>::Hx() const { return grid->mirror(FDTD_H, 0, i, j, k); }
+template <class Mode, class scalar>
+double& CellRef<Mode, scalar>::Hy() const { return grid->mirror(FDTD_H, 1, i, j, k); }
+template <class Mode, class scalar>
+double& CellRef<Mode, scalar>::Hz() const { return grid->mirror(FDTD_H, 2, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Dx() const { return grid->mirror(FDTD_D, 0, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Dy() const { return grid->mirror(FDTD_D, 1, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Dz() const { return grid->mirror(FDTD_D, 2, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Bx() const { return grid->mirror(FDTD_B, 0, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::By() const { return grid->mirror(FDTD_B, 1, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Bz() const { return grid->mirror(FDTD_B, 2, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Px() const { return grid->mirror(FDTD_P, 0, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Py() const { return grid->mirror(FDTD_P, 1, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Pz() const { return grid->mirror(FDTD_P, 2, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Jx() const { return grid->mirror(FDTD_J, 0, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Jy() const { return grid->mirror(FDTD_J, 1, i, j, k); }
+template <class Mode, class scalar>
+double CellRef<Mode, scalar>::Jz() const { return grid->mirror(FDTD_J, 2, i, j, k); }
+template <class Mode, class scalar>
+CellRef<Mode, scalar> CellRef<Mode, scalar>::getNeighborMin(int d) const {
+  return {grid, i - (d == 0), j - (d == 1), k - (d == 2)};
+}
+template <class Mode, class scalar>
+CellRef<Mode, scalar> CellRef<Mode, scalar>::getNeighborMax(int d) const {
+  return {grid, i + (d == 0), j + (d == 1), k + (d == 2)};
+}
+template <class Mode, class scalar>
+template <FieldType ft, Dir d>
+void CellRef<Mode, scalar>::setPMLParameters(double K, double S, double A, double) const {
+  grid->template setPMLParameters<ft, d>(d == Dir::X ? i : (d == Dir::Y ? j : k), K, S, A);
+}
 
 // --------------------------------------------------------------- functors
 // Value types naming one stage of the step; the Stepper checks the order.

@@ -84,3 +84,48 @@ def test_bloch_through_cpp_host_surface():
     tr = a["probe_trace_E_H_reim"].reshape(got.shape[0], -1, 2)
     assert rel_l2(got[:, :24], tr[:, :, 0]) <= 1e-12
     assert rel_l2(got[:, 24:], tr[:, :, 1]) <= 1e-12
+
+
+def test_cell_proxies_through_cpp_host_surface():
+    """The reference's per-cell idiom on the C++ host surface: CPML placed with
+    cell.setPMLParameters, the initial field written through cell.Ez() on the host
+    mirror, the result read by walking getNeighborMax(0): equal to the oracle's
+    run of the same scenario (1e-12), and Dz = eps_r eps0 Ez in the block."""
+    import math
+    from oracle import refcpu
+    from scenarios import D, E, H, rel_l2
+    exe = os.path.join(os.path.dirname(BIN), "tm_cell_proxy_host_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/bin/tm_cell_proxy_host_api not built (run __graft_entry__.build())")
+    print("[gpu-case] tm_cell_proxy_host_api", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    row = np.array([float(x) for x in lines[0].split()])
+    hx, hy = (float(x) for x in lines[1].split())
+    dz, ez = (float(x) for x in lines[2].split())
+    nx, ny, dx = 40, 36, 1e-6
+    c0 = refcpu.constants()["c0"]
+    dt = 0.5 * dx / (c0 * math.sqrt(2.0))
+    eps0 = refcpu.constants()["eps0"]
+    o = refcpu.Sim("TM", nx, ny, 1, dx, dt)
+    smax = 0.8 * (3.0 + 1) / (refcpu.constants()["imp0"] * dx)  # PMLParameterModel(dx) defaults
+    for ax in (0, 1):
+        o.set_pml_idiom(ax, 8, 3.0, 1.0, smax, 5.0, 0.05)
+    o.set_material(1, 2.0)
+    ids = np.zeros((1, ny, nx), np.uint8)
+    ids[0, 20:26, 24:30] = 1
+    o.set_matid(ids)
+    jj, ii = np.mgrid[0:ny, 0:nx]
+    e0 = np.exp(-((ii - 20.0) ** 2 + (jj - 18.0) ** 2) / 16.0)
+    e0[[0, -1], :] = 0.0
+    e0[:, [0, -1]] = 0.0
+    o.set(E, 2, e0[None])
+    o.set(D, 2, (e0 * eps0 * np.where(ids[0] == 1, 2.0, 1.0))[None])
+    o.step(60)
+    ez_o = o.get(E, 2)[0]
+    assert rel_l2(row, ez_o[18, 1:nx - 1]) <= 1e-12
+    assert abs(hx - o.get(H, 0)[0, 18, 20]) <= 1e-12 * np.abs(o.get(H, 0)).max()
+    assert abs(hy - o.get(H, 1)[0, 18, 20]) <= 1e-12 * np.abs(o.get(H, 1)).max()
+    assert abs(ez - ez_o[22, 26]) <= 1e-12 * np.abs(ez_o).max()
+    assert abs(dz - 2.0 * eps0 * ez) <= 1e-12 * abs(dz)
