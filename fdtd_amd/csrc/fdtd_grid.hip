@@ -813,7 +813,7 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     dd.base = a.dft_work;
     a.dft_work += long(h.nfreq) * h.ncells;
     dd.cbase = a.dft_cells;
-    a.dft_cells += h.ncells;
+    a.dft_cells += fdtdk::fold_segment(h.ncells);  // whole 256-thread blocks per monitor
     dd.sbase = a.dft_slots;
     a.dft_slots += long(h.ncomp) * fdtdk::fold_segment(h.ncells);
     gput(dd.smp, (T*)h.smp);

@@ -152,7 +152,7 @@ struct StepArgs {
   // running DFT
   int ndft;
   long dft_work;   // sum over monitors of nfreq x cells
-  long dft_cells;  // sum over monitors of cells
+  long dft_cells;  // the sampler's items: per monitor, its cells padded to whole 256-thread blocks
   long dft_slots;  // the fold's items: per monitor, components x its cells padded to whole blocks
   long dft_done0;  // steps folded into the accumulators when this chunk started
   DftDev<T> dft[kMaxDft];
@@ -645,15 +645,18 @@ __global__ __launch_bounds__(256) void dft_accumulate(const StepArgs<T>* __restr
 // the accumulators, one thread per (monitor, frequency, cell) keeping its
 // accumulators in registers across the block.  The additions happen in the
 // same order as step-by-step accumulation, so the sums are bitwise equal.
+// Each monitor's cells are padded to whole 256-thread blocks in the launch,
+// so a block's monitor (and its fields) are block-uniform scalar loads.
 template <typename T>
 __global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict__ ap, int s) {
   const StepArgs<T>& a = *ap;
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.dft_cells) return;
+  const long b0 = (long)blockIdx.x * 256;
+  if (b0 >= a.dft_cells) return;
   int q = 0;
-  while (q + 1 < a.ndft && g >= a.dft[q + 1].cbase) q++;
+  while (q + 1 < a.ndft && b0 >= a.dft[q + 1].cbase) q++;
   const DftDev<T>& d = a.dft[q];
-  const int lq = (int)(g - d.cbase);  // a monitor holds fewer than 2^31 cells (host-checked): 32-bit index math
+  const int lq = (int)(b0 - d.cbase) + (int)threadIdx.x;  // fewer than 2^31 cells (host-checked): 32-bit index math
+  if (lq >= d.ncells) return;
   const int wx = d.r[1] - d.r[0], wy = d.r[3] - d.r[2];
   const int lj = lq / wx, i = d.r[0] + (lq - lj * wx);
   const int lk = lj / wy, j = d.r[2] + (lj - lk * wy);
