@@ -285,14 +285,16 @@ def run_c5(args):
     t_fwd, _ = wall(lambda: (g.step(nsteps), g.synchronize()))
     g.close()
     two_pass = tile if tile[0] > 1 else (256, 1, 1)
-    # the gradient's own tile: with the recompute running beside the reverse
-    # steps, deeper z chunks (fewer, longer workgroups) share the GPU better;
-    # best of three by a short gradient run
-    trial = min(300, nsteps)
+    # the gradient's forward pass runs at the forward's tile; its reverse sweep
+    # and recompute at their own tile: with the recompute running beside the
+    # reverse steps, other shapes share the GPU better; best of four by a
+    # short gradient run
+    trial = min(600, nsteps)
     best = None
-    for cand in dict.fromkeys([two_pass, (128, 2, 4), (64, 4, 4)]):
+    for cand in dict.fromkeys([two_pass, (128, 2, 4), (64, 4, 4), (64, 2, 4)]):
         g = mk()
-        g.set_tile(*cand)
+        g.set_tile(*two_pass)
+        g.set_adjoint_tile(*cand)
         t, _ = wall(lambda: g.adjoint_eps_gradient(trial, 0, 4, checkpoint_every=100, history_mb=0.0))
         g.close()
         if best is None or t < best[0]:
@@ -302,11 +304,13 @@ def run_c5(args):
     for name, K, mb in (("checkpointed", 100, 0.0), ("history_resident", 100, 1e5)):
         # timed run (no per-kernel events), then a profiled run for the kernel averages
         g = mk()
-        g.set_tile(*adj_tile)
+        g.set_tile(*two_pass)
+        g.set_adjoint_tile(*adj_tile)
         t, (J, grad) = wall(lambda: g.adjoint_eps_gradient(nsteps, 0, 4, checkpoint_every=K, history_mb=mb))
         g.close()
         g = mk()
-        g.set_tile(*adj_tile)
+        g.set_tile(*two_pass)
+        g.set_adjoint_tile(*adj_tile)
         g.set_profiling(True)
         g.adjoint_eps_gradient(nsteps, 0, 4, checkpoint_every=K, history_mb=mb)
         avg = {}
@@ -325,7 +329,8 @@ def run_c5(args):
 
     def mkd(e):
         q = apply(Grid("ThreeD", n, n, n, sc["dx"], sc["dt"], precision="f32"), dict(sc, design=(box, e)))
-        q.set_tile(*adj_tile)
+        q.set_tile(*two_pass)
+        q.set_adjoint_tile(*adj_tile)
         return q
 
     t_seq = 0.0

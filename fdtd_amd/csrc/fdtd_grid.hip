@@ -260,6 +260,7 @@ struct fdtd_grid {
   long inc_rows = 0;
 
   Tile tile = {0, 0, 0};
+  Tile rtile = {0, 0, 0};  // the adjoint gradient's reverse sweep and recompute (0: the grid's tile)
   bool finalized = false;
   bool coef_dirty = true;
   bool mats_dirty = true;
@@ -1463,7 +1464,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   const auto* ap = (const fdtdk::StepArgs<T>*)adev;
   const auto* qp = (const fdtdk::AdjArgs<T>*)qdev;
 
-  const Tile t = (g->tile.by && g->tile.bx >= 2) ? g->tile : kDefaultTile3D;
+  const Tile t = g->rtile.by ? g->rtile : ((g->tile.by && g->tile.bx >= 2) ? g->tile : kDefaultTile3D);
   const int nzi = g->d.nz - 2;
   const dim3 block(t.bx, t.by, 1);
   const int ntx = (g->d.nx + t.bx - 1) / t.bx, nty = (g->d.ny - 2 + t.by - 1) / t.by, nkc = (nzi + t.kc - 1) / t.kc;
@@ -1502,7 +1503,10 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
     g->hist = X[b];
     g->hist_step0 = n0;
     g->hist_rows = len;
+    const Tile tile_saved = g->tile;
+    if (g->rtile.by) g->tile = g->rtile;
     if (!r) r = step_impl<T>(g, len);
+    g->tile = tile_saved;
     g->step = step_saved;
     g->recompute = false;
     g->hist = nullptr;
@@ -1988,6 +1992,18 @@ int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
     return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
   }
   g->tile = Tile{bx, by, kc};
+  return 0;
+}
+
+int fdtd_set_adjoint_tile(fdtd_grid* g, int bx, int by, int kc) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (bx == 0 && by == 0 && kc == 0) {
+    g->rtile = Tile{0, 0, 0};
+    return 0;
+  }
+  if (bx < 2 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
+    return fail(FDTD_EINVAL, "the reverse sweep takes a two-pass tile: a multiple of 64 threads, at most 256, kc >= 1");
+  g->rtile = Tile{bx, by, kc};
   return 0;
 }
 

@@ -28,7 +28,7 @@ ABI_SYMBOLS = [
     "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_dft", "fdtd_get_dft",
     "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
-    "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune",
+    "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune", "fdtd_set_adjoint_tile",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
     "fdtd_set_design", "fdtd_adjoint_eps_gradient", "fdtd_adjoint_eps_gradient_batch",
     "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph", "fdtd_set_boundary",
@@ -91,6 +91,7 @@ def load_library():
         "fdtd_surface_quantities": (_i, [_p, _ip, _dp]),
         "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
         "fdtd_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
+        "fdtd_set_adjoint_tile": (_i, [_p, _i, _i, _i]),
         "fdtd_batch_create": (_i, [ctypes.POINTER(_p), _i, ctypes.POINTER(_p)]),
         "fdtd_batch_destroy": (_i, [_p]),
         "fdtd_batch_set_graph": (_i, [_p, _i]),
@@ -297,6 +298,10 @@ class Grid:
         v = [ctypes.c_int() for _ in range(3)]
         _check(load_library().fdtd_get_tile(self.h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+    def set_adjoint_tile(self, bx, by, kc):
+        """Tile of the adjoint gradient's reverse sweep and recompute ((0, 0, 0): the grid's tile)."""
+        _check(load_library().fdtd_set_adjoint_tile(self.h, bx, by, kc))
 
     def autotune(self, trial_steps=20):
         """Pick the fastest tile by timing; the simulation state is left unchanged."""

@@ -255,6 +255,11 @@ int fdtd_set_design(fdtd_grid* g, const int box[6], const double* eps_r);
  * non-dispersive grids only. */
 int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, double history_mb, int dft,
                               int comp_mask, double* objective, double* grad);
+/* Tile of the gradient's reverse sweep and of its recomputed segments (the
+ * forward pass keeps fdtd_set_tile's): the two run side by side on two
+ * streams, where other shapes share the GPU best.  (0, 0, 0) = the grid's
+ * tile.  No counterpart in the reference. */
+int fdtd_set_adjoint_tile(fdtd_grid* g, int bx, int by, int kc);
 /* The same gradient for n independent grids (designs) at once: each runs on
  * its own host thread and stream (a non-blocking one is made for a grid on
  * the default stream), so the designs share the GPU.  objectives[q] and

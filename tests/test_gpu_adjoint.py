@@ -53,8 +53,12 @@ def test_checkpoint_recompute_is_bitwise(precision):
     print(f"[gpu-case] adjoint recompute {sc['name']} {precision}", flush=True)
     runs = []
     # (K, history MB): all recorded, 11 of 12 segments recomputed, none, 2 of 4, and a few resident
-    for K, mb in [(150, 1e3), (13, 0.0), (13, 1e3), (40, 0.0), (13, 0.08)]:
+    # (the last two: the reverse sweep and recompute at tiles of their own)
+    for K, mb, rtile in [(150, 1e3, None), (13, 0.0, None), (13, 1e3, None), (40, 0.0, None), (13, 0.08, None),
+                         (13, 0.0, (64, 4, 4)), (40, 0.08, (256, 1, 2))]:
         g = apply(_grid(sc, precision), sc)
+        if rtile:
+            g.set_adjoint_tile(*rtile)
         J, grad = g.adjoint_eps_gradient(sc["nsteps"], 0, 4, checkpoint_every=K, history_mb=mb)
         fields = [g.get(E, c) for c in range(3)] + [g.get(H, c) for c in range(3)]
         runs.append((J, grad, fields, g.get_dft(0), g.step_count()))
