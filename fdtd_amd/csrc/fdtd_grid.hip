@@ -1845,7 +1845,7 @@ int fdtd_get_dft(fdtd_grid* g, int idx, double* out) {
   const auto& d = g->dft[idx];
   const size_t n = size_t(d.ncomp) * d.nfreq * 2 * d.ncells;
   rc = download_T(g, out, d.acc, n);
-  if (rc || !g->cplx) return rc;
+  if (rc || !g->cplx || !d.acc_im) return rc;  // before the first step the imaginary part is zero
   // complex field: F = F_re + i F_im, each [comp][freq][re, im][cell]
   std::vector<double> fi(n);
   if ((rc = download_T(g, fi.data(), d.acc_im, n))) return rc;

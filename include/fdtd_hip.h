@@ -191,7 +191,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
  * UpdateSurfaceCurrentH (include/SurfaceQuantities.hpp:169-356) applied to
  * those cells.  out: quantity-major, x fastest; 3-D 7 quantities rho, Jex,
  * Jey, Jez, Jhx, Jhy, Jhz; TM 3: Jez, Jhx, Jhy (the reference's TM charge
- * density functor does nothing).  FP64 arithmetic; synchronises. */
+ * density functor does nothing).  FP64 arithmetic; synchronises.  On a
+ * complex grid (Bloch face) these are the quantities of the real part. */
 int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out);
 
 /* Boundary condition on one face (axis 0..ndim-1, side 0 = min, 1 = max).

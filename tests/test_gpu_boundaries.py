@@ -117,6 +117,7 @@ def test_bloch_complex_fields(name, precision):
     print(f"[gpu-case] bloch {name} {precision}", flush=True)
     g = apply(_grid(sc, precision), sc)
     o = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    assert np.all(g.get_dft(0) == 0) and np.all(g.get(E, 2, imag=True) == 0)  # before the first step
     g.step(sc["nsteps"])
     o.step(sc["nsteps"])
     for nm, f, c in _comps(sc):
