@@ -448,7 +448,9 @@ struct EStencil {
 // DISP = 0: the grid has no dispersive material (the ADE code is left out of
 // that instance, which keeps its register count down); 1: pole materials;
 // 2: also magnetized and fluid Drude materials.
-template <typename T, int DISP = 1>
+// IP: the in-place passes, whose psi is read and written in the same arrays
+// (one set of pointers instead of two: fewer scalar registers).
+template <typename T, int DISP = 1, bool IP = false>
 __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incH, int i, int j, int k,
                                           long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy,
                                           T fz, bool own, T out[3]) {
@@ -471,8 +473,8 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
     d[1] += g * p1;
     d[2] += g * p2;
     if (own) {
-      a.psio[0][1][0][o] = b * p1 - cc * dyx;
-      a.psio[0][2][0][o] = b * p2 + cc * dzx;
+      (IP ? a.psi : a.psio)[0][1][0][o] = b * p1 - cc * dyx;
+      (IP ? a.psi : a.psio)[0][2][0][o] = b * p2 + cc * dzx;
     }
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
@@ -483,8 +485,8 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
     d[0] += g * p0;
     d[2] += g * p2;
     if (own) {
-      a.psio[0][0][1][o] = b * p0 + cc * dxy;
-      a.psio[0][2][1][o] = b * p2 - cc * dzy;
+      (IP ? a.psi : a.psio)[0][0][1][o] = b * p0 + cc * dxy;
+      (IP ? a.psi : a.psio)[0][2][1][o] = b * p2 - cc * dzy;
     }
   }
   const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
@@ -495,8 +497,8 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
     d[0] += g * p0;
     d[1] += g * p1;
     if (own) {
-      a.psio[0][0][2][o] = b * p0 - cc * dxz;
-      a.psio[0][1][2][o] = b * p1 + cc * dyz;
+      (IP ? a.psi : a.psio)[0][0][2][o] = b * p0 - cc * dxz;
+      (IP ? a.psi : a.psio)[0][1][2][o] = b * p1 + cc * dyz;
     }
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
@@ -519,7 +521,7 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
 }
 
 // ---------------------------------------------------------------- H of one cell
-template <typename T>
+template <typename T, bool IP = false>
 __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incE, int i, int j, int k,
                                           long c, const EStencil<T>& e, T hx, T hy, T hz, T im, T fx, T fy, T fz,
                                           T out[3]) {
@@ -541,8 +543,8 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const
     const T p1 = a.psi[1][1][0][o], p2 = a.psi[1][2][0][o];
     d[1] -= g * p1;
     d[2] -= g * p2;
-    a.psio[1][1][0][o] = b * p1 - cc * dyx;
-    a.psio[1][2][0][o] = b * p2 + cc * dzx;
+    (IP ? a.psi : a.psio)[1][1][0][o] = b * p1 - cc * dyx;
+    (IP ? a.psi : a.psio)[1][2][0][o] = b * p2 + cc * dzx;
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
@@ -551,8 +553,8 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const
     const T p0 = a.psi[1][0][1][o], p2 = a.psi[1][2][1][o];
     d[0] -= g * p0;
     d[2] -= g * p2;
-    a.psio[1][0][1][o] = b * p0 + cc * dxy;
-    a.psio[1][2][1][o] = b * p2 - cc * dzy;
+    (IP ? a.psi : a.psio)[1][0][1][o] = b * p0 + cc * dxy;
+    (IP ? a.psi : a.psio)[1][2][1][o] = b * p2 - cc * dzy;
   }
   const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
   if (szi >= 0) {
@@ -561,8 +563,8 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const
     const T p0 = a.psi[1][0][2][o], p1 = a.psi[1][1][2][o];
     d[0] -= g * p0;
     d[1] -= g * p1;
-    a.psio[1][0][2][o] = b * p0 - cc * dxz;
-    a.psio[1][1][2][o] = b * p1 + cc * dyz;
+    (IP ? a.psi : a.psio)[1][0][2][o] = b * p0 - cc * dxz;
+    (IP ? a.psi : a.psio)[1][1][2][o] = b * p1 + cc * dyz;
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
     const bool act[3] = {true, true, true};
@@ -845,7 +847,7 @@ template <typename T>
 __device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a, int ft, int ndim, int i, int j, int k, long c,
                                          const T v[3]) {
   const int ci[3] = {i, j, k};
-  FDTD_G T* const* F = ft == 0 ? a.Eo : a.Ho;
+  FDTD_G T* const* F = ft == 0 ? a.E : a.H;  // callers are the in-place passes
   for (int ax = 0; ax < ndim; ax++)
     for (int side = 0; side < 2; side++) {
       if (ci[ax] != a.bc_src[ax][side]) continue;
@@ -894,8 +896,8 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     const int mid = a.mat[c];
     const T fz = a.cf[0][2][k];
     T o[3];
-    e_cell_3d<T, DISP>(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, cell_inv_eps(a, tie, mid, i, j, k), fx, fy, fz,
-                       true, o);
+    e_cell_3d<T, DISP, true>(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, cell_inv_eps(a, tie, mid, i, j, k), fx, fy,
+                             fz, true, o);
     a.Eo[0][c] = o[0];
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
@@ -907,13 +909,13 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
 template <typename T, typename Tab, bool BC = false>
 __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int i, int j, int k0, int k1, int s) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
-  // the E pass wrote Eo; the H pass reads it
-  const FDTD_G T* __restrict__ Ex = a.Eo[0];
-  const FDTD_G T* __restrict__ Ey = a.Eo[1];
-  const FDTD_G T* __restrict__ Ez = a.Eo[2];
-  const FDTD_G T* __restrict__ Hx = a.H[0];
-  const FDTD_G T* __restrict__ Hy = a.H[1];
-  const FDTD_G T* __restrict__ Hz = a.H[2];
+  // in place: the E pass wrote E; H is read and written through the same pointers
+  const FDTD_G T* __restrict__ Ex = a.E[0];
+  const FDTD_G T* __restrict__ Ey = a.E[1];
+  const FDTD_G T* __restrict__ Ez = a.E[2];
+  FDTD_G T* __restrict__ Hx = a.H[0];
+  FDTD_G T* __restrict__ Hy = a.H[1];
+  FDTD_G T* __restrict__ Hz = a.H[2];
   const long sy = a.sy, sz = a.sz;
   const long step = a.step0 + s;
   const FDTD_G double* incE = a.incE + (long)s * a.auxL;
@@ -935,10 +937,10 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int 
     const int mid = a.mat[c];
     const T fz = a.cf[1][2][k];
     T o[3];
-    h_cell_3d(a, step, incE, i, j, k, c, e, hx, hy, hz, tim[mid], fx, fy, fz, o);
-    a.Ho[0][c] = o[0];
-    a.Ho[1][c] = o[1];
-    a.Ho[2][c] = o[2];
+    h_cell_3d<T, true>(a, step, incE, i, j, k, c, e, hx, hy, hz, tim[mid], fx, fy, fz, o);
+    Hx[c] = o[0];
+    Hy[c] = o[1];
+    Hz[c] = o[2];
     if (BC && a.bc_any) bc_ghost_write(a, 1, 3, i, j, k, c, o);
     if (extra) {
       const T ev[3] = {e.ex, e.ey, e.ez};
