@@ -1030,8 +1030,12 @@ int step_impl(fdtd_grid* g, int nsteps) {
     if (g->args_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->args_ev[slot]));
     auto* host = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
     auto* host1 = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+    const auto* dev0 = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
+    const auto* dev1 =
+        reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
     set_state<T>(g, a, fused);
     *host = a;
+    gput(host->self, dev0);
     if (fused) {
       swap_state(g);
       set_state<T>(g, a, true);
@@ -1040,9 +1044,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
     } else if (g->cplx) {
       im_args<T>(g, a, *host1);  // variant 1: the imaginary part
     }
-    const auto* dev0 = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
-    const auto* dev1 =
-        reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+    gput(host1->self, dev1);
     HIP_TRY(hipMemcpyAsync((void*)dev0, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
     const long dft_work = host->dft_work, dft_cells = host->dft_cells, dft_slots = host->dft_slots;
     const long step0 = g->step;
@@ -1183,14 +1185,16 @@ int dft_flush_impl(fdtd_grid* g) {
   g->args_slot = (slot + 1) % fdtd_grid::kRing;
   if (g->args_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->args_ev[slot]));
   auto* host = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
-  *host = a;
-  if (g->cplx) {
-    auto* host1 = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
-    im_args<T>(g, a, *host1);
-  }
   const auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
   const auto* dev1 =
       reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+  *host = a;
+  gput(host->self, dev);
+  if (g->cplx) {
+    auto* host1 = reinterpret_cast<fdtdk::StepArgs<T>*>(g->args_host + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
+    im_args<T>(g, a, *host1);
+    gput(host1->self, dev1);
+  }
   HIP_TRY(hipMemcpyAsync((void*)dev, host, 2 * fdtd_grid::kArgsBytes, hipMemcpyHostToDevice, g->stream));
   for (int part = 0; part < (g->cplx ? 2 : 1); part++) {
     int rc = launch(g, 4, fdtdk::dft_fold<T>, dim3(unsigned((a.dft_slots + 255) / 256)), dim3(256), part ? dev1 : dev,
@@ -1459,6 +1463,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   if ((rc = sc.get(&adev, sizeof a))) return rc;
   if ((rc = sc.get(&qdev, sizeof q))) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));  // the stream-ordered allocations exist before the copies
+  gput(a.self, (const fdtdk::StepArgs<T>*)adev);
   HIP_TRY(hipMemcpy(adev, &a, sizeof a, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(qdev, &q, sizeof q, hipMemcpyHostToDevice));
   const auto* ap = (const fdtdk::StepArgs<T>*)adev;
@@ -2352,6 +2357,7 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
       a.step0 = g->step;
       gput(a.incE, g->incE);
       gput(a.incH, g->incH);
+      gput(a.self, reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + (slot * n + q) * sizeof(fdtdk::StepArgs<T>)));
       host[q] = a;
     }
     auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + slot * n * sizeof(fdtdk::StepArgs<T>));

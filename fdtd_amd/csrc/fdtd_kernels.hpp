@@ -127,6 +127,8 @@ struct StepArgs {
   FDTD_G T* Ddo[3];
   FDTD_G T* Pdo[kMaxPoles][3];
   FDTD_G T* Jdo[kMaxPoles][3];
+  // this block's own device address (set by the host; read by cold branches)
+  const FDTD_G StepArgs* self;
   // fluid Drude coefficients per dispersive-box cell: {cJ, cE} of the Drude
   // recursion (TM, or no magnetization) or {w2dt, op} of the magnetized one
   const FDTD_G T* fl[2];
@@ -237,9 +239,29 @@ __device__ __forceinline__ double lc3(int I, int J) {
 #define FDTD_COLD __forceinline__
 #endif
 
-template <typename T, int NDIM>
-__device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a, const FDTD_G double* row, int ft, int i, int j, int k, const bool actF[3],
+// The argument block as a cold branch should read it: through `self`, the
+// block's own device address stored in it by the host.  A pointer read from
+// memory is not known to be dereferenceable, so the branch's loads through it
+// cannot be hoisted out of the branch into scalar registers for the whole
+// kernel (where, with the hot path's values, they would spill into VGPR
+// lanes).  Blocks without `self` are read directly.  The view is typed in the
+// global address space so its loads are global, not flat, loads.
+template <typename T>
+__device__ __forceinline__ const FDTD_G StepArgs<T>& cold_args(const StepArgs<T>& a) {
+  const FDTD_G StepArgs<T>* p = a.self ? a.self : (const FDTD_G StepArgs<T>*)&a;
+  return *p;
+}
+
+// COLD: read the TFSF parameters through the cold view of the argument block
+// (the E pass of grids with dispersive materials, whose hoisted scalars would
+// otherwise spill: 77 -> 69 VGPRs, 6 -> 7 waves); the plain E pass and the H
+// pass keep them in scalar registers (the cold view measured slower there).  (The cold view is read as
+// a generic reference here: typed in the global address space it leaves the
+// E pass with more registers, measured.)
+template <typename T, int NDIM, bool COLD = false>
+__device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a0, const FDTD_G double* row, int ft, int i, int j, int k, const bool actF[3],
                           const bool actC[3], T delta[3]) {
+  const StepArgs<T>& a = COLD ? (const StepArgs<T>&)cold_args(a0) : a0;
   const int ci[3] = {i, j, k};
   for (int J = 0; J < NDIM; J++) {
     int side = (ci[J] == a.box[2 * J]) ? 0 : ((ci[J] == a.box[2 * J + 1]) ? 1 : -1);
@@ -278,8 +300,9 @@ __device__ __forceinline__ T pick3(T v0, T v1, T v2, int c) {
   return c == 0 ? v0 : (c == 1 ? v1 : v2);
 }
 
-template <typename T>
-__device__ FDTD_COLD void add_sources(const StepArgs<T>& a, long step, int field, long c, T d[3]) {
+template <typename T, bool COLD = true>
+__device__ FDTD_COLD void add_sources(const StepArgs<T>& a0, long step, int field, long c, T d[3]) {
+  const StepArgs<T>& a = COLD ? (const StepArgs<T>&)cold_args(a0) : a0;
   for (int s = 0; s < a.nsrc; s++) {
     const SrcDev& q = a.src[s];
     if (q.cell == c && q.field == field && step < q.n) {
@@ -464,7 +487,7 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
   d[0] = fy * dxy - fz * dxz;
   d[1] = fz * dyz - fx * dyx;
   d[2] = fx * dzx - fy * dzy;
-  if (a.nsrc) add_sources(a, step, 0, c, d);
+  if (a.nsrc) add_sources<T, (DISP > 0)>(a, step, 0, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, k);
@@ -503,7 +526,7 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
     const bool act[3] = {true, true, true};
-    tfsf_cell<T, 3>(a, incH, 0, i, j, k, act, act, d);
+    tfsf_cell<T, 3, (DISP > 0)>(a, incH, 0, i, j, k, act, act, d);
   }
   if (ie > T(0)) {
     out[0] = ex + d[0] * ie;
@@ -577,8 +600,9 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const
 
 // probes of one cell after its step
 template <typename T>
-__device__ FDTD_COLD void dft_probe(const StepArgs<T>& a, long step, long c, int i, int j, int k, const T e[3],
+__device__ FDTD_COLD void dft_probe(const StepArgs<T>& a0, long step, long c, int i, int j, int k, const T e[3],
                                     const T h[3]) {
+  const StepArgs<T>& a = (const StepArgs<T>&)cold_args(a0);
   for (int p = 0; p < a.nprobe; p++)
     if (a.probe[p].cell == c) {
       const int comp = a.probe[p].comp;
@@ -780,8 +804,9 @@ __global__ __launch_bounds__(256) void dft_accumulate_batch(const StepArgs<T>* _
 }
 
 // E increment of a design cell at `step`, kept for the adjoint's eps gradient
-template <typename T>
-__device__ FDTD_COLD void record_increment(const StepArgs<T>& a, long step, int i, int j, int k, T d0, T d1, T d2) {
+template <typename T, bool COLD = true>
+__device__ FDTD_COLD void record_increment(const StepArgs<T>& a0, long step, int i, int j, int k, T d0, T d1, T d2) {
+  const StepArgs<T>& a = COLD ? (const StepArgs<T>&)cold_args(a0) : a0;
   const long row = step - a.hist_step0;
   if (row < 0 || row >= a.hist_rows) return;
   const long l = design_index(a, i, j, k);
@@ -843,9 +868,10 @@ __device__ __forceinline__ void stage_materials(const StepArgs<T>& a, MatLds<T>&
 // reads a ghost of the field it writes, and the next pass sees them.
 // ghost copies of interior cell (i,j,k) (address c) whose new values are v
 // (ft 0: E, 1: H); ndim 2: TM (Ez, or Hx and Hy)
-template <typename T>
-__device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a, int ft, int ndim, int i, int j, int k, long c,
+template <typename T, bool COLD = true>
+__device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a0, int ft, int ndim, int i, int j, int k, long c,
                                          const T v[3]) {
+  const StepArgs<T>& a = COLD ? (const StepArgs<T>&)cold_args(a0) : a0;
   const int ci[3] = {i, j, k};
   FDTD_G T* const* F = ft == 0 ? a.E : a.H;  // callers are the in-place passes
   for (int ax = 0; ax < ndim; ax++)
@@ -901,8 +927,8 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     a.Eo[0][c] = o[0];
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
-    if (a.hist) record_increment(a, step, i, j, k, o[0] - ex, o[1] - ey, o[2] - ez);
-    if (BC && a.bc_any) bc_ghost_write(a, 0, 3, i, j, k, c, o);
+    if (a.hist) record_increment<T, (DISP > 0)>(a, step, i, j, k, o[0] - ex, o[1] - ey, o[2] - ez);
+    if (BC && a.bc_any) bc_ghost_write<T, true>(a, 0, 3, i, j, k, c, o);
   }
 }
 
