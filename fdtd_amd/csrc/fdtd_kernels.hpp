@@ -487,7 +487,7 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
   d[0] = fy * dxy - fz * dxz;
   d[1] = fz * dyz - fx * dyx;
   d[2] = fx * dzx - fy * dzy;
-  if (a.nsrc) add_sources<T, (DISP > 0)>(a, step, 0, c, d);
+  if (a.nsrc) add_sources<T, true>(a, step, 0, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, k);
@@ -927,7 +927,7 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     a.Eo[0][c] = o[0];
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
-    if (a.hist) record_increment<T, (DISP > 0)>(a, step, i, j, k, o[0] - ex, o[1] - ey, o[2] - ez);
+    if (a.hist) record_increment<T, true>(a, step, i, j, k, o[0] - ex, o[1] - ey, o[2] - ez);
     if (BC && a.bc_any) bc_ghost_write<T, true>(a, 0, 3, i, j, k, c, o);
   }
 }
