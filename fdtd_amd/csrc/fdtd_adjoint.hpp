@@ -35,7 +35,16 @@ struct AdjArgs {
   const FDTD_G T* obj_w;        // [slot][nfreq][2][ncells]: 2 Re F, 2 Im F
   const FDTD_G double* obj_ph;  // E phases of the monitor, [nphase][nfreq][2]
   int obj_nphase;
+  const FDTD_G AdjArgs* self;  // this block's own device address (null: none)
 };
+
+// The adjoint block as a cold branch reads it (the same device-address view as
+// cold_args: loads in the branch stay in the branch).
+template <typename T>
+__device__ __forceinline__ const AdjArgs<T>& cold_adj(const AdjArgs<T>& q) {
+  const FDTD_G AdjArgs<T>* p = q.self ? q.self : (const FDTD_G AdjArgs<T>*)&q;
+  return (const AdjArgs<T>&)*p;
+}
 
 // objective source of E at (i,j,k) for the step's E sample
 template <typename T>
@@ -147,7 +156,7 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
       w12m -= cc * q.lpsi[1][1][2][o];
     }
     T src[3] = {T(0), T(0), T(0)};
-    if (q.obj_w) adj_objective(q, step, i, j, k, src);
+    if (q.obj_w) adj_objective(cold_adj(q), step, i, j, k, src);
     n0 += ie * (src[0] + (w12 - w12m) - (w21 - w21m));
     n1 += ie * (src[1] + (w20 - w20m) - (w02 - w02m));
     n2 += ie * (src[2] + (w01 - w01m) - (w10 - w10m));
@@ -155,11 +164,13 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
     N1[c] = n1;
     N2[c] = n2;
     if (hist) {
-      const long l = design_index(a, i, j, k);
+      const StepArgs<T>& ac = (const StepArgs<T>&)cold_args(a);
+      const AdjArgs<T>& qc = cold_adj(q);
+      const long l = design_index(ac, i, j, k);
       if (l >= 0) {
-        const long nd = (long)(a.dsg[1] - a.dsg[0]) * (a.dsg[3] - a.dsg[2]) * (a.dsg[5] - a.dsg[4]);
+        const long nd = (long)(ac.dsg[1] - ac.dsg[0]) * (ac.dsg[3] - ac.dsg[2]) * (ac.dsg[5] - ac.dsg[4]);
         const T dot = n0 * hist[l] + n1 * hist[nd + l] + n2 * hist[2 * nd + l];
-        q.grad[l] -= q.eps0 * (double)dot;
+        qc.grad[l] -= qc.eps0 * (double)dot;
       }
     }
   }

@@ -1464,6 +1464,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   if ((rc = sc.get(&qdev, sizeof q))) return rc;
   HIP_TRY(hipStreamSynchronize(g->stream));  // the stream-ordered allocations exist before the copies
   gput(a.self, (const fdtdk::StepArgs<T>*)adev);
+  gput(q.self, (const fdtdk::AdjArgs<T>*)qdev);
   HIP_TRY(hipMemcpy(adev, &a, sizeof a, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(qdev, &q, sizeof q, hipMemcpyHostToDevice));
   const auto* ap = (const fdtdk::StepArgs<T>*)adev;
