@@ -287,18 +287,23 @@ def run_c5(args):
     two_pass = tile if tile[0] > 1 else (256, 1, 1)
     # the gradient's forward pass runs at the forward's tile; its reverse sweep
     # and recompute at their own tile: with the recompute running beside the
-    # reverse steps, other shapes share the GPU better; best of four by a
-    # short gradient run
+    # reverse steps, other shapes share the GPU better; best of four by short
+    # gradient runs (an untimed first run loads the adjoint kernels; each
+    # candidate keeps the faster of two runs)
     trial = min(600, nsteps)
     best = None
+    g = mk()
+    g.adjoint_eps_gradient(trial, 0, 4, checkpoint_every=100, history_mb=0.0)
+    g.close()
     for cand in dict.fromkeys([two_pass, (128, 2, 4), (64, 4, 4), (64, 2, 4)]):
-        g = mk()
-        g.set_tile(*two_pass)
-        g.set_adjoint_tile(*cand)
-        t, _ = wall(lambda: g.adjoint_eps_gradient(trial, 0, 4, checkpoint_every=100, history_mb=0.0))
-        g.close()
-        if best is None or t < best[0]:
-            best = (t, cand)
+        for _ in range(2):
+            g = mk()
+            g.set_tile(*two_pass)
+            g.set_adjoint_tile(*cand)
+            t, _ = wall(lambda: g.adjoint_eps_gradient(trial, 0, 4, checkpoint_every=100, history_mb=0.0))
+            g.close()
+            if best is None or t < best[0]:
+                best = (t, cand)
     adj_tile = best[1]
     res = {}
     for name, K, mb in (("checkpointed", 100, 0.0), ("history_resident", 100, 1e5)):
