@@ -261,6 +261,14 @@ struct fdtd_grid {
 
   Tile tile = {0, 0, 0};
   Tile rtile = {0, 0, 0};  // the adjoint gradient's reverse sweep and recompute (0: the grid's tile)
+  // Light-cone recompute of an adjoint segment ending at step cone_last: the
+  // passes of step cone_last - r run over the tiles cone_tab[cone_off[r] ..
+  // cone_off[r+1]), built for tile shape cone_tile (device table owned by the
+  // adjoint call; null: whole-grid passes)
+  const int4* cone_tab = nullptr;
+  std::vector<int> cone_off;
+  Tile cone_tile = {0, 0, 0};
+  long cone_last = -1;
   bool finalized = false;
   bool coef_dirty = true;
   bool mats_dirty = true;
@@ -981,7 +989,9 @@ int step_impl(fdtd_grid* g, int nsteps) {
     if (rc) return rc;
   }
   const int nzi = g->ndim == 3 ? g->d.nz - 2 : 1;
-  const bool graphs = g->use_graph && !g->prof;
+  const bool graphs = g->use_graph && !g->prof && !g->cone_tab;
+  const bool cone = g->cone_tab && g->recompute && !fused && !g->nbc && !g->cplx && !has_magnetized(g) &&
+                    t.bx == g->cone_tile.bx && t.by == g->cone_tile.by && t.kc == g->cone_tile.kc;
   dim3 block, grid;
   int ntx = 1, nty = 1, nkc = 1;
   if (wavef) {
@@ -1095,6 +1105,17 @@ int step_impl(fdtd_grid* g, int nsteps) {
                         dim3(256), dev0, dev1, bfaces, ft);
             if (rc) return rc;
           }
+        } else if (g->ndim == 3 && cone && g->cone_last - (step0 + s) >= 0 &&
+                   g->cone_last - (step0 + s) + 1 < (long)g->cone_off.size()) {
+          // light-cone recompute: the batch passes (the same cell code) over
+          // the step's tile table, the grid as a batch of one
+          const long r = g->cone_last - (step0 + s);
+          const int o0 = g->cone_off[r], cnt = g->cone_off[r + 1] - o0;
+          const int disp = g->max_np > 0 ? 1 : 0;
+          rc = disp ? launch(g, 0, fdtdk::e_pass_batch<T, 1>, dim3(unsigned(cnt)), block, dev0, g->cone_tab + o0, t.kc, s)
+                    : launch(g, 0, fdtdk::e_pass_batch<T, 0>, dim3(unsigned(cnt)), block, dev0, g->cone_tab + o0, t.kc, s);
+          if (rc) return rc;
+          rc = launch(g, 1, fdtdk::h_pass_batch<T>, dim3(unsigned(cnt)), block, dev0, g->cone_tab + o0, t.kc, s);
         } else if (g->ndim == 3) {
           // instances that also write the boundary ghost layers (BC)
           // and the E-pass instance without the ADE code when no material is
@@ -1317,6 +1338,9 @@ struct AdjointScope {
     g->hist_rows = 0;
     g->inc_allE = g->inc_allH = nullptr;
     g->inc_rows = 0;
+    g->cone_tab = nullptr;
+    g->cone_off.clear();
+    g->cone_last = -1;
   }
   // Temporaries come from the stream-ordered allocator: hipMalloc / hipFree
   // synchronise the whole device and would serialise adjoints of several
@@ -1495,6 +1519,40 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
     HIP_TRY(hipEventRecord(ev_fwd, M));
     HIP_TRY(hipStreamWaitEvent(sc.rs, ev_fwd, 0));
   }
+  // Light-cone tables for the recompute.  A segment's recompute only has to
+  // reproduce the design cells' E increments: E(i) reads H(i-1), H(i) and
+  // H(i) reads E(i), E(i+1), so if the passes of step m run over the design
+  // box grown by L - m + 1 cells on every side (L: the segment's last step),
+  // the cells they read are exact, and the box is exact at step L.  Cells
+  // outside the region keep stale values that no exact cell reads.  Table r
+  // holds the tiles (of the recompute's tile shape) meeting the box grown by
+  // r + 1.  Grids with boundary ghost writes, complex fields or magnetized
+  // materials recompute whole.
+  if (nck && !g->nbc && !g->cplx && !has_magnetized(g)) {
+    Tile tr = g->rtile.by ? g->rtile : g->tile;
+    if (!tr.by) tr = kDefaultTile3D;
+    if (tr.bx <= 1) tr = kDefaultTile3D;
+    std::vector<int4> tab;
+    std::vector<int> off(1, 0);
+    const int nx = g->d.nx, ny = g->d.ny, nz = g->d.nz;
+    for (int r = 0; r < K; r++) {
+      const int d = r + 1;
+      const int i0 = std::max(1, g->dsg[0] - d), i1 = std::min(nx - 2, g->dsg[1] - 1 + d);
+      const int j0 = std::max(1, g->dsg[2] - d), j1 = std::min(ny - 2, g->dsg[3] - 1 + d);
+      const int k0 = std::max(1, g->dsg[4] - d), k1 = std::min(nz - 2, g->dsg[5] - 1 + d);
+      for (int z = (k0 - 1) / tr.kc; z <= (k1 - 1) / tr.kc; z++)
+        for (int y = (j0 - 1) / tr.by; y <= (j1 - 1) / tr.by; y++)
+          for (int x = i0 / tr.bx; x <= i1 / tr.bx; x++) tab.push_back(make_int4(0, x, y, z));
+      off.push_back(int(tab.size()));
+    }
+    void* tdev = nullptr;
+    if ((rc = sc.get(&tdev, tab.size() * sizeof(int4)))) return rc;
+    HIP_TRY(hipStreamSynchronize(g->stream));  // the stream-ordered allocation exists before the copy
+    HIP_TRY(hipMemcpy(tdev, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
+    g->cone_tab = (const int4*)tdev;
+    g->cone_off = off;
+    g->cone_tile = tr;
+  }
   auto recompute = [&](int s) -> int {
     const long n0 = long(s) * K;
     const int len = int(std::min<long>(K, nsteps - n0));
@@ -1509,6 +1567,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
     g->hist = X[b];
     g->hist_step0 = n0;
     g->hist_rows = len;
+    g->cone_last = n0 + len - 1;
     const Tile tile_saved = g->tile;
     if (g->rtile.by) g->tile = g->rtile;
     if (!r) r = step_impl<T>(g, len);

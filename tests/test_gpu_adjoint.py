@@ -77,6 +77,28 @@ def test_checkpoint_recompute_is_bitwise(precision):
         assert np.array_equal(dft, plain_dft)
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_light_cone_recompute_is_bitwise(precision):
+    """A grid much larger than the design box: the recompute of a segment
+    runs each step's passes only over the box grown by the steps left in the
+    segment (+1), and must still give bitwise the gradient of fully recorded
+    history, at the grid's tile and at tiles of its own."""
+    sc = synth.grid3d_adjoint(n=48, nsteps=120, design=(6, 5, 3), npml=6, monitor=12)
+    print(f"[gpu-case] adjoint light cone {sc['name']} {precision}", flush=True)
+    ref = apply(_grid(sc, precision), sc)
+    J0, g0 = ref.adjoint_eps_gradient(sc["nsteps"], 0, 4, checkpoint_every=120, history_mb=1e3)
+    ref.close()
+    assert np.any(g0 != 0)
+    for K, rtile in [(40, None), (30, (64, 4, 4)), (40, (256, 1, 2)), (7, (64, 2, 4))]:
+        g = apply(_grid(sc, precision), sc)
+        if rtile:
+            g.set_adjoint_tile(*rtile)
+        J, grad = g.adjoint_eps_gradient(sc["nsteps"], 0, 4, checkpoint_every=K, history_mb=0.0)
+        g.close()
+        assert J == J0
+        assert np.array_equal(grad, g0), f"K {K} tile {rtile}"
+
+
 def test_adjoint_f32_matches_finite_differences():
     sc = synth.grid3d_adjoint(n=40, nsteps=500, design=(16, 16, 6), npml=6, monitor=20)
     box, eps = sc["design"]
