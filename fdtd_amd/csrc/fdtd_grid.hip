@@ -1521,12 +1521,13 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   }
   // Light-cone tables for the recompute.  A segment's recompute only has to
   // reproduce the design cells' E increments: E(i) reads H(i-1), H(i) and
-  // H(i) reads E(i), E(i+1), so if the passes of step m run over the design
-  // box grown by L - m + 1 cells on every side (L: the segment's last step),
-  // the cells they read are exact, and the box is exact at step L.  Cells
-  // outside the region keep stale values that no exact cell reads.  Table r
-  // holds the tiles (of the recompute's tile shape) meeting the box grown by
-  // r + 1.  Grids with boundary ghost writes, complex fields or magnetized
+  // H(i) reads E(i), E(i+1).  If the passes of step m run over the design box
+  // grown by D = L - m cells on every side (L: the segment's last step), then
+  // after step m E is exact on the box grown by D and H on it less its upper
+  // layer, which is all that step m + 1's E reads on the box grown by D - 1;
+  // at step L the box itself is exact.  Cells outside the region keep stale
+  // values that no exact cell reads.  Table r holds the tiles (of the
+  // recompute's tile shape) meeting the box grown by r.  Grids with boundary ghost writes, complex fields or magnetized
   // materials recompute whole.
   if (nck && !g->nbc && !g->cplx && !has_magnetized(g)) {
     Tile tr = g->rtile.by ? g->rtile : g->tile;
@@ -1534,24 +1535,33 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
     if (tr.bx <= 1) tr = kDefaultTile3D;
     std::vector<int4> tab;
     std::vector<int> off(1, 0);
+    // tables stop where the cone covers every tile: later steps of a segment
+    // take the whole-grid passes (so the tables hold at most a grid edge's
+    // worth of distances whatever checkpoint_every is)
     const int nx = g->d.nx, ny = g->d.ny, nz = g->d.nz;
+    const long all = long((nx + tr.bx - 1) / tr.bx) * ((ny - 2 + tr.by - 1) / tr.by) * ((nz - 2 + tr.kc - 1) / tr.kc);
     for (int r = 0; r < K; r++) {
-      const int d = r + 1;
+      const int d = r;
       const int i0 = std::max(1, g->dsg[0] - d), i1 = std::min(nx - 2, g->dsg[1] - 1 + d);
       const int j0 = std::max(1, g->dsg[2] - d), j1 = std::min(ny - 2, g->dsg[3] - 1 + d);
       const int k0 = std::max(1, g->dsg[4] - d), k1 = std::min(nz - 2, g->dsg[5] - 1 + d);
+      const long n = long(i1 / tr.bx - i0 / tr.bx + 1) * ((j1 - 1) / tr.by - (j0 - 1) / tr.by + 1) *
+                     ((k1 - 1) / tr.kc - (k0 - 1) / tr.kc + 1);
+      if (n >= all) break;
       for (int z = (k0 - 1) / tr.kc; z <= (k1 - 1) / tr.kc; z++)
         for (int y = (j0 - 1) / tr.by; y <= (j1 - 1) / tr.by; y++)
           for (int x = i0 / tr.bx; x <= i1 / tr.bx; x++) tab.push_back(make_int4(0, x, y, z));
       off.push_back(int(tab.size()));
     }
-    void* tdev = nullptr;
-    if ((rc = sc.get(&tdev, tab.size() * sizeof(int4)))) return rc;
-    HIP_TRY(hipStreamSynchronize(g->stream));  // the stream-ordered allocation exists before the copy
-    HIP_TRY(hipMemcpy(tdev, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
-    g->cone_tab = (const int4*)tdev;
-    g->cone_off = off;
-    g->cone_tile = tr;
+    if (!tab.empty()) {
+      void* tdev = nullptr;
+      if ((rc = sc.get(&tdev, tab.size() * sizeof(int4)))) return rc;
+      HIP_TRY(hipStreamSynchronize(g->stream));  // the stream-ordered allocation exists before the copy
+      HIP_TRY(hipMemcpy(tdev, tab.data(), tab.size() * sizeof(int4), hipMemcpyHostToDevice));
+      g->cone_tab = (const int4*)tdev;
+      g->cone_off = off;
+      g->cone_tile = tr;
+    }
   }
   auto recompute = [&](int s) -> int {
     const long n0 = long(s) * K;

@@ -252,7 +252,7 @@ int fdtd_set_design(fdtd_grid* g, const int box[6], const double* eps_r);
  * (at least one) are recorded during the forward pass, every earlier segment
  * restarts from a checkpoint of E, H and psi taken at its first step and is
  * recomputed just before its reverse steps (each step over the design box
- * grown by the steps left in the segment, plus one cell, on grids without
+ * grown by one cell per step left in the segment, on grids without
  * boundary ghost writes or complex fields).  On return the grid holds the
  * forward state after nsteps steps and the monitor holds F.  3-D,
  * non-dispersive grids only. */

@@ -81,7 +81,7 @@ def test_checkpoint_recompute_is_bitwise(precision):
 def test_light_cone_recompute_is_bitwise(precision):
     """A grid much larger than the design box: the recompute of a segment
     runs each step's passes only over the box grown by the steps left in the
-    segment (+1), and must still give bitwise the gradient of fully recorded
+    segment, and must still give bitwise the gradient of fully recorded
     history, at the grid's tile and at tiles of its own."""
     sc = synth.grid3d_adjoint(n=48, nsteps=120, design=(6, 5, 3), npml=6, monitor=12)
     print(f"[gpu-case] adjoint light cone {sc['name']} {precision}", flush=True)
