@@ -5,7 +5,10 @@ has no adjoint) without any GPU:
   block (D, B, psiE, psiH), so that no block's scale hides another's error;
 * central finite differences (FP64, h = 1e-3) of the eps gradient of a
   CPML + TFSF continuous-wave scenario with a design box;
-* a design box holding the material table's values steps bitwise like the table.
+* a design box holding the material table's values steps bitwise like the table;
+* the light-cone argument of the checkpointed recompute: updating step m only on
+  the design box grown by L - m leaves the box's E at step L bitwise exact, and
+  one cell less does not.
 """
 import numpy as np
 import pytest
@@ -130,3 +133,44 @@ def test_design_box_equals_material_table():
     for c in range(3):
         assert np.array_equal(a.get(1, c), b.get(1, c))
         assert np.array_equal(a.get(3, c), b.get(3, c))
+
+
+def _cone_run(sc, L, shrink):
+    """Step the oracle L + 1 steps from random fields; after step m roll every
+    E and H cell outside the design box grown by L - m - shrink (at least 0 on
+    the last step) back to its value before the step, as a recompute that only
+    updates that region leaves it.  Returns E of the box after step L."""
+    from scenarios import E, H
+    o = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
+    rng = np.random.default_rng(11)
+    for f in (E, H):
+        for c in range(3):
+            o.set(f, c, rng.standard_normal(o.shape))
+    x0, x1, y0, y1, z0, z1 = sc["design"][0]
+    kk, jj, ii = np.meshgrid(np.arange(sc["nz"]), np.arange(sc["ny"]), np.arange(sc["nx"]), indexing="ij")
+    for m in range(L + 1):
+        d = max(L - m - shrink, 0) if m < L else 0
+        inside = ((ii >= x0 - d) & (ii < x1 + d) & (jj >= y0 - d) & (jj < y1 + d) & (kk >= z0 - d) & (kk < z1 + d))
+        before = {(f, c): o.get(f, c) for f in (E, H) for c in range(3)}
+        o.step(1)
+        for (f, c), old in before.items():
+            new = o.get(f, c)
+            o.set(f, c, np.where(inside, new, old))
+    box = (slice(z0, z1), slice(y0, y1), slice(x0, x1))
+    return [o.get(E, c)[box] for c in range(3)]
+
+
+def test_light_cone_recompute_argument():
+    """The adjoint's recompute updates step m of a segment ending at L only on
+    the design box grown by L - m: E(i) reads H(i-1), H(i) and H(i) reads
+    E(i), E(i+1), so stale cells outside never reach the box.  Checked on the
+    oracle (CPML, TFSF, random initial fields): the box's E at L equals a
+    whole-grid run bit for bit, and a cone one cell narrower does not."""
+    sc = synth.grid3d_adjoint(n=20, nsteps=16, design=(4, 4, 3), npml=4, monitor=8)
+    L = 12
+    full = _cone_run(sc, L, shrink=-100)  # every cell updated every step
+    cone = _cone_run(sc, L, shrink=0)
+    narrow = _cone_run(sc, L, shrink=1)
+    for a, b in zip(cone, full):
+        assert np.array_equal(a, b)
+    assert any(not np.array_equal(a, b) for a, b in zip(narrow, full))
