@@ -1527,8 +1527,8 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   // layer, which is all that step m + 1's E reads on the box grown by D - 1;
   // at step L the box itself is exact.  Cells outside the region keep stale
   // values that no exact cell reads.  Table r holds the tiles (of the
-  // recompute's tile shape) meeting the box grown by r.  Grids with boundary ghost writes, complex fields or magnetized
-  // materials recompute whole.
+  // recompute's tile shape) meeting the box grown by r.  Grids with boundary
+  // ghost writes, complex fields or magnetized materials recompute whole.
   if (nck && !g->nbc && !g->cplx && !has_magnetized(g)) {
     Tile tr = g->rtile.by ? g->rtile : g->tile;
     if (!tr.by) tr = kDefaultTile3D;
