@@ -127,12 +127,14 @@ constexpr Tile kDefaultTileTM = {64, 4, 1};
 }  // namespace
 
 // what a captured chunk depends on beyond the staged arguments: the argument
-// slot, the chunk length, the step phase within a DFT block, the schedule and
-// the DFT launch sizes
+// slot, the chunk length, the step phase within a DFT block, the schedule,
+// the E-pass instance (a material set after stepping started can change it)
+// and the DFT launch sizes
 struct GraphKey {
-  int slot, chunk, phase, bx, by, kc;
+  int slot, chunk, phase, bx, by, kc, disp;
   long dft_work, dft_cells;
   bool operator<(const GraphKey& o) const {
+    if (disp != o.disp) return disp < o.disp;
     if (slot != o.slot) return slot < o.slot;
     if (chunk != o.chunk) return chunk < o.chunk;
     if (phase != o.phase) return phase < o.phase;
@@ -460,6 +462,10 @@ int upload_materials(fdtd_grid* g) {
         cP = -(h.freq[p] * w / b);
       }
       m[id].cJ[p] = T(cJ);
+      m[id].cJm[p] = T(p < h.np ? (h.kind[p] == FDTD_DRUDE ? 2.0 * (h.gamma[p] * dt * 0.5) / (1.0 + h.gamma[p] * dt * 0.5)
+                                   : h.kind[p] == FDTD_CONDUCTIVE ? 0.0
+                                                                  : 2.0 * h.gamma[p] * dt / (1.0 + h.gamma[p] * dt))
+                                : 1.0 - cJ);
       m[id].cE[p] = T(cE);
       m[id].cP[p] = T(cP);
       m[id].cD[p] = T(cD);
@@ -734,7 +740,7 @@ void im_args(const fdtd_grid* g, const fdtdk::StepArgs<T>& re, fdtdk::StepArgs<T
   a.nsrc = 0;
   a.tfsf = 0;
   for (int q = 0; q < g->ndft; q++) {
-    gput(a.dft[q].acc, (T*)g->dft[q].acc_im);
+    gput(a.dft[q].acc, (double*)g->dft[q].acc_im);
     gput(a.dft[q].smp, (T*)g->dft[q].smp_im);
   }
   gput(a.trace, g->trace_im);
@@ -815,7 +821,7 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     dd.nfreq = h.nfreq;
     dd.ncomp = h.ncomp;
     dd.ncells = h.ncells;
-    gput(dd.acc, (T*)h.acc);
+    gput(dd.acc, (double*)h.acc);
     gput(dd.phE, h.phE);
     gput(dd.phH, h.phH);
     dd.nphase = h.nphase;
@@ -920,7 +926,7 @@ int ensure_dft_im(fdtd_grid* g) {
   for (int q = 0; q < g->ndft; q++) {
     auto& d = g->dft[q];
     if (d.acc_im) continue;
-    int rc = g->alloc(&d.acc_im, size_t(d.ncomp) * d.nfreq * 2 * d.ncells * g->esz);
+    int rc = g->alloc(&d.acc_im, size_t(d.ncomp) * d.nfreq * 2 * d.ncells * 8);
     if (!rc) rc = g->alloc(&d.smp_im, size_t(fdtdk::kDftBlock) * d.ncomp * d.ncells * g->esz);
     if (rc) return rc;
   }
@@ -1156,7 +1162,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
       return 0;
     };
     if (graphs) {
-      const GraphKey key{slot, chunk, int(step0 % fdtdk::kDftBlock), t.bx, t.by, t.kc, dft_work, dft_cells};
+      const int disp = g->max_np > 0 ? (has_magnetized(g) ? 2 : 1) : 0;
+      const GraphKey key{slot, chunk, int(step0 % fdtdk::kDftBlock), t.bx, t.by, t.kc, disp, dft_work, dft_cells};
       auto it = g->graphs.find(key);
       if (it == g->graphs.end()) {
         hipGraph_t graph;
@@ -1438,7 +1445,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   const auto& d = g->dft[didx];
   const size_t per_slot = size_t(d.nfreq) * 2 * d.ncells;
   std::vector<double> acc(size_t(d.ncomp) * per_slot);
-  if ((rc = download_T(g, acc.data(), d.acc, acc.size()))) return rc;
+  HIP_TRY(hipMemcpy(acc.data(), d.acc, acc.size() * 8, hipMemcpyDeviceToHost));
   fdtdk::AdjArgs<T> q;
   memset(&q, 0, sizeof q);
   q.eps0 = kEps0;
@@ -1898,7 +1905,7 @@ int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, co
   d.ncomp = __builtin_popcount(comp_mask);
   d.ncells = long(r[1] - r[0]) * (r[3] - r[2]) * (r[5] - r[4]);
   if (d.ncells * d.ncomp * fdtdk::kDftBlock >= (1L << 31)) return fail(FDTD_EINVAL, "DFT monitor too large");
-  int rc = g->alloc(&d.acc, size_t(d.ncomp) * nfreq * 2 * d.ncells * g->esz);
+  int rc = g->alloc(&d.acc, size_t(d.ncomp) * nfreq * 2 * d.ncells * 8);  // FP64 accumulators
   if (!rc) rc = g->alloc(&d.smp, size_t(fdtdk::kDftBlock) * d.ncomp * d.ncells * g->esz);
   if (!rc) rc = g->alloc((void**)&d.phE, size_t(std::max(nphase, 1)) * nfreq * 2 * 8);
   if (!rc) rc = g->alloc((void**)&d.phH, size_t(std::max(nphase, 1)) * nfreq * 2 * 8);
@@ -1919,11 +1926,11 @@ int fdtd_get_dft(fdtd_grid* g, int idx, double* out) {
   HIP_TRY(hipStreamSynchronize(g->stream));
   const auto& d = g->dft[idx];
   const size_t n = size_t(d.ncomp) * d.nfreq * 2 * d.ncells;
-  rc = download_T(g, out, d.acc, n);
-  if (rc || !g->cplx || !d.acc_im) return rc;  // before the first step the imaginary part is zero
+  HIP_TRY(hipMemcpy(out, d.acc, n * 8, hipMemcpyDeviceToHost));
+  if (!g->cplx || !d.acc_im) return 0;  // before the first step the imaginary part is zero
   // complex field: F = F_re + i F_im, each [comp][freq][re, im][cell]
   std::vector<double> fi(n);
-  if ((rc = download_T(g, fi.data(), d.acc_im, n))) return rc;
+  HIP_TRY(hipMemcpy(fi.data(), d.acc_im, n * 8, hipMemcpyDeviceToHost));
   for (size_t b = 0; b < size_t(d.ncomp) * d.nfreq; b++) {
     double* o = out + b * 2 * d.ncells;
     const double* q = fi.data() + b * 2 * d.ncells;

@@ -50,6 +50,7 @@ struct MatDev {
   T inv_mu;   // 1/(mu_r mu0)
   int np;     // poles; 0 = constant material
   T cJ[kMaxPoles], cE[kMaxPoles], cP[kMaxPoles];  // J = cJ J + cE E + cP P
+  T cJm[kMaxPoles];                                 // 1 - cJ, formed in FP64
   // conductive pole: P = (P + cD D)/(1 + cD) with cD = dt sigma/(eps_r eps0)
   // (ConductiveUpdate::implicit_update, DispersiveMaterials.hpp:549-560); 0 for
   // Drude / Lorentz poles, whose P advances by dt J
@@ -80,7 +81,10 @@ struct DftDev {
   int r[6];
   int mask, nfreq, ncomp;
   long ncells;
-  FDTD_G T* acc;             // [ncomp][nfreq][2][ncells]
+  // [ncomp][nfreq][2][ncells], FP64 for every field precision: a sum over
+  // thousands of steps keeps FP64 rounding, so the bins of an FP32 grid carry
+  // the error of its fields only (read and written once per kDftBlock steps)
+  FDTD_G double* acc;
   const FDTD_G double* phE;  // [nphase][nfreq][2]
   const FDTD_G double* phH;
   int nphase;
@@ -348,14 +352,28 @@ __device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>
       num -= Pv[p];
     }
   }
-  const T Ev = num * m.inv_eps;
+  T Ev = num * m.inv_eps;
+  if (sizeof(T) == 4 && m.cD[0] == T(0) && m.cD[1] == T(0) && m.cD[2] == T(0)) {
+    // FP32, Drude / Lorentz poles: E_{n+1} = E_n + (d - dt sum_p J_p)/(eps_r eps0),
+    // equal to (D - sum_p P_p)/(eps_r eps0) (each P_p advances by dt J_p)
+    // without forming D - sum P, which inside a metal cancels to a small E and
+    // leaves it a few digits in FP32 (FP64 keeps the reference's form)
+    T inc = d;
+#pragma unroll
+    for (int p = 0; p < kMaxPoles; p++)
+      if (p < m.np) inc -= a.dt * Jv[p];
+    Ev = Eold + inc * m.inv_eps;
+  }
   if (own) {
     a.Ddo[I][bq] = D;
 #pragma unroll
     for (int p = 0; p < kMaxPoles; p++) {
       if (p < m.np) {
         a.Pdo[p][I][bq] = Pv[p];
-        a.Jdo[p][I][bq] = m.cJ[p] * Jv[p] + m.cE[p] * Ev + m.cP[p] * Pv[p];
+        // FP32: cJ J as J - (1 - cJ) J.  cJ is within a few 1e-3 of 1 and its FP32
+        // rounding (~6e-8) compounds over the steps (cJ^n); 1 - cJ is stored to
+        // FP32's relative precision, so the pole's decay stays that of FP64
+        a.Jdo[p][I][bq] = (sizeof(T) == 4 ? Jv[p] - m.cJm[p] * Jv[p] : m.cJ[p] * Jv[p]) + m.cE[p] * Ev + m.cP[p] * Pv[p];
       }
     }
   }
@@ -644,14 +662,14 @@ __device__ __forceinline__ void dft_item(const StepArgs<T>& a, long g, long step
   const long c = (long)i + a.sy * (long)j + a.sz * (long)k;
   const FDTD_G double* pe = d.phE + 2L * (d.nfreq * step + f);
   const FDTD_G double* ph = d.phH + 2L * (d.nfreq * step + f);
-  const T ce = (T)pe[0], se = (T)pe[1], chh = (T)ph[0], sh = (T)ph[1];
+  const double ce = pe[0], se = pe[1], chh = ph[0], sh = ph[1];
   int slot = 0;
 #pragma unroll
   for (int comp = 0; comp < 6; comp++) {
     if (!((d.mask >> comp) & 1)) continue;
     const FDTD_G T* src = comp < 3 ? a.Eo[comp] : a.Ho[comp - 3];
-    const T v = src ? src[c] : T(0);
-    FDTD_G T* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
+    const double v = src ? (double)src[c] : 0.0;
+    FDTD_G double* re = d.acc + ((long)slot * d.nfreq + f) * 2 * d.ncells + lq;
     re[0] += v * (comp < 3 ? ce : chh);
     re[d.ncells] += v * (comp < 3 ? se : sh);
     slot++;
@@ -738,10 +756,10 @@ __device__ __forceinline__ void dft_fold_items(const StepArgs<T>& a, long m0, lo
 #pragma unroll
   for (int t = 0; t < kDftBlock; t++)
     v[t] = t < nm ? d.smp[(long)(((r0 + t) % kDftBlock) * d.ncomp + slot) * nc + lq] : T(0);
-  FDTD_G T* acc = d.acc + (long)slot * nf * 2 * nc + lq;
+  FDTD_G double* acc = d.acc + (long)slot * nf * 2 * nc + lq;
   for (int f0 = 0; f0 < nf; f0 += 4) {
     const int nu = nf - f0 < 4 ? nf - f0 : 4;
-    T ar[4], ai[4];
+    double ar[4], ai[4];
 #pragma unroll
     for (int u = 0; u < 4; u++)
       if (u < nu) {
@@ -756,8 +774,8 @@ __device__ __forceinline__ void dft_fold_items(const StepArgs<T>& a, long m0, lo
           if (u < nu) {
             const int x = (t * nf + f0 + u) * 2;
             const double pr = lds ? sph[x] : ph[x], pi = lds ? sph[x + 1] : ph[x + 1];
-            ar[u] += v[t] * (T)pr;
-            ai[u] += v[t] * (T)pi;
+            ar[u] += (double)v[t] * pr;
+            ai[u] += (double)v[t] * pi;
           }
       }
     }

@@ -169,6 +169,15 @@ def gauss_table(nsteps, t0, spread, scale=EPS0):
     return scale * np.exp(-((n - t0) / spread) ** 2)
 
 
+def dipole_table(nsteps, t0, spread, scale=EPS0):
+    """Soft-source table of a point dipole whose moment is a Gaussian pulse: the impressed current is the
+    moment's time derivative, so the values sum to zero and the source leaves no static charge behind
+    (a Gaussian added to D would leave a net charge and its static Coulomb field)."""
+    n = np.arange(nsteps, dtype=np.float64)
+    u = (n - t0) / spread
+    return -scale * math.sqrt(2.0 * math.e) * u * np.exp(-u * u)
+
+
 def gmsin_signal(nsteps, dt, f0, tw, t0):
     """Gaussian-modulated sinusoid sampled at the aux line's accumulated time
     (t += dt before each use, include/Source.hpp:414)."""

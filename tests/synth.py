@@ -2,10 +2,10 @@
 
 grid3d_cpml_tfsf_dft     3-D CPML + dielectric blocks + TFSF plane wave + DFT planes
                          (256^3 x 2000 steps at full size)
-grid3d_dispersive_sphere 3-D Drude + 2-Lorentz sphere, dipole source, DFT box surface
+grid3d_dispersive_sphere 3-D Drude + 2-Lorentz sphere, z-dipole source (Gaussian moment), DFT box surface
                          (192^3 x 3000 steps at full size)
 tm2d_c1                  2-D TM 256^2, 10-cell CPML, centre Gaussian pulse, 4 probes
-batch_c4                 64 uneven 3-D grids (edges 64..160), point source, 4-frequency plane each
+batch_c4                 64 uneven 3-D grids (edges 64..160), z-dipole point source, 4-frequency plane each
 grid3d_adjoint           3-D CPML + TFSF continuous wave, per-cell eps design box, Ez DFT plane
                          at one frequency (128^3, design 64x64x16, 3000 steps at full size)
 """
@@ -13,7 +13,7 @@ import math
 
 import numpy as np
 
-from scenarios import B, C0, D, E, EPS0, IMP0, H, SplitMix64, cw_signal, dft_phases, gauss_table, gmsin_signal
+from scenarios import B, C0, D, E, EPS0, IMP0, H, SplitMix64, cw_signal, dft_phases, dipole_table, gauss_table, gmsin_signal
 
 DRUDE, LORENTZ = 1, 2
 
@@ -108,7 +108,8 @@ def grid3d_dispersive_sphere(n=192, radius=40, nsteps=3000, npml=None, nfreq=Non
         "name": f"grid3d_dispersive_sphere_{nx}x{ny}x{nz}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz,
         "dx": dx, "dt": dt, "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, True, True],
         "materials": [(1.0, 1.0, []), (1.0, 1.0, poles)], "matid": matid,
-        "sources": [(D, 2, src_x, cy, cz, gauss_table(nsteps, 30, 10))], "probes": [], "dfts": [], "tfsf": None,
+        # a z-dipole whose Gaussian moment (spread 49 steps) puts its spectrum across the monitored band
+        "sources": [(D, 2, src_x, cy, cz, dipole_table(nsteps, 200, 49))], "probes": [], "dfts": [], "tfsf": None,
     }
     freqs = np.linspace(1e14, 1.2e15, nfreq)
     pe, ph = dft_phases(nsteps, dt, freqs)
@@ -144,7 +145,7 @@ def tm2d_c1(nsteps=1000):
 
 def batch_c4(nsims=64, nsteps=1500, lo=64, hi=160, seed=4):
     """64 independent 3-D FP32 grids, edges seeded uniform in [64,160] per axis,
-    8-cell CPML, 10 eps blocks U[1,12], own Dz point source, 4-frequency DFT
+    8-cell CPML, 10 eps blocks U[1,12], own z-dipole point source, 4-frequency DFT
     of all components on the mid z-plane."""
     rng = SplitMix64(seed)
     out = []
@@ -162,7 +163,7 @@ def batch_c4(nsims=64, nsteps=1500, lo=64, hi=160, seed=4):
             "name": f"batch{q}_{nx}x{ny}x{nz}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
             "nsteps": nsteps, "pml": _pml(8, dx), "pml_axes": [True, True, True],
             "materials": [(1.0, 1.0, [])] + blocks, "matid": matid,
-            "sources": [(D, 2, sx, sy, sz, gauss_table(nsteps, 30 + q % 7, 10))], "probes": [],
+            "sources": [(D, 2, sx, sy, sz, dipole_table(nsteps, 40 + q % 7, 10))], "probes": [],
             "dfts": [([1, nx - 1, 1, ny - 1, nz // 2, nz // 2 + 1], 63, pe, ph)], "tfsf": None,
         })
     return out

@@ -156,9 +156,9 @@ def test_tables_at_their_limits():
     o.step(nsteps)
     for f, c in _comps("ThreeD"):
         assert rel_l2(g.get(f, c), o.get(f, c)) <= 1e-12, (f, c)
-    assert rel_l2(g.probes(), o.probes()) <= 1e-11
+    assert rel_l2(g.probes(), o.probes()) <= 1e-12
     for q in range(8):
-        assert rel_l2(g.get_dft(q), o.get_dft(q)) <= 1e-11, q
+        assert rel_l2(g.get_dft(q), o.get_dft(q)) <= 1e-12, q
     over = Grid("ThreeD", *n, dx, dt, precision="f32")
     for q in range(16):
         over.add_source(D, 2, 5, 5, 5, gauss_table(4, 1, 1))
