@@ -24,13 +24,16 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+DFT_BLOCK = 16  # steps of samples folded into the DFT accumulators at once
 METRIC = "Gcell-updates/s + %HBM roofline (fwd, adjoint), batched sims/s vs plain"
 
 
 def algorithmic_bytes(sc, esz):
     """Minimum HBM bytes of one step: every E and H component read and written
     once, one material byte per cell, CPML psi (4 per slab cell) read+written,
-    DFT accumulators read+written per monitor cell."""
+    and the time-blocked running DFT at its true minimum: per monitor cell and
+    component one sample written and read back, plus the FP64 accumulators
+    (re, im per frequency) read and written once per DFT_BLOCK steps."""
     nx, ny, nz = sc["nx"], sc["ny"], sc["nz"]
     interior = (nx - 2) * (ny - 2) * (nz - 2)
     b = interior * (12 * esz + 1)
@@ -41,28 +44,33 @@ def algorithmic_bytes(sc, esz):
         b += slab * (o1 - 2) * (o2 - 2) * 4 * 2 * esz
     for region, mask, pe, _ in sc["dfts"]:
         cells = (region[1] - region[0]) * (region[3] - region[2]) * (region[5] - region[4])
-        b += cells * bin(mask).count("1") * pe.shape[1] * 2 * 2 * esz
-    return b, interior
+        b += cells * bin(mask).count("1") * (2 * esz + pe.shape[1] * 2 * 2 * 8 / DFT_BLOCK)
+    return int(b), interior
+
+
+def host_cpu():
+    """CPU model and logical core count of this host."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
 
 
 def cpu_baseline(size, sample_steps):
-    harness = os.path.join(ROOT, "oracle", "_ref", "harness")
-    if os.path.exists(harness):
-        t0 = time.time()
-        r = subprocess.run([harness, "bench3d", str(size), str(sample_steps), "12"], capture_output=True, text=True,
-                           timeout=900)
-        if r.returncode == 0:
-            js = json.loads(r.stdout.strip().splitlines()[-1])
-            return {"value": js["cell_updates_per_s"] / 1e9, "unit": "Gcell-updates/s", "cores": 1,
-                    "kind": "reference",
-                    "sample": f"{size}^3, 12-cell CPML, 48 eps blocks, Dz soft source; {sample_steps} timed steps "
-                              f"after 1 untimed; the reference's YeeUpdate/UpdatePML/ConstantUpdate/VacuumUpdate "
-                              f"functors, one sweep each (TFSF/DFT have no 3-D form in the reference); "
-                              f"wall incl. set-up {time.time() - t0:.1f} s"}
-    # fallback: the FP64 oracle restatement with OpenMP
+    """The FP64 oracle (oracle/refcpu.c: structure of arrays, OpenMP over cells, the reference's per-functor
+    sweeps and operation order) on every host core, on `sample_steps` steps of the full workload; beside it,
+    where it was built, the reference's own update functors (oracle/_ref/harness, single-threaded like the
+    reference) on 2 steps of the same grid without TFSF and DFT (which the reference has no 3-D form of)."""
     from oracle import refcpu
     import synth
     from scenarios import apply
+    model, ncpu = host_cpu()
     sc = synth.grid3d_cpml_tfsf_dft(n=size, nsteps=sample_steps + 1)
     cores = refcpu.nthreads(0)
     o = apply(refcpu.Sim("ThreeD", size, size, size, sc["dx"], sc["dt"]), sc)
@@ -70,8 +78,44 @@ def cpu_baseline(size, sample_steps):
     t0 = time.perf_counter()
     o.step(sample_steps)
     dt = time.perf_counter() - t0
-    return {"value": (size - 2) ** 3 * sample_steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": cores,
-            "kind": "port", "sample": f"{size}^3 full workload, {sample_steps} timed steps, FP64 oracle"}
+    del o
+    out = {"value": round((size - 2) ** 3 * sample_steps / dt / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores,
+           "kind": "port",
+           "sample": f"{size}^3 full workload (CPML, 48 blocks, TFSF, 2 DFT planes x 8 freqs), {sample_steps} timed "
+                     f"steps after 1 untimed, FP64 oracle with OpenMP on {cores} threads ({dt:.1f} s)",
+           "host": {"cpu_model": model, "logical_cores": ncpu}}
+    harness = os.path.join(ROOT, "oracle", "_ref", "harness")
+    if os.path.exists(harness):
+        t0 = time.time()
+        r = subprocess.run([harness, "bench3d", str(size), "2", "12"], capture_output=True, text=True, timeout=900)
+        if r.returncode == 0:
+            js = json.loads(r.stdout.strip().splitlines()[-1])
+            out["reference_functors_1core"] = {
+                "value": round(js["cell_updates_per_s"] / 1e9, 6), "unit": "Gcell-updates/s", "cores": 1,
+                "kind": "reference",
+                "sample": f"{size}^3, 12-cell CPML, 48 eps blocks, Dz soft source; 2 timed steps after 1 untimed; "
+                          f"YeeUpdate/UpdatePML/ConstantUpdate/VacuumUpdate, one sweep each; wall incl. set-up "
+                          f"{time.time() - t0:.1f} s"}
+    return out
+
+
+def copy_rate(seconds=0.3):
+    """Measured device-copy bandwidth (1 GiB FP32 -> 1 GiB), GB/s of bytes read + written."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    gbs = 2 * n * 4 * reps / (time.perf_counter() - t0) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def timed_steps(g, steps):
@@ -305,7 +349,7 @@ def run_c5(args):
             if best is None or t < best[0]:
                 best = (t, cand)
     adj_tile = best[1]
-    res = {}
+    res, grads = {}, {}
     for name, K, mb in (("checkpointed", 100, 0.0), ("history_resident", 100, 1e5)):
         # timed run (no per-kernel events), then a profiled run for the kernel averages
         g = mk()
@@ -327,6 +371,11 @@ def run_c5(args):
         g.close()
         res[name] = {"seconds": round(t, 4), "ratio_to_forward": round(t / t_fwd, 3), "checkpoint_every": K,
                      "history_mb": mb, "kernel_avg_ms": avg, "objective": J}
+        grads[name] = (J, grad)
+    # the checkpointed gradient (recomputed segments) is the one the headline ratio times: it must equal the
+    # gradient from recorded history bit for bit, and it is the one checked against finite differences below
+    Jc, grad = grads["checkpointed"]
+    bitwise = Jc == grads["history_resident"][0] and bool(np.array_equal(grad, grads["history_resident"][1]))
     # a batch of 8 designs: one gradient call for all (concurrent streams) vs 8 calls one after another
     from fdtd_amd.grid import adjoint_eps_gradient_batch
     rng8 = np.random.default_rng(8)
@@ -391,8 +440,10 @@ def run_c5(args):
                                  f"on a {sc['dfts'][0][0][3] - sc['dfts'][0][0][2]}^2 plane, {nsteps} steps",
                      "forward_seconds": round(t_fwd, 4), "forward_tile": list(tile), "gradient_tile": list(adj_tile),
                      "forward_gcell_per_s": round(interior * nsteps / t_fwd / 1e9, 3),
-                     "gradient": res, "fd_check": {"cells": 16, "h": 1e-3, "reference": "FP64 GPU forward runs",
-                                                   "max_abs_err_over_max_abs_fd": fd_err},
+                     "gradient": res, "checkpointed_equals_history_resident_bitwise": bitwise,
+                     "fd_check": {"cells": 16, "h": 1e-3, "reference": "FP64 GPU forward runs",
+                                  "gradient": "checkpointed (K = 100, recomputed segments)",
+                                  "max_abs_err_over_max_abs_fd": fd_err},
                      "batch_of_designs": batch8},
                     higher=False)
     if achieved:
@@ -426,63 +477,69 @@ def run_copy(args):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3d256_f32.json")
+TRAFFIC_SOURCE = ("profiles/hbm_traffic_3d256_f32.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
+                  "workload at the same tile, each scaled by the counter's reading for a known 1 GiB device copy; "
+                  "not measured inside this run")
+PASS_KERNELS = ("fused3d_lds", "e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedw_3d", "fused_3d")
 
 
 def _short(name):
-    for k in ("e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "dft_accumulate", "fusedw_3d", "fused_3d",
-              "aux_line", "copyBuffer"):
+    for k in PASS_KERNELS + ("dft_accumulate", "aux_line", "copyBuffer"):
         if k in name:
             return k
     return name.split("(")[0][:60]
 
 
-def _counter_means(d, counter):
+def _counter_rows(d, counter):
     import csv
     agg = {}
     with open(os.path.join(d, "run_counter_collection.csv")) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] == counter:
                 agg.setdefault(_short(r["Kernel_Name"]), []).append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in agg.items()}
+    return agg
 
 
 def traffic_summary(args):
     """HBM bytes per launch of each kernel from rocprofv3 --pmc runs of this bench: FETCH_SIZE and WRITE_SIZE
     (KiB per dispatch) of the workload, each scaled by known bytes / counter for the 1 GiB device copy
-    (`--workload copy`, the runtime's copyBuffer kernel: 1 GiB read + 1 GiB written)."""
+    (`--workload copy`, the runtime's copyBuffer kernel: 1 GiB read + 1 GiB written).  Launches per step are
+    the kernel's dispatches over the step kernel's (h_pass_3d, or the single fused pass)."""
     fetch_dir, write_dir, cfetch_dir, cwrite_dir = args.traffic_summary
     known = float(1 << 30)
-    cf = _counter_means(cfetch_dir, "FETCH_SIZE")["copyBuffer"] * 1024.0
-    cw = _counter_means(cwrite_dir, "WRITE_SIZE")["copyBuffer"] * 1024.0
+    mean = lambda v: sum(v) / len(v)
+    cf = mean(_counter_rows(cfetch_dir, "FETCH_SIZE")["copyBuffer"]) * 1024.0
+    cw = mean(_counter_rows(cwrite_dir, "WRITE_SIZE")["copyBuffer"]) * 1024.0
     ff, wf = known / cf, known / cw
-    fetch = _counter_means(fetch_dir, "FETCH_SIZE")
-    write = _counter_means(write_dir, "WRITE_SIZE")
-    kern = {k: {"fetch_bytes": round(fetch[k] * 1024.0 * ff), "write_bytes": round(write.get(k, 0.0) * 1024.0 * wf)}
-            for k in fetch if k in ("e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedw_3d", "fused_3d")}
+    fetch = _counter_rows(fetch_dir, "FETCH_SIZE")
+    write = _counter_rows(write_dir, "WRITE_SIZE")
+    step_kernel = "fused3d_lds" if "fused3d_lds" in fetch else ("h_pass_3d" if "h_pass_3d" in fetch else "fusedw_3d")
+    nsteps = len(fetch[step_kernel])
+    kern = {k: {"fetch_bytes": round(mean(fetch[k]) * 1024.0 * ff),
+                "write_bytes": round(mean(write.get(k, [0.0])) * 1024.0 * wf),
+                "launches_per_step": round(len(fetch[k]) / nsteps, 5)}
+            for k in fetch if k in PASS_KERNELS}
+    per_step = sum(v["launches_per_step"] * (v["fetch_bytes"] + v["write_bytes"]) for v in kern.values())
     out = {"workload": "3-D 256^3 FP32 forward (bench.py default workload)", "tile": args.tile,
            "fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
-           "copy_counters_kib": {"FETCH_SIZE": cf / 1024.0, "WRITE_SIZE": cw / 1024.0}, "kernels": kern}
+           "copy_counters_kib": {"FETCH_SIZE": cf / 1024.0, "WRITE_SIZE": cw / 1024.0}, "kernels": kern,
+           "bytes_per_step": round(per_step)}
     with open(args.traffic_out, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out), flush=True)
 
 
-def step_traffic(tile, kernels_per_step):
-    """HBM bytes of one step from the committed counter summary (taken with one two-pass tile; the bytes a
-    two-pass step moves do not depend on the tile to first order), for a two-pass tile."""
+def step_traffic(tile, per_step=None, steps=None):
+    """HBM bytes of one step from the committed counter summary, when it was taken at this tile (the bytes a
+    schedule moves depend on it); None otherwise."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except OSError:
         return None
-    if tile[0] < 2:
+    if t.get("tile") and [int(v) for v in str(t["tile"]).split(",")] != list(tile):
         return None
-    total = 0.0
-    for k, per_step in kernels_per_step.items():
-        if k not in t["kernels"]:
-            return None
-        total += per_step * (t["kernels"][k]["fetch_bytes"] + t["kernels"][k]["write_bytes"])
-    return round(total)
+    return t.get("bytes_per_step")
 
 
 def main():
@@ -493,7 +550,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample-steps", type=int, default=150)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "unit_cell", "copy"],
                     help="c2 (default): the headline 3-D 256^3 forward; c1: 2-D TM 256^2; c3: dispersive sphere; "
@@ -558,6 +615,8 @@ def main():
     # device time of the kernels of one step (the aux line runs once per 256-step chunk)
     kern_ms = sum(v for k, v in per_step.items() if k != "aux_line")
     achieved = abytes / (kern_ms / 1e3) / 1e9
+    traffic = step_traffic(tile, per_step, steps=args.steps) if (args.size == 256 and args.precision == "f32"
+                                                                 and not strip) else None
     out = {
         "metric": METRIC, "value": round(gcells, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 5), "higher_is_better": True,
@@ -569,13 +628,15 @@ def main():
                    "graph_chunks": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": step_traffic(tile, {"e_pass_3d": 1.0, "h_pass_3d": 1.0, "dft_sample": 1.0,
-                                                    "dft_fold": 1.0 / 16}) if args.size == 256 and
-                     args.precision == "f32" and tile[0] >= 2 and not strip else None,
+                     "traffic": traffic,
+                     "traffic_source": TRAFFIC_SOURCE if traffic else None,
                      "kernel": "one step: " + " + ".join(k for k in avg if k != "aux_line"),
                      "algorithmic_bytes_per_step": abytes, "avg_ms": avg, "ms_per_step_by_kernel": per_step},
     }
     g.close()
+    copy_gbs = copy_rate()
+    out["roofline"]["copy_GBs"] = round(copy_gbs, 1)
+    out["roofline"]["frac_of_copy"] = round(achieved / copy_gbs, 4)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.size, args.cpu_sample_steps)
     print(json.dumps(out), flush=True)
