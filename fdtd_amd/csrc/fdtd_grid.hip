@@ -14,6 +14,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -1647,6 +1648,53 @@ int surface_impl(fdtd_grid* g, const fdtdk::SurfRegion& r, long cells, double* o
   return 0;
 }
 
+// Device staging of fdtd_surface_quantities_static: one copy per GPU, held
+// until the process ends.  Host code uses the address hipGetSymbolAddress
+// returns for the current device (scratch_ptr), never the array's name, which
+// on the host is only a stand-in symbol.
+constexpr long kSurfStaticMax = FDTD_SURFACE_STATIC_MAX_CELLS;
+static __device__ __align__(256) double scratch[7 * kSurfStaticMax];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the scratch array's address on the current device, resolved once per device
+int scratch_ptr(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(FDTD_EDEVICE, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(FDTD_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+template <typename T>
+int surface_static_impl(fdtd_grid* g, const fdtdk::SurfRegion& r, long cells, double* out) {
+  const int nq = g->ndim == 3 ? 7 : 3;
+  double* dev = nullptr;
+  int rc = scratch_ptr(&dev);
+  if (rc) return rc;
+  const long sy = g->d.nx, sz = long(g->d.nx) * g->d.ny;
+  auto F = [](void* p) { return (const T*)p; };
+  const T* z = nullptr;
+  // every output element is written by the kernel: nothing to clear
+  hipLaunchKernelGGL(fdtdk::surface_quantities<T>, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, g->stream,
+                     g->ndim == 3 ? F(g->E[0]) : z, g->ndim == 3 ? F(g->E[1]) : z, F(g->E[2]), F(g->H[0]), F(g->H[1]),
+                     g->ndim == 3 ? F(g->H[2]) : z, sy, sz, r, g->ndim == 3 ? 1 : 0, g->d.dx, kEps0, dev);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dev, size_t(nq) * cells * sizeof(double), hipMemcpyDeviceToHost, g->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+  if (e != hipSuccess) return fail(FDTD_EDEVICE, "surface quantities: %s", hipGetErrorString(e));
+  return 0;
+}
+
 extern "C" {
 
 const char* fdtd_last_error(void) { return g_err.c_str(); }
@@ -2212,6 +2260,22 @@ int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out) {
   const fdtdk::SurfRegion r{region[0], region[1], region[2], region[3], region[4], region[5]};
   const long cells = long(r.x1 - r.x0) * (r.y1 - r.y0) * (r.z1 - r.z0);
   return g->f64 ? surface_impl<double>(g, r, cells, out) : surface_impl<float>(g, r, cells, out);
+}
+
+int fdtd_surface_quantities_static(fdtd_grid* g, const int region[6], double* out) {
+  if (!g || !region || !out) return fail(FDTD_EINVAL, "null argument");
+  for (int a = 0; a < 3; a++) {
+    const int lo = a < g->ndim ? 1 : 0, hi = a < g->ndim ? g->len[a] - 1 : 1;
+    if (region[2 * a] < lo || region[2 * a + 1] > hi || region[2 * a] >= region[2 * a + 1])
+      return fail(FDTD_EINVAL, "region must be a non-empty box of interior cells");
+  }
+  const fdtdk::SurfRegion r{region[0], region[1], region[2], region[3], region[4], region[5]};
+  const long cells = long(r.x1 - r.x0) * (r.y1 - r.y0) * (r.z1 - r.z0);
+  if (cells > kSurfStaticMax)
+    return fail(FDTD_EINVAL, "region of %ld cells exceeds FDTD_SURFACE_STATIC_MAX_CELLS (%ld)", cells, kSurfStaticMax);
+  // the device set-up allocates: it must have run already
+  if (!g->finalized) return fail(FDTD_ESTATE, "grid not set up on the device yet: step it (or call fdtd_surface_quantities) once first");
+  return g->f64 ? surface_static_impl<double>(g, r, cells, out) : surface_static_impl<float>(g, r, cells, out);
 }
 
 int fdtd_set_graph(fdtd_grid* g, int on) {

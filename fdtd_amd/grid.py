@@ -34,6 +34,7 @@ ABI_SYMBOLS = [
     "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph", "fdtd_set_boundary",
     "fdtd_set_bloch_boundary", "fdtd_get_probes_imag",
     "fdtd_surface_quantities",
+    "fdtd_surface_quantities_static",
 ]
 
 
@@ -89,6 +90,7 @@ def load_library():
         "fdtd_set_graph": (_i, [_p, _i]),
         "fdtd_set_boundary": (_i, [_p, _i, _i, _i]),
         "fdtd_surface_quantities": (_i, [_p, _ip, _dp]),
+        "fdtd_surface_quantities_static": (_i, [_p, _ip, _dp]),
         "fdtd_get_tile": (_i, [_p, _ip, _ip, _ip]),
         "fdtd_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
         "fdtd_set_adjoint_tile": (_i, [_p, _i, _i, _i]),
@@ -279,15 +281,18 @@ class Grid:
         f = complex(factor)
         _check(load_library().fdtd_set_bloch_boundary(self.h, int(axis), int(side), f.real, f.imag))
 
-    def surface_quantities(self, region):
+    def surface_quantities(self, region, static=False):
         """Surface charge density and currents of the current state on `region`
         (x0, x1, y0, y1, z0, z1): array (nq, z, y, x); 3-D rho, Jex, Jey, Jez, Jhx,
-        Jhy, Jhz; TM Jez, Jhx, Jhy."""
+        Jhy, Jhz; TM Jez, Jhx, Jhy.  static=True: the entry point that allocates
+        nothing (regions of at most FDTD_SURFACE_STATIC_MAX_CELLS cells, a grid
+        already set up on the device)."""
         r = [int(v) for v in region]
         nq = 7 if self.mode == "ThreeD" else 3
         shape = (r[5] - r[4], r[3] - r[2], r[1] - r[0])
         out = np.zeros(nq * int(np.prod(shape)))
-        _check(load_library().fdtd_surface_quantities(self.h, (ctypes.c_int * 6)(*r), _dptr(out)))
+        fn = load_library().fdtd_surface_quantities_static if static else load_library().fdtd_surface_quantities
+        _check(fn(self.h, (ctypes.c_int * 6)(*r), _dptr(out)))
         return out.reshape((nq,) + shape)
 
     def set_graph(self, on=True):

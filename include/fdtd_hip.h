@@ -195,6 +195,18 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
  * complex grid (Bloch face) these are the quantities of the real part. */
 int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out);
 
+/* The same quantities as fdtd_surface_quantities, without allocating: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving or capture loops, harnesses that forbid hipMalloc).
+ * The device staging is a fixed array of the library on each GPU
+ * (7 x FDTD_SURFACE_STATIC_MAX_CELLS doubles, 3.5 MiB, held until the process
+ * ends); regions of more cells are refused (FDTD_EINVAL), and so is a grid
+ * whose device set-up has not run yet (FDTD_ESTATE: step it, or call
+ * fdtd_surface_quantities, once first).  Calls on one GPU share that array:
+ * calls that overlap in time on one GPU race; run them one after another. */
+#define FDTD_SURFACE_STATIC_MAX_CELLS 65536
+int fdtd_surface_quantities_static(fdtd_grid* g, const int region[6], double* out);
+
 /* Boundary condition on one face (axis 0..ndim-1, side 0 = min, 1 = max).
  * Replaces make_pec_boundary / make_pmc_boundary / make_symmetric_boundary /
  * make_antisymmetric_boundary / make_periodic_boundary + BoundaryData::
