@@ -116,9 +116,10 @@ constexpr int kChunk = 256;  // steps per incident-table chunk
 constexpr int kNumKernels = 8;
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
-// bx == 0 selects the LDS fused single pass with `by` rows per workgroup,
-// bx == 1 the wave-fused single pass with `by` rows per wave; bx >= 2 the
-// two passes with bx x by threads per workgroup
+// bx == 0 selects the streamed fused pass (3-D FP32) with E regions of `by`
+// rows and `kc` workgroups per compute unit, bx == 1 the wave-fused single
+// pass with `by` rows per wave; bx >= 2 the two passes with bx x by threads
+// per workgroup
 struct Tile {
   int bx, by, kc;
 };
@@ -262,6 +263,7 @@ struct fdtd_grid {
   double *inc_allE = nullptr, *inc_allH = nullptr;  // incident rows of steps [0, inc_rows)
   long inc_rows = 0;
 
+  int ncu = 256;  // compute units of the grid's device (the streamed pass launches a multiple of them)
   Tile tile = {0, 0, 0};
   Tile rtile = {0, 0, 0};  // the adjoint gradient's reverse sweep and recompute (0: the grid's tile)
   // Light-cone recompute of an adjoint segment ending at step cone_last: the
@@ -988,6 +990,9 @@ int step_impl(fdtd_grid* g, int nsteps) {
   if (g->ndim == 2) t.kc = 1;
   // design increments and boundary ghost writes are made by the two-pass kernels
   if ((g->force_two_pass || g->nbc || has_magnetized(g)) && t.bx <= 1) t = kDefaultTile3D;
+  // the streamed pass is FP32, addresses fields with 32-bit buffer offsets and
+  // keeps the z coefficients in LDS
+  if (t.bx == 0 && (sizeof(T) != 4 || a.field_bytes == 0 || g->d.nz > fdtdk::kSMaxZ || g->design)) t = kDefaultTile3D;
 
   const bool wavef = g->ndim == 3 && t.bx == 1;
   const bool fused = g->ndim == 3 && t.bx <= 1;
@@ -996,6 +1001,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
     if (rc) return rc;
   }
   const int nzi = g->ndim == 3 ? g->d.nz - 2 : 1;
+  long units = 0;  // streamed pass: (tile, plane) units
   const bool graphs = g->use_graph && !g->prof && !g->cone_tab;
   const bool cone = g->cone_tab && g->recompute && !fused && !g->nbc && !g->cplx && !has_magnetized(g) &&
                     t.bx == g->cone_tile.bx && t.by == g->cone_tile.by && t.kc == g->cone_tile.kc;
@@ -1008,11 +1014,13 @@ int step_impl(fdtd_grid* g, int nsteps) {
     block = dim3(64, fdtdk::kWavesY, 1);
     grid = dim3(ntx * nty * nkc, 1, 1);
   } else if (fused) {
-    ntx = (g->d.nx - 2 + fdtdk::kFusedCols - 1) / fdtdk::kFusedCols;
-    nty = (g->d.ny - 2 + t.by - 1) / t.by;
-    nkc = (nzi + t.kc - 1) / t.kc;
-    block = dim3(256, t.by + 1, 1);
-    grid = dim3(ntx * nty * nkc, 1, 1);
+    // streamed pass: E regions of kSCols x kSRows cells own one column and one
+    // row less; ncu * kc persistent workgroups split the (tile, plane) units
+    ntx = (g->d.nx - 2 + fdtdk::kSCols - 2) / (fdtdk::kSCols - 1);
+    nty = (g->d.ny - 2 + fdtdk::kSRows - 2) / (fdtdk::kSRows - 1);
+    units = long(ntx) * nty * nzi;
+    block = dim3(fdtdk::kSThreads, 1, 1);
+    grid = dim3(unsigned(std::min<long>(long(g->ncu) * t.kc, units)), 1, 1);
   } else if (g->ndim == 3) {
     ntx = (g->d.nx + t.bx - 1) / t.bx;
     nty = (g->d.ny - 2 + t.by - 1) / t.by;
@@ -1082,12 +1090,14 @@ int step_impl(fdtd_grid* g, int nsteps) {
           else
             rc = launch(g, 3, fdtdk::fusedw_3d<T, 6>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         } else if (fused) {
-          if (t.by == 1)
-            rc = launch(g, 3, fdtdk::fused_3d<T, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-          else if (t.by == 2)
-            rc = launch(g, 3, fdtdk::fused_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-          else
-            rc = launch(g, 3, fdtdk::fused_3d<T, 3>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          if constexpr (sizeof(T) == 4) {
+            if (g->max_np > 0)
+              rc = launch(g, 3, fdtdk::fused3d_lds<1>, grid, block, (const fdtdk::StepArgs<float>*)ap, s, ntx, units);
+            else
+              rc = launch(g, 3, fdtdk::fused3d_lds<0>, grid, block, (const fdtdk::StepArgs<float>*)ap, s, ntx, units);
+          } else {
+            rc = fail(FDTD_ESTATE, "the streamed pass is FP32 only");
+          }
         } else if (g->cplx) {
           // complex fields: each pass over both parts, then the Bloch ghosts
           // from both parts' new values
@@ -1711,6 +1721,7 @@ int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return fail(FDTD_EDEVICE, "no HIP device");
   fdtd_grid* g = new fdtd_grid();
+  if (hipDeviceGetAttribute(&g->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || g->ncu < 1) g->ncu = 256;
   g->d = *desc;
   g->d.nz = nz;
   g->ndim = desc->mode == FDTD_3D ? 3 : 2;
@@ -2113,8 +2124,11 @@ long fdtd_step_count(const fdtd_grid* g) { return g ? g->step : -1; }
 int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
   if (bx == 0) {
-    if (g->ndim != 3 || by < 1 || by > 3 || kc <= 0)
-      return fail(FDTD_EINVAL, "the fused schedule (bx = 0) is 3-D only, with by in {1, 2, 3} rows and kc >= 1");
+    if (g->ndim != 3 || by != fdtdk::kSRows || kc < 1 || kc > 2)
+      return fail(FDTD_EINVAL,
+                  "the streamed fused schedule (bx = 0) is 3-D only, with by = %d E-region rows and kc in 1..2 "
+                  "workgroups per compute unit (FP64 grids, and grids of more than %d planes, run it as the default "
+                  "two-pass tile)", fdtdk::kSRows, fdtdk::kSMaxZ);
   } else if (bx == 1) {
     if (g->ndim != 3 || (by != 2 && by != 4 && by != 6) || kc <= 0)
       return fail(FDTD_EINVAL, "the wave-fused schedule (bx = 1) is 3-D only, with by in {2, 4, 6} rows and kc >= 1");
@@ -2141,8 +2155,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   if (!g || trial_steps < 1) return fail(FDTD_EINVAL, "bad argument");
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
-  static const Tile cand3[] = {{256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {256, 1, 8}, {128, 2, 2}, {128, 2, 4},
-                               {64, 4, 1},  {64, 4, 2},  {0, 2, 16},  {0, 3, 16},  {1, 2, 8},   {1, 2, 16},
+  static const Tile cand3[] = {{0, 5, 1},  {0, 5, 2},  {256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {256, 1, 8},
+                               {128, 2, 2}, {128, 2, 4}, {64, 4, 1}, {64, 4, 2}, {1, 2, 8}, {1, 2, 16},
                                {1, 4, 8},   {1, 4, 16}};
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;
@@ -2158,8 +2172,9 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   std::copy(g->kcount, g->kcount + kNumKernels, kcount);
   const Tile saved = g->tile;
   double best = 1e300;
-  Tile bt = cand[0];
+  Tile bt = g->ndim == 3 ? kDefaultTile3D : cand[0];
   for (int c = 0; c < ncand && !rc; c++) {
+    if (g->ndim == 3 && cand[c].bx == 0 && g->f64) continue;  // the streamed pass is FP32 only
     g->tile = cand[c];
     if ((rc = fdtd_step(g, 1))) break;
     if ((rc = fdtd_set_profiling(g, 1))) break;

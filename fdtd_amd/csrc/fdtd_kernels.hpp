@@ -38,11 +38,19 @@
 
 namespace fdtdk {
 
+// The cell arithmetic below turns off automatic FMA contraction and writes
+// its fused multiply-adds explicitly: with contraction left to the compiler,
+// an expression such as a*b - c*d may be fused on either product depending on
+// the surrounding code, so schedules that inline the same cell function into
+// different kernels could round differently.  Explicit FMAs keep every
+// schedule bitwise equal.
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 constexpr int kMaxPoles = 3;
 constexpr int kMaxSrc = 16;
 constexpr int kMaxDft = 8;
 constexpr int kMaxProbe = 32;
-constexpr int kFusedCols = 254;  // owned columns of a fused-pass workgroup (256 threads)
 
 template <typename T>
 struct MatDev {
@@ -215,14 +223,16 @@ __device__ __forceinline__ long slab_off(const StepArgs<T>& a, int J, int s, int
 
 // Aux1D::interpolateE / interpolateH (include/Source.hpp:442-458) on a row of
 // the incident table.
-__device__ __forceinline__ double interp_row(const FDTD_G double* row, int L, double loc, double dx, bool h) {
+template <typename Row>
+__device__ __forceinline__ double interp_row(Row row, int L, double loc, double dx, bool h) {
+#pragma clang fp contract(off)
   double nd = (double)L / 2.0 + loc / dx;
   if (h) nd -= 0.5;
   long n = (long)nd;
   if (n < 0) n = 0;
   if (n > L - 2) n = L - 2;
   double frac = nd - (double)n;
-  return (1.0 - frac) * row[n] + frac * row[n + 1];
+  return fmadd(frac, row[n + 1], (1.0 - frac) * row[n]);
 }
 
 // Levi-Civita of distinct axes
@@ -262,20 +272,30 @@ __device__ __forceinline__ const FDTD_G StepArgs<T>& cold_args(const StepArgs<T>
 // pass keep them in scalar registers (the cold view measured slower there).  (The cold view is read as
 // a generic reference here: typed in the global address space it leaves the
 // E pass with more registers, measured.)
-template <typename T, int NDIM, bool COLD = false>
-__device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a0, const FDTD_G double* row, int ft, int i, int j, int k, const bool actF[3],
-                          const bool actC[3], T delta[3]) {
+// row: the step's incident row (global memory, or its copy in LDS); fn: the
+// cell's curl coefficients dt F/dx per axis for this field type (null: read
+// from the argument block, which keeps them out of registers until needed).
+template <typename T, int NDIM, bool COLD = false, typename Row = const FDTD_G double*>
+__device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a0, Row row, int ft, int i, int j, int k, const bool actF[3],
+                                    const bool actC[3], const T fn[3], T delta[3]) {
+#pragma clang fp contract(off)
   const StepArgs<T>& a = COLD ? (const StepArgs<T>&)cold_args(a0) : a0;
   const int ci[3] = {i, j, k};
+  // loops fully unrolled: every index of delta[] and ci[] is a constant, so
+  // neither array needs memory (a dynamically indexed one would be placed in
+  // LDS or scratch)
+#pragma unroll
   for (int J = 0; J < NDIM; J++) {
     int side = (ci[J] == a.box[2 * J]) ? 0 : ((ci[J] == a.box[2 * J + 1]) ? 1 : -1);
     if (side < 0) continue;
     const double nrm = side == 0 ? -1.0 : 1.0;
+#pragma unroll
     for (int I = 0; I < 3; I++) {
       if (I == J || !actF[I]) continue;
       const int K = 3 - I - J;
       if (!actC[K]) continue;
       bool inside = true;
+#pragma unroll
       for (int Tn = 0; Tn < NDIM; Tn++) {
         if (Tn == J) continue;
         bool half = ft == 0 ? (Tn != I) : (Tn == I);
@@ -285,14 +305,15 @@ __device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a0, const FDTD_G double* 
       }
       if (!inside) continue;
       double proj = 0.0;
+#pragma unroll
       for (int dd = 0; dd < NDIM; dd++) {
         double off = ft == 0 ? (dd == K ? -0.5 : 0.0) : (dd != K ? -0.5 : 0.0);
-        double loc = ((double)ci[dd] - a.origin[dd]) * a.dx + off * a.dx;
-        proj += a.kvec[dd] * loc;
+        double loc = fmadd(off, a.dx, ((double)ci[dd] - a.origin[dd]) * a.dx);
+        proj = fmadd(a.kvec[dd], loc, proj);
       }
       double inc = interp_row(row, a.auxL, proj, a.dx, ft == 0);
       double pv = ft == 0 ? a.polH[K] : a.pol[K];
-      double f = (double)a.cf[ft][J][ci[J]];
+      double f = fn ? (double)fn[J] : (double)a.cf[ft][J][ci[J]];
       double term = lc3(I, J) * f * nrm * inc * pv;
       delta[I] += (T)(ft == 0 ? term : -term);
     }
@@ -326,15 +347,16 @@ __device__ FDTD_COLD void add_sources(const StepArgs<T>& a0, long step, int fiel
 // (DrudeUpdate / LorentzUpdate implicit_update, DispersiveMaterials.hpp:710-735, 979-998).
 template <typename T, int I>
 __device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>& m, long bq, T d, T Eold, bool own) {
+#pragma clang fp contract(off)
   const T D = a.Dd[I][bq] + d;
   if (sizeof(T) == 4 && m.np == 1 && m.cD[0] != T(0)) {
     // FP32, a lone conductive pole: E_{n+1} = (E_n + d/(eps_r eps0))/(1 + f),
     // the same update without the cancellation of D - P (P -> D in a good
     // conductor, which in FP32 leaves E at a few digits); P = D - eps_r eps0 E
-    const T Ev = (Eold + d * m.inv_eps) * m.cG;
+    const T Ev = fmadd(d, m.inv_eps, Eold) * m.cG;
     if (own) {
       a.Ddo[I][bq] = D;
-      a.Pdo[0][I][bq] = D - m.eps_e0 * Ev;
+      a.Pdo[0][I][bq] = fmadd(-m.eps_e0, Ev, D);
       a.Jdo[0][I][bq] = a.Jd[0][I][bq];
     }
     return Ev;
@@ -346,9 +368,9 @@ __device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>
     if (p < m.np) {
       Jv[p] = a.Jd[p][I][bq];
       if (m.cD[p] != T(0))
-        Pv[p] = (a.Pd[p][I][bq] + m.cD[p] * D) / (T(1) + m.cD[p]);
+        Pv[p] = fmadd(m.cD[p], D, a.Pd[p][I][bq]) / (T(1) + m.cD[p]);
       else
-        Pv[p] = a.Pd[p][I][bq] + a.dt * Jv[p];
+        Pv[p] = fmadd(a.dt, Jv[p], a.Pd[p][I][bq]);
       num -= Pv[p];
     }
   }
@@ -361,8 +383,8 @@ __device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>
     T inc = d;
 #pragma unroll
     for (int p = 0; p < kMaxPoles; p++)
-      if (p < m.np) inc -= a.dt * Jv[p];
-    Ev = Eold + inc * m.inv_eps;
+      if (p < m.np) inc = fmadd(-a.dt, Jv[p], inc);
+    Ev = fmadd(inc, m.inv_eps, Eold);
   }
   if (own) {
     a.Ddo[I][bq] = D;
@@ -373,7 +395,8 @@ __device__ __forceinline__ T ade_component(const StepArgs<T>& a, const MatDev<T>
         // FP32: cJ J as J - (1 - cJ) J.  cJ is within a few 1e-3 of 1 and its FP32
         // rounding (~6e-8) compounds over the steps (cJ^n); 1 - cJ is stored to
         // FP32's relative precision, so the pole's decay stays that of FP64
-        a.Jdo[p][I][bq] = (sizeof(T) == 4 ? Jv[p] - m.cJm[p] * Jv[p] : m.cJ[p] * Jv[p]) + m.cE[p] * Ev + m.cP[p] * Pv[p];
+        const T jj = sizeof(T) == 4 ? fmadd(-m.cJm[p], Jv[p], Jv[p]) : m.cJ[p] * Jv[p];
+        a.Jdo[p][I][bq] = fmadd(m.cP[p], Pv[p], fmadd(m.cE[p], Ev, jj));
       }
     }
   }
@@ -472,6 +495,43 @@ __device__ FDTD_COLD void ade_cell(const StepArgs<T>& a, int mid, int i, int j, 
   out[2] = ade_component<T, 2>(a, m, bq, d[2], eold[2], own);
 }
 
+// Inputs of a cell update that a caller may gather ahead of it (the streamed
+// pass does, a plane early): per slab axis J the cell's two CPML psi of its
+// field type (components I != J in increasing order) and the axis
+// coefficients dt G, B and C/dx at the cell's coordinate.
+// One soft source's value at the current step, staged where the update
+// reads it (field 0: D, 2: B; the value already converted to the field type).
+template <typename T>
+struct SrcVal {
+  long cell;
+  int field, comp;
+  T v;
+};
+
+template <typename T>
+struct CellPre {
+  T psi[3][2];
+  T g[3], b[3], c[3];
+  const SrcVal<T>* src;  // the step's source values (null: none)
+  int nsrc;
+};
+
+// the staged form of add_sources: the same additions in the same order
+template <typename T>
+__device__ __forceinline__ void add_staged_sources(const CellPre<T>& pre, int field, long c, T d[3]) {
+  for (int s = 0; s < pre.nsrc; s++) {
+    const SrcVal<T>& q = pre.src[s];
+    if (q.cell == c && q.field == field) {
+      if (q.comp == 0)
+        d[0] += q.v;
+      else if (q.comp == 1)
+        d[1] += q.v;
+      else
+        d[2] += q.v;
+    }
+  }
+}
+
 // H values around a cell that the E update reads (backward neighbours)
 template <typename T>
 struct HStencil {
@@ -491,10 +551,13 @@ struct EStencil {
 // 2: also magnetized and fluid Drude materials.
 // IP: the in-place passes, whose psi is read and written in the same arrays
 // (one set of pointers instead of two: fewer scalar registers).
-template <typename T, int DISP = 1, bool IP = false>
-__device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incH, int i, int j, int k,
-                                          long c, const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy,
-                                          T fz, bool own, T out[3]) {
+// PRE: psi and the slab coefficients come from *pre (gathered by the caller)
+// instead of memory; Row: the incident row's pointer type (global or LDS).
+template <typename T, int DISP = 1, bool IP = false, bool PRE = false, typename Row = const FDTD_G double*>
+__device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, Row incH, int i, int j, int k, long c,
+                                          const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy, T fz,
+                                          bool own, T out[3], const CellPre<T>* pre = nullptr) {
+#pragma clang fp contract(off)
   const T dxy = h.hz - h.hz_ym;  // Ex, J=y, +1
   const T dxz = h.hy - h.hy_zm;  // Ex, J=z, -1
   const T dyz = h.hx - h.hx_zm;  // Ey, J=z, +1
@@ -502,54 +565,61 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
   const T dzx = h.hy - h.hy_xm;  // Ez, J=x, +1
   const T dzy = h.hx - h.hx_ym;  // Ez, J=y, -1
   T d[3];
-  d[0] = fy * dxy - fz * dxz;
-  d[1] = fz * dyz - fx * dyx;
-  d[2] = fx * dzx - fy * dzy;
-  if (a.nsrc) add_sources<T, true>(a, step, 0, c, d);
+  d[0] = fmadd(fy, dxy, -(fz * dxz));
+  d[1] = fmadd(fz, dyz, -(fx * dyx));
+  d[2] = fmadd(fx, dzx, -(fy * dzy));
+  if (PRE)
+    add_staged_sources(*pre, 0, c, d);
+  else if (a.nsrc)
+    add_sources<T, true>(a, step, 0, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, k);
-    const T g = a.cg[0][0][i], b = a.cb[0][0][i], cc = a.cc[0][0][i];
-    const T p1 = a.psi[0][1][0][o], p2 = a.psi[0][2][0][o];
-    d[1] += g * p1;
-    d[2] += g * p2;
+    const T g = PRE ? pre->g[0] : a.cg[0][0][i], b = PRE ? pre->b[0] : a.cb[0][0][i],
+            cc = PRE ? pre->c[0] : a.cc[0][0][i];
+    const T p1 = PRE ? pre->psi[0][0] : a.psi[0][1][0][o], p2 = PRE ? pre->psi[0][1] : a.psi[0][2][0][o];
+    d[1] = fmadd(g, p1, d[1]);
+    d[2] = fmadd(g, p2, d[2]);
     if (own) {
-      (IP ? a.psi : a.psio)[0][1][0][o] = b * p1 - cc * dyx;
-      (IP ? a.psi : a.psio)[0][2][0][o] = b * p2 + cc * dzx;
+      (IP ? a.psi : a.psio)[0][1][0][o] = fmadd(b, p1, -(cc * dyx));
+      (IP ? a.psi : a.psio)[0][2][0][o] = fmadd(b, p2, cc * dzx);
     }
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
     const long o = slab_off(a, 1, syi, i, j, k);
-    const T g = a.cg[0][1][j], b = a.cb[0][1][j], cc = a.cc[0][1][j];
-    const T p0 = a.psi[0][0][1][o], p2 = a.psi[0][2][1][o];
-    d[0] += g * p0;
-    d[2] += g * p2;
+    const T g = PRE ? pre->g[1] : a.cg[0][1][j], b = PRE ? pre->b[1] : a.cb[0][1][j],
+            cc = PRE ? pre->c[1] : a.cc[0][1][j];
+    const T p0 = PRE ? pre->psi[1][0] : a.psi[0][0][1][o], p2 = PRE ? pre->psi[1][1] : a.psi[0][2][1][o];
+    d[0] = fmadd(g, p0, d[0]);
+    d[2] = fmadd(g, p2, d[2]);
     if (own) {
-      (IP ? a.psi : a.psio)[0][0][1][o] = b * p0 + cc * dxy;
-      (IP ? a.psi : a.psio)[0][2][1][o] = b * p2 - cc * dzy;
+      (IP ? a.psi : a.psio)[0][0][1][o] = fmadd(b, p0, cc * dxy);
+      (IP ? a.psi : a.psio)[0][2][1][o] = fmadd(b, p2, -(cc * dzy));
     }
   }
   const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
   if (szi >= 0) {
     const long o = slab_off(a, 2, szi, i, j, k);
-    const T g = a.cg[0][2][k], b = a.cb[0][2][k], cc = a.cc[0][2][k];
-    const T p0 = a.psi[0][0][2][o], p1 = a.psi[0][1][2][o];
-    d[0] += g * p0;
-    d[1] += g * p1;
+    const T g = PRE ? pre->g[2] : a.cg[0][2][k], b = PRE ? pre->b[2] : a.cb[0][2][k],
+            cc = PRE ? pre->c[2] : a.cc[0][2][k];
+    const T p0 = PRE ? pre->psi[2][0] : a.psi[0][0][2][o], p1 = PRE ? pre->psi[2][1] : a.psi[0][1][2][o];
+    d[0] = fmadd(g, p0, d[0]);
+    d[1] = fmadd(g, p1, d[1]);
     if (own) {
-      (IP ? a.psi : a.psio)[0][0][2][o] = b * p0 - cc * dxz;
-      (IP ? a.psi : a.psio)[0][1][2][o] = b * p1 + cc * dyz;
+      (IP ? a.psi : a.psio)[0][0][2][o] = fmadd(b, p0, -(cc * dxz));
+      (IP ? a.psi : a.psio)[0][1][2][o] = fmadd(b, p1, cc * dyz);
     }
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
     const bool act[3] = {true, true, true};
-    tfsf_cell<T, 3, (DISP > 0)>(a, incH, 0, i, j, k, act, act, d);
+    const T fn[3] = {fx, fy, fz};
+    tfsf_cell<T, 3, (DISP > 0)>(a, incH, 0, i, j, k, act, act, PRE ? fn : nullptr, d);
   }
   if (ie > T(0)) {
-    out[0] = ex + d[0] * ie;
-    out[1] = ey + d[1] * ie;
-    out[2] = ez + d[2] * ie;
+    out[0] = fmadd(d[0], ie, ex);
+    out[1] = fmadd(d[1], ie, ey);
+    out[2] = fmadd(d[2], ie, ez);
   } else if (DISP && ie < T(0) && a.Dd[0]) {
     const T eold[3] = {ex, ey, ez};
     const T hown[3] = {h.hx, h.hy, h.hz};
@@ -562,10 +632,11 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, const
 }
 
 // ---------------------------------------------------------------- H of one cell
-template <typename T, bool IP = false>
-__device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const FDTD_G double* incE, int i, int j, int k,
-                                          long c, const EStencil<T>& e, T hx, T hy, T hz, T im, T fx, T fy, T fz,
-                                          T out[3]) {
+template <typename T, bool IP = false, bool PRE = false, typename Row = const FDTD_G double*>
+__device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, Row incE, int i, int j, int k, long c,
+                                          const EStencil<T>& e, T hx, T hy, T hz, T im, T fx, T fy, T fz, T out[3],
+                                          const CellPre<T>* pre = nullptr) {
+#pragma clang fp contract(off)
   const T dxy = e.ez_yp - e.ez;  // Hx, J=y, +1
   const T dxz = e.ey_zp - e.ey;  // Hx, J=z, -1
   const T dyz = e.ex_zp - e.ex;  // Hy, J=z, +1
@@ -573,47 +644,54 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, const
   const T dzx = e.ey_xp - e.ey;  // Hz, J=x, +1
   const T dzy = e.ex_yp - e.ex;  // Hz, J=y, -1
   T d[3];
-  d[0] = fz * dxz - fy * dxy;
-  d[1] = fx * dyx - fz * dyz;
-  d[2] = fy * dzy - fx * dzx;
-  if (a.nsrc) add_sources(a, step, 2, c, d);
+  d[0] = fmadd(fz, dxz, -(fy * dxy));
+  d[1] = fmadd(fx, dyx, -(fz * dyz));
+  d[2] = fmadd(fy, dzy, -(fx * dzx));
+  if (PRE)
+    add_staged_sources(*pre, 2, c, d);
+  else if (a.nsrc)
+    add_sources(a, step, 2, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, k);
-    const T g = a.cg[1][0][i], b = a.cb[1][0][i], cc = a.cc[1][0][i];
-    const T p1 = a.psi[1][1][0][o], p2 = a.psi[1][2][0][o];
-    d[1] -= g * p1;
-    d[2] -= g * p2;
-    (IP ? a.psi : a.psio)[1][1][0][o] = b * p1 - cc * dyx;
-    (IP ? a.psi : a.psio)[1][2][0][o] = b * p2 + cc * dzx;
+    const T g = PRE ? pre->g[0] : a.cg[1][0][i], b = PRE ? pre->b[0] : a.cb[1][0][i],
+            cc = PRE ? pre->c[0] : a.cc[1][0][i];
+    const T p1 = PRE ? pre->psi[0][0] : a.psi[1][1][0][o], p2 = PRE ? pre->psi[0][1] : a.psi[1][2][0][o];
+    d[1] = fmadd(-g, p1, d[1]);
+    d[2] = fmadd(-g, p2, d[2]);
+    (IP ? a.psi : a.psio)[1][1][0][o] = fmadd(b, p1, -(cc * dyx));
+    (IP ? a.psi : a.psio)[1][2][0][o] = fmadd(b, p2, cc * dzx);
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
     const long o = slab_off(a, 1, syi, i, j, k);
-    const T g = a.cg[1][1][j], b = a.cb[1][1][j], cc = a.cc[1][1][j];
-    const T p0 = a.psi[1][0][1][o], p2 = a.psi[1][2][1][o];
-    d[0] -= g * p0;
-    d[2] -= g * p2;
-    (IP ? a.psi : a.psio)[1][0][1][o] = b * p0 + cc * dxy;
-    (IP ? a.psi : a.psio)[1][2][1][o] = b * p2 - cc * dzy;
+    const T g = PRE ? pre->g[1] : a.cg[1][1][j], b = PRE ? pre->b[1] : a.cb[1][1][j],
+            cc = PRE ? pre->c[1] : a.cc[1][1][j];
+    const T p0 = PRE ? pre->psi[1][0] : a.psi[1][0][1][o], p2 = PRE ? pre->psi[1][1] : a.psi[1][2][1][o];
+    d[0] = fmadd(-g, p0, d[0]);
+    d[2] = fmadd(-g, p2, d[2]);
+    (IP ? a.psi : a.psio)[1][0][1][o] = fmadd(b, p0, cc * dxy);
+    (IP ? a.psi : a.psio)[1][2][1][o] = fmadd(b, p2, -(cc * dzy));
   }
   const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
   if (szi >= 0) {
     const long o = slab_off(a, 2, szi, i, j, k);
-    const T g = a.cg[1][2][k], b = a.cb[1][2][k], cc = a.cc[1][2][k];
-    const T p0 = a.psi[1][0][2][o], p1 = a.psi[1][1][2][o];
-    d[0] -= g * p0;
-    d[1] -= g * p1;
-    (IP ? a.psi : a.psio)[1][0][2][o] = b * p0 - cc * dxz;
-    (IP ? a.psi : a.psio)[1][1][2][o] = b * p1 + cc * dyz;
+    const T g = PRE ? pre->g[2] : a.cg[1][2][k], b = PRE ? pre->b[2] : a.cb[1][2][k],
+            cc = PRE ? pre->c[2] : a.cc[1][2][k];
+    const T p0 = PRE ? pre->psi[2][0] : a.psi[1][0][2][o], p1 = PRE ? pre->psi[2][1] : a.psi[1][1][2][o];
+    d[0] = fmadd(-g, p0, d[0]);
+    d[1] = fmadd(-g, p1, d[1]);
+    (IP ? a.psi : a.psio)[1][0][2][o] = fmadd(b, p0, -(cc * dxz));
+    (IP ? a.psi : a.psio)[1][1][2][o] = fmadd(b, p1, cc * dyz);
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
     const bool act[3] = {true, true, true};
-    tfsf_cell<T, 3>(a, incE, 1, i, j, k, act, act, d);
+    const T fn[3] = {fx, fy, fz};
+    tfsf_cell<T, 3>(a, incE, 1, i, j, k, act, act, PRE ? fn : nullptr, d);
   }
-  out[0] = hx + d[0] * im;
-  out[1] = hy + d[1] * im;
-  out[2] = hz + d[2] * im;
+  out[0] = fmadd(d[0], im, hx);
+  out[1] = fmadd(d[1], im, hy);
+  out[2] = fmadd(d[2], im, hz);
 }
 
 // probes of one cell after its step
@@ -1169,183 +1247,6 @@ __global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h
   h_pass_batch_body(args, tiles, kc, s);
 }
 
-// ---------------------------------------------------------------- fused pass
-// Workgroup = 256 x (TY+1) threads.  Thread (tx, ty) holds column
-// i = i0 - 1 + tx and row j = j0 + ty.  Columns i0..i0+253 and rows
-// j0..j0+TY-1 are owned; column i0-1 only supplies H(i-1), column i0+254 and
-// row j0+TY are the halo whose new E the owned cells need, and plane k1 is
-// the halo plane.  Per z plane: load the old H of plane k+1 and publish it in
-// LDS, compute the new E of plane k+1 (halo included), then the new H of
-// plane k from the new E of planes k and k+1 (x and y neighbours from LDS),
-// then publish the new E of plane k+1.  The old state is only read and the
-// new state only written, so halo recomputation never races.
-// old state of one plane at this thread's (i,j), loaded one plane ahead
-template <typename T>
-struct PlaneLoad {
-  T h[3], e[3];
-  T lx, lz;  // Hx, Hz of row j0-1 (first row of threads only)
-  int mid;
-};
-
-template <typename T>
-__device__ __forceinline__ void load_plane(const StepArgs<T>& a, int i, int j, int kk, bool mem, bool lead,
-                                           PlaneLoad<T>& p) {
-  const long c = (long)i + a.sy * (long)j + a.sz * (long)kk;
-  p.h[0] = p.h[1] = p.h[2] = T(0);
-  p.e[0] = p.e[1] = p.e[2] = T(0);
-  p.lx = p.lz = T(0);
-  p.mid = 0;
-  if (mem) {
-    p.h[0] = a.H[0][c];
-    p.h[1] = a.H[1][c];
-    p.h[2] = a.H[2][c];
-    p.e[0] = a.E[0][c];
-    p.e[1] = a.E[1][c];
-    p.e[2] = a.E[2][c];
-    p.mid = a.mat[c];
-  }
-  if (lead) {  // row j0-1 of this column: the y-neighbour of row j0
-    p.lx = a.H[0][c - a.sy];
-    p.lz = a.H[2][c - a.sy];
-  }
-}
-
-// publish the old H of plane kk in LDS, then the new E of (i,j,kk)
-template <typename T, int TY>
-__device__ __forceinline__ void fused_e_plane(const StepArgs<T>& a, const T* tie, long step, const FDTD_G double* incH,
-                                              int i, int j, int tx, int ty, int kk, bool own_e, bool ecell,
-                                              const PlaneLoad<T>& p, T hzx, T hzy, T (&sH)[3][TY + 2][256], T en[3]) {
-  sH[0][ty + 1][tx] = p.h[0];
-  sH[1][ty + 1][tx] = p.h[1];
-  sH[2][ty + 1][tx] = p.h[2];
-  if (ty == 0) {
-    sH[0][0][tx] = p.lx;
-    sH[1][0][tx] = T(0);
-    sH[2][0][tx] = p.lz;
-  }
-  __syncthreads();
-  en[0] = p.e[0];
-  en[1] = p.e[1];
-  en[2] = p.e[2];
-  if (ecell && kk >= 1 && kk <= a.nz - 2) {
-    const long c = (long)i + a.sy * (long)j + a.sz * (long)kk;
-    HStencil<T> h;
-    h.hx = p.h[0];
-    h.hy = p.h[1];
-    h.hz = p.h[2];
-    h.hz_ym = sH[2][ty][tx];
-    h.hx_ym = sH[0][ty][tx];
-    h.hz_xm = sH[2][ty + 1][tx - 1];
-    h.hy_xm = sH[1][ty + 1][tx - 1];
-    h.hy_zm = hzy;
-    h.hx_zm = hzx;
-    T o[3];
-    e_cell_3d(a, step, incH, i, j, kk, c, h, p.e[0], p.e[1], p.e[2], p.mid, cell_inv_eps(a, tie, p.mid, i, j, kk),
-              a.cf[0][0][i], a.cf[0][1][j], a.cf[0][2][kk], own_e, o);
-    en[0] = o[0];
-    en[1] = o[1];
-    en[2] = o[2];
-    if (own_e) {
-      a.Eo[0][c] = o[0];
-      a.Eo[1][c] = o[1];
-      a.Eo[2][c] = o[2];
-    }
-  }
-}
-
-
-template <typename T, int TY>
-__global__ __launch_bounds__(256 * (TY + 1)) void fused_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx,
-                                                            int nty, int nkc) {
-  const StepArgs<T>& a = *ap;
-  __shared__ T sH[3][TY + 2][256];
-  __shared__ T sE[3][TY + 1][256];
-  __shared__ MatLds<T> sm;
-  stage_materials(a, sm);
-  const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
-  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = 1 + bx * kFusedCols;
-  const int i = i0 - 1 + tx;
-  const int j0 = 1 + by * TY;
-  const int j = j0 + ty;
-  const int k0 = 1 + bz * kc;
-  const int k1 = min(k0 + kc, a.nz - 1);
-  const int nx = a.nx, ny = a.ny;
-  const long sy = a.sy, sz = a.sz;
-  const long step = a.step0 + s;
-  const FDTD_G double* incH = a.incH + (long)s * a.auxL;
-  const FDTD_G double* incE = a.incE + (long)s * a.auxL;
-  const bool mem = i < nx && j < ny;
-  const bool ecell = tx >= 1 && i <= nx - 2 && j <= ny - 2;
-  const bool own = ecell && tx <= kFusedCols && ty < TY;
-  const bool lead = ty == 0 && i < nx;
-  const bool extra = a.nprobe > 0;
-
-  T hk[3], ek[3], en[3];
-  int midk = 0;
-  T hzx = T(0), hzy = T(0);
-  if (mem) {  // old Hx, Hy of plane k0-1 (z-neighbour of plane k0)
-    const long c = (long)i + sy * (long)j + sz * (long)(k0 - 1);
-    hzx = a.H[0][c];
-    hzy = a.H[1][c];
-  }
-  PlaneLoad<T> cur, nxt;
-  load_plane(a, i, j, k0, mem, lead, cur);
-  load_plane(a, i, j, k0 + 1, mem, lead, nxt);  // plane k1 <= nz-1 always exists
-  fused_e_plane<T, TY>(a, sm.ie, step, incH, i, j, tx, ty, k0, own, ecell, cur, hzx, hzy, sH, ek);
-  hk[0] = cur.h[0];
-  hk[1] = cur.h[1];
-  hk[2] = cur.h[2];
-  midk = cur.mid;
-  cur = nxt;
-  __syncthreads();
-  sE[0][ty][tx] = ek[0];
-  sE[1][ty][tx] = ek[1];
-  sE[2][ty][tx] = ek[2];
-  for (int k = k0; k < k1; k++) {
-    // issue the loads of plane k+2 before working on planes k+1 and k
-    if (k + 2 <= k1) load_plane(a, i, j, k + 2, mem, lead, nxt);
-    fused_e_plane<T, TY>(a, sm.ie, step, incH, i, j, tx, ty, k + 1, own && k + 1 < k1, ecell, cur, hk[0], hk[1], sH,
-                         en);
-    if (own) {  // new H of plane k
-      const long c = (long)i + sy * (long)j + sz * (long)k;
-      EStencil<T> e;
-      e.ex = ek[0];
-      e.ey = ek[1];
-      e.ez = ek[2];
-      e.ez_yp = sE[2][ty + 1][tx];
-      e.ex_yp = sE[0][ty + 1][tx];
-      e.ey_xp = sE[1][ty][tx + 1];
-      e.ez_xp = sE[2][ty][tx + 1];
-      e.ey_zp = en[1];
-      e.ex_zp = en[0];
-      T o[3];
-      h_cell_3d(a, step, incE, i, j, k, c, e, hk[0], hk[1], hk[2], sm.im[midk], a.cf[1][0][i], a.cf[1][1][j],
-                a.cf[1][2][k], o);
-      a.Ho[0][c] = o[0];
-      a.Ho[1][c] = o[1];
-      a.Ho[2][c] = o[2];
-      if (extra) {
-        const T ev[3] = {e.ex, e.ey, e.ez};
-        dft_probe(a, step, c, i, j, k, ev, o);
-      }
-    }
-    __syncthreads();
-    sE[0][ty][tx] = en[0];
-    sE[1][ty][tx] = en[1];
-    sE[2][ty][tx] = en[2];
-    ek[0] = en[0];
-    ek[1] = en[1];
-    ek[2] = en[2];
-    hk[0] = cur.h[0];
-    hk[1] = cur.h[1];
-    hk[2] = cur.h[2];
-    midk = cur.mid;
-    cur = nxt;
-  }
-}
-
 // ---------------------------------------------------------------- wave-fused pass
 // One pass per step with no LDS and no barrier.  A wavefront owns a
 // 62-column x TY-row x kc-plane brick: lane l holds column i = i0 - 1 + l
@@ -1568,6 +1469,401 @@ __global__ __launch_bounds__(64 * kWavesY) void fusedw_3d(const StepArgs<T>* __r
   }
 }
 
+// ---------------------------------------------------------------- streamed fused pass
+// One pass per step (FP32), every E and H value read once and written once
+// per step.  A tile is an E region of kSCols x kSRows cells (x, y) marched
+// along z; its owned cells are the first kSCols-1 columns and kSRows-1 rows,
+// the last column and row being the +x / +y halo whose new E the owned cells'
+// H update needs.  Per plane k of the march: new E of plane k+1 (halo cells
+// too, from the old H of planes k+1 and k), then new H of plane k from the new
+// E of planes k (LDS: +x, +y neighbours) and k+1 (the thread's registers).
+// Old state is only read and new state only written (ping-pong copies), so
+// halo cells that two tiles recompute never race.  The cell arithmetic is
+// e_cell_3d / h_cell_3d, as in the two-pass schedule, so the fields are
+// bitwise equal.
+//
+// The waves of a workgroup have two roles.  Two producer waves stream the old
+// H of each plane, with the -x / -y halo the E update reads, into a ring of
+// LDS planes by direct buffer-to-LDS loads, kSDepth planes ahead of the plane
+// the consumers need; they issue nothing else, so a wait for "all but the
+// newest n groups" of their loads is exact.  The consumer waves (one cell per
+// thread) gather their own cell's old E, material id and CPML psi a plane
+// ahead into registers, compute, and store without ever waiting for their
+// stores: the barriers between planes wait only for LDS operations and for
+// the producers' loads.  The z coefficients of the grid and the step's
+// incident rows sit in LDS, so no update waits on a scalar load.
+//
+// The kernel is persistent: gridDim.x workgroups split the (tile, plane)
+// units evenly, each taking a contiguous range (a run of planes of one tile,
+// possibly ending one tile and starting the next), so every workgroup does the
+// same work and the launch has no tail.
+constexpr int kSCols = 128;                 // E-region columns (two waves across); 127 owned
+constexpr int kSRows = 5;                   // E-region rows; 4 owned
+constexpr int kSCons = 2 * kSRows;          // consumer waves, one cell per thread
+constexpr int kSProd = 2;                   // producer waves
+constexpr int kSThreads = 64 * (kSCons + kSProd);
+constexpr int kSDepth = 3;                  // planes of old H in flight ahead of the one needed
+constexpr int kSSlots = kSDepth + 2;        // plus the two planes being read
+constexpr int kSMaxAux = 1024;              // incident rows staged in LDS up to this length
+constexpr int kSMaxZ = 1024;                // z coefficients staged in LDS up to this many planes
+
+__host__ __device__ constexpr int pad64(int n) { return (n + 63) / 64 * 64; }
+struct SGeom {
+  static constexpr int nHx = (kSRows + 1) * kSCols;        // rows j0-1 .. j0+R-1, columns i0 .. i0+127
+  static constexpr int nHy = kSRows * (kSCols + 1);        // rows j0 .. j0+R-1, columns i0-1 .. i0+127
+  static constexpr int nHz = (kSRows + 1) * (kSCols + 1);  // rows j0-1 .. j0+R-1, columns i0-1 .. i0+127
+  static constexpr int nId = kSRows * kSCols;              // material ids of the E region
+  static constexpr int qHx = pad64(nHx) / 64, qHy = pad64(nHy) / 64, qHz = pad64(nHz) / 64, qId = pad64(nId) / 64;
+  static constexpr int nInst = qHx + qHy + qHz + qId;  // 64-lane loads per plane
+  static constexpr int perProd = nInst / kSProd;
+  static constexpr int oHy = pad64(nHx), oHz = oHy + pad64(nHy), slot = oHz + pad64(nHz);
+};
+static_assert(SGeom::nInst % kSProd == 0, "producer waves issue equal shares");
+static_assert(SGeom::perProd * (kSDepth - 1) <= 63, "a producer's wait count fits vmcnt");
+static_assert(kSThreads <= 1024, "workgroup size");
+
+struct StreamLds {
+  float h[kSSlots][SGeom::slot];          // ring of old-H planes
+  int id[kSSlots][pad64(SGeom::nId)];     // their material ids (one byte loaded into each int)
+  float en[3][kSRows * kSCols];           // new E of the plane below the one being computed
+  float zc[2][4][kSMaxZ];                 // per field type and plane: dt F/dx, dt G, B, C/dx
+  double inc[2][kSMaxAux];                // the step's incident rows: aux H (E update), aux E (H update)
+  SrcVal<float> src[kMaxSrc];             // the step's soft-source values
+  ProbeDev probe[kMaxProbe];
+  int nsrc;
+  MatLds<float> m;
+};
+
+__device__ __forceinline__ void lds_dma(__amdgpu_buffer_rsrc_t r, void* lds, int size, unsigned vo, unsigned so) {
+  if (size == 4)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 4, (int)vo, (int)so, 0, 0);
+  else
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 1, (int)vo, (int)so, 0, 0);
+}
+// workgroup barriers that wait only for what the wave's role needs (a
+// __syncthreads fence would also wait for every store and load in flight)
+__device__ __forceinline__ void s_bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+template <int N>
+__device__ __forceinline__ void s_bar_vm() { asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(N) : "memory"); }
+
+// the cell data a consumer gathers ahead of its use
+struct SPre {
+  float eo[3];               // old E of the E plane
+  float pe[3][2], ph[3][2];  // CPML psi of the E plane and of the H plane
+};
+
+template <int DISP>
+__global__ __launch_bounds__(kSThreads) void fused3d_lds(const StepArgs<float>* __restrict__ ap, int s, int ntx,
+                                                          long units) {
+  using T = float;
+  using Gm = SGeom;
+  const StepArgs<T>& a = *ap;
+  __shared__ StreamLds L;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, nzi = nz - 2;
+  const long sy = a.sy, sz = a.sz;
+  const long step = a.step0 + s;
+  // stage the material table, the z coefficients, the step's incident rows,
+  // source values and the probe table
+  stage_materials(a, L.m);
+  for (int q = threadIdx.x; q < 8 * nz; q += blockDim.x) {
+    const int ft = q / (4 * nz), w = (q / nz) % 4, kk = q % nz;
+    const FDTD_G T* src = w == 0 ? a.cf[ft][2] : (w == 1 ? a.cg[ft][2] : (w == 2 ? a.cb[ft][2] : a.cc[ft][2]));
+    L.zc[ft][w][kk] = src[kk];
+  }
+  const bool tfsf_lds = a.tfsf && a.auxL <= kSMaxAux;
+  if (tfsf_lds)
+    for (int q = threadIdx.x; q < a.auxL; q += blockDim.x) {
+      L.inc[0][q] = a.incH[(long)s * a.auxL + q];
+      L.inc[1][q] = a.incE[(long)s * a.auxL + q];
+    }
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int q = 0; q < a.nsrc; q++)  // add_sources adds a source while step < its table length
+      if (step < a.src[q].n) L.src[n++] = SrcVal<T>{a.src[q].cell, a.src[q].field, a.src[q].comp, (T)a.src[q].table[step]};
+    L.nsrc = n;
+  }
+  for (int q = threadIdx.x; q < a.nprobe; q += blockDim.x) L.probe[q] = a.probe[q];
+  __syncthreads();
+  const unsigned plane_bytes = (unsigned)(sz * (long)sizeof(T));
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const long u0 = units * (long)blockIdx.x / gridDim.x, u1 = units * ((long)blockIdx.x + 1) / gridDim.x;
+  if (w >= kSCons) {
+    // ------------------------------------------------ producer waves
+    const int pw = w - kSCons;
+    __amdgpu_buffer_rsrc_t rH[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) rH[q] = buf_rsrc(a.H[q], a.field_bytes);
+    const __amdgpu_buffer_rsrc_t rM = buf_rsrc(a.mat, a.mat_bytes);
+    for (long u = u0; u < u1;) {
+      const int t = (int)(u / nzi);
+      const long uend = min(u1, (long)(t + 1) * nzi);
+      const int ka = 1 + (int)(u - (long)t * nzi), kb = 1 + (int)(uend - (long)t * nzi);
+      u = uend;
+      const int i0 = 1 + (t % ntx) * (kSCols - 1), j0 = 1 + (t / ntx) * (kSRows - 1);
+      unsigned vo[Gm::perProd];  // byte offsets (plane 0) of this lane's elements, per load
+#pragma unroll
+      for (int q = 0; q < Gm::perProd; q++) {
+        const int g = pw * Gm::perProd + q;
+        int e, x, y;
+        if (g < Gm::qHx) {
+          e = min(g * 64 + lane, Gm::nHx - 1);
+          x = i0 + e % kSCols;
+          y = j0 - 1 + e / kSCols;
+        } else if (g < Gm::qHx + Gm::qHy) {
+          e = min((g - Gm::qHx) * 64 + lane, Gm::nHy - 1);
+          x = i0 - 1 + e % (kSCols + 1);
+          y = j0 + e / (kSCols + 1);
+        } else if (g < Gm::qHx + Gm::qHy + Gm::qHz) {
+          e = min((g - Gm::qHx - Gm::qHy) * 64 + lane, Gm::nHz - 1);
+          x = i0 - 1 + e % (kSCols + 1);
+          y = j0 - 1 + e / (kSCols + 1);
+        } else {
+          e = min((g - Gm::qHx - Gm::qHy - Gm::qHz) * 64 + lane, Gm::nId - 1);
+          x = i0 + e % kSCols;
+          y = j0 + e / kSCols;
+        }
+        const unsigned cell = (unsigned)((long)min(x, nx - 1) + sy * (long)min(y, ny - 1));
+        vo[q] = g < Gm::qHx + Gm::qHy + Gm::qHz ? cell * (unsigned)sizeof(T) : cell;
+      }
+      auto issue = [&](int k) {  // old H and material ids of plane k into ring slot k % kSSlots
+        const int sl = k % kSSlots;
+#pragma unroll
+        for (int q = 0; q < Gm::perProd; q++) {
+          const int g = pw * Gm::perProd + q;  // uniform per wave, and so is the choice of array
+          if (g < Gm::qHx)
+            lds_dma(rH[0], &L.h[sl][g * 64], 4, vo[q], (unsigned)k * plane_bytes);
+          else if (g < Gm::qHx + Gm::qHy)
+            lds_dma(rH[1], &L.h[sl][Gm::oHy + (g - Gm::qHx) * 64], 4, vo[q], (unsigned)k * plane_bytes);
+          else if (g < Gm::qHx + Gm::qHy + Gm::qHz)
+            lds_dma(rH[2], &L.h[sl][Gm::oHz + (g - Gm::qHx - Gm::qHy) * 64], 4, vo[q], (unsigned)k * plane_bytes);
+          else
+            lds_dma(rM, &L.id[sl][(g - Gm::qHx - Gm::qHy - Gm::qHz) * 64], 1, vo[q], (unsigned)(k * sz));
+        }
+      };
+      // planes ka-1 .. ka+3 (up to kb) fill the ring; the prologue needs ka-1 and ka
+      int issued = ka - 2;  // last plane issued
+      for (int p = ka - 1; p <= min(ka + 3, kb); p++) issue(issued = p);
+      // barrier once every plane <= need has landed (at most the two newest
+      // groups, planes need+1 and need+2, may stay in flight)
+      auto wait_bar = [&](int need) {
+        const int after = issued - need;
+        if (after >= 2)
+          s_bar_vm<Gm::perProd * 2>();
+        else if (after == 1)
+          s_bar_vm<Gm::perProd>();
+        else
+          s_bar_vm<0>();
+      };
+      wait_bar(ka);  // B0
+      for (int k = ka; k < kb; k++) {
+        wait_bar(k + 1);                         // B1: plane k+1 landed (k+2, k+3 in flight)
+        if (k + 4 <= kb) issue(issued = k + 4);  // into the slot of plane k-1, free since B2 of the last plane
+        s_bar_lds();                             // B2
+      }
+      s_bar_vm<0>();  // the range's end
+    }
+    return;
+  }
+  // ------------------------------------------------ consumer waves: one E-region cell per thread
+  const int cx = 64 * (w & 1) + lane, ry = w >> 1;
+  __amdgpu_buffer_rsrc_t rE[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) rE[q] = buf_rsrc(a.E[q], a.field_bytes);
+  const int nprobe = a.nprobe;
+  FDTD_G double* trace = a.trace;
+  const FDTD_G T* dummy = a.mie;  // a valid address for loads whose value is not used
+  for (long u = u0; u < u1;) {
+    const int t = (int)(u / nzi);
+    const long uend = min(u1, (long)(t + 1) * nzi);
+    const int ka = 1 + (int)(u - (long)t * nzi), kb = 1 + (int)(uend - (long)t * nzi);
+    u = uend;
+    const int i0 = 1 + (t % ntx) * (kSCols - 1), j0 = 1 + (t / ntx) * (kSRows - 1);
+    const int xi = i0 + cx, yj = j0 + ry;
+    const int xc = min(xi, nx - 1), yc = min(yj, ny - 1);
+    const bool ecell = xi <= nx - 2 && yj <= ny - 2;
+    const bool own = ecell && cx < kSCols - 1 && ry < kSRows - 1;
+    const unsigned co = (unsigned)((long)xc + sy * (long)yc) * (unsigned)sizeof(T);
+    const int sxi = slab_index(xc, a.slab_a[0], a.slab_b[0]), syi = slab_index(yc, a.slab_a[1], a.slab_b[1]);
+    // per-thread x and y coefficients of both field types
+    const T fx[2] = {a.cf[0][0][xc], a.cf[1][0][xc]}, fy[2] = {a.cf[0][1][yc], a.cf[1][1][yc]};
+    T gx[2], bx[2], cxc[2], gy[2], by[2], cy[2];
+#pragma unroll
+    for (int ft = 0; ft < 2; ft++) {
+      gx[ft] = sxi >= 0 ? a.cg[ft][0][xc] : T(0);
+      bx[ft] = sxi >= 0 ? a.cb[ft][0][xc] : T(0);
+      cxc[ft] = sxi >= 0 ? a.cc[ft][0][xc] : T(0);
+      gy[ft] = syi >= 0 ? a.cg[ft][1][yc] : T(0);
+      by[ft] = syi >= 0 ? a.cb[ft][1][yc] : T(0);
+      cy[ft] = syi >= 0 ? a.cc[ft][1][yc] : T(0);
+    }
+    // a psi value (0 .. 2: slab axis) that a cell may need: loaded whatever
+    // the cell needs, from the cell's slab entry or, where it has none, from a
+    // valid address, so that the gather has no branches (a load inside a
+    // branch makes the compiler wait for every memory operation at the join)
+    auto psi_ld = [&](const FDTD_G T* base, bool on, long o) -> T {
+      const FDTD_G T* p = base ? (on ? base + o : base) : dummy;
+      return *p;
+    };
+    // gather the cell's inputs: old E and E-field psi of plane kE, H-field psi of plane kH
+    auto gather = [&](int kE, int kH, SPre& p) {
+      const unsigned so = (unsigned)kE * plane_bytes;
+#pragma unroll
+      for (int q = 0; q < 3; q++) p.eo[q] = buf_ld<T>(rE[q], co, so);
+#pragma unroll
+      for (int ft = 0; ft < 2; ft++) {
+        const int kk = ft == 0 ? kE : max(kH, 1);
+        float(&c)[3][2] = ft == 0 ? p.pe : p.ph;
+        const bool use = ft == 0 ? ecell : own;
+        const int szi = slab_index(kk, a.slab_a[2], a.slab_b[2]);
+        const long ox = slab_off(a, 0, max(sxi, 0), xc, yc, kk), oy = slab_off(a, 1, max(syi, 0), xc, yc, kk),
+                   oz = slab_off(a, 2, max(szi, 0), xc, yc, kk);
+        c[0][0] = psi_ld(a.psi[ft][1][0], use && sxi >= 0, ox);
+        c[0][1] = psi_ld(a.psi[ft][2][0], use && sxi >= 0, ox);
+        c[1][0] = psi_ld(a.psi[ft][0][1], use && syi >= 0, oy);
+        c[1][1] = psi_ld(a.psi[ft][2][1], use && syi >= 0, oy);
+        c[2][0] = psi_ld(a.psi[ft][0][2], use && szi >= 0, oz);
+        c[2][1] = psi_ld(a.psi[ft][1][2], use && szi >= 0, oz);
+      }
+    };
+    // the update inputs of field type ft at plane kk: the gathered psi, the
+    // thread's x and y coefficients, the plane's z coefficients, the sources
+    auto pre_of = [&](int ft, int kk, const float (&psi)[3][2]) {
+      CellPre<T> c;
+#pragma unroll
+      for (int J = 0; J < 3; J++) {
+        c.psi[J][0] = psi[J][0];
+        c.psi[J][1] = psi[J][1];
+      }
+      c.g[0] = gx[ft];
+      c.b[0] = bx[ft];
+      c.c[0] = cxc[ft];
+      c.g[1] = gy[ft];
+      c.b[1] = by[ft];
+      c.c[1] = cy[ft];
+      c.g[2] = L.zc[ft][1][kk];
+      c.b[2] = L.zc[ft][2][kk];
+      c.c[2] = L.zc[ft][3][kk];
+      c.src = L.src;
+      c.nsrc = L.nsrc;
+      return c;
+    };
+    T en[3];  // new E of the cell's current E plane
+    // new E of plane kk from the gathered inputs p and the plane's LDS data
+    auto e_plane = [&](int kk, bool store, const SPre& p) {
+      en[0] = p.eo[0];
+      en[1] = p.eo[1];
+      en[2] = p.eo[2];
+      if (ecell && kk <= nz - 2) {
+        const float* h1 = L.h[kk % kSSlots];
+        const float* h0 = L.h[(kk - 1) % kSSlots];
+        const int mid = L.id[kk % kSSlots][ry * kSCols + cx];
+        HStencil<T> h;
+        h.hx = h1[(ry + 1) * kSCols + cx];
+        h.hx_ym = h1[ry * kSCols + cx];
+        h.hy = h1[Gm::oHy + ry * (kSCols + 1) + cx + 1];
+        h.hy_xm = h1[Gm::oHy + ry * (kSCols + 1) + cx];
+        h.hz = h1[Gm::oHz + (ry + 1) * (kSCols + 1) + cx + 1];
+        h.hz_ym = h1[Gm::oHz + ry * (kSCols + 1) + cx + 1];
+        h.hz_xm = h1[Gm::oHz + (ry + 1) * (kSCols + 1) + cx];
+        h.hx_zm = h0[(ry + 1) * kSCols + cx];
+        h.hy_zm = h0[Gm::oHy + ry * (kSCols + 1) + cx + 1];
+        const CellPre<T> pre = pre_of(0, kk, p.pe);
+        const long c = (long)xi + sy * (long)yj + sz * (long)kk;
+        const bool w_own = own && store;
+        T o[3];
+        const T ie = L.m.ie[mid];  // grids with a design box run the two-pass schedule
+        if (tfsf_lds)
+          e_cell_3d<T, DISP, false, true, const double*>(a, step, &L.inc[0][0], xi, yj, kk, c, h, p.eo[0], p.eo[1],
+                                                          p.eo[2], mid, ie, fx[0], fy[0], L.zc[0][0][kk], w_own, o,
+                                                          &pre);
+        else
+          e_cell_3d<T, DISP, false, true>(a, step, a.incH + (long)s * a.auxL, xi, yj, kk, c, h, p.eo[0], p.eo[1],
+                                          p.eo[2], mid, ie, fx[0], fy[0], L.zc[0][0][kk], w_own, o, &pre);
+        en[0] = o[0];
+        en[1] = o[1];
+        en[2] = o[2];
+        if (w_own) {
+          a.Eo[0][c] = o[0];
+          a.Eo[1][c] = o[1];
+          a.Eo[2][c] = o[2];
+        }
+      }
+    };
+    auto publish = [&]() {
+      const int x = ry * kSCols + cx;
+      L.en[0][x] = en[0];
+      L.en[1][x] = en[1];
+      L.en[2][x] = en[2];
+    };
+    T ek[3];
+    // plane k of the march: new E of plane k+1 from cur (gathered for planes
+    // k+1 / k), new H of plane k; the inputs of planes k+3 / k+2 are gathered
+    // into nxt, two planes ahead of their use (three sets rotate without copies)
+    auto plane = [&](int k, const SPre& cur, SPre& nxt) {
+      s_bar_lds();  // B1: old H and ids of plane k+1 landed, new E of plane k published
+      if (k + 2 < kb) gather(k + 3, k + 2, nxt);
+      e_plane(k + 1, k + 1 < kb, cur);  // the halo plane when k+1 == kb: computed, not stored
+      if (own) {  // new H of plane k
+        const float* hk = L.h[k % kSSlots];
+        const int midk = L.id[k % kSSlots][ry * kSCols + cx];
+        const long c = (long)xi + sy * (long)yj + sz * (long)k;
+        EStencil<T> e;
+        e.ex = ek[0];
+        e.ey = ek[1];
+        e.ez = ek[2];
+        e.ex_yp = L.en[0][(ry + 1) * kSCols + cx];
+        e.ez_yp = L.en[2][(ry + 1) * kSCols + cx];
+        e.ey_xp = L.en[1][ry * kSCols + cx + 1];
+        e.ez_xp = L.en[2][ry * kSCols + cx + 1];
+        e.ex_zp = en[0];
+        e.ey_zp = en[1];
+        const T hx = hk[(ry + 1) * kSCols + cx];
+        const T hy = hk[Gm::oHy + ry * (kSCols + 1) + cx + 1];
+        const T hz = hk[Gm::oHz + (ry + 1) * (kSCols + 1) + cx + 1];
+        const CellPre<T> pre = pre_of(1, k, cur.ph);
+        T o[3];
+        if (tfsf_lds)
+          h_cell_3d<T, false, true, const double*>(a, step, &L.inc[1][0], xi, yj, k, c, e, hx, hy, hz,
+                                                   L.m.im[midk], fx[1], fy[1], L.zc[1][0][k], o, &pre);
+        else
+          h_cell_3d<T, false, true>(a, step, a.incE + (long)s * a.auxL, xi, yj, k, c, e, hx, hy, hz, L.m.im[midk],
+                                    fx[1], fy[1], L.zc[1][0][k], o, &pre);
+        a.Ho[0][c] = o[0];
+        a.Ho[1][c] = o[1];
+        a.Ho[2][c] = o[2];
+        for (int q = 0; q < nprobe; q++)  // the probes of the cell (dft_probe's values, table in LDS)
+          if (L.probe[q].cell == c) {
+            const int comp = L.probe[q].comp;
+            const T v = L.probe[q].field == 1 ? pick3(e.ex, e.ey, e.ez, comp) : pick3(o[0], o[1], o[2], comp);
+            trace[step * nprobe + q] = (double)v;
+          }
+      }
+      s_bar_lds();  // B2: every read of the published plane done
+      ek[0] = en[0];
+      ek[1] = en[1];
+      ek[2] = en[2];
+      publish();
+    };
+    SPre P0, A, B, C;
+    gather(ka, 0, P0);                           // the prologue's E plane
+    gather(ka + 1, ka, A);                       // plane ka of the march
+    if (ka + 1 < kb) gather(ka + 2, ka + 1, B);  // plane ka+1
+    s_bar_lds();  // B0 (the producers wait for planes ka-1 and ka)
+    e_plane(ka, true, P0);
+    ek[0] = en[0];
+    ek[1] = en[1];
+    ek[2] = en[2];
+    publish();
+    for (int k = ka; k < kb; k += 3) {
+      plane(k, A, C);
+      if (k + 1 < kb) plane(k + 1, B, A);
+      if (k + 2 < kb) plane(k + 2, C, B);
+    }
+    s_bar_lds();  // the range's end
+  }
+}
+
 // ---------------------------------------------------------------- 2-D TM
 // Ez, Hx, Hy (include/YeeFields.hpp:123-148); CPML along x and y; in place.
 template <typename T, bool BC = false>
@@ -1605,7 +1901,7 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
-    tfsf_cell<T, 2>(a, a.incH + (long)s * a.auxL, 0, i, j, 0, aF, aC, d);
+    tfsf_cell<T, 2>(a, a.incH + (long)s * a.auxL, 0, i, j, 0, aF, aC, (const T*)nullptr, d);
   }
   T ezn = ez;
   if (ie > T(0)) {
@@ -1659,7 +1955,7 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {true, true, false}, aC[3] = {false, false, true};
-    tfsf_cell<T, 2>(a, a.incE + (long)s * a.auxL, 1, i, j, 0, aF, aC, d);
+    tfsf_cell<T, 2>(a, a.incE + (long)s * a.auxL, 1, i, j, 0, aF, aC, (const T*)nullptr, d);
   }
   T h[3] = {hx + d[0] * im, hy + d[1] * im, T(0)};
   a.Ho[0][c] = h[0];

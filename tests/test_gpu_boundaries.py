@@ -64,7 +64,7 @@ def test_boundaries_schedule_invariance(name):
     sc = SCEN[name]()
     ref = apply(_grid(sc, "f32"), sc)
     ref.step(sc["nsteps"])
-    for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 2, 16), False), ((128, 2, 1), True)):
+    for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 5, 2), False), ((128, 2, 1), True)):
         print(f"[gpu-case] boundaries {name} tile {tile} graph {graph}", flush=True)
         g = apply(_grid(sc, "f32"), sc)
         g.set_tile(*tile)
@@ -142,7 +142,7 @@ def test_bloch_schedule_invariance_and_errors():
     sc, _ = _bloch_scenario("grid3d_bloch")
     ref = apply(_grid(sc, "f32"), sc)
     ref.step(sc["nsteps"])
-    for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 2, 16), False), ((128, 2, 1), True)):
+    for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 5, 2), False), ((128, 2, 1), True)):
         print(f"[gpu-case] bloch tile {tile} graph {graph}", flush=True)
         g = apply(_grid(sc, "f32"), sc)
         g.set_tile(*tile)

@@ -100,7 +100,7 @@ def test_ragged_grid_every_schedule(precision):
     n, dx, setup = _ragged_setup(dt, nsteps)
     ref = None
     o = None
-    for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 3, 16), False), ((1, 6, 8), False),
+    for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 5, 1), False), ((1, 6, 8), False),
                         ((128, 2, 4), True)):
         print(f"[gpu-case] ragged {precision} tile {tile} graph {graph}", flush=True)
         g, oo = _pair("ThreeD", n, dx, dt, precision, setup)

@@ -173,9 +173,11 @@ def test_autotune_as_first_call():
         np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=nm)
 
 
-# (0, TY, kc): LDS fused pass; (1, TY, kc): wave-fused pass (ny=23 gives a partly
-# empty last wave row block, nx=300 five 62-column wave tiles)
-FUSED = [(0, 1, 8), (0, 2, 16), (0, 3, 5), (1, 2, 7), (1, 4, 16), (1, 6, 5)]
+# (0, 5, kc): streamed fused pass (128 x 5 E regions, kc persistent workgroups
+# per compute unit; FP64 grids run the two-pass default); (1, TY, kc):
+# wave-fused pass (ny=23 gives a partly empty last wave row block, nx=300 five
+# 62-column wave tiles and three streamed tiles, the last 44 columns wide)
+FUSED = [(0, 5, 1), (0, 5, 2), (1, 2, 7), (1, 4, 16), (1, 6, 5)]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -205,12 +207,40 @@ def test_fused_pass_equals_two_pass(precision):
         g.close()
 
 
+@pytest.mark.parametrize("kc", [1, 2])
+def test_streamed_pass_runs_cross_tiles(kc):
+    """Streamed fused pass (FP32) on grids whose (tile, plane) units give each persistent workgroup a run of
+    several planes that ends in one tile and continues in the next: bitwise equal to the two passes, on a CPML +
+    blocks + TFSF + DFT + probes grid and on a dispersive sphere with a dipole source."""
+    for sc in (synth.grid3d_cpml_tfsf_dft(n=(260, 40, 70), nsteps=30, npml=6, inset=10),
+               synth.grid3d_dispersive_sphere(n=(150, 44, 60), radius=12, nsteps=30)):
+        print(f"[gpu-case] streamed runs kc={kc} {sc['name']}", flush=True)
+        ref = apply(_grid(sc, "f32"), sc)
+        ref.set_tile(128, 2, 2)
+        ref.step(30)
+        g = apply(_grid(sc, "f32"), sc)
+        g.set_tile(0, 5, kc)
+        g.step(30)
+        for nm, f, c in _comps(sc):
+            np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=f"{kc} {nm}")
+        np.testing.assert_array_equal(g.probes(), ref.probes())
+        for idx in range(len(sc["dfts"])):
+            np.testing.assert_array_equal(g.get_dft(idx), ref.get_dft(idx))
+        if sc["name"].startswith("grid3d_dispersive"):
+            for p in range(3):
+                for c in range(3):
+                    np.testing.assert_array_equal(g.get(P, c, p), ref.get(P, c, p))
+                    np.testing.assert_array_equal(g.get(J, c, p), ref.get(J, c, p))
+        g.close()
+        ref.close()
+
+
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_fused_pass_against_oracle(precision):
     """Fused schedule on the dispersive sphere and on a CPML+TFSF+DFT grid,
     against the FP64 oracle."""
-    for sc, tile in ((synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (0, 3, 8)),
-                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (0, 3, 8)),
+    for sc, tile in ((synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (0, 5, 1)),
+                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (0, 5, 2)),
                      (synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (1, 4, 8)),
                      (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (1, 2, 16))):
         print(f"[gpu-case] fused-vs-oracle {sc['name']} {tile} {precision}", flush=True)
@@ -333,7 +363,7 @@ def test_3d_magnetized_schedules_and_errors():
         return g
 
     ref = run(None, False)
-    for tile, graph in (((0, 2, 16), False), ((1, 4, 2), False), ((64, 4, 2), False), ((256, 1, 4), True)):
+    for tile, graph in (((0, 5, 2), False), ((1, 4, 2), False), ((64, 4, 2), False), ((256, 1, 4), True)):
         g = run(tile, graph)
         for f in (E, H, P, J):
             for c in range(3):
