@@ -160,6 +160,16 @@ struct MatHost {
   int fluid = 0;
 };
 
+// One Aux1D incident line on the device (include/Source.hpp:280-458): its
+// state, CPML rows and source signal, and the rows it writes for the current
+// chunk (Ez after each step's advance, Hy before it).
+struct AuxLine {
+  int L = 0;
+  double *state = nullptr, *pE = nullptr, *pH = nullptr, *sig = nullptr;
+  int nsig = 0;
+  double *incE = nullptr, *incH = nullptr;
+};
+
 struct fdtd_grid {
   fdtd_grid_desc d{};
   int ndim = 3;
@@ -207,18 +217,20 @@ struct fdtd_grid {
   void* Pd2[FDTD_MAX_POLES][3] = {};
   void* Jd2[FDTD_MAX_POLES][3] = {};
 
-  // sources
-  int nsrc = 0;
+  // sources (nsrc_eh of them on E or H)
+  int nsrc = 0, nsrc_eh = 0;
   fdtdk::SrcDev src[FDTD_MAX_SOURCES];
 
-  // TFSF
+  // TFSF box
   bool tfsf = false;
   int box[6] = {};
   double kvec[3] = {}, pol[3] = {}, polH[3] = {}, origin[3] = {};
-  int auxL = 0;
-  double *aux_state = nullptr, *aux_pE = nullptr, *aux_pH = nullptr, *aux_sig = nullptr;
-  int nsig = 0;
-  double *incE = nullptr, *incH = nullptr;
+  AuxLine aux;
+
+  // single TFSF planes, each with its own line (dev: all but the row pointers)
+  int nplane = 0;
+  fdtdk::PlaneDev plane[FDTD_MAX_PLANES];
+  AuxLine plane_aux[FDTD_MAX_PLANES];
 
   // DFT
   int ndft = 0;
@@ -741,7 +753,9 @@ void im_args(const fdtd_grid* g, const fdtdk::StepArgs<T>& re, fdtdk::StepArgs<T
         gput(a.psio[ft][I][J], (T*)g->psi2[ft][I][J]);
       }
   a.nsrc = 0;
+  a.nsrc_eh = 0;
   a.tfsf = 0;
+  a.nplane = 0;
   for (int q = 0; q < g->ndft; q++) {
     gput(a.dft[q].acc, (double*)g->dft[q].acc_im);
     gput(a.dft[q].smp, (T*)g->dft[q].smp_im);
@@ -803,6 +817,7 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   gput(a.fl[1], (const T*)g->fl[1]);
   a.dt = T(g->d.dt);
   a.nsrc = g->nsrc;
+  a.nsrc_eh = g->nsrc_eh;
   for (int s = 0; s < g->nsrc; s++) a.src[s] = g->src[s];
   a.tfsf = g->tfsf;
   for (int x = 0; x < 6; x++) a.box[x] = g->box[x];
@@ -814,7 +829,13 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
     a.origin[x] = g->origin[x];
   }
   a.dx = g->d.dx;
-  a.auxL = g->auxL;
+  a.auxL = g->aux.L;
+  a.nplane = g->nplane;
+  for (int p = 0; p < g->nplane; p++) {
+    a.plane[p] = g->plane[p];
+    gput(a.plane[p].incE, g->plane_aux[p].incE);
+    gput(a.plane[p].incH, g->plane_aux[p].incH);
+  }
   a.ndft = g->ndft;
   for (int q = 0; q < g->ndft; q++) {
     const auto& h = g->dft[q];
@@ -936,31 +957,40 @@ int ensure_dft_im(fdtd_grid* g) {
   return 0;
 }
 
-int run_aux(fdtd_grid* g, long step0, int nsteps) {
+// advance one incident line nsteps from step0, its rows into outE / outH
+int run_aux_line(fdtd_grid* g, const AuxLine& l, double* outE, double* outH, long step0, int nsteps) {
   fdtdk::AuxArgs a;
-  a.L = g->auxL;
+  a.L = l.L;
   a.npml = 15;
   a.dt = g->d.dt;
   a.dx = g->d.dx;
   a.eps0 = kEps0;
   a.mu0 = kMu0;
-  double* s = g->aux_state;
+  double* s = l.state;
   a.Dz = s;
-  a.Ez = s + g->auxL;
-  a.By = s + 2 * g->auxL;
-  a.Hy = s + 3 * g->auxL;
-  a.psiE = s + 4 * g->auxL;
-  a.psiH = s + 5 * g->auxL;
-  a.pE = g->aux_pE;
-  a.pH = g->aux_pH;
-  a.signal = g->aux_sig;
-  a.nsignal = g->nsig;
+  a.Ez = s + l.L;
+  a.By = s + 2 * l.L;
+  a.Hy = s + 3 * l.L;
+  a.psiE = s + 4 * l.L;
+  a.psiH = s + 5 * l.L;
+  a.pE = l.pE;
+  a.pH = l.pH;
+  a.signal = l.sig;
+  a.nsignal = l.nsig;
   a.step0 = step0;
   a.nsteps = nsteps;
-  a.outE = g->incE;
-  a.outH = g->incH;
-  int threads = std::min(1024, ((g->auxL + 63) / 64) * 64);
+  a.outE = outE;
+  a.outH = outH;
+  int threads = std::min(1024, ((l.L + 63) / 64) * 64);
   return launch(g, 2, fdtdk::aux_line, dim3(1), dim3(threads), a);
+}
+
+// the chunk's rows of every incident line of the grid (the box's and the planes')
+int run_aux(fdtd_grid* g, long step0, int nsteps) {
+  int rc = g->tfsf ? run_aux_line(g, g->aux, g->aux.incE, g->aux.incH, step0, nsteps) : 0;
+  for (int p = 0; p < g->nplane && !rc; p++)
+    rc = run_aux_line(g, g->plane_aux[p], g->plane_aux[p].incE, g->plane_aux[p].incH, step0, nsteps);
+  return rc;
 }
 
 template <typename T>
@@ -989,7 +1019,9 @@ int step_impl(fdtd_grid* g, int nsteps) {
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   if (g->ndim == 2) t.kc = 1;
   // design increments and boundary ghost writes are made by the two-pass kernels
-  if ((g->force_two_pass || g->nbc || has_magnetized(g)) && t.bx <= 1) t = kDefaultTile3D;
+  // (E / H sources and single planes too: the general instance of the passes)
+  const bool general = g->nbc || g->nsrc_eh || g->nplane;
+  if ((g->force_two_pass || general || has_magnetized(g)) && t.bx <= 1) t = kDefaultTile3D;
   // the streamed pass is FP32, addresses fields with 32-bit buffer offsets and
   // keeps the z coefficients in LDS
   if (t.bx == 0 && (sizeof(T) != 4 || a.field_bytes == 0 || g->d.nz > fdtdk::kSMaxZ || g->design)) t = kDefaultTile3D;
@@ -1035,14 +1067,14 @@ int step_impl(fdtd_grid* g, int nsteps) {
   while (done < nsteps) {
     const int chunk = std::min(kChunk, nsteps - done);
     a.step0 = g->step;
-    gput(a.incE, g->incE);
-    gput(a.incH, g->incH);
+    gput(a.incE, g->aux.incE);
+    gput(a.incH, g->aux.incH);
     if (g->tfsf && g->inc_rows) {
       // incident rows computed ahead for every step (adjoint forward and recompute)
       if (g->step + chunk > g->inc_rows) return fail(FDTD_ESTATE, "step beyond the precomputed incident rows");
-      gput(a.incE, g->inc_allE + g->step * g->auxL);
-      gput(a.incH, g->inc_allH + g->step * g->auxL);
-    } else if (g->tfsf) {
+      gput(a.incE, g->inc_allE + g->step * g->aux.L);
+      gput(a.incH, g->inc_allH + g->step * g->aux.L);
+    } else if (g->tfsf || g->nplane) {
       int rc = run_aux(g, g->step, chunk);
       if (rc) return rc;
     }
@@ -1134,15 +1166,15 @@ int step_impl(fdtd_grid* g, int nsteps) {
           if (rc) return rc;
           rc = launch(g, 1, fdtdk::h_pass_batch<T>, dim3(unsigned(cnt)), block, dev0, g->cone_tab + o0, t.kc, s);
         } else if (g->ndim == 3) {
-          // instances that also write the boundary ghost layers (BC)
-          // and the E-pass instance without the ADE code when no material is
+          // the general instances (boundary ghost writes, E / H sources,
+          // single TFSF planes) and the E-pass instance without the ADE code when no material is
           // dispersive (with the magnetized Drude code only when one is magnetized)
           const int disp = g->max_np > 0 ? (has_magnetized(g) ? 2 : 1) : 0;
-          if (g->nbc && disp == 2)
+          if (general && disp == 2)
             rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 2>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-          else if (g->nbc && disp)
+          else if (general && disp)
             rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 1>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-          else if (g->nbc)
+          else if (general)
             rc = launch(g, 0, fdtdk::e_pass_3d<T, true, 0>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           else if (disp == 2)
             rc = launch(g, 0, fdtdk::e_pass_3d<T, false, 2>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
@@ -1151,11 +1183,11 @@ int step_impl(fdtd_grid* g, int nsteps) {
           else
             rc = launch(g, 0, fdtdk::e_pass_3d<T, false, 0>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           if (rc) return rc;
-          if (g->nbc)
+          if (general)
             rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           else
             rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
-        } else if (g->nbc) {
+        } else if (general) {
           rc = launch(g, 0, fdtdk::e_pass_tm<T, true>, grid, block, dev0, 1, s);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T, true>, grid, block, dev0, 1, s);
         } else {
@@ -1394,15 +1426,9 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   int rc = 0;
   g->force_two_pass = true;
   if (g->tfsf) {
-    if ((rc = sc.get((void**)&g->inc_allE, size_t(nsteps) * g->auxL * 8))) return rc;
-    if ((rc = sc.get((void**)&g->inc_allH, size_t(nsteps) * g->auxL * 8))) return rc;
-    double *e0 = g->incE, *h0 = g->incH;
-    g->incE = g->inc_allE;
-    g->incH = g->inc_allH;
-    rc = run_aux(g, 0, nsteps);
-    g->incE = e0;
-    g->incH = h0;
-    if (rc) return rc;
+    if ((rc = sc.get((void**)&g->inc_allE, size_t(nsteps) * g->aux.L * 8))) return rc;
+    if ((rc = sc.get((void**)&g->inc_allH, size_t(nsteps) * g->aux.L * 8))) return rc;
+    if ((rc = run_aux_line(g, g->aux, g->inc_allE, g->inc_allH, 0, nsteps))) return rc;
     g->inc_rows = nsteps;
   }
   const int S = (nsteps + K - 1) / K;
@@ -1880,7 +1906,7 @@ int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids) {
 int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, const double* table, int ntable) {
   if (!g || !table || ntable < 0) return fail(FDTD_EINVAL, "null argument");
   if (g->nsrc >= FDTD_MAX_SOURCES) return fail(FDTD_EINVAL, "too many sources (max %d)", FDTD_MAX_SOURCES);
-  if ((field != FDTD_D && field != FDTD_B) || !valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "bad source field/component");
+  if (field < FDTD_D || field > FDTD_H || !valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "bad source field/component");
   if (i < 1 || j < 1 || i >= g->d.nx - 1 || j >= g->d.ny - 1 || (g->ndim == 3 && (k < 1 || k >= g->d.nz - 1)) ||
       (g->ndim == 2 && k != 0))
     return fail(FDTD_EINVAL, "source cell must be an interior cell");
@@ -1894,6 +1920,30 @@ int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, cons
   s.comp = comp;
   gput(s.table, t);
   s.n = ntable;
+  if (field == FDTD_E || field == FDTD_H) g->nsrc_eh++;
+  return 0;
+}
+
+// Allocate and set up one incident line of aux_ncells cells (Aux1D's length
+// ncells + 2*15 + 20, Source.hpp:401-408) driven by signal[n] at its n-th step.
+int aux_line_init(fdtd_grid* g, int aux_ncells, const double* signal, int nsignal, AuxLine& l) {
+  l.L = aux_ncells + 2 * 15 + 20;
+  const int L = l.L;
+  int rc = g->alloc((void**)&l.state, size_t(6) * L * 8);
+  if (!rc) rc = g->alloc((void**)&l.pE, size_t(4) * L * 8);
+  if (!rc) rc = g->alloc((void**)&l.pH, size_t(4) * L * 8);
+  if (!rc) rc = g->alloc((void**)&l.sig, size_t(std::max(nsignal, 1)) * 8);
+  if (!rc) rc = g->alloc((void**)&l.incE, size_t(kChunk) * L * 8);
+  if (!rc) rc = g->alloc((void**)&l.incH, size_t(kChunk) * L * 8);
+  if (rc) return rc;
+  // Aux1D's own CPML: PMLParameterModel(dx) defaults, 15 layers (Source.hpp:362, 380-389)
+  std::vector<double> e, h;
+  const double m = 3;
+  pml_idiom_rows(L, 15, m, 1, 0.8 * (m + 1) / (kImp0 * g->d.dx), 5, 0.05, g->d.dt, e, h);
+  HIP_TRY(hipMemcpy(l.pE, e.data(), e.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(l.pH, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  if (nsignal) HIP_TRY(hipMemcpy(l.sig, signal, size_t(nsignal) * 8, hipMemcpyHostToDevice));
+  l.nsig = nsignal;
   return 0;
 }
 
@@ -1918,23 +1968,50 @@ int fdtd_set_tfsf(fdtd_grid* g, const int box[6], const double kvec[3], const do
     g->polH[d] = polH[d];
     g->origin[d] = origin[d];
   }
-  g->auxL = aux_ncells + 2 * 15 + 20;
-  const int L = g->auxL;
-  int rc = g->alloc((void**)&g->aux_state, size_t(6) * L * 8);
-  if (!rc) rc = g->alloc((void**)&g->aux_pE, size_t(4) * L * 8);
-  if (!rc) rc = g->alloc((void**)&g->aux_pH, size_t(4) * L * 8);
-  if (!rc) rc = g->alloc((void**)&g->aux_sig, size_t(std::max(nsignal, 1)) * 8);
-  if (!rc) rc = g->alloc((void**)&g->incE, size_t(kChunk) * L * 8);
-  if (!rc) rc = g->alloc((void**)&g->incH, size_t(kChunk) * L * 8);
+  return aux_line_init(g, aux_ncells, signal, nsignal, g->aux);
+}
+
+int fdtd_add_tfsf_plane(fdtd_grid* g, int axis, int side, const int cells[6], const double kvec[3],
+                        const double pol[3], const double polH[3], const double origin[3], int aux_ncells,
+                        const double* signal, int nsignal) {
+  if (!g || !cells || !kvec || !pol || !polH || !origin || (!signal && nsignal)) return fail(FDTD_EINVAL, "null argument");
+  if (g->nplane >= FDTD_MAX_PLANES) return fail(FDTD_EINVAL, "too many TFSF planes (max %d)", FDTD_MAX_PLANES);
+  if (axis < 0 || axis >= g->ndim || (side != FDTD_MIN && side != FDTD_MAX))
+    return fail(FDTD_EINVAL, "bad plane axis or side");
+  int c[6];
+  for (int x = 0; x < 6; x++) c[x] = cells[x];
+  if (g->ndim == 2) {
+    c[4] = 0;
+    c[5] = 1;
+  }
+  for (int a = 0; a < g->ndim; a++)
+    if (c[2 * a] < 1 || c[2 * a + 1] > g->len[a] - 1 || c[2 * a] >= c[2 * a + 1])
+      return fail(FDTD_EINVAL, "TFSF plane cells must be interior cells with lo < hi");
+  if (c[2 * axis + 1] != c[2 * axis] + 1) return fail(FDTD_EINVAL, "a TFSF plane is one cell layer thick along its axis");
+  if (aux_ncells < 2 || (aux_ncells + 50) > 8192) return fail(FDTD_EINVAL, "bad aux line length");
+  double K = 0;
+  for (int d = 0; d < g->ndim; d++) K += kvec[d] * kvec[d];
+  K = std::sqrt(K);
+  if (!(K > 0)) return fail(FDTD_EINVAL, "zero wave vector");
+  fdtdk::PlaneDev P{};
+  P.axis = axis;
+  P.nrm = side == FDTD_MAX ? 1.0 : -1.0;
+  for (int d = 0; d < 3; d++) {
+    P.lo[d] = c[2 * d];
+    P.hi[d] = c[2 * d + 1] - 1;
+    // TFSFPlane normalises the wave vector (include/Source.hpp:566-570)
+    P.kvec[d] = d < g->ndim ? kvec[d] / K : 0.0;
+    P.pol[d] = pol[d];
+    P.polH[d] = polH[d];
+    P.origin[d] = origin[d];
+  }
+  AuxLine l;
+  int rc = aux_line_init(g, aux_ncells, signal, nsignal, l);
   if (rc) return rc;
-  // Aux1D's own CPML: PMLParameterModel(dx) defaults, 15 layers (Source.hpp:362, 380-389)
-  std::vector<double> e, h;
-  const double m = 3;
-  pml_idiom_rows(L, 15, m, 1, 0.8 * (m + 1) / (kImp0 * g->d.dx), 5, 0.05, g->d.dt, e, h);
-  HIP_TRY(hipMemcpy(g->aux_pE, e.data(), e.size() * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(g->aux_pH, h.data(), h.size() * 8, hipMemcpyHostToDevice));
-  if (nsignal) HIP_TRY(hipMemcpy(g->aux_sig, signal, size_t(nsignal) * 8, hipMemcpyHostToDevice));
-  g->nsig = nsignal;
+  P.auxL = l.L;
+  g->plane[g->nplane] = P;
+  g->plane_aux[g->nplane] = l;
+  g->nplane++;
   return 0;
 }
 
@@ -2354,6 +2431,8 @@ int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, do
   if (!g->finalized && (rc = finalize(g))) return rc;
   if (g->max_np > 0) return fail(FDTD_EINVAL, "the adjoint gradient does not cover dispersive materials");
   if (g->nbc) return fail(FDTD_EINVAL, "the adjoint gradient does not cover boundary conditions");
+  if (g->nsrc_eh || g->nplane)
+    return fail(FDTD_EINVAL, "the adjoint gradient covers D / B sources and the TFSF box, not E / H sources or single planes");
   return g->f64 ? adjoint_impl<double>(g, nsteps, checkpoint_every, history_mb, dft, comp_mask, objective, grad)
                 : adjoint_impl<float>(g, nsteps, checkpoint_every, history_mb, dft, comp_mask, objective, grad);
 }
@@ -2490,6 +2569,8 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
   for (fdtd_grid* g : b->grids) {
     int rc = 0;
     if (has_magnetized(g)) return fail(FDTD_EINVAL, "batched grids cannot hold magnetized Drude materials (step them alone)");
+    if (g->nbc || g->nsrc_eh || g->nplane)
+      return fail(FDTD_EINVAL, "batched grids cannot have boundary conditions, E / H sources or single TFSF planes");
     if (g->mats_dirty && (rc = upload_materials<T>(g))) return rc;
     if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
     if ((rc = ensure_trace(g, g->step + nsteps))) return rc;
@@ -2498,7 +2579,7 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
   while (done < nsteps) {
     const int chunk = std::min(kChunk, nsteps - done);
     for (fdtd_grid* g : b->grids)
-      if (g->tfsf) {
+      if (g->tfsf || g->nplane) {
         int rc = run_aux(g, g->step, chunk);
         if (rc) return rc;
       }
@@ -2511,8 +2592,8 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
       fdtdk::StepArgs<T> a;
       fill_args<T>(g, a);
       a.step0 = g->step;
-      gput(a.incE, g->incE);
-      gput(a.incH, g->incH);
+      gput(a.incE, g->aux.incE);
+      gput(a.incH, g->aux.incH);
       gput(a.self, reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + (slot * n + q) * sizeof(fdtdk::StepArgs<T>)));
       host[q] = a;
     }
@@ -2582,6 +2663,8 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
     if (grids[q]->ndim != 3) return fail(FDTD_EINVAL, "batches hold 3-D grids");
     if (grids[q]->f64 != grids[0]->f64) return fail(FDTD_EINVAL, "batched grids must share one precision");
     if (grids[q]->nbc) return fail(FDTD_EINVAL, "batched grids cannot have boundary conditions (step them alone)");
+    if (grids[q]->nsrc_eh || grids[q]->nplane)
+      return fail(FDTD_EINVAL, "batched grids cannot have E / H sources or single TFSF planes (step them alone)");
     if (has_magnetized(grids[q]))
       return fail(FDTD_EINVAL, "batched grids cannot hold magnetized Drude materials (step them alone)");
     for (int r = 0; r < q; r++)

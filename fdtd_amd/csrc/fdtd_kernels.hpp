@@ -111,6 +111,20 @@ struct ProbeDev {
   int field, comp;
 };
 
+// One single TFSF plane (make_tfsf_plane, include/Source.hpp:624-634): the
+// cells [lo, hi] (inclusive; lo[axis] == hi[axis] is the plane's layer), the
+// outward normal nrm along `axis`, and its own incident line.
+constexpr int kMaxPlane = 6;
+struct PlaneDev {
+  int axis;
+  int lo[3], hi[3];
+  double nrm;
+  double kvec[3], pol[3], polH[3], origin[3];
+  int auxL;
+  const FDTD_G double* incE;  // the line's Ez after each step's advance, rows of the chunk
+  const FDTD_G double* incH;  // its Hy before the advance
+};
+
 template <typename T>
 struct StepArgs {
   FDTD_G T* E[3];   // state read by the step
@@ -184,6 +198,12 @@ struct StepArgs {
   FDTD_G T* hist;
   long hist_step0, hist_rows;
   unsigned field_bytes, mat_bytes;  // padded array sizes for buffer addressing (0: too large)
+  // read by the general instances only (kept last, so the offsets of the
+  // fields above stay small): soft sources on E or H among src[], single
+  // TFSF planes
+  int nsrc_eh;
+  int nplane;
+  PlaneDev plane[kMaxPlane];
 };
 
 // Extra elements allocated after every field array and the id volume: the
@@ -320,6 +340,52 @@ __device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a0, Row row, int ft, int 
   }
 }
 
+// Single TFSF planes through cell (i,j,k) (TFSFPlane::updateD / updateB,
+// include/Source.hpp:595-620): every plane holding the cell adds, for each
+// flux I other than its normal axis J, eps_IJK (dt F_J/dx) n_J inc_K pol_K
+// to the cell's delta (with the curl's sign: + for D, - for B), inc_K the
+// plane's own line at k.(r + off_K dx), r = (cell - origin) dx.  ft 0: D from
+// the line's H with polH; 1: B from its E with pol.  actF / actC: the mode's
+// flux and curl components (bit per axis).
+template <typename T, int NDIM>
+__device__ FDTD_COLD void plane_cells(const StepArgs<T>& a0, long step, int ft, int i, int j, int k, int actF,
+                                      int actC, T delta[3]) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = cold_args(a0);
+  const long s = step - a.step0;
+  for (int p = 0; p < a.nplane; p++) {
+    const PlaneDev& P = a.plane[p];
+    if (i < P.lo[0] || i > P.hi[0] || j < P.lo[1] || j > P.hi[1] || k < P.lo[2] || k > P.hi[2]) continue;
+    const int J = P.axis;
+    const FDTD_G double* row = (ft == 0 ? P.incH : P.incE) + s * P.auxL;
+    const double f = (double)a.cf[ft][J][J == 0 ? i : (J == 1 ? j : k)];
+#pragma unroll
+    for (int I = 0; I < 3; I++) {
+      if (I == J || !((actF >> I) & 1)) continue;
+      const int K = 3 - I - J;
+      if (!((actC >> K) & 1)) continue;
+      double proj = 0.0;
+#pragma unroll
+      for (int dd = 0; dd < NDIM; dd++) {
+        const int cd = dd == 0 ? i : (dd == 1 ? j : k);
+        const double off = ft == 0 ? (dd == K ? -0.5 : 0.0) : (dd != K ? -0.5 : 0.0);
+        const double loc = fmadd(off, a.dx, ((double)cd - P.origin[dd]) * a.dx);
+        proj = fmadd(P.kvec[dd], loc, proj);
+      }
+      const double inc = interp_row(row, P.auxL, proj, a.dx, ft == 0);
+      const double pv = ft == 0 ? P.polH[K] : P.pol[K];
+      const double term = lc3(I, J) * f * P.nrm * inc * pv;
+      const T v = (T)(ft == 0 ? term : -term);
+      if (I == 0)
+        delta[0] += v;
+      else if (I == 1)
+        delta[1] += v;
+      else
+        delta[2] += v;
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T pick3(T v0, T v1, T v2, int c) {
   return c == 0 ? v0 : (c == 1 ? v1 : v2);
@@ -338,6 +404,32 @@ __device__ FDTD_COLD void add_sources(const StepArgs<T>& a0, long step, int fiel
         d[1] += v;
       else
         d[2] += v;
+    }
+  }
+}
+
+// Soft sources on the derived fields, E (field 1) or H (field 3): SoftSource
+// on E / H, which the reference's loops apply after the material step
+// (include/Source.hpp:209-229; Aux1D places its own so, 421-425), so that the
+// next material step, recomputing the field from D or B, drops the value
+// again.  E and H are stored here without D and B (outside dispersive
+// cells), so the stored value carries the last step's source value: with
+// `back` the update first takes table[step - 1] off the cell's old value;
+// without, it adds table[step] to the new one.
+template <typename T>
+__device__ FDTD_COLD void soft_eh(const StepArgs<T>& a0, long step, int field, long c, bool back, T v[3]) {
+  const StepArgs<T>& a = cold_args(a0);
+  const long n = back ? step - 1 : step;
+  for (int s = 0; s < a.nsrc; s++) {
+    const SrcDev& q = a.src[s];
+    if (q.cell == c && q.field == field && n >= 0 && n < q.n) {
+      const T x = back ? -(T)q.table[n] : (T)q.table[n];
+      if (q.comp == 0)
+        v[0] += x;
+      else if (q.comp == 1)
+        v[1] += x;
+      else
+        v[2] += x;
     }
   }
 }
@@ -553,11 +645,21 @@ struct EStencil {
 // (one set of pointers instead of two: fewer scalar registers).
 // PRE: psi and the slab coefficients come from *pre (gathered by the caller)
 // instead of memory; Row: the incident row's pointer type (global or LDS).
-template <typename T, int DISP = 1, bool IP = false, bool PRE = false, typename Row = const FDTD_G double*>
+// GEN: the general instance, which also carries E / H soft sources and single
+// TFSF planes (code the other instances leave out, keeping their registers).
+template <typename T, int DISP = 1, bool IP = false, bool PRE = false, typename Row = const FDTD_G double*,
+          bool GEN = false>
 __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, Row incH, int i, int j, int k, long c,
                                           const HStencil<T>& h, T ex, T ey, T ez, int mid, T ie, T fx, T fy, T fz,
                                           bool own, T out[3], const CellPre<T>* pre = nullptr) {
 #pragma clang fp contract(off)
+  if (GEN && a.nsrc_eh) {  // the old E without the last step's E source value
+    T v[3] = {ex, ey, ez};
+    soft_eh(a, step, 1, c, true, v);
+    ex = v[0];
+    ey = v[1];
+    ez = v[2];
+  }
   const T dxy = h.hz - h.hz_ym;  // Ex, J=y, +1
   const T dxz = h.hy - h.hy_zm;  // Ex, J=z, -1
   const T dyz = h.hx - h.hx_zm;  // Ey, J=z, +1
@@ -616,27 +718,37 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, Row i
     const T fn[3] = {fx, fy, fz};
     tfsf_cell<T, 3, (DISP > 0)>(a, incH, 0, i, j, k, act, act, PRE ? fn : nullptr, d);
   }
+  if (GEN && a.nplane) plane_cells<T, 3>(a, step, 0, i, j, k, 7, 7, d);
   if (ie > T(0)) {
     out[0] = fmadd(d[0], ie, ex);
     out[1] = fmadd(d[1], ie, ey);
     out[2] = fmadd(d[2], ie, ez);
   } else if (DISP && ie < T(0) && a.Dd[0]) {
     const T eold[3] = {ex, ey, ez};
-    const T hown[3] = {h.hx, h.hy, h.hz};
+    T hown[3] = {h.hx, h.hy, h.hz};
+    if (DISP == 2 && GEN && a.nsrc_eh) soft_eh(a, step, 3, c, true, hown);  // B = mu H without the H source
     ade_cell<T, DISP == 2>(a, mid, i, j, k, d, eold, hown, own, out);
   } else {
     out[0] = ex;
     out[1] = ey;
     out[2] = ez;
   }
+  if (GEN && a.nsrc_eh) soft_eh(a, step, 1, c, false, out);
 }
 
 // ---------------------------------------------------------------- H of one cell
-template <typename T, bool IP = false, bool PRE = false, typename Row = const FDTD_G double*>
+template <typename T, bool IP = false, bool PRE = false, typename Row = const FDTD_G double*, bool GEN = false>
 __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, Row incE, int i, int j, int k, long c,
                                           const EStencil<T>& e, T hx, T hy, T hz, T im, T fx, T fy, T fz, T out[3],
                                           const CellPre<T>* pre = nullptr) {
 #pragma clang fp contract(off)
+  if (GEN && a.nsrc_eh) {  // the old H without the last step's H source value
+    T v[3] = {hx, hy, hz};
+    soft_eh(a, step, 3, c, true, v);
+    hx = v[0];
+    hy = v[1];
+    hz = v[2];
+  }
   const T dxy = e.ez_yp - e.ez;  // Hx, J=y, +1
   const T dxz = e.ey_zp - e.ey;  // Hx, J=z, -1
   const T dyz = e.ex_zp - e.ex;  // Hy, J=z, +1
@@ -689,9 +801,11 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, Row i
     const T fn[3] = {fx, fy, fz};
     tfsf_cell<T, 3>(a, incE, 1, i, j, k, act, act, PRE ? fn : nullptr, d);
   }
+  if (GEN && a.nplane) plane_cells<T, 3>(a, step, 1, i, j, k, 7, 7, d);
   out[0] = fmadd(d[0], im, hx);
   out[1] = fmadd(d[1], im, hy);
   out[2] = fmadd(d[2], im, hz);
+  if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, out);
 }
 
 // probes of one cell after its step
@@ -985,11 +1099,12 @@ __device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a0, int ft, int ndim
 // ---------------------------------------------------------------- two passes
 // One thread per cell and z plane: every load of the cell is issued before
 // any arithmetic, so a wave carries ~12 independent loads in flight.
-// BC: the instance for grids with boundary conditions (ghost writes); DISP:
-// for grids with dispersive materials (1: ADE poles, 2: also magnetized and
-// fluid Drude).  An instance without them carries none of that code, so its
-// registers stay low.
-template <typename T, bool BC = false, int DISP = 1>
+// GEN: the general instance, for grids with boundary conditions (ghost
+// writes), E / H soft sources or single TFSF planes; DISP: for grids with
+// dispersive materials (1: ADE poles, 2: also magnetized and fluid Drude).
+// An instance without them carries none of that code, so its registers stay
+// low.
+template <typename T, bool GEN = false, int DISP = 1>
 __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie, int i, int j, int k0, int k1, int s) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   const FDTD_G T* __restrict__ Hx = a.H[0];
@@ -1018,17 +1133,17 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     const int mid = a.mat[c];
     const T fz = a.cf[0][2][k];
     T o[3];
-    e_cell_3d<T, DISP, true>(a, step, incH, i, j, k, c, h, ex, ey, ez, mid, cell_inv_eps(a, tie, mid, i, j, k), fx, fy,
-                             fz, true, o);
+    e_cell_3d<T, DISP, true, false, const FDTD_G double*, GEN>(a, step, incH, i, j, k, c, h, ex, ey, ez, mid,
+                                                                cell_inv_eps(a, tie, mid, i, j, k), fx, fy, fz, true, o);
     a.Eo[0][c] = o[0];
     a.Eo[1][c] = o[1];
     a.Eo[2][c] = o[2];
     if (a.hist) record_increment<T, true>(a, step, i, j, k, o[0] - ex, o[1] - ey, o[2] - ez);
-    if (BC && a.bc_any) bc_ghost_write<T, true>(a, 0, 3, i, j, k, c, o);
+    if (GEN && a.bc_any) bc_ghost_write<T, true>(a, 0, 3, i, j, k, c, o);
   }
 }
 
-template <typename T, typename Tab, bool BC = false>
+template <typename T, typename Tab, bool GEN = false>
 __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int i, int j, int k0, int k1, int s) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   // in place: the E pass wrote E; H is read and written through the same pointers
@@ -1059,11 +1174,12 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int 
     const int mid = a.mat[c];
     const T fz = a.cf[1][2][k];
     T o[3];
-    h_cell_3d<T, true>(a, step, incE, i, j, k, c, e, hx, hy, hz, tim[mid], fx, fy, fz, o);
+    h_cell_3d<T, true, false, const FDTD_G double*, GEN>(a, step, incE, i, j, k, c, e, hx, hy, hz, tim[mid], fx, fy,
+                                                          fz, o);
     Hx[c] = o[0];
     Hy[c] = o[1];
     Hz[c] = o[2];
-    if (BC && a.bc_any) bc_ghost_write(a, 1, 3, i, j, k, c, o);
+    if (GEN && a.bc_any) bc_ghost_write(a, 1, 3, i, j, k, c, o);
     if (extra) {
       const T ev[3] = {e.ex, e.ey, e.ez};
       dft_probe(a, step, c, i, j, k, ev, o);
@@ -1073,7 +1189,7 @@ __device__ __forceinline__ void h_pass_cells(const StepArgs<T>& a, Tab tim, int 
 
 // One-dimensional launch of ntx x nty x nkc tiles of blockDim.x x blockDim.y
 // columns and kc planes, XCD-aware (tiles x fastest, then y, then z chunks).
-template <typename T, bool BC = false, int DISP = 1>
+template <typename T, bool GEN = false, int DISP = 1>
 __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                  int nkc) {
   const StepArgs<T>& a = *ap;
@@ -1082,7 +1198,7 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
   const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
   const int k0 = 1 + bz * kc;
-  e_pass_cells<T, BC, DISP>(a, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
+  e_pass_cells<T, GEN, DISP>(a, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
                             min(k0 + kc, a.nz - 1), s);
 }
 
@@ -1093,7 +1209,7 @@ __global__ __launch_bounds__(256) void e_pass_3d(const StepArgs<T>* __restrict__
 // FP32 instances are compiled for 7 waves per SIMD (the register count they
 // need anyway): at 256^3 on MI355X that schedule runs the H pass ~10 % faster
 // (0.151 vs 0.168 ms per step) than the compiler's default.
-template <typename T, bool BC>
+template <typename T, bool GEN>
 __device__ __forceinline__ void h_pass_3d_body(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                int nkc) {
   const StepArgs<T>& a = *ap;
@@ -1102,14 +1218,14 @@ __device__ __forceinline__ void h_pass_3d_body(const StepArgs<T>* __restrict__ a
   const int k0 = 1 + bz * kc;
   // the H pass reads 1/(mu_r mu0) from global memory: staging it in LDS
   // measured slower here (the E pass, with its larger stencil, gains)
-  h_pass_cells<T, decltype(a.mim), BC>(a, a.mim, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y,
+  h_pass_cells<T, decltype(a.mim), GEN>(a, a.mim, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y,
                                        k0, min(k0 + kc, a.nz - 1), s);
 }
 
-template <typename T, bool BC = false>
+template <typename T, bool GEN = false>
 __global__ __launch_bounds__(256) void h_pass_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx, int nty,
                                                  int nkc) {
-  h_pass_3d_body<T, BC>(ap, kc, s, ntx, nty, nkc);
+  h_pass_3d_body<T, GEN>(ap, kc, s, ntx, nty, nkc);
 }
 template <>
 __global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_3d<float, false>(
@@ -1866,7 +1982,9 @@ __global__ __launch_bounds__(kSThreads) void fused3d_lds(const StepArgs<float>* 
 
 // ---------------------------------------------------------------- 2-D TM
 // Ez, Hx, Hy (include/YeeFields.hpp:123-148); CPML along x and y; in place.
-template <typename T, bool BC = false>
+// GEN: the general instance (boundary ghost writes, E / H soft sources,
+// single TFSF planes).
+template <typename T, bool GEN = false>
 __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
   const StepArgs<T>& a = *ap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1877,7 +1995,12 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
   const long step = a.step0 + s;
   const long c = (long)i + a.sy * j;
   const T hx = Hx[c], hy = Hy[c], hy_xm = Hy[c - 1], hx_ym = Hx[c - a.sy];
-  const T ez = a.E[2][c];
+  T ez = a.E[2][c];
+  if (GEN && a.nsrc_eh) {  // the old Ez without the last step's E source value
+    T v[3] = {T(0), T(0), ez};
+    soft_eh(a, step, 1, c, true, v);
+    ez = v[2];
+  }
   const int mid = a.mat[c];
   const T ie = cell_inv_eps(a, a.mie, mid, i, j, 0);
   const T dzx = hy - hy_xm;  // J=x, +1
@@ -1903,6 +2026,7 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
     const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
     tfsf_cell<T, 2>(a, a.incH + (long)s * a.auxL, 0, i, j, 0, aF, aC, (const T*)nullptr, d);
   }
+  if (GEN && a.nplane) plane_cells<T, 2>(a, step, 0, i, j, 0, 4, 3, d);
   T ezn = ez;
   if (ie > T(0)) {
     ezn = ez + d[2] * ie;
@@ -1915,13 +2039,19 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
       ezn = ade_component<T, 2>(a, m, dbox_index(a, i, j, 0), d[2], ez, true);
     a.Eo[2][c] = ezn;
   }
-  if (BC && a.bc_any) {
+  if (GEN && a.nsrc_eh) {
+    T v[3] = {T(0), T(0), ezn};
+    soft_eh(a, step, 1, c, false, v);
+    ezn = v[2];
+    a.Eo[2][c] = ezn;
+  }
+  if (GEN && a.bc_any) {
     const T v[3] = {T(0), T(0), ezn};
     bc_ghost_write(a, 0, 2, i, j, 0, c, v);
   }
 }
 
-template <typename T, bool BC = false>
+template <typename T, bool GEN = false>
 __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
   const StepArgs<T>& a = *ap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1931,7 +2061,13 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   const long step = a.step0 + s;
   const long c = (long)i + a.sy * j;
   const T ez = Ez[c], ez_yp = Ez[c + a.sy], ez_xp = Ez[c + 1];
-  const T hx = a.H[0][c], hy = a.H[1][c];
+  T hx = a.H[0][c], hy = a.H[1][c];
+  if (GEN && a.nsrc_eh) {  // the old H without the last step's H source value
+    T v[3] = {hx, hy, T(0)};
+    soft_eh(a, step, 3, c, true, v);
+    hx = v[0];
+    hy = v[1];
+  }
   const T im = a.mim[a.mat[c]];
   const T dxy = ez_yp - ez;  // Hx, J=y, +1
   const T dyx = ez_xp - ez;  // Hy, J=x, -1
@@ -1957,10 +2093,12 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
     const bool aF[3] = {true, true, false}, aC[3] = {false, false, true};
     tfsf_cell<T, 2>(a, a.incE + (long)s * a.auxL, 1, i, j, 0, aF, aC, (const T*)nullptr, d);
   }
+  if (GEN && a.nplane) plane_cells<T, 2>(a, step, 1, i, j, 0, 3, 4, d);
   T h[3] = {hx + d[0] * im, hy + d[1] * im, T(0)};
+  if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, h);
   a.Ho[0][c] = h[0];
   a.Ho[1][c] = h[1];
-  if (BC && a.bc_any) bc_ghost_write(a, 1, 2, i, j, 0, c, h);
+  if (GEN && a.bc_any) bc_ghost_write(a, 1, 2, i, j, 0, c, h);
   if (a.nprobe) {
     const T e[3] = {T(0), T(0), ez};
     dft_probe(a, step, c, i, j, 0, e, h);

@@ -25,7 +25,7 @@ BOUNDARY = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "
 # every symbol declared in include/fdtd_hip.h
 ABI_SYMBOLS = [
     "fdtd_grid_create", "fdtd_grid_destroy", "fdtd_set_stream", "fdtd_set_pml_axis", "fdtd_set_pml_idiom",
-    "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_dft", "fdtd_get_dft",
+    "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_tfsf_plane", "fdtd_add_dft", "fdtd_get_dft",
     "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune", "fdtd_set_adjoint_tile",
@@ -72,6 +72,7 @@ def load_library():
         "fdtd_set_matid": (_i, [_p, ctypes.c_char_p]),
         "fdtd_add_source": (_i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
         "fdtd_set_tfsf": (_i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
+        "fdtd_add_tfsf_plane": (_i, [_p, _i, _i, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
         "fdtd_add_dft": (_i, [_p, _ip, _i, _i, _dp, _dp, _i, _ip]),
         "fdtd_get_dft": (_i, [_p, _i, _dp]),
         "fdtd_add_probe": (_i, [_p, _i, _i, _i, _i, _i]),
@@ -209,6 +210,14 @@ class Grid:
         k, p, ph, o, s = (_f64(x) for x in (kvec, pol, polH, origin, signal))
         _check(load_library().fdtd_set_tfsf(self.h, b, _dptr(k), _dptr(p), _dptr(ph), _dptr(o), aux_ncells,
                                             _dptr(s), s.size))
+
+    def add_tfsf_plane(self, axis, side, cells, kvec, pol, polH, origin, aux_ncells, signal):
+        """One TFSF plane with its own incident line (fdtd_add_tfsf_plane): normal
+        along `axis` toward `side` (0 MIN, 1 MAX), cells [x0,x1)x[y0,y1)x[z0,z1)."""
+        c = (ctypes.c_int * 6)(*cells)
+        k, p, ph, o, s = (_f64(x) for x in (kvec, pol, polH, origin, signal))
+        _check(load_library().fdtd_add_tfsf_plane(self.h, axis, side, c, _dptr(k), _dptr(p), _dptr(ph), _dptr(o),
+                                                  aux_ncells, _dptr(s), s.size))
 
     def add_dft(self, region, comp_mask, phaseE, phaseH):
         pe, ph = _f64(phaseE), _f64(phaseH)

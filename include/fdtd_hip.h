@@ -62,6 +62,9 @@ enum fdtd_boundary {
 #define FDTD_MAX_SOURCES 16
 #define FDTD_MAX_DFT 8
 #define FDTD_MAX_PROBES 32
+#define FDTD_MAX_PLANES 6
+/* the side of a face: Orientation::MIN / MAX (include/FDTDConstants.hpp) */
+enum fdtd_side { FDTD_MIN = 0, FDTD_MAX = 1 };
 
 typedef struct fdtd_grid fdtd_grid;
 
@@ -128,7 +131,14 @@ int fdtd_set_matid(fdtd_grid* g, const uint8_t* ids);
 
 /* make_soft_source + SoftSource::apply (include/Source.hpp:183-261): at step
  * n, table[n] is added to D_comp (field FDTD_D) or B_comp (FDTD_B) of cell
- * (i,j,k) right after the curl, before CPML and the material step. */
+ * (i,j,k) right after the curl, before CPML and the material step (a lasting
+ * source: testTM.cpp:105), or to E_comp (FDTD_E) or H_comp (FDTD_H) right
+ * after the material step, before the boundaries (the placement of Aux1D's
+ * own source, Source.hpp:421-425: the next material step recomputes E from
+ * D, H from B, so the value acts on one curl of the other field only).
+ * E and H are stored without D and B, so a value of E or H uploaded between
+ * steps stands for the field including the last step's source value, which
+ * the next step takes back off.  Steps n >= ntable add nothing. */
 int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, const double* table, int ntable);
 
 /* TFSF box = make_tfsf_plane x 2*dim faces sharing one Aux1D history
@@ -140,6 +150,21 @@ int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, cons
  * signal[n] is its soft-source value at its n-th step. */
 int fdtd_set_tfsf(fdtd_grid* g, const int box[6], const double kvec[3], const double pol[3], const double polH[3],
                   const double origin[3], int aux_ncells, const double* signal, int nsignal);
+
+/* One TFSF plane = make_tfsf_plane<Mode, Dir, Orient>(begin, end, loc, k, aux)
+ * + TFSFPlane::updateD / updateB (include/Source.hpp:472-634) with its own
+ * Aux1D: normal along `axis` (0 x, 1 y, 2 z) pointing to `side` (FDTD_MIN:
+ * -1, FDTD_MAX: +1); the plane's cells are the box [x0,x1)x[y0,y1)x[z0,z1)
+ * of `cells`, one layer thick along the axis (the iterator range; TM:
+ * z range ignored).  Every cell of it gets, in each step, the correction of
+ * every flux but the normal one, on D after the E-side CPML and on B after
+ * the H-side CPML, in the order the planes were added (after the box's).  The
+ * location functor is r = (cell - origin) dx.  kvec is normalised; pol / polH
+ * as for fdtd_set_tfsf (the reference's defaults per mode,
+ * Source.hpp:573-584, are the caller's to pass).  At most FDTD_MAX_PLANES. */
+int fdtd_add_tfsf_plane(fdtd_grid* g, int axis, int side, const int cells[6], const double kvec[3],
+                        const double pol[3], const double polH[3], const double origin[3], int aux_ncells,
+                        const double* signal, int nsignal);
 
 /* Running DFT over the half-open region [i0,i1)x[j0,j1)x[k0,k1) (no
  * counterpart in the reference: include/DataCapture.hpp:61-191 is empty).

@@ -357,6 +357,25 @@ struct TMScenario {
   int aux_ncells;
   double f0, tw, t0;
   bool save_full = true;
+  // soft sources on the derived fields, applied after the material step
+  // through the reference's SoftSource (include/Source.hpp:183-261):
+  // field 1 = E (comp 2: Ez), 3 = H (comp 0 / 1: Hx / Hy); table per step
+  struct Soft {
+    int field, comp, i, j;
+    std::vector<double> table;
+  };
+  std::vector<Soft> soft;
+  // single TFSF planes (make_tfsf_plane, Source.hpp:624-634), each with its
+  // own Aux1D driven by the Gaussian-modulated sinusoid (f0, tw, t0): normal
+  // along axis (0 x, 1 y), side 0 MIN / 1 MAX, on cell layer `layer`, cells
+  // lo..hi (inclusive) along the other axis; kx, ky, cx, cy, aux_ncells, f0
+  // and tw as for the box
+  struct Plane {
+    int axis, side, layer, lo, hi;
+    double t0;
+  };
+  std::vector<Plane> planes1;
+  double cx1 = 0, cy1 = 0;  // the planes' location origin (cell units) when there is no box
 };
 
 static void run_tm(const TMScenario& S, const std::string& dir) {
@@ -396,14 +415,25 @@ static void run_tm(const TMScenario& S, const std::string& dir) {
     });
 
   // TFSF: one plane per face of the box, each with its own Aux1D
-  // (include/Source.hpp:595-609 steps the aux inside updateD).
+  // (include/Source.hpp:595-609 steps the aux inside updateD), then the
+  // single planes, each with its own Aux1D too.
   typedef TFSFPlane<TM, LocTM, PlaneIt<CellTM>, Aux1D<double>> PlaneT;
   std::vector<std::vector<PlaneEntry<CellTM>>> faces;
   std::vector<PlaneT> planes;
   std::vector<double> aux_sig;
-  const double cx = std::floor(0.5 * (S.bx0 + S.bx1)) + 0.5, cy = std::floor(0.5 * (S.by0 + S.by1)) + 0.5;
+  std::vector<std::vector<double>> plane_sig(S.planes1.size());
+  const double cx = S.tfsf ? std::floor(0.5 * (S.bx0 + S.bx1)) + 0.5 : S.cx1,
+               cy = S.tfsf ? std::floor(0.5 * (S.by0 + S.by1)) + 0.5 : S.cy1;
+  faces.resize((S.tfsf ? 4 : 0) + S.planes1.size());
+  for (size_t p = 0; p < S.planes1.size(); p++) {
+    const auto& P = S.planes1[p];
+    auto& F = faces[(S.tfsf ? 4 : 0) + p];
+    for (int q = P.lo; q <= P.hi; q++) {
+      const int i = P.axis == 0 ? P.layer : q, j = P.axis == 0 ? q : P.layer;
+      F.push_back({&at(i, j), (i - cx) * dx, (j - cy) * dx, 0});
+    }
+  }
   if (S.tfsf) {
-    faces.resize(4);
     for (int j = S.by0 + 1; j <= S.by1; j++) {
       faces[0].push_back({&at(S.bx0, j), (S.bx0 - cx) * dx, (j - cy) * dx, 0});
       faces[1].push_back({&at(S.bx1, j), (S.bx1 - cx) * dx, (j - cy) * dx, 0});
@@ -429,6 +459,47 @@ static void run_tm(const TMScenario& S, const std::string& dir) {
       if (fidx == 3) planes.push_back(make_tfsf_plane<TM, Dir::Y, Orientation::MAX>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
     }
   }
+  for (size_t p = 0; p < S.planes1.size(); p++) {
+    const auto& P = S.planes1[p];
+    const double f0 = S.f0, tw = S.tw, t0 = P.t0;
+    std::vector<double>* rec = &plane_sig[p];
+    auto sig = [f0, tw, t0, rec](double t) {
+      double a = (t - t0) / tw;
+      double v = std::exp(-a * a) * std::sin(2 * pi * f0 * (t - t0));
+      rec->push_back(v);
+      return v;
+    };
+    auto aux = std::make_shared<Aux1D<double>>(S.aux_ncells, dx, dt, sig);
+    std::array<double, 2> K = {S.kx, S.ky};
+    auto& F = faces[(S.tfsf ? 4 : 0) + p];
+    PlaneIt<CellTM> b{F.data()}, e{F.data() + F.size()};
+    const int which = 2 * P.axis + P.side;
+    if (which == 0) planes.push_back(make_tfsf_plane<TM, Dir::X, Orientation::MIN>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+    if (which == 1) planes.push_back(make_tfsf_plane<TM, Dir::X, Orientation::MAX>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+    if (which == 2) planes.push_back(make_tfsf_plane<TM, Dir::Y, Orientation::MIN>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+    if (which == 3) planes.push_back(make_tfsf_plane<TM, Dir::Y, Orientation::MAX>(PlaneIt<CellTM>(b), PlaneIt<CellTM>(e), LocTM(), K, aux));
+  }
+  // the soft sources on E and H: the reference's SoftSource objects, sampled
+  // at t = step index
+  std::vector<std::function<void(double)>> softE, softH;
+  for (const auto& q : S.soft) {
+    auto tab = std::make_shared<std::vector<double>>(q.table);
+    auto sig = [tab](double t) {
+      const long n = std::lround(t);
+      return n < long(tab->size()) ? (*tab)[n] : 0.0;
+    };
+    CellTM& c = at(q.i, q.j);
+    if (q.field == 1) {
+      auto ss = make_soft_source<Ez>(sig, c);
+      softE.push_back([ss](double t) mutable { ss.apply(t); });
+    } else if (q.comp == 0) {
+      auto ss = make_soft_source<Hx>(sig, c);
+      softH.push_back([ss](double t) mutable { ss.apply(t); });
+    } else {
+      auto ss = make_soft_source<Hy>(sig, c);
+      softH.push_back([ss](double t) mutable { ss.apply(t); });
+    }
+  }
 
   std::vector<double> ptrace;
   for (int n = 0; n < S.nsteps; n++) {
@@ -442,6 +513,7 @@ static void run_tm(const TMScenario& S, const std::string& dir) {
     for (auto& pl : planes) pl.updateD(dt, dx);
     for (int j = 1; j < ny - 1; j++)
       for (int i = 1; i < nx - 1; i++) material_E<TM>(at(i, j), S.mats[S.matid[size_t(i) + size_t(nx) * j]], dt);
+    for (auto& f : softE) f(double(n));
     for (int j = 1; j < ny - 1; j++)
       for (int i = 1; i < nx - 1; i++) YeeUpdate<TM, Bx>(dt, dx)(at(i, j));
     for (int j = 1; j < ny - 1; j++)
@@ -453,6 +525,7 @@ static void run_tm(const TMScenario& S, const std::string& dir) {
     for (auto& pl : planes) pl.updateB(dt, dx);
     for (int j = 1; j < ny - 1; j++)
       for (int i = 1; i < nx - 1; i++) VacuumUpdate<TM, FieldType::Magnetic>()(at(i, j));
+    for (auto& f : softH) f(double(n));
     for (auto& pr : S.probes) {
       ptrace.push_back(at(pr.first, pr.second).Ez());
       ptrace.push_back(at(pr.first, pr.second).Hx());
@@ -488,6 +561,28 @@ static void run_tm(const TMScenario& S, const std::string& dir) {
     fx.p("tw", S.tw);
     fx.p("t0", S.t0);
     fx.arr("aux_signal", aux_sig, {aux_sig.size()});
+  }
+  if (!S.planes1.empty()) {
+    fx.p("kx", S.kx);
+    fx.p("ky", S.ky);
+    fx.pi("aux_ncells", S.aux_ncells);
+    fx.p("cx", cx);
+    fx.p("cy", cy);
+    fx.p("f0", S.f0);
+    fx.p("tw", S.tw);
+    std::vector<double> pl;
+    for (const auto& P : S.planes1) pl.insert(pl.end(), {double(P.axis), double(P.side), double(P.layer), double(P.lo), double(P.hi), P.t0});
+    fx.arr("planes_axis_side_layer_lo_hi_t0", pl, {S.planes1.size(), 6});
+    for (size_t p = 0; p < S.planes1.size(); p++) fx.arr("plane" + std::to_string(p) + "_signal", plane_sig[p], {plane_sig[p].size()});
+  }
+  if (!S.soft.empty()) {
+    std::vector<double> sp, st;
+    for (const auto& q : S.soft) {
+      sp.insert(sp.end(), {double(q.field), double(q.comp), double(q.i), double(q.j)});
+      st.insert(st.end(), q.table.begin(), q.table.end());
+    }
+    fx.arr("soft_field_comp_i_j", sp, {S.soft.size(), 4});
+    fx.arr("soft_tables", st, {S.soft.size(), size_t(S.nsteps)});
   }
   std::vector<double> mt;
   for (auto& m : S.mats) {
@@ -1345,6 +1440,48 @@ int main(int argc, char** argv) {
     S.tw = 2.0 / S.f0;
     S.t0 = 3.0 * S.tw;
     S.probes = {{30, 30}, {5 + 10, 30}, {14, 30}, {13, 30}, {49, 20}, {50, 20}, {31, 14}, {31, 13}};
+    run_tm(S, dir);
+  }
+  // 2D TM: soft sources on Ez (inside a Lorentz block) and on Hy, added
+  // after their material steps, and two single TFSF planes (an x-MIN face
+  // and a y-MAX face, each with its own Aux1D and its own pulse delay) under
+  // an oblique wave; CPML x+y, a dielectric block.
+  {
+    TMScenario S;
+    S.name = "tm2d_soft_planes";
+    S.nx = 56;
+    S.ny = 50;
+    S.nsteps = 170;
+    S.dx = 1e-6;
+    S.dt = 0.7 * S.dx / (c0 * std::sqrt(2.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {10, 3, 1, dm.msMax, 5, 0.05};
+    S.pmlx = S.pmly = true;
+    S.mats = {{0, 1, 0, 0, 0}, {1, 2.5, 0, 0, 0}};
+    S.mats.push_back({3, 2.0, 0.2 / S.dt, 0.01 / S.dt, 0.7});
+    S.matid.assign(size_t(S.nx) * S.ny, 0);
+    for (int j = 26; j < 34; j++)
+      for (int i = 30; i < 38; i++) S.matid[size_t(i) + size_t(S.nx) * j] = 1;
+    for (int j = 30; j < 36; j++)
+      for (int i = 18; i < 24; i++) S.matid[size_t(i) + size_t(S.nx) * j] = 2;
+    S.src_i = -1;
+    S.src_j = -1;
+    TMScenario::Soft se{1, 2, 20, 32, {}}, sh{3, 1, 34, 18, {}};
+    const double imp = std::sqrt(mu0 / eps0);
+    for (int n = 0; n < S.nsteps; n++) {
+      se.table.push_back(gauss_pulse(n, 30, 8));
+      sh.table.push_back(gauss_pulse(n, 45, 10) / imp);
+    }
+    S.soft = {se, sh};
+    S.kx = std::cos(25.0 * pi / 180.0);
+    S.ky = std::sin(25.0 * pi / 180.0);
+    S.aux_ncells = 100;
+    S.f0 = c0 / (20 * S.dx);
+    S.tw = 2.0 / S.f0;
+    S.cx1 = 27.5;
+    S.cy1 = 25.5;
+    S.planes1 = {{0, 0, 14, 12, 37, 3.0 * S.tw}, {1, 1, 38, 16, 40, 4.0 * S.tw}};
+    S.probes = {{20, 32}, {34, 18}, {14, 24}, {13, 24}, {28, 38}, {28, 39}, {40, 20}, {45, 45}};
     run_tm(S, dir);
   }
   // 3-D, small and anisotropic, CPML on all faces, two dielectric blocks,

@@ -242,6 +242,17 @@ typedef struct {
   size_t cell;
 } rc_probe;
 
+/* one single TFSF plane with its own incident line */
+typedef struct {
+  int axis;
+  double nrm; /* outward normal along axis: -1 MIN, +1 MAX */
+  int lo[3], hi[3]; /* cells, inclusive */
+  double kvec[3], pol[3], polH[3], origin[3];
+  rc_aux* aux;
+  double* signal;
+  int nsignal;
+} rc_plane;
+
 struct rc_sim {
   int mode, nx, ny, nz;
   size_t ncell;
@@ -262,6 +273,8 @@ struct rc_sim {
   rc_aux* aux;
   double* signal;
   int nsignal;
+  rc_plane plane[RC_MAXPLANE];
+  int nplane;
   rc_dft* dft;
   int ndft;
   rc_probe* probes;
@@ -379,6 +392,10 @@ void rc_destroy(rc_sim* s) {
   free(s->src);
   rc_aux_destroy(s->aux);
   free(s->signal);
+  for (int p = 0; p < s->nplane; p++) {
+    rc_aux_destroy(s->plane[p].aux);
+    free(s->plane[p].signal);
+  }
   for (int q = 0; q < s->ndft; q++) {
     free(s->dft[q].phE);
     free(s->dft[q].phH);
@@ -538,8 +555,8 @@ int rc_set_matid(rc_sim* s, const uint8_t* ids) {
 }
 
 int rc_add_source(rc_sim* s, int field, int comp, int i, int j, int k, const double* table, int ntable) {
-  if (field < 0 || field > 1 || comp < 0 || comp > 2) return -1;
-  if (!(field == 0 ? s->actE[comp] : s->actH[comp])) return -1;
+  if (field < 0 || field > 3 || comp < 0 || comp > 2) return -1;
+  if (!(field % 2 == 0 ? s->actE[comp] : s->actH[comp])) return -1;
   s->src = realloc(s->src, sizeof(rc_src) * (s->nsrc + 1));
   rc_src* q = &s->src[s->nsrc++];
   q->field = field;
@@ -570,6 +587,33 @@ int rc_set_tfsf(rc_sim* s, const int box[6], const double kvec[3], const double 
   s->signal = malloc(sizeof(double) * nsignal);
   memcpy(s->signal, signal, sizeof(double) * nsignal);
   s->nsignal = nsignal;
+  return 0;
+}
+
+int rc_add_tfsf_plane(rc_sim* s, int axis, int side, const int cells[6], const double kvec[3], const double pol[3],
+                      const double polH[3], const double origin[3], int aux_ncells, const double* signal,
+                      int nsignal) {
+  if (s->nplane >= RC_MAXPLANE || axis < 0 || axis > 2 || !s->axes[axis]) return -1;
+  rc_plane* P = &s->plane[s->nplane++];
+  P->axis = axis;
+  P->nrm = side ? 1.0 : -1.0;
+  for (int d = 0; d < 3; d++) {
+    P->lo[d] = cells[2 * d];
+    P->hi[d] = cells[2 * d + 1] - 1;
+  }
+  /* TFSFPlane normalises the wave vector, Source.hpp:566-570 */
+  int nd = s->mode == RC_3D ? 3 : (s->mode == RC_TEM ? 1 : 2);
+  double K = 0;
+  for (int d = 0; d < nd; d++) K += kvec[d] * kvec[d];
+  K = sqrt(K);
+  for (int d = 0; d < 3; d++) P->kvec[d] = d < nd ? kvec[d] / K : 0.0;
+  memcpy(P->pol, pol, sizeof(double) * 3);
+  memcpy(P->polH, polH, sizeof(double) * 3);
+  memcpy(P->origin, origin, sizeof(double) * 3);
+  P->aux = rc_aux_create(aux_ncells, s->dx, s->dt);
+  P->signal = malloc(sizeof(double) * (nsignal > 0 ? nsignal : 1));
+  memcpy(P->signal, signal, sizeof(double) * nsignal);
+  P->nsignal = nsignal;
   return 0;
 }
 
@@ -909,6 +953,52 @@ static void tfsf_faces(rc_sim* s, int ft) {
   }
 }
 
+/* one single plane's correction pass (TFSFPlane::updateD / updateB,
+ * Source.hpp:595-620): every cell of the plane, every flux I but the normal
+ * axis J (whose two curl directions have a zero normal component), with the
+ * per-cell arithmetic of TFSFPlane::field_update (506-553) */
+static void plane_faces(rc_sim* s, const rc_plane* P, int ft) {
+  const double dt = s->dt, dx = s->dx;
+  const int J = P->axis;
+  for (int I = 0; I < 3; I++) {
+    if (I == J) continue;
+    if (!(ft == 0 ? s->actE[I] : s->actH[I])) continue;
+    const int K = third(I, J);
+    if (!(ft == 0 ? s->actH[K] : s->actE[K])) continue;
+    double* F = ft == 0 ? s->D[I] : s->B[I];
+    const double* pml = s->pml[ft][J];
+    const double* polv = ft == 0 ? P->polH : P->pol;
+    const int e = lc(I, J, K);
+    for (int k = P->lo[2]; k <= P->hi[2]; k++)
+      for (int j = P->lo[1]; j <= P->hi[1]; j++)
+        for (int i = P->lo[0]; i <= P->hi[0]; i++) {
+          int ci[3] = {i, j, k};
+          double proj = 0;
+          for (int dd = 0; dd < 3; dd++) {
+            if (!s->axes[dd]) continue;
+            double off = ft == 0 ? ((dd == K) ? -0.5 : 0.0) : ((dd != K) ? -0.5 : 0.0);
+            double loc = (ci[dd] - P->origin[dd]) * dx;
+            loc += off * dx;
+            proj += P->kvec[dd] * loc;
+          }
+          double srcval = ft == 0 ? rc_aux_interp_H(P->aux, proj) : rc_aux_interp_E(P->aux, proj);
+          double term = e * pml[4 * ci[J] + 2] / (dx)*P->nrm * srcval * polv[K];
+          size_t c = cidx(s, i, j, k);
+          double coeff = ft == 0 ? 1.0 : -1.0;
+          F[c] += coeff * 1.0 * dt * (term + 0.0);
+        }
+  }
+}
+
+/* soft sources on the derived fields (field 2: E, 3: H), added after the
+ * material step (SoftSource::apply, Source.hpp:209-229, placed as Aux1D places
+ * its own, 421-425): the next material step recomputes the field from D or B */
+static void soft_derived(rc_sim* s, int field, long n) {
+  for (int q = 0; q < s->nsrc; q++)
+    if (s->src[q].field == field && n < s->src[q].n)
+      (field == 2 ? s->E : s->H)[s->src[q].comp][s->src[q].cell] += s->src[q].table[n];
+}
+
 /* One step in three phases (the E half step with its boundaries, the H half
  * step with its boundaries, then DFT and probes), so that the two parts of a
  * complex grid advance in lockstep with the Bloch coupling between them. */
@@ -970,6 +1060,11 @@ static void step_e(rc_sim* s) {
     tfsf_faces(s, 0);
     rc_aux_step(s->aux, n < s->nsignal ? s->signal[n] : 0.0);
   }
+  for (int p = 0; p < s->nplane; p++) {
+    rc_plane* P = &s->plane[p];
+    plane_faces(s, P, 0);
+    rc_aux_step(P->aux, n < P->nsignal ? P->signal[n] : 0.0);
+  }
   /* ---- E material (DispersiveMaterials.hpp:354-359, 710-735, 979-998) */
   {
     const int np = s->npoles_alloc;
@@ -1027,6 +1122,7 @@ static void step_e(rc_sim* s) {
           }
         }
   }
+  soft_derived(s, 2, n);
   apply_boundaries(s, 0);
 }
 
@@ -1084,6 +1180,7 @@ static void step_h(rc_sim* s) {
     }
   }
   if (s->tfsf) tfsf_faces(s, 1);
+  for (int p = 0; p < s->nplane; p++) plane_faces(s, &s->plane[p], 1);
   /* ---- H material: H = B/(mu_r mu0) */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = lo[2]; k < hi[2]; k++)
@@ -1094,6 +1191,7 @@ static void step_h(rc_sim* s) {
         for (int I = 2; I >= 0; I--)
           if (s->actH[I]) s->H[I][c] = s->B[I][c] / (m->mu * mu0);
       }
+  soft_derived(s, 3, n);
   apply_boundaries(s, 1);
 }
 

@@ -27,6 +27,7 @@ extern "C" {
 
 #define RC_MAXPOLES 3
 #define RC_MAXMAT 256
+#define RC_MAXPLANE 6
 
 /* modes: which components exist (FDTDConstants.hpp:424-445, 844-874) */
 enum { RC_TEM = 1, RC_TM = 2, RC_3D = 3, RC_TE = 4 };
@@ -74,12 +75,18 @@ int rc_set_magnetized_drude(rc_sim* s, int id, double eps_r, double mu_r, double
 int rc_set_fluid_drude(rc_sim* s, int id, double mu_r, int magnetization, const double* bias);
 int rc_set_fluid_state(rc_sim* s, const double* ne, const double* nu_m);
 int rc_set_matid(rc_sim* s, const uint8_t* ids); /* nx*ny*nz, x fastest */
-/* soft source: adds table[n] to D_comp (field 0) or B_comp (field 1) at cell after the curl of step n */
+/* soft source: adds table[n] to D_comp (field 0) or B_comp (field 1) at cell after the curl of step n,
+ * or to E_comp (field 2) / H_comp (field 3) after the material step of step n */
 int rc_add_source(rc_sim* s, int field, int comp, int i, int j, int k, const double* table, int ntable);
 /* TFSF box; faces at x0,x1 (etc); kvec/pol/polH 3-vectors; origin (cell units); aux line of ncells;
  * signal[n] is the aux source value at its n-th step */
 int rc_set_tfsf(rc_sim* s, const int box[6], const double kvec[3], const double pol[3], const double polH[3],
                 const double origin[3], int aux_ncells, const double* signal, int nsignal);
+/* single TFSF plane with its own aux line: normal along axis (side 0 MIN: -1, 1 MAX: +1), cells
+ * [x0,x1)x[y0,y1)x[z0,z1); every flux but the normal one of every cell is corrected */
+int rc_add_tfsf_plane(rc_sim* s, int axis, int side, const int cells[6], const double kvec[3], const double pol[3],
+                      const double polH[3], const double origin[3], int aux_ncells, const double* signal,
+                      int nsignal);
 /* running DFT over a box region; comps bitmask (bit c: E_c for c<3, H_{c-3}); phases [nsteps][nfreq][2]
  * for E and for H (already multiplied by dt) */
 int rc_add_dft(rc_sim* s, const int region[6], int comp_mask, int nfreq, const double* phaseE,

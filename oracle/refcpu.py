@@ -50,6 +50,7 @@ def lib():
             ("rc_set_matid", _i, [_p, ctypes.c_char_p]),
             ("rc_add_source", _i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
             ("rc_set_tfsf", _i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
+            ("rc_add_tfsf_plane", _i, [_p, _i, _i, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
             ("rc_add_dft", _i, [_p, _ip, _i, _i, _dp, _dp, _i]),
             ("rc_get_dft", _i, [_p, _i, _dp]),
             ("rc_add_probe", _i, [_p, _i, _i, _i, _i, _i]),
@@ -184,16 +185,23 @@ class Sim:
         assert lib().rc_set_matid(self.h, ids.tobytes()) == 0
 
     def add_source(self, field, comp, i, j, k, table):
-        """field: 0 (D) or 2 (B), the field codes of this class; rc_add_source numbers them 0 / 1."""
-        assert field in (0, 2), field
+        """field: 0 (D), 2 (B), 1 (E) or 3 (H), the field codes of this class;
+        rc_add_source numbers them D 0, B 1, E 2, H 3."""
+        code = {0: 0, 2: 1, 1: 2, 3: 3}[field]
         t = _f64(table)
-        assert lib().rc_add_source(self.h, 0 if field == 0 else 1, comp, i, j, k, _dptr(t), t.size) == 0
+        assert lib().rc_add_source(self.h, code, comp, i, j, k, _dptr(t), t.size) == 0
 
     def set_tfsf(self, box, kvec, pol, polH, origin, aux_ncells, signal):
         b = (ctypes.c_int * 6)(*box)
         k, p, ph, o, s = (_f64(x) for x in (kvec, pol, polH, origin, signal))
         assert lib().rc_set_tfsf(self.h, b, _dptr(k), _dptr(p), _dptr(ph), _dptr(o), aux_ncells,
                                  _dptr(s), s.size) == 0
+
+    def add_tfsf_plane(self, axis, side, cells, kvec, pol, polH, origin, aux_ncells, signal):
+        c = (ctypes.c_int * 6)(*cells)
+        k, p, ph, o, s = (_f64(x) for x in (kvec, pol, polH, origin, signal))
+        assert lib().rc_add_tfsf_plane(self.h, axis, side, c, _dptr(k), _dptr(p), _dptr(ph), _dptr(o),
+                                       aux_ncells, _dptr(s), s.size) == 0
 
     def add_dft(self, region, comp_mask, phaseE, phaseH):
         pe, ph = _f64(phaseE), _f64(phaseH)

@@ -103,6 +103,20 @@ def from_golden(name):
         sc["fluid"] = (a["fluid_ne"].reshape(shape), a["fluid_nu"].reshape(shape))
     if int(p.get("src_i", -1)) >= 0:
         sc["sources"].append((D, 2, int(p["src_i"]), int(p["src_j"]), int(p.get("src_k", 0)), a["src"]))
+    if "soft_field_comp_i_j" in a:
+        for row, tab in zip(a["soft_field_comp_i_j"], a["soft_tables"]):
+            f, c, i, j = (int(v) for v in row)
+            sc["sources"].append((f, c, i, j, 0, tab))
+    if "planes_axis_side_layer_lo_hi_t0" in a:
+        kx, ky = p["kx"], p["ky"]
+        k = np.array([kx, ky]) / math.hypot(kx, ky)
+        sc["planes"] = []
+        for q, row in enumerate(a["planes_axis_side_layer_lo_hi_t0"]):
+            axis, side, layer, lo, hi = (int(v) for v in row[:5])
+            cells = [layer, layer + 1, lo, hi + 1, 0, 1] if axis == 0 else [lo, hi + 1, layer, layer + 1, 0, 1]
+            sc["planes"].append(dict(axis=axis, side=side, cells=cells, kvec=[kx, ky, 0.0], pol=[0.0, 0.0, 1.0],
+                                     polH=[k[1], -k[0], 0.0], origin=[p["cx"], p["cy"], 0.0],
+                                     aux_ncells=int(p["aux_ncells"]), signal=a[f"plane{q}_signal"]))
     if "probes" in a:
         for pr in a["probes"]:
             ijk = [int(v) for v in pr] + [0] * (3 - len(pr))
@@ -141,6 +155,9 @@ def apply(sim, sc):
     if sc.get("tfsf"):
         t = sc["tfsf"]
         sim.set_tfsf(t["box"], t["kvec"], t["pol"], t["polH"], t["origin"], t["aux_ncells"], t["signal"])
+    for t in sc.get("planes", ()):
+        sim.add_tfsf_plane(t["axis"], t["side"], t["cells"], t["kvec"], t["pol"], t["polH"], t["origin"],
+                           t["aux_ncells"], t["signal"])
     for pr in sc["probes"]:
         sim.add_probe(*pr)
     for d in sc.get("dfts", []):

@@ -296,3 +296,68 @@ def dft_monitor(region, mask, dt, nsteps, freqs=(0.02, 0.035)):
     phH = np.stack([np.cos(2 * np.pi * f[None, :] * (n[:, None] + 1.5) * dt) * dt,
                     -np.sin(2 * np.pi * f[None, :] * (n[:, None] + 1.5) * dt) * dt], -1)
     return (list(region), mask, phE, phH)
+
+
+def _plane(axis, side, cells, kvec, origin, aux_ncells, signal, pol=(0.0, 0.0, 1.0)):
+    """A single TFSF plane's arguments; polH = pol x k (the aux line's Ez, Hy pair)."""
+    k = np.asarray(kvec, float)
+    k = k / np.linalg.norm(k)
+    return dict(axis=axis, side=side, cells=list(cells), kvec=list(kvec), pol=list(pol),
+                polH=list(np.cross(np.asarray(pol, float), k)), origin=list(origin), aux_ncells=aux_ncells,
+                signal=signal)
+
+
+def grid3d_sources_planes(n=(40, 30, 26), nsteps=80, npml=5):
+    """3-D: soft sources on Ez (inside a Drude block), Ex and Hy (added after the
+    material steps) beside one on Dz, and two single TFSF planes (x-MIN, y-MAX
+    faces) under an oblique wave, each plane with its own line and pulse
+    delay; CPML on every face, a dielectric block, a DFT monitor and probes."""
+    nx, ny, nz = n
+    dx = 1e-8
+    dt = 0.5 * dx / (C0 * math.sqrt(3.0))
+    matid = np.zeros((nz, ny, nx), np.uint8)
+    matid[10:16, 17:23, 20:27] = 1
+    matid[14:20, 8:13, 12:17] = 2
+    mats = [(1.0, 1.0, []), (3.0, 1.0, []), (1.5, 1.0, [(1, 1.37e16, 1e14, 1.0)])]
+    f0 = C0 / (20 * dx)
+    tw = 2.0 / f0
+    kvec = [math.cos(0.5), math.sin(0.5) * math.cos(0.3), math.sin(0.5) * math.sin(0.3)]
+    origin = [20.5, 15.5, 13.5]
+    planes = [_plane(0, 0, [8, 9, 8, 22, 7, 19], kvec, origin, 90, gmsin_signal(nsteps, dt, f0, tw, 3.0 * tw)),
+              _plane(1, 1, [9, 33, 22, 23, 7, 19], kvec, origin, 90, gmsin_signal(nsteps, dt, f0, tw, 4.0 * tw))]
+    sources = [(E, 2, 14, 10, 16, gauss_table(nsteps, 20, 6, scale=1.0)),
+               (E, 0, 28, 20, 9, gauss_table(nsteps, 30, 7, scale=1.0)),
+               (H, 1, 17, 24, 12, gauss_table(nsteps, 25, 6, scale=1.0 / IMP0)),
+               (D, 2, 30, 8, 18, gauss_table(nsteps, 15, 5))]
+    probes = [(E, 2, 14, 10, 16), (E, 0, 28, 20, 9), (H, 1, 17, 24, 12), (E, 2, 8, 15, 13), (E, 2, 7, 15, 13),
+              (H, 2, 20, 22, 12), (E, 1, 20, 23, 12)]
+    pe, ph = dft_phases(nsteps, dt, np.array([0.8, 1.1]) * f0)
+    return {
+        "name": "grid3d_sources_planes", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
+        "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, True, True], "materials": mats,
+        "matid": matid, "sources": sources, "probes": probes, "tfsf": None, "planes": planes,
+        "dfts": [([6, 34, 5, 25, 13, 14], 63, pe, ph)],
+    }
+
+
+def grid3d_plane_slab(n=(48, 12, 10), nsteps=140, npml=8, faces=(0, 1), layers=(14, 33)):
+    """Vacuum, CPML along x only, periodic y and z; single TFSF planes with
+    normal x on the given sides (0 MIN at layers[0], 1 MAX at layers[1]),
+    each covering every interior y, z cell, k = x: an x-propagating plane
+    wave confined to the total-field slab between them."""
+    nx, ny, nz = n
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(3.0))
+    f0 = C0 / (20 * dx)
+    tw = 2.0 / f0
+    sig = gmsin_signal(nsteps, dt, f0, tw, 3.0 * tw)
+    origin = [0.5 * (layers[0] + layers[1]), ny / 2, nz / 2]
+    planes = [_plane(0, side, [layers[side], layers[side] + 1, 1, ny - 1, 1, nz - 1], [1.0, 0.0, 0.0], origin,
+                     nx + 20, sig) for side in faces]
+    return {
+        "name": "grid3d_plane_slab", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
+        "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, False, False],
+        "materials": [(1.0, 1.0, [])], "matid": np.zeros((nz, ny, nx), np.uint8), "sources": [], "probes": [],
+        "dfts": [], "tfsf": None, "planes": planes,
+        "boundaries": [(ax, sd, "Periodic") for ax in (1, 2) for sd in (0, 1)],
+    }

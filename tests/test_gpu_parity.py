@@ -52,7 +52,7 @@ def field_errors(g, o, sc):
 
 
 GOLDEN = ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small", "grid3d_disp", "tm2d_c1",
-          "tm2d_conductive", "grid3d_magnetized", "grid3d_fluid", "tm2d_fluid"]
+          "tm2d_conductive", "grid3d_magnetized", "grid3d_fluid", "tm2d_fluid", "tm2d_soft_planes"]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])

@@ -148,7 +148,7 @@ def test_cell_fluid():
 
 @pytest.mark.parametrize("name", ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small",
                                   "grid3d_disp", "tm2d_c1", "tm2d_conductive", "grid3d_magnetized",
-                                  "grid3d_fluid", "tm2d_fluid"])
+                                  "grid3d_fluid", "tm2d_fluid", "tm2d_soft_planes"])
 def test_grid_scenarios_bit_exact(name):
     sc, p, a = from_golden(name)
     sim = refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])

@@ -8,14 +8,18 @@ mathematics fixes rather than a retyped formula:
 * running DFT: every accumulator equals the brute-force sum of the stored
   probe trace of that cell times the phase table;
 * multi-pole ADE: two identical poles of half weight step like one pole of
-  full weight (the polarisation equations are linear in the weight).
+  full weight (the polarisation equations are linear in the weight);
+* single TFSF planes: a pair bounding a slab of a y/z-periodic vacuum grid
+  confines an x-propagating wave to the slab (round-off outside), one plane
+  alone injects forward only, and in TM the four face planes of a box step
+  bit for bit like the box.
 """
 import numpy as np
 import pytest
 
 import synth
 from oracle import refcpu
-from scenarios import E, H, apply, dft_phases, rel_l2
+from scenarios import E, H, apply, dft_phases, from_golden, rel_l2
 
 
 def _sim(sc):
@@ -86,3 +90,73 @@ def test_split_pole_equals_single_pole(kind):
             ref = a.get(f, c)
             assert np.max(np.abs(ref)) > 0
             assert rel_l2(b.get(f, c), ref) <= 1e-13
+
+
+def _slab_fields(faces):
+    sc = synth.grid3d_plane_slab(faces=faces)
+    sim = _sim(sc)
+    sim.step(sc["nsteps"])
+    return sc, sim
+
+
+def test_tfsf_plane_pair_confines_the_wave_to_the_slab():
+    """Two single planes (x-MIN at layer a, x-MAX at layer b, each with its own
+    line and every interior y, z cell) in a y/z-periodic vacuum grid, k = x:
+    outside the total-field slab [a, b] the fields stay at round-off, inside
+    the incident wave has passed.  A plausible slip (a normal sign, the wrong
+    aux field or offset, a plane corrected one layer off) leaks at O(1)."""
+    sc, sim = _slab_fields((0, 1))
+    a, b = 14, 33
+    npml = sc["pml"]["n"]
+    x = np.arange(sc["nx"])
+    out = ((x < a - 1) | (x > b + 1)) & (x > npml + 1) & (x < sc["nx"] - npml - 2)
+    inside = (x > a) & (x < b)
+    for f in (E, H):
+        tot = max(np.max(np.abs(sim.get(f, c)[:, :, inside])) for c in range(3))
+        leak = max(np.max(np.abs(sim.get(f, c)[:, :, out])) for c in range(3))
+        assert tot > 0
+        assert leak <= 1e-12 * tot, (f, leak / tot)
+    # the wave is uniform in y and z (periodic faces, full-width planes)
+    ez = sim.get(E, 2)[1:-1, 1:-1, :]
+    assert np.max(np.abs(ez - ez[:1, :1, :])) <= 1e-12 * np.max(np.abs(ez))
+
+
+def test_tfsf_single_plane_injects_forward_only():
+    """One x-MIN plane alone: the scattered-field side (x < a) stays at
+    round-off while the wave runs on through x > a into the CPML."""
+    sc, sim = _slab_fields((0,))
+    a = 14
+    npml = sc["pml"]["n"]
+    x = np.arange(sc["nx"])
+    behind = (x < a - 1) & (x > npml + 1)
+    ahead = x > a
+    for f in (E, H):
+        tot = max(np.max(np.abs(sim.get(f, c)[:, :, ahead])) for c in range(3))
+        leak = max(np.max(np.abs(sim.get(f, c)[:, :, behind])) for c in range(3))
+        assert tot > 0
+        assert leak <= 1e-12 * tot, (f, leak / tot)
+
+
+def test_plane_faces_equal_the_box_in_tm():
+    """In TM a box face's component ranges coincide with a plane over the
+    face's cells (x faces: rows by0+1..by1; y faces: columns bx0+1..bx1), so
+    the box and four single planes sharing its signal step alike."""
+    sc, p, a = from_golden("tm2d_tfsf_oblique")
+    t = sc["tfsf"]
+    box = t["box"]
+    sim_box = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], 1, sc["dx"], sc["dt"]), sc)
+    sc2 = dict(sc, tfsf=None, planes=[])
+    for axis in (0, 1):
+        for side in (0, 1):
+            layer = box[2 * axis + side]
+            o = 1 - axis
+            cells = [0, 0, 0, 0, 0, 1]
+            cells[2 * axis], cells[2 * axis + 1] = layer, layer + 1
+            cells[2 * o], cells[2 * o + 1] = box[2 * o] + 1, box[2 * o + 1] + 1
+            sc2["planes"].append(dict(axis=axis, side=side, cells=cells, kvec=t["kvec"], pol=t["pol"], polH=t["polH"],
+                                      origin=t["origin"], aux_ncells=t["aux_ncells"], signal=t["signal"]))
+    sim_pl = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], 1, sc["dx"], sc["dt"]), sc2)
+    sim_box.step(sc["nsteps"])
+    sim_pl.step(sc["nsteps"])
+    for f, c in ((E, 2), (H, 0), (H, 1)):
+        np.testing.assert_array_equal(sim_pl.get(f, c), sim_box.get(f, c))
