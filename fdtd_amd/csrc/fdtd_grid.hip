@@ -166,7 +166,7 @@ struct MatHost {
 struct AuxLine {
   int L = 0;
   double *state = nullptr, *pE = nullptr, *pH = nullptr, *sig = nullptr;
-  int nsig = 0;
+  int nsig = 0, sig_cap = 0;  // signal entries in use / allocated
   double *incE = nullptr, *incH = nullptr;
 };
 
@@ -217,9 +217,11 @@ struct fdtd_grid {
   void* Pd2[FDTD_MAX_POLES][3] = {};
   void* Jd2[FDTD_MAX_POLES][3] = {};
 
-  // sources (nsrc_eh of them on E or H)
+  // sources (nsrc_eh of them on E or H); their tables and allocated lengths
   int nsrc = 0, nsrc_eh = 0;
   fdtdk::SrcDev src[FDTD_MAX_SOURCES];
+  double* src_table[FDTD_MAX_SOURCES] = {};
+  int src_cap[FDTD_MAX_SOURCES] = {};
 
   // TFSF box
   bool tfsf = false;
@@ -1914,6 +1916,8 @@ int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, cons
   int rc = g->alloc((void**)&t, size_t(std::max(ntable, 1)) * 8);
   if (rc) return rc;
   if (ntable) HIP_TRY(hipMemcpy(t, table, size_t(ntable) * 8, hipMemcpyHostToDevice));
+  g->src_table[g->nsrc] = t;
+  g->src_cap[g->nsrc] = std::max(ntable, 1);
   fdtdk::SrcDev& s = g->src[g->nsrc++];
   s.cell = g->q(i, j, k);
   s.field = field;
@@ -1944,6 +1948,27 @@ int aux_line_init(fdtd_grid* g, int aux_ncells, const double* signal, int nsigna
   HIP_TRY(hipMemcpy(l.pH, h.data(), h.size() * 8, hipMemcpyHostToDevice));
   if (nsignal) HIP_TRY(hipMemcpy(l.sig, signal, size_t(nsignal) * 8, hipMemcpyHostToDevice));
   l.nsig = nsignal;
+  l.sig_cap = std::max(nsignal, 1);
+  return 0;
+}
+
+// Write values[0..n) as entries step0.. of a device table of *len entries in
+// use and *cap allocated, growing it (zero-filled, contents kept) when needed.
+int table_write(fdtd_grid* g, double** table, int* len, int* cap, long step0, const double* values, int n) {
+  const long end = step0 + n;
+  if (end > (1L << 30)) return fail(FDTD_EINVAL, "table longer than 2^30 steps");
+  if (end > *cap) {
+    const int ncap = int(std::max<long>(end, std::min<long>(2L * *cap, 1L << 30)));
+    double* t = nullptr;
+    int rc = g->alloc((void**)&t, size_t(ncap) * 8);
+    if (rc) return rc;
+    if (*len) HIP_TRY(hipMemcpy(t, *table, size_t(*len) * 8, hipMemcpyDeviceToDevice));
+    g->release(*table);
+    *table = t;
+    *cap = ncap;
+  }
+  if (n) HIP_TRY(hipMemcpy(*table + step0, values, size_t(n) * 8, hipMemcpyHostToDevice));
+  *len = int(std::max<long>(*len, end));
   return 0;
 }
 
@@ -1969,6 +1994,24 @@ int fdtd_set_tfsf(fdtd_grid* g, const int box[6], const double kvec[3], const do
     g->origin[d] = origin[d];
   }
   return aux_line_init(g, aux_ncells, signal, nsignal, g->aux);
+}
+
+int fdtd_set_signal(fdtd_grid* g, int kind, int index, long step0, const double* values, int n) {
+  if (!g || (!values && n) || n < 0 || step0 < 0) return fail(FDTD_EINVAL, "bad argument");
+  // queued steps may still read the table being rewritten or replaced
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (kind == FDTD_SIGNAL_SOURCE) {
+    if (index < 0 || index >= g->nsrc) return fail(FDTD_EINVAL, "no source %d", index);
+    fdtdk::SrcDev& s = g->src[index];
+    int rc = table_write(g, &g->src_table[index], &s.n, &g->src_cap[index], step0, values, n);
+    gput(s.table, g->src_table[index]);
+    return rc;
+  }
+  AuxLine* l = nullptr;
+  if (kind == FDTD_SIGNAL_TFSF && index == 0 && g->tfsf) l = &g->aux;
+  if (kind == FDTD_SIGNAL_PLANE && index >= 0 && index < g->nplane) l = &g->plane_aux[index];
+  if (!l) return fail(FDTD_EINVAL, "no incident line of kind %d, index %d", kind, index);
+  return table_write(g, &l->sig, &l->nsig, &l->sig_cap, step0, values, n);
 }
 
 int fdtd_add_tfsf_plane(fdtd_grid* g, int axis, int side, const int cells[6], const double kvec[3],

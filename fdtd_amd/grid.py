@@ -25,7 +25,7 @@ BOUNDARY = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "
 # every symbol declared in include/fdtd_hip.h
 ABI_SYMBOLS = [
     "fdtd_grid_create", "fdtd_grid_destroy", "fdtd_set_stream", "fdtd_set_pml_axis", "fdtd_set_pml_idiom",
-    "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_tfsf_plane", "fdtd_add_dft", "fdtd_get_dft",
+    "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_tfsf_plane", "fdtd_set_signal", "fdtd_add_dft", "fdtd_get_dft",
     "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune", "fdtd_set_adjoint_tile",
@@ -73,6 +73,7 @@ def load_library():
         "fdtd_add_source": (_i, [_p, _i, _i, _i, _i, _i, _dp, _i]),
         "fdtd_set_tfsf": (_i, [_p, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
         "fdtd_add_tfsf_plane": (_i, [_p, _i, _i, _ip, _dp, _dp, _dp, _dp, _i, _dp, _i]),
+        "fdtd_set_signal": (_i, [_p, _i, _i, ctypes.c_long, _dp, _i]),
         "fdtd_add_dft": (_i, [_p, _ip, _i, _i, _dp, _dp, _i, _ip]),
         "fdtd_get_dft": (_i, [_p, _i, _dp]),
         "fdtd_add_probe": (_i, [_p, _i, _i, _i, _i, _i]),
@@ -218,6 +219,14 @@ class Grid:
         k, p, ph, o, s = (_f64(x) for x in (kvec, pol, polH, origin, signal))
         _check(load_library().fdtd_add_tfsf_plane(self.h, axis, side, c, _dptr(k), _dptr(p), _dptr(ph), _dptr(o),
                                                   aux_ncells, _dptr(s), s.size))
+
+    SIGNAL = {"source": 0, "tfsf": 1, "plane": 2}
+
+    def set_signal(self, kind, index, step0, values):
+        """Write values as the entries of steps step0.. of a source's table
+        (kind "source"), the TFSF box's signal ("tfsf") or a plane's ("plane")."""
+        v = _f64(values)
+        _check(load_library().fdtd_set_signal(self.h, self.SIGNAL[kind], index, step0, _dptr(v), v.size))
 
     def add_dft(self, region, comp_mask, phaseE, phaseH):
         pe, ph = _f64(phaseE), _f64(phaseH)

@@ -19,7 +19,11 @@
 //   MagnetizedFluidDrudeUpdate<Mode,ft>(Bx,By,Bz,dt), SelfMagnetizedFluidDrudeUpdate<Mode,ft>(dt)  2014-2021
 //   make_soft_source<Dz>(signal, cell)              make_soft_source<Dz>(signal, grid.cell(i,j,k))  Source.hpp:258-261
 //   Aux1D<double>(ncells,dx,dt,signal)              Aux1D<double>(ncells,dx,dt,signal)               Source.hpp:280-408
-//   make_tfsf_plane<Mode,Dir,Orient>(b,e,loc,k,aux) make_tfsf_box<Mode>(grid,box,k,aux)              Source.hpp:624-634
+//   make_tfsf_plane<Mode,Dir,Orient>(b,e,loc,k,aux) make_tfsf_plane<Mode,Dir,Orient>(grid,cells,origin,k,aux)  Source.hpp:624-634
+//     (2*dim faces, one history)                    make_tfsf_box<Mode>(grid,box,k,aux)
+//   plane.updateD(dt,dx) / updateB(dt,dx)           same, inside a for_each loop     Source.hpp:595-620
+//   src.apply(t)                                    same, inside a for_each loop     Source.hpp:225-229
+//   std::for_each(cells.begin(), cells.end(), F)    for_each(grid, F), for_each(grid.cells(id), M)  testTM.cpp:102-113
 //   make_pec_boundary<Mode,Dir,Orient>(b,e)         make_pec_boundary<Mode,Dir,Orient>(grid)         BoundaryUpdates.hpp:527-565
 //     (pmc, symmetric, antisymmetric, periodic)     same names; periodic image fixed (layer L-2 / 1)  BoundaryUpdates.hpp:578-832
 //   BoundaryData::updateE() / updateH()             Stepper::boundary(bd): ghost writes after E / H   BoundaryUpdates.hpp:340-362
@@ -29,14 +33,18 @@
 // Instead of sweeping each functor over a user container, the functors are
 // handed once to a Stepper, which checks that they form the reference's step
 // composition (D curl, sources, CPML E, TFSF, E material, B curl, CPML H,
-// TFSF, H material) and then runs whole steps on the GPU.  Errors from the C
+// TFSF, H material) and then runs whole steps on the GPU; or the reference's
+// loop is kept, one for_each(grid, F) per sweep, each recording a stage, the
+// recorded steps running on the GPU many at once.  Errors from the C
 // ABI surface as fdtd::hip::Error exceptions on the host side only.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <map>
 #include <memory>
@@ -113,6 +121,188 @@ struct PMLParameterModel {
   double A(double x) const { return x < 0.0 ? 0.0 : (x > 1.0 ? 0.0 : std::pow(1.0 - x, mMa) * maMax); }
 };
 
+// material updates: one table entry each (eps_r or mu_r plus poles)
+struct MaterialSpec {
+  double eps = 1.0, mu = 1.0;
+  std::vector<double> poles;  // kind, freq, gamma, weight per pole
+  int mag = 0;                // magnetized Drude: 1 fixed bias, 2 the cell's own B
+  double bias[3] = {0, 0, 0};
+  bool fluid = false;         // fluid Drude: omega_p, gamma per cell (YeeGrid::setFluidState)
+};
+
+namespace detail {
+// MaterialMap equivalent (include/MaterialMap.hpp:204-284): id -> material.
+inline void set_material(fdtd_grid* h, int id, const MaterialSpec& s) {
+  if (s.fluid) {
+    check(fdtd_set_fluid_drude(h, id, s.mu, s.mag, s.mag == 1 ? s.bias : nullptr), "fdtd_set_fluid_drude");
+    return;
+  }
+  if (s.mag) {
+    check(fdtd_set_magnetized_drude(h, id, s.eps, s.mu, s.poles[1], s.poles[2], s.mag == 1 ? s.bias : nullptr),
+          "fdtd_set_magnetized_drude");
+    return;
+  }
+  check(fdtd_set_material(h, id, s.eps, s.mu, int(s.poles.size() / 4), s.poles.empty() ? nullptr : s.poles.data()),
+        "fdtd_set_material");
+}
+
+inline std::string spec_text(const MaterialSpec& s) {
+  std::string t;
+  char b[32];
+  for (double v : {s.eps, s.mu, double(s.mag), s.bias[0], s.bias[1], s.bias[2], double(s.fluid)}) {
+    snprintf(b, sizeof b, "%.17g,", v);
+    t += b;
+  }
+  for (double v : s.poles) {
+    snprintf(b, sizeof b, "%.17g,", v);
+    t += b;
+  }
+  return t;
+}
+
+// Stages of the reference's step composition (tests/testTM.cpp:102-113,
+// tests/test3D.cpp:146-171, include/Source.hpp:413-435), in their order.
+enum Stage : int { kCurlD = 0, kPmlE = 1, kTfsfD = 2, kMatE = 3, kCurlB = 4, kPmlH = 5, kTfsfB = 6, kMatH = 7 };
+
+// The recorded time loop of one grid (for_each).  Each call of the loop names
+// one stage of the step; a step is complete when its H material stage has
+// been seen and the next step starts (or the grid is read).  The first step's
+// list of stages is the loop's composition: every later step must repeat it.
+// The steps run on the device, many at once, when the grid is read, at
+// flush(), or every flush_every steps; the per-step values of the soft
+// sources the loop applies and the incident lines' signals go to the device
+// with them.
+class Loop {
+ public:
+  int flush_every = 1024;
+  int pml_axes = 0;      // axes with a CPML profile (their UpdatePML stages must be in the loop)
+  int nsources = 0;      // sources of the grid (the loop's and others), the next one's index
+  int nplanes = 0;       // single TFSF planes of the grid, likewise
+  fdtd_grid* h = nullptr;
+  std::function<void()> before_run;  // the grid's pending set-up (CPML profiles)
+
+  // one stage of the current step, named by `tag` (the functor's signature)
+  void event(int stage, const std::string& tag) {
+    if (stage == kCurlD && stage_ >= kMatH) {
+      end_step();
+      if (pending_ >= flush_every) flush();
+    }
+    if (stage_ < 0 && stage != kCurlD) throw Error("a step of the time loop starts with the D curl (YeeUpdate of D)");
+    if (stage < stage_) throw Error(tag + " out of the reference's step order");
+    stage_ = stage;
+    now_.push_back(tag);
+  }
+  // SoftSource::apply inside the loop: v is this step's value.  D and B
+  // sources belong between their curl and their material stage, E right
+  // after the E material stage, H right after the H material stage.
+  void source(int field, int comp, int i, int j, int k, const std::shared_ptr<int>& id, double v) {
+    const bool ok = field == FDTD_D   ? stage_ >= kCurlD && stage_ < kMatE
+                    : field == FDTD_B ? stage_ >= kCurlB && stage_ < kMatH
+                    : field == FDTD_E ? stage_ == kMatE
+                                      : stage_ == kMatH;
+    if (!ok)
+      throw Error("SoftSource applied outside its place in the step (D, B: after their curl, before the material "
+                  "step; E, H: right after their material step)");
+    if (*id < 0) {
+      const double none = 0.0;
+      check(fdtd_add_source(h, field, comp, i, j, k, &none, 0), "fdtd_add_source");
+      *id = nsources++;
+    }
+    auto& q = vals_[*id];
+    q.resize(size_t(pending_ + 1), 0.0);
+    q[size_t(pending_)] += v;
+    now_.push_back("src" + std::to_string(*id));
+  }
+  // a TFSF box (kind FDTD_SIGNAL_TFSF) or plane (FDTD_SIGNAL_PLANE) whose
+  // updateD (stage kTfsfD) or updateB (kTfsfB) the loop calls; `add`
+  // registers it on first use and returns its index
+  void line(int kind, const std::shared_ptr<int>& id, const std::function<int()>& add,
+            const std::function<double(double)>& sig, double dt, int stage) {
+    if (*id < 0) {
+      if (stage != kTfsfD) throw Error("a TFSF box or plane's updateD comes before its updateB");
+      *id = add();
+      lines_.push_back({kind, *id, 0.0, dt, sig});
+    }
+    event(stage, "tfsf" + std::to_string(kind) + ":" + std::to_string(*id));
+  }
+  void material(int id, int ft, const MaterialSpec& m) {
+    event(ft ? kMatH : kMatE, "mat" + std::to_string(ft) + ":" + std::to_string(id) + ":" + spec_text(m));
+    if (!have_first_) (ft ? hmat_ : emat_)[id] = m;
+  }
+  // complete steps not yet run
+  long pending() const { return pending_ + (stage_ >= kMatH ? 1 : 0); }
+  // run the complete steps (reading the grid in the middle of a step is an error)
+  void flush() {
+    if (stage_ >= 0 && stage_ < kMatH) throw Error("the grid is read in the middle of a step of the time loop");
+    if (stage_ >= kMatH) end_step();
+    if (pending_ == 0) return;
+    if (before_run) before_run();
+    if (configure_) {
+      configure();
+      configure_ = false;
+    }
+    const long n0 = fdtd_step_count(h);
+    for (auto& kv : vals_) {
+      kv.second.resize(size_t(pending_), 0.0);
+      check(fdtd_set_signal(h, FDTD_SIGNAL_SOURCE, kv.first, n0, kv.second.data(), int(pending_)), "fdtd_set_signal");
+      kv.second.clear();
+    }
+    for (auto& l : lines_) {
+      // the line advances its time before sampling the signal (Source.hpp:414)
+      std::vector<double> v(static_cast<size_t>(pending_));
+      for (auto& x : v) {
+        l.t += l.dt;
+        x = l.sig(l.t);
+      }
+      check(fdtd_set_signal(h, l.kind, l.index, n0, v.data(), int(pending_)), "fdtd_set_signal");
+    }
+    check(fdtd_step(h, int(pending_)), "fdtd_step");
+    pending_ = 0;
+  }
+
+ private:
+  struct Line {
+    int kind, index;
+    double t, dt;
+    std::function<double(double)> sig;
+  };
+  void end_step() {
+    if (!have_first_) {
+      first_ = now_;
+      have_first_ = true;
+      configure_ = true;
+    } else if (now_ != first_) {
+      throw Error("every step of the time loop must repeat the first step's functors, sources and values");
+    }
+    now_.clear();
+    stage_ = -1;
+    pending_++;
+  }
+  // set up the device from the first step: its materials (the electric model
+  // and the magnetic mu_r per id); CPML profiles need their UpdatePML stages
+  void configure() {
+    for (int a = 0; a < 3; a++)
+      if ((pml_axes >> a) & 1)
+        for (int ft = 0; ft < 2; ft++)
+          if (std::find(first_.begin(), first_.end(), "pml" + std::to_string(ft) + std::to_string(a)) == first_.end())
+            throw Error("the grid has a CPML profile along an axis whose UpdatePML is not in the time loop");
+    std::map<int, MaterialSpec> all = emat_;
+    for (auto& kv : hmat_) {
+      if (!kv.second.poles.empty()) throw Error("magnetic dispersion has no device counterpart");
+      all[kv.first].mu = kv.second.mu;
+    }
+    for (auto& kv : all) set_material(h, kv.first, kv.second);
+  }
+  int stage_ = -1;
+  long pending_ = 0;
+  bool have_first_ = false, configure_ = false;
+  std::vector<std::string> now_, first_;
+  std::map<int, MaterialSpec> emat_, hmat_;
+  std::map<int, std::vector<double>> vals_;  // the loop's sources: values of the pending steps
+  std::vector<Line> lines_;
+};
+}  // namespace detail
+
 // ------------------------------------------------------------------- grid
 template <class Mode, class scalar = double>
 class YeeGrid;
@@ -152,6 +342,13 @@ struct CellRef {
   void setPMLParameters(double K, double S, double A, double dt) const;
 };
 
+// the cells of one material id of a grid (YeeGrid::cells)
+template <class Mode, class scalar>
+struct GridCells {
+  YeeGrid<Mode, scalar>* grid;
+  int id;
+};
+
 template <class Mode, class scalar>
 class YeeGrid {
  public:
@@ -166,8 +363,22 @@ class YeeGrid {
         ksa_[ft][a].assign(3 * L, 0.0);
         for (int i = 0; i < L; i++) ksa_[ft][a][3 * i] = 1.0;
       }
+    loop_.h = g;
+    loop_.before_run = [this]() { commit_pml(); };
   }
   YeeGrid(int nx, int ny, double dx, double dt) : YeeGrid(nx, ny, 1, dx, dt) {}
+  YeeGrid(const YeeGrid&) = delete;
+  YeeGrid& operator=(const YeeGrid&) = delete;
+
+  // the recorded time loop (for_each); flush() runs its complete steps,
+  // which every read of the grid does first
+  detail::Loop& loop() const { return loop_; }
+  void flush() const { loop_.flush(); }
+  // steps run so far plus the loop's complete steps not yet run
+  long steps() const { return fdtd_step_count(handle()) + loop_.pending(); }
+  void check_dt(double dt, double dx) const {
+    if (dt != dt_ || dx != dx_) throw Error("functor dt/dx differ from the grid's");
+  }
 
   fdtd_grid* handle() const { return g_.get(); }
   int nx() const { return nx_; }
@@ -177,6 +388,8 @@ class YeeGrid {
   double dt() const { return dt_; }
 
   CellRef<Mode, scalar> cell(int i, int j, int k = 0) { return {this, i, j, k}; }
+  // the cells of one material id: the range a material functor sweeps
+  GridCells<Mode, scalar> cells(int id) { return {this, id}; }
 
   // NewPML::setPMLParameters for every cell whose coordinate along `d` is idx
   template <FieldType ft, Dir d>
@@ -186,12 +399,14 @@ class YeeGrid {
     v[3 * idx + 1] = S;
     v[3 * idx + 2] = A;
     pml_dirty_[int(d)] = true;
+    if (S != 0.0 || K != 1.0 || A != 0.0) loop_.pml_axes |= 1 << int(d);
   }
 
   // the reference's placement idiom on one axis (tests/testTM.cpp:46-71)
   template <Dir d>
   void placePML(const PMLParameterModel& p, int n) {
     check(fdtd_set_pml_idiom(handle(), int(d), n, p.mM, p.mMa, p.msMax, p.mkMax, p.maMax), "fdtd_set_pml_idiom");
+    if (n > 0) loop_.pml_axes |= 1 << int(d);
   }
 
   // record one E or H component at a cell after every step
@@ -203,6 +418,7 @@ class YeeGrid {
   }
   // [steps][probes]
   std::vector<double> probes() const {
+    flush();
     long n = fdtd_step_count(handle());
     std::vector<double> v(size_t(std::max(1L, n)) * std::max(1, nprobes_));
     int got = 0;
@@ -213,6 +429,7 @@ class YeeGrid {
 
   // the imaginary part of the probes of a complex grid (a Bloch-periodic face)
   std::vector<double> probes_imag() const {
+    flush();
     long n = fdtd_step_count(handle());
     std::vector<double> v(size_t(std::max(1L, n)) * std::max(1, nprobes_));
     int got = 0;
@@ -227,6 +444,7 @@ class YeeGrid {
   // of every existing component to the host; sync_device copies the mirror's
   // E and H back.  Cell accessors read and write the mirror only.
   void sync_host() {
+    flush();
     const int fields[6] = {FDTD_E, FDTD_H, FDTD_D, FDTD_B, FDTD_P, FDTD_J};
     for (int f : fields)
       for (int c = 0; c < 3; c++) {
@@ -252,11 +470,13 @@ class YeeGrid {
   }
 
   std::vector<double> download(int field, int comp, int pole = 0) const {
+    flush();
     std::vector<double> v(size_t(nx_) * ny_ * nz_);
     check(fdtd_download_field(handle(), field, comp, pole, v.data()), "fdtd_download_field");
     return v;
   }
   void upload(int field, int comp, const std::vector<double>& v) {
+    flush();
     check(fdtd_upload_field(handle(), field, comp, 0, v.data()), "fdtd_upload_field");
   }
 
@@ -289,6 +509,7 @@ class YeeGrid {
   bool pml_dirty_[3] = {false, false, false};
   int nprobes_ = 0;
   std::vector<double> mirror_[6][3];  // [field code][component]
+  mutable detail::Loop loop_;
 };
 
 template <class Mode, class scalar>
@@ -356,14 +577,7 @@ struct UpdatePML {
   static constexpr int stage = ft == FieldType::Electric ? 1 : 5;
 };
 
-// material updates: one table entry each (eps_r or mu_r plus poles)
-struct MaterialSpec {
-  double eps = 1.0, mu = 1.0;
-  std::vector<double> poles;  // kind, freq, gamma, weight per pole
-  int mag = 0;                // magnetized Drude: 1 fixed bias, 2 the cell's own B
-  double bias[3] = {0, 0, 0};
-  bool fluid = false;         // fluid Drude: omega_p, gamma per cell (YeeGrid::setFluidState)
-};
+// material updates: one table entry each (eps_r or mu_r plus poles), MaterialSpec above
 template <class Mode, FieldType ft, bool forward = false>
 struct VacuumUpdate {
   MaterialSpec spec() const { return {}; }
@@ -479,17 +693,7 @@ struct SelfMagnetizedFluidDrudeUpdate {
 // MaterialMap equivalent (include/MaterialMap.hpp:204-284): id -> material.
 template <class Mode, class scalar>
 void set_material(YeeGrid<Mode, scalar>& g, int id, const MaterialSpec& s) {
-  if (s.fluid) {
-    check(fdtd_set_fluid_drude(g.handle(), id, s.mu, s.mag, s.mag == 1 ? s.bias : nullptr), "fdtd_set_fluid_drude");
-    return;
-  }
-  if (s.mag) {
-    check(fdtd_set_magnetized_drude(g.handle(), id, s.eps, s.mu, s.poles[1], s.poles[2], s.mag == 1 ? s.bias : nullptr),
-          "fdtd_set_magnetized_drude");
-    return;
-  }
-  check(fdtd_set_material(g.handle(), id, s.eps, s.mu, int(s.poles.size() / 4), s.poles.empty() ? nullptr : s.poles.data()),
-        "fdtd_set_material");
+  detail::set_material(g.handle(), id, s);
 }
 
 // MaterialPair / MaterialMap (include/MaterialMap.hpp:45-284): a named pair
@@ -513,17 +717,24 @@ inline MaterialSpec combine(const std::string& name, const MaterialPair& p) {
 }
 
 // ---------------------------------------------------------------- sources
-// SoftSource (include/Source.hpp:183-261): adds signal(t) to the cell's flux.
+// SoftSource (include/Source.hpp:183-261): adds signal(t) to one field of a
+// cell: D or B (after their curl: a lasting source) or E or H (after their
+// material step: acting on one curl of the other field, Source.hpp:421-425).
+// Handed to a Stepper, its signal is sampled once per step up front; called
+// as apply(t) / (t) inside a time loop of for_each calls, signal(t) is that
+// step's value, as in the reference's loops.
 template <class Mode, class scalar>
 struct SoftSource {
   std::function<double(double)> signal;
   CellRef<Mode, scalar> loc;
   int field, comp;
+  std::shared_ptr<int> id = std::make_shared<int>(-1);  // the device source, once the loop has added it
   double at(double t) const { return signal(t); }
+  void apply(double t) const { loc.grid->loop().source(field, comp, loc.i, loc.j, loc.k, id, signal(t)); }
+  void operator()(double t) const { apply(t); }
 };
 template <class F, class Signal, class Mode, class scalar>
 SoftSource<Mode, scalar> make_soft_source(Signal sig, CellRef<Mode, scalar> c) {
-  static_assert(F::field == FDTD_D || F::field == FDTD_B, "soft sources act on D or B (before the material step)");
   return {std::function<double(double)>(sig), c, F::field, F::comp};
 }
 
@@ -554,34 +765,119 @@ struct Aux1D {
   Aux1D(int n, double deltax, double deltat, Signal sig) : ncells(n), dx(deltax), dt(deltat), signal(sig) {}
 };
 
-// TFSF box of 2*dim make_tfsf_plane faces sharing one incident line.
-// Polarisations default to the reference's rule for the mode
-// (include/Source.hpp:573-584; ThreeD, which the reference leaves unset,
-// gets pol = z and polH = pol x k, consistent with the line's Ez/Hy).
+namespace detail {
+// Polarisations by the reference's rule for the mode (include/Source.hpp:
+// 573-584): TM pol = z, polH = (k_y, -k_x, 0).  ThreeD, which the reference
+// leaves unset, gets pol = z and polH = pol x k = (-k_y, k_x, 0).  (The TM
+// rule's polH is the negative of pol x k: with it the incident field the
+// faces subtract is not the line's own plane wave, so the scattered-field
+// region is not left empty; the rule is kept for parity with the reference,
+// and the explicit-polarisation overloads take pol x k instead.)
+inline void default_pol(int dim, const std::array<double, 3>& k, std::array<double, 3>& pol,
+                        std::array<double, 3>& polH) {
+  pol = {0, 0, 1};
+  if (dim == 2)
+    polH = {k[1], -k[0], 0};
+  else
+    polH = {-k[1], k[0], 0};
+}
+inline std::array<double, 3> unit(int dim, const std::array<double, 3>& kvec) {
+  double K = 0;
+  for (int d = 0; d < dim; d++) K += kvec[d] * kvec[d];
+  K = std::sqrt(K);
+  std::array<double, 3> k{};
+  for (int d = 0; d < 3; d++) k[d] = d < dim ? kvec[d] / K : 0.0;
+  return k;
+}
+}  // namespace detail
+
+// TFSF box of 2*dim make_tfsf_plane faces sharing one incident line.  Handed
+// to a Stepper (tfsf), or stepped by updateD / updateB inside a time loop.
 template <class Mode>
 struct TFSFBox {
   std::array<int, 6> box;
   std::array<double, 3> k, pol, polH, origin;
   std::shared_ptr<Aux1D<double>> aux;
+  detail::Loop* loop = nullptr;
+  std::shared_ptr<int> id = std::make_shared<int>(-1);
+  void updateD(double, double) const { step(detail::kTfsfD); }
+  void updateB(double, double) const { step(detail::kTfsfB); }
+
+ private:
+  void step(int stage) const {
+    TFSFBox b = *this;
+    loop->line(FDTD_SIGNAL_TFSF, id, [b]() {
+      check(fdtd_set_tfsf(b.loop->h, b.box.data(), b.k.data(), b.pol.data(), b.polH.data(), b.origin.data(),
+                          b.aux->ncells, nullptr, 0), "fdtd_set_tfsf");
+      return 0;
+    }, aux->signal, aux->dt, stage);
+  }
 };
 template <class Mode, class scalar>
 TFSFBox<Mode> make_tfsf_box(YeeGrid<Mode, scalar>& g, std::array<int, 6> box, std::array<double, 3> kvec,
                             std::shared_ptr<Aux1D<double>> aux) {
   TFSFBox<Mode> t;
   t.box = box;
-  double K = 0;
-  for (int d = 0; d < Mode::dim; d++) K += kvec[d] * kvec[d];
-  K = std::sqrt(K);
-  for (int d = 0; d < 3; d++) t.k[d] = d < Mode::dim ? kvec[d] / K : 0.0;
-  t.pol = {0, 0, 1};
-  if (Mode::dim == 2)
-    t.polH = {t.k[1], -t.k[0], 0};
-  else
-    t.polH = {-t.k[1], t.k[0], 0};  // z x k
+  t.k = detail::unit(Mode::dim, kvec);
+  detail::default_pol(Mode::dim, t.k, t.pol, t.polH);
   for (int d = 0; d < 3; d++) t.origin[d] = d < Mode::dim ? std::floor(0.5 * (box[2 * d] + box[2 * d + 1])) + 0.5 : 0.0;
   t.aux = aux;
-  (void)g;
+  t.loop = &g.loop();
   return t;
+}
+
+// One TFSF plane with its own incident line: make_tfsf_plane<Mode, Dir,
+// Orient>(begin, end, loc, k, aux) (include/Source.hpp:624-634) with the
+// iterator range given as the cells [x0,x1)x[y0,y1)x[z0,z1) of the grid (one
+// layer thick along Dir) and the location functor as r = (cell - origin) dx.
+// Handed to a Stepper (tfsf_plane), or stepped by updateD / updateB inside a
+// time loop (TFSFPlane::updateD / updateB, 595-620).
+template <class Mode>
+struct TFSFPlane {
+  int axis, side;
+  std::array<int, 6> cells;
+  std::array<double, 3> k, pol, polH, origin;
+  std::shared_ptr<Aux1D<double>> aux;
+  detail::Loop* loop = nullptr;
+  std::shared_ptr<int> id = std::make_shared<int>(-1);
+  void updateD(double, double) const { step(detail::kTfsfD); }
+  void updateB(double, double) const { step(detail::kTfsfB); }
+  int add(fdtd_grid* h, const double* sig, int nsig) const {
+    check(fdtd_add_tfsf_plane(h, axis, side, cells.data(), k.data(), pol.data(), polH.data(), origin.data(),
+                              aux->ncells, sig, nsig), "fdtd_add_tfsf_plane");
+    return loop->nplanes++;
+  }
+
+ private:
+  void step(int stage) const {
+    TFSFPlane p = *this;
+    loop->line(FDTD_SIGNAL_PLANE, id, [p]() { return p.add(p.loop->h, nullptr, 0); }, aux->signal, aux->dt, stage);
+  }
+};
+template <class Mode, Dir d, Orientation o, class scalar>
+TFSFPlane<Mode> make_tfsf_plane(YeeGrid<Mode, scalar>& g, std::array<int, 6> cells, std::array<double, 3> origin,
+                                std::array<double, 3> kvec, std::shared_ptr<Aux1D<double>> aux) {
+  static_assert(int(d) < Mode::dim, "the plane's normal must be an axis of the mode");
+  TFSFPlane<Mode> p;
+  p.axis = int(d);
+  p.side = int(o);
+  p.cells = cells;
+  p.k = detail::unit(Mode::dim, kvec);
+  detail::default_pol(Mode::dim, p.k, p.pol, p.polH);
+  p.origin = origin;
+  p.aux = aux;
+  p.loop = &g.loop();
+  return p;
+}
+// the same with explicit polarisations (pol multiplies the line's E, polH its H)
+template <class Mode, Dir d, Orientation o, class scalar>
+TFSFPlane<Mode> make_tfsf_plane(YeeGrid<Mode, scalar>& g, std::array<int, 6> cells, std::array<double, 3> origin,
+                                std::array<double, 3> kvec, std::array<double, 3> pol, std::array<double, 3> polH,
+                                std::shared_ptr<Aux1D<double>> aux) {
+  TFSFPlane<Mode> p = make_tfsf_plane<Mode, d, o>(g, cells, origin, kvec, aux);
+  p.pol = pol;
+  p.polH = polH;
+  return p;
 }
 
 // ---------------------------------------------------------------- boundaries
@@ -673,6 +969,10 @@ class Stepper {
     tfsf_.reset(new TFSFBox<Mode>(t));
     return *this;
   }
+  Stepper& tfsf_plane(const TFSFPlane<Mode>& p) {
+    planes_.push_back(p);
+    return *this;
+  }
   // ghost-layer writes of one face after every E and H update (BoundaryData::updateE / updateH)
   Stepper& boundary(const BoundaryData& b) {
     bcs_.push_back(b);
@@ -737,20 +1037,26 @@ class Stepper {
       for (size_t q = 0; q < tab.size(); q++) tab[q] = s.at(double(q) * unit);
       check(fdtd_add_source(g_.handle(), s.field, s.comp, s.loc.i, s.loc.j, s.loc.k, tab.data(), int(tab.size())),
             "fdtd_add_source");
+      g_.loop().nsources++;
     }
+    for (auto& p : planes_) p.add(g_.handle(), line_signal(*p.aux, n0 + n).data(), int(n0 + n));
     if (tfsf_) {
-      // the aux line advances its time before sampling the signal (Source.hpp:414)
-      std::vector<double> sig(size_t(n0 + n));
-      double t = 0;
-      for (auto& v : sig) {
-        t += tfsf_->aux->dt;
-        v = tfsf_->aux->signal(t);
-      }
+      std::vector<double> sig = line_signal(*tfsf_->aux, n0 + n);
       check(fdtd_set_tfsf(g_.handle(), tfsf_->box.data(), tfsf_->k.data(), tfsf_->pol.data(), tfsf_->polH.data(),
                           tfsf_->origin.data(), tfsf_->aux->ncells, sig.data(), int(sig.size())),
             "fdtd_set_tfsf");
     }
     committed_ = true;
+  }
+  // the line's signal for steps [0, n): it advances its time before sampling (Source.hpp:414)
+  static std::vector<double> line_signal(const Aux1D<double>& aux, long n) {
+    std::vector<double> sig(static_cast<size_t>(n));
+    double t = 0;
+    for (auto& v : sig) {
+      t += aux.dt;
+      v = aux.signal(t);
+    }
+    return sig;
   }
   YeeGrid<Mode, scalar>& g_;
   int stage_ = 0;
@@ -758,8 +1064,53 @@ class Stepper {
   bool committed_ = false;
   std::vector<SoftSource<Mode, scalar>> sources_;
   std::unique_ptr<TFSFBox<Mode>> tfsf_;
+  std::vector<TFSFPlane<Mode>> planes_;
   std::vector<BoundaryData> bcs_;
 };
+
+// ------------------------------------------------------------ time loop
+// for_each(grid, functor): the reference's sweep of one functor over the
+// grid's cells, std::for_each(cells.begin(), cells.end(), F(...))
+// (tests/testTM.cpp:102-113), so that the reference's time loop reads
+//
+//   for (int t = 0; t < n; t++) {
+//     for_each(grid, YeeUpdate<TM, Dz>(dt, dx));
+//     src.apply(t);                                      // SoftSource on Dz
+//     for_each(grid, UpdatePML<TM, FieldType::Electric, Dir::X>(dt, dx));
+//     ...
+//     plane.updateD(dt, dx);                             // TFSF plane or box
+//     for_each(grid.cells(1), ConstantUpdate<TM, FieldType::Electric>(4.0));
+//     for_each(grid, YeeUpdate<TM, Bx>(dt, dx));
+//     ...
+//     for_each(grid, VacuumUpdate<TM, FieldType::Magnetic>());
+//   }
+//
+// Each call records one stage of the step on the grid's loop; complete steps
+// run on the device as fused passes, many steps at once, when the grid is
+// read (probes, download, sync_host), at flush(), or every
+// loop().flush_every steps.  Every step must repeat the first one's
+// functors, sources and values.  A material functor sweeps every cell
+// (material id 0) or grid.cells(id); CPML runs where profiles are set, whose
+// axes need their UpdatePML stages in the loop.
+template <class Mode, class scalar, class Flux>
+void for_each(YeeGrid<Mode, scalar>& g, const YeeUpdate<Mode, Flux>& u) {
+  g.check_dt(u.dt, u.dx);
+  g.loop().event(u.stage, "curl" + std::to_string(Flux::field) + std::to_string(Flux::comp));
+}
+template <class Mode, class scalar, FieldType ft, Dir d>
+void for_each(YeeGrid<Mode, scalar>& g, const UpdatePML<Mode, ft, d>& u) {
+  static_assert(int(d) < Mode::dim, "UpdatePML along an axis the mode does not have");
+  g.check_dt(u.dt, u.dx);
+  g.loop().event(u.stage, "pml" + std::to_string(int(ft)) + std::to_string(int(d)));
+}
+template <class Mode, class scalar, template <class, FieldType, bool> class M, FieldType ft, bool fwd>
+void for_each(YeeGrid<Mode, scalar>& g, const M<Mode, ft, fwd>& m) {
+  g.loop().material(0, int(ft), m.spec());
+}
+template <class Mode, class scalar, template <class, FieldType, bool> class M, FieldType ft, bool fwd>
+void for_each(GridCells<Mode, scalar> c, const M<Mode, ft, fwd>& m) {
+  c.grid->loop().material(c.id, int(ft), m.spec());
+}
 
 }  // namespace hip
 }  // namespace fdtd

@@ -166,6 +166,20 @@ int fdtd_add_tfsf_plane(fdtd_grid* g, int axis, int side, const int cells[6], co
                         const double pol[3], const double polH[3], const double origin[3], int aux_ncells,
                         const double* signal, int nsignal);
 
+/* Rewrite or extend a per-step table between steps, for a time loop that
+ * computes each step's value as it goes (SoftSource::apply(t) in the loop,
+ * include/Source.hpp:225-229; the signal Aux1D::time_step samples, 414-425):
+ * values[q] becomes the entry of step step0 + q of
+ *   FDTD_SIGNAL_SOURCE: the table of source `index` (sources are numbered
+ *                       0, 1, ... in the order of fdtd_add_source calls),
+ *   FDTD_SIGNAL_TFSF:   the TFSF box's signal (index 0),
+ *   FDTD_SIGNAL_PLANE:  the signal of plane `index` (order of
+ *                       fdtd_add_tfsf_plane calls).
+ * The table grows as needed; entries never written are 0.  Waits for the
+ * grid's queued steps first. */
+enum fdtd_signal { FDTD_SIGNAL_SOURCE = 0, FDTD_SIGNAL_TFSF = 1, FDTD_SIGNAL_PLANE = 2 };
+int fdtd_set_signal(fdtd_grid* g, int kind, int index, long step0, const double* values, int n);
+
 /* Running DFT over the half-open region [i0,i1)x[j0,j1)x[k0,k1) (no
  * counterpart in the reference: include/DataCapture.hpp:61-191 is empty).
  * comp_mask bit c selects E_c (c<3) or H_{c-3}.  phaseE/phaseH:

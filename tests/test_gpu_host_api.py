@@ -129,3 +129,22 @@ def test_cell_proxies_through_cpp_host_surface():
     assert abs(hy - o.get(H, 1)[0, 18, 20]) <= 1e-12 * np.abs(o.get(H, 1)).max()
     assert abs(ez - ez_o[22, 26]) <= 1e-12 * np.abs(ez_o).max()
     assert abs(dz - 2.0 * eps0 * ez) <= 1e-12 * abs(dz)
+
+
+def test_for_each_time_loop_against_reference_fixture():
+    """The reference's own loop shape through for_each: SoftSource on Ez and Hy
+    applied inside the loop after their material steps, two single TFSF planes
+    stepped by updateD / updateB, material functors per id, 170 steps flushed
+    to the device in runs of 64: the probe traces equal the reference's own
+    (golden fixture tm2d_soft_planes, from the reference's templates)."""
+    exe = os.path.join(os.path.dirname(BIN), "tm_loop_host_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/bin/tm_loop_host_api not built (run __graft_entry__.build())")
+    p, a = load("tm2d_soft_planes")
+    print("[gpu-case] tm_loop_host_api", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
+    ref = a["probe_trace_Ez_Hx_Hy"].reshape(got.shape)
+    assert np.abs(ref).max() > 0
+    assert rel_l2(got, ref) <= 1e-12

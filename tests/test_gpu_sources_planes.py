@@ -122,3 +122,27 @@ def test_refusals():
     from fdtd_amd import Batch
     with pytest.raises(FdtdError, match="E / H sources"):
         Batch([a])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_signals_written_between_steps(precision):
+    """Sources and plane signals given as empty tables and written run by run
+    with fdtd_set_signal (the tables grow as they go) step exactly like the
+    same grid given whole tables up front."""
+    sc = synth.grid3d_sources_planes(nsteps=70)
+    n = sc["nsteps"]
+    print(f"[gpu-case] signals between steps {precision}", flush=True)
+    whole = _grid(sc, precision)
+    whole.step(n)
+    sc2 = dict(sc, sources=[s[:5] + (np.zeros(0),) for s in sc["sources"]],
+               planes=[dict(p, signal=np.zeros(0)) for p in sc["planes"]])
+    g = _grid(sc2, precision)
+    for s0, s1 in ((0, 5), (5, 33), (33, 34), (34, 70)):
+        for q, s in enumerate(sc["sources"]):
+            g.set_signal("source", q, s0, s[5][s0:s1])
+        for q, p in enumerate(sc["planes"]):
+            g.set_signal("plane", q, s0, p["signal"][s0:s1])
+        g.step(s1 - s0)
+    for f, c in COMPS:
+        np.testing.assert_array_equal(g.get(f, c), whole.get(f, c), err_msg=f"{f}{c}")
+    np.testing.assert_array_equal(g.probes(), whole.probes())
