@@ -160,3 +160,24 @@ def test_plane_faces_equal_the_box_in_tm():
     sim_pl.step(sc["nsteps"])
     for f, c in ((E, 2), (H, 0), (H, 1)):
         np.testing.assert_array_equal(sim_pl.get(f, c), sim_box.get(f, c))
+
+
+def test_tm_reference_default_polarisation_leaks_and_pol_cross_k_does_not():
+    """The reference's TM default polH = (k_y, -k_x, 0) (include/Source.hpp:
+    577-580), reproduced bit for bit by the golden fixtures, is the negative of
+    pol x k: the faces then remove the opposite of the line's wave and the
+    scattered-field region carries O(1) of the incident amplitude.  With
+    polH = pol x k the same box leaves it at round-off."""
+    sc, p, a = from_golden("tm2d_tfsf_x")
+    box = sc["tfsf"]["box"]
+    res = {}
+    for name, polH in (("reference", [0.0, -1.0, 0.0]), ("pol x k", [0.0, 1.0, 0.0])):
+        sc["tfsf"]["polH"] = polH
+        sim = apply(refcpu.Sim("TM", sc["nx"], sc["ny"], 1, sc["dx"], sc["dt"]), sc)
+        sim.step(sc["nsteps"])
+        ez = sim.get(E, 2)[0]
+        inside = np.abs(ez[box[2] + 1:box[3], box[0] + 1:box[1]]).max()
+        far = np.concatenate([ez[11:box[2] - 1, 11:-11].ravel(), ez[box[3] + 2:-11, 11:-11].ravel()])
+        res[name] = np.abs(far).max() / inside
+    assert res["reference"] > 0.1, res
+    assert res["pol x k"] < 1e-12, res
