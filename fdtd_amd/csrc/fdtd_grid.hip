@@ -14,6 +14,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <chrono>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -119,7 +120,8 @@ constexpr int kNumKernels = 8;
 // bx == 0 selects the streamed fused pass (3-D FP32) with E regions of `by`
 // rows and `kc` workgroups per compute unit, bx == 1 the wave-fused single
 // pass with `by` rows per wave; bx >= 2 the two passes with bx x by threads
-// per workgroup
+// per workgroup.  TM grids: bx == 0 selects the time-blocked pass, `by` steps
+// per launch on cores of kc x kc cells.
 struct Tile {
   int bx, by, kc;
 };
@@ -1019,18 +1021,28 @@ int step_impl(fdtd_grid* g, int nsteps) {
   fdtdk::StepArgs<T> a;
   fill_args<T>(g, a);
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
-  if (g->ndim == 2) t.kc = 1;
   // design increments and boundary ghost writes are made by the two-pass kernels
   // (E / H sources and single planes too: the general instance of the passes)
   const bool general = g->nbc || g->nsrc_eh || g->nplane;
-  if ((g->force_two_pass || general || has_magnetized(g)) && t.bx <= 1) t = kDefaultTile3D;
+  // the time-blocked TM pass covers non-dispersive grids without those, complex
+  // fields or DFT monitors
+  const bool tmb = g->ndim == 2 && t.bx == 0 && !general && !g->cplx && g->max_np == 0 && g->ndft == 0 &&
+                   !g->force_two_pass;
+  if (g->ndim == 2 && !tmb) {
+    if (t.bx == 0) t = kDefaultTileTM;
+    t.kc = 1;
+  }
+  if ((g->force_two_pass || general || has_magnetized(g)) && t.bx <= 1 && g->ndim == 3) t = kDefaultTile3D;
   // the streamed pass is FP32, addresses fields with 32-bit buffer offsets and
   // keeps the z coefficients in LDS
-  if (t.bx == 0 && (sizeof(T) != 4 || a.field_bytes == 0 || g->d.nz > fdtdk::kSMaxZ || g->design)) t = kDefaultTile3D;
+  if (g->ndim == 3 && t.bx == 0 && (sizeof(T) != 4 || a.field_bytes == 0 || g->d.nz > fdtdk::kSMaxZ || g->design))
+    t = kDefaultTile3D;
 
   const bool wavef = g->ndim == 3 && t.bx == 1;
   const bool fused = g->ndim == 3 && t.bx <= 1;
-  if (fused) {
+  // the fused and time-blocked passes write the second copy of the state
+  const bool pp = fused || tmb;
+  if (pp) {
     int rc = ensure_pingpong(g);
     if (rc) return rc;
   }
@@ -1061,6 +1073,13 @@ int step_impl(fdtd_grid* g, int nsteps) {
     nkc = (nzi + t.kc - 1) / t.kc;
     block = dim3(t.bx, t.by, 1);
     grid = dim3(ntx * nty * nkc, 1, 1);
+  } else if (tmb) {
+    // cores of kc x kc cells over the whole grid (the outer layer is copied)
+    ntx = (g->d.nx + t.kc - 1) / t.kc;
+    nty = (g->d.ny + t.kc - 1) / t.kc;
+    const int R = t.kc + 2 * t.by;
+    block = dim3(unsigned((R * R + 63) / 64 * 64), 1, 1);
+    grid = dim3(unsigned(ntx * nty), 1, 1);
   } else {
     block = dim3(t.bx, t.by, 1);
     grid = dim3((g->d.nx + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, 1);
@@ -1092,10 +1111,10 @@ int step_impl(fdtd_grid* g, int nsteps) {
     const auto* dev0 = reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + size_t(slot) * 2 * fdtd_grid::kArgsBytes);
     const auto* dev1 =
         reinterpret_cast<const fdtdk::StepArgs<T>*>(g->args_dev + (size_t(slot) * 2 + 1) * fdtd_grid::kArgsBytes);
-    set_state<T>(g, a, fused);
+    set_state<T>(g, a, pp);
     *host = a;
     gput(host->self, dev0);
-    if (fused) {
+    if (pp) {
       swap_state(g);
       set_state<T>(g, a, true);
       *host1 = a;
@@ -1112,7 +1131,24 @@ int step_impl(fdtd_grid* g, int nsteps) {
     // the chunk's launches; every step-dependent value comes from the staged
     // arguments and the step index s, so the sequence can be captured once
     // and replayed for later chunks of the same shape
+    const int nlaunch = tmb ? (chunk + t.by - 1) / t.by : chunk;  // launches that swap the state roles
     auto enqueue = [&]() -> int {
+      if (tmb) {
+        // launch l reads copy l % 2 and writes the other
+        for (int l = 0; l < nlaunch; l++) {
+          const auto* ap = (l % 2) ? dev1 : dev0;
+          const int s = l * t.by, nst = std::min(t.by, chunk - s);
+          int rc;
+          switch (t.by) {
+            case 2: rc = launch(g, 3, fdtdk::tm_block<T, 2>, grid, block, ap, t.kc, s, nst, ntx); break;
+            case 4: rc = launch(g, 3, fdtdk::tm_block<T, 4>, grid, block, ap, t.kc, s, nst, ntx); break;
+            case 6: rc = launch(g, 3, fdtdk::tm_block<T, 6>, grid, block, ap, t.kc, s, nst, ntx); break;
+            default: rc = launch(g, 3, fdtdk::tm_block<T, 8>, grid, block, ap, t.kc, s, nst, ntx); break;
+          }
+          if (rc) return rc;
+        }
+        return 0;
+      }
       for (int s = 0; s < chunk; s++) {
         int rc;
         const auto* ap = (fused && (s % 2)) ? dev1 : dev0;
@@ -1229,8 +1265,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
       if (rc) return rc;
     }
     // host-side effects of the chunk
-    if (fused && (chunk % 2)) swap_state(g);
-    if (fused) g->nswap += chunk - (chunk % 2);
+    if (pp && (nlaunch % 2)) swap_state(g);
+    if (pp) g->nswap += nlaunch - (nlaunch % 2);
     if (dft_work > 0) {
       const long last = ((step0 + chunk) / fdtdk::kDftBlock) * fdtdk::kDftBlock;  // last block end in the chunk
       if (last > step0) g->dft_done = last;
@@ -2243,7 +2279,11 @@ long fdtd_step_count(const fdtd_grid* g) { return g ? g->step : -1; }
 
 int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
-  if (bx == 0) {
+  if (bx == 0 && g->ndim == 2) {
+    if ((by != 2 && by != 4 && by != 6 && by != 8) || kc < 1 || kc + 2 * by > fdtdk::kTmMaxRegion)
+      return fail(FDTD_EINVAL, "the time-blocked TM schedule (bx = 0) takes by in {2, 4, 6, 8} steps per launch and "
+                               "cores of kc x kc cells with kc + 2 by <= %d", fdtdk::kTmMaxRegion);
+  } else if (bx == 0) {
     if (g->ndim != 3 || by != fdtdk::kSRows || kc < 1 || kc > 2)
       return fail(FDTD_EINVAL,
                   "the streamed fused schedule (bx = 0) is 3-D only, with by = %d E-region rows and kc in 1..2 "
@@ -2278,10 +2318,11 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   static const Tile cand3[] = {{0, 5, 1},  {0, 5, 2},  {256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {256, 1, 8},
                                {128, 2, 2}, {128, 2, 4}, {64, 4, 1}, {64, 4, 2}, {1, 2, 8}, {1, 2, 16},
                                {1, 4, 8},   {1, 4, 16}};
-  static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1}};
+  static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1},
+                               {0, 2, 16},  {0, 4, 16},  {0, 4, 24}, {0, 8, 16}, {0, 4, 8}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;
   const int ncand = g->ndim == 3 ? int(sizeof cand3 / sizeof cand3[0]) : int(sizeof cand2 / sizeof cand2[0]);
-  if ((rc = ensure_trace(g, g->step + long(ncand) * (trial_steps + 1)))) return rc;
+  if ((rc = ensure_trace(g, g->step + long(ncand) * (2 * trial_steps + 1)))) return rc;
   if (g->ndim == 3 && (rc = ensure_pingpong(g))) return rc;
   GridSnapshot snap;
   if ((rc = snapshot_take(g, snap))) return rc;
@@ -2296,15 +2337,26 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   for (int c = 0; c < ncand && !rc; c++) {
     if (g->ndim == 3 && cand[c].bx == 0 && g->f64) continue;  // the streamed pass is FP32 only
     g->tile = cand[c];
-    if ((rc = fdtd_step(g, 1))) break;
-    if ((rc = fdtd_set_profiling(g, 1))) break;
-    if ((rc = fdtd_step(g, trial_steps))) break;
-    double t = 0;  // device time of the step's kernels per step
-    for (int kid : {0, 1, 3, 4}) {
-      double ms = 0;
-      long nl = 0;
-      if ((rc = fdtd_kernel_time(g, kid, &ms, &nl))) break;
-      t += ms / trial_steps;
+    double t = 0;  // time per step
+    if (g->ndim == 2) {
+      // TM steps are bound by launches: wall time of a run after one of the
+      // same length (which captures its graphs, if any)
+      if ((rc = fdtd_step(g, trial_steps))) break;
+      if ((rc = fdtd_synchronize(g))) break;
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((rc = fdtd_step(g, trial_steps))) break;
+      if ((rc = fdtd_synchronize(g))) break;
+      t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial_steps;
+    } else {
+      if ((rc = fdtd_step(g, 1))) break;
+      if ((rc = fdtd_set_profiling(g, 1))) break;
+      if ((rc = fdtd_step(g, trial_steps))) break;
+      for (int kid : {0, 1, 3, 4}) {  // device time of the step's kernels
+        double ms = 0;
+        long nl = 0;
+        if ((rc = fdtd_kernel_time(g, kid, &ms, &nl))) break;
+        t += ms / trial_steps;
+      }
     }
     if (rc) break;
     if (t < best) {

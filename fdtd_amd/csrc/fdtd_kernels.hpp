@@ -1986,6 +1986,7 @@ __global__ __launch_bounds__(kSThreads) void fused3d_lds(const StepArgs<float>* 
 // single TFSF planes).
 template <typename T, bool GEN = false>
 __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
+#pragma clang fp contract(off)
   const StepArgs<T>& a = *ap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -2006,21 +2007,21 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
   const T dzx = hy - hy_xm;  // J=x, +1
   const T dzy = hx - hx_ym;  // J=y, -1
   T d[3] = {T(0), T(0), T(0)};
-  d[2] = a.cf[0][0][i] * dzx - a.cf[0][1][j] * dzy;
+  d[2] = fmadd(a.cf[0][0][i], dzx, -(a.cf[0][1][j] * dzy));
   if (a.nsrc) add_sources(a, step, 0, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, 0);
     const T p = a.psi[0][2][0][o];
-    d[2] += a.cg[0][0][i] * p;
-    a.psio[0][2][0][o] = a.cb[0][0][i] * p + a.cc[0][0][i] * dzx;
+    d[2] = fmadd(a.cg[0][0][i], p, d[2]);
+    a.psio[0][2][0][o] = fmadd(a.cb[0][0][i], p, a.cc[0][0][i] * dzx);
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
     const long o = slab_off(a, 1, syi, i, j, 0);
     const T p = a.psi[0][2][1][o];
-    d[2] += a.cg[0][1][j] * p;
-    a.psio[0][2][1][o] = a.cb[0][1][j] * p - a.cc[0][1][j] * dzy;
+    d[2] = fmadd(a.cg[0][1][j], p, d[2]);
+    a.psio[0][2][1][o] = fmadd(a.cb[0][1][j], p, -(a.cc[0][1][j] * dzy));
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
@@ -2029,7 +2030,7 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
   if (GEN && a.nplane) plane_cells<T, 2>(a, step, 0, i, j, 0, 4, 3, d);
   T ezn = ez;
   if (ie > T(0)) {
-    ezn = ez + d[2] * ie;
+    ezn = fmadd(d[2], ie, ez);
     a.Eo[2][c] = ezn;
   } else if (ie < T(0) && a.Dd[2]) {
     const MatDev<T>& m = a.mats[mid];
@@ -2053,6 +2054,7 @@ __global__ __launch_bounds__(256) void e_pass_tm(const StepArgs<T>* __restrict__
 
 template <typename T, bool GEN = false>
 __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__ ap, int, int s) {
+#pragma clang fp contract(off)
   const StepArgs<T>& a = *ap;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -2072,29 +2074,29 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   const T dxy = ez_yp - ez;  // Hx, J=y, +1
   const T dyx = ez_xp - ez;  // Hy, J=x, -1
   T d[3] = {T(0), T(0), T(0)};
-  d[0] = -a.cf[1][1][j] * dxy;
+  d[0] = -(a.cf[1][1][j] * dxy);
   d[1] = a.cf[1][0][i] * dyx;
   if (a.nsrc) add_sources(a, step, 2, c, d);
   const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
   if (sxi >= 0) {
     const long o = slab_off(a, 0, sxi, i, j, 0);
     const T p = a.psi[1][1][0][o];
-    d[1] -= a.cg[1][0][i] * p;
-    a.psio[1][1][0][o] = a.cb[1][0][i] * p - a.cc[1][0][i] * dyx;
+    d[1] = fmadd(-a.cg[1][0][i], p, d[1]);
+    a.psio[1][1][0][o] = fmadd(a.cb[1][0][i], p, -(a.cc[1][0][i] * dyx));
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
     const long o = slab_off(a, 1, syi, i, j, 0);
     const T p = a.psi[1][0][1][o];
-    d[0] -= a.cg[1][1][j] * p;
-    a.psio[1][0][1][o] = a.cb[1][1][j] * p + a.cc[1][1][j] * dxy;
+    d[0] = fmadd(-a.cg[1][1][j], p, d[0]);
+    a.psio[1][0][1][o] = fmadd(a.cb[1][1][j], p, a.cc[1][1][j] * dxy);
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
     const bool aF[3] = {true, true, false}, aC[3] = {false, false, true};
     tfsf_cell<T, 2>(a, a.incE + (long)s * a.auxL, 1, i, j, 0, aF, aC, (const T*)nullptr, d);
   }
   if (GEN && a.nplane) plane_cells<T, 2>(a, step, 1, i, j, 0, 3, 4, d);
-  T h[3] = {hx + d[0] * im, hy + d[1] * im, T(0)};
+  T h[3] = {fmadd(d[0], im, hx), fmadd(d[1], im, hy), T(0)};
   if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, h);
   a.Ho[0][c] = h[0];
   a.Ho[1][c] = h[1];
@@ -2102,6 +2104,224 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   if (a.nprobe) {
     const T e[3] = {T(0), T(0), ez};
     dft_probe(a, step, c, i, j, 0, e, h);
+  }
+}
+
+// ---------------------------------------------------------------- 2-D TM, time-blocked
+// One workgroup per C x C core of the grid advances it TB steps per launch:
+// it loads the core and a halo of TB cells on every side (the state at the
+// launch's first step), keeps one cell per thread in registers, passes the
+// neighbour values of each half step through LDS, and writes the core into
+// the other copy of the state.  Each step's E update consumes one halo layer
+// on the low side (it reads H at -x, -y) and the H update one on the high
+// side (E at +x, +y), so after TB steps the core is exact; each step updates
+// only the cells the core still depends on.  The redundant halo work buys TB
+// steps per launch on a grid that fits in the caches and whose one-pass steps
+// are bound by launch latency.
+// Grids with dispersive cells, boundary conditions, E / H sources, single
+// planes, complex fields or DFT monitors take the two passes.  The cell
+// arithmetic is that of e_pass_tm / h_pass_tm, operation for operation.
+constexpr int kTmMaxRegion = 32;  // C + 2 TB <= 32: one thread per region cell, 1024 threads
+constexpr int kTmMaxSteps = 8;
+constexpr int kTmRows = 3072;  // incident-row values (both fields, TB steps) staged in LDS at most
+template <typename T, int TB>
+__global__ __launch_bounds__(1024) void tm_block(const StepArgs<T>* __restrict__ ap, int core, int s0, int nst,
+                                                 int ntx) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = *ap;
+  __shared__ T sx[kTmMaxRegion * kTmMaxRegion];  // Hx
+  __shared__ T sy[kTmMaxRegion * kTmMaxRegion];  // Hy
+  __shared__ T se[kTmMaxRegion * kTmMaxRegion];  // Ez
+  // the launch's source values, [source][step], their lengths and components
+  __shared__ T sval[kMaxSrc][TB];
+  __shared__ int slen[kMaxSrc], scomp[kMaxSrc];
+  // the launch's incident rows: Hy for the D faces, Ez for the B faces
+  __shared__ double rowH[kTmRows / 2], rowE[kTmRows / 2];
+  const int R = core + 2 * TB;
+  const int t = threadIdx.x;
+  const bool live = t < R * R;
+  const int rx = live ? t % R : 0, ry = live ? t / R : 0;
+  const int bx = blockIdx.x % ntx, by = blockIdx.x / ntx;
+  const int i = bx * core - TB + rx, j = by * core - TB + ry;
+  const bool ingrid = live && i >= 0 && j >= 0 && i < a.nx && j < a.ny;
+  const bool inner = ingrid && i >= 1 && j >= 1 && i < a.nx - 1 && j < a.ny - 1;
+  const long c = (long)i + a.sy * j;
+  // stage the step-dependent tables of the launch (no global loads in the step loop)
+  const long stepA = a.step0 + s0;
+  for (int q = t; q < a.nsrc * TB; q += blockDim.x) {
+    const int sidx = q / TB, k = q % TB;
+    const SrcDev& sd = a.src[sidx];
+    sval[sidx][k] = (k < nst && stepA + k < sd.n) ? (T)sd.table[stepA + k] : T(0);
+    if (k == 0) {
+      slen[sidx] = sd.n;
+      scomp[sidx] = sd.comp;
+    }
+  }
+  const bool rows_lds = a.tfsf && a.auxL * nst <= kTmRows / 2;
+  if (rows_lds)
+    for (int q = t; q < a.auxL * nst; q += blockDim.x) {
+      rowH[q] = a.incH[(long)s0 * a.auxL + q];
+      rowE[q] = a.incE[(long)s0 * a.auxL + q];
+    }
+  T ez = T(0), hx = T(0), hy = T(0);
+  if (ingrid) {
+    ez = a.E[2][c];
+    hx = a.H[0][c];
+    hy = a.H[1][c];
+  }
+  // the cell's coefficients and CPML state
+  T ie = T(0), im = T(0), fEx = T(0), fEy = T(0), fHx = T(0), fHy = T(0);
+  const int sxi = inner ? slab_index(i, a.slab_a[0], a.slab_b[0]) : -1;
+  const int syi = inner ? slab_index(j, a.slab_a[1], a.slab_b[1]) : -1;
+  long ox = 0, oy = 0;
+  T pEx = T(0), pEy = T(0), pHx = T(0), pHy = T(0);  // psi of Dz (x, y slabs), of By (x slab), Bx (y slab)
+  T gEx = T(0), bEx = T(0), cEx = T(0), gEy = T(0), bEy = T(0), cEy = T(0);
+  T gHx = T(0), bHx = T(0), cHx = T(0), gHy = T(0), bHy = T(0), cHy = T(0);
+  unsigned smD = 0, smB = 0;  // the D / B sources of this cell (bit per source)
+  if (inner) {
+    const int mid = a.mat[c];
+    ie = cell_inv_eps(a, a.mie, mid, i, j, 0);
+    im = a.mim[mid];
+    fEx = a.cf[0][0][i];
+    fEy = a.cf[0][1][j];
+    fHx = a.cf[1][0][i];
+    fHy = a.cf[1][1][j];
+    if (sxi >= 0) {
+      ox = slab_off(a, 0, sxi, i, j, 0);
+      pEx = a.psi[0][2][0][ox];
+      pHx = a.psi[1][1][0][ox];
+      gEx = a.cg[0][0][i];
+      bEx = a.cb[0][0][i];
+      cEx = a.cc[0][0][i];
+      gHx = a.cg[1][0][i];
+      bHx = a.cb[1][0][i];
+      cHx = a.cc[1][0][i];
+    }
+    if (syi >= 0) {
+      oy = slab_off(a, 1, syi, i, j, 0);
+      pEy = a.psi[0][2][1][oy];
+      pHy = a.psi[1][0][1][oy];
+      gEy = a.cg[0][1][j];
+      bEy = a.cb[0][1][j];
+      cEy = a.cc[0][1][j];
+      gHy = a.cg[1][1][j];
+      bHy = a.cb[1][1][j];
+      cHy = a.cc[1][1][j];
+    }
+    for (int q = 0; q < a.nsrc; q++)
+      if (a.src[q].cell == c) {
+        if (a.src[q].field == 0) smD |= 1u << q;
+        if (a.src[q].field == 2) smB |= 1u << q;
+      }
+  }
+  const bool owner = inner && rx >= TB && ry >= TB && rx < TB + core && ry < TB + core;
+  // the probes this thread records: bit p of mEz / mHx / mHy
+  unsigned mEz = 0, mHx = 0, mHy = 0;
+  if (owner)
+    for (int p = 0; p < a.nprobe; p++)
+      if (a.probe[p].cell == c) {
+        const int f = a.probe[p].field, cp = a.probe[p].comp;
+        if (f == 1 && cp == 2) mEz |= 1u << p;
+        if (f == 3 && cp == 0) mHx |= 1u << p;
+        if (f == 3 && cp == 1) mHy |= 1u << p;
+      }
+  const bool tfsf = inner && a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3]);
+  __syncthreads();  // the staged tables
+  for (int q = 0; q < nst; q++) {
+    const int s = s0 + q;
+    const long step = a.step0 + s;
+    // the cells whose new values the core still depends on: E on [lo, hi],
+    // H on [lo, hi - 1] along each axis (the dependency cone of the core
+    // after the launch's remaining steps; lo >= 1 and hi <= R - 1, so the
+    // neighbours read are in the region)
+    const int lo = TB - (nst - 1 - q), hi = TB + core + (nst - 1 - q);
+    // E half step: Ez from Hx, Hy and their -y, -x neighbours
+    if (live) {
+      sx[t] = hx;
+      sy[t] = hy;
+    }
+    s_bar_lds();
+    if (inner && rx >= lo && ry >= lo && rx <= hi && ry <= hi) {
+      const T dzx = hy - sy[t - 1];
+      const T dzy = hx - sx[t - R];
+      T d[3] = {T(0), T(0), T(0)};
+      d[2] = fmadd(fEx, dzx, -(fEy * dzy));
+      for (unsigned m = smD; m; m &= m - 1) {  // add_sources, from the staged values
+        const int sq = __builtin_ctz(m);
+        if (step < slen[sq]) d[2] += sval[sq][q];
+      }
+      if (sxi >= 0) {
+        d[2] = fmadd(gEx, pEx, d[2]);
+        pEx = fmadd(bEx, pEx, cEx * dzx);
+      }
+      if (syi >= 0) {
+        d[2] = fmadd(gEy, pEy, d[2]);
+        pEy = fmadd(bEy, pEy, -(cEy * dzy));
+      }
+      if (tfsf) {
+        const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
+        if (rows_lds)
+          tfsf_cell<T, 2, false, const double*>(a, rowH + q * a.auxL, 0, i, j, 0, aF, aC, (const T*)nullptr, d);
+        else
+          tfsf_cell<T, 2>(a, a.incH + (long)s * a.auxL, 0, i, j, 0, aF, aC, (const T*)nullptr, d);
+      }
+      if (ie > T(0)) ez = fmadd(d[2], ie, ez);
+    }
+    if (live) se[t] = ez;
+    s_bar_lds();
+    // H half step: Hx, Hy from Ez and its +y, +x neighbours (the next step
+    // writes sx, sy only after every thread has passed this barrier, that
+    // is after every E update read them; se likewise after the next one)
+    if (inner && rx >= lo && ry >= lo && rx < hi && ry < hi) {
+      const T dxy = se[t + R] - ez;
+      const T dyx = se[t + 1] - ez;
+      T d[3] = {T(0), T(0), T(0)};
+      d[0] = -(fHy * dxy);
+      d[1] = fHx * dyx;
+      for (unsigned m = smB; m; m &= m - 1) {
+        const int sq = __builtin_ctz(m);
+        if (step < slen[sq]) {
+          if (scomp[sq] == 0)
+            d[0] += sval[sq][q];
+          else
+            d[1] += sval[sq][q];
+        }
+      }
+      if (sxi >= 0) {
+        d[1] = fmadd(-gHx, pHx, d[1]);
+        pHx = fmadd(bHx, pHx, -(cHx * dyx));
+      }
+      if (syi >= 0) {
+        d[0] = fmadd(-gHy, pHy, d[0]);
+        pHy = fmadd(bHy, pHy, cHy * dxy);
+      }
+      if (tfsf) {
+        const bool aF[3] = {true, true, false}, aC[3] = {false, false, true};
+        if (rows_lds)
+          tfsf_cell<T, 2, false, const double*>(a, rowE + q * a.auxL, 1, i, j, 0, aF, aC, (const T*)nullptr, d);
+        else
+          tfsf_cell<T, 2>(a, a.incE + (long)s * a.auxL, 1, i, j, 0, aF, aC, (const T*)nullptr, d);
+      }
+      hx = fmadd(d[0], im, hx);
+      hy = fmadd(d[1], im, hy);
+    }
+    for (unsigned m = mEz; m; m &= m - 1) a.trace[step * a.nprobe + __builtin_ctz(m)] = (double)ez;
+    for (unsigned m = mHx; m; m &= m - 1) a.trace[step * a.nprobe + __builtin_ctz(m)] = (double)hx;
+    for (unsigned m = mHy; m; m &= m - 1) a.trace[step * a.nprobe + __builtin_ctz(m)] = (double)hy;
+  }
+  // the core, into the other copy of the state (the outer layer unchanged)
+  if (ingrid && rx >= TB && ry >= TB && rx < TB + core && ry < TB + core) {
+    a.Eo[2][c] = ez;
+    a.Ho[0][c] = hx;
+    a.Ho[1][c] = hy;
+    if (sxi >= 0) {
+      a.psio[0][2][0][ox] = pEx;
+      a.psio[1][1][0][ox] = pHx;
+    }
+    if (syi >= 0) {
+      a.psio[0][2][1][oy] = pEy;
+      a.psio[1][0][1][oy] = pHy;
+    }
   }
 }
 

@@ -398,3 +398,55 @@ def test_fluid_state_changes_between_steps():
     g.set_matid(sc["matid"])
     with pytest.raises(FdtdError, match="fluid"):
         g.step(1)
+
+
+TM_BLOCK_TILES = [(0, 2, 16), (0, 4, 16), (0, 8, 16), (0, 4, 24), (0, 4, 8), (0, 6, 20)]
+
+
+@pytest.mark.parametrize("name", ["tm2d_c1", "tm2d_tfsf_oblique", "tm2d_small"])
+def test_tm_time_blocked_pass(name):
+    """The time-blocked TM pass (TB steps per launch on kc x kc cores with a TB
+    halo, ping-pong state) gives the two-pass bits on the reference's TM
+    scenarios (CPML, D source, probes; the TFSF box; tm2d_small has dispersive
+    blocks, which run the two passes): chunks of odd length, graph capture."""
+    sc, p, a = from_golden(name)
+    n = sc["nsteps"]
+    ref = apply(_grid(sc, "f64"), sc)
+    ref.step(n)
+    o = apply(_oracle(sc), sc)
+    o.step(n)
+    for nm, f, c in _comps(sc):
+        assert rel_l2(ref.get(f, c), o.get(f, c)) <= TOL["f64"], nm
+    for tile in TM_BLOCK_TILES:
+        for graph in (False, True):
+            print(f"[gpu-case] tm block {name} tile {tile} graph {graph}", flush=True)
+            g = apply(_grid(sc, "f64"), sc)
+            g.set_tile(*tile)
+            g.set_graph(graph)
+            g.step(37)
+            g.step(n - 37)
+            for nm, f, c in _comps(sc):
+                np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=f"{name} {tile} {graph} {nm}")
+            np.testing.assert_array_equal(g.probes(), ref.probes())
+            g.close()
+
+
+def test_tm_time_blocked_f32_and_autotune():
+    sc, p, a = from_golden("tm2d_c1")
+    n = 300
+    print("[gpu-case] tm block f32 + autotune", flush=True)
+    ref = apply(_grid(sc, "f32"), sc)
+    ref.step(n)
+    g = apply(_grid(sc, "f32"), sc)
+    g.set_tile(0, 4, 16)
+    g.step(n)
+    for nm, f, c in _comps(sc):
+        np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=nm)
+    t = apply(_grid(sc, "f64"), sc)
+    tile = t.autotune(trial_steps=20)
+    print("autotuned TM tile", tile, flush=True)
+    t.step(n)
+    o = apply(_oracle(sc), sc)
+    o.step(n)
+    for nm, f, c in _comps(sc):
+        assert rel_l2(t.get(f, c), o.get(f, c)) <= TOL["f64"], nm
