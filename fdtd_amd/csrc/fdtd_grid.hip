@@ -17,6 +17,7 @@
 #include <chrono>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fdtd_hip.h"
@@ -2609,6 +2610,9 @@ struct fdtd_batch {
   int ntiles = 0;
   long* dft_offs = nullptr;  // n+1 prefix sums of each grid's DFT items
   long dft_total = 0;
+  // the time-blocked DFT: n+1 prefix sums of each grid's sampler and fold blocks
+  long *smp_offs = nullptr, *fold_offs = nullptr;
+  long smp_total = 0, fold_total = 0;
   static constexpr int kRing = 2;
   char* args_dev = nullptr;
   char* args_host = nullptr;
@@ -2616,7 +2620,8 @@ struct fdtd_batch {
   bool ev_used[kRing] = {};
   int slot = 0;
   bool use_graph = true;
-  std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (slot, steps) -> executable graph
+  // (slot, steps, DFT block phase or -1) -> executable graph
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 
   ~fdtd_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -2625,6 +2630,8 @@ struct fdtd_batch {
       if (e) (void)hipEventDestroy(e);
     if (tiles) (void)hipFree(tiles);
     if (dft_offs) (void)hipFree(dft_offs);
+    if (smp_offs) (void)hipFree(smp_offs);
+    if (fold_offs) (void)hipFree(fold_offs);
     if (args_dev) (void)hipFree(args_dev);
     if (args_host) (void)hipHostFree(args_host);
     for (size_t q = 0; q < grids.size(); q++) grids[q]->stream = saved_streams[q];
@@ -2670,6 +2677,27 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
     if (g->coef_dirty && (rc = upload_coefs(g))) return rc;
     if ((rc = ensure_trace(g, g->step + nsteps))) return rc;
   }
+  {
+    long items = 0, sblocks = 0;
+    for (fdtd_grid* g : b->grids)
+      for (int m = 0; m < g->ndft; m++) {
+        items += long(g->dft[m].nfreq) * g->dft[m].ncells;
+        sblocks += fdtdk::fold_segment(g->dft[m].ncells) / 256;
+      }
+    if (items != b->dft_total || sblocks != b->smp_total)
+      return fail(FDTD_ESTATE, "a grid's DFT monitors changed after the batch was made (make a new batch)");
+  }
+  // the time-blocked DFT (samples every step, folded once per kDftBlock
+  // steps) when every grid is at the same phase of its block; otherwise the
+  // grids' pending samples are folded and the sums run step by step
+  int phase = b->smp_total ? int(b->grids[0]->step % fdtdk::kDftBlock) : -1;
+  for (fdtd_grid* g : b->grids)
+    if (g->step % fdtdk::kDftBlock != phase) phase = -1;
+  if (phase < 0)
+    for (fdtd_grid* g : b->grids) {
+      int rc = dft_flush(g);
+      if (rc) return rc;
+    }
   int done = 0;
   while (done < nsteps) {
     const int chunk = std::min(kChunk, nsteps - done);
@@ -2687,6 +2715,7 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
       fdtdk::StepArgs<T> a;
       fill_args<T>(g, a);
       a.step0 = g->step;
+      a.dft_done0 = g->dft_done;
       gput(a.incE, g->aux.incE);
       gput(a.incH, g->aux.incH);
       gput(a.self, reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + (slot * n + q) * sizeof(fdtdk::StepArgs<T>)));
@@ -2698,22 +2727,31 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
     // the E-pass instance without the ADE code when no grid is dispersive
     bool disp = false;
     for (fdtd_grid* g : b->grids) disp = disp || g->max_np > 0;
+    // one step's launches: the two passes over every grid's tiles, then the DFT
+    auto enqueue_step = [&](int s) {
+      if (disp)
+        hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+      else
+        hipLaunchKernelGGL((fdtdk::e_pass_batch<T, 0>), grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+      hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+      if (phase >= 0) {
+        hipLaunchKernelGGL(fdtdk::dft_sample_batch<T>, dim3(unsigned(b->smp_total)), dim3(256), 0, b->stream, dev,
+                           b->smp_offs, int(n), s);
+        if ((phase + s + 1) % fdtdk::kDftBlock == 0)
+          hipLaunchKernelGGL(fdtdk::dft_fold_batch<T>, dim3(unsigned(b->fold_total)), dim3(256), 0, b->stream, dev,
+                             b->fold_offs, int(n), s);
+      } else if (b->dft_total) {
+        hipLaunchKernelGGL(fdtdk::dft_accumulate_batch<T>, dim3(unsigned((b->dft_total + 255) / 256)), dim3(256), 0,
+                           b->stream, dev, b->dft_offs, int(n), s);
+      }
+    };
     if (b->use_graph) {
-      auto key = std::make_pair(slot, chunk);
+      auto key = std::make_tuple(slot, chunk, phase);
       auto it = b->graphs.find(key);
       if (it == b->graphs.end()) {
         hipGraph_t graph;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (int s = 0; s < chunk; s++) {
-          if (disp)
-            hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
-          else
-            hipLaunchKernelGGL((fdtdk::e_pass_batch<T, 0>), grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
-          hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
-          if (b->dft_total)
-            hipLaunchKernelGGL(fdtdk::dft_accumulate_batch<T>, dim3(unsigned((b->dft_total + 255) / 256)), dim3(256), 0,
-                               b->stream, dev, b->dft_offs, int(n), s);
-        }
+        for (int s = 0; s < chunk; s++) enqueue_step(s);
         HIP_TRY(hipStreamEndCapture(b->stream, &graph));
         hipGraphExec_t exec;
         hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -2723,24 +2761,20 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
       }
       HIP_TRY(hipGraphLaunch(it->second, b->stream));
     } else {
-      for (int s = 0; s < chunk; s++) {
-        if (disp)
-          hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
-        else
-          hipLaunchKernelGGL((fdtdk::e_pass_batch<T, 0>), grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
-        hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
-        if (b->dft_total)
-          hipLaunchKernelGGL(fdtdk::dft_accumulate_batch<T>, dim3(unsigned((b->dft_total + 255) / 256)), dim3(256), 0,
-                             b->stream, dev, b->dft_offs, int(n), s);
-      }
+      for (int s = 0; s < chunk; s++) enqueue_step(s);
       HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(b->ev[slot], b->stream));
     b->ev_used[slot] = true;
     for (fdtd_grid* g : b->grids) {
+      const long last = ((g->step + chunk) / fdtdk::kDftBlock) * fdtdk::kDftBlock;  // last block end in the chunk
+      if (phase < 0)
+        g->dft_done = g->step + chunk;  // summed step by step
+      else if (last > g->step)
+        g->dft_done = last;
       g->step += chunk;
-      g->dft_done = g->step;  // batches accumulate the DFT step by step
     }
+    if (phase >= 0) phase = int((phase + chunk) % fdtdk::kDftBlock);
     done += chunk;
   }
   return 0;
@@ -2789,6 +2823,27 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
   if (!rc && hipMalloc((void**)&b->dft_offs, offs.size() * sizeof(long)) != hipSuccess) rc = fail(FDTD_ENOMEM, "offsets");
   if (!rc && hipMemcpy(b->dft_offs, offs.data(), offs.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(FDTD_EDEVICE, "offset upload");
+  // sampler and fold blocks per grid: as the grid's own launches count them
+  // (each monitor's cells, and each (monitor, component) segment, padded to
+  // whole blocks)
+  std::vector<long> so(1, 0), fo(1, 0);
+  for (int q = 0; q < n; q++) {
+    long sb = 0, fb = 0;
+    for (int m = 0; m < grids[q]->ndft; m++) {
+      sb += fdtdk::fold_segment(grids[q]->dft[m].ncells) / 256;
+      fb += long(grids[q]->dft[m].ncomp) * fdtdk::fold_segment(grids[q]->dft[m].ncells) / 256;
+    }
+    so.push_back(so.back() + sb);
+    fo.push_back(fo.back() + fb);
+  }
+  b->smp_total = so.back();
+  b->fold_total = fo.back();
+  if (!rc && hipMalloc((void**)&b->smp_offs, so.size() * sizeof(long)) != hipSuccess) rc = fail(FDTD_ENOMEM, "offsets");
+  if (!rc && hipMalloc((void**)&b->fold_offs, fo.size() * sizeof(long)) != hipSuccess) rc = fail(FDTD_ENOMEM, "offsets");
+  if (!rc && hipMemcpy(b->smp_offs, so.data(), so.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(FDTD_EDEVICE, "offset upload");
+  if (!rc && hipMemcpy(b->fold_offs, fo.data(), fo.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(FDTD_EDEVICE, "offset upload");
   const size_t abytes = size_t(fdtd_batch::kRing) * n * sizeof(fdtdk::StepArgs<double>);
   if (!rc && hipMalloc((void**)&b->args_dev, abytes) != hipSuccess) rc = fail(FDTD_ENOMEM, "batch arguments");
   if (!rc && hipHostMalloc((void**)&b->args_host, abytes) != hipSuccess) rc = fail(FDTD_ENOMEM, "batch staging");
@@ -2816,8 +2871,9 @@ int fdtd_batch_set_tile(fdtd_batch* b, int bx, int by, int kc) {
 
 int fdtd_batch_autotune(fdtd_batch* b, int trial_steps, int* bx, int* by, int* kc) {
   if (!b || trial_steps < 1) return fail(FDTD_EINVAL, "bad argument");
+  // 32-wide rows waste fewer lanes on the edges of small grids
   static const Tile cand[] = {{64, 4, 1}, {64, 4, 2}, {64, 2, 4}, {128, 2, 2}, {128, 2, 4}, {128, 1, 4},
-                              {256, 1, 2}, {256, 1, 4}};
+                              {256, 1, 2}, {256, 1, 4}, {32, 8, 2}, {32, 8, 4}, {32, 4, 4}, {32, 2, 8}};
   int rc = 0;
   std::vector<GridSnapshot> snaps(b->grids.size());
   for (size_t q = 0; q < b->grids.size() && !rc; q++) {

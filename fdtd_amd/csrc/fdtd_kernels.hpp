@@ -884,9 +884,8 @@ __global__ __launch_bounds__(256) void dft_accumulate(const StepArgs<T>* __restr
 // Each monitor's cells are padded to whole 256-thread blocks in the launch,
 // so a block's monitor (and its fields) are block-uniform scalar loads.
 template <typename T>
-__global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict__ ap, int s) {
-  const StepArgs<T>& a = *ap;
-  const long b0 = (long)blockIdx.x * 256;
+__device__ __forceinline__ void dft_sample_block(const StepArgs<T>& a, long blk, int s) {
+  const long b0 = blk * 256;
   if (b0 >= a.dft_cells) return;
   int q = 0;
   while (q + 1 < a.ndft && b0 >= a.dft[q + 1].cbase) q++;
@@ -909,6 +908,11 @@ __global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict_
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void dft_sample(const StepArgs<T>* __restrict__ ap, int s) {
+  dft_sample_block(*ap, blockIdx.x, s);
+}
+
 // The fold: one thread per (component, monitor cell), cells fastest; each
 // (monitor, component) segment is padded to whole 256-thread blocks, so a
 // block stages the block's phase rows of its component's table in LDS once
@@ -920,9 +924,9 @@ constexpr int kFoldLdsFreq = 64;  // phase rows staged in LDS up to this many fr
 __host__ __device__ inline long fold_segment(long ncells) { return (ncells + kFoldBlock - 1) / kFoldBlock * kFoldBlock; }
 
 template <typename T>
-__device__ __forceinline__ void dft_fold_items(const StepArgs<T>& a, long m0, long m1) {
+__device__ __forceinline__ void dft_fold_items(const StepArgs<T>& a, long m0, long m1, long blk) {
   __shared__ double sph[kDftBlock * kFoldLdsFreq * 2];
-  const long b0 = (long)blockIdx.x * kFoldBlock;  // block-uniform: one (monitor, component) per block
+  const long b0 = blk * kFoldBlock;  // block-uniform: one (monitor, component) per block
   if (b0 >= a.dft_slots) return;
   int q = 0;
   while (q + 1 < a.ndft && b0 >= a.dft[q + 1].sbase) q++;
@@ -982,7 +986,7 @@ __device__ __forceinline__ void dft_fold_items(const StepArgs<T>& a, long m0, lo
 
 template <typename T>
 __global__ __launch_bounds__(256) void dft_fold(const StepArgs<T>* __restrict__ ap, long m0, long m1) {
-  dft_fold_items(*ap, m0, m1);
+  dft_fold_items(*ap, m0, m1, blockIdx.x);
 }
 
 // the fold after step s of a chunk whose step is the last of a block: the
@@ -992,15 +996,12 @@ __global__ __launch_bounds__(256) void dft_fold_step(const StepArgs<T>* __restri
   const StepArgs<T>& a = *ap;
   const long m1 = a.step0 + s + 1;
   const long m0 = m1 - kDftBlock > a.dft_done0 ? m1 - kDftBlock : a.dft_done0;
-  dft_fold_items(a, m0, m1);
+  dft_fold_items(a, m0, m1, blockIdx.x);
 }
 
-// batched grids: `offs` holds each grid's first item (prefix sums, n+1 entries)
-template <typename T>
-__global__ __launch_bounds__(256) void dft_accumulate_batch(const StepArgs<T>* __restrict__ args,
-                                                            const long* __restrict__ offs, int n, int s) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= offs[n]) return;
+// the grid of batch item (or block) g: `offs` holds each grid's first one
+// (prefix sums, n+1 entries)
+__device__ __forceinline__ int batch_grid(const long* __restrict__ offs, int n, long g) {
   int lo = 0, hi = n - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) / 2;
@@ -1009,6 +1010,34 @@ __global__ __launch_bounds__(256) void dft_accumulate_batch(const StepArgs<T>* _
     else
       hi = mid - 1;
   }
+  return lo;
+}
+
+// the time-blocked DFT of batched grids stepped in the same phase of their
+// blocks: the sampler after every step and the fold after a block's last
+// step, over every grid's blocks (`boffs`: each grid's first block)
+template <typename T>
+__global__ __launch_bounds__(256) void dft_sample_batch(const StepArgs<T>* __restrict__ args,
+                                                        const long* __restrict__ boffs, int n, int s) {
+  const int q = batch_grid(boffs, n, blockIdx.x);
+  dft_sample_block(args[q], blockIdx.x - boffs[q], s);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void dft_fold_batch(const StepArgs<T>* __restrict__ args,
+                                                      const long* __restrict__ boffs, int n, int s) {
+  const int q = batch_grid(boffs, n, blockIdx.x);
+  const StepArgs<T>& a = args[q];
+  const long m1 = a.step0 + s + 1;
+  const long m0 = m1 - kDftBlock > a.dft_done0 ? m1 - kDftBlock : a.dft_done0;
+  dft_fold_items(a, m0, m1, blockIdx.x - boffs[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void dft_accumulate_batch(const StepArgs<T>* __restrict__ args,
+                                                            const long* __restrict__ offs, int n, int s) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= offs[n]) return;
+  const int lo = batch_grid(offs, n, g);
   const StepArgs<T>& a = args[lo];
   dft_item(a, g - offs[lo], a.step0 + s);
 }

@@ -450,3 +450,39 @@ def test_tm_time_blocked_f32_and_autotune():
     o.step(n)
     for nm, f, c in _comps(sc):
         assert rel_l2(t.get(f, c), o.get(f, c)) <= TOL["f64"], nm
+
+
+def test_batch_dft_blocks_and_phases():
+    """The batch's DFT: time-blocked when every grid is at the same phase of
+    its block (a spectrum read in the middle of a block folds what is
+    pending), summed step by step when the phases differ (one grid stepped
+    alone first); the spectra equal those of the grids stepped alone."""
+    from fdtd_amd import Batch
+    scs = synth.batch_c4(nsims=3, nsteps=200, lo=26, hi=36, seed=12)
+    alone = []
+    for sc in scs:
+        g = apply(_grid(sc, "f32"), sc)
+        g.step(200)
+        alone.append(g)
+    print("[gpu-case] batch dft phases", flush=True)
+    gs = [apply(_grid(sc, "f32"), sc) for sc in scs]
+    b = Batch(gs)
+    b.step(21)                    # blocked, ends mid-block
+    spec = gs[1].get_dft(0)       # folds grid 1's pending samples
+    assert np.abs(spec).max() > 0
+    b.step(30)                    # grid 1 is now folded further than the others: still the same phase
+    b.close()
+    gs[0].step(5)                 # phases differ from here on
+    b = Batch(gs)
+    b.step(60)                    # summed step by step
+    b.close()
+    gs[1].step(5)
+    gs[2].step(5)
+    b = Batch(gs)                 # the same phase again: blocked
+    b.step(200 - 116)
+    b.close()
+    for sc, g, r in zip(scs, gs, alone):
+        assert g.step_count() == 200
+        np.testing.assert_array_equal(g.get(E, 2), r.get(E, 2))
+        for idx in range(len(sc["dfts"])):
+            np.testing.assert_array_equal(g.get_dft(idx), r.get_dft(idx), err_msg=sc["name"])
