@@ -484,7 +484,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3d256_f32.json")
 TRAFFIC_SOURCE = ("profiles/hbm_traffic_3d256_f32.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
                   "workload at the same tile, each scaled by the counter's reading for a known 1 GiB device copy; "
                   "not measured inside this run")
-PASS_KERNELS = ("fused3d_lds", "e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedw_3d", "fused_3d")
+PASS_KERNELS = ("fused3d_lds", "e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedz_3d", "tm_block")
 
 
 def _short(name):
@@ -517,7 +517,7 @@ def traffic_summary(args):
     ff, wf = known / cf, known / cw
     fetch = _counter_rows(fetch_dir, "FETCH_SIZE")
     write = _counter_rows(write_dir, "WRITE_SIZE")
-    step_kernel = "fused3d_lds" if "fused3d_lds" in fetch else ("h_pass_3d" if "h_pass_3d" in fetch else "fusedw_3d")
+    step_kernel = "fused3d_lds" if "fused3d_lds" in fetch else ("h_pass_3d" if "h_pass_3d" in fetch else "fusedz_3d")
     nsteps = len(fetch[step_kernel])
     kern = {k: {"fetch_bytes": round(mean(fetch[k]) * 1024.0 * ff),
                 "write_bytes": round(mean(write.get(k, [0.0])) * 1024.0 * wf),

@@ -119,8 +119,9 @@ constexpr int kNumKernels = 8;
 
 // tile of the per-step passes: threads per block in x and y, z planes per thread
 // bx == 0 selects the streamed fused pass (3-D FP32) with E regions of `by`
-// rows and `kc` workgroups per compute unit, bx == 1 the wave-fused single
-// pass with `by` rows per wave; bx >= 2 the two passes with bx x by threads
+// rows and `kc` workgroups per compute unit, bx == 1 the z-marching fused
+// pass with workgroups of 64 x `by` columns and `kc` planes; bx >= 2 the two
+// passes with bx x by threads
 // per workgroup.  TM grids: bx == 0 selects the time-blocked pass, `by` steps
 // per launch on cores of kc x kc cells.
 struct Tile {
@@ -866,7 +867,7 @@ void fill_args(fdtd_grid* g, fdtdk::StepArgs<T>& a) {
   for (int p = 0; p < g->nprobe; p++) a.probe[p] = g->probe[p];
   gput(a.trace, g->trace);
   // byte sizes of one (padded) field array and of the id volume, for the
-  // buffer-addressed wave-fused pass (0 when they do not fit 31 bits)
+  // buffer-addressed streamed pass (0 when they do not fit 31 bits)
   const size_t fb = size_t(g->ncell + fdtdk::field_pad(g->d.nx)) * sizeof(T);
   a.field_bytes = fb < (size_t(1) << 31) ? unsigned(fb) : 0u;
   a.mat_bytes = a.field_bytes ? unsigned(g->ncell + fdtdk::field_pad(g->d.nx)) : 0u;
@@ -1055,10 +1056,11 @@ int step_impl(fdtd_grid* g, int nsteps) {
   dim3 block, grid;
   int ntx = 1, nty = 1, nkc = 1;
   if (wavef) {
-    ntx = (g->d.nx - 2 + fdtdk::kWaveCols - 1) / fdtdk::kWaveCols;
-    nty = (g->d.ny - 2 + fdtdk::kWavesY * t.by - 1) / (fdtdk::kWavesY * t.by);
+    // z-marching fused pass: tiles of 63 x (by - 1) owned columns, kc planes
+    ntx = (g->d.nx - 2 + fdtdk::kZCols - 2) / (fdtdk::kZCols - 1);
+    nty = (g->d.ny - 2 + t.by - 2) / (t.by - 1);
     nkc = (nzi + t.kc - 1) / t.kc;
-    block = dim3(64, fdtdk::kWavesY, 1);
+    block = dim3(fdtdk::kZCols, t.by, 1);
     grid = dim3(ntx * nty * nkc, 1, 1);
   } else if (fused) {
     // streamed pass: E regions of kSCols x kSRows cells own one column and one
@@ -1154,12 +1156,16 @@ int step_impl(fdtd_grid* g, int nsteps) {
         int rc;
         const auto* ap = (fused && (s % 2)) ? dev1 : dev0;
         if (wavef) {
-          if (t.by == 2)
-            rc = launch(g, 3, fdtdk::fusedw_3d<T, 2>, grid, block, ap, t.kc, s, ntx, nty, nkc);
-          else if (t.by == 4)
-            rc = launch(g, 3, fdtdk::fusedw_3d<T, 4>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          const bool disp = g->max_np > 0;
+          if (t.by == 4)
+            rc = disp ? launch(g, 3, fdtdk::fusedz_3d<T, 4, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc)
+                      : launch(g, 3, fdtdk::fusedz_3d<T, 4, 0>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+          else if (t.by == 8)
+            rc = disp ? launch(g, 3, fdtdk::fusedz_3d<T, 8, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc)
+                      : launch(g, 3, fdtdk::fusedz_3d<T, 8, 0>, grid, block, ap, t.kc, s, ntx, nty, nkc);
           else
-            rc = launch(g, 3, fdtdk::fusedw_3d<T, 6>, grid, block, ap, t.kc, s, ntx, nty, nkc);
+            rc = disp ? launch(g, 3, fdtdk::fusedz_3d<T, 16, 1>, grid, block, ap, t.kc, s, ntx, nty, nkc)
+                      : launch(g, 3, fdtdk::fusedz_3d<T, 16, 0>, grid, block, ap, t.kc, s, ntx, nty, nkc);
         } else if (fused) {
           if constexpr (sizeof(T) == 4) {
             if (g->max_np > 0)
@@ -2291,8 +2297,9 @@ int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
                   "workgroups per compute unit (FP64 grids, and grids of more than %d planes, run it as the default "
                   "two-pass tile)", fdtdk::kSRows, fdtdk::kSMaxZ);
   } else if (bx == 1) {
-    if (g->ndim != 3 || (by != 2 && by != 4 && by != 6) || kc <= 0)
-      return fail(FDTD_EINVAL, "the wave-fused schedule (bx = 1) is 3-D only, with by in {2, 4, 6} rows and kc >= 1");
+    if (g->ndim != 3 || (by != 4 && by != 8 && by != 16) || kc <= 0)
+      return fail(FDTD_EINVAL, "the z-marching fused schedule (bx = 1) is 3-D only, with by in {4, 8, 16} rows of 64 "
+                               "columns and kc >= 1 planes per workgroup");
   } else if (bx < 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0) {
     return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
   }
@@ -2317,8 +2324,8 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
   static const Tile cand3[] = {{0, 5, 1},  {0, 5, 2},  {256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {256, 1, 8},
-                               {128, 2, 2}, {128, 2, 4}, {64, 4, 1}, {64, 4, 2}, {1, 2, 8}, {1, 2, 16},
-                               {1, 4, 8},   {1, 4, 16}};
+                               {128, 2, 2}, {128, 2, 4}, {64, 4, 1}, {64, 4, 2}, {1, 8, 16}, {1, 8, 32},
+                               {1, 16, 16}, {1, 4, 32}};
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1},
                                {0, 2, 16},  {0, 4, 16},  {0, 4, 24}, {0, 8, 16}, {0, 4, 8}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;

@@ -175,9 +175,10 @@ def test_autotune_as_first_call():
 
 # (0, 5, kc): streamed fused pass (128 x 5 E regions, kc persistent workgroups
 # per compute unit; FP64 grids run the two-pass default); (1, TY, kc):
-# wave-fused pass (ny=23 gives a partly empty last wave row block, nx=300 five
-# 62-column wave tiles and three streamed tiles, the last 44 columns wide)
-FUSED = [(0, 5, 1), (0, 5, 2), (1, 2, 7), (1, 4, 16), (1, 6, 5)]
+# z-marching fused pass (64 x TY workgroups owning 63 x (TY - 1) columns:
+# nx=300 gives five tiles, the last 47 columns wide, ny=23 a partly empty last
+# row of tiles, and the streamed pass three tiles, the last 44 columns wide)
+FUSED = [(0, 5, 1), (0, 5, 2), (1, 4, 7), (1, 8, 16), (1, 16, 5)]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -241,8 +242,8 @@ def test_fused_pass_against_oracle(precision):
     against the FP64 oracle."""
     for sc, tile in ((synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (0, 5, 1)),
                      (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (0, 5, 2)),
-                     (synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (1, 4, 8)),
-                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (1, 2, 16))):
+                     (synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (1, 8, 8)),
+                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (1, 4, 16))):
         print(f"[gpu-case] fused-vs-oracle {sc['name']} {tile} {precision}", flush=True)
         g = apply(_grid(sc, precision), sc)
         g.set_tile(*tile)
@@ -291,7 +292,7 @@ def test_batch_equals_standalone(graph):
     np.testing.assert_array_equal(batched[-1].probes(), alone[-1].probes())
 
 
-@pytest.mark.parametrize("tile", [(256, 1, 2), (1, 2, 8)])
+@pytest.mark.parametrize("tile", [(256, 1, 2), (1, 8, 8)])
 def test_graph_chunks_equal_plain_launches(tile):
     """Chunks captured in hipGraphs and replayed give the fields, DFTs and
     probes of plain launches, across uneven step calls and a DFT read-back
