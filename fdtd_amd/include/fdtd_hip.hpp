@@ -23,7 +23,7 @@
 //     (2*dim faces, one history)                    make_tfsf_box<Mode>(grid,box,k,aux)
 //   plane.updateD(dt,dx) / updateB(dt,dx)           same, inside a for_each loop     Source.hpp:595-620
 //   src.apply(t)                                    same, inside a for_each loop     Source.hpp:225-229
-//   std::for_each(cells.begin(), cells.end(), F)    for_each(grid, F), for_each(grid.cells(id), M)  testTM.cpp:102-113
+//   std::for_each(cells.begin(), cells.end(), F)    for_each(grid, F) or F(grid); M(grid.cells(id))  testTM.cpp:102-113
 //   make_pec_boundary<Mode,Dir,Orient>(b,e)         make_pec_boundary<Mode,Dir,Orient>(grid)         BoundaryUpdates.hpp:527-565
 //     (pmc, symmetric, antisymmetric, periodic)     same names; periodic image fixed (layer L-2 / 1)  BoundaryUpdates.hpp:578-832
 //   BoundaryData::updateE() / updateH()             Stepper::boundary(bd): ghost writes after E / H   BoundaryUpdates.hpp:340-362
@@ -563,15 +563,28 @@ void CellRef<Mode, scalar>::setPMLParameters(double K, double S, double A, doubl
 }
 
 // --------------------------------------------------------------- functors
-// Value types naming one stage of the step; the Stepper checks the order.
+// Value types naming one stage of the step.  Handed to a Stepper, which
+// checks their order; or called on a grid, f(grid) (a material functor also
+// on grid.cells(id)), which is for_each(grid, f): the reference's sweep of
+// the functor over the cells (tests/testTM.cpp:102-113).
+namespace detail {
+template <class F>
+struct Sweep {
+  template <class G>
+  void operator()(G&& g) const {
+    for_each(g, static_cast<const F&>(*this));
+  }
+};
+}  // namespace detail
+
 template <class Mode, class Flux>
-struct YeeUpdate {
+struct YeeUpdate : detail::Sweep<YeeUpdate<Mode, Flux>> {
   double dt, dx;
   YeeUpdate(double deltat, double deltax) : dt(deltat), dx(deltax) {}
   static constexpr int stage = Flux::field == FDTD_D || Flux::field == FDTD_E ? 0 : 4;
 };
 template <class Mode, FieldType ft, Dir d>
-struct UpdatePML {
+struct UpdatePML : detail::Sweep<UpdatePML<Mode, ft, d>> {
   double dt, dx;
   UpdatePML(double deltat, double deltax) : dt(deltat), dx(deltax) {}
   static constexpr int stage = ft == FieldType::Electric ? 1 : 5;
@@ -579,11 +592,11 @@ struct UpdatePML {
 
 // material updates: one table entry each (eps_r or mu_r plus poles), MaterialSpec above
 template <class Mode, FieldType ft, bool forward = false>
-struct VacuumUpdate {
+struct VacuumUpdate : detail::Sweep<VacuumUpdate<Mode, ft, forward>> {
   MaterialSpec spec() const { return {}; }
 };
 template <class Mode, FieldType ft, bool forward = false>
-struct ConstantUpdate {
+struct ConstantUpdate : detail::Sweep<ConstantUpdate<Mode, ft, forward>> {
   double K;
   explicit ConstantUpdate(double k) : K(k) {}
   MaterialSpec spec() const {
@@ -594,7 +607,7 @@ struct ConstantUpdate {
 };
 // Drude pole with the reference's DrudeStorage(K, Freq, Gamma, dt) arguments
 template <class Mode, FieldType ft, bool forward = false>
-struct DrudeUpdate {
+struct DrudeUpdate : detail::Sweep<DrudeUpdate<Mode, ft, forward>> {
   double K, freq, gamma, dt;
   DrudeUpdate(double k, double f, double g, double deltat) : K(k), freq(f), gamma(g), dt(deltat) {}
   MaterialSpec spec() const { return {K, 1.0, {double(FDTD_DRUDE), freq, gamma, 1.0}}; }
@@ -602,7 +615,7 @@ struct DrudeUpdate {
 // Lorentz pole with LorentzStorage(K, Delta, Freq, Gamma, dt); like the
 // reference's update, Delta does not enter (include/DispersiveMaterials.hpp:711-735)
 template <class Mode, FieldType ft, bool forward = false>
-struct LorentzUpdate {
+struct LorentzUpdate : detail::Sweep<LorentzUpdate<Mode, ft, forward>> {
   double K, delta, freq, gamma, dt;
   LorentzUpdate(double k, double de, double f, double g, double deltat) : K(k), delta(de), freq(f), gamma(g), dt(deltat) {}
   MaterialSpec spec() const { return {K, 1.0, {double(FDTD_LORENTZ), freq, gamma, 1.0}}; }
@@ -611,7 +624,7 @@ struct LorentzUpdate {
 // conductivity (include/DispersiveMaterials.hpp:508-680): P relaxes
 // implicitly towards D with factor dt Freq/(K eps0)
 template <class Mode, FieldType ft, bool forward = false>
-struct ConductiveUpdate {
+struct ConductiveUpdate : detail::Sweep<ConductiveUpdate<Mode, ft, forward>> {
   double K, freq, dt;
   ConductiveUpdate(double k, double f, double deltat) : K(k), freq(f), dt(deltat) {}
   MaterialSpec spec() const { return {K, 1.0, {double(FDTD_CONDUCTIVE), freq, 0.0, 1.0}}; }
@@ -621,7 +634,7 @@ struct ConductiveUpdate {
 // fixed bias field B [T] (include/DispersiveMaterials.hpp:1263-1340); in TM
 // the reference's update is the Drude one per component, and so is this one
 template <class Mode, FieldType ft, bool forward = false>
-struct MagnetizedDrudeUpdate {
+struct MagnetizedDrudeUpdate : detail::Sweep<MagnetizedDrudeUpdate<Mode, ft, forward>> {
   double Bx, By, Bz, K, freq, gamma, dt;
   MagnetizedDrudeUpdate(double bx, double by, double bz, double k, double f, double g, double deltat)
       : Bx(bx), By(by), Bz(bz), K(k), freq(f), gamma(g), dt(deltat) {}
@@ -638,7 +651,7 @@ struct MagnetizedDrudeUpdate {
 // cell; SelfMagnetizedDrudeStorage(dt) with its K, Freq, Gamma set through
 // the accessors, as in the reference (1898-1943)
 template <class Mode, FieldType ft, bool forward = false>
-struct SelfMagnetizedDrudeUpdate {
+struct SelfMagnetizedDrudeUpdate : detail::Sweep<SelfMagnetizedDrudeUpdate<Mode, ft, forward>> {
   double k_ = 1.0, freq_ = 0.0, gamma_ = 0.0, dt;
   explicit SelfMagnetizedDrudeUpdate(double deltat) : dt(deltat) {}
   double& K() { return k_; }
@@ -655,7 +668,7 @@ struct SelfMagnetizedDrudeUpdate {
 // SelfMagnetizedFluidDrudeStorage; include/DispersiveMaterials.hpp:1160-1199,
 // 1728-1893), K = 1; the cells' ne and nu_m are given by YeeGrid::setFluidState
 template <class Mode, FieldType ft, bool forward = false>
-struct FluidDrudeUpdate {
+struct FluidDrudeUpdate : detail::Sweep<FluidDrudeUpdate<Mode, ft, forward>> {
   double dt;
   explicit FluidDrudeUpdate(double deltat) : dt(deltat) {}
   MaterialSpec spec() const {
@@ -665,7 +678,7 @@ struct FluidDrudeUpdate {
   }
 };
 template <class Mode, FieldType ft, bool forward = false>
-struct MagnetizedFluidDrudeUpdate {
+struct MagnetizedFluidDrudeUpdate : detail::Sweep<MagnetizedFluidDrudeUpdate<Mode, ft, forward>> {
   double Bx, By, Bz, dt;
   MagnetizedFluidDrudeUpdate(double bx, double by, double bz, double deltat) : Bx(bx), By(by), Bz(bz), dt(deltat) {}
   MaterialSpec spec() const {
@@ -679,7 +692,7 @@ struct MagnetizedFluidDrudeUpdate {
   }
 };
 template <class Mode, FieldType ft, bool forward = false>
-struct SelfMagnetizedFluidDrudeUpdate {
+struct SelfMagnetizedFluidDrudeUpdate : detail::Sweep<SelfMagnetizedFluidDrudeUpdate<Mode, ft, forward>> {
   double dt;
   explicit SelfMagnetizedFluidDrudeUpdate(double deltat) : dt(deltat) {}
   MaterialSpec spec() const {

@@ -1,5 +1,6 @@
 // 2-D TM time loop written as the reference's own user loop
-// (tests/testTM.cpp:102-113): one for_each sweep per functor, soft sources
+// (tests/testTM.cpp:102-113): one sweep per functor (for_each(grid, f), or
+// the functor called on the grid or on a material's cells), soft sources
 // applied inside the loop (SoftSource on Ez right after the E material step
 // and on Hy right after the H material step, include/Source.hpp:421-425),
 // two single TFSF planes with their own Aux1D stepped by updateD / updateB.
@@ -59,13 +60,13 @@ int main() {
     p0.updateD(dt, dx);
     p1.updateD(dt, dx);
     for_each(grid, VacuumUpdate<TM, FieldType::Electric>());
-    for_each(grid.cells(1), ConstantUpdate<TM, FieldType::Electric>(2.5));
+    ConstantUpdate<TM, FieldType::Electric>(2.5)(grid.cells(1));  // a functor called on a range: the same sweep
     for_each(grid.cells(2), LorentzUpdate<TM, FieldType::Electric>(2.0, 0.7, 0.2 / dt, 0.01 / dt, dt));
     srcE.apply(t);
     for_each(grid, YeeUpdate<TM, Bx>(dt, dx));
-    for_each(grid, YeeUpdate<TM, By>(dt, dx));
+    YeeUpdate<TM, By>(dt, dx)(grid);
     for_each(grid, UpdatePML<TM, FieldType::Magnetic, Dir::X>(dt, dx));
-    for_each(grid, UpdatePML<TM, FieldType::Magnetic, Dir::Y>(dt, dx));
+    UpdatePML<TM, FieldType::Magnetic, Dir::Y>(dt, dx)(grid);
     p0.updateB(dt, dx);
     p1.updateB(dt, dx);
     for_each(grid, VacuumUpdate<TM, FieldType::Magnetic>());

@@ -139,7 +139,13 @@ def _cone_run(sc, L, shrink):
     """Step the oracle L + 1 steps from random fields; after step m roll every
     E and H cell outside the design box grown by L - m - shrink (at least 0 on
     the last step) back to its value before the step, as a recompute that only
-    updates that region leaves it.  Returns E of the box after step L."""
+    updates that region leaves it.  Returns E of the box after step L.
+
+    The CPML psi outside the region is left updated here, where the GPU's
+    recompute leaves it stale.  That difference cannot reach the box: a
+    cell's psi is read only by that cell's own update, so psi outside the
+    region feeds only E and H outside it, which are rolled back (stale) in
+    both; inside the region psi was updated on every earlier step in both."""
     from scenarios import E, H
     o = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
     rng = np.random.default_rng(11)
