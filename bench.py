@@ -484,7 +484,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3d256_f32.json")
 TRAFFIC_SOURCE = ("profiles/hbm_traffic_3d256_f32.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
                   "workload at the same tile, each scaled by the counter's reading for a known 1 GiB device copy; "
                   "not measured inside this run")
-PASS_KERNELS = ("fused3d_lds", "e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedz_3d", "tm_block")
+PASS_KERNELS = ("fused3d_lds", "e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedz_3d", "tm_block", "tb_3d")
 
 
 def _short(name):
@@ -563,9 +563,6 @@ def main():
     ap.add_argument("--tile", default=None,
                     help="bx,by,kc: fixed tile instead of the autotuner (c2; bx = 0 selects the fused pass)")
     ap.add_argument("--graph", action="store_true", help="capture chunks of steps in hipGraphs (c2)")
-    ap.add_argument("--strip", default="",
-                    help="c2 cost breakdown (not the headline): comma list of tfsf, dft, pml to leave out of the "
-                         "workload (without tfsf a Dz point source drives the grid)")
     ap.add_argument("--traffic-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
                     help="turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench and of --workload copy into "
                          "HBM bytes per launch (no GPU work)")
@@ -593,13 +590,7 @@ def main():
         print(json.dumps(fn(args)), flush=True)
         return
     total = args.warmup + 2 * args.steps  # timed run + profiled rerun
-    strip = set(filter(None, args.strip.split(",")))
-    if strip - {"tfsf", "dft", "pml"}:
-        sys.exit(f"--strip takes tfsf, dft, pml; got {args.strip}")
-    sc = synth.grid3d_cpml_tfsf_dft(n=args.size, nsteps=total, tfsf="tfsf" not in strip, dft="dft" not in strip,
-                                    **({"npml": 0} if "pml" in strip else {}))
-    if "pml" in strip:
-        sc["pml_axes"] = [False, False, False]
+    sc = synth.grid3d_cpml_tfsf_dft(n=args.size, nsteps=total)
     esz = 4 if args.precision == "f32" else 8
     g = apply(Grid("ThreeD", args.size, args.size, args.size, sc["dx"], sc["dt"], precision=args.precision), sc)
     if args.tile:
@@ -619,15 +610,14 @@ def main():
     # device time of the kernels of one step (the aux line runs once per 256-step chunk)
     kern_ms = sum(v for k, v in per_step.items() if k != "aux_line")
     achieved = abytes / (kern_ms / 1e3) / 1e9
-    traffic = step_traffic(tile, per_step, steps=args.steps) if (args.size == 256 and args.precision == "f32"
-                                                                 and not strip) else None
+    traffic = step_traffic(tile, per_step, steps=args.steps) if (args.size == 256 and args.precision == "f32") else None
     out = {
         "metric": METRIC, "value": round(gcells, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
         "config": {"workload": f"3-D {args.size}^3 {args.precision.upper()} forward: 12-cell CPML + 48 eps blocks "
                                f"+ TFSF box inset 20 (x-propagating Ez plane wave) + running DFT of 6 comps on "
-                               f"2 z-planes x 8 freqs" + (f" WITHOUT {', '.join(sorted(strip))}" if strip else ""),
+                               f"2 z-planes x 8 freqs",
                    "cells": args.size ** 3, "interior_cells": interior, "tile": list(tile),
                    "graph_chunks": bool(args.graph)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -1034,7 +1034,26 @@ int step_impl(fdtd_grid* g, int nsteps) {
     if (t.bx == 0) t = kDefaultTileTM;
     t.kc = 1;
   }
-  if ((g->force_two_pass || general || has_magnetized(g)) && t.bx <= 1 && g->ndim == 3) t = kDefaultTile3D;
+  if ((g->force_two_pass || general || has_magnetized(g)) && t.bx <= 2 && g->ndim == 3) t = kDefaultTile3D;
+  // the time-blocked 3-D pass covers grids without dispersive cells or complex
+  // fields whose DFT monitors lie inside the interior (it samples the cells
+  // it updates)
+  if (g->ndim == 3 && t.bx == 2) {
+    bool inner = true;
+    for (int q = 0; q < g->ndft; q++) {
+      const int* r = g->dft[q].r;
+      inner = inner && r[0] >= 1 && r[2] >= 1 && r[4] >= 1 && r[1] <= g->d.nx - 1 && r[3] <= g->d.ny - 1 &&
+              r[5] <= g->d.nz - 1;
+    }
+    if (g->cplx || g->max_np > 0 || !inner || g->design || (g->tfsf && g->aux.L > fdtdk::kTbRowMax))
+      t = kDefaultTile3D;
+    else if (sizeof(T) == 8) {
+      t.by = 8;  // FP64: 256 registers per lane at 8 rows (at 12 they would spill)
+      t.kc = std::min(t.kc, fdtdk::kTbMaxKc64);  // the chunk's TFSF tables in LDS
+    } else if (t.by == 16) {
+      t.by = 12;  // 16 rows would spill at 128 registers per lane
+    }
+  }
   // the streamed pass is FP32, addresses fields with 32-bit buffer offsets and
   // keeps the z coefficients in LDS
   if (g->ndim == 3 && t.bx == 0 && (sizeof(T) != 4 || a.field_bytes == 0 || g->d.nz > fdtdk::kSMaxZ || g->design))
@@ -1042,8 +1061,9 @@ int step_impl(fdtd_grid* g, int nsteps) {
 
   const bool wavef = g->ndim == 3 && t.bx == 1;
   const bool fused = g->ndim == 3 && t.bx <= 1;
+  const bool tb3 = g->ndim == 3 && t.bx == 2;
   // the fused and time-blocked passes write the second copy of the state
-  const bool pp = fused || tmb;
+  const bool pp = fused || tmb || tb3;
   if (pp) {
     int rc = ensure_pingpong(g);
     if (rc) return rc;
@@ -1055,7 +1075,14 @@ int step_impl(fdtd_grid* g, int nsteps) {
                     t.bx == g->cone_tile.bx && t.by == g->cone_tile.by && t.kc == g->cone_tile.kc;
   dim3 block, grid;
   int ntx = 1, nty = 1, nkc = 1;
-  if (wavef) {
+  if (tb3) {
+    // time-blocked pass: cores of 60 x (by - 4) columns, kc planes
+    ntx = (g->d.nx - 2 + fdtdk::kTbCore - 1) / fdtdk::kTbCore;
+    nty = (g->d.ny - 2 + (t.by - 4) - 1) / (t.by - 4);
+    nkc = (nzi + t.kc - 1) / t.kc;
+    block = dim3(fdtdk::kTbCols, t.by, 1);
+    grid = dim3(ntx * nty * nkc, 1, 1);
+  } else if (wavef) {
     // z-marching fused pass: tiles of 63 x (by - 1) owned columns, kc planes
     ntx = (g->d.nx - 2 + fdtdk::kZCols - 2) / (fdtdk::kZCols - 1);
     nty = (g->d.ny - 2 + t.by - 2) / (t.by - 1);
@@ -1134,8 +1161,35 @@ int step_impl(fdtd_grid* g, int nsteps) {
     // the chunk's launches; every step-dependent value comes from the staged
     // arguments and the step index s, so the sequence can be captured once
     // and replayed for later chunks of the same shape
-    const int nlaunch = tmb ? (chunk + t.by - 1) / t.by : chunk;  // launches that swap the state roles
+    // the time-blocked 3-D pass advances two steps per launch; with DFT
+    // monitors a launch starts on an even step, so that a DFT block never
+    // ends on a launch's first step (its fold follows the launch)
+    auto tb_steps = [&](int s) { return (chunk - s >= 2 && (dft_work == 0 || (step0 + s) % 2 == 0)) ? 2 : 1; };
+    int ntb = 0;
+    if (tb3)
+      for (int s = 0; s < chunk; s += tb_steps(s)) ntb++;
+    const int nlaunch = tmb ? (chunk + t.by - 1) / t.by : (tb3 ? ntb : chunk);  // launches that swap the state roles
     auto enqueue = [&]() -> int {
+      if (tb3) {
+        // launch l reads copy l % 2 and writes the other
+        int l = 0;
+        for (int s = 0; s < chunk; l++) {
+          const auto* ap = (l % 2) ? dev1 : dev0;
+          const int nst = tb_steps(s);
+          int rc;
+          if (sizeof(T) == 8 || t.by == 8)
+            rc = launch(g, 3, fdtdk::tb_3d<T, 8>, grid, block, ap, t.kc, s, nst, ntx, nty, nkc);
+          else
+            rc = launch(g, 3, fdtdk::tb_3d<float, 12>, grid, block, (const fdtdk::StepArgs<float>*)ap, t.kc, s, nst,
+                        ntx, nty, nkc);
+          if (!rc && dft_work > 0 && (step0 + s + nst) % fdtdk::kDftBlock == 0)
+            rc = launch(g, 4, fdtdk::dft_fold_step<T>, dim3(unsigned((dft_slots + 255) / 256)), dim3(256), dev0,
+                        s + nst - 1);
+          if (rc) return rc;
+          s += nst;
+        }
+        return 0;
+      }
       if (tmb) {
         // launch l reads copy l % 2 and writes the other
         for (int l = 0; l < nlaunch; l++) {
@@ -2300,8 +2354,13 @@ int fdtd_set_tile(fdtd_grid* g, int bx, int by, int kc) {
     if (g->ndim != 3 || (by != 4 && by != 8 && by != 16) || kc <= 0)
       return fail(FDTD_EINVAL, "the z-marching fused schedule (bx = 1) is 3-D only, with by in {4, 8, 16} rows of 64 "
                                "columns and kc >= 1 planes per workgroup");
-  } else if (bx < 0 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0) {
-    return fail(FDTD_EINVAL, "tile must have a multiple of 64 threads, at most 256, and kc >= 1");
+  } else if (bx == 2) {
+    if (g->ndim != 3 || (by != 8 && by != 12 && by != 16) || kc <= 0 || kc > fdtdk::kTbMaxKc)
+      return fail(FDTD_EINVAL, "the time-blocked schedule (bx = 2) is 3-D only, with by in {8, 12, 16} rows of 64 "
+                               "columns (FP32 runs 16 as 12, FP64 every by as 8 with kc <= 32) and 1 <= kc <= %d planes per workgroup",
+                  fdtdk::kTbMaxKc);
+  } else if (bx < 8 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0) {
+    return fail(FDTD_EINVAL, "a two-pass tile has bx >= 8, a multiple of 64 threads, at most 256, and kc >= 1");
   }
   g->tile = Tile{bx, by, kc};
   return 0;
@@ -2313,8 +2372,9 @@ int fdtd_set_adjoint_tile(fdtd_grid* g, int bx, int by, int kc) {
     g->rtile = Tile{0, 0, 0};
     return 0;
   }
-  if (bx < 2 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
-    return fail(FDTD_EINVAL, "the reverse sweep takes a two-pass tile: a multiple of 64 threads, at most 256, kc >= 1");
+  if (bx < 8 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
+    return fail(FDTD_EINVAL,
+                "the reverse sweep takes a two-pass tile: bx >= 8, a multiple of 64 threads, at most 256, kc >= 1");
   g->rtile = Tile{bx, by, kc};
   return 0;
 }
@@ -2325,7 +2385,7 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   if (!g->finalized && (rc = finalize(g))) return rc;
   static const Tile cand3[] = {{0, 5, 1},  {0, 5, 2},  {256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {256, 1, 8},
                                {128, 2, 2}, {128, 2, 4}, {64, 4, 1}, {64, 4, 2}, {1, 8, 16}, {1, 8, 32},
-                               {1, 16, 16}, {1, 4, 32}};
+                               {1, 16, 16}, {1, 4, 32}, {2, 12, 16}, {2, 12, 32}};
   static const Tile cand2[] = {{256, 1, 1}, {128, 2, 1}, {64, 4, 1}, {64, 1, 1}, {128, 1, 1},
                                {0, 2, 16},  {0, 4, 16},  {0, 4, 24}, {0, 8, 16}, {0, 4, 8}};
   const Tile* cand = g->ndim == 3 ? cand3 : cand2;

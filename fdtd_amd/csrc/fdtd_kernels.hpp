@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 // Pointers that kernels read out of argument blocks in device memory are
 // declared in the global address space for the device compile, so every
 // access through them is a global load/store (not a flat one, which would
@@ -295,7 +297,8 @@ __device__ __forceinline__ const FDTD_G StepArgs<T>& cold_args(const StepArgs<T>
 // row: the step's incident row (global memory, or its copy in LDS); fn: the
 // cell's curl coefficients dt F/dx per axis for this field type (null: read
 // from the argument block, which keeps them out of registers until needed).
-template <typename T, int NDIM, bool COLD = false, typename Row = const FDTD_G double*>
+// J0: the first face axis taken (the time-blocked pass adds the x faces' terms itself)
+template <typename T, int NDIM, bool COLD = false, typename Row = const FDTD_G double*, int J0 = 0>
 __device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a0, Row row, int ft, int i, int j, int k, const bool actF[3],
                                     const bool actC[3], const T fn[3], T delta[3]) {
 #pragma clang fp contract(off)
@@ -305,7 +308,7 @@ __device__ FDTD_COLD void tfsf_cell(const StepArgs<T>& a0, Row row, int ft, int 
   // neither array needs memory (a dynamically indexed one would be placed in
   // LDS or scratch)
 #pragma unroll
-  for (int J = 0; J < NDIM; J++) {
+  for (int J = J0; J < NDIM; J++) {
     int side = (ci[J] == a.box[2 * J]) ? 0 : ((ci[J] == a.box[2 * J + 1]) ? 1 : -1);
     if (side < 0) continue;
     const double nrm = side == 0 ? -1.0 : 1.0;
@@ -602,7 +605,7 @@ struct SrcVal {
 
 template <typename T>
 struct CellPre {
-  T psi[3][2];
+  mutable T psi[3][2];  // in (PRE bit 0) and, with PRE bit 2, out
   T g[3], b[3], c[3];
   const SrcVal<T>* src;  // the step's source values (null: none)
   int nsrc;
@@ -646,7 +649,8 @@ struct EStencil {
 // PRE: bit 0: psi and the slab coefficients come from *pre (gathered by the
 // caller ahead of the cell's arithmetic) instead of memory; bit 1: so do the
 // step's source values (staged), and the TFSF correction takes the curl
-// coefficients from the caller.  Row: the incident row's pointer type (global
+// coefficients from the caller; bit 2: the new psi goes back into pre->psi
+// instead of memory (the time-blocked pass keeps it in registers).  Row: the incident row's pointer type (global
 // or LDS).
 // GEN: the general instance, which also carries E / H soft sources and single
 // TFSF planes (code the other instances leave out, keeping their registers).
@@ -685,7 +689,10 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, Row i
     const T p1 = (PRE & 1) ? pre->psi[0][0] : a.psi[0][1][0][o], p2 = (PRE & 1) ? pre->psi[0][1] : a.psi[0][2][0][o];
     d[1] = fmadd(g, p1, d[1]);
     d[2] = fmadd(g, p2, d[2]);
-    if (own) {
+    if (PRE & 4) {
+      pre->psi[0][0] = fmadd(b, p1, -(cc * dyx));
+      pre->psi[0][1] = fmadd(b, p2, cc * dzx);
+    } else if (own) {
       (IP ? a.psi : a.psio)[0][1][0][o] = fmadd(b, p1, -(cc * dyx));
       (IP ? a.psi : a.psio)[0][2][0][o] = fmadd(b, p2, cc * dzx);
     }
@@ -698,7 +705,10 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, Row i
     const T p0 = (PRE & 1) ? pre->psi[1][0] : a.psi[0][0][1][o], p2 = (PRE & 1) ? pre->psi[1][1] : a.psi[0][2][1][o];
     d[0] = fmadd(g, p0, d[0]);
     d[2] = fmadd(g, p2, d[2]);
-    if (own) {
+    if (PRE & 4) {
+      pre->psi[1][0] = fmadd(b, p0, cc * dxy);
+      pre->psi[1][1] = fmadd(b, p2, -(cc * dzy));
+    } else if (own) {
       (IP ? a.psi : a.psio)[0][0][1][o] = fmadd(b, p0, cc * dxy);
       (IP ? a.psi : a.psio)[0][2][1][o] = fmadd(b, p2, -(cc * dzy));
     }
@@ -711,7 +721,10 @@ __device__ __forceinline__ void e_cell_3d(const StepArgs<T>& a, long step, Row i
     const T p0 = (PRE & 1) ? pre->psi[2][0] : a.psi[0][0][2][o], p1 = (PRE & 1) ? pre->psi[2][1] : a.psi[0][1][2][o];
     d[0] = fmadd(g, p0, d[0]);
     d[1] = fmadd(g, p1, d[1]);
-    if (own) {
+    if (PRE & 4) {
+      pre->psi[2][0] = fmadd(b, p0, -(cc * dxz));
+      pre->psi[2][1] = fmadd(b, p1, cc * dyz);
+    } else if (own) {
       (IP ? a.psi : a.psio)[0][0][2][o] = fmadd(b, p0, -(cc * dxz));
       (IP ? a.psi : a.psio)[0][1][2][o] = fmadd(b, p1, cc * dyz);
     }
@@ -774,8 +787,13 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, Row i
     const T p1 = (PRE & 1) ? pre->psi[0][0] : a.psi[1][1][0][o], p2 = (PRE & 1) ? pre->psi[0][1] : a.psi[1][2][0][o];
     d[1] = fmadd(-g, p1, d[1]);
     d[2] = fmadd(-g, p2, d[2]);
-    (IP ? a.psi : a.psio)[1][1][0][o] = fmadd(b, p1, -(cc * dyx));
-    (IP ? a.psi : a.psio)[1][2][0][o] = fmadd(b, p2, cc * dzx);
+    if (PRE & 4) {
+      pre->psi[0][0] = fmadd(b, p1, -(cc * dyx));
+      pre->psi[0][1] = fmadd(b, p2, cc * dzx);
+    } else {
+      (IP ? a.psi : a.psio)[1][1][0][o] = fmadd(b, p1, -(cc * dyx));
+      (IP ? a.psi : a.psio)[1][2][0][o] = fmadd(b, p2, cc * dzx);
+    }
   }
   const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
   if (syi >= 0) {
@@ -785,8 +803,13 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, Row i
     const T p0 = (PRE & 1) ? pre->psi[1][0] : a.psi[1][0][1][o], p2 = (PRE & 1) ? pre->psi[1][1] : a.psi[1][2][1][o];
     d[0] = fmadd(-g, p0, d[0]);
     d[2] = fmadd(-g, p2, d[2]);
-    (IP ? a.psi : a.psio)[1][0][1][o] = fmadd(b, p0, cc * dxy);
-    (IP ? a.psi : a.psio)[1][2][1][o] = fmadd(b, p2, -(cc * dzy));
+    if (PRE & 4) {
+      pre->psi[1][0] = fmadd(b, p0, cc * dxy);
+      pre->psi[1][1] = fmadd(b, p2, -(cc * dzy));
+    } else {
+      (IP ? a.psi : a.psio)[1][0][1][o] = fmadd(b, p0, cc * dxy);
+      (IP ? a.psi : a.psio)[1][2][1][o] = fmadd(b, p2, -(cc * dzy));
+    }
   }
   const int szi = slab_index(k, a.slab_a[2], a.slab_b[2]);
   if (szi >= 0) {
@@ -796,8 +819,13 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, Row i
     const T p0 = (PRE & 1) ? pre->psi[2][0] : a.psi[1][0][2][o], p1 = (PRE & 1) ? pre->psi[2][1] : a.psi[1][1][2][o];
     d[0] = fmadd(-g, p0, d[0]);
     d[1] = fmadd(-g, p1, d[1]);
-    (IP ? a.psi : a.psio)[1][0][2][o] = fmadd(b, p0, -(cc * dxz));
-    (IP ? a.psi : a.psio)[1][1][2][o] = fmadd(b, p1, cc * dyz);
+    if (PRE & 4) {
+      pre->psi[2][0] = fmadd(b, p0, -(cc * dxz));
+      pre->psi[2][1] = fmadd(b, p1, cc * dyz);
+    } else {
+      (IP ? a.psi : a.psio)[1][0][2][o] = fmadd(b, p0, -(cc * dxz));
+      (IP ? a.psi : a.psio)[1][1][2][o] = fmadd(b, p1, cc * dyz);
+    }
   }
   if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3] || k == a.box[4] || k == a.box[5])) {
     const bool act[3] = {true, true, true};
@@ -2037,6 +2065,587 @@ __global__ __launch_bounds__(kZCols * TY) void fusedz_3d(const StepArgs<T>* __re
     ep[2] = en[2];
     midp = cur.mid;
     cur = nxt;
+  }
+}
+
+// ---------------------------------------------------------------- time-blocked 3-D pass
+// Two steps per launch (nst = 2; nst = 1 runs the first step only).  A
+// workgroup of 64 x TY threads holds the region columns (x0 - 2 + l,
+// y0 - 2 + r), l < 64, r < TY, one cell per thread per plane, and marches z
+// over a chunk of planes [k0, k1) with the four half steps skewed by one
+// plane each: at iteration p it computes
+//   E2 at plane p - 2 (from H1 of planes p - 2 and p - 3),
+//   H2 at plane p - 3 (from E2 of planes p - 3 and p - 2),
+//   E1 at plane p     (from the old H of planes p and p - 1),
+//   H1 at plane p - 1 (from E1 of planes p - 1 and p).
+// E reads its -x, -y neighbours and H its +x, +y ones, so each half step
+// leaves one more column / row on one side without exact inputs: E1 is exact
+// on l >= 1, H1 on [1, 62], E2 on [2, 62], H2 on the core [2, 61] (rows
+// likewise), and the core's 60 x (TY - 4) columns hold both steps' exact
+// values.  The core writes them, with its CPML psi, to the second copy of the
+// state.  Every lane of a row computes each half step its row takes part in
+// (rows outside a half step's range skip it, a wave-uniform branch); the
+// values of the lanes outside the exact range are never read by an exact
+// one.  Cells outside the interior keep their old values (a select).  x
+// neighbours come from the adjacent lane (a wave is one region row, DPP wave
+// shifts), y neighbours through LDS, written in the iteration before the one
+// that reads them: one barrier per plane.  The first step's CPML psi of a
+// cell stays in registers until the second step's update of the cell; the
+// planes' values rotate through three register slots (the loop is unrolled
+// three times, so the rotation is register renaming).  Fields are addressed
+// by 32-bit byte offsets from their bases (one offset register per thread
+// and plane).  The cell arithmetic is e_cell_3d / h_cell_3d's, operation for
+// operation, so the fields, psi, DFT samples and probes are bitwise those of
+// the two passes.  Grids with dispersive cells, boundary conditions, E / H
+// sources, single planes, complex fields, a design box, recorded design
+// history or arrays of 2 GiB or more take the two passes.
+// Everything the march reads besides the fields, ids and psi is staged in LDS
+// first (z coefficients of the chunk, the two steps' incident rows and source
+// values, the DFT monitors of each plane): a uniform value read from global
+// memory inside the loop would be a vector load, and waiting for it would
+// wait for the prefetched planes too.
+constexpr int kTbCols = 64;
+constexpr int kTbCore = kTbCols - 4;
+constexpr int kTbMaxKc = 64;     // planes per workgroup at most (FP32; FP64: kTbMaxKc64)
+constexpr int kTbMaxKc64 = 32;
+constexpr int kTbRowMax = 1024;  // incident-line cells at most
+constexpr int kDppWaveShr1 = 0x138;  // lane l receives lane l - 1
+constexpr int kDppWaveShl1 = 0x130;  // lane l receives lane l + 1
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// an element at a 32-bit byte offset from an array's base (base in scalar
+// registers, offset in a vector register: one address register for all arrays)
+template <typename T>
+__device__ __forceinline__ T ldo(const FDTD_G T* base, unsigned off) {
+  return *(const FDTD_G T*)((const FDTD_G char*)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(FDTD_G T* base, unsigned off, T v) {
+  *(FDTD_G T*)((FDTD_G char*)base + off) = v;
+}
+
+// DFT samples of one cell after a step (the time-blocked form of
+// dft_sample_block: the same slots), fld 0: E values, 1: H values
+template <typename T>
+__device__ FDTD_COLD void tb_sample(const StepArgs<T>& a, long step, int i, int j, int k, int fld, const T v[3]) {
+  for (int q = 0; q < a.ndft; q++) {
+    const DftDev<T>& d = a.dft[q];
+    if (i < d.r[0] || i >= d.r[1] || j < d.r[2] || j >= d.r[3] || k < d.r[4] || k >= d.r[5]) continue;
+    const int wx = d.r[1] - d.r[0], wy = d.r[3] - d.r[2];
+    const long lq = (long)(i - d.r[0]) + (long)wx * ((long)(j - d.r[2]) + (long)wy * (k - d.r[4]));
+    const long slot0 = (long)((int)(step % kDftBlock) * d.ncomp);
+    int slot = 0;
+    for (int comp = 0; comp < 6; comp++) {
+      if (!((d.mask >> comp) & 1)) continue;
+      if ((comp < 3) == (fld == 0)) d.smp[(slot0 + slot) * d.ncells + lq] = v[comp % 3];
+      slot++;
+    }
+  }
+}
+
+template <int N>
+using IC = std::integral_constant<int, N>;
+
+template <typename T, int TY>
+__global__ __launch_bounds__(kTbCols * TY) void tb_3d(const StepArgs<T>* __restrict__ ap, int kc, int s0, int nst,
+                                                     int ntx, int nty, int nkc) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = *ap;
+  __shared__ MatLds<T> sm;
+  // y-neighbour exchange [buffer][quantity][row][lane]: H0(p+1) hz, hx; E1(p)
+  // ez, ex; H1(p-1) hz, hx; E2(p-2) ez, ex (written at iteration p)
+  __shared__ T xs[2][8][TY][kTbCols];
+  __shared__ T zc[2][4][kTbMaxKc + 5];   // z coefficients dt F/dx, dt G, B, C/dx of planes k0 - 2 .. k1 + 2
+  __shared__ double rows[4][kTbRowMax];  // incident rows: aux H of steps 1, 2; aux E of steps 1, 2
+  __shared__ SrcVal<T> ssrc[2][kMaxSrc];  // the two steps' source values (sources still running)
+  __shared__ int nss[2];
+  __shared__ unsigned char dpl[kTbMaxKc + 5];  // DFT monitors holding each plane (bit per monitor)
+  // TFSF x-face terms of every plane of the chunk, computed by the whole
+  // workgroup before the march (a few terms per thread) instead of by every
+  // wave holding a face lane at every half step: [plane][field type][step]
+  // [face side][row][I - 1], and whether the component's face cell lies
+  // inside the box's face (the term is added)
+  constexpr int kTab = (sizeof(T) == 4 ? kTbMaxKc : kTbMaxKc64) + 5;
+  __shared__ T xt[kTab][2][2][2][TY][2];
+  __shared__ unsigned char xin[kTab][2][2][2][TY][2];
+  __shared__ T xfc[2][2];  // dt F/dx along x at the two x faces [field type][side]
+  const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
+  const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
+  // a wave is one row of the region: the row index is wave-uniform
+  const int l = threadIdx.x, r = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int i = bx * kTbCore - 1 + l, j = by * (TY - 4) - 1 + r;
+  const int k0 = 1 + bz * kc, k1 = min(k0 + kc, a.nz - 1);
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int tid = l + kTbCols * r;
+  // ---- stage the launch's tables
+  const int kz0 = k0 - 2, nzs = min(k1 + 2, nz - 1) - kz0 + 1;
+  for (int q = tid; q < 8 * nzs; q += kTbCols * TY) {
+    const int ft = q / (4 * nzs), w = (q / nzs) % 4, z = q % nzs, k = kz0 + z;
+    const FDTD_G T* src = w == 0 ? a.cf[ft][2] : (w == 1 ? a.cg[ft][2] : (w == 2 ? a.cb[ft][2] : a.cc[ft][2]));
+    zc[ft][w][z] = k >= 0 ? src[k] : T(0);
+  }
+  for (int q = tid; q < nzs; q += kTbCols * TY) {
+    unsigned m = 0;
+    for (int d = 0; d < a.ndft; d++)
+      if (kz0 + q >= a.dft[d].r[4] && kz0 + q < a.dft[d].r[5]) m |= 1u << d;
+    dpl[q] = (unsigned char)m;
+  }
+  if (a.tfsf)
+    for (int q = tid; q < a.auxL * nst; q += kTbCols * TY) {
+      const int u = q / a.auxL, x = q - u * a.auxL;
+      rows[u][x] = a.incH[(long)(s0 + u) * a.auxL + x];
+      rows[2 + u][x] = a.incE[(long)(s0 + u) * a.auxL + x];
+    }
+  if (a.tfsf && tid < 4) xfc[tid >> 1][tid & 1] = a.cf[tid >> 1][0][min(max(a.box[tid & 1], 0), a.nx - 1)];
+  if (tid < nst) {
+    const long step = a.step0 + s0 + tid;
+    int n = 0;
+    for (int q = 0; q < a.nsrc; q++)  // add_sources adds a source while step < its table length
+      if (step < a.src[q].n)
+        ssrc[tid][n++] = SrcVal<T>{a.src[q].cell, a.src[q].field, a.src[q].comp, (T)a.src[q].table[step]};
+    nss[tid] = n;
+  }
+  stage_materials(a, sm);  // (its barrier also covers the tables above)
+
+  // ---- the thread's column
+  const bool incol = i >= 1 && j >= 1 && i <= nx - 2 && j <= ny - 2;  // an interior column (updated)
+  const bool own = incol && l >= 2 && l <= kTbCols - 3 && r >= 2 && r <= TY - 3;
+  const bool two = nst > 1;
+  const int rm = r > 0 ? r - 1 : 0, rp = r < TY - 1 ? r + 1 : TY - 1;
+  // columns off the grid load (and compute from) the nearest grid column:
+  // nothing exact reads them
+  const int ic = min(max(i, 0), nx - 1), jc = min(max(j, 0), ny - 1);
+  const unsigned c0 = (unsigned)ic + (unsigned)nx * (unsigned)jc;  // cell index of the column's plane 0
+  const unsigned pstride = (unsigned)nx * (unsigned)ny;
+  const unsigned vo = c0 * (unsigned)sizeof(T), pb = pstride * (unsigned)sizeof(T);
+  const long st1 = a.step0 + s0, st2 = st1 + 1;
+  const int ci = incol ? i : 1, cj = incol ? j : 1;
+  const T fxE = a.cf[0][0][ci], fyE = a.cf[0][1][cj], fxH = a.cf[1][0][ci], fyH = a.cf[1][1][cj];
+  const int sxi = incol ? slab_index(i, a.slab_a[0], a.slab_b[0]) : -1;
+  const int syi = incol ? slab_index(j, a.slab_a[1], a.slab_b[1]) : -1;
+  T gxE = T(0), bxE = T(0), cxE = T(0), gyE = T(0), byE = T(0), cyE = T(0);
+  T gxH = T(0), bxH = T(0), cxH = T(0), gyH = T(0), byH = T(0), cyH = T(0);
+  if (sxi >= 0) {
+    gxE = a.cg[0][0][i];
+    bxE = a.cb[0][0][i];
+    cxE = a.cc[0][0][i];
+    gxH = a.cg[1][0][i];
+    bxH = a.cb[1][0][i];
+    cxH = a.cc[1][0][i];
+  }
+  if (syi >= 0) {
+    gyE = a.cg[0][1][j];
+    byE = a.cb[0][1][j];
+    cyE = a.cc[0][1][j];
+    gyH = a.cg[1][1][j];
+    byH = a.cb[1][1][j];
+    cyH = a.cc[1][1][j];
+  }
+  // psi byte offsets: x slab (sxi + S0 (j + ny k)), y slab (i + nx (syi + S1 k)),
+  // z slab (i + nx (j + ny szi)); the k parts per plane
+  const unsigned oxb = sxi >= 0 ? ((unsigned)sxi + (unsigned)a.slab_S[0] * (unsigned)j) * (unsigned)sizeof(T) : 0u;
+  const unsigned oyb = syi >= 0 ? ((unsigned)i + (unsigned)nx * (unsigned)syi) * (unsigned)sizeof(T) : 0u;
+  const unsigned kxb = (unsigned)a.slab_S[0] * (unsigned)ny * (unsigned)sizeof(T);
+  const unsigned kyb = (unsigned)nx * (unsigned)a.slab_S[1] * (unsigned)sizeof(T);
+  // a column holding a source cell adds the staged source values; one holding
+  // a probe cell records it; DFT monitors whose x, y range holds the column
+  bool scol = false, pcol = false;
+  for (int q = 0; q < a.nsrc; q++) scol = scol || (incol && (unsigned)(a.src[q].cell % (long)pstride) == c0);
+  for (int q = 0; q < a.nprobe; q++) pcol = pcol || (own && (unsigned)(a.probe[q].cell % (long)pstride) == c0);
+  unsigned dcol = 0;
+  for (int q = 0; q < a.ndft; q++)
+    if (own && i >= a.dft[q].r[0] && i < a.dft[q].r[1] && j >= a.dft[q].r[2] && j < a.dft[q].r[3]) dcol |= 1u << q;
+  // TFSF: the x faces' lanes add the workgroup's precomputed terms; y-face
+  // rows and z-face planes compute theirs in the cell update (every lane of
+  // such a wave is a face cell)
+  const int lf0 = a.tfsf ? a.box[0] - (bx * kTbCore - 1) : -1, lf1 = a.tfsf ? a.box[1] - (bx * kTbCore - 1) : -1;
+  const bool xf0 = lf0 >= 0 && lf0 < kTbCols, xf1 = lf1 >= 0 && lf1 < kTbCols && a.box[1] != a.box[0];
+  const bool xfaces = xf0 || xf1;
+  const int xside = (xf0 && l == lf0) ? 0 : ((xf1 && l == lf1) ? 1 : -1);
+  const bool tfy = a.tfsf && incol && (j == a.box[2] || j == a.box[3]);
+  // wave-uniform guards of the lane-dependent branches (most waves take none:
+  // the uniform test costs a scalar branch, the lane mask a save / restore)
+  const bool wx = __ballot(sxi >= 0) != 0, wy = __ballot(syi >= 0) != 0, wsrc = __ballot(scol) != 0;
+  const bool wxf = __ballot(xside >= 0) != 0, wty = __ballot(tfy) != 0;
+  const bool wd = __ballot(dcol != 0) != 0, wp = __ballot(pcol) != 0;
+  // the half steps this row takes part in
+  const bool doE1 = r >= 1, doH1 = r >= 1 && r <= TY - 2, doE2 = two && r >= 2 && r <= TY - 2,
+             doH2 = two && r >= 2 && r <= TY - 3;
+  // the planes each half step covers
+  const int e1hi = min(k1 + 1, nz - 2), h1lo = max(1, k0 - 1), h1hi = min(k1, nz - 2), e2hi = min(k1, nz - 2);
+  const int sza = a.slab_a[2], szb = a.slab_b[2];
+  const bool tz = a.tfsf != 0;
+  const int bz4 = a.box[4], bz5 = a.box[5];
+  // the x-face terms: tfsf_cell's J = 0 terms, operation for operation
+  auto xterm = [&](int x) {
+    const int c = x & 1, rr = (x >> 1) % TY, sd = (x / (2 * TY)) & 1, u = (x / (4 * TY)) & 1,
+              ft = (x / (8 * TY)) & 1, z = x / (16 * TY);
+    if (!(sd == 0 ? xf0 : xf1)) return;
+    const int ci3[3] = {a.box[sd], by * (TY - 4) - 1 + rr, kz0 + z};
+    const int I = 1 + c, K = 2 - c;
+    bool inside = true;
+#pragma unroll
+    for (int Tn = 1; Tn < 3; Tn++) {
+      const bool half = ft == 0 ? (Tn != I) : (Tn == I);
+      const int lo = half ? a.box[2 * Tn] + 1 : a.box[2 * Tn];
+      const int hi = half ? a.box[2 * Tn + 1] : a.box[2 * Tn + 1] - 1;
+      if (ci3[Tn] < lo || ci3[Tn] > hi) inside = false;
+    }
+    double proj = 0.0;
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+      const double off = ft == 0 ? (dd == K ? -0.5 : 0.0) : (dd != K ? -0.5 : 0.0);
+      const double loc = fmadd(off, a.dx, ((double)ci3[dd] - a.origin[dd]) * a.dx);
+      proj = fmadd(a.kvec[dd], loc, proj);
+    }
+    const double inc = interp_row((const double*)rows[(ft ? 2 : 0) + u], a.auxL, proj, a.dx, ft == 0);
+    const double pv = ft == 0 ? a.polH[K] : a.pol[K];
+    const double f = (double)xfc[ft][sd];
+    const double term = lc3(I, 0) * f * (sd == 0 ? -1.0 : 1.0) * inc * pv;
+    xt[z][ft][u][sd][rr][c] = (T)(ft == 0 ? term : -term);
+    xin[z][ft][u][sd][rr][c] = inside ? 1 : 0;
+  };
+  if (xfaces) {
+    for (int x = tid; x < nzs * 16 * TY; x += kTbCols * TY) xterm(x);
+    __syncthreads();
+  }
+
+  // ---- the cell updates (e_cell_3d / h_cell_3d's arithmetic)
+  auto add_src = [&](int u, int field, int q, T d[3]) {
+    const long c = (long)c0 + (long)pstride * q;
+    for (int s = 0; s < nss[u]; s++) {
+      const SrcVal<T>& v = ssrc[u][s];
+      if (v.cell == c && v.field == field) {
+        if (v.comp == 0)
+          d[0] += v.v;
+        else if (v.comp == 1)
+          d[1] += v.v;
+        else
+          d[2] += v.v;
+      }
+    }
+  };
+  // new E of the cell at plane q, step u, from its H stencil; ps: psi in / out
+  auto e_upd = [&](int u, int q, T hx, T hy, T hz, T hz_ym, T hz_xm, T hy_xm, T hy_zm, T hx_ym, T hx_zm, T ex,
+                   T ey, T ez, T ie, T (&ps)[3][2], T out[3]) {
+    const int z = q - kz0;
+    const T fz = zc[0][0][z];
+    const T dxy = hz - hz_ym;  // Ex, J=y, +1
+    const T dxz = hy - hy_zm;  // Ex, J=z, -1
+    const T dyz = hx - hx_zm;  // Ey, J=z, +1
+    const T dyx = hz - hz_xm;  // Ey, J=x, -1
+    const T dzx = hy - hy_xm;  // Ez, J=x, +1
+    const T dzy = hx - hx_ym;  // Ez, J=y, -1
+    T d[3];
+    d[0] = fmadd(fyE, dxy, -(fz * dxz));
+    d[1] = fmadd(fz, dyz, -(fxE * dyx));
+    d[2] = fmadd(fxE, dzx, -(fyE * dzy));
+    if (wsrc && scol) add_src(u, 0, q, d);
+    if (wx && sxi >= 0) {
+      const T p1 = ps[0][0], p2 = ps[0][1];
+      d[1] = fmadd(gxE, p1, d[1]);
+      d[2] = fmadd(gxE, p2, d[2]);
+      ps[0][0] = fmadd(bxE, p1, -(cxE * dyx));
+      ps[0][1] = fmadd(bxE, p2, cxE * dzx);
+    }
+    if (wy && syi >= 0) {
+      const T p0 = ps[1][0], p2 = ps[1][1];
+      d[0] = fmadd(gyE, p0, d[0]);
+      d[2] = fmadd(gyE, p2, d[2]);
+      ps[1][0] = fmadd(byE, p0, cyE * dxy);
+      ps[1][1] = fmadd(byE, p2, -(cyE * dzy));
+    }
+    if (q < sza || q >= szb) {
+      const T g = zc[0][1][z], b = zc[0][2][z], cc = zc[0][3][z];
+      const T p0 = ps[2][0], p1 = ps[2][1];
+      d[0] = fmadd(g, p0, d[0]);
+      d[1] = fmadd(g, p1, d[1]);
+      ps[2][0] = fmadd(b, p0, -(cc * dxz));
+      ps[2][1] = fmadd(b, p1, cc * dyz);
+    }
+    if (wxf && xside >= 0) {
+      if (xin[z][0][u][xside][r][0]) d[1] += xt[z][0][u][xside][r][0];
+      if (xin[z][0][u][xside][r][1]) d[2] += xt[z][0][u][xside][r][1];
+    }
+    if ((tz && (q == bz4 || q == bz5)) || (wty && tfy)) {
+      const bool act[3] = {true, true, true};
+      const T fn[3] = {fxE, fyE, fz};
+      tfsf_cell<T, 3, false, const double*, 1>(a, rows[u], 0, i, j, q, act, act, fn, d);
+    }
+    if (ie > T(0)) {
+      out[0] = fmadd(d[0], ie, ex);
+      out[1] = fmadd(d[1], ie, ey);
+      out[2] = fmadd(d[2], ie, ez);
+    } else {
+      out[0] = ex;
+      out[1] = ey;
+      out[2] = ez;
+    }
+  };
+  auto h_upd = [&](int u, int q, T ex, T ey, T ez, T ez_yp, T ez_xp, T ey_xp, T ey_zp, T ex_yp, T ex_zp, T hx,
+                   T hy, T hz, T im, T (&ps)[3][2], T out[3]) {
+    const int z = q - kz0;
+    const T fz = zc[1][0][z];
+    const T dxy = ez_yp - ez;  // Hx, J=y, +1
+    const T dxz = ey_zp - ey;  // Hx, J=z, -1
+    const T dyz = ex_zp - ex;  // Hy, J=z, +1
+    const T dyx = ez_xp - ez;  // Hy, J=x, -1
+    const T dzx = ey_xp - ey;  // Hz, J=x, +1
+    const T dzy = ex_yp - ex;  // Hz, J=y, -1
+    T d[3];
+    d[0] = fmadd(fz, dxz, -(fyH * dxy));
+    d[1] = fmadd(fxH, dyx, -(fz * dyz));
+    d[2] = fmadd(fyH, dzy, -(fxH * dzx));
+    if (wsrc && scol) add_src(u, 2, q, d);
+    if (wx && sxi >= 0) {
+      const T p1 = ps[0][0], p2 = ps[0][1];
+      d[1] = fmadd(-gxH, p1, d[1]);
+      d[2] = fmadd(-gxH, p2, d[2]);
+      ps[0][0] = fmadd(bxH, p1, -(cxH * dyx));
+      ps[0][1] = fmadd(bxH, p2, cxH * dzx);
+    }
+    if (wy && syi >= 0) {
+      const T p0 = ps[1][0], p2 = ps[1][1];
+      d[0] = fmadd(-gyH, p0, d[0]);
+      d[2] = fmadd(-gyH, p2, d[2]);
+      ps[1][0] = fmadd(byH, p0, cyH * dxy);
+      ps[1][1] = fmadd(byH, p2, -(cyH * dzy));
+    }
+    if (q < sza || q >= szb) {
+      const T g = zc[1][1][z], b = zc[1][2][z], cc = zc[1][3][z];
+      const T p0 = ps[2][0], p1 = ps[2][1];
+      d[0] = fmadd(-g, p0, d[0]);
+      d[1] = fmadd(-g, p1, d[1]);
+      ps[2][0] = fmadd(b, p0, -(cc * dxz));
+      ps[2][1] = fmadd(b, p1, cc * dyz);
+    }
+    if (wxf && xside >= 0) {
+      if (xin[z][1][u][xside][r][0]) d[1] += xt[z][1][u][xside][r][0];
+      if (xin[z][1][u][xside][r][1]) d[2] += xt[z][1][u][xside][r][1];
+    }
+    if ((tz && (q == bz4 || q == bz5)) || (wty && tfy)) {
+      const bool act[3] = {true, true, true};
+      const T fn[3] = {fxH, fyH, fz};
+      tfsf_cell<T, 3, false, const double*, 1>(a, rows[2 + u], 1, i, j, q, act, act, fn, d);
+    }
+    out[0] = fmadd(d[0], im, hx);
+    out[1] = fmadd(d[1], im, hy);
+    out[2] = fmadd(d[2], im, hz);
+  };
+  // psi of field type ft at plane q (the slabs the cell is in)
+  auto load_psi = [&](int ft, int q, T (&ps)[3][2]) {
+    if (q < 1 || q > nz - 2) return;
+    if (wx && sxi >= 0) {
+      const unsigned o = oxb + kxb * (unsigned)q;
+      ps[0][0] = ldo(a.psi[ft][1][0], o);
+      ps[0][1] = ldo(a.psi[ft][2][0], o);
+    }
+    if (wy && syi >= 0) {
+      const unsigned o = oyb + kyb * (unsigned)q;
+      ps[1][0] = ldo(a.psi[ft][0][1], o);
+      ps[1][1] = ldo(a.psi[ft][2][1], o);
+    }
+    const int szi = slab_index(q, sza, szb);
+    if (szi >= 0) {
+      const unsigned o = vo + pb * (unsigned)szi;
+      ps[2][0] = ldo(a.psi[ft][0][2], o);
+      ps[2][1] = ldo(a.psi[ft][1][2], o);
+    }
+  };
+  auto store_psi = [&](int ft, int q, const T (&ps)[3][2]) {
+    if (wx && sxi >= 0) {
+      const unsigned o = oxb + kxb * (unsigned)q;
+      sto(a.psio[ft][1][0], o, ps[0][0]);
+      sto(a.psio[ft][2][0], o, ps[0][1]);
+    }
+    if (wy && syi >= 0) {
+      const unsigned o = oyb + kyb * (unsigned)q;
+      sto(a.psio[ft][0][1], o, ps[1][0]);
+      sto(a.psio[ft][2][1], o, ps[1][1]);
+    }
+    const int szi = slab_index(q, sza, szb);
+    if (szi >= 0) {
+      const unsigned o = vo + pb * (unsigned)szi;
+      sto(a.psio[ft][0][2], o, ps[2][0]);
+      sto(a.psio[ft][1][2], o, ps[2][1]);
+    }
+  };
+
+  // ---- register slots: plane q's values live in slot (q - k0 + 1) mod 3
+  struct Ld {
+    T e[3], h[3];
+    int mid;
+  };
+  Ld L[3];        // old state: planes p (slot A), p + 1 (C); plane p + 2 is loaded into p - 1's (B)
+  T E1[3][3];     // E1 of planes p (A), p - 1 (B), p - 2 (C)
+  T H1[3][3];     // H1 of planes p - 1 (B), p - 2 (C), p - 3 (A)
+  T E2[3][3];     // E2 of planes p - 2 (C), p - 3 (A)
+  T PE[3][3][2];  // first-step E psi of planes p - 2 (C), p - 1 (B), p (A: loaded old psi, then updated)
+  T PH[3][3][2];  // first-step H psi of planes p - 3 (A), p - 2 (C), p - 1 (B: loaded old psi, then updated)
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      L[s].e[c] = L[s].h[c] = E1[s][c] = H1[s][c] = E2[s][c] = T(0);
+      PE[s][c][0] = PE[s][c][1] = PH[s][c][0] = PH[s][c][1] = T(0);
+    }
+    L[s].mid = 0;
+  }
+  auto load = [&](int q, Ld& d) {
+    const int qc = min(max(q, 0), nz - 1);
+    const unsigned o = vo + pb * (unsigned)qc;
+    d.h[0] = ldo(a.H[0], o);
+    d.h[1] = ldo(a.H[1], o);
+    d.h[2] = ldo(a.H[2], o);
+    d.e[0] = ldo(a.E[0], o);
+    d.e[1] = ldo(a.E[1], o);
+    d.e[2] = ldo(a.E[2], o);
+    d.mid = ldo(a.mat, c0 + pstride * (unsigned)qc);
+  };
+  load(k0 - 2, L[2]);
+  load(k0 - 1, L[0]);
+  load(k0, L[1]);
+  load_psi(0, k0 - 1, PE[0]);
+  unsigned mids = (unsigned)L[2].mid;  // material ids of planes p - 1, p - 2, p - 3 (bytes 0..2)
+  // prologue: the y neighbours of the first iteration's old H
+  {
+    const int wb = (k0 - 2) & 1;
+#pragma unroll
+    for (int q = 2; q < 8; q++) xs[wb][q][r][l] = T(0);
+    xs[wb][0][r][l] = L[0].h[2];
+    xs[wb][1][r][l] = L[0].h[0];
+  }
+  s_bar_lds();
+
+  auto iter = [&](int p, auto sa, auto sb, auto sc) {
+    constexpr int A = decltype(sa)::value, B = decltype(sb)::value, C = decltype(sc)::value;
+    const int rb = (p - 1) & 1, wb = p & 1;
+    const int m0 = L[A].mid, m1 = mids & 255, m2 = (mids >> 8) & 255, m3 = (mids >> 16) & 255;
+    // plane p - 1's old H leaves slot B, plane p + 2's loads go into it
+    const T h0p[3] = {L[B].h[0], L[B].h[1], L[B].h[2]};
+    load(p + 2, L[B]);
+    // ---- E2 at plane q2 = p - 2 (second step)
+    const int q2 = p - 2;
+    if (doE2 && q2 >= k0 && q2 <= e2hi) {
+      const T hz_xm = dpp_mov<kDppWaveShr1>(H1[C][2]), hy_xm = dpp_mov<kDppWaveShr1>(H1[C][1]);
+      T o[3];
+      e_upd(1, q2, H1[C][0], H1[C][1], H1[C][2], xs[rb][4][rm][l], hz_xm, hy_xm, H1[A][1], xs[rb][5][rm][l],
+            H1[A][0], E1[C][0], E1[C][1], E1[C][2], sm.ie[m2], PE[C], o);
+#pragma unroll
+      for (int c = 0; c < 3; c++) E2[C][c] = incol ? o[c] : E1[C][c];
+      if (q2 < k1 && own) {
+        const unsigned so = vo + pb * (unsigned)q2;
+        sto(a.Eo[0], so, o[0]);
+        sto(a.Eo[1], so, o[1]);
+        sto(a.Eo[2], so, o[2]);
+        store_psi(0, q2, PE[C]);
+        if (wd && dcol && (dcol & dpl[q2 - kz0])) tb_sample(a, st2, i, j, q2, 0, o);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) E2[C][c] = E1[C][c];
+    }
+    // ---- H2 at plane q3 = p - 3 (second step, core rows)
+    const int q3 = p - 3;
+    if (doH2 && q3 >= k0 && q3 < k1) {
+      const T ez_xp = dpp_mov<kDppWaveShl1>(E2[A][2]), ey_xp = dpp_mov<kDppWaveShl1>(E2[A][1]);
+      T o[3];
+      h_upd(1, q3, E2[A][0], E2[A][1], E2[A][2], xs[rb][6][rp][l], ez_xp, ey_xp, E2[C][1], xs[rb][7][rp][l],
+            E2[C][0], H1[A][0], H1[A][1], H1[A][2], sm.im[m3], PH[A], o);
+      if (own) {
+        const unsigned so = vo + pb * (unsigned)q3;
+        sto(a.Ho[0], so, o[0]);
+        sto(a.Ho[1], so, o[1]);
+        sto(a.Ho[2], so, o[2]);
+        store_psi(1, q3, PH[A]);
+        if (wd && dcol && (dcol & dpl[q3 - kz0])) tb_sample(a, st2, i, j, q3, 1, o);
+        if (wp && pcol) {
+          const T ev[3] = {E2[A][0], E2[A][1], E2[A][2]};
+          dft_probe(a, st2, (long)c0 + (long)pstride * q3, i, j, q3, ev, o);
+        }
+      }
+    }
+    // ---- E1 at plane p (first step)
+    if (doE1 && p >= 1 && p <= e1hi) {
+      const T hz_xm = dpp_mov<kDppWaveShr1>(L[A].h[2]), hy_xm = dpp_mov<kDppWaveShr1>(L[A].h[1]);
+      T o[3];
+      e_upd(0, p, L[A].h[0], L[A].h[1], L[A].h[2], xs[rb][0][rm][l], hz_xm, hy_xm, h0p[1], xs[rb][1][rm][l],
+            h0p[0], L[A].e[0], L[A].e[1], L[A].e[2], sm.ie[m0], PE[A], o);
+#pragma unroll
+      for (int c = 0; c < 3; c++) E1[A][c] = incol ? o[c] : L[A].e[c];
+      if (p >= k0 && p < k1 && own) {
+        if (!two) {
+          const unsigned so = vo + pb * (unsigned)p;
+          sto(a.Eo[0], so, o[0]);
+          sto(a.Eo[1], so, o[1]);
+          sto(a.Eo[2], so, o[2]);
+          store_psi(0, p, PE[A]);
+        }
+        if (wd && dcol && (dcol & dpl[p - kz0])) tb_sample(a, st1, i, j, p, 0, o);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) E1[A][c] = L[A].e[c];
+    }
+    // ---- H1 at plane q1 = p - 1 (first step)
+    const int q1 = p - 1;
+    if (doH1 && q1 >= h1lo && q1 <= h1hi) {
+      const T ez_xp = dpp_mov<kDppWaveShl1>(E1[B][2]), ey_xp = dpp_mov<kDppWaveShl1>(E1[B][1]);
+      T o[3];
+      h_upd(0, q1, E1[B][0], E1[B][1], E1[B][2], xs[rb][2][rp][l], ez_xp, ey_xp, E1[A][1], xs[rb][3][rp][l],
+            E1[A][0], h0p[0], h0p[1], h0p[2], sm.im[m1], PH[B], o);
+#pragma unroll
+      for (int c = 0; c < 3; c++) H1[B][c] = incol ? o[c] : h0p[c];
+      if (q1 >= k0 && q1 < k1 && own) {
+        if (!two) {
+          const unsigned so = vo + pb * (unsigned)q1;
+          sto(a.Ho[0], so, o[0]);
+          sto(a.Ho[1], so, o[1]);
+          sto(a.Ho[2], so, o[2]);
+          store_psi(1, q1, PH[B]);
+        }
+        if (wd && dcol && (dcol & dpl[q1 - kz0])) tb_sample(a, st1, i, j, q1, 1, o);
+        if (wp && pcol) {
+          const T ev[3] = {E1[B][0], E1[B][1], E1[B][2]};
+          dft_probe(a, st1, (long)c0 + (long)pstride * q1, i, j, q1, ev, o);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) H1[B][c] = h0p[c];
+    }
+    // the next iteration's old psi: E at plane p + 1 (slot C, whose first-step
+    // psi of plane p - 2 the E2 update above consumed), H at plane p (slot A,
+    // whose plane p - 3 the H2 update consumed)
+    load_psi(0, p + 1, PE[C]);
+    load_psi(1, p, PH[A]);
+    // the y neighbours the next iteration reads
+    xs[wb][0][r][l] = L[C].h[2];
+    xs[wb][1][r][l] = L[C].h[0];
+    xs[wb][2][r][l] = E1[A][2];
+    xs[wb][3][r][l] = E1[A][0];
+    xs[wb][4][r][l] = H1[B][2];
+    xs[wb][5][r][l] = H1[B][0];
+    xs[wb][6][r][l] = E2[C][2];
+    xs[wb][7][r][l] = E2[C][0];
+    s_bar_lds();  // LDS only: the loads and stores in flight need not land first
+    mids = ((mids << 8) | (unsigned)m0) & 0xffffffu;
+  };
+  for (int p = k0 - 1; p <= k1 + 2; p += 3) {
+    iter(p, IC<0>{}, IC<2>{}, IC<1>{});
+    if (p + 1 <= k1 + 2) iter(p + 1, IC<1>{}, IC<0>{}, IC<2>{});
+    if (p + 2 <= k1 + 2) iter(p + 2, IC<2>{}, IC<1>{}, IC<0>{});
   }
 }
 

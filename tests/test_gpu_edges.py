@@ -3,8 +3,9 @@
 - The smallest grids (one interior cell: 3x3x3 and 3x3 TM) under the
   two-pass tiles, whose blocks are far larger than the grid.
 - A ragged 3-D grid (37 x 29 x 23, no size a multiple of any tile) under every
-  schedule: two-pass tiles, the streamed and z-marching fused passes, captured
-  chunks; all give the same bits and match the oracle.
+  schedule: two-pass tiles, the streamed and z-marching fused passes, the
+  time-blocked tile (which runs the two passes here: the grid is dispersive),
+  captured chunks; all give the same bits and match the oracle.
 - Every table at its limit (16 sources, 32 probes, 8 DFT monitors, 256
   material ids, 3 poles) and one past it (refused).
 - DFT monitors of a single cell and monitors whose phase tables end before the
@@ -101,7 +102,7 @@ def test_ragged_grid_every_schedule(precision):
     ref = None
     o = None
     for tile, graph in (((64, 4, 2), False), ((256, 1, 4), False), ((0, 5, 1), False), ((1, 8, 5), False),
-                        ((128, 2, 4), True)):
+                        ((2, 12, 6), False), ((128, 2, 4), True)):
         print(f"[gpu-case] ragged {precision} tile {tile} graph {graph}", flush=True)
         g, oo = _pair("ThreeD", n, dx, dt, precision, setup)
         g.set_tile(*tile)

@@ -13,7 +13,7 @@ import pytest
 
 from oracle import refcpu
 from parity import bin_errors, worst_bin
-from scenarios import D, E, B, H, P, J, apply, from_golden, rel_l2
+from scenarios import D, E, B, H, P, J, apply, from_golden, gauss_table, rel_l2
 import synth
 
 pytestmark = pytest.mark.gpu
@@ -128,7 +128,7 @@ def test_tile_independence_and_autotune_restores_state(precision):
     ref = apply(_grid(sc, precision), sc)
     ref.step(80)
     outs = {nm: ref.get(f, c) for nm, f, c in _comps(sc)}
-    for tile in [(64, 4, 1), (128, 2, 4), (64, 1, 8)]:
+    for tile in [(64, 4, 1), (128, 2, 4), (64, 1, 8), (2, 12, 8)]:
         g = apply(_grid(sc, precision), sc)
         g.set_tile(*tile)
         g.step(30)
@@ -177,8 +177,11 @@ def test_autotune_as_first_call():
 # per compute unit; FP64 grids run the two-pass default); (1, TY, kc):
 # z-marching fused pass (64 x TY workgroups owning 63 x (TY - 1) columns:
 # nx=300 gives five tiles, the last 47 columns wide, ny=23 a partly empty last
-# row of tiles, and the streamed pass three tiles, the last 44 columns wide)
-FUSED = [(0, 5, 1), (0, 5, 2), (1, 4, 7), (1, 8, 16), (1, 16, 5)]
+# row of tiles, and the streamed pass three tiles, the last 44 columns wide);
+# (2, TY, kc): time-blocked pass, two steps per launch (cores of 60 x (TY - 4)
+# columns: five tiles along x, the last 58 wide; 17 steps end on a one-step
+# launch)
+FUSED = [(0, 5, 1), (0, 5, 2), (1, 4, 7), (1, 8, 16), (1, 16, 5), (2, 8, 16), (2, 12, 7), (2, 16, 5)]
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -243,7 +246,9 @@ def test_fused_pass_against_oracle(precision):
     for sc, tile in ((synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (0, 5, 1)),
                      (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (0, 5, 2)),
                      (synth.grid3d_dispersive_sphere(n=30, radius=7, nsteps=90), (1, 8, 8)),
-                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (1, 4, 16))):
+                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (1, 4, 16)),
+                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (2, 12, 9)),
+                     (synth.grid3d_cpml_tfsf_dft(n=(37, 29, 33), nsteps=90), (2, 8, 40))):
         print(f"[gpu-case] fused-vs-oracle {sc['name']} {tile} {precision}", flush=True)
         g = apply(_grid(sc, precision), sc)
         g.set_tile(*tile)
@@ -487,3 +492,69 @@ def test_batch_dft_blocks_and_phases():
         np.testing.assert_array_equal(g.get(E, 2), r.get(E, 2))
         for idx in range(len(sc["dfts"])):
             np.testing.assert_array_equal(g.get_dft(idx), r.get_dft(idx), err_msg=sc["name"])
+
+
+def _tb_setup(nsteps, design):
+    """A ragged non-dispersive grid for the time-blocked pass: CPML on every
+    axis, four materials, D and B point sources (one in a CPML slab), probes of
+    E and H, a DFT monitor of all components and, optionally, a design box of
+    per-cell permittivities."""
+    n = (131, 37, 29)
+    dx = 1e-6
+    dt = 0.5 * dx / (refcpu.constants()["c0"] * np.sqrt(3.0))
+    rng = np.random.default_rng(21)
+    ids = rng.integers(0, 4, size=(n[2], n[1], n[0])).astype(np.uint8)
+
+    def setup(s):
+        for ax in range(3):
+            s.set_pml_idiom(ax, 4, 3.0, 1.0, 0.8 * 4 / (refcpu.constants()["imp0"] * dx), 5.0, 0.05)
+        s.set_material(1, 2.0)
+        s.set_material(2, 5.5)
+        s.set_material(3, 1.5, 2.0)
+        s.set_matid(ids)
+        s.add_source(D, 2, 64, 18, 14, gauss_table(nsteps, 15, 5))
+        s.add_source(B, 0, 2, 20, 6, 1e-9 * gauss_table(nsteps, 22, 6))
+        s.add_probe(E, 2, 64, 18, 14)
+        s.add_probe(H, 1, 61, 3, 25)
+        s.add_probe(E, 0, 1, 1, 1)
+        s.add_dft(*synth.dft_monitor([2, 129, 3, 34, 12, 14], 63, dt, nsteps))
+        if design:
+            s.set_design([50, 90, 5, 30, 8, 20], 1.0 + 11.0 * np.random.default_rng(5).random((12, 25, 40)))
+
+    return n, dx, dt, setup
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("design", [False, True])
+def test_time_blocked_pass(precision, design):
+    """Time-blocked pass (two steps per launch) against the two passes, bitwise
+    (fields, probe traces, DFT bins), for odd step counts and chunk lengths,
+    schedule switches and captured chunks; the first run against the oracle."""
+    from fdtd_amd import Grid
+    nsteps = 75
+    n, dx, dt, setup = _tb_setup(nsteps, design)
+    ref = Grid("ThreeD", *n, dx, dt, precision=precision)
+    setup(ref)
+    ref.set_tile(128, 2, 2)
+    ref.step(nsteps)
+    o = refcpu.Sim("ThreeD", *n, dx, dt)
+    setup(o)
+    o.step(nsteps)
+    comps = [(E, c) for c in range(3)] + [(H, c) for c in range(3)]
+    for f, c in comps:
+        assert rel_l2(ref.get(f, c), o.get(f, c)) <= TOL[precision], (f, c)
+    for tile, graph, split in (((2, 12, 7), False, (75,)), ((2, 8, 5), False, (1, 30, 44)),
+                               ((2, 16, 32), True, (33, 42)), ((2, 12, 64), True, (16, 17, 42))):
+        print(f"[gpu-case] time-blocked {precision} design={design} {tile} graph={graph} {split}", flush=True)
+        g = Grid("ThreeD", *n, dx, dt, precision=precision)
+        setup(g)
+        g.set_graph(graph)
+        for q, m in enumerate(split):
+            g.set_tile(*(tile if q % 2 == 0 else (64, 4, 2)))
+            g.step(m)
+        assert g.get_tile() == tuple(tile if (len(split) - 1) % 2 == 0 else (64, 4, 2))
+        for f, c in comps:
+            np.testing.assert_array_equal(g.get(f, c), ref.get(f, c), err_msg=f"{tile} {f}{c}")
+        np.testing.assert_array_equal(g.probes(), ref.probes())
+        np.testing.assert_array_equal(g.get_dft(0), ref.get_dft(0))
+        g.close()

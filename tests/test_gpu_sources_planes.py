@@ -46,7 +46,7 @@ def test_sources_and_planes_against_oracle_every_schedule(precision):
     o.step(n)
     ref = None
     for tile, graph in (((128, 2, 4), False), ((64, 4, 2), False), ((256, 1, 4), False), ((0, 5, 1), False),
-                        ((1, 8, 8), False), ((128, 2, 4), True)):
+                        ((1, 8, 8), False), ((2, 12, 8), False), ((128, 2, 4), True)):
         print(f"[gpu-case] sources+planes {precision} tile {tile} graph {graph}", flush=True)
         g = _grid(sc, precision)
         g.set_tile(*tile)
