@@ -332,18 +332,18 @@ def run_c5(args):
     g.set_tile(*tile)
     t_fwd, _ = wall(lambda: (g.step(nsteps), g.synchronize()))
     g.close()
-    two_pass = tile if tile[0] > 1 else (256, 1, 1)
-    # the gradient's forward pass runs at the forward's tile; its reverse sweep
-    # and recompute at their own tile: with the recompute running beside the
-    # reverse steps, other shapes share the GPU better; best of four by short
-    # gradient runs (an untimed first run loads the adjoint kernels; each
-    # candidate keeps the faster of two runs)
+    two_pass = tile if tile[0] >= 8 else (64, 4, 2)  # the gradient's recording forward runs two passes
+    # the gradient's forward pass runs at the forward's two-pass tile; its
+    # reverse sweep and recompute at their own tile: with the recompute running
+    # beside the reverse steps, other shapes share the GPU better; the best
+    # candidate by short gradient runs (an untimed first run loads the adjoint
+    # kernels; each candidate keeps the faster of two runs)
     trial = min(600, nsteps)
     best = None
     g = mk()
     g.adjoint_eps_gradient(trial, 0, 4, checkpoint_every=100, history_mb=0.0)
     g.close()
-    for cand in dict.fromkeys([two_pass, (128, 2, 4), (64, 4, 4), (64, 2, 4)]):
+    for cand in dict.fromkeys([two_pass, (128, 2, 4), (64, 4, 4), (64, 2, 4), (64, 4, 6), (64, 4, 8), (128, 2, 8)]):
         for _ in range(2):
             g = mk()
             g.set_tile(*two_pass)

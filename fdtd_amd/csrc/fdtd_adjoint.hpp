@@ -74,9 +74,14 @@ __device__ FDTD_COLD void adj_objective(const AdjArgs<T>& q, long step, int i, i
 
 // Adjoint E pass of step `step` over one column of cells.  hist: the step's
 // row of design-cell E increments (null: no design box).
+// The column's slab indices and CPML coefficients along x and y are read
+// once, before the march; the z coefficients come from zf, the block's rows
+// of dt F/dx (E pass: H's of planes k0 - 1 .. k1 - 1; H pass: E's of planes
+// k0 .. k1), so no uniform global load inside the march waits behind the
+// field loads.
 template <typename T>
-__device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<T>& q, const T* tie, int i, int j, int k0,
-                                            int k1, long step, const T* __restrict__ hist) {
+__device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<T>& q, const T* tie, const T* zf,
+                                            int i, int j, int k0, int k1, long step, const T* __restrict__ hist) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   const FDTD_G T* __restrict__ M0 = q.mu[0];
   const FDTD_G T* __restrict__ M1 = q.mu[1];
@@ -87,58 +92,82 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
   const long sy = a.sy, sz = a.sz;
   const T fx = a.cf[1][0][i], fy = a.cf[1][1][j];
   const T fxm = a.cf[1][0][i - 1], fym = a.cf[1][1][j - 1];
+  const int sx0 = slab_index(i, a.slab_a[0], a.slab_b[0]), sx1 = slab_index(i - 1, a.slab_a[0], a.slab_b[0]);
+  const int sy0 = slab_index(j, a.slab_a[1], a.slab_b[1]), sy1 = slab_index(j - 1, a.slab_a[1], a.slab_b[1]);
+  T ccx = T(0), bx = T(0), gx = T(0), ccxm = T(0), ccy = T(0), by = T(0), gy = T(0), ccym = T(0);
+  if (sx0 >= 0) {
+    ccx = a.cc[1][0][i];
+    bx = a.cb[0][0][i];
+    gx = a.cg[0][0][i];
+  }
+  if (sx1 >= 0) ccxm = a.cc[1][0][i - 1];
+  if (sy0 >= 0) {
+    ccy = a.cc[1][1][j];
+    by = a.cb[0][1][j];
+    gy = a.cg[0][1][j];
+  }
+  if (sy1 >= 0) ccym = a.cc[1][1][j - 1];
+  // whether the column meets the objective monitor and the design box (the
+  // cold branches below are then taken only on their planes)
+  const bool ocol = q.obj_w && i >= q.obj_r[0] && i < q.obj_r[1] && j >= q.obj_r[2] && j < q.obj_r[3];
+  const bool dcol = hist && i >= a.dsg[0] && i < a.dsg[1] && j >= a.dsg[2] && j < a.dsg[3];
+  // w12, w02 of the plane below (with their z-slab terms): the march carries
+  // them from plane to plane instead of reloading that plane's mu
+  T w12c, w02c;
+  {
+    const long c = (long)i + sy * (long)j + sz * (long)(k0 - 1);
+    w12c = zf[0] * M1[c];
+    w02c = zf[0] * M0[c];
+    const int s = slab_index(k0 - 1, a.slab_a[2], a.slab_b[2]);
+    if (s >= 0) {
+      const long o = slab_off(a, 2, s, i, j, k0 - 1);
+      const T cc = a.cc[1][2][k0 - 1];
+      w02c -= cc * q.lpsi[1][0][2][o];
+      w12c -= cc * q.lpsi[1][1][2][o];
+    }
+  }
   for (int k = k0; k < k1; k++) {
     const long c = (long)i + sy * (long)j + sz * (long)k;
     const T m0 = M0[c], m1 = M1[c], m2 = M2[c];
-    const T m1_zm = M1[c - sz], m2_ym = M2[c - sy], m0_zm = M0[c - sz];
+    const T m2_ym = M2[c - sy];
     const T m2_xm = M2[c - 1], m0_ym = M0[c - sy], m1_xm = M1[c - 1];
     T n0 = N0[c], n1 = N1[c], n2 = N2[c];
     const int mid = a.mat[c];
     const T ie = cell_inv_eps(a, tie, mid, i, j, k);
-    const T fz = a.cf[1][2][k], fzm = a.cf[1][2][k - 1];
+    const T fz = zf[k - k0 + 1];
     // w_KJ at the cell and at its lower J neighbour
-    T w12 = fz * m1, w12m = fzm * m1_zm;
+    T w12 = fz * m1, w12m = w12c;
     T w21 = fy * m2, w21m = fym * m2_ym;
-    T w02 = fz * m0, w02m = fzm * m0_zm;
+    T w02 = fz * m0, w02m = w02c;
     T w20 = fx * m2, w20m = fxm * m2_xm;
     T w01 = fy * m0, w01m = fym * m0_ym;
     T w10 = fx * m1, w10m = fxm * m1_xm;
     // x slabs: H psi (K=1,2; J=0) and the E psi adjoint (I=1,2; J=0) of the later step
-    int s = slab_index(i, a.slab_a[0], a.slab_b[0]);
-    if (s >= 0) {
-      const long o = slab_off(a, 0, s, i, j, k);
-      const T cc = a.cc[1][0][i];
-      w10 -= cc * q.lpsi[1][1][0][o];
-      w20 -= cc * q.lpsi[1][2][0][o];
-      const T b = a.cb[0][0][i], g = a.cg[0][0][i];
-      q.lpsi[0][1][0][o] = b * q.lpsi[0][1][0][o] + g * n1;
-      q.lpsi[0][2][0][o] = b * q.lpsi[0][2][0][o] + g * n2;
+    if (sx0 >= 0) {
+      const long o = slab_off(a, 0, sx0, i, j, k);
+      w10 -= ccx * q.lpsi[1][1][0][o];
+      w20 -= ccx * q.lpsi[1][2][0][o];
+      q.lpsi[0][1][0][o] = bx * q.lpsi[0][1][0][o] + gx * n1;
+      q.lpsi[0][2][0][o] = bx * q.lpsi[0][2][0][o] + gx * n2;
     }
-    s = slab_index(i - 1, a.slab_a[0], a.slab_b[0]);
-    if (s >= 0) {
-      const long o = slab_off(a, 0, s, i - 1, j, k);
-      const T cc = a.cc[1][0][i - 1];
-      w10m -= cc * q.lpsi[1][1][0][o];
-      w20m -= cc * q.lpsi[1][2][0][o];
+    if (sx1 >= 0) {
+      const long o = slab_off(a, 0, sx1, i - 1, j, k);
+      w10m -= ccxm * q.lpsi[1][1][0][o];
+      w20m -= ccxm * q.lpsi[1][2][0][o];
     }
-    s = slab_index(j, a.slab_a[1], a.slab_b[1]);
-    if (s >= 0) {
-      const long o = slab_off(a, 1, s, i, j, k);
-      const T cc = a.cc[1][1][j];
-      w01 -= cc * q.lpsi[1][0][1][o];
-      w21 -= cc * q.lpsi[1][2][1][o];
-      const T b = a.cb[0][1][j], g = a.cg[0][1][j];
-      q.lpsi[0][0][1][o] = b * q.lpsi[0][0][1][o] + g * n0;
-      q.lpsi[0][2][1][o] = b * q.lpsi[0][2][1][o] + g * n2;
+    if (sy0 >= 0) {
+      const long o = slab_off(a, 1, sy0, i, j, k);
+      w01 -= ccy * q.lpsi[1][0][1][o];
+      w21 -= ccy * q.lpsi[1][2][1][o];
+      q.lpsi[0][0][1][o] = by * q.lpsi[0][0][1][o] + gy * n0;
+      q.lpsi[0][2][1][o] = by * q.lpsi[0][2][1][o] + gy * n2;
     }
-    s = slab_index(j - 1, a.slab_a[1], a.slab_b[1]);
-    if (s >= 0) {
-      const long o = slab_off(a, 1, s, i, j - 1, k);
-      const T cc = a.cc[1][1][j - 1];
-      w01m -= cc * q.lpsi[1][0][1][o];
-      w21m -= cc * q.lpsi[1][2][1][o];
+    if (sy1 >= 0) {
+      const long o = slab_off(a, 1, sy1, i, j - 1, k);
+      w01m -= ccym * q.lpsi[1][0][1][o];
+      w21m -= ccym * q.lpsi[1][2][1][o];
     }
-    s = slab_index(k, a.slab_a[2], a.slab_b[2]);
+    const int s = slab_index(k, a.slab_a[2], a.slab_b[2]);
     if (s >= 0) {
       const long o = slab_off(a, 2, s, i, j, k);
       const T cc = a.cc[1][2][k];
@@ -148,22 +177,17 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
       q.lpsi[0][0][2][o] = b * q.lpsi[0][0][2][o] + g * n0;
       q.lpsi[0][1][2][o] = b * q.lpsi[0][1][2][o] + g * n1;
     }
-    s = slab_index(k - 1, a.slab_a[2], a.slab_b[2]);
-    if (s >= 0) {
-      const long o = slab_off(a, 2, s, i, j, k - 1);
-      const T cc = a.cc[1][2][k - 1];
-      w02m -= cc * q.lpsi[1][0][2][o];
-      w12m -= cc * q.lpsi[1][1][2][o];
-    }
+    w12c = w12;
+    w02c = w02;
     T src[3] = {T(0), T(0), T(0)};
-    if (q.obj_w) adj_objective(cold_adj(q), step, i, j, k, src);
+    if (ocol && k >= q.obj_r[4] && k < q.obj_r[5]) adj_objective(cold_adj(q), step, i, j, k, src);
     n0 += ie * (src[0] + (w12 - w12m) - (w21 - w21m));
     n1 += ie * (src[1] + (w20 - w20m) - (w02 - w02m));
     n2 += ie * (src[2] + (w01 - w01m) - (w10 - w10m));
     N0[c] = n0;
     N1[c] = n1;
     N2[c] = n2;
-    if (hist) {
+    if (dcol && k >= a.dsg[4] && k < a.dsg[5]) {
       const StepArgs<T>& ac = (const StepArgs<T>&)cold_args(a);
       const AdjArgs<T>& qc = cold_adj(q);
       const long l = design_index(ac, i, j, k);
@@ -178,8 +202,8 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
 
 // Adjoint H pass of one step over one column of cells.
 template <typename T>
-__device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<T>& q, const T* tim, int i, int j, int k0,
-                                            int k1) {
+__device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<T>& q, const T* tim, const T* zf, int i,
+                                            int j, int k0, int k1) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
   const FDTD_G T* __restrict__ N0 = q.nu[0];
   const FDTD_G T* __restrict__ N1 = q.nu[1];
@@ -190,57 +214,77 @@ __device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<
   const long sy = a.sy, sz = a.sz;
   const T fx = a.cf[0][0][i], fy = a.cf[0][1][j];
   const T fxp = a.cf[0][0][i + 1], fyp = a.cf[0][1][j + 1];
-  for (int k = k0; k < k1; k++) {
+  const int sx0 = slab_index(i, a.slab_a[0], a.slab_b[0]), sx1 = slab_index(i + 1, a.slab_a[0], a.slab_b[0]);
+  const int sy0 = slab_index(j, a.slab_a[1], a.slab_b[1]), sy1 = slab_index(j + 1, a.slab_a[1], a.slab_b[1]);
+  T ccx = T(0), bx = T(0), gx = T(0), ccxp = T(0), ccy = T(0), by = T(0), gy = T(0), ccyp = T(0);
+  if (sx0 >= 0) {
+    ccx = a.cc[0][0][i];
+    bx = a.cb[1][0][i];
+    gx = a.cg[1][0][i];
+  }
+  if (sx1 >= 0) ccxp = a.cc[0][0][i + 1];
+  if (sy0 >= 0) {
+    ccy = a.cc[0][1][j];
+    by = a.cb[1][1][j];
+    gy = a.cg[1][1][j];
+  }
+  if (sy1 >= 0) ccyp = a.cc[0][1][j + 1];
+  // the march runs down the planes and carries v12, v02 of the plane above
+  // (with their z-slab terms) instead of reloading that plane's nu
+  T v12c, v02c;
+  {
+    const long c = (long)i + sy * (long)j + sz * (long)k1;
+    v12c = zf[k1 - k0] * N1[c];
+    v02c = zf[k1 - k0] * N0[c];
+    const int s = slab_index(k1, a.slab_a[2], a.slab_b[2]);
+    if (s >= 0) {
+      const long o = slab_off(a, 2, s, i, j, k1);
+      const T cc = a.cc[0][2][k1];
+      v02c += cc * q.lpsi[0][0][2][o];
+      v12c += cc * q.lpsi[0][1][2][o];
+    }
+  }
+  for (int k = k1 - 1; k >= k0; k--) {
     const long c = (long)i + sy * (long)j + sz * (long)k;
     const T n0 = N0[c], n1 = N1[c], n2 = N2[c];
-    const T n1_zp = N1[c + sz], n2_yp = N2[c + sy], n0_zp = N0[c + sz];
+    const T n2_yp = N2[c + sy];
     const T n2_xp = N2[c + 1], n0_yp = N0[c + sy], n1_xp = N1[c + 1];
     const T h0 = M0[c], h1 = M1[c], h2 = M2[c];
     const T im = tim[a.mat[c]];
-    const T fz = a.cf[0][2][k], fzp = a.cf[0][2][k + 1];
+    const T fz = zf[k - k0];
     // v_IJ at the cell and at its upper J neighbour
-    T v12 = fz * n1, v12p = fzp * n1_zp;
+    T v12 = fz * n1, v12p = v12c;
     T v21 = fy * n2, v21p = fyp * n2_yp;
-    T v02 = fz * n0, v02p = fzp * n0_zp;
+    T v02 = fz * n0, v02p = v02c;
     T v20 = fx * n2, v20p = fxp * n2_xp;
     T v01 = fy * n0, v01p = fyp * n0_yp;
     T v10 = fx * n1, v10p = fxp * n1_xp;
     // x slabs: E psi adjoint (I=1,2; J=0) and the H psi adjoint update (K=1,2; J=0)
-    int s = slab_index(i, a.slab_a[0], a.slab_b[0]);
-    if (s >= 0) {
-      const long o = slab_off(a, 0, s, i, j, k);
-      const T cc = a.cc[0][0][i];
-      v10 += cc * q.lpsi[0][1][0][o];
-      v20 += cc * q.lpsi[0][2][0][o];
-      const T b = a.cb[1][0][i], g = a.cg[1][0][i];
-      q.lpsi[1][1][0][o] = b * q.lpsi[1][1][0][o] - g * h1;
-      q.lpsi[1][2][0][o] = b * q.lpsi[1][2][0][o] - g * h2;
+    if (sx0 >= 0) {
+      const long o = slab_off(a, 0, sx0, i, j, k);
+      v10 += ccx * q.lpsi[0][1][0][o];
+      v20 += ccx * q.lpsi[0][2][0][o];
+      q.lpsi[1][1][0][o] = bx * q.lpsi[1][1][0][o] - gx * h1;
+      q.lpsi[1][2][0][o] = bx * q.lpsi[1][2][0][o] - gx * h2;
     }
-    s = slab_index(i + 1, a.slab_a[0], a.slab_b[0]);
-    if (s >= 0) {
-      const long o = slab_off(a, 0, s, i + 1, j, k);
-      const T cc = a.cc[0][0][i + 1];
-      v10p += cc * q.lpsi[0][1][0][o];
-      v20p += cc * q.lpsi[0][2][0][o];
+    if (sx1 >= 0) {
+      const long o = slab_off(a, 0, sx1, i + 1, j, k);
+      v10p += ccxp * q.lpsi[0][1][0][o];
+      v20p += ccxp * q.lpsi[0][2][0][o];
     }
-    s = slab_index(j, a.slab_a[1], a.slab_b[1]);
-    if (s >= 0) {
-      const long o = slab_off(a, 1, s, i, j, k);
-      const T cc = a.cc[0][1][j];
-      v01 += cc * q.lpsi[0][0][1][o];
-      v21 += cc * q.lpsi[0][2][1][o];
-      const T b = a.cb[1][1][j], g = a.cg[1][1][j];
-      q.lpsi[1][0][1][o] = b * q.lpsi[1][0][1][o] - g * h0;
-      q.lpsi[1][2][1][o] = b * q.lpsi[1][2][1][o] - g * h2;
+    if (sy0 >= 0) {
+      const long o = slab_off(a, 1, sy0, i, j, k);
+      v01 += ccy * q.lpsi[0][0][1][o];
+      v21 += ccy * q.lpsi[0][2][1][o];
+      q.lpsi[1][0][1][o] = by * q.lpsi[1][0][1][o] - gy * h0;
+      q.lpsi[1][2][1][o] = by * q.lpsi[1][2][1][o] - gy * h2;
     }
-    s = slab_index(j + 1, a.slab_a[1], a.slab_b[1]);
-    if (s >= 0) {
-      const long o = slab_off(a, 1, s, i, j + 1, k);
-      const T cc = a.cc[0][1][j + 1];
-      v01p += cc * q.lpsi[0][0][1][o];
-      v21p += cc * q.lpsi[0][2][1][o];
+    if (sy1 >= 0) {
+      const long o = slab_off(a, 1, sy1, i, j + 1, k);
+      v01p += ccyp * q.lpsi[0][0][1][o];
+      v21p += ccyp * q.lpsi[0][2][1][o];
     }
-    s = slab_index(k, a.slab_a[2], a.slab_b[2]);
+    const int s = slab_index(k, a.slab_a[2], a.slab_b[2]);
     if (s >= 0) {
       const long o = slab_off(a, 2, s, i, j, k);
       const T cc = a.cc[0][2][k];
@@ -250,43 +294,45 @@ __device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<
       q.lpsi[1][0][2][o] = b * q.lpsi[1][0][2][o] - g * h0;
       q.lpsi[1][1][2][o] = b * q.lpsi[1][1][2][o] - g * h1;
     }
-    s = slab_index(k + 1, a.slab_a[2], a.slab_b[2]);
-    if (s >= 0) {
-      const long o = slab_off(a, 2, s, i, j, k + 1);
-      const T cc = a.cc[0][2][k + 1];
-      v02p += cc * q.lpsi[0][0][2][o];
-      v12p += cc * q.lpsi[0][1][2][o];
-    }
+    v12c = v12;
+    v02c = v02;
     M0[c] = h0 + im * ((v12 - v12p) - (v21 - v21p));
     M1[c] = h1 + im * ((v20 - v20p) - (v02 - v02p));
     M2[c] = h2 + im * ((v01 - v01p) - (v10 - v10p));
   }
 }
 
-// launched like the forward passes: ntx x nty x nkc tiles, XCD-aware order,
-// the H pass walking each XCD's range in reverse
+// launched like the forward passes: ntx x nty x nkc tiles (kc <= kAdjMaxKc),
+// XCD-aware order, the H pass walking each XCD's range in reverse
+constexpr int kAdjMaxKc = 64;
 template <typename T>
 __device__ __forceinline__ void adj_e_pass_body(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
                                                 int kc, int ntx, int nty, int nkc, long step,
                                                 const T* __restrict__ hist) {
   __shared__ MatLds<T> sm;
-  stage_materials(*ap, sm);
+  __shared__ T zf[kAdjMaxKc + 1];
   const int t = xcd_tile(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-  const int k0 = 1 + bz * kc;
-  adj_e_cells(*ap, *qp, sm.ie, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
-              min(k0 + kc, ap->nz - 1), step, hist);
+  const int k0 = 1 + bz * kc, k1 = min(k0 + kc, ap->nz - 1);
+  for (int x = threadIdx.x + blockDim.x * threadIdx.y; x <= k1 - k0; x += blockDim.x * blockDim.y)
+    zf[x] = ap->cf[1][2][k0 - 1 + x];
+  stage_materials(*ap, sm);  // (its barrier also covers zf)
+  adj_e_cells(*ap, *qp, sm.ie, (const T*)zf, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
+              k1, step, hist);
 }
 template <typename T>
 __device__ __forceinline__ void adj_h_pass_body(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
                                                 int kc, int ntx, int nty, int nkc) {
   __shared__ MatLds<T> sm;
-  stage_materials(*ap, sm);
+  __shared__ T zf[kAdjMaxKc + 1];
   const int t = xcd_tile_rev(blockIdx.x, ntx * nty * nkc);
   const int bx = t % ntx, by = (t / ntx) % nty, bz = t / (ntx * nty);
-  const int k0 = 1 + bz * kc;
-  adj_h_cells(*ap, *qp, sm.im, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
-              min(k0 + kc, ap->nz - 1));
+  const int k0 = 1 + bz * kc, k1 = min(k0 + kc, ap->nz - 1);
+  for (int x = threadIdx.x + blockDim.x * threadIdx.y; x <= k1 - k0; x += blockDim.x * blockDim.y)
+    zf[x] = ap->cf[0][2][k0 + x];
+  stage_materials(*ap, sm);  // (its barrier also covers zf)
+  adj_h_cells(*ap, *qp, sm.im, (const T*)zf, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
+              k1);
 }
 
 template <typename T>

@@ -120,10 +120,11 @@ constexpr int kNumKernels = 8;
 // tile of the per-step passes: threads per block in x and y, z planes per thread
 // bx == 0 selects the streamed fused pass (3-D FP32) with E regions of `by`
 // rows and `kc` workgroups per compute unit, bx == 1 the z-marching fused
-// pass with workgroups of 64 x `by` columns and `kc` planes; bx >= 2 the two
-// passes with bx x by threads
-// per workgroup.  TM grids: bx == 0 selects the time-blocked pass, `by` steps
-// per launch on cores of kc x kc cells.
+// pass with workgroups of 64 x `by` columns and `kc` planes, bx == 2 the
+// time-blocked pass (two steps per launch, 64 x `by` columns, `kc` planes);
+// bx >= 8 the two passes with bx x by threads per workgroup.  TM grids:
+// bx == 0 selects the time-blocked pass, `by` steps per launch on cores of
+// kc x kc cells.
 struct Tile {
   int bx, by, kc;
 };
@@ -1637,7 +1638,10 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   const auto* ap = (const fdtdk::StepArgs<T>*)adev;
   const auto* qp = (const fdtdk::AdjArgs<T>*)qdev;
 
-  const Tile t = g->rtile.by ? g->rtile : ((g->tile.by && g->tile.bx >= 2) ? g->tile : kDefaultTile3D);
+  // a two-pass tile (the fused and time-blocked tiles are not reverse-sweep
+  // shapes), at most kAdjMaxKc planes per block (the z coefficients in LDS)
+  Tile t = g->rtile.by ? g->rtile : ((g->tile.by && g->tile.bx >= 8) ? g->tile : kDefaultTile3D);
+  t.kc = std::min(t.kc, fdtdk::kAdjMaxKc);
   const int nzi = g->d.nz - 2;
   const dim3 block(t.bx, t.by, 1);
   const int ntx = (g->d.nx + t.bx - 1) / t.bx, nty = (g->d.ny - 2 + t.by - 1) / t.by, nkc = (nzi + t.kc - 1) / t.kc;
@@ -1675,7 +1679,7 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   if (nck && !g->nbc && !g->cplx && !has_magnetized(g)) {
     Tile tr = g->rtile.by ? g->rtile : g->tile;
     if (!tr.by) tr = kDefaultTile3D;
-    if (tr.bx <= 1) tr = kDefaultTile3D;
+    if (tr.bx < 8) tr = kDefaultTile3D;
     std::vector<int4> tab;
     std::vector<int> off(1, 0);
     // tables stop where the cone covers every tile: later steps of a segment
@@ -2931,8 +2935,8 @@ int fdtd_batch_destroy(fdtd_batch* b) {
 
 int fdtd_batch_set_tile(fdtd_batch* b, int bx, int by, int kc) {
   if (!b) return fail(FDTD_EINVAL, "null batch");
-  if (bx < 2 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
-    return fail(FDTD_EINVAL, "batch tiles are two-pass tiles: bx >= 2, bx*by a multiple of 64 up to 256, kc >= 1");
+  if (bx < 8 || by <= 0 || kc <= 0 || bx * by > 256 || (bx * by) % 64 != 0)
+    return fail(FDTD_EINVAL, "batch tiles are two-pass tiles: bx >= 8, bx*by a multiple of 64 up to 256, kc >= 1");
   return batch_tiles(b, Tile{bx, by, kc});
 }
 

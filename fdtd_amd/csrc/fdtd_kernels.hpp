@@ -1198,6 +1198,9 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     pre.b[1] = a.cb[0][1][j];
     pre.c[1] = a.cc[0][1][j];
   }
+  // the old H of the plane below (the z-1 neighbours of the E update): the
+  // march carries the cell's Hx, Hy from plane to plane instead of reloading
+  T hx_zm = Hx[(long)i + sy * (long)j + sz * (long)(k0 - 1)], hy_zm = Hy[(long)i + sy * (long)j + sz * (long)(k0 - 1)];
   for (int k = k0; k < k1; k++) {
     const long c = (long)i + sy * (long)j + sz * (long)k;
     if (sxi >= 0) {
@@ -1226,9 +1229,11 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     h.hz_ym = Hz[c - sy];
     h.hz_xm = Hz[c - 1];
     h.hy_xm = Hy[c - 1];
-    h.hy_zm = Hy[c - sz];
+    h.hy_zm = hy_zm;
     h.hx_ym = Hx[c - sy];
-    h.hx_zm = Hx[c - sz];
+    h.hx_zm = hx_zm;
+    hx_zm = h.hx;
+    hy_zm = h.hy;
     const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
     const int mid = a.mat[c];
     const T fz = a.cf[0][2][k];
