@@ -480,10 +480,12 @@ def run_copy(args):
                      {"workload": "device copy, 1 GiB FP32 -> 1 GiB FP32", "bytes_per_copy": 2 * n * 4})
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_3d256_f32.json")
-TRAFFIC_SOURCE = ("profiles/hbm_traffic_3d256_f32.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this "
-                  "workload at the same tile, each scaled by the counter's reading for a known 1 GiB device copy; "
-                  "not measured inside this run")
+# committed FETCH_SIZE / WRITE_SIZE summaries of the headline workload, per schedule (the bytes a schedule moves
+# depend on it): the two passes at tile 128,2,4 and the time-blocked pass at 2,12,32
+TRAFFIC_FILES = [os.path.join(ROOT, "profiles", "hbm_traffic_3d256_f32.json"),
+                 os.path.join(ROOT, "profiles", "hbm_traffic_3d256_f32_time_blocked.json")]
+TRAFFIC_NOTE = (": rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload at the same tile, each scaled by "
+                "the counter's reading for a known 1 GiB device copy; not measured inside this run")
 PASS_KERNELS = ("fused3d_lds", "e_pass_3d", "h_pass_3d", "dft_sample", "dft_fold", "fusedz_3d", "tm_block", "tb_3d")
 
 
@@ -517,8 +519,14 @@ def traffic_summary(args):
     ff, wf = known / cf, known / cw
     fetch = _counter_rows(fetch_dir, "FETCH_SIZE")
     write = _counter_rows(write_dir, "WRITE_SIZE")
-    step_kernel = "fused3d_lds" if "fused3d_lds" in fetch else ("h_pass_3d" if "h_pass_3d" in fetch else "fusedz_3d")
-    nsteps = len(fetch[step_kernel])
+    if "tb_3d" in fetch:
+        # the time-blocked pass advances two steps per launch (one at odd starts and ends)
+        if not args.traffic_steps:
+            sys.exit("--traffic-summary of the time-blocked pass needs --traffic-steps (the profiled run's steps)")
+        nsteps = args.traffic_steps
+    else:
+        step_kernel = "fused3d_lds" if "fused3d_lds" in fetch else ("h_pass_3d" if "h_pass_3d" in fetch else "fusedz_3d")
+        nsteps = len(fetch[step_kernel])
     kern = {k: {"fetch_bytes": round(mean(fetch[k]) * 1024.0 * ff),
                 "write_bytes": round(mean(write.get(k, [0.0])) * 1024.0 * wf),
                 "launches_per_step": round(len(fetch[k]) / nsteps, 5)}
@@ -534,16 +542,17 @@ def traffic_summary(args):
 
 
 def step_traffic(tile, per_step=None, steps=None):
-    """HBM bytes of one step from the committed counter summary, when it was taken at this tile (the bytes a
-    schedule moves depend on it); None otherwise."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
-    except OSError:
-        return None
-    if t.get("tile") and [int(v) for v in str(t["tile"]).split(",")] != list(tile):
-        return None
-    return t.get("bytes_per_step")
+    """HBM bytes of one step and their source from the committed counter summary taken at this tile (the bytes
+    a schedule moves depend on it); (None, None) when there is none."""
+    for path in TRAFFIC_FILES:
+        try:
+            with open(path) as f:
+                t = json.load(f)
+        except OSError:
+            continue
+        if t.get("tile") and [int(v) for v in str(t["tile"]).split(",")] == list(tile):
+            return t.get("bytes_per_step"), os.path.relpath(path, ROOT) + TRAFFIC_NOTE
+    return None, None
 
 
 def main():
@@ -567,6 +576,9 @@ def main():
                     help="turn rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this bench and of --workload copy into "
                          "HBM bytes per launch (no GPU work)")
     ap.add_argument("--traffic-out", default="hbm_traffic.json")
+    ap.add_argument("--traffic-steps", type=int, default=None,
+                    help="steps the profiled run took (warm-up + 2 x steps): per-step bytes of a pass that advances "
+                         "several steps per launch")
     args = ap.parse_args()
     if args.traffic_summary:
         traffic_summary(args)
@@ -610,7 +622,8 @@ def main():
     # device time of the kernels of one step (the aux line runs once per 256-step chunk)
     kern_ms = sum(v for k, v in per_step.items() if k != "aux_line")
     achieved = abytes / (kern_ms / 1e3) / 1e9
-    traffic = step_traffic(tile, per_step, steps=args.steps) if (args.size == 256 and args.precision == "f32") else None
+    traffic, traffic_src = (step_traffic(tile, per_step, steps=args.steps) if (args.size == 256 and args.precision == "f32")
+                            else (None, None))
     out = {
         "metric": METRIC, "value": round(gcells, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 5), "higher_is_better": True,
@@ -623,7 +636,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "traffic_source": TRAFFIC_SOURCE if traffic else None,
+                     "traffic_source": traffic_src,
                      "kernel": "one step: " + " + ".join(k for k in avg if k != "aux_line"),
                      "algorithmic_bytes_per_step": abytes, "avg_ms": avg, "ms_per_step_by_kernel": per_step},
     }
