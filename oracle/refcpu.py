@@ -26,7 +26,9 @@ _ip = ctypes.POINTER(ctypes.c_int)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "librefcpu.so")
+        # REFCPU_LIB: another build of the same source (the sanitizer build of
+        # oracle/Makefile's `asan` target, which the CPU tests run once)
+        path = os.environ.get("REFCPU_LIB") or os.path.join(_HERE, "librefcpu.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle library missing: run `make -C oracle refcpu`")
         L = ctypes.CDLL(path)
