@@ -139,6 +139,9 @@ typedef Nb<FluidCell<YeeCell<YeeFields<TM, double, std::array>, DoublePolarizati
 typedef Nb<FluidCell<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<double>,
                              VacuumMagnetization<double>, NewPML<ThreeD, double>>>>
     Cell3;
+typedef Nb<FluidCell<YeeCell<YeeFields<TE, double, std::array>, DoublePolarization<double>,
+                             VacuumMagnetization<double>, NewPML<TE, double>>>>
+    CellTE;
 
 // material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz, 4 conductive (freq = the conductivity),
 // 5 magnetized Drude with the bias (bx, by, bz) [T], 6 self-magnetized Drude (the cell's own B),
@@ -225,6 +228,9 @@ struct PlaneIt {
   }
   bool operator!=(const PlaneIt& o) const { return p != o.p; }
   bool operator==(const PlaneIt& o) const { return p == o.p; }
+};
+struct LocTE {
+  std::array<double, 2> operator()(const PlaneIt<CellTE>& it) const { return {it.p->x, it.p->y}; }
 };
 struct LocTM {
   std::array<double, 2> operator()(const PlaneIt<CellTM>& it) const { return {it.p->x, it.p->y}; }
@@ -335,6 +341,192 @@ static void scen_aux1d(const std::string& dir) {
   fx.arr("trace_EH", trace, {size_t(nsteps), locs.size(), 2});
   fx.arr("final_E_nodes", eall, {eall.size()});
   fx.arr("final_H_nodes", hall, {hall.size()});
+  fx.write(dir);
+}
+
+// ------------------------------------------------------------ scenario: 2D TE
+// YeeFields<TE> (Ex, Ey, Hz; include/YeeFields.hpp:94-119) stepped with the
+// reference's functors in the composition of tests/testTE.cpp: D curl, D
+// sources, CPML E along x and y, TFSF updateD, material; B curl, B source
+// (tests/testTE.cpp adds its pulse to Bz), CPML H, TFSF updateB, vacuum H.
+// The TFSF planes take the reference's TE polarisations (Source.hpp:573-576:
+// polH = z, pol = (k_y, -k_x, 0)).
+struct TEScenario {
+  std::string name;
+  int nx, ny, nsteps;
+  double dx, dt;
+  PmlSpec pml;
+  std::vector<Mat> mats;
+  std::vector<int> matid;  // nx*ny
+  struct Src {
+    int field, comp, i, j;  // field 0: D (comp 0 / 1), 2: B (comp 2)
+    std::vector<double> table;
+  };
+  std::vector<Src> src;
+  std::vector<std::pair<int, int>> probes;
+  bool tfsf = false;
+  int bx0 = 0, bx1 = 0, by0 = 0, by1 = 0;
+  double kx = 1, ky = 0;
+  int aux_ncells = 0;
+  double f0 = 0, tw = 0, t0 = 0;
+};
+
+static void run_te(const TEScenario& S, const std::string& dir) {
+  const int nx = S.nx, ny = S.ny;
+  std::vector<CellTE> cells(size_t(nx) * ny);
+  auto at = [&](int i, int j) -> CellTE& { return cells[size_t(i) + size_t(nx) * j]; };
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      CellTE& c = at(i, j);
+      c.mn[0] = &at(i - 1, j);
+      c.mx[0] = &at(i + 1, j);
+      c.mn[1] = &at(i, j - 1);
+      c.mx[1] = &at(i, j + 1);
+    }
+  const double dt = S.dt, dx = S.dx;
+  place_pml(nx, S.pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int j = 0; j < ny; j++) {
+      if (e)
+        at(idx, j).setPMLParameters<FieldType::Electric, Dir::X>(K, Sg, A, dt);
+      else
+        at(idx, j).setPMLParameters<FieldType::Magnetic, Dir::X>(K, Sg, A, dt);
+    }
+  });
+  place_pml(ny, S.pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int i = 0; i < nx; i++) {
+      if (e)
+        at(i, idx).setPMLParameters<FieldType::Electric, Dir::Y>(K, Sg, A, dt);
+      else
+        at(i, idx).setPMLParameters<FieldType::Magnetic, Dir::Y>(K, Sg, A, dt);
+    }
+  });
+  typedef TFSFPlane<TE, LocTE, PlaneIt<CellTE>, Aux1D<double>> PlaneT;
+  std::vector<std::vector<PlaneEntry<CellTE>>> faces(S.tfsf ? 4 : 0);
+  std::vector<PlaneT> planes;
+  std::vector<double> aux_sig;
+  const double cx = std::floor(0.5 * (S.bx0 + S.bx1)) + 0.5, cy = std::floor(0.5 * (S.by0 + S.by1)) + 0.5;
+  if (S.tfsf) {
+    // TE fluxes (Dx, Dy, Bz) carry no offset along the tangential axis of a
+    // face, so every face spans the cells lo .. hi-1 of that axis (TM's Dz,
+    // Bx, By span lo+1 .. hi)
+    for (int j = S.by0; j < S.by1; j++) {
+      faces[0].push_back({&at(S.bx0, j), (S.bx0 - cx) * dx, (j - cy) * dx, 0});
+      faces[1].push_back({&at(S.bx1, j), (S.bx1 - cx) * dx, (j - cy) * dx, 0});
+    }
+    for (int i = S.bx0; i < S.bx1; i++) {
+      faces[2].push_back({&at(i, S.by0), (i - cx) * dx, (S.by0 - cy) * dx, 0});
+      faces[3].push_back({&at(i, S.by1), (i - cx) * dx, (S.by1 - cy) * dx, 0});
+    }
+    const double f0 = S.f0, tw = S.tw, t0 = S.t0;
+    for (int fidx = 0; fidx < 4; fidx++) {
+      auto sig = [f0, tw, t0, fidx, &aux_sig](double t) {
+        double a = (t - t0) / tw;
+        double v = std::exp(-a * a) * std::sin(2 * pi * f0 * (t - t0));
+        if (fidx == 0) aux_sig.push_back(v);
+        return v;
+      };
+      auto aux = std::make_shared<Aux1D<double>>(S.aux_ncells, dx, dt, sig);
+      std::array<double, 2> K = {S.kx, S.ky};
+      PlaneIt<CellTE> b{faces[fidx].data()}, e{faces[fidx].data() + faces[fidx].size()};
+      if (fidx == 0) planes.push_back(make_tfsf_plane<TE, Dir::X, Orientation::MIN>(PlaneIt<CellTE>(b), PlaneIt<CellTE>(e), LocTE(), K, aux));
+      if (fidx == 1) planes.push_back(make_tfsf_plane<TE, Dir::X, Orientation::MAX>(PlaneIt<CellTE>(b), PlaneIt<CellTE>(e), LocTE(), K, aux));
+      if (fidx == 2) planes.push_back(make_tfsf_plane<TE, Dir::Y, Orientation::MIN>(PlaneIt<CellTE>(b), PlaneIt<CellTE>(e), LocTE(), K, aux));
+      if (fidx == 3) planes.push_back(make_tfsf_plane<TE, Dir::Y, Orientation::MAX>(PlaneIt<CellTE>(b), PlaneIt<CellTE>(e), LocTE(), K, aux));
+    }
+  }
+  std::vector<double> ptrace;
+  for (int n = 0; n < S.nsteps; n++) {
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TE, Dx>(dt, dx)(at(i, j));
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TE, Dy>(dt, dx)(at(i, j));
+    for (const auto& q : S.src)
+      if (q.field == 0) (q.comp == 0 ? at(q.i, q.j).Dx() : at(q.i, q.j).Dy()) += q.table[n];
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TE, FieldType::Electric, Dir::X>(dt, dx)(at(i, j));
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TE, FieldType::Electric, Dir::Y>(dt, dx)(at(i, j));
+    for (auto& pl : planes) pl.updateD(dt, dx);
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) material_E<TE>(at(i, j), S.mats[S.matid[size_t(i) + size_t(nx) * j]], dt);
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TE, Bz>(dt, dx)(at(i, j));
+    for (const auto& q : S.src)
+      if (q.field == 2) at(q.i, q.j).Bz() += q.table[n];
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TE, FieldType::Magnetic, Dir::X>(dt, dx)(at(i, j));
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TE, FieldType::Magnetic, Dir::Y>(dt, dx)(at(i, j));
+    for (auto& pl : planes) pl.updateB(dt, dx);
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) VacuumUpdate<TE, FieldType::Magnetic>()(at(i, j));
+    for (auto& pr : S.probes) {
+      ptrace.push_back(at(pr.first, pr.second).Ex());
+      ptrace.push_back(at(pr.first, pr.second).Ey());
+      ptrace.push_back(at(pr.first, pr.second).Hz());
+    }
+  }
+  Fixture fx;
+  fx.name = S.name;
+  fx.ps("mode", "TE");
+  fx.pi("nx", nx);
+  fx.pi("ny", ny);
+  fx.pi("nsteps", S.nsteps);
+  fx.p("dx", dx);
+  fx.p("dt", dt);
+  put_pml(fx, S.pml);
+  fx.pi("pmlx", 1);
+  fx.pi("pmly", 1);
+  fx.pi("tfsf", S.tfsf);
+  if (S.tfsf) {
+    fx.pi("bx0", S.bx0);
+    fx.pi("bx1", S.bx1);
+    fx.pi("by0", S.by0);
+    fx.pi("by1", S.by1);
+    fx.p("kx", S.kx);
+    fx.p("ky", S.ky);
+    fx.pi("aux_ncells", S.aux_ncells);
+    fx.p("cx", cx);
+    fx.p("cy", cy);
+    fx.p("f0", S.f0);
+    fx.p("tw", S.tw);
+    fx.p("t0", S.t0);
+    fx.arr("aux_signal", aux_sig, {aux_sig.size()});
+  }
+  if (!S.src.empty()) {
+    std::vector<double> sp, st;
+    for (const auto& q : S.src) {
+      sp.insert(sp.end(), {double(q.field), double(q.comp), double(q.i), double(q.j)});
+      st.insert(st.end(), q.table.begin(), q.table.end());
+    }
+    fx.arr("src_field_comp_i_j", sp, {S.src.size(), 4});
+    fx.arr("src_tables", st, {S.src.size(), size_t(S.nsteps)});
+  }
+  std::vector<double> mt;
+  for (auto& m : S.mats) mt.insert(mt.end(), {double(m.kind), m.eps, m.freq, m.gamma, m.delta});
+  fx.arr("mats_kind_eps_freq_gamma_delta", mt, {S.mats.size(), 5});
+  std::vector<double> mid(S.matid.begin(), S.matid.end());
+  fx.arr("matid", mid, {size_t(ny), size_t(nx)});
+  std::vector<double> pr;
+  for (auto& p : S.probes) pr.insert(pr.end(), {double(p.first), double(p.second)});
+  fx.arr("probes", pr, {S.probes.size(), 2});
+  fx.arr("probe_trace_Ex_Ey_Hz", ptrace, {size_t(S.nsteps), S.probes.size(), 3});
+  std::vector<double> Ex, Ey, Dxv, Dyv, Hz, Bzv;
+  for (auto& c : cells) {
+    Ex.push_back(c.Ex());
+    Ey.push_back(c.Ey());
+    Dxv.push_back(c.Dx());
+    Dyv.push_back(c.Dy());
+    Hz.push_back(c.Hz());
+    Bzv.push_back(c.Bz());
+  }
+  std::vector<size_t> sh = {size_t(ny), size_t(nx)};
+  fx.arr("Ex", Ex, sh);
+  fx.arr("Ey", Ey, sh);
+  fx.arr("Dx", Dxv, sh);
+  fx.arr("Dy", Dyv, sh);
+  fx.arr("Hz", Hz, sh);
+  fx.arr("Bz", Bzv, sh);
   fx.write(dir);
 }
 
@@ -1386,6 +1578,57 @@ int main(int argc, char** argv) {
     for (int n = 0; n < S.nsteps; n++) S.src.push_back(eps0 * gauss_pulse(n, 25, 8));
     S.probes = {{24, 20}, {17, 15}, {30, 25}, {12, 30}, {3, 3}};
     run_tm(S, dir);
+  }
+  // 2D TE (Ex, Ey, Hz): CPML x+y, two dielectric blocks, a Drude and a
+  // Lorentz block, a Gaussian soft source on Dx and the pulse of
+  // tests/testTE.cpp on Bz; and a TE TFSF box (oblique wave, the reference's
+  // TE polarisation) around a dielectric block.
+  for (int variant = 0; variant < 2; variant++) {
+    TEScenario S;
+    S.name = variant == 0 ? "te2d_small" : "te2d_tfsf";
+    S.nx = variant == 0 ? 48 : 64;
+    S.ny = variant == 0 ? 40 : 60;
+    S.nsteps = variant == 0 ? 150 : 160;
+    S.dx = 1e-6;
+    S.dt = (variant == 0 ? 0.9 : 0.5) * S.dx / (c0 * std::sqrt(2.0));
+    PMLParameterModel dm(S.dx);
+    S.pml = {10, 3, 1, dm.msMax, 5, 0.05};
+    S.mats = {{0, 1, 0, 0, 0}, {1, 4.0, 0, 0, 0}, {1, 2.25, 0, 0, 0}};
+    S.mats.push_back({2, 1.5, 0.3 / S.dt, 0.02 / S.dt, 0});
+    S.mats.push_back({3, 2.0, 0.2 / S.dt, 0.01 / S.dt, 0.7});
+    S.matid.assign(size_t(S.nx) * S.ny, 0);
+    auto fill = [&](int x0, int x1, int y0, int y1, int m) {
+      for (int j = y0; j < y1; j++)
+        for (int i = x0; i < x1; i++) S.matid[size_t(i) + size_t(S.nx) * j] = m;
+    };
+    if (variant == 0) {
+      fill(14, 22, 13, 19, 1);
+      fill(27, 33, 22, 29, 2);
+      fill(15, 20, 24, 30, 3);
+      fill(28, 34, 12, 17, 4);
+      TEScenario::Src d{0, 0, 24, 20, {}}, b{2, 2, 20, 25, {}};
+      for (int n = 0; n < S.nsteps; n++) {
+        d.table.push_back(eps0 * gauss_pulse(n, 25, 8));
+        b.table.push_back(std::exp(-(n - 10.0) * (n - 10.0) * c0 / (20 * S.dx) * S.dt * S.dt) * 1e-9);
+      }
+      S.src = {d, b};
+      S.probes = {{24, 20}, {17, 15}, {30, 25}, {12, 30}, {3, 3}, {20, 25}};
+    } else {
+      fill(28, 36, 26, 33, 1);
+      S.tfsf = true;
+      S.bx0 = 14;
+      S.bx1 = 49;
+      S.by0 = 14;
+      S.by1 = 45;
+      S.kx = std::cos(pi / 6);
+      S.ky = std::sin(pi / 6);
+      S.aux_ncells = 100;
+      S.f0 = c0 / (20 * S.dx);
+      S.tw = 2.0 / S.f0;
+      S.t0 = 3.0 * S.tw;
+      S.probes = {{30, 30}, {15, 30}, {14, 30}, {13, 30}, {49, 20}, {50, 20}, {31, 14}, {31, 13}};
+    }
+    run_te(S, dir);
   }
   // 2D TM at the 256^2 / 1000-step shape: dx = 1 m, c0 dt/dx = 0.5, 10-cell
   // CPML on all edges, vacuum, Dz[128,128] += eps0 exp(-((n-40)/12)^2),

@@ -103,6 +103,10 @@ def from_golden(name):
         sc["fluid"] = (a["fluid_ne"].reshape(shape), a["fluid_nu"].reshape(shape))
     if int(p.get("src_i", -1)) >= 0:
         sc["sources"].append((D, 2, int(p["src_i"]), int(p["src_j"]), int(p.get("src_k", 0)), a["src"]))
+    if "src_field_comp_i_j" in a:  # D / B point sources (TE fixtures): field 0 D, 2 B
+        for row, tab in zip(a["src_field_comp_i_j"], a["src_tables"]):
+            f, c, i, j = (int(v) for v in row)
+            sc["sources"].append((f, c, i, j, 0, tab))
     if "soft_field_comp_i_j" in a:
         for row, tab in zip(a["soft_field_comp_i_j"], a["soft_tables"]):
             f, c, i, j = (int(v) for v in row)
@@ -123,6 +127,9 @@ def from_golden(name):
             if mode == "ThreeD":
                 for f, c in [(E, 0), (E, 1), (E, 2), (H, 0), (H, 1), (H, 2)]:
                     sc["probes"].append((f, c, *ijk))
+            elif mode == "TE":
+                for f, c in [(E, 0), (E, 1), (H, 2)]:
+                    sc["probes"].append((f, c, *ijk))
             else:
                 for f, c in [(E, 2), (H, 0), (H, 1)]:
                     sc["probes"].append((f, c, *ijk))
@@ -137,6 +144,8 @@ def from_golden(name):
         k = np.array([kx, ky])
         k = k / math.sqrt(k[0] * k[0] + k[1] * k[1])
         sc["tfsf"]["polH"] = [k[1], -k[0], 0.0]
+        if mode == "TE":  # the reference's TE defaults, include/Source.hpp:573-576
+            sc["tfsf"]["pol"], sc["tfsf"]["polH"] = [k[1], -k[0], 0.0], [0.0, 0.0, 1.0]
     return sc, p, a
 
 
