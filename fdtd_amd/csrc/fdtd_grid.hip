@@ -1029,7 +1029,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
   const bool general = g->nbc || g->nsrc_eh || g->nplane;
   // the time-blocked TM pass covers non-dispersive grids without those, complex
   // fields or DFT monitors
-  const bool tmb = g->ndim == 2 && t.bx == 0 && !general && !g->cplx && g->max_np == 0 && g->ndft == 0 &&
+  const bool te = g->d.mode == FDTD_TE;
+  const bool tmb = g->ndim == 2 && !te && t.bx == 0 && !general && !g->cplx && g->max_np == 0 && g->ndft == 0 &&
                    !g->force_two_pass;
   if (g->ndim == 2 && !tmb) {
     if (t.bx == 0) t = kDefaultTileTM;
@@ -1237,7 +1238,10 @@ int step_impl(fdtd_grid* g, int nsteps) {
           for (int ft = 0; ft < 2; ft++) {
             for (int part = 0; part < 2; part++) {
               const auto* p = part ? dev1 : dev0;
-              if (g->ndim == 2)
+              if (g->ndim == 2 && te)
+                rc = ft ? launch(g, 1, fdtdk::h_pass_te<T, true>, grid, block, p, 1, s)
+                        : launch(g, 0, fdtdk::e_pass_te<T, true>, grid, block, p, 1, s);
+              else if (g->ndim == 2)
                 rc = ft ? launch(g, 1, fdtdk::h_pass_tm<T, true>, grid, block, p, 1, s)
                         : launch(g, 0, fdtdk::e_pass_tm<T, true>, grid, block, p, 1, s);
               else if (ft)
@@ -1287,6 +1291,12 @@ int step_impl(fdtd_grid* g, int nsteps) {
             rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           else
             rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+        } else if (te && general) {
+          rc = launch(g, 0, fdtdk::e_pass_te<T, true>, grid, block, dev0, 1, s);
+          if (!rc) rc = launch(g, 1, fdtdk::h_pass_te<T, true>, grid, block, dev0, 1, s);
+        } else if (te) {
+          rc = launch(g, 0, fdtdk::e_pass_te<T>, grid, block, dev0, 1, s);
+          if (!rc) rc = launch(g, 1, fdtdk::h_pass_te<T>, grid, block, dev0, 1, s);
         } else if (general) {
           rc = launch(g, 0, fdtdk::e_pass_tm<T, true>, grid, block, dev0, 1, s);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_tm<T, true>, grid, block, dev0, 1, s);
@@ -1768,17 +1778,19 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
 
 }  // namespace
 
+// quantities per cell of fdtd_surface_quantities: 3-D 7, TM 3, TE 4
+static int surface_count(const fdtd_grid* g) { return g->d.mode == FDTD_3D ? 7 : (g->d.mode == FDTD_TE ? 4 : 3); }
+
 template <typename T>
 int surface_impl(fdtd_grid* g, const fdtdk::SurfRegion& r, long cells, double* out) {
-  const int nq = g->ndim == 3 ? 7 : 3;
+  const int nq = surface_count(g);
   double* dev = nullptr;
   HIP_TRY(hipMallocAsync((void**)&dev, size_t(nq) * cells * sizeof(double), g->stream));
   const long sy = g->d.nx, sz = long(g->d.nx) * g->d.ny;
   auto F = [](void* p) { return (const T*)p; };
-  const T* z = nullptr;
   hipLaunchKernelGGL(fdtdk::surface_quantities<T>, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, g->stream,
-                     g->ndim == 3 ? F(g->E[0]) : z, g->ndim == 3 ? F(g->E[1]) : z, F(g->E[2]), F(g->H[0]), F(g->H[1]),
-                     g->ndim == 3 ? F(g->H[2]) : z, sy, sz, r, g->ndim == 3 ? 1 : 0, g->d.dx, kEps0, dev);
+                     F(g->E[0]), F(g->E[1]), F(g->E[2]), F(g->H[0]), F(g->H[1]), F(g->H[2]), sy, sz, r, g->d.mode,
+                     g->d.dx, kEps0, dev);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, dev, size_t(nq) * cells * sizeof(double), hipMemcpyDeviceToHost, g->stream);
   (void)hipFreeAsync(dev, g->stream);
@@ -1816,17 +1828,16 @@ int scratch_ptr(double** out) {
 
 template <typename T>
 int surface_static_impl(fdtd_grid* g, const fdtdk::SurfRegion& r, long cells, double* out) {
-  const int nq = g->ndim == 3 ? 7 : 3;
+  const int nq = surface_count(g);
   double* dev = nullptr;
   int rc = scratch_ptr(&dev);
   if (rc) return rc;
   const long sy = g->d.nx, sz = long(g->d.nx) * g->d.ny;
   auto F = [](void* p) { return (const T*)p; };
-  const T* z = nullptr;
   // every output element is written by the kernel: nothing to clear
   hipLaunchKernelGGL(fdtdk::surface_quantities<T>, dim3(unsigned((cells + 255) / 256)), dim3(256), 0, g->stream,
-                     g->ndim == 3 ? F(g->E[0]) : z, g->ndim == 3 ? F(g->E[1]) : z, F(g->E[2]), F(g->H[0]), F(g->H[1]),
-                     g->ndim == 3 ? F(g->H[2]) : z, sy, sz, r, g->ndim == 3 ? 1 : 0, g->d.dx, kEps0, dev);
+                     F(g->E[0]), F(g->E[1]), F(g->E[2]), F(g->H[0]), F(g->H[1]), F(g->H[2]), sy, sz, r, g->d.mode,
+                     g->d.dx, kEps0, dev);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, dev, size_t(nq) * cells * sizeof(double), hipMemcpyDeviceToHost, g->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
@@ -1841,7 +1852,8 @@ const char* fdtd_last_error(void) { return g_err.c_str(); }
 int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   if (!desc || !out) return fail(FDTD_EINVAL, "null argument");
   *out = nullptr;
-  if (desc->mode != FDTD_TM && desc->mode != FDTD_3D) return fail(FDTD_EINVAL, "mode must be FDTD_TM or FDTD_3D");
+  if (desc->mode != FDTD_TM && desc->mode != FDTD_TE && desc->mode != FDTD_3D)
+    return fail(FDTD_EINVAL, "mode must be FDTD_TM, FDTD_TE or FDTD_3D");
   if (desc->precision != FDTD_F32 && desc->precision != FDTD_F64) return fail(FDTD_EINVAL, "bad precision");
   const int nz = desc->mode == FDTD_3D ? desc->nz : 1;
   if (desc->nx < 3 || desc->ny < 3 || (desc->mode == FDTD_3D && nz < 3))
@@ -1862,6 +1874,9 @@ int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   g->ncell = long(desc->nx) * desc->ny * nz;
   if (g->ndim == 3) {
     for (int c = 0; c < 3; c++) g->actE[c] = g->actH[c] = true;
+  } else if (desc->mode == FDTD_TE) {
+    g->actE[0] = g->actE[1] = true;
+    g->actH[2] = true;
   } else {
     g->actE[2] = true;
     g->actH[0] = g->actH[1] = true;

@@ -1137,7 +1137,7 @@ __device__ __forceinline__ void stage_materials(const StepArgs<T>& a, MatLds<T>&
 // ghost copies (masks are +-1, so they are exact): nothing in the same pass
 // reads a ghost of the field it writes, and the next pass sees them.
 // ghost copies of interior cell (i,j,k) (address c) whose new values are v
-// (ft 0: E, 1: H); ndim 2: TM (Ez, or Hx and Hy)
+// (ft 0: E, 1: H); the components of the mode (non-null field pointers)
 template <typename T, bool COLD = true>
 __device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a0, int ft, int ndim, int i, int j, int k, long c,
                                          const T v[3]) {
@@ -1150,7 +1150,7 @@ __device__ FDTD_COLD void bc_ghost_write(const StepArgs<T>& a0, int ft, int ndim
       const long d = c + a.bc_off[ax][side];
       const int neg = a.bc_neg[ft][ax][side];
       for (int q = 0; q < 3; q++) {
-        if (ndim == 2 && (ft == 0) != (q == 2)) continue;  // TM: Ez, Hx, Hy
+        if (!F[q]) continue;  // a component the mode does not have
         F[q][d] = ((neg >> q) & 1) ? -v[q] : v[q];
       }
     }
@@ -1431,7 +1431,8 @@ __global__ __launch_bounds__(256) void bloch_ghosts(const StepArgs<T>* __restric
 // current E and H of the cells of region [x0,x1)x[y0,y1)x[z0,z1): one thread
 // per cell, FP64 arithmetic in the reference's operation order with no FMA
 // contraction, so FP64 grids give the reference's bits.  out: quantity-major,
-// x fastest; 3-D: rho, Jex, Jey, Jez, Jhx, Jhy, Jhz; TM: Jez, Jhx, Jhy.
+// x fastest; 3-D: rho, Jex, Jey, Jez, Jhx, Jhy, Jhz; TM: Jez, Jhx, Jhy; TE
+// (189-198, 253-263, 323-332): rho, Jex, Jey, Jhz.  mode: 3, 2 (TM) or 4 (TE).
 struct SurfRegion {
   int x0, x1, y0, y1, z0, z1;
 };
@@ -1440,7 +1441,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void surface_quantities(const T* __restrict__ Ex, const T* __restrict__ Ey,
                                                           const T* __restrict__ Ez, const T* __restrict__ Hx,
                                                           const T* __restrict__ Hy, const T* __restrict__ Hz, long sy,
-                                                          long sz, SurfRegion r, int three_d, double dx, double eps0,
+                                                          long sz, SurfRegion r, int mode, double dx, double eps0,
                                                           double* __restrict__ out) {
 #pragma clang fp contract(off)
   const long ex = r.x1 - r.x0, ey = r.y1 - r.y0, ez = r.z1 - r.z0;
@@ -1450,7 +1451,7 @@ __global__ __launch_bounds__(256) void surface_quantities(const T* __restrict__ 
   const int i = r.x0 + int(q % ex), j = r.y0 + int((q / ex) % ey), k = r.z0 + int(q / (ex * ey));
   const long c = (long)i + sy * (long)j + sz * (long)k;
   auto v = [](const T* f, long x) { return (double)f[x]; };
-  if (three_d) {
+  if (mode == 3) {
     out[0 * cells + q] = eps0 / dx *
                          (v(Ex, c) + v(Ey, c) + v(Ez, c) - v(Ex, c - 1) - v(Ey, c - sy) - v(Ez, c - sz));
     out[1 * cells + q] = 1.0 / dx * (v(Hz, c) - v(Hz, c - sy) - v(Hy, c) + v(Hy, c - sz));
@@ -1459,10 +1460,15 @@ __global__ __launch_bounds__(256) void surface_quantities(const T* __restrict__ 
     out[4 * cells + q] = -1.0 / dx * (v(Ez, c + sy) - v(Ez, c) - v(Ey, c + sz) + v(Ey, c));
     out[5 * cells + q] = -1.0 / dx * (v(Ex, c + sz) - v(Ex, c) - v(Ez, c + 1) + v(Ez, c));
     out[6 * cells + q] = -1.0 / dx * (v(Ey, c + 1) - v(Ey, c) - v(Ex, c + sy) + v(Ex, c));
-  } else {
+  } else if (mode == 2) {
     out[0 * cells + q] = 1.0 / dx * (v(Hy, c) - v(Hy, c - 1) - v(Hx, c) + v(Hx, c - sy));
     out[1 * cells + q] = -1.0 / dx * (v(Ez, c + sy) - v(Ez, c));
     out[2 * cells + q] = -1.0 / dx * (-v(Ez, c + 1) + v(Ez, c));
+  } else {
+    out[0 * cells + q] = eps0 / dx * (v(Ex, c) + v(Ey, c) - v(Ex, c - 1) - v(Ey, c - sy));
+    out[1 * cells + q] = 1.0 / dx * (v(Hz, c) - v(Hz, c - sy));
+    out[2 * cells + q] = 1.0 / dx * (-v(Hz, c) + v(Hz, c - 1));
+    out[3 * cells + q] = -1.0 / dx * (v(Ey, c + 1) - v(Ey, c) - v(Ex, c + sy) + v(Ex, c));
   }
 }
 
@@ -2777,6 +2783,131 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   if (GEN && a.bc_any) bc_ghost_write(a, 1, 2, i, j, 0, c, h);
   if (a.nprobe) {
     const T e[3] = {T(0), T(0), ez};
+    dft_probe(a, step, c, i, j, 0, e, h);
+  }
+}
+
+// ---------------------------------------------------------------- 2-D TE
+// Ex, Ey, Hz (FieldComponents<TE>, include/FDTDConstants.hpp:852); CPML along
+// x and y; in place.  The curl and CPML terms are those of e_cell_3d /
+// h_cell_3d with the z derivatives and the TM components absent:
+// Dx += dt F_y/dx (Hz - Hz[j-1]), Dy -= dt F_x/dx (Hz - Hz[i-1]),
+// Bz -= dt (F_x/dx (Ey[i+1] - Ey) - F_y/dx (Ex[j+1] - Ex)).
+template <typename T, bool GEN = false>
+__global__ __launch_bounds__(256) void e_pass_te(const StepArgs<T>* __restrict__ ap, int, int s) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = *ap;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
+  const FDTD_G T* __restrict__ Hz = a.H[2];
+  const long step = a.step0 + s;
+  const long c = (long)i + a.sy * j;
+  const T hz = Hz[c], hz_ym = Hz[c - a.sy], hz_xm = Hz[c - 1];
+  T ex = a.E[0][c], ey = a.E[1][c];
+  if (GEN && a.nsrc_eh) {  // the old E without the last step's E source value
+    T v[3] = {ex, ey, T(0)};
+    soft_eh(a, step, 1, c, true, v);
+    ex = v[0];
+    ey = v[1];
+  }
+  const int mid = a.mat[c];
+  const T ie = cell_inv_eps(a, a.mie, mid, i, j, 0);
+  const T dxy = hz - hz_ym;  // Ex, J=y, +1
+  const T dyx = hz - hz_xm;  // Ey, J=x, -1
+  T d[3] = {T(0), T(0), T(0)};
+  d[0] = a.cf[0][1][j] * dxy;
+  d[1] = -(a.cf[0][0][i] * dyx);
+  if (a.nsrc) add_sources(a, step, 0, c, d);
+  const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
+  if (sxi >= 0) {
+    const long o = slab_off(a, 0, sxi, i, j, 0);
+    const T p = a.psi[0][1][0][o];
+    d[1] = fmadd(a.cg[0][0][i], p, d[1]);
+    a.psio[0][1][0][o] = fmadd(a.cb[0][0][i], p, -(a.cc[0][0][i] * dyx));
+  }
+  const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
+  if (syi >= 0) {
+    const long o = slab_off(a, 1, syi, i, j, 0);
+    const T p = a.psi[0][0][1][o];
+    d[0] = fmadd(a.cg[0][1][j], p, d[0]);
+    a.psio[0][0][1][o] = fmadd(a.cb[0][1][j], p, a.cc[0][1][j] * dxy);
+  }
+  if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
+    const bool aF[3] = {true, true, false}, aC[3] = {false, false, true};
+    tfsf_cell<T, 2>(a, a.incH + (long)s * a.auxL, 0, i, j, 0, aF, aC, (const T*)nullptr, d);
+  }
+  if (GEN && a.nplane) plane_cells<T, 2>(a, step, 0, i, j, 0, 3, 4, d);
+  T e[3] = {ex, ey, T(0)};
+  if (ie > T(0)) {
+    e[0] = fmadd(d[0], ie, ex);
+    e[1] = fmadd(d[1], ie, ey);
+  } else if (ie < T(0) && a.Dd[0]) {
+    const MatDev<T>& m = a.mats[mid];
+    const long bq = dbox_index(a, i, j, 0);
+    if (m.fluid) {
+      e[0] = fluid_component<T, 0>(a, m, bq, d[0], true);
+      e[1] = fluid_component<T, 1>(a, m, bq, d[1], true);
+    } else {
+      e[0] = ade_component<T, 0>(a, m, bq, d[0], ex, true);
+      e[1] = ade_component<T, 1>(a, m, bq, d[1], ey, true);
+    }
+  }
+  if (GEN && a.nsrc_eh) soft_eh(a, step, 1, c, false, e);
+  a.Eo[0][c] = e[0];
+  a.Eo[1][c] = e[1];
+  if (GEN && a.bc_any) bc_ghost_write(a, 0, 2, i, j, 0, c, e);
+}
+
+template <typename T, bool GEN = false>
+__global__ __launch_bounds__(256) void h_pass_te(const StepArgs<T>* __restrict__ ap, int, int s) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = *ap;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
+  const FDTD_G T* __restrict__ Ex = a.Eo[0];
+  const FDTD_G T* __restrict__ Ey = a.Eo[1];
+  const long step = a.step0 + s;
+  const long c = (long)i + a.sy * j;
+  const T ex = Ex[c], ey = Ey[c], ex_yp = Ex[c + a.sy], ey_xp = Ey[c + 1];
+  T hz = a.H[2][c];
+  if (GEN && a.nsrc_eh) {  // the old H without the last step's H source value
+    T v[3] = {T(0), T(0), hz};
+    soft_eh(a, step, 3, c, true, v);
+    hz = v[2];
+  }
+  const T im = a.mim[a.mat[c]];
+  const T dzx = ey_xp - ey;  // Hz, J=x, +1
+  const T dzy = ex_yp - ex;  // Hz, J=y, -1
+  T d[3] = {T(0), T(0), T(0)};
+  d[2] = fmadd(a.cf[1][1][j], dzy, -(a.cf[1][0][i] * dzx));
+  if (a.nsrc) add_sources(a, step, 2, c, d);
+  const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
+  if (sxi >= 0) {
+    const long o = slab_off(a, 0, sxi, i, j, 0);
+    const T p = a.psi[1][2][0][o];
+    d[2] = fmadd(-a.cg[1][0][i], p, d[2]);
+    a.psio[1][2][0][o] = fmadd(a.cb[1][0][i], p, a.cc[1][0][i] * dzx);
+  }
+  const int syi = slab_index(j, a.slab_a[1], a.slab_b[1]);
+  if (syi >= 0) {
+    const long o = slab_off(a, 1, syi, i, j, 0);
+    const T p = a.psi[1][2][1][o];
+    d[2] = fmadd(-a.cg[1][1][j], p, d[2]);
+    a.psio[1][2][1][o] = fmadd(a.cb[1][1][j], p, -(a.cc[1][1][j] * dzy));
+  }
+  if (a.tfsf && (i == a.box[0] || i == a.box[1] || j == a.box[2] || j == a.box[3])) {
+    const bool aF[3] = {false, false, true}, aC[3] = {true, true, false};
+    tfsf_cell<T, 2>(a, a.incE + (long)s * a.auxL, 1, i, j, 0, aF, aC, (const T*)nullptr, d);
+  }
+  if (GEN && a.nplane) plane_cells<T, 2>(a, step, 1, i, j, 0, 4, 3, d);
+  T h[3] = {T(0), T(0), fmadd(d[2], im, hz)};
+  if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, h);
+  a.Ho[2][c] = h[2];
+  if (GEN && a.bc_any) bc_ghost_write(a, 1, 2, i, j, 0, c, h);
+  if (a.nprobe) {
+    const T e[3] = {ex, ey, T(0)};
     dft_probe(a, step, c, i, j, 0, e, h);
   }
 }

@@ -14,7 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfdtd_hip.so")
 
-TM, THREED = 2, 3
+TM, THREED, TE = 2, 3, 4
+MODES = {"TM": TM, "TE": TE, "ThreeD": THREED}
 F32, F64 = 0, 1
 D, E, B, H, P, J = 0, 1, 2, 3, 4, 5
 DRUDE, LORENTZ, CONDUCTIVE = 1, 2, 3
@@ -135,15 +136,16 @@ def _f64(a):
 
 
 class Grid:
-    """One simulation on the GPU.  ``mode`` is "TM" or "ThreeD";
+    """One simulation on the GPU.  ``mode`` is "TM", "TE" or "ThreeD";
     ``precision`` "f32" or "f64"."""
 
     def __init__(self, mode, nx, ny, nz=1, dx=1.0, dt=1.0, precision="f64", stream=None):
         L = load_library()
         self.mode = mode
         self.shape = (nz if mode == "ThreeD" else 1, ny, nx)
-        desc = GridDesc(THREED if mode == "ThreeD" else TM, F64 if precision == "f64" else F32, nx, ny,
-                        nz if mode == "ThreeD" else 1, dx, dt)
+        if mode not in MODES:
+            raise ValueError("mode must be one of %s" % ", ".join(MODES))
+        desc = GridDesc(MODES[mode], F64 if precision == "f64" else F32, nx, ny, nz if mode == "ThreeD" else 1, dx, dt)
         h = _p()
         _check(L.fdtd_grid_create(ctypes.byref(desc), ctypes.byref(h)))
         self.h = h
@@ -302,11 +304,11 @@ class Grid:
     def surface_quantities(self, region, static=False):
         """Surface charge density and currents of the current state on `region`
         (x0, x1, y0, y1, z0, z1): array (nq, z, y, x); 3-D rho, Jex, Jey, Jez, Jhx,
-        Jhy, Jhz; TM Jez, Jhx, Jhy.  static=True: the entry point that allocates
+        Jhy, Jhz; TM Jez, Jhx, Jhy; TE rho, Jex, Jey, Jhz.  static=True: the entry point that allocates
         nothing (regions of at most FDTD_SURFACE_STATIC_MAX_CELLS cells, a grid
         already set up on the device)."""
         r = [int(v) for v in region]
-        nq = 7 if self.mode == "ThreeD" else 3
+        nq = {"ThreeD": 7, "TM": 3, "TE": 4}[self.mode]
         shape = (r[5] - r[4], r[3] - r[2], r[1] - r[0])
         out = np.zeros(nq * int(np.prod(shape)))
         fn = load_library().fdtd_surface_quantities_static if static else load_library().fdtd_surface_quantities

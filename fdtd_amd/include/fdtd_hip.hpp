@@ -80,6 +80,9 @@ enum class Orientation : char { MIN = 0, MAX };
 struct TM {
   static constexpr int dim = 2, code = FDTD_TM;
 };
+struct TE {
+  static constexpr int dim = 2, code = FDTD_TE;
+};
 struct ThreeD {
   static constexpr int dim = 3, code = FDTD_3D;
 };
@@ -780,19 +783,24 @@ struct Aux1D {
 
 namespace detail {
 // Polarisations by the reference's rule for the mode (include/Source.hpp:
-// 573-584): TM pol = z, polH = (k_y, -k_x, 0).  ThreeD, which the reference
-// leaves unset, gets pol = z and polH = pol x k = (-k_y, k_x, 0).  (The TM
+// 573-584): TM pol = z, polH = (k_y, -k_x, 0); TE pol = (k_y, -k_x, 0),
+// polH = z.  ThreeD, which the reference leaves unset, gets pol = z and
+// polH = pol x k = (-k_y, k_x, 0).  (The TM
 // rule's polH is the negative of pol x k: with it the incident field the
 // faces subtract is not the line's own plane wave, so the scattered-field
 // region is not left empty; the rule is kept for parity with the reference,
 // and the explicit-polarisation overloads take pol x k instead.)
-inline void default_pol(int dim, const std::array<double, 3>& k, std::array<double, 3>& pol,
+inline void default_pol(int mode, const std::array<double, 3>& k, std::array<double, 3>& pol,
                         std::array<double, 3>& polH) {
   pol = {0, 0, 1};
-  if (dim == 2)
+  if (mode == FDTD_TE) {
+    pol = {k[1], -k[0], 0};
+    polH = {0, 0, 1};
+  } else if (mode == FDTD_TM) {
     polH = {k[1], -k[0], 0};
-  else
+  } else {
     polH = {-k[1], k[0], 0};
+  }
 }
 inline std::array<double, 3> unit(int dim, const std::array<double, 3>& kvec) {
   double K = 0;
@@ -832,7 +840,7 @@ TFSFBox<Mode> make_tfsf_box(YeeGrid<Mode, scalar>& g, std::array<int, 6> box, st
   TFSFBox<Mode> t;
   t.box = box;
   t.k = detail::unit(Mode::dim, kvec);
-  detail::default_pol(Mode::dim, t.k, t.pol, t.polH);
+  detail::default_pol(Mode::code, t.k, t.pol, t.polH);
   for (int d = 0; d < 3; d++) t.origin[d] = d < Mode::dim ? std::floor(0.5 * (box[2 * d] + box[2 * d + 1])) + 0.5 : 0.0;
   t.aux = aux;
   t.loop = &g.loop();
@@ -876,7 +884,7 @@ TFSFPlane<Mode> make_tfsf_plane(YeeGrid<Mode, scalar>& g, std::array<int, 6> cel
   p.side = int(o);
   p.cells = cells;
   p.k = detail::unit(Mode::dim, kvec);
-  detail::default_pol(Mode::dim, p.k, p.pol, p.polH);
+  detail::default_pol(Mode::code, p.k, p.pol, p.polH);
   p.origin = origin;
   p.aux = aux;
   p.loop = &g.loop();

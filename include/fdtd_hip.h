@@ -36,8 +36,9 @@ enum fdtd_status {
 };
 
 /* Modes, as in include/FDTDConstants.hpp:424-445 (TEM exists only as the
- * TFSF incident line inside the library). */
-enum fdtd_mode { FDTD_TM = 2, FDTD_3D = 3 };
+ * TFSF incident line inside the library): TM (Ez, Hx, Hy), TE (Ex, Ey, Hz),
+ * ThreeD. */
+enum fdtd_mode { FDTD_TM = 2, FDTD_3D = 3, FDTD_TE = 4 };
 enum fdtd_precision { FDTD_F32 = 0, FDTD_F64 = 1 };
 /* field codes */
 enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5 };
@@ -69,9 +70,9 @@ enum fdtd_side { FDTD_MIN = 0, FDTD_MAX = 1 };
 typedef struct fdtd_grid fdtd_grid;
 
 typedef struct {
-  int mode;      /* FDTD_TM or FDTD_3D */
+  int mode;      /* FDTD_TM, FDTD_TE or FDTD_3D */
   int precision; /* FDTD_F32 or FDTD_F64: storage and arithmetic type */
-  int nx, ny, nz; /* cells per axis including the outer layer; nz = 1 for TM */
+  int nx, ny, nz; /* cells per axis including the outer layer; nz = 1 for TM and TE */
   double dx;     /* cell size [m] */
   double dt;     /* time step [s] */
 } fdtd_grid_desc;
@@ -225,12 +226,12 @@ int fdtd_get_tile(fdtd_grid* g, int* bx, int* by, int* kc);
 int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc);
 
 /* Surface charge density and surface currents of the current state on the
- * cells of region [x0,x1)x[y0,y1)x[z0,z1) (interior cells; TM: z range
- * [0,1)): replaces UpdateSurfaceChargeDensity / UpdateSurfaceCurrentE /
+ * cells of region [x0,x1)x[y0,y1)x[z0,z1) (interior cells; TM and TE: z
+ * range [0,1)): replaces UpdateSurfaceChargeDensity / UpdateSurfaceCurrentE /
  * UpdateSurfaceCurrentH (include/SurfaceQuantities.hpp:169-356) applied to
  * those cells.  out: quantity-major, x fastest; 3-D 7 quantities rho, Jex,
  * Jey, Jez, Jhx, Jhy, Jhz; TM 3: Jez, Jhx, Jhy (the reference's TM charge
- * density functor does nothing).  FP64 arithmetic; synchronises.  On a
+ * density functor does nothing); TE 4: rho, Jex, Jey, Jhz.  FP64 arithmetic; synchronises.  On a
  * complex grid (Bloch face) these are the quantities of the real part. */
 int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out);
 

@@ -1190,6 +1190,54 @@ static void scen_surface(const std::string& dir) {
     fx.arr("initTM_Ez_Hx_Hy", init, {3, size_t(ny), size_t(nx)});
     fx.arr("sqTM_Jez_Jhx_Jhy", out, {size_t(ny - 2), size_t(nx - 2), 3});
   }
+  // TE: rho, Jex, Jey, Jhz
+  {
+    const int nx = 9, ny = 11;
+    std::vector<CellTE> cells(size_t(nx) * ny);
+    auto at = [&](int i, int j) -> CellTE& { return cells[size_t(i) + size_t(nx) * j]; };
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        CellTE& c = at(i, j);
+        c.mn[0] = i > 0 ? &at(i - 1, j) : nullptr;
+        c.mx[0] = i < nx - 1 ? &at(i + 1, j) : nullptr;
+        c.mn[1] = j > 0 ? &at(i, j - 1) : nullptr;
+        c.mx[1] = j < ny - 1 ? &at(i, j + 1) : nullptr;
+      }
+    SplitMix64 rng(19);
+    std::vector<double> init;
+    for (auto& c : cells) {
+      c.Ex() = rng.uniform() - 0.5;
+      c.Ey() = rng.uniform() - 0.5;
+      c.Hz() = (rng.uniform() - 0.5) * 2e-3;
+    }
+    for (int f = 0; f < 3; f++)
+      for (auto& c : cells) init.push_back(f == 0 ? c.Ex() : f == 1 ? c.Ey() : c.Hz());
+    std::vector<double> out;
+    UpdateSurfaceChargeDensity<TE> rho(dx);
+    UpdateSurfaceCurrentE<TE> je(dx);
+    UpdateSurfaceCurrentH<TE> jh(dx);
+    struct TEView : public SurfaceQuantities<TE> {
+      CellTE* c;
+      double& Ex() { return c->Ex(); }
+      double& Ey() { return c->Ey(); }
+      double& Hz() { return c->Hz(); }
+      CellTE& getNeighborMin(int d) { return c->getNeighborMin(d); }
+      CellTE& getNeighborMax(int d) { return c->getNeighborMax(d); }
+    };
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) {
+        TEView v;
+        v.c = &at(i, j);
+        rho(v);
+        je(v);
+        jh(v);
+        out.insert(out.end(), {v.rho(), v.Jex(), v.Jey(), v.Jhz()});
+      }
+    fx.pi("te_nx", nx);
+    fx.pi("te_ny", ny);
+    fx.arr("initTE_Ex_Ey_Hz", init, {3, size_t(ny), size_t(nx)});
+    fx.arr("sqTE_rho_Jex_Jey_Jhz", out, {size_t(ny - 2), size_t(nx - 2), 4});
+  }
   fx.write(dir);
 }
 

@@ -1298,6 +1298,14 @@ int rc_surface_quantities(const rc_sim* s, const int r[6], double* out) {
           out[4 * cells + q] = -1.0 / dx * (E[2][c + sy] - E[2][c] - E[1][c + sz] + E[1][c]);
           out[5 * cells + q] = -1.0 / dx * (E[0][c + sz] - E[0][c] - E[2][c + 1] + E[2][c]);
           out[6 * cells + q] = -1.0 / dx * (E[1][c + 1] - E[1][c] - E[0][c + sy] + E[0][c]);
+        } else if (s->mode == RC_TE) { /* SurfaceQuantities.hpp:189-198, 253-263, 323-332 */
+          const double* Ex = s->E[0];
+          const double* Ey = s->E[1];
+          const double* Hz = s->H[2];
+          out[0 * cells + q] = RC_EPS0 / dx * (Ex[c] + Ey[c] - Ex[c - 1] - Ey[c - sy]);
+          out[1 * cells + q] = 1.0 / dx * (Hz[c] - Hz[c - sy]);
+          out[2 * cells + q] = 1.0 / dx * (-Hz[c] + Hz[c - 1]);
+          out[3 * cells + q] = -1.0 / dx * (Ey[c + 1] - Ey[c] - Ex[c + sy] + Ex[c]);
         } else {
           const double* Ez = s->E[2];
           const double* Hx = s->H[0];

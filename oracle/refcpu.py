@@ -266,9 +266,10 @@ class Sim:
         lib().rc_apply_boundaries(self.h, ft)
 
     def surface_quantities(self, region):
-        """(nq, z, y, x) array: 3-D rho, Jex, Jey, Jez, Jhx, Jhy, Jhz; TM Jez, Jhx, Jhy."""
+        """(nq, z, y, x) array: 3-D rho, Jex, Jey, Jez, Jhx, Jhy, Jhz; TM Jez, Jhx, Jhy;
+        TE rho, Jex, Jey, Jhz."""
         r = [int(v) for v in region]
-        nq = 7 if self.mode == "ThreeD" else 3
+        nq = {"ThreeD": 7, "TM": 3, "TE": 4}[self.mode]
         shape = (r[5] - r[4], r[3] - r[2], r[1] - r[0])
         out = np.zeros(nq * int(np.prod(shape)))
         assert lib().rc_surface_quantities(self.h, (ctypes.c_int * 6)(*r), _dptr(out)) == 0, region

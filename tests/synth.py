@@ -361,3 +361,60 @@ def grid3d_plane_slab(n=(48, 12, 10), nsteps=140, npml=8, faces=(0, 1), layers=(
         "dfts": [], "tfsf": None, "planes": planes,
         "boundaries": [(ax, sd, "Periodic") for ax in (1, 2) for sd in (0, 1)],
     }
+
+
+def te2d_features(n=(52, 44), nsteps=160, npml=8):
+    """2-D TE (Ex, Ey, Hz): CPML along x, y closed by PEC (min) and PMC (max);
+    a dielectric, a Drude and a Lorentz block; soft sources on Dx, on Ey
+    inside the Lorentz block and on Hz (after the material steps) beside one
+    on Bz; a single TFSF plane on an x-MIN face under an oblique TE wave
+    (pol = (k_y, -k_x, 0), polH = pol x k = z); a DFT monitor of Ex, Ey, Hz."""
+    nx, ny = n
+    dx = 1e-7
+    dt = 0.5 * dx / (C0 * math.sqrt(2.0))
+    matid = np.zeros((1, ny, nx), np.uint8)
+    matid[0, 10:17, 24:31] = 1
+    matid[0, 26:33, 14:21] = 2
+    matid[0, 22:28, 32:38] = 3
+    mats = [(1.0, 1.0, []), (4.0, 1.0, []), (1.5, 1.0, [(1, 0.3 / dt, 0.02 / dt, 1.0)]),
+            (2.0, 1.0, [(2, 0.2 / dt, 0.01 / dt, 0.7)])]
+    f0 = C0 / (20 * dx)
+    tw = 2.0 / f0
+    kvec = [math.cos(0.4), math.sin(0.4), 0.0]
+    pol = (kvec[1], -kvec[0], 0.0)
+    planes = [_plane(0, 0, [12, 13, 6, 38, 0, 1], kvec, [26.5, 22.5, 0.0], 90,
+                     gmsin_signal(nsteps, dt, f0, tw, 3.0 * tw), pol=pol)]
+    sources = [(D, 0, 20, 14, 0, gauss_table(nsteps, 20, 6)),
+               (E, 1, 34, 24, 0, gauss_table(nsteps, 30, 7, scale=1.0)),
+               (H, 2, 17, 29, 0, gauss_table(nsteps, 25, 6, scale=1.0 / IMP0)),
+               (B, 2, 40, 12, 0, gauss_table(nsteps, 15, 5, scale=1e-9))]
+    probes = [(E, 0, 20, 14, 0), (E, 1, 34, 24, 0), (H, 2, 17, 29, 0), (E, 0, 30, 1, 0), (H, 2, 30, ny - 2, 0),
+              (E, 1, 11, 20, 0), (E, 1, 12, 20, 0)]
+    pe, ph = dft_phases(nsteps, dt, np.array([0.8, 1.1]) * f0)
+    return {
+        "name": "te2d_features", "mode": "TE", "nx": nx, "ny": ny, "nz": 1, "dx": dx, "dt": dt, "nsteps": nsteps,
+        "pml": _pml(npml, dx), "pml_axes": [True, False, False], "materials": mats, "matid": matid,
+        "sources": sources, "probes": probes, "tfsf": None, "planes": planes,
+        "dfts": [([4, nx - 4, 3, ny - 3, 0, 1], 1 | 2 | 32, pe, ph)],
+        "boundaries": [(1, 0, "PEC"), (1, 1, "PMC")],
+    }
+
+
+def te2d_bloch(n=(40, 44), nsteps=150, theta=0.9):
+    """2-D TE unit cell: x Bloch-periodic (e^{-/+ i theta} at the min / max face),
+    10-cell CPML along y, a dielectric block, a Dy source and a 2-frequency DFT
+    line of Ex, Ey, Hz across the cell."""
+    nx, ny = n
+    dx = 1e-6
+    dt = 0.5 * dx / (C0 * math.sqrt(2.0))
+    matid = np.zeros((1, ny, nx), np.uint8)
+    matid[0, 14:22, 6:19] = 1
+    f = complex(math.cos(theta), math.sin(theta))
+    return {
+        "name": "te2d_bloch", "mode": "TE", "nx": nx, "ny": ny, "nz": 1, "dx": dx, "dt": dt, "nsteps": nsteps,
+        "pml": _pml(10, dx), "pml_axes": [False, True, False], "materials": [(1.0, 1.0, []), (2.25, 1.0, [])],
+        "matid": matid, "sources": [(D, 1, nx // 3, ny // 2, 0, gauss_table(nsteps, 25, 8))],
+        "probes": [(E, 1, 1, ny // 2, 0), (E, 0, nx - 2, ny // 3, 0), (H, 2, nx // 2, 12, 0), (H, 2, 1, ny - 13, 0)],
+        "dfts": [dft_monitor([1, nx - 1, ny // 2 + 5, ny // 2 + 6, 0, 1], 1 | 2 | 32, dt, nsteps)], "tfsf": None,
+        "bloch": [(0, 0, f.conjugate()), (0, 1, f)],
+    }

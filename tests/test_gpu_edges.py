@@ -35,10 +35,12 @@ def _pair(mode, n, dx, dt, precision, setup):
 def _comps(mode):
     if mode == "TM":
         return [(E, 2), (H, 0), (H, 1)]
+    if mode == "TE":
+        return [(E, 0), (E, 1), (H, 2)]
     return [(E, 0), (E, 1), (E, 2), (H, 0), (H, 1), (H, 2)]
 
 
-@pytest.mark.parametrize("mode", ["ThreeD", "TM"])
+@pytest.mark.parametrize("mode", ["ThreeD", "TM", "TE"])
 @pytest.mark.parametrize("tile", [(64, 4, 2), (256, 1, 4), (128, 2, 1)])
 def test_one_interior_cell(mode, tile):
     n = (3, 3, 3) if mode == "ThreeD" else (3, 3, 1)
@@ -49,11 +51,14 @@ def test_one_interior_cell(mode, tile):
     def setup(s):
         s.set_material(1, 2.5)
         s.set_matid(np.ones((n[2], n[1], n[0]), np.uint8))
-        s.add_source(D, 2, 1, 1, 1 if mode == "ThreeD" else 0, gauss_table(40, 10, 4))
+        ec = 0 if mode == "TE" else 2  # an E component of the mode
+        s.add_source(D, ec, 1, 1, 1 if mode == "ThreeD" else 0, gauss_table(40, 10, 4))
         if mode == "ThreeD":
             s.add_source(B, 0, 1, 1, 1, 1e-9 * gauss_table(40, 14, 4))
-        s.add_probe(E, 2, 1, 1, 1 if mode == "ThreeD" else 0)
-        s.add_dft(*synth.dft_monitor([1, 2, 1, 2, 1, 2] if mode == "ThreeD" else [1, 2, 1, 2, 0, 1], 1 << 2, dt, 40))
+        if mode == "TE":
+            s.add_source(B, 2, 1, 1, 0, 1e-9 * gauss_table(40, 14, 4))
+        s.add_probe(E, ec, 1, 1, 1 if mode == "ThreeD" else 0)
+        s.add_dft(*synth.dft_monitor([1, 2, 1, 2, 1, 2] if mode == "ThreeD" else [1, 2, 1, 2, 0, 1], 1 << ec, dt, 40))
 
     g, o = _pair(mode, n, dx, dt, "f64", setup)
     if mode == "ThreeD" or tile[2] == 1:

@@ -148,3 +148,23 @@ def test_for_each_time_loop_against_reference_fixture():
     ref = a["probe_trace_Ez_Hx_Hy"].reshape(got.shape)
     assert np.abs(ref).max() > 0
     assert rel_l2(got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["small", "tfsf"])
+def test_te_time_loop_through_cpp_host_surface(name):
+    """The reference's TE loop shape (tests/testTE.cpp) through for_each on a
+    YeeGrid<TE>: Dx and Bz soft sources inside the loop, dielectric / Drude /
+    Lorentz functors per material id, or a TFSF box with the reference's TE
+    polarisation stepped by updateD / updateB: the probe traces equal the
+    reference's own (golden fixtures te2d_small, te2d_tfsf) to 1e-12."""
+    exe = os.path.join(os.path.dirname(BIN), "te_loop_host_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/bin/te_loop_host_api not built (run __graft_entry__.build())")
+    p, a = load("te2d_" + name)
+    print(f"[gpu-case] te_loop_host_api {name}", flush=True)
+    r = subprocess.run([exe, name], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
+    ref = a["probe_trace_Ex_Ey_Hz"].reshape(got.shape)
+    assert np.abs(ref).max() > 0
+    assert rel_l2(got, ref) <= 1e-12

@@ -186,7 +186,7 @@ def test_grid_scenarios_bit_exact(name):
 def test_surface_quantities_match_reference_functors():
     """rc_surface_quantities against the reference's UpdateSurfaceChargeDensity /
     UpdateSurfaceCurrentE / UpdateSurfaceCurrentH (include/SurfaceQuantities.hpp)
-    on seeded 3-D and TM field states, bit for bit."""
+    on seeded 3-D, TM and TE field states, bit for bit."""
     p, a = load("surface_quantities")
     nx, ny, nz, dx = int(p["nx"]), int(p["ny"]), int(p["nz"]), p["dx"]
     init = a["init3_E_H"].reshape(6, nz, ny, nx)
@@ -205,4 +205,13 @@ def test_surface_quantities_match_reference_functors():
     sim.set(H, 1, init[2])
     got = sim.surface_quantities([1, tnx - 1, 1, tny - 1, 0, 1])
     want = a["sqTM_Jez_Jhx_Jhy"].reshape(1, tny - 2, tnx - 2, 3)
+    np.testing.assert_array_equal(np.moveaxis(got, 0, -1), want)
+    enx, eny = int(p["te_nx"]), int(p["te_ny"])
+    init = a["initTE_Ex_Ey_Hz"].reshape(3, 1, eny, enx)
+    sim = refcpu.Sim("TE", enx, eny, 1, dx, 1e-15)
+    sim.set(E, 0, init[0])
+    sim.set(E, 1, init[1])
+    sim.set(H, 2, init[2])
+    got = sim.surface_quantities([1, enx - 1, 1, eny - 1, 0, 1])
+    want = a["sqTE_rho_Jex_Jey_Jhz"].reshape(1, eny - 2, enx - 2, 4)
     np.testing.assert_array_equal(np.moveaxis(got, 0, -1), want)
