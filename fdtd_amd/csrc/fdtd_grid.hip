@@ -112,7 +112,8 @@ void pml_idiom_rows(int L, int n, double m, double ma, double smax, double kmax,
   }
 }
 
-constexpr int kChunk = 256;  // steps per incident-table chunk
+constexpr int kChunk = 256;    // steps per incident-table chunk
+constexpr int kIncBlock = 64;  // steps per launch of the adjoint's precomputed incident rows
 // timed kernels: 0 E pass, 1 H pass, 2 aux line, 3 fused pass, 4 DFT,
 // 5 adjoint E pass, 6 adjoint H pass
 constexpr int kNumKernels = 8;
@@ -281,6 +282,14 @@ struct fdtd_grid {
   long hist_step0 = 0, hist_rows = 0;
   double *inc_allE = nullptr, *inc_allH = nullptr;  // incident rows of steps [0, inc_rows)
   long inc_rows = 0;
+  std::vector<hipEvent_t> inc_ev;  // block b (kIncBlock steps) of those rows is written
+
+  // the incident lines' own stream (default priority): the rows of a call's
+  // next chunk are computed there while the current chunk's passes run, into
+  // the other of two row buffers (line_ev: rows of buffer b written; pass_ev:
+  // the passes reading buffer b are done)
+  hipStream_t line_stream = nullptr;
+  hipEvent_t line_ev[2] = {}, pass_ev[2] = {}, start_ev = nullptr;
 
   int ncu = 256;  // compute units of the grid's device (the streamed pass launches a multiple of them)
   Tile tile = {0, 0, 0};
@@ -324,6 +333,12 @@ struct fdtd_grid {
   long kcount[kNumKernels] = {};
 
   ~fdtd_grid() {
+    if (line_stream) {
+      (void)hipStreamSynchronize(line_stream);
+      (void)hipStreamDestroy(line_stream);
+    }
+    for (hipEvent_t e : {line_ev[0], line_ev[1], pass_ev[0], pass_ev[1], start_ev})
+      if (e) (void)hipEventDestroy(e);
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     if (own_stream) (void)hipStreamDestroy(own_stream);
     for (void* p : allocs) (void)hipFree(p);
@@ -964,8 +979,10 @@ int ensure_dft_im(fdtd_grid* g) {
   return 0;
 }
 
-// advance one incident line nsteps from step0, its rows into outE / outH
-int run_aux_line(fdtd_grid* g, const AuxLine& l, double* outE, double* outH, long step0, int nsteps) {
+// advance one incident line nsteps from step0, its rows into outE / outH, on
+// stream s (the grid's stream by default)
+int run_aux_line(fdtd_grid* g, const AuxLine& l, double* outE, double* outH, long step0, int nsteps,
+                 hipStream_t s = nullptr, bool on_grid = true) {
   fdtdk::AuxArgs a;
   a.L = l.L;
   a.npml = 15;
@@ -973,13 +990,13 @@ int run_aux_line(fdtd_grid* g, const AuxLine& l, double* outE, double* outH, lon
   a.dx = g->d.dx;
   a.eps0 = kEps0;
   a.mu0 = kMu0;
-  double* s = l.state;
-  a.Dz = s;
-  a.Ez = s + l.L;
-  a.By = s + 2 * l.L;
-  a.Hy = s + 3 * l.L;
-  a.psiE = s + 4 * l.L;
-  a.psiH = s + 5 * l.L;
+  double* st = l.state;
+  a.Dz = st;
+  a.Ez = st + l.L;
+  a.By = st + 2 * l.L;
+  a.Hy = st + 3 * l.L;
+  a.psiE = st + 4 * l.L;
+  a.psiH = st + 5 * l.L;
   a.pE = l.pE;
   a.pH = l.pH;
   a.signal = l.sig;
@@ -988,8 +1005,59 @@ int run_aux_line(fdtd_grid* g, const AuxLine& l, double* outE, double* outH, lon
   a.nsteps = nsteps;
   a.outE = outE;
   a.outH = outH;
-  int threads = std::min(1024, ((l.L + 63) / 64) * 64);
-  return launch(g, 2, fdtdk::aux_line, dim3(1), dim3(threads), a);
+  // cells per thread: the fewest that fit the line in one workgroup
+  int cpt = 1;
+  while (cpt < 8 && (l.L + cpt - 1) / cpt > fdtdk::kAuxThreads) cpt *= 2;
+  if ((l.L + cpt - 1) / cpt > fdtdk::kAuxThreads || 2 * a.npml + 5 > fdtdk::kAuxPmlMax)
+    return fail(FDTD_EINVAL, "incident line of %d cells exceeds the line kernel", l.L);
+  const dim3 block(unsigned(((l.L + cpt - 1) / cpt + 63) / 64 * 64));
+  auto go = [&](auto kern) -> int {
+    if (on_grid) return launch(g, 2, kern, dim3(1), block, a);  // timed with the grid's kernels
+    hipLaunchKernelGGL(kern, dim3(1), block, 0, s, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+  };
+  switch (cpt) {
+    case 1: return go(fdtdk::aux_line<1>);
+    case 2: return go(fdtdk::aux_line<2>);
+    case 4: return go(fdtdk::aux_line<4>);
+    default: return go(fdtdk::aux_line<8>);
+  }
+}
+
+// the incident lines' stream and its events, made on first use
+int ensure_line_stream(fdtd_grid* g) {
+  if (g->line_stream) return 0;
+  HIP_TRY(hipStreamCreateWithFlags(&g->line_stream, hipStreamNonBlocking));
+  for (int b = 0; b < 2; b++) {
+    HIP_TRY(hipEventCreateWithFlags(&g->line_ev[b], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&g->pass_ev[b], hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventCreateWithFlags(&g->start_ev, hipEventDisableTiming));
+  return 0;
+}
+
+// The stream the lines run on: their own, or the grid's while kernels are
+// timed (fdtd_set_profiling), so that every launch is attributed in order.
+hipStream_t lines_on(const fdtd_grid* g) { return g->prof ? g->stream : g->line_stream; }
+
+// rows of steps [step0, step0 + n) of every incident line into row buffer b,
+// on the lines' stream; line_ev[b] marks them written
+int line_rows(fdtd_grid* g, int b, long step0, int n) {
+  hipStream_t ls = lines_on(g);
+  const bool timed = ls == g->stream;
+  int rc = 0;
+  if (g->tfsf)
+    rc = run_aux_line(g, g->aux, g->aux.incE + size_t(b) * kChunk * g->aux.L,
+                      g->aux.incH + size_t(b) * kChunk * g->aux.L, step0, n, ls, timed);
+  for (int p = 0; p < g->nplane && !rc; p++) {
+    const AuxLine& l = g->plane_aux[p];
+    rc = run_aux_line(g, l, l.incE + size_t(b) * kChunk * l.L, l.incH + size_t(b) * kChunk * l.L, step0, n, ls,
+                      timed);
+  }
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(g->line_ev[b], ls));
+  return 0;
 }
 
 // the chunk's rows of every incident line of the grid (the box's and the planes')
@@ -1116,9 +1184,20 @@ int step_impl(fdtd_grid* g, int nsteps) {
     block = dim3(t.bx, t.by, 1);
     grid = dim3((g->d.nx + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, 1);
   }
+  // incident lines stepped here (not precomputed): chunk c's rows go to row
+  // buffer c % 2, written on the lines' stream while chunk c - 1 runs
+  const bool lines = (g->tfsf || g->nplane) && !(g->tfsf && g->inc_rows);
+  if (lines && nsteps > 0) {
+    int rc = ensure_line_stream(g);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(g->start_ev, g->stream));  // earlier chunks are done with both buffers
+    HIP_TRY(hipStreamWaitEvent(lines_on(g), g->start_ev, 0));
+    if ((rc = line_rows(g, 0, g->step, std::min(kChunk, nsteps)))) return rc;
+  }
   int done = 0;
-  while (done < nsteps) {
+  for (int cidx = 0; done < nsteps; cidx++) {
     const int chunk = std::min(kChunk, nsteps - done);
+    const int lb = cidx % 2;
     a.step0 = g->step;
     gput(a.incE, g->aux.incE);
     gput(a.incH, g->aux.incH);
@@ -1127,9 +1206,16 @@ int step_impl(fdtd_grid* g, int nsteps) {
       if (g->step + chunk > g->inc_rows) return fail(FDTD_ESTATE, "step beyond the precomputed incident rows");
       gput(a.incE, g->inc_allE + g->step * g->aux.L);
       gput(a.incH, g->inc_allH + g->step * g->aux.L);
-    } else if (g->tfsf || g->nplane) {
-      int rc = run_aux(g, g->step, chunk);
-      if (rc) return rc;
+      const size_t blk = size_t((g->step + chunk - 1) / kIncBlock);
+      if (blk < g->inc_ev.size()) HIP_TRY(hipStreamWaitEvent(g->stream, g->inc_ev[blk], 0));
+    } else if (lines) {
+      HIP_TRY(hipStreamWaitEvent(g->stream, g->line_ev[lb], 0));
+      gput(a.incE, g->aux.incE + size_t(lb) * kChunk * g->aux.L);
+      gput(a.incH, g->aux.incH + size_t(lb) * kChunk * g->aux.L);
+      for (int p = 0; p < g->nplane; p++) {
+        gput(a.plane[p].incE, g->plane_aux[p].incE + size_t(lb) * kChunk * g->plane_aux[p].L);
+        gput(a.plane[p].incH, g->plane_aux[p].incH + size_t(lb) * kChunk * g->plane_aux[p].L);
+      }
     }
     // stage this chunk's arguments: variant 0 reads the current state,
     // variant 1 (fused pass only) the second copy; captured chunks alternate
@@ -1347,6 +1433,14 @@ int step_impl(fdtd_grid* g, int nsteps) {
     HIP_TRY(hipEventRecord(g->args_ev[slot], g->stream));
     g->args_ev_used[slot] = true;
     done += chunk;
+    if (lines && done < nsteps) {
+      // the next chunk's rows, into the other buffer once the chunk before
+      // this one has finished reading it
+      HIP_TRY(hipEventRecord(g->pass_ev[lb], g->stream));
+      if (cidx >= 1) HIP_TRY(hipStreamWaitEvent(lines_on(g), g->pass_ev[1 - lb], 0));
+      int rc = line_rows(g, 1 - lb, g->step, std::min(kChunk, nsteps - done));
+      if (rc) return rc;
+    }
   }
   return 0;
 }
@@ -1485,6 +1579,8 @@ struct AdjointScope {
   explicit AdjointScope(fdtd_grid* gg) : g(gg), main(gg->stream) {}
   ~AdjointScope() {
     g->stream = main;
+    if (g->line_stream) (void)hipStreamSynchronize(g->line_stream);  // precomputed rows still being written
+    g->inc_ev.clear();
     if (rs) {
       (void)hipStreamSynchronize(rs);
       (void)hipStreamDestroy(rs);
@@ -1536,9 +1632,24 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
   int rc = 0;
   g->force_two_pass = true;
   if (g->tfsf) {
+    // every step's incident rows, in blocks of kIncBlock steps on the lines'
+    // stream: the forward's chunks wait only for the blocks they read
     if ((rc = sc.get((void**)&g->inc_allE, size_t(nsteps) * g->aux.L * 8))) return rc;
     if ((rc = sc.get((void**)&g->inc_allH, size_t(nsteps) * g->aux.L * 8))) return rc;
-    if ((rc = run_aux_line(g, g->aux, g->inc_allE, g->inc_allH, 0, nsteps))) return rc;
+    if ((rc = ensure_line_stream(g))) return rc;
+    hipStream_t ls = lines_on(g);
+    HIP_TRY(hipEventRecord(g->start_ev, g->stream));
+    HIP_TRY(hipStreamWaitEvent(ls, g->start_ev, 0));
+    for (long b0 = 0; b0 < nsteps; b0 += kIncBlock) {
+      const int n = int(std::min<long>(kIncBlock, nsteps - b0));
+      const size_t off = size_t(b0) * g->aux.L;
+      if ((rc = run_aux_line(g, g->aux, g->inc_allE + off, g->inc_allH + off, b0, n, ls, ls == g->stream))) return rc;
+      hipEvent_t e;
+      HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      sc.evs.push_back(e);
+      HIP_TRY(hipEventRecord(e, ls));
+      g->inc_ev.push_back(e);
+    }
     g->inc_rows = nsteps;
   }
   const int S = (nsteps + K - 1) / K;
@@ -2053,8 +2164,9 @@ int aux_line_init(fdtd_grid* g, int aux_ncells, const double* signal, int nsigna
   if (!rc) rc = g->alloc((void**)&l.pE, size_t(4) * L * 8);
   if (!rc) rc = g->alloc((void**)&l.pH, size_t(4) * L * 8);
   if (!rc) rc = g->alloc((void**)&l.sig, size_t(std::max(nsignal, 1)) * 8);
-  if (!rc) rc = g->alloc((void**)&l.incE, size_t(kChunk) * L * 8);
-  if (!rc) rc = g->alloc((void**)&l.incH, size_t(kChunk) * L * 8);
+  // two chunk buffers of rows (a chunk's rows are written while the chunk before runs)
+  if (!rc) rc = g->alloc((void**)&l.incE, size_t(2) * kChunk * L * 8);
+  if (!rc) rc = g->alloc((void**)&l.incH, size_t(2) * kChunk * L * 8);
   if (rc) return rc;
   // Aux1D's own CPML: PMLParameterModel(dx) defaults, 15 layers (Source.hpp:362, 380-389)
   std::vector<double> e, h;

@@ -3133,7 +3133,15 @@ __global__ __launch_bounds__(1024) void tm_block(const StepArgs<T>* __restrict__
 // ---------------------------------------------------------------- Aux1D line
 // One workgroup advances the 1-D TEM incident line `nsteps` times
 // (Aux1D::time_step, include/Source.hpp:413-435) and writes, per step, the
-// line's Hy before the advance and its Ez after it.  Always FP64.
+// line's Hy before the advance and its Ez after it.  Always FP64, in the
+// operation order of the reference's update (no contraction).
+// Thread t owns the CPT consecutive cells [t CPT, t CPT + CPT): their Dz, By,
+// Ez, Hy and curl coefficients stay in registers for the whole launch; only
+// the first Ez and the last Hy of each thread cross to its neighbours, through
+// LDS, with barriers that wait for LDS alone (the row stores stay in flight).
+// The CPML cells (the first npml+2 and the last npml+3) keep psi and their
+// B, C, G coefficients in an LDS table; the signal is staged in LDS per block
+// of kAuxSigBlock steps.  Cells past L hold zero and are never updated.
 struct AuxArgs {
   int L, npml;
   double dt, dx, eps0, mu0;
@@ -3153,42 +3161,124 @@ struct AuxArgs {
   double* outH;
 };
 
-__global__ __launch_bounds__(1024) void aux_line(AuxArgs a) {
-  const int L = a.L, tid = threadIdx.x, nt = blockDim.x;
+constexpr int kAuxThreads = 1024;
+constexpr int kAuxPmlMax = 64;  // CPML cells of a line: 2 npml + 5 (npml <= 29)
+constexpr int kAuxSigBlock = 256;
+
+template <int CPT>
+__global__ __launch_bounds__(kAuxThreads) void aux_line(AuxArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double e_first[kAuxThreads], h_last[kAuxThreads];
+  __shared__ double pml_psi[2][kAuxPmlMax], pml_c[2][3][kAuxPmlMax];  // [E/H][B, C, G]
+  __shared__ double sig[kAuxSigBlock];
+  const int L = a.L, t = threadIdx.x, nt = blockDim.x;
   const double dt = a.dt, dx = a.dx;
-  const int src = a.npml + 5;
+  const int src = a.npml + 5, plo = a.npml + 2, phi = L - (a.npml + 3);
+  // table slot of a CPML cell, -1 for the others
+  auto pslot = [&](int n) { return n < plo ? n : (n >= phi && n < L ? plo + (n - phi) : -1); };
+  double D[CPT], B[CPT], E[CPT], H[CPT], fE[CPT], fH[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int n = t * CPT + q;
+    const bool in = n < L;
+    D[q] = in ? a.Dz[n] : 0.0;
+    B[q] = in ? a.By[n] : 0.0;
+    E[q] = in ? a.Ez[n] : 0.0;
+    H[q] = in ? a.Hy[n] : 0.0;
+    fE[q] = in ? a.pE[4 * n + 2] : 0.0;
+    fH[q] = in ? a.pH[4 * n + 2] : 0.0;
+    const int ps = in ? pslot(n) : -1;
+    if (ps >= 0) {
+      pml_psi[0][ps] = a.psiE[n];
+      pml_psi[1][ps] = a.psiH[n];
+      pml_c[0][0][ps] = a.pE[4 * n + 0];
+      pml_c[0][1][ps] = a.pE[4 * n + 1];
+      pml_c[0][2][ps] = a.pE[4 * n + 3];
+      pml_c[1][0][ps] = a.pH[4 * n + 0];
+      pml_c[1][1][ps] = a.pH[4 * n + 1];
+      pml_c[1][2][ps] = a.pH[4 * n + 3];
+    }
+  }
+  h_last[t] = H[CPT - 1];
+  // does this thread own a CPML cell or the source cell (wave-uniform tests keep
+  // the other waves' loops free of the table code)
+  const bool has_pml = t * CPT < plo || t * CPT + CPT - 1 >= phi;
+  const bool has_src = src >= t * CPT && src < t * CPT + CPT;
   for (int s = 0; s < a.nsteps; s++) {
-    for (int n = tid; n < L; n += nt) a.outH[(long)s * L + n] = a.Hy[n];
-    __syncthreads();
-    for (int n = tid; n < L; n += nt) {
-      double hm = n > 0 ? a.Hy[n - 1] : 0.0;
-      double diff = a.Hy[n] - hm;
-      double D = a.Dz[n] + dt * (a.pE[4 * n + 2] / dx * diff);
-      if (n < a.npml + 2 || n >= L - (a.npml + 3)) {
-        D += dt * a.psiE[n] * a.pE[4 * n + 3];
-        a.psiE[n] = a.pE[4 * n + 0] * a.psiE[n] + a.pE[4 * n + 1] / dx * diff;
+    if (s % kAuxSigBlock == 0) {
+      __syncthreads();
+      for (int x = t; x < kAuxSigBlock; x += nt) {
+        const long gs = a.step0 + s + x;
+        sig[x] = gs < a.nsignal ? a.signal[gs] : 0.0;
       }
-      a.Dz[n] = D;
-      double Ev = D / a.eps0;
-      long gs = a.step0 + s;
-      if (n == src) Ev += gs < a.nsignal ? a.signal[gs] : 0.0;
-      a.Ez[n] = Ev;
+      __syncthreads();
     }
-    __syncthreads();
-    for (int n = tid; n < L; n += nt) {
-      double ep = n + 1 < L ? a.Ez[n + 1] : 0.0;
-      double diff = ep - a.Ez[n];
-      double Bv = a.By[n] + dt * (a.pH[4 * n + 2] / dx * diff);
-      if (n < a.npml + 2 || n >= L - (a.npml + 3)) {
-        Bv -= dt * a.psiH[n] * a.pH[4 * n + 3];
-        a.psiH[n] = a.pH[4 * n + 0] * a.psiH[n] - a.pH[4 * n + 1] / dx * diff;
+    FDTD_G double* oH = (FDTD_G double*)a.outH + (long)s * L;
+    FDTD_G double* oE = (FDTD_G double*)a.outE + (long)s * L;
+#pragma unroll
+    for (int q = 0; q < CPT; q++)
+      if (t * CPT + q < L) oH[t * CPT + q] = H[q];
+    // E: Dz += dt F/dx (Hy - Hy[n-1]); CPML; Ez = Dz/eps0 (+ the signal at the source cell)
+    double hm = t > 0 ? h_last[t - 1] : 0.0;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int n = t * CPT + q;
+      const double diff = H[q] - hm;
+      hm = H[q];
+      if (n < L) {
+        D[q] = D[q] + dt * (fE[q] / dx * diff);
+        if (has_pml) {
+          const int ps = pslot(n);
+          if (ps >= 0) {
+            const double p = pml_psi[0][ps];
+            D[q] += dt * p * pml_c[0][2][ps];
+            pml_psi[0][ps] = pml_c[0][0][ps] * p + pml_c[0][1][ps] / dx * diff;
+          }
+        }
+        E[q] = D[q] / a.eps0;
+        if (has_src && n == src) E[q] += sig[s % kAuxSigBlock];
       }
-      a.By[n] = Bv;
-      a.outE[(long)s * L + n] = a.Ez[n];
     }
-    __syncthreads();
-    for (int n = tid; n < L; n += nt) a.Hy[n] = a.By[n] / a.mu0;
-    __syncthreads();
+    e_first[t] = E[0];
+    s_bar_lds();
+    // B: By += dt F/dx (Ez[n+1] - Ez); CPML; Hy = By/mu0
+    const double enext = t + 1 < nt ? e_first[t + 1] : 0.0;
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+      const int n = t * CPT + q;
+      const double ep = q + 1 < CPT ? E[q + 1] : enext;
+      if (n < L) {
+        const double diff = ep - E[q];
+        B[q] = B[q] + dt * (fH[q] / dx * diff);
+        if (has_pml) {
+          const int ps = pslot(n);
+          if (ps >= 0) {
+            const double p = pml_psi[1][ps];
+            B[q] -= dt * p * pml_c[1][2][ps];
+            pml_psi[1][ps] = pml_c[1][0][ps] * p - pml_c[1][1][ps] / dx * diff;
+          }
+        }
+        oE[n] = E[q];
+        H[q] = B[q] / a.mu0;
+      }
+    }
+    h_last[t] = H[CPT - 1];
+    s_bar_lds();
+  }
+#pragma unroll
+  for (int q = 0; q < CPT; q++) {
+    const int n = t * CPT + q;
+    if (n < L) {
+      a.Dz[n] = D[q];
+      a.By[n] = B[q];
+      a.Ez[n] = E[q];
+      a.Hy[n] = H[q];
+      const int ps = pslot(n);
+      if (ps >= 0) {
+        a.psiE[n] = pml_psi[0][ps];
+        a.psiH[n] = pml_psi[1][ps];
+      }
+    }
   }
 }
 
