@@ -14,6 +14,13 @@
 // (own cell) in the adjoint H pass, the psiE adjoint (own cell) in the next
 // adjoint E pass, so that no pass writes what another thread of it reads.
 // d J / d eps_r = -(1/eps_r) sum_n adj(E^{n+1}) . (E^{n+1} - E^n) = -eps0 sum_n nu . dE.
+// Cells of pole materials (Drude, Lorentz, conductive) carry D, P_p, J_p
+// instead of E: their step D' = D + d, P_p' = P_p + dt J_p (conductive:
+// (P_p + cD D')/(1 + cD)), E' = (D' - sum P_p') / (eps_r eps0),
+// J_p' = cJ J_p + cE E' + cP P_p' is linear, and its transpose (adj_pole_cell)
+// keeps the adjoints of D, P_p, J_p over the dispersive box and leaves in nu
+// the adjoint of the curl increment d, which is what the adjoint H pass and
+// the psiE adjoint read of every cell (ie adj(E) in a constant cell).
 // The reference has no adjoint; this is the build's own.
 #pragma once
 #include "fdtd_kernels.hpp"
@@ -22,8 +29,12 @@ namespace fdtdk {
 
 template <typename T>
 struct AdjArgs {
-  FDTD_G T* nu[3];          // ie * adjoint of E
+  FDTD_G T* nu[3];          // ie * adjoint of E (pole cells: adjoint of the curl increment)
   FDTD_G T* mu[3];          // im * adjoint of H
+  // adjoints of the pole cells' D, P_p, J_p over the dispersive box (null: none)
+  FDTD_G T* aD[3];
+  FDTD_G T* aP[kMaxPoles][3];
+  FDTD_G T* aJ[kMaxPoles][3];
   FDTD_G T* lpsi[2][3][3];  // adjoint of psi[ft][I][J] (the forward slab layout)
   FDTD_G double* grad;      // [ndes]: d J / d eps_r of the design cells
   double eps0;
@@ -72,14 +83,58 @@ __device__ FDTD_COLD void adj_objective(const AdjArgs<T>& q, long step, int i, i
   }
 }
 
+// Reverse of one pole cell's material step, component by component: e is the
+// adjoint of its new E gathered this step (objective and H pass); the D, P_p,
+// J_p adjoints are read and written over the dispersive box; out receives the
+// adjoint of the curl increment (= that of the new D).
+template <typename T>
+__device__ FDTD_COLD void adj_pole_cell(const StepArgs<T>& a0, const AdjArgs<T>& q0, int mid, long bq, const T e[3],
+                                        T out[3]) {
+  const StepArgs<T>& a = (const StepArgs<T>&)cold_args(a0);
+  const AdjArgs<T>& q = cold_adj(q0);
+  const MatDev<T>& m = a.mats[mid];
+#pragma unroll
+  for (int I = 0; I < 3; I++) {
+    T eb = e[I];
+    T jn[kMaxPoles], pt[kMaxPoles];
+#pragma unroll
+    for (int p = 0; p < kMaxPoles; p++) {
+      if (p >= m.np) continue;
+      const T jb = q.aJ[p][I][bq];
+      eb += m.cE[p] * jb;  // J' = cJ J + cE E' + cP P'
+      jn[p] = sizeof(T) == 4 ? jb - m.cJm[p] * jb : m.cJ[p] * jb;
+      pt[p] = q.aP[p][I][bq] + m.cP[p] * jb;
+    }
+    const T nb = eb * m.inv_eps;  // E' = (D' - sum P') / (eps_r eps0)
+    T db = q.aD[I][bq] + nb;
+#pragma unroll
+    for (int p = 0; p < kMaxPoles; p++) {
+      if (p >= m.np) continue;
+      pt[p] -= nb;
+      if (m.cD[p] != T(0)) {  // P' = (P + cD D') / (1 + cD)
+        const T t = pt[p] / (T(1) + m.cD[p]);
+        db += m.cD[p] * t;
+        pt[p] = t;
+      } else {  // P' = P + dt J
+        jn[p] += a.dt * pt[p];
+      }
+      q.aP[p][I][bq] = pt[p];
+      q.aJ[p][I][bq] = jn[p];
+    }
+    q.aD[I][bq] = db;  // D' = D + d
+    out[I] = db;
+  }
+}
+
 // Adjoint E pass of step `step` over one column of cells.  hist: the step's
-// row of design-cell E increments (null: no design box).
+// row of design-cell E increments (null: no design box).  ADE: the grid has
+// pole materials (their cells run adj_pole_cell).
 // The column's slab indices and CPML coefficients along x and y are read
 // once, before the march; the z coefficients come from zf, the block's rows
 // of dt F/dx (E pass: H's of planes k0 - 1 .. k1 - 1; H pass: E's of planes
 // k0 .. k1), so no uniform global load inside the march waits behind the
 // field loads.
-template <typename T>
+template <typename T, bool ADE = false>
 __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<T>& q, const T* tie, const T* zf,
                                             int i, int j, int k0, int k1, long step, const T* __restrict__ hist) {
   if (i < 1 || i >= a.nx - 1 || j >= a.ny - 1) return;
@@ -181,9 +236,19 @@ __device__ __forceinline__ void adj_e_cells(const StepArgs<T>& a, const AdjArgs<
     w02c = w02;
     T src[3] = {T(0), T(0), T(0)};
     if (ocol && k >= q.obj_r[4] && k < q.obj_r[5]) adj_objective(cold_adj(q), step, i, j, k, src);
-    n0 += ie * (src[0] + (w12 - w12m) - (w21 - w21m));
-    n1 += ie * (src[1] + (w20 - w20m) - (w02 - w02m));
-    n2 += ie * (src[2] + (w01 - w01m) - (w10 - w10m));
+    if (ADE && ie < T(0)) {
+      const T e[3] = {src[0] + (w12 - w12m) - (w21 - w21m), src[1] + (w20 - w20m) - (w02 - w02m),
+                      src[2] + (w01 - w01m) - (w10 - w10m)};
+      T o[3];
+      adj_pole_cell(a, q, mid, dbox_index(a, i, j, k), e, o);
+      n0 = o[0];
+      n1 = o[1];
+      n2 = o[2];
+    } else {
+      n0 += ie * (src[0] + (w12 - w12m) - (w21 - w21m));
+      n1 += ie * (src[1] + (w20 - w20m) - (w02 - w02m));
+      n2 += ie * (src[2] + (w01 - w01m) - (w10 - w10m));
+    }
     N0[c] = n0;
     N1[c] = n1;
     N2[c] = n2;
@@ -305,7 +370,7 @@ __device__ __forceinline__ void adj_h_cells(const StepArgs<T>& a, const AdjArgs<
 // launched like the forward passes: ntx x nty x nkc tiles (kc <= kAdjMaxKc),
 // XCD-aware order, the H pass walking each XCD's range in reverse
 constexpr int kAdjMaxKc = 64;
-template <typename T>
+template <typename T, bool ADE>
 __device__ __forceinline__ void adj_e_pass_body(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
                                                 int kc, int ntx, int nty, int nkc, long step,
                                                 const T* __restrict__ hist) {
@@ -317,8 +382,8 @@ __device__ __forceinline__ void adj_e_pass_body(const StepArgs<T>* __restrict__ 
   for (int x = threadIdx.x + blockDim.x * threadIdx.y; x <= k1 - k0; x += blockDim.x * blockDim.y)
     zf[x] = ap->cf[1][2][k0 - 1 + x];
   stage_materials(*ap, sm);  // (its barrier also covers zf)
-  adj_e_cells(*ap, *qp, sm.ie, (const T*)zf, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y, k0,
-              k1, step, hist);
+  adj_e_cells<T, ADE>(*ap, *qp, sm.ie, (const T*)zf, bx * blockDim.x + threadIdx.x, 1 + by * blockDim.y + threadIdx.y,
+                      k0, k1, step, hist);
 }
 template <typename T>
 __device__ __forceinline__ void adj_h_pass_body(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
@@ -335,11 +400,11 @@ __device__ __forceinline__ void adj_h_pass_body(const StepArgs<T>* __restrict__ 
               k1);
 }
 
-template <typename T>
+template <typename T, bool ADE = false>
 __global__ __launch_bounds__(256) void adj_e_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,
                                                      int kc, int ntx, int nty, int nkc, long step,
                                                      const T* __restrict__ hist) {
-  adj_e_pass_body(ap, qp, kc, ntx, nty, nkc, step, hist);
+  adj_e_pass_body<T, ADE>(ap, qp, kc, ntx, nty, nkc, step, hist);
 }
 template <typename T>
 __global__ __launch_bounds__(256) void adj_h_pass_3d(const StepArgs<T>* __restrict__ ap, const AdjArgs<T>* __restrict__ qp,

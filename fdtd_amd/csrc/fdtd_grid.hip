@@ -1563,6 +1563,11 @@ void state_arrays(fdtd_grid* g, std::vector<std::pair<void*, size_t>>& out) {
   for (int c = 0; c < 3; c++) {
     add(g->E[c]);
     add(g->H[c]);
+    add(g->Dd[c]);  // pole cells: D, P_p, J_p over the dispersive box
+    for (int p = 0; p < FDTD_MAX_POLES; p++) {
+      add(g->Pd[p][c]);
+      add(g->Jd[p][c]);
+    }
   }
   for (int ft = 0; ft < 2; ft++)
     for (int I = 0; I < 3; I++)
@@ -1745,6 +1750,16 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
         auto it = std::find(g->allocs.begin(), g->allocs.end(), p);
         if ((rc = sc.get((void**)&q.lpsi[ft][I][Jx], g->alloc_bytes[it - g->allocs.begin()]))) return rc;
       }
+  if (g->max_np > 0) {  // adjoints of the pole cells' D, P_p, J_p
+    const long nb = long(g->dbox[1] - g->dbox[0]) * (g->dbox[3] - g->dbox[2]) * (g->dbox[5] - g->dbox[4]);
+    for (int c = 0; c < 3; c++) {
+      if ((rc = sc.get((void**)&q.aD[c], size_t(nb) * g->esz))) return rc;
+      for (int p = 0; p < g->max_np; p++) {
+        if ((rc = sc.get((void**)&q.aP[p][c], size_t(nb) * g->esz))) return rc;
+        if ((rc = sc.get((void**)&q.aJ[p][c], size_t(nb) * g->esz))) return rc;
+      }
+    }
+  }
   if ((rc = sc.get((void**)&q.grad, size_t(nd) * 8))) return rc;
   fdtdk::StepArgs<T> a;
   fill_args<T>(g, a);
@@ -1874,7 +1889,8 @@ int adjoint_impl(fdtd_grid* g, int nsteps, int K, double history_mb, int didx, i
     if (s >= 1 && s < nck && (rc = recompute(s - 1))) return rc;
     for (long n = n0 + len - 1; n >= n0 && !rc; n--) {
       const T* hrow = (const T*)(hb + size_t(n - n0) * row_bytes);
-      rc = launch(g, 5, fdtdk::adj_e_pass_3d<T>, grd, block, ap, qp, t.kc, ntx, nty, nkc, n, hrow);
+      rc = g->max_np > 0 ? launch(g, 5, fdtdk::adj_e_pass_3d<T, true>, grd, block, ap, qp, t.kc, ntx, nty, nkc, n, hrow)
+                         : launch(g, 5, fdtdk::adj_e_pass_3d<T>, grd, block, ap, qp, t.kc, ntx, nty, nkc, n, hrow);
       if (!rc) rc = launch(g, 6, fdtdk::adj_h_pass_3d<T>, grd, block, ap, qp, t.kc, ntx, nty, nkc);
     }
     if (rc) return rc;
@@ -2723,7 +2739,12 @@ int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, do
     return fail(FDTD_EINVAL, "objective components must be E components recorded by the monitor");
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
-  if (g->max_np > 0) return fail(FDTD_EINVAL, "the adjoint gradient does not cover dispersive materials");
+  for (long c = 0; g->max_np > 0 && c < g->ncell; c++) {
+    const MatHost& m = g->mats[g->mat_host[c]];
+    if (m.np > 0 && (m.mag || m.fluid))
+      return fail(FDTD_EINVAL, "the adjoint gradient covers Drude, Lorentz and conductive poles, not magnetized or "
+                               "fluid Drude materials");
+  }
   if (g->nbc) return fail(FDTD_EINVAL, "the adjoint gradient does not cover boundary conditions");
   if (g->nsrc_eh || g->nplane)
     return fail(FDTD_EINVAL, "the adjoint gradient covers D / B sources and the TFSF box, not E / H sources or single planes");

@@ -298,16 +298,19 @@ int fdtd_set_design(fdtd_grid* g, const int box[6], const double* eps_r);
  * comp_mask (bit c = E_c, recorded by that monitor) and its frequencies of
  * |F|^2, and grad[cell] = dJ/d eps_r for every design cell (same order as
  * fdtd_set_design).  Reverse steps are the transpose of the forward step
- * (curl, CPML psi, per-cell material; sources and TFSF terms are
+ * (curl, CPML psi, per-cell material including the P / J recursions of
+ * Drude, Lorentz and conductive poles; sources and TFSF terms are
  * state-independent).  The design cells' E increments are kept for segments
  * of checkpoint_every steps: the last segments that fit history_mb megabytes
  * (at least one) are recorded during the forward pass, every earlier segment
- * restarts from a checkpoint of E, H and psi taken at its first step and is
+ * restarts from a checkpoint of E, H, psi and the pole cells' D, P, J taken
+ * at its first step and is
  * recomputed just before its reverse steps (each step over the design box
  * grown by one cell per step left in the segment, on grids without
  * boundary ghost writes or complex fields).  On return the grid holds the
- * forward state after nsteps steps and the monitor holds F.  3-D,
- * non-dispersive grids only. */
+ * forward state after nsteps steps and the monitor holds F.  3-D grids;
+ * magnetized and fluid Drude materials are refused (FDTD_EINVAL); design
+ * cells are non-dispersive (their own eps_r). */
 int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, double history_mb, int dft,
                               int comp_mask, double* objective, double* grad);
 /* Tile of the gradient's reverse sweep and of its recomputed segments (the

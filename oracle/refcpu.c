@@ -290,6 +290,7 @@ struct rc_sim {
    * adjoints are per-step accumulators of the derived fields */
   double *Db[3], *Bb[3], *Eb[3], *Hb[3];
   double* Pb[2][3][3];
+  double *PPb[RC_MAXPOLES][3], *JPb[RC_MAXPOLES][3]; /* pole P and J adjoints (dispersive cells) */
   int obj_dft, obj_mask;
   /* boundary condition per [axis][side] (RC_BC_*; 0: ghost layer left alone) */
   int bc[3][2];
@@ -414,6 +415,11 @@ void rc_destroy(rc_sim* s) {
   for (int ft = 0; ft < 2; ft++)
     for (int I = 0; I < 3; I++)
       for (int J = 0; J < 3; J++) free(s->Pb[ft][I][J]);
+  for (int p = 0; p < RC_MAXPOLES; p++)
+    for (int c = 0; c < 3; c++) {
+      free(s->PPb[p][c]);
+      free(s->JPb[p][c]);
+    }
   free(s->fluid_ne);
   free(s->fluid_nu);
   rc_destroy(s->im);
@@ -1319,11 +1325,13 @@ int rc_surface_quantities(const rc_sim* s, const int r[6], double* out) {
 }
 
 /* ---------------------------------------------------------------- adjoint
- * Build-defined (the reference has no adjoint).  Reverse mode of rc_step for
- * non-dispersive cells, op by op in the D/B form above: every forward update
- * y += a x becomes x_bar += a y_bar, run in the reverse order of the forward
- * ops; sources and TFSF terms do not depend on the state and drop out.  Each
- * material op E = D/(eps_r eps0) contributes dJ/d eps_r += E_bar (-E/eps_r).
+ * Build-defined (the reference has no adjoint).  Reverse mode of rc_step, op
+ * by op in the D/B form above: every forward update y += a x becomes
+ * x_bar += a y_bar, run in the reverse order of the forward ops; sources and
+ * TFSF terms do not depend on the state and drop out.  Each material op
+ * E = D/(eps_r eps0) contributes dJ/d eps_r += E_bar (-E/eps_r); the cells of
+ * pole materials (Drude, Lorentz, conductive) run the reverse of their P / J
+ * recursion (adj_pole_cell), with P and J adjoints carried across steps.
  * Written independently of the device adjoint, which works on E and H
  * increments with gathered (not scattered) transposes. */
 
@@ -1364,6 +1372,55 @@ void rc_adjoint_reset(rc_sim* s) {
           if (!s->Pb[ft][I][J]) s->Pb[ft][I][J] = malloc(sizeof(double) * s->ncell);
           memset(s->Pb[ft][I][J], 0, sizeof(double) * s->ncell);
         }
+  for (int p = 0; p < s->npoles_alloc; p++)
+    for (int c = 0; c < 3; c++)
+      if (s->actE[c]) {
+        if (!s->PPb[p][c]) s->PPb[p][c] = malloc(sizeof(double) * s->ncell);
+        if (!s->JPb[p][c]) s->JPb[p][c] = malloc(sizeof(double) * s->ncell);
+        memset(s->PPb[p][c], 0, sizeof(double) * s->ncell);
+        memset(s->JPb[p][c], 0, sizeof(double) * s->ncell);
+      }
+}
+
+/* Reverse of the pole material step of one component of one dispersive cell
+ * (the forward's op order in step_e: P_p (conductive: implicit), num = D -
+ * sum P, E = num/(eps_r eps0), J_p per kind), op by op from the last:
+ * Eb (the E adjoint gathered this step) is consumed; D, P_p and J_p
+ * adjoints carry across steps. */
+static void adj_pole_cell(rc_sim* s, const rc_mat* m, int I, size_t c) {
+  const double dt = s->dt;
+  double eb = s->Eb[I][c];
+  for (int p = m->np - 1; p >= 0; p--) {
+    double* jb = &s->JPb[p][I][c];
+    double* pb = &s->PPb[p][I][c];
+    const double wgt_eps0 = m->weight[p] * RC_EPS0;
+    if (m->kind[p] == 3) continue;
+    if (m->kind[p] == 1) { /* J = J (1-b)/(1+b) + w^2/dt wgt eps0 E */
+      const double b = m->gamma[p] * dt * 0.5, w = dt * m->freq[p];
+      eb += w * w / dt * wgt_eps0 * *jb;
+      *jb = *jb * (1.0 - b) / (1.0 + b);
+    } else { /* J = bm/b J + freq w/b wgt eps0 E - freq w/b P */
+      const double b = 1.0 + m->gamma[p] * dt, bm = 1.0 - m->gamma[p] * dt, w = m->freq[p] * dt;
+      eb += m->freq[p] * w / b * wgt_eps0 * *jb;
+      *pb += -(m->freq[p] * w / b) * *jb;
+      *jb = bm / b * *jb;
+    }
+  }
+  const double numb = eb / (m->eps * RC_EPS0); /* E = num/(eps_r eps0) */
+  s->Db[I][c] += numb;                         /* num = D - sum_p P_p */
+  for (int p = 0; p < m->np; p++) s->PPb[p][I][c] -= numb;
+  for (int p = m->np - 1; p >= 0; p--) {
+    double* pb = &s->PPb[p][I][c];
+    if (m->kind[p] == 3) { /* P = (P + f D)/(1 + f) */
+      const double f = dt * m->freq[p] / (m->eps * RC_EPS0);
+      const double t = *pb / (1.0 + f);
+      s->Db[I][c] += f * t;
+      *pb = t;
+    } else { /* P += dt J */
+      s->JPb[p][I][c] += dt * *pb;
+    }
+  }
+  s->Eb[I][c] = 0.0;
 }
 
 /* Reverse of forward step n.  w: objective weights 2F of monitor obj_dft
@@ -1454,15 +1511,21 @@ static void adj_step(rc_sim* s, long n, const double* w, const double* Edes, dou
           }
     }
   }
-  /* E = D/(eps_r eps0): D_bar += E_bar/(eps_r eps0), dJ/d eps_r += E_bar (-E/eps_r) */
+  /* E = D/(eps_r eps0): D_bar += E_bar/(eps_r eps0), dJ/d eps_r += E_bar (-E/eps_r);
+   * dispersive cells: the reverse of their pole step */
   for (int k = lo[2]; k < hi[2]; k++)
     for (int j = lo[1]; j < hi[1]; j++)
       for (int i = lo[0]; i < hi[0]; i++) {
         const size_t c = cidx(s, i, j, k);
         const double eps = cell_eps(s, i, j, k);
         const long dl = design_local(s, i, j, k);
+        const rc_mat* m = &s->mat[s->matid[c]];
         for (int I = 0; I < 3; I++) {
           if (!s->actE[I]) continue;
+          if (dl < 0 && m->np && s->npoles_alloc) {
+            adj_pole_cell(s, m, I, c);
+            continue;
+          }
           if (dl >= 0 && grad && Edes) grad[dl] += s->Eb[I][c] * (-Edes[(size_t)I * nd + dl] / eps);
           s->Db[I][c] += s->Eb[I][c] / (eps * RC_EPS0);
         }
@@ -1551,10 +1614,14 @@ void rc_sync_derived(rc_sim* s) {
       }
 }
 
+/* adjoint of D (field 0), B (2), psi E / H (6 / 7, comp 3 I + J), pole P / J
+ * (8 / 9, comp 3 p + I) */
 static double* adj_ptr(const rc_sim* s, int field, int comp) {
   if (field == 0 && comp >= 0 && comp < 3) return s->Db[comp];
   if (field == 2 && comp >= 0 && comp < 3) return s->Bb[comp];
   if ((field == 6 || field == 7) && comp >= 0 && comp < 9) return s->Pb[field - 6][comp / 3][comp % 3];
+  if ((field == 8 || field == 9) && comp >= 0 && comp < 3 * RC_MAXPOLES)
+    return field == 8 ? s->PPb[comp / 3][comp % 3] : s->JPb[comp / 3][comp % 3];
   return NULL;
 }
 
@@ -1577,10 +1644,13 @@ int rc_adjoint_eps_gradient(rc_sim* s, int nsteps, int dft, int comp_mask, doubl
   if (comp_mask < 1 || comp_mask > 7 || (comp_mask & ~s->dft[dft].mask)) return -1;
   int lo[3], hi[3];
   interior(s, lo, hi);
+  /* pole materials are transposed; magnetized and fluid models are not */
   for (int k = lo[2]; k < hi[2]; k++)
     for (int j = lo[1]; j < hi[1]; j++)
-      for (int i = lo[0]; i < hi[0]; i++)
-        if (s->npoles_alloc && s->mat[s->matid[cidx(s, i, j, k)]].np > 0 && design_local(s, i, j, k) < 0) return -1;
+      for (int i = lo[0]; i < hi[0]; i++) {
+        const rc_mat* m = &s->mat[s->matid[cidx(s, i, j, k)]];
+        if (s->npoles_alloc && m->np > 0 && (m->mag || m->fluid) && design_local(s, i, j, k) < 0) return -1;
+      }
   const long nd = (long)(s->dsg[1] - s->dsg[0]) * (s->dsg[3] - s->dsg[2]) * (s->dsg[5] - s->dsg[4]);
   double* hist = calloc((size_t)nsteps * 3 * nd, sizeof(double));
   for (int n = 0; n < nsteps; n++) {

@@ -124,7 +124,7 @@ void rc_sync_derived(rc_sim* s);
  * comp_mask and its frequencies of |F|^2, with respect to every design cell's eps_r. */
 int rc_adjoint_eps_gradient(rc_sim* s, int nsteps, int dft, int comp_mask, double* objective, double* grad);
 /* Transposed step, for the dot-product test: adjoint state D_bar (field 0), B_bar (2), psi_bar (6 E, 7 H;
- * comp = 3 I + J).  reset zeroes it; step applies the transpose of the homogeneous forward step n;
+ * comp = 3 I + J), pole P_bar / J_bar (8 / 9; comp = 3 p + I).  reset zeroes it; step applies the transpose of the homogeneous forward step n;
  * finish folds the adjoint of the step's input H = B/(mu_r mu0) into B_bar. */
 void rc_adjoint_reset(rc_sim* s);
 void rc_adjoint_step(rc_sim* s, long n);

@@ -169,11 +169,14 @@ def batch_c4(nsims=64, nsteps=1500, lo=64, hi=160, seed=4):
     return out
 
 
-def grid3d_adjoint(n=128, nsteps=3000, design=(64, 64, 16), npml=10, monitor=64, seed=5):
+def grid3d_adjoint(n=128, nsteps=3000, design=(64, 64, 16), npml=10, monitor=64, seed=5, poles=False):
     """Inverse-design gradient workload: x-propagating, Ez-polarised continuous
     wave (TFSF, one frequency f0 = c0/(20 dx)) through a design box of per-cell
     eps_r seeded U[1,12] centred in the grid; objective sum |F[Ez](f0)|^2 over a
-    monitor x-plane of monitor x monitor cells behind the design box."""
+    monitor x-plane of monitor x monitor cells behind the design box.
+    poles: pole materials on the wave's path: a Lorentz block before the design
+    box, a Drude slab between it and the monitor, a conductive + Lorentz block
+    beside it."""
     nx = ny = nz = n
     dx = 1e-6
     dt = 0.5 * dx / (C0 * math.sqrt(3.0))
@@ -193,10 +196,19 @@ def grid3d_adjoint(n=128, nsteps=3000, design=(64, 64, 16), npml=10, monitor=64,
     m = min(monitor, ny - 2 * inset - 2, nz - 2 * inset - 2)
     my0, mz0 = (ny - m) // 2, (nz - m) // 2
     pe, ph = dft_phases(nsteps, dt, [f0])
+    mats = [(1.0, 1.0, [])]
+    matid = np.zeros((nz, ny, nx), np.uint8)
+    if poles:
+        w0 = 2 * math.pi * f0
+        mats += [(1.0, 1.0, [(DRUDE, 1.2 * w0, 0.1 * w0, 1.0)]), (2.0, 1.0, [(LORENTZ, 0.8 * w0, 0.05 * w0, 1.0)]),
+                 (1.5, 1.0, [(3, 0.05 * EPS0 / dt, 0.0, 1.0), (LORENTZ, 1.3 * w0, 0.08 * w0, 1.0)])]
+        matid[z0 - 1:z0 + wz + 1, y0 - 1:y0 + wy + 1, x0 + wx:mx] = 1       # Drude slab before the monitor
+        matid[z0:z0 + wz, y0:y0 + wy, max(inset, x0 - 3):x0 - 1] = 2       # Lorentz block before the design box
+        matid[z0:z0 + wz, y0 + wy + 1:y0 + wy + 3, x0:x0 + wx] = 3         # conductive + Lorentz beside it
     return {
         "name": f"grid3d_adjoint_{n}", "mode": "ThreeD", "nx": nx, "ny": ny, "nz": nz, "dx": dx, "dt": dt,
-        "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, True, True], "materials": [(1.0, 1.0, [])],
-        "matid": np.zeros((nz, ny, nx), np.uint8), "sources": [], "probes": [],
+        "nsteps": nsteps, "pml": _pml(npml, dx), "pml_axes": [True, True, True], "materials": mats,
+        "matid": matid, "sources": [], "probes": [],
         "tfsf": dict(box=box, kvec=[1.0, 0.0, 0.0], pol=[0.0, 0.0, 1.0], polH=[0.0, 1.0, 0.0], origin=origin,
                      aux_ncells=ext, signal=cw_signal(nsteps, dt, f0)),
         "dfts": [([mx, mx + 1, my0, my0 + m, mz0, mz0 + m], 4, pe, ph)],

@@ -3,8 +3,11 @@ has no adjoint) without any GPU:
 
 * dot-product test <A v, w> = <v, A^T w> of one homogeneous step, block by
   block (D, B, psiE, psiH), so that no block's scale hides another's error;
+* the same test with pole materials (Drude, Lorentz, conductive + Lorentz),
+  their P and J among the blocks;
 * central finite differences (FP64, h = 1e-3) of the eps gradient of a
-  CPML + TFSF continuous-wave scenario with a design box;
+  CPML + TFSF continuous-wave scenario with a design box, without and with
+  pole materials on the wave's path;
 * a design box holding the material table's values steps bitwise like the table;
 * the light-cone argument of the checkpointed recompute: updating step m only on
   the design box grown by L - m leaves the box's E at step L bitwise exact, and
@@ -49,38 +52,87 @@ def _interior(shape):
     return m
 
 
-@pytest.mark.parametrize("src", ["D", "B", "psiE", "psiH"])
-@pytest.mark.parametrize("dst", ["D", "B", "psiE", "psiH"])
-def test_adjoint_step_dot_product(src, dst):
+def _dot_sim_poles():
+    """_dot_sim with pole materials: a Drude, a Lorentz and a conductive + Lorentz
+    block beside a dielectric one (coefficients of order one at this dt)."""
     sc = _dot_sim()
+    dt = sc["dt"]
+    sc["materials"] = [(1.0, 1.0, []), (2.0, 1.0, [(1, 0.3 / dt, 0.05 / dt, 1.0)]),
+                       (1.5, 1.2, [(2, 0.2 / dt, 0.02 / dt, 1.0)]),
+                       (3.0, 0.8, [(3, 0.4 * 3.0 * 8.854e-12 / dt, 0.0, 1.0), (2, 0.25 / dt, 0.03 / dt, 1.0)])]
+    return sc
+
+
+POLE_P, POLE_J, P_BAR, J_BAR = 4, 5, 8, 9
+
+
+def _run_dot(sc, src, dst):
     sim = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
     blocks = _blocks(sim)
     inner = _interior(sim.shape)
-    rng = np.random.default_rng(hash((src, dst)) % (1 << 32))
-    # v: random in block `src` (interior cells), zero elsewhere; H, E derived
+    # pole state: P and J of every pole of every component, on the cells of pole materials only
+    npole = max(len(m[2]) for m in sc["materials"])
+    has = np.array([len(m[2]) > 0 for m in sc["materials"]])[sc["matid"]]
+    masks = {}
+    for name, f in (("P", POLE_P), ("J", POLE_J)):
+        blocks[name] = [(f, c, p) for p in range(npole) for c in range(3)]
+    def mask(name):
+        return inner & has if name in ("P", "J") else inner
+    rng = np.random.default_rng(hash((src, dst, len(sc["materials"]))) % (1 << 32))
+
+    def fset(key, a):
+        f, c = key[0], key[1]
+        sim.set(f, c, a, *key[2:])
+
+    def fget(key):
+        return sim.get(key[0], key[1], *key[2:])
+
+    def adj_code(key):
+        if len(key) == 3:  # pole P / J: adjoint fields 8 / 9, comp 3 p + I
+            return (P_BAR if key[0] == POLE_P else J_BAR), 3 * key[2] + key[1]
+        return key
+
+    # v: random in block `src`, zero elsewhere; H, E derived
     v = {}
-    for f, c in blocks[src]:
-        a = np.where(inner, rng.standard_normal(sim.shape), 0.0)
-        v[(f, c)] = a
-        sim.set(f, c, a)
+    for key in blocks[src]:
+        a = np.where(mask(src), rng.standard_normal(sim.shape), 0.0)
+        v[key] = a
+        fset(key, a)
     sim.sync_derived()
     sim.step(1)
     w = {}
     lhs = 0.0
-    for f, c in blocks[dst]:
-        a = np.where(inner, rng.standard_normal(sim.shape), 0.0)
-        w[(f, c)] = a
-        lhs += float(np.sum(sim.get(f, c) * a))
+    for key in blocks[dst]:
+        a = np.where(mask(dst), rng.standard_normal(sim.shape), 0.0)
+        w[key] = a
+        lhs += float(np.sum(fget(key) * a))
     sim.adjoint_reset()
-    for (f, c), a in w.items():
-        sim.set_adjoint(f, c, a)
+    for key, a in w.items():
+        sim.set_adjoint(*adj_code(key), a)
     sim.adjoint_step(0)
     sim.adjoint_finish()
-    rhs = sum(float(np.sum(a * sim.get_adjoint(f, c))) for (f, c), a in v.items())
+    rhs = sum(float(np.sum(a * sim.get_adjoint(*adj_code(key)))) for key, a in v.items())
     # scale of the terms summed: a mistake in a transposed coefficient moves rhs by one of them
-    terms = sum(float(np.sum(np.abs(a * sim.get_adjoint(f, c)))) for (f, c), a in v.items())
+    terms = sum(float(np.sum(np.abs(a * sim.get_adjoint(*adj_code(key))))) for key, a in v.items())
     assert terms > 0 or lhs == 0
     assert abs(lhs - rhs) <= 1e-12 * max(terms, 1e-300), (src, dst, lhs, rhs, terms)
+    return terms
+
+
+@pytest.mark.parametrize("src", ["D", "B", "psiE", "psiH"])
+@pytest.mark.parametrize("dst", ["D", "B", "psiE", "psiH"])
+def test_adjoint_step_dot_product(src, dst):
+    _run_dot(_dot_sim(), src, dst)
+
+
+@pytest.mark.parametrize("src", ["D", "B", "psiE", "P", "J"])
+@pytest.mark.parametrize("dst", ["D", "B", "psiH", "P", "J"])
+def test_adjoint_step_dot_product_pole_materials(src, dst):
+    """The same test on a grid with Drude, Lorentz and conductive + Lorentz
+    cells, with their P and J as blocks of the state (on those cells)."""
+    terms = _run_dot(_dot_sim_poles(), src, dst)
+    if src in ("P", "J") and dst in ("B", "P", "J"):
+        assert terms > 0  # P and J reach the new E, hence B, P and J (not D): a dropped term would show
 
 
 def _grad_scenario():
@@ -95,8 +147,10 @@ def _objective(sc, eps):
     return float(np.sum(F * F))
 
 
-def test_oracle_gradient_matches_central_differences():
-    sc = _grad_scenario()
+@pytest.mark.parametrize("poles", [False, True])
+def test_oracle_gradient_matches_central_differences(poles):
+    sc = synth.grid3d_adjoint(n=24, nsteps=140, design=(4, 4, 3), npml=4, monitor=8, poles=True) if poles \
+        else _grad_scenario()
     box, eps = sc["design"]
     sim = apply(refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"]), sc)
     J, g = sim.adjoint_eps_gradient(sc["nsteps"], 0, 4)
