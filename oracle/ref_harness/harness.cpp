@@ -142,6 +142,11 @@ typedef Nb<FluidCell<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolari
 typedef Nb<FluidCell<YeeCell<YeeFields<TE, double, std::array>, DoublePolarization<double>,
                              VacuumMagnetization<double>, NewPML<TE, double>>>>
     CellTE;
+// a 3-D cell that also carries the magnetization M and magnetic current K of
+// the magnetic-side pole models (DoubleMagnetization, DispersiveMaterials.hpp:86-110)
+typedef Nb<FluidCell<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<double>,
+                             DoubleMagnetization<double>, NewPML<ThreeD, double>>>>
+    Cell3M;
 
 // material kinds: 0 vacuum, 1 constant, 2 Drude, 3 Lorentz, 4 conductive (freq = the conductivity),
 // 5 magnetized Drude with the bias (bx, by, bz) [T], 6 self-magnetized Drude (the cell's own B),
@@ -173,6 +178,25 @@ void material_E(C& c, const Mat& m, double dt) {
     case 7: (FluidDrudeUpdate<Mode, FieldType::Electric>(dt))(c); break;
     case 8: MagnetizedFluidDrudeUpdate<Mode, FieldType::Electric>(m.bx, m.by, m.bz, dt)(c); break;
     case 9: (SelfMagnetizedFluidDrudeUpdate<Mode, FieldType::Electric>(dt))(c); break;
+  }
+}
+
+// the magnetic side of a material (the reference's functors on FieldType::Magnetic,
+// which run the same updates on B, H, M, K with mu0 for eps0): kinds 0 vacuum,
+// 1 constant (mu = mu_r), 2 Drude, 3 Lorentz, 4 conductive (freq = the magnetic conductivity)
+struct MatM {
+  int kind;
+  double mu, freq, gamma, delta;
+};
+
+template <class Mode, class C>
+void material_H(C& c, const MatM& m, double dt) {
+  switch (m.kind) {
+    case 0: VacuumUpdate<Mode, FieldType::Magnetic>()(c); break;
+    case 1: ConstantUpdate<Mode, FieldType::Magnetic>(m.mu)(c); break;
+    case 2: DrudeUpdate<Mode, FieldType::Magnetic>(m.mu, m.freq, m.gamma, dt)(c); break;
+    case 3: LorentzUpdate<Mode, FieldType::Magnetic>(m.mu, m.delta, m.freq, m.gamma, dt)(c); break;
+    case 4: ConductiveUpdate<Mode, FieldType::Magnetic>(m.mu, m.freq, dt)(c); break;
   }
 }
 
@@ -1036,6 +1060,155 @@ static double gauss_pulse(int n, double t0, double spread) {
   return std::exp(-a * a);
 }
 
+// ------------------------------------------------------------ scenario: magnetic-side poles
+// 3-D grid, CPML on every face, material pairs whose magnetic side is a Drude,
+// Lorentz or conductive model (the reference's functors on FieldType::Magnetic,
+// stepping B, H, M, K), beside electric Drude / Lorentz / constant sides; a
+// Gaussian on Dz and one on Bx.
+static void scen_magnetic(const std::string& dir) {
+  const int nx = 22, ny = 20, nz = 18, nsteps = 130;
+  const double dx = 1e-6, dt = 0.5 * dx / (c0 * std::sqrt(3.0));
+  PMLParameterModel dm(dx);
+  const PmlSpec pml = {5, 3, 1, dm.msMax, 5, 0.05};
+  std::vector<Mat> me = {{0, 1, 0, 0, 0}, {1, 2.0, 0, 0, 0}, {3, 1.5, 0.2 / dt, 0.02 / dt, 0.6}, {1, 1.0, 0, 0, 0},
+                         {2, 1.2, 0.25 / dt, 0.03 / dt, 0}, {1, 3.0, 0, 0, 0}};
+  std::vector<MatM> mh = {{0, 1, 0, 0, 0},           {2, 1.0, 0.3 / dt, 0.02 / dt, 0}, {3, 1.5, 0.15 / dt, 0.01 / dt, 0.5},
+                          {4, 1.2, 0.04 * 1.2 * mu0 / dt, 0, 0}, {1, 2.0, 0, 0, 0},   {2, 2.5, 0.2 / dt, 0.05 / dt, 0}};
+  std::vector<int> matid(size_t(nx) * ny * nz, 0);
+  auto id = [&](int i, int j, int k) { return size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k); };
+  auto fill = [&](int x0, int x1, int y0, int y1, int z0, int z1, int m) {
+    for (int k = z0; k < z1; k++)
+      for (int j = y0; j < y1; j++)
+        for (int i = x0; i < x1; i++) matid[id(i, j, k)] = m;
+  };
+  fill(7, 11, 6, 10, 6, 11, 1);
+  fill(12, 16, 7, 12, 7, 10, 2);
+  fill(8, 13, 11, 14, 9, 13, 3);
+  fill(13, 16, 13, 16, 5, 9, 4);
+  fill(5, 8, 11, 15, 11, 14, 5);
+  std::vector<Cell3M> cells(size_t(nx) * ny * nz);
+  auto at = [&](int i, int j, int k) -> Cell3M& { return cells[id(i, j, k)]; };
+  for (int k = 1; k < nz - 1; k++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++) {
+        Cell3M& c = at(i, j, k);
+        c.mn[0] = &at(i - 1, j, k);
+        c.mx[0] = &at(i + 1, j, k);
+        c.mn[1] = &at(i, j - 1, k);
+        c.mx[1] = &at(i, j + 1, k);
+        c.mn[2] = &at(i, j, k - 1);
+        c.mx[2] = &at(i, j, k + 1);
+      }
+  place_pml(nx, pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++) {
+        if (e)
+          at(idx, j, k).setPMLParameters<FieldType::Electric, Dir::X>(K, Sg, A, dt);
+        else
+          at(idx, j, k).setPMLParameters<FieldType::Magnetic, Dir::X>(K, Sg, A, dt);
+      }
+  });
+  place_pml(ny, pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int k = 0; k < nz; k++)
+      for (int i = 0; i < nx; i++) {
+        if (e)
+          at(i, idx, k).setPMLParameters<FieldType::Electric, Dir::Y>(K, Sg, A, dt);
+        else
+          at(i, idx, k).setPMLParameters<FieldType::Magnetic, Dir::Y>(K, Sg, A, dt);
+      }
+  });
+  place_pml(nz, pml, [&](int idx, bool e, double K, double Sg, double A) {
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        if (e)
+          at(i, j, idx).setPMLParameters<FieldType::Electric, Dir::Z>(K, Sg, A, dt);
+        else
+          at(i, j, idx).setPMLParameters<FieldType::Magnetic, Dir::Z>(K, Sg, A, dt);
+      }
+  });
+  const int si = 10, sj = 9, sk = 8, bi = 14, bj = 10, bk = 8;
+  std::vector<double> sD, sB;
+  for (int n = 0; n < nsteps; n++) {
+    sD.push_back(eps0 * gauss_pulse(n, 20, 6));
+    sB.push_back(1e-9 * gauss_pulse(n, 30, 7));
+  }
+  const std::vector<std::array<int, 3>> probes = {{10, 9, 8}, {14, 10, 8}, {9, 8, 8}, {14, 9, 8}, {10, 12, 11}, {14, 14, 7}, {6, 13, 12}, {3, 3, 3}};
+  std::vector<double> ptrace;
+  auto interior = [&](auto&& fn) {
+    for (int k = 1; k < nz - 1; k++)
+      for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) fn(i, j, k);
+  };
+  for (int n = 0; n < nsteps; n++) {
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dx>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dy>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Dz>(dt, dx)(at(i, j, k)); });
+    at(si, sj, sk).Dz() += sD[n];
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Electric, Dir::X>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Electric, Dir::Y>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Electric, Dir::Z>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { material_E<ThreeD>(at(i, j, k), me[matid[id(i, j, k)]], dt); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Bx>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, By>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { YeeUpdate<ThreeD, Bz>(dt, dx)(at(i, j, k)); });
+    at(bi, bj, bk).Bx() += sB[n];
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Magnetic, Dir::X>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Magnetic, Dir::Y>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { UpdatePML<ThreeD, FieldType::Magnetic, Dir::Z>(dt, dx)(at(i, j, k)); });
+    interior([&](int i, int j, int k) { material_H<ThreeD>(at(i, j, k), mh[matid[id(i, j, k)]], dt); });
+    for (auto& p : probes) {
+      Cell3M& c = at(p[0], p[1], p[2]);
+      ptrace.insert(ptrace.end(), {c.Ex(), c.Ey(), c.Ez(), c.Hx(), c.Hy(), c.Hz()});
+    }
+  }
+  Fixture fx;
+  fx.name = "grid3d_magnetic";
+  fx.ps("mode", "ThreeD");
+  fx.pi("nx", nx);
+  fx.pi("ny", ny);
+  fx.pi("nz", nz);
+  fx.pi("nsteps", nsteps);
+  fx.p("dx", dx);
+  fx.p("dt", dt);
+  put_pml(fx, pml);
+  fx.pi("src_i", si);
+  fx.pi("src_j", sj);
+  fx.pi("src_k", sk);
+  fx.arr("src", sD, {sD.size()});
+  fx.arr("bsrc_comp_i_j_k", {0.0, double(bi), double(bj), double(bk)}, {4});
+  fx.arr("bsrc", sB, {sB.size()});
+  std::vector<double> mt, mm;
+  for (auto& m : me) mt.insert(mt.end(), {double(m.kind), m.eps, m.freq, m.gamma, m.delta});
+  for (auto& m : mh) mm.insert(mm.end(), {double(m.kind), m.mu, m.freq, m.gamma, m.delta});
+  fx.arr("mats_kind_eps_freq_gamma_delta", mt, {me.size(), 5});
+  fx.arr("mmats_kind_mu_freq_gamma_delta", mm, {mh.size(), 5});
+  std::vector<size_t> sh = {size_t(nz), size_t(ny), size_t(nx)};
+  std::vector<double> mid(matid.begin(), matid.end());
+  fx.arr("matid", mid, sh);
+  std::vector<double> pr;
+  for (auto& p : probes) pr.insert(pr.end(), {double(p[0]), double(p[1]), double(p[2])});
+  fx.arr("probes", pr, {probes.size(), 3});
+  fx.arr("probe_trace_E_H", ptrace, {size_t(nsteps), probes.size(), 6});
+  const char* names[6] = {"Ex", "Ey", "Ez", "Hx", "Hy", "Hz"};
+  for (int f = 0; f < 6; f++) {
+    std::vector<double> v;
+    for (auto& c : cells)
+      v.push_back(f == 0 ? c.Ex() : f == 1 ? c.Ey() : f == 2 ? c.Ez() : f == 3 ? c.Hx() : f == 4 ? c.Hy() : c.Hz());
+    fx.arr(names[f], v, sh);
+  }
+  std::vector<double> de, dh;
+  for (size_t q = 0; q < cells.size(); q++) {
+    Cell3M& c = cells[q];
+    if (me[matid[q]].kind >= 2)
+      de.insert(de.end(), {double(q), c.Dx(), c.Dy(), c.Dz(), c.Px(), c.Py(), c.Pz(), c.Jx(), c.Jy(), c.Jz()});
+    if (mh[matid[q]].kind >= 2)
+      dh.insert(dh.end(), {double(q), c.Bx(), c.By(), c.Bz(), c.Mx(), c.My(), c.Mz(), c.Kx(), c.Ky(), c.Kz()});
+  }
+  fx.arr("dispersive_q_D_P_J", de, {de.size() / 10, 10});
+  fx.arr("magnetic_q_B_M_K", dh, {dh.size() / 10, 10});
+  fx.write(dir);
+}
+
 // ------------------------------------------------------------ scenario: material map XML
 // The reference's MaterialMap / MaterialPair (include/MaterialMap.hpp:45-284)
 // holding one material of every kind the device path covers, written by its
@@ -1594,6 +1767,7 @@ int main(int argc, char** argv) {
   scen_surface(dir);
   scen_material_map(dir);
   scen_aux1d(dir);
+  scen_magnetic(dir);
 
   // 2D TM, small, non-square, CPML x+y, two dielectric blocks, one Drude and
   // one Lorentz block, Gaussian soft source on Dz.

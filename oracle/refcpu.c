@@ -221,6 +221,10 @@ typedef struct {
   int mag;        /* magnetized Drude: 0 none, 1 fixed bias, 2 the cell's own B */
   double bias[3]; /* [T] */
   int fluid;      /* fluid Drude: omega_p, gamma from the cell's ne, nu_m */
+  /* magnetic side: Drude (1), Lorentz (2), conductive (3) poles on B, H, M, K */
+  int mnp;
+  int mkind[RC_MAXPOLES];
+  double mfreq[RC_MAXPOLES], mgamma[RC_MAXPOLES], mweight[RC_MAXPOLES];
 } rc_mat;
 
 typedef struct {
@@ -265,6 +269,8 @@ struct rc_sim {
   rc_mat mat[RC_MAXMAT];
   double *P[RC_MAXPOLES][3], *J[RC_MAXPOLES][3];
   int npoles_alloc;
+  double *M[RC_MAXPOLES][3], *K[RC_MAXPOLES][3]; /* magnetization and magnetic current per pole */
+  int mpoles_alloc;
   rc_src* src;
   int nsrc;
   int tfsf;
@@ -381,6 +387,8 @@ void rc_destroy(rc_sim* s) {
     for (int p = 0; p < RC_MAXPOLES; p++) {
       free(s->P[p][c]);
       free(s->J[p][c]);
+      free(s->M[p][c]);
+      free(s->K[p][c]);
     }
   }
   for (int ft = 0; ft < 2; ft++)
@@ -457,6 +465,32 @@ int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, co
     s->npoles_alloc = npoles;
   }
   if (s->im) rc_set_material(s->im, id, eps_r, mu_r, npoles, poles);
+  return 0;
+}
+
+/* the magnetic side of material id: the reference's pole functors on
+ * FieldType::Magnetic (DispersiveMaterials.hpp, the *Call structs with
+ * val = mu0), which step B, H, M, K as the electric side steps D, E, P, J */
+int rc_set_material_magnetic(rc_sim* s, int id, int npoles, const double* poles) {
+  if (id < 0 || id >= RC_MAXMAT || npoles < 0 || npoles > RC_MAXPOLES) return -1;
+  rc_mat* m = &s->mat[id];
+  m->mnp = npoles;
+  for (int p = 0; p < npoles; p++) {
+    m->mkind[p] = (int)poles[4 * p];
+    m->mfreq[p] = poles[4 * p + 1];
+    m->mgamma[p] = poles[4 * p + 2];
+    m->mweight[p] = poles[4 * p + 3];
+  }
+  if (npoles > s->mpoles_alloc) {
+    for (int p = s->mpoles_alloc; p < npoles; p++)
+      for (int c = 0; c < 3; c++)
+        if (s->actH[c]) {
+          s->M[p][c] = calloc(s->ncell, sizeof(double));
+          s->K[p][c] = calloc(s->ncell, sizeof(double));
+        }
+    s->mpoles_alloc = npoles;
+  }
+  if (s->im) rc_set_material_magnetic(s->im, id, npoles, poles);
   return 0;
 }
 
@@ -688,6 +722,8 @@ static double* field_ptr(const rc_sim* s, int field, int comp, int pole) {
     case 3: return s->H[comp];
     case 4: return pole < s->npoles_alloc ? s->P[pole][comp] : NULL;
     case 5: return pole < s->npoles_alloc ? s->J[pole][comp] : NULL;
+    case 8: return pole < s->mpoles_alloc ? s->M[pole][comp] : NULL;
+    case 9: return pole < s->mpoles_alloc ? s->K[pole][comp] : NULL;
   }
   return NULL;
 }
@@ -801,6 +837,13 @@ static rc_sim* clone_part(const rc_sim* s) {
         t->J[p][c] = calloc(t->ncell, sizeof(double));
       }
   t->npoles_alloc = s->npoles_alloc;
+  for (int p = 0; p < s->mpoles_alloc; p++)
+    for (int c = 0; c < 3; c++)
+      if (t->actH[c]) {
+        t->M[p][c] = calloc(t->ncell, sizeof(double));
+        t->K[p][c] = calloc(t->ncell, sizeof(double));
+      }
+  t->mpoles_alloc = s->mpoles_alloc;
   memcpy(t->bc, s->bc, sizeof s->bc);
   memcpy(t->bloch, s->bloch, sizeof s->bloch);
   if (s->fluid_ne) rc_set_fluid_state(t, s->fluid_ne, s->fluid_nu);
@@ -1187,15 +1230,50 @@ static void step_h(rc_sim* s) {
   }
   if (s->tfsf) tfsf_faces(s, 1);
   for (int p = 0; p < s->nplane; p++) plane_faces(s, &s->plane[p], 1);
-  /* ---- H material: H = B/(mu_r mu0) */
+  /* ---- H material: H = B/(mu_r mu0); magnetic poles: the electric side's
+   * updates on B, H, M, K with mu0 (DispersiveMaterials.hpp:549-560, 710-735, 979-998) */
+  const int mnp_alloc = s->mpoles_alloc;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = lo[2]; k < hi[2]; k++)
     for (int j = lo[1]; j < hi[1]; j++)
       for (int i = lo[0]; i < hi[0]; i++) {
         size_t c = cidx(s, i, j, k);
         const rc_mat* m = &s->mat[s->matid[c]];
-        for (int I = 2; I >= 0; I--)
-          if (s->actH[I]) s->H[I][c] = s->B[I][c] / (m->mu * mu0);
+        for (int I = 2; I >= 0; I--) {
+          if (!s->actH[I]) continue;
+          if (m->mnp == 0 || mnp_alloc == 0) {
+            s->H[I][c] = s->B[I][c] / (m->mu * mu0);
+            continue;
+          }
+          for (int p = 0; p < m->mnp; p++) {
+            if (m->mkind[p] == 3) { /* ConductiveUpdate::implicit_update */
+              const double factor = dt * m->mfreq[p] / (m->mu * mu0);
+              s->M[p][I][c] = (s->M[p][I][c] + factor * s->B[I][c]) / (1.0 + factor);
+            } else {
+              s->M[p][I][c] += dt * s->K[p][I][c];
+            }
+          }
+          double num = s->B[I][c];
+          for (int p = 0; p < m->mnp; p++) num = num - s->M[p][I][c];
+          const double Hv = num / (m->mu * mu0);
+          s->H[I][c] = Hv;
+          for (int p = 0; p < m->mnp; p++) {
+            double* Kp = &s->K[p][I][c];
+            double* Mp = &s->M[p][I][c];
+            const double wgt_mu0 = m->mweight[p] * mu0;
+            if (m->mkind[p] == 3) continue; /* conductive: K untouched */
+            if (m->mkind[p] == 1) {
+              double b = m->mgamma[p] * dt * 0.5;
+              double w = dt * m->mfreq[p];
+              *Kp = *Kp * (1.0 - b) / (1.0 + b) + w * w / dt * wgt_mu0 * Hv;
+            } else {
+              double b = 1.0 + m->mgamma[p] * dt;
+              double bm = 1.0 - m->mgamma[p] * dt;
+              double w = m->mfreq[p] * dt;
+              *Kp = bm / b * *Kp + m->mfreq[p] * w / b * wgt_mu0 * Hv - m->mfreq[p] * w / b * *Mp;
+            }
+          }
+        }
       }
   soft_derived(s, 3, n);
   apply_boundaries(s, 1);
@@ -1650,6 +1728,7 @@ int rc_adjoint_eps_gradient(rc_sim* s, int nsteps, int dft, int comp_mask, doubl
       for (int i = lo[0]; i < hi[0]; i++) {
         const rc_mat* m = &s->mat[s->matid[cidx(s, i, j, k)]];
         if (s->npoles_alloc && m->np > 0 && (m->mag || m->fluid) && design_local(s, i, j, k) < 0) return -1;
+        if (s->mpoles_alloc && m->mnp > 0) return -1; /* magnetic poles: not transposed */
       }
   const long nd = (long)(s->dsg[1] - s->dsg[0]) * (s->dsg[3] - s->dsg[2]) * (s->dsg[5] - s->dsg[4]);
   double* hist = calloc((size_t)nsteps * 3 * nd, sizeof(double));

@@ -66,6 +66,8 @@ void rc_destroy(rc_sim* s);
 int rc_set_pml_axis(rc_sim* s, int axis, int ft, const double* coeffs);
 /* material table: eps_r, mu_r, npoles, then per pole {kind(1 Drude,2 Lorentz), freq, gamma, weight} */
 int rc_set_material(rc_sim* s, int id, double eps_r, double mu_r, int npoles, const double* poles);
+/* magnetic side of material id: poles (kind, freq, gamma, weight) on B, H, M, K; rc_get_field field 8 = M, 9 = K */
+int rc_set_material_magnetic(rc_sim* s, int id, int npoles, const double* poles);
 /* magnetized Drude material (bias NULL: self-magnetized by the cell's own B);
  * in 3-D the coupled update, otherwise the Drude pole {1, freq, gamma, 1} */
 int rc_set_magnetized_drude(rc_sim* s, int id, double eps_r, double mu_r, double freq, double gamma,

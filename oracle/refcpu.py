@@ -44,6 +44,7 @@ def lib():
             ("rc_create", _p, [_i, _i, _i, _i, _d, _d]), ("rc_destroy", None, [_p]),
             ("rc_set_pml_axis", _i, [_p, _i, _i, _dp]),
             ("rc_set_material", _i, [_p, _i, _d, _d, _i, _dp]),
+            ("rc_set_material_magnetic", _i, [_p, _i, _i, _dp]),
             ("rc_set_magnetized_drude", _i, [_p, _i, _d, _d, _d, _d, _dp]),
             ("rc_set_fluid_drude", _i, [_p, _i, _d, _i, _dp]),
             ("rc_set_bloch_boundary", _i, [_p, _i, _i, _d, _d]),
@@ -166,6 +167,12 @@ class Sim:
     def set_material(self, mid, eps_r, mu_r=1.0, poles=()):
         arr = _f64(np.array(poles, dtype=np.float64).reshape(-1)) if len(poles) else np.zeros(1)
         assert lib().rc_set_material(self.h, mid, eps_r, mu_r, len(poles), _dptr(arr)) == 0
+
+    def set_material_magnetic(self, mid, poles=()):
+        """Magnetic side of material mid: poles (kind 1 Drude, 2 Lorentz, 3 conductive, freq, gamma, weight)
+        on B, H, M, K (mu_r comes from set_material).  get(8, c, pole) is M, get(9, c, pole) K."""
+        arr = _f64(np.array(poles, dtype=np.float64).reshape(-1)) if len(poles) else np.zeros(1)
+        assert lib().rc_set_material_magnetic(self.h, mid, len(poles), _dptr(arr)) == 0
 
     def set_magnetized_drude(self, mid, eps_r, mu_r, freq, gamma, bias=None):
         """bias: the fixed field [T] (3 values), or None for the cell's own B."""

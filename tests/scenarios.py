@@ -65,6 +65,20 @@ def ref_material(kind, eps, freq, gamma, delta, bias=None):
     return (eps, 1.0, [(2, freq, gamma, 1.0)])
 
 
+def ref_magnetic(kind, mu, freq, gamma, delta):
+    """Map the magnetic side of a reference material (kind 0 vacuum, 1 constant,
+    2 Drude, 3 Lorentz, 4 conductive with freq = the magnetic conductivity) to
+    (mu_r, poles)."""
+    kind = int(kind)
+    if kind == 0:
+        return 1.0, []
+    if kind == 1:
+        return mu, []
+    if kind == 4:
+        return mu, [(3, freq, 0.0, 1.0)]
+    return mu, [(1 if kind == 2 else 2, freq, gamma, 1.0)]
+
+
 def set_ref_material(sim, mid, spec):
     """Apply a ref_material spec to a Grid or an oracle Sim."""
     if isinstance(spec, dict):
@@ -89,6 +103,7 @@ def from_golden(name):
         "pml_axes": [bool(p.get("pmlx", 1)), bool(p.get("pmly", 1)), mode == "ThreeD"],
         "materials": [ref_material(*row, bias=a["mats_bias"][m] if "mats_bias" in a else None)
                       for m, row in enumerate(a["mats_kind_eps_freq_gamma_delta"])],
+        "magnetic": [],
         "sources": [], "probes": [], "dfts": [], "tfsf": None,
     }
     shape = (nz if mode == "ThreeD" else 1, ny, nx)
@@ -103,6 +118,16 @@ def from_golden(name):
         sc["fluid"] = (a["fluid_ne"].reshape(shape), a["fluid_nu"].reshape(shape))
     if int(p.get("src_i", -1)) >= 0:
         sc["sources"].append((D, 2, int(p["src_i"]), int(p["src_j"]), int(p.get("src_k", 0)), a["src"]))
+    if "bsrc" in a:
+        c, i, j, k = (int(v) for v in a["bsrc_comp_i_j_k"])
+        sc["sources"].append((B, c, i, j, k, a["bsrc"]))
+    if "mmats_kind_mu_freq_gamma_delta" in a:  # magnetic sides: mu_r into the material, poles beside it
+        for mid, row in enumerate(a["mmats_kind_mu_freq_gamma_delta"]):
+            mu, poles = ref_magnetic(*row)
+            eps, _, epoles = sc["materials"][mid]
+            sc["materials"][mid] = (eps, mu, epoles)
+            if poles:
+                sc["magnetic"].append((mid, poles))
     if "src_field_comp_i_j" in a:  # D / B point sources (TE fixtures): field 0 D, 2 B
         for row, tab in zip(a["src_field_comp_i_j"], a["src_tables"]):
             f, c, i, j = (int(v) for v in row)
@@ -156,6 +181,8 @@ def apply(sim, sc):
             sim.set_pml_idiom(ax, pm["n"], pm["m"], pm["ma"], pm["smax"], pm["kmax"], pm["amax"])
     for mid, spec in enumerate(sc["materials"]):
         set_ref_material(sim, mid, spec)
+    for mid, poles in sc.get("magnetic", ()):
+        sim.set_material_magnetic(mid, poles)
     sim.set_matid(sc["matid"])
     if sc.get("fluid") is not None:
         sim.set_fluid_state(*sc["fluid"])

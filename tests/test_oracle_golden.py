@@ -148,7 +148,8 @@ def test_cell_fluid():
 
 @pytest.mark.parametrize("name", ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small",
                                   "grid3d_disp", "tm2d_c1", "tm2d_conductive", "grid3d_magnetized",
-                                  "grid3d_fluid", "tm2d_fluid", "tm2d_soft_planes", "te2d_small", "te2d_tfsf"])
+                                  "grid3d_fluid", "tm2d_fluid", "tm2d_soft_planes", "te2d_small", "te2d_tfsf",
+                                  "grid3d_magnetic"])
 def test_grid_scenarios_bit_exact(name):
     sc, p, a = from_golden(name)
     sim = refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])
@@ -181,6 +182,13 @@ def test_grid_scenarios_bit_exact(name):
             np.testing.assert_array_equal(sim.get(D, c).reshape(-1)[q], dq[:, 1 + c])
             np.testing.assert_array_equal(sim.get(4, c).reshape(-1)[q], dq[:, 4 + c])
             np.testing.assert_array_equal(sim.get(5, c).reshape(-1)[q], dq[:, 7 + c])
+    if "magnetic_q_B_M_K" in a:  # the magnetic side's B, M, K (oracle fields 2, 8, 9)
+        mq = a["magnetic_q_B_M_K"]
+        q = mq[:, 0].astype(int)
+        for c in range(3):
+            np.testing.assert_array_equal(sim.get(B, c).reshape(-1)[q], mq[:, 1 + c])
+            np.testing.assert_array_equal(sim.get(8, c).reshape(-1)[q], mq[:, 4 + c])
+            np.testing.assert_array_equal(sim.get(9, c).reshape(-1)[q], mq[:, 7 + c])
 
 
 def test_surface_quantities_match_reference_functors():
