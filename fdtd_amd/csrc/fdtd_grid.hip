@@ -164,6 +164,10 @@ struct MatHost {
   double bias[3] = {0, 0, 0};
   // fluid Drude: omega_p and gamma of each cell from fdtd_set_fluid_state
   int fluid = 0;
+  // magnetic side: poles on B, H, M, K (fdtd_set_material_magnetic)
+  int mnp = 0;
+  int mkind[FDTD_MAX_POLES] = {0, 0, 0};
+  double mfreq[FDTD_MAX_POLES] = {0, 0, 0}, mgamma[FDTD_MAX_POLES] = {0, 0, 0}, mweight[FDTD_MAX_POLES] = {0, 0, 0};
 };
 
 // One Aux1D incident line on the device (include/Source.hpp:280-458): its
@@ -222,6 +226,14 @@ struct fdtd_grid {
   void* Dd2[3] = {};
   void* Pd2[FDTD_MAX_POLES][3] = {};
   void* Jd2[FDTD_MAX_POLES][3] = {};
+  // magnetic-pole box: bounding box of the interior cells whose material has
+  // magnetic poles, and B, M, K over it (in place: such grids run the
+  // general two-pass instances)
+  int mbox[6] = {0, 0, 0, 0, 0, 0};
+  int max_mnp = 0;
+  void* Bd[3] = {};
+  void* Md[FDTD_MAX_POLES][3] = {};
+  void* Kd[FDTD_MAX_POLES][3] = {};
 
   // sources (nsrc_eh of them on E or H); their tables and allocated lengths
   int nsrc = 0, nsrc_eh = 0;
@@ -508,12 +520,41 @@ int upload_materials(fdtd_grid* g) {
       m[id].cD[p] = T(cD);
       if (p == 0) m[id].cG = T(1.0 / (1.0 + cD));
     }
+    // the magnetic side: the same recursions on B, H, M, K with mu_r mu0
+    m[id].mnp = h.mnp;
+    m[id].mu_m0 = T(h.mu * kMu0);
+    m[id].mcG = T(1.0);
+    for (int p = 0; p < fdtdk::kMaxPoles; p++) {
+      double cJ = 0, cE = 0, cP = 0, cD = 0, cJm = 1.0;
+      if (p < h.mnp && h.mkind[p] == FDTD_CONDUCTIVE) {
+        cD = dt * h.mfreq[p] / (h.mu * kMu0);
+        cJ = 1.0;
+        cJm = 0.0;
+      } else if (p < h.mnp && h.mkind[p] == FDTD_DRUDE) {
+        const double b = h.mgamma[p] * dt * 0.5, w = dt * h.mfreq[p];
+        cJ = (1.0 - b) / (1.0 + b);
+        cJm = 2.0 * b / (1.0 + b);
+        cE = w * w / dt * (h.mweight[p] * kMu0);
+      } else if (p < h.mnp) {
+        const double b = 1.0 + h.mgamma[p] * dt, bm = 1.0 - h.mgamma[p] * dt, w = h.mfreq[p] * dt;
+        cJ = bm / b;
+        cJm = 2.0 * h.mgamma[p] * dt / b;
+        cE = h.mfreq[p] * w / b * (h.mweight[p] * kMu0);
+        cP = -(h.mfreq[p] * w / b);
+      }
+      m[id].mcJ[p] = T(cJ);
+      m[id].mcJm[p] = T(cJm);
+      m[id].mcE[p] = T(cE);
+      m[id].mcP[p] = T(cP);
+      m[id].mcD[p] = T(cD);
+      if (p == 0) m[id].mcG = T(1.0 / (1.0 + cD));
+    }
   }
   HIP_TRY(hipMemcpy(g->mats_dev, m.data(), sizeof(fdtdk::MatDev<T>) * m.size(), hipMemcpyHostToDevice));
   std::vector<T> ie(FDTD_MAX_MATERIALS), im(FDTD_MAX_MATERIALS);
   for (int id = 0; id < FDTD_MAX_MATERIALS; id++) {
     ie[id] = g->mats[id].np > 0 ? -m[id].inv_eps : m[id].inv_eps;
-    im[id] = m[id].inv_mu;
+    im[id] = g->mats[id].mnp > 0 ? -m[id].inv_mu : m[id].inv_mu;  // negative: a magnetic-pole material
   }
   HIP_TRY(hipMemcpy(g->mie_dev, ie.data(), sizeof(T) * ie.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(g->mim_dev, im.data(), sizeof(T) * im.size(), hipMemcpyHostToDevice));
@@ -635,6 +676,58 @@ int finalize(fdtd_grid* g) {
       g->fluid_dirty = true;
     }
   }
+  // magnetic-pole box: bounding box of interior cells whose material has
+  // magnetic poles; B = mu_r mu0 H there from the current H
+  int mmx = 0;
+  for (int id = 0; id < FDTD_MAX_MATERIALS; id++) mmx = std::max(mmx, g->mats[id].mnp);
+  g->max_mnp = 0;
+  if (mmx > 0) {
+    int b[6] = {g->len[0], -1, g->len[1], -1, g->len[2], -1};
+    const int nz = g->ndim == 3 ? g->d.nz : 1;
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < g->d.ny; j++)
+        for (int i = 0; i < g->d.nx; i++)
+          if (g->mats[g->mat_host[g->q(i, j, k)]].mnp > 0) {
+            b[0] = std::min(b[0], i);
+            b[1] = std::max(b[1], i);
+            b[2] = std::min(b[2], j);
+            b[3] = std::max(b[3], j);
+            b[4] = std::min(b[4], k);
+            b[5] = std::max(b[5], k);
+          }
+    if (b[1] >= 0) {
+      if (g->cplx) return fail(FDTD_EINVAL, "magnetic poles are not stepped on complex (Bloch-periodic) grids");
+      for (int id = 0; id < FDTD_MAX_MATERIALS; id++)
+        if (g->mats[id].np > 0 && g->mats[id].mag == 2 && g->ndim == 3)
+          return fail(FDTD_EINVAL, "a self-magnetized Drude material (the cell's own B = mu_r mu0 H) cannot share a "
+                                   "grid with magnetic poles");
+      g->max_mnp = mmx;
+      for (int x = 0; x < 3; x++) {
+        g->mbox[2 * x] = b[2 * x];
+        g->mbox[2 * x + 1] = b[2 * x + 1] + 1;
+      }
+      const long nb = long(g->mbox[1] - g->mbox[0]) * (g->mbox[3] - g->mbox[2]) * (g->mbox[5] - g->mbox[4]);
+      for (int c = 0; c < 3; c++) {
+        if (!g->actH[c]) continue;
+        int rc = g->alloc(&g->Bd[c], size_t(nb) * g->esz);
+        if (rc) return rc;
+        std::vector<double> h(g->ncell), bb(nb);
+        if ((rc = download_T(g, h.data(), g->H[c], g->ncell))) return rc;
+        long t = 0;
+        for (int k = g->mbox[4]; k < g->mbox[5]; k++)
+          for (int j = g->mbox[2]; j < g->mbox[3]; j++)
+            for (int i = g->mbox[0]; i < g->mbox[1]; i++, t++) {
+              const long qq = g->q(i, j, k);
+              bb[t] = g->mats[g->mat_host[qq]].mu * kMu0 * h[qq];
+            }
+        if ((rc = upload_T(g, g->Bd[c], bb.data(), nb))) return rc;
+        for (int p = 0; p < mmx; p++) {
+          if ((rc = g->alloc(&g->Md[p][c], size_t(nb) * g->esz))) return rc;
+          if ((rc = g->alloc(&g->Kd[p][c], size_t(nb) * g->esz))) return rc;
+        }
+      }
+    }
+  }
   if (g->cplx) {
     int rc = ensure_pingpong(g);  // the imaginary part's state
     if (!rc) rc = ensure_dft_im(g);
@@ -660,8 +753,12 @@ void set_state(fdtd_grid* g, fdtdk::StepArgs<T>& a, bool pingpong) {
       gput(a.Jd[p][c], (T*)g->Jd[p][c]);
       gput(a.Pdo[p][c], (T*)(pingpong ? g->Pd2[p][c] : g->Pd[p][c]));
       gput(a.Jdo[p][c], (T*)(pingpong ? g->Jd2[p][c] : g->Jd[p][c]));
+      gput(a.Md[p][c], (T*)g->Md[p][c]);
+      gput(a.Kd[p][c], (T*)g->Kd[p][c]);
     }
+    gput(a.Bd[c], (T*)g->Bd[c]);
   }
+  for (int x = 0; x < 6; x++) a.mbox[x] = g->mbox[x];
   for (int ft = 0; ft < 2; ft++)
     for (int I = 0; I < 3; I++)
       for (int J = 0; J < 3; J++) {
@@ -1094,7 +1191,8 @@ int step_impl(fdtd_grid* g, int nsteps) {
   Tile t = g->tile.by ? g->tile : (g->ndim == 3 ? kDefaultTile3D : kDefaultTileTM);
   // design increments and boundary ghost writes are made by the two-pass kernels
   // (E / H sources and single planes too: the general instance of the passes)
-  const bool general = g->nbc || g->nsrc_eh || g->nplane;
+  // (and the H update of magnetic-pole cells)
+  const bool general = g->nbc || g->nsrc_eh || g->nplane || g->max_mnp > 0;
   // the time-blocked TM pass covers non-dispersive grids without those, complex
   // fields or DFT monitors
   const bool te = g->d.mode == FDTD_TE;
@@ -1537,7 +1635,7 @@ void snapshot_restore(fdtd_grid* g, GridSnapshot& s) {
 bool valid_comp(const fdtd_grid* g, int field, int comp) {
   if (comp < 0 || comp > 2) return false;
   if (field == FDTD_E || field == FDTD_D || field == FDTD_P || field == FDTD_J) return g->actE[comp];
-  if (field == FDTD_H || field == FDTD_B) return g->actH[comp];
+  if (field == FDTD_H || field == FDTD_B || field == FDTD_M || field == FDTD_K) return g->actH[comp];
   return false;
 }
 
@@ -2097,6 +2195,28 @@ int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npole
   return 0;
 }
 
+int fdtd_set_material_magnetic(fdtd_grid* g, int id, int npoles, const double* poles) {
+  if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (id < 0 || id >= FDTD_MAX_MATERIALS || npoles < 0 || npoles > FDTD_MAX_POLES || (npoles && !poles))
+    return fail(FDTD_EINVAL, "bad material id or pole count");
+  MatHost& m = g->mats[id];
+  if (g->finalized && (npoles > 0) != (m.mnp > 0))
+    return fail(FDTD_ESTATE, "the magnetic-pole layout is fixed once stepping has started");
+  if (g->finalized && npoles > g->max_mnp) return fail(FDTD_ESTATE, "pole count exceeds the allocated magnetic state");
+  for (int p = 0; p < npoles; p++)
+    if (int(poles[4 * p]) != FDTD_DRUDE && int(poles[4 * p]) != FDTD_LORENTZ && int(poles[4 * p]) != FDTD_CONDUCTIVE)
+      return fail(FDTD_EINVAL, "bad pole kind");
+  m.mnp = npoles;
+  for (int p = 0; p < npoles; p++) {
+    m.mkind[p] = int(poles[4 * p]);
+    m.mfreq[p] = poles[4 * p + 1];
+    m.mgamma[p] = poles[4 * p + 2];
+    m.mweight[p] = poles[4 * p + 3];
+  }
+  g->mats_dirty = true;
+  return 0;
+}
+
 int fdtd_set_magnetized_drude(fdtd_grid* g, int id, double eps_r, double mu_r, double freq, double gamma,
                               const double* bias) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
@@ -2434,10 +2554,29 @@ int fdtd_download_field(fdtd_grid* g, int field, int comp, int pole, double* hos
   HIP_TRY(hipStreamSynchronize(g->stream));
   if (field == FDTD_E) return download_T(g, host, g->E[comp], g->ncell);
   if (field == FDTD_H) return download_T(g, host, g->H[comp], g->ncell);
-  if (field == FDTD_B) {
-    int rc = download_T(g, host, g->H[comp], g->ncell);
+  if (field == FDTD_B || field == FDTD_M || field == FDTD_K) {
+    // mu_r mu0 H (B) or 0 (M, K), and the magnetic-pole box's stored values on its pole cells
+    if (field == FDTD_B) {
+      int rc = download_T(g, host, g->H[comp], g->ncell);
+      if (rc) return rc;
+      for (long i = 0; i < g->ncell; i++) host[i] *= g->mats[g->mat_host[i]].mu * kMu0;
+    } else {
+      std::fill(host, host + g->ncell, 0.0);
+      if (pole < 0 || pole >= FDTD_MAX_POLES) return fail(FDTD_EINVAL, "bad pole index");
+    }
+    void* src = field == FDTD_B ? g->Bd[comp] : (field == FDTD_M ? g->Md[pole][comp] : g->Kd[pole][comp]);
+    if (!src || g->max_mnp == 0) return 0;
+    const long nb = long(g->mbox[1] - g->mbox[0]) * (g->mbox[3] - g->mbox[2]) * (g->mbox[5] - g->mbox[4]);
+    std::vector<double> bx(nb);
+    int rc = download_T(g, bx.data(), src, nb);
     if (rc) return rc;
-    for (long i = 0; i < g->ncell; i++) host[i] *= g->mats[g->mat_host[i]].mu * kMu0;
+    long t = 0;
+    for (int k = g->mbox[4]; k < g->mbox[5]; k++)
+      for (int j = g->mbox[2]; j < g->mbox[3]; j++)
+        for (int i = g->mbox[0]; i < g->mbox[1]; i++, t++) {
+          const long qq = g->q(i, j, k);
+          if (g->mats[g->mat_host[qq]].mnp > 0) host[qq] = bx[t];
+        }
     return 0;
   }
   // D, P, J: dispersive box values where stored, eps_r eps0 E (D) or 0 elsewhere
@@ -2745,6 +2884,7 @@ int fdtd_adjoint_eps_gradient(fdtd_grid* g, int nsteps, int checkpoint_every, do
       return fail(FDTD_EINVAL, "the adjoint gradient covers Drude, Lorentz and conductive poles, not magnetized or "
                                "fluid Drude materials");
   }
+  if (g->max_mnp > 0) return fail(FDTD_EINVAL, "the adjoint gradient does not cover magnetic poles");
   if (g->nbc) return fail(FDTD_EINVAL, "the adjoint gradient does not cover boundary conditions");
   if (g->nsrc_eh || g->nplane)
     return fail(FDTD_EINVAL, "the adjoint gradient covers D / B sources and the TFSF box, not E / H sources or single planes");
@@ -3015,6 +3155,9 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
       return fail(FDTD_EINVAL, "batched grids cannot have E / H sources or single TFSF planes (step them alone)");
     if (has_magnetized(grids[q]))
       return fail(FDTD_EINVAL, "batched grids cannot hold magnetized Drude materials (step them alone)");
+    for (long c = 0; c < grids[q]->ncell; c++)
+      if (grids[q]->mats[grids[q]->mat_host[c]].mnp > 0)
+        return fail(FDTD_EINVAL, "batched grids cannot hold magnetic poles (step them alone)");
     for (int r = 0; r < q; r++)
       if (grids[r] == grids[q]) return fail(FDTD_EINVAL, "a grid appears twice in the batch");
   }

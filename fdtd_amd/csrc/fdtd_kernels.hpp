@@ -77,6 +77,12 @@ struct MatDev {
   T w2dt;  // (dt omega_p)^2 / dt
   T op;    // 1 + dt gamma + (dt omega_p)^2
   T vq;    // dt q_e/m_e: v = vq B for the cell's own B
+  // magnetic side: poles on B, H, M, K, the electric side's updates with mu0
+  // (K = mcJ K + mcE H + mcP M; conductive: M = (M + mcD B)/(1 + mcD))
+  int mnp;
+  T mcJ[kMaxPoles], mcE[kMaxPoles], mcP[kMaxPoles], mcJm[kMaxPoles], mcD[kMaxPoles];
+  T mcG;    // 1/(1 + mcD[0])
+  T mu_m0;  // mu_r mu0
 };
 
 struct SrcDev {
@@ -155,6 +161,12 @@ struct StepArgs {
   FDTD_G T* Ddo[3];
   FDTD_G T* Pdo[kMaxPoles][3];
   FDTD_G T* Jdo[kMaxPoles][3];
+  // cells of materials with magnetic poles, over their bounding box (updated
+  // in place by the general H passes): B, M_p, K_p
+  int mbox[6];
+  FDTD_G T* Bd[3];
+  FDTD_G T* Md[kMaxPoles][3];
+  FDTD_G T* Kd[kMaxPoles][3];
   // this block's own device address (set by the host; read by cold branches)
   const FDTD_G StepArgs* self;
   // fluid Drude coefficients per dispersive-box cell: {cJ, cE} of the Drude
@@ -504,6 +516,63 @@ __device__ __forceinline__ long dbox_index(const StepArgs<T>& a, int i, int j, i
          (long)(a.dbox[1] - a.dbox[0]) * ((long)(j - a.dbox[2]) + (long)(a.dbox[3] - a.dbox[2]) * (long)(k - a.dbox[4]));
 }
 
+template <typename T>
+__device__ __forceinline__ long mbox_index(const StepArgs<T>& a, int i, int j, int k) {
+  return (long)(i - a.mbox[0]) +
+         (long)(a.mbox[1] - a.mbox[0]) * ((long)(j - a.mbox[2]) + (long)(a.mbox[3] - a.mbox[2]) * (long)(k - a.mbox[4]));
+}
+
+// Magnetic-side pole cell, one component (the reference's pole functors on
+// FieldType::Magnetic, which run the electric updates on B, H, M, K with mu0):
+// B += d; M_p += dt K_p (conductive: (M_p + mcD B)/(1 + mcD));
+// H = (B - sum_p M_p)/(mu_r mu0); K_p = mcJ K_p + mcE H + mcP M_p.  FP32 takes
+// the same forms as ade_component (H from its old value where B - sum M
+// cancels).  In place: the general H passes own their cells.
+template <typename T, int I>
+__device__ FDTD_COLD T mag_component(const StepArgs<T>& a0, long c, long bq, T d, T Hold) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = (const StepArgs<T>&)cold_args(a0);
+  const MatDev<T>& m = a.mats[a.mat[c]];
+  const T B = a.Bd[I][bq] + d;
+  if (sizeof(T) == 4 && m.mnp == 1 && m.mcD[0] != T(0)) {
+    const T Hv = fmadd(d, m.inv_mu, Hold) * m.mcG;
+    a.Bd[I][bq] = B;
+    a.Md[0][I][bq] = fmadd(-m.mu_m0, Hv, B);
+    return Hv;
+  }
+  T num = B;
+  T Mv[kMaxPoles], Kv[kMaxPoles];
+#pragma unroll
+  for (int p = 0; p < kMaxPoles; p++) {
+    if (p < m.mnp) {
+      Kv[p] = a.Kd[p][I][bq];
+      if (m.mcD[p] != T(0))
+        Mv[p] = fmadd(m.mcD[p], B, a.Md[p][I][bq]) / (T(1) + m.mcD[p]);
+      else
+        Mv[p] = fmadd(a.dt, Kv[p], a.Md[p][I][bq]);
+      num -= Mv[p];
+    }
+  }
+  T Hv = num * m.inv_mu;
+  if (sizeof(T) == 4 && m.mcD[0] == T(0) && m.mcD[1] == T(0) && m.mcD[2] == T(0)) {
+    T inc = d;
+#pragma unroll
+    for (int p = 0; p < kMaxPoles; p++)
+      if (p < m.mnp) inc = fmadd(-a.dt, Kv[p], inc);
+    Hv = fmadd(inc, m.inv_mu, Hold);
+  }
+  a.Bd[I][bq] = B;
+#pragma unroll
+  for (int p = 0; p < kMaxPoles; p++) {
+    if (p < m.mnp) {
+      a.Md[p][I][bq] = Mv[p];
+      const T kk = sizeof(T) == 4 ? fmadd(-m.mcJm[p], Kv[p], Kv[p]) : m.mcJ[p] * Kv[p];
+      a.Kd[p][I][bq] = fmadd(m.mcP[p], Mv[p], fmadd(m.mcE[p], Hv, kk));
+    }
+  }
+  return Hv;
+}
+
 // Fluid Drude, one component, with the cell's own coefficients cJ, cE
 // (FluidDrudeStorage: omega_p = K sqrt(ne), gamma = nu_m, through
 // DrudeUpdate::implicit_update, DispersiveMaterials.hpp:979-998, 1160-1181):
@@ -833,9 +902,16 @@ __device__ __forceinline__ void h_cell_3d(const StepArgs<T>& a, long step, Row i
     tfsf_cell<T, 3>(a, incE, 1, i, j, k, act, act, (PRE & 2) ? fn : nullptr, d);
   }
   if (GEN && a.nplane) plane_cells<T, 3>(a, step, 1, i, j, k, 7, 7, d);
-  out[0] = fmadd(d[0], im, hx);
-  out[1] = fmadd(d[1], im, hy);
-  out[2] = fmadd(d[2], im, hz);
+  if (GEN && im < T(0)) {  // magnetic poles (the general instance only)
+    const long bq = mbox_index(a, i, j, k);
+    out[0] = mag_component<T, 0>(a, c, bq, d[0], hx);
+    out[1] = mag_component<T, 1>(a, c, bq, d[1], hy);
+    out[2] = mag_component<T, 2>(a, c, bq, d[2], hz);
+  } else {
+    out[0] = fmadd(d[0], im, hx);
+    out[1] = fmadd(d[1], im, hy);
+    out[2] = fmadd(d[2], im, hz);
+  }
   if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, out);
 }
 
@@ -2777,6 +2853,11 @@ __global__ __launch_bounds__(256) void h_pass_tm(const StepArgs<T>* __restrict__
   }
   if (GEN && a.nplane) plane_cells<T, 2>(a, step, 1, i, j, 0, 3, 4, d);
   T h[3] = {fmadd(d[0], im, hx), fmadd(d[1], im, hy), T(0)};
+  if (GEN && im < T(0)) {  // magnetic poles (the general instance only)
+    const long bq = mbox_index(a, i, j, 0);
+    h[0] = mag_component<T, 0>(a, c, bq, d[0], hx);
+    h[1] = mag_component<T, 1>(a, c, bq, d[1], hy);
+  }
   if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, h);
   a.Ho[0][c] = h[0];
   a.Ho[1][c] = h[1];
@@ -2903,6 +2984,7 @@ __global__ __launch_bounds__(256) void h_pass_te(const StepArgs<T>* __restrict__
   }
   if (GEN && a.nplane) plane_cells<T, 2>(a, step, 1, i, j, 0, 4, 3, d);
   T h[3] = {T(0), T(0), fmadd(d[2], im, hz)};
+  if (GEN && im < T(0)) h[2] = mag_component<T, 2>(a, c, mbox_index(a, i, j, 0), d[2], hz);  // magnetic poles
   if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, h);
   a.Ho[2][c] = h[2];
   if (GEN && a.bc_any) bc_ghost_write(a, 1, 2, i, j, 0, c, h);

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libfdtd_hip.so")
 TM, THREED, TE = 2, 3, 4
 MODES = {"TM": TM, "TE": TE, "ThreeD": THREED}
 F32, F64 = 0, 1
-D, E, B, H, P, J = 0, 1, 2, 3, 4, 5
+D, E, B, H, P, J, M, K = 0, 1, 2, 3, 4, 5, 6, 7
 DRUDE, LORENTZ, CONDUCTIVE = 1, 2, 3
 IMAG = 16  # field-code flag: the imaginary part of a complex grid (FDTD_IMAG)
 # boundary kinds (fdtd_boundary in include/fdtd_hip.h; names of the reference's fdtd::Boundary)
@@ -26,7 +26,7 @@ BOUNDARY = {"NONE": 0, "PEC": 1, "PMC": 2, "Symmetric": 3, "Antisymmetric": 4, "
 # every symbol declared in include/fdtd_hip.h
 ABI_SYMBOLS = [
     "fdtd_grid_create", "fdtd_grid_destroy", "fdtd_set_stream", "fdtd_set_pml_axis", "fdtd_set_pml_idiom",
-    "fdtd_set_material", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_tfsf_plane", "fdtd_set_signal", "fdtd_add_dft", "fdtd_get_dft",
+    "fdtd_set_material", "fdtd_set_material_magnetic", "fdtd_set_magnetized_drude", "fdtd_set_fluid_drude", "fdtd_set_fluid_state", "fdtd_set_matid", "fdtd_add_source", "fdtd_set_tfsf", "fdtd_add_tfsf_plane", "fdtd_set_signal", "fdtd_add_dft", "fdtd_get_dft",
     "fdtd_add_probe", "fdtd_get_probes", "fdtd_upload_field", "fdtd_download_field", "fdtd_step",
     "fdtd_synchronize", "fdtd_step_count", "fdtd_set_profiling", "fdtd_kernel_time", "fdtd_last_error",
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune", "fdtd_set_adjoint_tile",
@@ -67,6 +67,7 @@ def load_library():
         "fdtd_set_pml_axis": (_i, [_p, _i, _i, _dp, _dp, _dp]),
         "fdtd_set_pml_idiom": (_i, [_p, _i, _i, _d, _d, _d, _d, _d]),
         "fdtd_set_material": (_i, [_p, _i, _d, _d, _i, _dp]),
+        "fdtd_set_material_magnetic": (_i, [_p, _i, _i, _dp]),
         "fdtd_set_magnetized_drude": (_i, [_p, _i, _d, _d, _d, _d, _dp]),
         "fdtd_set_fluid_drude": (_i, [_p, _i, _d, _i, _dp]),
         "fdtd_set_fluid_state": (_i, [_p, _dp, _dp]),
@@ -176,6 +177,12 @@ class Grid:
     def set_material(self, mid, eps_r, mu_r=1.0, poles=()):
         arr = _f64(np.array(poles, dtype=np.float64).reshape(-1)) if len(poles) else np.zeros(1)
         _check(load_library().fdtd_set_material(self.h, mid, eps_r, mu_r, len(poles), _dptr(arr)))
+
+    def set_material_magnetic(self, mid, poles=()):
+        """Magnetic side of material mid: poles (kind DRUDE / LORENTZ / CONDUCTIVE, freq, gamma,
+        weight) on B, H, M, K; mu_r is set_material's.  get(M, c, pole) / get(K, c, pole) read them."""
+        arr = _f64(np.array(poles, dtype=np.float64).reshape(-1)) if len(poles) else np.zeros(1)
+        _check(load_library().fdtd_set_material_magnetic(self.h, mid, len(poles), _dptr(arr)))
 
     def set_magnetized_drude(self, mid, eps_r, mu_r, freq, gamma, bias=None):
         """Magnetized Drude material: bias = the fixed field [T] (MagnetizedDrudeUpdate),

@@ -131,6 +131,7 @@ struct MaterialSpec {
   int mag = 0;                // magnetized Drude: 1 fixed bias, 2 the cell's own B
   double bias[3] = {0, 0, 0};
   bool fluid = false;         // fluid Drude: omega_p, gamma per cell (YeeGrid::setFluidState)
+  std::vector<double> mpoles; // magnetic side: kind, freq, gamma, weight per pole (on B, H, M, K)
 };
 
 namespace detail {
@@ -147,6 +148,16 @@ inline void set_material(fdtd_grid* h, int id, const MaterialSpec& s) {
   }
   check(fdtd_set_material(h, id, s.eps, s.mu, int(s.poles.size() / 4), s.poles.empty() ? nullptr : s.poles.data()),
         "fdtd_set_material");
+  if (!s.mpoles.empty())
+    check(fdtd_set_material_magnetic(h, id, int(s.mpoles.size() / 4), s.mpoles.data()), "fdtd_set_material_magnetic");
+}
+
+// the spec of a pole model: K is eps_r of an electric model, mu_r of a magnetic one
+inline MaterialSpec pole_spec(FieldType ft, double K, std::vector<double> pole) {
+  MaterialSpec s;
+  (ft == FieldType::Electric ? s.eps : s.mu) = K;
+  s.poles = std::move(pole);
+  return s;
 }
 
 inline std::string spec_text(const MaterialSpec& s) {
@@ -158,6 +169,10 @@ inline std::string spec_text(const MaterialSpec& s) {
   }
   for (double v : s.poles) {
     snprintf(b, sizeof b, "%.17g,", v);
+    t += b;
+  }
+  for (double v : s.mpoles) {
+    snprintf(b, sizeof b, "m%.17g,", v);
     t += b;
   }
   return t;
@@ -282,7 +297,8 @@ class Loop {
     pending_++;
   }
   // set up the device from the first step: its materials (the electric model
-  // and the magnetic mu_r per id); CPML profiles need their UpdatePML stages
+  // and the magnetic model's mu_r and poles per id); CPML profiles need their
+  // UpdatePML stages
   void configure() {
     for (int a = 0; a < 3; a++)
       if ((pml_axes >> a) & 1)
@@ -291,8 +307,9 @@ class Loop {
             throw Error("the grid has a CPML profile along an axis whose UpdatePML is not in the time loop");
     std::map<int, MaterialSpec> all = emat_;
     for (auto& kv : hmat_) {
-      if (!kv.second.poles.empty()) throw Error("magnetic dispersion has no device counterpart");
+      if (kv.second.mag || kv.second.fluid) throw Error("a magnetized or fluid model on the magnetic side has no device counterpart");
       all[kv.first].mu = kv.second.mu;
+      all[kv.first].mpoles = kv.second.poles;
     }
     for (auto& kv : all) set_material(h, kv.first, kv.second);
   }
@@ -613,7 +630,7 @@ template <class Mode, FieldType ft, bool forward = false>
 struct DrudeUpdate : detail::Sweep<DrudeUpdate<Mode, ft, forward>> {
   double K, freq, gamma, dt;
   DrudeUpdate(double k, double f, double g, double deltat) : K(k), freq(f), gamma(g), dt(deltat) {}
-  MaterialSpec spec() const { return {K, 1.0, {double(FDTD_DRUDE), freq, gamma, 1.0}}; }
+  MaterialSpec spec() const { return detail::pole_spec(ft, K, {double(FDTD_DRUDE), freq, gamma, 1.0}); }
 };
 // Lorentz pole with LorentzStorage(K, Delta, Freq, Gamma, dt); like the
 // reference's update, Delta does not enter (include/DispersiveMaterials.hpp:711-735)
@@ -621,7 +638,7 @@ template <class Mode, FieldType ft, bool forward = false>
 struct LorentzUpdate : detail::Sweep<LorentzUpdate<Mode, ft, forward>> {
   double K, delta, freq, gamma, dt;
   LorentzUpdate(double k, double de, double f, double g, double deltat) : K(k), delta(de), freq(f), gamma(g), dt(deltat) {}
-  MaterialSpec spec() const { return {K, 1.0, {double(FDTD_LORENTZ), freq, gamma, 1.0}}; }
+  MaterialSpec spec() const { return detail::pole_spec(ft, K, {double(FDTD_LORENTZ), freq, gamma, 1.0}); }
 };
 // Conductive medium with ConductiveStorage(K, Freq, dt), Freq being the
 // conductivity (include/DispersiveMaterials.hpp:508-680): P relaxes
@@ -630,7 +647,7 @@ template <class Mode, FieldType ft, bool forward = false>
 struct ConductiveUpdate : detail::Sweep<ConductiveUpdate<Mode, ft, forward>> {
   double K, freq, dt;
   ConductiveUpdate(double k, double f, double deltat) : K(k), freq(f), dt(deltat) {}
-  MaterialSpec spec() const { return {K, 1.0, {double(FDTD_CONDUCTIVE), freq, 0.0, 1.0}}; }
+  MaterialSpec spec() const { return detail::pole_spec(ft, K, {double(FDTD_CONDUCTIVE), freq, 0.0, 1.0}); }
 };
 // Magnetized Drude plasma with MagnetizedDrudeStorage(Bx, By, Bz, K, Freq,
 // Gamma, dt): in 3-D the three components of P and J advance together in the
@@ -714,8 +731,8 @@ void set_material(YeeGrid<Mode, scalar>& g, int id, const MaterialSpec& s) {
 
 // MaterialPair / MaterialMap (include/MaterialMap.hpp:45-284): a named pair
 // of electric and magnetic models.  The device takes the electric model's
-// eps_r and poles and the magnetic model's mu_r (magnetic dispersion has no
-// device counterpart).  The reference's XML form is read and written by
+// eps_r and poles and the magnetic model's mu_r and poles (Drude, Lorentz,
+// conductive on B, H, M, K).  The reference's XML form is read and written by
 // fdtd_amd/materials.py.
 struct MaterialPair {
   MaterialSpec electric, magnetic;
@@ -726,9 +743,11 @@ MaterialPair make_material_pair(const E& e, const M& m) {
 }
 using MaterialMap = std::map<std::string, MaterialPair>;
 inline MaterialSpec combine(const std::string& name, const MaterialPair& p) {
-  if (!p.magnetic.poles.empty()) throw Error("material " + name + ": magnetic dispersion has no device counterpart");
+  if (p.magnetic.mag || p.magnetic.fluid)
+    throw Error("material " + name + ": a magnetized or fluid model on the magnetic side has no device counterpart");
   MaterialSpec s = p.electric;
   s.mu = p.magnetic.mu;
+  s.mpoles = p.magnetic.poles;
   return s;
 }
 

@@ -17,8 +17,10 @@ with ids = {identifier: material id}: electric Vacuum / Constant / Conductive
 SelfMagnetizedDrude (1898-2006) a magnetized Drude material, FluidDrude
 (1160-1199), MagnetizedFluidDrude (1728-1814) and SelfMagnetizedFluidDrude
 (1816-1893) a fluid Drude material (its density and collision rate per cell
-come from the grid's fluid state), magnetic Vacuum / Constant its mu_r.
-Magnetic dispersion has no device counterpart and raises.
+come from the grid's fluid state), magnetic Vacuum / Constant its mu_r,
+magnetic Drude / Lorentz / Conductive its mu_r and magnetic pole (B, H, M, K:
+``set_material_magnetic``).  Magnetized and fluid models on the magnetic side
+raise.
 """
 import xml.etree.ElementTree as ET
 from dataclasses import dataclass
@@ -92,13 +94,26 @@ class MaterialPair:
             raise NotImplementedError(f"electric model {e.kind} has no device counterpart")
         if m.kind == "Vacuum":
             mu = 1.0
-        elif m.kind == "Constant":
-            mu = m.K
+        elif m.kind in ("Constant", "Drude", "Lorentz", "Conductive"):
+            mu = m.K  # poles of the last three: magnetic_poles()
         else:
-            raise NotImplementedError(f"magnetic model {m.kind}: magnetic dispersion has no device counterpart")
+            raise NotImplementedError(f"magnetic model {m.kind} has no device counterpart")
         if fluid is not None:
             return {"fluid": fluid, "mu": mu, "bias": tuple(float(b) for b in e.B) if fluid == 1 else None}
         return (eps, mu, poles) + bias
+
+    def magnetic_poles(self):
+        """Poles of the magnetic model (the reference's Drude / Lorentz / Conductive
+        updates on FieldType::Magnetic, stepping B, H, M, K) for set_material_magnetic;
+        [] for Vacuum and Constant."""
+        m = self.magnetic
+        if m.kind == "Conductive":
+            return [(CONDUCTIVE, m.freq, 0.0, 1.0)]
+        if m.kind == "Drude":
+            return [(DRUDE, m.freq, m.gamma, 1.0)]
+        if m.kind == "Lorentz":
+            return [(LORENTZ, m.freq, m.gamma, 1.0)]
+        return []
 
 
 def _fmt(v):
@@ -177,4 +192,7 @@ class MaterialMap(dict):
                 sim.set_magnetized_drude(mid, eps, mu, poles[0][1], poles[0][2], bias)
             else:
                 sim.set_material(mid, *s)
+            mp = self[name].magnetic_poles()
+            if mp:
+                sim.set_material_magnetic(mid, mp)
         return sim

@@ -40,8 +40,8 @@ enum fdtd_status {
  * ThreeD. */
 enum fdtd_mode { FDTD_TM = 2, FDTD_3D = 3, FDTD_TE = 4 };
 enum fdtd_precision { FDTD_F32 = 0, FDTD_F64 = 1 };
-/* field codes */
-enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5 };
+/* field codes (M, K: magnetization and magnetic current of magnetic poles) */
+enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5, FDTD_M = 6, FDTD_K = 7 };
 /* OR'ed into the field of fdtd_upload_field / fdtd_download_field: the
  * imaginary part of E or H of a complex grid (one with a Bloch-periodic face) */
 #define FDTD_IMAG 16
@@ -102,6 +102,16 @@ int fdtd_set_pml_idiom(fdtd_grid* g, int axis, int n, double m, double ma, doubl
  * ConductiveUpdate (508-680): P = (P + f D)/(1 + f), f = dt sigma/(eps_r eps0),
  * E = (D - P)/(eps_r eps0). */
 int fdtd_set_material(fdtd_grid* g, int id, double eps_r, double mu_r, int npoles, const double* poles);
+/* The magnetic side of material `id`: replaces the reference's DrudeUpdate /
+ * LorentzUpdate / ConductiveUpdate on FieldType::Magnetic (the same
+ * recursions on B, H, M, K with mu0 and mu_r for eps0 and eps_r: the *Call
+ * structs of include/DispersiveMaterials.hpp, val = mu0; M and K are the
+ * DoubleMagnetization members, 86-110).  poles as for fdtd_set_material
+ * (FDTD_CONDUCTIVE: freq = the magnetic conductivity); mu_r is
+ * fdtd_set_material's.  npoles = 0 clears them.  Grids holding such cells run
+ * the general two-pass instances; batches, the adjoint gradient, complex
+ * (Bloch) grids and self-magnetized Drude materials refuse them. */
+int fdtd_set_material_magnetic(fdtd_grid* g, int id, int npoles, const double* poles);
 /* Material `id` as a magnetized Drude plasma: replaces MagnetizedDrudeUpdate
  * (bias != NULL: the fixed field bias[3] in tesla, MagnetizedDrudeStorage,
  * include/DispersiveMaterials.hpp:1619-1726) and SelfMagnetizedDrudeUpdate
@@ -199,7 +209,8 @@ int fdtd_get_probes(fdtd_grid* g, double* out, int max_steps, int* nsteps_out);
 int fdtd_get_probes_imag(fdtd_grid* g, double* out, int max_steps, int* nsteps_out);
 
 /* Host<->device copies of one component (nx*ny*nz float64 values).  D and B
- * read back as eps_r eps0 E and mu_r mu0 H outside the dispersive box. */
+ * read back as eps_r eps0 E and mu_r mu0 H outside the cells of pole
+ * materials; P, J (electric poles) and M, K (magnetic poles) take `pole`. */
 int fdtd_upload_field(fdtd_grid* g, int field, int comp, int pole, const double* host);
 int fdtd_download_field(fdtd_grid* g, int field, int comp, int pole, double* host);
 

@@ -1132,7 +1132,7 @@ static void scen_magnetic(const std::string& dir) {
     sD.push_back(eps0 * gauss_pulse(n, 20, 6));
     sB.push_back(1e-9 * gauss_pulse(n, 30, 7));
   }
-  const std::vector<std::array<int, 3>> probes = {{10, 9, 8}, {14, 10, 8}, {9, 8, 8}, {14, 9, 8}, {10, 12, 11}, {14, 14, 7}, {6, 13, 12}, {3, 3, 3}};
+  const std::vector<std::array<int, 3>> probes = {{10, 9, 8}, {14, 10, 8}, {9, 8, 8}, {14, 14, 7}, {6, 13, 12}};
   std::vector<double> ptrace;
   auto interior = [&](auto&& fn) {
     for (int k = 1; k < nz - 1; k++)

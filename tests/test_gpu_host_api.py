@@ -168,3 +168,20 @@ def test_te_time_loop_through_cpp_host_surface(name):
     ref = a["probe_trace_Ex_Ey_Hz"].reshape(got.shape)
     assert np.abs(ref).max() > 0
     assert rel_l2(got, ref) <= 1e-12
+
+
+def test_magnetic_poles_through_cpp_host_surface():
+    """Material functors on FieldType::Magnetic (Drude, Lorentz, conductive) in a
+    reference-style for_each loop: the probe traces equal the reference's own run
+    (golden fixture grid3d_magnetic) to 1e-12."""
+    exe = os.path.join(os.path.dirname(BIN), "grid3d_magnetic_host_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/bin/grid3d_magnetic_host_api not built (run __graft_entry__.build())")
+    p, a = load("grid3d_magnetic")
+    print("[gpu-case] grid3d_magnetic_host_api", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
+    ref = a["probe_trace_E_H"].reshape(got.shape)
+    assert np.abs(ref).max() > 0
+    assert rel_l2(got, ref) <= 1e-12

@@ -52,7 +52,8 @@ def field_errors(g, o, sc):
 
 
 GOLDEN = ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small", "grid3d_disp", "tm2d_c1",
-          "tm2d_conductive", "grid3d_magnetized", "grid3d_fluid", "tm2d_fluid", "tm2d_soft_planes"]
+          "tm2d_conductive", "grid3d_magnetized", "grid3d_fluid", "tm2d_fluid", "tm2d_soft_planes", "grid3d_magnetic"]
+M_FIELD, K_FIELD = 6, 7  # magnetization and magnetic current of magnetic poles (FDTD_M, FDTD_K)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -79,6 +80,13 @@ def test_golden_scenarios(name, precision):
             assert rel_l2(g.get(D, c).reshape(-1)[q], dq[:, 1 + c]) <= TOL[precision]
             assert rel_l2(g.get(P, c).reshape(-1)[q], dq[:, 4 + c]) <= TOL[precision]
             assert rel_l2(g.get(J, c).reshape(-1)[q], dq[:, 7 + c]) <= TOL[precision]
+    if "magnetic_q_B_M_K" in a:  # magnetic poles: B, M, K of their cells against the reference's
+        mq = a["magnetic_q_B_M_K"]
+        q = mq[:, 0].astype(int)
+        for c in range(3):
+            assert rel_l2(g.get(B, c).reshape(-1)[q], mq[:, 1 + c]) <= TOL[precision], ("B", c)
+            assert rel_l2(g.get(M_FIELD, c).reshape(-1)[q], mq[:, 4 + c]) <= TOL[precision], ("M", c)
+            assert rel_l2(g.get(K_FIELD, c).reshape(-1)[q], mq[:, 7 + c]) <= TOL[precision], ("K", c)
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -558,3 +566,53 @@ def test_time_blocked_pass(precision, design):
         np.testing.assert_array_equal(g.probes(), ref.probes())
         np.testing.assert_array_equal(g.get_dft(0), ref.get_dft(0))
         g.close()
+
+
+def _with_magnetic_poles(name):
+    """A 2-D reference scenario whose blocks also get magnetic sides: a Drude,
+    a Lorentz and a conductive pole and a constant mu_r (beside their
+    electric sides)."""
+    sc, p, a = from_golden(name)
+    dt = sc["dt"]
+    mats = list(sc["materials"])
+    for mid, mu in ((1, 1.3), (2, 1.0), (3, 1.6), (4, 2.0)):
+        eps, _, poles = mats[mid]
+        mats[mid] = (eps, mu, poles)
+    sc["materials"] = mats
+    sc["magnetic"] = [(1, [(1, 0.3 / dt, 0.02 / dt, 1.0)]), (2, [(2, 0.2 / dt, 0.01 / dt, 1.0)]),
+                      (3, [(3, 0.05 * 1.6 * 4e-7 * np.pi / dt, 0.0, 1.0), (2, 0.15 / dt, 0.02 / dt, 1.0)])]
+    return sc
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("name", ["tm2d_small", "te2d_small", "te2d_tfsf"])
+def test_magnetic_poles_2d(name, precision):
+    """Magnetic poles in TM and TE grids (H passes of both modes) against the oracle."""
+    sc = _with_magnetic_poles(name)
+    g, o = run_pair(sc, precision)
+    comps = _comps(sc) if sc["mode"] != "TE" else [("Ex", E, 0), ("Ey", E, 1), ("Hz", H, 2)]
+    for nm, f, c in comps:
+        e = rel_l2(g.get(f, c), o.get(f, c))
+        print(name, precision, nm, f"{e:.2e}", flush=True)
+        tol = 1e-4 if (name == "te2d_small" and precision == "f32" and f == H) else TOL[precision]
+        assert e <= tol, (nm, e)
+    hc = [c for _, f, c in comps if f == H]
+    for c in hc:  # the magnetic state of the pole cells
+        mask = np.isin(sc["matid"], [1, 2, 3]).reshape(-1)
+        assert rel_l2(g.get(M_FIELD, c).reshape(-1)[mask], o.get(8, c).reshape(-1)[mask]) <= 10 * TOL[precision]
+        assert rel_l2(g.get(K_FIELD, c).reshape(-1)[mask], o.get(9, c).reshape(-1)[mask]) <= 10 * TOL[precision]
+        assert np.abs(o.get(8, c)).max() > 0
+
+
+def test_magnetic_poles_refusals():
+    from fdtd_amd import Batch, FdtdError
+    sc, p, a = from_golden("grid3d_magnetic")
+    g = apply(_grid(sc, "f32"), sc)
+    with pytest.raises(FdtdError):
+        Batch([g])
+    g.step(2)
+    bl = apply(_grid(sc, "f32"), sc)
+    bl.set_bloch_boundary(0, 0, 1j)
+    bl.set_bloch_boundary(0, 1, -1j)
+    with pytest.raises(FdtdError):
+        bl.step(1)

@@ -44,13 +44,49 @@ def test_writes_the_reference_text():
 
 def test_unsupported_models_raise():
     mm = MaterialMap(x=MaterialPair(Dispersion("Anisotropic"), Dispersion("Vacuum")),
-                     y=MaterialPair(Dispersion("Constant", K=2.0), Dispersion("Drude", 1.0, 1e15, 1e13)))
+                     z=MaterialPair(Dispersion("Constant", K=2.0), Dispersion("MagnetizedDrude", 1.0, 1e15, 1e13)))
     with pytest.raises(NotImplementedError):
         mm["x"].spec()
     with pytest.raises(NotImplementedError, match="magnetic"):
-        mm["y"].spec()
+        mm["z"].spec()
     with pytest.raises(ValueError):
         Dispersion("Plasma")
+
+
+def test_magnetic_poles_from_the_map():
+    """A magnetic Drude / Lorentz / Conductive model gives mu_r = K and its pole for
+    set_material_magnetic; on the oracle the applied map steps like the poles set by hand."""
+    mm = MaterialMap(y=MaterialPair(Dispersion("Constant", K=2.0), Dispersion("Drude", 1.3, 2e14, 1e13)),
+                     w=MaterialPair(Dispersion("Vacuum"), Dispersion("Lorentz", 1.5, 3e14, 2e13, 0.4)),
+                     c=MaterialPair(Dispersion("Vacuum"), Dispersion("Conductive", 1.2, 50.0)))
+    assert mm["y"].spec() == (2.0, 1.3, [])
+    assert mm["y"].magnetic_poles() == [(1, 2e14, 1e13, 1.0)]
+    assert mm["w"].magnetic_poles() == [(2, 3e14, 2e13, 1.0)]
+    assert mm["c"].magnetic_poles() == [(3, 50.0, 0.0, 1.0)]
+    nx = ny = 20
+    dx = 1e-6
+    dt = 0.5 * dx / (refcpu.constants()["c0"] * np.sqrt(2.0))
+    matid = np.zeros((1, ny, nx), np.uint8)
+    matid[0, 4:9, 4:9], matid[0, 11:16, 4:9], matid[0, 4:9, 11:16] = 1, 2, 3
+    runs = []
+    for hand in (False, True):
+        s = refcpu.Sim("TM", nx, ny, 1, dx, dt)
+        if hand:
+            s.set_material(1, 2.0, 1.3)
+            s.set_material_magnetic(1, [(1, 2e14, 1e13, 1.0)])
+            s.set_material(2, 1.0, 1.5)
+            s.set_material_magnetic(2, [(2, 3e14, 2e13, 1.0)])
+            s.set_material(3, 1.0, 1.2)
+            s.set_material_magnetic(3, [(3, 50.0, 0.0, 1.0)])
+        else:
+            mm.apply(s, {"y": 1, "w": 2, "c": 3})
+        s.set_matid(matid)
+        s.add_source(0, 2, 10, 10, 0, 8.854e-12 * np.exp(-((np.arange(60) - 15) / 5.0) ** 2))
+        s.step(60)
+        runs.append([s.get(3, c) for c in (0, 1)] + [s.get(8, 0)])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+    assert np.abs(runs[0][2]).max() > 0
 
 
 def test_applied_map_steps_like_hand_set_materials():
