@@ -599,8 +599,8 @@ def test_magnetic_poles_2d(name, precision):
     hc = [c for _, f, c in comps if f == H]
     for c in hc:  # the magnetic state of the pole cells
         mask = np.isin(sc["matid"], [1, 2, 3]).reshape(-1)
-        assert rel_l2(g.get(M_FIELD, c).reshape(-1)[mask], o.get(8, c).reshape(-1)[mask]) <= 10 * TOL[precision]
-        assert rel_l2(g.get(K_FIELD, c).reshape(-1)[mask], o.get(9, c).reshape(-1)[mask]) <= 10 * TOL[precision]
+        assert rel_l2(g.get(M_FIELD, c).reshape(-1)[mask], o.get(8, c).reshape(-1)[mask]) <= TOL[precision]
+        assert rel_l2(g.get(K_FIELD, c).reshape(-1)[mask], o.get(9, c).reshape(-1)[mask]) <= TOL[precision]
         assert np.abs(o.get(8, c)).max() > 0
 
 
