@@ -2979,6 +2979,9 @@ struct fdtd_batch {
   bool ev_used[kRing] = {};
   int slot = 0;
   bool use_graph = true;
+  // persistent passes: pwg workgroups per compute unit walk the tile table
+  // (0: one workgroup per tile); ncu compute units
+  int pwg = 0, ncu = 256;
   // (slot, steps, DFT block phase or -1) -> executable graph
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 
@@ -3083,16 +3086,26 @@ int batch_step_impl(fdtd_batch* b, int nsteps) {
     auto* dev = reinterpret_cast<const fdtdk::StepArgs<T>*>(b->args_dev + slot * n * sizeof(fdtdk::StepArgs<T>));
     HIP_TRY(hipMemcpyAsync((void*)dev, host, n * sizeof(fdtdk::StepArgs<T>), hipMemcpyHostToDevice, b->stream));
     const dim3 block(b->tile.bx, b->tile.by, 1), grid(b->ntiles, 1, 1);
+    // persistent passes: a multiple of 8 workgroups (one set per XCD), at most one per tile
+    const dim3 pgrid(unsigned(std::max(8, std::min(b->ntiles / 8 * 8, b->pwg * b->ncu / 8 * 8))), 1, 1);
     // the E-pass instance without the ADE code when no grid is dispersive
     bool disp = false;
     for (fdtd_grid* g : b->grids) disp = disp || g->max_np > 0;
     // one step's launches: the two passes over every grid's tiles, then the DFT
     auto enqueue_step = [&](int s) {
-      if (disp)
+      if (b->pwg && disp)
+        hipLaunchKernelGGL(fdtdk::e_pass_batch_p<T>, pgrid, block, 0, b->stream, dev, b->tiles, b->ntiles, b->tile.kc, s);
+      else if (b->pwg)
+        hipLaunchKernelGGL((fdtdk::e_pass_batch_p<T, 0>), pgrid, block, 0, b->stream, dev, b->tiles, b->ntiles,
+                           b->tile.kc, s);
+      else if (disp)
         hipLaunchKernelGGL(fdtdk::e_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
       else
         hipLaunchKernelGGL((fdtdk::e_pass_batch<T, 0>), grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
-      hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
+      if (b->pwg)
+        hipLaunchKernelGGL(fdtdk::h_pass_batch_p<T>, pgrid, block, 0, b->stream, dev, b->tiles, b->ntiles, b->tile.kc, s);
+      else
+        hipLaunchKernelGGL(fdtdk::h_pass_batch<T>, grid, block, 0, b->stream, dev, b->tiles, b->tile.kc, s);
       if (phase >= 0) {
         hipLaunchKernelGGL(fdtdk::dft_sample_batch<T>, dim3(unsigned(b->smp_total)), dim3(256), 0, b->stream, dev,
                            b->smp_offs, int(n), s);
@@ -3163,6 +3176,7 @@ int fdtd_batch_create(fdtd_grid* const* grids, int n, fdtd_batch** out) {
   }
   fdtd_batch* b = new fdtd_batch();
   b->f64 = grids[0]->f64;
+  b->ncu = grids[0]->ncu;
   int rc = 0;
   if (hipStreamCreate(&b->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipStreamCreate failed");
   for (int q = 0; q < n && !rc; q++) {
@@ -3251,25 +3265,41 @@ int fdtd_batch_autotune(fdtd_batch* b, int trial_steps, int* bx, int* by, int* k
     rc = fail(FDTD_EDEVICE, "hipEventCreate failed");
   double best = 1e300;
   Tile bt = saved;
-  for (const Tile& t : cand) {
-    if (rc) break;
-    if ((rc = batch_tiles(b, t))) break;
-    if ((rc = fdtd_batch_step(b, 1))) break;
-    if (hipEventRecord(e0, b->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventRecord");
-    if (!rc) rc = fdtd_batch_step(b, trial_steps);
-    if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventRecord");
-    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventSynchronize");
+  const int saved_pwg = b->pwg;
+  int bp = 0;
+  // one trial: a step to settle, then trial_steps timed
+  auto trial = [&](const Tile& t, int pwg) -> int {
+    b->pwg = pwg;
+    int r = batch_tiles(b, t);
+    if (!r) r = fdtd_batch_step(b, 1);
+    if (!r && hipEventRecord(e0, b->stream) != hipSuccess) r = fail(FDTD_EDEVICE, "hipEventRecord");
+    if (!r) r = fdtd_batch_step(b, trial_steps);
+    if (!r && hipEventRecord(e1, b->stream) != hipSuccess) r = fail(FDTD_EDEVICE, "hipEventRecord");
+    if (!r && hipEventSynchronize(e1) != hipSuccess) r = fail(FDTD_EDEVICE, "hipEventSynchronize");
     float ms = 0;
-    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(FDTD_EDEVICE, "hipEventElapsedTime");
-    if (!rc && ms < best) {
+    if (!r && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) r = fail(FDTD_EDEVICE, "hipEventElapsedTime");
+    if (!r && ms < best) {
       best = ms;
       bt = t;
+      bp = pwg;
     }
+    return r;
+  };
+  for (const Tile& t : cand) {
+    if (rc) break;
+    rc = trial(t, 0);
+  }
+  // the best tile's persistent forms
+  const Tile tile_best = bt;
+  for (int pwg : {2, 4, 6, 8}) {
+    if (rc) break;
+    rc = trial(tile_best, pwg);
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   for (size_t q = 0; q < b->grids.size(); q++) snapshot_restore(b->grids[q], snaps[q]);
   b->use_graph = graph;
+  b->pwg = rc ? saved_pwg : bp;
   int rc2 = batch_tiles(b, rc ? saved : bt);
   if (rc) return rc;
   if (rc2) return rc2;
@@ -3277,6 +3307,13 @@ int fdtd_batch_autotune(fdtd_batch* b, int trial_steps, int* bx, int* by, int* k
   if (by) *by = bt.by;
   if (kc) *kc = bt.kc;
   return 0;
+}
+
+int fdtd_batch_set_persistent(fdtd_batch* b, int wg_per_cu) {
+  if (!b) return fail(FDTD_EINVAL, "null batch");
+  if (wg_per_cu < 0 || wg_per_cu > 32) return fail(FDTD_EINVAL, "workgroups per compute unit must be 0..32");
+  b->pwg = wg_per_cu;
+  return batch_tiles(b, b->tile);  // drops the graphs captured with the other passes
 }
 
 int fdtd_batch_set_graph(fdtd_batch* b, int on) {

@@ -1587,6 +1587,52 @@ __global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h
   h_pass_batch_body(args, tiles, kc, s);
 }
 
+// Persistent form of the batched passes: gridDim.x workgroups (a multiple of
+// 8) walk the tile table, workgroup b taking the tiles the one-per-tile launch
+// gives blocks b, b + gridDim.x, b + 2 gridDim.x, ... (so each XCD keeps its
+// tiles and its neighbours' timing); the E pass stages a grid's material
+// table in LDS only when its next tile belongs to another grid.  Same cell
+// code and order per cell: bitwise the one-per-tile result.
+template <typename T, int DISP = 1>
+__global__ __launch_bounds__(256) void e_pass_batch_p(const StepArgs<T>* __restrict__ args,
+                                                      const int4* __restrict__ tiles, int ntiles, int kc, int s) {
+  __shared__ MatLds<T> sm;
+  int cur = -1;
+  for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    const int4 t = tiles[xcd_tile(v, ntiles)];
+    const StepArgs<T>& a = args[t.x];
+    if (t.x != cur) {  // uniform over the workgroup
+      if (cur >= 0) __syncthreads();  // every wave is done with the previous table
+      stage_materials(a, sm);
+      cur = t.x;
+    }
+    const int k0 = 1 + t.w * kc;
+    e_pass_cells<T, false, DISP>(a, sm.ie, t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0,
+                                 min(k0 + kc, a.nz - 1), s);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void h_pass_batch_p_body(const StepArgs<T>* __restrict__ args,
+                                                    const int4* __restrict__ tiles, int ntiles, int kc, int s) {
+  for (int v = blockIdx.x; v < ntiles; v += gridDim.x) {
+    const int4 t = tiles[xcd_tile_rev(v, ntiles)];
+    const StepArgs<T>& a = args[t.x];
+    const int k0 = 1 + t.w * kc;
+    h_pass_cells(a, a.mim, t.y * blockDim.x + threadIdx.x, 1 + t.z * blockDim.y + threadIdx.y, k0,
+                 min(k0 + kc, a.nz - 1), s);
+  }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void h_pass_batch_p(const StepArgs<T>* __restrict__ args,
+                                                      const int4* __restrict__ tiles, int ntiles, int kc, int s) {
+  h_pass_batch_p_body(args, tiles, ntiles, kc, s);
+}
+template <>
+__global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_batch_p<float>(
+    const StepArgs<float>* __restrict__ args, const int4* __restrict__ tiles, int ntiles, int kc, int s) {
+  h_pass_batch_p_body(args, tiles, ntiles, kc, s);
+}
+
 // ---------------------------------------------------------------- buffer access
 // Raw buffer access (32-bit byte offsets: one lane offset shared by every
 // array, the row/plane part in a scalar register), which keeps the many rows

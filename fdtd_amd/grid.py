@@ -32,7 +32,7 @@ ABI_SYMBOLS = [
     "fdtd_set_tile", "fdtd_get_tile", "fdtd_autotune", "fdtd_set_adjoint_tile",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_set_graph", "fdtd_batch_step", "fdtd_batch_synchronize",
     "fdtd_set_design", "fdtd_adjoint_eps_gradient", "fdtd_adjoint_eps_gradient_batch",
-    "fdtd_batch_set_tile", "fdtd_batch_autotune", "fdtd_set_graph", "fdtd_set_boundary",
+    "fdtd_batch_set_tile", "fdtd_batch_set_persistent", "fdtd_batch_autotune", "fdtd_set_graph", "fdtd_set_boundary",
     "fdtd_set_bloch_boundary", "fdtd_get_probes_imag",
     "fdtd_surface_quantities",
     "fdtd_surface_quantities_static",
@@ -104,6 +104,7 @@ def load_library():
         "fdtd_batch_step": (_i, [_p, _i]),
         "fdtd_batch_synchronize": (_i, [_p]),
         "fdtd_batch_set_tile": (_i, [_p, _i, _i, _i]),
+        "fdtd_batch_set_persistent": (_i, [_p, _i]),
         "fdtd_batch_autotune": (_i, [_p, _i, _ip, _ip, _ip]),
         "fdtd_set_design": (_i, [_p, _ip, _dp]),
         "fdtd_adjoint_eps_gradient": (_i, [_p, _i, _i, _d, _i, _i, _dp, _dp]),
@@ -403,6 +404,10 @@ class Batch:
 
     def set_tile(self, bx, by, kc):
         _check(load_library().fdtd_batch_set_tile(self.h, bx, by, kc))
+
+    def set_persistent(self, wg_per_cu):
+        """Persistent passes: wg_per_cu workgroups per compute unit walk the tile table (0: one per tile)."""
+        _check(load_library().fdtd_batch_set_persistent(self.h, int(wg_per_cu)))
 
     def autotune(self, trial_steps=10):
         """Pick the fastest tile by timing whole batch steps; the grids' states are left unchanged."""

@@ -351,8 +351,15 @@ int fdtd_batch_set_graph(fdtd_batch* b, int on);
 /* Two-pass tile of the batch's passes (as fdtd_set_tile with bx >= 2; the
  * default is 64 x 4 x 2). */
 int fdtd_batch_set_tile(fdtd_batch* b, int bx, int by, int kc);
-/* Times candidate tiles over trial_steps steps of the whole batch, keeps the
- * fastest and restores every grid's device state and step count. */
+/* Persistent passes: wg_per_cu workgroups per compute unit (rounded to a
+ * multiple of 8, at most one per tile) walk the tile table in every pass,
+ * each staging a grid's material table once per run of that grid's tiles;
+ * 0 (the default) launches one workgroup per tile.  Bitwise the same
+ * results either way.  No counterpart in the reference. */
+int fdtd_batch_set_persistent(fdtd_batch* b, int wg_per_cu);
+/* Times candidate tiles over trial_steps steps of the whole batch, then the
+ * best tile's persistent forms (2, 4, 6, 8 workgroups per compute unit),
+ * keeps the fastest and restores every grid's device state and step count. */
 int fdtd_batch_autotune(fdtd_batch* b, int trial_steps, int* bx, int* by, int* kc);
 int fdtd_batch_step(fdtd_batch* b, int nsteps);
 int fdtd_batch_synchronize(fdtd_batch* b);

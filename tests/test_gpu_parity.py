@@ -275,7 +275,8 @@ def test_fused_pass_against_oracle(precision):
 def test_batch_equals_standalone(graph):
     """Grids of uneven sizes stepped as one batch (tile table, hipGraph chunks)
     give the same fields, DFTs and probes as each grid stepped on its own,
-    whatever tiles the batch uses and after its autotuner ran."""
+    whatever tiles the batch uses, one workgroup per tile or persistent
+    workgroups walking the table, and after its autotuner ran."""
     from fdtd_amd import Batch
     scs = synth.batch_c4(nsims=5, nsteps=300, lo=28, hi=44, seed=11)
     tf = synth.grid3d_cpml_tfsf_dft(n=(34, 30, 26), nsteps=300)  # one member with TFSF + probes
@@ -291,9 +292,14 @@ def test_batch_equals_standalone(graph):
     b.autotune(trial_steps=3)  # trial steps must leave every grid as it was
     b.step(7)
     b.set_tile(128, 2, 4)  # the tile changes speed only
-    b.step(256)
+    b.step(200)
+    b.set_persistent(4)  # so do persistent workgroups walking the tile table
+    b.step(56)
     b.set_tile(64, 4, 2)
-    b.step(37)
+    b.set_persistent(1)
+    b.step(20)
+    b.set_persistent(0)
+    b.step(17)
     b.synchronize()
     b.close()
     for sc, g, r in zip(scs, batched, alone):
