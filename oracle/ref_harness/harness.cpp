@@ -142,6 +142,9 @@ typedef Nb<FluidCell<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolari
 typedef Nb<FluidCell<YeeCell<YeeFields<TE, double, std::array>, DoublePolarization<double>,
                              VacuumMagnetization<double>, NewPML<TE, double>>>>
     CellTE;
+typedef Nb<FluidCell<YeeCell<YeeFields<TEM, double, std::array>, DoublePolarization<double>,
+                             VacuumMagnetization<double>, NewPML<TEM, double>>>>
+    CellTEM;
 // a 3-D cell that also carries the magnetization M and magnetic current K of
 // the magnetic-side pole models (DoubleMagnetization, DispersiveMaterials.hpp:86-110)
 typedef Nb<FluidCell<YeeCell<YeeFields<ThreeD, double, std::array>, DoublePolarization<double>,
@@ -258,6 +261,9 @@ struct LocTE {
 };
 struct LocTM {
   std::array<double, 2> operator()(const PlaneIt<CellTM>& it) const { return {it.p->x, it.p->y}; }
+};
+struct LocTEM {
+  std::array<double, 1> operator()(const PlaneIt<CellTEM>& it) const { return {it.p->x}; }
 };
 
 // ------------------------------------------------------------ scenario: PML
@@ -1209,6 +1215,147 @@ static void scen_magnetic(const std::string& dir) {
   fx.write(dir);
 }
 
+// ------------------------------------------------------------ scenario: 1-D TEM
+// The reference's TEM mode (Ez, Hy along x; tests/testTEM.cpp) on a line of
+// cells: CPML at both ends, a dielectric, a Drude, a Lorentz and a conductive
+// segment, a Gaussian on Dz and one on By (variant 0); a TFSF pair of planes
+// (one cell each, their own Aux1D, the reference's TEM polarisation) around a
+// dielectric segment (variant 1).
+static void scen_tem(const std::string& dir) {
+  for (int variant = 0; variant < 2; variant++) {
+    const int nx = 140, nsteps = variant == 0 ? 320 : 360;
+    const double dx = 1e-6, dt = 0.9 * dx / c0;
+    PMLParameterModel dm(dx);
+    const PmlSpec pml = {10, 3, 1, dm.msMax, 5, 0.05};
+    std::vector<Mat> mats = {{0, 1, 0, 0, 0}, {1, 2.25, 0, 0, 0}, {2, 1.5, 0.3 / dt, 0.02 / dt, 0},
+                             {3, 2.0, 0.2 / dt, 0.01 / dt, 0.7}, {4, 1.2, 0.05 * 1.2 * eps0 / dt, 0, 0}};
+    std::vector<int> matid(nx, 0);
+    auto fill = [&](int x0, int x1, int m) {
+      for (int i = x0; i < x1; i++) matid[i] = m;
+    };
+    if (variant == 0) {
+      fill(40, 52, 1);
+      fill(60, 70, 2);
+      fill(76, 88, 3);
+      fill(96, 104, 4);
+    } else {
+      fill(62, 78, 1);
+    }
+    std::vector<CellTEM> cells(nx);
+    for (int i = 1; i < nx - 1; i++) {
+      cells[i].mn[0] = &cells[i - 1];
+      cells[i].mx[0] = &cells[i + 1];
+    }
+    place_pml(nx, pml, [&](int idx, bool e, double K, double Sg, double A) {
+      if (e)
+        cells[idx].setPMLParameters<FieldType::Electric, Dir::X>(K, Sg, A, dt);
+      else
+        cells[idx].setPMLParameters<FieldType::Magnetic, Dir::X>(K, Sg, A, dt);
+    });
+    std::vector<double> sD, sB;
+    for (int n = 0; n < nsteps; n++) {
+      sD.push_back(eps0 * gauss_pulse(n, 30, 8));
+      sB.push_back(1e-9 * gauss_pulse(n, 45, 9));
+    }
+    const int si = 30, bi = 110;
+    // TFSF: planes at x = 40 (MIN) and 100 (MAX)
+    const int bx0 = 40, bx1 = 100;
+    const double cx = std::floor(0.5 * (bx0 + bx1)) + 0.5;
+    const double f0 = c0 / (20 * dx), tw = 2.0 / f0, t0 = 3.0 * tw;
+    std::vector<double> aux_sig;
+    typedef TFSFPlane<TEM, LocTEM, PlaneIt<CellTEM>, Aux1D<double>> PlaneT;
+    std::vector<std::vector<PlaneEntry<CellTEM>>> faces(variant == 1 ? 2 : 0);
+    std::vector<PlaneT> planes;
+    if (variant == 1) {
+      faces[0].push_back({&cells[bx0], (bx0 - cx) * dx, 0, 0});
+      faces[1].push_back({&cells[bx1], (bx1 - cx) * dx, 0, 0});
+      for (int fidx = 0; fidx < 2; fidx++) {
+        auto sig = [f0, tw, t0, fidx, &aux_sig](double t) {
+          double a = (t - t0) / tw;
+          double v = std::exp(-a * a) * std::sin(2 * pi * f0 * (t - t0));
+          if (fidx == 0) aux_sig.push_back(v);
+          return v;
+        };
+        auto aux = std::make_shared<Aux1D<double>>(100, dx, dt, sig);
+        std::array<double, 1> K = {1.0};
+        PlaneIt<CellTEM> b{faces[fidx].data()}, e{faces[fidx].data() + faces[fidx].size()};
+        if (fidx == 0)
+          planes.push_back(make_tfsf_plane<TEM, Dir::X, Orientation::MIN>(PlaneIt<CellTEM>(b), PlaneIt<CellTEM>(e), LocTEM(), K, aux));
+        else
+          planes.push_back(make_tfsf_plane<TEM, Dir::X, Orientation::MAX>(PlaneIt<CellTEM>(b), PlaneIt<CellTEM>(e), LocTEM(), K, aux));
+      }
+    }
+    const std::vector<int> probes = variant == 0 ? std::vector<int>{30, 45, 64, 80, 100, 110, 5}
+                                                 : std::vector<int>{70, 50, 40, 39, 100, 101, 20};
+    std::vector<double> ptrace;
+    for (int n = 0; n < nsteps; n++) {
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TEM, Dz>(dt, dx)(cells[i]);
+      if (variant == 0) cells[si].Dz() += sD[n];
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TEM, FieldType::Electric, Dir::X>(dt, dx)(cells[i]);
+      for (auto& pl : planes) pl.updateD(dt, dx);
+      for (int i = 1; i < nx - 1; i++) material_E<TEM>(cells[i], mats[matid[i]], dt);
+      for (int i = 1; i < nx - 1; i++) YeeUpdate<TEM, By>(dt, dx)(cells[i]);
+      if (variant == 0) cells[bi].By() += sB[n];
+      for (int i = 1; i < nx - 1; i++) UpdatePML<TEM, FieldType::Magnetic, Dir::X>(dt, dx)(cells[i]);
+      for (auto& pl : planes) pl.updateB(dt, dx);
+      for (int i = 1; i < nx - 1; i++) VacuumUpdate<TEM, FieldType::Magnetic>()(cells[i]);
+      for (int q : probes) {
+        ptrace.push_back(cells[q].Ez());
+        ptrace.push_back(cells[q].Hy());
+      }
+    }
+    Fixture fx;
+    fx.name = variant == 0 ? "tem1d_small" : "tem1d_tfsf";
+    fx.ps("mode", "TEM");
+    fx.pi("nx", nx);
+    fx.pi("ny", 1);
+    fx.pi("nsteps", nsteps);
+    fx.p("dx", dx);
+    fx.p("dt", dt);
+    put_pml(fx, pml);
+    fx.pi("pmlx", 1);
+    fx.pi("pmly", 0);
+    if (variant == 0) {
+      fx.arr("src_field_comp_i_j", {0.0, 2.0, double(si), 0.0, 2.0, 1.0, double(bi), 0.0}, {2, 4});
+      std::vector<double> st(sD);
+      st.insert(st.end(), sB.begin(), sB.end());
+      fx.arr("src_tables", st, {2, size_t(nsteps)});
+    } else {
+      fx.pi("tfsf", 1);
+      fx.pi("bx0", bx0);
+      fx.pi("bx1", bx1);
+      fx.p("kx", 1.0);
+      fx.pi("aux_ncells", 100);
+      fx.p("cx", cx);
+      fx.p("f0", f0);
+      fx.p("tw", tw);
+      fx.p("t0", t0);
+      fx.arr("aux_signal", aux_sig, {aux_sig.size()});
+    }
+    std::vector<double> mt;
+    for (auto& m : mats) mt.insert(mt.end(), {double(m.kind), m.eps, m.freq, m.gamma, m.delta});
+    fx.arr("mats_kind_eps_freq_gamma_delta", mt, {mats.size(), 5});
+    std::vector<double> mid(matid.begin(), matid.end());
+    fx.arr("matid", mid, {1, size_t(nx)});
+    std::vector<double> pr;
+    for (int q : probes) pr.push_back(double(q));
+    fx.arr("probes", pr, {probes.size(), 1});
+    fx.arr("probe_trace_Ez_Hy", ptrace, {size_t(nsteps), probes.size(), 2});
+    std::vector<double> Ez, Dzv, Hy, Byv;
+    for (auto& c : cells) {
+      Ez.push_back(c.Ez());
+      Dzv.push_back(c.Dz());
+      Hy.push_back(c.Hy());
+      Byv.push_back(c.By());
+    }
+    fx.arr("Ez", Ez, {1, size_t(nx)});
+    fx.arr("Dz", Dzv, {1, size_t(nx)});
+    fx.arr("Hy", Hy, {1, size_t(nx)});
+    fx.arr("By", Byv, {1, size_t(nx)});
+    fx.write(dir);
+  }
+}
+
 // ------------------------------------------------------------ scenario: material map XML
 // The reference's MaterialMap / MaterialPair (include/MaterialMap.hpp:45-284)
 // holding one material of every kind the device path covers, written by its
@@ -1768,6 +1915,7 @@ int main(int argc, char** argv) {
   scen_material_map(dir);
   scen_aux1d(dir);
   scen_magnetic(dir);
+  scen_tem(dir);
 
   // 2D TM, small, non-square, CPML x+y, two dielectric blocks, one Drude and
   // one Lorentz block, Gaussian soft source on Dz.

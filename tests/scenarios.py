@@ -155,10 +155,17 @@ def from_golden(name):
             elif mode == "TE":
                 for f, c in [(E, 0), (E, 1), (H, 2)]:
                     sc["probes"].append((f, c, *ijk))
+            elif mode == "TEM":
+                for f, c in [(E, 2), (H, 1)]:
+                    sc["probes"].append((f, c, *ijk))
             else:
                 for f, c in [(E, 2), (H, 0), (H, 1)]:
                     sc["probes"].append((f, c, *ijk))
-    if int(p.get("tfsf", 0)):
+    if int(p.get("tfsf", 0)) and mode == "TEM":  # a pair of planes along x; the reference's TEM defaults
+        sc["tfsf"] = dict(box=[int(p["bx0"]), int(p["bx1"]), 0, 0, 0, 0], kvec=[1.0, 0.0, 0.0], pol=[0.0, 0.0, 1.0],
+                          polH=[0.0, 1.0, 0.0], origin=[p["cx"], 0.0, 0.0], aux_ncells=int(p["aux_ncells"]),
+                          signal=a["aux_signal"])
+    elif int(p.get("tfsf", 0)):
         kx, ky = p["kx"], p["ky"]
         sc["tfsf"] = dict(
             box=[int(p["bx0"]), int(p["bx1"]), int(p["by0"]), int(p["by1"]), 0, 0],

@@ -149,7 +149,7 @@ def test_cell_fluid():
 @pytest.mark.parametrize("name", ["tm2d_small", "tm2d_tfsf_x", "tm2d_tfsf_oblique", "grid3d_small",
                                   "grid3d_disp", "tm2d_c1", "tm2d_conductive", "grid3d_magnetized",
                                   "grid3d_fluid", "tm2d_fluid", "tm2d_soft_planes", "te2d_small", "te2d_tfsf",
-                                  "grid3d_magnetic"])
+                                  "grid3d_magnetic", "tem1d_small", "tem1d_tfsf"])
 def test_grid_scenarios_bit_exact(name):
     sc, p, a = from_golden(name)
     sim = refcpu.Sim(sc["mode"], sc["nx"], sc["ny"], sc["nz"], sc["dx"], sc["dt"])
@@ -159,6 +159,8 @@ def test_grid_scenarios_bit_exact(name):
         comps = [("Ez", E, 2), ("Dz", D, 2), ("Hx", H, 0), ("Hy", H, 1), ("Bx", B, 0), ("By", B, 1)]
     elif sc["mode"] == "TE":
         comps = [("Ex", E, 0), ("Dx", D, 0), ("Ey", E, 1), ("Dy", D, 1), ("Hz", H, 2), ("Bz", B, 2)]
+    elif sc["mode"] == "TEM":
+        comps = [("Ez", E, 2), ("Dz", D, 2), ("Hy", H, 1), ("By", B, 1)]
     else:
         comps = [("Ex", E, 0), ("Ey", E, 1), ("Ez", E, 2), ("Hx", H, 0), ("Hy", H, 1), ("Hz", H, 2)]
     checked = 0
