@@ -1195,7 +1195,7 @@ int step_impl(fdtd_grid* g, int nsteps) {
   const bool general = g->nbc || g->nsrc_eh || g->nplane || g->max_mnp > 0;
   // the time-blocked TM pass covers non-dispersive grids without those, complex
   // fields or DFT monitors
-  const bool te = g->d.mode == FDTD_TE;
+  const bool te = g->d.mode == FDTD_TE, tem = g->ndim == 1;
   const bool tmb = g->ndim == 2 && !te && t.bx == 0 && !general && !g->cplx && g->max_np == 0 && g->ndft == 0 &&
                    !g->force_two_pass;
   if (g->ndim == 2 && !tmb) {
@@ -1278,6 +1278,9 @@ int step_impl(fdtd_grid* g, int nsteps) {
     const int R = t.kc + 2 * t.by;
     block = dim3(unsigned((R * R + 63) / 64 * 64), 1, 1);
     grid = dim3(unsigned(ntx * nty), 1, 1);
+  } else if (tem) {
+    block = dim3(256, 1, 1);  // a thread per cell of the line
+    grid = dim3(unsigned((g->d.nx + 255) / 256), 1, 1);
   } else {
     block = dim3(t.bx, t.by, 1);
     grid = dim3((g->d.nx + t.bx - 1) / t.bx, (g->d.ny - 2 + t.by - 1) / t.by, 1);
@@ -1475,6 +1478,12 @@ int step_impl(fdtd_grid* g, int nsteps) {
             rc = launch(g, 1, fdtdk::h_pass_3d<T, true>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
           else
             rc = launch(g, 1, fdtdk::h_pass_3d<T>, grid, block, dev0, t.kc, s, ntx, nty, nkc);
+        } else if (tem && general) {
+          rc = launch(g, 0, fdtdk::e_pass_tem<T, true>, grid, block, dev0, 1, s);
+          if (!rc) rc = launch(g, 1, fdtdk::h_pass_tem<T, true>, grid, block, dev0, 1, s);
+        } else if (tem) {
+          rc = launch(g, 0, fdtdk::e_pass_tem<T>, grid, block, dev0, 1, s);
+          if (!rc) rc = launch(g, 1, fdtdk::h_pass_tem<T>, grid, block, dev0, 1, s);
         } else if (te && general) {
           rc = launch(g, 0, fdtdk::e_pass_te<T, true>, grid, block, dev0, 1, s);
           if (!rc) rc = launch(g, 1, fdtdk::h_pass_te<T, true>, grid, block, dev0, 1, s);
@@ -1630,6 +1639,14 @@ void snapshot_restore(fdtd_grid* g, GridSnapshot& s) {
   if ((g->nswap - s.nswap) % 2) swap_state(g);
   g->step = s.step;
   g->dft_done = s.dft_done;
+}
+
+// (i, j, k) is an interior cell of the mode's axes, with coordinate 0 on the others
+static bool interior_cell(const fdtd_grid* g, int i, int j, int k) {
+  const int c[3] = {i, j, k};
+  for (int a = 0; a < 3; a++)
+    if (a < g->ndim ? (c[a] < 1 || c[a] >= g->len[a] - 1) : c[a] != 0) return false;
+  return true;
 }
 
 bool valid_comp(const fdtd_grid* g, int field, int comp) {
@@ -2077,11 +2094,12 @@ const char* fdtd_last_error(void) { return g_err.c_str(); }
 int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   if (!desc || !out) return fail(FDTD_EINVAL, "null argument");
   *out = nullptr;
-  if (desc->mode != FDTD_TM && desc->mode != FDTD_TE && desc->mode != FDTD_3D)
-    return fail(FDTD_EINVAL, "mode must be FDTD_TM, FDTD_TE or FDTD_3D");
+  if (desc->mode != FDTD_TM && desc->mode != FDTD_TE && desc->mode != FDTD_3D && desc->mode != FDTD_TEM)
+    return fail(FDTD_EINVAL, "mode must be FDTD_TEM, FDTD_TM, FDTD_TE or FDTD_3D");
   if (desc->precision != FDTD_F32 && desc->precision != FDTD_F64) return fail(FDTD_EINVAL, "bad precision");
   const int nz = desc->mode == FDTD_3D ? desc->nz : 1;
-  if (desc->nx < 3 || desc->ny < 3 || (desc->mode == FDTD_3D && nz < 3))
+  const int ny = desc->mode == FDTD_TEM ? 1 : desc->ny;
+  if (desc->nx < 3 || (desc->mode != FDTD_TEM && desc->ny < 3) || (desc->mode == FDTD_3D && nz < 3))
     return fail(FDTD_EINVAL, "grid must have at least 3 cells per axis");
   if (!(desc->dx > 0) || !(desc->dt > 0)) return fail(FDTD_EINVAL, "dx and dt must be positive");
   int dev = 0;
@@ -2090,18 +2108,22 @@ int fdtd_grid_create(const fdtd_grid_desc* desc, fdtd_grid** out) {
   if (hipDeviceGetAttribute(&g->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || g->ncu < 1) g->ncu = 256;
   g->d = *desc;
   g->d.nz = nz;
-  g->ndim = desc->mode == FDTD_3D ? 3 : 2;
+  g->d.ny = ny;
+  g->ndim = desc->mode == FDTD_3D ? 3 : (desc->mode == FDTD_TEM ? 1 : 2);
   g->f64 = desc->precision == FDTD_F64;
   g->esz = g->f64 ? 8 : 4;
   g->len[0] = desc->nx;
-  g->len[1] = desc->ny;
+  g->len[1] = ny;
   g->len[2] = nz;
-  g->ncell = long(desc->nx) * desc->ny * nz;
+  g->ncell = long(desc->nx) * ny * nz;
   if (g->ndim == 3) {
     for (int c = 0; c < 3; c++) g->actE[c] = g->actH[c] = true;
   } else if (desc->mode == FDTD_TE) {
     g->actE[0] = g->actE[1] = true;
     g->actH[2] = true;
+  } else if (desc->mode == FDTD_TEM) {
+    g->actE[2] = true;
+    g->actH[1] = true;
   } else {
     g->actE[2] = true;
     g->actH[0] = g->actH[1] = true;
@@ -2272,9 +2294,7 @@ int fdtd_add_source(fdtd_grid* g, int field, int comp, int i, int j, int k, cons
   if (!g || !table || ntable < 0) return fail(FDTD_EINVAL, "null argument");
   if (g->nsrc >= FDTD_MAX_SOURCES) return fail(FDTD_EINVAL, "too many sources (max %d)", FDTD_MAX_SOURCES);
   if (field < FDTD_D || field > FDTD_H || !valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "bad source field/component");
-  if (i < 1 || j < 1 || i >= g->d.nx - 1 || j >= g->d.ny - 1 || (g->ndim == 3 && (k < 1 || k >= g->d.nz - 1)) ||
-      (g->ndim == 2 && k != 0))
-    return fail(FDTD_EINVAL, "source cell must be an interior cell");
+  if (!interior_cell(g, i, j, k)) return fail(FDTD_EINVAL, "source cell must be an interior cell");
   double* t = nullptr;
   int rc = g->alloc((void**)&t, size_t(std::max(ntable, 1)) * 8);
   if (rc) return rc;
@@ -2345,7 +2365,7 @@ int fdtd_set_tfsf(fdtd_grid* g, const int box[6], const double kvec[3], const do
       return fail(FDTD_EINVAL, "TFSF box must lie on interior cells with lo < hi");
   if (aux_ncells < 2 || (aux_ncells + 50) > 8192) return fail(FDTD_EINVAL, "bad aux line length");
   g->tfsf = true;
-  for (int x = 0; x < 6; x++) g->box[x] = g->ndim == 3 || x < 4 ? box[x] : 0;
+  for (int x = 0; x < 6; x++) g->box[x] = x < 2 * g->ndim ? box[x] : 0;
   // TFSFPlane normalises the wave vector (include/Source.hpp:566-570)
   double K = 0;
   for (int d = 0; d < g->ndim; d++) K += kvec[d] * kvec[d];
@@ -2387,9 +2407,9 @@ int fdtd_add_tfsf_plane(fdtd_grid* g, int axis, int side, const int cells[6], co
     return fail(FDTD_EINVAL, "bad plane axis or side");
   int c[6];
   for (int x = 0; x < 6; x++) c[x] = cells[x];
-  if (g->ndim == 2) {
-    c[4] = 0;
-    c[5] = 1;
+  for (int a = g->ndim; a < 3; a++) {  // axes the mode does not have: the one layer 0
+    c[2 * a] = 0;
+    c[2 * a + 1] = 1;
   }
   for (int a = 0; a < g->ndim; a++)
     if (c[2 * a] < 1 || c[2 * a + 1] > g->len[a] - 1 || c[2 * a] >= c[2 * a + 1])
@@ -2428,9 +2448,9 @@ int fdtd_add_dft(fdtd_grid* g, const int region[6], int comp_mask, int nfreq, co
   if (g->ndft >= FDTD_MAX_DFT) return fail(FDTD_EINVAL, "too many DFT monitors (max %d)", FDTD_MAX_DFT);
   int r[6];
   for (int x = 0; x < 6; x++) r[x] = region[x];
-  if (g->ndim == 2) {
-    r[4] = 0;
-    r[5] = 1;
+  for (int a = g->ndim; a < 3; a++) {  // axes the mode does not have: the one layer 0
+    r[2 * a] = 0;
+    r[2 * a + 1] = 1;
   }
   for (int a = 0; a < 3; a++)
     if (r[2 * a] < 0 || r[2 * a + 1] > g->len[a] || r[2 * a] >= r[2 * a + 1]) return fail(FDTD_EINVAL, "bad DFT region");
@@ -2489,9 +2509,7 @@ int fdtd_add_probe(fdtd_grid* g, int field, int comp, int i, int j, int k) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
   if (g->nprobe >= FDTD_MAX_PROBES) return fail(FDTD_EINVAL, "too many probes (max %d)", FDTD_MAX_PROBES);
   if ((field != FDTD_E && field != FDTD_H) || !valid_comp(g, field, comp)) return fail(FDTD_EINVAL, "probes record E or H components");
-  if (i < 1 || j < 1 || i >= g->d.nx - 1 || j >= g->d.ny - 1 || (g->ndim == 3 && (k < 1 || k >= g->d.nz - 1)) ||
-      (g->ndim == 2 && k != 0))
-    return fail(FDTD_EINVAL, "probe cell must be an interior cell");
+  if (!interior_cell(g, i, j, k)) return fail(FDTD_EINVAL, "probe cell must be an interior cell");
   if (g->step > 0) return fail(FDTD_ESTATE, "probes must be added before stepping");
   g->probe[g->nprobe].cell = g->q(i, j, k);
   g->probe[g->nprobe].field = field;
@@ -2669,6 +2687,13 @@ int fdtd_autotune(fdtd_grid* g, int trial_steps, int* bx, int* by, int* kc) {
   if (!g || trial_steps < 1) return fail(FDTD_EINVAL, "bad argument");
   int rc = 0;
   if (!g->finalized && (rc = finalize(g))) return rc;
+  if (g->ndim == 1) {  // TEM: one schedule (a thread per cell), nothing to choose
+    const Tile t = g->tile.by ? g->tile : kDefaultTileTM;
+    if (bx) *bx = t.bx;
+    if (by) *by = t.by;
+    if (kc) *kc = t.kc;
+    return 0;
+  }
   static const Tile cand3[] = {{0, 5, 1},  {0, 5, 2},  {256, 1, 1}, {256, 1, 2}, {256, 1, 4}, {256, 1, 8},
                                {128, 2, 2}, {128, 2, 4}, {64, 4, 1}, {64, 4, 2}, {1, 8, 16}, {1, 8, 32},
                                {1, 16, 16}, {1, 4, 32}, {2, 12, 16}, {2, 12, 32}};
@@ -2779,6 +2804,7 @@ int fdtd_set_boundary(fdtd_grid* g, int axis, int side, int kind) {
 
 int fdtd_set_bloch_boundary(fdtd_grid* g, int axis, int side, double factor_re, double factor_im) {
   if (!g) return fail(FDTD_EINVAL, "null grid");
+  if (g->ndim == 1) return fail(FDTD_EINVAL, "Bloch-periodic faces are for 2-D and 3-D grids");
   if (axis < 0 || axis >= g->ndim || side < 0 || side > 1) return fail(FDTD_EINVAL, "bad axis or side");
   if (g->finalized) return fail(FDTD_ESTATE, "boundary conditions are fixed once stepping has started");
   g->nbc += (g->bc[axis][side] == FDTD_BC_NONE);
@@ -2791,6 +2817,7 @@ int fdtd_set_bloch_boundary(fdtd_grid* g, int axis, int side, double factor_re, 
 
 int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out) {
   if (!g || !region || !out) return fail(FDTD_EINVAL, "null argument");
+  if (g->ndim == 1) return fail(FDTD_EINVAL, "the reference defines surface quantities for TE, TM and 3-D");
   for (int a = 0; a < 3; a++) {
     const int lo = a < g->ndim ? 1 : 0, hi = a < g->ndim ? g->len[a] - 1 : 1;
     if (region[2 * a] < lo || region[2 * a + 1] > hi || region[2 * a] >= region[2 * a + 1])
@@ -2805,6 +2832,7 @@ int fdtd_surface_quantities(fdtd_grid* g, const int region[6], double* out) {
 
 int fdtd_surface_quantities_static(fdtd_grid* g, const int region[6], double* out) {
   if (!g || !region || !out) return fail(FDTD_EINVAL, "null argument");
+  if (g->ndim == 1) return fail(FDTD_EINVAL, "the reference defines surface quantities for TE, TM and 3-D");
   for (int a = 0; a < 3; a++) {
     const int lo = a < g->ndim ? 1 : 0, hi = a < g->ndim ? g->len[a] - 1 : 1;
     if (region[2 * a] < lo || region[2 * a + 1] > hi || region[2 * a] >= region[2 * a + 1])
@@ -2836,9 +2864,9 @@ int fdtd_set_design(fdtd_grid* g, const int box[6], const double* eps_r) {
   if (!g || !box || !eps_r) return fail(FDTD_EINVAL, "null argument");
   int b[6];
   for (int x = 0; x < 6; x++) b[x] = box[x];
-  if (g->ndim == 2) {
-    b[4] = 0;
-    b[5] = 1;
+  for (int a = g->ndim; a < 3; a++) {  // axes the mode does not have: the one layer 0
+    b[2 * a] = 0;
+    b[2 * a + 1] = 1;
   }
   for (int a = 0; a < g->ndim; a++)
     if (b[2 * a] < 1 || b[2 * a + 1] > g->len[a] - 1 || b[2 * a] >= b[2 * a + 1])

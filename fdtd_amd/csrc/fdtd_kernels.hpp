@@ -3040,6 +3040,100 @@ __global__ __launch_bounds__(256) void h_pass_te(const StepArgs<T>* __restrict__
   }
 }
 
+// ---------------------------------------------------------------- 1-D TEM
+// Ez, Hy along x (FieldComponents<TEM>, include/FDTDConstants.hpp:868-874);
+// CPML along x; in place; one thread per cell.  The x terms of the TM passes:
+// Dz += dt F_x/dx (Hy - Hy[i-1]), By += dt F_x/dx (Ez[i+1] - Ez).
+template <typename T, bool GEN = false>
+__global__ __launch_bounds__(256) void e_pass_tem(const StepArgs<T>* __restrict__ ap, int, int s) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = *ap;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i >= a.nx - 1) return;
+  const long step = a.step0 + s;
+  const long c = i;
+  const T hy = a.H[1][c], hy_xm = a.H[1][c - 1];
+  T ez = a.E[2][c];
+  if (GEN && a.nsrc_eh) {  // the old Ez without the last step's E source value
+    T v[3] = {T(0), T(0), ez};
+    soft_eh(a, step, 1, c, true, v);
+    ez = v[2];
+  }
+  const int mid = a.mat[c];
+  const T ie = cell_inv_eps(a, a.mie, mid, i, 0, 0);
+  const T dzx = hy - hy_xm;  // J=x, +1
+  T d[3] = {T(0), T(0), a.cf[0][0][i] * dzx};
+  if (a.nsrc) add_sources(a, step, 0, c, d);
+  const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
+  if (sxi >= 0) {
+    const long o = slab_off(a, 0, sxi, i, 0, 0);
+    const T p = a.psi[0][2][0][o];
+    d[2] = fmadd(a.cg[0][0][i], p, d[2]);
+    a.psio[0][2][0][o] = fmadd(a.cb[0][0][i], p, a.cc[0][0][i] * dzx);
+  }
+  if (a.tfsf && (i == a.box[0] || i == a.box[1])) {
+    const bool aF[3] = {false, false, true}, aC[3] = {false, true, false};
+    tfsf_cell<T, 1>(a, a.incH + (long)s * a.auxL, 0, i, 0, 0, aF, aC, (const T*)nullptr, d);
+  }
+  if (GEN && a.nplane) plane_cells<T, 1>(a, step, 0, i, 0, 0, 4, 2, d);
+  T ezn = ez;
+  if (ie > T(0)) {
+    ezn = fmadd(d[2], ie, ez);
+  } else if (ie < T(0) && a.Dd[2]) {
+    const MatDev<T>& m = a.mats[mid];
+    if (m.fluid)
+      ezn = fluid_component<T, 2>(a, m, dbox_index(a, i, 0, 0), d[2], true);
+    else
+      ezn = ade_component<T, 2>(a, m, dbox_index(a, i, 0, 0), d[2], ez, true);
+  }
+  T e[3] = {T(0), T(0), ezn};
+  if (GEN && a.nsrc_eh) soft_eh(a, step, 1, c, false, e);
+  a.Eo[2][c] = e[2];
+  if (GEN && a.bc_any) bc_ghost_write(a, 0, 1, i, 0, 0, c, e);
+}
+
+template <typename T, bool GEN = false>
+__global__ __launch_bounds__(256) void h_pass_tem(const StepArgs<T>* __restrict__ ap, int, int s) {
+#pragma clang fp contract(off)
+  const StepArgs<T>& a = *ap;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 1 || i >= a.nx - 1) return;
+  const long step = a.step0 + s;
+  const long c = i;
+  const T ez = a.Eo[2][c], ez_xp = a.Eo[2][c + 1];
+  T hy = a.H[1][c];
+  if (GEN && a.nsrc_eh) {  // the old H without the last step's H source value
+    T v[3] = {T(0), hy, T(0)};
+    soft_eh(a, step, 3, c, true, v);
+    hy = v[1];
+  }
+  const T im = a.mim[a.mat[c]];
+  const T dyx = ez_xp - ez;  // Hy, J=x, -1
+  T d[3] = {T(0), a.cf[1][0][i] * dyx, T(0)};
+  if (a.nsrc) add_sources(a, step, 2, c, d);
+  const int sxi = slab_index(i, a.slab_a[0], a.slab_b[0]);
+  if (sxi >= 0) {
+    const long o = slab_off(a, 0, sxi, i, 0, 0);
+    const T p = a.psi[1][1][0][o];
+    d[1] = fmadd(-a.cg[1][0][i], p, d[1]);
+    a.psio[1][1][0][o] = fmadd(a.cb[1][0][i], p, -(a.cc[1][0][i] * dyx));
+  }
+  if (a.tfsf && (i == a.box[0] || i == a.box[1])) {
+    const bool aF[3] = {false, true, false}, aC[3] = {false, false, true};
+    tfsf_cell<T, 1>(a, a.incE + (long)s * a.auxL, 1, i, 0, 0, aF, aC, (const T*)nullptr, d);
+  }
+  if (GEN && a.nplane) plane_cells<T, 1>(a, step, 1, i, 0, 0, 2, 4, d);
+  T h[3] = {T(0), fmadd(d[1], im, hy), T(0)};
+  if (GEN && im < T(0)) h[1] = mag_component<T, 1>(a, c, mbox_index(a, i, 0, 0), d[1], hy);  // magnetic poles
+  if (GEN && a.nsrc_eh) soft_eh(a, step, 3, c, false, h);
+  a.Ho[1][c] = h[1];
+  if (GEN && a.bc_any) bc_ghost_write(a, 1, 1, i, 0, 0, c, h);
+  if (a.nprobe) {
+    const T ev[3] = {T(0), T(0), ez};
+    dft_probe(a, step, c, i, 0, 0, ev, h);
+  }
+}
+
 // ---------------------------------------------------------------- 2-D TM, time-blocked
 // One workgroup per C x C core of the grid advances it TB steps per launch:
 // it loads the core and a halo of TB cells on every side (the state at the

@@ -14,8 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfdtd_hip.so")
 
-TM, THREED, TE = 2, 3, 4
-MODES = {"TM": TM, "TE": TE, "ThreeD": THREED}
+TEM, TM, THREED, TE = 1, 2, 3, 4
+MODES = {"TEM": TEM, "TM": TM, "TE": TE, "ThreeD": THREED}
 F32, F64 = 0, 1
 D, E, B, H, P, J, M, K = 0, 1, 2, 3, 4, 5, 6, 7
 DRUDE, LORENTZ, CONDUCTIVE = 1, 2, 3
@@ -138,12 +138,13 @@ def _f64(a):
 
 
 class Grid:
-    """One simulation on the GPU.  ``mode`` is "TM", "TE" or "ThreeD";
+    """One simulation on the GPU.  ``mode`` is "TEM" (a line along x: ny = nz = 1), "TM", "TE" or "ThreeD";
     ``precision`` "f32" or "f64"."""
 
     def __init__(self, mode, nx, ny, nz=1, dx=1.0, dt=1.0, precision="f64", stream=None):
         L = load_library()
         self.mode = mode
+        ny = 1 if mode == "TEM" else ny
         self.shape = (nz if mode == "ThreeD" else 1, ny, nx)
         if mode not in MODES:
             raise ValueError("mode must be one of %s" % ", ".join(MODES))
@@ -316,7 +317,7 @@ class Grid:
         nothing (regions of at most FDTD_SURFACE_STATIC_MAX_CELLS cells, a grid
         already set up on the device)."""
         r = [int(v) for v in region]
-        nq = {"ThreeD": 7, "TM": 3, "TE": 4}[self.mode]
+        nq = {"ThreeD": 7, "TM": 3, "TE": 4}.get(self.mode, 1)  # TEM: refused by the library
         shape = (r[5] - r[4], r[3] - r[2], r[1] - r[0])
         out = np.zeros(nq * int(np.prod(shape)))
         fn = load_library().fdtd_surface_quantities_static if static else load_library().fdtd_surface_quantities

@@ -77,6 +77,9 @@ enum class Dir : char { X = 0, Y, Z };
 enum class FieldType : char { Electric = 0, Magnetic };
 enum class Orientation : char { MIN = 0, MAX };
 
+struct TEM {
+  static constexpr int dim = 1, code = FDTD_TEM;
+};
 struct TM {
   static constexpr int dim = 2, code = FDTD_TM;
 };
@@ -372,7 +375,8 @@ struct GridCells {
 template <class Mode, class scalar>
 class YeeGrid {
  public:
-  YeeGrid(int nx, int ny, int nz, double dx, double dt) : nx_(nx), ny_(ny), nz_(Mode::dim == 3 ? nz : 1), dx_(dx), dt_(dt) {
+  YeeGrid(int nx, int ny, int nz, double dx, double dt)
+      : nx_(nx), ny_(Mode::dim >= 2 ? ny : 1), nz_(Mode::dim == 3 ? nz : 1), dx_(dx), dt_(dt) {
     fdtd_grid_desc d{Mode::code, std::is_same<scalar, double>::value ? FDTD_F64 : FDTD_F32, nx_, ny_, nz_, dx, dt};
     fdtd_grid* g = nullptr;
     check(fdtd_grid_create(&d, &g), "fdtd_grid_create");
@@ -387,6 +391,7 @@ class YeeGrid {
     loop_.before_run = [this]() { commit_pml(); };
   }
   YeeGrid(int nx, int ny, double dx, double dt) : YeeGrid(nx, ny, 1, dx, dt) {}
+  YeeGrid(int nx, double dx, double dt) : YeeGrid(nx, 1, 1, dx, dt) {}  // TEM: a line of nx cells
   YeeGrid(const YeeGrid&) = delete;
   YeeGrid& operator=(const YeeGrid&) = delete;
 
@@ -408,6 +413,7 @@ class YeeGrid {
   double dt() const { return dt_; }
 
   CellRef<Mode, scalar> cell(int i, int j, int k = 0) { return {this, i, j, k}; }
+  CellRef<Mode, scalar> cell(int i) { return {this, i, 0, 0}; }  // TEM: cell i of the line
   // the cells of one material id: the range a material functor sweeps
   GridCells<Mode, scalar> cells(int id) { return {this, id}; }
 
@@ -430,6 +436,10 @@ class YeeGrid {
   }
 
   // record one E or H component at a cell after every step
+  template <class F>
+  void add_probe(int i) {  // TEM: cell i of the line
+    add_probe<F>(i, 0, 0);
+  }
   template <class F>
   void add_probe(int i, int j, int k = 0) {
     static_assert(F::field == FDTD_E || F::field == FDTD_H, "probes record E or H");
@@ -803,7 +813,7 @@ struct Aux1D {
 namespace detail {
 // Polarisations by the reference's rule for the mode (include/Source.hpp:
 // 573-584): TM pol = z, polH = (k_y, -k_x, 0); TE pol = (k_y, -k_x, 0),
-// polH = z.  ThreeD, which the reference leaves unset, gets pol = z and
+// polH = z; TEM pol = z, polH = y.  ThreeD, which the reference leaves unset, gets pol = z and
 // polH = pol x k = (-k_y, k_x, 0).  (The TM
 // rule's polH is the negative of pol x k: with it the incident field the
 // faces subtract is not the line's own plane wave, so the scattered-field
@@ -815,6 +825,8 @@ inline void default_pol(int mode, const std::array<double, 3>& k, std::array<dou
   if (mode == FDTD_TE) {
     pol = {k[1], -k[0], 0};
     polH = {0, 0, 1};
+  } else if (mode == FDTD_TEM) {
+    polH = {0, 1, 0};
   } else if (mode == FDTD_TM) {
     polH = {k[1], -k[0], 0};
   } else {

@@ -35,10 +35,9 @@ enum fdtd_status {
   FDTD_ESTATE = -4    /* call not valid in the grid's current state */
 };
 
-/* Modes, as in include/FDTDConstants.hpp:424-445 (TEM exists only as the
- * TFSF incident line inside the library): TM (Ez, Hx, Hy), TE (Ex, Ey, Hz),
- * ThreeD. */
-enum fdtd_mode { FDTD_TM = 2, FDTD_3D = 3, FDTD_TE = 4 };
+/* Modes, as in include/FDTDConstants.hpp:424-445: TEM (Ez, Hy along x; a
+ * line of nx cells, ny = nz = 1), TM (Ez, Hx, Hy), TE (Ex, Ey, Hz), ThreeD. */
+enum fdtd_mode { FDTD_TEM = 1, FDTD_TM = 2, FDTD_3D = 3, FDTD_TE = 4 };
 enum fdtd_precision { FDTD_F32 = 0, FDTD_F64 = 1 };
 /* field codes (M, K: magnetization and magnetic current of magnetic poles) */
 enum fdtd_field { FDTD_D = 0, FDTD_E = 1, FDTD_B = 2, FDTD_H = 3, FDTD_P = 4, FDTD_J = 5, FDTD_M = 6, FDTD_K = 7 };
@@ -70,9 +69,9 @@ enum fdtd_side { FDTD_MIN = 0, FDTD_MAX = 1 };
 typedef struct fdtd_grid fdtd_grid;
 
 typedef struct {
-  int mode;      /* FDTD_TM, FDTD_TE or FDTD_3D */
+  int mode;      /* FDTD_TEM, FDTD_TM, FDTD_TE or FDTD_3D */
   int precision; /* FDTD_F32 or FDTD_F64: storage and arithmetic type */
-  int nx, ny, nz; /* cells per axis including the outer layer; nz = 1 for TM and TE */
+  int nx, ny, nz; /* cells per axis including the outer layer; nz = 1 for TM and TE, ny = nz = 1 for TEM */
   double dx;     /* cell size [m] */
   double dt;     /* time step [s] */
 } fdtd_grid_desc;

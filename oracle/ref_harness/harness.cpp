@@ -1223,7 +1223,7 @@ static void scen_magnetic(const std::string& dir) {
 // dielectric segment (variant 1).
 static void scen_tem(const std::string& dir) {
   for (int variant = 0; variant < 2; variant++) {
-    const int nx = 140, nsteps = variant == 0 ? 320 : 360;
+    const int nx = 140, nsteps = variant == 0 ? 320 : 200;
     const double dx = 1e-6, dt = 0.9 * dx / c0;
     PMLParameterModel dm(dx);
     const PmlSpec pml = {10, 3, 1, dm.msMax, 5, 0.05};

@@ -185,3 +185,20 @@ def test_magnetic_poles_through_cpp_host_surface():
     ref = a["probe_trace_E_H"].reshape(got.shape)
     assert np.abs(ref).max() > 0
     assert rel_l2(got, ref) <= 1e-12
+
+
+def test_tem_time_loop_through_cpp_host_surface():
+    """The reference's TEM loop (tests/testTEM.cpp) through for_each on a
+    YeeGrid<TEM> line: the probe traces equal the reference's own (golden
+    fixture tem1d_small) to 1e-12."""
+    exe = os.path.join(os.path.dirname(BIN), "tem_loop_host_api")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/bin/tem_loop_host_api not built (run __graft_entry__.build())")
+    p, a = load("tem1d_small")
+    print("[gpu-case] tem_loop_host_api", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.array([[float(x) for x in line.split()] for line in r.stdout.strip().splitlines()])
+    ref = a["probe_trace_Ez_Hy"].reshape(got.shape)
+    assert np.abs(ref).max() > 0
+    assert rel_l2(got, ref) <= 1e-12

@@ -167,4 +167,4 @@ def test_te_refusals():
         Batch([g])
     g.step(2)
     with pytest.raises(ValueError):
-        Grid("TEM", 8, 8)
+        Grid("TX", 8, 8)
