@@ -2615,19 +2615,23 @@ __global__ __launch_bounds__(kTbCols * TY) void tb_3d(const StepArgs<T>* __restr
     }
   };
 
-  // ---- register slots: plane q's values live in slot (q - k0 + 1) mod 3
+  // ---- register slots: plane q's values live in slot (q - k0 + 1) mod 4.  Four
+  // slots (planes p - 1 .. p + 2 of the old state in flight or in use) make the
+  // rotation over a four-times unrolled loop a pure renaming: plane p + 2 loads
+  // into the slot of plane p - 2, no longer read, while plane p - 1's old H is
+  // still read in place
   struct Ld {
     T e[3], h[3];
     int mid;
   };
-  Ld L[3];        // old state: planes p (slot A), p + 1 (C); plane p + 2 is loaded into p - 1's (B)
-  T E1[3][3];     // E1 of planes p (A), p - 1 (B), p - 2 (C)
-  T H1[3][3];     // H1 of planes p - 1 (B), p - 2 (C), p - 3 (A)
-  T E2[3][3];     // E2 of planes p - 2 (C), p - 3 (A)
-  T PE[3][3][2];  // first-step E psi of planes p - 2 (C), p - 1 (B), p (A: loaded old psi, then updated)
-  T PH[3][3][2];  // first-step H psi of planes p - 3 (A), p - 2 (C), p - 1 (B: loaded old psi, then updated)
+  Ld L[4];        // old state of planes p - 1 .. p + 2
+  T E1[4][3];     // E1 of planes p, p - 1, p - 2
+  T H1[4][3];     // H1 of planes p - 1, p - 2, p - 3
+  T E2[4][3];     // E2 of planes p - 2, p - 3
+  T PE[4][3][2];  // first-step E psi of planes p - 2, p - 1, p (loaded old psi, then updated)
+  T PH[4][3][2];  // first-step H psi of planes p - 3, p - 2, p - 1 (loaded old psi, then updated)
 #pragma unroll
-  for (int s = 0; s < 3; s++) {
+  for (int s = 0; s < 4; s++) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       L[s].e[c] = L[s].h[c] = E1[s][c] = H1[s][c] = E2[s][c] = T(0);
@@ -2646,11 +2650,11 @@ __global__ __launch_bounds__(kTbCols * TY) void tb_3d(const StepArgs<T>* __restr
     d.e[2] = ldo(a.E[2], o);
     d.mid = ldo(a.mat, c0 + pstride * (unsigned)qc);
   };
-  load(k0 - 2, L[2]);
+  load(k0 - 2, L[3]);
   load(k0 - 1, L[0]);
   load(k0, L[1]);
   load_psi(0, k0 - 1, PE[0]);
-  unsigned mids = (unsigned)L[2].mid;  // material ids of planes p - 1, p - 2, p - 3 (bytes 0..2)
+  unsigned mids = (unsigned)L[3].mid;  // material ids of planes p - 1, p - 2, p - 3 (bytes 0..2)
   // prologue: the y neighbours of the first iteration's old H
   {
     const int wb = (k0 - 2) & 1;
@@ -2661,50 +2665,51 @@ __global__ __launch_bounds__(kTbCols * TY) void tb_3d(const StepArgs<T>* __restr
   }
   s_bar_lds();
 
-  auto iter = [&](int p, auto sa, auto sb, auto sc) {
-    constexpr int A = decltype(sa)::value, B = decltype(sb)::value, C = decltype(sc)::value;
+  // one iteration; S is plane p's slot: plane p - 1 in slot (S + 3) & 3,
+  // p - 2 and p + 2 in (S + 2) & 3, p - 3 and p + 1 in (S + 1) & 3
+  auto iter = [&](int p, auto sp) {
+    constexpr int A = decltype(sp)::value, M1 = (A + 3) & 3, M2 = (A + 2) & 3, M3 = (A + 1) & 3;
     const int rb = (p - 1) & 1, wb = p & 1;
     const int m0 = L[A].mid, m1 = mids & 255, m2 = (mids >> 8) & 255, m3 = (mids >> 16) & 255;
-    // plane p - 1's old H leaves slot B, plane p + 2's loads go into it
-    const T h0p[3] = {L[B].h[0], L[B].h[1], L[B].h[2]};
-    load(p + 2, L[B]);
+    const T(&h0p)[3] = L[M1].h;  // plane p - 1's old H
+    load(p + 2, L[M2]);
     // ---- E2 at plane q2 = p - 2 (second step)
     const int q2 = p - 2;
     if (doE2 && q2 >= k0 && q2 <= e2hi) {
-      const T hz_xm = dpp_mov<kDppWaveShr1>(H1[C][2]), hy_xm = dpp_mov<kDppWaveShr1>(H1[C][1]);
+      const T hz_xm = dpp_mov<kDppWaveShr1>(H1[M2][2]), hy_xm = dpp_mov<kDppWaveShr1>(H1[M2][1]);
       T o[3];
-      e_upd(1, q2, H1[C][0], H1[C][1], H1[C][2], xs[rb][4][rm][l], hz_xm, hy_xm, H1[A][1], xs[rb][5][rm][l],
-            H1[A][0], E1[C][0], E1[C][1], E1[C][2], sm.ie[m2], PE[C], o);
+      e_upd(1, q2, H1[M2][0], H1[M2][1], H1[M2][2], xs[rb][4][rm][l], hz_xm, hy_xm, H1[M3][1], xs[rb][5][rm][l],
+            H1[M3][0], E1[M2][0], E1[M2][1], E1[M2][2], sm.ie[m2], PE[M2], o);
 #pragma unroll
-      for (int c = 0; c < 3; c++) E2[C][c] = incol ? o[c] : E1[C][c];
+      for (int c = 0; c < 3; c++) E2[M2][c] = incol ? o[c] : E1[M2][c];
       if (q2 < k1 && own) {
         const unsigned so = vo + pb * (unsigned)q2;
         sto(a.Eo[0], so, o[0]);
         sto(a.Eo[1], so, o[1]);
         sto(a.Eo[2], so, o[2]);
-        store_psi(0, q2, PE[C]);
+        store_psi(0, q2, PE[M2]);
         if (wd && dcol && (dcol & dpl[q2 - kz0])) tb_sample(a, st2, i, j, q2, 0, o);
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < 3; c++) E2[C][c] = E1[C][c];
+      for (int c = 0; c < 3; c++) E2[M2][c] = E1[M2][c];
     }
     // ---- H2 at plane q3 = p - 3 (second step, core rows)
     const int q3 = p - 3;
     if (doH2 && q3 >= k0 && q3 < k1) {
-      const T ez_xp = dpp_mov<kDppWaveShl1>(E2[A][2]), ey_xp = dpp_mov<kDppWaveShl1>(E2[A][1]);
+      const T ez_xp = dpp_mov<kDppWaveShl1>(E2[M3][2]), ey_xp = dpp_mov<kDppWaveShl1>(E2[M3][1]);
       T o[3];
-      h_upd(1, q3, E2[A][0], E2[A][1], E2[A][2], xs[rb][6][rp][l], ez_xp, ey_xp, E2[C][1], xs[rb][7][rp][l],
-            E2[C][0], H1[A][0], H1[A][1], H1[A][2], sm.im[m3], PH[A], o);
+      h_upd(1, q3, E2[M3][0], E2[M3][1], E2[M3][2], xs[rb][6][rp][l], ez_xp, ey_xp, E2[M2][1], xs[rb][7][rp][l],
+            E2[M2][0], H1[M3][0], H1[M3][1], H1[M3][2], sm.im[m3], PH[M3], o);
       if (own) {
         const unsigned so = vo + pb * (unsigned)q3;
         sto(a.Ho[0], so, o[0]);
         sto(a.Ho[1], so, o[1]);
         sto(a.Ho[2], so, o[2]);
-        store_psi(1, q3, PH[A]);
+        store_psi(1, q3, PH[M3]);
         if (wd && dcol && (dcol & dpl[q3 - kz0])) tb_sample(a, st2, i, j, q3, 1, o);
         if (wp && pcol) {
-          const T ev[3] = {E2[A][0], E2[A][1], E2[A][2]};
+          const T ev[3] = {E2[M3][0], E2[M3][1], E2[M3][2]};
           dft_probe(a, st2, (long)c0 + (long)pstride * q3, i, j, q3, ev, o);
         }
       }
@@ -2734,51 +2739,56 @@ __global__ __launch_bounds__(kTbCols * TY) void tb_3d(const StepArgs<T>* __restr
     // ---- H1 at plane q1 = p - 1 (first step)
     const int q1 = p - 1;
     if (doH1 && q1 >= h1lo && q1 <= h1hi) {
-      const T ez_xp = dpp_mov<kDppWaveShl1>(E1[B][2]), ey_xp = dpp_mov<kDppWaveShl1>(E1[B][1]);
+      const T ez_xp = dpp_mov<kDppWaveShl1>(E1[M1][2]), ey_xp = dpp_mov<kDppWaveShl1>(E1[M1][1]);
       T o[3];
-      h_upd(0, q1, E1[B][0], E1[B][1], E1[B][2], xs[rb][2][rp][l], ez_xp, ey_xp, E1[A][1], xs[rb][3][rp][l],
-            E1[A][0], h0p[0], h0p[1], h0p[2], sm.im[m1], PH[B], o);
+      h_upd(0, q1, E1[M1][0], E1[M1][1], E1[M1][2], xs[rb][2][rp][l], ez_xp, ey_xp, E1[A][1], xs[rb][3][rp][l],
+            E1[A][0], h0p[0], h0p[1], h0p[2], sm.im[m1], PH[M1], o);
 #pragma unroll
-      for (int c = 0; c < 3; c++) H1[B][c] = incol ? o[c] : h0p[c];
+      for (int c = 0; c < 3; c++) H1[M1][c] = incol ? o[c] : h0p[c];
       if (q1 >= k0 && q1 < k1 && own) {
         if (!two) {
           const unsigned so = vo + pb * (unsigned)q1;
           sto(a.Ho[0], so, o[0]);
           sto(a.Ho[1], so, o[1]);
           sto(a.Ho[2], so, o[2]);
-          store_psi(1, q1, PH[B]);
+          store_psi(1, q1, PH[M1]);
         }
         if (wd && dcol && (dcol & dpl[q1 - kz0])) tb_sample(a, st1, i, j, q1, 1, o);
         if (wp && pcol) {
-          const T ev[3] = {E1[B][0], E1[B][1], E1[B][2]};
+          const T ev[3] = {E1[M1][0], E1[M1][1], E1[M1][2]};
           dft_probe(a, st1, (long)c0 + (long)pstride * q1, i, j, q1, ev, o);
         }
       }
     } else {
 #pragma unroll
-      for (int c = 0; c < 3; c++) H1[B][c] = h0p[c];
+      for (int c = 0; c < 3; c++) H1[M1][c] = h0p[c];
     }
-    // the next iteration's old psi: E at plane p + 1 (slot C, whose first-step
-    // psi of plane p - 2 the E2 update above consumed), H at plane p (slot A,
-    // whose plane p - 3 the H2 update consumed)
-    load_psi(0, p + 1, PE[C]);
+    // the next iteration's old psi: E at plane p + 1 (the slot of plane p - 3,
+    // whose E psi is no longer read), H at plane p (the slot of plane p - 4)
+    load_psi(0, p + 1, PE[M3]);
     load_psi(1, p, PH[A]);
     // the y neighbours the next iteration reads
-    xs[wb][0][r][l] = L[C].h[2];
-    xs[wb][1][r][l] = L[C].h[0];
+    xs[wb][0][r][l] = L[M3].h[2];
+    xs[wb][1][r][l] = L[M3].h[0];
     xs[wb][2][r][l] = E1[A][2];
     xs[wb][3][r][l] = E1[A][0];
-    xs[wb][4][r][l] = H1[B][2];
-    xs[wb][5][r][l] = H1[B][0];
-    xs[wb][6][r][l] = E2[C][2];
-    xs[wb][7][r][l] = E2[C][0];
+    xs[wb][4][r][l] = H1[M1][2];
+    xs[wb][5][r][l] = H1[M1][0];
+    xs[wb][6][r][l] = E2[M2][2];
+    xs[wb][7][r][l] = E2[M2][0];
     s_bar_lds();  // LDS only: the loads and stores in flight need not land first
     mids = ((mids << 8) | (unsigned)m0) & 0xffffffu;
   };
-  for (int p = k0 - 1; p <= k1 + 2; p += 3) {
-    iter(p, IC<0>{}, IC<2>{}, IC<1>{});
-    if (p + 1 <= k1 + 2) iter(p + 1, IC<1>{}, IC<0>{}, IC<2>{});
-    if (p + 2 <= k1 + 2) iter(p + 2, IC<2>{}, IC<1>{}, IC<0>{});
+  // whole groups of four iterations: the last group's iterations past k1 + 2
+  // update nothing (every half step and store checks its plane range; loads
+  // clamp to the grid), and an unconditional body keeps the slot rotation a
+  // renaming: skipped iterations would make the loop carry values from several
+  // paths, i.e. register copies of loads still in flight, which wait for them
+  for (int p = k0 - 1; p <= k1 + 2; p += 4) {
+    iter(p, IC<0>{});
+    iter(p + 1, IC<1>{});
+    iter(p + 2, IC<2>{});
+    iter(p + 3, IC<3>{});
   }
 }
 
