@@ -1993,7 +1993,9 @@ __global__ __launch_bounds__(kSThreads) void fused3d_lds(const StepArgs<float>* 
     // into nxt, two planes ahead of their use (three sets rotate without copies)
     auto plane = [&](int k, const SPre& cur, SPre& nxt) {
       s_bar_lds();  // B1: old H and ids of plane k+1 landed, new E of plane k published
-      if (k + 2 < kb) gather(k + 3, k + 2, nxt);
+      // (near the range's end it gathers planes already read: nothing uses
+      // them, and an unconditional gather keeps the rotation a renaming)
+      gather(min(k + 3, kb), min(k + 2, kb - 1), nxt);
       e_plane(k + 1, k + 1 < kb, cur);  // the halo plane when k+1 == kb: computed, not stored
       if (own) {  // new H of plane k
         const float* hk = L.h[k % kSSlots];
@@ -2046,11 +2048,17 @@ __global__ __launch_bounds__(kSThreads) void fused3d_lds(const StepArgs<float>* 
     ek[1] = en[1];
     ek[2] = en[2];
     publish();
-    for (int k = ka; k < kb; k += 3) {
+    // whole groups of three in the loop (skipped calls inside it would make
+    // the loop carry gathered values from several paths: register copies
+    // that wait for loads in flight), the rest after it
+    int k = ka;
+    for (; k + 2 < kb; k += 3) {
       plane(k, A, C);
-      if (k + 1 < kb) plane(k + 1, B, A);
-      if (k + 2 < kb) plane(k + 2, C, B);
+      plane(k + 1, B, A);
+      plane(k + 2, C, B);
     }
+    if (k < kb) plane(k, A, C);
+    if (k + 1 < kb) plane(k + 1, B, A);
     s_bar_lds();  // the range's end
   }
 }
@@ -2071,6 +2079,9 @@ __global__ __launch_bounds__(kSThreads) void fused3d_lds(const StepArgs<float>* 
 // Three LDS buffers of new E: a thread one plane ahead of another (they are
 // never further apart: one barrier per plane) writes the buffer the other
 // no longer reads.
+template <int N>
+using IC = std::integral_constant<int, N>;
+
 constexpr int kZCols = 64;
 template <typename T, int TY, int DISP>
 __global__ __launch_bounds__(kZCols * TY) void fusedz_3d(const StepArgs<T>* __restrict__ ap, int kc, int s, int ntx,
@@ -2094,15 +2105,7 @@ __global__ __launch_bounds__(kZCols * TY) void fusedz_3d(const StepArgs<T>* __re
   const T fxE = a.cf[0][0][incol ? i : 1], fyE = a.cf[0][1][incol ? j : 1];
   const T fxH = a.cf[1][0][incol ? i : 1], fyH = a.cf[1][1][incol ? j : 1];
   const long c0 = (long)i + sy * (long)j;
-  // old H of the cell at the plane below: the z-1 neighbour of plane k0's E
-  T hxm = T(0), hym = T(0);
-  if (inside) {
-    hxm = a.H[0][c0 + sz * (long)(k0 - 1)];
-    hym = a.H[1][c0 + sz * (long)(k0 - 1)];
-  }
-  T hp[3] = {T(0), T(0), T(0)};  // old H of the cell at the previous plane
   T ep[3] = {T(0), T(0), T(0)};  // new E of the cell at the previous plane
-  int midp = 0;
   // the old state of a plane: the cell's H and E, its id and the -x, -y H
   // neighbours; loaded one plane ahead, so a plane's loads are in flight
   // while the previous plane computes
@@ -2128,15 +2131,30 @@ __global__ __launch_bounds__(kZCols * TY) void fusedz_3d(const StepArgs<T>* __re
       q.hy_xm = a.H[1][c - 1];
     }
   };
-  Plane cur, nxt;
-  cur.h[0] = cur.h[1] = cur.h[2] = cur.e[0] = cur.e[1] = cur.e[2] = T(0);
-  cur.hz_ym = cur.hx_ym = cur.hz_xm = cur.hy_xm = T(0);
-  cur.mid = 0;
-  nxt = cur;
-  load(k0, cur);
-  for (int k = k0; k <= k1; k++) {
+  // three plane slots, plane k in slot (k - k0 + 1) mod 3, over a three times
+  // unrolled loop: plane k + 1 loads into the slot of plane k - 2 while plane
+  // k - 1's old H and id are read in place, so the loop carries no copy of a
+  // plane whose loads are still in flight (a copy would wait for them and
+  // leave nothing to overlap with the next plane's arithmetic)
+  Plane P[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    P[q].h[0] = P[q].h[1] = P[q].h[2] = P[q].e[0] = P[q].e[1] = P[q].e[2] = T(0);
+    P[q].hz_ym = P[q].hx_ym = P[q].hz_xm = P[q].hy_xm = T(0);
+    P[q].mid = 0;
+  }
+  // old H of the cell at the plane below: the z-1 neighbour of plane k0's E
+  if (inside) {
+    P[0].h[0] = a.H[0][c0 + sz * (long)(k0 - 1)];
+    P[0].h[1] = a.H[1][c0 + sz * (long)(k0 - 1)];
+  }
+  load(k0, P[1]);
+  auto it = [&](int k, auto sk) {
+    constexpr int A = decltype(sk)::value, M1 = (A + 2) % 3, N1 = (A + 1) % 3;
+    Plane& cur = P[A];
+    const Plane& prv = P[M1];  // plane k - 1
     const long c = c0 + sz * (long)k;
-    if (k < k1) load(k + 1, nxt);
+    load(k < k1 ? k + 1 : k1, P[N1]);  // (the last plane loads one nothing reads)
     T en[3] = {cur.e[0], cur.e[1], cur.e[2]};
     if (incol && k <= a.nz - 2) {
       HStencil<T> hs;
@@ -2147,8 +2165,8 @@ __global__ __launch_bounds__(kZCols * TY) void fusedz_3d(const StepArgs<T>* __re
       hs.hx_ym = cur.hx_ym;
       hs.hz_xm = cur.hz_xm;
       hs.hy_xm = cur.hy_xm;
-      hs.hy_zm = hym;
-      hs.hx_zm = hxm;
+      hs.hy_zm = prv.h[1];
+      hs.hx_zm = prv.h[0];
       const bool own = owner && k < k1;
       T o[3];
       e_cell_3d<T, DISP>(a, step, incH, i, j, k, c, hs, cur.e[0], cur.e[1], cur.e[2], cur.mid,
@@ -2182,23 +2200,25 @@ __global__ __launch_bounds__(kZCols * TY) void fusedz_3d(const StepArgs<T>* __re
       es.ey_zp = en[1];
       es.ex_zp = en[0];
       T o[3];
-      h_cell_3d(a, step, incE, i, j, k - 1, cp, es, hp[0], hp[1], hp[2], sm.im[midp], fxH, fyH, a.cf[1][2][k - 1], o);
+      h_cell_3d(a, step, incE, i, j, k - 1, cp, es, prv.h[0], prv.h[1], prv.h[2], sm.im[prv.mid], fxH, fyH,
+                a.cf[1][2][k - 1], o);
       a.Ho[0][cp] = o[0];
       a.Ho[1][cp] = o[1];
       a.Ho[2][cp] = o[2];
       if (a.nprobe) dft_probe(a, step, cp, i, j, k - 1, ep, o);
     }
-    hxm = cur.h[0];
-    hym = cur.h[1];
-    hp[0] = cur.h[0];
-    hp[1] = cur.h[1];
-    hp[2] = cur.h[2];
     ep[0] = en[0];
     ep[1] = en[1];
     ep[2] = en[2];
-    midp = cur.mid;
-    cur = nxt;
+  };
+  int k = k0;
+  for (; k + 2 <= k1; k += 3) {
+    it(k, IC<1>{});
+    it(k + 1, IC<2>{});
+    it(k + 2, IC<0>{});
   }
+  if (k <= k1) it(k, IC<1>{});
+  if (k + 1 <= k1) it(k + 1, IC<2>{});
 }
 
 // ---------------------------------------------------------------- time-blocked 3-D pass
@@ -2287,8 +2307,6 @@ __device__ FDTD_COLD void tb_sample(const StepArgs<T>& a, long step, int i, int 
   }
 }
 
-template <int N>
-using IC = std::integral_constant<int, N>;
 
 template <typename T, int TY>
 __global__ __launch_bounds__(kTbCols * TY) void tb_3d(const StepArgs<T>* __restrict__ ap, int kc, int s0, int nst,
