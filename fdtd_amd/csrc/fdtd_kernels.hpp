@@ -2244,8 +2244,8 @@ __global__ __launch_bounds__(kZCols * TY) void fusedz_3d(const StepArgs<T>* __re
 // shifts), y neighbours through LDS, written in the iteration before the one
 // that reads them: one barrier per plane.  The first step's CPML psi of a
 // cell stays in registers until the second step's update of the cell; the
-// planes' values rotate through three register slots (the loop is unrolled
-// three times, so the rotation is register renaming).  Fields are addressed
+// planes' values rotate through four register slots (the loop is unrolled
+// four times, so the rotation is register renaming).  Fields are addressed
 // by 32-bit byte offsets from their bases (one offset register per thread
 // and plane).  The cell arithmetic is e_cell_3d / h_cell_3d's, operation for
 // operation, so the fields, psi, DFT samples and probes are bitwise those of
