@@ -1305,9 +1305,13 @@ __device__ __forceinline__ void e_pass_cells(const StepArgs<T>& a, const T* tie,
     h.hz_ym = Hz[c - sy];
     h.hz_xm = Hz[c - 1];
     h.hy_xm = Hy[c - 1];
-    h.hy_zm = hy_zm;
+    // the general instance reloads the plane below (a cache hit) instead: its
+    // carried pair, loaded before the loop, was copied at the loop header,
+    // and the wait for those loads there also waited, every plane, for the
+    // previous plane's stores (measured 10 % slower on a periodic 256^3 grid)
+    h.hy_zm = GEN ? Hy[c - sz] : hy_zm;
     h.hx_ym = Hx[c - sy];
-    h.hx_zm = hx_zm;
+    h.hx_zm = GEN ? Hx[c - sz] : hx_zm;
     hx_zm = h.hx;
     hy_zm = h.hy;
     const T ex = Ex[c], ey = Ey[c], ez = Ez[c];
@@ -1450,8 +1454,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h
     const StepArgs<float>* __restrict__ ap, int kc, int s, int ntx, int nty, int nkc) {
   h_pass_3d_body<float, false>(ap, kc, s, ntx, nty, nkc);
 }
+// (the general instance at 6 waves: at 7 it spills 68 bytes and runs 4 % slower)
 template <>
-__global__ __attribute__((amdgpu_waves_per_eu(7))) __launch_bounds__(256) void h_pass_3d<float, true>(
+__global__ __attribute__((amdgpu_waves_per_eu(6))) __launch_bounds__(256) void h_pass_3d<float, true>(
     const StepArgs<float>* __restrict__ ap, int kc, int s, int ntx, int nty, int nkc) {
   h_pass_3d_body<float, true>(ap, kc, s, ntx, nty, nkc);
 }
